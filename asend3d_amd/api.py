@@ -1,0 +1,268 @@
+"""ctypes binding of libasend_gpu.so (the C ABI declared in include/asend_gpu.h).
+
+There is no CPU fallback: if the library is missing this module raises at
+import, and device calls raise when the HIP runtime reports an error.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libasend_gpu.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is not built; run `python -m asend3d_amd.build` (hipcc, gfx950)")
+
+_lib = ctypes.CDLL(LIB_PATH)
+
+BRICK8 = 8
+BRICKIM = 81
+SHELL4 = 41
+
+SOLVE_PCG_FP64 = 0
+SOLVE_IR_MIXED = 1
+
+PC_JACOBI = 0
+PC_BLOCK_JACOBI = 1
+PC_MG = 2
+PC_LEARNED = 3
+
+STOP_REL_RESIDUAL = 0
+STOP_REF_ABS = 1
+
+_i64 = ctypes.c_int64
+_dbl = ctypes.c_double
+_vp = ctypes.c_void_p
+
+
+class SolveOpts(ctypes.Structure):
+    _fields_ = [("method", ctypes.c_int), ("precond", ctypes.c_int), ("stop_rule", ctypes.c_int),
+                ("check_every", ctypes.c_int), ("tol", ctypes.c_double), ("inner_tol", ctypes.c_double),
+                ("max_it", ctypes.c_int64), ("max_outer", ctypes.c_int64)]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int64), ("outer_iterations", ctypes.c_int64),
+                ("rel_residual", ctypes.c_double), ("ref_residual", ctypes.c_double),
+                ("converged", ctypes.c_int), ("fallback_used", ctypes.c_int), ("device_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib.asend_last_error.restype = ctypes.c_char_p
+_lib.asend_last_error.argtypes = [_vp]
+_lib.asend_version.restype = ctypes.c_char_p
+_lib.asend_block_dim_auto.restype = ctypes.c_int64
+_lib.asend_block_dim_auto.argtypes = [ctypes.c_int64]
+_lib.asend_penalty_scale.restype = ctypes.c_double
+_lib.asend_penalty_scale.argtypes = [_vp]
+_lib.asend_ctx_destroy.restype = None
+_lib.asend_ctx_destroy.argtypes = [_vp]
+
+EXPORTED = ["asend_ctx_create", "asend_ctx_destroy", "asend_last_error", "asend_version", "asend_sync",
+            "asend_reorder_nodes", "asend_block_dim_auto", "asend_surface_traction", "asend_lognormal_moduli",
+            "asend_upload_mesh", "asend_set_grid", "asend_update_moduli", "asend_set_penalty",
+            "asend_penalty_scale", "asend_assemble", "asend_get_matrix", "asend_spmv", "asend_solve",
+            "asend_element_sens", "asend_static_compliance", "asend_bench_spmv"]
+
+
+def lib():
+    return _lib
+
+
+def _p(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _i64a(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+class AsendError(RuntimeError):
+    pass
+
+
+def version():
+    return _lib.asend_version().decode()
+
+
+# ---------------------------------------------------------------- host prep --
+def reorder_nodes(xyz, conn, dof_per_nd, constraints, block_dim):
+    """Node reordering + DOF numbering (replaces Model::reorder_nodes,
+    S/model_prep.rs:56-303).  Same arguments as the oracle's binding."""
+    xyz = _f64(xyz)
+    conn = _i32(conn)
+    n_nodes = xyz.shape[0]
+    n_el, nn = conn.shape
+    cptr, tptr, nodes = [0], [0], []
+    for c in constraints:
+        for t in c:
+            nodes.extend(int(v) for v in t)
+            tptr.append(len(nodes))
+        cptr.append(len(tptr) - 1)
+    cptr = _i64a(cptr)
+    tptr = _i64a(tptr)
+    setn = _i32(nodes if nodes else [0])
+    ordered = np.zeros(n_nodes, dtype=np.int32)
+    dof_index = np.zeros((n_nodes, 6), dtype=np.int64)
+    ndof = _i64(0)
+    rc = _lib.asend_reorder_nodes(_i64(n_nodes), _p(xyz), _i64(n_el), ctypes.c_int(nn), ctypes.c_int(dof_per_nd),
+                                  _p(conn), _i64(len(constraints)), _p(cptr), _p(tptr), _p(setn), _i64(block_dim),
+                                  _p(ordered), _p(dof_index), ctypes.byref(ndof))
+    if rc != 0:
+        raise AsendError(f"asend_reorder_nodes failed ({rc})")
+    return ordered, dof_index, int(ndof.value)
+
+
+def block_dim_auto(n_nodes):
+    return int(_lib.asend_block_dim_auto(n_nodes))
+
+
+def surface_traction(etype, xyz, conn, dof_index, n_dof, el_set, load, norm_dir, norm_tol_deg, f=None):
+    xyz = _f64(xyz)
+    conn = _i32(conn)
+    di = _i64a(dof_index)
+    es = _i32(el_set)
+    ld = _f64(load)
+    nd = _f64(norm_dir)
+    if f is None:
+        f = np.zeros(n_dof)
+    rc = _lib.asend_surface_traction(ctypes.c_int(etype), _i64(xyz.shape[0]), _p(xyz), _i64(conn.shape[0]), _p(conn),
+                                     _p(di), _i64(es.shape[0]), _p(es), _p(ld), _p(nd), _dbl(norm_tol_deg),
+                                     _i64(n_dof), _p(f))
+    if rc != 0:
+        raise AsendError(f"asend_surface_traction failed ({rc})")
+    return f
+
+
+def lognormal_moduli(seed, n, median, sigma_ln):
+    out = np.zeros(n)
+    rc = _lib.asend_lognormal_moduli(ctypes.c_uint64(seed), _i64(n), _dbl(median), _dbl(sigma_ln), _p(out))
+    if rc != 0:
+        raise AsendError(f"asend_lognormal_moduli failed ({rc})")
+    return out
+
+
+# ------------------------------------------------------------------ context --
+class Context:
+    """One device context (one HIP stream) holding a mesh and its operator."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        rc = _lib.asend_ctx_create(ctypes.c_int(device), ctypes.byref(h))
+        if rc != 0:
+            raise AsendError(f"asend_ctx_create failed ({rc}): no usable GPU")
+        self.h = h
+        self.n_dof = 0
+        self.n_el = 0
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            _lib.asend_ctx_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, name):
+        if rc != 0:
+            msg = _lib.asend_last_error(self.h).decode()
+            raise AsendError(f"{name} failed ({rc}): {msg}")
+
+    def upload_mesh(self, etype, xyz, dof_index, n_dof, conn, E, nu, rho=None):
+        xyz = _f64(xyz)
+        di = _i64a(dof_index)
+        conn = _i32(conn)
+        E = _f64(E)
+        nu = _f64(nu)
+        rho = None if rho is None else _f64(rho)
+        self._chk(_lib.asend_upload_mesh(self.h, ctypes.c_int(etype), _i64(xyz.shape[0]), _p(xyz), _p(di),
+                                         _i64(n_dof), _i64(conn.shape[0]), _p(conn), _p(E), _p(nu), _p(rho)),
+                  "asend_upload_mesh")
+        self.n_dof = n_dof
+        self.n_el = conn.shape[0]
+
+    def set_grid(self, nx, ny, nz):
+        self._chk(_lib.asend_set_grid(self.h, _i64(nx), _i64(ny), _i64(nz)), "asend_set_grid")
+
+    def update_moduli(self, E):
+        E = _f64(E)
+        self._chk(_lib.asend_update_moduli(self.h, _p(E)), "asend_update_moduli")
+
+    def set_penalty(self, dofs, sf=0.0):
+        d = _i64a(dofs)
+        self._chk(_lib.asend_set_penalty(self.h, _i64(d.shape[0]), _p(d), _dbl(sf)), "asend_set_penalty")
+
+    def penalty_scale(self):
+        return float(_lib.asend_penalty_scale(self.h))
+
+    def assemble(self, c_mass=0.0):
+        self._chk(_lib.asend_assemble(self.h, _dbl(c_mass)), "asend_assemble")
+
+    def get_matrix(self, with_penalty=True):
+        nnz = _i64(0)
+        self._chk(_lib.asend_get_matrix(self.h, None, None, None, ctypes.byref(nnz), ctypes.c_int(1)),
+                  "asend_get_matrix")
+        rowptr = np.zeros(self.n_dof + 1, dtype=np.int64)
+        cols = np.zeros(nnz.value, dtype=np.int32)
+        vals = np.zeros(nnz.value)
+        self._chk(_lib.asend_get_matrix(self.h, _p(rowptr), _p(cols), _p(vals), ctypes.byref(nnz),
+                                        ctypes.c_int(1 if with_penalty else 0)), "asend_get_matrix")
+        return rowptr, cols, vals
+
+    def spmv(self, x, precision=64):
+        x = _f64(x)
+        y = np.zeros(self.n_dof)
+        self._chk(_lib.asend_spmv(self.h, ctypes.c_int(precision), _p(x), _p(y)), "asend_spmv")
+        return y
+
+    @staticmethod
+    def opts(method=SOLVE_PCG_FP64, precond=PC_JACOBI, stop_rule=STOP_REL_RESIDUAL, tol=1e-8, inner_tol=1e-4,
+             max_it=0, max_outer=30, check_every=16):
+        return SolveOpts(method, precond, stop_rule, check_every, tol, inner_tol, max_it, max_outer)
+
+    def solve(self, rhs, opts):
+        rhs = _f64(rhs)
+        x = np.zeros(self.n_dof)
+        st = SolveStats()
+        self._chk(_lib.asend_solve(self.h, _p(rhs), _p(x), ctypes.byref(opts), ctypes.byref(st)), "asend_solve")
+        return x, st
+
+    def element_sens(self, u, lam):
+        u = _f64(u)
+        lam = _f64(lam)
+        out = np.zeros(self.n_el)
+        self._chk(_lib.asend_element_sens(self.h, _p(u), _p(lam), _p(out)), "asend_element_sens")
+        return out
+
+    def static_compliance(self, f, opts):
+        f = _f64(f)
+        u = np.zeros(self.n_dof)
+        s = np.zeros(self.n_el)
+        J = _dbl(0)
+        st = SolveStats()
+        self._chk(_lib.asend_static_compliance(self.h, _p(f), _p(u), _p(s), ctypes.byref(J), ctypes.byref(opts),
+                                               ctypes.byref(st)), "asend_static_compliance")
+        return u, s, float(J.value), st
+
+    def bench_spmv(self, precision=32, n_iter=50):
+        ms = _dbl(0)
+        self._chk(_lib.asend_bench_spmv(self.h, ctypes.c_int(precision), ctypes.c_int(n_iter), ctypes.byref(ms)),
+                  "asend_bench_spmv")
+        return float(ms.value)
+
+    def sync(self):
+        self._chk(_lib.asend_sync(self.h), "asend_sync")

@@ -1,0 +1,793 @@
+// Device context and the C ABI of include/asend_gpu.h.
+//
+// Host orchestration of the solve path: mesh upload (node-block pattern,
+// element colouring, element-to-block map), assembly, Krylov drivers with
+// device-resident scalars, sensitivities.  The reference runs all of this in
+// one serial Rust process (Model::solve / Model::get_obj_gradient,
+// S/model_analysis.rs:560-673, :1544-1706); here every per-element and
+// per-iteration step is a HIP kernel on the context's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "asend_gpu.h"
+#include "kernels.hpp"
+#include "precond.hpp"
+
+using namespace asend;
+
+struct asend_ctx {
+    int device = 0;
+    hipStream_t st = nullptr;
+    std::string err;
+
+    int etype = 0, bs = 3;
+    int64_t n_nodes = 0, n_dof = 0, n_el = 0, nnzb = 0;
+    int64_t grid[3] = {0, 0, 0};
+    std::vector<int32_t> h_rp, h_bc, h_color_ptr;
+
+    int32_t *d_conn_r = nullptr, *d_rp = nullptr, *d_bc = nullptr, *d_kdiag = nullptr, *d_elems = nullptr;
+    uint8_t* d_slot = nullptr;
+    double *d_xyz_r = nullptr, *d_E = nullptr, *d_nu = nullptr, *d_rho = nullptr;
+    double* d_K64 = nullptr;
+    float* d_K32 = nullptr;
+    double* d_dinv64 = nullptr;
+    float* d_dinv32 = nullptr;
+
+    int64_t* d_pen = nullptr;
+    int32_t* d_pen_mult = nullptr;
+    int n_pen = 0;
+    double sf_user = 0.0, sf = 0.0;
+    bool sf_fixed = false, assembled = false;
+
+    // work vectors
+    double *d_f = nullptr, *d_u = nullptr, *d_r = nullptr, *d_p = nullptr, *d_q = nullptr, *d_lam = nullptr,
+           *d_sens = nullptr, *d_x64 = nullptr, *d_r2 = nullptr;
+    float *d_r32 = nullptr, *d_x32 = nullptr, *d_p32 = nullptr, *d_q32 = nullptr, *d_w32 = nullptr;
+
+    Scalars* d_sc = nullptr;  // [0] outer/FP64 loop, [1] inner FP32 loop, [2] misc
+    Scalars* h_sc = nullptr;  // pinned mirror
+    double* d_partials = nullptr;
+    unsigned* d_counter = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    Multigrid* mg = nullptr;
+};
+
+namespace {
+
+int fail(asend_ctx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                                 \
+    do {                                                                                             \
+        hipError_t e_ = (expr);                                                                      \
+        if (e_ != hipSuccess) return fail(ctx, ASEND_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+int dalloc(asend_ctx* ctx, T** p, size_t n) {
+    if (*p) {
+        (void)hipFree(*p);
+        *p = nullptr;
+    }
+    if (n == 0) n = 1;
+    HIPCHK(hipMalloc((void**)p, n * sizeof(T)));
+    return ASEND_OK;
+}
+
+template <typename T>
+void dfree(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+int ensure_work(asend_ctx* ctx) {
+    int64_t n = ctx->n_dof;
+    if (ctx->d_f) return ASEND_OK;
+    int rc;
+    if ((rc = dalloc(ctx, &ctx->d_f, n)) || (rc = dalloc(ctx, &ctx->d_u, n)) || (rc = dalloc(ctx, &ctx->d_r, n)) ||
+        (rc = dalloc(ctx, &ctx->d_p, n)) || (rc = dalloc(ctx, &ctx->d_q, n)) || (rc = dalloc(ctx, &ctx->d_lam, n)) ||
+        (rc = dalloc(ctx, &ctx->d_x64, n)) || (rc = dalloc(ctx, &ctx->d_sens, ctx->n_el)) ||
+        (rc = dalloc(ctx, &ctx->d_r2, n)) ||
+        (rc = dalloc(ctx, &ctx->d_r32, n)) || (rc = dalloc(ctx, &ctx->d_x32, n)) ||
+        (rc = dalloc(ctx, &ctx->d_p32, n)) || (rc = dalloc(ctx, &ctx->d_q32, n)) ||
+        (rc = dalloc(ctx, &ctx->d_w32, n)))
+        return rc;
+    return ASEND_OK;
+}
+
+int read_scalars(asend_ctx* ctx, int idx) {
+    HIPCHK(hipMemcpyAsync(&ctx->h_sc[idx], &ctx->d_sc[idx], sizeof(Scalars), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return ASEND_OK;
+}
+
+int write_scalars(asend_ctx* ctx, int idx) {
+    HIPCHK(hipMemcpyAsync(&ctx->d_sc[idx], &ctx->h_sc[idx], sizeof(Scalars), hipMemcpyHostToDevice, ctx->st));
+    return ASEND_OK;
+}
+
+void free_mesh(asend_ctx* ctx) {
+    dfree(ctx->d_conn_r);
+    dfree(ctx->d_rp);
+    dfree(ctx->d_bc);
+    dfree(ctx->d_kdiag);
+    dfree(ctx->d_elems);
+    dfree(ctx->d_slot);
+    dfree(ctx->d_xyz_r);
+    dfree(ctx->d_E);
+    dfree(ctx->d_nu);
+    dfree(ctx->d_rho);
+    dfree(ctx->d_K64);
+    dfree(ctx->d_K32);
+    dfree(ctx->d_dinv64);
+    dfree(ctx->d_dinv32);
+    dfree(ctx->d_f);
+    dfree(ctx->d_u);
+    dfree(ctx->d_r);
+    dfree(ctx->d_p);
+    dfree(ctx->d_q);
+    dfree(ctx->d_lam);
+    dfree(ctx->d_x64);
+    dfree(ctx->d_r2);
+    dfree(ctx->d_sens);
+    dfree(ctx->d_r32);
+    dfree(ctx->d_x32);
+    dfree(ctx->d_p32);
+    dfree(ctx->d_q32);
+    dfree(ctx->d_w32);
+    if (ctx->mg) {
+        mg_destroy(ctx->mg);
+        ctx->mg = nullptr;
+    }
+    ctx->assembled = false;
+}
+
+// ---------------------------------------------------------------------------
+// Krylov drivers
+// ---------------------------------------------------------------------------
+
+// Jacobi PCG on the operator of precision T.  b, x on device (length n_dof).
+// Scalars slot `si`.  Returns iterations through the slot.
+template <typename T>
+int run_jacobi_pcg(asend_ctx* ctx, int si, const T* b, T* x, double tol, bool stop_abs, int64_t max_it,
+                   int check_every) {
+    const int64_t n = ctx->n_dof;
+    const int nr = (int)ctx->n_nodes;
+    const T* K;
+    const T* dinv;
+    T *r, *p, *q;
+    if (sizeof(T) == 8) {
+        K = (const T*)ctx->d_K64;
+        dinv = (const T*)ctx->d_dinv64;
+        r = (T*)ctx->d_r2;
+        p = (T*)ctx->d_p;
+        q = (T*)ctx->d_q;
+    } else {
+        K = (const T*)ctx->d_K32;
+        dinv = (const T*)ctx->d_dinv32;
+        r = (T*)ctx->d_w32;
+        p = (T*)ctx->d_p32;
+        q = (T*)ctx->d_q32;
+    }
+    Scalars& h = ctx->h_sc[si];
+    std::memset(&h, 0, sizeof(Scalars));
+    h.tol = tol;
+    h.tol2 = tol * tol;
+    h.stop_abs = stop_abs ? 1 : 0;
+    h.max_it = max_it;
+    int rc;
+    if ((rc = write_scalars(ctx, si))) return rc;
+    Scalars* sc = ctx->d_sc + si;
+    launch_pcg_init<T>(ctx->st, n, b, dinv, x, r, p, ctx->d_partials, ctx->d_counter, sc);
+    for (;;) {
+        for (int k = 0; k < check_every; ++k) {
+            launch_spmv3<T>(ctx->st, 1, nr, ctx->d_rp, ctx->d_bc, K, p, q, nullptr, ctx->d_partials, ctx->d_counter,
+                            sc);
+            launch_pcg_update<T>(ctx->st, n, dinv, x, r, p, q, ctx->d_partials, ctx->d_counter, sc);
+            launch_pcg_dir<T>(ctx->st, n, dinv, r, p, sc);
+        }
+        HIPCHK(hipGetLastError());
+        if ((rc = read_scalars(ctx, si))) return rc;
+        if (h.done) break;
+    }
+    return ASEND_OK;
+}
+
+// FP64 residual r = f - K u, sc[0].rr = ||r||^2.
+int residual64(asend_ctx* ctx, const double* f, const double* u, double* r) {
+    launch_spmv3<double>(ctx->st, 2, (int)ctx->n_nodes, ctx->d_rp, ctx->d_bc, ctx->d_K64, u, r, f, ctx->d_partials,
+                         ctx->d_counter, ctx->d_sc + 0);
+    HIPCHK(hipGetLastError());
+    return ASEND_OK;
+}
+
+int solve_device(asend_ctx* ctx, const asend_solve_opts* o, asend_solve_stats* st) {
+    const int64_t n = ctx->n_dof;
+    int64_t max_it = o->max_it > 0 ? o->max_it : n;
+    int check = o->check_every > 0 ? o->check_every : 16;
+    int64_t max_outer = o->max_outer > 0 ? o->max_outer : 20;
+    int rc;
+    asend_solve_stats s;
+    std::memset(&s, 0, sizeof(s));
+    // ||f||^2 with u = 0: r = f - K*0
+    HIPCHK(hipMemsetAsync(ctx->d_u, 0, n * sizeof(double), ctx->st));
+    {
+        Scalars& h = ctx->h_sc[0];
+        std::memset(&h, 0, sizeof(Scalars));
+        if ((rc = write_scalars(ctx, 0))) return rc;
+    }
+    if ((rc = residual64(ctx, ctx->d_f, ctx->d_u, ctx->d_r))) return rc;
+    if ((rc = read_scalars(ctx, 0))) return rc;
+    const double ff = ctx->h_sc[0].rr;
+    if (ff == 0.0) {
+        s.converged = 1;
+        s.outer_iterations = 0;
+        *st = s;
+        return ASEND_OK;
+    }
+    double rr = ff;
+    int64_t total_it = 0;
+
+    if (o->method == ASEND_SOLVE_PCG_FP64) {
+        if (o->precond != ASEND_PC_JACOBI) return fail(ctx, ASEND_ERR_UNSUPPORTED, "FP64 PCG supports Jacobi only");
+        // FP64 PCG; restarted from the true residual if the recursive one drifted
+        int64_t outer = 0;
+        for (; outer < max_outer; ++outer) {
+            if (o->stop_rule == ASEND_STOP_REL_RESIDUAL && std::sqrt(rr / ff) <= o->tol) break;
+            double tol_in = (o->stop_rule == ASEND_STOP_REL_RESIDUAL) ? o->tol * std::sqrt(ff / rr) : o->tol;
+            if ((rc = run_jacobi_pcg<double>(ctx, 1, ctx->d_r, ctx->d_x64, tol_in, o->stop_rule == ASEND_STOP_REF_ABS,
+                                             max_it - total_it, check)))
+                return rc;
+            total_it += ctx->h_sc[1].it;
+            s.ref_residual = ctx->h_sc[1].rz;
+            // u += x ; r = f - K u
+            launch_axpy64(ctx->st, n, 1.0, ctx->d_x64, ctx->d_u);
+            if ((rc = residual64(ctx, ctx->d_f, ctx->d_u, ctx->d_r))) return rc;
+            if ((rc = read_scalars(ctx, 0))) return rc;
+            rr = ctx->h_sc[0].rr;
+            if (o->stop_rule == ASEND_STOP_REF_ABS) {
+                ++outer;
+                break;
+            }
+            if (total_it >= max_it) {
+                ++outer;
+                break;
+            }
+        }
+        s.outer_iterations = outer;
+        s.iterations = total_it;
+        s.rel_residual = std::sqrt(rr / ff);
+        s.converged = (o->stop_rule == ASEND_STOP_REL_RESIDUAL) ? (s.rel_residual <= o->tol)
+                                                                 : (s.ref_residual <= o->tol);
+        *st = s;
+        return ASEND_OK;
+    }
+
+    if (o->method != ASEND_SOLVE_IR_MIXED) return fail(ctx, ASEND_ERR_ARG, "unknown solve method %d", o->method);
+    if (o->stop_rule != ASEND_STOP_REL_RESIDUAL)
+        return fail(ctx, ASEND_ERR_UNSUPPORTED, "mixed solver supports the relative-residual rule only");
+    double inner_tol = o->inner_tol > 0 ? o->inner_tol : 1e-4;
+    int64_t outer = 0;
+    for (; outer < max_outer; ++outer) {
+        if (std::sqrt(rr / ff) <= o->tol) break;
+        if (total_it >= max_it) break;
+        launch_scale_to_f32(ctx->st, n, ctx->d_r, ctx->d_r32, ctx->d_sc + 0);
+        int64_t it_in = 0;
+        if (o->precond == ASEND_PC_JACOBI) {
+            if ((rc = run_jacobi_pcg<float>(ctx, 1, ctx->d_r32, ctx->d_x32, inner_tol, false, max_it - total_it,
+                                            check)))
+                return rc;
+            it_in = ctx->h_sc[1].it;
+        } else {
+            if (!ctx->mg) return fail(ctx, ASEND_ERR_STATE, "multigrid preconditioner not built (asend_set_grid)");
+            int fb = 0;
+            if ((rc = mg_fcg_solve(ctx->mg, ctx->st, ctx->d_r32, ctx->d_x32, inner_tol, max_it - total_it, check,
+                                   &it_in, &fb)))
+                return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
+            s.fallback_used |= fb;
+        }
+        total_it += it_in;
+        launch_axpy_from_f32(ctx->st, n, ctx->d_u, ctx->d_x32, ctx->d_sc + 0);
+        if ((rc = residual64(ctx, ctx->d_f, ctx->d_u, ctx->d_r))) return rc;
+        if ((rc = read_scalars(ctx, 0))) return rc;
+        rr = ctx->h_sc[0].rr;
+    }
+    s.outer_iterations = outer;
+    s.iterations = total_it;
+    s.rel_residual = std::sqrt(rr / ff);
+    s.converged = s.rel_residual <= o->tol;
+    *st = s;
+    return ASEND_OK;
+}
+
+int assemble_device(asend_ctx* ctx, double c_mass) {
+    const int64_t nval = ctx->nnzb * ctx->bs * ctx->bs;
+    HIPCHK(hipMemsetAsync(ctx->d_K64, 0, nval * sizeof(double), ctx->st));
+    int ncol = (int)ctx->h_color_ptr.size() - 1;
+    for (int c = 0; c < ncol; ++c) {
+        int b = ctx->h_color_ptr[c], e = ctx->h_color_ptr[c + 1];
+        launch_hex8_assemble(ctx->st, ctx->d_elems + b, e - b, ctx->d_conn_r, ctx->d_xyz_r, ctx->d_E, ctx->d_nu,
+                             ctx->d_rho, c_mass, ctx->d_rp, ctx->d_slot, ctx->d_K64);
+    }
+    HIPCHK(hipGetLastError());
+    int rc;
+    if (ctx->n_pen > 0) {
+        if (!ctx->sf_fixed) {
+            if (ctx->sf_user > 0) {
+                ctx->sf = ctx->sf_user;
+            } else {
+                // reference rule: sf = 1e5 * max|K_ij| of the first assembly (S/model_analysis.rs:255-259)
+                launch_maxabs(ctx->st, ctx->d_K64, nval, ctx->d_partials, ctx->d_counter, ctx->d_sc + 2);
+                HIPCHK(hipGetLastError());
+                if ((rc = read_scalars(ctx, 2))) return rc;
+                ctx->sf = 1.0e5 * ctx->h_sc[2].maxabs;
+            }
+            ctx->sf_fixed = true;
+        }
+        ctx->h_sc[2].sf = ctx->sf;
+        if ((rc = write_scalars(ctx, 2))) return rc;
+        launch_penalty_diag(ctx->st, ctx->d_pen, ctx->d_pen_mult, ctx->n_pen, ctx->d_rp, ctx->d_kdiag, ctx->bs,
+                            ctx->d_K64, ctx->d_sc + 2);
+    }
+    launch_diag_inv(ctx->st, (int)ctx->n_nodes, ctx->bs, ctx->d_rp, ctx->d_kdiag, ctx->d_K64, ctx->d_dinv64,
+                    ctx->d_dinv32);
+    launch_to_f32(ctx->st, ctx->d_K64, ctx->d_K32, nval);
+    HIPCHK(hipGetLastError());
+    ctx->assembled = true;
+    if (ctx->mg) {
+        if ((rc = mg_setup(ctx->mg, ctx->st, ctx->d_E, ctx->d_nu, ctx->d_K32, ctx->d_dinv32, ctx->sf)))
+            return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
+    }
+    return ASEND_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int asend_ctx_create(int device, asend_ctx** out) {
+    if (!out) return ASEND_ERR_ARG;
+    *out = nullptr;
+    asend_ctx* ctx = new asend_ctx();
+    ctx->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) {
+        delete ctx;
+        return ASEND_ERR_HIP;
+    }
+    if (hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_sc, 4 * sizeof(Scalars)) != hipSuccess ||
+        hipHostMalloc((void**)&ctx->h_sc, 4 * sizeof(Scalars), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_partials, kMaxPartials * 2 * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_counter, 16 * sizeof(unsigned)) != hipSuccess ||
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+        asend_ctx_destroy(ctx);
+        return ASEND_ERR_HIP;
+    }
+    if (hipMemset(ctx->d_counter, 0, 16 * sizeof(unsigned)) != hipSuccess ||
+        hipMemset(ctx->d_sc, 0, 4 * sizeof(Scalars)) != hipSuccess) {
+        asend_ctx_destroy(ctx);
+        return ASEND_ERR_HIP;
+    }
+    std::memset(ctx->h_sc, 0, 4 * sizeof(Scalars));
+    *out = ctx;
+    return ASEND_OK;
+}
+
+void asend_ctx_destroy(asend_ctx* ctx) {
+    if (!ctx) return;
+    if (ctx->st) (void)hipStreamSynchronize(ctx->st);
+    free_mesh(ctx);
+    dfree(ctx->d_pen);
+    dfree(ctx->d_pen_mult);
+    dfree(ctx->d_sc);
+    dfree(ctx->d_partials);
+    dfree(ctx->d_counter);
+    if (ctx->h_sc) (void)hipHostFree(ctx->h_sc);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->st) (void)hipStreamDestroy(ctx->st);
+    delete ctx;
+}
+
+const char* asend_last_error(const asend_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int asend_sync(asend_ctx* ctx) {
+    if (!ctx) return ASEND_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return ASEND_OK;
+}
+
+int asend_upload_mesh(asend_ctx* ctx, int elem_type, int64_t n_nodes, const double* xyz, const int64_t* dof_index,
+                      int64_t n_dof, int64_t n_el, const int32_t* conn, const double* E, const double* nu,
+                      const double* rho) {
+    if (!ctx) return ASEND_ERR_ARG;
+    if (elem_type != ASEND_BRICK8)
+        return fail(ctx, ASEND_ERR_UNSUPPORTED, "element type %d has no device integration kernel", elem_type);
+    if (n_nodes <= 0 || n_el <= 0 || !xyz || !dof_index || !conn || !E || !nu)
+        return fail(ctx, ASEND_ERR_ARG, "bad mesh arguments");
+    if (n_dof != 3 * n_nodes) return fail(ctx, ASEND_ERR_ARG, "n_dof must equal 3*n_nodes for brick8 meshes");
+    if (n_nodes >= (1ll << 31) / 9) return fail(ctx, ASEND_ERR_UNSUPPORTED, "mesh too large for int32 indices");
+    HIPCHK(hipSetDevice(ctx->device));
+    free_mesh(ctx);
+    ctx->etype = elem_type;
+    ctx->bs = 3;
+    ctx->n_nodes = n_nodes;
+    ctx->n_dof = n_dof;
+    ctx->n_el = n_el;
+    ctx->sf_fixed = false;
+    // node rank from the DOF numbering: dof_index[6n+c] = 3*rank + c
+    std::vector<int32_t> rank(n_nodes);
+    std::vector<uint8_t> seen(n_nodes, 0);
+    for (int64_t nd = 0; nd < n_nodes; ++nd) {
+        int64_t d0 = dof_index[6 * nd];
+        if (d0 < 0 || d0 % 3 != 0 || d0 >= n_dof || dof_index[6 * nd + 1] != d0 + 1 || dof_index[6 * nd + 2] != d0 + 2)
+            return fail(ctx, ASEND_ERR_ARG, "dof_index of node %lld is not a 3-DOF node block", (long long)nd);
+        int64_t r = d0 / 3;
+        if (seen[r]) return fail(ctx, ASEND_ERR_ARG, "dof_index is not a permutation");
+        seen[r] = 1;
+        rank[nd] = (int32_t)r;
+    }
+    std::vector<int32_t> conn_r(8 * n_el);
+    for (int64_t i = 0; i < 8 * n_el; ++i) {
+        int32_t v = conn[i];
+        if (v < 0 || v >= n_nodes) return fail(ctx, ASEND_ERR_ARG, "connectivity out of range");
+        conn_r[i] = rank[v];
+    }
+    std::vector<double> xyz_r(3 * n_nodes);
+    for (int64_t nd = 0; nd < n_nodes; ++nd)
+        for (int c = 0; c < 3; ++c) xyz_r[3 * (int64_t)rank[nd] + c] = xyz[3 * nd + c];
+    // node -> element adjacency
+    std::vector<int32_t> n2e_ptr(n_nodes + 1, 0), n2e;
+    for (int64_t i = 0; i < 8 * n_el; ++i) n2e_ptr[conn_r[i] + 1]++;
+    for (int64_t r = 0; r < n_nodes; ++r) n2e_ptr[r + 1] += n2e_ptr[r];
+    n2e.resize(n2e_ptr[n_nodes]);
+    {
+        std::vector<int32_t> fill(n2e_ptr.begin(), n2e_ptr.end() - 1);
+        for (int64_t e = 0; e < n_el; ++e)
+            for (int a = 0; a < 8; ++a) n2e[fill[conn_r[8 * e + a]]++] = (int32_t)e;
+    }
+    // block rows: sorted unique neighbour ranks
+    ctx->h_rp.assign(n_nodes + 1, 0);
+    std::vector<std::vector<int32_t>> rows(n_nodes);
+    {
+        std::vector<int32_t> tmp;
+        for (int64_t r = 0; r < n_nodes; ++r) {
+            tmp.clear();
+            for (int32_t k = n2e_ptr[r]; k < n2e_ptr[r + 1]; ++k) {
+                int32_t e = n2e[k];
+                for (int a = 0; a < 8; ++a) tmp.push_back(conn_r[8 * (int64_t)e + a]);
+            }
+            if (tmp.empty()) tmp.push_back((int32_t)r);  // isolated node keeps its diagonal
+            std::sort(tmp.begin(), tmp.end());
+            tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+            if (tmp.size() > 255) return fail(ctx, ASEND_ERR_UNSUPPORTED, "node with more than 255 neighbours");
+            rows[r] = tmp;
+            ctx->h_rp[r + 1] = ctx->h_rp[r] + (int32_t)tmp.size();
+        }
+    }
+    ctx->nnzb = ctx->h_rp[n_nodes];
+    ctx->h_bc.resize(ctx->nnzb);
+    std::vector<int32_t> kdiag(n_nodes);
+    for (int64_t r = 0; r < n_nodes; ++r) {
+        std::copy(rows[r].begin(), rows[r].end(), ctx->h_bc.begin() + ctx->h_rp[r]);
+        kdiag[r] = (int32_t)(std::lower_bound(rows[r].begin(), rows[r].end(), (int32_t)r) - rows[r].begin());
+    }
+    // element -> in-row slot of each (a, b) block
+    std::vector<uint8_t> slot(64 * n_el);
+    for (int64_t e = 0; e < n_el; ++e)
+        for (int a = 0; a < 8; ++a) {
+            const auto& row = rows[conn_r[8 * e + a]];
+            for (int b = 0; b < 8; ++b)
+                slot[64 * e + 8 * a + b] =
+                    (uint8_t)(std::lower_bound(row.begin(), row.end(), conn_r[8 * e + b]) - row.begin());
+        }
+    rows.clear();
+    rows.shrink_to_fit();
+    // greedy element colouring: no two elements of a colour share a node
+    std::vector<uint64_t> mask(n_nodes, 0);
+    std::vector<int8_t> col(n_el);
+    int ncol = 0;
+    for (int64_t e = 0; e < n_el; ++e) {
+        uint64_t used = 0;
+        for (int a = 0; a < 8; ++a) used |= mask[conn_r[8 * e + a]];
+        if (~used == 0) return fail(ctx, ASEND_ERR_UNSUPPORTED, "more than 64 element colours");
+        int c = __builtin_ctzll(~used);
+        col[e] = (int8_t)c;
+        ncol = std::max(ncol, c + 1);
+        for (int a = 0; a < 8; ++a) mask[conn_r[8 * e + a]] |= (1ull << c);
+    }
+    ctx->h_color_ptr.assign(ncol + 1, 0);
+    for (int64_t e = 0; e < n_el; ++e) ctx->h_color_ptr[col[e] + 1]++;
+    for (int c = 0; c < ncol; ++c) ctx->h_color_ptr[c + 1] += ctx->h_color_ptr[c];
+    std::vector<int32_t> elems(n_el);
+    {
+        std::vector<int32_t> fill(ctx->h_color_ptr.begin(), ctx->h_color_ptr.end() - 1);
+        for (int64_t e = 0; e < n_el; ++e) elems[fill[col[e]]++] = (int32_t)e;
+    }
+    int rc;
+    const int64_t nval = ctx->nnzb * 9;
+    if ((rc = dalloc(ctx, &ctx->d_conn_r, 8 * n_el)) || (rc = dalloc(ctx, &ctx->d_rp, n_nodes + 1)) ||
+        (rc = dalloc(ctx, &ctx->d_bc, ctx->nnzb)) || (rc = dalloc(ctx, &ctx->d_kdiag, n_nodes)) ||
+        (rc = dalloc(ctx, &ctx->d_elems, n_el)) || (rc = dalloc(ctx, &ctx->d_slot, 64 * n_el)) ||
+        (rc = dalloc(ctx, &ctx->d_xyz_r, 3 * n_nodes)) || (rc = dalloc(ctx, &ctx->d_E, n_el)) ||
+        (rc = dalloc(ctx, &ctx->d_nu, n_el)) || (rc = dalloc(ctx, &ctx->d_rho, n_el)) ||
+        (rc = dalloc(ctx, &ctx->d_K64, nval)) || (rc = dalloc(ctx, &ctx->d_K32, nval)) ||
+        (rc = dalloc(ctx, &ctx->d_dinv64, n_dof)) || (rc = dalloc(ctx, &ctx->d_dinv32, n_dof)))
+        return rc;
+    HIPCHK(hipMemcpy(ctx->d_conn_r, conn_r.data(), conn_r.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_rp, ctx->h_rp.data(), ctx->h_rp.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_bc, ctx->h_bc.data(), ctx->h_bc.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_kdiag, kdiag.data(), kdiag.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_elems, elems.data(), elems.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_slot, slot.data(), slot.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_xyz_r, xyz_r.data(), xyz_r.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_E, E, n_el * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_nu, nu, n_el * 8, hipMemcpyHostToDevice));
+    if (rho) HIPCHK(hipMemcpy(ctx->d_rho, rho, n_el * 8, hipMemcpyHostToDevice));
+    else HIPCHK(hipMemset(ctx->d_rho, 0, n_el * 8));
+    if ((rc = ensure_work(ctx))) return rc;
+    return ASEND_OK;
+}
+
+int asend_set_grid(asend_ctx* ctx, int64_t nx, int64_t ny, int64_t nz) {
+    if (!ctx) return ASEND_ERR_ARG;
+    if (!ctx->d_K64) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
+    if (nx * ny * nz != ctx->n_el || (nx + 1) * (ny + 1) * (nz + 1) != ctx->n_nodes)
+        return fail(ctx, ASEND_ERR_ARG, "grid %lldx%lldx%lld does not match the mesh", (long long)nx, (long long)ny,
+                    (long long)nz);
+    ctx->grid[0] = nx;
+    ctx->grid[1] = ny;
+    ctx->grid[2] = nz;
+    if (ctx->mg) mg_destroy(ctx->mg);
+    ctx->mg = nullptr;
+    // rank of structured node (i,j,k) -> needed by the transfer operators
+    std::vector<int32_t> rank_of_grid(ctx->n_nodes);
+    std::vector<int32_t> conn_r(8 * ctx->n_el);
+    HIPCHK(hipMemcpy(conn_r.data(), ctx->d_conn_r, conn_r.size() * 4, hipMemcpyDeviceToHost));
+    // element e = i + nx*(j + ny*k) has node 0 at grid (i,j,k), node 6 at (i+1,j+1,k+1)
+    const int corner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+    for (int64_t k = 0; k < nz; ++k)
+        for (int64_t j = 0; j < ny; ++j)
+            for (int64_t i = 0; i < nx; ++i) {
+                int64_t e = i + nx * (j + ny * k);
+                for (int a = 0; a < 8; ++a) {
+                    int64_t gi = i + corner[a][0], gj = j + corner[a][1], gk = k + corner[a][2];
+                    rank_of_grid[gi + (nx + 1) * (gj + (ny + 1) * gk)] = conn_r[8 * e + a];
+                }
+            }
+    int rc = mg_create(&ctx->mg, (int)nx, (int)ny, (int)nz, rank_of_grid.data(), ctx->d_xyz_r, ctx->d_conn_r,
+                       ctx->d_rp, ctx->d_bc, ctx->d_slot, ctx->h_rp.data(), ctx->h_bc.data(), ctx->n_nodes,
+                       ctx->n_el);
+    if (rc) {
+        std::string m = ctx->mg ? mg_last_error(ctx->mg) : "multigrid create failed";
+        if (ctx->mg) mg_destroy(ctx->mg);
+        ctx->mg = nullptr;
+        return fail(ctx, rc, "%s", m.c_str());
+    }
+    return ASEND_OK;
+}
+
+int asend_update_moduli(asend_ctx* ctx, const double* E) {
+    if (!ctx || !E) return ASEND_ERR_ARG;
+    if (!ctx->d_E) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
+    HIPCHK(hipMemcpyAsync(ctx->d_E, E, ctx->n_el * 8, hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    ctx->assembled = false;
+    return ASEND_OK;
+}
+
+int asend_set_penalty(asend_ctx* ctx, int64_t n, const int64_t* dofs, double sf) {
+    if (!ctx || n < 0 || (n > 0 && !dofs)) return ASEND_ERR_ARG;
+    if (!ctx->d_K64) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
+    std::vector<int64_t> v(dofs, dofs + n);
+    for (int64_t d : v)
+        if (d < 0 || d >= ctx->n_dof) return fail(ctx, ASEND_ERR_ARG, "penalty DOF out of range");
+    std::sort(v.begin(), v.end());
+    std::vector<int64_t> u;
+    std::vector<int32_t> m;
+    for (size_t i = 0; i < v.size(); ++i) {
+        if (!u.empty() && u.back() == v[i]) m.back()++;
+        else {
+            u.push_back(v[i]);
+            m.push_back(1);
+        }
+    }
+    int rc;
+    if ((rc = dalloc(ctx, &ctx->d_pen, u.size())) || (rc = dalloc(ctx, &ctx->d_pen_mult, m.size()))) return rc;
+    if (!u.empty()) {
+        HIPCHK(hipMemcpy(ctx->d_pen, u.data(), u.size() * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ctx->d_pen_mult, m.data(), m.size() * 4, hipMemcpyHostToDevice));
+    }
+    ctx->n_pen = (int)u.size();
+    ctx->sf_user = sf;
+    ctx->sf_fixed = false;
+    ctx->assembled = false;
+    return ASEND_OK;
+}
+
+double asend_penalty_scale(const asend_ctx* ctx) { return ctx ? ctx->sf : 0.0; }
+
+int asend_assemble(asend_ctx* ctx, double c_mass) {
+    if (!ctx) return ASEND_ERR_ARG;
+    if (!ctx->d_K64) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
+    int rc = assemble_device(ctx, c_mass);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return ASEND_OK;
+}
+
+int asend_get_matrix(asend_ctx* ctx, int64_t* rowptr, int32_t* cols, double* vals, int64_t* nnz, int with_penalty) {
+    if (!ctx || !nnz) return ASEND_ERR_ARG;
+    if (!ctx->assembled) return fail(ctx, ASEND_ERR_STATE, "assemble first");
+    const int64_t n_nodes = ctx->n_nodes;
+    *nnz = ctx->nnzb * 9;
+    if (!rowptr) return ASEND_OK;
+    for (int64_t r = 0; r < n_nodes; ++r) {
+        int64_t nb = ctx->h_rp[r + 1] - ctx->h_rp[r];
+        for (int i = 0; i < 3; ++i) rowptr[3 * r + i] = 9 * (int64_t)ctx->h_rp[r] + 3 * nb * i;
+    }
+    rowptr[3 * n_nodes] = 9 * ctx->nnzb;
+    if (!cols || !vals) return ASEND_OK;
+    std::vector<double> K(9 * ctx->nnzb);
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    HIPCHK(hipMemcpy(K.data(), ctx->d_K64, K.size() * 8, hipMemcpyDeviceToHost));
+    for (int64_t r = 0; r < n_nodes; ++r) {
+        int64_t s = ctx->h_rp[r], nb = ctx->h_rp[r + 1] - s;
+        for (int i = 0; i < 3; ++i) {
+            int64_t o = rowptr[3 * r + i];
+            for (int64_t k = 0; k < nb; ++k)
+                for (int j = 0; j < 3; ++j) {
+                    cols[o + 3 * k + j] = 3 * ctx->h_bc[s + k] + j;
+                    vals[o + 3 * k + j] = K[9 * s + (3 * i + j) * nb + k];
+                }
+        }
+    }
+    if (!with_penalty && ctx->n_pen > 0) {
+        std::vector<int64_t> pen(ctx->n_pen);
+        std::vector<int32_t> mult(ctx->n_pen);
+        HIPCHK(hipMemcpy(pen.data(), ctx->d_pen, pen.size() * 8, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(mult.data(), ctx->d_pen_mult, mult.size() * 4, hipMemcpyDeviceToHost));
+        for (int i = 0; i < ctx->n_pen; ++i) {
+            int64_t d = pen[i];
+            for (int64_t k = rowptr[d]; k < rowptr[d + 1]; ++k)
+                if (cols[k] == d) vals[k] -= mult[i] * ctx->sf;
+        }
+    }
+    return ASEND_OK;
+}
+
+int asend_spmv(asend_ctx* ctx, int precision, const double* x, double* y) {
+    if (!ctx || !x || !y) return ASEND_ERR_ARG;
+    if (!ctx->assembled) return fail(ctx, ASEND_ERR_STATE, "assemble first");
+    const int64_t n = ctx->n_dof;
+    if (precision == 64) {
+        HIPCHK(hipMemcpy(ctx->d_p, x, n * 8, hipMemcpyHostToDevice));
+        launch_spmv3<double>(ctx->st, 0, (int)ctx->n_nodes, ctx->d_rp, ctx->d_bc, ctx->d_K64, ctx->d_p, ctx->d_q,
+                             nullptr, nullptr, nullptr, nullptr);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(ctx->st));
+        HIPCHK(hipMemcpy(y, ctx->d_q, n * 8, hipMemcpyDeviceToHost));
+    } else if (precision == 32) {
+        std::vector<float> xf(x, x + n), yf(n);
+        HIPCHK(hipMemcpy(ctx->d_p32, xf.data(), n * 4, hipMemcpyHostToDevice));
+        launch_spmv3<float>(ctx->st, 0, (int)ctx->n_nodes, ctx->d_rp, ctx->d_bc, ctx->d_K32, ctx->d_p32, ctx->d_q32,
+                            nullptr, nullptr, nullptr, nullptr);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(ctx->st));
+        HIPCHK(hipMemcpy(yf.data(), ctx->d_q32, n * 4, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n; ++i) y[i] = yf[i];
+    } else {
+        return fail(ctx, ASEND_ERR_ARG, "precision must be 32 or 64");
+    }
+    return ASEND_OK;
+}
+
+int asend_solve(asend_ctx* ctx, const double* rhs, double* x, const asend_solve_opts* opts, asend_solve_stats* stats) {
+    if (!ctx || !rhs || !x || !opts) return ASEND_ERR_ARG;
+    if (!ctx->assembled) return fail(ctx, ASEND_ERR_STATE, "assemble first");
+    const int64_t n = ctx->n_dof;
+    HIPCHK(hipMemcpy(ctx->d_f, rhs, n * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->st));
+    asend_solve_stats s;
+    int rc = solve_device(ctx, opts, &s);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->st));
+    HIPCHK(hipEventSynchronize(ctx->ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    s.device_ms = ms;
+    HIPCHK(hipMemcpy(x, ctx->d_u, n * 8, hipMemcpyDeviceToHost));
+    if (stats) *stats = s;
+    return ASEND_OK;
+}
+
+int asend_element_sens(asend_ctx* ctx, const double* u, const double* lam, double* dJdE) {
+    if (!ctx || !u || !lam || !dJdE) return ASEND_ERR_ARG;
+    if (!ctx->d_E) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
+    const int64_t n = ctx->n_dof;
+    HIPCHK(hipMemcpy(ctx->d_x64, u, n * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_lam, lam, n * 8, hipMemcpyHostToDevice));
+    launch_hex8_sens(ctx->st, (int)ctx->n_el, ctx->d_conn_r, ctx->d_xyz_r, ctx->d_E, ctx->d_nu, ctx->d_x64,
+                     ctx->d_lam, ctx->d_sens);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    HIPCHK(hipMemcpy(dJdE, ctx->d_sens, ctx->n_el * 8, hipMemcpyDeviceToHost));
+    return ASEND_OK;
+}
+
+int asend_static_compliance(asend_ctx* ctx, const double* f, double* u, double* dJdE, double* compliance,
+                            const asend_solve_opts* opts, asend_solve_stats* stats) {
+    if (!ctx || !f || !opts) return ASEND_ERR_ARG;
+    if (!ctx->d_K64) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
+    const int64_t n = ctx->n_dof;
+    HIPCHK(hipMemcpy(ctx->d_f, f, n * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->st));
+    int rc = assemble_device(ctx, 0.0);
+    if (rc) return rc;
+    asend_solve_stats s;
+    if ((rc = solve_device(ctx, opts, &s))) return rc;
+    // compliance J = f.u ; adjoint rhs dJ/du = f, so lambda = u (same operator)
+    launch_dot64(ctx->st, n, ctx->d_f, ctx->d_u, ctx->d_partials, ctx->d_counter, ctx->d_sc + 2);
+    launch_hex8_sens(ctx->st, (int)ctx->n_el, ctx->d_conn_r, ctx->d_xyz_r, ctx->d_E, ctx->d_nu, ctx->d_u, ctx->d_u,
+                     ctx->d_sens);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->st));
+    HIPCHK(hipEventSynchronize(ctx->ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    s.device_ms = ms;
+    if ((rc = read_scalars(ctx, 2))) return rc;
+    if (compliance) *compliance = ctx->h_sc[2].dot;
+    if (u) HIPCHK(hipMemcpy(u, ctx->d_u, n * 8, hipMemcpyDeviceToHost));
+    if (dJdE) HIPCHK(hipMemcpy(dJdE, ctx->d_sens, ctx->n_el * 8, hipMemcpyDeviceToHost));
+    if (stats) *stats = s;
+    return ASEND_OK;
+}
+
+int asend_bench_spmv(asend_ctx* ctx, int precision, int n_iter, double* avg_ms) {
+    if (!ctx || n_iter <= 0 || !avg_ms) return ASEND_ERR_ARG;
+    if (!ctx->assembled) return fail(ctx, ASEND_ERR_STATE, "assemble first");
+    const int nr = (int)ctx->n_nodes;
+    // warm-up launch, then n_iter timed launches between two events
+    for (int pass = 0; pass < 2; ++pass) {
+        int cnt = pass == 0 ? 1 : n_iter;
+        if (pass == 1) HIPCHK(hipEventRecord(ctx->ev0, ctx->st));
+        for (int i = 0; i < cnt; ++i) {
+            if (precision == 64)
+                launch_spmv3<double>(ctx->st, 0, nr, ctx->d_rp, ctx->d_bc, ctx->d_K64, ctx->d_u, ctx->d_q, nullptr,
+                                     nullptr, nullptr, nullptr);
+            else
+                launch_spmv3<float>(ctx->st, 0, nr, ctx->d_rp, ctx->d_bc, ctx->d_K32, ctx->d_x32, ctx->d_q32,
+                                    nullptr, nullptr, nullptr, nullptr);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->st));
+    HIPCHK(hipEventSynchronize(ctx->ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    *avg_ms = ms / n_iter;
+    return ASEND_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,74 @@
+"""Synthetic structured meshes for the solve-path workloads.
+
+Node and element label order follows the reference's swept-mesh generator:
+nodes x-fastest, then y, then z (P/meshing/Mesh3D.py:111-136); hex nodes in the
+brick8 local order (-1,-1,-1), (1,-1,-1), (1,1,-1), (-1,1,-1), then the same at
+z = +1 (S/element_meth.rs:207-231).
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass
+class HexBlock:
+    nx: int
+    ny: int
+    nz: int
+    lx: float
+    ly: float
+    lz: float
+    xyz: np.ndarray = field(repr=False)
+    conn: np.ndarray = field(repr=False)
+
+    @property
+    def n_nodes(self):
+        return self.xyz.shape[0]
+
+    @property
+    def n_el(self):
+        return self.conn.shape[0]
+
+    def node_id(self, i, j, k):
+        return i + (self.nx + 1) * (j + (self.ny + 1) * k)
+
+    def nodes_at_x(self, i):
+        """Node labels of the plane x-index i, in label order."""
+        j, k = np.meshgrid(np.arange(self.ny + 1), np.arange(self.nz + 1), indexing="xy")
+        ids = self.node_id(i, j.reshape(-1), k.reshape(-1))
+        return np.sort(ids).astype(np.int32)
+
+    def elements_at_x(self, ex):
+        ey, ez = np.meshgrid(np.arange(self.ny), np.arange(self.nz), indexing="xy")
+        ids = ex + self.nx * (ey.reshape(-1) + self.ny * ez.reshape(-1))
+        return np.sort(ids).astype(np.int32)
+
+
+def hex_block(nx, ny, nz, lx, ly, lz):
+    """Structured brick8 block [0,lx]x[0,ly]x[0,lz] with nx*ny*nz elements."""
+    xs = np.arange(nx + 1, dtype=np.float64) * (lx / nx)
+    ys = np.arange(ny + 1, dtype=np.float64) * (ly / ny)
+    zs = np.arange(nz + 1, dtype=np.float64) * (lz / nz)
+    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+    xyz = np.stack([X.reshape(-1), Y.reshape(-1), Z.reshape(-1)], axis=1)
+    ez, ey, ex = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    ex = ex.reshape(-1)
+    ey = ey.reshape(-1)
+    ez = ez.reshape(-1)
+
+    def nid(i, j, k):
+        return i + (nx + 1) * (j + (ny + 1) * k)
+
+    conn = np.stack([nid(ex, ey, ez), nid(ex + 1, ey, ez), nid(ex + 1, ey + 1, ez), nid(ex, ey + 1, ez),
+                     nid(ex, ey, ez + 1), nid(ex + 1, ey, ez + 1), nid(ex + 1, ey + 1, ez + 1),
+                     nid(ex, ey + 1, ez + 1)], axis=1).astype(np.int32)
+    return HexBlock(nx, ny, nz, lx, ly, lz, np.ascontiguousarray(xyz), np.ascontiguousarray(conn))
+
+
+def block_dim_auto(n_nodes):
+    """Iterative-solver block size: smallest multiple of 6 whose square reaches
+    the node count (S/model_prep.rs:888-902)."""
+    b = 6
+    while b * b < n_nodes:
+        b += 6
+    return b

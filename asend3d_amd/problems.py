@@ -1,0 +1,82 @@
+"""Static cantilever workloads of the solve path, built with the library's
+host-side preparation (node reordering, traction loads, seeded moduli).
+
+Clamp: one single-DOF penalty constraint per component on the x = 0 node set
+(three constraints, as the reference input writes a clamp,
+S/const_meth.rs:30-90).  Load: uniform surface traction on the x = lx face
+elements with normDir (1,0,0) and normTol 5 degrees
+(S/element_equations.rs:1833-2056).
+"""
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import api
+from .meshgen import HexBlock, hex_block
+
+C2_SEED = 20261202
+
+
+@dataclass
+class StaticProblem:
+    mesh: HexBlock
+    etype: int
+    E: np.ndarray
+    nu: np.ndarray
+    block_dim: int
+    ordered: np.ndarray
+    dof_index: np.ndarray
+    n_dof: int
+    clamp_nodes: np.ndarray
+    pen_dofs: np.ndarray
+    tip_elements: np.ndarray
+    traction: tuple
+    f: np.ndarray
+
+    @property
+    def constraints(self):
+        return [[self.clamp_nodes], [self.clamp_nodes], [self.clamp_nodes]]
+
+
+def cantilever(nx, ny, nz, lx=1.0, ly=0.2, lz=0.2, E=200e9, nu=0.3, traction=(0.0, 0.0, -1.0e6), etype=8,
+               block_dim=None):
+    m = hex_block(nx, ny, nz, lx, ly, lz)
+    if np.isscalar(E):
+        E = np.full(m.n_el, float(E))
+    E = np.ascontiguousarray(E, dtype=np.float64)
+    nu_a = np.full(m.n_el, float(nu)) if np.isscalar(nu) else np.ascontiguousarray(nu, dtype=np.float64)
+    clamp = m.nodes_at_x(0)
+    bd = api.block_dim_auto(m.n_nodes) if block_dim is None else block_dim
+    constraints = [[clamp], [clamp], [clamp]]
+    ordered, di, n_dof = api.reorder_nodes(m.xyz, m.conn, 3, constraints, bd)
+    pen = np.concatenate([di[clamp, 0], di[clamp, 1], di[clamp, 2]]).astype(np.int64)
+    tip = m.elements_at_x(nx - 1)
+    f = api.surface_traction(etype, m.xyz, m.conn, di, n_dof, tip, traction, (1.0, 0.0, 0.0), 5.0)
+    return StaticProblem(m, etype, E, nu_a, bd, ordered, di, n_dof, clamp, pen, tip, tuple(traction), f)
+
+
+def config1(etype=8):
+    """20x4x4 brick8 cantilever, 1.0 x 0.2 x 0.2 m, E = 200 GPa, nu = 0.3."""
+    return cantilever(20, 4, 4, E=200e9, nu=0.3, etype=etype)
+
+
+def heterogeneous(nx, ny, nz, seed=C2_SEED, median=70e3, sigma_ln=1.0, nu=0.3, lx=1000.0, ly=200.0, lz=200.0,
+                  traction=(0.0, 0.0, -1.0)):
+    """Lognormal per-element moduli, posed in mm-N-MPa (median 70e3 MPa =
+    70 GPa, traction 1 MPa).  In these units 1/E stays far above the 1e-12
+    pivot clamp of the reference's QR inverse (S/matrix_functions.rs:189-191),
+    which in SI units would corrupt the material matrix of the stiff tail
+    (E > ~7e11 Pa) of a sigma_ln = 1 distribution."""
+    n_el = nx * ny * nz
+    E = api.lognormal_moduli(seed, n_el, median, sigma_ln)
+    return cantilever(nx, ny, nz, lx, ly, lz, E=E, nu=nu, traction=traction)
+
+
+def config2():
+    """200x40x40 brick8 block (337,881 nodes, 1,013,643 DOF), lognormal E."""
+    return heterogeneous(200, 40, 40)
+
+
+def upload(ctx, prob, rho=None):
+    ctx.upload_mesh(prob.etype, prob.mesh.xyz, prob.dof_index, prob.n_dof, prob.mesh.conn, prob.E, prob.nu, rho)
+    ctx.set_penalty(prob.pen_dofs, 0.0)

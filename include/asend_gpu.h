@@ -1,0 +1,206 @@
+/*
+ * asend_gpu.h -- C ABI of the MI355X (gfx950) solve path for ASenD3D models.
+ *
+ * The reference has no plugin/FFI layer: its hot path is the in-process call
+ * chain Model::solve -> solve_step -> build_elastic_soln_load / get_ru_dfd0 ->
+ * conj_grad_sparse, and Model::get_obj_gradient -> solve_for_adjoint ->
+ * d_relasticd_d (reference paths below are relative to the reference checkout,
+ * S/ = solverSrc/asend_run_job/src/).  Each entry point here replaces one link
+ * of that chain and is what a thin Rust FFI crate (bindgen over this header)
+ * would bind in place of the Rust routine named in its comment.
+ *
+ * Conventions
+ *   - Plain C types only; every array is a host pointer owned by the caller,
+ *     read-only unless documented as output.  No pointer is retained past the
+ *     call.  The context owns every device buffer.
+ *   - Node coordinates xyz[3*n + c]; element connectivity conn[nn*e + a] in
+ *     the reference's local node order; dof_index[6*n + d] = global DOF of
+ *     node n, component d (-1 when the node has fewer DOFs), exactly as
+ *     produced by asend_reorder_nodes.
+ *   - Global vectors are in the reference's DOF numbering.
+ *   - Every function returns ASEND_OK (0) or a negative ASEND_ERR_* code and
+ *     never throws across the ABI; asend_last_error() gives the message.
+ *     Non-convergence is not an error: the call returns ASEND_OK and sets
+ *     stats->converged = 0 (the reference only warns, S/matrix_functions.rs:354-356).
+ *   - A context is not thread-safe; all its device work runs on one HIP
+ *     stream created at default priority.
+ */
+#ifndef ASEND_GPU_H
+#define ASEND_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ASEND_OK 0
+#define ASEND_ERR_ARG (-1)
+#define ASEND_ERR_HIP (-2)
+#define ASEND_ERR_STATE (-3)
+#define ASEND_ERR_UNSUPPORTED (-4)
+#define ASEND_ERR_NUMERIC (-5)
+
+/* element type codes of the reference (S/model_input.rs:405-423) */
+#define ASEND_BRICK8 8
+#define ASEND_BRICKIM 81
+#define ASEND_SHELL4 41
+
+/* solver methods */
+#define ASEND_SOLVE_PCG_FP64 0    /* FP64 PCG on the assembled FP64 operator          */
+#define ASEND_SOLVE_IR_MIXED 1    /* FP32 flexible PCG inside FP64 iterative refinement */
+
+/* preconditioners */
+#define ASEND_PC_JACOBI 0         /* point Jacobi                                       */
+#define ASEND_PC_BLOCK_JACOBI 1   /* inverse of the per-node diagonal blocks            */
+#define ASEND_PC_MG 2             /* geometric multigrid, Galerkin coarse operators     */
+#define ASEND_PC_LEARNED 3        /* multigrid whose fine smoother is the learned L L^T */
+
+/* stopping rules */
+#define ASEND_STOP_REL_RESIDUAL 0 /* ||f - K u||_2 / ||f||_2 <= tol, evaluated in FP64 */
+#define ASEND_STOP_REF_ABS 1      /* g^T M^-1 g <= tol (conj_grad_sparse rule,
+                                     S/matrix_functions.rs:328), FP64 PCG only       */
+
+typedef struct asend_ctx asend_ctx;
+
+typedef struct asend_solve_opts {
+    int method;           /* ASEND_SOLVE_*                                   */
+    int precond;          /* ASEND_PC_*                                      */
+    int stop_rule;        /* ASEND_STOP_*                                    */
+    int check_every;      /* iterations between convergence checks (>=1)    */
+    double tol;           /* outer tolerance for the stop rule               */
+    double inner_tol;     /* relative residual drop per inner FP32 solve     */
+    int64_t max_it;       /* max total (inner) iterations; <=0: n_dof        */
+    int64_t max_outer;    /* max refinement steps (mixed)                    */
+} asend_solve_opts;
+
+typedef struct asend_solve_stats {
+    int64_t iterations;       /* total Krylov iterations (inner for mixed)   */
+    int64_t outer_iterations; /* refinement steps (mixed), 1 for FP64        */
+    double rel_residual;      /* final ||f - K u|| / ||f|| in FP64            */
+    double ref_residual;      /* final g^T M^-1 g of the last Krylov loop     */
+    int converged;            /* 1 when the stop rule was met                */
+    int fallback_used;        /* 1 when the learned preconditioner was
+                                 rejected and Jacobi PCG took over          */
+    double device_ms;         /* HIP-event time of the solve on the stream   */
+} asend_solve_stats;
+
+/* ---- context ------------------------------------------------------------ */
+int asend_ctx_create(int device, asend_ctx** out);
+void asend_ctx_destroy(asend_ctx* ctx);
+const char* asend_last_error(const asend_ctx* ctx);
+const char* asend_version(void);
+/* Block until all work queued on the context's stream has finished. */
+int asend_sync(asend_ctx* ctx);
+
+/* ---- host-side model preparation (no device work) ----------------------- */
+
+/* Replaces Model::reorder_nodes + Model::add_constraint_conn
+ * (S/model_prep.rs:56-303, :20-54): level-order node insertion from the
+ * least-connected node with a restart at the nearest uninserted node every
+ * block_dim nodes; DOFs numbered node by node in insertion order.
+ * Constraint c has terms [cterm_ptr[c], cterm_ptr[c+1]); term t's node set is
+ * set_nodes[term_set_ptr[t] .. term_set_ptr[t+1]).
+ * Out: ordered[n_nodes], dof_index[6*n_nodes], *n_dof_out. */
+int asend_reorder_nodes(int64_t n_nodes, const double* xyz, int64_t n_el, int nn, int dof_per_nd,
+                        const int32_t* conn, int64_t n_const, const int64_t* cterm_ptr,
+                        const int64_t* term_set_ptr, const int32_t* set_nodes, int64_t block_dim,
+                        int32_t* ordered, int64_t* dof_index, int64_t* n_dof_out);
+
+/* Replaces the iterative block-size rule of Model::analysis_prep
+ * (S/model_prep.rs:888-902). */
+int64_t asend_block_dim_auto(int64_t n_nodes);
+
+/* Replaces the surfaceTraction branch of Element::get_app_load_dfd0
+ * (S/element_equations.rs:1833-2056) with find_surface_faces
+ * (S/model_prep.rs:510-554): accumulates (+=) the nodal loads of a uniform
+ * traction on the surface faces of el_set whose outward normal lies within
+ * norm_tol_deg of norm_dir.  Solid hex types only. */
+int asend_surface_traction(int elem_type, int64_t n_nodes, const double* xyz, int64_t n_el,
+                           const int32_t* conn, const int64_t* dof_index, int64_t n_set,
+                           const int32_t* el_set, const double* load, const double* norm_dir,
+                           double norm_tol_deg, int64_t n_dof, double* f);
+
+/* Seeded per-element moduli E_e = median * exp(sigma_ln * z_e), z_e standard
+ * normal from splitmix64(seed) + Box-Muller (cosine branch, one pair per
+ * element).  Input synthesis for heterogeneous designs; no reference
+ * counterpart (the reference takes per-element values through design
+ * variables, S/model_input.rs:1377-1401). */
+int asend_lognormal_moduli(uint64_t seed, int64_t n, double median, double sigma_ln, double* out);
+
+/* ---- device model ------------------------------------------------------- */
+
+/* Upload a homogeneous mesh: the replacement for the element/node arrays that
+ * Model::analysis_prep hands to build_elastic_soln_load
+ * (S/model_prep.rs:881-952, S/model_analysis.rs:179-201).  E, nu, rho are per
+ * element (isotropic; G = E/(2(1+nu)) as P/model/Material.py:24-26 writes
+ * it).  rho may be NULL for static work.  Builds the node-block sparsity
+ * pattern, the element colouring and the element-to-block map. */
+int asend_upload_mesh(asend_ctx* ctx, int elem_type, int64_t n_nodes, const double* xyz,
+                      const int64_t* dof_index, int64_t n_dof, int64_t n_el, const int32_t* conn,
+                      const double* E, const double* nu, const double* rho);
+
+/* Optional structured-grid description (element (i,j,k) is label
+ * i + nx*(j + ny*k), nodes likewise with nx+1, ny+1), used by the multigrid
+ * preconditioners.  No reference counterpart. */
+int asend_set_grid(asend_ctx* ctx, int64_t nx, int64_t ny, int64_t nz);
+
+/* Replace the per-element moduli on the device (design update). */
+int asend_update_moduli(asend_ctx* ctx, const double* E);
+
+/* Penalty constraints of single-DOF clamp type: K_eff = K + sf * sum C^T C
+ * with one unit entry per constrained DOF (Constraint::build_mat,
+ * S/const_meth.rs:30-90; get_total_vec_mult :161-168).  sf <= 0 selects the
+ * reference rule sf = 1e5 * max|K_ij| evaluated on the next assembly
+ * (Model::scale_const, S/model_analysis.rs:255-259). */
+int asend_set_penalty(asend_ctx* ctx, int64_t n, const int64_t* dofs, double sf);
+double asend_penalty_scale(const asend_ctx* ctx);
+
+/* Element integration + colored atomic-free assembly of the elastic operator:
+ * replaces build_elastic_soln_load(true) with get_ruk_dfd0 / get_ru_dfd0
+ * (S/model_analysis.rs:179-201, S/element_equations.rs:206-418, :852-1062).
+ * c_mass != 0 adds c_mass * M (consistent mass) for the Newmark operator
+ * K + M/(dt^2 (gamma - beta)) (S/element_equations.rs:977-1005).
+ * Produces the FP64 operator (penalty folded into its diagonal), its FP32
+ * copy and the Jacobi diagonal. */
+int asend_assemble(asend_ctx* ctx, double c_mass);
+
+/* Download the assembled FP64 operator as scalar CSR (columns ascending):
+ * rowptr[n_dof+1]; pass cols = vals = NULL to query *nnz only. */
+int asend_get_matrix(asend_ctx* ctx, int64_t* rowptr, int32_t* cols, double* vals, int64_t* nnz,
+                     int with_penalty);
+
+/* y = K_eff x with the FP64 (precision 64) or FP32 (precision 32) operator:
+ * SparseMat::vector_multiply + ConstraintList::get_total_vec_mult
+ * (S/list_ent_meth.rs:64-79, S/const_meth.rs:92-103). */
+int asend_spmv(asend_ctx* ctx, int precision, const double* x, double* y);
+
+/* Solve K_eff x = rhs: replaces conj_grad_sparse (S/matrix_functions.rs:285-362)
+ * and the direct ldl_solve branch of solve_step (S/model_analysis.rs:495-501). */
+int asend_solve(asend_ctx* ctx, const double* rhs, double* x, const asend_solve_opts* opts,
+                asend_solve_stats* stats);
+
+/* Compliance adjoint sensitivities for per-element modulus design variables:
+ * dJdE[e] = -lambda_e^T (dK_e/dE_e) u_e, FP64 reduction.  Replaces the DV
+ * loop of Model::get_obj_gradient with d_relasticd_d
+ * (S/model_analysis.rs:1679-1701, :1379-1457).  brick8 only. */
+int asend_element_sens(asend_ctx* ctx, const double* u, const double* lam, double* dJdE);
+
+/* The static compliance pipeline on resident data: assemble, solve K u = f,
+ * compliance J = f^T u, adjoint K lambda = dJ/du (= f, so lambda = u without a
+ * second solve when the objective is the compliance), dJ/dE.  Replaces
+ * Model::solve (static) + Model::get_obj_gradient for a compliance objective
+ * (S/model_analysis.rs:560-673, :1544-1706).  device_ms in stats covers the
+ * whole pipeline. */
+int asend_static_compliance(asend_ctx* ctx, const double* f, double* u, double* dJdE, double* compliance,
+                            const asend_solve_opts* opts, asend_solve_stats* stats);
+
+/* Timing helper: average duration (ms) of n_iter back-to-back launches of the
+ * SpMV kernel of the given precision, HIP events on the context stream. */
+int asend_bench_spmv(asend_ctx* ctx, int precision, int n_iter, double* avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ASEND_GPU_H */

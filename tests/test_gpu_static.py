@@ -1,0 +1,141 @@
+"""Device parity of the static solve path against the CPU oracle.
+Every device call goes through the C ABI (libasend_gpu.so)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from asend3d_amd import api, problems
+from conftest import gpu_case
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_system(p):
+    K = O.assemble_solid(8, p.mesh.xyz, p.mesh.conn, p.E, p.nu, p.dof_index, p.n_dof)
+    sf = 1e5 * O.max_abs(K)
+    return K, sf
+
+
+def small_problems():
+    yield "C1", problems.config1()
+    yield "het-14x5x4", problems.heterogeneous(14, 5, 4, seed=5)
+
+
+@pytest.mark.parametrize("which", [0, 1])
+def test_assembly_matches_oracle(ctx, which):
+    name, p = list(small_problems())[which]
+    gpu_case(f"assembly {name}")
+    problems.upload(ctx, p)
+    ctx.assemble()
+    K, sf = oracle_system(p)
+    rp, cols, vals = ctx.get_matrix(with_penalty=False)
+    # identical sparsity (DOF numbering and connectivity are bit-exact)
+    assert np.array_equal(rp, K.rowptr)
+    assert np.array_equal(cols, K.cols)
+    scale = np.abs(K.vals).max()
+    assert np.abs(vals - K.vals).max() <= 1e-12 * scale
+    assert abs(ctx.penalty_scale() - sf) <= 1e-12 * sf
+
+
+def test_spmv_matches_oracle(ctx):
+    name, p = list(small_problems())[1]
+    gpu_case(f"spmv {name}")
+    problems.upload(ctx, p)
+    ctx.assemble()
+    K, sf = oracle_system(p)
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(p.n_dof)
+    y_ref = O.spmv_penalty(K, p.pen_dofs, ctx.penalty_scale(), x)
+    y64 = ctx.spmv(x, 64)
+    assert np.abs(y64 - y_ref).max() <= 1e-12 * np.abs(y_ref).max()
+    y32 = ctx.spmv(x, 32)
+    assert np.abs(y32 - y_ref).max() <= 1e-5 * np.abs(y_ref).max()
+
+
+@pytest.mark.parametrize("which", [0, 1])
+def test_fp64_pcg_matches_ldl(ctx, which):
+    name, p = list(small_problems())[which]
+    gpu_case(f"fp64 pcg {name}")
+    problems.upload(ctx, p)
+    ctx.assemble()
+    K, sf = oracle_system(p)
+    u_ref = O.LDL(K, p.pen_dofs, sf, p.n_dof).solve(p.f)
+    u, st = ctx.solve(p.f, ctx.opts(api.SOLVE_PCG_FP64, api.PC_JACOBI, tol=1e-13, max_it=200000))
+    assert st.converged and st.rel_residual <= 1e-13
+    assert np.abs(u - u_ref).max() <= 1e-10 * np.abs(u_ref).max()
+
+
+def test_mixed_ir_jacobi_reaches_fp64_residual(ctx):
+    name, p = list(small_problems())[1]
+    gpu_case(f"mixed IR {name}")
+    problems.upload(ctx, p)
+    ctx.assemble()
+    K, sf = oracle_system(p)
+    u_ref = O.LDL(K, p.pen_dofs, sf, p.n_dof).solve(p.f)
+    u, st = ctx.solve(p.f, ctx.opts(api.SOLVE_IR_MIXED, api.PC_JACOBI, tol=1e-10, inner_tol=1e-4, max_it=400000))
+    assert st.converged and st.rel_residual <= 1e-10
+    assert st.outer_iterations >= 2  # FP32 inner alone cannot reach 1e-10
+    assert np.abs(u - u_ref).max() <= 1e-7 * np.abs(u_ref).max()
+    r = p.f - O.spmv_penalty(K, p.pen_dofs, sf, u)
+    assert np.linalg.norm(r) <= 1.01e-10 * np.linalg.norm(p.f)
+
+
+def test_reference_stop_rule(ctx):
+    # g^T M^-1 g <= conv_tol (absolute), S/matrix_functions.rs:328
+    name, p = list(small_problems())[0]
+    gpu_case(f"abs stop rule {name}")
+    problems.upload(ctx, p)
+    ctx.assemble()
+    u, st = ctx.solve(p.f, ctx.opts(api.SOLVE_PCG_FP64, api.PC_JACOBI, stop_rule=api.STOP_REF_ABS, tol=1e-12,
+                                    max_it=100000, check_every=1))
+    assert st.converged and st.ref_residual <= 1e-12
+    u1, st1 = ctx.solve(p.f, ctx.opts(api.SOLVE_PCG_FP64, api.PC_JACOBI, stop_rule=api.STOP_REF_ABS, tol=1e-30,
+                                      max_it=3, check_every=1))
+    assert st1.iterations == 3 and not st1.converged
+
+
+def test_sensitivity_matches_dual_number_oracle(ctx):
+    name, p = list(small_problems())[1]
+    gpu_case(f"sens {name}")
+    problems.upload(ctx, p)
+    ctx.assemble()
+    K, sf = oracle_system(p)
+    u = O.LDL(K, p.pen_dofs, sf, p.n_dof).solve(p.f)
+    rng = np.random.default_rng(4)
+    lam = u + 0.3 * np.abs(u).max() * rng.standard_normal(u.shape)
+    s_ref = O.solid_sens(8, p.mesh.xyz, p.mesh.conn, p.E, p.nu, p.dof_index, u, lam)
+    s = ctx.element_sens(u, lam)
+    assert np.abs(s - s_ref).max() <= 1e-10 * np.abs(s_ref).max()
+
+
+@pytest.mark.parametrize("method", [api.SOLVE_PCG_FP64, api.SOLVE_IR_MIXED])
+def test_compliance_pipeline(ctx, method):
+    name, p = list(small_problems())[1]
+    gpu_case(f"compliance pipeline {name} method {method}")
+    problems.upload(ctx, p)
+    K, sf = oracle_system(p)
+    u_ref = O.LDL(K, p.pen_dofs, sf, p.n_dof).solve(p.f)
+    J_ref = p.f @ u_ref
+    s_ref = O.solid_sens(8, p.mesh.xyz, p.mesh.conn, p.E, p.nu, p.dof_index, u_ref, u_ref)
+    tol = 1e-12 if method == api.SOLVE_PCG_FP64 else 1e-9
+    u, s, J, st = ctx.static_compliance(p.f, ctx.opts(method, api.PC_JACOBI, tol=tol, max_it=400000))
+    assert st.converged
+    etol = 1e-10 if method == api.SOLVE_PCG_FP64 else 1e-6
+    assert abs(J - J_ref) <= etol * abs(J_ref)
+    assert np.abs(u - u_ref).max() <= etol * np.abs(u_ref).max()
+    assert np.abs(s - s_ref).max() <= etol * np.abs(s_ref).max()
+    assert st.device_ms > 0
+
+
+def test_assembly_deterministic(ctx):
+    name, p = list(small_problems())[1]
+    gpu_case(f"determinism {name}")
+    problems.upload(ctx, p)
+    ctx.assemble()
+    a = ctx.get_matrix()[2]
+    ctx.assemble()
+    b = ctx.get_matrix()[2]
+    assert np.array_equal(a, b)
+    u1, _ = ctx.solve(p.f, ctx.opts(tol=1e-10, max_it=100000))
+    u2, _ = ctx.solve(p.f, ctx.opts(tol=1e-10, max_it=100000))
+    assert np.array_equal(u1, u2)

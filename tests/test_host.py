@@ -1,0 +1,95 @@
+"""Host-side product code: library exports, node reordering, traction loads,
+seeded moduli.  CPU only (no device calls)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+from asend3d_amd import api
+from asend3d_amd.meshgen import hex_block
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "asend_gpu.h")).read()
+    declared = set(re.findall(r"\b(asend_[a-z0-9_]+)\s*\(", hdr))
+    assert declared, "no declarations parsed"
+    lib = api.lib()
+    for name in sorted(declared):
+        assert hasattr(lib, name), f"{name} declared but not exported"
+    assert set(api.EXPORTED) <= declared
+    assert "gfx950" in api.version()
+
+
+@pytest.mark.parametrize("dims,bd", [((20, 4, 4), None), ((13, 7, 3), None), ((9, 9, 9), 12), ((40, 8, 8), 30),
+                                     ((6, 3, 2), 2000000000)])
+def test_reorder_matches_oracle_bit_exact(dims, bd):
+    m = hex_block(*dims, 1.0, 0.37, 0.21)
+    clamp = m.nodes_at_x(0)
+    cons = [[clamp], [clamp], [clamp]]
+    bd = api.block_dim_auto(m.n_nodes) if bd is None else bd
+    a = api.reorder_nodes(m.xyz, m.conn, 3, cons, bd)
+    b = O.reorder_nodes(m.xyz, m.conn, 3, cons, bd)
+    assert np.array_equal(a[0], b[0])
+    assert np.array_equal(a[1], b[1])
+    assert a[2] == b[2]
+
+
+def test_reorder_tie_breaks_and_multi_term_constraints():
+    # uniform grid: many equal distances at restarts; a two-set tie constraint
+    # couples nodes pairwise (S/model_prep.rs:31-41)
+    m = hex_block(8, 6, 5, 1.0, 1.0, 1.0)
+    s1 = m.nodes_at_x(0)
+    s2 = m.nodes_at_x(m.nx)
+    single = np.array([5], dtype=np.int32)
+    cons = [[s1, s2], [single, s2[:1]], [s1]]
+    for bd in (7, 18, 33):
+        a = api.reorder_nodes(m.xyz, m.conn, 3, cons, bd)
+        b = O.reorder_nodes(m.xyz, m.conn, 3, cons, bd)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_reorder_shell_dofs():
+    # 6-DOF nodes number six DOFs per node in insertion order
+    m = hex_block(4, 3, 2, 1.0, 1.0, 1.0)
+    a = api.reorder_nodes(m.xyz, m.conn, 6, [], 10)
+    b = O.reorder_nodes(m.xyz, m.conn, 6, [], 10)
+    assert np.array_equal(a[1], b[1]) and a[2] == 6 * m.n_nodes
+
+
+def test_block_dim_rule():
+    assert api.block_dim_auto(525) == 24
+    assert api.block_dim_auto(337881) == 582
+    assert api.block_dim_auto(66049) == 258
+
+
+@pytest.mark.parametrize("dims", [(20, 4, 4), (7, 3, 5)])
+def test_traction_matches_oracle(dims):
+    m = hex_block(*dims, 1.3, 0.2, 0.45)
+    rng = np.random.default_rng(0)
+    xyz = m.xyz + 0.02 * rng.standard_normal(m.xyz.shape) * np.array([1, 1, 1]) * (m.xyz[:, :1] < 1.2)
+    clamp = m.nodes_at_x(0)
+    _, di, nd = O.reorder_nodes(xyz, m.conn, 3, [[clamp]] * 3, 24)
+    tip = m.elements_at_x(m.nx - 1)
+    for load in ([0, 0, -1e6], [3.0, -1.0, 2.0]):
+        a = api.surface_traction(8, xyz, m.conn, di, nd, tip, load, [1, 0, 0], 5.0)
+        b = O.surface_traction(8, xyz, m.conn, di, nd, tip, load, [1, 0, 0], 5.0)
+        assert np.allclose(a, b, rtol=1e-12, atol=1e-12 * np.abs(b).max())
+    # all elements in the set: only faces facing +x within 5 degrees load
+    every = np.arange(m.n_el, dtype=np.int32)
+    a = api.surface_traction(8, xyz, m.conn, di, nd, every, [0, 0, -1.0], [1, 0, 0], 5.0)
+    b = O.surface_traction(8, xyz, m.conn, di, nd, every, [0, 0, -1.0], [1, 0, 0], 5.0)
+    assert np.allclose(a, b, rtol=1e-12, atol=1e-14)
+
+
+def test_lognormal_moduli_seeded():
+    a = api.lognormal_moduli(20261202, 200000, 70e3, 1.0)
+    b = api.lognormal_moduli(20261202, 200000, 70e3, 1.0)
+    assert np.array_equal(a, b)
+    z = np.log(a / 70e3)
+    assert abs(z.mean()) < 0.01 and abs(z.std() - 1.0) < 0.01
+    c = api.lognormal_moduli(1, 10, 70e3, 1.0)
+    assert not np.allclose(a[:10], c)

@@ -1,0 +1,270 @@
+"""Pins the CPU oracle (oracle/liboracle.so) against closed forms, invariants,
+library routines and finite differences.  CPU only."""
+import numpy as np
+import pytest
+
+import oracle as O
+from asend3d_amd.meshgen import hex_block, block_dim_auto
+
+UNIT = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], float)
+
+
+def distorted():
+    rng = np.random.default_rng(7)
+    return UNIT * np.array([2.0, 1.0, 0.5]) + 0.08 * rng.standard_normal((8, 3))
+
+
+def lame(E, nu):
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    mu = E / (2 * (1 + nu))
+    C = np.zeros((6, 6))
+    C[:3, :3] = lam
+    C[np.arange(3), np.arange(3)] = lam + 2 * mu
+    C[3, 3] = C[4, 4] = C[5, 5] = mu
+    return C
+
+
+def test_cmat_pivot_clamp_quirk():
+    # The reference inverts the compliance with a Givens QR that replaces any
+    # pivot below 1e-12 by 1e-12 (S/matrix_functions.rs:189-191, constants.rs:12).
+    # In SI units that corrupts C once 1/E drops under ~1e-12 (E >~ 7e11 Pa);
+    # the oracle keeps the quirk, which is why heterogeneous workloads are
+    # posed in mm-N-MPa units where 1/E stays far above the clamp.
+    assert np.allclose(O.solid_cmat(2e11, 0.3), lame(2e11, 0.3), rtol=1e-12, atol=1e-3)
+    Cbad = O.solid_cmat(5e12, 0.3)
+    assert not np.allclose(Cbad, lame(5e12, 0.3), rtol=1e-3)
+    assert np.allclose(O.solid_cmat(5e6, 0.3), lame(5e6, 0.3), rtol=1e-12, atol=1e-9)
+
+
+def test_cmat_is_lame_matrix():
+    # QR inverse of the compliance == textbook isotropic stiffness
+    for E, nu in [(200e9, 0.3), (1e6, 0.0), (70e9, 0.33)]:
+        C = O.solid_cmat(E, nu)
+        assert np.allclose(C, lame(E, nu), rtol=1e-13, atol=1e-13 * E)
+
+
+def rigid_modes(X):
+    modes = []
+    for d in range(3):
+        m = np.zeros((8, 3))
+        m[:, d] = 1
+        modes.append(m.reshape(-1))
+    for axis in range(3):
+        w = np.zeros(3)
+        w[axis] = 1
+        modes.append(np.cross(w, X).reshape(-1))
+    return np.array(modes)
+
+
+@pytest.mark.parametrize("etype", [8, 81])
+def test_element_symmetry_rigid_modes_rank(etype):
+    X = distorted()
+    K, _ = O.solid_element(etype, X, O.solid_cmat(1e6, 0.25))
+    assert np.abs(K - K.T).max() <= 1e-12 * np.abs(K).max()
+    R = rigid_modes(X)
+    assert np.abs(K @ R.T).max() <= 1e-10 * np.abs(K).max()
+    ev = np.linalg.eigvalsh(K)
+    assert np.sum(ev > 1e-8 * ev.max()) == 18
+
+
+def affine():
+    A = np.array([[2.0, 0.3, -0.1], [0.2, 1.0, 0.25], [0.1, -0.2, 0.5]])
+    return UNIT @ A.T + np.array([0.3, -0.2, 1.0])
+
+
+@pytest.mark.parametrize("etype,shape", [(8, "distorted"), (81, "affine")])
+def test_patch_linear_field_energy(etype, shape):
+    # linear displacement u = A x: strain energy is exactly V eps:C:eps / 2 (a
+    # trilinear hex reproduces linear fields; 2x2x2 Gauss integrates det J
+    # exactly).  The incompatible-mode element (bubbles with the centroid
+    # Jacobian only, S/element_soln_fields.rs:429-461) passes this patch test
+    # on parallelepipeds; on a general trilinear shape its bubbles couple.
+    X = distorted() if shape == "distorted" else affine()
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((3, 3)) * 1e-3
+    u = (X @ A.T).reshape(-1)
+    C = O.solid_cmat(3e9, 0.31)
+    K, R = O.solid_element(etype, X, C, ue=u)
+    eps = 0.5 * (A + A.T)
+    ev = np.array([eps[0, 0], eps[1, 1], eps[2, 2], 2 * eps[0, 1], 2 * eps[0, 2], 2 * eps[1, 2]])
+    M = O.solid_mass(etype, X, 1.0)
+    V = M[0::3, 0::3].sum()
+    e_ref = 0.5 * V * ev @ C @ ev
+    assert abs(0.5 * u @ K @ u - e_ref) <= 1e-10 * e_ref
+    assert np.allclose(R, K @ u, rtol=1e-10, atol=1e-10 * np.abs(R).max())
+
+
+def test_stiffness_linear_in_E_and_dual_derivative():
+    X = distorted()
+    K1, dK = O.solid_element_dE(8, X, 2e9, 0.3)
+    K2, _ = O.solid_element_dE(8, X, 4e9, 0.3)
+    assert np.allclose(K2, 2 * K1, rtol=1e-12, atol=1e-12 * np.abs(K2).max())
+    # dual-number tangent == K/E (K linear in E at fixed nu) == central difference
+    assert np.allclose(dK, K1 / 2e9, rtol=1e-12, atol=1e-12 * np.abs(dK).max())
+    h = 1e3
+    Kp, _ = O.solid_element_dE(8, X, 2e9 + h, 0.3)
+    Km, _ = O.solid_element_dE(8, X, 2e9 - h, 0.3)
+    assert np.allclose(dK, (Kp - Km) / (2 * h), rtol=1e-6, atol=1e-6 * np.abs(dK).max())
+
+
+def test_brickim_dual_matches_fd():
+    X = distorted()
+    K, dK = O.solid_element_dE(81, X, 5e9, 0.2)
+    h = 5e3
+    Kp, _ = O.solid_element_dE(81, X, 5e9 + h, 0.2)
+    Km, _ = O.solid_element_dE(81, X, 5e9 - h, 0.2)
+    assert np.allclose(dK, (Kp - Km) / (2 * h), rtol=1e-6, atol=1e-6 * np.abs(dK).max())
+
+
+def test_mass_total():
+    X = distorted()
+    M = O.solid_mass(8, X, 2700.0)
+    V = np.abs(np.linalg.det(np.array([X[1] - X[0], X[3] - X[0], X[4] - X[0]])))  # rough scale only
+    tot = M[0::3, 0::3].sum()
+    assert np.allclose(M[0::3, 0::3].sum(), M[1::3, 1::3].sum())
+    assert np.allclose(M, M.T)
+    assert tot > 0 and 0.5 * V * 2700 < tot < 2.0 * V * 2700
+    # unit cube: total mass == rho exactly
+    Mu = O.solid_mass(8, UNIT, 3.0)
+    assert abs(Mu[0::3, 0::3].sum() - 3.0) < 1e-13
+
+
+def c1_problem(etype=8, nx=20, ny=4, nz=4):
+    m = hex_block(nx, ny, nz, 1.0, 0.2, 0.2)
+    clamp = m.nodes_at_x(0)
+    bd = block_dim_auto(m.n_nodes)
+    ordered, di, ndof = O.reorder_nodes(m.xyz, m.conn, 3, [[clamp], [clamp], [clamp]], bd)
+    E = np.full(m.n_el, 200e9)
+    nu = np.full(m.n_el, 0.3)
+    K = O.assemble_solid(etype, m.xyz, m.conn, E, nu, di, ndof)
+    pen = np.concatenate([di[clamp, 0], di[clamp, 1], di[clamp, 2]])
+    sf = 1e5 * O.max_abs(K)
+    f = O.surface_traction(etype, m.xyz, m.conn, di, ndof, m.elements_at_x(nx - 1), [0, 0, -1e6], [1, 0, 0], 5.0)
+    return m, di, ndof, E, nu, K, pen, sf, f, bd
+
+
+def test_c1_sizes_and_pattern():
+    m, di, ndof, E, nu, K, pen, sf, f, bd = c1_problem()
+    assert (m.n_nodes, ndof, K.vals.size, bd) == (525, 1575, 92781, 24)
+    S = K.to_scipy()
+    assert abs(S - S.T).max() <= 1e-15 * abs(S).max()
+    # rows sum to zero before the penalty (rigid translation)
+    assert np.abs(S @ np.ones(ndof)).max() <= 1e-9 * abs(S).max()
+    assert abs(f.sum() + 40000.0) < 1e-8
+
+
+def test_traction_quarter_area_per_node():
+    m, di, ndof, E, nu, K, pen, sf, f, bd = c1_problem()
+    fz = f[di[:, 2]]
+    tip = m.nodes_at_x(m.nx)
+    a = (0.2 / 4) ** 2 * 1e6
+    counts = np.zeros(m.n_nodes)
+    for e in m.elements_at_x(m.nx - 1):
+        for loc in (1, 2, 6, 5):
+            counts[m.conn[e, loc]] += 1
+    assert np.allclose(fz[tip], -0.25 * a * counts[tip], rtol=1e-12)
+    assert np.abs(np.delete(f, di[tip, 2])).max() == 0.0
+
+
+def test_ldl_direct_matches_dense_solve():
+    m, di, ndof, E, nu, K, pen, sf, f, bd = c1_problem()
+    A = K.to_scipy().toarray()
+    A[pen, pen] += sf
+    u_dense = np.linalg.solve(A, f)
+    ldl = O.LDL(K, pen, sf, ndof)
+    assert ldl.pos_def()
+    u = ldl.solve(f)
+    assert np.abs(u - u_dense).max() <= 1e-10 * np.abs(u_dense).max()
+
+
+def test_conj_grad_sparse_matches_ldl():
+    m, di, ndof, E, nu, K, pen, sf, f, bd = c1_problem()
+    u_lu = O.LDL(K, pen, sf, ndof).solve(f)
+    pc = O.LDL(K, pen, sf, 6 * bd)
+    u, it, res = O.conj_grad_sparse(K, pen, sf, pc, f, 1e-30, 10 * ndof)
+    assert res <= 1e-30 and 0 < it < ndof
+    assert np.abs(u - u_lu).max() <= 1e-10 * np.abs(u_lu).max()
+    # max_it bound is honoured (reference only warns on non-convergence)
+    u1, it1, res1 = O.conj_grad_sparse(K, pen, sf, pc, f, 1e-30, 1)
+    assert it1 == 1 and res1 > 1e-30
+
+
+def test_jacobi_pcg_relative_rule():
+    m, di, ndof, E, nu, K, pen, sf, f, bd = c1_problem()
+    u_lu = O.LDL(K, pen, sf, ndof).solve(f)
+    u, it, rr = O.jacobi_pcg(K, pen, sf, f, 1e-12, 100000)
+    assert rr <= 1e-11
+    assert np.abs(u - u_lu).max() <= 1e-8 * np.abs(u_lu).max()
+
+
+def test_adjoint_identity_and_fd():
+    m, di, ndof, E, nu, K, pen, sf, f, bd = c1_problem()
+    rng = np.random.default_rng(11)
+    E = 70e9 * np.exp(0.5 * rng.standard_normal(m.n_el))
+    K = O.assemble_solid(8, m.xyz, m.conn, E, nu, di, ndof)
+    u = O.LDL(K, pen, sf, ndof).solve(f)
+    J = f @ u
+    s = O.solid_sens(8, m.xyz, m.conn, E, nu, di, u, u)
+    # closed form dJ/dE_e = -u_e^T K_e u_e / E_e summed with weights E_e;
+    # sf is held fixed as in the reference, so the penalty energy remains
+    ident = -J + sf * np.sum(u[pen] ** 2)
+    assert abs((E * s).sum() - ident) <= 1e-10 * abs(ident)
+    # global central difference on one element's modulus
+    e0 = int(np.argmax(np.abs(s)))
+    h = 1e-4 * E[e0]
+    Jpm = []
+    for sg in (1, -1):
+        E2 = E.copy()
+        E2[e0] += sg * h
+        K2 = O.assemble_solid(8, m.xyz, m.conn, E2, nu, di, ndof)
+        Jpm.append(f @ O.LDL(K2, pen, sf, ndof).solve(f))
+    fd = (Jpm[0] - Jpm[1]) / (2 * h)
+    assert abs(fd - s[e0]) <= 1e-6 * abs(s[e0])
+
+
+def test_hexbeam_known_answer():
+    # E/testCases/hexBeam/notes.txt test 1: brickIM 100x10x1, E=1e6, nu=0, tip
+    # load 2 -> tip deflection FL^3/3EI = 8, dw/dE = -8e-6 (discretisation tol.)
+    m = hex_block(100, 10, 1, 10.0, 1.0, 0.1)
+    clamp = m.nodes_at_x(0)
+    ordered, di, ndof = O.reorder_nodes(m.xyz, m.conn, 3, [[clamp], [clamp], [clamp]], 2000000000)
+    E = np.full(m.n_el, 1e6)
+    nu = np.zeros(m.n_el)
+    K = O.assemble_solid(81, m.xyz, m.conn, E, nu, di, ndof)
+    pen = np.concatenate([di[clamp, 0], di[clamp, 1], di[clamp, 2]])
+    sf = 1e5 * O.max_abs(K)
+    tip = m.nodes_at_x(m.nx)
+    f = np.zeros(ndof)
+    f[di[tip, 2]] = 0.0909091
+    u = O.LDL(K, pen, sf, ndof).solve(f)
+    w = 0.0454545 * u[di[tip, 2]].sum()
+    assert abs(w - 8.0) < 0.02 * 8.0
+    lam = np.zeros(ndof)
+    lam[:] = u * (0.0454545 / 0.0909091)   # dJ/du = 0.0454545 on the tip z-DOFs = f/2
+    s = O.solid_sens(81, m.xyz, m.conn, E, nu, di, u, lam)
+    assert abs(s.sum() - (-8e-6)) < 0.03 * 8e-6
+
+
+def test_brickim_softer_than_brick8_in_bending():
+    w = {}
+    for et in (8, 81):
+        m, di, ndof, E, nu, K, pen, sf, f, bd = c1_problem(et)
+        u = O.LDL(K, pen, sf, ndof).solve(f)
+        w[et] = u[di[m.nodes_at_x(m.nx), 2]].mean()
+    eb = -40000.0 / (3 * 200e9 * 0.2 ** 4 / 12)  # Euler-Bernoulli tip deflection
+    assert abs(w[81]) > abs(w[8])
+    assert abs(w[81] - eb) < 0.05 * abs(eb)
+
+
+def test_reorder_invariants():
+    m = hex_block(12, 5, 4, 1.0, 0.3, 0.2)
+    clamp = m.nodes_at_x(0)
+    bd = 18
+    ordered, di, ndof = O.reorder_nodes(m.xyz, m.conn, 3, [[clamp], [clamp], [clamp]], bd)
+    assert np.array_equal(np.sort(ordered), np.arange(m.n_nodes))
+    assert ndof == 3 * m.n_nodes
+    assert np.array_equal(di[ordered, 0], 3 * np.arange(m.n_nodes))
+    assert np.all(di[:, 3:] == -1)
+    # first node has the least connectivity (a corner, 7 neighbours, not clamped)
+    first = ordered[0]
+    assert first not in set(clamp.tolist())
