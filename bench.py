@@ -93,10 +93,10 @@ def main():
                     max_it=2000000, max_outer=40, check_every=32 if pc == api.PC_JACOBI else 4)
     base = ctx.opts(api.SOLVE_PCG_FP64, api.PC_JACOBI, tol=args.tol, max_it=2000000, check_every=32)
 
-    # reference solution: FP64 PCG to 1e-12
-    u_ref, _, J_ref, st_ref = ctx.static_compliance(prob.f, ctx.opts(api.SOLVE_PCG_FP64, api.PC_JACOBI, tol=1e-12,
+    # reference solution: FP64 PCG to 1e-11 (about the attainable FP64 residual here)
+    u_ref, _, J_ref, st_ref = ctx.static_compliance(prob.f, ctx.opts(api.SOLVE_PCG_FP64, api.PC_JACOBI, tol=1e-11,
                                                                      max_it=4000000, check_every=64))
-    log(f"FP64 reference to 1e-12: {st_ref.iterations} it, {st_ref.device_ms / 1e3:.2f} s")
+    log(f"FP64 reference to 1e-11: {st_ref.iterations} it, {st_ref.device_ms / 1e3:.2f} s")
 
     for _ in range(args.warmup):
         ctx.static_compliance(prob.f, fast)
@@ -161,7 +161,7 @@ def main():
         "fast_outer": int(st.outer_iterations),
         "fp64_iterations": int(stb.iterations),
         "rel_residual": st.rel_residual,
-        "u_rel_err_vs_fp64_1e-12": err_u,
+        "u_rel_err_vs_fp64_1e-11": err_u,
         "compliance_rel_err": err_J,
         "dJdE_rel_err_vs_fp64_path": err_s,
         "dof_per_s": prob.n_dof / t_fast,
