@@ -193,12 +193,11 @@ int run_jacobi_pcg(asend_ctx* ctx, int si, const T* b, T* x, double tol, bool st
     int rc;
     if ((rc = write_scalars(ctx, si))) return rc;
     Scalars* sc = ctx->d_sc + si;
-    launch_pcg_init<T>(ctx->st, n, b, dinv, x, r, p, ctx->d_partials, ctx->d_counter, sc);
+    launch_pcg_init<T>(ctx->st, n, b, dinv, x, r, p, ctx->d_partials, sc);
     for (;;) {
         for (int k = 0; k < check_every; ++k) {
-            launch_spmv3<T>(ctx->st, 1, nr, ctx->d_rp, ctx->d_bc, K, p, q, nullptr, ctx->d_partials, ctx->d_counter,
-                            sc);
-            launch_pcg_update<T>(ctx->st, n, dinv, x, r, p, q, ctx->d_partials, ctx->d_counter, sc);
+            launch_spmv3<T>(ctx->st, 1, nr, ctx->d_rp, ctx->d_bc, K, p, q, nullptr, ctx->d_partials, sc);
+            launch_pcg_update<T>(ctx->st, n, dinv, x, r, p, q, ctx->d_partials, sc);
             launch_pcg_dir<T>(ctx->st, n, dinv, r, p, sc);
         }
         HIPCHK(hipGetLastError());
@@ -210,8 +209,7 @@ int run_jacobi_pcg(asend_ctx* ctx, int si, const T* b, T* x, double tol, bool st
 
 // FP64 residual r = f - K u, sc[0].rr = ||r||^2.
 int residual64(asend_ctx* ctx, const double* f, const double* u, double* r) {
-    launch_spmv3<double>(ctx->st, 2, (int)ctx->n_nodes, ctx->d_rp, ctx->d_bc, ctx->d_K64, u, r, f, ctx->d_partials,
-                         ctx->d_counter, ctx->d_sc + 0);
+    launch_spmv3<double>(ctx->st, 2, (int)ctx->n_nodes, ctx->d_rp, ctx->d_bc, ctx->d_K64, u, r, f, ctx->d_partials, ctx->d_sc + 0);
     HIPCHK(hipGetLastError());
     return ASEND_OK;
 }
@@ -332,7 +330,7 @@ int assemble_device(asend_ctx* ctx, double c_mass) {
                 ctx->sf = ctx->sf_user;
             } else {
                 // reference rule: sf = 1e5 * max|K_ij| of the first assembly (S/model_analysis.rs:255-259)
-                launch_maxabs(ctx->st, ctx->d_K64, nval, ctx->d_partials, ctx->d_counter, ctx->d_sc + 2);
+                launch_maxabs(ctx->st, ctx->d_K64, nval, ctx->d_partials, ctx->d_sc + 2);
                 HIPCHK(hipGetLastError());
                 if ((rc = read_scalars(ctx, 2))) return rc;
                 ctx->sf = 1.0e5 * ctx->h_sc[2].maxabs;
@@ -682,7 +680,7 @@ int asend_spmv(asend_ctx* ctx, int precision, const double* x, double* y) {
     if (precision == 64) {
         HIPCHK(hipMemcpy(ctx->d_p, x, n * 8, hipMemcpyHostToDevice));
         launch_spmv3<double>(ctx->st, 0, (int)ctx->n_nodes, ctx->d_rp, ctx->d_bc, ctx->d_K64, ctx->d_p, ctx->d_q,
-                             nullptr, nullptr, nullptr, nullptr);
+                             nullptr, nullptr, nullptr);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(ctx->st));
         HIPCHK(hipMemcpy(y, ctx->d_q, n * 8, hipMemcpyDeviceToHost));
@@ -690,7 +688,7 @@ int asend_spmv(asend_ctx* ctx, int precision, const double* x, double* y) {
         std::vector<float> xf(x, x + n), yf(n);
         HIPCHK(hipMemcpy(ctx->d_p32, xf.data(), n * 4, hipMemcpyHostToDevice));
         launch_spmv3<float>(ctx->st, 0, (int)ctx->n_nodes, ctx->d_rp, ctx->d_bc, ctx->d_K32, ctx->d_p32, ctx->d_q32,
-                            nullptr, nullptr, nullptr, nullptr);
+                            nullptr, nullptr, nullptr);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(ctx->st));
         HIPCHK(hipMemcpy(yf.data(), ctx->d_q32, n * 4, hipMemcpyDeviceToHost));
@@ -747,7 +745,7 @@ int asend_static_compliance(asend_ctx* ctx, const double* f, double* u, double* 
     asend_solve_stats s;
     if ((rc = solve_device(ctx, opts, &s))) return rc;
     // compliance J = f.u ; adjoint rhs dJ/du = f, so lambda = u (same operator)
-    launch_dot64(ctx->st, n, ctx->d_f, ctx->d_u, ctx->d_partials, ctx->d_counter, ctx->d_sc + 2);
+    launch_dot64(ctx->st, n, ctx->d_f, ctx->d_u, ctx->d_partials, ctx->d_sc + 2);
     launch_hex8_sens(ctx->st, (int)ctx->n_el, ctx->d_conn_r, ctx->d_xyz_r, ctx->d_E, ctx->d_nu, ctx->d_u, ctx->d_u,
                      ctx->d_sens);
     HIPCHK(hipGetLastError());
@@ -775,10 +773,10 @@ int asend_bench_spmv(asend_ctx* ctx, int precision, int n_iter, double* avg_ms) 
         for (int i = 0; i < cnt; ++i) {
             if (precision == 64)
                 launch_spmv3<double>(ctx->st, 0, nr, ctx->d_rp, ctx->d_bc, ctx->d_K64, ctx->d_u, ctx->d_q, nullptr,
-                                     nullptr, nullptr, nullptr);
+                                     nullptr, nullptr);
             else
                 launch_spmv3<float>(ctx->st, 0, nr, ctx->d_rp, ctx->d_bc, ctx->d_K32, ctx->d_x32, ctx->d_q32,
-                                    nullptr, nullptr, nullptr, nullptr);
+                                    nullptr, nullptr, nullptr);
         }
     }
     HIPCHK(hipGetLastError());

@@ -62,71 +62,27 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
     __syncthreads();
 }
 
-// Deterministic grid-wide sum: every block stores its NV partial sums, the
-// last block to arrive adds all partials in a fixed order and calls
-// fin(totals) from thread 0.  The counter is reset for the next launch.
-// Partial stores/loads and the arrival counter use agent-scope atomics so
-// they are coherent across the XCDs' separate L2 caches.
-template <int NV, class Fin>
-__device__ __forceinline__ void grid_sum(double (&v)[NV], double* partials, unsigned* counter, Fin fin) {
+// Block-level partial sums for a two-kernel grid reduction: thread 0 of every
+// block stores its NV sums at partials[NV*blockIdx.x + i]; a single-block
+// finalize kernel launched next on the same stream combines them.
+template <int NV>
+__device__ __forceinline__ void block_partials(double (&v)[NV], double* partials) {
     block_sum<NV>(v);
-    __shared__ bool last;
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0)
 #pragma unroll
-        for (int i = 0; i < NV; ++i)
-            __hip_atomic_store(&partials[(size_t)NV * blockIdx.x + i], v[i], __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return;
-    double t[NV];
-#pragma unroll
-    for (int i = 0; i < NV; ++i) t[i] = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
-#pragma unroll
-        for (int i = 0; i < NV; ++i)
-            t[i] += __hip_atomic_load(&partials[(size_t)NV * b + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    block_sum<NV>(t);
-    if (threadIdx.x == 0) {
-        fin(t);
-        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+        for (int i = 0; i < NV; ++i) partials[(size_t)NV * blockIdx.x + i] = v[i];
 }
 
-// Grid-wide max of one non-negative value, same scheme.
-template <class Fin>
-__device__ __forceinline__ void grid_max(double v, double* partials, unsigned* counter, Fin fin) {
+// Block-level maximum of a non-negative value, same scheme.
+__device__ __forceinline__ void block_partial_max(double v, double* partials) {
     __shared__ double sh[kBlock / kWave];
-    __shared__ bool last;
-    int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
     v = wave_max(v);
-    if (lane == 0) sh[w] = v;
+    if (threadIdx.x % kWave == 0) sh[threadIdx.x / kWave] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
         double m = 0.0;
         for (int k = 0; k < (int)(blockDim.x / kWave); ++k) m = m > sh[k] ? m : sh[k];
-        __hip_atomic_store(&partials[blockIdx.x], m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return;
-    double m = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
-        double x = __hip_atomic_load(&partials[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        m = m > x ? m : x;
-    }
-    m = wave_max(m);
-    __syncthreads();
-    if (lane == 0) sh[w] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double mm = 0.0;
-        for (int k = 0; k < (int)(blockDim.x / kWave); ++k) mm = mm > sh[k] ? mm : sh[k];
-        fin(mm);
-        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        partials[blockIdx.x] = m;
     }
 }
 

@@ -15,11 +15,6 @@
 namespace asend {
 
 // ---------------------------------------------------------------------------
-// Scalars of one Krylov loop, resident in device memory so kernels chain
-// without host round trips.
-// ---------------------------------------------------------------------------
-
-// ---------------------------------------------------------------------------
 // brick8 element integration + colored assembly (FP64).
 // One 64-lane wavefront per element; lane (a, b) owns the 3x3 block K_ab.
 // Gauss-point geometry (J^-1, det J, physical gradients) is computed once
@@ -136,21 +131,90 @@ void launch_hex8_assemble(hipStream_t st, const int32_t* elems, int n_elems, con
 }
 
 // ---------------------------------------------------------------------------
+// Grid reductions: every block writes its partial sums with plain stores;
+// a single-block finalize kernel (stream-ordered after the producer, so the
+// partials are visible without device-scope fences) adds them in a fixed
+// order and updates the loop scalars.  Deterministic run to run.
+// ---------------------------------------------------------------------------
+enum FinOp { FIN_PQ = 0, FIN_RR = 1, FIN_INIT = 2, FIN_UPDATE = 3, FIN_DOT = 4, FIN_MAX = 5 };
+
+__global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ partials, int nparts, int op,
+                                                     Scalars* sc) {
+    if ((op == FIN_PQ || op == FIN_UPDATE) && sc->done) return;
+    if (op == FIN_MAX) {
+        double m = 0.0;
+        for (int b = threadIdx.x; b < nparts; b += blockDim.x) m = fmax(m, partials[b]);
+        m = wave_max(m);
+        __shared__ double sh[kBlock / kWave];
+        if (threadIdx.x % kWave == 0) sh[threadIdx.x / kWave] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double mm = 0.0;
+            for (int k = 0; k < (int)(blockDim.x / kWave); ++k) mm = fmax(mm, sh[k]);
+            sc->maxabs = mm;
+        }
+        return;
+    }
+    const int nv = (op == FIN_INIT || op == FIN_UPDATE) ? 2 : 1;
+    double t[2] = {0.0, 0.0};
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
+        t[0] += partials[(size_t)nv * b];
+        if (nv == 2) t[1] += partials[(size_t)nv * b + 1];
+    }
+    block_sum<2>(t);
+    if (threadIdx.x != 0) return;
+    switch (op) {
+        case FIN_PQ:
+            sc->pq = t[0];
+            sc->alpha = sc->rz / t[0];
+            break;
+        case FIN_RR:
+            sc->rr = t[0];
+            break;
+        case FIN_DOT:
+            sc->dot = t[0];
+            break;
+        case FIN_INIT:
+            sc->rz = t[0];
+            sc->rr = t[1];
+            sc->ff = t[1];
+            sc->it = 0;
+            sc->done = (sc->stop_abs ? (t[0] <= sc->tol) : (t[1] <= sc->tol2 * t[1])) || t[1] == 0.0;
+            break;
+        case FIN_UPDATE: {
+            sc->beta = t[0] / sc->rz;
+            sc->rz = t[0];
+            sc->rr = t[1];
+            sc->it += 1;
+            bool conv = sc->stop_abs ? (t[0] <= sc->tol) : (t[1] <= sc->tol2 * sc->ff);
+            if (conv || sc->it >= sc->max_it) sc->done = 1;
+            break;
+        }
+        default:
+            break;
+    }
+}
+
+static void finalize(hipStream_t st, const double* partials, int nparts, int op, Scalars* sc) {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, st, partials, nparts, op, sc);
+}
+
+// ---------------------------------------------------------------------------
 // Post-assembly: max |K| (penalty scale), penalty diagonal, FP32 copy,
 // Jacobi diagonal.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_maxabs(const double* __restrict__ v, int64_t n, double* partials,
-                                                    unsigned* counter, Scalars* sc) {
+constexpr int kMaxBlocks = 1024;
+
+__global__ __launch_bounds__(kBlock) void k_maxabs(const double* __restrict__ v, int64_t n, double* partials) {
     double m = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        double a = fabs(v[i]);
-        m = m > a ? m : a;
-    }
-    grid_max(m, partials, counter, [&](double t) { sc->maxabs = t; });
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        m = fmax(m, fabs(v[i]));
+    block_partial_max(m, partials);
 }
 
-void launch_maxabs(hipStream_t st, const double* v, int64_t n, double* partials, unsigned* counter, Scalars* sc) {
-    hipLaunchKernelGGL(k_maxabs, dim3(1024), dim3(kBlock), 0, st, v, n, partials, counter, sc);
+void launch_maxabs(hipStream_t st, const double* v, int64_t n, double* partials, Scalars* sc) {
+    hipLaunchKernelGGL(k_maxabs, dim3(kMaxBlocks), dim3(kBlock), 0, st, v, n, partials);
+    finalize(st, partials, kMaxBlocks, FIN_MAX, sc);
 }
 
 // Adds mult[i] * sf to the diagonal entry of DOF pen[i] (unique DOFs).
@@ -203,17 +267,16 @@ void launch_diag_inv(hipStream_t st, int n_rows, int bs, const int32_t* rp, cons
 // ---------------------------------------------------------------------------
 // Block-row SpMV, b = 3, G lanes per block row, component-major values.
 // MODE 0: y = K x
-// MODE 1: y = K x and (x . y) summed over the grid -> sc->pq, alpha = rz/pq
-// MODE 2: y = f - K x (x = current solution, f = load) and ||y||^2 -> sc->rr
+// MODE 1: y = K x and (x . y) -> finalize: sc->pq, sc->alpha = sc->rz / pq
+// MODE 2: y = f - K x (x = current solution, f = load) -> finalize: sc->rr = |y|^2
 // Kernels with a scalars pointer return at once when sc->done is set.
 // ---------------------------------------------------------------------------
 template <typename T, int G, int MODE>
 __global__ __launch_bounds__(kBlock) void k_spmv3(int n_rows, int rows_per_block, const int32_t* __restrict__ rp,
                                                   const int32_t* __restrict__ bc, const T* __restrict__ val,
                                                   const T* __restrict__ x, T* __restrict__ y,
-                                                  const T* __restrict__ f, double* partials, unsigned* counter,
-                                                  Scalars* sc) {
-    if (MODE != 0 && sc->done) return;
+                                                  const T* __restrict__ f, double* partials, const Scalars* sc) {
+    if (MODE == 1 && sc->done) return;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int sub = threadIdx.x % G;
     const int grp = threadIdx.x / G;
@@ -258,49 +321,44 @@ __global__ __launch_bounds__(kBlock) void k_spmv3(int n_rows, int rows_per_block
     }
     if (MODE == 0) return;
     double t[1] = {part};
-    grid_sum<1>(t, partials, counter, [&](double* tot) {
-        if (MODE == 1) {
-            sc->pq = tot[0];
-            sc->alpha = sc->rz / tot[0];
-        } else {
-            sc->rr = tot[0];
-        }
-    });
+    block_partials<1>(t, partials);
 }
 
 template <typename T>
 void launch_spmv3(hipStream_t st, int mode, int n_rows, const int32_t* rp, const int32_t* bc, const T* val,
-                  const T* x, T* y, const T* f, double* partials, unsigned* counter, Scalars* sc) {
+                  const T* x, T* y, const T* f, double* partials, Scalars* sc) {
     constexpr int G = 32;
     const int nblk = kSpmvBlocks;
     int rpb = (n_rows + nblk - 1) / nblk;
-    if (mode == 0)
+    if (mode == 0) {
         hipLaunchKernelGGL((k_spmv3<T, G, 0>), dim3(nblk), dim3(kBlock), 0, st, n_rows, rpb, rp, bc, val, x, y, f,
-                           partials, counter, sc);
-    else if (mode == 1)
+                           partials, sc);
+    } else if (mode == 1) {
         hipLaunchKernelGGL((k_spmv3<T, G, 1>), dim3(nblk), dim3(kBlock), 0, st, n_rows, rpb, rp, bc, val, x, y, f,
-                           partials, counter, sc);
-    else
+                           partials, sc);
+        finalize(st, partials, nblk, FIN_PQ, sc);
+    } else {
         hipLaunchKernelGGL((k_spmv3<T, G, 2>), dim3(nblk), dim3(kBlock), 0, st, n_rows, rpb, rp, bc, val, x, y, f,
-                           partials, counter, sc);
+                           partials, sc);
+        finalize(st, partials, nblk, FIN_RR, sc);
+    }
 }
 
 template void launch_spmv3<double>(hipStream_t, int, int, const int32_t*, const int32_t*, const double*,
-                                   const double*, double*, const double*, double*, unsigned*, Scalars*);
+                                   const double*, double*, const double*, double*, Scalars*);
 template void launch_spmv3<float>(hipStream_t, int, int, const int32_t*, const int32_t*, const float*, const float*,
-                                  float*, const float*, double*, unsigned*, Scalars*);
+                                  float*, const float*, double*, Scalars*);
 
 // ---------------------------------------------------------------------------
 // Jacobi PCG vector kernels (T = double for the FP64 solver, float for the
 // inner FP32 solver).  Reductions are always combined in FP64.
 // ---------------------------------------------------------------------------
 
-// r = b, x = 0, p = z = Dinv r; rz = r.z, rr = r.r, ff = r.r, done if
-// already below tolerance.
+// r = b, x = 0, p = z = Dinv r; partials of r.z and r.r
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pcg_init(int64_t n, const T* __restrict__ b, const T* __restrict__ dinv,
                                                      T* __restrict__ x, T* __restrict__ r, T* __restrict__ p,
-                                                     double* partials, unsigned* counter, Scalars* sc) {
+                                                     double* partials) {
     double t[2] = {0.0, 0.0};
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         T ri = b[i];
@@ -311,21 +369,15 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(int64_t n, const T* __restr
         t[0] += (double)ri * zi;
         t[1] += (double)ri * ri;
     }
-    grid_sum<2>(t, partials, counter, [&](double* tot) {
-        sc->rz = tot[0];
-        sc->rr = tot[1];
-        sc->ff = tot[1];
-        sc->it = 0;
-        sc->done = (sc->stop_abs ? (tot[0] <= sc->tol) : (tot[1] <= sc->tol2 * tot[1])) || tot[1] == 0.0;
-    });
+    block_partials<2>(t, partials);
 }
 
-// x += alpha p; r -= alpha q; rz_new = r.(Dinv r), rr = r.r; beta; done test.
+// x += alpha p; r -= alpha q; partials of r.(Dinv r) and r.r
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(int64_t n, const T* __restrict__ dinv, T* __restrict__ x,
                                                        T* __restrict__ r, const T* __restrict__ p,
-                                                       const T* __restrict__ q, double* partials, unsigned* counter,
-                                                       Scalars* sc) {
+                                                       const T* __restrict__ q, double* partials,
+                                                       const Scalars* sc) {
     if (sc->done) return;
     const T alpha = (T)sc->alpha;
     double t[2] = {0.0, 0.0};
@@ -336,14 +388,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int64_t n, const T* __res
         t[0] += (double)ri * (dinv[i] * ri);
         t[1] += (double)ri * ri;
     }
-    grid_sum<2>(t, partials, counter, [&](double* tot) {
-        sc->beta = tot[0] / sc->rz;
-        sc->rz = tot[0];
-        sc->rr = tot[1];
-        sc->it += 1;
-        bool conv = sc->stop_abs ? (tot[0] <= sc->tol) : (tot[1] <= sc->tol2 * sc->ff);
-        if (conv || sc->it >= sc->max_it) sc->done = 1;
-    });
+    block_partials<2>(t, partials);
 }
 
 // p = Dinv r + beta p
@@ -358,26 +403,25 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(int64_t n, const T* __restri
 
 template <typename T>
 void launch_pcg_init(hipStream_t st, int64_t n, const T* b, const T* dinv, T* x, T* r, T* p, double* partials,
-                     unsigned* counter, Scalars* sc) {
-    hipLaunchKernelGGL(k_pcg_init<T>, dim3(kVecBlocks), dim3(kBlock), 0, st, n, b, dinv, x, r, p, partials, counter,
-                       sc);
+                     Scalars* sc) {
+    hipLaunchKernelGGL(k_pcg_init<T>, dim3(kVecBlocks), dim3(kBlock), 0, st, n, b, dinv, x, r, p, partials);
+    finalize(st, partials, kVecBlocks, FIN_INIT, sc);
 }
 template <typename T>
 void launch_pcg_update(hipStream_t st, int64_t n, const T* dinv, T* x, T* r, const T* p, const T* q,
-                       double* partials, unsigned* counter, Scalars* sc) {
-    hipLaunchKernelGGL(k_pcg_update<T>, dim3(kVecBlocks), dim3(kBlock), 0, st, n, dinv, x, r, p, q, partials, counter,
-                       sc);
+                       double* partials, Scalars* sc) {
+    hipLaunchKernelGGL(k_pcg_update<T>, dim3(kVecBlocks), dim3(kBlock), 0, st, n, dinv, x, r, p, q, partials, sc);
+    finalize(st, partials, kVecBlocks, FIN_UPDATE, sc);
 }
 template <typename T>
 void launch_pcg_dir(hipStream_t st, int64_t n, const T* dinv, const T* r, T* p, const Scalars* sc) {
     hipLaunchKernelGGL(k_pcg_dir<T>, dim3(kVecBlocks), dim3(kBlock), 0, st, n, dinv, r, p, sc);
 }
 
-#define INST_PCG(T)                                                                                            \
-    template void launch_pcg_init<T>(hipStream_t, int64_t, const T*, const T*, T*, T*, T*, double*, unsigned*, \
-                                     Scalars*);                                                                \
-    template void launch_pcg_update<T>(hipStream_t, int64_t, const T*, T*, T*, const T*, const T*, double*,    \
-                                       unsigned*, Scalars*);                                                   \
+#define INST_PCG(T)                                                                                               \
+    template void launch_pcg_init<T>(hipStream_t, int64_t, const T*, const T*, T*, T*, T*, double*, Scalars*);    \
+    template void launch_pcg_update<T>(hipStream_t, int64_t, const T*, T*, T*, const T*, const T*, double*,       \
+                                       Scalars*);                                                                 \
     template void launch_pcg_dir<T>(hipStream_t, int64_t, const T*, const T*, T*, const Scalars*);
 INST_PCG(double)
 INST_PCG(float)
@@ -407,18 +451,17 @@ void launch_axpy_from_f32(hipStream_t st, int64_t n, double* u, const float* d, 
     hipLaunchKernelGGL(k_axpy_from_f32, dim3(kVecBlocks), dim3(kBlock), 0, st, n, u, d, sc_outer);
 }
 
-// dot(a, b) in FP64 -> sc->pq (used for the compliance f.u)
+// dot(a, b) in FP64 -> sc->dot (used for the compliance f.u)
 __global__ __launch_bounds__(kBlock) void k_dot64(int64_t n, const double* __restrict__ a,
-                                                  const double* __restrict__ b, double* partials, unsigned* counter,
-                                                  Scalars* sc) {
+                                                  const double* __restrict__ b, double* partials) {
     double t[1] = {0.0};
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         t[0] += a[i] * b[i];
-    grid_sum<1>(t, partials, counter, [&](double* tot) { sc->dot = tot[0]; });
+    block_partials<1>(t, partials);
 }
-void launch_dot64(hipStream_t st, int64_t n, const double* a, const double* b, double* partials, unsigned* counter,
-                  Scalars* sc) {
-    hipLaunchKernelGGL(k_dot64, dim3(kVecBlocks), dim3(kBlock), 0, st, n, a, b, partials, counter, sc);
+void launch_dot64(hipStream_t st, int64_t n, const double* a, const double* b, double* partials, Scalars* sc) {
+    hipLaunchKernelGGL(k_dot64, dim3(kVecBlocks), dim3(kBlock), 0, st, n, a, b, partials);
+    finalize(st, partials, kVecBlocks, FIN_DOT, sc);
 }
 
 __global__ void k_axpy64(int64_t n, double alpha, const double* __restrict__ x, double* __restrict__ y) {
