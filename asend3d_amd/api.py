@@ -65,7 +65,8 @@ EXPORTED = ["asend_ctx_create", "asend_ctx_destroy", "asend_last_error", "asend_
             "asend_reorder_nodes", "asend_block_dim_auto", "asend_surface_traction", "asend_lognormal_moduli",
             "asend_upload_mesh", "asend_set_grid", "asend_update_moduli", "asend_set_penalty",
             "asend_penalty_scale", "asend_assemble", "asend_get_matrix", "asend_spmv", "asend_solve",
-            "asend_element_sens", "asend_static_compliance", "asend_bench_spmv"]
+            "asend_element_sens", "asend_static_compliance", "asend_bench_spmv", "asend_mg_info", "asend_mg_apply",
+            "asend_mg_restrict", "asend_mg_prolong", "asend_mg_vcycle"]
 
 
 def lib():
@@ -266,3 +267,31 @@ class Context:
 
     def sync(self):
         self._chk(_lib.asend_sync(self.h), "asend_sync")
+
+    # ---- multigrid diagnostics ----
+    def mg_info(self):
+        n = ctypes.c_int(0)
+        nd = np.zeros(16, dtype=np.int64)
+        self._chk(_lib.asend_mg_info(self.h, ctypes.byref(n), _p(nd)), "asend_mg_info")
+        return [int(v) for v in nd[:n.value]]
+
+    def _mg_io(self, fn, level, x, n_out):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros(n_out, dtype=np.float32)
+        self._chk(fn(self.h, ctypes.c_int(level), _p(x), _p(y)), fn.__name__)
+        return y
+
+    def mg_apply(self, level, x):
+        return self._mg_io(_lib.asend_mg_apply, level, x, self.mg_info()[level])
+
+    def mg_restrict(self, level, x):
+        return self._mg_io(_lib.asend_mg_restrict, level, x, self.mg_info()[level + 1])
+
+    def mg_prolong(self, level, x):
+        return self._mg_io(_lib.asend_mg_prolong, level, x, self.mg_info()[level])
+
+    def mg_vcycle(self, b):
+        b = np.ascontiguousarray(b, dtype=np.float32)
+        x = np.zeros(b.shape[0], dtype=np.float32)
+        self._chk(_lib.asend_mg_vcycle(self.h, _p(b), _p(x)), "asend_mg_vcycle")
+        return x

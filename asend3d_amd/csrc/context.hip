@@ -43,6 +43,8 @@ struct asend_ctx {
     int64_t* d_pen = nullptr;
     int32_t* d_pen_mult = nullptr;
     int n_pen = 0;
+    std::vector<int64_t> h_pen;
+    std::vector<int32_t> h_pen_mult;
     double sf_user = 0.0, sf = 0.0;
     bool sf_fixed = false, assembled = false;
 
@@ -293,6 +295,14 @@ int solve_device(asend_ctx* ctx, const asend_solve_opts* o, asend_solve_stats* s
             it_in = ctx->h_sc[1].it;
         } else {
             if (!ctx->mg) return fail(ctx, ASEND_ERR_STATE, "multigrid preconditioner not built (asend_set_grid)");
+            if (o->precond == ASEND_PC_MG) {
+                mg_set_fine_smoother(ctx->mg, 0);
+            } else if (o->precond == ASEND_PC_LEARNED) {
+                if (mg_set_fine_smoother(ctx->mg, 1))
+                    return fail(ctx, ASEND_ERR_STATE, "no learned preconditioner loaded");
+            } else {
+                return fail(ctx, ASEND_ERR_ARG, "unknown preconditioner %d", o->precond);
+            }
             int fb = 0;
             if ((rc = mg_fcg_solve(ctx->mg, ctx->st, ctx->d_r32, ctx->d_x32, inner_tol, max_it - total_it, check,
                                    &it_in, &fb)))
@@ -348,7 +358,8 @@ int assemble_device(asend_ctx* ctx, double c_mass) {
     HIPCHK(hipGetLastError());
     ctx->assembled = true;
     if (ctx->mg) {
-        if ((rc = mg_setup(ctx->mg, ctx->st, ctx->d_E, ctx->d_nu, ctx->d_K32, ctx->d_dinv32, ctx->sf)))
+        if ((rc = mg_setup(ctx->mg, ctx->st, ctx->d_E, ctx->d_nu, ctx->d_K32, ctx->d_dinv32, ctx->sf, ctx->n_pen,
+                           ctx->h_pen.data(), ctx->h_pen_mult.data())))
             return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
     }
     return ASEND_OK;
@@ -558,7 +569,7 @@ int asend_set_grid(asend_ctx* ctx, int64_t nx, int64_t ny, int64_t nz) {
     if (ctx->mg) mg_destroy(ctx->mg);
     ctx->mg = nullptr;
     // rank of structured node (i,j,k) -> needed by the transfer operators
-    std::vector<int32_t> rank_of_grid(ctx->n_nodes);
+    std::vector<int32_t> rank_of_grid(ctx->n_nodes, -1);
     std::vector<int32_t> conn_r(8 * ctx->n_el);
     HIPCHK(hipMemcpy(conn_r.data(), ctx->d_conn_r, conn_r.size() * 4, hipMemcpyDeviceToHost));
     // element e = i + nx*(j + ny*k) has node 0 at grid (i,j,k), node 6 at (i+1,j+1,k+1)
@@ -569,7 +580,11 @@ int asend_set_grid(asend_ctx* ctx, int64_t nx, int64_t ny, int64_t nz) {
                 int64_t e = i + nx * (j + ny * k);
                 for (int a = 0; a < 8; ++a) {
                     int64_t gi = i + corner[a][0], gj = j + corner[a][1], gk = k + corner[a][2];
-                    rank_of_grid[gi + (nx + 1) * (gj + (ny + 1) * gk)] = conn_r[8 * e + a];
+                    int32_t& slotr = rank_of_grid[gi + (nx + 1) * (gj + (ny + 1) * gk)];
+                    if (slotr >= 0 && slotr != conn_r[8 * e + a])
+                        return fail(ctx, ASEND_ERR_ARG, "mesh is not the structured grid %lldx%lldx%lld in label order",
+                                    (long long)nx, (long long)ny, (long long)nz);
+                    slotr = conn_r[8 * e + a];
                 }
             }
     int rc = mg_create(&ctx->mg, (int)nx, (int)ny, (int)nz, rank_of_grid.data(), ctx->d_xyz_r, ctx->d_conn_r,
@@ -616,6 +631,8 @@ int asend_set_penalty(asend_ctx* ctx, int64_t n, const int64_t* dofs, double sf)
         HIPCHK(hipMemcpy(ctx->d_pen_mult, m.data(), m.size() * 4, hipMemcpyHostToDevice));
     }
     ctx->n_pen = (int)u.size();
+    ctx->h_pen = u;
+    ctx->h_pen_mult = m;
     ctx->sf_user = sf;
     ctx->sf_fixed = false;
     ctx->assembled = false;
@@ -787,5 +804,61 @@ int asend_bench_spmv(asend_ctx* ctx, int precision, int n_iter, double* avg_ms) 
     *avg_ms = ms / n_iter;
     return ASEND_OK;
 }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Multigrid diagnostics (host arrays, FP32): used by the parity tests to check
+// the Galerkin identity A_{l+1} = P^T A_l P and the V-cycle on the device.
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int asend_mg_info(asend_ctx* ctx, int* n_levels, int64_t* level_ndof) {
+    if (!ctx || !n_levels) return ASEND_ERR_ARG;
+    if (!ctx->mg) return fail(ctx, ASEND_ERR_STATE, "no multigrid hierarchy (asend_set_grid)");
+    *n_levels = mg_levels(ctx->mg);
+    if (level_ndof)
+        for (int l = 0; l < *n_levels && l < 16; ++l) level_ndof[l] = mg_level_ndof(ctx->mg, l);
+    return ASEND_OK;
+}
+
+static int mg_io(asend_ctx* ctx, int op, int level, const float* in, float* out) {
+    if (!ctx || !in || !out) return ASEND_ERR_ARG;
+    if (!ctx->mg) return fail(ctx, ASEND_ERR_STATE, "no multigrid hierarchy (asend_set_grid)");
+    if (!ctx->assembled) return fail(ctx, ASEND_ERR_STATE, "assemble first");
+    int nl = mg_levels(ctx->mg);
+    if (level < 0 || level >= nl || (op >= 1 && level + 1 >= nl))
+        return fail(ctx, ASEND_ERR_ARG, "bad level %d", level);
+    int64_t nin = 0, nout = 0;
+    if (op == 0) nin = nout = mg_level_ndof(ctx->mg, level);
+    if (op == 1) { nin = mg_level_ndof(ctx->mg, level); nout = mg_level_ndof(ctx->mg, level + 1); }
+    if (op == 2) { nin = mg_level_ndof(ctx->mg, level + 1); nout = mg_level_ndof(ctx->mg, level); }
+    if (op == 3) nin = nout = mg_level_ndof(ctx->mg, 0);
+    float *din = nullptr, *dout = nullptr;
+    HIPCHK(hipMalloc((void**)&din, nin * sizeof(float)));
+    HIPCHK(hipMalloc((void**)&dout, nout * sizeof(float)));
+    HIPCHK(hipMemcpy(din, in, nin * sizeof(float), hipMemcpyHostToDevice));
+    int rc = 0;
+    if (op == 0) rc = mg_apply_level(ctx->mg, ctx->st, level, din, dout);
+    if (op == 1) rc = mg_restrict(ctx->mg, ctx->st, level, din, dout);
+    if (op == 2) rc = mg_prolong(ctx->mg, ctx->st, level, din, dout);
+    if (op == 3) rc = mg_vcycle(ctx->mg, ctx->st, din, dout);
+    hipError_t e1 = hipStreamSynchronize(ctx->st);
+    hipError_t e2 = hipMemcpy(out, dout, nout * sizeof(float), hipMemcpyDeviceToHost);
+    (void)hipFree(din);
+    (void)hipFree(dout);
+    if (rc) return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
+    if (e1 != hipSuccess || e2 != hipSuccess) return fail(ctx, ASEND_ERR_HIP, "multigrid diagnostic copy failed");
+    return ASEND_OK;
+}
+
+int asend_mg_apply(asend_ctx* ctx, int level, const float* x, float* y) { return mg_io(ctx, 0, level, x, y); }
+int asend_mg_restrict(asend_ctx* ctx, int level, const float* fine, float* coarse) {
+    return mg_io(ctx, 1, level, fine, coarse);
+}
+int asend_mg_prolong(asend_ctx* ctx, int level, const float* coarse, float* fine) {
+    return mg_io(ctx, 2, level, coarse, fine);
+}
+int asend_mg_vcycle(asend_ctx* ctx, const float* b, float* x) { return mg_io(ctx, 3, 0, b, x); }
 
 }  // extern "C"

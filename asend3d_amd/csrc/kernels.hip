@@ -269,6 +269,7 @@ void launch_diag_inv(hipStream_t st, int n_rows, int bs, const int32_t* rp, cons
 // MODE 0: y = K x
 // MODE 1: y = K x and (x . y) -> finalize: sc->pq, sc->alpha = sc->rz / pq
 // MODE 2: y = f - K x (x = current solution, f = load) -> finalize: sc->rr = |y|^2
+// MODE 3: y = f - K x without the norm
 // Kernels with a scalars pointer return at once when sc->done is set.
 // ---------------------------------------------------------------------------
 template <typename T, int G, int MODE>
@@ -303,7 +304,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv3(int n_rows, int rows_per_block
             a2 += __shfl_xor(a2, o, G);
         }
         if (sub == 0) {
-            if (MODE == 2) {
+            if (MODE == 2 || MODE == 3) {
                 T y0 = f[3 * (int64_t)r] - a0, y1 = f[3 * (int64_t)r + 1] - a1, y2 = f[3 * (int64_t)r + 2] - a2;
                 y[3 * (int64_t)r] = y0;
                 y[3 * (int64_t)r + 1] = y1;
@@ -319,7 +320,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv3(int n_rows, int rows_per_block
             }
         }
     }
-    if (MODE == 0) return;
+    if (MODE == 0 || MODE == 3) return;
     double t[1] = {part};
     block_partials<1>(t, partials);
 }
@@ -337,10 +338,13 @@ void launch_spmv3(hipStream_t st, int mode, int n_rows, const int32_t* rp, const
         hipLaunchKernelGGL((k_spmv3<T, G, 1>), dim3(nblk), dim3(kBlock), 0, st, n_rows, rpb, rp, bc, val, x, y, f,
                            partials, sc);
         finalize(st, partials, nblk, FIN_PQ, sc);
-    } else {
+    } else if (mode == 2) {
         hipLaunchKernelGGL((k_spmv3<T, G, 2>), dim3(nblk), dim3(kBlock), 0, st, n_rows, rpb, rp, bc, val, x, y, f,
                            partials, sc);
         finalize(st, partials, nblk, FIN_RR, sc);
+    } else {
+        hipLaunchKernelGGL((k_spmv3<T, G, 3>), dim3(nblk), dim3(kBlock), 0, st, n_rows, rpb, rp, bc, val, x, y, f,
+                           partials, sc);
     }
 }
 

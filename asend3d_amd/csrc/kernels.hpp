@@ -27,7 +27,7 @@ void launch_diag_inv(hipStream_t st, int n_rows, int bs, const int32_t* rp, cons
                      double* d64, float* d32);
 
 // mode 0: y = K x; 1: y = K x, sc->pq = x.y, sc->alpha = sc->rz/pq; 2: y = f - K x, sc->rr = |y|^2
-// (modes 1 and 2 launch their single-block finalize kernel after the SpMV)
+// 3: y = f - K x (no norm).  Modes 1 and 2 launch their single-block finalize kernel after the SpMV.
 template <typename T>
 void launch_spmv3(hipStream_t st, int mode, int n_rows, const int32_t* rp, const int32_t* bc, const T* val,
                   const T* x, T* y, const T* f, double* partials, Scalars* sc);

@@ -1,29 +1,1201 @@
-// Multigrid preconditioner (placeholder until the hierarchy is built).
-#include "precond.hpp"
+// Geometric multigrid preconditioner for structured brick8 grids (gfx950).
+//
+// Hierarchy: each level pairs the cells of the finer level along every axis
+// (an odd count leaves its last cell single), until the coarsest level is a
+// few hundred DOFs.  Prolongation is piecewise-linear interpolation in grid
+// index space; restriction is its transpose.  Coarse operators are exactly
+// Galerkin, A_{l+1} = P^T A_l P, computed element by element:
+//   level 1: integrate each coarse element's trilinear basis over its fine
+//            cells' Gauss points with the fine cells' moduli;
+//   level l+1 >= 2: K_E = sum_children P_c^T K_c P_c (24x24 element matrices);
+//   penalty rows: sf * P^T D P per component, formed on the host.
+// Smoother: Chebyshev polynomial in D^-1 A (or in the learned node-block
+// preconditioner on the finest level), degree 2 pre and post.  Coarsest
+// level: explicit FP64 inverse (single-workgroup Gauss-Jordan, SPD, no
+// pivoting) applied as a dense GEMV.
+// The Krylov wrapper is flexible CG with one truncation vector, since a
+// V-cycle evaluated in FP32 is only approximately a fixed SPD operator.
+#include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
 #include <string>
+#include <vector>
 
 #include "asend_gpu.h"
+#include "device_common.hpp"
+#include "kernels.hpp"
+#include "precond.hpp"
 
 namespace asend {
 
-struct Multigrid {
-    std::string err;
+namespace {
+
+constexpr int kChebDegree = 2;
+constexpr int kMaxCoarseNodes = 30;
+constexpr int kPowerIts = 20;
+
+// ---------------------------------------------------------------------------
+// Host-side axis maps between a level and the next coarser level.
+// ---------------------------------------------------------------------------
+struct AxisMap {
+    int nf = 0, nc = 0;          // fine / coarse cell counts
+    std::vector<int> X;          // coarse node I -> fine node index (nc+1 entries)
+    std::vector<int> pi;         // fine node i -> 2 coarse parents (-1 unused)
+    std::vector<float> pw;       //   and weights
+    std::vector<int> ri;         // coarse node I -> 3 fine children (-1 unused)
+    std::vector<float> rw;
 };
 
-int mg_create(Multigrid** out, int, int, int, const int32_t*, const double*, const int32_t*, const int32_t*,
-              const int32_t*, const uint8_t*, const int32_t*, const int32_t*, int64_t, int64_t) {
-    *out = new Multigrid();
-    (*out)->err = "multigrid preconditioner not available in this build";
-    return ASEND_ERR_UNSUPPORTED;
+AxisMap make_axis(int nf) {
+    AxisMap a;
+    a.nf = nf;
+    a.nc = (nf + 1) / 2;
+    a.X.resize(a.nc + 1);
+    for (int I = 0; I <= a.nc; ++I) a.X[I] = std::min(2 * I, nf);
+    a.pi.assign(2 * (nf + 1), -1);
+    a.pw.assign(2 * (nf + 1), 0.f);
+    for (int i = 0; i <= nf; ++i) {
+        int I = -1;
+        if (i % 2 == 0) I = i / 2;
+        else if (i == nf) I = a.nc;
+        if (I >= 0) {
+            a.pi[2 * i] = I;
+            a.pw[2 * i] = 1.f;
+        } else {
+            a.pi[2 * i] = (i - 1) / 2;
+            a.pw[2 * i] = 0.5f;
+            a.pi[2 * i + 1] = (i - 1) / 2 + 1;
+            a.pw[2 * i + 1] = 0.5f;
+        }
+    }
+    a.ri.assign(3 * (a.nc + 1), -1);
+    a.rw.assign(3 * (a.nc + 1), 0.f);
+    std::vector<int> cnt(a.nc + 1, 0);
+    for (int i = 0; i <= nf; ++i)
+        for (int s = 0; s < 2; ++s) {
+            int I = a.pi[2 * i + s];
+            if (I < 0) continue;
+            a.ri[3 * I + cnt[I]] = i;
+            a.rw[3 * I + cnt[I]] = a.pw[2 * i + s];
+            cnt[I]++;
+        }
+    return a;
 }
-void mg_destroy(Multigrid* mg) { delete mg; }
+
+// Device copy of the three axis maps for one level transition.
+struct DevAxes {
+    int* pi[3] = {nullptr, nullptr, nullptr};
+    float* pw[3] = {nullptr, nullptr, nullptr};
+    int* ri[3] = {nullptr, nullptr, nullptr};
+    float* rw[3] = {nullptr, nullptr, nullptr};
+    int* X[3] = {nullptr, nullptr, nullptr};
+};
+
+struct Level {
+    int n[3] = {0, 0, 0};   // cells per axis
+    int64_t n_nodes = 0, n_dof = 0, n_el = 0, nnzb = 0;
+    bool owns_op = false;   // level 0 borrows the context's operator
+    const int32_t* d_rp = nullptr;
+    const int32_t* d_bc = nullptr;
+    const float* d_val = nullptr;
+    const float* d_dinv = nullptr;
+    int32_t *o_rp = nullptr, *o_bc = nullptr, *d_kdiag = nullptr;
+    float *o_val = nullptr, *o_dinv = nullptr;
+    std::vector<int32_t> h_rp, h_bc;
+    // elements (levels >= 1)
+    int32_t* d_conn = nullptr;
+    uint8_t* d_slot = nullptr;
+    float* d_Ke = nullptr;
+    int32_t* d_elems = nullptr;
+    std::vector<int> color_ptr;
+    // transfer to the next finer level (levels >= 1)
+    AxisMap ax[3];
+    DevAxes dax;
+    // penalty Galerkin additions
+    int64_t n_padd = 0;
+    int64_t* d_pidx = nullptr;
+    float* d_pval = nullptr;
+    // work vectors
+    float *x = nullptr, *b = nullptr, *r = nullptr, *d = nullptr;
+    double lmax = 0.0;
+};
+
+}  // namespace
+
+struct Multigrid {
+    std::string err;
+    std::vector<Level> L;
+    const int32_t* d_rank0 = nullptr;  // level-0 grid node -> fine row
+    int32_t* o_rank0 = nullptr;
+    std::vector<int32_t> h_rank0;
+    const double* d_xyz_r = nullptr;
+    const int32_t* d_conn_r = nullptr;
+    int64_t n_el0 = 0;
+    // coarsest dense inverse
+    int nco = 0;
+    double* d_Ainv = nullptr;
+    // FCG work vectors (fine level)
+    float *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+    double* partials = nullptr;
+    double* sv = nullptr;      // device scalars of the FCG loop
+    double* h_sv = nullptr;    // pinned mirror
+    // learned fine smoother
+    int fine_smoother = 0;
+    const float* d_blocks = nullptr;
+    bool ready = false;
+};
+
+namespace {
+
+int mgfail(Multigrid* mg, int code, const char* msg) {
+    if (mg) mg->err = msg;
+    return code;
+}
+
+#define MGCHK(expr)                                                                                   \
+    do {                                                                                              \
+        hipError_t e_ = (expr);                                                                       \
+        if (e_ != hipSuccess) {                                                                       \
+            mg->err = std::string(#expr) + ": " + hipGetErrorString(e_);                              \
+            return ASEND_ERR_HIP;                                                                     \
+        }                                                                                             \
+    } while (0)
+
+template <typename T>
+hipError_t up(T** d, const T* h, size_t n) {
+    hipError_t e = hipMalloc((void**)d, std::max<size_t>(n, 1) * sizeof(T));
+    if (e != hipSuccess) return e;
+    if (n) e = hipMemcpy(*d, h, n * sizeof(T), hipMemcpyHostToDevice);
+    return e;
+}
+
+template <typename T>
+void fr(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+__constant__ int kCorner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+
+// Restriction: coarse node (I,J,K) <- sum of up to 27 fine nodes.
+__global__ void k_restrict3(int ncx, int ncy, int ncz, int nfx, int nfy, const int* __restrict__ rix,
+                            const float* __restrict__ rwx, const int* __restrict__ riy, const float* __restrict__ rwy,
+                            const int* __restrict__ riz, const float* __restrict__ rwz,
+                            const int32_t* __restrict__ frank, const float* __restrict__ rf, float* __restrict__ bc) {
+    int64_t N = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t total = (int64_t)(ncx + 1) * (ncy + 1) * (ncz + 1);
+    if (N >= total) return;
+    int I = (int)(N % (ncx + 1));
+    int J = (int)((N / (ncx + 1)) % (ncy + 1));
+    int K = (int)(N / ((int64_t)(ncx + 1) * (ncy + 1)));
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+    for (int c = 0; c < 3; ++c) {
+        int k = riz[3 * K + c];
+        if (k < 0) continue;
+        float wz = rwz[3 * K + c];
+        for (int b = 0; b < 3; ++b) {
+            int j = riy[3 * J + b];
+            if (j < 0) continue;
+            float wyz = wz * rwy[3 * J + b];
+            for (int a = 0; a < 3; ++a) {
+                int i = rix[3 * I + a];
+                if (i < 0) continue;
+                float w = wyz * rwx[3 * I + a];
+                int64_t g = i + (int64_t)(nfx + 1) * (j + (int64_t)(nfy + 1) * k);
+                int64_t row = frank ? frank[g] : g;
+                s0 += w * rf[3 * row];
+                s1 += w * rf[3 * row + 1];
+                s2 += w * rf[3 * row + 2];
+            }
+        }
+    }
+    bc[3 * N] = s0;
+    bc[3 * N + 1] = s1;
+    bc[3 * N + 2] = s2;
+}
+
+// Prolongation-add: fine node (i,j,k) += sum of up to 8 coarse parents.
+__global__ void k_prolong_add3(int nfx, int nfy, int nfz, int ncx, int ncy, const int* __restrict__ pix,
+                               const float* __restrict__ pwx, const int* __restrict__ piy,
+                               const float* __restrict__ pwy, const int* __restrict__ piz,
+                               const float* __restrict__ pwz, const int32_t* __restrict__ frank,
+                               const float* __restrict__ xc, float* __restrict__ xf) {
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t total = (int64_t)(nfx + 1) * (nfy + 1) * (nfz + 1);
+    if (g >= total) return;
+    int i = (int)(g % (nfx + 1));
+    int j = (int)((g / (nfx + 1)) % (nfy + 1));
+    int k = (int)(g / ((int64_t)(nfx + 1) * (nfy + 1)));
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+    for (int c = 0; c < 2; ++c) {
+        int K = piz[2 * k + c];
+        if (K < 0) continue;
+        float wz = pwz[2 * k + c];
+        for (int b = 0; b < 2; ++b) {
+            int J = piy[2 * j + b];
+            if (J < 0) continue;
+            float wyz = wz * pwy[2 * j + b];
+            for (int a = 0; a < 2; ++a) {
+                int I = pix[2 * i + a];
+                if (I < 0) continue;
+                float w = wyz * pwx[2 * i + a];
+                int64_t N = I + (int64_t)(ncx + 1) * (J + (int64_t)(ncy + 1) * K);
+                s0 += w * xc[3 * N];
+                s1 += w * xc[3 * N + 1];
+                s2 += w * xc[3 * N + 2];
+            }
+        }
+    }
+    int64_t row = frank ? frank[g] : g;
+    xf[3 * row] += s0;
+    xf[3 * row + 1] += s1;
+    xf[3 * row + 2] += s2;
+}
+
+// Level-1 element matrices from the fine mesh.  One wave per coarse element,
+// lane (A, B) owns the 3x3 block; each of the (up to 8) fine cells in the
+// coarse element's index box is integrated with its own 2x2x2 Gauss rule and
+// moduli, using the coarse trilinear basis restricted to that cell.
+__global__ __launch_bounds__(64) void k_level1_ke(int ncx, int ncy, int ncz, int nfx, int nfy,
+                                                  const int* __restrict__ Xx, const int* __restrict__ Xy,
+                                                  const int* __restrict__ Xz, const int32_t* __restrict__ conn_r,
+                                                  const double* __restrict__ xyz_r, const double* __restrict__ E,
+                                                  const double* __restrict__ nu, float* __restrict__ Ke) {
+    const int lane = threadIdx.x;
+    const int64_t ce = blockIdx.x;
+    const int I = (int)(ce % ncx), J = (int)((ce / ncx) % ncy), K = (int)(ce / ((int64_t)ncx * ncy));
+    const int x0 = Xx[I], x1 = Xx[I + 1], y0 = Xy[J], y1 = Xy[J + 1], z0 = Xz[K], z1 = Xz[K + 1];
+    __shared__ float Xs[8][3];
+    __shared__ float Ji[8][9];
+    __shared__ float W[8];
+    __shared__ float Gf[8][8][3];   // fine gradients [gp][a]
+    __shared__ float Gc[8][8][3];   // coarse gradients [gp][A]
+    __shared__ float P[8][8];       // fine node a -> coarse node A weights
+    const int A = lane / 8, B = lane % 8;
+    float k[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    const float g = 0.5773502691896257f;
+    for (int fz = z0; fz < z1; ++fz)
+        for (int fy = y0; fy < y1; ++fy)
+            for (int fx = x0; fx < x1; ++fx) {
+                const int64_t e = fx + (int64_t)nfx * (fy + (int64_t)nfy * fz);
+                __syncthreads();
+                if (lane < 24) {
+                    int a = lane / 3, c = lane % 3;
+                    Xs[a][c] = (float)xyz_r[3 * (int64_t)conn_r[8 * e + a] + c];
+                }
+                {
+                    // P[a][A] for this fine cell: per-axis linear weights in index space
+                    int a = lane / 8, Ac = lane % 8;
+                    float tx = (float)(fx + kCorner[a][0] - x0) / (float)(x1 - x0);
+                    float ty = (float)(fy + kCorner[a][1] - y0) / (float)(y1 - y0);
+                    float tz = (float)(fz + kCorner[a][2] - z0) / (float)(z1 - z0);
+                    float w = (kCorner[Ac][0] ? tx : 1.f - tx) * (kCorner[Ac][1] ? ty : 1.f - ty) *
+                              (kCorner[Ac][2] ? tz : 1.f - tz);
+                    P[a][Ac] = w;
+                }
+                __syncthreads();
+                if (lane < 8) {
+                    const int gp = lane;
+                    float s0 = (gp & 1) ? g : -g, s1 = (gp & 2) ? g : -g, s2 = (gp & 4) ? g : -g;
+                    float Jm[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+                    for (int a = 0; a < 8; ++a) {
+                        float sx = kCorner[a][0] ? 1.f : -1.f, sy = kCorner[a][1] ? 1.f : -1.f,
+                              sz = kCorner[a][2] ? 1.f : -1.f;
+                        float fxx = 1.f + sx * s0, fyy = 1.f + sy * s1, fzz = 1.f + sz * s2;
+                        float d0 = 0.125f * sx * fyy * fzz, d1 = 0.125f * fxx * sy * fzz, d2 = 0.125f * fxx * fyy * sz;
+                        for (int i = 0; i < 3; ++i) {
+                            Jm[i][0] += Xs[a][i] * d0;
+                            Jm[i][1] += Xs[a][i] * d1;
+                            Jm[i][2] += Xs[a][i] * d2;
+                        }
+                    }
+                    float c00 = Jm[1][1] * Jm[2][2] - Jm[1][2] * Jm[2][1];
+                    float c01 = Jm[1][2] * Jm[2][0] - Jm[1][0] * Jm[2][2];
+                    float c02 = Jm[1][0] * Jm[2][1] - Jm[1][1] * Jm[2][0];
+                    float det = Jm[0][0] * c00 + Jm[0][1] * c01 + Jm[0][2] * c02;
+                    float id = 1.f / det;
+                    Ji[gp][0] = c00 * id;
+                    Ji[gp][1] = (Jm[0][2] * Jm[2][1] - Jm[0][1] * Jm[2][2]) * id;
+                    Ji[gp][2] = (Jm[0][1] * Jm[1][2] - Jm[0][2] * Jm[1][1]) * id;
+                    Ji[gp][3] = c01 * id;
+                    Ji[gp][4] = (Jm[0][0] * Jm[2][2] - Jm[0][2] * Jm[2][0]) * id;
+                    Ji[gp][5] = (Jm[0][2] * Jm[1][0] - Jm[0][0] * Jm[1][2]) * id;
+                    Ji[gp][6] = c02 * id;
+                    Ji[gp][7] = (Jm[0][1] * Jm[2][0] - Jm[0][0] * Jm[2][1]) * id;
+                    Ji[gp][8] = (Jm[0][0] * Jm[1][1] - Jm[0][1] * Jm[1][0]) * id;
+                    W[gp] = det;
+                }
+                __syncthreads();
+                {
+                    const int gp = lane / 8, a = lane % 8;
+                    float s0 = (gp & 1) ? g : -g, s1 = (gp & 2) ? g : -g, s2 = (gp & 4) ? g : -g;
+                    float sx = kCorner[a][0] ? 1.f : -1.f, sy = kCorner[a][1] ? 1.f : -1.f,
+                          sz = kCorner[a][2] ? 1.f : -1.f;
+                    float fxx = 1.f + sx * s0, fyy = 1.f + sy * s1, fzz = 1.f + sz * s2;
+                    float d0 = 0.125f * sx * fyy * fzz, d1 = 0.125f * fxx * sy * fzz, d2 = 0.125f * fxx * fyy * sz;
+                    for (int c = 0; c < 3; ++c) Gf[gp][a][c] = d0 * Ji[gp][c] + d1 * Ji[gp][3 + c] + d2 * Ji[gp][6 + c];
+                }
+                __syncthreads();
+                {
+                    const int gp = lane / 8, Ac = lane % 8;
+                    float s[3] = {0.f, 0.f, 0.f};
+                    for (int a = 0; a < 8; ++a) {
+                        float w = P[a][Ac];
+                        s[0] += w * Gf[gp][a][0];
+                        s[1] += w * Gf[gp][a][1];
+                        s[2] += w * Gf[gp][a][2];
+                    }
+                    Gc[gp][Ac][0] = s[0];
+                    Gc[gp][Ac][1] = s[1];
+                    Gc[gp][Ac][2] = s[2];
+                }
+                __syncthreads();
+                const float Ee = (float)E[e], ne = (float)nu[e];
+                const float lam = Ee * ne / ((1.f + ne) * (1.f - 2.f * ne));
+                const float mu = Ee / (2.f * (1.f + ne));
+                for (int gp = 0; gp < 8; ++gp) {
+                    const float w = W[gp];
+                    const float* ga = Gc[gp][A];
+                    const float* gb = Gc[gp][B];
+                    float dot = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+                    for (int i = 0; i < 3; ++i)
+                        for (int j = 0; j < 3; ++j)
+                            k[i][j] += w * (lam * ga[i] * gb[j] + mu * ga[j] * gb[i] + (i == j ? mu * dot : 0.f));
+                }
+            }
+    float* out = Ke + 576 * ce;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) out[(3 * A + i) * 24 + 3 * B + j] = k[i][j];
+}
+
+// Galerkin element matrices of level l+1 from level l: K_E = sum_c P_c^T K_c P_c.
+__global__ __launch_bounds__(64) void k_galerkin_ke(int ncx, int ncy, int ncz, int nfx, int nfy,
+                                                    const int* __restrict__ Xx, const int* __restrict__ Xy,
+                                                    const int* __restrict__ Xz, const float* __restrict__ Kf,
+                                                    float* __restrict__ Kc) {
+    const int lane = threadIdx.x;
+    const int64_t ce = blockIdx.x;
+    const int I = (int)(ce % ncx), J = (int)((ce / ncx) % ncy), K = (int)(ce / ((int64_t)ncx * ncy));
+    const int x0 = Xx[I], x1 = Xx[I + 1], y0 = Xy[J], y1 = Xy[J + 1], z0 = Xz[K], z1 = Xz[K + 1];
+    __shared__ float Ks[576];
+    __shared__ float P[8][8];
+    const int A = lane / 8, B = lane % 8;
+    float k[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int fz = z0; fz < z1; ++fz)
+        for (int fy = y0; fy < y1; ++fy)
+            for (int fx = x0; fx < x1; ++fx) {
+                const int64_t e = fx + (int64_t)nfx * (fy + (int64_t)nfy * fz);
+                __syncthreads();
+                for (int t = lane; t < 576; t += 64) Ks[t] = Kf[576 * e + t];
+                {
+                    int a = lane / 8, Ac = lane % 8;
+                    float tx = (float)(fx + kCorner[a][0] - x0) / (float)(x1 - x0);
+                    float ty = (float)(fy + kCorner[a][1] - y0) / (float)(y1 - y0);
+                    float tz = (float)(fz + kCorner[a][2] - z0) / (float)(z1 - z0);
+                    P[a][Ac] = (kCorner[Ac][0] ? tx : 1.f - tx) * (kCorner[Ac][1] ? ty : 1.f - ty) *
+                               (kCorner[Ac][2] ? tz : 1.f - tz);
+                }
+                __syncthreads();
+                for (int a = 0; a < 8; ++a) {
+                    float wa = P[a][A];
+                    if (wa == 0.f) continue;
+                    for (int b = 0; b < 8; ++b) {
+                        float w = wa * P[b][B];
+                        if (w == 0.f) continue;
+                        for (int i = 0; i < 3; ++i)
+                            for (int j = 0; j < 3; ++j) k[i][j] += w * Ks[(3 * a + i) * 24 + 3 * b + j];
+                    }
+                }
+            }
+    float* out = Kc + 576 * ce;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) out[(3 * A + i) * 24 + 3 * B + j] = k[i][j];
+}
+
+// Coloured scatter of element matrices into the level's component-major BSR.
+__global__ __launch_bounds__(64) void k_asm_ke(const int32_t* __restrict__ elems, const int32_t* __restrict__ conn,
+                                               const float* __restrict__ Ke, const int32_t* __restrict__ rp,
+                                               const uint8_t* __restrict__ slot, float* __restrict__ K) {
+    const int lane = threadIdx.x;
+    const int e = elems[blockIdx.x];
+    const int A = lane / 8, B = lane % 8;
+    const int ra = conn[8 * (int64_t)e + A];
+    const int s = rp[ra], nb = rp[ra + 1] - s;
+    const int kk = slot[64 * (int64_t)e + lane];
+    const float* src = Ke + 576 * (int64_t)e;
+    float* base = K + 9 * (int64_t)s + kk;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) base[(int64_t)(3 * i + j) * nb] += src[(3 * A + i) * 24 + 3 * B + j];
+}
+
+__global__ void k_add_values(int64_t n, const int64_t* __restrict__ idx, const float* __restrict__ v, float scale,
+                             float* __restrict__ K) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) K[idx[i]] += scale * v[i];
+}
+
+__global__ void k_dinv32(int n_rows, const int32_t* __restrict__ rp, const int32_t* __restrict__ kdiag,
+                         const float* __restrict__ K, float* __restrict__ dinv) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_rows) return;
+    int s = rp[r], nb = rp[r + 1] - s;
+    for (int c = 0; c < 3; ++c) dinv[3 * r + c] = 1.f / K[9 * (int64_t)s + (4 * c) * nb + kdiag[r]];
+}
+
+__global__ void k_fill(int64_t n, float v, float* __restrict__ x) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = v;
+}
+
+// Deterministic pseudo-random start vector in [-1, 1] (integer hash of the index).
+__global__ void k_fill_hash(int64_t n, float* __restrict__ x) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t h = (uint32_t)(i * 2654435761ull) ^ 0x9E3779B9u;
+        h ^= h >> 16;
+        h *= 0x7feb352du;
+        h ^= h >> 15;
+        h *= 0x846ca68bu;
+        h ^= h >> 16;
+        x[i] = (float)h * (2.0f / 4294967296.0f) - 1.0f;
+    }
+}
+
+__global__ void k_copy(int64_t n, const float* __restrict__ a, float* __restrict__ b) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
+// Chebyshev, first step from x = 0: d = M r / theta; x = d.
+// M is D^-1 (blocks == nullptr) or the learned 3x3 node blocks.
+__global__ void k_cheb_first(int64_t n_nodes, const float* __restrict__ dinv, const float* __restrict__ blocks,
+                             const float* __restrict__ r, float inv_theta, float* __restrict__ d,
+                             float* __restrict__ x) {
+    for (int64_t nd = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; nd < n_nodes;
+         nd += (int64_t)gridDim.x * blockDim.x) {
+        float r0 = r[3 * nd], r1 = r[3 * nd + 1], r2 = r[3 * nd + 2];
+        float z0, z1, z2;
+        if (blocks) {
+            const float* m = blocks + 9 * nd;
+            z0 = m[0] * r0 + m[1] * r1 + m[2] * r2;
+            z1 = m[3] * r0 + m[4] * r1 + m[5] * r2;
+            z2 = m[6] * r0 + m[7] * r1 + m[8] * r2;
+        } else {
+            z0 = dinv[3 * nd] * r0;
+            z1 = dinv[3 * nd + 1] * r1;
+            z2 = dinv[3 * nd + 2] * r2;
+        }
+        d[3 * nd] = z0 * inv_theta;
+        d[3 * nd + 1] = z1 * inv_theta;
+        d[3 * nd + 2] = z2 * inv_theta;
+        x[3 * nd] = z0 * inv_theta;
+        x[3 * nd + 1] = z1 * inv_theta;
+        x[3 * nd + 2] = z2 * inv_theta;
+    }
+}
+
+// Chebyshev step: d = c1 d + c2 M r; x += d   (r = b - A x of the current x)
+__global__ void k_cheb_step(int64_t n_nodes, const float* __restrict__ dinv, const float* __restrict__ blocks,
+                            const float* __restrict__ r, float c1, float c2, float* __restrict__ d,
+                            float* __restrict__ x) {
+    for (int64_t nd = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; nd < n_nodes;
+         nd += (int64_t)gridDim.x * blockDim.x) {
+        float r0 = r[3 * nd], r1 = r[3 * nd + 1], r2 = r[3 * nd + 2];
+        float z0, z1, z2;
+        if (blocks) {
+            const float* m = blocks + 9 * nd;
+            z0 = m[0] * r0 + m[1] * r1 + m[2] * r2;
+            z1 = m[3] * r0 + m[4] * r1 + m[5] * r2;
+            z2 = m[6] * r0 + m[7] * r1 + m[8] * r2;
+        } else {
+            z0 = dinv[3 * nd] * r0;
+            z1 = dinv[3 * nd + 1] * r1;
+            z2 = dinv[3 * nd + 2] * r2;
+        }
+        float e0 = c1 * d[3 * nd] + c2 * z0, e1 = c1 * d[3 * nd + 1] + c2 * z1, e2 = c1 * d[3 * nd + 2] + c2 * z2;
+        d[3 * nd] = e0;
+        d[3 * nd + 1] = e1;
+        d[3 * nd + 2] = e2;
+        x[3 * nd] += e0;
+        x[3 * nd + 1] += e1;
+        x[3 * nd + 2] += e2;
+    }
+}
+
+// Power iteration helper: w = M (A v) already in w (A v), scale by dinv,
+// partials of |w|^2.
+__global__ __launch_bounds__(kBlock) void k_dinv_scale_norm(int64_t n, const float* __restrict__ dinv,
+                                                            float* __restrict__ w, double* partials) {
+    double t[1] = {0.0};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        float v = dinv[i] * w[i];
+        w[i] = v;
+        t[0] += (double)v * v;
+    }
+    block_partials<1>(t, partials);
+}
+
+__global__ void k_scale(int64_t n, const float* __restrict__ a, float s, float* __restrict__ b) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = s * a[i];
+}
+
+// Sum the partials of `nv` interleaved values into out[0..nv).
+__global__ __launch_bounds__(kBlock) void k_sum_partials(const double* __restrict__ partials, int nparts, int nv,
+                                                         double* __restrict__ out) {
+    double t[4] = {0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x)
+        for (int i = 0; i < nv; ++i) t[i] += partials[(size_t)nv * b + i];
+    block_sum<4>(t);
+    if (threadIdx.x == 0)
+        for (int i = 0; i < nv; ++i) out[i] = t[i];
+}
+
+// Coarsest level: BSR (FP32) -> dense FP64 row-major.
+__global__ void k_bsr_to_dense(int n_rows, const int32_t* __restrict__ rp, const int32_t* __restrict__ bc,
+                               const float* __restrict__ K, double* __restrict__ D) {
+    int r = blockIdx.x;
+    if (r >= n_rows) return;
+    int s = rp[r], nb = rp[r + 1] - s;
+    int n = 3 * n_rows;
+    for (int t = threadIdx.x; t < 9 * nb; t += blockDim.x) {
+        int c = t / nb, k = t % nb;
+        int i = c / 3, j = c % 3;
+        D[(int64_t)(3 * r + i) * n + 3 * bc[s + k] + j] = (double)K[9 * (int64_t)s + t];
+    }
+}
+
+// In-place Gauss-Jordan inverse of an SPD matrix (no pivoting), one
+// workgroup; column k is staged in LDS before each elimination step.
+__global__ __launch_bounds__(1024) void k_gauss_jordan(int n, double* __restrict__ A) {
+    extern __shared__ double colk[];
+    for (int k = 0; k < n; ++k) {
+        for (int i = threadIdx.x; i < n; i += blockDim.x) colk[i] = A[(int64_t)i * n + k];
+        __syncthreads();
+        const double ip = 1.0 / colk[k];
+        for (int j = threadIdx.x; j < n; j += blockDim.x) A[(int64_t)k * n + j] = (j == k ? 1.0 : A[(int64_t)k * n + j]) * ip;
+        __syncthreads();
+        for (int64_t t = threadIdx.x; t < (int64_t)n * n; t += blockDim.x) {
+            int i = (int)(t / n), j = (int)(t % n);
+            if (i == k) continue;
+            double aij = (j == k) ? 0.0 : A[t];
+            A[t] = aij - colk[i] * A[(int64_t)k * n + j];
+        }
+        __syncthreads();
+    }
+}
+
+// Same elimination with the whole matrix resident in LDS (n*n*8 <= 64 KiB).
+__global__ __launch_bounds__(1024) void k_gauss_jordan_lds(int n, double* __restrict__ Ag) {
+    extern __shared__ double sm[];
+    double* A = sm;
+    double* colk = sm + (size_t)n * n;
+    for (int64_t t = threadIdx.x; t < (int64_t)n * n; t += blockDim.x) A[t] = Ag[t];
+    __syncthreads();
+    for (int k = 0; k < n; ++k) {
+        for (int i = threadIdx.x; i < n; i += blockDim.x) colk[i] = A[i * n + k];
+        __syncthreads();
+        const double ip = 1.0 / colk[k];
+        for (int j = threadIdx.x; j < n; j += blockDim.x) A[k * n + j] = (j == k ? 1.0 : A[k * n + j]) * ip;
+        __syncthreads();
+        for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+            int i = t / n, j = t % n;
+            if (i == k) continue;
+            double aij = (j == k) ? 0.0 : A[t];
+            A[t] = aij - colk[i] * A[k * n + j];
+        }
+        __syncthreads();
+    }
+    for (int64_t t = threadIdx.x; t < (int64_t)n * n; t += blockDim.x) Ag[t] = A[t];
+}
+
+// x = Ainv b (FP64 accumulation), one wave per row.
+__global__ __launch_bounds__(256) void k_dense_gemv(int n, const double* __restrict__ Ainv, const float* __restrict__ b,
+                                                    float* __restrict__ x) {
+    int row = blockIdx.x * 4 + threadIdx.x / 64;
+    int lane = threadIdx.x % 64;
+    if (row >= n) return;
+    double s = 0.0;
+    for (int j = lane; j < n; j += 64) s += Ainv[(int64_t)row * n + j] * (double)b[j];
+    s = wave_sum(s);
+    if (lane == 0) x[row] = (float)s;
+}
+
+// FCG helpers --------------------------------------------------------------
+// sv layout: 0 pq, 1 pr, 2 rr, 3 rr0, 4 alpha, 5 beta, 6 zq, 7 tol2, 8 it, 9 max_it, 10 done
+enum { SV_PQ = 0, SV_PR, SV_RR, SV_RR0, SV_ALPHA, SV_BETA, SV_ZQ, SV_TOL2, SV_IT, SV_MAXIT, SV_DONE, SV_N };
+
+__global__ __launch_bounds__(kBlock) void k_dot2(int64_t n, const float* __restrict__ a, const float* __restrict__ b,
+                                                 const float* __restrict__ c, const float* __restrict__ d,
+                                                 double* partials, const double* __restrict__ sv) {
+    if (sv[SV_DONE] != 0.0) return;
+    double t[2] = {0.0, 0.0};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        t[0] += (double)a[i] * b[i];
+        t[1] += (double)c[i] * d[i];
+    }
+    block_partials<2>(t, partials);
+}
+
+// op 0: pq, pr -> alpha ; op 1: rr -> done test ; op 2: zq -> beta ; op 3: rr0 init
+__global__ __launch_bounds__(kBlock) void k_fcg_fin(const double* __restrict__ partials, int nparts, int op,
+                                                    double* __restrict__ sv) {
+    if (op != 3 && sv[SV_DONE] != 0.0) return;
+    const int nv = (op == 0 || op == 2) ? 2 : 1;
+    double t[2] = {0.0, 0.0};
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
+        t[0] += partials[(size_t)nv * b];
+        if (nv == 2) t[1] += partials[(size_t)nv * b + 1];
+    }
+    block_sum<2>(t);
+    if (threadIdx.x != 0) return;
+    if (op == 0) {
+        sv[SV_PQ] = t[0];
+        sv[SV_PR] = t[1];
+        sv[SV_ALPHA] = t[1] / t[0];
+    } else if (op == 1) {
+        sv[SV_RR] = t[0];
+        sv[SV_IT] += 1.0;
+        if (t[0] <= sv[SV_TOL2] * sv[SV_RR0] || sv[SV_IT] >= sv[SV_MAXIT] || !(t[0] == t[0])) sv[SV_DONE] = 1.0;
+    } else if (op == 2) {
+        sv[SV_ZQ] = t[0];
+        sv[SV_BETA] = -t[0] / sv[SV_PQ];
+    } else {
+        sv[SV_RR0] = t[0];
+        sv[SV_RR] = t[0];
+        sv[SV_IT] = 0.0;
+        sv[SV_DONE] = (t[0] == 0.0) ? 1.0 : 0.0;
+    }
+}
+
+// x += alpha p; r -= alpha q; partial r.r
+__global__ __launch_bounds__(kBlock) void k_fcg_update(int64_t n, float* __restrict__ x, float* __restrict__ r,
+                                                       const float* __restrict__ p, const float* __restrict__ q,
+                                                       double* partials, const double* __restrict__ sv) {
+    if (sv[SV_DONE] != 0.0) return;
+    const float alpha = (float)sv[SV_ALPHA];
+    double t[1] = {0.0};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        x[i] += alpha * p[i];
+        float ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        t[0] += (double)ri * ri;
+    }
+    block_partials<1>(t, partials);
+}
+
+// p = z + beta p
+__global__ void k_fcg_dir(int64_t n, const float* __restrict__ z, float* __restrict__ p,
+                          const double* __restrict__ sv) {
+    if (sv[SV_DONE] != 0.0) return;
+    const float beta = (float)sv[SV_BETA];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = z[i] + beta * p[i];
+}
+
+// x = 0, r = b, partial b.b
+__global__ __launch_bounds__(kBlock) void k_fcg_init(int64_t n, const float* __restrict__ b, float* __restrict__ x,
+                                                     float* __restrict__ r, double* partials) {
+    double t[1] = {0.0};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        float v = b[i];
+        x[i] = 0.f;
+        r[i] = v;
+        t[0] += (double)v * v;
+    }
+    block_partials<1>(t, partials);
+}
+
+inline int nb_for(int64_t n, int bs = 256) { return (int)std::min<int64_t>((n + bs - 1) / bs, 4096); }
+
+// ---------------------------------------------------------------------------
+// Host helpers
+// ---------------------------------------------------------------------------
+
+// 27-point node-block pattern of a structured level in natural order, the
+// element slot map and the parity colouring.
+void build_structured_pattern(Level& lv, std::vector<int32_t>& kdiag, std::vector<int32_t>& conn,
+                              std::vector<uint8_t>& slot, std::vector<int32_t>& elems) {
+    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
+    const int mx = nx + 1, my = ny + 1, mz = nz + 1;
+    lv.n_nodes = (int64_t)mx * my * mz;
+    lv.n_dof = 3 * lv.n_nodes;
+    lv.n_el = (int64_t)nx * ny * nz;
+    lv.h_rp.assign(lv.n_nodes + 1, 0);
+    lv.h_bc.clear();
+    kdiag.assign(lv.n_nodes, 0);
+    for (int k = 0; k < mz; ++k)
+        for (int j = 0; j < my; ++j)
+            for (int i = 0; i < mx; ++i) {
+                int64_t r = i + (int64_t)mx * (j + (int64_t)my * k);
+                for (int dk = -1; dk <= 1; ++dk)
+                    for (int dj = -1; dj <= 1; ++dj)
+                        for (int di = -1; di <= 1; ++di) {
+                            int ii = i + di, jj = j + dj, kk = k + dk;
+                            if (ii < 0 || jj < 0 || kk < 0 || ii >= mx || jj >= my || kk >= mz) continue;
+                            int64_t c = ii + (int64_t)mx * (jj + (int64_t)my * kk);
+                            if (c == r) kdiag[r] = (int32_t)(lv.h_bc.size() - lv.h_rp[r]);
+                            lv.h_bc.push_back((int32_t)c);
+                        }
+                lv.h_rp[r + 1] = (int32_t)lv.h_bc.size();
+            }
+    lv.nnzb = (int64_t)lv.h_bc.size();
+    conn.assign(8 * lv.n_el, 0);
+    slot.assign(64 * lv.n_el, 0);
+    static const int cor[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                int64_t e = i + (int64_t)nx * (j + (int64_t)ny * k);
+                for (int a = 0; a < 8; ++a)
+                    conn[8 * e + a] = (int32_t)((i + cor[a][0]) + (int64_t)mx * ((j + cor[a][1]) + (int64_t)my * (k + cor[a][2])));
+                for (int a = 0; a < 8; ++a) {
+                    int64_t r = conn[8 * e + a];
+                    const int32_t* row = lv.h_bc.data() + lv.h_rp[r];
+                    int nbk = lv.h_rp[r + 1] - lv.h_rp[r];
+                    for (int b = 0; b < 8; ++b)
+                        slot[64 * e + 8 * a + b] = (uint8_t)(std::lower_bound(row, row + nbk, conn[8 * e + b]) - row);
+                }
+            }
+    // 8-colour parity colouring
+    lv.color_ptr.assign(9, 0);
+    elems.resize(lv.n_el);
+    std::vector<int> cnt(8, 0);
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) cnt[(i & 1) + 2 * (j & 1) + 4 * (k & 1)]++;
+    for (int c = 0; c < 8; ++c) lv.color_ptr[c + 1] = lv.color_ptr[c] + cnt[c];
+    std::vector<int> fill(lv.color_ptr.begin(), lv.color_ptr.end() - 1);
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i)
+                elems[fill[(i & 1) + 2 * (j & 1) + 4 * (k & 1)]++] = (int32_t)(i + (int64_t)nx * (j + (int64_t)ny * k));
+}
+
+int apply_level(Multigrid* mg, hipStream_t st, int l, const float* x, float* y) {
+    Level& lv = mg->L[l];
+    launch_spmv3<float>(st, 0, (int)lv.n_nodes, lv.d_rp, lv.d_bc, lv.d_val, x, y, nullptr, nullptr, nullptr);
+    return ASEND_OK;
+}
+
+void residual_level(hipStream_t st, Level& lv, const float* b, const float* x, float* r) {
+    launch_spmv3<float>(st, 3, (int)lv.n_nodes, lv.d_rp, lv.d_bc, lv.d_val, x, r, b, nullptr, nullptr);
+}
+
+void smooth(Multigrid* mg, hipStream_t st, int l, const float* b, float* x, bool zero_start) {
+    Level& lv = mg->L[l];
+    const float* blocks = (l == 0 && mg->fine_smoother == 1) ? mg->d_blocks : nullptr;
+    const double hi = lv.lmax * 1.15, lo = lv.lmax * 0.1;
+    const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+    double rho0 = 1.0 / sigma;
+    const int nblk = nb_for(lv.n_nodes);
+    if (zero_start) {
+        hipLaunchKernelGGL(k_cheb_first, dim3(nblk), dim3(256), 0, st, lv.n_nodes, lv.d_dinv, blocks, b,
+                           (float)(1.0 / theta), lv.d, x);
+    } else {
+        residual_level(st, lv, b, x, lv.r);
+        hipLaunchKernelGGL(k_cheb_step, dim3(nblk), dim3(256), 0, st, lv.n_nodes, lv.d_dinv, blocks, lv.r, 0.f,
+                           (float)(1.0 / theta), lv.d, x);
+    }
+    for (int k = 1; k < kChebDegree; ++k) {
+        residual_level(st, lv, b, x, lv.r);
+        double rho1 = 1.0 / (2.0 * sigma - rho0);
+        hipLaunchKernelGGL(k_cheb_step, dim3(nblk), dim3(256), 0, st, lv.n_nodes, lv.d_dinv, blocks, lv.r,
+                           (float)(rho1 * rho0), (float)(2.0 * rho1 / delta), lv.d, x);
+        rho0 = rho1;
+    }
+}
+
+void restrict_to(Multigrid* mg, hipStream_t st, int l, const float* rf, float* bc) {
+    Level& c = mg->L[l + 1];
+    Level& f = mg->L[l];
+    const int32_t* frank = (l == 0) ? mg->d_rank0 : nullptr;
+    int64_t nnodes = c.n_nodes;
+    hipLaunchKernelGGL(k_restrict3, dim3((unsigned)((nnodes + 255) / 256)), dim3(256), 0, st, c.n[0], c.n[1], c.n[2],
+                       f.n[0], f.n[1], c.dax.ri[0], c.dax.rw[0], c.dax.ri[1], c.dax.rw[1], c.dax.ri[2], c.dax.rw[2],
+                       frank, rf, bc);
+}
+
+void prolong_add(Multigrid* mg, hipStream_t st, int l, const float* xc, float* xf) {
+    Level& c = mg->L[l + 1];
+    Level& f = mg->L[l];
+    const int32_t* frank = (l == 0) ? mg->d_rank0 : nullptr;
+    int64_t nnodes = f.n_nodes;
+    hipLaunchKernelGGL(k_prolong_add3, dim3((unsigned)((nnodes + 255) / 256)), dim3(256), 0, st, f.n[0], f.n[1],
+                       f.n[2], c.n[0], c.n[1], c.dax.pi[0], c.dax.pw[0], c.dax.pi[1], c.dax.pw[1], c.dax.pi[2],
+                       c.dax.pw[2], frank, xc, xf);
+}
+
+// One V-cycle: x = M b on level l (x overwritten).
+void vcycle(Multigrid* mg, hipStream_t st, int l, const float* b, float* x) {
+    const int nl = (int)mg->L.size();
+    if (l == nl - 1) {
+        hipLaunchKernelGGL(k_dense_gemv, dim3((mg->nco + 3) / 4), dim3(256), 0, st, mg->nco, mg->d_Ainv, b, x);
+        return;
+    }
+    Level& lv = mg->L[l];
+    Level& cv = mg->L[l + 1];
+    smooth(mg, st, l, b, x, true);
+    residual_level(st, lv, b, x, lv.r);
+    restrict_to(mg, st, l, lv.r, cv.b);
+    vcycle(mg, st, l + 1, cv.b, cv.x);
+    prolong_add(mg, st, l, cv.x, x);
+    smooth(mg, st, l, b, x, false);
+}
+
+// lambda_max of M A by power iteration (M = D^-1).
+int estimate_lmax(Multigrid* mg, hipStream_t st, int l) {
+    Level& lv = mg->L[l];
+    const int64_t n = lv.n_dof;
+    // deterministic pseudo-random start vector (a smooth start such as the
+    // constant vector lies near the rigid-body modes and converges slowly)
+    hipLaunchKernelGGL(k_fill_hash, dim3(nb_for(n)), dim3(256), 0, st, n, lv.x);
+    double lam = 0.0;
+    const int nblk = kVecBlocks;
+    for (int it = 0; it < kPowerIts; ++it) {
+        apply_level(mg, st, l, lv.x, lv.r);
+        hipLaunchKernelGGL(k_dinv_scale_norm, dim3(nblk), dim3(kBlock), 0, st, n, lv.d_dinv, lv.r, mg->partials);
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, st, mg->partials, nblk, 1, mg->sv + SV_N);
+        MGCHK(hipMemcpyAsync(mg->h_sv, mg->sv + SV_N, sizeof(double), hipMemcpyDeviceToHost, st));
+        MGCHK(hipStreamSynchronize(st));
+        double nrm = std::sqrt(mg->h_sv[0]);
+        if (!(nrm > 0.0) || !std::isfinite(nrm)) return mgfail(mg, ASEND_ERR_NUMERIC, "power iteration broke down");
+        lam = nrm;
+        hipLaunchKernelGGL(k_scale, dim3(nb_for(n)), dim3(256), 0, st, n, lv.r, (float)(1.0 / nrm), lv.x);
+    }
+    lv.lmax = lam;
+    return ASEND_OK;
+}
+
+void free_level(Level& lv) {
+    fr(lv.o_rp);
+    fr(lv.o_bc);
+    fr(lv.o_val);
+    fr(lv.o_dinv);
+    fr(lv.d_kdiag);
+    fr(lv.d_conn);
+    fr(lv.d_slot);
+    fr(lv.d_Ke);
+    fr(lv.d_elems);
+    fr(lv.d_pidx);
+    fr(lv.d_pval);
+    fr(lv.x);
+    fr(lv.b);
+    fr(lv.r);
+    fr(lv.d);
+    for (int a = 0; a < 3; ++a) {
+        fr(lv.dax.pi[a]);
+        fr(lv.dax.pw[a]);
+        fr(lv.dax.ri[a]);
+        fr(lv.dax.rw[a]);
+        fr(lv.dax.X[a]);
+    }
+}
+
+}  // namespace
+
+// ===========================================================================
+
+int mg_create(Multigrid** out, int nx, int ny, int nz, const int32_t* rank_of_grid, const double* d_xyz_r,
+              const int32_t* d_conn_r, const int32_t* d_rp, const int32_t* d_bc, const uint8_t* d_slot,
+              const int32_t* h_rp, const int32_t* h_bc, int64_t n_nodes, int64_t n_el) {
+    (void)d_slot;
+    (void)h_rp;
+    (void)h_bc;
+    Multigrid* mg = new Multigrid();
+    *out = mg;
+    mg->d_xyz_r = d_xyz_r;
+    mg->d_conn_r = d_conn_r;
+    mg->n_el0 = n_el;
+    mg->h_rank0.assign(rank_of_grid, rank_of_grid + n_nodes);
+    MGCHK(up(&mg->o_rank0, rank_of_grid, (size_t)n_nodes));
+    mg->d_rank0 = mg->o_rank0;
+    // level 0 = fine grid (operator borrowed at setup)
+    Level l0;
+    l0.n[0] = nx;
+    l0.n[1] = ny;
+    l0.n[2] = nz;
+    l0.n_nodes = n_nodes;
+    l0.n_dof = 3 * n_nodes;
+    l0.n_el = n_el;
+    l0.d_rp = d_rp;
+    l0.d_bc = d_bc;
+    mg->L.push_back(l0);
+    // coarse levels
+    while (true) {
+        Level& f = mg->L.back();
+        if (f.n_nodes <= kMaxCoarseNodes || (f.n[0] == 1 && f.n[1] == 1 && f.n[2] == 1)) break;
+        Level c;
+        for (int a = 0; a < 3; ++a) {
+            c.ax[a] = make_axis(f.n[a]);
+            c.n[a] = c.ax[a].nc;
+        }
+        std::vector<int32_t> kdiag, conn, elems;
+        std::vector<uint8_t> slot;
+        build_structured_pattern(c, kdiag, conn, slot, elems);
+        c.owns_op = true;
+        MGCHK(up(&c.o_rp, c.h_rp.data(), c.h_rp.size()));
+        MGCHK(up(&c.o_bc, c.h_bc.data(), c.h_bc.size()));
+        MGCHK(up(&c.d_kdiag, kdiag.data(), kdiag.size()));
+        MGCHK(up(&c.d_conn, conn.data(), conn.size()));
+        MGCHK(up(&c.d_slot, slot.data(), slot.size()));
+        MGCHK(up(&c.d_elems, elems.data(), elems.size()));
+        MGCHK(hipMalloc((void**)&c.o_val, 9 * c.nnzb * sizeof(float)));
+        MGCHK(hipMalloc((void**)&c.o_dinv, c.n_dof * sizeof(float)));
+        MGCHK(hipMalloc((void**)&c.d_Ke, 576 * c.n_el * sizeof(float)));
+        c.d_rp = c.o_rp;
+        c.d_bc = c.o_bc;
+        c.d_val = c.o_val;
+        c.d_dinv = c.o_dinv;
+        for (int a = 0; a < 3; ++a) {
+            MGCHK(up(&c.dax.pi[a], c.ax[a].pi.data(), c.ax[a].pi.size()));
+            MGCHK(up(&c.dax.pw[a], c.ax[a].pw.data(), c.ax[a].pw.size()));
+            MGCHK(up(&c.dax.ri[a], c.ax[a].ri.data(), c.ax[a].ri.size()));
+            MGCHK(up(&c.dax.rw[a], c.ax[a].rw.data(), c.ax[a].rw.size()));
+            MGCHK(up(&c.dax.X[a], c.ax[a].X.data(), c.ax[a].X.size()));
+        }
+        mg->L.push_back(c);
+        if (mg->L.size() > 16) break;
+    }
+    if (mg->L.size() < 2) return mgfail(mg, ASEND_ERR_UNSUPPORTED, "grid too small for a multigrid hierarchy");
+    for (auto& lv : mg->L) {
+        MGCHK(hipMalloc((void**)&lv.x, lv.n_dof * sizeof(float)));
+        MGCHK(hipMalloc((void**)&lv.b, lv.n_dof * sizeof(float)));
+        MGCHK(hipMalloc((void**)&lv.r, lv.n_dof * sizeof(float)));
+        MGCHK(hipMalloc((void**)&lv.d, lv.n_dof * sizeof(float)));
+    }
+    mg->nco = (int)mg->L.back().n_dof;
+    MGCHK(hipMalloc((void**)&mg->d_Ainv, (size_t)mg->nco * mg->nco * sizeof(double)));
+    const int64_t n0 = mg->L[0].n_dof;
+    MGCHK(hipMalloc((void**)&mg->r, n0 * sizeof(float)));
+    MGCHK(hipMalloc((void**)&mg->z, n0 * sizeof(float)));
+    MGCHK(hipMalloc((void**)&mg->p, n0 * sizeof(float)));
+    MGCHK(hipMalloc((void**)&mg->q, n0 * sizeof(float)));
+    MGCHK(hipMalloc((void**)&mg->partials, 2 * 4096 * sizeof(double)));
+    MGCHK(hipMalloc((void**)&mg->sv, 32 * sizeof(double)));
+    MGCHK(hipHostMalloc((void**)&mg->h_sv, 32 * sizeof(double), hipHostMallocDefault));
+    return ASEND_OK;
+}
+
+void mg_destroy(Multigrid* mg) {
+    if (!mg) return;
+    for (auto& lv : mg->L) free_level(lv);
+    fr(mg->o_rank0);
+    fr(mg->d_Ainv);
+    fr(mg->r);
+    fr(mg->z);
+    fr(mg->p);
+    fr(mg->q);
+    fr(mg->partials);
+    fr(mg->sv);
+    if (mg->h_sv) (void)hipHostFree(mg->h_sv);
+    delete mg;
+}
+
 const char* mg_last_error(const Multigrid* mg) { return mg ? mg->err.c_str() : ""; }
-int mg_setup(Multigrid*, hipStream_t, const double*, const double*, const float*, const float*, double) {
-    return ASEND_ERR_UNSUPPORTED;
+
+int mg_set_fine_smoother(Multigrid* mg, int kind) {
+    if (kind != 0 && kind != 1) return mgfail(mg, ASEND_ERR_ARG, "unknown smoother");
+    if (kind == 1 && !mg->d_blocks) return mgfail(mg, ASEND_ERR_STATE, "no learned blocks set");
+    mg->fine_smoother = kind;
+    return ASEND_OK;
 }
-int mg_fcg_solve(Multigrid*, hipStream_t, const float*, float*, double, int64_t, int, int64_t*, int*) {
-    return ASEND_ERR_UNSUPPORTED;
+
+int mg_set_learned_blocks(Multigrid* mg, const float* d_blocks) {
+    mg->d_blocks = d_blocks;
+    return ASEND_OK;
+}
+
+int mg_setup(Multigrid* mg, hipStream_t st, const double* d_E, const double* d_nu, const float* d_K32,
+             const float* d_dinv32, double sf, int n_pen, const int64_t* h_pen, const int32_t* h_mult) {
+    mg->ready = false;
+    Level& l0 = mg->L[0];
+    l0.d_val = d_K32;
+    l0.d_dinv = d_dinv32;
+    // penalty of the fine level per component as a node-level diagonal
+    // (grid node -> weight), used for the Galerkin coarse penalty
+    std::vector<int32_t> grid_of_rank(l0.n_nodes);
+    for (int64_t g = 0; g < l0.n_nodes; ++g) grid_of_rank[mg->h_rank0[g]] = (int32_t)g;
+    std::vector<std::map<std::pair<int64_t, int64_t>, double>> S(3);
+    for (int i = 0; i < n_pen; ++i) {
+        int64_t d = h_pen[i];
+        int64_t g = grid_of_rank[d / 3];
+        S[d % 3][{g, g}] += (double)h_mult[i];
+    }
+    for (size_t l = 1; l < mg->L.size(); ++l) {
+        Level& c = mg->L[l];
+        Level& f = mg->L[l - 1];
+        const int fmx = f.n[0] + 1, fmy = f.n[1] + 1;
+        const int cmx = c.n[0] + 1, cmy = c.n[1] + 1;
+        // element matrices
+        if (l == 1) {
+            hipLaunchKernelGGL(k_level1_ke, dim3((unsigned)c.n_el), dim3(64), 0, st, c.n[0], c.n[1], c.n[2], f.n[0],
+                               f.n[1], c.dax.X[0], c.dax.X[1], c.dax.X[2], mg->d_conn_r, mg->d_xyz_r, d_E, d_nu,
+                               c.d_Ke);
+        } else {
+            hipLaunchKernelGGL(k_galerkin_ke, dim3((unsigned)c.n_el), dim3(64), 0, st, c.n[0], c.n[1], c.n[2], f.n[0],
+                               f.n[1], c.dax.X[0], c.dax.X[1], c.dax.X[2], f.d_Ke, c.d_Ke);
+        }
+        // assemble
+        hipLaunchKernelGGL(k_fill, dim3(nb_for(9 * c.nnzb)), dim3(256), 0, st, 9 * c.nnzb, 0.f, c.o_val);
+        for (int col = 0; col < 8; ++col) {
+            int b = c.color_ptr[col], e = c.color_ptr[col + 1];
+            if (e > b)
+                hipLaunchKernelGGL(k_asm_ke, dim3(e - b), dim3(64), 0, st, c.d_elems + b, c.d_conn, c.d_Ke, c.o_rp,
+                                   c.d_slot, c.o_val);
+        }
+        // Galerkin penalty: S_c = P^T S_f P per component
+        std::vector<std::map<std::pair<int64_t, int64_t>, double>> Sc(3);
+        for (int comp = 0; comp < 3; ++comp)
+            for (auto& kv : S[comp]) {
+                int64_t g1 = kv.first.first, g2 = kv.first.second;
+                int i1 = (int)(g1 % fmx), j1 = (int)((g1 / fmx) % fmy), k1 = (int)(g1 / ((int64_t)fmx * fmy));
+                int i2 = (int)(g2 % fmx), j2 = (int)((g2 / fmx) % fmy), k2 = (int)(g2 / ((int64_t)fmx * fmy));
+                for (int a1 = 0; a1 < 8; ++a1) {
+                    int I1 = c.ax[0].pi[2 * i1 + (a1 & 1)], J1 = c.ax[1].pi[2 * j1 + ((a1 >> 1) & 1)],
+                        K1 = c.ax[2].pi[2 * k1 + ((a1 >> 2) & 1)];
+                    if (I1 < 0 || J1 < 0 || K1 < 0) continue;
+                    double w1 = (double)c.ax[0].pw[2 * i1 + (a1 & 1)] * c.ax[1].pw[2 * j1 + ((a1 >> 1) & 1)] *
+                                c.ax[2].pw[2 * k1 + ((a1 >> 2) & 1)];
+                    int64_t N1 = I1 + (int64_t)cmx * (J1 + (int64_t)cmy * K1);
+                    for (int a2 = 0; a2 < 8; ++a2) {
+                        int I2 = c.ax[0].pi[2 * i2 + (a2 & 1)], J2 = c.ax[1].pi[2 * j2 + ((a2 >> 1) & 1)],
+                            K2 = c.ax[2].pi[2 * k2 + ((a2 >> 2) & 1)];
+                        if (I2 < 0 || J2 < 0 || K2 < 0) continue;
+                        double w2 = (double)c.ax[0].pw[2 * i2 + (a2 & 1)] * c.ax[1].pw[2 * j2 + ((a2 >> 1) & 1)] *
+                                    c.ax[2].pw[2 * k2 + ((a2 >> 2) & 1)];
+                        int64_t N2 = I2 + (int64_t)cmx * (J2 + (int64_t)cmy * K2);
+                        Sc[comp][{N1, N2}] += w1 * kv.second * w2;
+                    }
+                }
+            }
+        S.swap(Sc);
+        std::vector<int64_t> pidx;
+        std::vector<float> pval;
+        for (int comp = 0; comp < 3; ++comp)
+            for (auto& kv : S[comp]) {
+                int64_t N1 = kv.first.first, N2 = kv.first.second;
+                const int32_t* row = c.h_bc.data() + c.h_rp[N1];
+                int nbk = c.h_rp[N1 + 1] - c.h_rp[N1];
+                const int32_t* it = std::lower_bound(row, row + nbk, (int32_t)N2);
+                if (it == row + nbk || *it != N2) return mgfail(mg, ASEND_ERR_STATE, "penalty coupling outside pattern");
+                int64_t slotk = it - row;
+                pidx.push_back(9 * (int64_t)c.h_rp[N1] + (4 * comp) * nbk + slotk);
+                pval.push_back((float)kv.second);
+            }
+        fr(c.d_pidx);
+        fr(c.d_pval);
+        c.n_padd = (int64_t)pidx.size();
+        MGCHK(up(&c.d_pidx, pidx.data(), pidx.size()));
+        MGCHK(up(&c.d_pval, pval.data(), pval.size()));
+        if (c.n_padd)
+            hipLaunchKernelGGL(k_add_values, dim3((unsigned)((c.n_padd + 255) / 256)), dim3(256), 0, st, c.n_padd,
+                               c.d_pidx, c.d_pval, (float)sf, c.o_val);
+        hipLaunchKernelGGL(k_dinv32, dim3((unsigned)((c.n_nodes + 255) / 256)), dim3(256), 0, st, (int)c.n_nodes,
+                           c.o_rp, c.d_kdiag, c.o_val, c.o_dinv);
+    }
+    MGCHK(hipGetLastError());
+    // smoother bounds (all but the coarsest level)
+    for (size_t l = 0; l + 1 < mg->L.size(); ++l) {
+        int rc = estimate_lmax(mg, st, (int)l);
+        if (rc) return rc;
+    }
+    // coarsest: dense FP64 inverse
+    Level& lc = mg->L.back();
+    hipLaunchKernelGGL(k_fill, dim3(nb_for((int64_t)mg->nco * mg->nco / 2)), dim3(256), 0, st,
+                       (int64_t)mg->nco * mg->nco * 2, 0.f, (float*)mg->d_Ainv);
+    hipLaunchKernelGGL(k_bsr_to_dense, dim3((unsigned)lc.n_nodes), dim3(64), 0, st, (int)lc.n_nodes, lc.d_rp,
+                       lc.d_bc, lc.d_val, mg->d_Ainv);
+    if ((size_t)mg->nco * (mg->nco + 1) * sizeof(double) <= 65536)
+        hipLaunchKernelGGL(k_gauss_jordan_lds, dim3(1), dim3(1024), (size_t)mg->nco * (mg->nco + 1) * sizeof(double), st,
+                           mg->nco, mg->d_Ainv);
+    else
+        hipLaunchKernelGGL(k_gauss_jordan, dim3(1), dim3(1024), mg->nco * sizeof(double), st, mg->nco, mg->d_Ainv);
+    MGCHK(hipGetLastError());
+    MGCHK(hipStreamSynchronize(st));
+    mg->ready = true;
+    return ASEND_OK;
+}
+
+int mg_fcg_solve(Multigrid* mg, hipStream_t st, const float* b, float* x, double rel_tol, int64_t max_it,
+                 int check_every, int64_t* it_out, int* fallback_used) {
+    if (!mg->ready) return mgfail(mg, ASEND_ERR_STATE, "multigrid not set up");
+    Level& l0 = mg->L[0];
+    const int64_t n = l0.n_dof;
+    const int nvb = kVecBlocks;
+    if (fallback_used) *fallback_used = 0;
+    // scalars
+    double init[SV_N];
+    std::memset(init, 0, sizeof(init));
+    init[SV_TOL2] = rel_tol * rel_tol;
+    init[SV_MAXIT] = (double)max_it;
+    std::memcpy(mg->h_sv, init, sizeof(init));
+    MGCHK(hipMemcpyAsync(mg->sv, mg->h_sv, sizeof(init), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_fcg_init, dim3(nvb), dim3(kBlock), 0, st, n, b, x, mg->r, mg->partials);
+    hipLaunchKernelGGL(k_fcg_fin, dim3(1), dim3(kBlock), 0, st, mg->partials, nvb, 3, mg->sv);
+    // z = M r ; p = z
+    vcycle(mg, st, 0, mg->r, mg->z);
+    hipLaunchKernelGGL(k_copy, dim3(nb_for(n)), dim3(256), 0, st, n, mg->z, mg->p);
+    int64_t it = 0;
+    for (;;) {
+        for (int k = 0; k < check_every; ++k) {
+            // q = A p ; alpha = (p.r)/(p.q)
+            apply_level(mg, st, 0, mg->p, mg->q);
+            hipLaunchKernelGGL(k_dot2, dim3(nvb), dim3(kBlock), 0, st, n, mg->p, mg->q, mg->p, mg->r, mg->partials,
+                               mg->sv);
+            hipLaunchKernelGGL(k_fcg_fin, dim3(1), dim3(kBlock), 0, st, mg->partials, nvb, 0, mg->sv);
+            hipLaunchKernelGGL(k_fcg_update, dim3(nvb), dim3(kBlock), 0, st, n, x, mg->r, mg->p, mg->q, mg->partials,
+                               mg->sv);
+            hipLaunchKernelGGL(k_fcg_fin, dim3(1), dim3(kBlock), 0, st, mg->partials, nvb, 1, mg->sv);
+            // z = M r ; beta = -(z.q)/(p.q) ; p = z + beta p
+            vcycle(mg, st, 0, mg->r, mg->z);
+            hipLaunchKernelGGL(k_dot2, dim3(nvb), dim3(kBlock), 0, st, n, mg->z, mg->q, mg->z, mg->q, mg->partials,
+                               mg->sv);
+            hipLaunchKernelGGL(k_fcg_fin, dim3(1), dim3(kBlock), 0, st, mg->partials, nvb, 2, mg->sv);
+            hipLaunchKernelGGL(k_fcg_dir, dim3(nvb), dim3(kBlock), 0, st, n, mg->z, mg->p, mg->sv);
+        }
+        MGCHK(hipGetLastError());
+        MGCHK(hipMemcpyAsync(mg->h_sv, mg->sv, SV_N * sizeof(double), hipMemcpyDeviceToHost, st));
+        MGCHK(hipStreamSynchronize(st));
+        it = (int64_t)mg->h_sv[SV_IT];
+        if (mg->h_sv[SV_DONE] != 0.0) break;
+    }
+    if (it_out) *it_out = it;
+    if (!(mg->h_sv[SV_RR] == mg->h_sv[SV_RR]))
+        return mgfail(mg, ASEND_ERR_NUMERIC, "multigrid FCG produced a non-finite residual");
+    return ASEND_OK;
+}
+
+int mg_levels(const Multigrid* mg) { return (int)mg->L.size(); }
+int64_t mg_level_ndof(const Multigrid* mg, int l) { return mg->L[l].n_dof; }
+
+int mg_apply_level(Multigrid* mg, hipStream_t st, int l, const float* d_x, float* d_y) {
+    if (l < 0 || l >= (int)mg->L.size()) return mgfail(mg, ASEND_ERR_ARG, "bad level");
+    return apply_level(mg, st, l, d_x, d_y);
+}
+
+int mg_restrict(Multigrid* mg, hipStream_t st, int l, const float* d_fine, float* d_coarse) {
+    if (l < 0 || l + 1 >= (int)mg->L.size()) return mgfail(mg, ASEND_ERR_ARG, "bad level");
+    restrict_to(mg, st, l, d_fine, d_coarse);
+    return ASEND_OK;
+}
+
+int mg_prolong(Multigrid* mg, hipStream_t st, int l, const float* d_coarse, float* d_fine) {
+    if (l < 0 || l + 1 >= (int)mg->L.size()) return mgfail(mg, ASEND_ERR_ARG, "bad level");
+    hipLaunchKernelGGL(k_fill, dim3(nb_for(mg->L[l].n_dof)), dim3(256), 0, st, mg->L[l].n_dof, 0.f, d_fine);
+    prolong_add(mg, st, l, d_coarse, d_fine);
+    return ASEND_OK;
+}
+
+int mg_vcycle(Multigrid* mg, hipStream_t st, const float* d_b, float* d_x) {
+    if (!mg->ready) return mgfail(mg, ASEND_ERR_STATE, "multigrid not set up");
+    vcycle(mg, st, 0, d_b, d_x);
+    return ASEND_OK;
 }
 
 }  // namespace asend

@@ -84,7 +84,7 @@ def main():
     if args.fast_precond in ("auto", "mg", "learned"):
         try:
             ctx.set_grid(args.nx, args.ny, args.nz)
-            pc = api.PC_LEARNED if args.fast_precond in ("auto", "learned") else api.PC_MG
+            pc = api.PC_LEARNED if args.fast_precond == "learned" else api.PC_MG
         except api.AsendError as e:
             if args.fast_precond != "auto":
                 raise

@@ -199,6 +199,16 @@ int asend_static_compliance(asend_ctx* ctx, const double* f, double* u, double* 
  * SpMV kernel of the given precision, HIP events on the context stream. */
 int asend_bench_spmv(asend_ctx* ctx, int precision, int n_iter, double* avg_ms);
 
+/* ---- multigrid diagnostics (no reference counterpart) --------------------
+ * Host FP32 arrays.  Level 0 uses the fine DOF numbering, coarser levels the
+ * lexicographic node order of their grid.  Used to verify on the device that
+ * A_{l+1} = P^T A_l P and that one V-cycle is a symmetric positive operator. */
+int asend_mg_info(asend_ctx* ctx, int* n_levels, int64_t* level_ndof /* [16] */);
+int asend_mg_apply(asend_ctx* ctx, int level, const float* x, float* y);
+int asend_mg_restrict(asend_ctx* ctx, int level, const float* fine, float* coarse);
+int asend_mg_prolong(asend_ctx* ctx, int level, const float* coarse, float* fine);
+int asend_mg_vcycle(asend_ctx* ctx, const float* b, float* x);
+
 #ifdef __cplusplus
 }
 #endif
