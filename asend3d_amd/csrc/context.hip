@@ -33,6 +33,19 @@ struct asend_ctx {
     std::vector<int32_t> h_rp, h_bc, h_color_ptr;
 
     int32_t *d_conn_r = nullptr, *d_rp = nullptr, *d_bc = nullptr, *d_kdiag = nullptr, *d_elems = nullptr;
+    // fine operator storage: sliced ELL of 3x3 blocks, 64-row slices
+    int64_t n_slices = 0, n_slots = 0;
+    std::vector<int32_t> h_sptr;
+    int32_t *d_sptr = nullptr, *d_scol = nullptr;
+    Sell3 fine() const {
+        Sell3 A;
+        A.n_rows = (int)n_nodes;
+        A.n_slices = (int)n_slices;
+        A.sptr = d_sptr;
+        A.scol = d_scol;
+        return A;
+    }
+    int64_t nval() const { return 9 * 64 * n_slots; }
     uint8_t* d_slot = nullptr;
     double *d_xyz_r = nullptr, *d_E = nullptr, *d_nu = nullptr, *d_rho = nullptr;
     double* d_K64 = nullptr;
@@ -127,6 +140,8 @@ void free_mesh(asend_ctx* ctx) {
     dfree(ctx->d_conn_r);
     dfree(ctx->d_rp);
     dfree(ctx->d_bc);
+    dfree(ctx->d_sptr);
+    dfree(ctx->d_scol);
     dfree(ctx->d_kdiag);
     dfree(ctx->d_elems);
     dfree(ctx->d_slot);
@@ -198,7 +213,7 @@ int run_jacobi_pcg(asend_ctx* ctx, int si, const T* b, T* x, double tol, bool st
     launch_pcg_init<T>(ctx->st, n, b, dinv, x, r, p, ctx->d_partials, sc);
     for (;;) {
         for (int k = 0; k < check_every; ++k) {
-            launch_spmv3<T>(ctx->st, 1, nr, ctx->d_rp, ctx->d_bc, K, p, q, nullptr, ctx->d_partials, sc);
+            launch_sell3<T>(ctx->st, 1, ctx->fine(), K, p, q, nullptr, ctx->d_partials, sc);
             launch_pcg_update<T>(ctx->st, n, dinv, x, r, p, q, ctx->d_partials, sc);
             launch_pcg_dir<T>(ctx->st, n, dinv, r, p, sc);
         }
@@ -211,7 +226,7 @@ int run_jacobi_pcg(asend_ctx* ctx, int si, const T* b, T* x, double tol, bool st
 
 // FP64 residual r = f - K u, sc[0].rr = ||r||^2.
 int residual64(asend_ctx* ctx, const double* f, const double* u, double* r) {
-    launch_spmv3<double>(ctx->st, 2, (int)ctx->n_nodes, ctx->d_rp, ctx->d_bc, ctx->d_K64, u, r, f, ctx->d_partials, ctx->d_sc + 0);
+    launch_sell3<double>(ctx->st, 2, ctx->fine(), ctx->d_K64, u, r, f, ctx->d_partials, ctx->d_sc + 0);
     HIPCHK(hipGetLastError());
     return ASEND_OK;
 }
@@ -324,13 +339,13 @@ int solve_device(asend_ctx* ctx, const asend_solve_opts* o, asend_solve_stats* s
 }
 
 int assemble_device(asend_ctx* ctx, double c_mass) {
-    const int64_t nval = ctx->nnzb * ctx->bs * ctx->bs;
+    const int64_t nval = ctx->nval();
     HIPCHK(hipMemsetAsync(ctx->d_K64, 0, nval * sizeof(double), ctx->st));
     int ncol = (int)ctx->h_color_ptr.size() - 1;
     for (int c = 0; c < ncol; ++c) {
         int b = ctx->h_color_ptr[c], e = ctx->h_color_ptr[c + 1];
         launch_hex8_assemble(ctx->st, ctx->d_elems + b, e - b, ctx->d_conn_r, ctx->d_xyz_r, ctx->d_E, ctx->d_nu,
-                             ctx->d_rho, c_mass, ctx->d_rp, ctx->d_slot, ctx->d_K64);
+                             ctx->d_rho, c_mass, ctx->d_sptr, ctx->d_slot, ctx->d_K64);
     }
     HIPCHK(hipGetLastError());
     int rc;
@@ -349,10 +364,10 @@ int assemble_device(asend_ctx* ctx, double c_mass) {
         }
         ctx->h_sc[2].sf = ctx->sf;
         if ((rc = write_scalars(ctx, 2))) return rc;
-        launch_penalty_diag(ctx->st, ctx->d_pen, ctx->d_pen_mult, ctx->n_pen, ctx->d_rp, ctx->d_kdiag, ctx->bs,
+        launch_penalty_diag(ctx->st, ctx->d_pen, ctx->d_pen_mult, ctx->n_pen, ctx->d_sptr, ctx->d_kdiag,
                             ctx->d_K64, ctx->d_sc + 2);
     }
-    launch_diag_inv(ctx->st, (int)ctx->n_nodes, ctx->bs, ctx->d_rp, ctx->d_kdiag, ctx->d_K64, ctx->d_dinv64,
+    launch_diag_inv(ctx->st, (int)ctx->n_nodes, ctx->d_sptr, ctx->d_kdiag, ctx->d_K64, ctx->d_dinv64,
                     ctx->d_dinv32);
     launch_to_f32(ctx->st, ctx->d_K64, ctx->d_K32, nval);
     HIPCHK(hipGetLastError());
@@ -509,6 +524,28 @@ int asend_upload_mesh(asend_ctx* ctx, int elem_type, int64_t n_nodes, const doub
                 slot[64 * e + 8 * a + b] =
                     (uint8_t)(std::lower_bound(row.begin(), row.end(), conn_r[8 * e + b]) - row.begin());
         }
+    // sliced-ELL layout of the fine operator: 64-row slices padded to the
+    // widest row of the slice; padding slots point at the row itself with
+    // zero values
+    ctx->n_slices = (n_nodes + 63) / 64;
+    ctx->h_sptr.assign(ctx->n_slices + 1, 0);
+    for (int64_t sl = 0; sl < ctx->n_slices; ++sl) {
+        int w = 0;
+        for (int64_t r = 64 * sl; r < std::min<int64_t>(64 * sl + 64, n_nodes); ++r) w = std::max<int>(w, (int)rows[r].size());
+        ctx->h_sptr[sl + 1] = ctx->h_sptr[sl] + w;
+    }
+    ctx->n_slots = ctx->h_sptr[ctx->n_slices];
+    std::vector<int32_t> scol(64 * ctx->n_slots, 0);
+    for (int64_t sl = 0; sl < ctx->n_slices; ++sl)
+        for (int lane = 0; lane < 64; ++lane) {
+            int64_t r = 64 * sl + lane;
+            int w = ctx->h_sptr[sl + 1] - ctx->h_sptr[sl];
+            for (int k = 0; k < w; ++k) {
+                int32_t c = 0;
+                if (r < n_nodes) c = (k < (int)rows[r].size()) ? rows[r][k] : (int32_t)r;
+                scol[64 * (int64_t)(ctx->h_sptr[sl] + k) + lane] = c;
+            }
+        }
     rows.clear();
     rows.shrink_to_fit();
     // greedy element colouring: no two elements of a colour share a node
@@ -533,8 +570,9 @@ int asend_upload_mesh(asend_ctx* ctx, int elem_type, int64_t n_nodes, const doub
         for (int64_t e = 0; e < n_el; ++e) elems[fill[col[e]]++] = (int32_t)e;
     }
     int rc;
-    const int64_t nval = ctx->nnzb * 9;
-    if ((rc = dalloc(ctx, &ctx->d_conn_r, 8 * n_el)) || (rc = dalloc(ctx, &ctx->d_rp, n_nodes + 1)) ||
+    const int64_t nval = ctx->nval();
+    if ((rc = dalloc(ctx, &ctx->d_sptr, ctx->n_slices + 1)) || (rc = dalloc(ctx, &ctx->d_scol, scol.size())) ||
+        (rc = dalloc(ctx, &ctx->d_conn_r, 8 * n_el)) || (rc = dalloc(ctx, &ctx->d_rp, n_nodes + 1)) ||
         (rc = dalloc(ctx, &ctx->d_bc, ctx->nnzb)) || (rc = dalloc(ctx, &ctx->d_kdiag, n_nodes)) ||
         (rc = dalloc(ctx, &ctx->d_elems, n_el)) || (rc = dalloc(ctx, &ctx->d_slot, 64 * n_el)) ||
         (rc = dalloc(ctx, &ctx->d_xyz_r, 3 * n_nodes)) || (rc = dalloc(ctx, &ctx->d_E, n_el)) ||
@@ -542,7 +580,11 @@ int asend_upload_mesh(asend_ctx* ctx, int elem_type, int64_t n_nodes, const doub
         (rc = dalloc(ctx, &ctx->d_K64, nval)) || (rc = dalloc(ctx, &ctx->d_K32, nval)) ||
         (rc = dalloc(ctx, &ctx->d_dinv64, n_dof)) || (rc = dalloc(ctx, &ctx->d_dinv32, n_dof)))
         return rc;
+    HIPCHK(hipMemcpy(ctx->d_sptr, ctx->h_sptr.data(), ctx->h_sptr.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_scol, scol.data(), scol.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->d_conn_r, conn_r.data(), conn_r.size() * 4, hipMemcpyHostToDevice));
+    // padding values must read as zero in every later assembly
+    HIPCHK(hipMemset(ctx->d_K64, 0, nval * sizeof(double)));
     HIPCHK(hipMemcpy(ctx->d_rp, ctx->h_rp.data(), ctx->h_rp.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->d_bc, ctx->h_bc.data(), ctx->h_bc.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->d_kdiag, kdiag.data(), kdiag.size() * 4, hipMemcpyHostToDevice));
@@ -588,8 +630,7 @@ int asend_set_grid(asend_ctx* ctx, int64_t nx, int64_t ny, int64_t nz) {
                 }
             }
     int rc = mg_create(&ctx->mg, (int)nx, (int)ny, (int)nz, rank_of_grid.data(), ctx->d_xyz_r, ctx->d_conn_r,
-                       ctx->d_rp, ctx->d_bc, ctx->d_slot, ctx->h_rp.data(), ctx->h_bc.data(), ctx->n_nodes,
-                       ctx->n_el);
+                       ctx->fine(), ctx->n_nodes, ctx->n_el);
     if (rc) {
         std::string m = ctx->mg ? mg_last_error(ctx->mg) : "multigrid create failed";
         if (ctx->mg) mg_destroy(ctx->mg);
@@ -662,7 +703,7 @@ int asend_get_matrix(asend_ctx* ctx, int64_t* rowptr, int32_t* cols, double* val
     }
     rowptr[3 * n_nodes] = 9 * ctx->nnzb;
     if (!cols || !vals) return ASEND_OK;
-    std::vector<double> K(9 * ctx->nnzb);
+    std::vector<double> K(ctx->nval());
     HIPCHK(hipStreamSynchronize(ctx->st));
     HIPCHK(hipMemcpy(K.data(), ctx->d_K64, K.size() * 8, hipMemcpyDeviceToHost));
     for (int64_t r = 0; r < n_nodes; ++r) {
@@ -672,7 +713,7 @@ int asend_get_matrix(asend_ctx* ctx, int64_t* rowptr, int32_t* cols, double* val
             for (int64_t k = 0; k < nb; ++k)
                 for (int j = 0; j < 3; ++j) {
                     cols[o + 3 * k + j] = 3 * ctx->h_bc[s + k] + j;
-                    vals[o + 3 * k + j] = K[9 * s + (3 * i + j) * nb + k];
+                    vals[o + 3 * k + j] = K[sell_index(ctx->h_sptr.data(), (int)r, (int)k, 3 * i + j)];
                 }
         }
     }
@@ -696,16 +737,14 @@ int asend_spmv(asend_ctx* ctx, int precision, const double* x, double* y) {
     const int64_t n = ctx->n_dof;
     if (precision == 64) {
         HIPCHK(hipMemcpy(ctx->d_p, x, n * 8, hipMemcpyHostToDevice));
-        launch_spmv3<double>(ctx->st, 0, (int)ctx->n_nodes, ctx->d_rp, ctx->d_bc, ctx->d_K64, ctx->d_p, ctx->d_q,
-                             nullptr, nullptr, nullptr);
+        launch_sell3<double>(ctx->st, 0, ctx->fine(), ctx->d_K64, ctx->d_p, ctx->d_q, nullptr, nullptr, nullptr);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(ctx->st));
         HIPCHK(hipMemcpy(y, ctx->d_q, n * 8, hipMemcpyDeviceToHost));
     } else if (precision == 32) {
         std::vector<float> xf(x, x + n), yf(n);
         HIPCHK(hipMemcpy(ctx->d_p32, xf.data(), n * 4, hipMemcpyHostToDevice));
-        launch_spmv3<float>(ctx->st, 0, (int)ctx->n_nodes, ctx->d_rp, ctx->d_bc, ctx->d_K32, ctx->d_p32, ctx->d_q32,
-                            nullptr, nullptr, nullptr);
+        launch_sell3<float>(ctx->st, 0, ctx->fine(), ctx->d_K32, ctx->d_p32, ctx->d_q32, nullptr, nullptr, nullptr);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(ctx->st));
         HIPCHK(hipMemcpy(yf.data(), ctx->d_q32, n * 4, hipMemcpyDeviceToHost));
@@ -782,18 +821,17 @@ int asend_static_compliance(asend_ctx* ctx, const double* f, double* u, double* 
 int asend_bench_spmv(asend_ctx* ctx, int precision, int n_iter, double* avg_ms) {
     if (!ctx || n_iter <= 0 || !avg_ms) return ASEND_ERR_ARG;
     if (!ctx->assembled) return fail(ctx, ASEND_ERR_STATE, "assemble first");
-    const int nr = (int)ctx->n_nodes;
     // warm-up launch, then n_iter timed launches between two events
     for (int pass = 0; pass < 2; ++pass) {
         int cnt = pass == 0 ? 1 : n_iter;
         if (pass == 1) HIPCHK(hipEventRecord(ctx->ev0, ctx->st));
         for (int i = 0; i < cnt; ++i) {
             if (precision == 64)
-                launch_spmv3<double>(ctx->st, 0, nr, ctx->d_rp, ctx->d_bc, ctx->d_K64, ctx->d_u, ctx->d_q, nullptr,
-                                     nullptr, nullptr);
+                launch_sell3<double>(ctx->st, 0, ctx->fine(), ctx->d_K64, ctx->d_u, ctx->d_q, nullptr, nullptr,
+                                     nullptr);
             else
-                launch_spmv3<float>(ctx->st, 0, nr, ctx->d_rp, ctx->d_bc, ctx->d_K32, ctx->d_x32, ctx->d_q32,
-                                    nullptr, nullptr, nullptr);
+                launch_sell3<float>(ctx->st, 0, ctx->fine(), ctx->d_K32, ctx->d_x32, ctx->d_q32, nullptr, nullptr,
+                                    nullptr);
         }
     }
     HIPCHK(hipGetLastError());

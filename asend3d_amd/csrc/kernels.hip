@@ -32,7 +32,7 @@ __global__ __launch_bounds__(64) void k_hex8_assemble(const int32_t* __restrict_
                                                       const double* __restrict__ xyz_r,
                                                       const double* __restrict__ E, const double* __restrict__ nu,
                                                       const double* __restrict__ rho, double c_mass,
-                                                      const int32_t* __restrict__ rp,
+                                                      const int32_t* __restrict__ sptr,
                                                       const uint8_t* __restrict__ slot, double* __restrict__ K) {
     const int lane = threadIdx.x;
     const int ei = blockIdx.x;
@@ -115,19 +115,18 @@ __global__ __launch_bounds__(64) void k_hex8_assemble(const int32_t* __restrict_
         k[2][2] += cm;
     }
     const int ra = conn_r[8 * (int64_t)e + a];
-    const int s = rp[ra], nb = rp[ra + 1] - s;
     const int kk = slot[64 * (int64_t)e + lane];
-    double* base = K + 9 * (int64_t)s + kk;
+    double* base = K + sell_index(sptr, ra, kk, 0);
     for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) base[(int64_t)(3 * i + j) * nb] += k[i][j];
+        for (int j = 0; j < 3; ++j) base[64 * (3 * i + j)] += k[i][j];
 }
 
 void launch_hex8_assemble(hipStream_t st, const int32_t* elems, int n_elems, const int32_t* conn_r,
                           const double* xyz_r, const double* E, const double* nu, const double* rho, double c_mass,
-                          const int32_t* rp, const uint8_t* slot, double* K) {
+                          const int32_t* sptr, const uint8_t* slot, double* K) {
     if (n_elems <= 0) return;
     hipLaunchKernelGGL(k_hex8_assemble, dim3(n_elems), dim3(64), 0, st, elems, n_elems, conn_r, xyz_r, E, nu, rho,
-                       c_mass, rp, slot, K);
+                       c_mass, sptr, slot, K);
 }
 
 // ---------------------------------------------------------------------------
@@ -219,20 +218,19 @@ void launch_maxabs(hipStream_t st, const double* v, int64_t n, double* partials,
 
 // Adds mult[i] * sf to the diagonal entry of DOF pen[i] (unique DOFs).
 __global__ void k_penalty_diag(const int64_t* __restrict__ pen, const int32_t* __restrict__ mult, int n_pen,
-                               const int32_t* __restrict__ rp, const int32_t* __restrict__ kdiag, int bs,
+                               const int32_t* __restrict__ sptr, const int32_t* __restrict__ kdiag,
                                double* __restrict__ K, const Scalars* __restrict__ sc) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_pen) return;
     int64_t d = pen[i];
-    int r = (int)(d / bs), c = (int)(d % bs);
-    int s = rp[r], nb = rp[r + 1] - s;
-    K[(int64_t)bs * bs * s + (int64_t)(bs * c + c) * nb + kdiag[r]] += mult[i] * sc->sf;
+    int r = (int)(d / 3), c = (int)(d % 3);
+    K[sell_index(sptr, r, kdiag[r], 4 * c)] += mult[i] * sc->sf;
 }
 
-void launch_penalty_diag(hipStream_t st, const int64_t* pen, const int32_t* mult, int n_pen, const int32_t* rp,
-                         const int32_t* kdiag, int bs, double* K, const Scalars* sc) {
+void launch_penalty_diag(hipStream_t st, const int64_t* pen, const int32_t* mult, int n_pen, const int32_t* sptr,
+                         const int32_t* kdiag, double* K, const Scalars* sc) {
     if (n_pen <= 0) return;
-    hipLaunchKernelGGL(k_penalty_diag, dim3((n_pen + 255) / 256), dim3(256), 0, st, pen, mult, n_pen, rp, kdiag, bs, K,
+    hipLaunchKernelGGL(k_penalty_diag, dim3((n_pen + 255) / 256), dim3(256), 0, st, pen, mult, n_pen, sptr, kdiag, K,
                        sc);
 }
 
@@ -246,23 +244,109 @@ void launch_to_f32(hipStream_t st, const double* a, float* b, int64_t n) {
 }
 
 // Jacobi diagonal: dinv[d] = 1 / K_dd (FP64 and FP32 copies).
-__global__ void k_diag_inv(int n_rows, int bs, const int32_t* __restrict__ rp, const int32_t* __restrict__ kdiag,
+__global__ void k_diag_inv(int n_rows, const int32_t* __restrict__ sptr, const int32_t* __restrict__ kdiag,
                            const double* __restrict__ K, double* __restrict__ d64, float* __restrict__ d32) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n_rows) return;
-    int s = rp[r], nb = rp[r + 1] - s;
-    for (int c = 0; c < bs; ++c) {
-        double v = K[(int64_t)bs * bs * s + (int64_t)(bs * c + c) * nb + kdiag[r]];
-        double iv = 1.0 / v;
-        d64[(int64_t)bs * r + c] = iv;
-        d32[(int64_t)bs * r + c] = (float)iv;
+    for (int c = 0; c < 3; ++c) {
+        double iv = 1.0 / K[sell_index(sptr, r, kdiag[r], 4 * c)];
+        d64[3 * (int64_t)r + c] = iv;
+        d32[3 * (int64_t)r + c] = (float)iv;
     }
 }
 
-void launch_diag_inv(hipStream_t st, int n_rows, int bs, const int32_t* rp, const int32_t* kdiag, const double* K,
+void launch_diag_inv(hipStream_t st, int n_rows, const int32_t* sptr, const int32_t* kdiag, const double* K,
                      double* d64, float* d32) {
-    hipLaunchKernelGGL(k_diag_inv, dim3((n_rows + 255) / 256), dim3(256), 0, st, n_rows, bs, rp, kdiag, K, d64, d32);
+    hipLaunchKernelGGL(k_diag_inv, dim3((n_rows + 255) / 256), dim3(256), 0, st, n_rows, sptr, kdiag, K, d64, d32);
 }
+
+// ---------------------------------------------------------------------------
+// Sliced-ELL SpMV of the fine operator (3x3 blocks, 64-row slices): one
+// wavefront per slice, lane = row.  Slot k of a slice holds one block per
+// row (zero blocks pad rows shorter than the slice width), its column index
+// at scol[64*k + lane] and its 9 values at val[64*(9k + c) + lane], so every
+// load of the wavefront is one contiguous 256 B (FP32) / 512 B (FP64) burst
+// and no cross-lane reduction is needed.  Modes as the block-row SpMV.
+// ---------------------------------------------------------------------------
+template <typename T, int MODE>
+__global__ __launch_bounds__(kBlock) void k_sell3(int n_rows, int n_slices, const int32_t* __restrict__ sptr,
+                                                  const int32_t* __restrict__ scol, const T* __restrict__ val,
+                                                  const T* __restrict__ x, T* __restrict__ y,
+                                                  const T* __restrict__ f, double* partials, const Scalars* sc) {
+    if (MODE == 1 && sc->done) return;
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int slice = lb * (kBlock / kWave) + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    const int r = slice * kWave + lane;
+    double part = 0.0;
+    if (slice < n_slices) {
+        const int k0 = sptr[slice], k1 = sptr[slice + 1];
+        const int32_t* cp = scol + (int64_t)k0 * kWave + lane;
+        const T* vp = val + (int64_t)k0 * 9 * kWave + lane;
+        T a0 = 0, a1 = 0, a2 = 0;
+#pragma unroll 2
+        for (int k = k0; k < k1; ++k) {
+            const int c = *cp;
+            const T x0 = x[3 * (int64_t)c], x1 = x[3 * (int64_t)c + 1], x2 = x[3 * (int64_t)c + 2];
+            a0 += vp[0 * kWave] * x0 + vp[1 * kWave] * x1 + vp[2 * kWave] * x2;
+            a1 += vp[3 * kWave] * x0 + vp[4 * kWave] * x1 + vp[5 * kWave] * x2;
+            a2 += vp[6 * kWave] * x0 + vp[7 * kWave] * x1 + vp[8 * kWave] * x2;
+            cp += kWave;
+            vp += 9 * kWave;
+        }
+        if (r < n_rows) {
+            if (MODE == 2 || MODE == 3) {
+                T y0 = f[3 * (int64_t)r] - a0, y1 = f[3 * (int64_t)r + 1] - a1, y2 = f[3 * (int64_t)r + 2] - a2;
+                y[3 * (int64_t)r] = y0;
+                y[3 * (int64_t)r + 1] = y1;
+                y[3 * (int64_t)r + 2] = y2;
+                part = (double)y0 * y0 + (double)y1 * y1 + (double)y2 * y2;
+            } else {
+                y[3 * (int64_t)r] = a0;
+                y[3 * (int64_t)r + 1] = a1;
+                y[3 * (int64_t)r + 2] = a2;
+                if (MODE == 1)
+                    part = (double)a0 * x[3 * (int64_t)r] + (double)a1 * x[3 * (int64_t)r + 1] +
+                           (double)a2 * x[3 * (int64_t)r + 2];
+            }
+        }
+    }
+    if (MODE == 0 || MODE == 3) return;
+    double t[1] = {part};
+    block_partials<1>(t, partials);
+}
+
+template <typename T>
+void launch_sell3(hipStream_t st, int mode, const Sell3& A, const T* val, const T* x, T* y, const T* f,
+                  double* partials, Scalars* sc) {
+    const int per = kBlock / kWave;
+    const int nblk = (A.n_slices + per - 1) / per;
+    switch (mode) {
+        case 0:
+            hipLaunchKernelGGL((k_sell3<T, 0>), dim3(nblk), dim3(kBlock), 0, st, A.n_rows, A.n_slices, A.sptr, A.scol,
+                               val, x, y, f, partials, sc);
+            break;
+        case 1:
+            hipLaunchKernelGGL((k_sell3<T, 1>), dim3(nblk), dim3(kBlock), 0, st, A.n_rows, A.n_slices, A.sptr, A.scol,
+                               val, x, y, f, partials, sc);
+            finalize(st, partials, nblk, FIN_PQ, sc);
+            break;
+        case 2:
+            hipLaunchKernelGGL((k_sell3<T, 2>), dim3(nblk), dim3(kBlock), 0, st, A.n_rows, A.n_slices, A.sptr, A.scol,
+                               val, x, y, f, partials, sc);
+            finalize(st, partials, nblk, FIN_RR, sc);
+            break;
+        default:
+            hipLaunchKernelGGL((k_sell3<T, 3>), dim3(nblk), dim3(kBlock), 0, st, A.n_rows, A.n_slices, A.sptr, A.scol,
+                               val, x, y, f, partials, sc);
+            break;
+    }
+}
+
+template void launch_sell3<double>(hipStream_t, int, const Sell3&, const double*, const double*, double*,
+                                   const double*, double*, Scalars*);
+template void launch_sell3<float>(hipStream_t, int, const Sell3&, const float*, const float*, float*, const float*,
+                                  double*, Scalars*);
 
 // ---------------------------------------------------------------------------
 // Block-row SpMV, b = 3, G lanes per block row, component-major values.

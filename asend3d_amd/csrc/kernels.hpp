@@ -16,15 +16,32 @@ struct Scalars {
     int done, stop_abs, pad0, pad1;
 };
 
+// Sliced-ELL fine operator: 64-row slices of 3x3 blocks (see kernels.hip).
+struct Sell3 {
+    int n_rows = 0, n_slices = 0;
+    const int32_t* sptr = nullptr;  // slot offset of each slice (n_slices + 1)
+    const int32_t* scol = nullptr;  // column (node row) of slot k, lane l at 64k + l
+};
+
+// Index of component c (= 3i + j) of block k of row r in the value array.
+__host__ __device__ inline int64_t sell_index(const int32_t* sptr, int r, int k, int c) {
+    return ((int64_t)9 * (sptr[r >> 6] + k) + c) * 64 + (r & 63);
+}
+
 void launch_hex8_assemble(hipStream_t st, const int32_t* elems, int n_elems, const int32_t* conn_r,
                           const double* xyz_r, const double* E, const double* nu, const double* rho, double c_mass,
-                          const int32_t* rp, const uint8_t* slot, double* K);
+                          const int32_t* sptr, const uint8_t* slot, double* K);
 void launch_maxabs(hipStream_t st, const double* v, int64_t n, double* partials, Scalars* sc);
-void launch_penalty_diag(hipStream_t st, const int64_t* pen, const int32_t* mult, int n_pen, const int32_t* rp,
-                         const int32_t* kdiag, int bs, double* K, const Scalars* sc);
+void launch_penalty_diag(hipStream_t st, const int64_t* pen, const int32_t* mult, int n_pen, const int32_t* sptr,
+                         const int32_t* kdiag, double* K, const Scalars* sc);
 void launch_to_f32(hipStream_t st, const double* a, float* b, int64_t n);
-void launch_diag_inv(hipStream_t st, int n_rows, int bs, const int32_t* rp, const int32_t* kdiag, const double* K,
+void launch_diag_inv(hipStream_t st, int n_rows, const int32_t* sptr, const int32_t* kdiag, const double* K,
                      double* d64, float* d32);
+
+// Sliced-ELL SpMV, same modes as launch_spmv3.
+template <typename T>
+void launch_sell3(hipStream_t st, int mode, const Sell3& A, const T* val, const T* x, T* y, const T* f,
+                  double* partials, Scalars* sc);
 
 // mode 0: y = K x; 1: y = K x, sc->pq = x.y, sc->alpha = sc->rz/pq; 2: y = f - K x, sc->rr = |y|^2
 // 3: y = f - K x (no norm).  Modes 1 and 2 launch their single-block finalize kernel after the SpMV.

@@ -134,6 +134,7 @@ struct Multigrid {
     const double* d_xyz_r = nullptr;
     const int32_t* d_conn_r = nullptr;
     int64_t n_el0 = 0;
+    Sell3 fine;                        // level-0 operator layout (sliced ELL)
     // coarsest dense inverse
     int nco = 0;
     double* d_Ainv = nullptr;
@@ -779,12 +780,15 @@ void build_structured_pattern(Level& lv, std::vector<int32_t>& kdiag, std::vecto
 
 int apply_level(Multigrid* mg, hipStream_t st, int l, const float* x, float* y) {
     Level& lv = mg->L[l];
-    launch_spmv3<float>(st, 0, (int)lv.n_nodes, lv.d_rp, lv.d_bc, lv.d_val, x, y, nullptr, nullptr, nullptr);
+    if (l == 0) launch_sell3<float>(st, 0, mg->fine, lv.d_val, x, y, nullptr, nullptr, nullptr);
+    else launch_spmv3<float>(st, 0, (int)lv.n_nodes, lv.d_rp, lv.d_bc, lv.d_val, x, y, nullptr, nullptr, nullptr);
     return ASEND_OK;
 }
 
-void residual_level(hipStream_t st, Level& lv, const float* b, const float* x, float* r) {
-    launch_spmv3<float>(st, 3, (int)lv.n_nodes, lv.d_rp, lv.d_bc, lv.d_val, x, r, b, nullptr, nullptr);
+void residual_level(Multigrid* mg, hipStream_t st, int l, const float* b, const float* x, float* r) {
+    Level& lv = mg->L[l];
+    if (l == 0) launch_sell3<float>(st, 3, mg->fine, lv.d_val, x, r, b, nullptr, nullptr);
+    else launch_spmv3<float>(st, 3, (int)lv.n_nodes, lv.d_rp, lv.d_bc, lv.d_val, x, r, b, nullptr, nullptr);
 }
 
 void smooth(Multigrid* mg, hipStream_t st, int l, const float* b, float* x, bool zero_start) {
@@ -798,12 +802,12 @@ void smooth(Multigrid* mg, hipStream_t st, int l, const float* b, float* x, bool
         hipLaunchKernelGGL(k_cheb_first, dim3(nblk), dim3(256), 0, st, lv.n_nodes, lv.d_dinv, blocks, b,
                            (float)(1.0 / theta), lv.d, x);
     } else {
-        residual_level(st, lv, b, x, lv.r);
+        residual_level(mg, st, l, b, x, lv.r);
         hipLaunchKernelGGL(k_cheb_step, dim3(nblk), dim3(256), 0, st, lv.n_nodes, lv.d_dinv, blocks, lv.r, 0.f,
                            (float)(1.0 / theta), lv.d, x);
     }
     for (int k = 1; k < kChebDegree; ++k) {
-        residual_level(st, lv, b, x, lv.r);
+        residual_level(mg, st, l, b, x, lv.r);
         double rho1 = 1.0 / (2.0 * sigma - rho0);
         hipLaunchKernelGGL(k_cheb_step, dim3(nblk), dim3(256), 0, st, lv.n_nodes, lv.d_dinv, blocks, lv.r,
                            (float)(rho1 * rho0), (float)(2.0 * rho1 / delta), lv.d, x);
@@ -841,7 +845,7 @@ void vcycle(Multigrid* mg, hipStream_t st, int l, const float* b, float* x) {
     Level& lv = mg->L[l];
     Level& cv = mg->L[l + 1];
     smooth(mg, st, l, b, x, true);
-    residual_level(st, lv, b, x, lv.r);
+    residual_level(mg, st, l, b, x, lv.r);
     restrict_to(mg, st, l, lv.r, cv.b);
     vcycle(mg, st, l + 1, cv.b, cv.x);
     prolong_add(mg, st, l, cv.x, x);
@@ -902,12 +906,9 @@ void free_level(Level& lv) {
 // ===========================================================================
 
 int mg_create(Multigrid** out, int nx, int ny, int nz, const int32_t* rank_of_grid, const double* d_xyz_r,
-              const int32_t* d_conn_r, const int32_t* d_rp, const int32_t* d_bc, const uint8_t* d_slot,
-              const int32_t* h_rp, const int32_t* h_bc, int64_t n_nodes, int64_t n_el) {
-    (void)d_slot;
-    (void)h_rp;
-    (void)h_bc;
+              const int32_t* d_conn_r, const Sell3& fine, int64_t n_nodes, int64_t n_el) {
     Multigrid* mg = new Multigrid();
+    mg->fine = fine;
     *out = mg;
     mg->d_xyz_r = d_xyz_r;
     mg->d_conn_r = d_conn_r;
@@ -923,8 +924,6 @@ int mg_create(Multigrid** out, int nx, int ny, int nz, const int32_t* rank_of_gr
     l0.n_nodes = n_nodes;
     l0.n_dof = 3 * n_nodes;
     l0.n_el = n_el;
-    l0.d_rp = d_rp;
-    l0.d_bc = d_bc;
     mg->L.push_back(l0);
     // coarse levels
     while (true) {

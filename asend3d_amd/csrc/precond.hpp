@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "kernels.hpp"
+
 namespace asend {
 
 struct Multigrid;
@@ -13,8 +15,7 @@ struct Multigrid;
 // block row.  Device pointers are borrowed from the context and must outlive
 // the multigrid object.
 int mg_create(Multigrid** out, int nx, int ny, int nz, const int32_t* rank_of_grid, const double* d_xyz_r,
-              const int32_t* d_conn_r, const int32_t* d_rp, const int32_t* d_bc, const uint8_t* d_slot,
-              const int32_t* h_rp, const int32_t* h_bc, int64_t n_nodes, int64_t n_el);
+              const int32_t* d_conn_r, const Sell3& fine, int64_t n_nodes, int64_t n_el);
 void mg_destroy(Multigrid* mg);
 const char* mg_last_error(const Multigrid* mg);
 

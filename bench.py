@@ -168,7 +168,7 @@ def main():
         "wall_s_timed": wall,
         "roofline": {"bound": "hbm", "achieved": gbs32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs32 / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_spmv3<float,32,0> block-row SpMV", "bytes_per_launch": spmv32_bytes,
+                     "kernel": "k_sell3<float,0> sliced-ELL SpMV", "bytes_per_launch": spmv32_bytes,
                      "avg_ms": ms32},
         "spmv_fp64": {"achieved": gbs64, "frac": gbs64 / HBM_PEAK_GBS, "avg_ms": ms64,
                       "bytes_per_launch": spmv64_bytes},
