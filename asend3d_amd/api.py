@@ -66,7 +66,7 @@ EXPORTED = ["asend_ctx_create", "asend_ctx_destroy", "asend_last_error", "asend_
             "asend_upload_mesh", "asend_set_grid", "asend_update_moduli", "asend_set_penalty",
             "asend_penalty_scale", "asend_assemble", "asend_get_matrix", "asend_spmv", "asend_solve",
             "asend_element_sens", "asend_static_compliance", "asend_bench_spmv", "asend_mg_info", "asend_mg_apply",
-            "asend_mg_restrict", "asend_mg_prolong", "asend_mg_vcycle"]
+            "asend_mg_restrict", "asend_mg_prolong", "asend_mg_vcycle", "asend_bench_copy"]
 
 
 def lib():
@@ -267,6 +267,12 @@ class Context:
 
     def sync(self):
         self._chk(_lib.asend_sync(self.h), "asend_sync")
+
+    def bench_copy(self, nbytes, n_iter=20):
+        ms = _dbl(0)
+        self._chk(_lib.asend_bench_copy(self.h, _i64(nbytes), ctypes.c_int(n_iter), ctypes.byref(ms)),
+                  "asend_bench_copy")
+        return float(ms.value)
 
     # ---- multigrid diagnostics ----
     def mg_info(self):

@@ -900,3 +900,23 @@ int asend_mg_prolong(asend_ctx* ctx, int level, const float* coarse, float* fine
 int asend_mg_vcycle(asend_ctx* ctx, const float* b, float* x) { return mg_io(ctx, 3, 0, b, x); }
 
 }  // extern "C"
+
+extern "C" int asend_bench_copy(asend_ctx* ctx, int64_t bytes, int n_iter, double* avg_ms) {
+    if (!ctx || bytes <= 0 || n_iter <= 0 || !avg_ms) return ASEND_ERR_ARG;
+    const int64_t n4 = bytes / 16;
+    float *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc((void**)&a, n4 * 16));
+    HIPCHK(hipMalloc((void**)&b, n4 * 16));
+    HIPCHK(hipMemsetAsync(a, 0, n4 * 16, ctx->st));
+    launch_copy_f32(ctx->st, n4, a, b);
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->st));
+    for (int i = 0; i < n_iter; ++i) launch_copy_f32(ctx->st, n4, a, b);
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->st));
+    HIPCHK(hipEventSynchronize(ctx->ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    *avg_ms = ms / n_iter;
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return ASEND_OK;
+}

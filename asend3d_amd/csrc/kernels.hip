@@ -638,3 +638,18 @@ void launch_hex8_sens(hipStream_t st, int n_el, const int32_t* conn_r, const dou
 }
 
 }  // namespace asend
+
+namespace asend {
+
+// Streaming copy used to calibrate HBM traffic counters and to measure the
+// achievable copy bandwidth (read n*4 + write n*4 bytes per launch).
+__global__ __launch_bounds__(kBlock) void k_copy_f32(int64_t n, const float4* __restrict__ a, float4* __restrict__ b) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
+void launch_copy_f32(hipStream_t st, int64_t n4, const float* a, float* b) {
+    hipLaunchKernelGGL(k_copy_f32, dim3(4096), dim3(kBlock), 0, st, n4, (const float4*)a, (float4*)b);
+}
+
+}  // namespace asend

@@ -67,3 +67,7 @@ void launch_hex8_sens(hipStream_t st, int n_el, const int32_t* conn_r, const dou
                       const double* nu, const double* u, const double* lam, double* out);
 
 }  // namespace asend
+
+namespace asend {
+void launch_copy_f32(hipStream_t st, int64_t n4, const float* a, float* b);
+}  // namespace asend

@@ -30,6 +30,19 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def measured_traffic(key):
+    """Calibrated HBM bytes per launch of the SpMV on the 200x40x40 operator,
+    from the committed rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE scaled by
+    a copy kernel of known size); None when absent."""
+    path = os.path.join(ROOT, "profiles", "sell3_spmv_c2_pmc.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(key + ":"):
+            return int(line.split(":")[1])
+    return None
+
+
 def cpu_baseline(prob, fp64_iters, nx, ny, nz):
     """Serial oracle (reference-algorithm restatement) on a bounded slice of the
     workload, extrapolated linearly to the full mesh at the iteration count the
@@ -66,6 +79,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fast-precond", default="auto", choices=["auto", "jacobi", "mg", "learned"])
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="assemble, then time only the SpMV and copy kernels (short run for counter profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding is a design "
@@ -80,6 +95,16 @@ def main():
         f"({time.perf_counter() - t0:.1f} s host prep)")
     ctx = api.Context(0)
     problems.upload(ctx, prob)
+    if args.kernels_only:
+        ctx.assemble()
+        ms32 = ctx.bench_spmv(32, 20)
+        ms64 = ctx.bench_spmv(64, 20)
+        copy_bytes = 1 << 30
+        msc = ctx.bench_copy(copy_bytes, 20)
+        print(json.dumps({"spmv32_ms": ms32, "spmv64_ms": ms64, "copy_bytes": copy_bytes, "copy_ms": msc,
+                          "copy_gbs": 2 * copy_bytes / (msc * 1e-3) / 1e9}), flush=True)
+        ctx.close()
+        return
     pc = api.PC_JACOBI
     if args.fast_precond in ("auto", "mg", "learned"):
         try:
@@ -131,7 +156,11 @@ def main():
     ms64 = ctx.bench_spmv(64, 100)
     gbs32 = spmv32_bytes / (ms32 * 1e-3) / 1e9
     gbs64 = spmv64_bytes / (ms64 * 1e-3) / 1e9
-    log(f"SpMV fp32 {ms32 * 1e3:.1f} us ({gbs32:.0f} GB/s), fp64 {ms64 * 1e3:.1f} us ({gbs64:.0f} GB/s)")
+    copy_bytes = 1 << 30
+    ms_copy = ctx.bench_copy(copy_bytes, 20)
+    copy_gbs = 2 * copy_bytes / (ms_copy * 1e-3) / 1e9
+    log(f"SpMV fp32 {ms32 * 1e3:.1f} us ({gbs32:.0f} GB/s), fp64 {ms64 * 1e3:.1f} us ({gbs64:.0f} GB/s); "
+        f"copy {copy_gbs:.0f} GB/s")
 
     cpu = None
     if not args.no_cpu:
@@ -167,11 +196,15 @@ def main():
         "dof_per_s": prob.n_dof / t_fast,
         "wall_s_timed": wall,
         "roofline": {"bound": "hbm", "achieved": gbs32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs32 / HBM_PEAK_GBS, "traffic": None,
+                     "frac": gbs32 / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("traffic_bytes_per_launch_fp32") if (args.nx, args.ny, args.nz) == (
+                         200, 40, 40) else None,
                      "kernel": "k_sell3<float,0> sliced-ELL SpMV", "bytes_per_launch": spmv32_bytes,
                      "avg_ms": ms32},
         "spmv_fp64": {"achieved": gbs64, "frac": gbs64 / HBM_PEAK_GBS, "avg_ms": ms64,
                       "bytes_per_launch": spmv64_bytes},
+        "copy_achievable_gbs": copy_gbs,
+        "spmv32_frac_of_copy": gbs32 / copy_gbs,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

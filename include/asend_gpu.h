@@ -199,6 +199,11 @@ int asend_static_compliance(asend_ctx* ctx, const double* f, double* u, double* 
  * SpMV kernel of the given precision, HIP events on the context stream. */
 int asend_bench_spmv(asend_ctx* ctx, int precision, int n_iter, double* avg_ms);
 
+/* Timing helper: average duration (ms) of a streaming copy of `bytes` bytes
+ * (read + write), for the achievable-bandwidth figure and for calibrating
+ * HBM traffic counters. */
+int asend_bench_copy(asend_ctx* ctx, int64_t bytes, int n_iter, double* avg_ms);
+
 /* ---- multigrid diagnostics (no reference counterpart) --------------------
  * Host FP32 arrays.  Level 0 uses the fine DOF numbering, coarser levels the
  * lexicographic node order of their grid.  Used to verify on the device that
