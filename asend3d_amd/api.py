@@ -27,6 +27,7 @@ PC_JACOBI = 0
 PC_BLOCK_JACOBI = 1
 PC_MG = 2
 PC_LEARNED = 3
+PC_LEARNED_PCG = 4
 
 STOP_REL_RESIDUAL = 0
 STOP_REF_ABS = 1
@@ -56,6 +57,8 @@ _lib.asend_last_error.argtypes = [_vp]
 _lib.asend_version.restype = ctypes.c_char_p
 _lib.asend_block_dim_auto.restype = ctypes.c_int64
 _lib.asend_block_dim_auto.argtypes = [ctypes.c_int64]
+_lib.asend_learned_train.argtypes = [_vp, ctypes.c_int, ctypes.c_float, ctypes.c_uint64, ctypes.c_int, _vp]
+_lib.asend_learned_init.argtypes = [_vp, ctypes.c_uint64]
 _lib.asend_penalty_scale.restype = ctypes.c_double
 _lib.asend_penalty_scale.argtypes = [_vp]
 _lib.asend_ctx_destroy.restype = None
@@ -66,7 +69,10 @@ EXPORTED = ["asend_ctx_create", "asend_ctx_destroy", "asend_last_error", "asend_
             "asend_upload_mesh", "asend_set_grid", "asend_update_moduli", "asend_set_penalty",
             "asend_penalty_scale", "asend_assemble", "asend_get_matrix", "asend_spmv", "asend_solve",
             "asend_element_sens", "asend_static_compliance", "asend_bench_spmv", "asend_mg_info", "asend_mg_apply",
-            "asend_mg_restrict", "asend_mg_prolong", "asend_mg_vcycle", "asend_bench_copy"]
+            "asend_mg_restrict", "asend_mg_prolong", "asend_mg_vcycle", "asend_bench_copy",
+            "asend_learned_num_params", "asend_learned_init", "asend_learned_get_params",
+            "asend_learned_set_params", "asend_learned_train", "asend_learned_eval", "asend_learned_apply",
+            "asend_learned_edges", "asend_mg_set_smoother"]
 
 
 def lib():
@@ -301,3 +307,57 @@ class Context:
         x = np.zeros(b.shape[0], dtype=np.float32)
         self._chk(_lib.asend_mg_vcycle(self.h, _p(b), _p(x)), "asend_mg_vcycle")
         return x
+
+    def mg_set_smoother(self, kind=1, degree=2, lo_frac=0.1, hi_frac=1.15):
+        """kind 0 = point Jacobi, 1 = 3x3 block Jacobi; takes effect at the next assembly."""
+        self._chk(_lib.asend_mg_set_smoother(self.h, int(kind), int(degree), _dbl(lo_frac), _dbl(hi_frac)),
+                  "asend_mg_set_smoother")
+
+    # -------------------------------------------------- learned preconditioner --
+    def learned_init(self, seed=1):
+        self._chk(_lib.asend_learned_init(self.h, ctypes.c_uint64(seed)), "asend_learned_init")
+
+    def learned_params(self):
+        out = np.zeros(learned_num_params(), dtype=np.float32)
+        self._chk(_lib.asend_learned_get_params(self.h, _p(out), out.size), "asend_learned_get_params")
+        return out
+
+    def learned_set_params(self, params):
+        a = np.ascontiguousarray(params, dtype=np.float32)
+        self._chk(_lib.asend_learned_set_params(self.h, _p(a), a.size), "asend_learned_set_params")
+
+    def learned_train(self, n_steps, lr=1e-3, seed=1, solver_samples=True):
+        losses = np.zeros(max(n_steps, 1), dtype=np.float64)
+        self._chk(_lib.asend_learned_train(self.h, int(n_steps), ctypes.c_float(lr), ctypes.c_uint64(seed),
+                                           1 if solver_samples else 0, _p(losses)), "asend_learned_train")
+        return losses[:n_steps]
+
+    def learned_eval(self, e, n_nodes):
+        e = np.ascontiguousarray(e, dtype=np.float64)
+        loss = ctypes.c_double(0.0)
+        grad = np.zeros(learned_num_params(), dtype=np.float32)
+        feat = np.zeros((n_nodes, 8), dtype=np.float32)
+        out = np.zeros((n_nodes, 24), dtype=np.float32)
+        self._chk(_lib.asend_learned_eval(self.h, _p(e), ctypes.byref(loss), _p(grad), _p(feat), _p(out)),
+                  "asend_learned_eval")
+        return loss.value, grad, feat, out
+
+    def learned_apply(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.zeros_like(r)
+        self._chk(_lib.asend_learned_apply(self.h, _p(r), _p(z)), "asend_learned_apply")
+        return z
+
+    def learned_edges(self, n_nodes):
+        w = ctypes.c_int(0)
+        self._chk(_lib.asend_learned_edges(self.h, None, None, ctypes.byref(w)), "asend_learned_edges")
+        cnt = max(w.value, 1) * n_nodes
+        enb = np.zeros(cnt, dtype=np.int32)
+        erev = np.zeros(cnt, dtype=np.int32)
+        self._chk(_lib.asend_learned_edges(self.h, _p(enb), _p(erev), ctypes.byref(w)), "asend_learned_edges")
+        return enb.reshape(-1, n_nodes)[:w.value], erev.reshape(-1, n_nodes)[:w.value]
+
+
+def learned_num_params():
+    return int(_lib.asend_learned_num_params())
+

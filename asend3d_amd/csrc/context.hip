@@ -19,6 +19,7 @@
 #include "asend_gpu.h"
 #include "kernels.hpp"
 #include "precond.hpp"
+#include "learned.hpp"
 
 using namespace asend;
 
@@ -73,6 +74,15 @@ struct asend_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     Multigrid* mg = nullptr;
+
+    // learned preconditioner: network, graph data, and the assembly its factor belongs to
+    Learned* learned = nullptr;
+    int32_t *d_n2e_ptr = nullptr, *d_n2e = nullptr;
+    uint8_t* d_clamped = nullptr;
+    int32_t *d_enb = nullptr, *d_erev = nullptr, *d_eslot = nullptr;
+    int n_edge = 0;
+    double mean_log_e = 0.0;
+    int64_t asm_count = 0, factor_asm = -1;
 };
 
 namespace {
@@ -167,11 +177,25 @@ void free_mesh(asend_ctx* ctx) {
     dfree(ctx->d_p32);
     dfree(ctx->d_q32);
     dfree(ctx->d_w32);
+    dfree(ctx->d_n2e_ptr);
+    dfree(ctx->d_n2e);
+    dfree(ctx->d_clamped);
+    dfree(ctx->d_enb);
+    dfree(ctx->d_erev);
+    dfree(ctx->d_eslot);
+    ctx->n_edge = 0;
+    ctx->factor_asm = -1;
     if (ctx->mg) {
         mg_destroy(ctx->mg);
         ctx->mg = nullptr;
     }
     ctx->assembled = false;
+}
+
+__global__ void k_sub_f32(int64_t n, const float* __restrict__ a, const float* __restrict__ b,
+                          float* __restrict__ c) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        c[i] = a[i] - b[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -228,6 +252,54 @@ int run_jacobi_pcg(asend_ctx* ctx, int si, const T* b, T* x, double tol, bool st
 int residual64(asend_ctx* ctx, const double* f, const double* u, double* r) {
     launch_sell3<double>(ctx->st, 2, ctx->fine(), ctx->d_K64, u, r, f, ctx->d_partials, ctx->d_sc + 0);
     HIPCHK(hipGetLastError());
+    return ASEND_OK;
+}
+
+// Inner-iteration budget of one learned-preconditioned FCG solve; an inner
+// solve that needs more has failed to cut iterations and the solve falls back
+// to Jacobi PCG.
+constexpr int64_t kLearnedInnerBudget = 300;
+constexpr int64_t kLearnedPcgBudget = 20000;
+
+LearnedGraph learned_graph(const asend_ctx* ctx) {
+    LearnedGraph g;
+    g.n_nodes = (int)ctx->n_nodes;
+    g.rp = ctx->d_rp;
+    g.bc = ctx->d_bc;
+    g.n2e_ptr = ctx->d_n2e_ptr;
+    g.n2e = ctx->d_n2e;
+    g.E = ctx->d_E;
+    g.clamped = ctx->d_clamped;
+    g.n_edge = ctx->n_edge;
+    g.enb = ctx->d_enb;
+    g.erev = ctx->d_erev;
+    g.eslot = ctx->d_eslot;
+    g.op = ctx->fine();
+    g.K32 = ctx->d_K32;
+    g.kdiag = ctx->d_kdiag;
+    g.mean_log_e = ctx->mean_log_e;
+    return g;
+}
+
+// Any call that reuses the network's buffers (training, diagnostics)
+// invalidates the factor the multigrid smoother borrows.
+void invalidate_factor(asend_ctx* ctx) {
+    ctx->factor_asm = -1;
+    if (ctx->mg) mg_clear_learned_factor(ctx->mg);
+}
+
+// Network inference for the current operator (once per assembly) and
+// registration of the factor with the multigrid object; the Chebyshev bounds
+// lambda_max(P A) are estimated when the factor is used as the smoother.
+int ensure_factor(asend_ctx* ctx, bool smoother) {
+    if (ctx->factor_asm == ctx->asm_count && (!smoother || mg_factor_has_bounds(ctx->mg))) return ASEND_OK;
+    invalidate_factor(ctx);
+    Factor F;
+    int rc = learned_infer(ctx->learned, ctx->st, learned_graph(ctx), &F);
+    if (rc) return fail(ctx, rc, "%s", learned_error(ctx->learned));
+    if ((rc = mg_set_learned_factor(ctx->mg, ctx->st, F, smoother)))
+        return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
+    ctx->factor_asm = ctx->asm_count;
     return ASEND_OK;
 }
 
@@ -303,26 +375,52 @@ int solve_device(asend_ctx* ctx, const asend_solve_opts* o, asend_solve_stats* s
         if (total_it >= max_it) break;
         launch_scale_to_f32(ctx->st, n, ctx->d_r, ctx->d_r32, ctx->d_sc + 0);
         int64_t it_in = 0;
-        if (o->precond == ASEND_PC_JACOBI) {
+        const bool learned_pc = (o->precond == ASEND_PC_LEARNED || o->precond == ASEND_PC_LEARNED_PCG);
+        bool use_learned = learned_pc && !s.fallback_used;
+        if (learned_pc) {
+            if (!ctx->mg) return fail(ctx, ASEND_ERR_STATE, "multigrid preconditioner not built (asend_set_grid)");
+            if (!ctx->learned) return fail(ctx, ASEND_ERR_STATE, "no learned preconditioner (asend_learned_init)");
+        }
+        if (use_learned) {
+            // learned factor as the multigrid smoother (PC_LEARNED) or alone
+            // (PC_LEARNED_PCG); Jacobi PCG takes over for the rest of the solve
+            // if it breaks down or needs more than the inner budget
+            const bool smoother = (o->precond == ASEND_PC_LEARNED);
+            const int64_t budget = smoother ? kLearnedInnerBudget : kLearnedPcgBudget;
+            const int rc_factor = ensure_factor(ctx, smoother);
+            rc = rc_factor;
+            if (rc == ASEND_OK) {
+                if (smoother) mg_set_fine_smoother(ctx->mg, 1);
+                int fb = 0;
+                rc = mg_fcg_solve(ctx->mg, ctx->st, ctx->d_r32, ctx->d_x32, inner_tol,
+                                  std::min<int64_t>(budget, max_it - total_it), check, &it_in, &fb, smoother ? 0 : 1);
+                mg_set_fine_smoother(ctx->mg, 0);
+            }
+            if (rc == ASEND_ERR_NUMERIC || (rc == ASEND_OK && it_in >= budget)) {
+                total_it += it_in;
+                s.fallback_used = 1;
+                use_learned = false;
+                it_in = 0;
+                launch_scale_to_f32(ctx->st, n, ctx->d_r, ctx->d_r32, ctx->d_sc + 0);
+            } else if (rc) {
+                return rc_factor ? rc_factor : fail(ctx, rc, "%s", mg_last_error(ctx->mg));
+            }
+        }
+        if (use_learned) {
+            // done above
+        } else if (o->precond == ASEND_PC_JACOBI || learned_pc) {
             if ((rc = run_jacobi_pcg<float>(ctx, 1, ctx->d_r32, ctx->d_x32, inner_tol, false, max_it - total_it,
                                             check)))
                 return rc;
             it_in = ctx->h_sc[1].it;
         } else {
             if (!ctx->mg) return fail(ctx, ASEND_ERR_STATE, "multigrid preconditioner not built (asend_set_grid)");
-            if (o->precond == ASEND_PC_MG) {
-                mg_set_fine_smoother(ctx->mg, 0);
-            } else if (o->precond == ASEND_PC_LEARNED) {
-                if (mg_set_fine_smoother(ctx->mg, 1))
-                    return fail(ctx, ASEND_ERR_STATE, "no learned preconditioner loaded");
-            } else {
-                return fail(ctx, ASEND_ERR_ARG, "unknown preconditioner %d", o->precond);
-            }
+            if (o->precond != ASEND_PC_MG) return fail(ctx, ASEND_ERR_ARG, "unknown preconditioner %d", o->precond);
+            mg_set_fine_smoother(ctx->mg, 0);
             int fb = 0;
             if ((rc = mg_fcg_solve(ctx->mg, ctx->st, ctx->d_r32, ctx->d_x32, inner_tol, max_it - total_it, check,
-                                   &it_in, &fb)))
+                                   &it_in, &fb, 0)))
                 return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
-            s.fallback_used |= fb;
         }
         total_it += it_in;
         launch_axpy_from_f32(ctx->st, n, ctx->d_u, ctx->d_x32, ctx->d_sc + 0);
@@ -372,9 +470,10 @@ int assemble_device(asend_ctx* ctx, double c_mass) {
     launch_to_f32(ctx->st, ctx->d_K64, ctx->d_K32, nval);
     HIPCHK(hipGetLastError());
     ctx->assembled = true;
+    ctx->asm_count += 1;
     if (ctx->mg) {
-        if ((rc = mg_setup(ctx->mg, ctx->st, ctx->d_E, ctx->d_nu, ctx->d_K32, ctx->d_dinv32, ctx->sf, ctx->n_pen,
-                           ctx->h_pen.data(), ctx->h_pen_mult.data())))
+        if ((rc = mg_setup(ctx->mg, ctx->st, ctx->d_E, ctx->d_nu, ctx->d_K32, ctx->d_dinv32, ctx->d_kdiag, ctx->sf,
+                           ctx->n_pen, ctx->h_pen.data(), ctx->h_pen_mult.data())))
             return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
     }
     return ASEND_OK;
@@ -420,6 +519,8 @@ void asend_ctx_destroy(asend_ctx* ctx) {
     if (!ctx) return;
     if (ctx->st) (void)hipStreamSynchronize(ctx->st);
     free_mesh(ctx);
+    if (ctx->learned) learned_destroy(ctx->learned);
+    ctx->learned = nullptr;
     dfree(ctx->d_pen);
     dfree(ctx->d_pen_mult);
     dfree(ctx->d_sc);
@@ -515,6 +616,38 @@ int asend_upload_mesh(asend_ctx* ctx, int elem_type, int64_t n_nodes, const doub
         std::copy(rows[r].begin(), rows[r].end(), ctx->h_bc.begin() + ctx->h_rp[r]);
         kdiag[r] = (int32_t)(std::lower_bound(rows[r].begin(), rows[r].end(), (int32_t)r) - rows[r].begin());
     }
+    // mesh-edge neighbours (the 12 edges of each hex) for the learned factor:
+    // [slot][rank] tables, -1 padded, with the reverse slot and the operator slot
+    std::vector<int32_t> enb, erev, eslot;
+    int n_edge = 0;
+    {
+        static const int hex_edges[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                                             {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+        std::vector<std::vector<int32_t>> en(n_nodes);
+        for (int64_t e = 0; e < n_el; ++e)
+            for (const auto& ed : hex_edges) {
+                int32_t a = conn_r[8 * e + ed[0]], b = conn_r[8 * e + ed[1]];
+                if (a == b) continue;
+                en[a].push_back(b);
+                en[b].push_back(a);
+            }
+        for (auto& v : en) {
+            std::sort(v.begin(), v.end());
+            v.erase(std::unique(v.begin(), v.end()), v.end());
+            n_edge = std::max<int>(n_edge, (int)v.size());
+        }
+        if (n_edge > kMaxEdges) return fail(ctx, ASEND_ERR_UNSUPPORTED, "node with more than %d edge neighbours", kMaxEdges);
+        enb.assign((size_t)std::max(n_edge, 1) * n_nodes, -1);
+        erev.assign(enb.size(), -1);
+        eslot.assign(enb.size(), -1);
+        for (int64_t r = 0; r < n_nodes; ++r)
+            for (size_t s = 0; s < en[r].size(); ++s) {
+                int32_t j = en[r][s];
+                enb[s * n_nodes + r] = j;
+                erev[s * n_nodes + r] = (int32_t)(std::lower_bound(en[j].begin(), en[j].end(), (int32_t)r) - en[j].begin());
+                eslot[s * n_nodes + r] = (int32_t)(std::lower_bound(rows[r].begin(), rows[r].end(), j) - rows[r].begin());
+            }
+    }
     // element -> in-row slot of each (a, b) block
     std::vector<uint8_t> slot(64 * n_el);
     for (int64_t e = 0; e < n_el; ++e)
@@ -578,8 +711,23 @@ int asend_upload_mesh(asend_ctx* ctx, int elem_type, int64_t n_nodes, const doub
         (rc = dalloc(ctx, &ctx->d_xyz_r, 3 * n_nodes)) || (rc = dalloc(ctx, &ctx->d_E, n_el)) ||
         (rc = dalloc(ctx, &ctx->d_nu, n_el)) || (rc = dalloc(ctx, &ctx->d_rho, n_el)) ||
         (rc = dalloc(ctx, &ctx->d_K64, nval)) || (rc = dalloc(ctx, &ctx->d_K32, nval)) ||
-        (rc = dalloc(ctx, &ctx->d_dinv64, n_dof)) || (rc = dalloc(ctx, &ctx->d_dinv32, n_dof)))
+        (rc = dalloc(ctx, &ctx->d_dinv64, n_dof)) || (rc = dalloc(ctx, &ctx->d_dinv32, n_dof)) ||
+        (rc = dalloc(ctx, &ctx->d_n2e_ptr, n_nodes + 1)) || (rc = dalloc(ctx, &ctx->d_n2e, n2e.size())) ||
+        (rc = dalloc(ctx, &ctx->d_clamped, n_nodes)) || (rc = dalloc(ctx, &ctx->d_enb, enb.size())) ||
+        (rc = dalloc(ctx, &ctx->d_erev, erev.size())) || (rc = dalloc(ctx, &ctx->d_eslot, eslot.size())))
         return rc;
+    ctx->n_edge = n_edge;
+    HIPCHK(hipMemcpy(ctx->d_n2e_ptr, n2e_ptr.data(), n2e_ptr.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_n2e, n2e.data(), n2e.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(ctx->d_clamped, 0, n_nodes));
+    HIPCHK(hipMemcpy(ctx->d_enb, enb.data(), enb.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_erev, erev.data(), erev.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_eslot, eslot.data(), eslot.size() * 4, hipMemcpyHostToDevice));
+    {
+        double s = 0.0;
+        for (int64_t e = 0; e < n_el; ++e) s += std::log(E[e]);
+        ctx->mean_log_e = s / (double)n_el;
+    }
     HIPCHK(hipMemcpy(ctx->d_sptr, ctx->h_sptr.data(), ctx->h_sptr.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->d_scol, scol.data(), scol.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->d_conn_r, conn_r.data(), conn_r.size() * 4, hipMemcpyHostToDevice));
@@ -645,6 +793,9 @@ int asend_update_moduli(asend_ctx* ctx, const double* E) {
     if (!ctx->d_E) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
     HIPCHK(hipMemcpyAsync(ctx->d_E, E, ctx->n_el * 8, hipMemcpyHostToDevice, ctx->st));
     HIPCHK(hipStreamSynchronize(ctx->st));
+    double s = 0.0;
+    for (int64_t e = 0; e < ctx->n_el; ++e) s += std::log(E[e]);
+    ctx->mean_log_e = s / (double)ctx->n_el;
     ctx->assembled = false;
     return ASEND_OK;
 }
@@ -670,6 +821,11 @@ int asend_set_penalty(asend_ctx* ctx, int64_t n, const int64_t* dofs, double sf)
     if (!u.empty()) {
         HIPCHK(hipMemcpy(ctx->d_pen, u.data(), u.size() * 8, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(ctx->d_pen_mult, m.data(), m.size() * 4, hipMemcpyHostToDevice));
+    }
+    {
+        std::vector<uint8_t> cl(ctx->n_nodes, 0);
+        for (int64_t d : u) cl[d / 3] = 1;
+        HIPCHK(hipMemcpy(ctx->d_clamped, cl.data(), cl.size(), hipMemcpyHostToDevice));
     }
     ctx->n_pen = (int)u.size();
     ctx->h_pen = u;
@@ -843,6 +999,126 @@ int asend_bench_spmv(asend_ctx* ctx, int precision, int n_iter, double* avg_ms) 
     return ASEND_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Learned preconditioner
+// ---------------------------------------------------------------------------
+int asend_learned_num_params(void) { return kNumParams; }
+
+int asend_learned_init(asend_ctx* ctx, uint64_t seed) {
+    if (!ctx) return ASEND_ERR_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    invalidate_factor(ctx);
+    if (ctx->learned) learned_destroy(ctx->learned);
+    ctx->learned = nullptr;
+    Learned* lp = nullptr;
+    int rc = learned_create(&lp, seed);
+    if (rc) {
+        std::string m = learned_error(lp);
+        learned_destroy(lp);
+        return fail(ctx, rc, "%s", m.c_str());
+    }
+    ctx->learned = lp;
+    return ASEND_OK;
+}
+
+int asend_learned_get_params(asend_ctx* ctx, float* out, int n) {
+    if (!ctx || !out) return ASEND_ERR_ARG;
+    if (!ctx->learned) return fail(ctx, ASEND_ERR_STATE, "no learned preconditioner (asend_learned_init)");
+    if (n != kNumParams) return fail(ctx, ASEND_ERR_ARG, "expected %d parameters", kNumParams);
+    int rc = learned_get_params(ctx->learned, out, n);
+    return rc ? fail(ctx, rc, "%s", learned_error(ctx->learned)) : ASEND_OK;
+}
+
+int asend_learned_set_params(asend_ctx* ctx, const float* in, int n) {
+    if (!ctx || !in) return ASEND_ERR_ARG;
+    if (!ctx->learned) return fail(ctx, ASEND_ERR_STATE, "no learned preconditioner (asend_learned_init)");
+    if (n != kNumParams) return fail(ctx, ASEND_ERR_ARG, "expected %d parameters", kNumParams);
+    invalidate_factor(ctx);
+    int rc = learned_set_params(ctx->learned, in, n);
+    return rc ? fail(ctx, rc, "%s", learned_error(ctx->learned)) : ASEND_OK;
+}
+
+int asend_learned_train(asend_ctx* ctx, int n_steps, float lr, uint64_t seed, int solver_samples, double* losses) {
+    if (!ctx || n_steps < 0 || !(lr > 0.f)) return ASEND_ERR_ARG;
+    if (!ctx->learned) return fail(ctx, ASEND_ERR_STATE, "no learned preconditioner (asend_learned_init)");
+    if (!ctx->assembled) return fail(ctx, ASEND_ERR_STATE, "assemble first");
+    if (solver_samples && !ctx->mg) return fail(ctx, ASEND_ERR_STATE, "solver samples need the multigrid hierarchy");
+    invalidate_factor(ctx);
+    const LearnedGraph g = learned_graph(ctx);
+    const int64_t n = ctx->n_dof;
+    for (int k = 0; k < n_steps; ++k) {
+        const uint64_t s = seed * 1000003ull + (uint64_t)k;
+        const float* e = nullptr;
+        const int kind = solver_samples ? k % 3 : 0;
+        if (kind == 1) {
+            // error left by one V-cycle B of the solver: e = (I - B A) e0
+            learned_sample(ctx->st, g, s, ctx->d_p32);
+            launch_sell3<float>(ctx->st, 0, ctx->fine(), ctx->d_K32, ctx->d_p32, ctx->d_r32, nullptr, nullptr,
+                                nullptr);
+            mg_set_fine_smoother(ctx->mg, 0);
+            int rc = mg_vcycle(ctx->mg, ctx->st, ctx->d_r32, ctx->d_x32);
+            if (rc) return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
+            hipLaunchKernelGGL(k_sub_f32, dim3(kVecBlocks), dim3(256), 0, ctx->st, n, ctx->d_p32, ctx->d_x32,
+                               ctx->d_q32);
+            e = ctx->d_q32;
+        } else if (kind == 2) {
+            // approximate solution of A e = b for a random load b: e = B b
+            learned_sample(ctx->st, g, s, ctx->d_r32);
+            mg_set_fine_smoother(ctx->mg, 0);
+            int rc = mg_vcycle(ctx->mg, ctx->st, ctx->d_r32, ctx->d_q32);
+            if (rc) return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
+            e = ctx->d_q32;
+        }
+        double loss = 0.0;
+        int rc = learned_train_step(ctx->learned, ctx->st, g, e, s, lr, losses ? &loss : nullptr);
+        if (rc) return fail(ctx, rc, "%s", learned_error(ctx->learned));
+        if (losses) losses[k] = loss;
+    }
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return ASEND_OK;
+}
+
+int asend_learned_eval(asend_ctx* ctx, const double* e, double* loss, float* grad, float* feat, float* out) {
+    if (!ctx || !e) return ASEND_ERR_ARG;
+    if (!ctx->learned) return fail(ctx, ASEND_ERR_STATE, "no learned preconditioner (asend_learned_init)");
+    if (!ctx->assembled) return fail(ctx, ASEND_ERR_STATE, "assemble first");
+    invalidate_factor(ctx);
+    const int64_t n = ctx->n_dof;
+    std::vector<float> ef(e, e + n);
+    HIPCHK(hipMemcpy(ctx->d_q32, ef.data(), n * 4, hipMemcpyHostToDevice));
+    int rc = learned_eval(ctx->learned, ctx->st, learned_graph(ctx), ctx->d_q32, loss, grad, feat, out);
+    return rc ? fail(ctx, rc, "%s", learned_error(ctx->learned)) : ASEND_OK;
+}
+
+int asend_learned_apply(asend_ctx* ctx, const double* r, double* z) {
+    if (!ctx || !r || !z) return ASEND_ERR_ARG;
+    if (!ctx->learned) return fail(ctx, ASEND_ERR_STATE, "no learned preconditioner (asend_learned_init)");
+    if (!ctx->assembled) return fail(ctx, ASEND_ERR_STATE, "assemble first");
+    invalidate_factor(ctx);
+    const int64_t n = ctx->n_dof;
+    Factor F;
+    int rc = learned_infer(ctx->learned, ctx->st, learned_graph(ctx), &F);
+    if (rc) return fail(ctx, rc, "%s", learned_error(ctx->learned));
+    std::vector<float> buf(r, r + n);
+    HIPCHK(hipMemcpyAsync(ctx->d_r32, buf.data(), n * 4, hipMemcpyHostToDevice, ctx->st));
+    launch_factor_apply(ctx->st, F, ctx->d_r32, ctx->d_w32, ctx->d_x32);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(buf.data(), ctx->d_x32, n * 4, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    for (int64_t i = 0; i < n; ++i) z[i] = buf[i];
+    return ASEND_OK;
+}
+
+int asend_learned_edges(asend_ctx* ctx, int32_t* enb, int32_t* erev, int* n_edge) {
+    if (!ctx || !n_edge) return ASEND_ERR_ARG;
+    if (!ctx->d_enb) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
+    *n_edge = ctx->n_edge;
+    const size_t cnt = (size_t)std::max(ctx->n_edge, 1) * ctx->n_nodes;
+    if (enb) HIPCHK(hipMemcpy(enb, ctx->d_enb, cnt * 4, hipMemcpyDeviceToHost));
+    if (erev) HIPCHK(hipMemcpy(erev, ctx->d_erev, cnt * 4, hipMemcpyDeviceToHost));
+    return ASEND_OK;
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
@@ -898,6 +1174,15 @@ int asend_mg_prolong(asend_ctx* ctx, int level, const float* coarse, float* fine
     return mg_io(ctx, 2, level, coarse, fine);
 }
 int asend_mg_vcycle(asend_ctx* ctx, const float* b, float* x) { return mg_io(ctx, 3, 0, b, x); }
+
+int asend_mg_set_smoother(asend_ctx* ctx, int kind, int degree, double lo_frac, double hi_frac) {
+    if (!ctx) return ASEND_ERR_ARG;
+    if (!ctx->mg) return fail(ctx, ASEND_ERR_STATE, "no multigrid hierarchy (asend_set_grid)");
+    int rc = mg_set_smoother(ctx->mg, kind, degree, lo_frac, hi_frac);
+    if (rc) return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
+    if (!mg_ready(ctx->mg)) ctx->assembled = false;
+    return ASEND_OK;
+}
 
 }  // extern "C"
 

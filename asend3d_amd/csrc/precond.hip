@@ -28,13 +28,16 @@
 #include "device_common.hpp"
 #include "kernels.hpp"
 #include "precond.hpp"
+#include "learned.hpp"
 
 namespace asend {
 
 namespace {
 
-constexpr int kChebDegree = 2;
+constexpr int kChebDegreeDefault = 2;
+constexpr int kChebMaxDegree = 8;
 constexpr int kMaxCoarseNodes = 30;
+constexpr int kMaxCoarseDenseNodes = 400;
 constexpr int kPowerIts = 20;
 
 // ---------------------------------------------------------------------------
@@ -104,6 +107,7 @@ struct Level {
     const float* d_dinv = nullptr;
     int32_t *o_rp = nullptr, *o_bc = nullptr, *d_kdiag = nullptr;
     float *o_val = nullptr, *o_dinv = nullptr;
+    float* binv = nullptr;  // 3x3 block inverses, component-major (n_nodes x 9)
     std::vector<int32_t> h_rp, h_bc;
     // elements (levels >= 1)
     int32_t* d_conn = nullptr;
@@ -143,9 +147,17 @@ struct Multigrid {
     double* partials = nullptr;
     double* sv = nullptr;      // device scalars of the FCG loop
     double* h_sv = nullptr;    // pinned mirror
-    // learned fine smoother
+    // Chebyshev smoother: point (0) or block (1) Jacobi, degree and interval
+    // [lo, hi] * lambda_max
+    int smoother_kind = 1;
+    int cheb_degree = kChebDegreeDefault;
+    double cheb_lo = 0.1, cheb_hi = 1.15;
+    // learned fine smoother: factor P = L L^T, its lambda_max(P A), scratch
     int fine_smoother = 0;
-    const float* d_blocks = nullptr;
+    bool has_factor = false;
+    Factor factor;
+    double lmax_factor = 0.0;
+    float *ft = nullptr, *fz = nullptr;
     bool ready = false;
 };
 
@@ -449,6 +461,73 @@ __global__ void k_dinv32(int n_rows, const int32_t* __restrict__ rp, const int32
     for (int c = 0; c < 3; ++c) dinv[3 * r + c] = 1.f / K[9 * (int64_t)s + (4 * c) * nb + kdiag[r]];
 }
 
+// Inverse of the symmetric part of each 3x3 diagonal block (FP64 arithmetic),
+// stored component-major: binv[c * n + r].  sptr != null selects the sliced-ELL
+// layout of the finest level, else the component-major BSR of coarse levels.
+__global__ void k_binv32(int n_rows, const int32_t* __restrict__ sptr, const int32_t* __restrict__ rp,
+                         const int32_t* __restrict__ kdiag, const float* __restrict__ K, float* __restrict__ binv) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_rows) return;
+    double B[9];
+    if (sptr) {
+        for (int c = 0; c < 9; ++c) B[c] = K[sell_index(sptr, r, kdiag[r], c)];
+    } else {
+        const int s = rp[r], nb = rp[r + 1] - s;
+        for (int c = 0; c < 9; ++c) B[c] = K[9 * (int64_t)s + c * nb + kdiag[r]];
+    }
+    const double a = B[0], b = 0.5 * (B[1] + B[3]), c = 0.5 * (B[2] + B[6]);
+    const double d = B[4], e = 0.5 * (B[5] + B[7]), f = B[8];
+    const double A00 = d * f - e * e, A01 = c * e - b * f, A02 = b * e - c * d;
+    const double A11 = a * f - c * c, A12 = b * c - a * e, A22 = a * d - b * b;
+    const double id = 1.0 / (a * A00 + b * A01 + c * A02);
+    const double inv[9] = {A00 * id, A01 * id, A02 * id, A01 * id, A11 * id, A12 * id, A02 * id, A12 * id, A22 * id};
+    for (int k = 0; k < 9; ++k) binv[(int64_t)k * n_rows + r] = (float)inv[k];
+}
+
+// Block-Jacobi Chebyshev steps (z_r = Binv_r r_r per node):
+//   first: d = z / theta, x = d;   step: d = c1 d + c2 z, x += d
+template <bool FIRST>
+__global__ void k_cheb_block(int n_nodes, const float* __restrict__ binv, const float* __restrict__ r, float c1,
+                             float c2, float* __restrict__ d, float* __restrict__ x) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_nodes; i += gridDim.x * blockDim.x) {
+        const float r0 = r[3 * i], r1 = r[3 * i + 1], r2 = r[3 * i + 2];
+        float m[9];
+        for (int k = 0; k < 9; ++k) m[k] = binv[(int64_t)k * n_nodes + i];
+        const float z[3] = {m[0] * r0 + m[1] * r1 + m[2] * r2, m[3] * r0 + m[4] * r1 + m[5] * r2,
+                            m[6] * r0 + m[7] * r1 + m[8] * r2};
+        for (int c = 0; c < 3; ++c) {
+            if (FIRST) {
+                const float v = c2 * z[c];
+                d[3 * i + c] = v;
+                x[3 * i + c] = v;
+            } else {
+                const float v = c1 * d[3 * i + c] + c2 * z[c];
+                d[3 * i + c] = v;
+                x[3 * i + c] += v;
+            }
+        }
+    }
+}
+
+// Power iteration helper for the block-Jacobi smoother: w = Binv w, partials of |w|^2
+__global__ __launch_bounds__(kBlock) void k_binv_scale_norm(int n_nodes, const float* __restrict__ binv,
+                                                            float* __restrict__ w, double* partials) {
+    double t[1] = {0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_nodes; i += gridDim.x * blockDim.x) {
+        const float r0 = w[3 * i], r1 = w[3 * i + 1], r2 = w[3 * i + 2];
+        float m[9];
+        for (int k = 0; k < 9; ++k) m[k] = binv[(int64_t)k * n_nodes + i];
+        const float z0 = m[0] * r0 + m[1] * r1 + m[2] * r2;
+        const float z1 = m[3] * r0 + m[4] * r1 + m[5] * r2;
+        const float z2 = m[6] * r0 + m[7] * r1 + m[8] * r2;
+        w[3 * i] = z0;
+        w[3 * i + 1] = z1;
+        w[3 * i + 2] = z2;
+        t[0] += (double)z0 * z0 + (double)z1 * z1 + (double)z2 * z2;
+    }
+    block_partials<1>(t, partials);
+}
+
 __global__ void k_fill(int64_t n, float v, float* __restrict__ x) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         x[i] = v;
@@ -472,73 +551,47 @@ __global__ void k_copy(int64_t n, const float* __restrict__ a, float* __restrict
         b[i] = a[i];
 }
 
-// Chebyshev, first step from x = 0: d = M r / theta; x = d.
-// M is D^-1 (blocks == nullptr) or the learned 3x3 node blocks.
-__global__ void k_cheb_first(int64_t n_nodes, const float* __restrict__ dinv, const float* __restrict__ blocks,
-                             const float* __restrict__ r, float inv_theta, float* __restrict__ d,
-                             float* __restrict__ x) {
-    for (int64_t nd = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; nd < n_nodes;
-         nd += (int64_t)gridDim.x * blockDim.x) {
-        float r0 = r[3 * nd], r1 = r[3 * nd + 1], r2 = r[3 * nd + 2];
-        float z0, z1, z2;
-        if (blocks) {
-            const float* m = blocks + 9 * nd;
-            z0 = m[0] * r0 + m[1] * r1 + m[2] * r2;
-            z1 = m[3] * r0 + m[4] * r1 + m[5] * r2;
-            z2 = m[6] * r0 + m[7] * r1 + m[8] * r2;
-        } else {
-            z0 = dinv[3 * nd] * r0;
-            z1 = dinv[3 * nd + 1] * r1;
-            z2 = dinv[3 * nd + 2] * r2;
-        }
-        d[3 * nd] = z0 * inv_theta;
-        d[3 * nd + 1] = z1 * inv_theta;
-        d[3 * nd + 2] = z2 * inv_theta;
-        x[3 * nd] = z0 * inv_theta;
-        x[3 * nd + 1] = z1 * inv_theta;
-        x[3 * nd + 2] = z2 * inv_theta;
+// Chebyshev, first step from x = 0: d = z / theta; x = d, with z = D^-1 r
+// (dinv given) or z = r when r already holds the preconditioned residual.
+__global__ void k_cheb_first(int64_t n, const float* __restrict__ dinv, const float* __restrict__ r, float inv_theta,
+                             float* __restrict__ d, float* __restrict__ x) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float z = (dinv ? dinv[i] * r[i] : r[i]) * inv_theta;
+        d[i] = z;
+        x[i] = z;
     }
 }
 
-// Chebyshev step: d = c1 d + c2 M r; x += d   (r = b - A x of the current x)
-__global__ void k_cheb_step(int64_t n_nodes, const float* __restrict__ dinv, const float* __restrict__ blocks,
-                            const float* __restrict__ r, float c1, float c2, float* __restrict__ d,
-                            float* __restrict__ x) {
-    for (int64_t nd = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; nd < n_nodes;
-         nd += (int64_t)gridDim.x * blockDim.x) {
-        float r0 = r[3 * nd], r1 = r[3 * nd + 1], r2 = r[3 * nd + 2];
-        float z0, z1, z2;
-        if (blocks) {
-            const float* m = blocks + 9 * nd;
-            z0 = m[0] * r0 + m[1] * r1 + m[2] * r2;
-            z1 = m[3] * r0 + m[4] * r1 + m[5] * r2;
-            z2 = m[6] * r0 + m[7] * r1 + m[8] * r2;
-        } else {
-            z0 = dinv[3 * nd] * r0;
-            z1 = dinv[3 * nd + 1] * r1;
-            z2 = dinv[3 * nd + 2] * r2;
-        }
-        float e0 = c1 * d[3 * nd] + c2 * z0, e1 = c1 * d[3 * nd + 1] + c2 * z1, e2 = c1 * d[3 * nd + 2] + c2 * z2;
-        d[3 * nd] = e0;
-        d[3 * nd + 1] = e1;
-        d[3 * nd + 2] = e2;
-        x[3 * nd] += e0;
-        x[3 * nd + 1] += e1;
-        x[3 * nd + 2] += e2;
+// Chebyshev step: d = c1 d + c2 z; x += d   (z = D^-1 r or the preconditioned r)
+__global__ void k_cheb_step(int64_t n, const float* __restrict__ dinv, const float* __restrict__ r, float c1,
+                            float c2, float* __restrict__ d, float* __restrict__ x) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float z = dinv ? dinv[i] * r[i] : r[i];
+        const float e = c1 * d[i] + c2 * z;
+        d[i] = e;
+        x[i] += e;
     }
 }
 
-// Power iteration helper: w = M (A v) already in w (A v), scale by dinv,
+// Power iteration helper: w holds A v (or P A v); scale by dinv when given,
 // partials of |w|^2.
 __global__ __launch_bounds__(kBlock) void k_dinv_scale_norm(int64_t n, const float* __restrict__ dinv,
                                                             float* __restrict__ w, double* partials) {
     double t[1] = {0.0};
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        float v = dinv[i] * w[i];
+        float v = dinv ? dinv[i] * w[i] : w[i];
         w[i] = v;
         t[0] += (double)v * v;
     }
     block_partials<1>(t, partials);
+}
+
+// b = a / sqrt(nrm2[0])  (power-iteration normalisation with a device-side norm)
+__global__ void k_scale_dev(int64_t n, const float* __restrict__ a, const double* __restrict__ nrm2,
+                            float* __restrict__ b) {
+    const float s = (float)(1.0 / sqrt(nrm2[0]));
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = s * a[i];
 }
 
 __global__ void k_scale(int64_t n, const float* __restrict__ a, float s, float* __restrict__ b) {
@@ -791,26 +844,51 @@ void residual_level(Multigrid* mg, hipStream_t st, int l, const float* b, const 
     else launch_spmv3<float>(st, 3, (int)lv.n_nodes, lv.d_rp, lv.d_bc, lv.d_val, x, r, b, nullptr, nullptr);
 }
 
+// Chebyshev smoothing of degree mg->cheb_degree on level l.  The finest level
+// uses the learned factor P = L L^T as the Chebyshev preconditioner when it is
+// selected; every other case uses point Jacobi D^-1.
 void smooth(Multigrid* mg, hipStream_t st, int l, const float* b, float* x, bool zero_start) {
     Level& lv = mg->L[l];
-    const float* blocks = (l == 0 && mg->fine_smoother == 1) ? mg->d_blocks : nullptr;
-    const double hi = lv.lmax * 1.15, lo = lv.lmax * 0.1;
+    const bool learned = (l == 0 && mg->fine_smoother == 1);
+    const double lmax = learned ? mg->lmax_factor : lv.lmax;
+    const double hi = lmax * mg->cheb_hi, lo = lmax * mg->cheb_lo;
     const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
     double rho0 = 1.0 / sigma;
     const int nblk = nb_for(lv.n_nodes);
+    const bool block = !learned && mg->smoother_kind == 1;
+    const float* dinv = (learned || block) ? nullptr : lv.d_dinv;
+    // z = P r for the learned case (into mg->fz), else r itself (scaled in the kernel)
+    auto precond = [&](const float* r) -> const float* {
+        if (!learned) return r;
+        launch_factor_apply(st, mg->factor, r, mg->ft, mg->fz);
+        return mg->fz;
+    };
+    auto step = [&](bool first, const float* r, float c1, float c2) {
+        if (block) {
+            if (first)
+                hipLaunchKernelGGL(k_cheb_block<true>, dim3(nblk), dim3(256), 0, st, (int)lv.n_nodes, lv.binv, r, c1,
+                                   c2, lv.d, x);
+            else
+                hipLaunchKernelGGL(k_cheb_block<false>, dim3(nblk), dim3(256), 0, st, (int)lv.n_nodes, lv.binv, r,
+                                   c1, c2, lv.d, x);
+            return;
+        }
+        const float* z = precond(r);
+        if (first)
+            hipLaunchKernelGGL(k_cheb_first, dim3(nblk), dim3(256), 0, st, lv.n_dof, dinv, z, c2, lv.d, x);
+        else
+            hipLaunchKernelGGL(k_cheb_step, dim3(nblk), dim3(256), 0, st, lv.n_dof, dinv, z, c1, c2, lv.d, x);
+    };
     if (zero_start) {
-        hipLaunchKernelGGL(k_cheb_first, dim3(nblk), dim3(256), 0, st, lv.n_nodes, lv.d_dinv, blocks, b,
-                           (float)(1.0 / theta), lv.d, x);
+        step(true, b, 0.f, (float)(1.0 / theta));
     } else {
         residual_level(mg, st, l, b, x, lv.r);
-        hipLaunchKernelGGL(k_cheb_step, dim3(nblk), dim3(256), 0, st, lv.n_nodes, lv.d_dinv, blocks, lv.r, 0.f,
-                           (float)(1.0 / theta), lv.d, x);
+        step(false, lv.r, 0.f, (float)(1.0 / theta));
     }
-    for (int k = 1; k < kChebDegree; ++k) {
+    for (int k = 1; k < mg->cheb_degree; ++k) {
         residual_level(mg, st, l, b, x, lv.r);
         double rho1 = 1.0 / (2.0 * sigma - rho0);
-        hipLaunchKernelGGL(k_cheb_step, dim3(nblk), dim3(256), 0, st, lv.n_nodes, lv.d_dinv, blocks, lv.r,
-                           (float)(rho1 * rho0), (float)(2.0 * rho1 / delta), lv.d, x);
+        step(false, lv.r, (float)(rho1 * rho0), (float)(2.0 * rho1 / delta));
         rho0 = rho1;
     }
 }
@@ -852,27 +930,39 @@ void vcycle(Multigrid* mg, hipStream_t st, int l, const float* b, float* x) {
     smooth(mg, st, l, b, x, false);
 }
 
-// lambda_max of M A by power iteration (M = D^-1).
-int estimate_lmax(Multigrid* mg, hipStream_t st, int l) {
+// lambda_max of M A by power iteration (M = D^-1, or the learned factor on
+// the finest level when `learned`).
+int estimate_lmax(Multigrid* mg, hipStream_t st, int l, bool learned, double* out) {
     Level& lv = mg->L[l];
     const int64_t n = lv.n_dof;
     // deterministic pseudo-random start vector (a smooth start such as the
     // constant vector lies near the rigid-body modes and converges slowly)
     hipLaunchKernelGGL(k_fill_hash, dim3(nb_for(n)), dim3(256), 0, st, n, lv.x);
-    double lam = 0.0;
     const int nblk = kVecBlocks;
+    // kPowerIts steps without host round trips; ||M A x|| with ||x|| = 1 of the
+    // last step is the estimate
     for (int it = 0; it < kPowerIts; ++it) {
         apply_level(mg, st, l, lv.x, lv.r);
-        hipLaunchKernelGGL(k_dinv_scale_norm, dim3(nblk), dim3(kBlock), 0, st, n, lv.d_dinv, lv.r, mg->partials);
+        float* w = lv.r;
+        if (learned) {
+            launch_factor_apply(st, mg->factor, lv.r, mg->ft, mg->fz);
+            w = mg->fz;
+        }
+        if (!learned && mg->smoother_kind == 1)
+            hipLaunchKernelGGL(k_binv_scale_norm, dim3(nblk), dim3(kBlock), 0, st, (int)lv.n_nodes, lv.binv, w,
+                               mg->partials);
+        else
+            hipLaunchKernelGGL(k_dinv_scale_norm, dim3(nblk), dim3(kBlock), 0, st, n, learned ? nullptr : lv.d_dinv,
+                               w, mg->partials);
         hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, st, mg->partials, nblk, 1, mg->sv + SV_N);
-        MGCHK(hipMemcpyAsync(mg->h_sv, mg->sv + SV_N, sizeof(double), hipMemcpyDeviceToHost, st));
-        MGCHK(hipStreamSynchronize(st));
-        double nrm = std::sqrt(mg->h_sv[0]);
-        if (!(nrm > 0.0) || !std::isfinite(nrm)) return mgfail(mg, ASEND_ERR_NUMERIC, "power iteration broke down");
-        lam = nrm;
-        hipLaunchKernelGGL(k_scale, dim3(nb_for(n)), dim3(256), 0, st, n, lv.r, (float)(1.0 / nrm), lv.x);
+        hipLaunchKernelGGL(k_scale_dev, dim3(nb_for(n)), dim3(256), 0, st, n, w, mg->sv + SV_N, lv.x);
     }
-    lv.lmax = lam;
+    MGCHK(hipGetLastError());
+    MGCHK(hipMemcpyAsync(mg->h_sv, mg->sv + SV_N, sizeof(double), hipMemcpyDeviceToHost, st));
+    MGCHK(hipStreamSynchronize(st));
+    const double lam = std::sqrt(mg->h_sv[0]);
+    if (!(lam > 0.0) || !std::isfinite(lam)) return mgfail(mg, ASEND_ERR_NUMERIC, "power iteration broke down");
+    *out = lam;
     return ASEND_OK;
 }
 
@@ -881,6 +971,7 @@ void free_level(Level& lv) {
     fr(lv.o_bc);
     fr(lv.o_val);
     fr(lv.o_dinv);
+    fr(lv.binv);
     fr(lv.d_kdiag);
     fr(lv.d_conn);
     fr(lv.d_slot);
@@ -929,6 +1020,14 @@ int mg_create(Multigrid** out, int nx, int ny, int nz, const int32_t* rank_of_gr
     while (true) {
         Level& f = mg->L.back();
         if (f.n_nodes <= kMaxCoarseNodes || (f.n[0] == 1 && f.n[1] == 1 && f.n[2] == 1)) break;
+        // Trilinear coarse elements lock in bending once a direction that is
+        // resolved by two or more elements drops to one; the Galerkin operator
+        // of such a level is far too stiff and the coarse correction stalls
+        // (a 200x40x40 cantilever needs 79 instead of 43 iterations).  Stop
+        // above it while the dense coarsest inverse stays small.
+        bool locks = false;
+        for (int a = 0; a < 3; ++a) locks |= (f.n[a] >= 2 && (f.n[a] + 1) / 2 < 2);
+        if (locks && f.n_nodes <= kMaxCoarseDenseNodes) break;
         Level c;
         for (int a = 0; a < 3; ++a) {
             c.ax[a] = make_axis(f.n[a]);
@@ -967,6 +1066,7 @@ int mg_create(Multigrid** out, int nx, int ny, int nz, const int32_t* rank_of_gr
         MGCHK(hipMalloc((void**)&lv.b, lv.n_dof * sizeof(float)));
         MGCHK(hipMalloc((void**)&lv.r, lv.n_dof * sizeof(float)));
         MGCHK(hipMalloc((void**)&lv.d, lv.n_dof * sizeof(float)));
+        MGCHK(hipMalloc((void**)&lv.binv, 9 * lv.n_nodes * sizeof(float)));
     }
     mg->nco = (int)mg->L.back().n_dof;
     MGCHK(hipMalloc((void**)&mg->d_Ainv, (size_t)mg->nco * mg->nco * sizeof(double)));
@@ -992,6 +1092,8 @@ void mg_destroy(Multigrid* mg) {
     fr(mg->q);
     fr(mg->partials);
     fr(mg->sv);
+    fr(mg->ft);
+    fr(mg->fz);
     if (mg->h_sv) (void)hipHostFree(mg->h_sv);
     delete mg;
 }
@@ -1000,22 +1102,65 @@ const char* mg_last_error(const Multigrid* mg) { return mg ? mg->err.c_str() : "
 
 int mg_set_fine_smoother(Multigrid* mg, int kind) {
     if (kind != 0 && kind != 1) return mgfail(mg, ASEND_ERR_ARG, "unknown smoother");
-    if (kind == 1 && !mg->d_blocks) return mgfail(mg, ASEND_ERR_STATE, "no learned blocks set");
+    if (kind == 1 && !mg_factor_has_bounds(mg)) return mgfail(mg, ASEND_ERR_STATE, "no learned factor set");
     mg->fine_smoother = kind;
     return ASEND_OK;
 }
 
-int mg_set_learned_blocks(Multigrid* mg, const float* d_blocks) {
-    mg->d_blocks = d_blocks;
+int mg_set_smoother(Multigrid* mg, int kind, int degree, double lo_frac, double hi_frac) {
+    if ((kind != 0 && kind != 1) || degree < 1 || degree > kChebMaxDegree || !(lo_frac > 0.0) ||
+        !(hi_frac > lo_frac))
+        return mgfail(mg, ASEND_ERR_ARG, "bad Chebyshev smoother parameters");
+    const bool rebound = (kind != mg->smoother_kind);
+    mg->smoother_kind = kind;
+    mg->cheb_degree = degree;
+    mg->cheb_lo = lo_frac;
+    mg->cheb_hi = hi_frac;
+    // new Jacobi kind: the bounds of the current operator are stale
+    if (rebound) mg->ready = false;
     return ASEND_OK;
 }
 
+int mg_set_learned_factor(Multigrid* mg, hipStream_t st, const Factor& F, bool smoother_bounds) {
+    if (!mg->ready) return mgfail(mg, ASEND_ERR_STATE, "multigrid not set up");
+    Level& l0 = mg->L[0];
+    if (F.n != l0.n_nodes) return mgfail(mg, ASEND_ERR_ARG, "factor size does not match the finest level");
+    if (!mg->ft) {
+        MGCHK(hipMalloc((void**)&mg->ft, l0.n_dof * sizeof(float)));
+        MGCHK(hipMalloc((void**)&mg->fz, l0.n_dof * sizeof(float)));
+    }
+    mg->factor = F;
+    mg->has_factor = true;
+    mg->lmax_factor = 0.0;
+    if (!smoother_bounds) return ASEND_OK;
+    double lam = 0.0;
+    int rc = estimate_lmax(mg, st, 0, true, &lam);
+    if (rc) {
+        mg->has_factor = false;
+        return rc;
+    }
+    mg->lmax_factor = lam;
+    return ASEND_OK;
+}
+
+bool mg_ready(const Multigrid* mg) { return mg->ready; }
+
+bool mg_factor_has_bounds(const Multigrid* mg) { return mg->has_factor && mg->lmax_factor > 0.0; }
+
+void mg_clear_learned_factor(Multigrid* mg) {
+    mg->has_factor = false;
+    mg->fine_smoother = 0;
+}
+
 int mg_setup(Multigrid* mg, hipStream_t st, const double* d_E, const double* d_nu, const float* d_K32,
-             const float* d_dinv32, double sf, int n_pen, const int64_t* h_pen, const int32_t* h_mult) {
+             const float* d_dinv32, const int32_t* d_kdiag0, double sf, int n_pen, const int64_t* h_pen,
+             const int32_t* h_mult) {
     mg->ready = false;
     Level& l0 = mg->L[0];
     l0.d_val = d_K32;
     l0.d_dinv = d_dinv32;
+    hipLaunchKernelGGL(k_binv32, dim3((unsigned)((l0.n_nodes + 255) / 256)), dim3(256), 0, st, (int)l0.n_nodes,
+                       mg->fine.sptr, (const int32_t*)nullptr, d_kdiag0, d_K32, l0.binv);
     // penalty of the fine level per component as a node-level diagonal
     // (grid node -> weight), used for the Galerkin coarse penalty
     std::vector<int32_t> grid_of_rank(l0.n_nodes);
@@ -1097,11 +1242,13 @@ int mg_setup(Multigrid* mg, hipStream_t st, const double* d_E, const double* d_n
                                c.d_pidx, c.d_pval, (float)sf, c.o_val);
         hipLaunchKernelGGL(k_dinv32, dim3((unsigned)((c.n_nodes + 255) / 256)), dim3(256), 0, st, (int)c.n_nodes,
                            c.o_rp, c.d_kdiag, c.o_val, c.o_dinv);
+        hipLaunchKernelGGL(k_binv32, dim3((unsigned)((c.n_nodes + 255) / 256)), dim3(256), 0, st, (int)c.n_nodes,
+                           (const int32_t*)nullptr, c.o_rp, c.d_kdiag, c.o_val, c.binv);
     }
     MGCHK(hipGetLastError());
     // smoother bounds (all but the coarsest level)
     for (size_t l = 0; l + 1 < mg->L.size(); ++l) {
-        int rc = estimate_lmax(mg, st, (int)l);
+        int rc = estimate_lmax(mg, st, (int)l, false, &mg->L[l].lmax);
         if (rc) return rc;
     }
     // coarsest: dense FP64 inverse
@@ -1122,8 +1269,14 @@ int mg_setup(Multigrid* mg, hipStream_t st, const double* d_E, const double* d_n
 }
 
 int mg_fcg_solve(Multigrid* mg, hipStream_t st, const float* b, float* x, double rel_tol, int64_t max_it,
-                 int check_every, int64_t* it_out, int* fallback_used) {
+                 int check_every, int64_t* it_out, int* fallback_used, int precond) {
     if (!mg->ready) return mgfail(mg, ASEND_ERR_STATE, "multigrid not set up");
+    if (precond == 1 && !mg->has_factor) return mgfail(mg, ASEND_ERR_STATE, "no learned factor set");
+    // z = B r: one V-cycle, or the learned factor alone
+    auto apply_b = [&](const float* r, float* z) {
+        if (precond == 1) launch_factor_apply(st, mg->factor, r, mg->ft, z);
+        else vcycle(mg, st, 0, r, z);
+    };
     Level& l0 = mg->L[0];
     const int64_t n = l0.n_dof;
     const int nvb = kVecBlocks;
@@ -1138,7 +1291,7 @@ int mg_fcg_solve(Multigrid* mg, hipStream_t st, const float* b, float* x, double
     hipLaunchKernelGGL(k_fcg_init, dim3(nvb), dim3(kBlock), 0, st, n, b, x, mg->r, mg->partials);
     hipLaunchKernelGGL(k_fcg_fin, dim3(1), dim3(kBlock), 0, st, mg->partials, nvb, 3, mg->sv);
     // z = M r ; p = z
-    vcycle(mg, st, 0, mg->r, mg->z);
+    apply_b(mg->r, mg->z);
     hipLaunchKernelGGL(k_copy, dim3(nb_for(n)), dim3(256), 0, st, n, mg->z, mg->p);
     int64_t it = 0;
     for (;;) {
@@ -1152,7 +1305,7 @@ int mg_fcg_solve(Multigrid* mg, hipStream_t st, const float* b, float* x, double
                                mg->sv);
             hipLaunchKernelGGL(k_fcg_fin, dim3(1), dim3(kBlock), 0, st, mg->partials, nvb, 1, mg->sv);
             // z = M r ; beta = -(z.q)/(p.q) ; p = z + beta p
-            vcycle(mg, st, 0, mg->r, mg->z);
+            apply_b(mg->r, mg->z);
             hipLaunchKernelGGL(k_dot2, dim3(nvb), dim3(kBlock), 0, st, n, mg->z, mg->q, mg->z, mg->q, mg->partials,
                                mg->sv);
             hipLaunchKernelGGL(k_fcg_fin, dim3(1), dim3(kBlock), 0, st, mg->partials, nvb, 2, mg->sv);

@@ -5,6 +5,7 @@
 #include <cstdint>
 
 #include "kernels.hpp"
+#include "learned.hpp"
 
 namespace asend {
 
@@ -23,19 +24,29 @@ const char* mg_last_error(const Multigrid* mg);
 // current fine operator (FP32 copy with the penalty folded in).  pen / mult
 // list the penalised fine DOFs (unique) and their multiplicities.
 int mg_setup(Multigrid* mg, hipStream_t st, const double* d_E, const double* d_nu, const float* d_K32,
-             const float* d_dinv32, double sf, int n_pen, const int64_t* h_pen, const int32_t* h_mult);
+             const float* d_dinv32, const int32_t* d_kdiag0, double sf, int n_pen, const int64_t* h_pen,
+             const int32_t* h_mult);
 
-// Smoother of the finest level: 0 = Chebyshev-Jacobi, 1 = Chebyshev with the
-// learned node-block preconditioner (set by mg_set_learned_blocks).
+// Smoother of the finest level: 0 = Chebyshev with point Jacobi, 1 = Chebyshev
+// with the learned factored preconditioner P = L L^T (mg_set_learned_factor).
 int mg_set_fine_smoother(Multigrid* mg, int kind);
-// Learned SPD node blocks for the fine smoother: d_blocks holds n_nodes
-// row-major 3x3 blocks M_r (FP32, device), applied as z_r = M_r r_r.
-int mg_set_learned_blocks(Multigrid* mg, const float* d_blocks);
+// Chebyshev smoother of every smoothed level: point (kind 0) or 3x3 block
+// (kind 1) Jacobi, degree 1..8, interval [lo_frac, hi_frac] * lambda_max
+// (default block, 2, [0.1, 1.15]).  Changing the kind needs a new mg_setup.
+int mg_set_smoother(Multigrid* mg, int kind, int degree, double lo_frac, double hi_frac);
+// Register the learned factor (device arrays borrowed, must stay valid while
+// it is in use); with smoother_bounds also estimate lambda_max(P A), which the
+// Chebyshev smoother needs.
+int mg_set_learned_factor(Multigrid* mg, hipStream_t st, const Factor& F, bool smoother_bounds);
+bool mg_factor_has_bounds(const Multigrid* mg);
+bool mg_ready(const Multigrid* mg);
+void mg_clear_learned_factor(Multigrid* mg);
 
 // Flexible CG (one truncation vector) on the FP32 fine operator
-// preconditioned by one V-cycle: solves A x = b to ||r|| <= rel_tol ||b||.
+// preconditioned by one V-cycle (precond 0) or by the learned factor alone
+// (precond 1): solves A x = b to ||r|| <= rel_tol ||b||.
 int mg_fcg_solve(Multigrid* mg, hipStream_t st, const float* b, float* x, double rel_tol, int64_t max_it,
-                 int check_every, int64_t* it_out, int* fallback_used);
+                 int check_every, int64_t* it_out, int* fallback_used, int precond);
 
 // Diagnostics for tests: y = A_l x on level l (natural node order for l >= 1,
 // fine order for l = 0), and the transfer operators between l and l+1.

@@ -55,6 +55,7 @@ extern "C" {
 #define ASEND_PC_BLOCK_JACOBI 1   /* inverse of the per-node diagonal blocks            */
 #define ASEND_PC_MG 2             /* geometric multigrid, Galerkin coarse operators     */
 #define ASEND_PC_LEARNED 3        /* multigrid whose fine smoother is the learned L L^T */
+#define ASEND_PC_LEARNED_PCG 4    /* the learned L L^T alone                            */
 
 /* stopping rules */
 #define ASEND_STOP_REL_RESIDUAL 0 /* ||f - K u||_2 / ||f||_2 <= tol, evaluated in FP64 */
@@ -213,6 +214,48 @@ int asend_mg_apply(asend_ctx* ctx, int level, const float* x, float* y);
 int asend_mg_restrict(asend_ctx* ctx, int level, const float* fine, float* coarse);
 int asend_mg_prolong(asend_ctx* ctx, int level, const float* coarse, float* fine);
 int asend_mg_vcycle(asend_ctx* ctx, const float* b, float* x);
+/* Chebyshev smoother of every smoothed level: M = point Jacobi (kind 0) or the
+ * inverse of the 3x3 node blocks (kind 1), degree 1..8, interval
+ * [lo_frac, hi_frac] * lambda_max(M A).  Defaults 1, 2, 0.1, 1.15.  Kept across
+ * assemblies of the same hierarchy; a new kind takes effect at the next
+ * assembly. */
+int asend_mg_set_smoother(asend_ctx* ctx, int kind, int degree, double lo_frac, double hi_frac);
+
+/* ---- learned preconditioner (no reference counterpart) --------------------
+ * A 3-layer message-passing network on the mesh node graph (hidden 64, BF16
+ * weights and activations on MFMA, FP32 accumulation) maps node features
+ * (element moduli around the node, degree, clamp flag, diagonal-block shape)
+ * and edge features (the diagonally scaled operator block of each mesh edge)
+ * to a sparse factor L with one 3x3 block per node and one per mesh-edge
+ * neighbour; P = L L^T is SPD.  ASEND_PC_LEARNED uses P as the Chebyshev
+ * preconditioner of the finest multigrid level inside flexible PCG; when the
+ * learned solve breaks down or needs more than its inner budget the solver
+ * switches to Jacobi PCG and sets stats.fallback_used.  The FP64 outer
+ * residual decides convergence in every case.  The network lives in the
+ * context and survives asend_upload_mesh, so one network is trained over
+ * several meshes. */
+int asend_learned_num_params(void);
+/* Create the network (deterministic init from seed; untrained it gives L = C^-T,
+ * block Jacobi). */
+int asend_learned_init(asend_ctx* ctx, uint64_t seed);
+int asend_learned_get_params(asend_ctx* ctx, float* out, int n);
+int asend_learned_set_params(asend_ctx* ctx, const float* in, int n);
+/* n_steps Adam steps (learning rate lr) on the current assembled mesh, each on
+ * one pair (r = K e, e).  Without solver_samples e is Gaussian (zero on
+ * penalised nodes); with it the steps cycle through a Gaussian e, the error
+ * e = (I - B K) e0 left by one multigrid V-cycle B of the solver, and the
+ * approximate solution e = B b of a random load b.  Loss: the energy left by
+ * the best damped step, min_w ||e - w P K e||_K^2 / ||e||_K^2 (independent of
+ * the scale of P).  losses[k] receives each step's loss when non-null. */
+int asend_learned_train(asend_ctx* ctx, int n_steps, float lr, uint64_t seed, int solver_samples, double* losses);
+/* Diagnostics: loss and parameter gradient for the sample e (n_dof), node
+ * features (n_nodes x 8) and head outputs (n_nodes x 24); null outputs skipped.
+ * Node order is the DOF-block order. */
+int asend_learned_eval(asend_ctx* ctx, const double* e, double* loss, float* grad, float* feat, float* out);
+/* z = L L^T r with the factor of the current operator (FP32 inside). */
+int asend_learned_apply(asend_ctx* ctx, const double* r, double* z);
+/* Mesh-edge neighbour tables, [n_edge][n_nodes] each (-1 = none), as used by L. */
+int asend_learned_edges(asend_ctx* ctx, int32_t* enb, int32_t* erev, int* n_edge);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,126 @@
+"""Device checks of the learned factored preconditioner against the CPU
+checker oracle/mpnn.py: node features, head outputs, the training loss and
+its parameter gradient (BF16-MFMA forward/backward), the apply z = L L^T r;
+training reduces the loss and is deterministic; solves with the learned
+preconditioner reach the FP64 solution, and a broken network falls back to
+Jacobi PCG."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import mpnn
+from asend3d_amd import api, problems
+from conftest import gpu_case, oracle_learned_graph
+
+pytestmark = pytest.mark.gpu
+
+GROUPS = [("W0", mpnn.OFF_W0, mpnn.OFF_B0), ("b0", mpnn.OFF_B0, mpnn.OFF_L1)] + \
+    [(f"layer{l + 1}", mpnn.OFF_L1 + l * mpnn.LAYER_SIZE, mpnn.OFF_L1 + (l + 1) * mpnn.LAYER_SIZE)
+     for l in range(mpnn.LAYERS)] + [("head", mpnn.OFF_WH, mpnn.OFF_WA), ("Wa", mpnn.OFF_WA, mpnn.NUM_PARAMS)]
+
+
+def small_problem(dims=(6, 4, 3), seed=5):
+    return problems.heterogeneous(*dims, seed=seed, lx=5.0 * dims[0], ly=5.0 * dims[1], lz=5.0 * dims[2])
+
+
+def test_network_parity_with_oracle(ctx):
+    gpu_case("learned parity 6x4x3")
+    p = small_problem()
+    problems.upload(ctx, p)
+    ctx.assemble()
+    ctx.learned_init(1)
+    prm = mpnn.random_params(11).astype(np.float32)
+    ctx.learned_set_params(prm)
+    n = p.n_dof // 3
+    g, _ = oracle_learned_graph(p)
+    rng = np.random.default_rng(0)
+    e = rng.standard_normal(p.n_dof)
+    e[np.repeat(g.X[:, 4] > 0, 3)] = 0.0
+    loss_d, grad_d, feat_d, out_d = ctx.learned_eval(e, n)
+    # features: FP32 on the device, FP64 here
+    assert np.abs(feat_d - g.X).max() < 1e-5
+    # head outputs and loss: BF16 activations on both sides; a rounding flip
+    # in one activation moves the outputs by ~1e-4 relative
+    fw = mpnn.forward(g, prm.astype(np.float64))
+    assert np.linalg.norm(out_d - fw["O"]) <= 2e-3 * np.linalg.norm(fw["O"])
+    loss_o, grad_o = mpnn.loss_and_grad(g, prm.astype(np.float64), e)
+    assert abs(loss_d - loss_o) <= 1e-4 * abs(loss_o)
+    for name, a, b in GROUPS:
+        err = np.linalg.norm(grad_d[a:b] - grad_o[a:b]) / np.linalg.norm(grad_o[a:b])
+        assert err < 1e-2, (name, err)
+    # the apply kernels, given the device's own head outputs
+    r = rng.standard_normal(p.n_dof)
+    z_d = ctx.learned_apply(r)
+    Wa = mpnn.bf16(prm[mpnn.OFF_WA:].astype(np.float64)).reshape(9, 9)
+    Ld, Le = mpnn.factor_blocks(g, out_d.astype(np.float64), Wa)
+    L = mpnn.factor_matrix(g, {"Ld": Ld, "Le": Le})
+    z_o = L @ (L.T @ r)
+    assert np.linalg.norm(z_d - z_o) <= 1e-5 * np.linalg.norm(z_o)
+
+
+def test_untrained_network_is_block_jacobi(ctx):
+    gpu_case("learned untrained = block Jacobi")
+    p = small_problem((8, 3, 3), seed=2)
+    problems.upload(ctx, p)
+    ctx.assemble()
+    ctx.learned_init(3)
+    rp, cols, vals = ctx.get_matrix(with_penalty=True)
+    K = sp.csr_matrix((vals, cols, rp), shape=(p.n_dof, p.n_dof))
+    n = p.n_dof // 3
+    B = np.zeros((n, 3, 3))
+    for a in range(3):
+        for b in range(3):
+            B[:, a, b] = np.asarray(K[3 * np.arange(n) + a, 3 * np.arange(n) + b]).ravel()
+    Bs = 0.5 * (B + B.transpose(0, 2, 1))
+    r = np.random.default_rng(1).standard_normal(p.n_dof)
+    z = ctx.learned_apply(r)
+    z_ref = np.einsum("iab,ib->ia", np.linalg.inv(Bs), r.reshape(-1, 3)).ravel()
+    assert np.linalg.norm(z - z_ref) <= 1e-5 * np.linalg.norm(z_ref)
+
+
+def test_training_reduces_loss_and_is_deterministic(ctx):
+    dims = (24, 6, 6)
+    gpu_case(f"learned training {dims}")
+    p = small_problem(dims, seed=8)
+    problems.upload(ctx, p)
+    ctx.set_grid(*dims)
+    ctx.assemble()
+    ctx.learned_init(5)
+    losses = ctx.learned_train(240, lr=2e-3, seed=3, solver_samples=True)
+    assert np.all(np.isfinite(losses))
+    # the three sample kinds (Gaussian, V-cycle error, V-cycle solution) have
+    # different loss levels; the Gaussian and V-cycle-error losses must drop
+    for kind in (0, 1):
+        lk = losses[kind::3]
+        assert lk[-10:].mean() < 0.85 * lk[:10].mean(), (kind, lk[:10].mean(), lk[-10:].mean())
+    w1 = ctx.learned_params()
+    ctx.learned_init(5)
+    ctx.learned_train(240, lr=2e-3, seed=3, solver_samples=True)
+    w2 = ctx.learned_params()
+    assert np.array_equal(w1, w2)
+
+
+def test_learned_solve_matches_fp64_and_falls_back(ctx):
+    dims = (30, 6, 6)
+    gpu_case(f"learned solve + fallback {dims}")
+    p = small_problem(dims, seed=4)
+    problems.upload(ctx, p)
+    ctx.set_grid(*dims)
+    ctx.assemble()
+    ref = ctx.opts(api.SOLVE_PCG_FP64, api.PC_JACOBI, tol=1e-11, max_it=400000, check_every=32)
+    u_ref, s_ref, J_ref, _ = ctx.static_compliance(p.f, ref)
+    ctx.learned_init(9)
+    ctx.learned_train(60, lr=1e-3, seed=1)
+    for pc in (api.PC_LEARNED, api.PC_LEARNED_PCG):
+        o = ctx.opts(api.SOLVE_IR_MIXED, pc, tol=1e-8, inner_tol=1e-3, max_it=100000, max_outer=40, check_every=4)
+        u, s, J, st = ctx.static_compliance(p.f, o)
+        assert st.converged and st.fallback_used == 0, (pc, st.as_dict())
+        assert np.abs(u - u_ref).max() <= 1e-6 * np.abs(u_ref).max()
+        assert abs(J - J_ref) <= 1e-8 * abs(J_ref)
+    # a network with non-finite weights must not take the solve down
+    ctx.learned_set_params(np.full(api.learned_num_params(), np.nan, dtype=np.float32))
+    o = ctx.opts(api.SOLVE_IR_MIXED, api.PC_LEARNED, tol=1e-8, inner_tol=1e-3, max_it=400000, max_outer=40,
+                 check_every=4)
+    u, s, J, st = ctx.static_compliance(p.f, o)
+    assert st.converged and st.fallback_used == 1, st.as_dict()
+    assert np.abs(u - u_ref).max() <= 1e-6 * np.abs(u_ref).max()
