@@ -72,7 +72,7 @@ EXPORTED = ["asend_ctx_create", "asend_ctx_destroy", "asend_last_error", "asend_
             "asend_mg_restrict", "asend_mg_prolong", "asend_mg_vcycle", "asend_bench_copy",
             "asend_learned_num_params", "asend_learned_init", "asend_learned_get_params",
             "asend_learned_set_params", "asend_learned_train", "asend_learned_eval", "asend_learned_apply",
-            "asend_learned_edges", "asend_mg_set_smoother"]
+            "asend_learned_edges", "asend_mg_set_smoother", "asend_learned_vcycle_eval", "asend_set_graphs"]
 
 
 def lib():
@@ -308,6 +308,9 @@ class Context:
         self._chk(_lib.asend_mg_vcycle(self.h, _p(b), _p(x)), "asend_mg_vcycle")
         return x
 
+    def set_graphs(self, on=True):
+        self._chk(_lib.asend_set_graphs(self.h, 1 if on else 0), "asend_set_graphs")
+
     def mg_set_smoother(self, kind=1, degree=2, lo_frac=0.1, hi_frac=1.15):
         """kind 0 = point Jacobi, 1 = 3x3 block Jacobi; takes effect at the next assembly."""
         self._chk(_lib.asend_mg_set_smoother(self.h, int(kind), int(degree), _dbl(lo_frac), _dbl(hi_frac)),
@@ -326,10 +329,12 @@ class Context:
         a = np.ascontiguousarray(params, dtype=np.float32)
         self._chk(_lib.asend_learned_set_params(self.h, _p(a), a.size), "asend_learned_set_params")
 
-    def learned_train(self, n_steps, lr=1e-3, seed=1, solver_samples=True):
+    def learned_train(self, n_steps, lr=1e-3, seed=1, mode=1):
+        """mode 0: Gaussian pairs, one-step loss; 1: solver samples, one-step
+        loss; 2: end to end through one multigrid V-cycle."""
         losses = np.zeros(max(n_steps, 1), dtype=np.float64)
         self._chk(_lib.asend_learned_train(self.h, int(n_steps), ctypes.c_float(lr), ctypes.c_uint64(seed),
-                                           1 if solver_samples else 0, _p(losses)), "asend_learned_train")
+                                           int(mode), _p(losses)), "asend_learned_train")
         return losses[:n_steps]
 
     def learned_eval(self, e, n_nodes):
@@ -341,6 +346,14 @@ class Context:
         self._chk(_lib.asend_learned_eval(self.h, _p(e), ctypes.byref(loss), _p(grad), _p(feat), _p(out)),
                   "asend_learned_eval")
         return loss.value, grad, feat, out
+
+    def learned_vcycle_eval(self, e, keep_bounds=False):
+        e = np.ascontiguousarray(e, dtype=np.float64)
+        loss = ctypes.c_double(0.0)
+        grad = np.zeros(learned_num_params(), dtype=np.float32)
+        self._chk(_lib.asend_learned_vcycle_eval(self.h, _p(e), 1 if keep_bounds else 0, ctypes.byref(loss), _p(grad)),
+                  "asend_learned_vcycle_eval")
+        return loss.value, grad
 
     def learned_apply(self, r):
         r = np.ascontiguousarray(r, dtype=np.float64)

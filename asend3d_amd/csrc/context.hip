@@ -83,6 +83,7 @@ struct asend_ctx {
     int n_edge = 0;
     double mean_log_e = 0.0;
     int64_t asm_count = 0, factor_asm = -1;
+    bool use_graphs = true;
 };
 
 namespace {
@@ -779,6 +780,7 @@ int asend_set_grid(asend_ctx* ctx, int64_t nx, int64_t ny, int64_t nz) {
             }
     int rc = mg_create(&ctx->mg, (int)nx, (int)ny, (int)nz, rank_of_grid.data(), ctx->d_xyz_r, ctx->d_conn_r,
                        ctx->fine(), ctx->n_nodes, ctx->n_el);
+    if (ctx->mg) mg_use_graphs(ctx->mg, ctx->use_graphs);
     if (rc) {
         std::string m = ctx->mg ? mg_last_error(ctx->mg) : "multigrid create failed";
         if (ctx->mg) mg_destroy(ctx->mg);
@@ -1038,16 +1040,40 @@ int asend_learned_set_params(asend_ctx* ctx, const float* in, int n) {
     return rc ? fail(ctx, rc, "%s", learned_error(ctx->learned)) : ASEND_OK;
 }
 
-int asend_learned_train(asend_ctx* ctx, int n_steps, float lr, uint64_t seed, int solver_samples, double* losses) {
-    if (!ctx || n_steps < 0 || !(lr > 0.f)) return ASEND_ERR_ARG;
+int asend_learned_train(asend_ctx* ctx, int n_steps, float lr, uint64_t seed, int mode, double* losses) {
+    if (!ctx || n_steps < 0 || !(lr > 0.f) || mode < 0 || mode > 2) return ASEND_ERR_ARG;
     if (!ctx->learned) return fail(ctx, ASEND_ERR_STATE, "no learned preconditioner (asend_learned_init)");
     if (!ctx->assembled) return fail(ctx, ASEND_ERR_STATE, "assemble first");
-    if (solver_samples && !ctx->mg) return fail(ctx, ASEND_ERR_STATE, "solver samples need the multigrid hierarchy");
+    if (mode > 0 && !ctx->mg) return fail(ctx, ASEND_ERR_STATE, "this training mode needs the multigrid hierarchy");
     invalidate_factor(ctx);
     const LearnedGraph g = learned_graph(ctx);
     const int64_t n = ctx->n_dof;
+    const bool solver_samples = (mode == 1);
+    double ema = 0.0;  // running level of the accepted mode-2 losses
     for (int k = 0; k < n_steps; ++k) {
         const uint64_t s = seed * 1000003ull + (uint64_t)k;
+        if (mode == 2) {
+            // end to end through one V-cycle with the current factor as the
+            // finest smoother (its Chebyshev bounds re-estimated every step)
+            learned_sample(ctx->st, g, s, ctx->d_p32);
+            Factor F;
+            int rc = learned_infer(ctx->learned, ctx->st, g, &F);
+            if (rc) return fail(ctx, rc, "%s", learned_error(ctx->learned));
+            if ((rc = mg_set_learned_factor(ctx->mg, ctx->st, F, true)))
+                return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
+            FactorUses uses;
+            double loss = 0.0;
+            rc = mg_vcycle_uses(ctx->mg, ctx->st, ctx->d_p32, &uses, &loss);
+            if (rc && rc != ASEND_ERR_NUMERIC) return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
+            if (losses) losses[k] = rc ? NAN : loss;
+            // a sample whose cycle diverged (loss far above the running level,
+            // typically an underestimated lambda_max) is skipped, not learned from
+            if (rc || !(loss < 4.0 * ema || k < 8)) continue;
+            ema = (k == 0) ? loss : 0.95 * ema + 0.05 * loss;
+            if ((rc = learned_step_from_uses(ctx->learned, ctx->st, g, uses, lr)))
+                return fail(ctx, rc, "%s", learned_error(ctx->learned));
+            continue;
+        }
         const float* e = nullptr;
         const int kind = solver_samples ? k % 3 : 0;
         if (kind == 1) {
@@ -1075,6 +1101,7 @@ int asend_learned_train(asend_ctx* ctx, int n_steps, float lr, uint64_t seed, in
         if (losses) losses[k] = loss;
     }
     HIPCHK(hipStreamSynchronize(ctx->st));
+    invalidate_factor(ctx);
     return ASEND_OK;
 }
 
@@ -1088,6 +1115,29 @@ int asend_learned_eval(asend_ctx* ctx, const double* e, double* loss, float* gra
     HIPCHK(hipMemcpy(ctx->d_q32, ef.data(), n * 4, hipMemcpyHostToDevice));
     int rc = learned_eval(ctx->learned, ctx->st, learned_graph(ctx), ctx->d_q32, loss, grad, feat, out);
     return rc ? fail(ctx, rc, "%s", learned_error(ctx->learned)) : ASEND_OK;
+}
+
+int asend_learned_vcycle_eval(asend_ctx* ctx, const double* e, int keep_bounds, double* loss, float* grad) {
+    if (!ctx || !e || !loss) return ASEND_ERR_ARG;
+    if (!ctx->learned) return fail(ctx, ASEND_ERR_STATE, "no learned preconditioner (asend_learned_init)");
+    if (!ctx->assembled || !ctx->mg) return fail(ctx, ASEND_ERR_STATE, "assemble with a multigrid hierarchy first");
+    const LearnedGraph g = learned_graph(ctx);
+    const int64_t n = ctx->n_dof;
+    std::vector<float> ef(e, e + n);
+    HIPCHK(hipMemcpy(ctx->d_p32, ef.data(), n * 4, hipMemcpyHostToDevice));
+    ctx->factor_asm = -1;
+    Factor F;
+    int rc = learned_infer(ctx->learned, ctx->st, g, &F);
+    if (rc) return fail(ctx, rc, "%s", learned_error(ctx->learned));
+    mg_keep_factor_bounds(ctx->mg, keep_bounds != 0);
+    rc = mg_set_learned_factor(ctx->mg, ctx->st, F, true);
+    mg_keep_factor_bounds(ctx->mg, false);
+    if (rc) return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
+    FactorUses uses;
+    if ((rc = mg_vcycle_uses(ctx->mg, ctx->st, ctx->d_p32, &uses, loss))) return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
+    if ((rc = learned_grad_from_uses(ctx->learned, ctx->st, g, uses, grad)))
+        return fail(ctx, rc, "%s", learned_error(ctx->learned));
+    return ASEND_OK;
 }
 
 int asend_learned_apply(asend_ctx* ctx, const double* r, double* z) {
@@ -1174,6 +1224,13 @@ int asend_mg_prolong(asend_ctx* ctx, int level, const float* coarse, float* fine
     return mg_io(ctx, 2, level, coarse, fine);
 }
 int asend_mg_vcycle(asend_ctx* ctx, const float* b, float* x) { return mg_io(ctx, 3, 0, b, x); }
+
+int asend_set_graphs(asend_ctx* ctx, int on) {
+    if (!ctx) return ASEND_ERR_ARG;
+    ctx->use_graphs = (on != 0);
+    if (ctx->mg) mg_use_graphs(ctx->mg, ctx->use_graphs);
+    return ASEND_OK;
+}
 
 int asend_mg_set_smoother(asend_ctx* ctx, int kind, int degree, double lo_frac, double hi_frac) {
     if (!ctx) return ASEND_ERR_ARG;

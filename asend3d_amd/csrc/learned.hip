@@ -48,6 +48,7 @@ constexpr int kNodesPerWave = 32;
 constexpr int kWgChunk = 1024;  // nodes per block of the weight-gradient kernel
 constexpr int kHeadPad = 32;    // head rows padded to one MFMA tile
 constexpr float kExpClamp = 10.f;
+constexpr int kMaxPairs = 2 * kMaxUses;  // outer-product pairs of one gradient
 
 inline int blocks_for(int64_t n, int per) { return (int)((n + per - 1) / per); }
 
@@ -264,33 +265,25 @@ __global__ void k_factor(int n, int W, const float* __restrict__ O, const float*
     }
 }
 
-// t_j = sum over the blocks L_ij of column j of L_ij^T r_i  (= (L^T r)_j)
-__global__ __launch_bounds__(256) void k_factor_lt(int n, int W, const float* __restrict__ Ls,
-                                                   const int32_t* __restrict__ enb, const int32_t* __restrict__ erev,
-                                                   const float* __restrict__ r, float* __restrict__ t) {
+// L^T in the same row layout as L: LT block s of row j is (L_ij)^T for the
+// node i of slot s (s = 0: i = j; s = 1 + k: i = enb[k][j], whose slot for j
+// is 1 + erev[k][j]).  Built once per factor so that both halves of
+// z = L (L^T r) are coalesced row gathers.
+__global__ __launch_bounds__(256) void k_factor_transpose(int n, int W, const float* __restrict__ Ls,
+                                                          const int32_t* __restrict__ enb,
+                                                          const int32_t* __restrict__ erev, float* __restrict__ LT) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
-    float r0 = r[3 * j], r1 = r[3 * j + 1], r2 = r[3 * j + 2];
-    float b[9];
-    for (int c = 0; c < 9; ++c) b[c] = Ls[(int64_t)c * n + j];
-    float t0 = b[0] * r0 + b[3] * r1 + b[6] * r2;
-    float t1 = b[1] * r0 + b[4] * r1 + b[7] * r2;
-    float t2 = b[2] * r0 + b[5] * r1 + b[8] * r2;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) LT[(int64_t)(3 * a + b) * n + j] = Ls[(int64_t)(3 * b + a) * n + j];
     for (int s = 0; s < W; ++s) {
         const int i = enb[(int64_t)s * n + j];
-        if (i < 0) continue;
-        const int slot = 1 + erev[(int64_t)s * n + j];
-        for (int c = 0; c < 9; ++c) b[c] = Ls[((int64_t)slot * 9 + c) * n + i];
-        r0 = r[3 * i];
-        r1 = r[3 * i + 1];
-        r2 = r[3 * i + 2];
-        t0 += b[0] * r0 + b[3] * r1 + b[6] * r2;
-        t1 += b[1] * r0 + b[4] * r1 + b[7] * r2;
-        t2 += b[2] * r0 + b[5] * r1 + b[8] * r2;
+        const int slot = i < 0 ? 0 : 1 + erev[(int64_t)s * n + j];
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                LT[((int64_t)(s + 1) * 9 + 3 * a + b) * n + j] =
+                    i < 0 ? 0.f : Ls[((int64_t)slot * 9 + 3 * b + a) * n + i];
     }
-    t[3 * j] = t0;
-    t[3 * j + 1] = t1;
-    t[3 * j + 2] = t2;
 }
 
 // z_i = sum_s L_is t_{col(s)};  SUB: z = e - (L t)
@@ -396,28 +389,36 @@ __global__ void k_grad_vectors(int64_t n, const double* __restrict__ sv, const f
     }
 }
 
-// dLoss/dL_ij = g1_i t_j^T + g2_i u_j^T with t = L^T r, u = L^T q (q = A P r).
-// Chained through L_ij = Ci_i^T Lambda_ij to the head outputs:
-// dO[i][0..5] (node block), dO[i][6..14] (sum over i's edges), per-edge
-// gradients DOE (gathered by k_grad_to into dO[j][15..23]) and the per-node
-// contribution to dWa (Gea, 81 values).
-__global__ void k_grad_factor(int n, int W, const float* __restrict__ g1, const float* __restrict__ g2,
-                              const float* __restrict__ t, const float* __restrict__ u,
-                              const float* __restrict__ O, const float* __restrict__ Ci,
-                              const float* __restrict__ Ahat, const int32_t* __restrict__ enb, float* __restrict__ dO,
-                              float* __restrict__ DOE, float* __restrict__ Gea) {
+// dLoss/dL_ij = sum_q a_q[i] b_q[j]^T, the outer-product pairs (a_q, b_q)
+// describing the loss gradient with respect to L.  Chained through
+// L_ij = Ci_i^T Lambda_ij to the head outputs: dO[i][0..5] (node block),
+// dO[i][6..14] (sum over i's edges), per-edge gradients DOE (gathered by
+// k_grad_to into dO[j][15..23]) and the per-node contribution to dWa (Gea).
+struct PairSet {
+    int q;
+    const float* a[kMaxPairs];
+    const float* b[kMaxPairs];
+};
+
+__global__ void k_grad_factor(int n, int W, const PairSet ps, const float* __restrict__ O,
+                              const float* __restrict__ Ci, const float* __restrict__ Ahat,
+                              const int32_t* __restrict__ enb, float* __restrict__ dO, float* __restrict__ DOE,
+                              float* __restrict__ Gea) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float ci[9];
     for (int c = 0; c < 9; ++c) ci[c] = Ci[9 * (int64_t)i + c];
-    const float wi[3] = {g1[3 * i], g1[3 * i + 1], g1[3 * i + 2]};
-    const float ri[3] = {g2[3 * i], g2[3 * i + 1], g2[3 * i + 2]};
     float dlam[9], dl[9];
     auto block_grad = [&](int j) {
-        const float tj[3] = {t[3 * j], t[3 * j + 1], t[3 * j + 2]};
-        const float uj[3] = {u[3 * j], u[3 * j + 1], u[3 * j + 2]};
-        for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b) dl[3 * a + b] = wi[a] * tj[b] + ri[a] * uj[b];
+        for (int c = 0; c < 9; ++c) dl[c] = 0.f;
+        for (int q = 0; q < ps.q; ++q) {
+            const float* aq = ps.a[q];
+            const float* bq = ps.b[q];
+            const float ai[3] = {aq[3 * i], aq[3 * i + 1], aq[3 * i + 2]};
+            const float bj[3] = {bq[3 * j], bq[3 * j + 1], bq[3 * j + 2]};
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) dl[3 * a + b] += ai[a] * bj[b];
+        }
         // dLambda = Ci dL
         for (int a = 0; a < 3; ++a)
             for (int b = 0; b < 3; ++b)
@@ -637,7 +638,8 @@ __global__ void k_refresh_bf16(const float* __restrict__ p, bf16* __restrict__ w
 
 void launch_factor_apply(hipStream_t st, const Factor& F, const float* r, float* t, float* z) {
     const int nb = blocks_for(F.n, 256);
-    hipLaunchKernelGGL(k_factor_lt, dim3(nb), dim3(256), 0, st, F.n, F.n_edge, F.Ls, F.enb, F.erev, r, t);
+    hipLaunchKernelGGL((k_factor_l<false>), dim3(nb), dim3(256), 0, st, F.n, F.n_edge, F.LsT, F.enb, r,
+                       (const float*)nullptr, t);
     hipLaunchKernelGGL((k_factor_l<false>), dim3(nb), dim3(256), 0, st, F.n, F.n_edge, F.Ls, F.enb, t,
                        (const float*)nullptr, z);
 }
@@ -655,9 +657,10 @@ struct Learned {
     bf16* A[kLayers] = {nullptr, nullptr, nullptr};
     float *dH = nullptr, *dX = nullptr, *O = nullptr, *dO = nullptr, *dz0 = nullptr;
     bf16* dz = nullptr;
-    float *Ci = nullptr, *Ahat = nullptr, *Ls = nullptr, *DOE = nullptr, *Gea = nullptr;
+    float *Ci = nullptr, *Ahat = nullptr, *Ls = nullptr, *LsT = nullptr, *DOE = nullptr, *Gea = nullptr;
     float *e = nullptr, *r = nullptr, *t = nullptr, *y = nullptr, *w = nullptr, *u = nullptr;
     float* wpart = nullptr;
+    float* pairbuf = nullptr;  // L^T images of the factor-use vectors (2 kMaxUses x 3n)
     double* partials = nullptr;
     double* sv = nullptr;
 };
@@ -692,6 +695,7 @@ void free_work(Learned* lp) {
     lfree(lp->Ci);
     lfree(lp->Ahat);
     lfree(lp->Ls);
+    lfree(lp->LsT);
     lfree(lp->DOE);
     lfree(lp->Gea);
     lfree(lp->e);
@@ -701,6 +705,7 @@ void free_work(Learned* lp) {
     lfree(lp->w);
     lfree(lp->u);
     lfree(lp->wpart);
+    lfree(lp->pairbuf);
     lp->cap = 0;
     lp->cap_w = -1;
 }
@@ -722,12 +727,14 @@ int ensure_cap(Learned* lp, int n, int W) {
     LCHK(hipMalloc((void**)&lp->Ci, c * 9 * sizeof(float)));
     LCHK(hipMalloc((void**)&lp->Ahat, wslots * 9 * c * sizeof(float)));
     LCHK(hipMalloc((void**)&lp->Ls, (wslots + 1) * 9 * c * sizeof(float)));
+    LCHK(hipMalloc((void**)&lp->LsT, (wslots + 1) * 9 * c * sizeof(float)));
     LCHK(hipMalloc((void**)&lp->DOE, wslots * 9 * c * sizeof(float)));
     LCHK(hipMalloc((void**)&lp->Gea, c * 81 * sizeof(float)));
     for (float** q : {&lp->e, &lp->r, &lp->t, &lp->y, &lp->w, &lp->u})
         LCHK(hipMalloc((void**)q, c * 3 * sizeof(float)));
     const int nwg = std::max(blocks_for(n, kWgChunk), 64);
     LCHK(hipMalloc((void**)&lp->wpart, (size_t)nwg * kLayerSize * sizeof(float)));
+    LCHK(hipMalloc((void**)&lp->pairbuf, (size_t)2 * kMaxUses * c * 3 * sizeof(float)));
     lp->cap = n;
     lp->cap_w = W;
     return ASEND_OK;
@@ -738,6 +745,7 @@ Factor factor_of(const Learned* lp, const LearnedGraph& g) {
     F.n = g.n_nodes;
     F.n_edge = g.n_edge;
     F.Ls = lp->Ls;
+    F.LsT = lp->LsT;
     F.enb = g.enb;
     F.erev = g.erev;
     return F;
@@ -764,7 +772,10 @@ void forward(Learned* lp, hipStream_t st, const LearnedGraph& g) {
                        lp->H[kLayers], lp->whp, lp->p + kOffBh, (void*)lp->O);
     hipLaunchKernelGGL(k_factor, dim3(nb), dim3(256), 0, st, n, W, lp->O, lp->Ci, lp->Ahat, g.enb, lp->wbf + kOffWa,
                        lp->Ls);
+    hipLaunchKernelGGL(k_factor_transpose, dim3(nb), dim3(256), 0, st, n, W, lp->Ls, g.enb, g.erev, lp->LsT);
 }
+
+void backward_from_pairs(Learned* lp, hipStream_t st, const LearnedGraph& g, const PairSet& ps);
 
 // loss for the sample in lp->e and the parameter gradient into lp->g
 void loss_and_backward(Learned* lp, hipStream_t st, const LearnedGraph& g) {
@@ -773,7 +784,8 @@ void loss_and_backward(Learned* lp, hipStream_t st, const LearnedGraph& g) {
     const int nb = blocks_for(n, 256);
     // r = A e, t = L^T r, z = L t, q = A z
     launch_sell3<float>(st, 0, g.op, g.K32, lp->e, lp->r, nullptr, nullptr, nullptr);
-    hipLaunchKernelGGL(k_factor_lt, dim3(nb), dim3(256), 0, st, n, W, lp->Ls, g.enb, g.erev, lp->r, lp->t);
+    hipLaunchKernelGGL((k_factor_l<false>), dim3(nb), dim3(256), 0, st, n, W, lp->LsT, g.enb, lp->r,
+                       (const float*)nullptr, lp->t);
     hipLaunchKernelGGL((k_factor_l<false>), dim3(nb), dim3(256), 0, st, n, W, lp->Ls, g.enb, lp->t,
                        (const float*)nullptr, lp->y);
     launch_sell3<float>(st, 0, g.op, g.K32, lp->y, lp->w, nullptr, nullptr, nullptr);
@@ -781,10 +793,25 @@ void loss_and_backward(Learned* lp, hipStream_t st, const LearnedGraph& g) {
                        lp->partials);
     hipLaunchKernelGGL(k_loss_fin, dim3(1), dim3(kBlock), 0, st, lp->partials, kVecBlocks, lp->sv);
     // u = L^T q ; left vectors (reusing e and y, no longer needed) ; factor gradient
-    hipLaunchKernelGGL(k_factor_lt, dim3(nb), dim3(256), 0, st, n, W, lp->Ls, g.enb, g.erev, lp->w, lp->u);
+    hipLaunchKernelGGL((k_factor_l<false>), dim3(nb), dim3(256), 0, st, n, W, lp->LsT, g.enb, lp->w,
+                       (const float*)nullptr, lp->u);
     hipLaunchKernelGGL(k_grad_vectors, dim3(kVecBlocks), dim3(256), 0, st, nd, lp->sv, lp->r, lp->w, lp->e, lp->y);
-    hipLaunchKernelGGL(k_grad_factor, dim3(nb), dim3(256), 0, st, n, W, lp->e, lp->y, lp->t, lp->u, lp->O, lp->Ci,
-                       lp->Ahat, g.enb, lp->dO, lp->DOE, lp->Gea);
+    PairSet ps;
+    ps.q = 2;
+    ps.a[0] = lp->e;
+    ps.b[0] = lp->t;
+    ps.a[1] = lp->y;
+    ps.b[1] = lp->u;
+    backward_from_pairs(lp, st, g, ps);
+}
+
+// Parameter gradient from dLoss/dL = sum_q a_q b_q^T (forward state of the
+// last forward() call).
+void backward_from_pairs(Learned* lp, hipStream_t st, const LearnedGraph& g, const PairSet& ps) {
+    const int n = g.n_nodes, W = g.n_edge;
+    const int nb = blocks_for(n, 256);
+    hipLaunchKernelGGL(k_grad_factor, dim3(nb), dim3(256), 0, st, n, W, ps, lp->O, lp->Ci, lp->Ahat, g.enb, lp->dO,
+                       lp->DOE, lp->Gea);
     hipLaunchKernelGGL(k_grad_to, dim3(nb), dim3(256), 0, st, n, W, g.enb, g.erev, lp->DOE, lp->dO);
     // edge map Wa: column sums of Gea
     const int nsb = 64;
@@ -818,6 +845,15 @@ void loss_and_backward(Learned* lp, hipStream_t st, const LearnedGraph& g) {
                        lp->wpart);
     hipLaunchKernelGGL(k_reduce_partials, dim3(blocks_for(kHid * kFeat + kHid, 256)), dim3(256), 0, st, nsb,
                        kHid * kFeat + kHid, lp->wpart, lp->g + kOffW0);
+}
+
+void adam_step(Learned* lp, hipStream_t st, float lr) {
+    lp->step += 1;
+    const float b1t = (float)std::pow(0.9, (double)lp->step), b2t = (float)std::pow(0.999, (double)lp->step);
+    hipLaunchKernelGGL(k_adam, dim3(blocks_for(kNumParams, 256)), dim3(256), 0, st, kNumParams, lp->p, lp->g, lp->m,
+                       lp->v, lr, b1t, b2t);
+    hipLaunchKernelGGL(k_refresh_bf16, dim3(blocks_for(kNumParams, 256)), dim3(256), 0, st, lp->p, lp->wbf,
+                       lp->wcatT, lp->whp);
 }
 
 uint64_t splitmix(uint64_t& s) {
@@ -924,12 +960,7 @@ int learned_train_step(Learned* lp, hipStream_t st, const LearnedGraph& g, const
     if (d_e) LCHK(hipMemcpyAsync(lp->e, d_e, 3 * (size_t)g.n_nodes * sizeof(float), hipMemcpyDeviceToDevice, st));
     else learned_sample(st, g, sample_seed, lp->e);
     loss_and_backward(lp, st, g);
-    lp->step += 1;
-    const float b1t = (float)std::pow(0.9, (double)lp->step), b2t = (float)std::pow(0.999, (double)lp->step);
-    hipLaunchKernelGGL(k_adam, dim3(blocks_for(kNumParams, 256)), dim3(256), 0, st, kNumParams, lp->p, lp->g, lp->m,
-                       lp->v, lr, b1t, b2t);
-    hipLaunchKernelGGL(k_refresh_bf16, dim3(blocks_for(kNumParams, 256)), dim3(256), 0, st, lp->p, lp->wbf,
-                       lp->wcatT, lp->whp);
+    adam_step(lp, st, lr);
     LCHK(hipGetLastError());
     if (loss_out) {
         double h[8];
@@ -937,6 +968,47 @@ int learned_train_step(Learned* lp, hipStream_t st, const LearnedGraph& g, const
         LCHK(hipStreamSynchronize(st));
         *loss_out = h[3];
     }
+    return ASEND_OK;
+}
+
+int learned_grad_from_uses(Learned* lp, hipStream_t st, const LearnedGraph& g, const FactorUses& uses,
+                           float* h_grad) {
+    if (uses.m < 1 || uses.m > kMaxUses) {
+        lp->err = "learned preconditioner: bad number of factor uses";
+        return ASEND_ERR_ARG;
+    }
+    const int n = g.n_nodes, W = g.n_edge;
+    const int nb = blocks_for(n, 256);
+    // dLoss/dP = sum_k zbar_k v_k^T with P = L L^T:
+    //   dLoss/dL = sum_k zbar_k (L^T v_k)^T + v_k (L^T zbar_k)^T
+    PairSet ps;
+    ps.q = 2 * uses.m;
+    for (int k = 0; k < uses.m; ++k) {
+        float* tv = lp->pairbuf + (int64_t)(2 * k) * 3 * n;
+        float* tz = lp->pairbuf + (int64_t)(2 * k + 1) * 3 * n;
+        hipLaunchKernelGGL((k_factor_l<false>), dim3(nb), dim3(256), 0, st, n, W, lp->LsT, g.enb, uses.v[k],
+                           (const float*)nullptr, tv);
+        hipLaunchKernelGGL((k_factor_l<false>), dim3(nb), dim3(256), 0, st, n, W, lp->LsT, g.enb, uses.zbar[k],
+                           (const float*)nullptr, tz);
+        ps.a[2 * k] = uses.zbar[k];
+        ps.b[2 * k] = tv;
+        ps.a[2 * k + 1] = uses.v[k];
+        ps.b[2 * k + 1] = tz;
+    }
+    backward_from_pairs(lp, st, g, ps);
+    LCHK(hipGetLastError());
+    if (h_grad) {
+        LCHK(hipStreamSynchronize(st));
+        LCHK(hipMemcpy(h_grad, lp->g, kNumParams * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    return ASEND_OK;
+}
+
+int learned_step_from_uses(Learned* lp, hipStream_t st, const LearnedGraph& g, const FactorUses& uses, float lr) {
+    int rc = learned_grad_from_uses(lp, st, g, uses, nullptr);
+    if (rc) return rc;
+    adam_step(lp, st, lr);
+    LCHK(hipGetLastError());
     return ASEND_OK;
 }
 

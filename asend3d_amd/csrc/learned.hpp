@@ -53,12 +53,23 @@ struct Factor {
     int n = 0;
     int n_edge = 0;
     const float* Ls = nullptr;
+    const float* LsT = nullptr;  // L^T in the same row layout (block s of row j = (L_ij)^T)
     const int32_t* enb = nullptr;
     const int32_t* erev = nullptr;
 };
 
-// z = L L^T r (t is an n-vector of scratch).  Both kernels gather; no atomics.
+// z = L L^T r (t is an n-vector of scratch): two coalesced row gathers, one
+// over L^T and one over L; no atomics.
 void launch_factor_apply(hipStream_t st, const Factor& F, const float* r, float* t, float* z);
+
+// The factor's uses inside a larger linear computation: P was applied to
+// v[k] (z_k = P v_k) and zbar[k] is dLoss/dz_k, so dLoss/dP = sum_k zbar_k v_k^T.
+constexpr int kMaxUses = 8;
+struct FactorUses {
+    int m = 0;
+    const float* v[kMaxUses];
+    const float* zbar[kMaxUses];
+};
 
 struct Learned;
 
@@ -76,6 +87,12 @@ int learned_infer(Learned* lp, hipStream_t st, const LearnedGraph& g, Factor* F)
 // loss_out may be null.
 int learned_train_step(Learned* lp, hipStream_t st, const LearnedGraph& g, const float* d_e, uint64_t sample_seed,
                        float lr, double* loss_out);
+// Backward pass and Adam step for a loss whose dependence on P is given by
+// its uses (after learned_infer on the same graph and parameters).
+int learned_step_from_uses(Learned* lp, hipStream_t st, const LearnedGraph& g, const FactorUses& uses, float lr);
+// The same gradient without the update, copied to h_grad (kNumParams) when non-null.
+int learned_grad_from_uses(Learned* lp, hipStream_t st, const LearnedGraph& g, const FactorUses& uses,
+                           float* h_grad);
 // Seeded Gaussian sample (zero on penalised nodes) into d_e (3n floats).
 void learned_sample(hipStream_t st, const LearnedGraph& g, uint64_t seed, float* d_e);
 // Diagnostics: loss and parameter gradient for a given sample e (device

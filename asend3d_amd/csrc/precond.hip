@@ -152,16 +152,58 @@ struct Multigrid {
     int smoother_kind = 1;
     int cheb_degree = kChebDegreeDefault;
     double cheb_lo = 0.1, cheb_hi = 1.15;
+    // captured FCG iteration blocks (hipGraph), reused while their key matches
+    struct FcgGraph {
+        hipGraphExec_t exec = nullptr;
+        int precond = -1, fine_smoother = -1, check = -1, degree = -1, kind = -1;
+        const float* b = nullptr;
+        float* x = nullptr;
+        const float* ls = nullptr;
+        const float* lst = nullptr;
+        uint64_t use = 0;
+    };
+    FcgGraph graphs[4];
+    uint64_t graph_clock = 0;
+    bool use_graphs = true;
+    // step coefficients (c1, c2) per level and step; slot L.size() is the
+    // learned finest-level smoother
+    float* d_coef = nullptr;
+    std::vector<float> h_coef;
     // learned fine smoother: factor P = L L^T, its lambda_max(P A), scratch
     int fine_smoother = 0;
     bool has_factor = false;
     Factor factor;
     double lmax_factor = 0.0;
     float *ft = nullptr, *fz = nullptr;
+    bool keep_factor_bounds = false;  // diagnostics: reuse lambda_max(P A) of the previous factor
+    // training of the learned smoother through one V-cycle: recorded vectors
+    std::vector<float*> tv, tz;          // v_k (input of the k-th use of P) and dLoss/dz_k
+    float *tx = nullptr, *td = nullptr, *txb = nullptr, *tdb = nullptr, *tb = nullptr, *tq = nullptr;
     bool ready = false;
 };
 
 namespace {
+
+int coef_slot(const Multigrid* mg, int level, int step) {
+    (void)mg;
+    return 2 * (level * kChebMaxDegree + step);
+}
+
+// Chebyshev coefficients of one smoothing call for bounds [lo, hi] * lmax:
+// step 0: (0, 1/theta); step k: (rho_k rho_{k-1}, 2 rho_k / delta)
+void cheb_coefficients(const Multigrid* mg, double lmax, float* out) {
+    const double hi = lmax * mg->cheb_hi, lo = lmax * mg->cheb_lo;
+    const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+    double rho0 = 1.0 / sigma;
+    out[0] = 0.f;
+    out[1] = (float)(1.0 / theta);
+    for (int k = 1; k < mg->cheb_degree; ++k) {
+        const double rho1 = 1.0 / (2.0 * sigma - rho0);
+        out[2 * k] = (float)(rho1 * rho0);
+        out[2 * k + 1] = (float)(2.0 * rho1 / delta);
+        rho0 = rho1;
+    }
+}
 
 int mgfail(Multigrid* mg, int code, const char* msg) {
     if (mg) mg->err = msg;
@@ -484,11 +526,13 @@ __global__ void k_binv32(int n_rows, const int32_t* __restrict__ sptr, const int
     for (int k = 0; k < 9; ++k) binv[(int64_t)k * n_rows + r] = (float)inv[k];
 }
 
-// Block-Jacobi Chebyshev steps (z_r = Binv_r r_r per node):
-//   first: d = z / theta, x = d;   step: d = c1 d + c2 z, x += d
+// Block-Jacobi Chebyshev steps (z_r = Binv_r r_r per node), coefficients
+// (c1, c2) read from device memory so that captured graphs stay valid when
+// the bounds change:   first: d = c2 z, x = d;   step: d = c1 d + c2 z, x += d
 template <bool FIRST>
-__global__ void k_cheb_block(int n_nodes, const float* __restrict__ binv, const float* __restrict__ r, float c1,
-                             float c2, float* __restrict__ d, float* __restrict__ x) {
+__global__ void k_cheb_block(int n_nodes, const float* __restrict__ binv, const float* __restrict__ r,
+                             const float* __restrict__ coef, float* __restrict__ d, float* __restrict__ x) {
+    const float c1 = coef[0], c2 = coef[1];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_nodes; i += gridDim.x * blockDim.x) {
         const float r0 = r[3 * i], r1 = r[3 * i + 1], r2 = r[3 * i + 2];
         float m[9];
@@ -509,6 +553,20 @@ __global__ void k_cheb_block(int n_nodes, const float* __restrict__ binv, const 
     }
 }
 
+// The same per DOF, z = dinv r (point Jacobi) or z = r (already preconditioned)
+template <bool FIRST>
+__global__ void k_cheb_dof(int64_t n, const float* __restrict__ dinv, const float* __restrict__ r,
+                           const float* __restrict__ coef, float* __restrict__ d, float* __restrict__ x) {
+    const float c1 = coef[0], c2 = coef[1];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float z = dinv ? dinv[i] * r[i] : r[i];
+        const float v = FIRST ? c2 * z : c1 * d[i] + c2 * z;
+        d[i] = v;
+        if (FIRST) x[i] = v;
+        else x[i] += v;
+    }
+}
+
 // Power iteration helper for the block-Jacobi smoother: w = Binv w, partials of |w|^2
 __global__ __launch_bounds__(kBlock) void k_binv_scale_norm(int n_nodes, const float* __restrict__ binv,
                                                             float* __restrict__ w, double* partials) {
@@ -526,6 +584,35 @@ __global__ __launch_bounds__(kBlock) void k_binv_scale_norm(int n_nodes, const f
         t[0] += (double)z0 * z0 + (double)z1 * z1 + (double)z2 * z2;
     }
     block_partials<1>(t, partials);
+}
+
+// out = a X + b Y
+__global__ void k_lincomb(int64_t n, float a, const float* __restrict__ X, float b, const float* __restrict__ Y,
+                          float* __restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = a * X[i] + b * Y[i];
+}
+
+// Reverse step of a Chebyshev update d = c1 d_prev + c2 z, x = x_prev + d:
+// with t = dbar + xbar:  zbar = c2 t, dbar_prev = c1 t  (xbar passes through)
+__global__ void k_cheb_adjoint(int64_t n, float c1, float c2, const float* __restrict__ xbar, float* __restrict__ dbar,
+                               float* __restrict__ zbar) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float t = dbar[i] + xbar[i];
+        zbar[i] = c2 * t;
+        dbar[i] = c1 * t;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_dot2_partials(int64_t n, const float* __restrict__ a,
+                                                          const float* __restrict__ b, const float* __restrict__ c,
+                                                          const float* __restrict__ d, double* partials) {
+    double t[2] = {0.0, 0.0};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        t[0] += (double)a[i] * b[i];
+        t[1] += (double)c[i] * d[i];
+    }
+    block_partials<2>(t, partials);
 }
 
 __global__ void k_fill(int64_t n, float v, float* __restrict__ x) {
@@ -549,17 +636,6 @@ __global__ void k_fill_hash(int64_t n, float* __restrict__ x) {
 __global__ void k_copy(int64_t n, const float* __restrict__ a, float* __restrict__ b) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         b[i] = a[i];
-}
-
-// Chebyshev, first step from x = 0: d = z / theta; x = d, with z = D^-1 r
-// (dinv given) or z = r when r already holds the preconditioned residual.
-__global__ void k_cheb_first(int64_t n, const float* __restrict__ dinv, const float* __restrict__ r, float inv_theta,
-                             float* __restrict__ d, float* __restrict__ x) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const float z = (dinv ? dinv[i] * r[i] : r[i]) * inv_theta;
-        d[i] = z;
-        x[i] = z;
-    }
 }
 
 // Chebyshev step: d = c1 d + c2 z; x += d   (z = D^-1 r or the preconditioned r)
@@ -627,18 +703,33 @@ __global__ void k_bsr_to_dense(int n_rows, const int32_t* __restrict__ rp, const
 // In-place Gauss-Jordan inverse of an SPD matrix (no pivoting), one
 // workgroup; column k is staged in LDS before each elimination step.
 __global__ __launch_bounds__(1024) void k_gauss_jordan(int n, double* __restrict__ A) {
-    extern __shared__ double colk[];
+    // pivot column and pivot row of each step staged in LDS; the update of the
+    // other rows reads A once and writes it once (L2-resident for the sizes used)
+    extern __shared__ double sm_gj[];
+    double* colk = sm_gj;
+    double* rowk = sm_gj + n;
+    const int64_t nn = (int64_t)n * n;
     for (int k = 0; k < n; ++k) {
         for (int i = threadIdx.x; i < n; i += blockDim.x) colk[i] = A[(int64_t)i * n + k];
         __syncthreads();
         const double ip = 1.0 / colk[k];
-        for (int j = threadIdx.x; j < n; j += blockDim.x) A[(int64_t)k * n + j] = (j == k ? 1.0 : A[(int64_t)k * n + j]) * ip;
+        for (int j = threadIdx.x; j < n; j += blockDim.x) {
+            const double v = (j == k ? 1.0 : A[(int64_t)k * n + j]) * ip;
+            rowk[j] = v;
+            A[(int64_t)k * n + j] = v;
+        }
         __syncthreads();
-        for (int64_t t = threadIdx.x; t < (int64_t)n * n; t += blockDim.x) {
-            int i = (int)(t / n), j = (int)(t % n);
-            if (i == k) continue;
-            double aij = (j == k) ? 0.0 : A[t];
-            A[t] = aij - colk[i] * A[(int64_t)k * n + j];
+        int i = threadIdx.x / n, j = threadIdx.x % n;
+        const int di = blockDim.x / n, dj = blockDim.x % n;
+        for (int64_t t = threadIdx.x; t < nn; t += blockDim.x) {
+            if (i != k) A[t] = (j == k ? 0.0 : A[t]) - colk[i] * rowk[j];
+            // advance (i, j) by blockDim.x without a division
+            i += di;
+            j += dj;
+            if (j >= n) {
+                j -= n;
+                ++i;
+            }
         }
         __syncthreads();
     }
@@ -844,52 +935,43 @@ void residual_level(Multigrid* mg, hipStream_t st, int l, const float* b, const 
     else launch_spmv3<float>(st, 3, (int)lv.n_nodes, lv.d_rp, lv.d_bc, lv.d_val, x, r, b, nullptr, nullptr);
 }
 
-// Chebyshev smoothing of degree mg->cheb_degree on level l.  The finest level
-// uses the learned factor P = L L^T as the Chebyshev preconditioner when it is
-// selected; every other case uses point Jacobi D^-1.
+// Chebyshev smoothing of degree mg->cheb_degree on level l with the
+// preconditioner M = 3x3 block Jacobi (default) or point Jacobi; the finest
+// level uses the learned factor P = L L^T instead when it is selected.  The
+// step coefficients live in device memory (coef_slot).
 void smooth(Multigrid* mg, hipStream_t st, int l, const float* b, float* x, bool zero_start) {
     Level& lv = mg->L[l];
     const bool learned = (l == 0 && mg->fine_smoother == 1);
-    const double lmax = learned ? mg->lmax_factor : lv.lmax;
-    const double hi = lmax * mg->cheb_hi, lo = lmax * mg->cheb_lo;
-    const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
-    double rho0 = 1.0 / sigma;
-    const int nblk = nb_for(lv.n_nodes);
     const bool block = !learned && mg->smoother_kind == 1;
     const float* dinv = (learned || block) ? nullptr : lv.d_dinv;
-    // z = P r for the learned case (into mg->fz), else r itself (scaled in the kernel)
-    auto precond = [&](const float* r) -> const float* {
-        if (!learned) return r;
-        launch_factor_apply(st, mg->factor, r, mg->ft, mg->fz);
-        return mg->fz;
-    };
-    auto step = [&](bool first, const float* r, float c1, float c2) {
+    const float* coef = mg->d_coef + coef_slot(mg, learned ? (int)mg->L.size() : l, 0);
+    const int nblk = nb_for(lv.n_nodes);
+    for (int k = 0; k < mg->cheb_degree; ++k) {
+        const bool first = (k == 0 && zero_start);
+        const float* r = b;
+        if (!first) {
+            residual_level(mg, st, l, b, x, lv.r);
+            r = lv.r;
+        }
         if (block) {
             if (first)
-                hipLaunchKernelGGL(k_cheb_block<true>, dim3(nblk), dim3(256), 0, st, (int)lv.n_nodes, lv.binv, r, c1,
-                                   c2, lv.d, x);
+                hipLaunchKernelGGL(k_cheb_block<true>, dim3(nblk), dim3(256), 0, st, (int)lv.n_nodes, lv.binv, r,
+                                   coef + 2 * k, lv.d, x);
             else
                 hipLaunchKernelGGL(k_cheb_block<false>, dim3(nblk), dim3(256), 0, st, (int)lv.n_nodes, lv.binv, r,
-                                   c1, c2, lv.d, x);
-            return;
+                                   coef + 2 * k, lv.d, x);
+            continue;
         }
-        const float* z = precond(r);
+        if (learned) {
+            launch_factor_apply(st, mg->factor, r, mg->ft, mg->fz);
+            r = mg->fz;
+        }
         if (first)
-            hipLaunchKernelGGL(k_cheb_first, dim3(nblk), dim3(256), 0, st, lv.n_dof, dinv, z, c2, lv.d, x);
+            hipLaunchKernelGGL(k_cheb_dof<true>, dim3(nblk), dim3(256), 0, st, lv.n_dof, dinv, r, coef + 2 * k, lv.d,
+                               x);
         else
-            hipLaunchKernelGGL(k_cheb_step, dim3(nblk), dim3(256), 0, st, lv.n_dof, dinv, z, c1, c2, lv.d, x);
-    };
-    if (zero_start) {
-        step(true, b, 0.f, (float)(1.0 / theta));
-    } else {
-        residual_level(mg, st, l, b, x, lv.r);
-        step(false, lv.r, 0.f, (float)(1.0 / theta));
-    }
-    for (int k = 1; k < mg->cheb_degree; ++k) {
-        residual_level(mg, st, l, b, x, lv.r);
-        double rho1 = 1.0 / (2.0 * sigma - rho0);
-        step(false, lv.r, (float)(rho1 * rho0), (float)(2.0 * rho1 / delta));
-        rho0 = rho1;
+            hipLaunchKernelGGL(k_cheb_dof<false>, dim3(nblk), dim3(256), 0, st, lv.n_dof, dinv, r, coef + 2 * k, lv.d,
+                               x);
     }
 }
 
@@ -1069,6 +1151,8 @@ int mg_create(Multigrid** out, int nx, int ny, int nz, const int32_t* rank_of_gr
         MGCHK(hipMalloc((void**)&lv.binv, 9 * lv.n_nodes * sizeof(float)));
     }
     mg->nco = (int)mg->L.back().n_dof;
+    mg->h_coef.assign(2 * kChebMaxDegree * (mg->L.size() + 1), 0.f);
+    MGCHK(hipMalloc((void**)&mg->d_coef, mg->h_coef.size() * sizeof(float)));
     MGCHK(hipMalloc((void**)&mg->d_Ainv, (size_t)mg->nco * mg->nco * sizeof(double)));
     const int64_t n0 = mg->L[0].n_dof;
     MGCHK(hipMalloc((void**)&mg->r, n0 * sizeof(float)));
@@ -1094,6 +1178,17 @@ void mg_destroy(Multigrid* mg) {
     fr(mg->sv);
     fr(mg->ft);
     fr(mg->fz);
+    fr(mg->d_coef);
+    for (auto& g : mg->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    for (auto& q : mg->tv) fr(q);
+    for (auto& q : mg->tz) fr(q);
+    fr(mg->tx);
+    fr(mg->td);
+    fr(mg->txb);
+    fr(mg->tdb);
+    fr(mg->tb);
+    fr(mg->tq);
     if (mg->h_sv) (void)hipHostFree(mg->h_sv);
     delete mg;
 }
@@ -1121,6 +1216,16 @@ int mg_set_smoother(Multigrid* mg, int kind, int degree, double lo_frac, double 
     return ASEND_OK;
 }
 
+int upload_factor_coefficients(Multigrid* mg, hipStream_t st) {
+    const int slot = coef_slot(mg, (int)mg->L.size(), 0);
+    cheb_coefficients(mg, mg->lmax_factor, mg->h_coef.data() + slot);
+    MGCHK(hipMemcpyAsync(mg->d_coef + slot, mg->h_coef.data() + slot, 2 * kChebMaxDegree * sizeof(float),
+                         hipMemcpyHostToDevice, st));
+    // the host copy is read by the copy engine: keep it unchanged until done
+    MGCHK(hipStreamSynchronize(st));
+    return ASEND_OK;
+}
+
 int mg_set_learned_factor(Multigrid* mg, hipStream_t st, const Factor& F, bool smoother_bounds) {
     if (!mg->ready) return mgfail(mg, ASEND_ERR_STATE, "multigrid not set up");
     Level& l0 = mg->L[0];
@@ -1129,9 +1234,14 @@ int mg_set_learned_factor(Multigrid* mg, hipStream_t st, const Factor& F, bool s
         MGCHK(hipMalloc((void**)&mg->ft, l0.n_dof * sizeof(float)));
         MGCHK(hipMalloc((void**)&mg->fz, l0.n_dof * sizeof(float)));
     }
+    const double keep = mg->lmax_factor;
     mg->factor = F;
     mg->has_factor = true;
     mg->lmax_factor = 0.0;
+    if (smoother_bounds && mg->keep_factor_bounds && keep > 0.0) {
+        mg->lmax_factor = keep;
+        return upload_factor_coefficients(mg, st);
+    }
     if (!smoother_bounds) return ASEND_OK;
     double lam = 0.0;
     int rc = estimate_lmax(mg, st, 0, true, &lam);
@@ -1140,7 +1250,7 @@ int mg_set_learned_factor(Multigrid* mg, hipStream_t st, const Factor& F, bool s
         return rc;
     }
     mg->lmax_factor = lam;
-    return ASEND_OK;
+    return upload_factor_coefficients(mg, st);
 }
 
 bool mg_ready(const Multigrid* mg) { return mg->ready; }
@@ -1151,6 +1261,10 @@ void mg_clear_learned_factor(Multigrid* mg) {
     mg->has_factor = false;
     mg->fine_smoother = 0;
 }
+
+void mg_keep_factor_bounds(Multigrid* mg, bool keep) { mg->keep_factor_bounds = keep; }
+
+void mg_use_graphs(Multigrid* mg, bool on) { mg->use_graphs = on; }
 
 int mg_setup(Multigrid* mg, hipStream_t st, const double* d_E, const double* d_nu, const float* d_K32,
              const float* d_dinv32, const int32_t* d_kdiag0, double sf, int n_pen, const int64_t* h_pen,
@@ -1250,7 +1364,10 @@ int mg_setup(Multigrid* mg, hipStream_t st, const double* d_E, const double* d_n
     for (size_t l = 0; l + 1 < mg->L.size(); ++l) {
         int rc = estimate_lmax(mg, st, (int)l, false, &mg->L[l].lmax);
         if (rc) return rc;
+        cheb_coefficients(mg, mg->L[l].lmax, mg->h_coef.data() + coef_slot(mg, (int)l, 0));
     }
+    MGCHK(hipMemcpyAsync(mg->d_coef, mg->h_coef.data(), mg->h_coef.size() * sizeof(float), hipMemcpyHostToDevice,
+                         st));
     // coarsest: dense FP64 inverse
     Level& lc = mg->L.back();
     hipLaunchKernelGGL(k_fill, dim3(nb_for((int64_t)mg->nco * mg->nco / 2)), dim3(256), 0, st,
@@ -1261,7 +1378,8 @@ int mg_setup(Multigrid* mg, hipStream_t st, const double* d_E, const double* d_n
         hipLaunchKernelGGL(k_gauss_jordan_lds, dim3(1), dim3(1024), (size_t)mg->nco * (mg->nco + 1) * sizeof(double), st,
                            mg->nco, mg->d_Ainv);
     else
-        hipLaunchKernelGGL(k_gauss_jordan, dim3(1), dim3(1024), mg->nco * sizeof(double), st, mg->nco, mg->d_Ainv);
+        hipLaunchKernelGGL(k_gauss_jordan, dim3(1), dim3(1024), 2 * mg->nco * sizeof(double), st, mg->nco,
+                           mg->d_Ainv);
     MGCHK(hipGetLastError());
     MGCHK(hipStreamSynchronize(st));
     mg->ready = true;
@@ -1293,8 +1411,9 @@ int mg_fcg_solve(Multigrid* mg, hipStream_t st, const float* b, float* x, double
     // z = M r ; p = z
     apply_b(mg->r, mg->z);
     hipLaunchKernelGGL(k_copy, dim3(nb_for(n)), dim3(256), 0, st, n, mg->z, mg->p);
-    int64_t it = 0;
-    for (;;) {
+    // one block of check_every iterations; kernels only (scalars stay on the
+    // device), so the block can be captured once and replayed as a graph
+    auto iterations = [&]() {
         for (int k = 0; k < check_every; ++k) {
             // q = A p ; alpha = (p.r)/(p.q)
             apply_level(mg, st, 0, mg->p, mg->q);
@@ -1311,6 +1430,54 @@ int mg_fcg_solve(Multigrid* mg, hipStream_t st, const float* b, float* x, double
             hipLaunchKernelGGL(k_fcg_fin, dim3(1), dim3(kBlock), 0, st, mg->partials, nvb, 2, mg->sv);
             hipLaunchKernelGGL(k_fcg_dir, dim3(nvb), dim3(kBlock), 0, st, n, mg->z, mg->p, mg->sv);
         }
+    };
+    Multigrid::FcgGraph* gr = nullptr;
+    if (mg->use_graphs) {
+        const float* ls = mg->has_factor ? mg->factor.Ls : nullptr;
+        const float* lst = mg->has_factor ? mg->factor.LsT : nullptr;
+        for (auto& g : mg->graphs)
+            if (g.exec && g.precond == precond && g.fine_smoother == mg->fine_smoother && g.check == check_every &&
+                g.degree == mg->cheb_degree && g.kind == mg->smoother_kind && g.b == b && g.x == x && g.ls == ls &&
+                g.lst == lst)
+                gr = &g;
+        if (!gr) {
+            // capture into the least recently used slot
+            gr = &mg->graphs[0];
+            for (auto& g : mg->graphs)
+                if (g.use < gr->use) gr = &g;
+            if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
+            gr->exec = nullptr;
+            hipGraph_t graph = nullptr;
+            MGCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            iterations();
+            hipError_t e1 = hipGetLastError();
+            hipError_t e2 = hipStreamEndCapture(st, &graph);
+            if (e1 != hipSuccess || e2 != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return mgfail(mg, ASEND_ERR_HIP, "capturing the FCG iteration graph failed");
+            }
+            hipError_t e3 = hipGraphInstantiate(&gr->exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e3 != hipSuccess) {
+                gr->exec = nullptr;
+                return mgfail(mg, ASEND_ERR_HIP, "instantiating the FCG iteration graph failed");
+            }
+            gr->precond = precond;
+            gr->fine_smoother = mg->fine_smoother;
+            gr->check = check_every;
+            gr->degree = mg->cheb_degree;
+            gr->kind = mg->smoother_kind;
+            gr->b = b;
+            gr->x = x;
+            gr->ls = ls;
+            gr->lst = lst;
+        }
+        gr->use = ++mg->graph_clock;
+    }
+    int64_t it = 0;
+    for (;;) {
+        if (gr) MGCHK(hipGraphLaunch(gr->exec, st));
+        else iterations();
         MGCHK(hipGetLastError());
         MGCHK(hipMemcpyAsync(mg->h_sv, mg->sv, SV_N * sizeof(double), hipMemcpyDeviceToHost, st));
         MGCHK(hipStreamSynchronize(st));
@@ -1320,6 +1487,107 @@ int mg_fcg_solve(Multigrid* mg, hipStream_t st, const float* b, float* x, double
     if (it_out) *it_out = it;
     if (!(mg->h_sv[SV_RR] == mg->h_sv[SV_RR]))
         return mgfail(mg, ASEND_ERR_NUMERIC, "multigrid FCG produced a non-finite residual");
+    return ASEND_OK;
+}
+
+// One V-cycle x = B b on b = A e with the learned smoother, its error energy
+// ratio loss = (e - x)^T A (e - x) / e^T A e, and the reverse pass that gives
+// dLoss/dz_k for every use z_k = P v_k of the factor (2 x Chebyshev degree
+// uses on the finest level; the coarse correction Q = Pr V_1 Pr^T is a fixed
+// symmetric operator).  The Chebyshev coefficients are held fixed.
+int mg_vcycle_uses(Multigrid* mg, hipStream_t st, const float* e, FactorUses* uses, double* loss) {
+    if (!mg->ready || !mg_factor_has_bounds(mg)) return mgfail(mg, ASEND_ERR_STATE, "learned smoother not set up");
+    const int D = mg->cheb_degree;
+    if (2 * D > kMaxUses) return mgfail(mg, ASEND_ERR_UNSUPPORTED, "Chebyshev degree too high for training");
+    Level& lv = mg->L[0];
+    Level& cv = mg->L[1];
+    const int64_t n = lv.n_dof;
+    if ((int)mg->tv.size() < 2 * D) {
+        for (int k = (int)mg->tv.size(); k < 2 * D; ++k) {
+            float *a = nullptr, *b = nullptr;
+            MGCHK(hipMalloc((void**)&a, n * sizeof(float)));
+            MGCHK(hipMalloc((void**)&b, n * sizeof(float)));
+            mg->tv.push_back(a);
+            mg->tz.push_back(b);
+        }
+    }
+    if (!mg->tx) {
+        for (float** q : {&mg->tx, &mg->td, &mg->txb, &mg->tdb, &mg->tb, &mg->tq})
+            MGCHK(hipMalloc((void**)q, n * sizeof(float)));
+    }
+    const int nv = nb_for(n);
+    // Chebyshev coefficients of one smoothing call (the same for pre and post)
+    const double lmax = mg->lmax_factor;
+    const double hi = lmax * mg->cheb_hi, lo = lmax * mg->cheb_lo;
+    const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+    std::vector<float> c1(D), c2(D);
+    double rho0 = 1.0 / sigma;
+    c1[0] = 0.f;
+    c2[0] = (float)(1.0 / theta);
+    for (int k = 1; k < D; ++k) {
+        const double rho1 = 1.0 / (2.0 * sigma - rho0);
+        c1[k] = (float)(rho1 * rho0);
+        c2[k] = (float)(2.0 * rho1 / delta);
+        rho0 = rho1;
+    }
+    // forward: b = A e, x = B b, recording the inputs of P
+    apply_level(mg, st, 0, e, mg->tb);
+    hipLaunchKernelGGL(k_fill, dim3(nv), dim3(256), 0, st, n, 0.f, mg->tx);
+    hipLaunchKernelGGL(k_fill, dim3(nv), dim3(256), 0, st, n, 0.f, mg->td);
+    for (int j = 0; j < 2 * D; ++j) {
+        const int k = j % D;
+        if (j == D) {
+            // coarse correction x += Q (b - A x)
+            residual_level(mg, st, 0, mg->tb, mg->tx, lv.r);
+            restrict_to(mg, st, 0, lv.r, cv.b);
+            vcycle(mg, st, 1, cv.b, cv.x);
+            prolong_add(mg, st, 0, cv.x, mg->tx);
+        }
+        if (j == 0) hipLaunchKernelGGL(k_copy, dim3(nv), dim3(256), 0, st, n, mg->tb, mg->tv[0]);
+        else residual_level(mg, st, 0, mg->tb, mg->tx, mg->tv[j]);
+        launch_factor_apply(st, mg->factor, mg->tv[j], mg->ft, mg->fz);
+        hipLaunchKernelGGL(k_cheb_step, dim3(nv), dim3(256), 0, st, n, (const float*)nullptr, mg->fz, c1[k], c2[k],
+                           mg->td, mg->tx);
+    }
+    // loss: e' = e - x, w = A e'
+    hipLaunchKernelGGL(k_lincomb, dim3(nv), dim3(256), 0, st, n, 1.f, e, -1.f, mg->tx, mg->tq);
+    apply_level(mg, st, 0, mg->tq, mg->txb);
+    hipLaunchKernelGGL(k_dot2_partials, dim3(kVecBlocks), dim3(kBlock), 0, st, n, mg->tq, mg->txb, e, mg->tb,
+                       mg->partials);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, st, mg->partials, kVecBlocks, 2, mg->sv + SV_N);
+    MGCHK(hipMemcpyAsync(mg->h_sv, mg->sv + SV_N, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    MGCHK(hipStreamSynchronize(st));
+    const double num = mg->h_sv[0], den = mg->h_sv[1];
+    if (!(den > 0.0) || !std::isfinite(num)) return mgfail(mg, ASEND_ERR_NUMERIC, "V-cycle training loss not finite");
+    *loss = num / den;
+    // reverse pass: xbar = dLoss/dx = -2 A e' / e^T A e, dbar = 0
+    hipLaunchKernelGGL(k_scale, dim3(nv), dim3(256), 0, st, n, mg->txb, (float)(-2.0 / den), mg->txb);
+    hipLaunchKernelGGL(k_fill, dim3(nv), dim3(256), 0, st, n, 0.f, mg->tdb);
+    for (int j = 2 * D - 1; j >= 0; --j) {
+        const int k = j % D;
+        hipLaunchKernelGGL(k_cheb_adjoint, dim3(nv), dim3(256), 0, st, n, c1[k], c2[k], mg->txb, mg->tdb, mg->tz[j]);
+        if (j > 0) {
+            // v_j = b - A x_prev:  xbar -= A P zbar_j
+            launch_factor_apply(st, mg->factor, mg->tz[j], mg->ft, mg->fz);
+            apply_level(mg, st, 0, mg->fz, mg->tq);
+            hipLaunchKernelGGL(k_lincomb, dim3(nv), dim3(256), 0, st, n, 1.f, mg->txb, -1.f, mg->tq, mg->txb);
+        }
+        if (j == D) {
+            // through x += Q (b - A x):  xbar -= A Q xbar
+            restrict_to(mg, st, 0, mg->txb, cv.b);
+            vcycle(mg, st, 1, cv.b, cv.x);
+            hipLaunchKernelGGL(k_fill, dim3(nv), dim3(256), 0, st, n, 0.f, mg->tq);
+            prolong_add(mg, st, 0, cv.x, mg->tq);
+            apply_level(mg, st, 0, mg->tq, lv.r);
+            hipLaunchKernelGGL(k_lincomb, dim3(nv), dim3(256), 0, st, n, 1.f, mg->txb, -1.f, lv.r, mg->txb);
+        }
+    }
+    MGCHK(hipGetLastError());
+    uses->m = 2 * D;
+    for (int j = 0; j < 2 * D; ++j) {
+        uses->v[j] = mg->tv[j];
+        uses->zbar[j] = mg->tz[j];
+    }
     return ASEND_OK;
 }
 

@@ -38,7 +38,18 @@ int mg_set_smoother(Multigrid* mg, int kind, int degree, double lo_frac, double 
 // it is in use); with smoother_bounds also estimate lambda_max(P A), which the
 // Chebyshev smoother needs.
 int mg_set_learned_factor(Multigrid* mg, hipStream_t st, const Factor& F, bool smoother_bounds);
+// Training through one V-cycle with the learned smoother: loss = energy of the
+// error left by one cycle on b = A e, relative to e; fills the factor uses
+// (inputs v_k of P and dLoss/dz_k) for learned_step_from_uses.  The returned
+// pointers stay valid until the next call.
+int mg_vcycle_uses(Multigrid* mg, hipStream_t st, const float* e, FactorUses* uses, double* loss);
 bool mg_factor_has_bounds(const Multigrid* mg);
+// Diagnostics: keep the previous factor's lambda_max when a new factor is set
+// (finite-difference checks of the training gradient, which holds it fixed).
+void mg_keep_factor_bounds(Multigrid* mg, bool keep);
+// Replay each block of check_every FCG iterations as a captured hipGraph
+// (default on); off launches the same kernels one by one.
+void mg_use_graphs(Multigrid* mg, bool on);
 bool mg_ready(const Multigrid* mg);
 void mg_clear_learned_factor(Multigrid* mg);
 

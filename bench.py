@@ -7,10 +7,15 @@ compliance, adjoint (lambda = u for compliance) and dJ/dE for all elements.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--nx 200 --ny 40 --nz 40]
 
-Prints one JSON line.  `value` is the mean time-to-solution of the fast path
-(learned/multigrid-preconditioned FP32 flexible PCG inside FP64 refinement when
-available, else FP32 Jacobi PCG inside FP64 refinement); the plain FP64 Jacobi
-PCG pipeline on the same GPU is timed alongside (`fp64_jacobi_s`, `speedup`).
+Prints one JSON line.  `value` is the mean time-to-solution of the learned +
+mixed-precision path: FP32 flexible PCG inside FP64 refinement, preconditioned
+by a multigrid V-cycle whose finest Chebyshev smoother is the learned factor
+L L^T.  The network is first trained on the GPU (8 lognormal 110x22x22 meshes,
+250 steps each = 2000 solver-generated samples; time reported as
+`train_s`, outside `value`), then inferred inside every timed step.  The plain
+FP64 Jacobi PCG pipeline on the same GPU is timed alongside (`fp64_jacobi_s`,
+`speedup_vs_fp64_jacobi`), and so is the same multigrid without the network
+(`mg_no_network_s`).
 """
 import argparse
 import json
@@ -68,6 +73,32 @@ def cpu_baseline(prob, fp64_iters, nx, ny, nz):
                        f"and {fp64_iters} iterations")}
 
 
+def train_network(ctx, n_meshes, steps):
+    """GPU training of the learned preconditioner on lognormal 110x22x22 brick8
+    meshes (53,240 elements each, sigma_ln cycling 0.6..1.15, i.e. moduli
+    contrast up to ~1e3 at +-3 sigma), `steps` solver-generated samples per
+    mesh."""
+    from asend3d_amd import api, problems
+    ctx.learned_init(7)
+    t0 = time.perf_counter()
+    first, last = [], []
+    for m in range(n_meshes):
+        q = problems.heterogeneous(110, 22, 22, seed=1000 + m, sigma_ln=(0.6, 0.8, 1.0, 1.15)[m % 4], lx=550.0,
+                                   ly=110.0, lz=110.0)
+        problems.upload(ctx, q)
+        ctx.set_grid(110, 22, 22)
+        ctx.assemble()
+        L = ctx.learned_train(steps, lr=1e-3, seed=m + 1, mode=1)
+        first.append(float(np.mean(L[0:30:3])))
+        last.append(float(np.mean(L[-30::3])))
+    ctx.sync()
+    t = time.perf_counter() - t0
+    log(f"trained the network on {n_meshes} meshes x {steps} steps in {t:.1f} s "
+        f"(Gaussian-sample loss {first[0]:.3f} -> {last[-1]:.3f})")
+    return {"meshes": n_meshes, "steps_per_mesh": steps, "samples": n_meshes * steps, "elements_per_mesh": 53240,
+            "train_s": t, "loss_first": first[0], "loss_last": last[-1]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,7 +109,9 @@ def main():
     ap.add_argument("--nz", type=int, default=40)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--fast-precond", default="auto", choices=["auto", "jacobi", "mg", "learned"])
+    ap.add_argument("--fast-precond", default="learned", choices=["jacobi", "mg", "learned"])
+    ap.add_argument("--train-meshes", type=int, default=8)
+    ap.add_argument("--train-steps", type=int, default=250)
     ap.add_argument("--kernels-only", action="store_true",
                     help="assemble, then time only the SpMV and copy kernels (short run for counter profiling)")
     args = ap.parse_args()
@@ -105,15 +138,13 @@ def main():
                           "copy_gbs": 2 * copy_bytes / (msc * 1e-3) / 1e9}), flush=True)
         ctx.close()
         return
-    pc = api.PC_JACOBI
-    if args.fast_precond in ("auto", "mg", "learned"):
-        try:
-            ctx.set_grid(args.nx, args.ny, args.nz)
-            pc = api.PC_LEARNED if args.fast_precond == "learned" else api.PC_MG
-        except api.AsendError as e:
-            if args.fast_precond != "auto":
-                raise
-            log(f"multigrid unavailable ({e}); fast path uses Jacobi")
+    pc = {"jacobi": api.PC_JACOBI, "mg": api.PC_MG, "learned": api.PC_LEARNED}[args.fast_precond]
+    train = None
+    if pc == api.PC_LEARNED:
+        train = train_network(ctx, args.train_meshes, args.train_steps)
+        problems.upload(ctx, prob)
+    if pc != api.PC_JACOBI:
+        ctx.set_grid(args.nx, args.ny, args.nz)
     fast = ctx.opts(api.SOLVE_IR_MIXED, pc, tol=args.tol, inner_tol=1e-4 if pc == api.PC_JACOBI else 1e-3,
                     max_it=2000000, max_outer=40, check_every=32 if pc == api.PC_JACOBI else 4)
     base = ctx.opts(api.SOLVE_PCG_FP64, api.PC_JACOBI, tol=args.tol, max_it=2000000, check_every=32)
@@ -143,6 +174,15 @@ def main():
     t_base = stb.device_ms / 1e3
     log(f"FP64 Jacobi PCG: {t_base:.3f} s, {stb.iterations} it, relres {stb.rel_residual:.2e}")
     err_s = float(np.abs(s - sb).max() / np.abs(sb).max())
+    ablation = None
+    if pc == api.PC_LEARNED:
+        # the same pipeline without the network (block-Jacobi Chebyshev smoother)
+        mg_o = ctx.opts(api.SOLVE_IR_MIXED, api.PC_MG, tol=args.tol, inner_tol=1e-3, max_it=2000000, max_outer=40,
+                        check_every=4)
+        ctx.static_compliance(prob.f, mg_o)
+        _, _, _, stm = ctx.static_compliance(prob.f, mg_o)
+        ablation = {"mg_no_network_s": stm.device_ms / 1e3, "mg_no_network_iterations": int(stm.iterations)}
+        log(f"multigrid without the network: {stm.device_ms / 1e3:.3f} s, {stm.iterations} it")
 
     # roofline of the dominant kernel: FP32 block-row SpMV, HIP events on the context stream
     import ctypes
@@ -183,10 +223,13 @@ def main():
         "config": {"workload": f"hex8 cantilever {args.nx}x{args.ny}x{args.nz} static + compliance adjoint",
                    "n_dof": prob.n_dof, "n_elements": prob.mesh.n_el, "tol": args.tol,
                    "fast_precond": {api.PC_JACOBI: "jacobi", api.PC_MG: "mg", api.PC_LEARNED: "learned"}[pc],
+                   "training": train,
                    "parallelism": "single GPU"},
         "fp64_jacobi_s": t_base,
         "speedup_vs_fp64_jacobi": t_base / t_fast,
         "fast_iterations": int(st.iterations),
+        "fallback_used": int(st.fallback_used),
+        "ablation": ablation,
         "fast_outer": int(st.outer_iterations),
         "fp64_iterations": int(stb.iterations),
         "rel_residual": st.rel_residual,

@@ -214,6 +214,11 @@ int asend_mg_apply(asend_ctx* ctx, int level, const float* x, float* y);
 int asend_mg_restrict(asend_ctx* ctx, int level, const float* fine, float* coarse);
 int asend_mg_prolong(asend_ctx* ctx, int level, const float* coarse, float* fine);
 int asend_mg_vcycle(asend_ctx* ctx, const float* b, float* x);
+/* Replay the launch sequences of the iterative loops (a block of check_every
+ * FCG iterations, a Newmark time step) as captured hipGraphs (on = 1, the
+ * default) or launch the same kernels one by one (on = 0).  Results are
+ * identical either way. */
+int asend_set_graphs(asend_ctx* ctx, int on);
 /* Chebyshev smoother of every smoothed level: M = point Jacobi (kind 0) or the
  * inverse of the 3x3 node blocks (kind 1), degree 1..8, interval
  * [lo_frac, hi_frac] * lambda_max(M A).  Defaults 1, 2, 0.1, 1.15.  Kept across
@@ -240,18 +245,27 @@ int asend_learned_num_params(void);
 int asend_learned_init(asend_ctx* ctx, uint64_t seed);
 int asend_learned_get_params(asend_ctx* ctx, float* out, int n);
 int asend_learned_set_params(asend_ctx* ctx, const float* in, int n);
-/* n_steps Adam steps (learning rate lr) on the current assembled mesh, each on
- * one pair (r = K e, e).  Without solver_samples e is Gaussian (zero on
- * penalised nodes); with it the steps cycle through a Gaussian e, the error
- * e = (I - B K) e0 left by one multigrid V-cycle B of the solver, and the
- * approximate solution e = B b of a random load b.  Loss: the energy left by
- * the best damped step, min_w ||e - w P K e||_K^2 / ||e||_K^2 (independent of
- * the scale of P).  losses[k] receives each step's loss when non-null. */
-int asend_learned_train(asend_ctx* ctx, int n_steps, float lr, uint64_t seed, int solver_samples, double* losses);
+/* n_steps Adam steps (learning rate lr) on the current assembled mesh.
+ * mode 0: one pair (r = K e, e) per step, e Gaussian (zero on penalised
+ *   nodes); loss = energy left by the best damped step,
+ *   min_w ||e - w P K e||_K^2 / ||e||_K^2 (independent of the scale of P).
+ * mode 1: as mode 0, but the steps cycle through a Gaussian e, the error
+ *   e = (I - B K) e0 left by one multigrid V-cycle B of the solver, and the
+ *   approximate solution e = B b of a random load b.
+ * mode 2: end to end through the solver: P is the finest Chebyshev smoother
+ *   of one V-cycle B; loss = ||(I - B K) e||_K^2 / ||e||_K^2 for a Gaussian e,
+ *   differentiated through every use of P in the cycle.
+ * losses[k] receives each step's loss when non-null. */
+int asend_learned_train(asend_ctx* ctx, int n_steps, float lr, uint64_t seed, int mode, double* losses);
 /* Diagnostics: loss and parameter gradient for the sample e (n_dof), node
  * features (n_nodes x 8) and head outputs (n_nodes x 24); null outputs skipped.
  * Node order is the DOF-block order. */
 int asend_learned_eval(asend_ctx* ctx, const double* e, double* loss, float* grad, float* feat, float* out);
+/* Diagnostics of training mode 2: loss of one V-cycle on the sample e (n_dof)
+ * and its parameter gradient (grad may be null).  keep_bounds != 0 reuses the
+ * Chebyshev bounds of the previous call instead of re-estimating them, which
+ * is what the gradient assumes (finite-difference checks). */
+int asend_learned_vcycle_eval(asend_ctx* ctx, const double* e, int keep_bounds, double* loss, float* grad);
 /* z = L L^T r with the factor of the current operator (FP32 inside). */
 int asend_learned_apply(asend_ctx* ctx, const double* r, double* z);
 /* Mesh-edge neighbour tables, [n_edge][n_nodes] each (-1 = none), as used by L. */

@@ -86,16 +86,17 @@ def test_training_reduces_loss_and_is_deterministic(ctx):
     ctx.set_grid(*dims)
     ctx.assemble()
     ctx.learned_init(5)
-    losses = ctx.learned_train(240, lr=2e-3, seed=3, solver_samples=True)
+    losses = ctx.learned_train(240, lr=2e-3, seed=3, mode=1)
     assert np.all(np.isfinite(losses))
     # the three sample kinds (Gaussian, V-cycle error, V-cycle solution) have
     # different loss levels; the Gaussian and V-cycle-error losses must drop
-    for kind in (0, 1):
+    # (observed: Gaussian 0.21 -> 0.08, V-cycle error 0.133 -> 0.113)
+    for kind, factor in ((0, 0.7), (1, 0.93)):
         lk = losses[kind::3]
-        assert lk[-10:].mean() < 0.85 * lk[:10].mean(), (kind, lk[:10].mean(), lk[-10:].mean())
+        assert lk[-10:].mean() < factor * lk[:10].mean(), (kind, lk[:10].mean(), lk[-10:].mean())
     w1 = ctx.learned_params()
     ctx.learned_init(5)
-    ctx.learned_train(240, lr=2e-3, seed=3, solver_samples=True)
+    ctx.learned_train(240, lr=2e-3, seed=3, mode=1)
     w2 = ctx.learned_params()
     assert np.array_equal(w1, w2)
 
@@ -124,3 +125,64 @@ def test_learned_solve_matches_fp64_and_falls_back(ctx):
     u, s, J, st = ctx.static_compliance(p.f, o)
     assert st.converged and st.fallback_used == 1, st.as_dict()
     assert np.abs(u - u_ref).max() <= 1e-6 * np.abs(u_ref).max()
+
+
+def test_vcycle_training_gradient_matches_finite_differences(ctx):
+    # end-to-end training differentiates through every use of P in one
+    # V-cycle (Chebyshev bounds held fixed); check along directions of the
+    # head and edge map, in which the loss is smooth
+    dims = (24, 6, 6)
+    gpu_case(f"learned V-cycle gradient FD {dims}")
+    p = small_problem(dims, seed=3)
+    problems.upload(ctx, p)
+    ctx.set_grid(*dims)
+    ctx.assemble()
+    ctx.learned_init(2)
+    prm = mpnn.random_params(5, head_scale=0.05).astype(np.float32)
+    ctx.learned_set_params(prm)
+    rng = np.random.default_rng(0)
+    e = rng.standard_normal(p.n_dof)
+    _, g0 = ctx.learned_vcycle_eval(e)
+    h = 3e-3
+    for _ in range(3):
+        d = np.zeros(prm.size, np.float32)
+        d[mpnn.OFF_WH:] = rng.standard_normal(prm.size - mpnn.OFF_WH).astype(np.float32)
+        ctx.learned_set_params(prm + h * d)
+        lp, _ = ctx.learned_vcycle_eval(e, keep_bounds=True)
+        ctx.learned_set_params(prm - h * d)
+        lm, _ = ctx.learned_vcycle_eval(e, keep_bounds=True)
+        fd = (lp - lm) / (2 * h)
+        # FP32 cycle: agreement to ~1e-2 relative (observed <= 6e-3)
+        assert abs(fd - g0 @ d) <= 2e-2 * abs(fd), (fd, float(g0 @ d))
+        ctx.learned_set_params(prm)
+        ctx.learned_vcycle_eval(e)
+
+
+def test_trained_network_cuts_pcg_iterations(ctx):
+    # what the network gains over the method without it: the untrained
+    # network is exactly block Jacobi (head and edge map zero); after training
+    # on four other 40x8x8 lognormal meshes the learned L L^T must need fewer
+    # PCG iterations on a held-out mesh (observed 1648 -> 1368)
+    dims = (40, 8, 8)
+    gpu_case(f"learned gain {dims}")
+    o = ctx.opts(api.SOLVE_IR_MIXED, api.PC_LEARNED_PCG, tol=1e-8, inner_tol=1e-3, max_it=400000, max_outer=40,
+                 check_every=4)
+
+    def setup(seed, sig=1.0):
+        q = problems.heterogeneous(*dims, seed=seed, sigma_ln=sig, lx=200.0, ly=40.0, lz=40.0)
+        problems.upload(ctx, q)
+        ctx.set_grid(*dims)
+        ctx.assemble()
+        return q
+
+    ctx.learned_init(7)
+    q = setup(4242)
+    u0, _, _, st0 = ctx.static_compliance(q.f, o)
+    for m, sig in enumerate((0.6, 0.8, 1.0, 1.15)):
+        setup(100 + m, sig)
+        ctx.learned_train(150, lr=1e-3, seed=m + 1, mode=1)
+    q = setup(4242)
+    u1, _, _, st1 = ctx.static_compliance(q.f, o)
+    assert st0.converged and st1.converged and st1.fallback_used == 0
+    assert st1.iterations <= 0.92 * st0.iterations, (st0.iterations, st1.iterations)
+    assert np.abs(u1 - u0).max() <= 1e-6 * np.abs(u0).max()

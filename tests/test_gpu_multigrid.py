@@ -114,3 +114,29 @@ def test_mg_compliance_pipeline_matches_oracle(ctx):
     assert st.converged
     assert abs(J - p.f @ u_ref) <= 1e-8 * abs(J)
     assert np.abs(s - s_ref).max() <= 1e-7 * np.abs(s_ref).max()
+
+
+def test_graph_replay_matches_eager_launches(ctx):
+    # the FCG iteration blocks replayed from captured hipGraphs give bitwise the
+    # results of launching the same kernels one by one, over several replays
+    # with eager work in between (two pipelines, smoother changes)
+    dims = (20, 6, 6)
+    gpu_case(f"graph replay {dims}")
+    p = problems.heterogeneous(*dims, seed=12)
+    setup(ctx, p, dims)
+    o = ctx.opts(api.SOLVE_IR_MIXED, api.PC_MG, tol=1e-8, inner_tol=1e-3, max_it=100000, max_outer=40, check_every=3)
+    res = {}
+    for on in (False, True, True):
+        ctx.set_graphs(on)
+        out = []
+        for kind in (1, 0, 1):
+            ctx.mg_set_smoother(kind, 2, 0.1, 1.15)
+            u, s, J, st = ctx.static_compliance(p.f, o)
+            assert st.converged
+            out.append((u.copy(), st.iterations))
+        res.setdefault(on, []).append(out)
+    ctx.set_graphs(True)
+    ref = res[False][0]
+    for run in res[True]:
+        for (u1, it1), (u0, it0) in zip(run, ref):
+            assert it1 == it0 and np.array_equal(u1, u0)
