@@ -43,6 +43,15 @@ class SolveOpts(ctypes.Structure):
                 ("max_it", ctypes.c_int64), ("max_outer", ctypes.c_int64)]
 
 
+class NewmarkStats(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int64), ("total_iterations", ctypes.c_int64),
+                ("max_step_iterations", ctypes.c_int64), ("unconverged_steps", ctypes.c_int64),
+                ("graph_used", ctypes.c_int), ("device_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class SolveStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int64), ("outer_iterations", ctypes.c_int64),
                 ("rel_residual", ctypes.c_double), ("ref_residual", ctypes.c_double),
@@ -72,7 +81,8 @@ EXPORTED = ["asend_ctx_create", "asend_ctx_destroy", "asend_last_error", "asend_
             "asend_mg_restrict", "asend_mg_prolong", "asend_mg_vcycle", "asend_bench_copy",
             "asend_learned_num_params", "asend_learned_init", "asend_learned_get_params",
             "asend_learned_set_params", "asend_learned_train", "asend_learned_eval", "asend_learned_apply",
-            "asend_learned_edges", "asend_mg_set_smoother", "asend_learned_vcycle_eval", "asend_set_graphs"]
+            "asend_learned_edges", "asend_mg_set_smoother", "asend_learned_vcycle_eval", "asend_set_graphs",
+            "asend_newmark_setup", "asend_newmark_run"]
 
 
 def lib():
@@ -307,6 +317,28 @@ class Context:
         x = np.zeros(b.shape[0], dtype=np.float32)
         self._chk(_lib.asend_mg_vcycle(self.h, _p(b), _p(x)), "asend_mg_vcycle")
         return x
+
+    def newmark_setup(self, dt, beta=0.25, gamma=0.5):
+        self._chk(_lib.asend_newmark_setup(self.h, _dbl(dt), _dbl(beta), _dbl(gamma)), "asend_newmark_setup")
+
+    def newmark_run(self, n_steps, f_shape, load_fac, watch=(), u0=None, v0=None, a0=None, tol=1e-10,
+                    max_it_step=50):
+        """Returns (hist [n_steps x n_watch], u, v, a, NewmarkStats)."""
+        f_shape = _f64(f_shape)
+        load_fac = _f64(load_fac)
+        watch = _i64a(np.asarray(watch, dtype=np.int64).reshape(-1))
+        hist = np.zeros((n_steps, max(watch.size, 1)))
+        u = np.zeros(self.n_dof)
+        v = np.zeros(self.n_dof)
+        a = np.zeros(self.n_dof)
+        st = NewmarkStats()
+        opt = [None if x is None else _f64(x) for x in (u0, v0, a0)]
+        self._chk(_lib.asend_newmark_run(self.h, ctypes.c_int64(n_steps), _p(f_shape), _p(load_fac), _p(opt[0]),
+                                         _p(opt[1]), _p(opt[2]), ctypes.c_int64(watch.size),
+                                         _p(watch) if watch.size else None, _p(hist) if watch.size else None,
+                                         _p(u), _p(v), _p(a), _dbl(tol), ctypes.c_int(max_it_step),
+                                         ctypes.byref(st)), "asend_newmark_run")
+        return hist[:, :watch.size], u, v, a, st
 
     def set_graphs(self, on=True):
         self._chk(_lib.asend_set_graphs(self.h, 1 if on else 0), "asend_set_graphs")

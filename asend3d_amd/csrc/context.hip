@@ -83,7 +83,22 @@ struct asend_ctx {
     int n_edge = 0;
     double mean_log_e = 0.0;
     int64_t asm_count = 0, factor_asm = -1;
+    double asm_cmass = 0.0;  // mass coefficient of the current assembly
     bool use_graphs = true;
+
+    // Newmark time stepping: mass operator, state and per-run buffers
+    struct Newmark {
+        bool ready = false;
+        double dt = 0, beta = 0, gamma = 0, c_m = 0, c2 = 0;
+        double* M64 = nullptr;
+        double *v = nullptr, *a = nullptr, *w = nullptr, *q = nullptr, *rhs0 = nullptr, *rhs = nullptr,
+               *fshape = nullptr;
+        double *fac = nullptr, *hist = nullptr;
+        int64_t *step = nullptr, *its = nullptr, *watch = nullptr;
+        int64_t fac_cap = 0, hist_cap = 0, watch_cap = 0;
+        hipGraphExec_t exec = nullptr;
+        int graph_iters = -1, graph_watch = -1;
+    } nm;
 };
 
 namespace {
@@ -147,6 +162,18 @@ int write_scalars(asend_ctx* ctx, int idx) {
     return ASEND_OK;
 }
 
+void newmark_free(asend_ctx* ctx) {
+    auto& nm = ctx->nm;
+    for (double** q : {&nm.M64, &nm.v, &nm.a, &nm.w, &nm.q, &nm.rhs0, &nm.rhs, &nm.fshape, &nm.fac, &nm.hist})
+        dfree(*q);
+    for (int64_t** q : {&nm.step, &nm.its, &nm.watch}) dfree(*q);
+    nm.fac_cap = nm.hist_cap = nm.watch_cap = 0;
+    if (nm.exec) (void)hipGraphExecDestroy(nm.exec);
+    nm.exec = nullptr;
+    nm.graph_iters = nm.graph_watch = -1;
+    nm.ready = false;
+}
+
 void free_mesh(asend_ctx* ctx) {
     dfree(ctx->d_conn_r);
     dfree(ctx->d_rp);
@@ -178,6 +205,7 @@ void free_mesh(asend_ctx* ctx) {
     dfree(ctx->d_p32);
     dfree(ctx->d_q32);
     dfree(ctx->d_w32);
+    newmark_free(ctx);
     dfree(ctx->d_n2e_ptr);
     dfree(ctx->d_n2e);
     dfree(ctx->d_clamped);
@@ -472,6 +500,7 @@ int assemble_device(asend_ctx* ctx, double c_mass) {
     HIPCHK(hipGetLastError());
     ctx->assembled = true;
     ctx->asm_count += 1;
+    ctx->asm_cmass = c_mass;
     if (ctx->mg) {
         if ((rc = mg_setup(ctx->mg, ctx->st, ctx->d_E, ctx->d_nu, ctx->d_K32, ctx->d_dinv32, ctx->d_kdiag, ctx->sf,
                            ctx->n_pen, ctx->h_pen.data(), ctx->h_pen_mult.data())))
@@ -998,6 +1027,187 @@ int asend_bench_spmv(asend_ctx* ctx, int precision, int n_iter, double* avg_ms) 
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     *avg_ms = ms / n_iter;
+    return ASEND_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Newmark dynamics
+// ---------------------------------------------------------------------------
+int asend_newmark_setup(asend_ctx* ctx, double dt, double beta, double gamma) {
+    if (!ctx || !(dt > 0.0)) return ASEND_ERR_ARG;
+    if (!ctx->d_K64) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
+    if (!(gamma > beta)) return fail(ctx, ASEND_ERR_ARG, "Newmark needs gamma > beta for an SPD effective operator");
+    HIPCHK(hipSetDevice(ctx->device));
+    newmark_free(ctx);
+    auto& nm = ctx->nm;
+    nm.dt = dt;
+    nm.beta = beta;
+    nm.gamma = gamma;
+    nm.c_m = 1.0 / (dt * dt * (gamma - beta));  // = -c1 of update_vel_acc
+    nm.c2 = dt * dt * (0.5 + beta - gamma);
+    const int64_t n = ctx->n_dof, nval = ctx->nval();
+    int rc;
+    if ((rc = dalloc(ctx, &nm.M64, nval)) || (rc = dalloc(ctx, &nm.v, n)) || (rc = dalloc(ctx, &nm.a, n)) ||
+        (rc = dalloc(ctx, &nm.w, n)) || (rc = dalloc(ctx, &nm.q, n)) || (rc = dalloc(ctx, &nm.rhs0, n)) ||
+        (rc = dalloc(ctx, &nm.rhs, n)) || (rc = dalloc(ctx, &nm.fshape, n)) || (rc = dalloc(ctx, &nm.step, 1)))
+        return rc;
+    // consistent mass alone: the element kernel with zero moduli and c_mass = 1
+    double* zeroE = nullptr;
+    if ((rc = dalloc(ctx, &zeroE, ctx->n_el))) return rc;
+    HIPCHK(hipMemsetAsync(zeroE, 0, ctx->n_el * 8, ctx->st));
+    HIPCHK(hipMemsetAsync(nm.M64, 0, nval * 8, ctx->st));
+    const int ncol = (int)ctx->h_color_ptr.size() - 1;
+    for (int c = 0; c < ncol; ++c) {
+        int b = ctx->h_color_ptr[c], e = ctx->h_color_ptr[c + 1];
+        launch_hex8_assemble(ctx->st, ctx->d_elems + b, e - b, ctx->d_conn_r, ctx->d_xyz_r, zeroE, ctx->d_nu,
+                             ctx->d_rho, 1.0, ctx->d_sptr, ctx->d_slot, nm.M64);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    dfree(zeroE);
+    // K_eff = K + c_m M (+ penalty).  The reference scales its penalty once,
+    // on the first matrix a solve step assembles, which for a dynamic job
+    // already holds the mass term (S/model_analysis.rs:255-259, :486-488)
+    ctx->sf_fixed = false;
+    if ((rc = assemble_device(ctx, nm.c_m))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    nm.ready = true;
+    return ASEND_OK;
+}
+
+int asend_newmark_run(asend_ctx* ctx, int64_t n_steps, const double* f_shape, const double* load_fac,
+                      const double* u0, const double* v0, const double* a0, int64_t n_watch, const int64_t* watch,
+                      double* hist, double* u_out, double* v_out, double* a_out, double tol, int max_it_step,
+                      asend_newmark_stats* stats) {
+    if (!ctx || n_steps < 0 || !f_shape || !load_fac || n_watch < 0 || (n_watch > 0 && (!watch || !hist)) ||
+        !(tol > 0.0) || max_it_step < 1)
+        return ASEND_ERR_ARG;
+    auto& nm = ctx->nm;
+    if (!nm.ready) return fail(ctx, ASEND_ERR_STATE, "call asend_newmark_setup first");
+    const int64_t n = ctx->n_dof;
+    for (int64_t k = 0; k < n_watch; ++k)
+        if (watch[k] < 0 || watch[k] >= n) return fail(ctx, ASEND_ERR_ARG, "watched DOF out of range");
+    int rc;
+    // the operator must be K_eff of this setup (a static assembly may have replaced it)
+    if (!ctx->assembled || ctx->asm_cmass != nm.c_m) {
+        if ((rc = assemble_device(ctx, nm.c_m))) return rc;
+    }
+    bool realloc = false;
+    if (n_steps > nm.fac_cap) {
+        if ((rc = dalloc(ctx, &nm.fac, n_steps)) || (rc = dalloc(ctx, &nm.its, n_steps))) return rc;
+        nm.fac_cap = n_steps;
+        realloc = true;
+    }
+    if (n_steps * std::max<int64_t>(n_watch, 1) > nm.hist_cap) {
+        if ((rc = dalloc(ctx, &nm.hist, n_steps * std::max<int64_t>(n_watch, 1)))) return rc;
+        nm.hist_cap = n_steps * std::max<int64_t>(n_watch, 1);
+        realloc = true;
+    }
+    if (std::max<int64_t>(n_watch, 1) > nm.watch_cap) {
+        if ((rc = dalloc(ctx, &nm.watch, std::max<int64_t>(n_watch, 1)))) return rc;
+        nm.watch_cap = std::max<int64_t>(n_watch, 1);
+        realloc = true;
+    }
+    if (realloc && nm.exec) {
+        (void)hipGraphExecDestroy(nm.exec);
+        nm.exec = nullptr;
+    }
+    HIPCHK(hipMemcpy(nm.fshape, f_shape, n * 8, hipMemcpyHostToDevice));
+    if (n_steps) HIPCHK(hipMemcpy(nm.fac, load_fac, n_steps * 8, hipMemcpyHostToDevice));
+    if (n_watch) HIPCHK(hipMemcpy(nm.watch, watch, n_watch * 8, hipMemcpyHostToDevice));
+    if (u0) HIPCHK(hipMemcpy(ctx->d_u, u0, n * 8, hipMemcpyHostToDevice));
+    else HIPCHK(hipMemset(ctx->d_u, 0, n * 8));
+    if (v0) HIPCHK(hipMemcpy(nm.v, v0, n * 8, hipMemcpyHostToDevice));
+    else HIPCHK(hipMemset(nm.v, 0, n * 8));
+    if (a0) HIPCHK(hipMemcpy(nm.a, a0, n * 8, hipMemcpyHostToDevice));
+    else HIPCHK(hipMemset(nm.a, 0, n * 8));
+    HIPCHK(hipMemset(nm.step, 0, 8));
+    // per-step solve: FP64 Jacobi PCG on K_eff, relative tolerance, at most
+    // max_it_step iterations (the kernels skip their work once converged)
+    {
+        Scalars& h = ctx->h_sc[1];
+        std::memset(&h, 0, sizeof(Scalars));
+        h.tol = tol;
+        h.tol2 = tol * tol;
+        h.max_it = max_it_step;
+        if ((rc = write_scalars(ctx, 1))) return rc;
+    }
+    Scalars* sc = ctx->d_sc + 1;
+    const int nw = (int)n_watch;
+    auto one_step = [&]() {
+        launch_nm_predict(ctx->st, n, ctx->d_u, nm.v, nm.a, nm.dt, nm.c2, nm.w);
+        launch_sell3<double>(ctx->st, 0, ctx->fine(), nm.M64, nm.w, nm.q, nullptr, nullptr, nullptr);
+        launch_nm_rhs(ctx->st, n, nm.step, nm.fac, nm.fshape, nm.c_m, nm.q, nm.rhs0);
+        launch_sell3<double>(ctx->st, 3, ctx->fine(), ctx->d_K64, ctx->d_u, nm.rhs, nm.rhs0, nullptr, nullptr);
+        launch_pcg_init<double>(ctx->st, n, nm.rhs, ctx->d_dinv64, ctx->d_x64, ctx->d_r2, ctx->d_p,
+                                ctx->d_partials, sc);
+        for (int k = 0; k < max_it_step; ++k) {
+            launch_sell3<double>(ctx->st, 1, ctx->fine(), ctx->d_K64, ctx->d_p, ctx->d_q, nullptr, ctx->d_partials,
+                                 sc);
+            launch_pcg_update<double>(ctx->st, n, ctx->d_dinv64, ctx->d_x64, ctx->d_r2, ctx->d_p, ctx->d_q,
+                                      ctx->d_partials, sc);
+            launch_pcg_dir<double>(ctx->st, n, ctx->d_dinv64, ctx->d_r2, ctx->d_p, sc);
+        }
+        launch_nm_update(ctx->st, n, ctx->d_x64, nm.w, nm.c_m, nm.dt, nm.gamma, ctx->d_u, nm.v, nm.a);
+        launch_nm_record(ctx->st, nw, nm.watch, ctx->d_u, nm.hist, nm.step, nm.its, sc);
+    };
+    bool graph = false;
+    if (ctx->use_graphs && n_steps > 1) {
+        // one time step captured once and replayed; every pointer and
+        // coefficient of the step is fixed for this setup
+        if (!nm.exec || nm.graph_iters != max_it_step || nm.graph_watch != nw) {
+            if (nm.exec) (void)hipGraphExecDestroy(nm.exec);
+            nm.exec = nullptr;
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
+            one_step();
+            hipError_t e1 = hipGetLastError();
+            hipError_t e2 = hipStreamEndCapture(ctx->st, &g);
+            if (e1 != hipSuccess || e2 != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return fail(ctx, ASEND_ERR_HIP, "capturing the Newmark step graph failed");
+            }
+            hipError_t e3 = hipGraphInstantiate(&nm.exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e3 != hipSuccess) {
+                nm.exec = nullptr;
+                return fail(ctx, ASEND_ERR_HIP, "instantiating the Newmark step graph failed");
+            }
+            nm.graph_iters = max_it_step;
+            nm.graph_watch = nw;
+        }
+        graph = true;
+    }
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->st));
+    for (int64_t s = 0; s < n_steps; ++s) {
+        if (graph) HIPCHK(hipGraphLaunch(nm.exec, ctx->st));
+        else one_step();
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->st));
+    HIPCHK(hipEventSynchronize(ctx->ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    std::vector<int64_t> its(std::max<int64_t>(n_steps, 1));
+    if (n_steps) HIPCHK(hipMemcpy(its.data(), nm.its, n_steps * 8, hipMemcpyDeviceToHost));
+    if (n_watch && n_steps) HIPCHK(hipMemcpy(hist, nm.hist, n_steps * n_watch * 8, hipMemcpyDeviceToHost));
+    if (u_out) HIPCHK(hipMemcpy(u_out, ctx->d_u, n * 8, hipMemcpyDeviceToHost));
+    if (v_out) HIPCHK(hipMemcpy(v_out, nm.v, n * 8, hipMemcpyDeviceToHost));
+    if (a_out) HIPCHK(hipMemcpy(a_out, nm.a, n * 8, hipMemcpyDeviceToHost));
+    if (stats) {
+        asend_newmark_stats st;
+        std::memset(&st, 0, sizeof(st));
+        st.steps = n_steps;
+        for (int64_t s = 0; s < n_steps; ++s) {
+            const int64_t it = its[s] >= 0 ? its[s] : -its[s] - 1;
+            st.total_iterations += it;
+            st.max_step_iterations = std::max(st.max_step_iterations, it);
+            if (its[s] < 0) st.unconverged_steps += 1;
+        }
+        st.graph_used = graph ? 1 : 0;
+        st.device_ms = ms;
+        *stats = st;
+    }
     return ASEND_OK;
 }
 

@@ -1,0 +1,97 @@
+// Newmark time stepping on the device (FP64 state and effective load).
+//
+// The reference advances every node with update_vel_acc / advance_disp
+// (S/node_meth.rs:127-199) around one linear solve per step in solve_step
+// (S/model_analysis.rs:312-558, loop in Model::solve :560-615).  In its
+// (beta - gamma) form, with c1 = 1/(dt^2 (beta - gamma)),
+// c2 = dt^2 (0.5 + beta - gamma):
+//   a(u) = c1 (u_n - u + dt v_n + c2 a_n)
+//   solve (K + c_m M) du = f(t_n) - K u_n - M a(u_n),  c_m = -c1
+//   u_{n+1} = u_n + du,  a_{n+1} = a(u_{n+1}),
+//   v_{n+1} = v_n + dt ((1 - gamma) a_n + gamma a_{n+1}).
+// With w = u_n + dt v_n + c2 a_n:  a(u_n) = c_m (u_n - w), a_{n+1} = c_m (u_{n+1} - w),
+// and since only K_eff = K + c_m M (penalty included) and M are stored,
+//   rhs = f(t_n) - K u_n - M a(u_n) = f(t_n) + c_m M w - K_eff u_n.
+// The load of the step that produces u_{n+1} is the one at t_n (the reference
+// calls solve_step(time) before advancing time).
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace asend {
+
+namespace {
+
+// w = u + dt v + c2 a
+__global__ void k_nm_predict(int64_t n, const double* __restrict__ u, const double* __restrict__ v,
+                             const double* __restrict__ a, double dt, double c2, double* __restrict__ w) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        w[i] = u[i] + dt * v[i] + c2 * a[i];
+}
+
+// rhs0 = fac[step] f_shape + c_m q  (q = M w); K_eff u_n is subtracted by the
+// residual SpMV that follows
+__global__ void k_nm_rhs(int64_t n, const int64_t* __restrict__ step, const double* __restrict__ fac,
+                         const double* __restrict__ f_shape, double c_m, const double* __restrict__ q,
+                         double* __restrict__ rhs) {
+    const double s = fac[*step];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        rhs[i] = s * f_shape[i] + c_m * q[i];
+}
+
+// u += du ; a_new = c_m (u - w) ; v += dt ((1 - gamma) a + gamma a_new) ; a = a_new
+__global__ void k_nm_update(int64_t n, const double* __restrict__ du, const double* __restrict__ w, double c_m,
+                            double dt, double gamma, double* __restrict__ u, double* __restrict__ v,
+                            double* __restrict__ a) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double un = u[i] + du[i];
+        const double an = c_m * (un - w[i]);
+        v[i] += dt * ((1.0 - gamma) * a[i] + gamma * an);
+        u[i] = un;
+        a[i] = an;
+    }
+}
+
+// history of the watched DOFs, per-step iteration count and convergence flag;
+// then advance the device step counter (one block: the increment follows
+// every read of the counter)
+__global__ __launch_bounds__(256) void k_nm_record(int n_watch, const int64_t* __restrict__ watch,
+                                                   const double* __restrict__ u, double* __restrict__ hist,
+                                                   int64_t* __restrict__ step, int64_t* __restrict__ its,
+                                                   const Scalars* __restrict__ sc) {
+    const int64_t s = *step;
+    for (int k = threadIdx.x; k < n_watch; k += blockDim.x) hist[s * n_watch + k] = u[watch[k]];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const bool conv = sc->done && sc->rr <= sc->tol2 * sc->ff;
+        its[s] = conv ? sc->it : -sc->it - 1;
+        *step = s + 1;
+    }
+}
+
+int nb(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 4096); }
+
+}  // namespace
+
+void launch_nm_predict(hipStream_t st, int64_t n, const double* u, const double* v, const double* a, double dt,
+                       double c2, double* w) {
+    hipLaunchKernelGGL(k_nm_predict, dim3(nb(n)), dim3(256), 0, st, n, u, v, a, dt, c2, w);
+}
+
+void launch_nm_rhs(hipStream_t st, int64_t n, const int64_t* step, const double* fac, const double* f_shape,
+                   double c_m, const double* q, double* rhs) {
+    hipLaunchKernelGGL(k_nm_rhs, dim3(nb(n)), dim3(256), 0, st, n, step, fac, f_shape, c_m, q, rhs);
+}
+
+void launch_nm_update(hipStream_t st, int64_t n, const double* du, const double* w, double c_m, double dt,
+                      double gamma, double* u, double* v, double* a) {
+    hipLaunchKernelGGL(k_nm_update, dim3(nb(n)), dim3(256), 0, st, n, du, w, c_m, dt, gamma, u, v, a);
+}
+
+void launch_nm_record(hipStream_t st, int n_watch, const int64_t* watch, const double* u, double* hist,
+                      int64_t* step, int64_t* its, const Scalars* sc) {
+    hipLaunchKernelGGL(k_nm_record, dim3(1), dim3(256), 0, st, n_watch, watch, u, hist, step, its, sc);
+}
+
+}  // namespace asend

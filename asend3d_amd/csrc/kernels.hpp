@@ -70,4 +70,15 @@ void launch_hex8_sens(hipStream_t st, int n_el, const int32_t* conn_r, const dou
 
 namespace asend {
 void launch_copy_f32(hipStream_t st, int64_t n4, const float* a, float* b);
+
+// Newmark step kernels (dynamics.hip)
+void launch_nm_predict(hipStream_t st, int64_t n, const double* u, const double* v, const double* a, double dt,
+                       double c2, double* w);
+void launch_nm_rhs(hipStream_t st, int64_t n, const int64_t* step, const double* fac, const double* f_shape,
+                   double c_m, const double* q, double* rhs);
+void launch_nm_update(hipStream_t st, int64_t n, const double* du, const double* w, double c_m, double dt,
+                      double gamma, double* u, double* v, double* a);
+void launch_nm_record(hipStream_t st, int n_watch, const int64_t* watch, const double* u, double* hist,
+                      int64_t* step, int64_t* its, const Scalars* sc);
+
 }  // namespace asend

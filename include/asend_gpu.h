@@ -205,6 +205,37 @@ int asend_bench_spmv(asend_ctx* ctx, int precision, int n_iter, double* avg_ms);
  * HBM traffic counters. */
 int asend_bench_copy(asend_ctx* ctx, int64_t bytes, int n_iter, double* avg_ms);
 
+/* ---- Newmark dynamics ---------------------------------------------------
+ * Replaces the dynamic branch of Model::solve (S/model_analysis.rs:560-615)
+ * with solve_step's elastic solve (:312-558) and the node kinematics
+ * update_vel_acc / advance_disp (S/node_meth.rs:127-199), in the reference's
+ * (beta - gamma) form: c1 = 1/(dt^2 (beta - gamma)), c2 = dt^2 (0.5 + beta -
+ * gamma), a(u) = c1 (u_n - u + dt v_n + c2 a_n); each step solves
+ * (K + c_m M) du = f(t_n) - K u_n - M a(u_n) with c_m = -c1 (the load of the
+ * step is taken at its start time, as solve_step(time) does), then
+ * u += du, a = a(u), v += dt ((1 - gamma) a_n + gamma a).  State, effective
+ * load and update in FP64; each step is one FP64 Jacobi PCG solve on the
+ * device, and the whole step is replayed as a captured hipGraph. */
+typedef struct asend_newmark_stats {
+    int64_t steps;
+    int64_t total_iterations;     /* PCG iterations over all steps            */
+    int64_t max_step_iterations;
+    int64_t unconverged_steps;    /* steps that hit max_it_step before tol    */
+    int graph_used;               /* 1 when the steps ran as graph replays    */
+    double device_ms;             /* HIP-event time of the step loop          */
+} asend_newmark_stats;
+/* Assemble M (consistent, rho per element) and K_eff = K + c_m M with the
+ * penalty of asend_set_penalty; requires gamma > beta.  brick8 meshes with rho. */
+int asend_newmark_setup(asend_ctx* ctx, double dt, double beta, double gamma);
+/* n_steps steps from (u0, v0, a0) (null = zero).  The load of step n is
+ * load_fac[n] * f_shape.  hist[n * n_watch + k] = u[watch[k]] after step n.
+ * Each step's solve stops at ||r|| <= tol ||rhs|| or max_it_step iterations.
+ * Final u, v, a into the out arrays when non-null. */
+int asend_newmark_run(asend_ctx* ctx, int64_t n_steps, const double* f_shape, const double* load_fac,
+                      const double* u0, const double* v0, const double* a0, int64_t n_watch, const int64_t* watch,
+                      double* hist, double* u_out, double* v_out, double* a_out, double tol, int max_it_step,
+                      asend_newmark_stats* stats);
+
 /* ---- multigrid diagnostics (no reference counterpart) --------------------
  * Host FP32 arrays.  Level 0 uses the fine DOF numbering, coarser levels the
  * lexicographic node order of their grid.  Used to verify on the device that

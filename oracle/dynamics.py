@@ -1,0 +1,74 @@
+"""CPU restatement of the reference's Newmark time stepping (numpy/scipy, FP64).
+
+TEST INFRASTRUCTURE ONLY: the checker of asend_newmark_run; only tests/ import
+it.  Follows, per step of Model::solve's dynamic loop
+(S/model_analysis.rs:560-615, S/ = solverSrc/asend_run_job/src/):
+
+  solve_step(time)            (S/model_analysis.rs:312-558, elastic branch)
+      rhs = f(time) - R(u) + penalty load, R(u) = K u + M a with the node
+      accelerations a of the current state; one linear solve with
+      K_eff = K + c_m M (+ sf C^T C), c_m = -c1; u += du;
+      update_vel_acc           (S/node_meth.rs:127-140)
+          c1 = 1/(dt^2 (beta - gamma)), c2 = dt^2 (0.5 + beta - gamma)
+          a = c1 (u_prev - u + dt v_prev + c2 a_prev)
+          v = v_prev + dt ((1 - gamma) a_prev + gamma a)
+  advance_disp + update_vel_acc (S/node_meth.rs:163-169) before the next step,
+  which makes the next residual use a(u_n) = c1 (dt v_n + c2 a_n).
+  time += dt after the step: the load of the step that produces u_{n+1} is the
+  one at t_n.
+
+first_step="reference" reproduces the reference literally: its first step
+uses the node acceleration as initialised (a0), because update_vel_acc has
+not run yet; first_step="predictor" uses a(u_0) like every later step (the
+device path).  From rest (u0 = v0 = a0 = 0) the two coincide.
+
+Pinning (tests/test_dynamics_oracle.py): single-DOF free vibration against
+the closed form u_n = cos(n phi), tan(phi/2) = omega dt / 2 of the average
+acceleration rule; energy conservation of a free brick8 mesh; the two
+first-step modes agree from rest.
+"""
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+
+def newmark(K, M, f_shape, load_fac, dt, beta=0.25, gamma=0.5, u0=None, v0=None, a0=None, pen_dofs=(),
+            sf=None, watch=(), first_step="predictor", on_step=None):
+    """Run len(load_fac) steps.  K, M: FP64 sparse (no penalty); pen_dofs: DOFs
+    clamped by a penalty sf (default 1e5 max|K + c_m M|, the reference scale
+    taken on the first dynamic assembly).  on_step(n, u, v, a, Kp) is called
+    after every step.  Returns (hist [n x len(watch)], u, v, a)."""
+    K = sp.csr_matrix(K)
+    M = sp.csr_matrix(M)
+    n = K.shape[0]
+    c1 = 1.0 / (dt * dt * (beta - gamma))
+    c2 = dt * dt * (0.5 + beta - gamma)
+    c_m = -c1
+    keff = (K + c_m * M).tocsr()
+    pen = np.zeros(n)
+    if len(pen_dofs):
+        if sf is None:
+            sf = 1e5 * abs(keff).max()
+        np.add.at(pen, np.asarray(pen_dofs, dtype=np.int64), sf)
+    Kp = (K + sp.diags(pen)).tocsr()
+    solve = spla.splu((keff + sp.diags(pen)).tocsc()).solve
+    u = np.zeros(n) if u0 is None else np.array(u0, dtype=np.float64)
+    v = np.zeros(n) if v0 is None else np.array(v0, dtype=np.float64)
+    a = np.zeros(n) if a0 is None else np.array(a0, dtype=np.float64)
+    watch = np.asarray(watch, dtype=np.int64)
+    hist = np.zeros((len(load_fac), watch.size))
+    for step, fac in enumerate(load_fac):
+        if step == 0 and first_step == "reference":
+            acc = a
+        else:
+            acc = c1 * (dt * v + c2 * a)
+        rhs = fac * f_shape - Kp @ u - M @ acc
+        du = solve(rhs)
+        un = u + du
+        an = c1 * (u - un + dt * v + c2 * a)
+        v = v + dt * ((1.0 - gamma) * a + gamma * an)
+        u, a = un, an
+        hist[step] = u[watch]
+        if on_step is not None:
+            on_step(step, u, v, a, Kp)
+    return hist, u, v, a

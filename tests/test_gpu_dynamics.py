@@ -1,0 +1,66 @@
+"""Device Newmark stepping against the CPU restatement of the reference's
+stepping (oracle/dynamics.py) on a brick8 cantilever under a half-sine tip
+pulse; graph replay and eager launches give identical histories."""
+import numpy as np
+import pytest
+
+import dynamics
+import oracle as O
+from asend3d_amd import api, problems
+from conftest import gpu_case
+
+pytestmark = pytest.mark.gpu
+
+RHO = 7850.0
+
+
+def cantilever_case(dims=(20, 4, 4)):
+    p = problems.cantilever(*dims)
+    K = O.assemble_solid(8, p.mesh.xyz, p.mesh.conn, p.E, p.nu, p.dof_index, p.n_dof)
+    M = O.assemble_solid_mass(8, p.mesh.xyz, p.mesh.conn, np.full(p.mesh.n_el, RHO), p.dof_index, K)
+    return p, K.to_scipy(), M.to_scipy()
+
+
+def tip_dofs(p):
+    di = np.asarray(p.dof_index).reshape(-1, 6)
+    tip = np.where(np.isclose(p.mesh.xyz[:, 0], p.mesh.xyz[:, 0].max()))[0]
+    return di[tip, 2]
+
+
+def test_newmark_matches_oracle(ctx):
+    gpu_case("newmark 20x4x4 vs oracle")
+    p, K, M = cantilever_case()
+    dt, n = 1e-4, 60
+    fac = np.where(np.arange(n) < 30, np.sin(np.pi * np.arange(n) / 30.0), 0.0)
+    watch = tip_dofs(p)
+    hist_o, u_o, v_o, a_o = dynamics.newmark(K, M, p.f, fac, dt, pen_dofs=p.pen_dofs, watch=watch)
+    problems.upload(ctx, p, rho=np.full(p.mesh.n_el, RHO))
+    ctx.newmark_setup(dt, 0.25, 0.5)
+    hist, u, v, a, st = ctx.newmark_run(n, p.f, fac, watch=watch, tol=1e-12, max_it_step=200)
+    assert st.unconverged_steps == 0 and st.steps == n, st.as_dict()
+    scale = np.abs(hist_o).max()
+    assert np.abs(hist - hist_o).max() <= 1e-8 * scale
+    for x, xo in ((u, u_o), (v, v_o), (a, a_o)):
+        assert np.abs(x - xo).max() <= 1e-7 * np.abs(xo).max()
+
+
+def test_newmark_graph_replay_matches_eager(ctx):
+    gpu_case("newmark graph vs eager")
+    p, _, _ = cantilever_case((12, 3, 3))
+    problems.upload(ctx, p, rho=np.full(p.mesh.n_el, RHO))
+    ctx.newmark_setup(5e-5, 0.25, 0.5)
+    n = 40
+    fac = np.sin(np.pi * np.arange(n) / n)
+    watch = tip_dofs(p)
+    runs = []
+    for on in (False, True, True):
+        ctx.set_graphs(on)
+        hist, u, v, a, st = ctx.newmark_run(n, p.f, fac, watch=watch, tol=1e-11, max_it_step=80)
+        assert st.graph_used == (1 if on else 0) and st.unconverged_steps == 0
+        runs.append((hist, u, v, a))
+        # eager work between replays: a static solve on the same context
+        ctx.static_compliance(p.f, ctx.opts(api.SOLVE_PCG_FP64, api.PC_JACOBI, tol=1e-8, max_it=100000))
+    ctx.set_graphs(True)
+    for r in runs[1:]:
+        for x, x0 in zip(r, runs[0]):
+            assert np.array_equal(x, x0)
