@@ -52,6 +52,15 @@ class NewmarkStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class BatchStats(ctypes.Structure):
+    _fields_ = [("outer_iterations", ctypes.c_int64), ("lockstep_iterations", ctypes.c_int64),
+                ("system_iterations", ctypes.c_int64), ("unconverged", ctypes.c_int64),
+                ("max_rel_residual", ctypes.c_double), ("device_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class SolveStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int64), ("outer_iterations", ctypes.c_int64),
                 ("rel_residual", ctypes.c_double), ("ref_residual", ctypes.c_double),
@@ -68,6 +77,9 @@ _lib.asend_block_dim_auto.restype = ctypes.c_int64
 _lib.asend_block_dim_auto.argtypes = [ctypes.c_int64]
 _lib.asend_learned_train.argtypes = [_vp, ctypes.c_int, ctypes.c_float, ctypes.c_uint64, ctypes.c_int, _vp]
 _lib.asend_learned_init.argtypes = [_vp, ctypes.c_uint64]
+_lib.asend_batch_error.restype = ctypes.c_char_p
+_lib.asend_batch_error.argtypes = [_vp]
+_lib.asend_batch_destroy.argtypes = [_vp]
 _lib.asend_penalty_scale.restype = ctypes.c_double
 _lib.asend_penalty_scale.argtypes = [_vp]
 _lib.asend_ctx_destroy.restype = None
@@ -82,7 +94,8 @@ EXPORTED = ["asend_ctx_create", "asend_ctx_destroy", "asend_last_error", "asend_
             "asend_learned_num_params", "asend_learned_init", "asend_learned_get_params",
             "asend_learned_set_params", "asend_learned_train", "asend_learned_eval", "asend_learned_apply",
             "asend_learned_edges", "asend_mg_set_smoother", "asend_learned_vcycle_eval", "asend_set_graphs",
-            "asend_newmark_setup", "asend_newmark_run"]
+            "asend_newmark_setup", "asend_newmark_run", "asend_batch_create", "asend_batch_destroy",
+            "asend_batch_error", "asend_batch_solve"]
 
 
 def lib():
@@ -406,3 +419,47 @@ class Context:
 def learned_num_params():
     return int(_lib.asend_learned_num_params())
 
+
+
+class Batch:
+    """Lockstep multi-RHS solves over assembled variant contexts
+    (asend_batch_create / asend_batch_solve)."""
+
+    def __init__(self, ctxs):
+        self.ctxs = list(ctxs)
+        arr = (ctypes.c_void_p * len(self.ctxs))(*[c.h for c in self.ctxs])
+        h = ctypes.c_void_p()
+        rc = _lib.asend_batch_create(arr, len(self.ctxs), ctypes.byref(h))
+        if rc != 0:
+            msg = _lib.asend_batch_error(h).decode() if h.value else "bad batch arguments"
+            if h.value:
+                _lib.asend_batch_destroy(h)
+            raise AsendError(f"asend_batch_create failed ({rc}): {msg}")
+        self.h = h
+
+    def solve(self, rhs, tol=1e-8, inner_tol=1e-3, max_it=100000, check_every=16, verbose=False):
+        """rhs: list (per variant) of (nrhs, n_dof) arrays; returns (list of
+        solutions, BatchStats)."""
+        nrhs = rhs[0].shape[0]
+        rs = [np.ascontiguousarray(r, dtype=np.float64) for r in rhs]
+        xs = [np.zeros_like(r) for r in rs]
+        rp = (ctypes.c_void_p * len(rs))(*[r.ctypes.data for r in rs])
+        xp = (ctypes.c_void_p * len(xs))(*[x.ctypes.data for x in xs])
+        st = BatchStats()
+        rc = _lib.asend_batch_solve(self.h, ctypes.c_int(nrhs), rp, xp, _dbl(tol), _dbl(inner_tol),
+                                    ctypes.c_int64(max_it), ctypes.c_int(check_every), ctypes.c_int(1 if verbose else 0),
+                                    ctypes.byref(st))
+        if rc != 0:
+            raise AsendError(f"asend_batch_solve failed ({rc}): {_lib.asend_batch_error(self.h).decode()}")
+        return xs, st
+
+    def close(self):
+        if self.h:
+            _lib.asend_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -20,6 +20,7 @@
 #include "kernels.hpp"
 #include "precond.hpp"
 #include "learned.hpp"
+#include "batched.hpp"
 
 using namespace asend;
 
@@ -112,6 +113,14 @@ int fail(asend_ctx* c, int code, const char* fmt, ...) {
     if (c) c->err = buf;
     return code;
 }
+
+#define HIPCHK_CTX(expr)                      \
+    do {                                      \
+        if ((expr) != hipSuccess) {           \
+            delete ab;                        \
+            return ASEND_ERR_HIP;             \
+        }                                     \
+    } while (0)
 
 #define HIPCHK(expr)                                                                                 \
     do {                                                                                             \
@@ -1027,6 +1036,94 @@ int asend_bench_spmv(asend_ctx* ctx, int precision, int n_iter, double* avg_ms) 
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     *avg_ms = ms / n_iter;
+    return ASEND_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched multi-RHS solves over variants
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+struct asend_batch {
+    Batch* b = nullptr;
+    std::vector<int64_t> n_dof;
+    std::string err;
+};
+
+extern "C" {
+
+int asend_batch_create(asend_ctx* const* variants, int n_variants, asend_batch** out) {
+    if (!variants || n_variants < 1 || !out) return ASEND_ERR_ARG;
+    *out = nullptr;
+    std::vector<OperatorView> ops(n_variants);
+    asend_batch* ab = new asend_batch();
+    for (int v = 0; v < n_variants; ++v) {
+        asend_ctx* c = variants[v];
+        if (!c || !c->assembled || c->etype != ASEND_BRICK8) {
+            delete ab;
+            return ASEND_ERR_STATE;
+        }
+        OperatorView& o = ops[v];
+        o.n_rows = (int)c->n_nodes;
+        o.n_slices = (int)c->n_slices;
+        o.n_slots = c->n_slots;
+        o.h_sptr = c->h_sptr.data();
+        o.scol = c->d_scol;
+        o.K32 = c->d_K32;
+        o.K64 = c->d_K64;
+        o.dinv32 = c->d_dinv32;
+        ab->n_dof.push_back(c->n_dof);
+        HIPCHK_CTX(hipStreamSynchronize(c->st));
+    }
+    int rc = batch_create(&ab->b, ops);
+    if (rc) {
+        ab->err = batch_error(ab->b);
+        batch_destroy(ab->b);
+        ab->b = nullptr;
+    }
+    *out = ab;
+    return rc;
+}
+
+void asend_batch_destroy(asend_batch* ab) {
+    if (!ab) return;
+    batch_destroy(ab->b);
+    delete ab;
+}
+
+const char* asend_batch_error(const asend_batch* ab) { return ab ? ab->err.c_str() : "null batch"; }
+
+int asend_batch_solve(asend_batch* ab, int nrhs, const double* const* rhs, double* const* x, double tol,
+                      double inner_tol, int64_t max_it, int check_every, int verbose, asend_batch_stats* stats) {
+    if (!ab || !ab->b || !rhs || !x || nrhs < 1 || nrhs > kNrhs) return ASEND_ERR_ARG;
+    Batch* b = ab->b;
+    const int nv = (int)ab->n_dof.size();
+    const int64_t ne = batch_rows_total(b) * 3 * kNrhs;
+    std::vector<double> F(ne, 0.0), U(ne, 0.0);
+    for (int v = 0; v < nv; ++v) {
+        const int64_t nd = ab->n_dof[v], off = batch_row_offset(b, v) * 3;
+        for (int j = 0; j < nrhs; ++j)
+            for (int64_t d = 0; d < nd; ++d) F[(off + d) * kNrhs + j] = rhs[v][j * nd + d];
+    }
+    BatchStats bs;
+    int rc = batch_solve(b, F.data(), U.data(), nrhs, tol, inner_tol, max_it, check_every, verbose, &bs);
+    if (rc) {
+        ab->err = batch_error(b);
+        return rc;
+    }
+    for (int v = 0; v < nv; ++v) {
+        const int64_t nd = ab->n_dof[v], off = batch_row_offset(b, v) * 3;
+        for (int j = 0; j < nrhs; ++j)
+            for (int64_t d = 0; d < nd; ++d) x[v][j * nd + d] = U[(off + d) * kNrhs + j];
+    }
+    if (stats) {
+        stats->outer_iterations = bs.outer_iterations;
+        stats->lockstep_iterations = bs.lockstep_iterations;
+        stats->system_iterations = bs.system_iterations;
+        stats->unconverged = bs.unconverged;
+        stats->max_rel_residual = bs.max_rel_residual;
+        stats->device_ms = bs.device_ms;
+    }
     return ASEND_OK;
 }
 

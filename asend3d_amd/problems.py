@@ -80,3 +80,47 @@ def config2():
 def upload(ctx, prob, rho=None):
     ctx.upload_mesh(prob.etype, prob.mesh.xyz, prob.dof_index, prob.n_dof, prob.mesh.conn, prob.E, prob.nu, rho)
     ctx.set_penalty(prob.pen_dofs, 0.0)
+
+
+@dataclass
+class BatchVariant:
+    """One design variant of a batch: the cantilever problem plus its load
+    cases and adjoint right-hand sides (rows of `rhs`, n_rhs x n_dof)."""
+    prob: StaticProblem
+    rhs: np.ndarray
+    n_load: int
+
+
+def batch_variants(n_variants, seed=7, min_el=10_000, max_el=200_000, n_load=8, n_adjoint=8, sigma_ln=1.0,
+                   progress=None):
+    """Cantilever variants with aspect 5:1:1 and element counts drawn uniformly
+    from [min_el, max_el] (nx = 5k, ny = nz = k), 5 mm cubes, lognormal moduli
+    (seeded per variant).  Right-hand sides: n_load random unit tip-face
+    tractions (1 MPa in a random direction) and n_adjoint adjoint right-hand
+    sides of tip-displacement objectives (unit loads on random tip DOFs; for a
+    compliance objective the adjoint right-hand side is the load case itself)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for v in range(n_variants):
+        target = rng.uniform(min_el, max_el)
+        k = max(2, int(round((target / 5.0) ** (1.0 / 3.0))))
+        nx, ny, nz = 5 * k, k, k
+        E = api.lognormal_moduli(seed * 1000 + v, nx * ny * nz, 70e3, sigma_ln)
+        p = cantilever(nx, ny, nz, 5.0 * nx, 5.0 * ny, 5.0 * nz, E=E, nu=0.3, traction=(0.0, 0.0, -1.0))
+        rows = []
+        for _ in range(n_load):
+            d = rng.standard_normal(3)
+            d /= np.linalg.norm(d)
+            rows.append(api.surface_traction(p.etype, p.mesh.xyz, p.mesh.conn, p.dof_index, p.n_dof,
+                                             p.tip_elements, tuple(d), (1.0, 0.0, 0.0), 5.0))
+        tip_nodes = p.mesh.nodes_at_x(nx)
+        di = np.asarray(p.dof_index).reshape(-1, 6)
+        for _ in range(n_adjoint):
+            nd = tip_nodes[rng.integers(len(tip_nodes))]
+            r = np.zeros(p.n_dof)
+            r[di[nd, rng.integers(3)]] = 1.0
+            rows.append(r)
+        out.append(BatchVariant(p, np.array(rows), n_load))
+        if progress is not None:
+            progress(v, p)
+    return out

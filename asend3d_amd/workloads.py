@@ -1,0 +1,119 @@
+"""Measured workloads besides the headline static solve (bench.py):
+
+  batched  -- 64 brick8 cantilever variants (10k-200k elements, aspect
+              5:1:1, lognormal moduli), 8 load cases + 8 adjoint right-hand
+              sides each: lockstep multi-RHS PCG over the whole batch against
+              solving one system at a time with the same mixed-precision
+              Jacobi PCG.
+  newmark  -- Newmark time stepping of a cantilever with a half-sine tip
+              pulse, each step replayed as a hipGraph.
+
+    python -m asend3d_amd.workloads batched [--variants 64] [--sample 2]
+    python -m asend3d_amd.workloads newmark [--nx 100 --ny 20 --nz 20 --steps 1000]
+
+Each prints one JSON line.
+"""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import api, problems
+
+
+def log(msg):
+    print(f"[workloads] {msg}", file=sys.stderr, flush=True)
+
+
+def batched(n_variants=64, sample=2, seed=7, tol=1e-8):
+    t0 = time.perf_counter()
+    vs = problems.batch_variants(n_variants, seed=seed,
+                                 progress=lambda v, p: (v % 16 == 15) and log(f"prepared {v + 1} variants"))
+    n_el = [v.prob.mesh.n_el for v in vs]
+    log(f"{n_variants} variants, {sum(n_el)} elements ({min(n_el)}..{max(n_el)}), "
+        f"{sum(v.prob.n_dof for v in vs)} DOF, host prep {time.perf_counter() - t0:.1f} s")
+    ctxs = []
+    for k, v in enumerate(vs):
+        c = api.Context(0)
+        problems.upload(c, v.prob)
+        c.assemble()
+        ctxs.append(c)
+        if k % 16 == 15:
+            log(f"assembled {k + 1} variants")
+    b = api.Batch(ctxs)
+    rhs = [v.rhs for v in vs]
+    xs, st = b.solve(rhs, tol=tol, inner_tol=1e-3, check_every=16, verbose=True)
+    n_sys = sum(r.shape[0] for r in rhs)
+    log(f"batched: {st.device_ms / 1e3:.3f} s for {n_sys} systems, {st.lockstep_iterations} lockstep iterations, "
+        f"max rel residual {st.max_rel_residual:.2e}")
+    # one system at a time, same algorithm (FP32 Jacobi PCG inside FP64 refinement), on a sample
+    o = api.Context.opts(api.SOLVE_IR_MIXED, api.PC_JACOBI, tol=tol, inner_tol=1e-3, max_it=2000000, max_outer=60,
+                         check_every=16)
+    single_ms, n_single, worst = 0.0, 0, 0.0
+    for k, (c, v, x) in enumerate(zip(ctxs, vs, xs)):
+        for j in range(min(sample, v.rhs.shape[0])):
+            u, s = c.solve(v.rhs[j], o)
+            single_ms += s.device_ms
+            n_single += 1
+            worst = max(worst, float(np.abs(x[j] - u).max() / max(np.abs(u).max(), 1e-300)))
+        log(f"single solves: variant {k + 1}/{len(vs)}, {single_ms / 1e3:.1f} s so far")
+    per_single = single_ms / n_single
+    est_single_s = per_single * n_sys / 1e3
+    b.close()
+    for c in ctxs:
+        c.close()
+    return {"workload": "batched", "variants": n_variants, "systems": n_sys, "elements_total": int(sum(n_el)),
+            "dof_total": int(sum(v.prob.n_dof for v in vs)), "batched_s": st.device_ms / 1e3,
+            "lockstep_iterations": int(st.lockstep_iterations), "outer_iterations": int(st.outer_iterations),
+            "max_rel_residual": st.max_rel_residual, "one_at_a_time_s_estimate": est_single_s,
+            "one_at_a_time_sample": f"{n_single} systems ({sample} per variant), {single_ms / 1e3:.2f} s, scaled "
+                                    f"by {n_sys}/{n_single}",
+            "speedup": est_single_s / (st.device_ms / 1e3), "max_rel_diff_vs_single": worst,
+            "tol": tol}
+
+
+def newmark(nx=100, ny=20, nz=20, steps=1000, dt=1e-5, graphs=True, tol=1e-8, max_it=400):
+    from .problems import cantilever
+    p = cantilever(nx, ny, nz, lx=1.0, ly=0.2, lz=0.2, E=70e9, nu=0.33, traction=(0.0, 0.0, -1.0e6))
+    ctx = api.Context(0)
+    problems.upload(ctx, p, rho=np.full(p.mesh.n_el, 2700.0))
+    ctx.set_graphs(graphs)
+    ctx.newmark_setup(dt, 0.25, 0.5)
+    n_pulse = int(round(1e-3 / dt))
+    fac = np.where(np.arange(steps) < n_pulse, np.sin(np.pi * np.arange(steps) / n_pulse), 0.0)
+    di = np.asarray(p.dof_index).reshape(-1, 6)
+    tip = p.mesh.nodes_at_x(nx)
+    watch = di[tip, 2]
+    ctx.newmark_run(min(steps, 20), p.f, fac, watch=watch, tol=tol, max_it_step=max_it)  # warm-up
+    hist, u, v, a, st = ctx.newmark_run(steps, p.f, fac, watch=watch, tol=tol, max_it_step=max_it)
+    ctx.close()
+    return {"workload": "newmark", "mesh": f"{nx}x{ny}x{nz} brick8", "dof": int(p.n_dof), "steps": steps,
+            "dt": dt, "device_s": st.device_ms / 1e3, "us_per_step": st.device_ms * 1e3 / steps,
+            "dof_steps_per_s": p.n_dof * steps / (st.device_ms / 1e3), "pcg_iterations": int(st.total_iterations),
+            "max_step_iterations": int(st.max_step_iterations), "unconverged_steps": int(st.unconverged_steps),
+            "graph": bool(st.graph_used), "step_tol": tol, "max_it_step": max_it,
+            "tip_w_min": float(hist.mean(axis=1).min())}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("which", choices=["batched", "newmark"])
+    ap.add_argument("--variants", type=int, default=64)
+    ap.add_argument("--sample", type=int, default=2)
+    ap.add_argument("--nx", type=int, default=100)
+    ap.add_argument("--ny", type=int, default=20)
+    ap.add_argument("--nz", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--no-graphs", action="store_true")
+    a = ap.parse_args()
+    if a.which == "batched":
+        out = batched(a.variants, a.sample)
+    else:
+        out = newmark(a.nx, a.ny, a.nz, a.steps, graphs=not a.no_graphs)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

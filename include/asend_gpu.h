@@ -205,6 +205,32 @@ int asend_bench_spmv(asend_ctx* ctx, int precision, int n_iter, double* avg_ms);
  * HBM traffic counters. */
 int asend_bench_copy(asend_ctx* ctx, int64_t bytes, int n_iter, double* avg_ms);
 
+/* ---- batched multi-RHS solves over design variants ----------------------
+ * No reference counterpart (the reference solves one model and one load at a
+ * time, S/model_analysis.rs:560-673).  Each variant is an assembled brick8
+ * context; their operators are laid side by side as one block-diagonal
+ * sliced-ELL operator and up to 16 right-hand sides per variant advance in
+ * lockstep: FP32 Jacobi PCG (one SpMM launch per iteration for the whole
+ * batch, per-system coefficients) inside FP64 iterative refinement. */
+typedef struct asend_batch asend_batch;
+typedef struct asend_batch_stats {
+    int64_t outer_iterations;
+    int64_t lockstep_iterations;  /* inner iterations of the slowest system, summed over refinements */
+    int64_t system_iterations;    /* inner iterations summed over all systems */
+    int64_t unconverged;          /* systems above tol after the last refinement */
+    double max_rel_residual;      /* max over systems of ||f - K u|| / ||f|| (FP64) */
+    double device_ms;
+} asend_batch_stats;
+/* The contexts must stay alive and assembled while the batch exists (their
+ * operators are copied; their device arrays are not borrowed afterwards). */
+int asend_batch_create(asend_ctx* const* variants, int n_variants, asend_batch** out);
+void asend_batch_destroy(asend_batch* b);
+const char* asend_batch_error(const asend_batch* b);
+/* rhs[v] and x[v]: nrhs x n_dof_v, right-hand side major (rhs[v][j * n_dof_v + d]).
+ * verbose != 0 prints one progress line per refinement to stderr. */
+int asend_batch_solve(asend_batch* b, int nrhs, const double* const* rhs, double* const* x, double tol,
+                      double inner_tol, int64_t max_it, int check_every, int verbose, asend_batch_stats* stats);
+
 /* ---- Newmark dynamics ---------------------------------------------------
  * Replaces the dynamic branch of Model::solve (S/model_analysis.rs:560-615)
  * with solve_step's elastic solve (:312-558) and the node kinematics
