@@ -61,6 +61,23 @@ class BatchStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+TUNE_SPMV = 1
+TUNE_SOLVER = 2
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("spmv_waves_per_block", ctypes.c_int), ("spmv_ms_fp32", ctypes.c_double * 4),
+                ("spmv_ms_fp64", ctypes.c_double * 4), ("cheb_degree", ctypes.c_int), ("inner_tol", ctypes.c_double),
+                ("check_every", ctypes.c_int), ("solve_ms", ctypes.c_double), ("n_trials", ctypes.c_int)]
+
+    def as_dict(self):
+        d = {}
+        for k, _ in self._fields_:
+            v = getattr(self, k)
+            d[k] = list(v) if hasattr(v, "__len__") else v
+        return d
+
+
 class SolveStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int64), ("outer_iterations", ctypes.c_int64),
                 ("rel_residual", ctypes.c_double), ("ref_residual", ctypes.c_double),
@@ -95,7 +112,7 @@ EXPORTED = ["asend_ctx_create", "asend_ctx_destroy", "asend_last_error", "asend_
             "asend_learned_set_params", "asend_learned_train", "asend_learned_eval", "asend_learned_apply",
             "asend_learned_edges", "asend_mg_set_smoother", "asend_learned_vcycle_eval", "asend_set_graphs",
             "asend_newmark_setup", "asend_newmark_run", "asend_batch_create", "asend_batch_destroy",
-            "asend_batch_error", "asend_batch_solve"]
+            "asend_batch_error", "asend_batch_solve", "asend_autotune"]
 
 
 def lib():
@@ -352,6 +369,12 @@ class Context:
                                          _p(u), _p(v), _p(a), _dbl(tol), ctypes.c_int(max_it_step),
                                          ctypes.byref(st)), "asend_newmark_run")
         return hist[:, :watch.size], u, v, a, st
+
+    def autotune(self, rhs=None, what=TUNE_SPMV | TUNE_SOLVER):
+        t = Tuning()
+        r = None if rhs is None else _f64(rhs)
+        self._chk(_lib.asend_autotune(self.h, _p(r), ctypes.c_int(what), ctypes.byref(t)), "asend_autotune")
+        return t
 
     def set_graphs(self, on=True):
         self._chk(_lib.asend_set_graphs(self.h, 1 if on else 0), "asend_set_graphs")

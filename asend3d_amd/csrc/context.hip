@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -45,6 +46,7 @@ struct asend_ctx {
         A.n_slices = (int)n_slices;
         A.sptr = d_sptr;
         A.scol = d_scol;
+        A.waves_per_block = spmv_wpb;
         return A;
     }
     int64_t nval() const { return 9 * 64 * n_slots; }
@@ -68,6 +70,7 @@ struct asend_ctx {
            *d_sens = nullptr, *d_x64 = nullptr, *d_r2 = nullptr;
     float *d_r32 = nullptr, *d_x32 = nullptr, *d_p32 = nullptr, *d_q32 = nullptr, *d_w32 = nullptr;
 
+    int64_t partials_cap = kMaxPartials;  // blocks whose partial sums d_partials can hold
     Scalars* d_sc = nullptr;  // [0] outer/FP64 loop, [1] inner FP32 loop, [2] misc
     Scalars* h_sc = nullptr;  // pinned mirror
     double* d_partials = nullptr;
@@ -86,6 +89,11 @@ struct asend_ctx {
     int64_t asm_count = 0, factor_asm = -1;
     double asm_cmass = 0.0;  // mass coefficient of the current assembly
     bool use_graphs = true;
+    // autotuned choices: SpMV workgroup geometry, inner tolerance and check
+    // interval used when a solve passes 0 for them
+    int spmv_wpb = 4;
+    double tuned_inner_tol = 0.0;
+    int tuned_check = 0;
 
     // Newmark time stepping: mass operator, state and per-run buffers
     struct Newmark {
@@ -97,7 +105,7 @@ struct asend_ctx {
         double *fac = nullptr, *hist = nullptr;
         int64_t *step = nullptr, *its = nullptr, *watch = nullptr;
         int64_t fac_cap = 0, hist_cap = 0, watch_cap = 0;
-        hipGraphExec_t exec = nullptr;
+        hipGraphExec_t exec_head = nullptr, exec_more = nullptr, exec_tail = nullptr;
         int graph_iters = -1, graph_watch = -1;
     } nm;
 };
@@ -177,8 +185,10 @@ void newmark_free(asend_ctx* ctx) {
         dfree(*q);
     for (int64_t** q : {&nm.step, &nm.its, &nm.watch}) dfree(*q);
     nm.fac_cap = nm.hist_cap = nm.watch_cap = 0;
-    if (nm.exec) (void)hipGraphExecDestroy(nm.exec);
-    nm.exec = nullptr;
+    for (hipGraphExec_t* e : {&nm.exec_head, &nm.exec_more, &nm.exec_tail}) {
+        if (*e) (void)hipGraphExecDestroy(*e);
+        *e = nullptr;
+    }
     nm.graph_iters = nm.graph_watch = -1;
     nm.ready = false;
 }
@@ -297,6 +307,7 @@ int residual64(asend_ctx* ctx, const double* f, const double* u, double* r) {
 // solve that needs more has failed to cut iterations and the solve falls back
 // to Jacobi PCG.
 constexpr int64_t kLearnedInnerBudget = 300;
+constexpr int kNewmarkBlock = 24;  // PCG iterations per captured block of a Newmark step
 constexpr int64_t kLearnedPcgBudget = 20000;
 
 LearnedGraph learned_graph(const asend_ctx* ctx) {
@@ -344,7 +355,7 @@ int ensure_factor(asend_ctx* ctx, bool smoother) {
 int solve_device(asend_ctx* ctx, const asend_solve_opts* o, asend_solve_stats* st) {
     const int64_t n = ctx->n_dof;
     int64_t max_it = o->max_it > 0 ? o->max_it : n;
-    int check = o->check_every > 0 ? o->check_every : 16;
+    int check = o->check_every > 0 ? o->check_every : (ctx->tuned_check > 0 ? ctx->tuned_check : 16);
     int64_t max_outer = o->max_outer > 0 ? o->max_outer : 20;
     int rc;
     asend_solve_stats s;
@@ -406,7 +417,7 @@ int solve_device(asend_ctx* ctx, const asend_solve_opts* o, asend_solve_stats* s
     if (o->method != ASEND_SOLVE_IR_MIXED) return fail(ctx, ASEND_ERR_ARG, "unknown solve method %d", o->method);
     if (o->stop_rule != ASEND_STOP_REL_RESIDUAL)
         return fail(ctx, ASEND_ERR_UNSUPPORTED, "mixed solver supports the relative-residual rule only");
-    double inner_tol = o->inner_tol > 0 ? o->inner_tol : 1e-4;
+    double inner_tol = o->inner_tol > 0 ? o->inner_tol : (ctx->tuned_inner_tol > 0 ? ctx->tuned_inner_tol : 1e-4);
     int64_t outer = 0;
     for (; outer < max_outer; ++outer) {
         if (std::sqrt(rr / ff) <= o->tol) break;
@@ -707,6 +718,12 @@ int asend_upload_mesh(asend_ctx* ctx, int elem_type, int64_t n_nodes, const doub
         ctx->h_sptr[sl + 1] = ctx->h_sptr[sl] + w;
     }
     ctx->n_slots = ctx->h_sptr[ctx->n_slices];
+    // the SpMV with one slice per workgroup writes one partial per slice
+    if (ctx->n_slices > ctx->partials_cap) {
+        int rc0 = dalloc(ctx, &ctx->d_partials, 2 * (size_t)ctx->n_slices);
+        if (rc0) return rc0;
+        ctx->partials_cap = ctx->n_slices;
+    }
     std::vector<int32_t> scol(64 * ctx->n_slots, 0);
     for (int64_t sl = 0; sl < ctx->n_slices; ++sl)
         for (int lane = 0; lane < 64; ++lane) {
@@ -1205,10 +1222,7 @@ int asend_newmark_run(asend_ctx* ctx, int64_t n_steps, const double* f_shape, co
         nm.watch_cap = std::max<int64_t>(n_watch, 1);
         realloc = true;
     }
-    if (realloc && nm.exec) {
-        (void)hipGraphExecDestroy(nm.exec);
-        nm.exec = nullptr;
-    }
+    if (realloc) nm.graph_iters = -1;  // recapture: the step graphs hold the old buffers
     HIPCHK(hipMemcpy(nm.fshape, f_shape, n * 8, hipMemcpyHostToDevice));
     if (n_steps) HIPCHK(hipMemcpy(nm.fac, load_fac, n_steps * 8, hipMemcpyHostToDevice));
     if (n_watch) HIPCHK(hipMemcpy(nm.watch, watch, n_watch * 8, hipMemcpyHostToDevice));
@@ -1231,54 +1245,73 @@ int asend_newmark_run(asend_ctx* ctx, int64_t n_steps, const double* f_shape, co
     }
     Scalars* sc = ctx->d_sc + 1;
     const int nw = (int)n_watch;
-    auto one_step = [&]() {
-        launch_nm_predict(ctx->st, n, ctx->d_u, nm.v, nm.a, nm.dt, nm.c2, nm.w);
-        launch_sell3<double>(ctx->st, 0, ctx->fine(), nm.M64, nm.w, nm.q, nullptr, nullptr, nullptr);
-        launch_nm_rhs(ctx->st, n, nm.step, nm.fac, nm.fshape, nm.c_m, nm.q, nm.rhs0);
-        launch_sell3<double>(ctx->st, 3, ctx->fine(), ctx->d_K64, ctx->d_u, nm.rhs, nm.rhs0, nullptr, nullptr);
-        launch_pcg_init<double>(ctx->st, n, nm.rhs, ctx->d_dinv64, ctx->d_x64, ctx->d_r2, ctx->d_p,
-                                ctx->d_partials, sc);
-        for (int k = 0; k < max_it_step; ++k) {
+    // A step is three launch sequences: head (effective load, PCG start and a
+    // first block of iterations), more (another block, repeated until the
+    // step's solve has converged: one small device-to-host read per block),
+    // tail (state update and history).  Each is captured once as a hipGraph.
+    const int blk = std::min(max_it_step, kNewmarkBlock);
+    auto iterations = [&]() {
+        for (int k = 0; k < blk; ++k) {
             launch_sell3<double>(ctx->st, 1, ctx->fine(), ctx->d_K64, ctx->d_p, ctx->d_q, nullptr, ctx->d_partials,
                                  sc);
             launch_pcg_update<double>(ctx->st, n, ctx->d_dinv64, ctx->d_x64, ctx->d_r2, ctx->d_p, ctx->d_q,
                                       ctx->d_partials, sc);
             launch_pcg_dir<double>(ctx->st, n, ctx->d_dinv64, ctx->d_r2, ctx->d_p, sc);
         }
+    };
+    auto head = [&]() {
+        launch_nm_predict(ctx->st, n, ctx->d_u, nm.v, nm.a, nm.dt, nm.c2, nm.w);
+        launch_sell3<double>(ctx->st, 0, ctx->fine(), nm.M64, nm.w, nm.q, nullptr, nullptr, nullptr);
+        launch_nm_rhs(ctx->st, n, nm.step, nm.fac, nm.fshape, nm.c_m, nm.q, nm.rhs0);
+        launch_sell3<double>(ctx->st, 3, ctx->fine(), ctx->d_K64, ctx->d_u, nm.rhs, nm.rhs0, nullptr, nullptr);
+        launch_pcg_init<double>(ctx->st, n, nm.rhs, ctx->d_dinv64, ctx->d_x64, ctx->d_r2, ctx->d_p,
+                                ctx->d_partials, sc);
+        iterations();
+    };
+    auto tail = [&]() {
         launch_nm_update(ctx->st, n, ctx->d_x64, nm.w, nm.c_m, nm.dt, nm.gamma, ctx->d_u, nm.v, nm.a);
         launch_nm_record(ctx->st, nw, nm.watch, ctx->d_u, nm.hist, nm.step, nm.its, sc);
     };
-    bool graph = false;
-    if (ctx->use_graphs && n_steps > 1) {
-        // one time step captured once and replayed; every pointer and
-        // coefficient of the step is fixed for this setup
-        if (!nm.exec || nm.graph_iters != max_it_step || nm.graph_watch != nw) {
-            if (nm.exec) (void)hipGraphExecDestroy(nm.exec);
-            nm.exec = nullptr;
-            hipGraph_t g = nullptr;
-            HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
-            one_step();
-            hipError_t e1 = hipGetLastError();
-            hipError_t e2 = hipStreamEndCapture(ctx->st, &g);
-            if (e1 != hipSuccess || e2 != hipSuccess) {
-                if (g) (void)hipGraphDestroy(g);
-                return fail(ctx, ASEND_ERR_HIP, "capturing the Newmark step graph failed");
-            }
-            hipError_t e3 = hipGraphInstantiate(&nm.exec, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (e3 != hipSuccess) {
-                nm.exec = nullptr;
-                return fail(ctx, ASEND_ERR_HIP, "instantiating the Newmark step graph failed");
-            }
-            nm.graph_iters = max_it_step;
-            nm.graph_watch = nw;
+    auto capture = [&](hipGraphExec_t* exec, const std::function<void()>& seq) -> int {
+        if (*exec) (void)hipGraphExecDestroy(*exec);
+        *exec = nullptr;
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
+        seq();
+        hipError_t e1 = hipGetLastError();
+        hipError_t e2 = hipStreamEndCapture(ctx->st, &g);
+        if (e1 != hipSuccess || e2 != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return fail(ctx, ASEND_ERR_HIP, "capturing a Newmark step graph failed");
         }
-        graph = true;
+        hipError_t e3 = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e3 != hipSuccess) {
+            *exec = nullptr;
+            return fail(ctx, ASEND_ERR_HIP, "instantiating a Newmark step graph failed");
+        }
+        return ASEND_OK;
+    };
+    const bool graph = ctx->use_graphs;
+    if (graph && (!nm.exec_head || nm.graph_iters != blk || nm.graph_watch != nw)) {
+        if ((rc = capture(&nm.exec_head, head)) || (rc = capture(&nm.exec_more, iterations)) ||
+            (rc = capture(&nm.exec_tail, tail)))
+            return rc;
+        nm.graph_iters = blk;
+        nm.graph_watch = nw;
     }
     HIPCHK(hipEventRecord(ctx->ev0, ctx->st));
     for (int64_t s = 0; s < n_steps; ++s) {
-        if (graph) HIPCHK(hipGraphLaunch(nm.exec, ctx->st));
-        else one_step();
+        if (graph) HIPCHK(hipGraphLaunch(nm.exec_head, ctx->st));
+        else head();
+        for (;;) {
+            if ((rc = read_scalars(ctx, 1))) return rc;
+            if (ctx->h_sc[1].done) break;
+            if (graph) HIPCHK(hipGraphLaunch(nm.exec_more, ctx->st));
+            else iterations();
+        }
+        if (graph) HIPCHK(hipGraphLaunch(nm.exec_tail, ctx->st));
+        else tail();
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ctx->ev1, ctx->st));
@@ -1531,6 +1564,93 @@ int asend_mg_prolong(asend_ctx* ctx, int level, const float* coarse, float* fine
     return mg_io(ctx, 2, level, coarse, fine);
 }
 int asend_mg_vcycle(asend_ctx* ctx, const float* b, float* x) { return mg_io(ctx, 3, 0, b, x); }
+
+// ---------------------------------------------------------------------------
+// Autotuner
+// ---------------------------------------------------------------------------
+int asend_autotune(asend_ctx* ctx, const double* rhs, int what, asend_tuning* out) {
+    if (!ctx || !out) return ASEND_ERR_ARG;
+    if (!ctx->assembled) return fail(ctx, ASEND_ERR_STATE, "assemble first");
+    std::memset(out, 0, sizeof(*out));
+    int rc;
+    if (what & ASEND_TUNE_SPMV) {
+        // SpMV workgroup geometry: 1, 2, 4 or 8 slices per workgroup, timed
+        // with HIP events on the context stream (FP32 decides, FP64 reported)
+        double best = 1e300;
+        int best_w = ctx->spmv_wpb;
+        for (int k = 0; k < 4; ++k) {
+            const int w = 1 << k;
+            if ((ctx->n_slices + w - 1) / w > ctx->partials_cap) continue;
+            ctx->spmv_wpb = w;
+            double ms32 = 0, ms64 = 0;
+            if ((rc = asend_bench_spmv(ctx, 32, 50, &ms32)) || (rc = asend_bench_spmv(ctx, 64, 20, &ms64))) return rc;
+            out->spmv_ms_fp32[k] = ms32;
+            out->spmv_ms_fp64[k] = ms64;
+            out->n_trials += 1;
+            if (ms32 < best) {
+                best = ms32;
+                best_w = w;
+            }
+        }
+        ctx->spmv_wpb = best_w;
+        if (ctx->mg) mg_set_fine_geometry(ctx->mg, best_w);
+    }
+    out->spmv_waves_per_block = ctx->spmv_wpb;
+    if (what & ASEND_TUNE_SOLVER) {
+        // multigrid-preconditioned mixed solve: Chebyshev degree, inner
+        // tolerance and convergence-check interval, timed on full solves of rhs
+        if (!rhs) return fail(ctx, ASEND_ERR_ARG, "solver tuning needs a right-hand side");
+        if (!ctx->mg) return fail(ctx, ASEND_ERR_STATE, "solver tuning needs the multigrid hierarchy");
+        HIPCHK(hipMemcpy(ctx->d_f, rhs, ctx->n_dof * 8, hipMemcpyHostToDevice));
+        const int degrees[2] = {2, 3};
+        const double tols[3] = {1e-2, 1e-3, 1e-4};
+        const int checks[3] = {2, 4, 8};
+        double best = 1e300;
+        int bd = 2, bc = 4;
+        double bt = 1e-3;
+        for (int d : degrees) {
+            if ((rc = mg_set_smoother(ctx->mg, 1, d, 0.1, 1.15)) || (rc = mg_refresh_coefficients(ctx->mg, ctx->st)))
+                return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
+            if (!mg_ready(ctx->mg) && (rc = assemble_device(ctx, ctx->asm_cmass))) return rc;
+            for (double t : tols)
+                for (int c : checks) {
+                    asend_solve_opts o;
+                    std::memset(&o, 0, sizeof(o));
+                    o.method = ASEND_SOLVE_IR_MIXED;
+                    o.precond = ASEND_PC_MG;
+                    o.stop_rule = ASEND_STOP_REL_RESIDUAL;
+                    o.tol = 1e-8;
+                    o.inner_tol = t;
+                    o.check_every = c;
+                    o.max_it = 100000;
+                    o.max_outer = 60;
+                    asend_solve_stats st;
+                    HIPCHK(hipEventRecord(ctx->ev0, ctx->st));
+                    if ((rc = solve_device(ctx, &o, &st))) return rc;
+                    HIPCHK(hipEventRecord(ctx->ev1, ctx->st));
+                    HIPCHK(hipEventSynchronize(ctx->ev1));
+                    float ms = 0;
+                    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+                    out->n_trials += 1;
+                    if (st.converged && ms < best) {
+                        best = ms;
+                        bd = d;
+                        bt = t;
+                        bc = c;
+                    }
+                }
+        }
+        if ((rc = mg_set_smoother(ctx->mg, 1, bd, 0.1, 1.15)) || (rc = mg_refresh_coefficients(ctx->mg, ctx->st)))
+            return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
+        ctx->tuned_inner_tol = bt;
+        ctx->tuned_check = bc;
+        out->solve_ms = best;
+    }
+    if (ctx->mg) out->cheb_degree = mg_smoother_degree(ctx->mg);
+    out->inner_tol = ctx->tuned_inner_tol;
+    out->check_every = ctx->tuned_check;
+    return ASEND_OK;
+}
 
 int asend_set_graphs(asend_ctx* ctx, int on) {
     if (!ctx) return ASEND_ERR_ARG;

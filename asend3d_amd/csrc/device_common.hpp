@@ -10,6 +10,7 @@ namespace asend {
 constexpr int kWave = 64;
 constexpr int kBlock = 256;  // threads per block for streaming kernels
 constexpr int kNumXcd = 8;
+constexpr int kMaxBlockWaves = 16;  // block reductions accept up to 1024 threads
 
 // Logical block index such that each XCD (blocks are dispatched round-robin
 // over the 8 XCDs) processes one contiguous range of logical blocks, so rows
@@ -43,7 +44,7 @@ __device__ __forceinline__ T wave_max(T v) {
 // Sum of NV values over the whole block; result valid in thread 0.
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV]) {
-    __shared__ double sh[NV][kBlock / kWave];
+    __shared__ double sh[NV][kMaxBlockWaves];
     int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
 #pragma unroll
     for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
@@ -75,7 +76,7 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], double* partials
 
 // Block-level maximum of a non-negative value, same scheme.
 __device__ __forceinline__ void block_partial_max(double v, double* partials) {
-    __shared__ double sh[kBlock / kWave];
+    __shared__ double sh[kMaxBlockWaves];
     v = wave_max(v);
     if (threadIdx.x % kWave == 0) sh[threadIdx.x / kWave] = v;
     __syncthreads();

@@ -269,13 +269,14 @@ void launch_diag_inv(hipStream_t st, int n_rows, const int32_t* sptr, const int3
 // and no cross-lane reduction is needed.  Modes as the block-row SpMV.
 // ---------------------------------------------------------------------------
 template <typename T, int MODE>
-__global__ __launch_bounds__(kBlock) void k_sell3(int n_rows, int n_slices, const int32_t* __restrict__ sptr,
+__global__ __launch_bounds__(kSellMaxWaves * kWave) void k_sell3(int n_rows, int n_slices,
+                                                                 const int32_t* __restrict__ sptr,
                                                   const int32_t* __restrict__ scol, const T* __restrict__ val,
                                                   const T* __restrict__ x, T* __restrict__ y,
                                                   const T* __restrict__ f, double* partials, const Scalars* sc) {
     if (MODE == 1 && sc->done) return;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
-    const int slice = lb * (kBlock / kWave) + threadIdx.x / kWave;
+    const int slice = lb * (int)(blockDim.x / kWave) + threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
     const int r = slice * kWave + lane;
     double part = 0.0;
@@ -319,25 +320,26 @@ __global__ __launch_bounds__(kBlock) void k_sell3(int n_rows, int n_slices, cons
 template <typename T>
 void launch_sell3(hipStream_t st, int mode, const Sell3& A, const T* val, const T* x, T* y, const T* f,
                   double* partials, Scalars* sc) {
-    const int per = kBlock / kWave;
+    const int per = A.waves_per_block;
     const int nblk = (A.n_slices + per - 1) / per;
+    const int nthr = per * kWave;
     switch (mode) {
         case 0:
-            hipLaunchKernelGGL((k_sell3<T, 0>), dim3(nblk), dim3(kBlock), 0, st, A.n_rows, A.n_slices, A.sptr, A.scol,
+            hipLaunchKernelGGL((k_sell3<T, 0>), dim3(nblk), dim3(nthr), 0, st, A.n_rows, A.n_slices, A.sptr, A.scol,
                                val, x, y, f, partials, sc);
             break;
         case 1:
-            hipLaunchKernelGGL((k_sell3<T, 1>), dim3(nblk), dim3(kBlock), 0, st, A.n_rows, A.n_slices, A.sptr, A.scol,
+            hipLaunchKernelGGL((k_sell3<T, 1>), dim3(nblk), dim3(nthr), 0, st, A.n_rows, A.n_slices, A.sptr, A.scol,
                                val, x, y, f, partials, sc);
             finalize(st, partials, nblk, FIN_PQ, sc);
             break;
         case 2:
-            hipLaunchKernelGGL((k_sell3<T, 2>), dim3(nblk), dim3(kBlock), 0, st, A.n_rows, A.n_slices, A.sptr, A.scol,
+            hipLaunchKernelGGL((k_sell3<T, 2>), dim3(nblk), dim3(nthr), 0, st, A.n_rows, A.n_slices, A.sptr, A.scol,
                                val, x, y, f, partials, sc);
             finalize(st, partials, nblk, FIN_RR, sc);
             break;
         default:
-            hipLaunchKernelGGL((k_sell3<T, 3>), dim3(nblk), dim3(kBlock), 0, st, A.n_rows, A.n_slices, A.sptr, A.scol,
+            hipLaunchKernelGGL((k_sell3<T, 3>), dim3(nblk), dim3(nthr), 0, st, A.n_rows, A.n_slices, A.sptr, A.scol,
                                val, x, y, f, partials, sc);
             break;
     }

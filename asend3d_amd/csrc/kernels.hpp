@@ -21,7 +21,9 @@ struct Sell3 {
     int n_rows = 0, n_slices = 0;
     const int32_t* sptr = nullptr;  // slot offset of each slice (n_slices + 1)
     const int32_t* scol = nullptr;  // column (node row) of slot k, lane l at 64k + l
+    int waves_per_block = 4;        // slices (wavefronts) per workgroup of the SpMV: 1, 2, 4 or 8
 };
+constexpr int kSellMaxWaves = 8;
 
 // Index of component c (= 3i + j) of block k of row r in the value array.
 __host__ __device__ inline int64_t sell_index(const int32_t* sptr, int r, int k, int c) {

@@ -1216,6 +1216,25 @@ int mg_set_smoother(Multigrid* mg, int kind, int degree, double lo_frac, double 
     return ASEND_OK;
 }
 
+// Recompute the device Chebyshev coefficients from the stored bounds (after a
+// change of degree or interval that keeps the Jacobi kind).
+int mg_refresh_coefficients(Multigrid* mg, hipStream_t st) {
+    if (!mg->ready) return ASEND_OK;
+    for (size_t l = 0; l + 1 < mg->L.size(); ++l)
+        cheb_coefficients(mg, mg->L[l].lmax, mg->h_coef.data() + coef_slot(mg, (int)l, 0));
+    if (mg->lmax_factor > 0.0)
+        cheb_coefficients(mg, mg->lmax_factor, mg->h_coef.data() + coef_slot(mg, (int)mg->L.size(), 0));
+    MGCHK(hipMemcpyAsync(mg->d_coef, mg->h_coef.data(), mg->h_coef.size() * sizeof(float), hipMemcpyHostToDevice,
+                         st));
+    MGCHK(hipStreamSynchronize(st));
+    return ASEND_OK;
+}
+
+void mg_set_fine_geometry(Multigrid* mg, int waves_per_block) { mg->fine.waves_per_block = waves_per_block; }
+
+int mg_smoother_degree(const Multigrid* mg) { return mg->cheb_degree;
+}
+
 int upload_factor_coefficients(Multigrid* mg, hipStream_t st) {
     const int slot = coef_slot(mg, (int)mg->L.size(), 0);
     cheb_coefficients(mg, mg->lmax_factor, mg->h_coef.data() + slot);

@@ -34,6 +34,12 @@ int mg_set_fine_smoother(Multigrid* mg, int kind);
 // (kind 1) Jacobi, degree 1..8, interval [lo_frac, hi_frac] * lambda_max
 // (default block, 2, [0.1, 1.15]).  Changing the kind needs a new mg_setup.
 int mg_set_smoother(Multigrid* mg, int kind, int degree, double lo_frac, double hi_frac);
+// Re-derive the Chebyshev step coefficients after mg_set_smoother changed the
+// degree or interval (same Jacobi kind: the stored bounds stay valid).
+int mg_refresh_coefficients(Multigrid* mg, hipStream_t st);
+int mg_smoother_degree(const Multigrid* mg);
+// Workgroup geometry of the finest-level SpMV (waves per workgroup).
+void mg_set_fine_geometry(Multigrid* mg, int waves_per_block);
 // Register the learned factor (device arrays borrowed, must stay valid while
 // it is in use); with smoother_bounds also estimate lambda_max(P A), which the
 // Chebyshev smoother needs.

@@ -145,8 +145,19 @@ def main():
         problems.upload(ctx, prob)
     if pc != api.PC_JACOBI:
         ctx.set_grid(args.nx, args.ny, args.nz)
-    fast = ctx.opts(api.SOLVE_IR_MIXED, pc, tol=args.tol, inner_tol=1e-4 if pc == api.PC_JACOBI else 1e-3,
-                    max_it=2000000, max_outer=40, check_every=32 if pc == api.PC_JACOBI else 4)
+    tuning = None
+    if pc != api.PC_JACOBI:
+        # project autotuner: SpMV workgroup geometry and the multigrid solve's
+        # Chebyshev degree / inner tolerance / check interval (setup, not timed)
+        ctx.assemble()
+        t0 = time.perf_counter()
+        tn = ctx.autotune(prob.f)
+        tuning = tn.as_dict()
+        tuning["tune_s"] = time.perf_counter() - t0
+        log(f"autotuned in {tuning['tune_s']:.1f} s: {tn.spmv_waves_per_block} waves/workgroup, Chebyshev degree "
+            f"{tn.cheb_degree}, inner tol {tn.inner_tol:g}, check every {tn.check_every}")
+    fast = ctx.opts(api.SOLVE_IR_MIXED, pc, tol=args.tol, inner_tol=1e-4 if pc == api.PC_JACOBI else 0.0,
+                    max_it=2000000, max_outer=40, check_every=32 if pc == api.PC_JACOBI else 0)
     base = ctx.opts(api.SOLVE_PCG_FP64, api.PC_JACOBI, tol=args.tol, max_it=2000000, check_every=32)
 
     # reference solution: FP64 PCG to 1e-11 (about the attainable FP64 residual here)
@@ -177,8 +188,8 @@ def main():
     ablation = None
     if pc == api.PC_LEARNED:
         # the same pipeline without the network (block-Jacobi Chebyshev smoother)
-        mg_o = ctx.opts(api.SOLVE_IR_MIXED, api.PC_MG, tol=args.tol, inner_tol=1e-3, max_it=2000000, max_outer=40,
-                        check_every=4)
+        mg_o = ctx.opts(api.SOLVE_IR_MIXED, api.PC_MG, tol=args.tol, inner_tol=0.0, max_it=2000000, max_outer=40,
+                        check_every=0)
         ctx.static_compliance(prob.f, mg_o)
         _, _, _, stm = ctx.static_compliance(prob.f, mg_o)
         ablation = {"mg_no_network_s": stm.device_ms / 1e3, "mg_no_network_iterations": int(stm.iterations)}
@@ -224,6 +235,7 @@ def main():
                    "n_dof": prob.n_dof, "n_elements": prob.mesh.n_el, "tol": args.tol,
                    "fast_precond": {api.PC_JACOBI: "jacobi", api.PC_MG: "mg", api.PC_LEARNED: "learned"}[pc],
                    "training": train,
+                   "autotune": tuning,
                    "parallelism": "single GPU"},
         "fp64_jacobi_s": t_base,
         "speedup_vs_fp64_jacobi": t_base / t_fast,

@@ -276,6 +276,26 @@ int asend_mg_vcycle(asend_ctx* ctx, const float* b, float* x);
  * default) or launch the same kernels one by one (on = 0).  Results are
  * identical either way. */
 int asend_set_graphs(asend_ctx* ctx, int on);
+/* Autotuner (no reference counterpart).  ASEND_TUNE_SPMV times the fine SpMV
+ * with 1, 2, 4 and 8 slices (wavefronts) per workgroup and keeps the fastest;
+ * ASEND_TUNE_SOLVER times full multigrid-preconditioned mixed solves of rhs
+ * over Chebyshev degree {2, 3} x inner tolerance {1e-2, 1e-3, 1e-4} x check
+ * interval {2, 4, 8} and keeps the fastest converged choice; a later solve
+ * uses the tuned inner tolerance / check interval where its options pass 0. */
+#define ASEND_TUNE_SPMV 1
+#define ASEND_TUNE_SOLVER 2
+typedef struct asend_tuning {
+    int spmv_waves_per_block;  /* chosen geometry */
+    double spmv_ms_fp32[4];    /* measured, for 1, 2, 4, 8 waves (0 = not tried) */
+    double spmv_ms_fp64[4];
+    int cheb_degree;
+    double inner_tol;
+    int check_every;
+    double solve_ms;           /* best tuned solve */
+    int n_trials;
+} asend_tuning;
+int asend_autotune(asend_ctx* ctx, const double* rhs, int what, asend_tuning* out);
+
 /* Chebyshev smoother of every smoothed level: M = point Jacobi (kind 0) or the
  * inverse of the 3x3 node blocks (kind 1), degree 1..8, interval
  * [lo_frac, hi_frac] * lambda_max(M A).  Defaults 1, 2, 0.1, 1.15.  Kept across

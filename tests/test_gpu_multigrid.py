@@ -140,3 +140,24 @@ def test_graph_replay_matches_eager_launches(ctx):
     for run in res[True]:
         for (u1, it1), (u0, it0) in zip(run, ref):
             assert it1 == it0 and np.array_equal(u1, u0)
+
+
+def test_autotuner_choices_are_used(ctx):
+    dims = (24, 6, 6)
+    gpu_case(f"autotune {dims}")
+    p = problems.heterogeneous(*dims, seed=21)
+    setup(ctx, p, dims)
+    t = ctx.autotune(p.f)
+    assert t.spmv_waves_per_block in (1, 2, 4, 8)
+    assert all(t.spmv_ms_fp32[k] > 0 for k in range(4))
+    assert t.cheb_degree in (2, 3) and t.inner_tol in (1e-2, 1e-3, 1e-4) and t.check_every in (2, 4, 8)
+    assert t.n_trials == 4 + 18 and t.solve_ms > 0
+    # a solve that passes 0 for the inner tolerance and check interval uses the tuned ones
+    o = ctx.opts(api.SOLVE_IR_MIXED, api.PC_MG, tol=1e-9, inner_tol=0.0, max_it=100000, max_outer=60,
+                 check_every=0)
+    u, s, J, st = ctx.static_compliance(p.f, o)
+    assert st.converged
+    ref = ctx.opts(api.SOLVE_PCG_FP64, api.PC_JACOBI, tol=1e-11, max_it=400000, check_every=32)
+    u_ref = ctx.static_compliance(p.f, ref)[0]
+    assert np.abs(u - u_ref).max() <= 1e-7 * np.abs(u_ref).max()
+    ctx.mg_set_smoother(1, 2, 0.1, 1.15)
