@@ -72,3 +72,50 @@ def block_dim_auto(n_nodes):
     while b * b < n_nodes:
         b += 6
     return b
+
+
+@dataclass
+class QuadGrid:
+    nx: int
+    ny: int
+    lx: float
+    ly: float
+    xyz: np.ndarray = field(repr=False)
+    conn: np.ndarray = field(repr=False)
+
+    @property
+    def n_nodes(self):
+        return self.xyz.shape[0]
+
+    @property
+    def n_el(self):
+        return self.conn.shape[0]
+
+    def node_id(self, i, j):
+        return i + (self.nx + 1) * j
+
+    def boundary_nodes(self):
+        i = np.arange(self.nx + 1)
+        j = np.arange(self.ny + 1)
+        ids = np.concatenate([self.node_id(i, 0), self.node_id(i, self.ny), self.node_id(0, j),
+                              self.node_id(self.nx, j)])
+        return np.unique(ids).astype(np.int32)
+
+    def nodes_at_x(self, i):
+        return np.sort(self.node_id(i, np.arange(self.ny + 1))).astype(np.int32)
+
+
+def quad_grid(nx, ny, lx, ly):
+    """Structured shell4 grid on [0,lx]x[0,ly] in the plane z = 0, nodes
+    x-fastest, element nodes counter-clockwise from (-1,-1) (the quad4 order of
+    S/element_soln_fields.rs:194-213)."""
+    xs = np.arange(nx + 1, dtype=np.float64) * (lx / nx)
+    ys = np.arange(ny + 1, dtype=np.float64) * (ly / ny)
+    Y, X = np.meshgrid(ys, xs, indexing="ij")
+    xyz = np.stack([X.reshape(-1), Y.reshape(-1), np.zeros(X.size)], axis=1)
+    ey, ex = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    ex = ex.reshape(-1)
+    ey = ey.reshape(-1)
+    n0 = ex + (nx + 1) * ey
+    conn = np.stack([n0, n0 + 1, n0 + 1 + (nx + 1), n0 + (nx + 1)], axis=1).astype(np.int32)
+    return QuadGrid(nx, ny, lx, ly, xyz, conn)

@@ -12,7 +12,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import api
-from .meshgen import HexBlock, hex_block
+from .meshgen import HexBlock, QuadGrid, hex_block, quad_grid
 
 C2_SEED = 20261202
 
@@ -124,3 +124,38 @@ def batch_variants(n_variants, seed=7, min_el=10_000, max_el=200_000, n_load=8, 
         if progress is not None:
             progress(v, p)
     return out
+
+
+@dataclass
+class ShellProblem:
+    mesh: QuadGrid
+    E: np.ndarray
+    nu: np.ndarray
+    t: np.ndarray
+    rho: np.ndarray
+    block_dim: int
+    dof_index: np.ndarray
+    n_dof: int
+    clamp_nodes: np.ndarray
+    pen_dofs: np.ndarray
+    etype: int = 41
+
+
+def shell_plate(nx, ny, lx, ly, t, E, nu, rho, clamp="edges"):
+    """Flat shell4 plate in the xy plane; clamp = "edges" (all four edges, all
+    six DOFs) or "x0" (the x = 0 edge)."""
+    m = quad_grid(nx, ny, lx, ly)
+    n_el = m.n_el
+    clamp_nodes = m.boundary_nodes() if clamp == "edges" else m.nodes_at_x(0)
+    bd = api.block_dim_auto(m.n_nodes)
+    constraints = [[clamp_nodes]] * 6
+    _, di, n_dof = api.reorder_nodes(m.xyz, m.conn, 6, constraints, bd)
+    pen = np.concatenate([di[clamp_nodes, d] for d in range(6)]).astype(np.int64)
+    full = lambda v: np.full(n_el, float(v)) if np.isscalar(v) else np.ascontiguousarray(v, dtype=np.float64)
+    return ShellProblem(m, full(E), full(nu), full(t), full(rho), bd, di, n_dof, clamp_nodes, pen)
+
+
+def config3():
+    """1 x 1 m plate, 256 x 256 shell4, t = 5 mm, E = 70 GPa, nu = 0.33,
+    rho = 2700 kg/m^3, edges clamped (66,049 nodes, 396,294 DOF)."""
+    return shell_plate(256, 256, 1.0, 1.0, 0.005, 70e9, 0.33, 2700.0)

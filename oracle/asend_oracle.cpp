@@ -355,6 +355,183 @@ void solid_ke_iso(int type, const double* X, double E, double nu, double* Ke) {
     else std::memcpy(Ke, K.data(), sizeof(double) * 576);
 }
 
+
+/* ---- shell4 (type 41) --------------------------------------------------
+ * Element data (S/element_meth.rs:253-297): 4 nodes x 6 DOF + 8 internal DOF
+ * (n_dim 10: 4 bilinear functions and 6 bubbles, S/element_soln_fields.rs:194-243),
+ * 2x2 Gauss points (+-1/sqrt3, weight 1), section deformation
+ * [e11, e22, g12, k11, k22, k12, g13, g23, drill]. */
+void shell4_shape(const double* s, double* N, double* dN /* 10 x 3 */) {
+    const double x = s[0], y = s[1];
+    const double n[10] = {0.25 * (1 - x) * (1 - y), 0.25 * (1 + x) * (1 - y), 0.25 * (1 + x) * (1 + y),
+                          0.25 * (1 - x) * (1 + y), 1 - x * x, 1 - y * y, (1 - x * x) * (1 - y),
+                          (1 - y * y) * (1 + x), (1 - x * x) * (1 + y), (1 - y * y) * (1 - x)};
+    const double d[10][2] = {{-0.25 * (1 - y), -0.25 * (1 - x)}, {0.25 * (1 - y), -0.25 * (1 + x)},
+                             {0.25 * (1 + y), 0.25 * (1 + x)},   {-0.25 * (1 + y), 0.25 * (1 - x)},
+                             {-2 * x, 0.0},                       {0.0, -2 * y},
+                             {-2 * x * (1 - y), -1 + x * x},      {1 - y * y, -2 * y * (1 + x)},
+                             {-2 * x * (1 + y), 1 - x * x},       {-1 + y * y, -2 * y * (1 - x)}};
+    for (int a = 0; a < 10; ++a) {
+        N[a] = n[a];
+        dN[3 * a] = d[a][0];
+        dN[3 * a + 1] = d[a][1];
+        dN[3 * a + 2] = 0.0;
+    }
+}
+
+/* Jacobian from the 4 nodal functions with J[2][2] = sign of the in-plane
+ * determinant (get_ip_data_dfd0, S/element_soln_fields.rs:383-445). */
+void shell4_jac(const double* loc /* 3 x 4 */, const double* dN, double* det, double* Ji) {
+    double J[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double acc = 0.0;
+            for (int a = 0; a < 4; ++a) acc += loc[4 * i + a] * dN[3 * a + j];
+            J[3 * i + j] = acc;
+        }
+    J[8] = (J[0] * J[4] - J[3] * J[1] > 0.0) ? 1.0 : -1.0;
+    det_inv(*det, Ji, J, 3);
+}
+
+/* Element stiffness (condensed to the 24 nodal DOFs) and consistent mass of a
+ * single isotropic layer of thickness t at the mid-plane (z offset 0).  The
+ * element frame is the identity when the element normal is parallel to z
+ * (correct_orient_dfd0, S/element_properties.rs:2052-2140, leaves the section
+ * orientation untouched then); other elements are rejected (-2). */
+int shell4_element(const double* X /* 3 x 4 SoA */, double E, double nu, double t, double rho, double* Ke,
+                   double* Me) {
+    double v1[3], v2[3], v3[3];
+    for (int k = 0; k < 3; ++k) {
+        v1[k] = X[4 * k + 2] - X[4 * k + 0];
+        v2[k] = X[4 * k + 3] - X[4 * k + 1];
+    }
+    v3[0] = v1[1] * v2[2] - v1[2] * v2[1];
+    v3[1] = v1[2] * v2[0] - v1[0] * v2[2];
+    v3[2] = v1[0] * v2[1] - v1[1] * v2[0];
+    if (v3[2] < 0.0)
+        for (double& c : v3) c = -c;
+    // cross of the section z axis (0,0,1) with v3
+    if (std::sqrt(v3[0] * v3[0] + v3[1] * v3[1]) >= 1e-12) return -2;
+    const double* loc = X;  // loc_ori = identity
+    // ABD (get_abd_dfd0, S/element_properties.rs:766-876) of the one layer:
+    // Q = S^-1 (get_layer_q_dfd0 :138-221), A = Q t, B = 0, D = Q t^3/12,
+    // transverse shear and drill stiffness = A66
+    double mod[3] = {E, E, E}, G = E / (2.0 * (1.0 + nu));
+    double S[9] = {1.0 / mod[0], -nu / mod[0], 0.0, -nu / mod[0], 1.0 / mod[1], 0.0, 0.0, 0.0, 1.0 / G};
+    double Q[9], dq;
+    det_inv(dq, Q, S, 3);
+    double C[81] = {0};
+    const double zmin = -0.5 * t, zmax = 0.5 * t;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            C[9 * i + j] = (zmax - zmin) * Q[3 * i + j];
+            C[9 * (i + 3) + j + 3] = (zmax * zmax * zmax - zmin * zmin * zmin) / 3.0 * Q[3 * i + j];
+            C[9 * i + j + 3] = C[9 * (i + 3) + j] = 0.5 * (zmax * zmax - zmin * zmin) * Q[3 * i + j];
+        }
+    C[60] = C[70] = C[80] = C[20];
+    // mass (get_shell_mass_dfd0, :1354-1408)
+    double Mm[36] = {0};
+    Mm[0] = Mm[7] = Mm[14] = rho * t;
+    const double m1 = 0.5 * rho * (zmax * zmax - zmin * zmin), m2 = rho * (zmax * zmax * zmax - zmin * zmin * zmin) / 3.0;
+    Mm[4] += m1;
+    Mm[24] += m1;
+    Mm[9] -= m1;
+    Mm[19] -= m1;
+    Mm[21] = Mm[28] = Mm[35] = m2;
+    const int tot = 32;
+    std::vector<double> K(tot * tot, 0.0);
+    std::fill(Me, Me + 576, 0.0);
+    const double g = 0.5773502691896257;
+    const double ip[4][2] = {{-g, -g}, {g, -g}, {-g, g}, {g, g}};
+    double N[10], dN[30], Nc[10], dNc[30], det, Ji[9], detc, Jic[9];
+    const double zero[3] = {0, 0, 0};
+    shell4_shape(zero, Nc, dNc);
+    shell4_jac(loc, dNc, &detc, Jic);
+    for (int q = 0; q < 4; ++q) {
+        const double s[3] = {ip[q][0], ip[q][1], 0.0};
+        shell4_shape(s, N, dN);
+        shell4_jac(loc, dN, &det, Ji);
+        double dx[10][3];
+        for (int a = 0; a < 10; ++a) {
+            const double* Jm = a < 4 ? Ji : Jic;
+            for (int j = 0; j < 3; ++j) dx[a][j] = dN[3 * a] * Jm[j] + dN[3 * a + 1] * Jm[3 + j] + dN[3 * a + 2] * Jm[6 + j];
+        }
+        // section-deformation columns (get_section_def_dfd0 for one unit DOF,
+        // S/element_soln_fields.rs:1160-1343)
+        double B[9][32] = {{0}};
+        for (int a = 0; a < 4; ++a) {
+            const double Nx = dx[a][0], Ny = dx[a][1];
+            for (int c = 0; c < 3; ++c) {  // translation along global c: local vector e_c
+                double l[3] = {0, 0, 0};
+                l[c] = 1.0;
+                const int col = 6 * a + c;
+                B[0][col] = l[0] * Nx;
+                B[1][col] = l[1] * Ny;
+                B[2][col] = l[0] * Ny + l[1] * Nx;
+                B[6][col] = l[2] * Nx;
+                B[7][col] = l[2] * Ny;
+                B[8][col] = l[0] * Ny - l[1] * Nx;
+            }
+            for (int c = 0; c < 3; ++c) {  // rotation about global c
+                double m[3] = {0, 0, 0};
+                m[c] = 1.0;
+                const int col = 6 * a + 3 + c;
+                B[3][col] = m[1] * Nx;
+                B[4][col] = -m[0] * Ny;
+                B[5][col] = m[1] * Ny - m[0] * Nx;
+                B[6][col] = m[1] * N[a];
+                B[7][col] = -m[0] * N[a];
+                B[8][col] = 2.0 * m[2] * N[a];
+            }
+        }
+        // internal DOFs 24..31: (bubble, component) = (4,u) (5,u) (4,v) (5,v) (6,w) (7,w) (8,w) (9,w)
+        const int ib[8] = {4, 5, 4, 5, 6, 7, 8, 9}, ic[8] = {0, 0, 1, 1, 2, 2, 2, 2};
+        for (int k = 0; k < 8; ++k) {
+            const double Nx = dx[ib[k]][0], Ny = dx[ib[k]][1];
+            const int col = 24 + k;
+            if (ic[k] == 0) {
+                B[0][col] = Nx;
+                B[2][col] = Ny;
+                B[8][col] = Ny;
+            } else if (ic[k] == 1) {
+                B[1][col] = Ny;
+                B[2][col] = Nx;
+                B[8][col] = -Nx;
+            } else {
+                B[6][col] = Nx;
+                B[7][col] = Ny;
+            }
+        }
+        double CB[9][32];
+        for (int i = 0; i < 9; ++i)
+            for (int j = 0; j < 32; ++j) {
+                double acc = 0.0;
+                for (int k = 0; k < 9; ++k) acc += C[9 * i + k] * B[k][j];
+                CB[i][j] = acc * det;
+            }
+        for (int i = 0; i < 32; ++i)
+            for (int j = 0; j < 32; ++j) {
+                double acc = 0.0;
+                for (int k = 0; k < 9; ++k) acc += B[k][i] * CB[k][j];
+                K[i * tot + j] += acc;
+            }
+        // mass: b (6 x 24) = N_a times the local unit vector of each nodal DOF
+        // (get_rum_dfd0, S/element_equations.rs:514-581)
+        double b[6][24] = {{0}};
+        for (int a = 0; a < 4; ++a)
+            for (int c = 0; c < 6; ++c) b[c][6 * a + c] = N[a];
+        for (int i = 0; i < 24; ++i)
+            for (int j = 0; j < 24; ++j) {
+                double acc = 0.0;
+                for (int k = 0; k < 6; ++k)
+                    for (int l = 0; l < 6; ++l) acc += b[k][i] * Mm[6 * k + l] * b[l][j];
+                Me[24 * i + j] += acc * det;
+            }
+    }
+    condense(K.data(), tot, 24, Ke);
+    return 0;
+}
+
 struct Pattern {
     std::vector<int64_t> rowptr;
     std::vector<int32_t> cols;
@@ -946,6 +1123,44 @@ int orc_solid_sens(int type, int64_t n_nodes, const double* xyz, int64_t n_el, c
         double s = 0.0;
         for (int i = 0; i < tot; ++i) s += le[i] * Rf[i].d;
         dJdE[e] = -s;
+    }
+    return 0;
+}
+
+
+int orc_shell4_element(const double* X, double E, double nu, double t, double rho, double* Ke, double* Me) {
+    double Mtmp[576];
+    return shell4_element(X, E, nu, t, rho, Ke, Me ? Me : Mtmp);
+}
+
+int orc_assemble_shell(int64_t n_nodes, const double* xyz, int64_t n_el, const int32_t* conn, const double* E,
+                       const double* nu, const double* t, const double* rho, const int64_t* dof_index, int64_t n_dof,
+                       int64_t* rowptr, int32_t* cols, double* vals, double* mvals, int64_t* nnz_out) {
+    (void)n_nodes;
+    Pattern p = build_pattern(n_el, 4, conn, dof_index, n_dof, 6);
+    if (nnz_out) *nnz_out = p.rowptr[n_dof];
+    if (!rowptr) return 0;
+    std::memcpy(rowptr, p.rowptr.data(), sizeof(int64_t) * (n_dof + 1));
+    if (!cols || !vals) return 0;
+    std::memcpy(cols, p.cols.data(), sizeof(int32_t) * p.cols.size());
+    std::memset(vals, 0, sizeof(double) * p.cols.size());
+    if (mvals) std::memset(mvals, 0, sizeof(double) * p.cols.size());
+    double X[12], Ke[576], Me[576];
+    int64_t gd[24];
+    for (int64_t e = 0; e < n_el; ++e) {
+        const int32_t* c = conn + 4 * e;
+        gather_X(xyz, c, 4, X);
+        int rc = shell4_element(X, E[e], nu[e], t[e], rho ? rho[e] : 0.0, Ke, Me);
+        if (rc) return rc;
+        for (int a = 0; a < 4; ++a)
+            for (int d = 0; d < 6; ++d) gd[6 * a + d] = dof_index[6 * (int64_t)c[a] + d];
+        for (int i = 0; i < 24; ++i)
+            for (int j = 0; j < 24; ++j) {
+                int64_t k = find_col(rowptr, cols, gd[i], (int32_t)gd[j]);
+                if (k < 0) return -3;
+                vals[k] += Ke[i * 24 + j];
+                if (mvals) mvals[k] += Me[i * 24 + j];
+            }
     }
     return 0;
 }

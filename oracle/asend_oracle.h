@@ -142,6 +142,20 @@ int orc_solid_sens(int type, int64_t n_nodes, const double* xyz, int64_t n_el,
                    const int32_t* conn, const double* E, const double* nu,
                    const int64_t* dof_index, const double* u, const double* lam, double* dJdE);
 
+
+/* shell4 (type 41) single isotropic layer of thickness t at the mid-plane:
+ * condensed 24x24 stiffness and 24x24 consistent mass of one element
+ * (S/element_equations.rs:206-418, :420-626, S/element_soln_fields.rs:1160-1343,
+ * S/element_properties.rs:766-876, :1354-1408).  X is 3 x 4 SoA.  Elements
+ * whose normal is not parallel to z return -2 (the identity element frame is
+ * the only one restated). */
+int orc_shell4_element(const double* X, double E, double nu, double t, double rho, double* Ke, double* Me);
+/* Shell assembly into CSR (6 DOF per node, dof_index[6n + d]); mvals (may be
+ * null) receives the consistent mass on the same pattern. */
+int orc_assemble_shell(int64_t n_nodes, const double* xyz, int64_t n_el, const int32_t* conn, const double* E,
+                       const double* nu, const double* t, const double* rho, const int64_t* dof_index, int64_t n_dof,
+                       int64_t* rowptr, int32_t* cols, double* vals, double* mvals, int64_t* nnz_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -263,3 +263,33 @@ def solid_sens(etype, xyz, conn, E, nu, dof_index, u, lam):
     _chk(lib().orc_solid_sens(ctypes.c_int(etype), _i64(xyz.shape[0]), _ptr(xyz), _i64(conn.shape[0]), _ptr(conn),
                               _ptr(E), _ptr(nu), _ptr(di), _ptr(u), _ptr(lam), _ptr(out)), "solid_sens")
     return out
+
+
+def shell4_element(X, E, nu, t, rho):
+    """Condensed 24x24 stiffness and 24x24 consistent mass of one shell4
+    (X: 3 x 4 SoA coordinates)."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    Ke = np.zeros(576)
+    Me = np.zeros(576)
+    _chk(lib().orc_shell4_element(_ptr(X), _dbl(E), _dbl(nu), _dbl(t), _dbl(rho), _ptr(Ke), _ptr(Me)), "shell4")
+    return Ke.reshape(24, 24), Me.reshape(24, 24)
+
+
+def assemble_shell(xyz, conn, E, nu, t, rho, dof_index, n_dof):
+    """Shell4 stiffness and consistent mass (CSR pair) on the same pattern."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    conn = np.ascontiguousarray(conn, dtype=np.int32)
+    args = [np.ascontiguousarray(a, dtype=np.float64) for a in (E, nu, t, rho)]
+    di = np.ascontiguousarray(dof_index, dtype=np.int64)
+    nnz = ctypes.c_int64(0)
+    rowptr = np.zeros(n_dof + 1, dtype=np.int64)
+    _chk(lib().orc_assemble_shell(_i64(xyz.shape[0]), _ptr(xyz), _i64(conn.shape[0]), _ptr(conn), *[_ptr(a) for a in args],
+                                  _ptr(di), _i64(n_dof), _ptr(rowptr), None, None, None, ctypes.byref(nnz)),
+         "assemble_shell")
+    cols = np.zeros(nnz.value, dtype=np.int32)
+    vals = np.zeros(nnz.value)
+    mvals = np.zeros(nnz.value)
+    _chk(lib().orc_assemble_shell(_i64(xyz.shape[0]), _ptr(xyz), _i64(conn.shape[0]), _ptr(conn), *[_ptr(a) for a in args],
+                                  _ptr(di), _i64(n_dof), _ptr(rowptr), _ptr(cols), _ptr(vals), _ptr(mvals),
+                                  ctypes.byref(nnz)), "assemble_shell")
+    return CSR(rowptr, cols, vals), CSR(rowptr, cols, mvals)
