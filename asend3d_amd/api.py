@@ -104,7 +104,7 @@ _lib.asend_ctx_destroy.argtypes = [_vp]
 
 EXPORTED = ["asend_ctx_create", "asend_ctx_destroy", "asend_last_error", "asend_version", "asend_sync",
             "asend_reorder_nodes", "asend_block_dim_auto", "asend_surface_traction", "asend_lognormal_moduli",
-            "asend_upload_mesh", "asend_set_grid", "asend_update_moduli", "asend_set_penalty",
+            "asend_upload_mesh", "asend_set_thickness", "asend_set_grid", "asend_update_moduli", "asend_set_penalty",
             "asend_penalty_scale", "asend_assemble", "asend_get_matrix", "asend_spmv", "asend_solve",
             "asend_element_sens", "asend_static_compliance", "asend_bench_spmv", "asend_mg_info", "asend_mg_apply",
             "asend_mg_restrict", "asend_mg_prolong", "asend_mg_vcycle", "asend_bench_copy",
@@ -241,6 +241,10 @@ class Context:
                   "asend_upload_mesh")
         self.n_dof = n_dof
         self.n_el = conn.shape[0]
+
+    def set_thickness(self, t):
+        t = _f64(t)
+        self._chk(_lib.asend_set_thickness(self.h, _p(t)), "asend_set_thickness")
 
     def set_grid(self, nx, ny, nz):
         self._chk(_lib.asend_set_grid(self.h, _i64(nx), _i64(ny), _i64(nz)), "asend_set_grid")

@@ -52,6 +52,10 @@ struct asend_ctx {
     int64_t nval() const { return 9 * 64 * n_slots; }
     uint8_t* d_slot = nullptr;
     double *d_xyz_r = nullptr, *d_E = nullptr, *d_nu = nullptr, *d_rho = nullptr;
+    // shell4 meshes: per-element thickness (asend_set_thickness); each node is
+    // two block rows (translations 2r, rotations 2r+1) of the operator
+    double* d_thk = nullptr;
+    bool thk_set = false;
     double* d_K64 = nullptr;
     float* d_K32 = nullptr;
     double* d_dinv64 = nullptr;
@@ -206,6 +210,16 @@ void free_mesh(asend_ctx* ctx) {
     dfree(ctx->d_E);
     dfree(ctx->d_nu);
     dfree(ctx->d_rho);
+    // penalty DOFs are numbered on the mesh they were set for
+    dfree(ctx->d_pen);
+    dfree(ctx->d_pen_mult);
+    ctx->n_pen = 0;
+    ctx->h_pen.clear();
+    ctx->h_pen_mult.clear();
+    ctx->sf_user = ctx->sf = 0.0;
+    ctx->sf_fixed = false;
+    dfree(ctx->d_thk);
+    ctx->thk_set = false;
     dfree(ctx->d_K64);
     dfree(ctx->d_K32);
     dfree(ctx->d_dinv64);
@@ -485,18 +499,33 @@ int solve_device(asend_ctx* ctx, const asend_solve_opts* o, asend_solve_stats* s
     return ASEND_OK;
 }
 
-int assemble_device(asend_ctx* ctx, double c_mass) {
-    const int64_t nval = ctx->nval();
-    HIPCHK(hipMemsetAsync(ctx->d_K64, 0, nval * sizeof(double), ctx->st));
-    int ncol = (int)ctx->h_color_ptr.size() - 1;
+// Element integration into K (cleared first): k_scale K_e(E) + c_mass M_e
+// of every element, one launch per element colour.  The brick kernel has no
+// stiffness scale (k_scale must be 1; pass zero moduli for the mass alone).
+int integrate_elements(asend_ctx* ctx, const double* E, double k_scale, double c_mass, double* K) {
+    HIPCHK(hipMemsetAsync(K, 0, ctx->nval() * sizeof(double), ctx->st));
+    const int ncol = (int)ctx->h_color_ptr.size() - 1;
     for (int c = 0; c < ncol; ++c) {
-        int b = ctx->h_color_ptr[c], e = ctx->h_color_ptr[c + 1];
-        launch_hex8_assemble(ctx->st, ctx->d_elems + b, e - b, ctx->d_conn_r, ctx->d_xyz_r, ctx->d_E, ctx->d_nu,
-                             ctx->d_rho, c_mass, ctx->d_sptr, ctx->d_slot, ctx->d_K64);
+        const int b = ctx->h_color_ptr[c], e = ctx->h_color_ptr[c + 1];
+        if (ctx->etype == ASEND_SHELL4)
+            launch_shell4_assemble(ctx->st, ctx->d_elems + b, e - b, ctx->d_conn_r, ctx->d_xyz_r, E, ctx->d_nu,
+                                   ctx->d_thk, ctx->d_rho, k_scale, c_mass, ctx->d_sptr, ctx->d_slot, K);
+        else
+            launch_hex8_assemble(ctx->st, ctx->d_elems + b, e - b, ctx->d_conn_r, ctx->d_xyz_r, E, ctx->d_nu,
+                                 ctx->d_rho, c_mass, ctx->d_sptr, ctx->d_slot, K);
     }
     HIPCHK(hipGetLastError());
-    int rc;
+    return ASEND_OK;
+}
+
+int assemble_device(asend_ctx* ctx, double c_mass) {
+    const int64_t nval = ctx->nval();
+    if (ctx->etype == ASEND_SHELL4 && !ctx->thk_set)
+        return fail(ctx, ASEND_ERR_STATE, "shell4 mesh without thickness (asend_set_thickness)");
+    int rc = integrate_elements(ctx, ctx->d_E, 1.0, c_mass, ctx->d_K64);
+    if (rc) return rc;
     if (ctx->n_pen > 0) {
+        if (ctx->h_pen.back() >= ctx->n_dof) return fail(ctx, ASEND_ERR_STATE, "penalty DOF beyond the mesh");
         if (!ctx->sf_fixed) {
             if (ctx->sf_user > 0) {
                 ctx->sf = ctx->sf_user;
@@ -591,16 +620,40 @@ int asend_sync(asend_ctx* ctx) {
     return ASEND_OK;
 }
 
-int asend_upload_mesh(asend_ctx* ctx, int elem_type, int64_t n_nodes, const double* xyz, const int64_t* dof_index,
-                      int64_t n_dof, int64_t n_el, const int32_t* conn, const double* E, const double* nu,
-                      const double* rho) {
+int asend_upload_mesh(asend_ctx* ctx, int elem_type, int64_t n_mesh_nodes, const double* xyz,
+                      const int64_t* dof_index, int64_t n_dof, int64_t n_el, const int32_t* conn, const double* E,
+                      const double* nu, const double* rho) {
     if (!ctx) return ASEND_ERR_ARG;
-    if (elem_type != ASEND_BRICK8)
+    if (elem_type != ASEND_BRICK8 && elem_type != ASEND_SHELL4)
         return fail(ctx, ASEND_ERR_UNSUPPORTED, "element type %d has no device integration kernel", elem_type);
-    if (n_nodes <= 0 || n_el <= 0 || !xyz || !dof_index || !conn || !E || !nu)
+    if (n_mesh_nodes <= 0 || n_el <= 0 || !xyz || !dof_index || !conn || !E || !nu)
         return fail(ctx, ASEND_ERR_ARG, "bad mesh arguments");
-    if (n_dof != 3 * n_nodes) return fail(ctx, ASEND_ERR_ARG, "n_dof must equal 3*n_nodes for brick8 meshes");
+    // a brick8 node is one 3x3 block row; a shell4 node two (translations,
+    // rotations), so every element touches 8 block rows either way
+    const bool shell = (elem_type == ASEND_SHELL4);
+    const int dpn = shell ? 6 : 3, npe = shell ? 4 : 8;
+    const int64_t n_nodes = shell ? 2 * n_mesh_nodes : n_mesh_nodes;  // block rows
+    if (n_dof != dpn * n_mesh_nodes)
+        return fail(ctx, ASEND_ERR_ARG, "n_dof must equal %d*n_nodes for this element type", dpn);
     if (n_nodes >= (1ll << 31) / 9) return fail(ctx, ASEND_ERR_UNSUPPORTED, "mesh too large for int32 indices");
+    if (shell) {
+        // flat elements only: the diagonals' normal must be parallel to z
+        for (int64_t e = 0; e < n_el; ++e) {
+            double d1[3], d2[3];
+            for (int c = 0; c < 4; ++c)
+                if (conn[4 * e + c] < 0 || conn[4 * e + c] >= n_mesh_nodes)
+                    return fail(ctx, ASEND_ERR_ARG, "connectivity out of range");
+            for (int k = 0; k < 3; ++k) {
+                d1[k] = xyz[3 * (int64_t)conn[4 * e + 2] + k] - xyz[3 * (int64_t)conn[4 * e + 0] + k];
+                d2[k] = xyz[3 * (int64_t)conn[4 * e + 3] + k] - xyz[3 * (int64_t)conn[4 * e + 1] + k];
+            }
+            const double nx = d1[1] * d2[2] - d1[2] * d2[1], ny = d1[2] * d2[0] - d1[0] * d2[2];
+            if (std::sqrt(nx * nx + ny * ny) >= 1e-12)
+                return fail(ctx, ASEND_ERR_UNSUPPORTED,
+                            "shell4 element %lld is not normal to z (only the identity element frame is supported)",
+                            (long long)e);
+        }
+    }
     HIPCHK(hipSetDevice(ctx->device));
     free_mesh(ctx);
     ctx->etype = elem_type;
@@ -609,26 +662,32 @@ int asend_upload_mesh(asend_ctx* ctx, int elem_type, int64_t n_nodes, const doub
     ctx->n_dof = n_dof;
     ctx->n_el = n_el;
     ctx->sf_fixed = false;
-    // node rank from the DOF numbering: dof_index[6n+c] = 3*rank + c
-    std::vector<int32_t> rank(n_nodes);
-    std::vector<uint8_t> seen(n_nodes, 0);
-    for (int64_t nd = 0; nd < n_nodes; ++nd) {
+    // node rank from the DOF numbering: dof_index[6n+c] = dpn*rank + c
+    std::vector<int32_t> rank(n_mesh_nodes);
+    std::vector<uint8_t> seen(n_mesh_nodes, 0);
+    for (int64_t nd = 0; nd < n_mesh_nodes; ++nd) {
         int64_t d0 = dof_index[6 * nd];
-        if (d0 < 0 || d0 % 3 != 0 || d0 >= n_dof || dof_index[6 * nd + 1] != d0 + 1 || dof_index[6 * nd + 2] != d0 + 2)
-            return fail(ctx, ASEND_ERR_ARG, "dof_index of node %lld is not a 3-DOF node block", (long long)nd);
-        int64_t r = d0 / 3;
+        bool ok = d0 >= 0 && d0 % dpn == 0 && d0 < n_dof;
+        for (int c = 1; c < dpn && ok; ++c) ok = dof_index[6 * nd + c] == d0 + c;
+        if (!ok) return fail(ctx, ASEND_ERR_ARG, "dof_index of node %lld is not a %d-DOF node block", (long long)nd, dpn);
+        int64_t r = d0 / dpn;
         if (seen[r]) return fail(ctx, ASEND_ERR_ARG, "dof_index is not a permutation");
         seen[r] = 1;
         rank[nd] = (int32_t)r;
     }
     std::vector<int32_t> conn_r(8 * n_el);
-    for (int64_t i = 0; i < 8 * n_el; ++i) {
+    for (int64_t i = 0; i < npe * n_el; ++i) {
         int32_t v = conn[i];
-        if (v < 0 || v >= n_nodes) return fail(ctx, ASEND_ERR_ARG, "connectivity out of range");
-        conn_r[i] = rank[v];
+        if (v < 0 || v >= n_mesh_nodes) return fail(ctx, ASEND_ERR_ARG, "connectivity out of range");
+        if (shell) {
+            conn_r[2 * i] = 2 * rank[v];
+            conn_r[2 * i + 1] = 2 * rank[v] + 1;
+        } else {
+            conn_r[i] = rank[v];
+        }
     }
-    std::vector<double> xyz_r(3 * n_nodes);
-    for (int64_t nd = 0; nd < n_nodes; ++nd)
+    std::vector<double> xyz_r(3 * n_mesh_nodes);
+    for (int64_t nd = 0; nd < n_mesh_nodes; ++nd)
         for (int c = 0; c < 3; ++c) xyz_r[3 * (int64_t)rank[nd] + c] = xyz[3 * nd + c];
     // node -> element adjacency
     std::vector<int32_t> n2e_ptr(n_nodes + 1, 0), n2e;
@@ -674,7 +733,7 @@ int asend_upload_mesh(asend_ctx* ctx, int elem_type, int64_t n_nodes, const doub
         static const int hex_edges[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
                                              {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
         std::vector<std::vector<int32_t>> en(n_nodes);
-        for (int64_t e = 0; e < n_el; ++e)
+        for (int64_t e = 0; e < (shell ? 0 : n_el); ++e)
             for (const auto& ed : hex_edges) {
                 int32_t a = conn_r[8 * e + ed[0]], b = conn_r[8 * e + ed[1]];
                 if (a == b) continue;
@@ -784,6 +843,7 @@ int asend_upload_mesh(asend_ctx* ctx, int elem_type, int64_t n_nodes, const doub
         for (int64_t e = 0; e < n_el; ++e) s += std::log(E[e]);
         ctx->mean_log_e = s / (double)n_el;
     }
+    if (shell && (rc = dalloc(ctx, &ctx->d_thk, n_el))) return rc;
     HIPCHK(hipMemcpy(ctx->d_sptr, ctx->h_sptr.data(), ctx->h_sptr.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->d_scol, scol.data(), scol.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->d_conn_r, conn_r.data(), conn_r.size() * 4, hipMemcpyHostToDevice));
@@ -803,9 +863,23 @@ int asend_upload_mesh(asend_ctx* ctx, int elem_type, int64_t n_nodes, const doub
     return ASEND_OK;
 }
 
+int asend_set_thickness(asend_ctx* ctx, const double* t) {
+    if (!ctx || !t) return ASEND_ERR_ARG;
+    if (!ctx->d_K64) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
+    if (ctx->etype != ASEND_SHELL4) return fail(ctx, ASEND_ERR_UNSUPPORTED, "thickness applies to shell4 meshes");
+    for (int64_t e = 0; e < ctx->n_el; ++e)
+        if (!(t[e] > 0.0)) return fail(ctx, ASEND_ERR_ARG, "thickness of element %lld is not positive", (long long)e);
+    HIPCHK(hipMemcpy(ctx->d_thk, t, ctx->n_el * 8, hipMemcpyHostToDevice));
+    ctx->thk_set = true;
+    ctx->assembled = false;
+    return ASEND_OK;
+}
+
 int asend_set_grid(asend_ctx* ctx, int64_t nx, int64_t ny, int64_t nz) {
     if (!ctx) return ASEND_ERR_ARG;
     if (!ctx->d_K64) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
+    if (ctx->etype != ASEND_BRICK8)
+        return fail(ctx, ASEND_ERR_UNSUPPORTED, "the multigrid hierarchy is built for brick8 grids");
     if (nx * ny * nz != ctx->n_el || (nx + 1) * (ny + 1) * (nz + 1) != ctx->n_nodes)
         return fail(ctx, ASEND_ERR_ARG, "grid %lldx%lldx%lld does not match the mesh", (long long)nx, (long long)ny,
                     (long long)nz);
@@ -990,6 +1064,8 @@ int asend_solve(asend_ctx* ctx, const double* rhs, double* x, const asend_solve_
 int asend_element_sens(asend_ctx* ctx, const double* u, const double* lam, double* dJdE) {
     if (!ctx || !u || !lam || !dJdE) return ASEND_ERR_ARG;
     if (!ctx->d_E) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
+    if (ctx->etype != ASEND_BRICK8)
+        return fail(ctx, ASEND_ERR_UNSUPPORTED, "element sensitivities are implemented for brick8 meshes");
     const int64_t n = ctx->n_dof;
     HIPCHK(hipMemcpy(ctx->d_x64, u, n * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->d_lam, lam, n * 8, hipMemcpyHostToDevice));
@@ -1005,6 +1081,8 @@ int asend_static_compliance(asend_ctx* ctx, const double* f, double* u, double* 
                             const asend_solve_opts* opts, asend_solve_stats* stats) {
     if (!ctx || !f || !opts) return ASEND_ERR_ARG;
     if (!ctx->d_K64) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
+    if (ctx->etype != ASEND_BRICK8)
+        return fail(ctx, ASEND_ERR_UNSUPPORTED, "element sensitivities are implemented for brick8 meshes");
     const int64_t n = ctx->n_dof;
     HIPCHK(hipMemcpy(ctx->d_f, f, n * 8, hipMemcpyHostToDevice));
     HIPCHK(hipDeviceSynchronize());
@@ -1165,20 +1243,21 @@ int asend_newmark_setup(asend_ctx* ctx, double dt, double beta, double gamma) {
         (rc = dalloc(ctx, &nm.w, n)) || (rc = dalloc(ctx, &nm.q, n)) || (rc = dalloc(ctx, &nm.rhs0, n)) ||
         (rc = dalloc(ctx, &nm.rhs, n)) || (rc = dalloc(ctx, &nm.fshape, n)) || (rc = dalloc(ctx, &nm.step, 1)))
         return rc;
-    // consistent mass alone: the element kernel with zero moduli and c_mass = 1
-    double* zeroE = nullptr;
-    if ((rc = dalloc(ctx, &zeroE, ctx->n_el))) return rc;
-    HIPCHK(hipMemsetAsync(zeroE, 0, ctx->n_el * 8, ctx->st));
-    HIPCHK(hipMemsetAsync(nm.M64, 0, nval * 8, ctx->st));
-    const int ncol = (int)ctx->h_color_ptr.size() - 1;
-    for (int c = 0; c < ncol; ++c) {
-        int b = ctx->h_color_ptr[c], e = ctx->h_color_ptr[c + 1];
-        launch_hex8_assemble(ctx->st, ctx->d_elems + b, e - b, ctx->d_conn_r, ctx->d_xyz_r, zeroE, ctx->d_nu,
-                             ctx->d_rho, 1.0, ctx->d_sptr, ctx->d_slot, nm.M64);
+    // consistent mass alone: the element kernel with c_mass = 1 and no
+    // stiffness (zero moduli for the brick kernel, k_scale = 0 for the shell)
+    if (ctx->etype == ASEND_SHELL4) {
+        if (!ctx->thk_set) return fail(ctx, ASEND_ERR_STATE, "shell4 mesh without thickness (asend_set_thickness)");
+        if ((rc = integrate_elements(ctx, ctx->d_E, 0.0, 1.0, nm.M64))) return rc;
+        HIPCHK(hipStreamSynchronize(ctx->st));
+    } else {
+        double* zeroE = nullptr;
+        if ((rc = dalloc(ctx, &zeroE, ctx->n_el))) return rc;
+        HIPCHK(hipMemsetAsync(zeroE, 0, ctx->n_el * 8, ctx->st));
+        rc = integrate_elements(ctx, zeroE, 1.0, 1.0, nm.M64);
+        HIPCHK(hipStreamSynchronize(ctx->st));
+        dfree(zeroE);
+        if (rc) return rc;
     }
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(ctx->st));
-    dfree(zeroE);
     // K_eff = K + c_m M (+ penalty).  The reference scales its penalty once,
     // on the first matrix a solve step assembles, which for a dynamic job
     // already holds the mass term (S/model_analysis.rs:255-259, :486-488)
@@ -1382,6 +1461,8 @@ int asend_learned_set_params(asend_ctx* ctx, const float* in, int n) {
 
 int asend_learned_train(asend_ctx* ctx, int n_steps, float lr, uint64_t seed, int mode, double* losses) {
     if (!ctx || n_steps < 0 || !(lr > 0.f) || mode < 0 || mode > 2) return ASEND_ERR_ARG;
+    if (ctx->d_K64 && ctx->etype != ASEND_BRICK8)
+        return fail(ctx, ASEND_ERR_UNSUPPORTED, "the learned preconditioner is defined on brick8 meshes");
     if (!ctx->learned) return fail(ctx, ASEND_ERR_STATE, "no learned preconditioner (asend_learned_init)");
     if (!ctx->assembled) return fail(ctx, ASEND_ERR_STATE, "assemble first");
     if (mode > 0 && !ctx->mg) return fail(ctx, ASEND_ERR_STATE, "this training mode needs the multigrid hierarchy");
@@ -1447,6 +1528,8 @@ int asend_learned_train(asend_ctx* ctx, int n_steps, float lr, uint64_t seed, in
 
 int asend_learned_eval(asend_ctx* ctx, const double* e, double* loss, float* grad, float* feat, float* out) {
     if (!ctx || !e) return ASEND_ERR_ARG;
+    if (ctx->d_K64 && ctx->etype != ASEND_BRICK8)
+        return fail(ctx, ASEND_ERR_UNSUPPORTED, "the learned preconditioner is defined on brick8 meshes");
     if (!ctx->learned) return fail(ctx, ASEND_ERR_STATE, "no learned preconditioner (asend_learned_init)");
     if (!ctx->assembled) return fail(ctx, ASEND_ERR_STATE, "assemble first");
     invalidate_factor(ctx);
@@ -1459,6 +1542,8 @@ int asend_learned_eval(asend_ctx* ctx, const double* e, double* loss, float* gra
 
 int asend_learned_vcycle_eval(asend_ctx* ctx, const double* e, int keep_bounds, double* loss, float* grad) {
     if (!ctx || !e || !loss) return ASEND_ERR_ARG;
+    if (ctx->d_K64 && ctx->etype != ASEND_BRICK8)
+        return fail(ctx, ASEND_ERR_UNSUPPORTED, "the learned preconditioner is defined on brick8 meshes");
     if (!ctx->learned) return fail(ctx, ASEND_ERR_STATE, "no learned preconditioner (asend_learned_init)");
     if (!ctx->assembled || !ctx->mg) return fail(ctx, ASEND_ERR_STATE, "assemble with a multigrid hierarchy first");
     const LearnedGraph g = learned_graph(ctx);
@@ -1482,6 +1567,8 @@ int asend_learned_vcycle_eval(asend_ctx* ctx, const double* e, int keep_bounds, 
 
 int asend_learned_apply(asend_ctx* ctx, const double* r, double* z) {
     if (!ctx || !r || !z) return ASEND_ERR_ARG;
+    if (ctx->d_K64 && ctx->etype != ASEND_BRICK8)
+        return fail(ctx, ASEND_ERR_UNSUPPORTED, "the learned preconditioner is defined on brick8 meshes");
     if (!ctx->learned) return fail(ctx, ASEND_ERR_STATE, "no learned preconditioner (asend_learned_init)");
     if (!ctx->assembled) return fail(ctx, ASEND_ERR_STATE, "assemble first");
     invalidate_factor(ctx);

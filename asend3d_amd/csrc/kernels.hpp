@@ -33,6 +33,12 @@ __host__ __device__ inline int64_t sell_index(const int32_t* sptr, int r, int k,
 void launch_hex8_assemble(hipStream_t st, const int32_t* elems, int n_elems, const int32_t* conn_r,
                           const double* xyz_r, const double* E, const double* nu, const double* rho, double c_mass,
                           const int32_t* sptr, const uint8_t* slot, double* K);
+// shell4: adds k_scale K_e + m_scale M_e; conn_p holds the 8 pseudo-node rows
+// (2 per node: translations, rotations) of each element, xyz_r is per node
+void launch_shell4_assemble(hipStream_t st, const int32_t* elems, int n_elems, const int32_t* conn_p,
+                            const double* xyz_r, const double* E, const double* nu, const double* thk,
+                            const double* rho, double k_scale, double m_scale, const int32_t* sptr,
+                            const uint8_t* slot, double* K);
 void launch_maxabs(hipStream_t st, const double* v, int64_t n, double* partials, Scalars* sc);
 void launch_penalty_diag(hipStream_t st, const int64_t* pen, const int32_t* mult, int n_pen, const int32_t* sptr,
                          const int32_t* kdiag, double* K, const Scalars* sc);

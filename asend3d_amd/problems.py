@@ -82,6 +82,13 @@ def upload(ctx, prob, rho=None):
     ctx.set_penalty(prob.pen_dofs, 0.0)
 
 
+def upload_shell(ctx, prob):
+    """Shell mesh, thickness and clamps onto a device context."""
+    ctx.upload_mesh(prob.etype, prob.mesh.xyz, prob.dof_index, prob.n_dof, prob.mesh.conn, prob.E, prob.nu, prob.rho)
+    ctx.set_thickness(prob.t)
+    ctx.set_penalty(prob.pen_dofs, 0.0)
+
+
 @dataclass
 class BatchVariant:
     """One design variant of a batch: the cantilever problem plus its load

@@ -7,9 +7,13 @@
               Jacobi PCG.
   newmark  -- Newmark time stepping of a cantilever with a half-sine tip
               pulse, each step replayed as a hipGraph.
+  shell    -- Newmark time stepping of a clamped shell4 plate (1 x 1 m,
+              256 x 256 elements, t = 5 mm, aluminium) under a 1 ms half-sine
+              10 kN centre load, dt = 1e-5 s, 1000 steps, hipGraph steps.
 
     python -m asend3d_amd.workloads batched [--variants 64] [--sample 2]
     python -m asend3d_amd.workloads newmark [--nx 100 --ny 20 --nz 20 --steps 1000]
+    python -m asend3d_amd.workloads shell [--nx 256 --ny 256 --steps 1000]
 
 Each prints one JSON line.
 """
@@ -97,21 +101,53 @@ def newmark(nx=100, ny=20, nz=20, steps=1000, dt=1e-5, graphs=True, tol=1e-8, ma
             "tip_w_min": float(hist.mean(axis=1).min())}
 
 
+def shell(nx=256, ny=256, steps=1000, dt=1e-5, graphs=True, tol=1e-8, max_it=2000, load=1.0e4):
+    p = problems.shell_plate(nx, ny, 1.0, 1.0, 0.005, 70e9, 0.33, 2700.0)
+    ctx = api.Context(0)
+    problems.upload_shell(ctx, p)
+    ctx.set_graphs(graphs)
+    t0 = time.perf_counter()
+    ctx.newmark_setup(dt, 0.25, 0.5)
+    log(f"shell {nx}x{ny}: {p.n_dof} DOF, assembly of M and K + c_m M {time.perf_counter() - t0:.2f} s")
+    di = np.asarray(p.dof_index).reshape(-1, 6)
+    centre = p.mesh.node_id(nx // 2, ny // 2)
+    f = np.zeros(p.n_dof)
+    f[di[centre, 2]] = load
+    n_pulse = int(round(1e-3 / dt))
+    fac = np.where(np.arange(steps) < n_pulse, np.sin(np.pi * np.arange(steps) / n_pulse), 0.0)
+    watch = di[centre, :3]
+    ctx.newmark_run(min(steps, 20), f, fac, watch=watch, tol=tol, max_it_step=max_it)  # warm-up
+    log("warm-up done")
+    hist, u, v, a, st = ctx.newmark_run(steps, f, fac, watch=watch, tol=tol, max_it_step=max_it)
+    ctx.close()
+    w = hist[:, 2]
+    return {"workload": "shell", "mesh": f"{nx}x{ny} shell4 plate, edges clamped", "dof": int(p.n_dof),
+            "steps": steps, "dt": dt, "load": f"{load} N half-sine over {n_pulse} steps at the centre",
+            "device_s": st.device_ms / 1e3, "us_per_step": st.device_ms * 1e3 / steps,
+            "dof_steps_per_s": p.n_dof * steps / (st.device_ms / 1e3), "pcg_iterations": int(st.total_iterations),
+            "max_step_iterations": int(st.max_step_iterations), "unconverged_steps": int(st.unconverged_steps),
+            "graph": bool(st.graph_used), "step_tol": tol, "max_it_step": max_it,
+            "centre_w_min": float(w.min()), "centre_w_max": float(w.max()),
+            "centre_w_min_step": int(np.argmin(w))}
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("which", choices=["batched", "newmark"])
+    ap.add_argument("which", choices=["batched", "newmark", "shell"])
     ap.add_argument("--variants", type=int, default=64)
     ap.add_argument("--sample", type=int, default=2)
-    ap.add_argument("--nx", type=int, default=100)
-    ap.add_argument("--ny", type=int, default=20)
+    ap.add_argument("--nx", type=int, default=None, help="default 100 (newmark), 256 (shell)")
+    ap.add_argument("--ny", type=int, default=None, help="default 20 (newmark), 256 (shell)")
     ap.add_argument("--nz", type=int, default=20)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--no-graphs", action="store_true")
     a = ap.parse_args()
     if a.which == "batched":
         out = batched(a.variants, a.sample)
+    elif a.which == "shell":
+        out = shell(a.nx or 256, a.ny or 256, a.steps, graphs=not a.no_graphs)
     else:
-        out = newmark(a.nx, a.ny, a.nz, a.steps, graphs=not a.no_graphs)
+        out = newmark(a.nx or 100, a.ny or 20, a.nz, a.steps, graphs=not a.no_graphs)
     print(json.dumps(out), flush=True)
 
 

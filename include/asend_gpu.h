@@ -136,10 +136,22 @@ int asend_lognormal_moduli(uint64_t seed, int64_t n, double median, double sigma
  * (S/model_prep.rs:881-952, S/model_analysis.rs:179-201).  E, nu, rho are per
  * element (isotropic; G = E/(2(1+nu)) as P/model/Material.py:24-26 writes
  * it).  rho may be NULL for static work.  Builds the node-block sparsity
- * pattern, the element colouring and the element-to-block map. */
+ * pattern, the element colouring and the element-to-block map.
+ * elem_type ASEND_BRICK8: conn holds 8 nodes per element, n_dof = 3 n_nodes.
+ * elem_type ASEND_SHELL4: conn holds 4 nodes per element, n_dof = 6 n_nodes
+ * (dof_index[6n + c] = 6 rank + c), one isotropic layer per element whose
+ * thickness asend_set_thickness supplies; every element must lie in a plane
+ * normal to z (the identity element frame).  A new mesh clears the penalty
+ * set (asend_set_penalty numbers DOFs on the mesh it is called for). */
 int asend_upload_mesh(asend_ctx* ctx, int elem_type, int64_t n_nodes, const double* xyz,
                       const int64_t* dof_index, int64_t n_dof, int64_t n_el, const int32_t* conn,
                       const double* E, const double* nu, const double* rho);
+
+/* Per-element thickness of a shell4 mesh: the single layer of the shell
+ * section (Layer.thickness, S/section.rs:93, read by get_abd_dfd0,
+ * S/element_properties.rs:766-876), centred on the mid-plane.  Required
+ * before the first assembly of a shell4 mesh. */
+int asend_set_thickness(asend_ctx* ctx, const double* t);
 
 /* Optional structured-grid description (element (i,j,k) is label
  * i + nx*(j + ny*k), nodes likewise with nx+1, ny+1), used by the multigrid
