@@ -1,0 +1,84 @@
+"""Per-kernel summaries of a rocprofv3 kernel trace (`--kernel-trace
+--output-format csv`), optionally restricted to a time window delimited by
+launches of named kernels.
+
+    python -m asend3d_amd.profiling TRACE.csv [--after NAME:K] [--until NAME:K] [--top 25]
+
+--after NAME:K starts the window at the end of the K-th (1-based) launch of
+the kernel whose name contains NAME; --until NAME:K ends it at the end of the
+K-th launch.  Prints a markdown table: kernel, launches, total, average and
+share of the window's kernel time, plus the window's span.
+"""
+import argparse
+import csv
+import sys
+from collections import defaultdict
+
+
+def load(path):
+    rows = []
+    with open(path, newline="") as fh:
+        for r in csv.DictReader(fh):
+            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]))
+    rows.sort()
+    return rows
+
+
+def nth_end(rows, name, k):
+    seen = 0
+    for s, e, n in rows:
+        if name in n:
+            seen += 1
+            if seen == k:
+                return e
+    raise SystemExit(f"kernel containing {name!r} launched {seen} times, fewer than {k}")
+
+
+def short(name, width=90):
+    """Kernel name without return type, namespaces' anonymous part and
+    argument list."""
+    n = name.replace("(anonymous namespace)::", "")
+    n = n.split("(")[0]
+    if n.startswith("void "):
+        n = n[5:]
+    return n if len(n) <= width else n[:width - 3] + "..."
+
+
+def summarize(rows, t0=None, t1=None, top=25):
+    sel = [(s, e, n) for s, e, n in rows if (t0 is None or s >= t0) and (t1 is None or e <= t1)]
+    if not sel:
+        raise SystemExit("no kernels in the window")
+    agg = defaultdict(lambda: [0, 0])
+    for s, e, n in sel:
+        agg[n][0] += 1
+        agg[n][1] += e - s
+    busy = sum(v[1] for v in agg.values())
+    span = max(e for _, e, _ in sel) - min(s for s, _, _ in sel)
+    out = [f"window: {len(sel)} launches, kernel time {busy / 1e6:.3f} ms, span {span / 1e6:.3f} ms "
+           f"(first launch to last end)", "",
+           "| kernel | launches | total ms | avg us | share |", "|---|---:|---:|---:|---:|"]
+    for n, (c, t) in sorted(agg.items(), key=lambda kv: -kv[1][1])[:top]:
+        out.append(f"| `{short(n)}` | {c} | {t / 1e6:.3f} | {t / c / 1e3:.2f} | {100.0 * t / busy:.1f}% |")
+    return "\n".join(out)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("trace")
+    ap.add_argument("--after", default=None)
+    ap.add_argument("--until", default=None)
+    ap.add_argument("--top", type=int, default=25)
+    a = ap.parse_args()
+    rows = load(a.trace)
+    t0 = t1 = None
+    if a.after:
+        name, k = a.after.rsplit(":", 1)
+        t0 = nth_end(rows, name, int(k))
+    if a.until:
+        name, k = a.until.rsplit(":", 1)
+        t1 = nth_end(rows, name, int(k))
+    print(summarize(rows, t0, t1, a.top))
+
+
+if __name__ == "__main__":
+    sys.exit(main())

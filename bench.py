@@ -112,6 +112,8 @@ def main():
     ap.add_argument("--fast-precond", default="learned", choices=["jacobi", "mg", "learned"])
     ap.add_argument("--train-meshes", type=int, default=8)
     ap.add_argument("--train-steps", type=int, default=250)
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch the captured iteration blocks eagerly (kernel-trace profiling runs)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="assemble, then time only the SpMV and copy kernels (short run for counter profiling)")
     args = ap.parse_args()
@@ -127,6 +129,8 @@ def main():
     log(f"prepared {args.nx}x{args.ny}x{args.nz}: {prob.n_dof} DOF, {prob.mesh.n_el} elements "
         f"({time.perf_counter() - t0:.1f} s host prep)")
     ctx = api.Context(0)
+    if args.no_graphs:
+        ctx.set_graphs(False)
     problems.upload(ctx, prob)
     if args.kernels_only:
         ctx.assemble()

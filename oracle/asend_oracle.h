@@ -29,7 +29,11 @@ extern "C" {
  * terms [cterm_ptr[c], cterm_ptr[c+1]); term t's node set is
  * set_nodes[term_set_ptr[t] .. term_set_ptr[t+1]).
  * Outputs: ordered[n_nodes] (insertion order), dof_index[6*n_nodes],
- * n_dof_out (= el_mat_dim). */
+ * n_dof_out (= el_mat_dim).
+ * Parity unpinned: the reference ships no node orderings and its Rust cannot
+ * run here; tests pin only invariants (a permutation, a least-connected start
+ * node, contiguous DOF blocks, restart blocks of block_dim), not the exact
+ * order. */
 int orc_reorder_nodes(int64_t n_nodes, const double* xyz,
                       int64_t n_el, int nn, int dof_per_nd, const int32_t* conn,
                       int64_t n_const, const int64_t* cterm_ptr,
