@@ -105,9 +105,9 @@ _lib.asend_ctx_destroy.argtypes = [_vp]
 EXPORTED = ["asend_ctx_create", "asend_ctx_destroy", "asend_last_error", "asend_version", "asend_sync",
             "asend_reorder_nodes", "asend_block_dim_auto", "asend_surface_traction", "asend_lognormal_moduli",
             "asend_upload_mesh", "asend_set_thickness", "asend_set_grid", "asend_update_moduli", "asend_set_penalty",
-            "asend_penalty_scale", "asend_assemble", "asend_get_matrix", "asend_spmv", "asend_solve",
+            "asend_penalty_scale", "asend_uniform_elements", "asend_assemble", "asend_get_matrix", "asend_spmv", "asend_solve",
             "asend_element_sens", "asend_static_compliance", "asend_bench_spmv", "asend_mg_info", "asend_mg_apply",
-            "asend_mg_restrict", "asend_mg_prolong", "asend_mg_vcycle", "asend_bench_copy",
+            "asend_mg_restrict", "asend_mg_prolong", "asend_mg_vcycle", "asend_mg_coarse_solve", "asend_bench_copy",
             "asend_learned_num_params", "asend_learned_init", "asend_learned_get_params",
             "asend_learned_set_params", "asend_learned_train", "asend_learned_eval", "asend_learned_apply",
             "asend_learned_edges", "asend_mg_set_smoother", "asend_learned_vcycle_eval", "asend_set_graphs",
@@ -257,6 +257,9 @@ class Context:
         d = _i64a(dofs)
         self._chk(_lib.asend_set_penalty(self.h, _i64(d.shape[0]), _p(d), _dbl(sf)), "asend_set_penalty")
 
+    def uniform_elements(self):
+        return bool(_lib.asend_uniform_elements(self.h))
+
     def penalty_scale(self):
         return float(_lib.asend_penalty_scale(self.h))
 
@@ -350,6 +353,12 @@ class Context:
         b = np.ascontiguousarray(b, dtype=np.float32)
         x = np.zeros(b.shape[0], dtype=np.float32)
         self._chk(_lib.asend_mg_vcycle(self.h, _p(b), _p(x)), "asend_mg_vcycle")
+        return x
+
+    def mg_coarse_solve(self, b):
+        b = np.ascontiguousarray(b, dtype=np.float32)
+        x = np.zeros(b.shape[0], dtype=np.float32)
+        self._chk(_lib.asend_mg_coarse_solve(self.h, _p(b), _p(x)), "asend_mg_coarse_solve")
         return x
 
     def newmark_setup(self, dt, beta=0.25, gamma=0.5):

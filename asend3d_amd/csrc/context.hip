@@ -56,6 +56,10 @@ struct asend_ctx {
     // two block rows (translations 2r, rotations 2r+1) of the operator
     double* d_thk = nullptr;
     bool thk_set = false;
+    // brick8 meshes of translated copies of one element with one Poisson
+    // ratio: reference blocks Khat (E = 1) and Mhat (rho = 1) of that element
+    bool uniform_hex = false;
+    double *d_khat = nullptr, *d_mhat = nullptr;
     double* d_K64 = nullptr;
     float* d_K32 = nullptr;
     double* d_dinv64 = nullptr;
@@ -91,6 +95,7 @@ struct asend_ctx {
     int n_edge = 0;
     double mean_log_e = 0.0;
     int64_t asm_count = 0, factor_asm = -1;
+    int64_t mg_asm = -1;  // assembly the multigrid hierarchy was last set up for
     double asm_cmass = 0.0;  // mass coefficient of the current assembly
     bool use_graphs = true;
     // autotuned choices: SpMV workgroup geometry, inner tolerance and check
@@ -220,6 +225,9 @@ void free_mesh(asend_ctx* ctx) {
     ctx->sf_fixed = false;
     dfree(ctx->d_thk);
     ctx->thk_set = false;
+    dfree(ctx->d_khat);
+    dfree(ctx->d_mhat);
+    ctx->uniform_hex = false;
     dfree(ctx->d_K64);
     dfree(ctx->d_K32);
     dfree(ctx->d_dinv64);
@@ -351,6 +359,20 @@ void invalidate_factor(asend_ctx* ctx) {
     if (ctx->mg) mg_clear_learned_factor(ctx->mg);
 }
 
+// Multigrid setup (coarse operators, smoother bounds, coarsest inverse) for
+// the current assembly, on first use: a solve that never asks for the
+// hierarchy does not pay for it.
+int ensure_mg(asend_ctx* ctx) {
+    if (!ctx->mg) return fail(ctx, ASEND_ERR_STATE, "multigrid preconditioner not built (asend_set_grid)");
+    if (!ctx->assembled) return fail(ctx, ASEND_ERR_STATE, "assemble first");
+    if (ctx->mg_asm == ctx->asm_count && mg_ready(ctx->mg)) return ASEND_OK;
+    int rc = mg_setup(ctx->mg, ctx->st, ctx->d_E, ctx->d_nu, ctx->d_K32, ctx->d_dinv32, ctx->d_kdiag, ctx->sf,
+                      ctx->n_pen, ctx->h_pen.data(), ctx->h_pen_mult.data());
+    if (rc) return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
+    ctx->mg_asm = ctx->asm_count;
+    return ASEND_OK;
+}
+
 // Network inference for the current operator (once per assembly) and
 // registration of the factor with the multigrid object; the Chebyshev bounds
 // lambda_max(P A) are estimated when the factor is used as the smoother.
@@ -441,8 +463,8 @@ int solve_device(asend_ctx* ctx, const asend_solve_opts* o, asend_solve_stats* s
         const bool learned_pc = (o->precond == ASEND_PC_LEARNED || o->precond == ASEND_PC_LEARNED_PCG);
         bool use_learned = learned_pc && !s.fallback_used;
         if (learned_pc) {
-            if (!ctx->mg) return fail(ctx, ASEND_ERR_STATE, "multigrid preconditioner not built (asend_set_grid)");
             if (!ctx->learned) return fail(ctx, ASEND_ERR_STATE, "no learned preconditioner (asend_learned_init)");
+            if ((rc = ensure_mg(ctx))) return rc;
         }
         if (use_learned) {
             // learned factor as the multigrid smoother (PC_LEARNED) or alone
@@ -477,8 +499,8 @@ int solve_device(asend_ctx* ctx, const asend_solve_opts* o, asend_solve_stats* s
                 return rc;
             it_in = ctx->h_sc[1].it;
         } else {
-            if (!ctx->mg) return fail(ctx, ASEND_ERR_STATE, "multigrid preconditioner not built (asend_set_grid)");
             if (o->precond != ASEND_PC_MG) return fail(ctx, ASEND_ERR_ARG, "unknown preconditioner %d", o->precond);
+            if ((rc = ensure_mg(ctx))) return rc;
             mg_set_fine_smoother(ctx->mg, 0);
             int fb = 0;
             if ((rc = mg_fcg_solve(ctx->mg, ctx->st, ctx->d_r32, ctx->d_x32, inner_tol, max_it - total_it, check,
@@ -499,20 +521,28 @@ int solve_device(asend_ctx* ctx, const asend_solve_opts* o, asend_solve_stats* s
     return ASEND_OK;
 }
 
-// Element integration into K (cleared first): k_scale K_e(E) + c_mass M_e
-// of every element, one launch per element colour.  The brick kernel has no
-// stiffness scale (k_scale must be 1; pass zero moduli for the mass alone).
+// Element integration into K: k_scale K_e(E) + c_mass M_e of every element.
+// brick8: one row-wise gather launch that writes every value of K (the brick
+// kernel has no stiffness scale: k_scale must be 1; pass zero moduli for the
+// mass alone).  shell4: K is cleared, then one scatter launch per element
+// colour.
 int integrate_elements(asend_ctx* ctx, const double* E, double k_scale, double c_mass, double* K) {
-    HIPCHK(hipMemsetAsync(K, 0, ctx->nval() * sizeof(double), ctx->st));
-    const int ncol = (int)ctx->h_color_ptr.size() - 1;
-    for (int c = 0; c < ncol; ++c) {
-        const int b = ctx->h_color_ptr[c], e = ctx->h_color_ptr[c + 1];
-        if (ctx->etype == ASEND_SHELL4)
+    if (ctx->etype == ASEND_SHELL4) {
+        HIPCHK(hipMemsetAsync(K, 0, ctx->nval() * sizeof(double), ctx->st));
+        const int ncol = (int)ctx->h_color_ptr.size() - 1;
+        for (int c = 0; c < ncol; ++c) {
+            const int b = ctx->h_color_ptr[c], e = ctx->h_color_ptr[c + 1];
             launch_shell4_assemble(ctx->st, ctx->d_elems + b, e - b, ctx->d_conn_r, ctx->d_xyz_r, E, ctx->d_nu,
                                    ctx->d_thk, ctx->d_rho, k_scale, c_mass, ctx->d_sptr, ctx->d_slot, K);
-        else
-            launch_hex8_assemble(ctx->st, ctx->d_elems + b, e - b, ctx->d_conn_r, ctx->d_xyz_r, E, ctx->d_nu,
-                                 ctx->d_rho, c_mass, ctx->d_sptr, ctx->d_slot, K);
+        }
+    } else if (ctx->uniform_hex) {
+        launch_hex8_assemble_uniform(ctx->st, (int)ctx->n_nodes, (int)ctx->n_slices, ctx->d_sptr, ctx->d_scol,
+                                     ctx->d_rp, ctx->d_n2e_ptr, ctx->d_n2e, ctx->d_conn_r, ctx->d_khat, ctx->d_mhat, E,
+                                     ctx->d_rho, c_mass, K);
+    } else {
+        launch_hex8_assemble_rows(ctx->st, (int)ctx->n_nodes, (int)ctx->n_slices, ctx->d_sptr, ctx->d_scol, ctx->d_rp,
+                                  ctx->d_n2e_ptr, ctx->d_n2e, ctx->d_conn_r, ctx->d_xyz_r, E, ctx->d_nu, ctx->d_rho,
+                                  c_mass, K);
     }
     HIPCHK(hipGetLastError());
     return ASEND_OK;
@@ -550,11 +580,6 @@ int assemble_device(asend_ctx* ctx, double c_mass) {
     ctx->assembled = true;
     ctx->asm_count += 1;
     ctx->asm_cmass = c_mass;
-    if (ctx->mg) {
-        if ((rc = mg_setup(ctx->mg, ctx->st, ctx->d_E, ctx->d_nu, ctx->d_K32, ctx->d_dinv32, ctx->d_kdiag, ctx->sf,
-                           ctx->n_pen, ctx->h_pen.data(), ctx->h_pen_mult.data())))
-            return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
-    }
     return ASEND_OK;
 }
 
@@ -698,6 +723,32 @@ int asend_upload_mesh(asend_ctx* ctx, int elem_type, int64_t n_mesh_nodes, const
         std::vector<int32_t> fill(n2e_ptr.begin(), n2e_ptr.end() - 1);
         for (int64_t e = 0; e < n_el; ++e)
             for (int a = 0; a < 8; ++a) n2e[fill[conn_r[8 * e + a]]++] = (int32_t)e;
+    }
+    // brick8 meshes whose elements are all translates of the first one (to
+    // 1e-12 of the mesh extent) with one Poisson ratio and at most 8 elements
+    // per node assemble from scaled reference blocks
+    bool uniform = !shell;
+    if (uniform) {
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int64_t nd = 0; nd < n_mesh_nodes; ++nd)
+            for (int c = 0; c < 3; ++c) {
+                lo[c] = std::min(lo[c], xyz[3 * nd + c]);
+                hi[c] = std::max(hi[c], xyz[3 * nd + c]);
+            }
+        const double tol = 1e-12 * std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-300});
+        double ref[8][3];
+        for (int t = 0; t < 8; ++t)
+            for (int c = 0; c < 3; ++c) ref[t][c] = xyz[3 * (int64_t)conn[t] + c] - xyz[3 * (int64_t)conn[0] + c];
+        for (int64_t e = 0; e < n_el && uniform; ++e) {
+            if (nu[e] != nu[0]) uniform = false;
+            for (int t = 0; t < 8 && uniform; ++t)
+                for (int c = 0; c < 3; ++c)
+                    if (std::fabs(xyz[3 * (int64_t)conn[8 * e + t] + c] - xyz[3 * (int64_t)conn[8 * e] + c] - ref[t][c]) >
+                        tol)
+                        uniform = false;
+        }
+        for (int64_t r = 0; r < n_nodes && uniform; ++r)
+            if (n2e_ptr[r + 1] - n2e_ptr[r] > 8) uniform = false;
     }
     // block rows: sorted unique neighbour ranks
     ctx->h_rp.assign(n_nodes + 1, 0);
@@ -859,6 +910,13 @@ int asend_upload_mesh(asend_ctx* ctx, int elem_type, int64_t n_mesh_nodes, const
     HIPCHK(hipMemcpy(ctx->d_nu, nu, n_el * 8, hipMemcpyHostToDevice));
     if (rho) HIPCHK(hipMemcpy(ctx->d_rho, rho, n_el * 8, hipMemcpyHostToDevice));
     else HIPCHK(hipMemset(ctx->d_rho, 0, n_el * 8));
+    if (uniform) {
+        if ((rc = dalloc(ctx, &ctx->d_khat, 64 * 9)) || (rc = dalloc(ctx, &ctx->d_mhat, 64))) return rc;
+        launch_hex8_reference(ctx->st, ctx->d_conn_r, ctx->d_xyz_r, nu[0], ctx->d_khat, ctx->d_mhat);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(ctx->st));
+        ctx->uniform_hex = true;
+    }
     if ((rc = ensure_work(ctx))) return rc;
     return ASEND_OK;
 }
@@ -968,6 +1026,8 @@ int asend_set_penalty(asend_ctx* ctx, int64_t n, const int64_t* dofs, double sf)
 }
 
 double asend_penalty_scale(const asend_ctx* ctx) { return ctx ? ctx->sf : 0.0; }
+
+int asend_uniform_elements(const asend_ctx* ctx) { return (ctx && ctx->uniform_hex) ? 1 : 0; }
 
 int asend_assemble(asend_ctx* ctx, double c_mass) {
     if (!ctx) return ASEND_ERR_ARG;
@@ -1466,6 +1526,8 @@ int asend_learned_train(asend_ctx* ctx, int n_steps, float lr, uint64_t seed, in
     if (!ctx->learned) return fail(ctx, ASEND_ERR_STATE, "no learned preconditioner (asend_learned_init)");
     if (!ctx->assembled) return fail(ctx, ASEND_ERR_STATE, "assemble first");
     if (mode > 0 && !ctx->mg) return fail(ctx, ASEND_ERR_STATE, "this training mode needs the multigrid hierarchy");
+    int rc0;
+    if (mode > 0 && (rc0 = ensure_mg(ctx))) return rc0;
     invalidate_factor(ctx);
     const LearnedGraph g = learned_graph(ctx);
     const int64_t n = ctx->n_dof;
@@ -1546,6 +1608,10 @@ int asend_learned_vcycle_eval(asend_ctx* ctx, const double* e, int keep_bounds, 
         return fail(ctx, ASEND_ERR_UNSUPPORTED, "the learned preconditioner is defined on brick8 meshes");
     if (!ctx->learned) return fail(ctx, ASEND_ERR_STATE, "no learned preconditioner (asend_learned_init)");
     if (!ctx->assembled || !ctx->mg) return fail(ctx, ASEND_ERR_STATE, "assemble with a multigrid hierarchy first");
+    {
+        int rc0 = ensure_mg(ctx);
+        if (rc0) return rc0;
+    }
     const LearnedGraph g = learned_graph(ctx);
     const int64_t n = ctx->n_dof;
     std::vector<float> ef(e, e + n);
@@ -1617,14 +1683,20 @@ static int mg_io(asend_ctx* ctx, int op, int level, const float* in, float* out)
     if (!ctx || !in || !out) return ASEND_ERR_ARG;
     if (!ctx->mg) return fail(ctx, ASEND_ERR_STATE, "no multigrid hierarchy (asend_set_grid)");
     if (!ctx->assembled) return fail(ctx, ASEND_ERR_STATE, "assemble first");
+    {
+        int rc0 = ensure_mg(ctx);
+        if (rc0) return rc0;
+    }
     int nl = mg_levels(ctx->mg);
-    if (level < 0 || level >= nl || (op >= 1 && level + 1 >= nl))
+    if (op == 4) level = nl - 1;
+    if (level < 0 || level >= nl || (op >= 1 && op <= 3 && level + 1 >= nl))
         return fail(ctx, ASEND_ERR_ARG, "bad level %d", level);
     int64_t nin = 0, nout = 0;
     if (op == 0) nin = nout = mg_level_ndof(ctx->mg, level);
     if (op == 1) { nin = mg_level_ndof(ctx->mg, level); nout = mg_level_ndof(ctx->mg, level + 1); }
     if (op == 2) { nin = mg_level_ndof(ctx->mg, level + 1); nout = mg_level_ndof(ctx->mg, level); }
     if (op == 3) nin = nout = mg_level_ndof(ctx->mg, 0);
+    if (op == 4) nin = nout = mg_level_ndof(ctx->mg, level);
     float *din = nullptr, *dout = nullptr;
     HIPCHK(hipMalloc((void**)&din, nin * sizeof(float)));
     HIPCHK(hipMalloc((void**)&dout, nout * sizeof(float)));
@@ -1634,6 +1706,7 @@ static int mg_io(asend_ctx* ctx, int op, int level, const float* in, float* out)
     if (op == 1) rc = mg_restrict(ctx->mg, ctx->st, level, din, dout);
     if (op == 2) rc = mg_prolong(ctx->mg, ctx->st, level, din, dout);
     if (op == 3) rc = mg_vcycle(ctx->mg, ctx->st, din, dout);
+    if (op == 4) rc = mg_coarse_solve(ctx->mg, ctx->st, din, dout);
     hipError_t e1 = hipStreamSynchronize(ctx->st);
     hipError_t e2 = hipMemcpy(out, dout, nout * sizeof(float), hipMemcpyDeviceToHost);
     (void)hipFree(din);
@@ -1651,6 +1724,7 @@ int asend_mg_prolong(asend_ctx* ctx, int level, const float* coarse, float* fine
     return mg_io(ctx, 2, level, coarse, fine);
 }
 int asend_mg_vcycle(asend_ctx* ctx, const float* b, float* x) { return mg_io(ctx, 3, 0, b, x); }
+int asend_mg_coarse_solve(asend_ctx* ctx, const float* b, float* x) { return mg_io(ctx, 4, 0, b, x); }
 
 // ---------------------------------------------------------------------------
 // Autotuner
@@ -1698,7 +1772,7 @@ int asend_autotune(asend_ctx* ctx, const double* rhs, int what, asend_tuning* ou
         for (int d : degrees) {
             if ((rc = mg_set_smoother(ctx->mg, 1, d, 0.1, 1.15)) || (rc = mg_refresh_coefficients(ctx->mg, ctx->st)))
                 return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
-            if (!mg_ready(ctx->mg) && (rc = assemble_device(ctx, ctx->asm_cmass))) return rc;
+            if ((rc = ensure_mg(ctx))) return rc;
             for (double t : tols)
                 for (int c : checks) {
                     asend_solve_opts o;
@@ -1751,7 +1825,6 @@ int asend_mg_set_smoother(asend_ctx* ctx, int kind, int degree, double lo_frac, 
     if (!ctx->mg) return fail(ctx, ASEND_ERR_STATE, "no multigrid hierarchy (asend_set_grid)");
     int rc = mg_set_smoother(ctx->mg, kind, degree, lo_frac, hi_frac);
     if (rc) return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
-    if (!mg_ready(ctx->mg)) ctx->assembled = false;
     return ASEND_OK;
 }
 

@@ -15,118 +15,319 @@
 namespace asend {
 
 // ---------------------------------------------------------------------------
-// brick8 element integration + colored assembly (FP64).
-// One 64-lane wavefront per element; lane (a, b) owns the 3x3 block K_ab.
-// Gauss-point geometry (J^-1, det J, physical gradients) is computed once
-// per element and shared through LDS.  Elements of one colour share no node,
-// so the read-modify-write of each block is race-free without atomics.
+// brick8 element integration + assembly (FP64).
 //   K_ab,ij = sum_gp w detJ [lam g_a,i g_b,j + mu g_a,j g_b,i + mu (g_a.g_b) d_ij]
 // is B_a^T C B_b for isotropic C (Lame lam, mu), the same operator as
-// get_ruk_dfd0's B^T C B (S/element_equations.rs:398-411).
+// get_ruk_dfd0's B^T C B (S/element_equations.rs:398-411), with g_a the
+// physical gradient of shape function a at Gauss point gp (8 points,
+// +-1/sqrt(3), weight 1).
 // ---------------------------------------------------------------------------
 __constant__ double kHexNdSign[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
                                         {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
 
-__global__ __launch_bounds__(64) void k_hex8_assemble(const int32_t* __restrict__ elems, int n_elems,
-                                                      const int32_t* __restrict__ conn_r,
-                                                      const double* __restrict__ xyz_r,
-                                                      const double* __restrict__ E, const double* __restrict__ nu,
-                                                      const double* __restrict__ rho, double c_mass,
-                                                      const int32_t* __restrict__ sptr,
-                                                      const uint8_t* __restrict__ slot, double* __restrict__ K) {
-    const int lane = threadIdx.x;
-    const int ei = blockIdx.x;
-    if (ei >= n_elems) return;
-    const int e = elems[ei];
-    __shared__ double X[8][3];
-    __shared__ double Ji[8][9];
-    __shared__ double W[8];
-    __shared__ double Gs[8][8][3];
-    __shared__ double Ns[8][8];
-    if (lane < 24) {
-        int a = lane / 3, c = lane % 3;
-        X[a][c] = xyz_r[3 * (int64_t)conn_r[8 * (int64_t)e + a] + c];
-    }
-    __syncthreads();
+// natural derivatives dN_t/ds_c of the brick8 shape functions at the 8 Gauss
+// points (+-1/sqrt(3), point gp has s_c = +g when bit c of gp is set)
+struct HexDN {
+    double d[8][8][3];
+};
+constexpr HexDN make_hex_dn() {
+    HexDN h{};
     const double g = 0.5773502691896257;
-    if (lane < 8) {
-        const int gp = lane;
-        double s0 = (gp & 1) ? g : -g, s1 = (gp & 2) ? g : -g, s2 = (gp & 4) ? g : -g;
-        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-        for (int a = 0; a < 8; ++a) {
-            double fx = 1.0 + kHexNdSign[a][0] * s0, fy = 1.0 + kHexNdSign[a][1] * s1,
-                   fz = 1.0 + kHexNdSign[a][2] * s2;
-            double d0 = 0.125 * kHexNdSign[a][0] * fy * fz, d1 = 0.125 * fx * kHexNdSign[a][1] * fz,
-                   d2 = 0.125 * fx * fy * kHexNdSign[a][2];
-            Ns[gp][a] = 0.125 * fx * fy * fz;
-            for (int i = 0; i < 3; ++i) {
-                J[i][0] += X[a][i] * d0;
-                J[i][1] += X[a][i] * d1;
-                J[i][2] += X[a][i] * d2;
-            }
-        }
-        double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-        double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-        double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-        double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
-        double id = 1.0 / det;
-        Ji[gp][0] = c00 * id;
-        Ji[gp][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
-        Ji[gp][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
-        Ji[gp][3] = c01 * id;
-        Ji[gp][4] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
-        Ji[gp][5] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
-        Ji[gp][6] = c02 * id;
-        Ji[gp][7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
-        Ji[gp][8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
-        W[gp] = det;  // Gauss weight 1
-    }
-    __syncthreads();
-    {
-        const int gp = lane / 8, a = lane % 8;
-        double s0 = (gp & 1) ? g : -g, s1 = (gp & 2) ? g : -g, s2 = (gp & 4) ? g : -g;
-        double fx = 1.0 + kHexNdSign[a][0] * s0, fy = 1.0 + kHexNdSign[a][1] * s1, fz = 1.0 + kHexNdSign[a][2] * s2;
-        double d0 = 0.125 * kHexNdSign[a][0] * fy * fz, d1 = 0.125 * fx * kHexNdSign[a][1] * fz,
-               d2 = 0.125 * fx * fy * kHexNdSign[a][2];
-        // dN/dx_k = sum_j dN/ds_j (J^-1)_jk
-        for (int k = 0; k < 3; ++k) Gs[gp][a][k] = d0 * Ji[gp][0 + k] + d1 * Ji[gp][3 + k] + d2 * Ji[gp][6 + k];
-    }
-    __syncthreads();
-    const int a = lane / 8, b = lane % 8;
-    const double Ee = E[e], ne = nu[e];
-    const double lam = Ee * ne / ((1.0 + ne) * (1.0 - 2.0 * ne));
-    const double mu = Ee / (2.0 * (1.0 + ne));
-    double k[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    double m = 0.0;
+    const double sg[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                             {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
     for (int gp = 0; gp < 8; ++gp) {
-        const double w = W[gp];
-        double ga[3] = {Gs[gp][a][0], Gs[gp][a][1], Gs[gp][a][2]};
-        double gb[3] = {Gs[gp][b][0], Gs[gp][b][1], Gs[gp][b][2]};
-        double dot = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j)
-                k[i][j] += w * (lam * ga[i] * gb[j] + mu * ga[j] * gb[i] + (i == j ? mu * dot : 0.0));
-        m += w * Ns[gp][a] * Ns[gp][b];
+        const double p0 = (gp & 1) ? g : -g, p1 = (gp & 2) ? g : -g, p2 = (gp & 4) ? g : -g;
+        for (int t = 0; t < 8; ++t) {
+            const double fx = 1.0 + sg[t][0] * p0, fy = 1.0 + sg[t][1] * p1, fz = 1.0 + sg[t][2] * p2;
+            h.d[gp][t][0] = 0.125 * sg[t][0] * fy * fz;
+            h.d[gp][t][1] = 0.125 * fx * sg[t][1] * fz;
+            h.d[gp][t][2] = 0.125 * fx * fy * sg[t][2];
+        }
     }
-    if (c_mass != 0.0) {
-        double cm = c_mass * rho[e] * m;
-        k[0][0] += cm;
-        k[1][1] += cm;
-        k[2][2] += cm;
-    }
-    const int ra = conn_r[8 * (int64_t)e + a];
-    const int kk = slot[64 * (int64_t)e + lane];
-    double* base = K + sell_index(sptr, ra, kk, 0);
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) base[64 * (3 * i + j)] += k[i][j];
+    return h;
+}
+__constant__ HexDN kHexDN = make_hex_dn();
+
+// node sign pattern of the brick8 reference element (kHexNdSign by index bits)
+__device__ __forceinline__ void hex_sign(int t, double& sx, double& sy, double& sz) {
+    sx = (((t ^ (t >> 1)) & 1) != 0) ? 1.0 : -1.0;
+    sy = (t & 2) ? 1.0 : -1.0;
+    sz = (t & 4) ? 1.0 : -1.0;
 }
 
-void launch_hex8_assemble(hipStream_t st, const int32_t* elems, int n_elems, const int32_t* conn_r,
-                          const double* xyz_r, const double* E, const double* nu, const double* rho, double c_mass,
-                          const int32_t* sptr, const uint8_t* slot, double* K) {
-    if (n_elems <= 0) return;
-    hipLaunchKernelGGL(k_hex8_assemble, dim3(n_elems), dim3(64), 0, st, elems, n_elems, conn_r, xyz_r, E, nu, rho,
-                       c_mass, sptr, slot, K);
+// Row-wise (gather) brick8 assembly: one lane per block row r, one wave per
+// 64-row slice of the sliced-ELL operator.  For each slot of the row (column
+// node c) the lane integrates, in registers, the 3x3 block K_e[a][b] (+ the
+// mass term) of every element e adjacent to r that also holds c (a, b = the
+// local indices of r and c in e) and sums them; the wave then stores the
+// slot's nine components of its 64 rows as nine contiguous 512 B writes.
+// Every value, padding included, is written once: no read-modify-write, no
+// colouring and no clearing pass.  The element geometry is recomputed per
+// (row, slot) pair, trading FP64 arithmetic for coalesced stores.
+__global__ __launch_bounds__(256, 3) void k_hex8_assemble_rows(int n_rows, int n_slices, const int32_t* __restrict__ sptr,
+                                                            const int32_t* __restrict__ scol,
+                                                            const int32_t* __restrict__ rp,
+                                                            const int32_t* __restrict__ n2e_ptr,
+                                                            const int32_t* __restrict__ n2e,
+                                                            const int32_t* __restrict__ conn_r,
+                                                            const double* __restrict__ xyz_r,
+                                                            const double* __restrict__ E,
+                                                            const double* __restrict__ nu,
+                                                            const double* __restrict__ rho, double c_mass,
+                                                            double* __restrict__ K) {
+    const int lane = threadIdx.x & 63;
+    const int s = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (s >= n_slices) return;
+    const int r = 64 * s + lane;
+    const bool live = r < n_rows;
+    const int s0 = sptr[s], w = sptr[s + 1] - s0;
+    const int nrow = live ? rp[r + 1] - rp[r] : 0;
+    const int q0 = live ? n2e_ptr[r] : 0, q1 = live ? n2e_ptr[r + 1] : 0;
+    const double g = 0.5773502691896257;
+    for (int k = 0; k < w; ++k) {
+        double acc[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) acc[c] = 0.0;
+        if (k < nrow) {
+            const int col = scol[64 * (int64_t)(s0 + k) + lane];
+            for (int q = q0; q < q1; ++q) {
+                const int e = n2e[q];
+                int nodes[8];
+                int a = -1, b = -1;
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    nodes[t] = conn_r[8 * (int64_t)e + t];
+                    if (nodes[t] == r) a = t;
+                    if (nodes[t] == col) b = t;
+                }
+                if (a < 0 || b < 0) continue;
+                double X[8][3];
+#pragma unroll
+                for (int t = 0; t < 8; ++t)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) X[t][c] = xyz_r[3 * (int64_t)nodes[t] + c];
+                const double Ee = E[e], ne = nu[e];
+                const double lam = Ee * ne / ((1.0 + ne) * (1.0 - 2.0 * ne));
+                const double mu = Ee / (2.0 * (1.0 + ne));
+                double sa[3], sb[3];
+                hex_sign(a, sa[0], sa[1], sa[2]);
+                hex_sign(b, sb[0], sb[1], sb[2]);
+                double kb[9];
+#pragma unroll
+                for (int c = 0; c < 9; ++c) kb[c] = 0.0;
+                double m = 0.0;
+#pragma unroll 1
+                for (int gp = 0; gp < 8; ++gp) {
+                    const double p0 = (gp & 1) ? g : -g, p1 = (gp & 2) ? g : -g, p2 = (gp & 4) ? g : -g;
+                    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+                    for (int t = 0; t < 8; ++t) {
+                        const double d0 = kHexDN.d[gp][t][0], d1 = kHexDN.d[gp][t][1], d2 = kHexDN.d[gp][t][2];
+#pragma unroll
+                        for (int i = 0; i < 3; ++i) {
+                            J[i][0] += X[t][i] * d0;
+                            J[i][1] += X[t][i] * d1;
+                            J[i][2] += X[t][i] * d2;
+                        }
+                    }
+                    const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+                    const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+                    const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+                    const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+                    const double id = 1.0 / det;
+                    const double Ji[9] = {c00 * id,
+                                          (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id,
+                                          (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id,
+                                          c01 * id,
+                                          (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id,
+                                          (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id,
+                                          c02 * id,
+                                          (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id,
+                                          (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id};
+                    const double fxa = 1.0 + sa[0] * p0, fya = 1.0 + sa[1] * p1, fza = 1.0 + sa[2] * p2;
+                    const double fxb = 1.0 + sb[0] * p0, fyb = 1.0 + sb[1] * p1, fzb = 1.0 + sb[2] * p2;
+                    const double da[3] = {0.125 * sa[0] * fya * fza, 0.125 * fxa * sa[1] * fza,
+                                          0.125 * fxa * fya * sa[2]};
+                    const double db[3] = {0.125 * sb[0] * fyb * fzb, 0.125 * fxb * sb[1] * fzb,
+                                          0.125 * fxb * fyb * sb[2]};
+                    double ga[3], gb[3];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        ga[c] = da[0] * Ji[0 + c] + da[1] * Ji[3 + c] + da[2] * Ji[6 + c];
+                        gb[c] = db[0] * Ji[0 + c] + db[1] * Ji[3 + c] + db[2] * Ji[6 + c];
+                    }
+                    const double dot = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+#pragma unroll
+                        for (int j = 0; j < 3; ++j)
+                            kb[3 * i + j] += det * (lam * ga[i] * gb[j] + mu * ga[j] * gb[i] + (i == j ? mu * dot : 0.0));
+                    m += det * (0.125 * fxa * fya * fza) * (0.125 * fxb * fyb * fzb);
+                }
+                if (c_mass != 0.0) {
+                    const double cm = c_mass * rho[e] * m;
+                    kb[0] += cm;
+                    kb[4] += cm;
+                    kb[8] += cm;
+                }
+#pragma unroll
+                for (int c = 0; c < 9; ++c) acc[c] += kb[c];
+            }
+        }
+        double* base = K + (int64_t)9 * 64 * (s0 + k) + lane;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) base[64 * c] = acc[c];
+    }
+}
+
+// Reference blocks of one brick8 element with E = 1 and rho = 1: lane (a, b)
+// writes Khat[9 (8a + b) + 3i + j] = K_ab,ij / E and Mhat[8a + b] =
+// sum_gp detJ N_a N_b, by the same arithmetic as the general kernel.  On a
+// mesh whose elements are all translates of this one with one Poisson ratio,
+// K_e = E_e Khat and M_e = rho_e Mhat exactly (up to rounding).
+__global__ __launch_bounds__(64) void k_hex8_reference(const int32_t* __restrict__ conn_r,
+                                                       const double* __restrict__ xyz_r, double nu,
+                                                       double* __restrict__ Khat, double* __restrict__ Mhat) {
+    const int lane = threadIdx.x;
+    const int a = lane / 8, b = lane % 8;
+    double X[8][3];
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) X[t][c] = xyz_r[3 * (int64_t)conn_r[t] + c];
+    const double lam = nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    const double mu = 1.0 / (2.0 * (1.0 + nu));
+    double sa[3], sb[3];
+    hex_sign(a, sa[0], sa[1], sa[2]);
+    hex_sign(b, sb[0], sb[1], sb[2]);
+    const double g = 0.5773502691896257;
+    double kb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double m = 0.0;
+    for (int gp = 0; gp < 8; ++gp) {
+        const double p0 = (gp & 1) ? g : -g, p1 = (gp & 2) ? g : -g, p2 = (gp & 4) ? g : -g;
+        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (int t = 0; t < 8; ++t) {
+            const double d0 = kHexDN.d[gp][t][0], d1 = kHexDN.d[gp][t][1], d2 = kHexDN.d[gp][t][2];
+            for (int i = 0; i < 3; ++i) {
+                J[i][0] += X[t][i] * d0;
+                J[i][1] += X[t][i] * d1;
+                J[i][2] += X[t][i] * d2;
+            }
+        }
+        const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+        const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+        const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+        const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+        const double id = 1.0 / det;
+        const double Ji[9] = {c00 * id, (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id,
+                              (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id, c01 * id,
+                              (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id, (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id,
+                              c02 * id, (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id,
+                              (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id};
+        const double fxa = 1.0 + sa[0] * p0, fya = 1.0 + sa[1] * p1, fza = 1.0 + sa[2] * p2;
+        const double fxb = 1.0 + sb[0] * p0, fyb = 1.0 + sb[1] * p1, fzb = 1.0 + sb[2] * p2;
+        const double da[3] = {0.125 * sa[0] * fya * fza, 0.125 * fxa * sa[1] * fza, 0.125 * fxa * fya * sa[2]};
+        const double db[3] = {0.125 * sb[0] * fyb * fzb, 0.125 * fxb * sb[1] * fzb, 0.125 * fxb * fyb * sb[2]};
+        double ga[3], gb[3];
+        for (int c = 0; c < 3; ++c) {
+            ga[c] = da[0] * Ji[0 + c] + da[1] * Ji[3 + c] + da[2] * Ji[6 + c];
+            gb[c] = db[0] * Ji[0 + c] + db[1] * Ji[3 + c] + db[2] * Ji[6 + c];
+        }
+        const double dot = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                kb[3 * i + j] += det * (lam * ga[i] * gb[j] + mu * ga[j] * gb[i] + (i == j ? mu * dot : 0.0));
+        m += det * (0.125 * fxa * fya * fza) * (0.125 * fxb * fyb * fzb);
+    }
+    for (int c = 0; c < 9; ++c) Khat[9 * lane + c] = kb[c];
+    Mhat[lane] = m;
+}
+
+// Row-wise brick8 assembly for meshes of translated copies of one element:
+// K_row,slot = sum over the elements e holding both the row node and the slot
+// node of E_e Khat[a][b] (+ c_mass rho_e Mhat[a][b] I).  The (at most 8)
+// elements of the row node and their connectivity are held in registers; the
+// stores are the same coalesced slot-by-slot writes as the general kernel.
+__global__ __launch_bounds__(256) void k_hex8_assemble_uniform(
+    int n_rows, int n_slices, const int32_t* __restrict__ sptr, const int32_t* __restrict__ scol,
+    const int32_t* __restrict__ rp, const int32_t* __restrict__ n2e_ptr, const int32_t* __restrict__ n2e,
+    const int32_t* __restrict__ conn_r, const double* __restrict__ Khat, const double* __restrict__ Mhat,
+    const double* __restrict__ E, const double* __restrict__ rho, double c_mass, double* __restrict__ K) {
+    const int lane = threadIdx.x & 63;
+    const int s = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (s >= n_slices) return;
+    const int r = 64 * s + lane;
+    const bool live = r < n_rows;
+    const int s0 = sptr[s], w = sptr[s + 1] - s0;
+    const int nrow = live ? rp[r + 1] - rp[r] : 0;
+    const int q0 = live ? n2e_ptr[r] : 0, nq = live ? n2e_ptr[r + 1] - q0 : 0;
+    int cn[8][8], av[8];
+    double ev[8], mv[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        av[q] = 0;
+        ev[q] = mv[q] = 0.0;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) cn[q][t] = -1;
+        if (q < nq) {
+            const int e = n2e[q0 + q];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                cn[q][t] = conn_r[8 * (int64_t)e + t];
+                if (cn[q][t] == r) av[q] = t;
+            }
+            ev[q] = E[e];
+            mv[q] = c_mass * rho[e];
+        }
+    }
+    for (int k = 0; k < w; ++k) {
+        double acc[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) acc[c] = 0.0;
+        if (k < nrow) {
+            const int col = scol[64 * (int64_t)(s0 + k) + lane];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                int bq = -1;
+#pragma unroll
+                for (int t = 0; t < 8; ++t) bq = (cn[q][t] == col) ? t : bq;
+                if (bq >= 0) {
+                    const int ab = 8 * av[q] + bq;
+                    for (int c = 0; c < 9; ++c) acc[c] += ev[q] * Khat[9 * ab + c];
+                    const double mm = mv[q] * Mhat[ab];
+                    acc[0] += mm;
+                    acc[4] += mm;
+                    acc[8] += mm;
+                }
+            }
+        }
+        double* base = K + (int64_t)9 * 64 * (s0 + k) + lane;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) base[64 * c] = acc[c];
+    }
+}
+
+void launch_hex8_assemble_rows(hipStream_t st, int n_rows, int n_slices, const int32_t* sptr, const int32_t* scol,
+                               const int32_t* rp, const int32_t* n2e_ptr, const int32_t* n2e, const int32_t* conn_r,
+                               const double* xyz_r, const double* E, const double* nu, const double* rho,
+                               double c_mass, double* K) {
+    if (n_slices <= 0) return;
+    hipLaunchKernelGGL(k_hex8_assemble_rows, dim3((unsigned)((n_slices + 3) / 4)), dim3(256), 0, st, n_rows, n_slices,
+                       sptr, scol, rp, n2e_ptr, n2e, conn_r, xyz_r, E, nu, rho, c_mass, K);
+}
+
+void launch_hex8_reference(hipStream_t st, const int32_t* conn_r, const double* xyz_r, double nu, double* Khat,
+                           double* Mhat) {
+    hipLaunchKernelGGL(k_hex8_reference, dim3(1), dim3(64), 0, st, conn_r, xyz_r, nu, Khat, Mhat);
+}
+
+void launch_hex8_assemble_uniform(hipStream_t st, int n_rows, int n_slices, const int32_t* sptr, const int32_t* scol,
+                                  const int32_t* rp, const int32_t* n2e_ptr, const int32_t* n2e,
+                                  const int32_t* conn_r, const double* Khat, const double* Mhat, const double* E,
+                                  const double* rho, double c_mass, double* K) {
+    if (n_slices <= 0) return;
+    hipLaunchKernelGGL(k_hex8_assemble_uniform, dim3((unsigned)((n_slices + 3) / 4)), dim3(256), 0, st, n_rows,
+                       n_slices, sptr, scol, rp, n2e_ptr, n2e, conn_r, Khat, Mhat, E, rho, c_mass, K);
 }
 
 // ---------------------------------------------------------------------------

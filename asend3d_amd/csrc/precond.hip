@@ -9,10 +9,11 @@
 //            cells' Gauss points with the fine cells' moduli;
 //   level l+1 >= 2: K_E = sum_children P_c^T K_c P_c (24x24 element matrices);
 //   penalty rows: sf * P^T D P per component, formed on the host.
-// Smoother: Chebyshev polynomial in D^-1 A (or in the learned node-block
-// preconditioner on the finest level), degree 2 pre and post.  Coarsest
-// level: explicit FP64 inverse (single-workgroup Gauss-Jordan, SPD, no
-// pivoting) applied as a dense GEMV.
+// Smoother: Chebyshev polynomial in the 3x3 block-Jacobi (or point-Jacobi, or
+// on the finest level the learned factor) preconditioned operator, degree 2
+// or 3 pre and post.  Coarsest level: explicit FP64 inverse (Gauss-Jordan,
+// SPD, no pivoting: whole-matrix LDS for small levels, else blocked over
+// 32-column pivot panels) applied as a dense GEMV.
 // The Krylov wrapper is flexible CG with one truncation vector, since a
 // V-cycle evaluated in FP32 is only approximately a fixed SPD operator.
 #include <hip/hip_runtime.h>
@@ -172,6 +173,10 @@ struct Multigrid {
     // learned fine smoother: factor P = L L^T, its lambda_max(P A), scratch
     int fine_smoother = 0;
     bool has_factor = false;
+    // penalty set the coarse penalty images (Level::d_pidx, d_pval) belong to
+    std::vector<int64_t> pen_key;
+    std::vector<int32_t> mult_key;
+    bool pen_ready = false;
     Factor factor;
     double lmax_factor = 0.0;
     float *ft = nullptr, *fz = nullptr;
@@ -700,41 +705,6 @@ __global__ void k_bsr_to_dense(int n_rows, const int32_t* __restrict__ rp, const
     }
 }
 
-// In-place Gauss-Jordan inverse of an SPD matrix (no pivoting), one
-// workgroup; column k is staged in LDS before each elimination step.
-__global__ __launch_bounds__(1024) void k_gauss_jordan(int n, double* __restrict__ A) {
-    // pivot column and pivot row of each step staged in LDS; the update of the
-    // other rows reads A once and writes it once (L2-resident for the sizes used)
-    extern __shared__ double sm_gj[];
-    double* colk = sm_gj;
-    double* rowk = sm_gj + n;
-    const int64_t nn = (int64_t)n * n;
-    for (int k = 0; k < n; ++k) {
-        for (int i = threadIdx.x; i < n; i += blockDim.x) colk[i] = A[(int64_t)i * n + k];
-        __syncthreads();
-        const double ip = 1.0 / colk[k];
-        for (int j = threadIdx.x; j < n; j += blockDim.x) {
-            const double v = (j == k ? 1.0 : A[(int64_t)k * n + j]) * ip;
-            rowk[j] = v;
-            A[(int64_t)k * n + j] = v;
-        }
-        __syncthreads();
-        int i = threadIdx.x / n, j = threadIdx.x % n;
-        const int di = blockDim.x / n, dj = blockDim.x % n;
-        for (int64_t t = threadIdx.x; t < nn; t += blockDim.x) {
-            if (i != k) A[t] = (j == k ? 0.0 : A[t]) - colk[i] * rowk[j];
-            // advance (i, j) by blockDim.x without a division
-            i += di;
-            j += dj;
-            if (j >= n) {
-                j -= n;
-                ++i;
-            }
-        }
-        __syncthreads();
-    }
-}
-
 // Same elimination with the whole matrix resident in LDS (n*n*8 <= 64 KiB).
 __global__ __launch_bounds__(1024) void k_gauss_jordan_lds(int n, double* __restrict__ Ag) {
     extern __shared__ double sm[];
@@ -757,6 +727,84 @@ __global__ __launch_bounds__(1024) void k_gauss_jordan_lds(int n, double* __rest
         __syncthreads();
     }
     for (int64_t t = threadIdx.x; t < (int64_t)n * n; t += blockDim.x) Ag[t] = A[t];
+}
+
+// Blocked in-place Gauss-Jordan inverse of an SPD matrix (no pivoting),
+// kGjBs pivots per step K (columns k0 .. k0 + kb - 1), two launches per step:
+//   k_gj_pivot: P = A_KK^-1, then the pivot row panel A_{K,:} <- P A_{K,:}
+//               (A_KK <- P);
+//   k_gj_update: every other row i, A_ij <- (j in K ? 0 : A_ij) - sum_l A_il(old) (new A_K)_lj.
+// This is the scalar elimination applied kb pivots at a time; A is streamed
+// once per step by many workgroups instead of once per pivot by one.
+constexpr int kGjBs = 32;
+
+// one workgroup of 256 threads
+__global__ __launch_bounds__(256) void k_gj_pivot(int n, int k0, double* __restrict__ A) {
+    __shared__ double P[kGjBs * kGjBs];
+    const int kb = min(kGjBs, n - k0);
+    const int tid = threadIdx.x;
+    for (int t = tid; t < kb * kb; t += blockDim.x) P[t] = A[(int64_t)(k0 + t / kb) * n + k0 + t % kb];
+    __syncthreads();
+    // P <- P^-1 in one wavefront without barriers: lane j < 32 holds column j,
+    // padded with the identity to 32 x 32; the pivot column of each step is
+    // read from lane kk by shuffles
+    if (tid < 64) {
+        const int j = tid;
+        double a[kGjBs];
+#pragma unroll
+        for (int i = 0; i < kGjBs; ++i) a[i] = (j < kb && i < kb) ? P[i * kb + j] : (i == j ? 1.0 : 0.0);
+        for (int kk = 0; kk < kb; ++kk) {
+            double akk = 0.0;
+#pragma unroll
+            for (int i = 0; i < kGjBs; ++i) akk = (i == kk) ? a[i] : akk;
+            const double ip = 1.0 / __shfl(akk, kk);
+            const double rk = (j == kk ? 1.0 : akk) * ip;
+#pragma unroll
+            for (int i = 0; i < kGjBs; ++i) {
+                const double ci = __shfl(a[i], kk);
+                a[i] = (i == kk) ? rk : (j == kk ? 0.0 : a[i]) - ci * rk;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kGjBs; ++i)
+            if (j < kb && i < kb) P[i * kb + j] = a[i];
+    }
+    __syncthreads();
+    // row panel, one column per thread (each thread reads its column before
+    // writing it)
+    for (int j = tid; j < n; j += blockDim.x) {
+        if (j >= k0 && j < k0 + kb) {
+            for (int m = 0; m < kb; ++m) A[(int64_t)(k0 + m) * n + j] = P[m * kb + (j - k0)];
+        } else {
+            double col[kGjBs];
+#pragma unroll
+            for (int l = 0; l < kGjBs; ++l) col[l] = l < kb ? A[(int64_t)(k0 + l) * n + j] : 0.0;
+            for (int m = 0; m < kb; ++m) {
+                double acc = 0.0;
+#pragma unroll
+                for (int l = 0; l < kGjBs; ++l)
+                    if (l < kb) acc += P[m * kb + l] * col[l];
+                A[(int64_t)(k0 + m) * n + j] = acc;
+            }
+        }
+    }
+}
+
+// one workgroup (256 threads) per row outside the pivot block
+__global__ __launch_bounds__(256) void k_gj_update(int n, int k0, double* __restrict__ A) {
+    __shared__ double c[kGjBs];
+    const int kb = min(kGjBs, n - k0);
+    const int i = (int)blockIdx.x < k0 ? (int)blockIdx.x : (int)blockIdx.x + kb;
+    if ((int)threadIdx.x < kb) c[threadIdx.x] = A[(int64_t)i * n + k0 + threadIdx.x];
+    __syncthreads();
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        const bool in_k = (j >= k0 && j < k0 + kb);
+        double acc = in_k ? 0.0 : A[(int64_t)i * n + j];
+#pragma unroll
+        for (int l = 0; l < kGjBs; ++l)
+            if (l < kb) acc -= c[l] * A[(int64_t)(k0 + l) * n + j];
+        A[(int64_t)i * n + j] = acc;
+    }
 }
 
 // x = Ainv b (FP64 accumulation), one wave per row.
@@ -1285,17 +1333,17 @@ void mg_keep_factor_bounds(Multigrid* mg, bool keep) { mg->keep_factor_bounds = 
 
 void mg_use_graphs(Multigrid* mg, bool on) { mg->use_graphs = on; }
 
-int mg_setup(Multigrid* mg, hipStream_t st, const double* d_E, const double* d_nu, const float* d_K32,
-             const float* d_dinv32, const int32_t* d_kdiag0, double sf, int n_pen, const int64_t* h_pen,
-             const int32_t* h_mult) {
-    mg->ready = false;
+// Galerkin images of the penalty on every coarse level, S_l = P^T S_{l-1} P
+// per component (S_0 = the penalised fine DOFs with their multiplicities), as
+// lists of (value index, weight); each setup adds sf * weight.  Built on the
+// host once per penalty set, since the penalty set does not change between
+// the assemblies of a design loop.
+static int prepare_penalty(Multigrid* mg, int n_pen, const int64_t* h_pen, const int32_t* h_mult) {
+    std::vector<int64_t> pk(h_pen, h_pen + n_pen);
+    std::vector<int32_t> mk(h_mult, h_mult + n_pen);
+    if (mg->pen_ready && pk == mg->pen_key && mk == mg->mult_key) return ASEND_OK;
+    mg->pen_ready = false;
     Level& l0 = mg->L[0];
-    l0.d_val = d_K32;
-    l0.d_dinv = d_dinv32;
-    hipLaunchKernelGGL(k_binv32, dim3((unsigned)((l0.n_nodes + 255) / 256)), dim3(256), 0, st, (int)l0.n_nodes,
-                       mg->fine.sptr, (const int32_t*)nullptr, d_kdiag0, d_K32, l0.binv);
-    // penalty of the fine level per component as a node-level diagonal
-    // (grid node -> weight), used for the Galerkin coarse penalty
     std::vector<int32_t> grid_of_rank(l0.n_nodes);
     for (int64_t g = 0; g < l0.n_nodes; ++g) grid_of_rank[mg->h_rank0[g]] = (int32_t)g;
     std::vector<std::map<std::pair<int64_t, int64_t>, double>> S(3);
@@ -1309,23 +1357,6 @@ int mg_setup(Multigrid* mg, hipStream_t st, const double* d_E, const double* d_n
         Level& f = mg->L[l - 1];
         const int fmx = f.n[0] + 1, fmy = f.n[1] + 1;
         const int cmx = c.n[0] + 1, cmy = c.n[1] + 1;
-        // element matrices
-        if (l == 1) {
-            hipLaunchKernelGGL(k_level1_ke, dim3((unsigned)c.n_el), dim3(64), 0, st, c.n[0], c.n[1], c.n[2], f.n[0],
-                               f.n[1], c.dax.X[0], c.dax.X[1], c.dax.X[2], mg->d_conn_r, mg->d_xyz_r, d_E, d_nu,
-                               c.d_Ke);
-        } else {
-            hipLaunchKernelGGL(k_galerkin_ke, dim3((unsigned)c.n_el), dim3(64), 0, st, c.n[0], c.n[1], c.n[2], f.n[0],
-                               f.n[1], c.dax.X[0], c.dax.X[1], c.dax.X[2], f.d_Ke, c.d_Ke);
-        }
-        // assemble
-        hipLaunchKernelGGL(k_fill, dim3(nb_for(9 * c.nnzb)), dim3(256), 0, st, 9 * c.nnzb, 0.f, c.o_val);
-        for (int col = 0; col < 8; ++col) {
-            int b = c.color_ptr[col], e = c.color_ptr[col + 1];
-            if (e > b)
-                hipLaunchKernelGGL(k_asm_ke, dim3(e - b), dim3(64), 0, st, c.d_elems + b, c.d_conn, c.d_Ke, c.o_rp,
-                                   c.d_slot, c.o_val);
-        }
         // Galerkin penalty: S_c = P^T S_f P per component
         std::vector<std::map<std::pair<int64_t, int64_t>, double>> Sc(3);
         for (int comp = 0; comp < 3; ++comp)
@@ -1370,6 +1401,46 @@ int mg_setup(Multigrid* mg, hipStream_t st, const double* d_E, const double* d_n
         c.n_padd = (int64_t)pidx.size();
         MGCHK(up(&c.d_pidx, pidx.data(), pidx.size()));
         MGCHK(up(&c.d_pval, pval.data(), pval.size()));
+    }
+    mg->pen_key.swap(pk);
+    mg->mult_key.swap(mk);
+    mg->pen_ready = true;
+    return ASEND_OK;
+}
+
+int mg_setup(Multigrid* mg, hipStream_t st, const double* d_E, const double* d_nu, const float* d_K32,
+             const float* d_dinv32, const int32_t* d_kdiag0, double sf, int n_pen, const int64_t* h_pen,
+             const int32_t* h_mult) {
+    mg->ready = false;
+    Level& l0 = mg->L[0];
+    l0.d_val = d_K32;
+    l0.d_dinv = d_dinv32;
+    hipLaunchKernelGGL(k_binv32, dim3((unsigned)((l0.n_nodes + 255) / 256)), dim3(256), 0, st, (int)l0.n_nodes,
+                       mg->fine.sptr, (const int32_t*)nullptr, d_kdiag0, d_K32, l0.binv);
+    {
+        int rc = prepare_penalty(mg, n_pen, h_pen, h_mult);
+        if (rc) return rc;
+    }
+    for (size_t l = 1; l < mg->L.size(); ++l) {
+        Level& c = mg->L[l];
+        Level& f = mg->L[l - 1];
+        // element matrices
+        if (l == 1) {
+            hipLaunchKernelGGL(k_level1_ke, dim3((unsigned)c.n_el), dim3(64), 0, st, c.n[0], c.n[1], c.n[2], f.n[0],
+                               f.n[1], c.dax.X[0], c.dax.X[1], c.dax.X[2], mg->d_conn_r, mg->d_xyz_r, d_E, d_nu,
+                               c.d_Ke);
+        } else {
+            hipLaunchKernelGGL(k_galerkin_ke, dim3((unsigned)c.n_el), dim3(64), 0, st, c.n[0], c.n[1], c.n[2], f.n[0],
+                               f.n[1], c.dax.X[0], c.dax.X[1], c.dax.X[2], f.d_Ke, c.d_Ke);
+        }
+        // assemble
+        hipLaunchKernelGGL(k_fill, dim3(nb_for(9 * c.nnzb)), dim3(256), 0, st, 9 * c.nnzb, 0.f, c.o_val);
+        for (int col = 0; col < 8; ++col) {
+            int b = c.color_ptr[col], e = c.color_ptr[col + 1];
+            if (e > b)
+                hipLaunchKernelGGL(k_asm_ke, dim3(e - b), dim3(64), 0, st, c.d_elems + b, c.d_conn, c.d_Ke, c.o_rp,
+                                   c.d_slot, c.o_val);
+        }
         if (c.n_padd)
             hipLaunchKernelGGL(k_add_values, dim3((unsigned)((c.n_padd + 255) / 256)), dim3(256), 0, st, c.n_padd,
                                c.d_pidx, c.d_pval, (float)sf, c.o_val);
@@ -1397,10 +1468,15 @@ int mg_setup(Multigrid* mg, hipStream_t st, const double* d_E, const double* d_n
         hipLaunchKernelGGL(k_gauss_jordan_lds, dim3(1), dim3(1024), (size_t)mg->nco * (mg->nco + 1) * sizeof(double), st,
                            mg->nco, mg->d_Ainv);
     else
-        hipLaunchKernelGGL(k_gauss_jordan, dim3(1), dim3(1024), 2 * mg->nco * sizeof(double), st, mg->nco,
-                           mg->d_Ainv);
+        for (int k0 = 0; k0 < mg->nco; k0 += kGjBs) {
+            const int kb = std::min(kGjBs, mg->nco - k0);
+            hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(256), 0, st, mg->nco, k0, mg->d_Ainv);
+            if (mg->nco > kb)
+                hipLaunchKernelGGL(k_gj_update, dim3((unsigned)(mg->nco - kb)), dim3(256), 0, st, mg->nco, k0,
+                                   mg->d_Ainv);
+        }
     MGCHK(hipGetLastError());
-    MGCHK(hipStreamSynchronize(st));
+    // no host wait: everything that uses the hierarchy is ordered on st
     mg->ready = true;
     return ASEND_OK;
 }
@@ -1634,6 +1710,12 @@ int mg_prolong(Multigrid* mg, hipStream_t st, int l, const float* d_coarse, floa
 int mg_vcycle(Multigrid* mg, hipStream_t st, const float* d_b, float* d_x) {
     if (!mg->ready) return mgfail(mg, ASEND_ERR_STATE, "multigrid not set up");
     vcycle(mg, st, 0, d_b, d_x);
+    return ASEND_OK;
+}
+
+int mg_coarse_solve(Multigrid* mg, hipStream_t st, const float* d_b, float* d_x) {
+    if (!mg->ready) return mgfail(mg, ASEND_ERR_STATE, "multigrid not set up");
+    vcycle(mg, st, (int)mg->L.size() - 1, d_b, d_x);
     return ASEND_OK;
 }
 

@@ -73,5 +73,7 @@ int mg_apply_level(Multigrid* mg, hipStream_t st, int l, const float* d_x, float
 int mg_restrict(Multigrid* mg, hipStream_t st, int l, const float* d_fine, float* d_coarse);
 int mg_prolong(Multigrid* mg, hipStream_t st, int l, const float* d_coarse, float* d_fine);
 int mg_vcycle(Multigrid* mg, hipStream_t st, const float* d_b, float* d_x);
+// x = A_c^-1 b on the coarsest level (its explicit FP64 inverse)
+int mg_coarse_solve(Multigrid* mg, hipStream_t st, const float* d_b, float* d_x);
 
 }  // namespace asend

@@ -2,12 +2,14 @@
 --output-format csv`), optionally restricted to a time window delimited by
 launches of named kernels.
 
-    python -m asend3d_amd.profiling TRACE.csv [--after NAME:K] [--until NAME:K] [--top 25]
+    python -m asend3d_amd.profiling TRACE.csv [--after NAME:K] [--until NAME:K] [--top 25] [--gaps 15]
 
 --after NAME:K starts the window at the end of the K-th (1-based) launch of
 the kernel whose name contains NAME; --until NAME:K ends it at the end of the
 K-th launch.  Prints a markdown table: kernel, launches, total, average and
-share of the window's kernel time, plus the window's span.
+share of the window's kernel time, plus the window's span; --gaps N adds the N
+longest idle intervals of the device (no kernel running) with the kernels on
+either side, which is where the host held the stream.
 """
 import argparse
 import csv
@@ -62,12 +64,29 @@ def summarize(rows, t0=None, t1=None, top=25):
     return "\n".join(out)
 
 
+def gaps(rows, t0=None, t1=None, top=15):
+    sel = [(s, e, n) for s, e, n in rows if (t0 is None or s >= t0) and (t1 is None or e <= t1)]
+    out, busy_end, prev = [], None, None
+    for s, e, n in sel:
+        if busy_end is not None and s > busy_end:
+            out.append((s - busy_end, prev, n))
+        if busy_end is None or e > busy_end:
+            busy_end, prev = e, n
+    out.sort(key=lambda g: -g[0])
+    idle = sum(g[0] for g in out)
+    lines = [f"idle: {idle / 1e6:.3f} ms in {len(out)} gaps", "", "| gap us | after | before |", "|---:|---|---|"]
+    for g, a, b in out[:top]:
+        lines.append(f"| {g / 1e3:.1f} | `{short(a, 60)}` | `{short(b, 60)}` |")
+    return "\n".join(lines)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("trace")
     ap.add_argument("--after", default=None)
     ap.add_argument("--until", default=None)
     ap.add_argument("--top", type=int, default=25)
+    ap.add_argument("--gaps", type=int, default=0)
     a = ap.parse_args()
     rows = load(a.trace)
     t0 = t1 = None
@@ -78,6 +97,9 @@ def main():
         name, k = a.until.rsplit(":", 1)
         t1 = nth_end(rows, name, int(k))
     print(summarize(rows, t0, t1, a.top))
+    if a.gaps:
+        print()
+        print(gaps(rows, t0, t1, a.gaps))
 
 
 if __name__ == "__main__":

@@ -169,6 +169,12 @@ int asend_update_moduli(asend_ctx* ctx, const double* E);
 int asend_set_penalty(asend_ctx* ctx, int64_t n, const int64_t* dofs, double sf);
 double asend_penalty_scale(const asend_ctx* ctx);
 
+/* 1 when the uploaded brick8 mesh consists of translated copies of one
+ * element with one Poisson ratio (at most 8 elements per node): assembly then
+ * scales that element's E = 1 / rho = 1 blocks by E_e / rho_e instead of
+ * integrating every element.  No reference counterpart (diagnostic). */
+int asend_uniform_elements(const asend_ctx* ctx);
+
 /* Element integration + colored atomic-free assembly of the elastic operator:
  * replaces build_elastic_soln_load(true) with get_ruk_dfd0 / get_ru_dfd0
  * (S/model_analysis.rs:179-201, S/element_equations.rs:206-418, :852-1062).
@@ -283,6 +289,8 @@ int asend_mg_apply(asend_ctx* ctx, int level, const float* x, float* y);
 int asend_mg_restrict(asend_ctx* ctx, int level, const float* fine, float* coarse);
 int asend_mg_prolong(asend_ctx* ctx, int level, const float* coarse, float* fine);
 int asend_mg_vcycle(asend_ctx* ctx, const float* b, float* x);
+/* x = A_c^-1 b with the explicit inverse of the coarsest operator. */
+int asend_mg_coarse_solve(asend_ctx* ctx, const float* b, float* x);
 /* Replay the launch sequences of the iterative loops (a block of check_every
  * FCG iterations, a Newmark time step) as captured hipGraphs (on = 1, the
  * default) or launch the same kernels one by one (on = 0).  Results are

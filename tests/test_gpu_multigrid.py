@@ -35,6 +35,29 @@ def test_galerkin_identity(ctx, dims):
         assert err < 2e-5, (l, err)
 
 
+@pytest.mark.parametrize("dims", [(4, 4, 4), (13, 3, 3), (40, 4, 4)])
+def test_coarsest_inverse_is_exact(ctx, dims):
+    # coarsest sizes 81, 216 and 567 DOF: the whole-matrix LDS, the blocked
+    # panel and the streaming Gauss-Jordan kernels; the operator is read back
+    # column by column through the level SpMV (FP32 values, exact products)
+    gpu_case(f"coarsest inverse {dims}")
+    p = problems.heterogeneous(*dims, seed=4)
+    setup(ctx, p, dims)
+    nd = ctx.mg_info()
+    lc = len(nd) - 1
+    n = nd[lc]
+    A = np.zeros((n, n))
+    e = np.zeros(n, dtype=np.float32)
+    for j in range(n):
+        e[j] = 1.0
+        A[:, j] = ctx.mg_apply(lc, e)
+        e[j] = 0.0
+    b = np.random.default_rng(5).standard_normal(n).astype(np.float32)
+    x = ctx.mg_coarse_solve(b).astype(np.float64)
+    x_ref = np.linalg.solve(A, b.astype(np.float64))
+    assert np.abs(x - x_ref).max() <= 1e-5 * np.abs(x_ref).max()
+
+
 def test_prolongation_interpolates_linear_fields(ctx):
     # P reproduces fields linear in the grid index exactly
     dims = (13, 5, 3)

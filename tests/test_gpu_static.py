@@ -1,5 +1,7 @@
 """Device parity of the static solve path against the CPU oracle.
 Every device call goes through the C ABI (libasend_gpu.so)."""
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -16,16 +18,30 @@ def oracle_system(p):
     return K, sf
 
 
+def distorted(p, seed):
+    """The same problem on a mesh with randomly moved interior nodes and a
+    per-element Poisson ratio (the general, per-element integration path)."""
+    rng = np.random.default_rng(seed)
+    m = p.mesh
+    xyz = m.xyz.copy()
+    h = min(m.lx / m.nx, m.ly / m.ny, m.lz / m.nz)
+    inner = np.all((xyz > xyz.min(axis=0) + 1e-9) & (xyz < xyz.max(axis=0) - 1e-9), axis=1)
+    xyz[inner] += rng.uniform(-0.2, 0.2, (int(inner.sum()), 3)) * h
+    return dataclasses.replace(p, mesh=dataclasses.replace(m, xyz=xyz), nu=rng.uniform(0.2, 0.4, m.n_el))
+
+
 def small_problems():
     yield "C1", problems.config1()
     yield "het-14x5x4", problems.heterogeneous(14, 5, 4, seed=5)
+    yield "het-14x5x4 distorted", distorted(problems.heterogeneous(14, 5, 4, seed=5), seed=6)
 
 
-@pytest.mark.parametrize("which", [0, 1])
+@pytest.mark.parametrize("which", [0, 1, 2])
 def test_assembly_matches_oracle(ctx, which):
     name, p = list(small_problems())[which]
     gpu_case(f"assembly {name}")
     problems.upload(ctx, p)
+    assert ctx.uniform_elements() == (which < 2)
     ctx.assemble()
     K, sf = oracle_system(p)
     assert abs(ctx.penalty_scale() - sf) <= 1e-12 * sf
@@ -44,6 +60,28 @@ def test_assembly_matches_oracle(ctx, which):
     scale = np.abs(K.vals).max()
     assert np.abs(vals[mask] - ref[mask]).max() <= 1e-12 * scale
     assert np.abs(vals[~mask] - ref[~mask]).max() <= 1e-14 * sf
+
+
+@pytest.mark.parametrize("which", [1, 2])
+def test_mass_assembly_matches_oracle(ctx, which):
+    # K + c M with per-element densities, scaled reference blocks (which = 1)
+    # and per-element integration (which = 2)
+    name, p = list(small_problems())[which]
+    gpu_case(f"mass assembly {name}")
+    rho = np.random.default_rng(3).uniform(1e-9, 8e-9, p.mesh.n_el)
+    problems.upload(ctx, p, rho=rho)
+    K, _ = oracle_system(p)
+    M = O.assemble_solid_mass(8, p.mesh.xyz, p.mesh.conn, rho, p.dof_index, K)
+    Ks, Ms = K.to_scipy(), M.to_scipy()
+    c = abs(Ks).max() / abs(Ms).max()
+    ctx.assemble(c)
+    rp, cols, vals = ctx.get_matrix(with_penalty=False)
+    import scipy.sparse as sp
+    A = sp.csr_matrix((vals, cols, rp), shape=(p.n_dof, p.n_dof))
+    ref = Ks + c * Ms
+    # FP64 rounding of two comparable terms summed in a different order
+    # (measured 5e-12 of max|K + c M| on the scaled-reference path)
+    assert abs(A - ref).max() <= 1e-11 * abs(ref).max()
 
 
 def test_spmv_matches_oracle(ctx):
