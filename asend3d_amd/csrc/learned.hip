@@ -166,8 +166,25 @@ __global__ __launch_bounds__(256) void k_aggregate(int n, const int32_t* __restr
     if (node >= n) return;
     float s = 0.f;
     int deg = 0;
-    for (int k = rp[node]; k < rp[node + 1]; ++k) {
-        int j = bc[k];
+    int k = rp[node];
+    const int end = rp[node + 1];
+    // four neighbour rows in flight per step; summed in row order
+    for (; k + 4 <= end; k += 4) {
+        int j[4];
+        float h[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) j[u] = bc[k + u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) h[u] = (j[u] != node) ? bf2f(H[(int64_t)kHid * j[u] + c]) : 0.f;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (j[u] != node) {
+                s += h[u];
+                ++deg;
+            }
+    }
+    for (; k < end; ++k) {
+        const int j = bc[k];
         if (j == node) continue;
         s += bf2f(H[(int64_t)kHid * j + c]);
         ++deg;
