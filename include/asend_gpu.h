@@ -269,7 +269,8 @@ typedef struct asend_newmark_stats {
     double device_ms;             /* HIP-event time of the step loop          */
 } asend_newmark_stats;
 /* Assemble M (consistent, rho per element) and K_eff = K + c_m M with the
- * penalty of asend_set_penalty; requires gamma > beta.  brick8 meshes with rho. */
+ * penalty of asend_set_penalty; requires gamma > beta.  brick8 and shell4
+ * meshes uploaded with rho (shell4: after asend_set_thickness). */
 int asend_newmark_setup(asend_ctx* ctx, double dt, double beta, double gamma);
 /* n_steps steps from (u0, v0, a0) (null = zero).  The load of step n is
  * load_fac[n] * f_shape.  hist[n * n_watch + k] = u[watch[k]] after step n.
@@ -319,8 +320,8 @@ int asend_autotune(asend_ctx* ctx, const double* rhs, int what, asend_tuning* ou
 /* Chebyshev smoother of every smoothed level: M = point Jacobi (kind 0) or the
  * inverse of the 3x3 node blocks (kind 1), degree 1..8, interval
  * [lo_frac, hi_frac] * lambda_max(M A).  Defaults 1, 2, 0.1, 1.15.  Kept across
- * assemblies of the same hierarchy; a new kind takes effect at the next
- * assembly. */
+ * assemblies of the same hierarchy; a new kind rebuilds the hierarchy's
+ * smoother data at the next solve. */
 int asend_mg_set_smoother(asend_ctx* ctx, int kind, int degree, double lo_frac, double hi_frac);
 
 /* ---- learned preconditioner (no reference counterpart) --------------------
