@@ -522,12 +522,25 @@ int solve_device(asend_ctx* ctx, const asend_solve_opts* o, asend_solve_stats* s
 }
 
 // Element integration into K: k_scale K_e(E) + c_mass M_e of every element.
-// brick8: one row-wise gather launch that writes every value of K (the brick
-// kernel has no stiffness scale: k_scale must be 1; pass zero moduli for the
-// mass alone).  shell4: K is cleared, then one scatter launch per element
-// colour.
+// Translated-element meshes (brick8, brickIM): one row-wise launch scaling the
+// reference blocks.  Other brick8 meshes: one row-wise gather launch that
+// integrates per element (no stiffness scale there: k_scale must be 1; pass
+// zero moduli for the mass alone).  brickIM and shell4: K is cleared, then
+// one scatter launch per element colour.
 int integrate_elements(asend_ctx* ctx, const double* E, double k_scale, double c_mass, double* K) {
-    if (ctx->etype == ASEND_SHELL4) {
+    if (ctx->uniform_hex) {
+        launch_hex8_assemble_uniform(ctx->st, (int)ctx->n_nodes, (int)ctx->n_slices, ctx->d_sptr, ctx->d_scol,
+                                     ctx->d_rp, ctx->d_n2e_ptr, ctx->d_n2e, ctx->d_conn_r, ctx->d_khat, ctx->d_mhat, E,
+                                     ctx->d_rho, k_scale, c_mass, K);
+    } else if (ctx->etype == ASEND_BRICKIM) {
+        HIPCHK(hipMemsetAsync(K, 0, ctx->nval() * sizeof(double), ctx->st));
+        const int ncol = (int)ctx->h_color_ptr.size() - 1;
+        for (int c = 0; c < ncol; ++c) {
+            const int b = ctx->h_color_ptr[c], e = ctx->h_color_ptr[c + 1];
+            launch_brickim_assemble(ctx->st, ctx->d_elems + b, e - b, ctx->d_conn_r, ctx->d_xyz_r, E, ctx->d_nu,
+                                    ctx->d_rho, k_scale, c_mass, ctx->d_sptr, ctx->d_slot, K);
+        }
+    } else if (ctx->etype == ASEND_SHELL4) {
         HIPCHK(hipMemsetAsync(K, 0, ctx->nval() * sizeof(double), ctx->st));
         const int ncol = (int)ctx->h_color_ptr.size() - 1;
         for (int c = 0; c < ncol; ++c) {
@@ -535,10 +548,6 @@ int integrate_elements(asend_ctx* ctx, const double* E, double k_scale, double c
             launch_shell4_assemble(ctx->st, ctx->d_elems + b, e - b, ctx->d_conn_r, ctx->d_xyz_r, E, ctx->d_nu,
                                    ctx->d_thk, ctx->d_rho, k_scale, c_mass, ctx->d_sptr, ctx->d_slot, K);
         }
-    } else if (ctx->uniform_hex) {
-        launch_hex8_assemble_uniform(ctx->st, (int)ctx->n_nodes, (int)ctx->n_slices, ctx->d_sptr, ctx->d_scol,
-                                     ctx->d_rp, ctx->d_n2e_ptr, ctx->d_n2e, ctx->d_conn_r, ctx->d_khat, ctx->d_mhat, E,
-                                     ctx->d_rho, c_mass, K);
     } else {
         launch_hex8_assemble_rows(ctx->st, (int)ctx->n_nodes, (int)ctx->n_slices, ctx->d_sptr, ctx->d_scol, ctx->d_rp,
                                   ctx->d_n2e_ptr, ctx->d_n2e, ctx->d_conn_r, ctx->d_xyz_r, E, ctx->d_nu, ctx->d_rho,
@@ -546,6 +555,16 @@ int integrate_elements(asend_ctx* ctx, const double* E, double k_scale, double c
     }
     HIPCHK(hipGetLastError());
     return ASEND_OK;
+}
+
+// dJ/dE_e = -lam_e^T (dK_e/dE_e) u_e for every element into d_sens
+void launch_sens(asend_ctx* ctx, const double* u, const double* lam) {
+    if (ctx->etype == ASEND_BRICKIM)
+        launch_brickim_sens(ctx->st, (int)ctx->n_el, ctx->d_conn_r, ctx->d_xyz_r, ctx->d_E, ctx->d_nu, u, lam,
+                            ctx->d_sens);
+    else
+        launch_hex8_sens(ctx->st, (int)ctx->n_el, ctx->d_conn_r, ctx->d_xyz_r, ctx->d_E, ctx->d_nu, u, lam,
+                         ctx->d_sens);
 }
 
 int assemble_device(asend_ctx* ctx, double c_mass) {
@@ -649,7 +668,7 @@ int asend_upload_mesh(asend_ctx* ctx, int elem_type, int64_t n_mesh_nodes, const
                       const int64_t* dof_index, int64_t n_dof, int64_t n_el, const int32_t* conn, const double* E,
                       const double* nu, const double* rho) {
     if (!ctx) return ASEND_ERR_ARG;
-    if (elem_type != ASEND_BRICK8 && elem_type != ASEND_SHELL4)
+    if (elem_type != ASEND_BRICK8 && elem_type != ASEND_BRICKIM && elem_type != ASEND_SHELL4)
         return fail(ctx, ASEND_ERR_UNSUPPORTED, "element type %d has no device integration kernel", elem_type);
     if (n_mesh_nodes <= 0 || n_el <= 0 || !xyz || !dof_index || !conn || !E || !nu)
         return fail(ctx, ASEND_ERR_ARG, "bad mesh arguments");
@@ -912,7 +931,10 @@ int asend_upload_mesh(asend_ctx* ctx, int elem_type, int64_t n_mesh_nodes, const
     else HIPCHK(hipMemset(ctx->d_rho, 0, n_el * 8));
     if (uniform) {
         if ((rc = dalloc(ctx, &ctx->d_khat, 64 * 9)) || (rc = dalloc(ctx, &ctx->d_mhat, 64))) return rc;
-        launch_hex8_reference(ctx->st, ctx->d_conn_r, ctx->d_xyz_r, nu[0], ctx->d_khat, ctx->d_mhat);
+        if (elem_type == ASEND_BRICKIM)
+            launch_brickim_reference(ctx->st, ctx->d_conn_r, ctx->d_xyz_r, ctx->d_nu, ctx->d_khat, ctx->d_mhat);
+        else
+            launch_hex8_reference(ctx->st, ctx->d_conn_r, ctx->d_xyz_r, nu[0], ctx->d_khat, ctx->d_mhat);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(ctx->st));
         ctx->uniform_hex = true;
@@ -1124,13 +1146,12 @@ int asend_solve(asend_ctx* ctx, const double* rhs, double* x, const asend_solve_
 int asend_element_sens(asend_ctx* ctx, const double* u, const double* lam, double* dJdE) {
     if (!ctx || !u || !lam || !dJdE) return ASEND_ERR_ARG;
     if (!ctx->d_E) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
-    if (ctx->etype != ASEND_BRICK8)
-        return fail(ctx, ASEND_ERR_UNSUPPORTED, "element sensitivities are implemented for brick8 meshes");
+    if (ctx->etype == ASEND_SHELL4)
+        return fail(ctx, ASEND_ERR_UNSUPPORTED, "element sensitivities are implemented for brick8 and brickIM meshes");
     const int64_t n = ctx->n_dof;
     HIPCHK(hipMemcpy(ctx->d_x64, u, n * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->d_lam, lam, n * 8, hipMemcpyHostToDevice));
-    launch_hex8_sens(ctx->st, (int)ctx->n_el, ctx->d_conn_r, ctx->d_xyz_r, ctx->d_E, ctx->d_nu, ctx->d_x64,
-                     ctx->d_lam, ctx->d_sens);
+    launch_sens(ctx, ctx->d_x64, ctx->d_lam);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->st));
     HIPCHK(hipMemcpy(dJdE, ctx->d_sens, ctx->n_el * 8, hipMemcpyDeviceToHost));
@@ -1141,8 +1162,8 @@ int asend_static_compliance(asend_ctx* ctx, const double* f, double* u, double* 
                             const asend_solve_opts* opts, asend_solve_stats* stats) {
     if (!ctx || !f || !opts) return ASEND_ERR_ARG;
     if (!ctx->d_K64) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
-    if (ctx->etype != ASEND_BRICK8)
-        return fail(ctx, ASEND_ERR_UNSUPPORTED, "element sensitivities are implemented for brick8 meshes");
+    if (ctx->etype == ASEND_SHELL4)
+        return fail(ctx, ASEND_ERR_UNSUPPORTED, "element sensitivities are implemented for brick8 and brickIM meshes");
     const int64_t n = ctx->n_dof;
     HIPCHK(hipMemcpy(ctx->d_f, f, n * 8, hipMemcpyHostToDevice));
     HIPCHK(hipDeviceSynchronize());
@@ -1153,8 +1174,7 @@ int asend_static_compliance(asend_ctx* ctx, const double* f, double* u, double* 
     if ((rc = solve_device(ctx, opts, &s))) return rc;
     // compliance J = f.u ; adjoint rhs dJ/du = f, so lambda = u (same operator)
     launch_dot64(ctx->st, n, ctx->d_f, ctx->d_u, ctx->d_partials, ctx->d_sc + 2);
-    launch_hex8_sens(ctx->st, (int)ctx->n_el, ctx->d_conn_r, ctx->d_xyz_r, ctx->d_E, ctx->d_nu, ctx->d_u, ctx->d_u,
-                     ctx->d_sens);
+    launch_sens(ctx, ctx->d_u, ctx->d_u);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ctx->ev1, ctx->st));
     HIPCHK(hipEventSynchronize(ctx->ev1));
@@ -1214,7 +1234,7 @@ int asend_batch_create(asend_ctx* const* variants, int n_variants, asend_batch**
     asend_batch* ab = new asend_batch();
     for (int v = 0; v < n_variants; ++v) {
         asend_ctx* c = variants[v];
-        if (!c || !c->assembled || c->etype != ASEND_BRICK8) {
+        if (!c || !c->assembled || (c->etype != ASEND_BRICK8 && c->etype != ASEND_BRICKIM)) {
             delete ab;
             return ASEND_ERR_STATE;
         }
@@ -1303,10 +1323,12 @@ int asend_newmark_setup(asend_ctx* ctx, double dt, double beta, double gamma) {
         (rc = dalloc(ctx, &nm.w, n)) || (rc = dalloc(ctx, &nm.q, n)) || (rc = dalloc(ctx, &nm.rhs0, n)) ||
         (rc = dalloc(ctx, &nm.rhs, n)) || (rc = dalloc(ctx, &nm.fshape, n)) || (rc = dalloc(ctx, &nm.step, 1)))
         return rc;
-    // consistent mass alone: the element kernel with c_mass = 1 and no
-    // stiffness (zero moduli for the brick kernel, k_scale = 0 for the shell)
-    if (ctx->etype == ASEND_SHELL4) {
-        if (!ctx->thk_set) return fail(ctx, ASEND_ERR_STATE, "shell4 mesh without thickness (asend_set_thickness)");
+    // consistent mass alone: the element kernels with c_mass = 1 and no
+    // stiffness (zero moduli for the brick8 kernels, k_scale = 0 for the
+    // kernels that condense internal DOFs, whose stiffness must stay regular)
+    if (ctx->etype == ASEND_SHELL4 || ctx->etype == ASEND_BRICKIM) {
+        if (ctx->etype == ASEND_SHELL4 && !ctx->thk_set)
+            return fail(ctx, ASEND_ERR_STATE, "shell4 mesh without thickness (asend_set_thickness)");
         if ((rc = integrate_elements(ctx, ctx->d_E, 0.0, 1.0, nm.M64))) return rc;
         HIPCHK(hipStreamSynchronize(ctx->st));
     } else {

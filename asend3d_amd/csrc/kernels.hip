@@ -243,16 +243,17 @@ __global__ __launch_bounds__(64) void k_hex8_reference(const int32_t* __restrict
     Mhat[lane] = m;
 }
 
-// Row-wise brick8 assembly for meshes of translated copies of one element:
-// K_row,slot = sum over the elements e holding both the row node and the slot
-// node of E_e Khat[a][b] (+ c_mass rho_e Mhat[a][b] I).  The (at most 8)
+// Row-wise brick8 / brickIM assembly for meshes of translated copies of one
+// element: K_row,slot = sum over the elements e holding both the row node and
+// the slot node of k_scale E_e Khat[a][b] (+ c_mass rho_e Mhat[a][b] I).  The (at most 8)
 // elements of the row node and their connectivity are held in registers; the
 // stores are the same coalesced slot-by-slot writes as the general kernel.
 __global__ __launch_bounds__(256) void k_hex8_assemble_uniform(
     int n_rows, int n_slices, const int32_t* __restrict__ sptr, const int32_t* __restrict__ scol,
     const int32_t* __restrict__ rp, const int32_t* __restrict__ n2e_ptr, const int32_t* __restrict__ n2e,
     const int32_t* __restrict__ conn_r, const double* __restrict__ Khat, const double* __restrict__ Mhat,
-    const double* __restrict__ E, const double* __restrict__ rho, double c_mass, double* __restrict__ K) {
+    const double* __restrict__ E, const double* __restrict__ rho, double k_scale, double c_mass,
+    double* __restrict__ K) {
     const int lane = threadIdx.x & 63;
     const int s = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (s >= n_slices) return;
@@ -276,7 +277,7 @@ __global__ __launch_bounds__(256) void k_hex8_assemble_uniform(
                 cn[q][t] = conn_r[8 * (int64_t)e + t];
                 if (cn[q][t] == r) av[q] = t;
             }
-            ev[q] = E[e];
+            ev[q] = k_scale * E[e];
             mv[q] = c_mass * rho[e];
         }
     }
@@ -324,10 +325,10 @@ void launch_hex8_reference(hipStream_t st, const int32_t* conn_r, const double* 
 void launch_hex8_assemble_uniform(hipStream_t st, int n_rows, int n_slices, const int32_t* sptr, const int32_t* scol,
                                   const int32_t* rp, const int32_t* n2e_ptr, const int32_t* n2e,
                                   const int32_t* conn_r, const double* Khat, const double* Mhat, const double* E,
-                                  const double* rho, double c_mass, double* K) {
+                                  const double* rho, double k_scale, double c_mass, double* K) {
     if (n_slices <= 0) return;
     hipLaunchKernelGGL(k_hex8_assemble_uniform, dim3((unsigned)((n_slices + 3) / 4)), dim3(256), 0, st, n_rows,
-                       n_slices, sptr, scol, rp, n2e_ptr, n2e, conn_r, Khat, Mhat, E, rho, c_mass, K);
+                       n_slices, sptr, scol, rp, n2e_ptr, n2e, conn_r, Khat, Mhat, E, rho, k_scale, c_mass, K);
 }
 
 // ---------------------------------------------------------------------------

@@ -39,12 +39,23 @@ void launch_hex8_assemble_rows(hipStream_t st, int n_rows, int n_slices, const i
 // Khat (64 x 9) and Mhat (64) of the element conn_r[0..7] with E = rho = 1
 void launch_hex8_reference(hipStream_t st, const int32_t* conn_r, const double* xyz_r, double nu, double* Khat,
                            double* Mhat);
-// brick8 by rows on a mesh of translated copies of one element (at most 8
-// elements per node): K = sum E_e Khat + c_mass rho_e Mhat, every slot written
+// brick8 / brickIM by rows on a mesh of translated copies of one element (at
+// most 8 elements per node): K = sum k_scale E_e Khat + c_mass rho_e Mhat,
+// every slot written
 void launch_hex8_assemble_uniform(hipStream_t st, int n_rows, int n_slices, const int32_t* sptr, const int32_t* scol,
                                   const int32_t* rp, const int32_t* n2e_ptr, const int32_t* n2e,
                                   const int32_t* conn_r, const double* Khat, const double* Mhat, const double* E,
-                                  const double* rho, double c_mass, double* K);
+                                  const double* rho, double k_scale, double c_mass, double* K);
+// brickIM (condensed bubbles): adds k_scale K_e + c_mass M_e per element colour;
+// reference blocks of the element conn_r[0..7] (E = rho = 1, its nu);
+// dJ/dE_e = -lam_e^T (K_e / E_e) u_e for every element
+void launch_brickim_assemble(hipStream_t st, const int32_t* elems, int n_elems, const int32_t* conn_r,
+                             const double* xyz_r, const double* E, const double* nu, const double* rho,
+                             double k_scale, double c_mass, const int32_t* sptr, const uint8_t* slot, double* K);
+void launch_brickim_reference(hipStream_t st, const int32_t* conn_r, const double* xyz_r, const double* nu,
+                              double* Khat, double* Mhat);
+void launch_brickim_sens(hipStream_t st, int n_el, const int32_t* conn_r, const double* xyz_r, const double* E,
+                         const double* nu, const double* u, const double* lam, double* sens);
 // shell4: adds k_scale K_e + m_scale M_e; conn_p holds the 8 pseudo-node rows
 // (2 per node: translations, rotations) of each element, xyz_r is per node
 void launch_shell4_assemble(hipStream_t st, const int32_t* elems, int n_elems, const int32_t* conn_p,

@@ -61,7 +61,7 @@ def cpu_baseline(prob, fp64_iters, nx, ny, nz):
     K = O.assemble_solid(8, sl.mesh.xyz, sl.mesh.conn, sl.E, sl.nu, sl.dof_index, sl.n_dof)
     t_asm = time.perf_counter() - t0
     sf = 1e5 * O.max_abs(K)
-    n_it = 150
+    n_it = 1500
     t0 = time.perf_counter()
     u, _, _ = O.jacobi_pcg(K, sl.pen_dofs, sf, sl.f, 1e-30, n_it)
     t_it = (time.perf_counter() - t0) / n_it

@@ -137,7 +137,8 @@ int asend_lognormal_moduli(uint64_t seed, int64_t n, double median, double sigma
  * element (isotropic; G = E/(2(1+nu)) as P/model/Material.py:24-26 writes
  * it).  rho may be NULL for static work.  Builds the node-block sparsity
  * pattern, the element colouring and the element-to-block map.
- * elem_type ASEND_BRICK8: conn holds 8 nodes per element, n_dof = 3 n_nodes.
+ * elem_type ASEND_BRICK8 or ASEND_BRICKIM (brick8 + 3 incompatible bubble modes
+ * condensed per element): conn holds 8 nodes per element, n_dof = 3 n_nodes.
  * elem_type ASEND_SHELL4: conn holds 4 nodes per element, n_dof = 6 n_nodes
  * (dof_index[6n + c] = 6 rank + c), one isotropic layer per element whose
  * thickness asend_set_thickness supplies; every element must lie in a plane
@@ -202,7 +203,9 @@ int asend_solve(asend_ctx* ctx, const double* rhs, double* x, const asend_solve_
 /* Compliance adjoint sensitivities for per-element modulus design variables:
  * dJdE[e] = -lambda_e^T (dK_e/dE_e) u_e, FP64 reduction.  Replaces the DV
  * loop of Model::get_obj_gradient with d_relasticd_d
- * (S/model_analysis.rs:1679-1701, :1379-1457).  brick8 only. */
+ * (S/model_analysis.rs:1679-1701, :1379-1457).  brick8 and brickIM (with the
+ * condensed bubbles: lambda_e^T (K_e / E_e) u_e, as update_internal's
+ * condensed internal DOFs give); not shell4. */
 int asend_element_sens(asend_ctx* ctx, const double* u, const double* lam, double* dJdE);
 
 /* The static compliance pipeline on resident data: assemble, solve K u = f,
