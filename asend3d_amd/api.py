@@ -111,7 +111,7 @@ EXPORTED = ["asend_ctx_create", "asend_ctx_destroy", "asend_last_error", "asend_
             "asend_learned_num_params", "asend_learned_init", "asend_learned_get_params",
             "asend_learned_set_params", "asend_learned_train", "asend_learned_eval", "asend_learned_apply",
             "asend_learned_edges", "asend_mg_set_smoother", "asend_learned_vcycle_eval", "asend_set_graphs",
-            "asend_newmark_setup", "asend_newmark_run", "asend_batch_create", "asend_batch_destroy",
+            "asend_newmark_setup", "asend_newmark_run", "asend_newmark_adjoint", "asend_batch_create", "asend_batch_destroy",
             "asend_batch_error", "asend_batch_solve", "asend_autotune"]
 
 
@@ -383,6 +383,21 @@ class Context:
                                          ctypes.byref(st)), "asend_newmark_run")
         return hist[:, :watch.size], u, v, a, st
 
+    def newmark_adjoint(self, n_steps, f_shape, load_fac, q, w, tol=1e-10, max_it_step=50):
+        """Forward sweep from rest and the transient adjoint of
+        L = sum_n w[n] q.u_n; returns (L, dL/dE per element, forward stats,
+        adjoint stats)."""
+        f_shape = _f64(f_shape)
+        load_fac = _f64(load_fac)
+        q = _f64(q)
+        w = _f64(w)
+        J = _dbl(0)
+        dJ = np.zeros(self.n_el)
+        fs, as_ = NewmarkStats(), NewmarkStats()
+        self._chk(_lib.asend_newmark_adjoint(self.h, _i64(n_steps), _p(f_shape), _p(load_fac), _p(q), _p(w),
+                                             _dbl(tol), ctypes.c_int(max_it_step), ctypes.byref(J), _p(dJ),
+                                             ctypes.byref(fs), ctypes.byref(as_)), "asend_newmark_adjoint")
+        return float(J.value), dJ, fs, as_
     def autotune(self, rhs=None, what=TUNE_SPMV | TUNE_SOLVER):
         t = Tuning()
         r = None if rhs is None else _f64(rhs)

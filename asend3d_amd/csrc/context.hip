@@ -116,6 +116,14 @@ struct asend_ctx {
         int64_t fac_cap = 0, hist_cap = 0, watch_cap = 0;
         hipGraphExec_t exec_head = nullptr, exec_more = nullptr, exec_tail = nullptr;
         int graph_iters = -1, graph_watch = -1;
+        // displacement history for the adjoint sweep (save_u: where the step
+        // tail stores u, null when not recording) and the adjoint vectors
+        double* save_u = nullptr;
+        const double* graph_save = nullptr;
+        double* uhist = nullptr;
+        int64_t uhist_cap = 0;
+        double *lam = nullptr, *aadj = nullptr, *vadj = nullptr, *mlam = nullptr, *qv = nullptr,
+               *sens_acc = nullptr;
     } nm;
 };
 
@@ -190,10 +198,13 @@ int write_scalars(asend_ctx* ctx, int idx) {
 
 void newmark_free(asend_ctx* ctx) {
     auto& nm = ctx->nm;
-    for (double** q : {&nm.M64, &nm.v, &nm.a, &nm.w, &nm.q, &nm.rhs0, &nm.rhs, &nm.fshape, &nm.fac, &nm.hist})
+    for (double** q : {&nm.M64, &nm.v, &nm.a, &nm.w, &nm.q, &nm.rhs0, &nm.rhs, &nm.fshape, &nm.fac, &nm.hist,
+                       &nm.uhist, &nm.lam, &nm.aadj, &nm.vadj, &nm.mlam, &nm.qv, &nm.sens_acc})
         dfree(*q);
     for (int64_t** q : {&nm.step, &nm.its, &nm.watch}) dfree(*q);
-    nm.fac_cap = nm.hist_cap = nm.watch_cap = 0;
+    nm.fac_cap = nm.hist_cap = nm.watch_cap = nm.uhist_cap = 0;
+    nm.save_u = nullptr;
+    nm.graph_save = nullptr;
     for (hipGraphExec_t* e : {&nm.exec_head, &nm.exec_more, &nm.exec_tail}) {
         if (*e) (void)hipGraphExecDestroy(*e);
         *e = nullptr;
@@ -1431,6 +1442,7 @@ int asend_newmark_run(asend_ctx* ctx, int64_t n_steps, const double* f_shape, co
     };
     auto tail = [&]() {
         launch_nm_update(ctx->st, n, ctx->d_x64, nm.w, nm.c_m, nm.dt, nm.gamma, ctx->d_u, nm.v, nm.a);
+        if (nm.save_u) launch_nm_save(ctx->st, n, nm.step, ctx->d_u, nm.save_u);
         launch_nm_record(ctx->st, nw, nm.watch, ctx->d_u, nm.hist, nm.step, nm.its, sc);
     };
     auto capture = [&](hipGraphExec_t* exec, const std::function<void()>& seq) -> int {
@@ -1454,12 +1466,13 @@ int asend_newmark_run(asend_ctx* ctx, int64_t n_steps, const double* f_shape, co
         return ASEND_OK;
     };
     const bool graph = ctx->use_graphs;
-    if (graph && (!nm.exec_head || nm.graph_iters != blk || nm.graph_watch != nw)) {
+    if (graph && (!nm.exec_head || nm.graph_iters != blk || nm.graph_watch != nw || nm.graph_save != nm.save_u)) {
         if ((rc = capture(&nm.exec_head, head)) || (rc = capture(&nm.exec_more, iterations)) ||
             (rc = capture(&nm.exec_tail, tail)))
             return rc;
         nm.graph_iters = blk;
         nm.graph_watch = nw;
+        nm.graph_save = nm.save_u;
     }
     HIPCHK(hipEventRecord(ctx->ev0, ctx->st));
     for (int64_t s = 0; s < n_steps; ++s) {
@@ -1499,6 +1512,75 @@ int asend_newmark_run(asend_ctx* ctx, int64_t n_steps, const double* f_shape, co
         st.device_ms = ms;
         *stats = st;
     }
+    return ASEND_OK;
+}
+
+int asend_newmark_adjoint(asend_ctx* ctx, int64_t n_steps, const double* f_shape, const double* load_fac,
+                          const double* q, const double* w, double tol, int max_it_step, double* J, double* dJdE,
+                          asend_newmark_stats* fwd_stats, asend_newmark_stats* adj_stats) {
+    if (!ctx || n_steps < 1 || !f_shape || !load_fac || !q || !w || !(tol > 0.0) || max_it_step < 1)
+        return ASEND_ERR_ARG;
+    auto& nm = ctx->nm;
+    if (!nm.ready) return fail(ctx, ASEND_ERR_STATE, "call asend_newmark_setup first");
+    if (ctx->etype == ASEND_SHELL4)
+        return fail(ctx, ASEND_ERR_UNSUPPORTED, "element sensitivities are implemented for brick8 and brickIM meshes");
+    const int64_t n = ctx->n_dof;
+    int rc;
+    if (n_steps * n > nm.uhist_cap) {
+        if ((rc = dalloc(ctx, &nm.uhist, n_steps * n))) return rc;
+        nm.uhist_cap = n_steps * n;
+    }
+    if (!nm.lam) {
+        if ((rc = dalloc(ctx, &nm.lam, n)) || (rc = dalloc(ctx, &nm.aadj, n)) || (rc = dalloc(ctx, &nm.vadj, n)) ||
+            (rc = dalloc(ctx, &nm.mlam, n)) || (rc = dalloc(ctx, &nm.qv, n)) || (rc = dalloc(ctx, &nm.sens_acc, ctx->n_el)))
+            return rc;
+    }
+    // forward sweep from rest, recording u after every step
+    nm.save_u = nm.uhist;
+    rc = asend_newmark_run(ctx, n_steps, f_shape, load_fac, nullptr, nullptr, nullptr, 0, nullptr, nullptr, nullptr,
+                           nullptr, nullptr, tol, max_it_step, fwd_stats);
+    nm.save_u = nullptr;
+    if (rc) return rc;
+    // backward sweep (Model::get_obj_gradient, dynamic branch,
+    // S/model_analysis.rs:1563-1631): one adjoint solve with K_eff per step
+    const double c1 = -nm.c_m;  // 1 / (dt^2 (beta - gamma))
+    HIPCHK(hipMemcpy(nm.qv, q, n * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemsetAsync(nm.aadj, 0, n * 8, ctx->st));
+    HIPCHK(hipMemsetAsync(nm.vadj, 0, n * 8, ctx->st));
+    HIPCHK(hipMemsetAsync(nm.sens_acc, 0, ctx->n_el * 8, ctx->st));
+    const int check = ctx->tuned_check > 0 ? ctx->tuned_check : 16;
+    double Jsum = 0.0;
+    asend_newmark_stats st;
+    std::memset(&st, 0, sizeof(st));
+    st.steps = n_steps;
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->st));
+    for (int64_t k = n_steps - 1; k >= 0; --k) {
+        const double* uk = nm.uhist + k * n;
+        const int last = (k == n_steps - 1) ? 1 : 0;
+        if (!last) launch_sell3<double>(ctx->st, 0, ctx->fine(), nm.M64, nm.lam, nm.mlam, nullptr, nullptr, nullptr);
+        launch_adj_prep(ctx->st, n, nm.qv, w[k], c1, nm.c2, nm.dt, nm.gamma, last, nm.mlam, nm.aadj, nm.vadj, nm.rhs);
+        HIPCHK(hipGetLastError());
+        if ((rc = run_jacobi_pcg<double>(ctx, 1, nm.rhs, nm.lam, tol, false, max_it_step, check))) return rc;
+        const Scalars& h = ctx->h_sc[1];
+        st.total_iterations += h.it;
+        st.max_step_iterations = std::max<int64_t>(st.max_step_iterations, h.it);
+        if (!(h.rr <= h.tol2 * h.ff)) st.unconverged_steps += 1;
+        // dJ/dE += -lam_k^T (dK/dE) u_k ; J += w_k q.u_k
+        launch_sens(ctx, uk, nm.lam);
+        launch_axpy64(ctx->st, ctx->n_el, 1.0, ctx->d_sens, nm.sens_acc);
+        launch_dot64(ctx->st, n, nm.qv, uk, ctx->d_partials, ctx->d_sc + 2);
+        HIPCHK(hipGetLastError());
+        if ((rc = read_scalars(ctx, 2))) return rc;
+        Jsum += w[k] * ctx->h_sc[2].dot;
+    }
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->st));
+    HIPCHK(hipEventSynchronize(ctx->ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    st.device_ms = ms;
+    if (J) *J = Jsum;
+    if (dJdE) HIPCHK(hipMemcpy(dJdE, nm.sens_acc, ctx->n_el * 8, hipMemcpyDeviceToHost));
+    if (adj_stats) *adj_stats = st;
     return ASEND_OK;
 }
 

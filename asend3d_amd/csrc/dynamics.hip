@@ -70,6 +70,40 @@ __global__ __launch_bounds__(256) void k_nm_record(int n_watch, const int64_t* _
     }
 }
 
+// uhist[step] = u, the displacement history kept for the adjoint sweep
+// (launched before k_nm_record advances the step counter)
+__global__ void k_nm_save(int64_t n, const int64_t* __restrict__ step, const double* __restrict__ u,
+                          double* __restrict__ uhist) {
+    double* dst = uhist + (*step) * n;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = u[i];
+}
+
+// One backward step of the transient adjoint (no damping).  Objective
+// partials at this step: dL/du = wn q, dL/dv = dL/da = 0.  Unless this is the
+// last step, the coupling to step n+1 (augmentd_ld_u, S/model_analysis.rs:
+// 936-1063) with t = a_adj - M lam_{n+1}:
+//   gu += c1 t ; ga = c1 c2 t + dt (1 - gamma) v_adj ; gv = c1 dt t + v_adj
+// then (solve_for_adjoint, :1089-1100):
+//   v_adj = gv ; a_adj = ga + dt gamma v_adj ; rhs = gu - c1 a_adj.
+__global__ void k_adj_prep(int64_t n, const double* __restrict__ q, double wn, double c1, double c2, double dt,
+                           double gamma, int last, const double* __restrict__ mlam, double* __restrict__ aadj,
+                           double* __restrict__ vadj, double* __restrict__ rhs) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double gu = wn * q[i], ga = 0.0, gv = 0.0;
+        if (!last) {
+            const double t = aadj[i] - mlam[i];
+            gu += c1 * t;
+            ga = c1 * c2 * t + dt * (1.0 - gamma) * vadj[i];
+            gv = c1 * dt * t + vadj[i];
+        }
+        const double va = gv, aa = ga + dt * gamma * va;
+        vadj[i] = va;
+        aadj[i] = aa;
+        rhs[i] = gu - c1 * aa;
+    }
+}
+
 int nb(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 4096); }
 
 }  // namespace
@@ -87,6 +121,16 @@ void launch_nm_rhs(hipStream_t st, int64_t n, const int64_t* step, const double*
 void launch_nm_update(hipStream_t st, int64_t n, const double* du, const double* w, double c_m, double dt,
                       double gamma, double* u, double* v, double* a) {
     hipLaunchKernelGGL(k_nm_update, dim3(nb(n)), dim3(256), 0, st, n, du, w, c_m, dt, gamma, u, v, a);
+}
+
+void launch_nm_save(hipStream_t st, int64_t n, const int64_t* step, const double* u, double* uhist) {
+    hipLaunchKernelGGL(k_nm_save, dim3(nb(n)), dim3(256), 0, st, n, step, u, uhist);
+}
+
+void launch_adj_prep(hipStream_t st, int64_t n, const double* q, double wn, double c1, double c2, double dt,
+                     double gamma, int last, const double* mlam, double* aadj, double* vadj, double* rhs) {
+    hipLaunchKernelGGL(k_adj_prep, dim3(nb(n)), dim3(256), 0, st, n, q, wn, c1, c2, dt, gamma, last, mlam, aadj, vadj,
+                       rhs);
 }
 
 void launch_nm_record(hipStream_t st, int n_watch, const int64_t* watch, const double* u, double* hist,

@@ -111,5 +111,8 @@ void launch_nm_update(hipStream_t st, int64_t n, const double* du, const double*
                       double gamma, double* u, double* v, double* a);
 void launch_nm_record(hipStream_t st, int n_watch, const int64_t* watch, const double* u, double* hist,
                       int64_t* step, int64_t* its, const Scalars* sc);
+void launch_nm_save(hipStream_t st, int64_t n, const int64_t* step, const double* u, double* uhist);
+void launch_adj_prep(hipStream_t st, int64_t n, const double* q, double wn, double c1, double c2, double dt,
+                     double gamma, int last, const double* mlam, double* aadj, double* vadj, double* rhs);
 
 }  // namespace asend

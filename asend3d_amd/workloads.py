@@ -7,6 +7,9 @@
               Jacobi PCG.
   newmark  -- Newmark time stepping of a cantilever with a half-sine tip
               pulse, each step replayed as a hipGraph.
+  adjoint  -- the newmark workload followed by the transient adjoint of the
+              tip deflection summed over the history, with dL/dE for every
+              element (one K_eff solve per step backwards).
   shell    -- Newmark time stepping of a clamped shell4 plate (1 x 1 m,
               256 x 256 elements, t = 5 mm, aluminium) under a 1 ms half-sine
               10 kN centre load, dt = 1e-5 s, 1000 steps, hipGraph steps.
@@ -14,6 +17,7 @@
     python -m asend3d_amd.workloads batched [--variants 64] [--sample 2]
     python -m asend3d_amd.workloads newmark [--nx 100 --ny 20 --nz 20 --steps 1000]
     python -m asend3d_amd.workloads shell [--nx 256 --ny 256 --steps 1000]
+    python -m asend3d_amd.workloads adjoint [--nx 100 --ny 20 --nz 20 --steps 1000]
 
 Each prints one JSON line.
 """
@@ -101,6 +105,30 @@ def newmark(nx=100, ny=20, nz=20, steps=1000, dt=1e-5, graphs=True, tol=1e-8, ma
             "tip_w_min": float(hist.mean(axis=1).min())}
 
 
+def adjoint(nx=100, ny=20, nz=20, steps=1000, dt=1e-5, graphs=True, tol=1e-8, max_it=400):
+    from .problems import cantilever
+    p = cantilever(nx, ny, nz, lx=1.0, ly=0.2, lz=0.2, E=70e9, nu=0.33, traction=(0.0, 0.0, -1.0e6))
+    ctx = api.Context(0)
+    problems.upload(ctx, p, rho=np.full(p.mesh.n_el, 2700.0))
+    ctx.set_graphs(graphs)
+    ctx.newmark_setup(dt, 0.25, 0.5)
+    n_pulse = int(round(1e-3 / dt))
+    fac = np.where(np.arange(steps) < n_pulse, np.sin(np.pi * np.arange(steps) / n_pulse), 0.0)
+    di = np.asarray(p.dof_index).reshape(-1, 6)
+    q = np.zeros(p.n_dof)
+    q[di[p.mesh.nodes_at_x(nx), 2]] = 1.0
+    w = np.full(steps, dt)
+    log(f"adjoint {nx}x{ny}x{nz}: {p.n_dof} DOF, {steps} steps, history {steps * p.n_dof * 8 / 2**30:.2f} GiB")
+    L, dL, fst, ast = ctx.newmark_adjoint(steps, p.f, fac, q, w, tol=tol, max_it_step=max_it)
+    ctx.close()
+    return {"workload": "newmark-adjoint", "mesh": f"{nx}x{ny}x{nz} brick8", "dof": int(p.n_dof), "steps": steps,
+            "dt": dt, "objective": "sum_n dt * (sum of tip w)_n", "L": L,
+            "forward_s": fst.device_ms / 1e3, "adjoint_s": ast.device_ms / 1e3,
+            "forward_pcg_iterations": int(fst.total_iterations), "adjoint_pcg_iterations": int(ast.total_iterations),
+            "unconverged_steps": int(fst.unconverged_steps + ast.unconverged_steps),
+            "dLdE_abs_max": float(np.abs(dL).max()), "dLdE_sum_E": float(np.sum(dL * p.E)), "step_tol": tol}
+
+
 def shell(nx=256, ny=256, steps=1000, dt=1e-5, graphs=True, tol=1e-8, max_it=2000, load=1.0e4):
     p = problems.shell_plate(nx, ny, 1.0, 1.0, 0.005, 70e9, 0.33, 2700.0)
     ctx = api.Context(0)
@@ -133,7 +161,7 @@ def shell(nx=256, ny=256, steps=1000, dt=1e-5, graphs=True, tol=1e-8, max_it=200
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("which", choices=["batched", "newmark", "shell"])
+    ap.add_argument("which", choices=["batched", "newmark", "shell", "adjoint"])
     ap.add_argument("--variants", type=int, default=64)
     ap.add_argument("--sample", type=int, default=2)
     ap.add_argument("--nx", type=int, default=None, help="default 100 (newmark), 256 (shell)")
@@ -144,6 +172,8 @@ def main():
     a = ap.parse_args()
     if a.which == "batched":
         out = batched(a.variants, a.sample)
+    elif a.which == "adjoint":
+        out = adjoint(a.nx or 100, a.ny or 20, a.nz, a.steps, graphs=not a.no_graphs)
     elif a.which == "shell":
         out = shell(a.nx or 256, a.ny or 256, a.steps, graphs=not a.no_graphs)
     else:

@@ -284,6 +284,24 @@ int asend_newmark_run(asend_ctx* ctx, int64_t n_steps, const double* f_shape, co
                       double* hist, double* u_out, double* v_out, double* a_out, double tol, int max_it_step,
                       asend_newmark_stats* stats);
 
+/* Transient adjoint: replaces the dynamic branch of Model::get_obj_gradient
+ * (S/model_analysis.rs:1563-1631) with augmentd_ld_u (:936-1063) and the
+ * dynamic elastic part of solve_for_adjoint (:1089-1100), no damping, for the
+ * objective L = sum_n w[n] q^T u_n (displacement objective terms of exponent
+ * 1, S/objective_meth.rs:618-663).  Runs the forward sweep from rest
+ * (load_fac[n] f_shape as asend_newmark_run, keeping u_n of every step in
+ * HBM: n_steps * n_dof doubles), then for n = N..1
+ *   t = a_adj - M lam_{n+1} (not on the last step),
+ *   v_adj = c1 dt t + v_adj, a_adj = c1 c2 t + dt (1 - gamma) v_adj_old + dt gamma v_adj,
+ *   K_eff lam_n = w[n] q + c1 t - c1 a_adj  (FP64 Jacobi PCG to tol),
+ *   dJdE += -lam_n^T (dK/dE) u_n,
+ * with c1 = 1/(dt^2 (beta - gamma)), c2 = dt^2 (0.5 + beta - gamma).  The
+ * reference evaluates every step's objective terms at the final time; for
+ * these objectives the time does not enter.  brick8 and brickIM meshes. */
+int asend_newmark_adjoint(asend_ctx* ctx, int64_t n_steps, const double* f_shape, const double* load_fac,
+                          const double* q, const double* w, double tol, int max_it_step, double* J, double* dJdE,
+                          asend_newmark_stats* fwd_stats, asend_newmark_stats* adj_stats);
+
 /* ---- multigrid diagnostics (no reference counterpart) --------------------
  * Host FP32 arrays.  Level 0 uses the fine DOF numbering, coarser levels the
  * lexicographic node order of their grid.  Used to verify on the device that

@@ -22,10 +22,22 @@ uses the node acceleration as initialised (a0), because update_vel_acc has
 not run yet; first_step="predictor" uses a(u_0) like every later step (the
 device path).  From rest (u0 = v0 = a0 = 0) the two coincide.
 
+newmark_adjoint restates the backward sweep of Model::get_obj_gradient for a
+dynamic elastic job without damping (S/model_analysis.rs:1563-1631): for
+n = N..1, augmentd_ld_u (:936-1063) adds step n+1's coupling to step n,
+solve_for_adjoint (:1089-1100) forms the velocity and acceleration adjoints
+and solves K_eff lam_n = dL/du_n, and the DV loop accumulates
+-lam_n^T dR_n/dd with the state u_n (d_relasticd_d, :1379-1457).  The
+objective is a linear functional of the displacement history,
+L = sum_n w_n q^T u_n (displacement terms, S/objective_meth.rs:618-663, with
+exponent 1).
+
 Pinning (tests/test_dynamics_oracle.py): single-DOF free vibration against
 the closed form u_n = cos(n phi), tan(phi/2) = omega dt / 2 of the average
 acceleration rule; energy conservation of a free brick8 mesh; the two
-first-step modes agree from rest.
+first-step modes agree from rest; newmark_adjoint against central finite
+differences of the forward run (penalty scale held fixed, as the reference
+does not differentiate it).
 """
 import numpy as np
 import scipy.sparse as sp
@@ -72,3 +84,45 @@ def newmark(K, M, f_shape, load_fac, dt, beta=0.25, gamma=0.5, u0=None, v0=None,
         if on_step is not None:
             on_step(step, u, v, a, Kp)
     return hist, u, v, a
+
+
+def newmark_adjoint(K, M, f_shape, load_fac, dt, beta, gamma, q, w, elem_sens, pen_dofs=(), sf=None):
+    """Forward Newmark run from rest, then the reference's backward adjoint
+    sweep.  elem_sens(u, lam) -> per-element -lam^T (dK_e/dE_e) u.  Returns
+    (L, dL/dE) with L = sum_n w[n] q.u_n."""
+    K = sp.csr_matrix(K)
+    M = sp.csr_matrix(M)
+    n = K.shape[0]
+    c1 = 1.0 / (dt * dt * (beta - gamma))
+    c2 = dt * dt * (0.5 + beta - gamma)
+    keff = (K - c1 * M).tocsr()
+    pen = np.zeros(n)
+    if len(pen_dofs):
+        if sf is None:
+            sf = 1e5 * abs(keff).max()
+        np.add.at(pen, np.asarray(pen_dofs, dtype=np.int64), sf)
+    us = []
+    newmark(K, M, f_shape, load_fac, dt, beta, gamma, pen_dofs=pen_dofs, sf=sf,
+            on_step=lambda k, u, v, a, Kp: us.append(u.copy()))
+    q = np.asarray(q, dtype=np.float64)
+    L = float(sum(w[k] * (q @ us[k]) for k in range(len(us))))
+    solve = spla.splu((keff + sp.diags(pen)).tocsc()).solve
+    lam = None
+    a_adj = np.zeros(n)
+    v_adj = np.zeros(n)
+    dL = None
+    for k in reversed(range(len(us))):
+        gu = w[k] * q
+        ga = np.zeros(n)
+        gv = np.zeros(n)
+        if lam is not None:
+            t = a_adj - M @ lam
+            gu = gu + c1 * t
+            ga = c1 * c2 * t + dt * (1.0 - gamma) * v_adj
+            gv = c1 * dt * t + v_adj
+        v_adj = gv
+        a_adj = ga + dt * gamma * v_adj
+        lam = solve(gu - c1 * a_adj)
+        s = elem_sens(us[k], lam)
+        dL = s if dL is None else dL + s
+    return L, dL

@@ -57,3 +57,44 @@ def test_reference_first_step_equals_predictor_from_rest():
     h2, *_ = dynamics.newmark(K, M, p.f, fac, 1e-5, pen_dofs=p.pen_dofs, watch=[p.n_dof - 1],
                               first_step="predictor")
     assert np.array_equal(h1, h2)
+
+
+def test_adjoint_matches_finite_differences():
+    # L = sum_n w_n q.u_n (the tip deflection weighted over the history);
+    # central differences of the forward run in single element moduli, with
+    # the penalty scale held at the unperturbed value
+    p = problems.cantilever(6, 2, 2, E=70e9)
+    rng = np.random.default_rng(3)
+    E = 70e9 * np.exp(0.4 * rng.standard_normal(p.mesh.n_el))
+    rho = np.full(p.mesh.n_el, 2700.0)
+
+    def km(Ev):
+        K = O.assemble_solid(8, p.mesh.xyz, p.mesh.conn, Ev, p.nu, p.dof_index, p.n_dof)
+        M = O.assemble_solid_mass(8, p.mesh.xyz, p.mesh.conn, rho, p.dof_index, K)
+        return K.to_scipy(), M.to_scipy()
+
+    K, M = km(E)
+    dt, n = 2e-5, 24
+    fac = np.where(np.arange(n) < 12, np.sin(np.pi * np.arange(n) / 12.0), 0.0)
+    di = np.asarray(p.dof_index).reshape(-1, 6)
+    q = np.zeros(p.n_dof)
+    q[di[p.mesh.nodes_at_x(6), 2]] = 1.0
+    w = 1.0 + 0.1 * np.arange(n)
+    c1 = 1.0 / (dt * dt * (0.25 - 0.5))
+    sf = 1e5 * abs(K - c1 * M).max()
+    sens = lambda u, lam: O.solid_sens(8, p.mesh.xyz, p.mesh.conn, E, p.nu, p.dof_index, u, lam)
+    L, dL = dynamics.newmark_adjoint(K, M, p.f, fac, dt, 0.25, 0.5, q, w, sens, pen_dofs=p.pen_dofs, sf=sf)
+    for e in (0, 7, 23):
+        h = 1e-5 * E[e]
+        Lp, Lm = [], []
+        for sgn, out in ((1.0, Lp), (-1.0, Lm)):
+            Ev = E.copy()
+            Ev[e] += sgn * h
+            Kv, Mv = km(Ev)
+            hist, *_ = dynamics.newmark(Kv, Mv, p.f, fac, dt, pen_dofs=p.pen_dofs, sf=sf, watch=np.flatnonzero(q))
+            out.append(float(np.sum(w[:, None] * hist)))
+        fd = (Lp[0] - Lm[0]) / (2 * h)
+        assert abs(dL[e] - fd) <= 1e-5 * np.abs(dL).max(), (e, dL[e], fd)
+    # the objective itself is the weighted history
+    hist, *_ = dynamics.newmark(K, M, p.f, fac, dt, pen_dofs=p.pen_dofs, sf=sf, watch=np.flatnonzero(q))
+    assert np.isclose(L, np.sum(w[:, None] * hist), rtol=1e-12)

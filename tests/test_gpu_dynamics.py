@@ -64,3 +64,30 @@ def test_newmark_graph_replay_matches_eager(ctx):
     for r in runs[1:]:
         for x, x0 in zip(r, runs[0]):
             assert np.array_equal(x, x0)
+
+
+@pytest.mark.parametrize("etype", [8, 81])
+def test_newmark_adjoint_matches_oracle(ctx, etype):
+    # L = sum_n w_n q.u_n (tip deflection over the history); the device's
+    # forward + backward sweep against the oracle's (itself pinned by finite
+    # differences of the forward run)
+    gpu_case(f"newmark adjoint etype {etype}")
+    p = problems.cantilever(12, 3, 3, E=70e9 * np.exp(0.3 * np.random.default_rng(5).standard_normal(108)),
+                            etype=etype)
+    rho = np.full(p.mesh.n_el, RHO)
+    K = O.assemble_solid(etype, p.mesh.xyz, p.mesh.conn, p.E, p.nu, p.dof_index, p.n_dof)
+    M = O.assemble_solid_mass(etype, p.mesh.xyz, p.mesh.conn, rho, p.dof_index, K)
+    dt, n = 5e-5, 30
+    fac = np.where(np.arange(n) < 15, np.sin(np.pi * np.arange(n) / 15.0), 0.0)
+    q = np.zeros(p.n_dof)
+    q[tip_dofs(p)] = 1.0
+    w = 1.0 + 0.05 * np.arange(n)
+    sens = lambda u, lam: O.solid_sens(etype, p.mesh.xyz, p.mesh.conn, p.E, p.nu, p.dof_index, u, lam)
+    L_o, dL_o = dynamics.newmark_adjoint(K.to_scipy(), M.to_scipy(), p.f, fac, dt, 0.25, 0.5, q, w, sens,
+                                         pen_dofs=p.pen_dofs)
+    problems.upload(ctx, p, rho=rho)
+    ctx.newmark_setup(dt, 0.25, 0.5)
+    L, dL, fst, ast = ctx.newmark_adjoint(n, p.f, fac, q, w, tol=1e-12, max_it_step=400)
+    assert fst.unconverged_steps == 0 and ast.unconverged_steps == 0, (fst.as_dict(), ast.as_dict())
+    assert abs(L - L_o) <= 1e-8 * abs(L_o)
+    assert np.abs(dL - dL_o).max() <= 1e-7 * np.abs(dL_o).max()
