@@ -52,6 +52,16 @@ class NewmarkStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ModalStats(ctypes.Structure):
+    _fields_ = [("lanczos_steps", ctypes.c_int64), ("modes_found", ctypes.c_int64),
+                ("solve_iterations", ctypes.c_int64), ("max_ritz_residual", ctypes.c_double),
+                ("masses_raised", ctypes.c_int64), ("unconverged_solves", ctypes.c_int64),
+                ("device_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class BatchStats(ctypes.Structure):
     _fields_ = [("outer_iterations", ctypes.c_int64), ("lockstep_iterations", ctypes.c_int64),
                 ("system_iterations", ctypes.c_int64), ("unconverged", ctypes.c_int64),
@@ -111,7 +121,7 @@ EXPORTED = ["asend_ctx_create", "asend_ctx_destroy", "asend_last_error", "asend_
             "asend_learned_num_params", "asend_learned_init", "asend_learned_get_params",
             "asend_learned_set_params", "asend_learned_train", "asend_learned_eval", "asend_learned_apply",
             "asend_learned_edges", "asend_mg_set_smoother", "asend_learned_vcycle_eval", "asend_set_graphs",
-            "asend_newmark_setup", "asend_newmark_run", "asend_newmark_adjoint", "asend_batch_create", "asend_batch_destroy",
+            "asend_newmark_setup", "asend_newmark_run", "asend_newmark_adjoint", "asend_modal", "asend_batch_create", "asend_batch_destroy",
             "asend_batch_error", "asend_batch_solve", "asend_autotune"]
 
 
@@ -398,6 +408,19 @@ class Context:
                                              _dbl(tol), ctypes.c_int(max_it_step), ctypes.byref(J), _p(dJ),
                                              ctypes.byref(fs), ctypes.byref(as_)), "asend_newmark_adjoint")
         return float(J.value), dJ, fs, as_
+
+    def modal(self, n_modes, shift=0.0, opts=None, vectors=True):
+        """Natural frequencies: (eigenvalues, frequencies in Hz, vectors
+        [n_modes x n_dof] or None, ModalStats)."""
+        if opts is None:
+            opts = self.opts(SOLVE_PCG_FP64, PC_JACOBI, tol=1e-11, max_it=2000000)
+        vals = np.zeros(n_modes)
+        fr = np.zeros(n_modes)
+        vecs = np.zeros((n_modes, self.n_dof)) if vectors else None
+        st = ModalStats()
+        self._chk(_lib.asend_modal(self.h, int(n_modes), _dbl(shift), ctypes.byref(opts), _p(vals), _p(fr),
+                                   _p(vecs) if vectors else None, ctypes.byref(st)), "asend_modal")
+        return vals, fr, vecs, st
     def autotune(self, rhs=None, what=TUNE_SPMV | TUNE_SOLVER):
         t = Tuning()
         r = None if rhs is None else _f64(rhs)

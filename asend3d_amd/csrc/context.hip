@@ -273,6 +273,11 @@ void free_mesh(asend_ctx* ctx) {
     ctx->assembled = false;
 }
 
+__global__ void k_scale_inplace(int64_t n, double s, double* __restrict__ x) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] *= s;
+}
+
 __global__ void k_sub_f32(int64_t n, const float* __restrict__ a, const float* __restrict__ b,
                           float* __restrict__ c) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -1581,6 +1586,252 @@ int asend_newmark_adjoint(asend_ctx* ctx, int64_t n_steps, const double* f_shape
     if (J) *J = Jsum;
     if (dJdE) HIPCHK(hipMemcpy(dJdE, nm.sens_acc, ctx->n_el * 8, hipMemcpyDeviceToHost));
     if (adj_stats) *adj_stats = st;
+    return ASEND_OK;
+}
+
+// Eigenpairs of a small dense symmetric matrix A (m x m, row-major, destroyed)
+// by cyclic Jacobi rotations: values into w, eigenvectors into the columns of
+// Z.  Used on the Lanczos tridiagonal (m <= a few hundred).
+static void jacobi_eigen(int m, std::vector<double>& A, std::vector<double>& w, std::vector<double>& Z) {
+    Z.assign((size_t)m * m, 0.0);
+    for (int i = 0; i < m; ++i) Z[(size_t)i * m + i] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int i = 0; i < m; ++i)
+            for (int j = 0; j < m; ++j) {
+                const double a = A[(size_t)i * m + j] * A[(size_t)i * m + j];
+                tot += a;
+                if (i != j) off += a;
+            }
+        if (off <= 1e-30 * tot) break;
+        for (int p = 0; p < m - 1; ++p)
+            for (int q = p + 1; q < m; ++q) {
+                const double apq = A[(size_t)p * m + q];
+                if (apq == 0.0) continue;
+                const double app = A[(size_t)p * m + p], aqq = A[(size_t)q * m + q];
+                // rotation angle that zeroes A_pq: tan(2 phi) = 2 A_pq / (A_qq - A_pp)
+                const double tau = (aqq - app) / (2.0 * apq);
+                const double t = (tau >= 0.0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+                const double c = 1.0 / std::sqrt(1.0 + t * t), s = t * c;
+                for (int k = 0; k < m; ++k) {
+                    const double akp = A[(size_t)k * m + p], akq = A[(size_t)k * m + q];
+                    A[(size_t)k * m + p] = c * akp - s * akq;
+                    A[(size_t)k * m + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < m; ++k) {
+                    const double apk = A[(size_t)p * m + k], aqk = A[(size_t)q * m + k];
+                    A[(size_t)p * m + k] = c * apk - s * aqk;
+                    A[(size_t)q * m + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < m; ++k) {
+                    const double zkp = Z[(size_t)k * m + p], zkq = Z[(size_t)k * m + q];
+                    Z[(size_t)k * m + p] = c * zkp - s * zkq;
+                    Z[(size_t)k * m + q] = s * zkp + c * zkq;
+                }
+            }
+    }
+    w.resize(m);
+    for (int i = 0; i < m; ++i) w[i] = A[(size_t)i * m + i];
+}
+
+int asend_modal(asend_ctx* ctx, int n_modes, double shift, const asend_solve_opts* opts, double* eig_vals,
+                double* freqs_hz, double* eig_vecs, asend_modal_stats* stats) {
+    if (!ctx || n_modes < 1 || !opts || !eig_vals) return ASEND_ERR_ARG;
+    if (!ctx->d_K64) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
+    if (ctx->etype == ASEND_SHELL4 && !ctx->thk_set)
+        return fail(ctx, ASEND_ERR_STATE, "shell4 mesh without thickness (asend_set_thickness)");
+    const int64_t n = ctx->n_dof, nval = ctx->nval();
+    const int m = (int)std::min<int64_t>({(int64_t)std::max(5 * n_modes, n_modes + 20), (int64_t)modal_max_vectors() - 1, n});
+    if (n_modes > m) return fail(ctx, ASEND_ERR_ARG, "too many modes requested");
+    int rc;
+    // lumped mass: row sums of the consistent mass, small entries raised to
+    // 1e-6 of the mean as the reference does (Model::eigen_solve,
+    // S/model_analysis.rs:744-773)
+    double *Mtmp = nullptr, *ones = nullptr, *dm = nullptr, *ds = nullptr, *dsi = nullptr, *V = nullptr,
+           *w = nullptr, *c = nullptr, *part = nullptr;
+    auto cleanup = [&]() {
+        for (double** p : {&Mtmp, &ones, &dm, &ds, &dsi, &V, &w, &c, &part}) dfree(*p);
+    };
+    const int nb = (int)std::min<int64_t>((n + 255) / 256, 1024);
+    if ((rc = dalloc(ctx, &Mtmp, nval)) || (rc = dalloc(ctx, &ones, n)) || (rc = dalloc(ctx, &dm, n)) ||
+        (rc = dalloc(ctx, &ds, n)) || (rc = dalloc(ctx, &dsi, n)) || (rc = dalloc(ctx, &V, (size_t)(m + 1) * n)) ||
+        (rc = dalloc(ctx, &w, n)) || (rc = dalloc(ctx, &c, m + 1)) || (rc = dalloc(ctx, &part, (size_t)nb * (m + 1)))) {
+        cleanup();
+        return rc;
+    }
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->st));
+    // consistent mass alone (zero moduli for the brick8 kernels, no stiffness
+    // term for the kernels that condense internal DOFs)
+    if (ctx->etype == ASEND_SHELL4 || ctx->etype == ASEND_BRICKIM) {
+        rc = integrate_elements(ctx, ctx->d_E, 0.0, 1.0, Mtmp);
+    } else {
+        double* zeroE = nullptr;
+        if (!(rc = dalloc(ctx, &zeroE, ctx->n_el))) {
+            HIPCHK(hipMemsetAsync(zeroE, 0, ctx->n_el * 8, ctx->st));
+            rc = integrate_elements(ctx, zeroE, 1.0, 1.0, Mtmp);
+            HIPCHK(hipStreamSynchronize(ctx->st));
+            dfree(zeroE);
+        }
+    }
+    if (rc) {
+        cleanup();
+        return rc;
+    }
+    {
+        std::vector<double> h1(n, 1.0);
+        HIPCHK(hipMemcpy(ones, h1.data(), n * 8, hipMemcpyHostToDevice));
+    }
+    launch_sell3<double>(ctx->st, 0, ctx->fine(), Mtmp, ones, dm, nullptr, nullptr, nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ctx->st));  // the context stream does not order null-stream copies
+    std::vector<double> hm(n);
+    HIPCHK(hipMemcpy(hm.data(), dm, n * 8, hipMemcpyDeviceToHost));
+    double m_avg = 0.0;
+    for (double v : hm) m_avg += std::fabs(v);
+    m_avg /= (double)n;
+    if (!(m_avg > 0.0)) {
+        cleanup();
+        return fail(ctx, ASEND_ERR_STATE, "modal analysis needs element densities (upload rho)");
+    }
+    int n_raised = 0;
+    for (double& v : hm)
+        if (v < 1e-6 * m_avg) {
+            v = 1e-6 * m_avg;
+            ++n_raised;
+        }
+    {
+        std::vector<double> hs(n), hsi(n);
+        for (int64_t i = 0; i < n; ++i) {
+            hs[i] = std::sqrt(hm[i]);
+            hsi[i] = 1.0 / hs[i];
+        }
+        HIPCHK(hipMemcpy(dm, hm.data(), n * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(ds, hs.data(), n * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dsi, hsi.data(), n * 8, hipMemcpyHostToDevice));
+    }
+    dfree(Mtmp);
+    // shifted operator K_eff - shift diag(m) (penalty included)
+    if ((rc = assemble_device(ctx, 0.0))) {
+        cleanup();
+        return rc;
+    }
+    if (shift != 0.0) {
+        launch_add_diag(ctx->st, (int)ctx->n_nodes, ctx->d_sptr, ctx->d_kdiag, ctx->d_K64, dm, -shift);
+        launch_diag_inv(ctx->st, (int)ctx->n_nodes, ctx->d_sptr, ctx->d_kdiag, ctx->d_K64, ctx->d_dinv64,
+                        ctx->d_dinv32);
+        launch_to_f32(ctx->st, ctx->d_K64, ctx->d_K32, nval);
+        HIPCHK(hipGetLastError());
+    }
+    // Lanczos on B = S (K - shift M)^-1 S, S = diag(sqrt(m)), with full
+    // reorthogonalisation (two classical Gram-Schmidt passes per step)
+    {
+        std::vector<double> v0(n);
+        double nrm = 0.0;
+        for (int64_t i = 0; i < n; ++i) {
+            v0[i] = std::sin((double)(i + 1));
+            nrm += v0[i] * v0[i];
+        }
+        nrm = 1.0 / std::sqrt(nrm);
+        for (double& x : v0) x *= nrm;
+        HIPCHK(hipMemcpy(V, v0.data(), n * 8, hipMemcpyHostToDevice));
+    }
+    std::vector<double> alpha, beta;
+    int64_t solve_its = 0, unconverged = 0;
+    int steps = 0;
+    std::vector<double> hc(m + 1), hc2(m + 1);
+    for (int j = 0; j < m; ++j) {
+        double* vj = V + (int64_t)j * n;
+        launch_vmul(ctx->st, n, ds, vj, ctx->d_f);
+        asend_solve_stats st;
+        if ((rc = solve_device(ctx, opts, &st))) {
+            cleanup();
+            return rc;
+        }
+        if (!st.converged) ++unconverged;  // reported, as the reference warns and goes on
+        solve_its += st.iterations;
+        launch_vmul(ctx->st, n, ds, ctx->d_u, w);
+        launch_vtw(ctx->st, n, j + 1, V, w, part, nb, c);
+        HIPCHK(hipMemcpyAsync(hc.data(), c, (j + 1) * 8, hipMemcpyDeviceToHost, ctx->st));
+        launch_add_vc(ctx->st, n, j + 1, V, c, -1.0, w);
+        launch_vtw(ctx->st, n, j + 1, V, w, part, nb, c);
+        HIPCHK(hipMemcpyAsync(hc2.data(), c, (j + 1) * 8, hipMemcpyDeviceToHost, ctx->st));
+        launch_add_vc(ctx->st, n, j + 1, V, c, -1.0, w);
+        launch_dot64(ctx->st, n, w, w, ctx->d_partials, ctx->d_sc + 2);
+        HIPCHK(hipGetLastError());
+        if ((rc = read_scalars(ctx, 2))) {
+            cleanup();
+            return rc;
+        }
+        const double a = hc[j] + hc2[j];
+        const double b = std::sqrt(ctx->h_sc[2].dot);
+        alpha.push_back(a);
+        steps = j + 1;
+        if (j == m - 1 || !(b > 1e-13 * std::fabs(a))) break;  // Krylov space exhausted
+        beta.push_back(b);
+        HIPCHK(hipMemcpyAsync(V + (int64_t)(j + 1) * n, w, n * 8, hipMemcpyDeviceToDevice, ctx->st));
+        hipLaunchKernelGGL(k_scale_inplace, dim3(256), dim3(256), 0, ctx->st, n, 1.0 / b, V + (int64_t)(j + 1) * n);
+    }
+    const double b_last = std::sqrt(ctx->h_sc[2].dot);
+    // Ritz pairs of the tridiagonal: largest theta = eigenvalues nearest the shift
+    std::vector<double> T((size_t)steps * steps, 0.0), th, Z;
+    for (int i = 0; i < steps; ++i) {
+        T[(size_t)i * steps + i] = alpha[i];
+        if (i + 1 < steps) T[(size_t)i * steps + i + 1] = T[(size_t)(i + 1) * steps + i] = beta[i];
+    }
+    jacobi_eigen(steps, T, th, Z);
+    std::vector<int> order(steps);
+    for (int i = 0; i < steps; ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](int x, int y) { return th[x] > th[y]; });
+    const int nm_out = std::min(n_modes, steps);
+    double max_res = 0.0;
+    for (int k = 0; k < nm_out; ++k) {
+        const int i = order[k];
+        const double lam = shift + 1.0 / th[i];
+        eig_vals[k] = lam;
+        if (freqs_hz) freqs_hz[k] = lam >= 0.0 ? std::sqrt(lam) / (2.0 * M_PI) : -1.0;
+        // residual estimate of the Ritz pair in B: |b_last * z_last| / theta
+        max_res = std::max(max_res, std::fabs(b_last * Z[(size_t)(steps - 1) * steps + i]) / std::fabs(th[i]));
+        if (eig_vecs) {
+            std::vector<double> zc(steps);
+            for (int j = 0; j < steps; ++j) zc[j] = Z[(size_t)j * steps + i];
+            HIPCHK(hipMemcpy(c, zc.data(), steps * 8, hipMemcpyHostToDevice));
+            HIPCHK(hipMemsetAsync(w, 0, n * 8, ctx->st));
+            launch_add_vc(ctx->st, n, steps, V, c, 1.0, w);
+            launch_vmul(ctx->st, n, dsi, w, ctx->d_f);
+            launch_dot64(ctx->st, n, ctx->d_f, ctx->d_f, ctx->d_partials, ctx->d_sc + 2);
+            if ((rc = read_scalars(ctx, 2))) {
+                cleanup();
+                return rc;
+            }
+            const double sc = 1.0 / std::sqrt(ctx->h_sc[2].dot);
+            hipLaunchKernelGGL(k_scale_inplace, dim3(256), dim3(256), 0, ctx->st, n, sc, ctx->d_f);
+            HIPCHK(hipStreamSynchronize(ctx->st));
+            HIPCHK(hipMemcpy(eig_vecs + (int64_t)k * n, ctx->d_f, n * 8, hipMemcpyDeviceToHost));
+        }
+    }
+    for (int k = nm_out; k < n_modes; ++k) {
+        eig_vals[k] = 0.0;
+        if (freqs_hz) freqs_hz[k] = -1.0;
+    }
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->st));
+    HIPCHK(hipEventSynchronize(ctx->ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    cleanup();
+    // the operator now holds the shift: a static solve must assemble again
+    if (shift != 0.0) ctx->assembled = false;
+    if (stats) {
+        asend_modal_stats s;
+        std::memset(&s, 0, sizeof(s));
+        s.lanczos_steps = steps;
+        s.modes_found = nm_out;
+        s.solve_iterations = solve_its;
+        s.max_ritz_residual = max_res;
+        s.masses_raised = n_raised;
+        s.unconverged_solves = unconverged;
+        s.device_ms = ms;
+        *stats = s;
+    }
     return ASEND_OK;
 }
 

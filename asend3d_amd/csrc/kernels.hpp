@@ -112,6 +112,14 @@ void launch_nm_update(hipStream_t st, int64_t n, const double* du, const double*
 void launch_nm_record(hipStream_t st, int n_watch, const int64_t* watch, const double* u, double* hist,
                       int64_t* step, int64_t* its, const Scalars* sc);
 void launch_nm_save(hipStream_t st, int64_t n, const int64_t* step, const double* u, double* uhist);
+// modal analysis (shift-invert Lanczos) vector kernels
+int modal_max_vectors();
+void launch_add_diag(hipStream_t st, int n_rows, const int32_t* sptr, const int32_t* kdiag, double* K, const double* d,
+                     double alpha);
+void launch_vmul(hipStream_t st, int64_t n, const double* a, const double* x, double* y);
+void launch_vtw(hipStream_t st, int64_t n, int m, const double* V, const double* w, double* partials, int nb,
+                double* c);
+void launch_add_vc(hipStream_t st, int64_t n, int m, const double* V, const double* c, double alpha, double* w);
 void launch_adj_prep(hipStream_t st, int64_t n, const double* q, double wn, double c1, double c2, double dt,
                      double gamma, int last, const double* mlam, double* aadj, double* vadj, double* rhs);
 

@@ -10,6 +10,9 @@
   adjoint  -- the newmark workload followed by the transient adjoint of the
               tip deflection summed over the history, with dL/dE for every
               element (one K_eff solve per step backwards).
+  modal    -- the 10 lowest natural frequencies of the 1M-DOF heterogeneous
+              hex8 block (200x40x40, lumped mass), shift-invert Lanczos whose
+              operator applications are multigrid mixed-precision solves.
   shell    -- Newmark time stepping of a clamped shell4 plate (1 x 1 m,
               256 x 256 elements, t = 5 mm, aluminium) under a 1 ms half-sine
               10 kN centre load, dt = 1e-5 s, 1000 steps, hipGraph steps.
@@ -18,6 +21,7 @@
     python -m asend3d_amd.workloads newmark [--nx 100 --ny 20 --nz 20 --steps 1000]
     python -m asend3d_amd.workloads shell [--nx 256 --ny 256 --steps 1000]
     python -m asend3d_amd.workloads adjoint [--nx 100 --ny 20 --nz 20 --steps 1000]
+    python -m asend3d_amd.workloads modal [--modes 10]
 
 Each prints one JSON line.
 """
@@ -129,6 +133,24 @@ def adjoint(nx=100, ny=20, nz=20, steps=1000, dt=1e-5, graphs=True, tol=1e-8, ma
             "dLdE_abs_max": float(np.abs(dL).max()), "dLdE_sum_E": float(np.sum(dL * p.E)), "step_tol": tol}
 
 
+def modal(n_modes=10, nx=200, ny=40, nz=40):
+    p = problems.heterogeneous(nx, ny, nz)
+    ctx = api.Context(0)
+    problems.upload(ctx, p, rho=np.full(p.mesh.n_el, 2.7e-9))
+    ctx.set_grid(nx, ny, nz)
+    o = api.Context.opts(api.SOLVE_IR_MIXED, api.PC_MG, tol=1e-8, inner_tol=1e-2, max_it=200000, max_outer=60,
+                         check_every=4)
+    log(f"modal {nx}x{ny}x{nz}: {p.n_dof} DOF, {n_modes} modes")
+    ctx.modal(2, opts=o, vectors=False)  # warm-up (hierarchy, graphs)
+    vals, fr, _, st = ctx.modal(n_modes, opts=o, vectors=True)
+    ctx.close()
+    return {"workload": "modal", "mesh": f"{nx}x{ny}x{nz} brick8, lognormal E", "dof": int(p.n_dof),
+            "modes": n_modes, "freqs_hz": [float(f) for f in fr], "device_s": st.device_ms / 1e3,
+            "lanczos_steps": int(st.lanczos_steps), "solve_iterations": int(st.solve_iterations),
+            "max_ritz_residual": float(st.max_ritz_residual), "unconverged_solves": int(st.unconverged_solves),
+            "solve_tol": 1e-8}
+
+
 def shell(nx=256, ny=256, steps=1000, dt=1e-5, graphs=True, tol=1e-8, max_it=2000, load=1.0e4):
     p = problems.shell_plate(nx, ny, 1.0, 1.0, 0.005, 70e9, 0.33, 2700.0)
     ctx = api.Context(0)
@@ -161,7 +183,8 @@ def shell(nx=256, ny=256, steps=1000, dt=1e-5, graphs=True, tol=1e-8, max_it=200
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("which", choices=["batched", "newmark", "shell", "adjoint"])
+    ap.add_argument("which", choices=["batched", "newmark", "shell", "adjoint", "modal"])
+    ap.add_argument("--modes", type=int, default=10)
     ap.add_argument("--variants", type=int, default=64)
     ap.add_argument("--sample", type=int, default=2)
     ap.add_argument("--nx", type=int, default=None, help="default 100 (newmark), 256 (shell)")
@@ -172,6 +195,8 @@ def main():
     a = ap.parse_args()
     if a.which == "batched":
         out = batched(a.variants, a.sample)
+    elif a.which == "modal":
+        out = modal(a.modes)
     elif a.which == "adjoint":
         out = adjoint(a.nx or 100, a.ny or 20, a.nz, a.steps, graphs=not a.no_graphs)
     elif a.which == "shell":

@@ -302,6 +302,35 @@ int asend_newmark_adjoint(asend_ctx* ctx, int64_t n_steps, const double* f_shape
                           const double* q, const double* w, double tol, int max_it_step, double* J, double* dJdE,
                           asend_newmark_stats* fwd_stats, asend_newmark_stats* adj_stats);
 
+/* ---- natural frequencies -------------------------------------------------
+ * Replaces Model::eigen_solve for type "frequency" (S/model_analysis.rs:713-888)
+ * and eigen_sparse_direct (S/matrix_functions.rs:1099-1265): the lumped mass
+ * m = M 1 (row sums of the consistent mass; entries below 1e-6 of the mean
+ * raised to that value), the shifted operator K_eff - shift diag(m) (penalty
+ * included), and Lanczos on diag(sqrt m) (K_eff - shift diag m)^-1 diag(sqrt m)
+ * with min(max(5 n_modes, n_modes + 20), 255) vectors in HBM and full
+ * reorthogonalisation (the reference orthogonalises against two vectors and
+ * filters repeated Ritz vectors).  Each operator application is one solve
+ * with opts (e.g. the multigrid mixed solver); the solves are iterative, so
+ * the shifted operator must stay positive definite (shift below the lowest
+ * eigenvalue; the reference factors it directly).  eig_vals[k] = shift +
+ * 1/theta_k for the n_modes largest Ritz values theta, ascending in
+ * eigenvalue; freqs_hz[k] = sqrt(eig_vals[k]) / (2 pi), or -1 for a negative
+ * eigenvalue; eig_vecs (optional, n_modes x n_dof) are the Ritz vectors
+ * diag(1/sqrt m) y normalised to unit 2-norm.  With a nonzero shift the
+ * operator stays shifted: assemble again before a static solve. */
+typedef struct asend_modal_stats {
+    int64_t lanczos_steps;
+    int64_t modes_found;
+    int64_t solve_iterations;   /* inner iterations over all operator applications */
+    double max_ritz_residual;   /* max |beta_m z_m| / theta over the returned modes */
+    int64_t masses_raised;      /* lumped masses raised to 1e-6 of the mean */
+    int64_t unconverged_solves; /* operator applications that missed opts->tol */
+    double device_ms;
+} asend_modal_stats;
+int asend_modal(asend_ctx* ctx, int n_modes, double shift, const asend_solve_opts* opts, double* eig_vals,
+                double* freqs_hz, double* eig_vecs, asend_modal_stats* stats);
+
 /* ---- multigrid diagnostics (no reference counterpart) --------------------
  * Host FP32 arrays.  Level 0 uses the fine DOF numbering, coarser levels the
  * lexicographic node order of their grid.  Used to verify on the device that

@@ -1,0 +1,81 @@
+"""Device natural frequencies (shift-invert Lanczos with the device solves)
+against scipy's shift-invert eigsh on the oracle's stiffness (penalty
+included) and the reference's lumped mass (row sums of the consistent mass),
+and the reference's hexBeam known answer (E/testCases/hexBeam/notes.txt
+test 8: first natural frequency (0.56/L^2) sqrt(EI/m) = 0.161658 Hz)."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import oracle as O
+from asend3d_amd import api, problems
+from asend3d_amd.meshgen import hex_block
+from conftest import gpu_case
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_modes(p, rho, k):
+    K = O.assemble_solid(p.etype, p.mesh.xyz, p.mesh.conn, p.E, p.nu, p.dof_index, p.n_dof)
+    M = O.assemble_solid_mass(p.etype, p.mesh.xyz, p.mesh.conn, rho, p.dof_index, K).to_scipy()
+    Ks = K.to_scipy()
+    sf = 1e5 * abs(Ks).max()
+    pen = np.zeros(p.n_dof)
+    np.add.at(pen, p.pen_dofs, sf)
+    m = M @ np.ones(p.n_dof)
+    m = np.maximum(m, 1e-6 * np.abs(m).mean())
+    vals, vecs = spla.eigsh((Ks + sp.diags(pen)).tocsc(), k=k, M=sp.diags(m).tocsc(), sigma=0.0, which="LM")
+    order = np.argsort(vals)
+    return vals[order], vecs[:, order]
+
+
+def test_modal_matches_shift_invert_eigsh(ctx):
+    gpu_case("modal 12x4x2 vs eigsh")
+    E = api.lognormal_moduli(3, 96, 70e3, 0.5)
+    p = problems.cantilever(12, 4, 2, lx=1200.0, ly=400.0, lz=200.0, E=E, traction=(0.0, 0.0, -1.0))
+    rho = np.full(p.mesh.n_el, 2.7e-9)
+    problems.upload(ctx, p, rho=rho)
+    k = 4
+    vals_o, vecs_o = oracle_modes(p, rho, k)
+    vals, fr, vecs, st = ctx.modal(k)
+    assert st.modes_found == k and st.max_ritz_residual < 1e-8 and st.unconverged_solves == 0, st.as_dict()
+    assert np.abs(vals - vals_o).max() <= 1e-8 * np.abs(vals_o).max(), (vals, vals_o)
+    assert np.allclose(fr, np.sqrt(vals) / (2 * np.pi), rtol=1e-14)
+    for i in range(k):
+        a = vecs[i] / np.linalg.norm(vecs[i])
+        b = vecs_o[:, i] / np.linalg.norm(vecs_o[:, i])
+        assert abs(a @ b) >= 1 - 1e-7, (i, a @ b)
+
+
+def test_modal_shifted_and_multigrid_solves(ctx):
+    # the multigrid mixed solver for the operator applications, and a shift
+    # below the spectrum (the shifted operator stays positive definite for the
+    # iterative solves), give the same modes
+    gpu_case("modal shifted + multigrid")
+    E = api.lognormal_moduli(4, 16 * 4 * 4, 70e3, 0.5)
+    p = problems.cantilever(16, 4, 4, lx=1600.0, ly=400.0, lz=400.0, E=E, traction=(0.0, 0.0, -1.0))
+    rho = np.full(p.mesh.n_el, 2.7e-9)
+    problems.upload(ctx, p, rho=rho)
+    vals_o, _ = oracle_modes(p, rho, 6)
+    ctx.set_grid(16, 4, 4)
+    o = api.Context.opts(api.SOLVE_IR_MIXED, api.PC_MG, tol=1e-11, inner_tol=1e-3, max_it=100000, max_outer=60)
+    vals, _, _, st = ctx.modal(6, opts=o, vectors=False)
+    assert np.abs(vals - vals_o).max() <= 1e-8 * np.abs(vals_o).max(), (vals, vals_o)
+    vs, _, _, st = ctx.modal(3, shift=-0.5 * vals_o[0], vectors=False)
+    assert np.abs(vs - vals_o[:3]).max() <= 1e-8 * np.abs(vals_o).max(), (vs, vals_o)
+
+
+def test_hexbeam_first_frequency(ctx):
+    gpu_case("modal hexBeam brickIM 100x10x1")
+    m = hex_block(100, 10, 1, 10.0, 1.0, 0.1)
+    clamp = m.nodes_at_x(0)
+    _, di, ndof = api.reorder_nodes(m.xyz, m.conn, 3, [[clamp], [clamp], [clamp]], api.block_dim_auto(m.n_nodes))
+    pen = np.concatenate([di[clamp, d] for d in range(3)]).astype(np.int64)
+    ctx.upload_mesh(81, m.xyz, di, ndof, m.conn, np.full(m.n_el, 1e6), np.zeros(m.n_el), np.full(m.n_el, 1.0))
+    ctx.set_penalty(pen, 0.0)
+    # the slender penalised beam's FP64 PCG floor is ~1e-8; the frequency
+    # check needs far less
+    o = api.Context.opts(api.SOLVE_PCG_FP64, api.PC_JACOBI, tol=1e-7, max_it=2000000, max_outer=60)
+    vals, fr, _, st = ctx.modal(1, opts=o, vectors=False)
+    assert abs(fr[0] - 0.16165804304147693) <= 0.01 * 0.16165804304147693, (fr, st.as_dict())
