@@ -121,7 +121,7 @@ EXPORTED = ["asend_ctx_create", "asend_ctx_destroy", "asend_last_error", "asend_
             "asend_learned_num_params", "asend_learned_init", "asend_learned_get_params",
             "asend_learned_set_params", "asend_learned_train", "asend_learned_eval", "asend_learned_apply",
             "asend_learned_edges", "asend_mg_set_smoother", "asend_learned_vcycle_eval", "asend_set_graphs",
-            "asend_newmark_setup", "asend_newmark_run", "asend_newmark_adjoint", "asend_modal", "asend_batch_create", "asend_batch_destroy",
+            "asend_set_rayleigh_damping", "asend_newmark_setup", "asend_newmark_run", "asend_newmark_adjoint", "asend_modal", "asend_batch_create", "asend_batch_destroy",
             "asend_batch_error", "asend_batch_solve", "asend_autotune"]
 
 
@@ -370,6 +370,10 @@ class Context:
         x = np.zeros(b.shape[0], dtype=np.float32)
         self._chk(_lib.asend_mg_coarse_solve(self.h, _p(b), _p(x)), "asend_mg_coarse_solve")
         return x
+
+    def set_rayleigh_damping(self, cm=0.0, ck=0.0):
+        """D = cm M + ck K for the next newmark_setup (rayDampCM / rayDampCK)."""
+        self._chk(_lib.asend_set_rayleigh_damping(self.h, _dbl(cm), _dbl(ck)), "asend_set_rayleigh_damping")
 
     def newmark_setup(self, dt, beta=0.25, gamma=0.5):
         self._chk(_lib.asend_newmark_setup(self.h, _dbl(dt), _dbl(beta), _dbl(gamma)), "asend_newmark_setup")

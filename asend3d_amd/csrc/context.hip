@@ -96,7 +96,10 @@ struct asend_ctx {
     double mean_log_e = 0.0;
     int64_t asm_count = 0, factor_asm = -1;
     int64_t mg_asm = -1;  // assembly the multigrid hierarchy was last set up for
-    double asm_cmass = 0.0;  // mass coefficient of the current assembly
+    double asm_cmass = 0.0;   // mass coefficient of the current assembly
+    double asm_kscale = 1.0;  // stiffness factor of the current assembly
+    // Rayleigh damping D = ray_cm M + ray_ck K of the next Newmark setup
+    double ray_cm = 0.0, ray_ck = 0.0;
     bool use_graphs = true;
     // autotuned choices: SpMV workgroup geometry, inner tolerance and check
     // interval used when a solve passes 0 for them
@@ -124,6 +127,18 @@ struct asend_ctx {
         int64_t uhist_cap = 0;
         double *lam = nullptr, *aadj = nullptr, *vadj = nullptr, *mlam = nullptr, *qv = nullptr,
                *sens_acc = nullptr;
+        // single-reduction PCG of the step solves: u = Dinv r and s = K p
+        double *cg_u = nullptr, *cg_s = nullptr;
+        // Rayleigh damping: coefficients, cd = c_m dt gamma (the weight of D
+        // in K_eff), D itself, the damped load, and the history of
+        // u + ck v the adjoint's dR/dE acts on (save_y: where the tail
+        // stores it, null when not recording)
+        double cm = 0, ck = 0, cd = 0;
+        double keff_k = 1.0, keff_m = 0.0;  // K_eff = keff_k K + keff_m M (+ penalty)
+        double *D64 = nullptr, *rhsd = nullptr, *yhist = nullptr, *save_y = nullptr;
+        int64_t yhist_cap = 0;
+        const double* graph_save_y = nullptr;
+        bool damped() const { return cm != 0.0 || ck != 0.0; }
     } nm;
 };
 
@@ -199,8 +214,12 @@ int write_scalars(asend_ctx* ctx, int idx) {
 void newmark_free(asend_ctx* ctx) {
     auto& nm = ctx->nm;
     for (double** q : {&nm.M64, &nm.v, &nm.a, &nm.w, &nm.q, &nm.rhs0, &nm.rhs, &nm.fshape, &nm.fac, &nm.hist,
-                       &nm.uhist, &nm.lam, &nm.aadj, &nm.vadj, &nm.mlam, &nm.qv, &nm.sens_acc})
+                       &nm.uhist, &nm.lam, &nm.aadj, &nm.vadj, &nm.mlam, &nm.qv, &nm.sens_acc, &nm.cg_u, &nm.cg_s,
+                       &nm.D64, &nm.rhsd, &nm.yhist})
         dfree(*q);
+    nm.yhist_cap = 0;
+    nm.save_y = nullptr;
+    nm.graph_save_y = nullptr;
     for (int64_t** q : {&nm.step, &nm.its, &nm.watch}) dfree(*q);
     nm.fac_cap = nm.hist_cap = nm.watch_cap = nm.uhist_cap = 0;
     nm.save_u = nullptr;
@@ -326,6 +345,34 @@ int run_jacobi_pcg(asend_ctx* ctx, int si, const T* b, T* x, double tol, bool st
             launch_sell3<T>(ctx->st, 1, ctx->fine(), K, p, q, nullptr, ctx->d_partials, sc);
             launch_pcg_update<T>(ctx->st, n, dinv, x, r, p, q, ctx->d_partials, sc);
             launch_pcg_dir<T>(ctx->st, n, dinv, r, p, sc);
+        }
+        HIPCHK(hipGetLastError());
+        if ((rc = read_scalars(ctx, si))) return rc;
+        if (h.done) break;
+    }
+    return ASEND_OK;
+}
+
+// FP64 single-reduction Jacobi PCG on the fine operator, relative tolerance
+// (|b - K x| <= tol |b|); warm: start from the x passed in, else from zero.
+// u and s are work vectors of n_dof doubles.
+int run_cg1_pcg(asend_ctx* ctx, int si, const double* b, double* x, bool warm, double tol, int64_t max_it,
+                int check_every, double* u, double* s) {
+    const int64_t n = ctx->n_dof;
+    Scalars& h = ctx->h_sc[si];
+    std::memset(&h, 0, sizeof(Scalars));
+    h.tol = tol;
+    h.tol2 = tol * tol;
+    h.max_it = max_it;
+    int rc;
+    if ((rc = write_scalars(ctx, si))) return rc;
+    Scalars* sc = ctx->d_sc + si;
+    if (warm) launch_sell3<double>(ctx->st, 3, ctx->fine(), ctx->d_K64, x, ctx->d_r2, b, nullptr, nullptr);
+    launch_cg1_init<double>(ctx->st, n, b, !warm, ctx->d_dinv64, x, ctx->d_r2, u, ctx->d_p, s, ctx->d_partials, sc);
+    for (;;) {
+        for (int k = 0; k < check_every; ++k) {
+            launch_sell3<double>(ctx->st, 4, ctx->fine(), ctx->d_K64, u, ctx->d_q, ctx->d_r2, ctx->d_partials, sc);
+            launch_cg1_update<double>(ctx->st, n, ctx->d_dinv64, x, ctx->d_r2, u, ctx->d_q, ctx->d_p, s, sc);
         }
         HIPCHK(hipGetLastError());
         if ((rc = read_scalars(ctx, si))) return rc;
@@ -567,7 +614,7 @@ int integrate_elements(asend_ctx* ctx, const double* E, double k_scale, double c
     } else {
         launch_hex8_assemble_rows(ctx->st, (int)ctx->n_nodes, (int)ctx->n_slices, ctx->d_sptr, ctx->d_scol, ctx->d_rp,
                                   ctx->d_n2e_ptr, ctx->d_n2e, ctx->d_conn_r, ctx->d_xyz_r, E, ctx->d_nu, ctx->d_rho,
-                                  c_mass, K);
+                                  k_scale, c_mass, K);
     }
     HIPCHK(hipGetLastError());
     return ASEND_OK;
@@ -583,11 +630,12 @@ void launch_sens(asend_ctx* ctx, const double* u, const double* lam) {
                          ctx->d_sens);
 }
 
-int assemble_device(asend_ctx* ctx, double c_mass) {
+// K_eff = k_scale K + c_mass M (+ penalty), its Jacobi diagonal and FP32 copy
+int assemble_device(asend_ctx* ctx, double k_scale, double c_mass) {
     const int64_t nval = ctx->nval();
     if (ctx->etype == ASEND_SHELL4 && !ctx->thk_set)
         return fail(ctx, ASEND_ERR_STATE, "shell4 mesh without thickness (asend_set_thickness)");
-    int rc = integrate_elements(ctx, ctx->d_E, 1.0, c_mass, ctx->d_K64);
+    int rc = integrate_elements(ctx, ctx->d_E, k_scale, c_mass, ctx->d_K64);
     if (rc) return rc;
     if (ctx->n_pen > 0) {
         if (ctx->h_pen.back() >= ctx->n_dof) return fail(ctx, ASEND_ERR_STATE, "penalty DOF beyond the mesh");
@@ -615,6 +663,7 @@ int assemble_device(asend_ctx* ctx, double c_mass) {
     ctx->assembled = true;
     ctx->asm_count += 1;
     ctx->asm_cmass = c_mass;
+    ctx->asm_kscale = k_scale;
     return ASEND_OK;
 }
 
@@ -1070,7 +1119,7 @@ int asend_uniform_elements(const asend_ctx* ctx) { return (ctx && ctx->uniform_h
 int asend_assemble(asend_ctx* ctx, double c_mass) {
     if (!ctx) return ASEND_ERR_ARG;
     if (!ctx->d_K64) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
-    int rc = assemble_device(ctx, c_mass);
+    int rc = assemble_device(ctx, 1.0, c_mass);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(ctx->st));
     return ASEND_OK;
@@ -1184,7 +1233,7 @@ int asend_static_compliance(asend_ctx* ctx, const double* f, double* u, double* 
     HIPCHK(hipMemcpy(ctx->d_f, f, n * 8, hipMemcpyHostToDevice));
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipEventRecord(ctx->ev0, ctx->st));
-    int rc = assemble_device(ctx, 0.0);
+    int rc = assemble_device(ctx, 1.0, 0.0);
     if (rc) return rc;
     asend_solve_stats s;
     if ((rc = solve_device(ctx, opts, &s))) return rc;
@@ -1321,6 +1370,13 @@ int asend_batch_solve(asend_batch* ab, int nrhs, const double* const* rhs, doubl
 // ---------------------------------------------------------------------------
 // Newmark dynamics
 // ---------------------------------------------------------------------------
+int asend_set_rayleigh_damping(asend_ctx* ctx, double cm, double ck) {
+    if (!ctx || !(cm >= 0.0) || !(ck >= 0.0) || !std::isfinite(cm) || !std::isfinite(ck)) return ASEND_ERR_ARG;
+    ctx->ray_cm = cm;
+    ctx->ray_ck = ck;
+    return ASEND_OK;
+}
+
 int asend_newmark_setup(asend_ctx* ctx, double dt, double beta, double gamma) {
     if (!ctx || !(dt > 0.0)) return ASEND_ERR_ARG;
     if (!ctx->d_K64) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
@@ -1333,11 +1389,21 @@ int asend_newmark_setup(asend_ctx* ctx, double dt, double beta, double gamma) {
     nm.gamma = gamma;
     nm.c_m = 1.0 / (dt * dt * (gamma - beta));  // = -c1 of update_vel_acc
     nm.c2 = dt * dt * (0.5 + beta - gamma);
+    nm.cm = ctx->ray_cm;
+    nm.ck = ctx->ray_ck;
+    nm.cd = nm.c_m * dt * gamma;
+    // The reference adds stiffness-proportional damping to the external block
+    // of elements with condensed internal DOFs before condensing
+    // (S/element_equations.rs:991-1003); that form is not built here.
+    if (nm.ck != 0.0 && ctx->etype != ASEND_BRICK8)
+        return fail(ctx, ASEND_ERR_UNSUPPORTED,
+                    "stiffness-proportional damping is implemented for brick8 meshes (mass-proportional for all)");
     const int64_t n = ctx->n_dof, nval = ctx->nval();
     int rc;
     if ((rc = dalloc(ctx, &nm.M64, nval)) || (rc = dalloc(ctx, &nm.v, n)) || (rc = dalloc(ctx, &nm.a, n)) ||
         (rc = dalloc(ctx, &nm.w, n)) || (rc = dalloc(ctx, &nm.q, n)) || (rc = dalloc(ctx, &nm.rhs0, n)) ||
-        (rc = dalloc(ctx, &nm.rhs, n)) || (rc = dalloc(ctx, &nm.fshape, n)) || (rc = dalloc(ctx, &nm.step, 1)))
+        (rc = dalloc(ctx, &nm.rhs, n)) || (rc = dalloc(ctx, &nm.fshape, n)) || (rc = dalloc(ctx, &nm.step, 1)) ||
+        (rc = dalloc(ctx, &nm.cg_u, n)) || (rc = dalloc(ctx, &nm.cg_s, n)))
         return rc;
     // consistent mass alone: the element kernels with c_mass = 1 and no
     // stiffness (zero moduli for the brick8 kernels, k_scale = 0 for the
@@ -1356,11 +1422,19 @@ int asend_newmark_setup(asend_ctx* ctx, double dt, double beta, double gamma) {
         dfree(zeroE);
         if (rc) return rc;
     }
-    // K_eff = K + c_m M (+ penalty).  The reference scales its penalty once,
-    // on the first matrix a solve step assembles, which for a dynamic job
-    // already holds the mass term (S/model_analysis.rs:255-259, :486-488)
+    // D = cm M + ck K, the damped load's work vector
+    if (nm.damped()) {
+        if ((rc = dalloc(ctx, &nm.D64, nval)) || (rc = dalloc(ctx, &nm.rhsd, n))) return rc;
+        if ((rc = integrate_elements(ctx, ctx->d_E, nm.ck, nm.cm, nm.D64))) return rc;
+    }
+    // K_eff = K + c_m M + cd D (+ penalty) = (1 + cd ck) K + (c_m + cd cm) M.
+    // The reference scales its penalty once, on the first matrix a solve
+    // step assembles, which for a dynamic job already holds the mass and
+    // damping terms (S/model_analysis.rs:255-259, :486-488)
+    nm.keff_k = 1.0 + nm.cd * nm.ck;
+    nm.keff_m = nm.c_m + nm.cd * nm.cm;
     ctx->sf_fixed = false;
-    if ((rc = assemble_device(ctx, nm.c_m))) return rc;
+    if ((rc = assemble_device(ctx, nm.keff_k, nm.keff_m))) return rc;
     HIPCHK(hipStreamSynchronize(ctx->st));
     nm.ready = true;
     return ASEND_OK;
@@ -1380,8 +1454,8 @@ int asend_newmark_run(asend_ctx* ctx, int64_t n_steps, const double* f_shape, co
         if (watch[k] < 0 || watch[k] >= n) return fail(ctx, ASEND_ERR_ARG, "watched DOF out of range");
     int rc;
     // the operator must be K_eff of this setup (a static assembly may have replaced it)
-    if (!ctx->assembled || ctx->asm_cmass != nm.c_m) {
-        if ((rc = assemble_device(ctx, nm.c_m))) return rc;
+    if (!ctx->assembled || ctx->asm_cmass != nm.keff_m || ctx->asm_kscale != nm.keff_k) {
+        if ((rc = assemble_device(ctx, nm.keff_k, nm.keff_m))) return rc;
     }
     bool realloc = false;
     if (n_steps > nm.fac_cap) {
@@ -1410,6 +1484,7 @@ int asend_newmark_run(asend_ctx* ctx, int64_t n_steps, const double* f_shape, co
     if (a0) HIPCHK(hipMemcpy(nm.a, a0, n * 8, hipMemcpyHostToDevice));
     else HIPCHK(hipMemset(nm.a, 0, n * 8));
     HIPCHK(hipMemset(nm.step, 0, 8));
+    HIPCHK(hipMemset(ctx->d_x64, 0, n * 8));  // increment of the step before the first
     // per-step solve: FP64 Jacobi PCG on K_eff, relative tolerance, at most
     // max_it_step iterations (the kernels skip their work once converged)
     {
@@ -1427,27 +1502,37 @@ int asend_newmark_run(asend_ctx* ctx, int64_t n_steps, const double* f_shape, co
     // step's solve has converged: one small device-to-host read per block),
     // tail (state update and history).  Each is captured once as a hipGraph.
     const int blk = std::min(max_it_step, kNewmarkBlock);
+    // single-reduction Jacobi PCG: x = du, r, u = Dinv r, w = K u, p, s = K p
     auto iterations = [&]() {
         for (int k = 0; k < blk; ++k) {
-            launch_sell3<double>(ctx->st, 1, ctx->fine(), ctx->d_K64, ctx->d_p, ctx->d_q, nullptr, ctx->d_partials,
+            launch_sell3<double>(ctx->st, 4, ctx->fine(), ctx->d_K64, nm.cg_u, ctx->d_q, ctx->d_r2, ctx->d_partials,
                                  sc);
-            launch_pcg_update<double>(ctx->st, n, ctx->d_dinv64, ctx->d_x64, ctx->d_r2, ctx->d_p, ctx->d_q,
-                                      ctx->d_partials, sc);
-            launch_pcg_dir<double>(ctx->st, n, ctx->d_dinv64, ctx->d_r2, ctx->d_p, sc);
+            launch_cg1_update<double>(ctx->st, n, ctx->d_dinv64, ctx->d_x64, ctx->d_r2, nm.cg_u, ctx->d_q, ctx->d_p,
+                                      nm.cg_s, sc);
         }
     };
     auto head = [&]() {
         launch_nm_predict(ctx->st, n, ctx->d_u, nm.v, nm.a, nm.dt, nm.c2, nm.w);
         launch_sell3<double>(ctx->st, 0, ctx->fine(), nm.M64, nm.w, nm.q, nullptr, nullptr, nullptr);
         launch_nm_rhs(ctx->st, n, nm.step, nm.fac, nm.fshape, nm.c_m, nm.q, nm.rhs0);
-        launch_sell3<double>(ctx->st, 3, ctx->fine(), ctx->d_K64, ctx->d_u, nm.rhs, nm.rhs0, nullptr, nullptr);
-        launch_pcg_init<double>(ctx->st, n, nm.rhs, ctx->d_dinv64, ctx->d_x64, ctx->d_r2, ctx->d_p,
-                                ctx->d_partials, sc);
+        const double* load = nm.rhs0;
+        if (nm.damped()) {  // rhsd = rhs0 - D z (q is free again)
+            launch_nm_damp_z(ctx->st, n, nm.v, nm.a, nm.w, nm.dt, nm.gamma, nm.cd, nm.q);
+            launch_sell3<double>(ctx->st, 3, ctx->fine(), nm.D64, nm.q, nm.rhsd, nm.rhs0, nullptr, nullptr);
+            load = nm.rhsd;
+        }
+        launch_sell3<double>(ctx->st, 3, ctx->fine(), ctx->d_K64, ctx->d_u, nm.rhs, load, nullptr, nullptr);
+        // warm start from the previous step's increment (zero before the first
+        // step); the tolerance stays relative to |rhs|
+        launch_sell3<double>(ctx->st, 3, ctx->fine(), ctx->d_K64, ctx->d_x64, ctx->d_r2, nm.rhs, nullptr, nullptr);
+        launch_cg1_init<double>(ctx->st, n, nm.rhs, false, ctx->d_dinv64, ctx->d_x64, ctx->d_r2, nm.cg_u, ctx->d_p,
+                                nm.cg_s, ctx->d_partials, sc);
         iterations();
     };
     auto tail = [&]() {
         launch_nm_update(ctx->st, n, ctx->d_x64, nm.w, nm.c_m, nm.dt, nm.gamma, ctx->d_u, nm.v, nm.a);
         if (nm.save_u) launch_nm_save(ctx->st, n, nm.step, ctx->d_u, nm.save_u);
+        if (nm.save_y) launch_nm_save_y(ctx->st, n, nm.step, ctx->d_u, nm.v, nm.ck, nm.save_y);
         launch_nm_record(ctx->st, nw, nm.watch, ctx->d_u, nm.hist, nm.step, nm.its, sc);
     };
     auto capture = [&](hipGraphExec_t* exec, const std::function<void()>& seq) -> int {
@@ -1471,13 +1556,15 @@ int asend_newmark_run(asend_ctx* ctx, int64_t n_steps, const double* f_shape, co
         return ASEND_OK;
     };
     const bool graph = ctx->use_graphs;
-    if (graph && (!nm.exec_head || nm.graph_iters != blk || nm.graph_watch != nw || nm.graph_save != nm.save_u)) {
+    if (graph && (!nm.exec_head || nm.graph_iters != blk || nm.graph_watch != nw || nm.graph_save != nm.save_u ||
+                  nm.graph_save_y != nm.save_y)) {
         if ((rc = capture(&nm.exec_head, head)) || (rc = capture(&nm.exec_more, iterations)) ||
             (rc = capture(&nm.exec_tail, tail)))
             return rc;
         nm.graph_iters = blk;
         nm.graph_watch = nw;
         nm.graph_save = nm.save_u;
+        nm.graph_save_y = nm.save_y;
     }
     HIPCHK(hipEventRecord(ctx->ev0, ctx->st));
     for (int64_t s = 0; s < n_steps; ++s) {
@@ -1535,6 +1622,11 @@ int asend_newmark_adjoint(asend_ctx* ctx, int64_t n_steps, const double* f_shape
         if ((rc = dalloc(ctx, &nm.uhist, n_steps * n))) return rc;
         nm.uhist_cap = n_steps * n;
     }
+    // under stiffness-proportional damping dR/dE acts on u + ck v
+    if (nm.ck != 0.0 && n_steps * n > nm.yhist_cap) {
+        if ((rc = dalloc(ctx, &nm.yhist, n_steps * n))) return rc;
+        nm.yhist_cap = n_steps * n;
+    }
     if (!nm.lam) {
         if ((rc = dalloc(ctx, &nm.lam, n)) || (rc = dalloc(ctx, &nm.aadj, n)) || (rc = dalloc(ctx, &nm.vadj, n)) ||
             (rc = dalloc(ctx, &nm.mlam, n)) || (rc = dalloc(ctx, &nm.qv, n)) || (rc = dalloc(ctx, &nm.sens_acc, ctx->n_el)))
@@ -1542,13 +1634,21 @@ int asend_newmark_adjoint(asend_ctx* ctx, int64_t n_steps, const double* f_shape
     }
     // forward sweep from rest, recording u after every step
     nm.save_u = nm.uhist;
+    nm.save_y = nm.ck != 0.0 ? nm.yhist : nullptr;
     rc = asend_newmark_run(ctx, n_steps, f_shape, load_fac, nullptr, nullptr, nullptr, 0, nullptr, nullptr, nullptr,
                            nullptr, nullptr, tol, max_it_step, fwd_stats);
     nm.save_u = nullptr;
+    nm.save_y = nullptr;
     if (rc) return rc;
     // backward sweep (Model::get_obj_gradient, dynamic branch,
     // S/model_analysis.rs:1563-1631): one adjoint solve with K_eff per step
     const double c1 = -nm.c_m;  // 1 / (dt^2 (beta - gamma))
+    // velocity of step n+1 with respect to (u, a, v) of step n (d_vd_up,
+    // d_vd_ap, d_vd_vp of augmentd_ld_u, S/model_analysis.rs:953-955)
+    const double dvu = nm.dt * nm.gamma * c1;
+    const double dva = nm.dt * (1.0 - nm.gamma) + nm.dt * nm.gamma * c1 * nm.c2;
+    const double dvv = 1.0 + nm.dt * nm.gamma * c1 * nm.dt;
+    double* dlam = nm.damped() ? nm.rhsd : nullptr;  // D lam_{n+1}
     HIPCHK(hipMemcpy(nm.qv, q, n * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemsetAsync(nm.aadj, 0, n * 8, ctx->st));
     HIPCHK(hipMemsetAsync(nm.vadj, 0, n * 8, ctx->st));
@@ -1561,17 +1661,22 @@ int asend_newmark_adjoint(asend_ctx* ctx, int64_t n_steps, const double* f_shape
     HIPCHK(hipEventRecord(ctx->ev0, ctx->st));
     for (int64_t k = n_steps - 1; k >= 0; --k) {
         const double* uk = nm.uhist + k * n;
+        const double* yk = nm.ck != 0.0 ? nm.yhist + k * n : uk;
         const int last = (k == n_steps - 1) ? 1 : 0;
-        if (!last) launch_sell3<double>(ctx->st, 0, ctx->fine(), nm.M64, nm.lam, nm.mlam, nullptr, nullptr, nullptr);
-        launch_adj_prep(ctx->st, n, nm.qv, w[k], c1, nm.c2, nm.dt, nm.gamma, last, nm.mlam, nm.aadj, nm.vadj, nm.rhs);
+        if (!last) {
+            launch_sell3<double>(ctx->st, 0, ctx->fine(), nm.M64, nm.lam, nm.mlam, nullptr, nullptr, nullptr);
+            if (dlam) launch_sell3<double>(ctx->st, 0, ctx->fine(), nm.D64, nm.lam, dlam, nullptr, nullptr, nullptr);
+        }
+        launch_adj_prep(ctx->st, n, nm.qv, w[k], c1, nm.c2, nm.dt, nm.gamma, last, nm.mlam, dlam, dvu, dva, dvv,
+                        nm.aadj, nm.vadj, nm.rhs);
         HIPCHK(hipGetLastError());
-        if ((rc = run_jacobi_pcg<double>(ctx, 1, nm.rhs, nm.lam, tol, false, max_it_step, check))) return rc;
+        if ((rc = run_cg1_pcg(ctx, 1, nm.rhs, nm.lam, !last, tol, max_it_step, check, nm.cg_u, nm.cg_s))) return rc;
         const Scalars& h = ctx->h_sc[1];
         st.total_iterations += h.it;
         st.max_step_iterations = std::max<int64_t>(st.max_step_iterations, h.it);
         if (!(h.rr <= h.tol2 * h.ff)) st.unconverged_steps += 1;
-        // dJ/dE += -lam_k^T (dK/dE) u_k ; J += w_k q.u_k
-        launch_sens(ctx, uk, nm.lam);
+        // dJ/dE += -lam_k^T (dK/dE) (u_k + ck v_k) ; J += w_k q.u_k
+        launch_sens(ctx, yk, nm.lam);
         launch_axpy64(ctx->st, ctx->n_el, 1.0, ctx->d_sens, nm.sens_acc);
         launch_dot64(ctx->st, n, nm.qv, uk, ctx->d_partials, ctx->d_sc + 2);
         HIPCHK(hipGetLastError());
@@ -1711,7 +1816,7 @@ int asend_modal(asend_ctx* ctx, int n_modes, double shift, const asend_solve_opt
     }
     dfree(Mtmp);
     // shifted operator K_eff - shift diag(m) (penalty included)
-    if ((rc = assemble_device(ctx, 0.0))) {
+    if ((rc = assemble_device(ctx, 1.0, 0.0))) {
         cleanup();
         return rc;
     }

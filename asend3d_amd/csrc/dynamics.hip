@@ -79,23 +79,44 @@ __global__ void k_nm_save(int64_t n, const int64_t* __restrict__ step, const dou
         dst[i] = u[i];
 }
 
-// One backward step of the transient adjoint (no damping).  Objective
-// partials at this step: dL/du = wn q, dL/dv = dL/da = 0.  Unless this is the
-// last step, the coupling to step n+1 (augmentd_ld_u, S/model_analysis.rs:
-// 936-1063) with t = a_adj - M lam_{n+1}:
-//   gu += c1 t ; ga = c1 c2 t + dt (1 - gamma) v_adj ; gv = c1 dt t + v_adj
+// yhist[step] = u + ck v
+__global__ void k_nm_save_y(int64_t n, const int64_t* __restrict__ step, const double* __restrict__ u,
+                            const double* __restrict__ v, double ck, double* __restrict__ yhist) {
+    double* dst = yhist + (*step) * n;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = u[i] + ck * v[i];
+}
+
+// Rayleigh damping D = cm M + ck K enters the step's residual through the
+// velocity v(u) = v + dt ((1 - gamma) a + gamma c_m (u - w)); with
+// K_eff = K + c_m M + cd D (cd = c_m dt gamma) the residual at u_n is
+// f + c_m M w - D z - K_eff u_n with z = v + dt (1 - gamma) a - cd w.
+__global__ void k_nm_damp_z(int64_t n, const double* __restrict__ v, const double* __restrict__ a,
+                            const double* __restrict__ w, double dt, double gamma, double cd, double* __restrict__ z) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        z[i] = v[i] + dt * (1.0 - gamma) * a[i] - cd * w[i];
+}
+
+// One backward step of the transient adjoint.  Objective partials at this
+// step: dL/du = wn q, dL/dv = dL/da = 0.  Unless this is the last step, the
+// coupling to step n+1 (augmentd_ld_u, S/model_analysis.rs:936-1063) with
+// t = a_adj - M lam_{n+1} and d = D lam_{n+1} (Rayleigh damping, :1033-1056):
+//   gu += c1 t - dvu d ; ga = c1 c2 t + dt (1 - gamma) v_adj - dva d ;
+//   gv = c1 dt t + v_adj - dvv d
 // then (solve_for_adjoint, :1089-1100):
 //   v_adj = gv ; a_adj = ga + dt gamma v_adj ; rhs = gu - c1 a_adj.
 __global__ void k_adj_prep(int64_t n, const double* __restrict__ q, double wn, double c1, double c2, double dt,
-                           double gamma, int last, const double* __restrict__ mlam, double* __restrict__ aadj,
-                           double* __restrict__ vadj, double* __restrict__ rhs) {
+                           double gamma, int last, const double* __restrict__ mlam, const double* __restrict__ dlam,
+                           double dvu, double dva, double dvv, double* __restrict__ aadj, double* __restrict__ vadj,
+                           double* __restrict__ rhs) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         double gu = wn * q[i], ga = 0.0, gv = 0.0;
         if (!last) {
             const double t = aadj[i] - mlam[i];
-            gu += c1 * t;
-            ga = c1 * c2 * t + dt * (1.0 - gamma) * vadj[i];
-            gv = c1 * dt * t + vadj[i];
+            const double d = dlam ? dlam[i] : 0.0;
+            gu += c1 * t - dvu * d;
+            ga = c1 * c2 * t + dt * (1.0 - gamma) * vadj[i] - dva * d;
+            gv = c1 * dt * t + vadj[i] - dvv * d;
         }
         const double va = gv, aa = ga + dt * gamma * va;
         vadj[i] = va;
@@ -127,10 +148,21 @@ void launch_nm_save(hipStream_t st, int64_t n, const int64_t* step, const double
     hipLaunchKernelGGL(k_nm_save, dim3(nb(n)), dim3(256), 0, st, n, step, u, uhist);
 }
 
+void launch_nm_save_y(hipStream_t st, int64_t n, const int64_t* step, const double* u, const double* v, double ck,
+                      double* yhist) {
+    hipLaunchKernelGGL(k_nm_save_y, dim3(nb(n)), dim3(256), 0, st, n, step, u, v, ck, yhist);
+}
+
+void launch_nm_damp_z(hipStream_t st, int64_t n, const double* v, const double* a, const double* w, double dt,
+                      double gamma, double cd, double* z) {
+    hipLaunchKernelGGL(k_nm_damp_z, dim3(nb(n)), dim3(256), 0, st, n, v, a, w, dt, gamma, cd, z);
+}
+
 void launch_adj_prep(hipStream_t st, int64_t n, const double* q, double wn, double c1, double c2, double dt,
-                     double gamma, int last, const double* mlam, double* aadj, double* vadj, double* rhs) {
-    hipLaunchKernelGGL(k_adj_prep, dim3(nb(n)), dim3(256), 0, st, n, q, wn, c1, c2, dt, gamma, last, mlam, aadj, vadj,
-                       rhs);
+                     double gamma, int last, const double* mlam, const double* dlam, double dvu, double dva,
+                     double dvv, double* aadj, double* vadj, double* rhs) {
+    hipLaunchKernelGGL(k_adj_prep, dim3(nb(n)), dim3(256), 0, st, n, q, wn, c1, c2, dt, gamma, last, mlam, dlam, dvu,
+                       dva, dvv, aadj, vadj, rhs);
 }
 
 void launch_nm_record(hipStream_t st, int n_watch, const int64_t* watch, const double* u, double* hist,

@@ -73,8 +73,8 @@ __global__ __launch_bounds__(256, 3) void k_hex8_assemble_rows(int n_rows, int n
                                                             const double* __restrict__ xyz_r,
                                                             const double* __restrict__ E,
                                                             const double* __restrict__ nu,
-                                                            const double* __restrict__ rho, double c_mass,
-                                                            double* __restrict__ K) {
+                                                            const double* __restrict__ rho, double k_scale,
+                                                            double c_mass, double* __restrict__ K) {
     const int lane = threadIdx.x & 63;
     const int s = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (s >= n_slices) return;
@@ -106,7 +106,7 @@ __global__ __launch_bounds__(256, 3) void k_hex8_assemble_rows(int n_rows, int n
                 for (int t = 0; t < 8; ++t)
 #pragma unroll
                     for (int c = 0; c < 3; ++c) X[t][c] = xyz_r[3 * (int64_t)nodes[t] + c];
-                const double Ee = E[e], ne = nu[e];
+                const double Ee = k_scale * E[e], ne = nu[e];
                 const double lam = Ee * ne / ((1.0 + ne) * (1.0 - 2.0 * ne));
                 const double mu = Ee / (2.0 * (1.0 + ne));
                 double sa[3], sb[3];
@@ -311,10 +311,10 @@ __global__ __launch_bounds__(256) void k_hex8_assemble_uniform(
 void launch_hex8_assemble_rows(hipStream_t st, int n_rows, int n_slices, const int32_t* sptr, const int32_t* scol,
                                const int32_t* rp, const int32_t* n2e_ptr, const int32_t* n2e, const int32_t* conn_r,
                                const double* xyz_r, const double* E, const double* nu, const double* rho,
-                               double c_mass, double* K) {
+                               double k_scale, double c_mass, double* K) {
     if (n_slices <= 0) return;
     hipLaunchKernelGGL(k_hex8_assemble_rows, dim3((unsigned)((n_slices + 3) / 4)), dim3(256), 0, st, n_rows, n_slices,
-                       sptr, scol, rp, n2e_ptr, n2e, conn_r, xyz_r, E, nu, rho, c_mass, K);
+                       sptr, scol, rp, n2e_ptr, n2e, conn_r, xyz_r, E, nu, rho, k_scale, c_mass, K);
 }
 
 void launch_hex8_reference(hipStream_t st, const int32_t* conn_r, const double* xyz_r, double nu, double* Khat,
@@ -337,11 +337,21 @@ void launch_hex8_assemble_uniform(hipStream_t st, int n_rows, int n_slices, cons
 // partials are visible without device-scope fences) adds them in a fixed
 // order and updates the loop scalars.  Deterministic run to run.
 // ---------------------------------------------------------------------------
-enum FinOp { FIN_PQ = 0, FIN_RR = 1, FIN_INIT = 2, FIN_UPDATE = 3, FIN_DOT = 4, FIN_MAX = 5 };
+enum FinOp {
+    FIN_PQ = 0,
+    FIN_RR = 1,
+    FIN_INIT = 2,
+    FIN_UPDATE = 3,
+    FIN_DOT = 4,
+    FIN_MAX = 5,
+    FIN_INIT_WARM = 6,
+    FIN_CG1_INIT = 7,
+    FIN_CG1 = 8
+};
 
 __global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ partials, int nparts, int op,
                                                      Scalars* sc) {
-    if ((op == FIN_PQ || op == FIN_UPDATE) && sc->done) return;
+    if ((op == FIN_PQ || op == FIN_UPDATE || op == FIN_CG1) && sc->done) return;
     if (op == FIN_MAX) {
         double m = 0.0;
         for (int b = threadIdx.x; b < nparts; b += blockDim.x) m = fmax(m, partials[b]);
@@ -356,13 +366,14 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ 
         }
         return;
     }
-    const int nv = (op == FIN_INIT || op == FIN_UPDATE) ? 2 : 1;
-    double t[2] = {0.0, 0.0};
+    const int nv = (op == FIN_INIT || op == FIN_UPDATE) ? 2 : ((op == FIN_INIT_WARM || op == FIN_CG1) ? 3 : 1);
+    double t[3] = {0.0, 0.0, 0.0};
     for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
         t[0] += partials[(size_t)nv * b];
-        if (nv == 2) t[1] += partials[(size_t)nv * b + 1];
+        if (nv >= 2) t[1] += partials[(size_t)nv * b + 1];
+        if (nv == 3) t[2] += partials[(size_t)nv * b + 2];
     }
-    block_sum<2>(t);
+    block_sum<3>(t);
     if (threadIdx.x != 0) return;
     switch (op) {
         case FIN_PQ:
@@ -382,6 +393,43 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ 
             sc->it = 0;
             sc->done = (sc->stop_abs ? (t[0] <= sc->tol) : (t[1] <= sc->tol2 * t[1])) || t[1] == 0.0;
             break;
+        case FIN_INIT_WARM:
+            // started from a nonzero x: the tolerance stays relative to |b|
+            sc->rz = t[0];
+            sc->rr = t[1];
+            sc->ff = t[2];
+            sc->it = 0;
+            sc->done = (sc->stop_abs ? (t[0] <= sc->tol) : (t[1] <= sc->tol2 * t[2])) || t[2] == 0.0;
+            break;
+        case FIN_CG1_INIT:
+            // single-reduction PCG start: t0 = |b|^2
+            sc->ff = t[0];
+            sc->rz = 0.0;
+            sc->it = 0;
+            sc->done = t[0] == 0.0;
+            break;
+        case FIN_CG1: {
+            // single-reduction PCG: t0 = gamma = r.u, t1 = delta = (K u).u,
+            // t2 = r.r of the current residual
+            const double g = t[0], d = t[1];
+            sc->rr = t[2];
+            const bool conv = sc->stop_abs ? (g <= sc->tol) : (t[2] <= sc->tol2 * sc->ff);
+            if (conv || sc->it >= sc->max_it) {
+                sc->done = 1;
+                break;
+            }
+            const double beta = sc->it == 0 ? 0.0 : g / sc->rz;
+            const double den = sc->it == 0 ? d : d - beta * g / sc->alpha;
+            if (!(den > 0.0)) {  // lost positive definiteness: stop, reported unconverged
+                sc->done = 1;
+                break;
+            }
+            sc->beta = beta;
+            sc->alpha = g / den;
+            sc->rz = g;
+            sc->it += 1;
+            break;
+        }
         case FIN_UPDATE: {
             sc->beta = t[0] / sc->rz;
             sc->rz = t[0];
@@ -476,12 +524,12 @@ __global__ __launch_bounds__(kSellMaxWaves * kWave) void k_sell3(int n_rows, int
                                                   const int32_t* __restrict__ scol, const T* __restrict__ val,
                                                   const T* __restrict__ x, T* __restrict__ y,
                                                   const T* __restrict__ f, double* partials, const Scalars* sc) {
-    if (MODE == 1 && sc->done) return;
+    if ((MODE == 1 || MODE == 4) && sc->done) return;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int slice = lb * (int)(blockDim.x / kWave) + threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
     const int r = slice * kWave + lane;
-    double part = 0.0;
+    double part = 0.0, part1 = 0.0, part2 = 0.0;
     if (slice < n_slices) {
         const int k0 = sptr[slice], k1 = sptr[slice + 1];
         const int32_t* cp = scol + (int64_t)k0 * kWave + lane;
@@ -504,6 +552,16 @@ __global__ __launch_bounds__(kSellMaxWaves * kWave) void k_sell3(int n_rows, int
                 y[3 * (int64_t)r + 1] = y1;
                 y[3 * (int64_t)r + 2] = y2;
                 part = (double)y0 * y0 + (double)y1 * y1 + (double)y2 * y2;
+            } else if (MODE == 4) {
+                // x = u (preconditioned residual), f = r: partials r.u, (K u).u, r.r
+                y[3 * (int64_t)r] = a0;
+                y[3 * (int64_t)r + 1] = a1;
+                y[3 * (int64_t)r + 2] = a2;
+                const double u0 = x[3 * (int64_t)r], u1 = x[3 * (int64_t)r + 1], u2 = x[3 * (int64_t)r + 2];
+                const double r0 = f[3 * (int64_t)r], r1 = f[3 * (int64_t)r + 1], r2 = f[3 * (int64_t)r + 2];
+                part = r0 * u0 + r1 * u1 + r2 * u2;
+                part1 = (double)a0 * u0 + (double)a1 * u1 + (double)a2 * u2;
+                part2 = r0 * r0 + r1 * r1 + r2 * r2;
             } else {
                 y[3 * (int64_t)r] = a0;
                 y[3 * (int64_t)r + 1] = a1;
@@ -515,6 +573,11 @@ __global__ __launch_bounds__(kSellMaxWaves * kWave) void k_sell3(int n_rows, int
         }
     }
     if (MODE == 0 || MODE == 3) return;
+    if (MODE == 4) {
+        double t[3] = {part, part1, part2};
+        block_partials<3>(t, partials);
+        return;
+    }
     double t[1] = {part};
     block_partials<1>(t, partials);
 }
@@ -540,6 +603,11 @@ void launch_sell3(hipStream_t st, int mode, const Sell3& A, const T* val, const 
                                val, x, y, f, partials, sc);
             finalize(st, partials, nblk, FIN_RR, sc);
             break;
+        case 4:
+            hipLaunchKernelGGL((k_sell3<T, 4>), dim3(nblk), dim3(nthr), 0, st, A.n_rows, A.n_slices, A.sptr, A.scol,
+                               val, x, y, f, partials, sc);
+            finalize(st, partials, nblk, FIN_CG1, sc);
+            break;
         default:
             hipLaunchKernelGGL((k_sell3<T, 3>), dim3(nblk), dim3(nthr), 0, st, A.n_rows, A.n_slices, A.sptr, A.scol,
                                val, x, y, f, partials, sc);
@@ -558,6 +626,8 @@ template void launch_sell3<float>(hipStream_t, int, const Sell3&, const float*, 
 // MODE 1: y = K x and (x . y) -> finalize: sc->pq, sc->alpha = sc->rz / pq
 // MODE 2: y = f - K x (x = current solution, f = load) -> finalize: sc->rr = |y|^2
 // MODE 3: y = f - K x without the norm
+// MODE 4 (sliced-ELL only): y = K x with x = u = Dinv r and f = r, partials of
+//   r.u, y.u and r.r -> finalize: the single-reduction PCG step scalars
 // Kernels with a scalars pointer return at once when sc->done is set.
 // ---------------------------------------------------------------------------
 template <typename T, int G, int MODE>
@@ -664,6 +734,24 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(int64_t n, const T* __restr
     block_partials<2>(t, partials);
 }
 
+// warm start: r = b - K x already formed; p = z = Dinv r; partials of r.z,
+// r.r and b.b
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_init_warm(int64_t n, const T* __restrict__ b,
+                                                          const T* __restrict__ dinv, const T* __restrict__ r,
+                                                          T* __restrict__ p, double* partials) {
+    double t[3] = {0.0, 0.0, 0.0};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const T ri = r[i];
+        const T zi = dinv[i] * ri;
+        p[i] = zi;
+        t[0] += (double)ri * zi;
+        t[1] += (double)ri * ri;
+        t[2] += (double)b[i] * b[i];
+    }
+    block_partials<3>(t, partials);
+}
+
 // x += alpha p; r -= alpha q; partials of r.(Dinv r) and r.r
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(int64_t n, const T* __restrict__ dinv, T* __restrict__ x,
@@ -693,11 +781,82 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(int64_t n, const T* __restri
         p[i] = dinv[i] * r[i] + beta * p[i];
 }
 
+// Single-reduction (Chronopoulos-Gear) Jacobi PCG: one SpMV with all three
+// dot products fused (K u with r.u, (K u).u, r.r) and one vector update per
+// iteration, against an SpMV, two vector kernels and two reductions for the
+// textbook recurrence.  Same Krylov iterates in exact arithmetic:
+//   beta = gamma / gamma_old, alpha = gamma / (delta - beta gamma / alpha_old)
+//   p = u + beta p ; s = w + beta s (= K p) ; x += alpha p ; r -= alpha s ;
+//   u = Dinv r
+// Start: cold (x = 0, r = b) or warm (r = b - K x already formed in r);
+// u = Dinv r, p = s = 0; partials of |b|^2.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg1_init(int64_t n, const T* __restrict__ b, int cold,
+                                                     const T* __restrict__ dinv, T* __restrict__ x, T* __restrict__ r,
+                                                     T* __restrict__ u, T* __restrict__ p, T* __restrict__ s,
+                                                     double* partials) {
+    double t[1] = {0.0};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const T bi = b[i];
+        T ri;
+        if (cold) {
+            ri = bi;
+            r[i] = ri;
+            x[i] = 0;
+        } else {
+            ri = r[i];
+        }
+        u[i] = dinv[i] * ri;
+        p[i] = 0;
+        s[i] = 0;
+        t[0] += (double)bi * bi;
+    }
+    block_partials<1>(t, partials);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg1_update(int64_t n, const T* __restrict__ dinv, T* __restrict__ x,
+                                                       T* __restrict__ r, T* __restrict__ u, const T* __restrict__ w,
+                                                       T* __restrict__ p, T* __restrict__ s,
+                                                       const Scalars* __restrict__ sc) {
+    if (sc->done) return;
+    const T alpha = (T)sc->alpha, beta = (T)sc->beta;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const T pi = u[i] + beta * p[i];
+        const T si = w[i] + beta * s[i];
+        p[i] = pi;
+        s[i] = si;
+        x[i] += alpha * pi;
+        const T ri = r[i] - alpha * si;
+        r[i] = ri;
+        u[i] = dinv[i] * ri;
+    }
+}
+
+template <typename T>
+void launch_cg1_init(hipStream_t st, int64_t n, const T* b, bool cold, const T* dinv, T* x, T* r, T* u, T* p, T* s,
+                     double* partials, Scalars* sc) {
+    hipLaunchKernelGGL(k_cg1_init<T>, dim3(kVecBlocks), dim3(kBlock), 0, st, n, b, cold ? 1 : 0, dinv, x, r, u, p, s,
+                       partials);
+    finalize(st, partials, kVecBlocks, FIN_CG1_INIT, sc);
+}
+template <typename T>
+void launch_cg1_update(hipStream_t st, int64_t n, const T* dinv, T* x, T* r, T* u, const T* w, T* p, T* s,
+                       const Scalars* sc) {
+    hipLaunchKernelGGL(k_cg1_update<T>, dim3(kVecBlocks), dim3(kBlock), 0, st, n, dinv, x, r, u, w, p, s, sc);
+}
+
 template <typename T>
 void launch_pcg_init(hipStream_t st, int64_t n, const T* b, const T* dinv, T* x, T* r, T* p, double* partials,
                      Scalars* sc) {
     hipLaunchKernelGGL(k_pcg_init<T>, dim3(kVecBlocks), dim3(kBlock), 0, st, n, b, dinv, x, r, p, partials);
     finalize(st, partials, kVecBlocks, FIN_INIT, sc);
+}
+template <typename T>
+void launch_pcg_init_warm(hipStream_t st, int64_t n, const T* b, const T* dinv, const T* r, T* p, double* partials,
+                          Scalars* sc) {
+    hipLaunchKernelGGL(k_pcg_init_warm<T>, dim3(kVecBlocks), dim3(kBlock), 0, st, n, b, dinv, r, p, partials);
+    finalize(st, partials, kVecBlocks, FIN_INIT_WARM, sc);
 }
 template <typename T>
 void launch_pcg_update(hipStream_t st, int64_t n, const T* dinv, T* x, T* r, const T* p, const T* q,
@@ -712,9 +871,15 @@ void launch_pcg_dir(hipStream_t st, int64_t n, const T* dinv, const T* r, T* p, 
 
 #define INST_PCG(T)                                                                                               \
     template void launch_pcg_init<T>(hipStream_t, int64_t, const T*, const T*, T*, T*, T*, double*, Scalars*);    \
+    template void launch_pcg_init_warm<T>(hipStream_t, int64_t, const T*, const T*, const T*, T*, double*,        \
+                                          Scalars*);                                                              \
     template void launch_pcg_update<T>(hipStream_t, int64_t, const T*, T*, T*, const T*, const T*, double*,       \
                                        Scalars*);                                                                 \
-    template void launch_pcg_dir<T>(hipStream_t, int64_t, const T*, const T*, T*, const Scalars*);
+    template void launch_pcg_dir<T>(hipStream_t, int64_t, const T*, const T*, T*, const Scalars*);               \
+    template void launch_cg1_init<T>(hipStream_t, int64_t, const T*, bool, const T*, T*, T*, T*, T*, T*, double*, \
+                                     Scalars*);                                                                   \
+    template void launch_cg1_update<T>(hipStream_t, int64_t, const T*, T*, T*, T*, const T*, T*, T*,              \
+                                       const Scalars*);
 INST_PCG(double)
 INST_PCG(float)
 

@@ -30,12 +30,13 @@ __host__ __device__ inline int64_t sell_index(const int32_t* sptr, int r, int k,
     return ((int64_t)9 * (sptr[r >> 6] + k) + c) * 64 + (r & 63);
 }
 
-// brick8 by rows: writes K (every slot, padding as zeros) = sum of K_e + c_mass M_e
-// over the elements of each row's node; n2e lists each node's elements
+// brick8 by rows: writes K (every slot, padding as zeros) = sum of
+// k_scale K_e + c_mass M_e over the elements of each row's node; n2e lists
+// each node's elements
 void launch_hex8_assemble_rows(hipStream_t st, int n_rows, int n_slices, const int32_t* sptr, const int32_t* scol,
                                const int32_t* rp, const int32_t* n2e_ptr, const int32_t* n2e, const int32_t* conn_r,
                                const double* xyz_r, const double* E, const double* nu, const double* rho,
-                               double c_mass, double* K);
+                               double k_scale, double c_mass, double* K);
 // Khat (64 x 9) and Mhat (64) of the element conn_r[0..7] with E = rho = 1
 void launch_hex8_reference(hipStream_t st, const int32_t* conn_r, const double* xyz_r, double nu, double* Khat,
                            double* Mhat);
@@ -69,7 +70,8 @@ void launch_to_f32(hipStream_t st, const double* a, float* b, int64_t n);
 void launch_diag_inv(hipStream_t st, int n_rows, const int32_t* sptr, const int32_t* kdiag, const double* K,
                      double* d64, float* d32);
 
-// Sliced-ELL SpMV, same modes as launch_spmv3.
+// Sliced-ELL SpMV, same modes as launch_spmv3, plus mode 4: y = K x with
+// x = u, f = r and the single-reduction PCG scalars (launch_cg1_*).
 template <typename T>
 void launch_sell3(hipStream_t st, int mode, const Sell3& A, const T* val, const T* x, T* y, const T* f,
                   double* partials, Scalars* sc);
@@ -83,11 +85,24 @@ void launch_spmv3(hipStream_t st, int mode, int n_rows, const int32_t* rp, const
 template <typename T>
 void launch_pcg_init(hipStream_t st, int64_t n, const T* b, const T* dinv, T* x, T* r, T* p, double* partials,
                      Scalars* sc);
+// warm start: r = b - K x already formed, x kept; tolerance relative to |b|
+template <typename T>
+void launch_pcg_init_warm(hipStream_t st, int64_t n, const T* b, const T* dinv, const T* r, T* p, double* partials,
+                          Scalars* sc);
 template <typename T>
 void launch_pcg_update(hipStream_t st, int64_t n, const T* dinv, T* x, T* r, const T* p, const T* q,
                        double* partials, Scalars* sc);
 template <typename T>
 void launch_pcg_dir(hipStream_t st, int64_t n, const T* dinv, const T* r, T* p, const Scalars* sc);
+// Single-reduction Jacobi PCG: start (cold: x = 0, r = b; warm: r = b - K x
+// formed by the caller), then per iteration launch_sell3 mode 4 (w = K u)
+// followed by launch_cg1_update.
+template <typename T>
+void launch_cg1_init(hipStream_t st, int64_t n, const T* b, bool cold, const T* dinv, T* x, T* r, T* u, T* p, T* s,
+                     double* partials, Scalars* sc);
+template <typename T>
+void launch_cg1_update(hipStream_t st, int64_t n, const T* dinv, T* x, T* r, T* u, const T* w, T* p, T* s,
+                       const Scalars* sc);
 
 void launch_scale_to_f32(hipStream_t st, int64_t n, const double* r, float* r32, const Scalars* sc);
 void launch_axpy_from_f32(hipStream_t st, int64_t n, double* u, const float* d, const Scalars* sc_outer);
@@ -112,6 +127,12 @@ void launch_nm_update(hipStream_t st, int64_t n, const double* du, const double*
 void launch_nm_record(hipStream_t st, int n_watch, const int64_t* watch, const double* u, double* hist,
                       int64_t* step, int64_t* its, const Scalars* sc);
 void launch_nm_save(hipStream_t st, int64_t n, const int64_t* step, const double* u, double* uhist);
+// yhist[step] = u + ck v (the state dR/dE acts on under stiffness-proportional damping)
+void launch_nm_save_y(hipStream_t st, int64_t n, const int64_t* step, const double* u, const double* v, double ck,
+                      double* yhist);
+// z = v + dt (1 - gamma) a - cd w: the velocity part of the damped residual
+void launch_nm_damp_z(hipStream_t st, int64_t n, const double* v, const double* a, const double* w, double dt,
+                      double gamma, double cd, double* z);
 // modal analysis (shift-invert Lanczos) vector kernels
 int modal_max_vectors();
 void launch_add_diag(hipStream_t st, int n_rows, const int32_t* sptr, const int32_t* kdiag, double* K, const double* d,
@@ -120,7 +141,10 @@ void launch_vmul(hipStream_t st, int64_t n, const double* a, const double* x, do
 void launch_vtw(hipStream_t st, int64_t n, int m, const double* V, const double* w, double* partials, int nb,
                 double* c);
 void launch_add_vc(hipStream_t st, int64_t n, int m, const double* V, const double* c, double alpha, double* w);
+// dlam = D lam_{n+1} (null without damping); dvu, dva, dvv: derivatives of
+// the velocity of step n+1 with respect to (u, a, v) of step n
 void launch_adj_prep(hipStream_t st, int64_t n, const double* q, double wn, double c1, double c2, double dt,
-                     double gamma, int last, const double* mlam, double* aadj, double* vadj, double* rhs);
+                     double gamma, int last, const double* mlam, const double* dlam, double dvu, double dva,
+                     double dvv, double* aadj, double* vadj, double* rhs);
 
 }  // namespace asend

@@ -271,12 +271,23 @@ typedef struct asend_newmark_stats {
     int graph_used;               /* 1 when the steps ran as graph replays    */
     double device_ms;             /* HIP-event time of the step loop          */
 } asend_newmark_stats;
-/* Assemble M (consistent, rho per element) and K_eff = K + c_m M with the
- * penalty of asend_set_penalty; requires gamma > beta.  brick8 and shell4
- * meshes uploaded with rho (shell4: after asend_set_thickness). */
+/* Rayleigh damping D = cm M + ck K for the next asend_newmark_setup (job keys
+ * rayDampCM / rayDampCK, JobCommand::ray_damp_cm / ray_damp_ck,
+ * S/job.rs:15-16; Element::get_rud_dfd0, S/element_equations.rs:699-753).
+ * cm, ck >= 0; default 0.  ck > 0 is implemented for brick8 meshes (the
+ * reference adds it to the uncondensed external block of elements with
+ * internal DOFs); cm for every element type.  Material damping matrices are
+ * not implemented. */
+int asend_set_rayleigh_damping(asend_ctx* ctx, double cm, double ck);
+/* Assemble M (consistent, rho per element), D (with damping) and
+ * K_eff = K + c_m M + c_m dt gamma D, c_m = 1/(dt^2 (gamma - beta)), with the
+ * penalty of asend_set_penalty (Element::get_ru_dfd0,
+ * S/element_equations.rs:948-1003); requires gamma > beta.  brick8, brickIM
+ * and shell4 meshes uploaded with rho (shell4: after asend_set_thickness). */
 int asend_newmark_setup(asend_ctx* ctx, double dt, double beta, double gamma);
 /* n_steps steps from (u0, v0, a0) (null = zero).  The load of step n is
- * load_fac[n] * f_shape.  hist[n * n_watch + k] = u[watch[k]] after step n.
+ * load_fac[n] * f_shape; the step's residual is f - K u - M a(u) - D v(u) at
+ * the predicted state (S/model_analysis.rs:312-558, S/node_meth.rs:127-199).  hist[n * n_watch + k] = u[watch[k]] after step n.
  * Each step's solve stops at ||r|| <= tol ||rhs|| or max_it_step iterations.
  * Final u, v, a into the out arrays when non-null. */
 int asend_newmark_run(asend_ctx* ctx, int64_t n_steps, const double* f_shape, const double* load_fac,
@@ -286,16 +297,20 @@ int asend_newmark_run(asend_ctx* ctx, int64_t n_steps, const double* f_shape, co
 
 /* Transient adjoint: replaces the dynamic branch of Model::get_obj_gradient
  * (S/model_analysis.rs:1563-1631) with augmentd_ld_u (:936-1063) and the
- * dynamic elastic part of solve_for_adjoint (:1089-1100), no damping, for the
- * objective L = sum_n w[n] q^T u_n (displacement objective terms of exponent
+ * dynamic elastic part of solve_for_adjoint (:1089-1100), with the Rayleigh
+ * damping of the setup, for the objective L = sum_n w[n] q^T u_n (displacement objective terms of exponent
  * 1, S/objective_meth.rs:618-663).  Runs the forward sweep from rest
  * (load_fac[n] f_shape as asend_newmark_run, keeping u_n of every step in
  * HBM: n_steps * n_dof doubles), then for n = N..1
- *   t = a_adj - M lam_{n+1} (not on the last step),
- *   v_adj = c1 dt t + v_adj, a_adj = c1 c2 t + dt (1 - gamma) v_adj_old + dt gamma v_adj,
- *   K_eff lam_n = w[n] q + c1 t - c1 a_adj  (FP64 Jacobi PCG to tol),
- *   dJdE += -lam_n^T (dK/dE) u_n,
- * with c1 = 1/(dt^2 (beta - gamma)), c2 = dt^2 (0.5 + beta - gamma).  The
+ *   t = a_adj - M lam_{n+1}, d = D lam_{n+1} (not on the last step),
+ *   v_adj = c1 dt t + v_adj - dvv d,
+ *   a_adj = c1 c2 t + dt (1 - gamma) v_adj_old - dva d + dt gamma v_adj,
+ *   K_eff lam_n = w[n] q + c1 t - dvu d - c1 a_adj  (FP64 Jacobi PCG to tol,
+ *   started from lam_{n+1}),
+ *   dJdE += -lam_n^T (dK/dE) (u_n + ck v_n),
+ * with c1 = 1/(dt^2 (beta - gamma)), c2 = dt^2 (0.5 + beta - gamma),
+ * dvu = dt gamma c1, dva = dt (1 - gamma) + dt gamma c1 c2,
+ * dvv = 1 + dt^2 gamma c1.  The
  * reference evaluates every step's objective terms at the final time; for
  * these objectives the time does not enter.  brick8 and brickIM meshes. */
 int asend_newmark_adjoint(asend_ctx* ctx, int64_t n_steps, const double* f_shape, const double* load_fac,

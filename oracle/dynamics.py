@@ -5,9 +5,13 @@ it.  Follows, per step of Model::solve's dynamic loop
 (S/model_analysis.rs:560-615, S/ = solverSrc/asend_run_job/src/):
 
   solve_step(time)            (S/model_analysis.rs:312-558, elastic branch)
-      rhs = f(time) - R(u) + penalty load, R(u) = K u + M a with the node
-      accelerations a of the current state; one linear solve with
-      K_eff = K + c_m M (+ sf C^T C), c_m = -c1; u += du;
+      rhs = f(time) - R(u) + penalty load, R(u) = K u + M a + D v with the
+      node accelerations a and velocities v of the current state; one linear
+      solve with K_eff = K + c_m M + c_m dt gamma D (+ sf C^T C), c_m = -c1
+      (Element::get_ru_dfd0, S/element_equations.rs:948-1003); u += du;
+      D = ray_damp_cm M + ray_damp_ck K is Rayleigh damping
+      (Element::get_rud_dfd0, S/element_equations.rs:699-753; material
+      damping matrices are not restated);
       update_vel_acc           (S/node_meth.rs:127-140)
           c1 = 1/(dt^2 (beta - gamma)), c2 = dt^2 (0.5 + beta - gamma)
           a = c1 (u_prev - u + dt v_prev + c2 a_prev)
@@ -23,21 +27,26 @@ not run yet; first_step="predictor" uses a(u_0) like every later step (the
 device path).  From rest (u0 = v0 = a0 = 0) the two coincide.
 
 newmark_adjoint restates the backward sweep of Model::get_obj_gradient for a
-dynamic elastic job without damping (S/model_analysis.rs:1563-1631): for
-n = N..1, augmentd_ld_u (:936-1063) adds step n+1's coupling to step n,
+dynamic elastic job with Rayleigh damping (S/model_analysis.rs:1563-1631):
+for n = N..1, augmentd_ld_u (:936-1063, the M and D terms of :1033-1056)
+adds step n+1's coupling to step n,
 solve_for_adjoint (:1089-1100) forms the velocity and acceleration adjoints
 and solves K_eff lam_n = dL/du_n, and the DV loop accumulates
--lam_n^T dR_n/dd with the state u_n (d_relasticd_d, :1379-1457).  The
+-lam_n^T dR_n/dd with the state of step n (d_relasticd_d, :1379-1457):
+dR/dE_e = dK/dE_e (u_n + ray_damp_ck v_n).  The
 objective is a linear functional of the displacement history,
 L = sum_n w_n q^T u_n (displacement terms, S/objective_meth.rs:618-663, with
 exponent 1).
 
 Pinning (tests/test_dynamics_oracle.py): single-DOF free vibration against
 the closed form u_n = cos(n phi), tan(phi/2) = omega dt / 2 of the average
-acceleration rule; energy conservation of a free brick8 mesh; the two
-first-step modes agree from rest; newmark_adjoint against central finite
-differences of the forward run (penalty scale held fixed, as the reference
-does not differentiate it).
+acceleration rule; the damped single DOF against the analytic decaying
+oscillation; energy conservation of a free brick8 mesh, and with Rayleigh
+damping the exact discrete energy balance E_{n+1} - E_n = -dt vbar^T D vbar
+of the trapezoidal rule; the two first-step modes agree from rest;
+newmark_adjoint against central finite differences of the forward run, with
+and without damping (penalty scale held fixed, as the reference does not
+differentiate it).
 """
 import numpy as np
 import scipy.sparse as sp
@@ -45,18 +54,20 @@ import scipy.sparse.linalg as spla
 
 
 def newmark(K, M, f_shape, load_fac, dt, beta=0.25, gamma=0.5, u0=None, v0=None, a0=None, pen_dofs=(),
-            sf=None, watch=(), first_step="predictor", on_step=None):
+            sf=None, watch=(), first_step="predictor", on_step=None, damp_cm=0.0, damp_ck=0.0):
     """Run len(load_fac) steps.  K, M: FP64 sparse (no penalty); pen_dofs: DOFs
-    clamped by a penalty sf (default 1e5 max|K + c_m M|, the reference scale
-    taken on the first dynamic assembly).  on_step(n, u, v, a, Kp) is called
-    after every step.  Returns (hist [n x len(watch)], u, v, a)."""
+    clamped by a penalty sf (default 1e5 max|K_eff|, the reference scale
+    taken on the first dynamic assembly); damp_cm, damp_ck: Rayleigh
+    coefficients (job keys rayDampCM, rayDampCK).  on_step(n, u, v, a, Kp) is
+    called after every step.  Returns (hist [n x len(watch)], u, v, a)."""
     K = sp.csr_matrix(K)
     M = sp.csr_matrix(M)
     n = K.shape[0]
     c1 = 1.0 / (dt * dt * (beta - gamma))
     c2 = dt * dt * (0.5 + beta - gamma)
     c_m = -c1
-    keff = (K + c_m * M).tocsr()
+    D = (damp_cm * M + damp_ck * K).tocsr()
+    keff = (K + c_m * M + (c_m * dt * gamma) * D).tocsr()
     pen = np.zeros(n)
     if len(pen_dofs):
         if sf is None:
@@ -71,10 +82,11 @@ def newmark(K, M, f_shape, load_fac, dt, beta=0.25, gamma=0.5, u0=None, v0=None,
     hist = np.zeros((len(load_fac), watch.size))
     for step, fac in enumerate(load_fac):
         if step == 0 and first_step == "reference":
-            acc = a
+            acc, vel = a, v
         else:
             acc = c1 * (dt * v + c2 * a)
-        rhs = fac * f_shape - Kp @ u - M @ acc
+            vel = v + dt * ((1.0 - gamma) * a + gamma * acc)
+        rhs = fac * f_shape - Kp @ u - M @ acc - D @ vel
         du = solve(rhs)
         un = u + du
         an = c1 * (u - un + dt * v + c2 * a)
@@ -86,7 +98,8 @@ def newmark(K, M, f_shape, load_fac, dt, beta=0.25, gamma=0.5, u0=None, v0=None,
     return hist, u, v, a
 
 
-def newmark_adjoint(K, M, f_shape, load_fac, dt, beta, gamma, q, w, elem_sens, pen_dofs=(), sf=None):
+def newmark_adjoint(K, M, f_shape, load_fac, dt, beta, gamma, q, w, elem_sens, pen_dofs=(), sf=None,
+                    damp_cm=0.0, damp_ck=0.0):
     """Forward Newmark run from rest, then the reference's backward adjoint
     sweep.  elem_sens(u, lam) -> per-element -lam^T (dK_e/dE_e) u.  Returns
     (L, dL/dE) with L = sum_n w[n] q.u_n."""
@@ -95,15 +108,26 @@ def newmark_adjoint(K, M, f_shape, load_fac, dt, beta, gamma, q, w, elem_sens, p
     n = K.shape[0]
     c1 = 1.0 / (dt * dt * (beta - gamma))
     c2 = dt * dt * (0.5 + beta - gamma)
-    keff = (K - c1 * M).tocsr()
+    D = (damp_cm * M + damp_ck * K).tocsr()
+    keff = (K - c1 * M - (c1 * dt * gamma) * D).tocsr()
     pen = np.zeros(n)
     if len(pen_dofs):
         if sf is None:
             sf = 1e5 * abs(keff).max()
         np.add.at(pen, np.asarray(pen_dofs, dtype=np.int64), sf)
-    us = []
-    newmark(K, M, f_shape, load_fac, dt, beta, gamma, pen_dofs=pen_dofs, sf=sf,
-            on_step=lambda k, u, v, a, Kp: us.append(u.copy()))
+    us, vs = [], []
+
+    def keep(k, u, v, a, Kp):
+        us.append(u.copy())
+        vs.append(v.copy())
+
+    newmark(K, M, f_shape, load_fac, dt, beta, gamma, pen_dofs=pen_dofs, sf=sf, on_step=keep, damp_cm=damp_cm,
+            damp_ck=damp_ck)
+    # partial derivatives of (a, v) of step n+1 with respect to (u, v, a) of
+    # step n (augmentd_ld_u's d_ad_* and d_vd_*)
+    d_vd_up = dt * gamma * c1
+    d_vd_vp = 1.0 + dt * gamma * c1 * dt
+    d_vd_ap = dt * (1.0 - gamma) + dt * gamma * c1 * c2
     q = np.asarray(q, dtype=np.float64)
     L = float(sum(w[k] * (q @ us[k]) for k in range(len(us))))
     solve = spla.splu((keff + sp.diags(pen)).tocsc()).solve
@@ -117,12 +141,13 @@ def newmark_adjoint(K, M, f_shape, load_fac, dt, beta, gamma, q, w, elem_sens, p
         gv = np.zeros(n)
         if lam is not None:
             t = a_adj - M @ lam
-            gu = gu + c1 * t
-            ga = c1 * c2 * t + dt * (1.0 - gamma) * v_adj
-            gv = c1 * dt * t + v_adj
+            dlam = D @ lam
+            gu = gu + c1 * t - d_vd_up * dlam
+            ga = c1 * c2 * t + dt * (1.0 - gamma) * v_adj - d_vd_ap * dlam
+            gv = c1 * dt * t + v_adj - d_vd_vp * dlam
         v_adj = gv
         a_adj = ga + dt * gamma * v_adj
         lam = solve(gu - c1 * a_adj)
-        s = elem_sens(us[k], lam)
+        s = elem_sens(us[k] + damp_ck * vs[k], lam)
         dL = s if dL is None else dL + s
     return L, dL

@@ -1,6 +1,8 @@
 """Device Newmark stepping against the CPU restatement of the reference's
 stepping (oracle/dynamics.py) on a brick8 cantilever under a half-sine tip
-pulse; graph replay and eager launches give identical histories."""
+pulse, undamped and with Rayleigh damping; graph replay and eager launches
+give identical histories; the transient adjoint against the oracle's sweep
+with and without damping."""
 import numpy as np
 import pytest
 
@@ -88,6 +90,66 @@ def test_newmark_adjoint_matches_oracle(ctx, etype):
     problems.upload(ctx, p, rho=rho)
     ctx.newmark_setup(dt, 0.25, 0.5)
     L, dL, fst, ast = ctx.newmark_adjoint(n, p.f, fac, q, w, tol=1e-12, max_it_step=400)
+    assert fst.unconverged_steps == 0 and ast.unconverged_steps == 0, (fst.as_dict(), ast.as_dict())
+    assert abs(L - L_o) <= 1e-8 * abs(L_o)
+    assert np.abs(dL - dL_o).max() <= 1e-7 * np.abs(dL_o).max()
+
+
+@pytest.mark.parametrize("beta,gamma", [(0.25, 0.5), (0.25, 0.55)])
+def test_damped_newmark_matches_oracle(ctx, beta, gamma):
+    # Rayleigh damping D = cm M + ck K (rayDampCM / rayDampCK): about 10% of
+    # critical on the first bending mode from cm, 1% from ck
+    gpu_case(f"damped newmark 12x3x3 beta {beta} gamma {gamma}")
+    p, K, M = cantilever_case((12, 3, 3))
+    cm, ck = 200.0, 2e-5
+    dt, n = 1e-4, 50
+    fac = np.where(np.arange(n) < 20, np.sin(np.pi * np.arange(n) / 20.0), 0.0)
+    watch = tip_dofs(p)
+    hist_o, u_o, v_o, a_o = dynamics.newmark(K, M, p.f, fac, dt, beta, gamma, pen_dofs=p.pen_dofs, watch=watch,
+                                             damp_cm=cm, damp_ck=ck)
+    problems.upload(ctx, p, rho=np.full(p.mesh.n_el, RHO))
+    try:
+        ctx.set_rayleigh_damping(cm, ck)
+        ctx.newmark_setup(dt, beta, gamma)
+        hist, u, v, a, st = ctx.newmark_run(n, p.f, fac, watch=watch, tol=1e-12, max_it_step=400)
+    finally:
+        ctx.set_rayleigh_damping(0.0, 0.0)
+    assert st.unconverged_steps == 0 and st.steps == n, st.as_dict()
+    scale = np.abs(hist_o).max()
+    assert np.abs(hist - hist_o).max() <= 1e-8 * scale
+    for x, xo in ((u, u_o), (v, v_o), (a, a_o)):
+        assert np.abs(x - xo).max() <= 1e-7 * np.abs(xo).max()
+    # the damping is visible: the undamped history differs at the percent level
+    hist_0, *_ = dynamics.newmark(K, M, p.f, fac, dt, beta, gamma, pen_dofs=p.pen_dofs, watch=watch)
+    assert np.abs(hist_0 - hist_o).max() > 1e-2 * scale
+
+
+@pytest.mark.parametrize("etype,cm,ck,beta,gamma", [(8, 300.0, 4e-6, 0.25, 0.55), (81, 300.0, 0.0, 0.25, 0.5)])
+def test_damped_newmark_adjoint_matches_oracle(ctx, etype, cm, ck, beta, gamma):
+    # the transient adjoint with Rayleigh damping: D in K_eff, the D lam
+    # coupling terms (the acceleration one is nonzero for gamma != 2 beta) and
+    # dR/dE acting on u + ck v
+    gpu_case(f"damped newmark adjoint etype {etype} cm {cm} ck {ck}")
+    p = problems.cantilever(12, 3, 3, E=70e9 * np.exp(0.3 * np.random.default_rng(5).standard_normal(108)),
+                            etype=etype)
+    rho = np.full(p.mesh.n_el, RHO)
+    K = O.assemble_solid(etype, p.mesh.xyz, p.mesh.conn, p.E, p.nu, p.dof_index, p.n_dof)
+    M = O.assemble_solid_mass(etype, p.mesh.xyz, p.mesh.conn, rho, p.dof_index, K)
+    dt, n = 5e-5, 30
+    fac = np.where(np.arange(n) < 15, np.sin(np.pi * np.arange(n) / 15.0), 0.0)
+    q = np.zeros(p.n_dof)
+    q[tip_dofs(p)] = 1.0
+    w = 1.0 + 0.05 * np.arange(n)
+    sens = lambda u, lam: O.solid_sens(etype, p.mesh.xyz, p.mesh.conn, p.E, p.nu, p.dof_index, u, lam)
+    L_o, dL_o = dynamics.newmark_adjoint(K.to_scipy(), M.to_scipy(), p.f, fac, dt, beta, gamma, q, w, sens,
+                                         pen_dofs=p.pen_dofs, damp_cm=cm, damp_ck=ck)
+    problems.upload(ctx, p, rho=rho)
+    try:
+        ctx.set_rayleigh_damping(cm, ck)
+        ctx.newmark_setup(dt, beta, gamma)
+        L, dL, fst, ast = ctx.newmark_adjoint(n, p.f, fac, q, w, tol=1e-12, max_it_step=400)
+    finally:
+        ctx.set_rayleigh_damping(0.0, 0.0)
     assert fst.unconverged_steps == 0 and ast.unconverged_steps == 0, (fst.as_dict(), ast.as_dict())
     assert abs(L - L_o) <= 1e-8 * abs(L_o)
     assert np.abs(dL - dL_o).max() <= 1e-7 * np.abs(dL_o).max()

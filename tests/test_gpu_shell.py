@@ -142,6 +142,40 @@ def test_shell_newmark_matches_oracle(ctx):
         assert np.abs(x - xo).max() <= 1e-6 * np.abs(xo).max()
 
 
+def test_shell_mass_damped_newmark_matches_oracle(ctx):
+    # mass-proportional Rayleigh damping on the plate (stiffness-proportional
+    # damping of elements with condensed DOFs is not built: an error)
+    gpu_case("shell4 newmark 12x12 with mass damping vs oracle")
+    p = problems.shell_plate(12, 12, 1.0, 1.0, 0.005, 70e9, 0.33, 2700.0)
+    K, M = oracle_km(p)
+    di = np.asarray(p.dof_index).reshape(-1, 6)
+    centre = p.mesh.node_id(6, 6)
+    f = np.zeros(p.n_dof)
+    f[di[centre, 2]] = 1.0e4
+    dt, n = 1e-5, 60
+    fac = np.where(np.arange(n) < 40, np.sin(np.pi * np.arange(n) / 40.0), 0.0)
+    watch = di[centre, :3]
+    cm = 2000.0
+    hist_o, u_o, v_o, a_o = dynamics.newmark(K, M, f, fac, dt, pen_dofs=p.pen_dofs, watch=watch, damp_cm=cm)
+    problems.upload_shell(ctx, p)
+    try:
+        ctx.set_rayleigh_damping(0.0, 1e-6)
+        with pytest.raises(api.AsendError, match="stiffness-proportional"):
+            ctx.newmark_setup(dt, 0.25, 0.5)
+        ctx.set_rayleigh_damping(cm, 0.0)
+        ctx.newmark_setup(dt, 0.25, 0.5)
+        hist, u, v, a, st = ctx.newmark_run(n, f, fac, watch=watch, tol=1e-12, max_it_step=4000)
+    finally:
+        ctx.set_rayleigh_damping(0.0, 0.0)
+    assert st.unconverged_steps == 0 and st.steps == n, st.as_dict()
+    scale = np.abs(hist_o[:, 2]).max()
+    assert np.abs(hist[:, 2] - hist_o[:, 2]).max() <= 1e-7 * scale
+    for x, xo in ((u, u_o), (v, v_o)):
+        assert np.abs(x - xo).max() <= 1e-6 * np.abs(xo).max()
+    hist_0, *_ = dynamics.newmark(K, M, f, fac, dt, pen_dofs=p.pen_dofs, watch=watch)
+    assert np.abs(hist_0[:, 2] - hist_o[:, 2]).max() > 1e-3 * scale
+
+
 def test_shell_error_paths(ctx):
     gpu_case("shell4 error paths")
     p = problems.shell_plate(4, 3, 1.0, 1.0, 0.01, 70e9, 0.3, 2700.0)
