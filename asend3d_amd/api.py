@@ -22,6 +22,8 @@ SHELL4 = 41
 
 SOLVE_PCG_FP64 = 0
 SOLVE_IR_MIXED = 1
+NM_FP64 = 0   # Newmark step solver: FP64 single-reduction Jacobi PCG (default)
+NM_MIXED = 1  # FP32 single-reduction Jacobi PCG inside FP64 refinement
 
 PC_JACOBI = 0
 PC_BLOCK_JACOBI = 1
@@ -121,7 +123,7 @@ EXPORTED = ["asend_ctx_create", "asend_ctx_destroy", "asend_last_error", "asend_
             "asend_learned_num_params", "asend_learned_init", "asend_learned_get_params",
             "asend_learned_set_params", "asend_learned_train", "asend_learned_eval", "asend_learned_apply",
             "asend_learned_edges", "asend_mg_set_smoother", "asend_learned_vcycle_eval", "asend_set_graphs",
-            "asend_set_rayleigh_damping", "asend_newmark_setup", "asend_newmark_run", "asend_newmark_adjoint", "asend_modal", "asend_batch_create", "asend_batch_destroy",
+            "asend_set_rayleigh_damping", "asend_newmark_set_solver", "asend_newmark_setup", "asend_newmark_run", "asend_newmark_adjoint", "asend_modal", "asend_batch_create", "asend_batch_destroy",
             "asend_batch_error", "asend_batch_solve", "asend_autotune"]
 
 
@@ -370,6 +372,11 @@ class Context:
         x = np.zeros(b.shape[0], dtype=np.float32)
         self._chk(_lib.asend_mg_coarse_solve(self.h, _p(b), _p(x)), "asend_mg_coarse_solve")
         return x
+
+    def newmark_set_solver(self, mode=NM_FP64, inner_tol=1e-4):
+        """Step solver of newmark_run (NM_MIXED or NM_FP64)."""
+        self._chk(_lib.asend_newmark_set_solver(self.h, ctypes.c_int(mode), _dbl(inner_tol)),
+                  "asend_newmark_set_solver")
 
     def set_rayleigh_damping(self, cm=0.0, ck=0.0):
         """D = cm M + ck K for the next newmark_setup (rayDampCM / rayDampCK)."""

@@ -54,10 +54,11 @@ __global__ __launch_bounds__(kBlock) void k_sell3_mrhs(int n_slices, const int32
     const int32_t* cp = scol + (int64_t)k0 * kWave + lane;
     const T* vp = val + (int64_t)k0 * 9 * kWave + lane;
     for (int k = k0; k < k1; ++k) {
-        const int64_t c = *cp;
+        // values and columns streamed once: non-temporal, keeping X in L2
+        const int64_t c = __builtin_nontemporal_load(cp);
         T a[9];
 #pragma unroll
-        for (int q = 0; q < 9; ++q) a[q] = vp[q * kWave];
+        for (int q = 0; q < 9; ++q) a[q] = __builtin_nontemporal_load(vp + q * kWave);
         const T* xc = X + c * 3 * kNrhs + j0;
 #pragma unroll
         for (int j = 0; j < NR; ++j) {

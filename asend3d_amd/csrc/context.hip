@@ -100,6 +100,9 @@ struct asend_ctx {
     double asm_kscale = 1.0;  // stiffness factor of the current assembly
     // Rayleigh damping D = ray_cm M + ray_ck K of the next Newmark setup
     double ray_cm = 0.0, ray_ck = 0.0;
+    // Newmark step solver (asend_newmark_set_solver)
+    int nm_mode = ASEND_NM_FP64;
+    double nm_inner_tol = 1e-4;
     bool use_graphs = true;
     // autotuned choices: SpMV workgroup geometry, inner tolerance and check
     // interval used when a solve passes 0 for them
@@ -139,6 +142,13 @@ struct asend_ctx {
         int64_t yhist_cap = 0;
         const double* graph_save_y = nullptr;
         bool damped() const { return cm != 0.0 || ck != 0.0; }
+        // step solver: mixed (FP32 single-reduction PCG inside FP64
+        // refinement) or FP64 single-reduction PCG (the default); the FP32
+        // work vectors of the mixed solver (7 n floats)
+        int mixed = 0;
+        double inner_tol = 1e-4;
+        float* f32 = nullptr;
+        int graph_mixed = -1;
     } nm;
 };
 
@@ -217,6 +227,8 @@ void newmark_free(asend_ctx* ctx) {
                        &nm.uhist, &nm.lam, &nm.aadj, &nm.vadj, &nm.mlam, &nm.qv, &nm.sens_acc, &nm.cg_u, &nm.cg_s,
                        &nm.D64, &nm.rhsd, &nm.yhist})
         dfree(*q);
+    dfree(nm.f32);
+    nm.graph_mixed = -1;
     nm.yhist_cap = 0;
     nm.save_y = nullptr;
     nm.graph_save_y = nullptr;
@@ -1377,6 +1389,17 @@ int asend_set_rayleigh_damping(asend_ctx* ctx, double cm, double ck) {
     return ASEND_OK;
 }
 
+int asend_newmark_set_solver(asend_ctx* ctx, int mode, double inner_tol) {
+    if (!ctx || (mode != ASEND_NM_MIXED && mode != ASEND_NM_FP64) ||
+        (mode == ASEND_NM_MIXED && !(inner_tol > 0.0 && inner_tol < 1.0)))
+        return ASEND_ERR_ARG;
+    ctx->nm_mode = mode;
+    if (mode == ASEND_NM_MIXED) ctx->nm_inner_tol = inner_tol;
+    ctx->nm.mixed = mode == ASEND_NM_MIXED ? 1 : 0;
+    ctx->nm.inner_tol = ctx->nm_inner_tol;
+    return ASEND_OK;
+}
+
 int asend_newmark_setup(asend_ctx* ctx, double dt, double beta, double gamma) {
     if (!ctx || !(dt > 0.0)) return ASEND_ERR_ARG;
     if (!ctx->d_K64) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
@@ -1384,6 +1407,8 @@ int asend_newmark_setup(asend_ctx* ctx, double dt, double beta, double gamma) {
     HIPCHK(hipSetDevice(ctx->device));
     newmark_free(ctx);
     auto& nm = ctx->nm;
+    nm.mixed = ctx->nm_mode == ASEND_NM_MIXED ? 1 : 0;
+    nm.inner_tol = ctx->nm_inner_tol;
     nm.dt = dt;
     nm.beta = beta;
     nm.gamma = gamma;
@@ -1403,7 +1428,7 @@ int asend_newmark_setup(asend_ctx* ctx, double dt, double beta, double gamma) {
     if ((rc = dalloc(ctx, &nm.M64, nval)) || (rc = dalloc(ctx, &nm.v, n)) || (rc = dalloc(ctx, &nm.a, n)) ||
         (rc = dalloc(ctx, &nm.w, n)) || (rc = dalloc(ctx, &nm.q, n)) || (rc = dalloc(ctx, &nm.rhs0, n)) ||
         (rc = dalloc(ctx, &nm.rhs, n)) || (rc = dalloc(ctx, &nm.fshape, n)) || (rc = dalloc(ctx, &nm.step, 1)) ||
-        (rc = dalloc(ctx, &nm.cg_u, n)) || (rc = dalloc(ctx, &nm.cg_s, n)))
+        (rc = dalloc(ctx, &nm.cg_u, n)) || (rc = dalloc(ctx, &nm.cg_s, n)) || (rc = dalloc(ctx, &nm.f32, 7 * n)))
         return rc;
     // consistent mass alone: the element kernels with c_mass = 1 and no
     // stiffness (zero moduli for the brick8 kernels, k_scale = 0 for the
@@ -1494,22 +1519,61 @@ int asend_newmark_run(asend_ctx* ctx, int64_t n_steps, const double* f_shape, co
         h.tol2 = tol * tol;
         h.max_it = max_it_step;
         if ((rc = write_scalars(ctx, 1))) return rc;
+        // inner FP32 solve of the mixed step solver, relative to its
+        // normalised right-hand side
+        Scalars& hi = ctx->h_sc[3];
+        std::memset(&hi, 0, sizeof(Scalars));
+        hi.tol = nm.inner_tol;
+        hi.tol2 = nm.inner_tol * nm.inner_tol;
+        hi.max_it = max_it_step;
+        hi.done = 1;
+        if ((rc = write_scalars(ctx, 3))) return rc;
     }
     Scalars* sc = ctx->d_sc + 1;
+    const Sell3 A = ctx->fine();
+    MixedSolve mx;
+    mx.A = &A;
+    mx.K64 = ctx->d_K64;
+    mx.dinv32 = ctx->d_dinv32;
+    mx.rhs = nm.rhs;
+    mx.x64 = ctx->d_x64;
+    mx.r64 = ctx->d_r2;
+    float* f = nm.f32;
+    mx.b32 = f;
+    mx.d32 = f + n;
+    mx.r32 = f + 2 * n;
+    mx.u32 = f + 3 * n;
+    mx.p32 = f + 4 * n;
+    mx.s32 = f + 5 * n;
+    float* w32 = f + 6 * n;
+    mx.partials = ctx->d_partials;
+    mx.so = sc;
+    mx.si = ctx->d_sc + 3;
+    mx.n = n;
     const int nw = (int)n_watch;
     // A step is three launch sequences: head (effective load, PCG start and a
     // first block of iterations), more (another block, repeated until the
     // step's solve has converged: one small device-to-host read per block),
     // tail (state update and history).  Each is captured once as a hipGraph.
     const int blk = std::min(max_it_step, kNewmarkBlock);
-    // single-reduction Jacobi PCG: x = du, r, u = Dinv r, w = K u, p, s = K p
+    // single-reduction Jacobi PCG: x = du, r, u = Dinv r, w = K u, p, s = K p;
+    // mixed: the same recurrence in FP32 on the normalised refinement
+    // residual, then a refinement transition (a no-op until the inner solve
+    // has finished)
+    const bool mixed = nm.mixed != 0;
     auto iterations = [&]() {
         for (int k = 0; k < blk; ++k) {
-            launch_sell3<double>(ctx->st, 4, ctx->fine(), ctx->d_K64, nm.cg_u, ctx->d_q, ctx->d_r2, ctx->d_partials,
-                                 sc);
-            launch_cg1_update<double>(ctx->st, n, ctx->d_dinv64, ctx->d_x64, ctx->d_r2, nm.cg_u, ctx->d_q, ctx->d_p,
-                                      nm.cg_s, sc);
+            if (mixed) {
+                launch_sell3<float>(ctx->st, 4, A, ctx->d_K32, mx.u32, w32, mx.r32, ctx->d_partials, mx.si);
+                launch_cg1_update<float>(ctx->st, n, ctx->d_dinv32, mx.d32, mx.r32, mx.u32, w32, mx.p32, mx.s32,
+                                         mx.si);
+            } else {
+                launch_sell3<double>(ctx->st, 4, A, ctx->d_K64, nm.cg_u, ctx->d_q, ctx->d_r2, ctx->d_partials, sc);
+                launch_cg1_update<double>(ctx->st, n, ctx->d_dinv64, ctx->d_x64, ctx->d_r2, nm.cg_u, ctx->d_q,
+                                          ctx->d_p, nm.cg_s, sc);
+            }
         }
+        if (mixed) launch_mixed_transition(ctx->st, mx);
     };
     auto head = [&]() {
         launch_nm_predict(ctx->st, n, ctx->d_u, nm.v, nm.a, nm.dt, nm.c2, nm.w);
@@ -1524,9 +1588,13 @@ int asend_newmark_run(asend_ctx* ctx, int64_t n_steps, const double* f_shape, co
         launch_sell3<double>(ctx->st, 3, ctx->fine(), ctx->d_K64, ctx->d_u, nm.rhs, load, nullptr, nullptr);
         // warm start from the previous step's increment (zero before the first
         // step); the tolerance stays relative to |rhs|
-        launch_sell3<double>(ctx->st, 3, ctx->fine(), ctx->d_K64, ctx->d_x64, ctx->d_r2, nm.rhs, nullptr, nullptr);
-        launch_cg1_init<double>(ctx->st, n, nm.rhs, false, ctx->d_dinv64, ctx->d_x64, ctx->d_r2, nm.cg_u, ctx->d_p,
-                                nm.cg_s, ctx->d_partials, sc);
+        if (mixed) {
+            launch_mixed_start(ctx->st, mx);
+        } else {
+            launch_sell3<double>(ctx->st, 3, A, ctx->d_K64, ctx->d_x64, ctx->d_r2, nm.rhs, nullptr, nullptr);
+            launch_cg1_init<double>(ctx->st, n, nm.rhs, false, ctx->d_dinv64, ctx->d_x64, ctx->d_r2, nm.cg_u,
+                                    ctx->d_p, nm.cg_s, ctx->d_partials, sc);
+        }
         iterations();
     };
     auto tail = [&]() {
@@ -1557,7 +1625,7 @@ int asend_newmark_run(asend_ctx* ctx, int64_t n_steps, const double* f_shape, co
     };
     const bool graph = ctx->use_graphs;
     if (graph && (!nm.exec_head || nm.graph_iters != blk || nm.graph_watch != nw || nm.graph_save != nm.save_u ||
-                  nm.graph_save_y != nm.save_y)) {
+                  nm.graph_save_y != nm.save_y || nm.graph_mixed != nm.mixed)) {
         if ((rc = capture(&nm.exec_head, head)) || (rc = capture(&nm.exec_more, iterations)) ||
             (rc = capture(&nm.exec_tail, tail)))
             return rc;
@@ -1565,6 +1633,7 @@ int asend_newmark_run(asend_ctx* ctx, int64_t n_steps, const double* f_shape, co
         nm.graph_watch = nw;
         nm.graph_save = nm.save_u;
         nm.graph_save_y = nm.save_y;
+        nm.graph_mixed = nm.mixed;
     }
     HIPCHK(hipEventRecord(ctx->ev0, ctx->st));
     for (int64_t s = 0; s < n_steps; ++s) {

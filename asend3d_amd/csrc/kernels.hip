@@ -346,12 +346,14 @@ enum FinOp {
     FIN_MAX = 5,
     FIN_INIT_WARM = 6,
     FIN_CG1_INIT = 7,
-    FIN_CG1 = 8
+    FIN_CG1 = 8,
+    FIN_MX_RES = 9
 };
 
 __global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ partials, int nparts, int op,
                                                      Scalars* sc) {
     if ((op == FIN_PQ || op == FIN_UPDATE || op == FIN_CG1) && sc->done) return;
+    if (op == FIN_MX_RES && !sc->gate) return;
     if (op == FIN_MAX) {
         double m = 0.0;
         for (int b = threadIdx.x; b < nparts; b += blockDim.x) m = fmax(m, partials[b]);
@@ -430,6 +432,15 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ 
             sc->it += 1;
             break;
         }
+        case FIN_MX_RES:
+            // refinement residual: converged, or a cap reached, ends the solve
+            sc->rr = t[0];
+            sc->n_resid += 1;
+            if (t[0] <= sc->tol2 * sc->ff || sc->n_resid >= kMixedMaxResiduals || sc->it >= sc->max_it) {
+                sc->done = 1;
+                sc->gate = 0;
+            }
+            break;
         case FIN_UPDATE: {
             sc->beta = t[0] / sc->rz;
             sc->rz = t[0];
@@ -525,6 +536,7 @@ __global__ __launch_bounds__(kSellMaxWaves * kWave) void k_sell3(int n_rows, int
                                                   const T* __restrict__ x, T* __restrict__ y,
                                                   const T* __restrict__ f, double* partials, const Scalars* sc) {
     if ((MODE == 1 || MODE == 4) && sc->done) return;
+    if (MODE == 5 && !sc->gate) return;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int slice = lb * (int)(blockDim.x / kWave) + threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
@@ -537,16 +549,21 @@ __global__ __launch_bounds__(kSellMaxWaves * kWave) void k_sell3(int n_rows, int
         T a0 = 0, a1 = 0, a2 = 0;
 #pragma unroll 2
         for (int k = k0; k < k1; ++k) {
-            const int c = *cp;
+            // values and columns are streamed once: non-temporal loads keep
+            // them from evicting the gathered x from L2
+            const int c = __builtin_nontemporal_load(cp);
             const T x0 = x[3 * (int64_t)c], x1 = x[3 * (int64_t)c + 1], x2 = x[3 * (int64_t)c + 2];
-            a0 += vp[0 * kWave] * x0 + vp[1 * kWave] * x1 + vp[2 * kWave] * x2;
-            a1 += vp[3 * kWave] * x0 + vp[4 * kWave] * x1 + vp[5 * kWave] * x2;
-            a2 += vp[6 * kWave] * x0 + vp[7 * kWave] * x1 + vp[8 * kWave] * x2;
+            T v[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) v[q] = __builtin_nontemporal_load(vp + q * kWave);
+            a0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
+            a1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
+            a2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
             cp += kWave;
             vp += 9 * kWave;
         }
         if (r < n_rows) {
-            if (MODE == 2 || MODE == 3) {
+            if (MODE == 2 || MODE == 3 || MODE == 5) {
                 T y0 = f[3 * (int64_t)r] - a0, y1 = f[3 * (int64_t)r + 1] - a1, y2 = f[3 * (int64_t)r + 2] - a2;
                 y[3 * (int64_t)r] = y0;
                 y[3 * (int64_t)r + 1] = y1;
@@ -608,6 +625,11 @@ void launch_sell3(hipStream_t st, int mode, const Sell3& A, const T* val, const 
                                val, x, y, f, partials, sc);
             finalize(st, partials, nblk, FIN_CG1, sc);
             break;
+        case 5:
+            hipLaunchKernelGGL((k_sell3<T, 5>), dim3(nblk), dim3(nthr), 0, st, A.n_rows, A.n_slices, A.sptr, A.scol,
+                               val, x, y, f, partials, sc);
+            finalize(st, partials, nblk, FIN_MX_RES, sc);
+            break;
         default:
             hipLaunchKernelGGL((k_sell3<T, 3>), dim3(nblk), dim3(nthr), 0, st, A.n_rows, A.n_slices, A.sptr, A.scol,
                                val, x, y, f, partials, sc);
@@ -628,6 +650,8 @@ template void launch_sell3<float>(hipStream_t, int, const Sell3&, const float*, 
 // MODE 3: y = f - K x without the norm
 // MODE 4 (sliced-ELL only): y = K x with x = u = Dinv r and f = r, partials of
 //   r.u, y.u and r.r -> finalize: the single-reduction PCG step scalars
+// MODE 5 (sliced-ELL only): MODE 2 when sc->gate is set (mixed solve's
+//   refinement residual) -> finalize: sc->rr, convergence of the refinement
 // Kernels with a scalars pointer return at once when sc->done is set.
 // ---------------------------------------------------------------------------
 template <typename T, int G, int MODE>
@@ -794,7 +818,8 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cg1_init(int64_t n, const T* __restrict__ b, int cold,
                                                      const T* __restrict__ dinv, T* __restrict__ x, T* __restrict__ r,
                                                      T* __restrict__ u, T* __restrict__ p, T* __restrict__ s,
-                                                     double* partials) {
+                                                     double* partials, const Scalars* __restrict__ gate) {
+    if (gate && !gate->gate) return;
     double t[1] = {0.0};
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const T bi = b[i];
@@ -837,8 +862,74 @@ template <typename T>
 void launch_cg1_init(hipStream_t st, int64_t n, const T* b, bool cold, const T* dinv, T* x, T* r, T* u, T* p, T* s,
                      double* partials, Scalars* sc) {
     hipLaunchKernelGGL(k_cg1_init<T>, dim3(kVecBlocks), dim3(kBlock), 0, st, n, b, cold ? 1 : 0, dinv, x, r, u, p, s,
-                       partials);
+                       partials, (const Scalars*)nullptr);
     finalize(st, partials, kVecBlocks, FIN_CG1_INIT, sc);
+}
+
+// --- mixed-precision solve under device-side control -----------------------
+// so->gate is the transition's condition; every transition kernel checks it.
+__global__ void k_mx_start(Scalars* __restrict__ so, Scalars* __restrict__ si) {
+    so->ff = so->dot;
+    so->rr = so->dot;
+    so->it = 0;
+    so->n_resid = 0;
+    so->done = so->dot == 0.0;
+    so->gate = !so->done;
+    si->done = 1;
+}
+__global__ void k_mx_gate(Scalars* __restrict__ so, const Scalars* __restrict__ si) {
+    const int g = si->done && !so->done;
+    so->gate = g;
+    if (g) so->it += si->it;
+}
+// x += |r| d
+__global__ void k_mx_accept(int64_t n, double* __restrict__ x, const float* __restrict__ d,
+                            const Scalars* __restrict__ so) {
+    if (!so->gate) return;
+    const double s = sqrt(so->rr);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] += s * (double)d[i];
+}
+// b = r / |r| in FP32
+__global__ void k_mx_scale(int64_t n, const double* __restrict__ r, float* __restrict__ b,
+                           const Scalars* __restrict__ so) {
+    if (!so->gate) return;
+    const double s = 1.0 / sqrt(so->rr);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = (float)(r[i] * s);
+}
+// inner start's scalars: |b|^2, iteration count, done
+__global__ __launch_bounds__(kBlock) void k_mx_fin_init(const double* __restrict__ partials, int nparts,
+                                                        const Scalars* __restrict__ so, Scalars* __restrict__ si) {
+    if (!so->gate) return;
+    double t[1] = {0.0};
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) t[0] += partials[b];
+    block_sum<1>(t);
+    if (threadIdx.x != 0) return;
+    si->ff = t[0];
+    si->rz = 0.0;
+    si->it = 0;
+    si->done = t[0] == 0.0;
+}
+
+static void mixed_residual_and_restart(hipStream_t st, const MixedSolve& m) {
+    launch_sell3<double>(st, 5, *m.A, m.K64, m.x64, m.r64, m.rhs, m.partials, m.so);
+    hipLaunchKernelGGL(k_mx_scale, dim3(kVecBlocks), dim3(kBlock), 0, st, m.n, m.r64, m.b32, m.so);
+    hipLaunchKernelGGL(k_cg1_init<float>, dim3(kVecBlocks), dim3(kBlock), 0, st, m.n, m.b32, 1, m.dinv32, m.d32,
+                       m.r32, m.u32, m.p32, m.s32, m.partials, (const Scalars*)m.so);
+    hipLaunchKernelGGL(k_mx_fin_init, dim3(1), dim3(kBlock), 0, st, m.partials, kVecBlocks, m.so, m.si);
+}
+
+void launch_mixed_start(hipStream_t st, const MixedSolve& m) {
+    launch_dot64(st, m.n, m.rhs, m.rhs, m.partials, m.so);
+    hipLaunchKernelGGL(k_mx_start, dim3(1), dim3(1), 0, st, m.so, m.si);
+    mixed_residual_and_restart(st, m);
+}
+
+void launch_mixed_transition(hipStream_t st, const MixedSolve& m) {
+    hipLaunchKernelGGL(k_mx_gate, dim3(1), dim3(1), 0, st, m.so, m.si);
+    hipLaunchKernelGGL(k_mx_accept, dim3(kVecBlocks), dim3(kBlock), 0, st, m.n, m.x64, m.d32, m.so);
+    mixed_residual_and_restart(st, m);
 }
 template <typename T>
 void launch_cg1_update(hipStream_t st, int64_t n, const T* dinv, T* x, T* r, T* u, const T* w, T* p, T* s,

@@ -13,7 +13,9 @@ constexpr int kMaxPartials = 8192; // partial-sum slots per reduction value
 struct Scalars {
     double rz, pq, rr, ff, alpha, beta, tol, tol2, maxabs, sf, dot, zq;
     int64_t it, max_it;
-    int done, stop_abs, pad0, pad1;
+    int done, stop_abs;
+    int gate;     // mixed step solve: a refinement transition is due
+    int n_resid;  // mixed step solve: FP64 residual evaluations so far
 };
 
 // Sliced-ELL fine operator: 64-row slices of 3x3 blocks (see kernels.hip).
@@ -103,6 +105,31 @@ void launch_cg1_init(hipStream_t st, int64_t n, const T* b, bool cold, const T* 
 template <typename T>
 void launch_cg1_update(hipStream_t st, int64_t n, const T* dinv, T* x, T* r, T* u, const T* w, T* p, T* s,
                        const Scalars* sc);
+
+// Mixed-precision solve of K x = rhs under device-side control, for launch
+// sequences captured once and replayed: FP64 iterative refinement around a
+// single-reduction FP32 Jacobi PCG (launch_sell3<float> mode 4 +
+// launch_cg1_update<float> on si, stopping at |r| <= si->tol |b|, at most
+// si->max_it iterations per start).  so holds the refinement: ff = |rhs|^2,
+// rr = |rhs - K x|^2 of the latest FP64 residual, it = inner iterations so
+// far (cap so->max_it), done when rr <= so->tol2 ff or a cap is reached.
+struct MixedSolve {
+    const Sell3* A = nullptr;
+    const double* K64 = nullptr;
+    const float* dinv32 = nullptr;
+    const double* rhs = nullptr;
+    double *x64 = nullptr, *r64 = nullptr;
+    float *b32 = nullptr, *d32 = nullptr, *r32 = nullptr, *u32 = nullptr, *p32 = nullptr, *s32 = nullptr;
+    double* partials = nullptr;
+    Scalars *so = nullptr, *si = nullptr;
+    int64_t n = 0;
+};
+constexpr int kMixedMaxResiduals = 40;
+// start from the x64 passed in: |rhs|^2, first residual, first inner start
+void launch_mixed_start(hipStream_t st, const MixedSolve& m);
+// refinement transition, a no-op unless the inner solve has finished and the
+// outer one has not: x += |r| d, new FP64 residual, next inner start
+void launch_mixed_transition(hipStream_t st, const MixedSolve& m);
 
 void launch_scale_to_f32(hipStream_t st, int64_t n, const double* r, float* r32, const Scalars* sc);
 void launch_axpy_from_f32(hipStream_t st, int64_t n, double* u, const float* d, const Scalars* sc_outer);

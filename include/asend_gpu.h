@@ -271,6 +271,17 @@ typedef struct asend_newmark_stats {
     int graph_used;               /* 1 when the steps ran as graph replays    */
     double device_ms;             /* HIP-event time of the step loop          */
 } asend_newmark_stats;
+/* Step solver of asend_newmark_run: ASEND_NM_FP64 (default) runs a
+ * single-reduction Jacobi PCG in FP64; ASEND_NM_MIXED solves each
+ * step's K_eff du = rhs by FP64 iterative refinement (residual rhs - K_eff du
+ * and the update in FP64) around a single-reduction FP32 Jacobi PCG run to
+ * inner_tol relative to its normalised right-hand side (default 1e-4;
+ * inner_tol is ignored for ASEND_NM_FP64).  Either stops at
+ * ||rhs - K_eff du|| <= tol ||rhs|| in FP64.
+ * Takes effect at the next asend_newmark_run. */
+#define ASEND_NM_FP64 0
+#define ASEND_NM_MIXED 1
+int asend_newmark_set_solver(asend_ctx* ctx, int mode, double inner_tol);
 /* Rayleigh damping D = cm M + ck K for the next asend_newmark_setup (job keys
  * rayDampCM / rayDampCK, JobCommand::ray_damp_cm / ray_damp_ck,
  * S/job.rs:15-16; Element::get_rud_dfd0, S/element_equations.rs:699-753).
@@ -288,7 +299,9 @@ int asend_newmark_setup(asend_ctx* ctx, double dt, double beta, double gamma);
 /* n_steps steps from (u0, v0, a0) (null = zero).  The load of step n is
  * load_fac[n] * f_shape; the step's residual is f - K u - M a(u) - D v(u) at
  * the predicted state (S/model_analysis.rs:312-558, S/node_meth.rs:127-199).  hist[n * n_watch + k] = u[watch[k]] after step n.
- * Each step's solve stops at ||r|| <= tol ||rhs|| or max_it_step iterations.
+ * Each step's solve stops at ||r|| <= tol ||rhs|| or max_it_step iterations
+ * (inner iterations for the mixed solver, which also stops after 40 FP64
+ * residuals); it starts from the previous step's increment.
  * Final u, v, a into the out arrays when non-null. */
 int asend_newmark_run(asend_ctx* ctx, int64_t n_steps, const double* f_shape, const double* load_fac,
                       const double* u0, const double* v0, const double* a0, int64_t n_watch, const int64_t* watch,

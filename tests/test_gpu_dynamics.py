@@ -29,8 +29,10 @@ def tip_dofs(p):
     return di[tip, 2]
 
 
-def test_newmark_matches_oracle(ctx):
-    gpu_case("newmark 20x4x4 vs oracle")
+@pytest.mark.parametrize("mode", [api.NM_MIXED, api.NM_FP64])
+def test_newmark_matches_oracle(ctx, mode):
+    # both step solvers: FP32 PCG inside FP64 refinement, and FP64 PCG
+    gpu_case(f"newmark 20x4x4 vs oracle, solver {mode}")
     p, K, M = cantilever_case()
     dt, n = 1e-4, 60
     fac = np.where(np.arange(n) < 30, np.sin(np.pi * np.arange(n) / 30.0), 0.0)
@@ -38,7 +40,11 @@ def test_newmark_matches_oracle(ctx):
     hist_o, u_o, v_o, a_o = dynamics.newmark(K, M, p.f, fac, dt, pen_dofs=p.pen_dofs, watch=watch)
     problems.upload(ctx, p, rho=np.full(p.mesh.n_el, RHO))
     ctx.newmark_setup(dt, 0.25, 0.5)
-    hist, u, v, a, st = ctx.newmark_run(n, p.f, fac, watch=watch, tol=1e-12, max_it_step=200)
+    try:
+        ctx.newmark_set_solver(mode)
+        hist, u, v, a, st = ctx.newmark_run(n, p.f, fac, watch=watch, tol=1e-12, max_it_step=400)
+    finally:
+        ctx.newmark_set_solver(api.NM_FP64)
     assert st.unconverged_steps == 0 and st.steps == n, st.as_dict()
     scale = np.abs(hist_o).max()
     assert np.abs(hist - hist_o).max() <= 1e-8 * scale

@@ -120,8 +120,9 @@ def test_shell_beam_known_answers(ctx):
     assert np.allclose(u[di[tip, 0]], 2e-4, rtol=1e-7), u[di[tip, 0]]
 
 
-def test_shell_newmark_matches_oracle(ctx):
-    gpu_case("shell4 newmark 16x16 clamped plate vs oracle")
+@pytest.mark.parametrize("mode", [api.NM_MIXED, api.NM_FP64])
+def test_shell_newmark_matches_oracle(ctx, mode):
+    gpu_case(f"shell4 newmark 16x16 clamped plate vs oracle, solver {mode}")
     p = problems.shell_plate(16, 16, 1.0, 1.0, 0.005, 70e9, 0.33, 2700.0)
     K, M = oracle_km(p)
     di = np.asarray(p.dof_index).reshape(-1, 6)
@@ -134,7 +135,11 @@ def test_shell_newmark_matches_oracle(ctx):
     hist_o, u_o, v_o, a_o = dynamics.newmark(K, M, f, fac, dt, pen_dofs=p.pen_dofs, watch=watch)
     problems.upload_shell(ctx, p)
     ctx.newmark_setup(dt, 0.25, 0.5)
-    hist, u, v, a, st = ctx.newmark_run(n, f, fac, watch=watch, tol=1e-12, max_it_step=4000)
+    try:
+        ctx.newmark_set_solver(mode)
+        hist, u, v, a, st = ctx.newmark_run(n, f, fac, watch=watch, tol=1e-12, max_it_step=4000)
+    finally:
+        ctx.newmark_set_solver(api.NM_FP64)
     assert st.unconverged_steps == 0 and st.steps == n, st.as_dict()
     scale = np.abs(hist_o[:, 2]).max()
     assert np.abs(hist[:, 2] - hist_o[:, 2]).max() <= 1e-7 * scale
