@@ -26,42 +26,53 @@ namespace asend {
 
 namespace {
 
-constexpr int kChunkRows = 2048;  // block rows per reduction chunk (never spans two variants)
+// block rows per reduction chunk (never spans two variants); 512 rows keep
+// about 4 workgroups per CU busy on a batch of 8 variants of ~100k rows
+constexpr int kChunkRows = 512;
 
 // Y[r][d][j0 + j] = (F -) sum_k K_rk X[c_k][.][j0 + j] for j < NR; vectors are
-// [row][3][kNrhs].  One wavefront per 64-row slice, lane = row; each K block is
-// loaded once and used NR times.
+// [row][3][kNrhs].  LPR wavefronts per 64-row slice: a wavefront holds
+// 64 / LPR rows, each row's NR right-hand sides split over LPR lanes (NR / LPR
+// each).  The LPR lanes of a row load the same K block addresses in one
+// instruction, so every block is fetched once per slice; splitting the right-
+// hand sides cuts the accumulators and X registers (NR = 16: 140 VGPRs at
+// LPR = 1, 78 at 2, 53 at 4), so more wavefronts stay resident per SIMD.
 // Slices of variants whose systems have all converged are skipped when
 // `active` is given.
-template <typename T, int NR, bool SUB>
+template <typename T, int NR, int LPR, bool SUB>
 __global__ __launch_bounds__(kBlock) void k_sell3_mrhs(int n_slices, const int32_t* __restrict__ sptr,
                                                        const int32_t* __restrict__ scol, const T* __restrict__ val,
                                                        const T* __restrict__ X, T* __restrict__ Y,
                                                        const T* __restrict__ F, int j0,
                                                        const int32_t* __restrict__ slice_var,
                                                        const uint8_t* __restrict__ active) {
-    const int slice = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    constexpr int RPW = kWave / LPR;  // rows per wavefront
+    constexpr int NRL = NR / LPR;     // right-hand sides per lane
+    const int gw = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    const int slice = gw / LPR;
     const int lane = threadIdx.x % kWave;
+    const int rl = (gw % LPR) * RPW + lane % RPW;  // row within the slice
+    const int jq = j0 + (lane / RPW) * NRL;        // first right-hand side of the lane
     if (slice >= n_slices) return;
     if (active && !active[slice_var[slice]]) return;
-    const int64_t r = (int64_t)slice * kWave + lane;
-    T acc[3][NR];
+    const int64_t r = (int64_t)slice * kWave + rl;
+    T acc[3][NRL];
 #pragma unroll
     for (int d = 0; d < 3; ++d)
 #pragma unroll
-        for (int j = 0; j < NR; ++j) acc[d][j] = 0;
+        for (int j = 0; j < NRL; ++j) acc[d][j] = 0;
     const int k0 = sptr[slice], k1 = sptr[slice + 1];
-    const int32_t* cp = scol + (int64_t)k0 * kWave + lane;
-    const T* vp = val + (int64_t)k0 * 9 * kWave + lane;
+    const int32_t* cp = scol + (int64_t)k0 * kWave + rl;
+    const T* vp = val + (int64_t)k0 * 9 * kWave + rl;
     for (int k = k0; k < k1; ++k) {
         // values and columns streamed once: non-temporal, keeping X in L2
         const int64_t c = __builtin_nontemporal_load(cp);
         T a[9];
 #pragma unroll
         for (int q = 0; q < 9; ++q) a[q] = __builtin_nontemporal_load(vp + q * kWave);
-        const T* xc = X + c * 3 * kNrhs + j0;
+        const T* xc = X + c * 3 * kNrhs + jq;
 #pragma unroll
-        for (int j = 0; j < NR; ++j) {
+        for (int j = 0; j < NRL; ++j) {
             const T x0 = xc[j], x1 = xc[kNrhs + j], x2 = xc[2 * kNrhs + j];
             acc[0][j] += a[0] * x0 + a[1] * x1 + a[2] * x2;
             acc[1][j] += a[3] * x0 + a[4] * x1 + a[5] * x2;
@@ -70,12 +81,12 @@ __global__ __launch_bounds__(kBlock) void k_sell3_mrhs(int n_slices, const int32
         cp += kWave;
         vp += 9 * kWave;
     }
-    T* yr = Y + r * 3 * kNrhs + j0;
-    const T* fr = F ? F + r * 3 * kNrhs + j0 : nullptr;
+    T* yr = Y + r * 3 * kNrhs + jq;
+    const T* fr = F ? F + r * 3 * kNrhs + jq : nullptr;
 #pragma unroll
     for (int d = 0; d < 3; ++d)
 #pragma unroll
-        for (int j = 0; j < NR; ++j) yr[d * kNrhs + j] = SUB ? fr[d * kNrhs + j] - acc[d][j] : acc[d][j];
+        for (int j = 0; j < NRL; ++j) yr[d * kNrhs + j] = SUB ? fr[d * kNrhs + j] - acc[d][j] : acc[d][j];
 }
 
 // Per-system scalars, index s = variant * kNrhs + j
@@ -97,6 +108,7 @@ __global__ __launch_bounds__(256) void k_chunk_dot(const int32_t* __restrict__ c
     const int seg = chunk_seg[ch];
     double t = 0.0;
     if (OP == 1 || sv[(seg * kNrhs + j) * B_N + B_DONE] == 0.0) {
+#pragma unroll 4
         for (int64_t e = e0 + threadIdx.x / kNrhs; e < e1; e += 256 / kNrhs) {
             const int64_t i = e * kNrhs + j;
             t += OP == 0 ? (double)a[i] * b[i] : (double)a[i] * a[i];
@@ -126,6 +138,7 @@ __global__ __launch_bounds__(256) void k_chunk_update(const int32_t* __restrict_
     double t0 = 0.0, t1 = 0.0;
     if (sv[s * B_N + B_DONE] == 0.0) {
         const float alpha = (float)sv[s * B_N + B_ALPHA];
+#pragma unroll 4
         for (int64_t e = e0 + threadIdx.x / kNrhs; e < e1; e += 256 / kNrhs) {
             const int64_t i = e * kNrhs + j;
             X[i] += alpha * P[i];
@@ -161,6 +174,7 @@ __global__ __launch_bounds__(256) void k_chunk_init(const int32_t* __restrict__ 
     const int j = threadIdx.x % kNrhs;
     const int64_t e0 = (int64_t)chunk_row[ch] * 3, e1 = (int64_t)chunk_row[ch + 1] * 3;
     double t0 = 0.0, t1 = 0.0;
+#pragma unroll 4
     for (int64_t e = e0 + threadIdx.x / kNrhs; e < e1; e += 256 / kNrhs) {
         const int64_t i = e * kNrhs + j;
         const float bi = B[i], zi = dinv[e] * bi;
@@ -197,30 +211,49 @@ __global__ __launch_bounds__(256) void k_chunk_dir(const int32_t* __restrict__ c
     if (sv[s * B_N + B_DONE] != 0.0) return;
     const float beta = (float)sv[s * B_N + B_BETA];
     const int64_t e0 = (int64_t)chunk_row[ch] * 3, e1 = (int64_t)chunk_row[ch + 1] * 3;
+#pragma unroll 4
     for (int64_t e = e0 + threadIdx.x / kNrhs; e < e1; e += 256 / kNrhs) {
         const int64_t i = e * kNrhs + j;
         P[i] = dinv[e] * R[i] + beta * P[i];
     }
 }
 
-// Per-system finalize over the chunks of its variant.
+// Per-system finalize over the chunks of its variant: one workgroup per
+// variant; the 256 / kNrhs lanes of each right-hand side add a strided share
+// of the chunks, then a fixed-order tree over those lanes (deterministic).
 //   OP 0 (init):   rz, rr0 = rr, it = 0, done when rr0 = 0 or the system is frozen
 //   OP 1 (p.q):    pq, alpha = rz / pq
 //   OP 2 (update): beta = rz'/rz, rz = rz', rr, it += 1, done on rr <= tol2 rr0 or it >= max
 //   OP 3 (|r|^2):  rr (FP64 residual norm) into out[s]
-__global__ void k_sys_fin(int n_sys, const int32_t* __restrict__ seg_chunk, const double* __restrict__ partials,
-                          int op, double* __restrict__ sv, const uint8_t* __restrict__ frozen,
-                          double* __restrict__ out) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n_sys) return;
-    const int seg = s / kNrhs, j = s % kNrhs;
+__global__ __launch_bounds__(256) void k_sys_fin(const int32_t* __restrict__ seg_chunk,
+                                                 const double* __restrict__ partials, int op,
+                                                 double* __restrict__ sv, const uint8_t* __restrict__ frozen,
+                                                 double* __restrict__ out) {
+    __shared__ double sh[2][256];
+    const int seg = blockIdx.x;
+    const int j = threadIdx.x % kNrhs, q = threadIdx.x / kNrhs;
+    const int s = seg * kNrhs + j;
     double* v = sv + (int64_t)s * B_N;
-    if ((op == 1 || op == 2) && v[B_DONE] != 0.0) return;
+    const bool skip = (op == 1 || op == 2) && v[B_DONE] != 0.0;
     double t0 = 0.0, t1 = 0.0;
-    for (int ch = seg_chunk[seg]; ch < seg_chunk[seg + 1]; ++ch) {
-        t0 += partials[(int64_t)ch * 2 * kNrhs + j];
-        t1 += partials[(int64_t)ch * 2 * kNrhs + kNrhs + j];
+    if (!skip)
+        for (int ch = seg_chunk[seg] + q; ch < seg_chunk[seg + 1]; ch += 256 / kNrhs) {
+            t0 += partials[(int64_t)ch * 2 * kNrhs + j];
+            t1 += partials[(int64_t)ch * 2 * kNrhs + kNrhs + j];
+        }
+    sh[0][threadIdx.x] = t0;
+    sh[1][threadIdx.x] = t1;
+    __syncthreads();
+    for (int w = 128; w >= kNrhs; w >>= 1) {
+        if (threadIdx.x < w) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
+        }
+        __syncthreads();
     }
+    if (q != 0 || skip) return;
+    t0 = sh[0][j];
+    t1 = sh[1][j];
     if (op == 0) {
         v[B_RZ] = t0;
         v[B_RR] = t1;
@@ -328,18 +361,19 @@ void bfree(T*& p) {
 void residual64(Batch* b) {
     const int nblk = (b->n_slices + kBlock / kWave - 1) / (kBlock / kWave);
     for (int j0 = 0; j0 < kNrhs; j0 += 8)
-        hipLaunchKernelGGL((k_sell3_mrhs<double, 8, true>), dim3(nblk), dim3(kBlock), 0, b->st, b->n_slices, b->sptr,
+        hipLaunchKernelGGL((k_sell3_mrhs<double, 8, 1, true>), dim3(nblk), dim3(kBlock), 0, b->st, b->n_slices, b->sptr,
                            b->scol, b->K64, b->U, b->R64, b->F, j0, (const int32_t*)nullptr,
                            (const uint8_t*)nullptr);
     hipLaunchKernelGGL((k_chunk_dot<double, 1>), dim3(b->n_chunks), dim3(256), 0, b->st, b->chunk_row, b->chunk_seg,
                        b->R64, (const double*)nullptr, b->sv, b->partials);
-    hipLaunchKernelGGL(k_sys_fin, dim3((b->n_var * kNrhs + 255) / 256), dim3(256), 0, b->st, b->n_var * kNrhs,
+    hipLaunchKernelGGL(k_sys_fin, dim3(b->n_var), dim3(256), 0, b->st,
                        b->seg_chunk, b->partials, 3, b->sv, b->frozen, b->rr);
 }
 
 void spmm32(Batch* b, const float* X, float* Y) {
-    const int nblk = (b->n_slices + kBlock / kWave - 1) / (kBlock / kWave);
-    hipLaunchKernelGGL((k_sell3_mrhs<float, kNrhs, false>), dim3(nblk), dim3(kBlock), 0, b->st, b->n_slices, b->sptr,
+    constexpr int lpr = 4;  // measured on 8 variants: 1 -> 12.7 s, 2 -> 11.4 s, 4 -> 10.8 s, 8 -> 13.5 s
+    const int nblk = (lpr * b->n_slices + kBlock / kWave - 1) / (kBlock / kWave);
+    hipLaunchKernelGGL((k_sell3_mrhs<float, kNrhs, lpr, false>), dim3(nblk), dim3(kBlock), 0, b->st, b->n_slices, b->sptr,
                        b->scol, b->K32, X, Y, (const float*)nullptr, 0, b->slice_var, b->active);
 }
 
@@ -476,7 +510,6 @@ int batch_solve(Batch* b, const double* h_F, double* h_U, int nrhs, double tol, 
     }
     const int n_sys = b->n_var * kNrhs;
     const int64_t ne = b->rows_total * 3 * kNrhs;
-    const int nsb = (n_sys + 255) / 256;
     BCHK(hipMemcpyAsync(b->F, h_F, ne * sizeof(double), hipMemcpyHostToDevice, b->st));
     BCHK(hipMemsetAsync(b->U, 0, ne * sizeof(double), b->st));
     BCHK(hipEventRecord(b->ev0, b->st));
@@ -509,7 +542,7 @@ int batch_solve(Batch* b, const double* h_F, double* h_U, int nrhs, double tol, 
                            b->B32);
         hipLaunchKernelGGL(k_chunk_init, dim3(b->n_chunks), dim3(256), 0, b->st, b->chunk_row, b->dinv, b->B32, b->X,
                            b->R, b->P, b->partials);
-        hipLaunchKernelGGL(k_sys_fin, dim3(nsb), dim3(256), 0, b->st, n_sys, b->seg_chunk, b->partials, 0, b->sv,
+        hipLaunchKernelGGL(k_sys_fin, dim3(b->n_var), dim3(256), 0, b->st, b->seg_chunk, b->partials, 0, b->sv,
                            b->frozen, (double*)nullptr);
         hipLaunchKernelGGL(k_var_active, dim3((b->n_var + 255) / 256), dim3(256), 0, b->st, b->n_var, b->sv,
                            b->active);
@@ -518,11 +551,11 @@ int batch_solve(Batch* b, const double* h_F, double* h_U, int nrhs, double tol, 
                 spmm32(b, b->P, b->Q);
                 hipLaunchKernelGGL((k_chunk_dot<float, 0>), dim3(b->n_chunks), dim3(256), 0, b->st, b->chunk_row,
                                    b->chunk_seg, b->P, b->Q, b->sv, b->partials);
-                hipLaunchKernelGGL(k_sys_fin, dim3(nsb), dim3(256), 0, b->st, n_sys, b->seg_chunk, b->partials, 1,
+                hipLaunchKernelGGL(k_sys_fin, dim3(b->n_var), dim3(256), 0, b->st, b->seg_chunk, b->partials, 1,
                                    b->sv, b->frozen, (double*)nullptr);
                 hipLaunchKernelGGL(k_chunk_update, dim3(b->n_chunks), dim3(256), 0, b->st, b->chunk_row, b->chunk_seg,
                                    b->dinv, b->X, b->R, b->P, b->Q, b->sv, b->partials);
-                hipLaunchKernelGGL(k_sys_fin, dim3(nsb), dim3(256), 0, b->st, n_sys, b->seg_chunk, b->partials, 2,
+                hipLaunchKernelGGL(k_sys_fin, dim3(b->n_var), dim3(256), 0, b->st, b->seg_chunk, b->partials, 2,
                                    b->sv, b->frozen, (double*)nullptr);
                 hipLaunchKernelGGL(k_var_active, dim3((b->n_var + 255) / 256), dim3(256), 0, b->st, b->n_var, b->sv,
                                    b->active);
