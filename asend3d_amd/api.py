@@ -23,7 +23,8 @@ SHELL4 = 41
 SOLVE_PCG_FP64 = 0
 SOLVE_IR_MIXED = 1
 NM_FP64 = 0   # Newmark step solver: FP64 single-reduction Jacobi PCG (default)
-NM_MIXED = 1  # FP32 single-reduction Jacobi PCG inside FP64 refinement
+NM_MIXED = 1  # FP32 single-reduction Jacobi PCG, FP64 increment, reliable updates
+NM_AUTO = 2   # mixed when the FP64 operator exceeds 128 MiB, else FP64 (default)
 
 PC_JACOBI = 0
 PC_BLOCK_JACOBI = 1
@@ -48,7 +49,7 @@ class SolveOpts(ctypes.Structure):
 class NewmarkStats(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int64), ("total_iterations", ctypes.c_int64),
                 ("max_step_iterations", ctypes.c_int64), ("unconverged_steps", ctypes.c_int64),
-                ("graph_used", ctypes.c_int), ("device_ms", ctypes.c_double)]
+                ("graph_used", ctypes.c_int), ("device_ms", ctypes.c_double), ("step_solver", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -373,9 +374,10 @@ class Context:
         self._chk(_lib.asend_mg_coarse_solve(self.h, _p(b), _p(x)), "asend_mg_coarse_solve")
         return x
 
-    def newmark_set_solver(self, mode=NM_FP64, inner_tol=1e-4):
-        """Step solver of newmark_run (NM_MIXED or NM_FP64)."""
-        self._chk(_lib.asend_newmark_set_solver(self.h, ctypes.c_int(mode), _dbl(inner_tol)),
+    def newmark_set_solver(self, mode=NM_AUTO, ru_factor=1e-3):
+        """Step solver of newmark_run (NM_MIXED or NM_FP64); ru_factor: the
+        residual reduction between FP64 reliable updates of the mixed solver."""
+        self._chk(_lib.asend_newmark_set_solver(self.h, ctypes.c_int(mode), _dbl(ru_factor)),
                   "asend_newmark_set_solver")
 
     def set_rayleigh_damping(self, cm=0.0, ck=0.0):

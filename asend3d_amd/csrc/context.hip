@@ -47,6 +47,7 @@ struct asend_ctx {
         A.sptr = d_sptr;
         A.scol = d_scol;
         A.waves_per_block = spmv_wpb;
+        A.n_slots = n_slots;
         return A;
     }
     int64_t nval() const { return 9 * 64 * n_slots; }
@@ -101,8 +102,8 @@ struct asend_ctx {
     // Rayleigh damping D = ray_cm M + ray_ck K of the next Newmark setup
     double ray_cm = 0.0, ray_ck = 0.0;
     // Newmark step solver (asend_newmark_set_solver)
-    int nm_mode = ASEND_NM_FP64;
-    double nm_inner_tol = 1e-4;
+    int nm_mode = ASEND_NM_AUTO;
+    double nm_inner_tol = 1e-3;  // reliable-update factor of the mixed step solver
     bool use_graphs = true;
     // autotuned choices: SpMV workgroup geometry, inner tolerance and check
     // interval used when a solve passes 0 for them
@@ -142,11 +143,12 @@ struct asend_ctx {
         int64_t yhist_cap = 0;
         const double* graph_save_y = nullptr;
         bool damped() const { return cm != 0.0 || ck != 0.0; }
-        // step solver: mixed (FP32 single-reduction PCG inside FP64
-        // refinement) or FP64 single-reduction PCG (the default); the FP32
-        // work vectors of the mixed solver (7 n floats)
-        int mixed = 0;
-        double inner_tol = 1e-4;
+        // step solver (ASEND_NM_*): FP32 single-reduction PCG with an FP64
+        // increment and reliable updates, FP64 single-reduction PCG, or by
+        // operator size; the FP32 work vectors of the mixed solver (5 n
+        // floats)
+        int mixed = ASEND_NM_AUTO;
+        double inner_tol = 1e-3;  // reliable-update factor
         float* f32 = nullptr;
         int graph_mixed = -1;
     } nm;
@@ -405,6 +407,9 @@ int residual64(asend_ctx* ctx, const double* f, const double* u, double* r) {
 // to Jacobi PCG.
 constexpr int64_t kLearnedInnerBudget = 300;
 constexpr int kNewmarkBlock = 24;  // PCG iterations per captured block of a Newmark step
+constexpr int kMixedRefreshEvery = 4;  // mixed step solver: iterations between reliable-update slots
+// ASEND_NM_AUTO takes the mixed step solver above this FP64 operator size
+constexpr int64_t kMixedAutoBytes = 128ll << 20;
 constexpr int64_t kLearnedPcgBudget = 20000;
 
 LearnedGraph learned_graph(const asend_ctx* ctx) {
@@ -1389,13 +1394,13 @@ int asend_set_rayleigh_damping(asend_ctx* ctx, double cm, double ck) {
     return ASEND_OK;
 }
 
-int asend_newmark_set_solver(asend_ctx* ctx, int mode, double inner_tol) {
-    if (!ctx || (mode != ASEND_NM_MIXED && mode != ASEND_NM_FP64) ||
-        (mode == ASEND_NM_MIXED && !(inner_tol > 0.0 && inner_tol < 1.0)))
+int asend_newmark_set_solver(asend_ctx* ctx, int mode, double ru_factor) {
+    if (!ctx || (mode != ASEND_NM_MIXED && mode != ASEND_NM_FP64 && mode != ASEND_NM_AUTO) ||
+        (mode != ASEND_NM_FP64 && !(ru_factor >= 1e-6 && ru_factor < 1.0)))
         return ASEND_ERR_ARG;
     ctx->nm_mode = mode;
-    if (mode == ASEND_NM_MIXED) ctx->nm_inner_tol = inner_tol;
-    ctx->nm.mixed = mode == ASEND_NM_MIXED ? 1 : 0;
+    if (mode != ASEND_NM_FP64) ctx->nm_inner_tol = ru_factor;
+    ctx->nm.mixed = mode;
     ctx->nm.inner_tol = ctx->nm_inner_tol;
     return ASEND_OK;
 }
@@ -1407,7 +1412,7 @@ int asend_newmark_setup(asend_ctx* ctx, double dt, double beta, double gamma) {
     HIPCHK(hipSetDevice(ctx->device));
     newmark_free(ctx);
     auto& nm = ctx->nm;
-    nm.mixed = ctx->nm_mode == ASEND_NM_MIXED ? 1 : 0;
+    nm.mixed = ctx->nm_mode;
     nm.inner_tol = ctx->nm_inner_tol;
     nm.dt = dt;
     nm.beta = beta;
@@ -1428,7 +1433,7 @@ int asend_newmark_setup(asend_ctx* ctx, double dt, double beta, double gamma) {
     if ((rc = dalloc(ctx, &nm.M64, nval)) || (rc = dalloc(ctx, &nm.v, n)) || (rc = dalloc(ctx, &nm.a, n)) ||
         (rc = dalloc(ctx, &nm.w, n)) || (rc = dalloc(ctx, &nm.q, n)) || (rc = dalloc(ctx, &nm.rhs0, n)) ||
         (rc = dalloc(ctx, &nm.rhs, n)) || (rc = dalloc(ctx, &nm.fshape, n)) || (rc = dalloc(ctx, &nm.step, 1)) ||
-        (rc = dalloc(ctx, &nm.cg_u, n)) || (rc = dalloc(ctx, &nm.cg_s, n)) || (rc = dalloc(ctx, &nm.f32, 7 * n)))
+        (rc = dalloc(ctx, &nm.cg_u, n)) || (rc = dalloc(ctx, &nm.cg_s, n)) || (rc = dalloc(ctx, &nm.f32, 5 * n)))
         return rc;
     // consistent mass alone: the element kernels with c_mass = 1 and no
     // stiffness (zero moduli for the brick8 kernels, k_scale = 0 for the
@@ -1519,37 +1524,27 @@ int asend_newmark_run(asend_ctx* ctx, int64_t n_steps, const double* f_shape, co
         h.tol2 = tol * tol;
         h.max_it = max_it_step;
         if ((rc = write_scalars(ctx, 1))) return rc;
-        // inner FP32 solve of the mixed step solver, relative to its
-        // normalised right-hand side
-        Scalars& hi = ctx->h_sc[3];
-        std::memset(&hi, 0, sizeof(Scalars));
-        hi.tol = nm.inner_tol;
-        hi.tol2 = nm.inner_tol * nm.inner_tol;
-        hi.max_it = max_it_step;
-        hi.done = 1;
-        if ((rc = write_scalars(ctx, 3))) return rc;
     }
     Scalars* sc = ctx->d_sc + 1;
     const Sell3 A = ctx->fine();
     MixedSolve mx;
     mx.A = &A;
     mx.K64 = ctx->d_K64;
+    mx.K32 = ctx->d_K32;
     mx.dinv32 = ctx->d_dinv32;
     mx.rhs = nm.rhs;
     mx.x64 = ctx->d_x64;
     mx.r64 = ctx->d_r2;
     float* f = nm.f32;
-    mx.b32 = f;
-    mx.d32 = f + n;
-    mx.r32 = f + 2 * n;
-    mx.u32 = f + 3 * n;
-    mx.p32 = f + 4 * n;
-    mx.s32 = f + 5 * n;
-    float* w32 = f + 6 * n;
+    mx.r32 = f;
+    mx.u32 = f + n;
+    mx.w32 = f + 2 * n;
+    mx.p32 = f + 3 * n;
+    mx.s32 = f + 4 * n;
     mx.partials = ctx->d_partials;
-    mx.so = sc;
-    mx.si = ctx->d_sc + 3;
+    mx.sc = sc;
     mx.n = n;
+    mx.ru_factor = nm.inner_tol;
     const int nw = (int)n_watch;
     // A step is three launch sequences: head (effective load, PCG start and a
     // first block of iterations), more (another block, repeated until the
@@ -1557,23 +1552,21 @@ int asend_newmark_run(asend_ctx* ctx, int64_t n_steps, const double* f_shape, co
     // tail (state update and history).  Each is captured once as a hipGraph.
     const int blk = std::min(max_it_step, kNewmarkBlock);
     // single-reduction Jacobi PCG: x = du, r, u = Dinv r, w = K u, p, s = K p;
-    // mixed: the same recurrence in FP32 on the normalised refinement
-    // residual, then a refinement transition (a no-op until the inner solve
-    // has finished)
-    const bool mixed = nm.mixed != 0;
+    // mixed: the same recurrence in FP32 with x in FP64, and a reliable-update
+    // slot after every kMixedRefreshEvery iterations (a no-op unless a refresh
+    // is pending; the iterations pause while one is)
+    const bool mixed = nm.mixed == ASEND_NM_MIXED || (nm.mixed == ASEND_NM_AUTO && 8 * ctx->nval() > kMixedAutoBytes);
     auto iterations = [&]() {
         for (int k = 0; k < blk; ++k) {
             if (mixed) {
-                launch_sell3<float>(ctx->st, 4, A, ctx->d_K32, mx.u32, w32, mx.r32, ctx->d_partials, mx.si);
-                launch_cg1_update<float>(ctx->st, n, ctx->d_dinv32, mx.d32, mx.r32, mx.u32, w32, mx.p32, mx.s32,
-                                         mx.si);
+                launch_mixed_iteration(ctx->st, mx);
+                if ((k + 1) % kMixedRefreshEvery == 0 || k + 1 == blk) launch_mixed_refresh(ctx->st, mx);
             } else {
                 launch_sell3<double>(ctx->st, 4, A, ctx->d_K64, nm.cg_u, ctx->d_q, ctx->d_r2, ctx->d_partials, sc);
                 launch_cg1_update<double>(ctx->st, n, ctx->d_dinv64, ctx->d_x64, ctx->d_r2, nm.cg_u, ctx->d_q,
                                           ctx->d_p, nm.cg_s, sc);
             }
         }
-        if (mixed) launch_mixed_transition(ctx->st, mx);
     };
     auto head = [&]() {
         launch_nm_predict(ctx->st, n, ctx->d_u, nm.v, nm.a, nm.dt, nm.c2, nm.w);
@@ -1625,7 +1618,7 @@ int asend_newmark_run(asend_ctx* ctx, int64_t n_steps, const double* f_shape, co
     };
     const bool graph = ctx->use_graphs;
     if (graph && (!nm.exec_head || nm.graph_iters != blk || nm.graph_watch != nw || nm.graph_save != nm.save_u ||
-                  nm.graph_save_y != nm.save_y || nm.graph_mixed != nm.mixed)) {
+                  nm.graph_save_y != nm.save_y || nm.graph_mixed != (int)mixed)) {
         if ((rc = capture(&nm.exec_head, head)) || (rc = capture(&nm.exec_more, iterations)) ||
             (rc = capture(&nm.exec_tail, tail)))
             return rc;
@@ -1633,7 +1626,7 @@ int asend_newmark_run(asend_ctx* ctx, int64_t n_steps, const double* f_shape, co
         nm.graph_watch = nw;
         nm.graph_save = nm.save_u;
         nm.graph_save_y = nm.save_y;
-        nm.graph_mixed = nm.mixed;
+        nm.graph_mixed = (int)mixed;
     }
     HIPCHK(hipEventRecord(ctx->ev0, ctx->st));
     for (int64_t s = 0; s < n_steps; ++s) {
@@ -1671,6 +1664,7 @@ int asend_newmark_run(asend_ctx* ctx, int64_t n_steps, const double* f_shape, co
         }
         st.graph_used = graph ? 1 : 0;
         st.device_ms = ms;
+        st.step_solver = mixed ? ASEND_NM_MIXED : ASEND_NM_FP64;
         *stats = st;
     }
     return ASEND_OK;

@@ -347,13 +347,15 @@ enum FinOp {
     FIN_INIT_WARM = 6,
     FIN_CG1_INIT = 7,
     FIN_CG1 = 8,
-    FIN_MX_RES = 9
+    FIN_RU_RES = 9,
+    FIN_CG1_RU = 10
 };
 
 __global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ partials, int nparts, int op,
                                                      Scalars* sc) {
     if ((op == FIN_PQ || op == FIN_UPDATE || op == FIN_CG1) && sc->done) return;
-    if (op == FIN_MX_RES && !sc->gate) return;
+    if (op == FIN_RU_RES && !sc->gate) return;
+    if (op == FIN_CG1_RU && (sc->done || sc->gate)) return;
     if (op == FIN_MAX) {
         double m = 0.0;
         for (int b = threadIdx.x; b < nparts; b += blockDim.x) m = fmax(m, partials[b]);
@@ -368,7 +370,8 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ 
         }
         return;
     }
-    const int nv = (op == FIN_INIT || op == FIN_UPDATE) ? 2 : ((op == FIN_INIT_WARM || op == FIN_CG1) ? 3 : 1);
+    const int nv =
+        (op == FIN_INIT || op == FIN_UPDATE) ? 2 : ((op == FIN_INIT_WARM || op == FIN_CG1 || op == FIN_CG1_RU) ? 3 : 1);
     double t[3] = {0.0, 0.0, 0.0};
     for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
         t[0] += partials[(size_t)nv * b];
@@ -432,15 +435,40 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ 
             sc->it += 1;
             break;
         }
-        case FIN_MX_RES:
-            // refinement residual: converged, or a cap reached, ends the solve
+        case FIN_RU_RES:
+            // reliable update: the FP64 residual decides convergence
             sc->rr = t[0];
             sc->n_resid += 1;
-            if (t[0] <= sc->tol2 * sc->ff || sc->n_resid >= kMixedMaxResiduals || sc->it >= sc->max_it) {
+            if (t[0] <= sc->tol2 * sc->ff || sc->n_resid >= kMixedMaxRefresh || sc->it >= sc->max_it) {
                 sc->done = 1;
                 sc->gate = 0;
             }
             break;
+        case FIN_CG1_RU: {
+            // FP32 single-reduction step; sc->rr is the FP64 residual of the
+            // last refresh, sc->zq the refresh factor squared
+            const double g = t[0], d = t[1];
+            sc->pq = t[2];  // recursive residual, FP32 recurrence
+            if (t[2] <= sc->zq * sc->rr || t[2] <= sc->tol2 * sc->ff) {
+                sc->gate = 1;
+                break;
+            }
+            if (sc->it >= sc->max_it) {
+                sc->done = 1;
+                break;
+            }
+            const double beta = sc->it == 0 ? 0.0 : g / sc->rz;
+            const double den = sc->it == 0 ? d : d - beta * g / sc->alpha;
+            if (!(den > 0.0)) {
+                sc->done = 1;
+                break;
+            }
+            sc->beta = beta;
+            sc->alpha = g / den;
+            sc->rz = g;
+            sc->it += 1;
+            break;
+        }
         case FIN_UPDATE: {
             sc->beta = t[0] / sc->rz;
             sc->rz = t[0];
@@ -529,13 +557,20 @@ void launch_diag_inv(hipStream_t st, int n_rows, const int32_t* sptr, const int3
 // load of the wavefront is one contiguous 256 B (FP32) / 512 B (FP64) burst
 // and no cross-lane reduction is needed.  Modes as the block-row SpMV.
 // ---------------------------------------------------------------------------
-template <typename T, int MODE>
+template <bool NT, typename T>
+__device__ __forceinline__ T sell_load(const T* p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
+template <typename T, int MODE, bool NT>
 __global__ __launch_bounds__(kSellMaxWaves * kWave) void k_sell3(int n_rows, int n_slices,
                                                                  const int32_t* __restrict__ sptr,
                                                   const int32_t* __restrict__ scol, const T* __restrict__ val,
                                                   const T* __restrict__ x, T* __restrict__ y,
                                                   const T* __restrict__ f, double* partials, const Scalars* sc) {
     if ((MODE == 1 || MODE == 4) && sc->done) return;
+    if (MODE == 6 && (sc->done || sc->gate)) return;
     if (MODE == 5 && !sc->gate) return;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int slice = lb * (int)(blockDim.x / kWave) + threadIdx.x / kWave;
@@ -549,13 +584,14 @@ __global__ __launch_bounds__(kSellMaxWaves * kWave) void k_sell3(int n_rows, int
         T a0 = 0, a1 = 0, a2 = 0;
 #pragma unroll 2
         for (int k = k0; k < k1; ++k) {
-            // values and columns are streamed once: non-temporal loads keep
-            // them from evicting the gathered x from L2
-            const int c = __builtin_nontemporal_load(cp);
+            // values and columns of operators larger than the Infinity Cache
+            // are streamed once: non-temporal loads keep them from evicting
+            // the gathered x from L2
+            const int c = sell_load<NT>(cp);
             const T x0 = x[3 * (int64_t)c], x1 = x[3 * (int64_t)c + 1], x2 = x[3 * (int64_t)c + 2];
             T v[9];
 #pragma unroll
-            for (int q = 0; q < 9; ++q) v[q] = __builtin_nontemporal_load(vp + q * kWave);
+            for (int q = 0; q < 9; ++q) v[q] = sell_load<NT>(vp + q * kWave);
             a0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
             a1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
             a2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
@@ -569,7 +605,7 @@ __global__ __launch_bounds__(kSellMaxWaves * kWave) void k_sell3(int n_rows, int
                 y[3 * (int64_t)r + 1] = y1;
                 y[3 * (int64_t)r + 2] = y2;
                 part = (double)y0 * y0 + (double)y1 * y1 + (double)y2 * y2;
-            } else if (MODE == 4) {
+            } else if (MODE == 4 || MODE == 6) {
                 // x = u (preconditioned residual), f = r: partials r.u, (K u).u, r.r
                 y[3 * (int64_t)r] = a0;
                 y[3 * (int64_t)r + 1] = a1;
@@ -590,7 +626,7 @@ __global__ __launch_bounds__(kSellMaxWaves * kWave) void k_sell3(int n_rows, int
         }
     }
     if (MODE == 0 || MODE == 3) return;
-    if (MODE == 4) {
+    if (MODE == 4 || MODE == 6) {
         double t[3] = {part, part1, part2};
         block_partials<3>(t, partials);
         return;
@@ -599,40 +635,50 @@ __global__ __launch_bounds__(kSellMaxWaves * kWave) void k_sell3(int n_rows, int
     block_partials<1>(t, partials);
 }
 
+template <typename T, int MODE>
+static void sell3_go(hipStream_t st, bool nt, int nblk, int nthr, const Sell3& A, const T* val, const T* x, T* y,
+                     const T* f, double* partials, Scalars* sc) {
+    if (nt)
+        hipLaunchKernelGGL((k_sell3<T, MODE, true>), dim3(nblk), dim3(nthr), 0, st, A.n_rows, A.n_slices, A.sptr,
+                           A.scol, val, x, y, f, partials, sc);
+    else
+        hipLaunchKernelGGL((k_sell3<T, MODE, false>), dim3(nblk), dim3(nthr), 0, st, A.n_rows, A.n_slices, A.sptr,
+                           A.scol, val, x, y, f, partials, sc);
+}
+
 template <typename T>
 void launch_sell3(hipStream_t st, int mode, const Sell3& A, const T* val, const T* x, T* y, const T* f,
                   double* partials, Scalars* sc) {
     const int per = A.waves_per_block;
     const int nblk = (A.n_slices + per - 1) / per;
     const int nthr = per * kWave;
+    const bool nt = (int64_t)sizeof(T) * 9 * kWave * A.n_slots > kSellStreamBytes;
     switch (mode) {
         case 0:
-            hipLaunchKernelGGL((k_sell3<T, 0>), dim3(nblk), dim3(nthr), 0, st, A.n_rows, A.n_slices, A.sptr, A.scol,
-                               val, x, y, f, partials, sc);
+            sell3_go<T, 0>(st, nt, nblk, nthr, A, val, x, y, f, partials, sc);
             break;
         case 1:
-            hipLaunchKernelGGL((k_sell3<T, 1>), dim3(nblk), dim3(nthr), 0, st, A.n_rows, A.n_slices, A.sptr, A.scol,
-                               val, x, y, f, partials, sc);
+            sell3_go<T, 1>(st, nt, nblk, nthr, A, val, x, y, f, partials, sc);
             finalize(st, partials, nblk, FIN_PQ, sc);
             break;
         case 2:
-            hipLaunchKernelGGL((k_sell3<T, 2>), dim3(nblk), dim3(nthr), 0, st, A.n_rows, A.n_slices, A.sptr, A.scol,
-                               val, x, y, f, partials, sc);
+            sell3_go<T, 2>(st, nt, nblk, nthr, A, val, x, y, f, partials, sc);
             finalize(st, partials, nblk, FIN_RR, sc);
             break;
         case 4:
-            hipLaunchKernelGGL((k_sell3<T, 4>), dim3(nblk), dim3(nthr), 0, st, A.n_rows, A.n_slices, A.sptr, A.scol,
-                               val, x, y, f, partials, sc);
+            sell3_go<T, 4>(st, nt, nblk, nthr, A, val, x, y, f, partials, sc);
             finalize(st, partials, nblk, FIN_CG1, sc);
             break;
         case 5:
-            hipLaunchKernelGGL((k_sell3<T, 5>), dim3(nblk), dim3(nthr), 0, st, A.n_rows, A.n_slices, A.sptr, A.scol,
-                               val, x, y, f, partials, sc);
-            finalize(st, partials, nblk, FIN_MX_RES, sc);
+            sell3_go<T, 5>(st, nt, nblk, nthr, A, val, x, y, f, partials, sc);
+            finalize(st, partials, nblk, FIN_RU_RES, sc);
+            break;
+        case 6:
+            sell3_go<T, 6>(st, nt, nblk, nthr, A, val, x, y, f, partials, sc);
+            finalize(st, partials, nblk, FIN_CG1_RU, sc);
             break;
         default:
-            hipLaunchKernelGGL((k_sell3<T, 3>), dim3(nblk), dim3(nthr), 0, st, A.n_rows, A.n_slices, A.sptr, A.scol,
-                               val, x, y, f, partials, sc);
+            sell3_go<T, 3>(st, nt, nblk, nthr, A, val, x, y, f, partials, sc);
             break;
     }
 }
@@ -651,7 +697,9 @@ template void launch_sell3<float>(hipStream_t, int, const Sell3&, const float*, 
 // MODE 4 (sliced-ELL only): y = K x with x = u = Dinv r and f = r, partials of
 //   r.u, y.u and r.r -> finalize: the single-reduction PCG step scalars
 // MODE 5 (sliced-ELL only): MODE 2 when sc->gate is set (mixed solve's
-//   refinement residual) -> finalize: sc->rr, convergence of the refinement
+//   reliable update) -> finalize: sc->rr, convergence on the FP64 residual
+// MODE 6 (sliced-ELL only): MODE 4 paused while sc->gate is set -> finalize:
+//   the mixed solve's FP32 step scalars, or a refresh request
 // Kernels with a scalars pointer return at once when sc->done is set.
 // ---------------------------------------------------------------------------
 template <typename T, int G, int MODE>
@@ -866,71 +914,77 @@ void launch_cg1_init(hipStream_t st, int64_t n, const T* b, bool cold, const T* 
     finalize(st, partials, kVecBlocks, FIN_CG1_INIT, sc);
 }
 
-// --- mixed-precision solve under device-side control -----------------------
-// so->gate is the transition's condition; every transition kernel checks it.
-__global__ void k_mx_start(Scalars* __restrict__ so, Scalars* __restrict__ si) {
-    so->ff = so->dot;
-    so->rr = so->dot;
-    so->it = 0;
-    so->n_resid = 0;
-    so->done = so->dot == 0.0;
-    so->gate = !so->done;
-    si->done = 1;
+// --- mixed-precision solve under device-side control ----------------------
+__global__ void k_ru_start(Scalars* __restrict__ sc, double ru2) {
+    sc->ff = sc->dot;
+    sc->rr = sc->dot;
+    sc->zq = ru2;
+    sc->it = 0;
+    sc->n_resid = 0;
+    sc->rz = 0.0;
+    sc->done = sc->dot == 0.0;
+    sc->gate = !sc->done;
 }
-__global__ void k_mx_gate(Scalars* __restrict__ so, const Scalars* __restrict__ si) {
-    const int g = si->done && !so->done;
-    so->gate = g;
-    if (g) so->it += si->it;
+// FP32 step with the solution in FP64: p = u + beta p ; s = w + beta s ;
+// x += alpha p ; r -= alpha s ; u = Dinv r
+__global__ __launch_bounds__(kBlock) void k_cg1_update_mx(int64_t n, const float* __restrict__ dinv,
+                                                          double* __restrict__ x, float* __restrict__ r,
+                                                          float* __restrict__ u, const float* __restrict__ w,
+                                                          float* __restrict__ p, float* __restrict__ s,
+                                                          const Scalars* __restrict__ sc) {
+    if (sc->done || sc->gate) return;
+    const float alpha = (float)sc->alpha, beta = (float)sc->beta;
+    const double alpha64 = sc->alpha;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float pi = u[i] + beta * p[i];
+        const float si = w[i] + beta * s[i];
+        p[i] = pi;
+        s[i] = si;
+        x[i] += alpha64 * (double)pi;
+        const float ri = r[i] - alpha * si;
+        r[i] = ri;
+        u[i] = dinv[i] * ri;
+    }
 }
-// x += |r| d
-__global__ void k_mx_accept(int64_t n, double* __restrict__ x, const float* __restrict__ d,
-                            const Scalars* __restrict__ so) {
-    if (!so->gate) return;
-    const double s = sqrt(so->rr);
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        x[i] += s * (double)d[i];
+// reliable update: r = (float) r64, u = Dinv r (and p = s = 0 at the start)
+__global__ __launch_bounds__(kBlock) void k_ru_load(int64_t n, const double* __restrict__ r64,
+                                                    const float* __restrict__ dinv, float* __restrict__ r,
+                                                    float* __restrict__ u, float* __restrict__ p,
+                                                    float* __restrict__ s, int restart,
+                                                    const Scalars* __restrict__ sc) {
+    if (!sc->gate) return;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float ri = (float)r64[i];
+        r[i] = ri;
+        u[i] = dinv[i] * ri;
+        if (restart) {
+            p[i] = 0.f;
+            s[i] = 0.f;
+        }
+    }
 }
-// b = r / |r| in FP32
-__global__ void k_mx_scale(int64_t n, const double* __restrict__ r, float* __restrict__ b,
-                           const Scalars* __restrict__ so) {
-    if (!so->gate) return;
-    const double s = 1.0 / sqrt(so->rr);
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        b[i] = (float)(r[i] * s);
-}
-// inner start's scalars: |b|^2, iteration count, done
-__global__ __launch_bounds__(kBlock) void k_mx_fin_init(const double* __restrict__ partials, int nparts,
-                                                        const Scalars* __restrict__ so, Scalars* __restrict__ si) {
-    if (!so->gate) return;
-    double t[1] = {0.0};
-    for (int b = threadIdx.x; b < nparts; b += blockDim.x) t[0] += partials[b];
-    block_sum<1>(t);
-    if (threadIdx.x != 0) return;
-    si->ff = t[0];
-    si->rz = 0.0;
-    si->it = 0;
-    si->done = t[0] == 0.0;
-}
+__global__ void k_ru_clear(Scalars* __restrict__ sc) { sc->gate = 0; }
 
-static void mixed_residual_and_restart(hipStream_t st, const MixedSolve& m) {
-    launch_sell3<double>(st, 5, *m.A, m.K64, m.x64, m.r64, m.rhs, m.partials, m.so);
-    hipLaunchKernelGGL(k_mx_scale, dim3(kVecBlocks), dim3(kBlock), 0, st, m.n, m.r64, m.b32, m.so);
-    hipLaunchKernelGGL(k_cg1_init<float>, dim3(kVecBlocks), dim3(kBlock), 0, st, m.n, m.b32, 1, m.dinv32, m.d32,
-                       m.r32, m.u32, m.p32, m.s32, m.partials, (const Scalars*)m.so);
-    hipLaunchKernelGGL(k_mx_fin_init, dim3(1), dim3(kBlock), 0, st, m.partials, kVecBlocks, m.so, m.si);
+static void mixed_refresh(hipStream_t st, const MixedSolve& m, int restart) {
+    launch_sell3<double>(st, 5, *m.A, m.K64, m.x64, m.r64, m.rhs, m.partials, m.sc);
+    hipLaunchKernelGGL(k_ru_load, dim3(kVecBlocks), dim3(kBlock), 0, st, m.n, m.r64, m.dinv32, m.r32, m.u32, m.p32,
+                       m.s32, restart, (const Scalars*)m.sc);
+    hipLaunchKernelGGL(k_ru_clear, dim3(1), dim3(1), 0, st, m.sc);
 }
 
 void launch_mixed_start(hipStream_t st, const MixedSolve& m) {
-    launch_dot64(st, m.n, m.rhs, m.rhs, m.partials, m.so);
-    hipLaunchKernelGGL(k_mx_start, dim3(1), dim3(1), 0, st, m.so, m.si);
-    mixed_residual_and_restart(st, m);
+    launch_dot64(st, m.n, m.rhs, m.rhs, m.partials, m.sc);
+    hipLaunchKernelGGL(k_ru_start, dim3(1), dim3(1), 0, st, m.sc, m.ru_factor * m.ru_factor);
+    mixed_refresh(st, m, 1);
 }
 
-void launch_mixed_transition(hipStream_t st, const MixedSolve& m) {
-    hipLaunchKernelGGL(k_mx_gate, dim3(1), dim3(1), 0, st, m.so, m.si);
-    hipLaunchKernelGGL(k_mx_accept, dim3(kVecBlocks), dim3(kBlock), 0, st, m.n, m.x64, m.d32, m.so);
-    mixed_residual_and_restart(st, m);
+void launch_mixed_iteration(hipStream_t st, const MixedSolve& m) {
+    launch_sell3<float>(st, 6, *m.A, m.K32, m.u32, m.w32, m.r32, m.partials, m.sc);
+    hipLaunchKernelGGL(k_cg1_update_mx, dim3(kVecBlocks), dim3(kBlock), 0, st, m.n, m.dinv32, m.x64, m.r32, m.u32,
+                       m.w32, m.p32, m.s32, (const Scalars*)m.sc);
 }
+
+void launch_mixed_refresh(hipStream_t st, const MixedSolve& m) { mixed_refresh(st, m, 0); }
 template <typename T>
 void launch_cg1_update(hipStream_t st, int64_t n, const T* dinv, T* x, T* r, T* u, const T* w, T* p, T* s,
                        const Scalars* sc) {

@@ -24,7 +24,13 @@ struct Sell3 {
     const int32_t* sptr = nullptr;  // slot offset of each slice (n_slices + 1)
     const int32_t* scol = nullptr;  // column (node row) of slot k, lane l at 64k + l
     int waves_per_block = 4;        // slices (wavefronts) per workgroup of the SpMV: 1, 2, 4 or 8
+    int64_t n_slots = 0;            // slots over all slices (sptr[n_slices])
 };
+// Operators whose values exceed this are streamed with non-temporal loads
+// (they cannot stay in the 256 MB Infinity Cache, and the loads then keep the
+// gathered x in L2); smaller ones are read with plain loads so that they stay
+// cache-resident between SpMVs.
+constexpr int64_t kSellStreamBytes = 192ll << 20;
 constexpr int kSellMaxWaves = 8;
 
 // Index of component c (= 3i + j) of block k of row r in the value array.
@@ -107,29 +113,35 @@ void launch_cg1_update(hipStream_t st, int64_t n, const T* dinv, T* x, T* r, T* 
                        const Scalars* sc);
 
 // Mixed-precision solve of K x = rhs under device-side control, for launch
-// sequences captured once and replayed: FP64 iterative refinement around a
-// single-reduction FP32 Jacobi PCG (launch_sell3<float> mode 4 +
-// launch_cg1_update<float> on si, stopping at |r| <= si->tol |b|, at most
-// si->max_it iterations per start).  so holds the refinement: ff = |rhs|^2,
-// rr = |rhs - K x|^2 of the latest FP64 residual, it = inner iterations so
-// far (cap so->max_it), done when rr <= so->tol2 ff or a cap is reached.
+// sequences captured once and replayed: single-reduction Jacobi PCG in FP32
+// (K32, Dinv32; r, u = Dinv r, w = K u, p, s = K p) with the solution x
+// accumulated in FP64 and reliable updates: when the recursive residual has
+// fallen by ru_factor since the last refresh, or below the tolerance, a
+// refresh slot recomputes r = rhs - K64 x in FP64 and reloads r and u in FP32,
+// keeping the search directions.  Convergence is decided on the FP64
+// residual only: sc->rr = |rhs - K64 x|^2 of the last refresh, sc->ff =
+// |rhs|^2, done when rr <= tol2 ff, or when it >= max_it, or after
+// kMixedMaxRefresh refreshes.  sc->gate is set while a refresh is pending
+// (the iteration kernels pause).
 struct MixedSolve {
     const Sell3* A = nullptr;
     const double* K64 = nullptr;
-    const float* dinv32 = nullptr;
+    const float *K32 = nullptr, *dinv32 = nullptr;
     const double* rhs = nullptr;
     double *x64 = nullptr, *r64 = nullptr;
-    float *b32 = nullptr, *d32 = nullptr, *r32 = nullptr, *u32 = nullptr, *p32 = nullptr, *s32 = nullptr;
+    float *r32 = nullptr, *u32 = nullptr, *w32 = nullptr, *p32 = nullptr, *s32 = nullptr;
     double* partials = nullptr;
-    Scalars *so = nullptr, *si = nullptr;
+    Scalars* sc = nullptr;
     int64_t n = 0;
+    double ru_factor = 0.01;
 };
-constexpr int kMixedMaxResiduals = 40;
-// start from the x64 passed in: |rhs|^2, first residual, first inner start
+constexpr int kMixedMaxRefresh = 200;
+// start from the x64 passed in: |rhs|^2, first FP64 residual, FP32 start
 void launch_mixed_start(hipStream_t st, const MixedSolve& m);
-// refinement transition, a no-op unless the inner solve has finished and the
-// outer one has not: x += |r| d, new FP64 residual, next inner start
-void launch_mixed_transition(hipStream_t st, const MixedSolve& m);
+// one FP32 iteration (a no-op once done or while a refresh is pending)
+void launch_mixed_iteration(hipStream_t st, const MixedSolve& m);
+// refresh slot (a no-op unless a refresh is pending)
+void launch_mixed_refresh(hipStream_t st, const MixedSolve& m);
 
 void launch_scale_to_f32(hipStream_t st, int64_t n, const double* r, float* r32, const Scalars* sc);
 void launch_axpy_from_f32(hipStream_t st, int64_t n, double* u, const float* d, const Scalars* sc_outer);

@@ -86,14 +86,14 @@ def batched(n_variants=64, sample=2, seed=7, tol=1e-8):
             "tol": tol}
 
 
-def newmark(nx=100, ny=20, nz=20, steps=1000, dt=1e-5, graphs=True, tol=1e-8, max_it=400, solver="fp64", inner_tol=1e-4):
+def newmark(nx=100, ny=20, nz=20, steps=1000, dt=1e-5, graphs=True, tol=1e-8, max_it=400, solver="auto", ru_factor=1e-3):
     from .problems import cantilever
     p = cantilever(nx, ny, nz, lx=1.0, ly=0.2, lz=0.2, E=70e9, nu=0.33, traction=(0.0, 0.0, -1.0e6))
     ctx = api.Context(0)
     problems.upload(ctx, p, rho=np.full(p.mesh.n_el, 2700.0))
     ctx.set_graphs(graphs)
     ctx.newmark_setup(dt, 0.25, 0.5)
-    ctx.newmark_set_solver(api.NM_MIXED if solver == "mixed" else api.NM_FP64, inner_tol)
+    ctx.newmark_set_solver({"mixed": api.NM_MIXED, "fp64": api.NM_FP64, "auto": api.NM_AUTO}[solver], ru_factor)
     n_pulse = int(round(1e-3 / dt))
     fac = np.where(np.arange(steps) < n_pulse, np.sin(np.pi * np.arange(steps) / n_pulse), 0.0)
     di = np.asarray(p.dof_index).reshape(-1, 6)
@@ -106,18 +106,18 @@ def newmark(nx=100, ny=20, nz=20, steps=1000, dt=1e-5, graphs=True, tol=1e-8, ma
             "dt": dt, "device_s": st.device_ms / 1e3, "us_per_step": st.device_ms * 1e3 / steps,
             "dof_steps_per_s": p.n_dof * steps / (st.device_ms / 1e3), "pcg_iterations": int(st.total_iterations),
             "max_step_iterations": int(st.max_step_iterations), "unconverged_steps": int(st.unconverged_steps),
-            "graph": bool(st.graph_used), "step_solver": solver, "inner_tol": inner_tol if solver == "mixed" else None, "step_tol": tol, "max_it_step": max_it,
+            "graph": bool(st.graph_used), "step_solver": "mixed" if st.step_solver == api.NM_MIXED else "fp64", "ru_factor": ru_factor if st.step_solver == api.NM_MIXED else None, "step_tol": tol, "max_it_step": max_it,
             "tip_w_min": float(hist.mean(axis=1).min())}
 
 
-def adjoint(nx=100, ny=20, nz=20, steps=1000, dt=1e-5, graphs=True, tol=1e-8, max_it=400, solver="fp64", inner_tol=1e-4):
+def adjoint(nx=100, ny=20, nz=20, steps=1000, dt=1e-5, graphs=True, tol=1e-8, max_it=400, solver="auto", ru_factor=1e-3):
     from .problems import cantilever
     p = cantilever(nx, ny, nz, lx=1.0, ly=0.2, lz=0.2, E=70e9, nu=0.33, traction=(0.0, 0.0, -1.0e6))
     ctx = api.Context(0)
     problems.upload(ctx, p, rho=np.full(p.mesh.n_el, 2700.0))
     ctx.set_graphs(graphs)
     ctx.newmark_setup(dt, 0.25, 0.5)
-    ctx.newmark_set_solver(api.NM_MIXED if solver == "mixed" else api.NM_FP64, inner_tol)
+    ctx.newmark_set_solver({"mixed": api.NM_MIXED, "fp64": api.NM_FP64, "auto": api.NM_AUTO}[solver], ru_factor)
     n_pulse = int(round(1e-3 / dt))
     fac = np.where(np.arange(steps) < n_pulse, np.sin(np.pi * np.arange(steps) / n_pulse), 0.0)
     di = np.asarray(p.dof_index).reshape(-1, 6)
@@ -129,7 +129,7 @@ def adjoint(nx=100, ny=20, nz=20, steps=1000, dt=1e-5, graphs=True, tol=1e-8, ma
     ctx.close()
     return {"workload": "newmark-adjoint", "mesh": f"{nx}x{ny}x{nz} brick8", "dof": int(p.n_dof), "steps": steps,
             "dt": dt, "objective": "sum_n dt * (sum of tip w)_n", "L": L,
-            "forward_s": fst.device_ms / 1e3, "adjoint_s": ast.device_ms / 1e3, "forward_step_solver": solver,
+            "forward_s": fst.device_ms / 1e3, "adjoint_s": ast.device_ms / 1e3, "forward_step_solver": "mixed" if fst.step_solver == api.NM_MIXED else "fp64",
             "forward_pcg_iterations": int(fst.total_iterations), "adjoint_pcg_iterations": int(ast.total_iterations),
             "unconverged_steps": int(fst.unconverged_steps + ast.unconverged_steps),
             "dLdE_abs_max": float(np.abs(dL).max()), "dLdE_sum_E": float(np.sum(dL * p.E)), "step_tol": tol}
@@ -153,14 +153,14 @@ def modal(n_modes=10, nx=200, ny=40, nz=40):
             "solve_tol": 1e-8}
 
 
-def shell(nx=256, ny=256, steps=1000, dt=1e-5, graphs=True, tol=1e-8, max_it=2000, load=1.0e4, solver="fp64", inner_tol=1e-4):
+def shell(nx=256, ny=256, steps=1000, dt=1e-5, graphs=True, tol=1e-8, max_it=2000, load=1.0e4, solver="auto", ru_factor=1e-3):
     p = problems.shell_plate(nx, ny, 1.0, 1.0, 0.005, 70e9, 0.33, 2700.0)
     ctx = api.Context(0)
     problems.upload_shell(ctx, p)
     ctx.set_graphs(graphs)
     t0 = time.perf_counter()
     ctx.newmark_setup(dt, 0.25, 0.5)
-    ctx.newmark_set_solver(api.NM_MIXED if solver == "mixed" else api.NM_FP64, inner_tol)
+    ctx.newmark_set_solver({"mixed": api.NM_MIXED, "fp64": api.NM_FP64, "auto": api.NM_AUTO}[solver], ru_factor)
     log(f"shell {nx}x{ny}: {p.n_dof} DOF, assembly of M and K + c_m M {time.perf_counter() - t0:.2f} s")
     di = np.asarray(p.dof_index).reshape(-1, 6)
     centre = p.mesh.node_id(nx // 2, ny // 2)
@@ -179,7 +179,7 @@ def shell(nx=256, ny=256, steps=1000, dt=1e-5, graphs=True, tol=1e-8, max_it=200
             "device_s": st.device_ms / 1e3, "us_per_step": st.device_ms * 1e3 / steps,
             "dof_steps_per_s": p.n_dof * steps / (st.device_ms / 1e3), "pcg_iterations": int(st.total_iterations),
             "max_step_iterations": int(st.max_step_iterations), "unconverged_steps": int(st.unconverged_steps),
-            "graph": bool(st.graph_used), "step_solver": solver, "inner_tol": inner_tol if solver == "mixed" else None, "step_tol": tol, "max_it_step": max_it,
+            "graph": bool(st.graph_used), "step_solver": "mixed" if st.step_solver == api.NM_MIXED else "fp64", "ru_factor": ru_factor if st.step_solver == api.NM_MIXED else None, "step_tol": tol, "max_it_step": max_it,
             "centre_w_min": float(w.min()), "centre_w_max": float(w.max()),
             "centre_w_min_step": int(np.argmin(w))}
 
@@ -195,19 +195,20 @@ def main():
     ap.add_argument("--nz", type=int, default=20)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--no-graphs", action="store_true")
-    ap.add_argument("--solver", choices=["mixed", "fp64"], default="fp64", help="Newmark step solver")
-    ap.add_argument("--inner-tol", type=float, default=1e-4, help="inner FP32 tolerance of the mixed step solver")
+    ap.add_argument("--solver", choices=["auto", "mixed", "fp64"], default="auto", help="Newmark step solver")
+    ap.add_argument("--ru-factor", type=float, default=1e-3,
+                    help="mixed step solver: residual reduction between FP64 reliable updates")
     a = ap.parse_args()
     if a.which == "batched":
         out = batched(a.variants, a.sample)
     elif a.which == "modal":
         out = modal(a.modes)
     elif a.which == "adjoint":
-        out = adjoint(a.nx or 100, a.ny or 20, a.nz, a.steps, graphs=not a.no_graphs, solver=a.solver, inner_tol=a.inner_tol)
+        out = adjoint(a.nx or 100, a.ny or 20, a.nz, a.steps, graphs=not a.no_graphs, solver=a.solver, ru_factor=a.ru_factor)
     elif a.which == "shell":
-        out = shell(a.nx or 256, a.ny or 256, a.steps, graphs=not a.no_graphs, solver=a.solver, inner_tol=a.inner_tol)
+        out = shell(a.nx or 256, a.ny or 256, a.steps, graphs=not a.no_graphs, solver=a.solver, ru_factor=a.ru_factor)
     else:
-        out = newmark(a.nx or 100, a.ny or 20, a.nz, a.steps, graphs=not a.no_graphs, solver=a.solver, inner_tol=a.inner_tol)
+        out = newmark(a.nx or 100, a.ny or 20, a.nz, a.steps, graphs=not a.no_graphs, solver=a.solver, ru_factor=a.ru_factor)
     print(json.dumps(out), flush=True)
 
 

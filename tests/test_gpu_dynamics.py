@@ -44,8 +44,8 @@ def test_newmark_matches_oracle(ctx, mode):
         ctx.newmark_set_solver(mode)
         hist, u, v, a, st = ctx.newmark_run(n, p.f, fac, watch=watch, tol=1e-12, max_it_step=400)
     finally:
-        ctx.newmark_set_solver(api.NM_FP64)
-    assert st.unconverged_steps == 0 and st.steps == n, st.as_dict()
+        ctx.newmark_set_solver(api.NM_AUTO)
+    assert st.unconverged_steps == 0 and st.steps == n and st.step_solver == mode, st.as_dict()
     scale = np.abs(hist_o).max()
     assert np.abs(hist - hist_o).max() <= 1e-8 * scale
     for x, xo in ((u, u_o), (v, v_o), (a, a_o)):
