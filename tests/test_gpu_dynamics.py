@@ -52,8 +52,11 @@ def test_newmark_matches_oracle(ctx, mode):
         assert np.abs(x - xo).max() <= 1e-7 * np.abs(xo).max()
 
 
-def test_newmark_graph_replay_matches_eager(ctx):
-    gpu_case("newmark graph vs eager")
+@pytest.mark.parametrize("mode", [api.NM_FP64, api.NM_MIXED])
+def test_newmark_graph_replay_matches_eager(ctx, mode):
+    # the step graphs (with the mixed solver's gated reliable-update slots)
+    # replayed repeatedly give the eager launches' results bit for bit
+    gpu_case(f"newmark graph vs eager, solver {mode}")
     p, _, _ = cantilever_case((12, 3, 3))
     problems.upload(ctx, p, rho=np.full(p.mesh.n_el, RHO))
     ctx.newmark_setup(5e-5, 0.25, 0.5)
@@ -61,14 +64,18 @@ def test_newmark_graph_replay_matches_eager(ctx):
     fac = np.sin(np.pi * np.arange(n) / n)
     watch = tip_dofs(p)
     runs = []
-    for on in (False, True, True):
-        ctx.set_graphs(on)
-        hist, u, v, a, st = ctx.newmark_run(n, p.f, fac, watch=watch, tol=1e-11, max_it_step=80)
-        assert st.graph_used == (1 if on else 0) and st.unconverged_steps == 0
-        runs.append((hist, u, v, a))
-        # eager work between replays: a static solve on the same context
-        ctx.static_compliance(p.f, ctx.opts(api.SOLVE_PCG_FP64, api.PC_JACOBI, tol=1e-8, max_it=100000))
-    ctx.set_graphs(True)
+    try:
+        ctx.newmark_set_solver(mode)
+        for on in (False, True, True):
+            ctx.set_graphs(on)
+            hist, u, v, a, st = ctx.newmark_run(n, p.f, fac, watch=watch, tol=1e-11, max_it_step=160)
+            assert st.graph_used == (1 if on else 0) and st.unconverged_steps == 0 and st.step_solver == mode
+            runs.append((hist, u, v, a))
+            # eager work between replays: a static solve on the same context
+            ctx.static_compliance(p.f, ctx.opts(api.SOLVE_PCG_FP64, api.PC_JACOBI, tol=1e-8, max_it=100000))
+    finally:
+        ctx.set_graphs(True)
+        ctx.newmark_set_solver(api.NM_AUTO)
     for r in runs[1:]:
         for x, x0 in zip(r, runs[0]):
             assert np.array_equal(x, x0)
