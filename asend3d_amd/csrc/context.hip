@@ -534,6 +534,27 @@ int solve_device(asend_ctx* ctx, const asend_solve_opts* o, asend_solve_stats* s
     if (o->stop_rule != ASEND_STOP_REL_RESIDUAL)
         return fail(ctx, ASEND_ERR_UNSUPPORTED, "mixed solver supports the relative-residual rule only");
     double inner_tol = o->inner_tol > 0 ? o->inner_tol : (ctx->tuned_inner_tol > 0 ? ctx->tuned_inner_tol : 1e-4);
+    // Reliable updates for the multigrid FCG (plain and learned smoother):
+    // one FCG run keeps its search directions through the FP64 refreshes,
+    // which come whenever the recursive FP32 residual has fallen by inner_tol
+    // (and when it claims convergence); the FP64 residual decides.
+    FcgRefresh ru;
+    ru.next = inner_tol * inner_tol;
+    ru.run = [&](double* ratio, double* next, bool* conv) -> int {
+        const double s_old = std::sqrt(rr);
+        launch_axpy_from_f32(ctx->st, n, ctx->d_u, ctx->d_x32, ctx->d_sc + 0);  // u += |r_old| x
+        HIPCHK(hipMemsetAsync(ctx->d_x32, 0, n * sizeof(float), ctx->st));
+        int rc2;
+        if ((rc2 = residual64(ctx, ctx->d_f, ctx->d_u, ctx->d_r))) return rc2;
+        if ((rc2 = read_scalars(ctx, 0))) return rc2;
+        rr = ctx->h_sc[0].rr;
+        *conv = !(rr > 0.0) || std::sqrt(rr / ff) <= o->tol;
+        if (*conv) return ASEND_OK;
+        launch_scale_to_f32(ctx->st, n, ctx->d_r, ctx->d_r32, ctx->d_sc + 0);
+        *ratio = s_old / std::sqrt(rr);
+        *next = std::max(inner_tol * inner_tol, o->tol * o->tol * ff / rr);
+        return ASEND_OK;
+    };
     int64_t outer = 0;
     for (; outer < max_outer; ++outer) {
         if (std::sqrt(rr / ff) <= o->tol) break;
@@ -558,7 +579,8 @@ int solve_device(asend_ctx* ctx, const asend_solve_opts* o, asend_solve_stats* s
                 if (smoother) mg_set_fine_smoother(ctx->mg, 1);
                 int fb = 0;
                 rc = mg_fcg_solve(ctx->mg, ctx->st, ctx->d_r32, ctx->d_x32, inner_tol,
-                                  std::min<int64_t>(budget, max_it - total_it), check, &it_in, &fb, smoother ? 0 : 1);
+                                  std::min<int64_t>(budget, max_it - total_it), check, &it_in, &fb, smoother ? 0 : 1,
+                                  smoother ? &ru : nullptr);
                 mg_set_fine_smoother(ctx->mg, 0);
             }
             if (rc == ASEND_ERR_NUMERIC || (rc == ASEND_OK && it_in >= budget)) {
@@ -584,10 +606,14 @@ int solve_device(asend_ctx* ctx, const asend_solve_opts* o, asend_solve_stats* s
             mg_set_fine_smoother(ctx->mg, 0);
             int fb = 0;
             if ((rc = mg_fcg_solve(ctx->mg, ctx->st, ctx->d_r32, ctx->d_x32, inner_tol, max_it - total_it, check,
-                                   &it_in, &fb, 0)))
+                                   &it_in, &fb, 0, &ru)))
                 return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
         }
         total_it += it_in;
+        if (ru.converged) {  // the last refresh took the solution and its residual
+            outer += ru.refreshes;
+            break;
+        }
         launch_axpy_from_f32(ctx->st, n, ctx->d_u, ctx->d_x32, ctx->d_sc + 0);
         if ((rc = residual64(ctx, ctx->d_f, ctx->d_u, ctx->d_r))) return rc;
         if ((rc = read_scalars(ctx, 0))) return rc;
@@ -604,9 +630,8 @@ int solve_device(asend_ctx* ctx, const asend_solve_opts* o, asend_solve_stats* s
 // Element integration into K: k_scale K_e(E) + c_mass M_e of every element.
 // Translated-element meshes (brick8, brickIM): one row-wise launch scaling the
 // reference blocks.  Other brick8 meshes: one row-wise gather launch that
-// integrates per element (no stiffness scale there: k_scale must be 1; pass
-// zero moduli for the mass alone).  brickIM and shell4: K is cleared, then
-// one scatter launch per element colour.
+// integrates per element.  brickIM and shell4: K is cleared, then one scatter
+// launch per element colour.
 int integrate_elements(asend_ctx* ctx, const double* E, double k_scale, double c_mass, double* K) {
     if (ctx->uniform_hex) {
         launch_hex8_assemble_uniform(ctx->st, (int)ctx->n_nodes, (int)ctx->n_slices, ctx->d_sptr, ctx->d_scol,

@@ -643,6 +643,11 @@ __global__ void k_copy(int64_t n, const float* __restrict__ a, float* __restrict
         b[i] = a[i];
 }
 
+__global__ void k_scal(int64_t n, float a, float* __restrict__ x) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] *= a;
+}
+
 // Chebyshev step: d = c1 d + c2 z; x += d   (z = D^-1 r or the preconditioned r)
 __global__ void k_cheb_step(int64_t n, const float* __restrict__ dinv, const float* __restrict__ r, float c1,
                             float c2, float* __restrict__ d, float* __restrict__ x) {
@@ -1482,7 +1487,7 @@ int mg_setup(Multigrid* mg, hipStream_t st, const double* d_E, const double* d_n
 }
 
 int mg_fcg_solve(Multigrid* mg, hipStream_t st, const float* b, float* x, double rel_tol, int64_t max_it,
-                 int check_every, int64_t* it_out, int* fallback_used, int precond) {
+                 int check_every, int64_t* it_out, int* fallback_used, int precond, FcgRefresh* ru) {
     if (!mg->ready) return mgfail(mg, ASEND_ERR_STATE, "multigrid not set up");
     if (precond == 1 && !mg->has_factor) return mgfail(mg, ASEND_ERR_STATE, "no learned factor set");
     // z = B r: one V-cycle, or the learned factor alone
@@ -1497,7 +1502,9 @@ int mg_fcg_solve(Multigrid* mg, hipStream_t st, const float* b, float* x, double
     // scalars
     double init[SV_N];
     std::memset(init, 0, sizeof(init));
-    init[SV_TOL2] = rel_tol * rel_tol;
+    // with reliable updates the FP64 residual decides convergence: the
+    // device-side test stops only at max_it
+    init[SV_TOL2] = ru ? 0.0 : rel_tol * rel_tol;
     init[SV_MAXIT] = (double)max_it;
     std::memcpy(mg->h_sv, init, sizeof(init));
     MGCHK(hipMemcpyAsync(mg->sv, mg->h_sv, sizeof(init), hipMemcpyHostToDevice, st));
@@ -1578,6 +1585,23 @@ int mg_fcg_solve(Multigrid* mg, hipStream_t st, const float* b, float* x, double
         MGCHK(hipStreamSynchronize(st));
         it = (int64_t)mg->h_sv[SV_IT];
         if (mg->h_sv[SV_DONE] != 0.0) break;
+        if (ru && mg->h_sv[SV_RR] <= ru->next) {
+            double ratio = 1.0, next = ru->next;
+            bool conv = false;
+            const int rc = ru->run(&ratio, &next, &conv);
+            if (rc) return rc;
+            ru->refreshes += 1;
+            if (conv) {
+                ru->converged = true;
+                break;
+            }
+            // r = b (the FP64 residual, normalised); p keeps its direction in
+            // the new normalisation
+            hipLaunchKernelGGL(k_copy, dim3(nb_for(n)), dim3(256), 0, st, n, b, mg->r);
+            hipLaunchKernelGGL(k_scal, dim3(nb_for(n)), dim3(256), 0, st, n, (float)ratio, mg->p);
+            MGCHK(hipGetLastError());
+            ru->next = next;
+        }
     }
     if (it_out) *it_out = it;
     if (!(mg->h_sv[SV_RR] == mg->h_sv[SV_RR]))

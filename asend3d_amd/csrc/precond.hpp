@@ -2,6 +2,8 @@
 // solver (definitions in precond.hip).
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <functional>
 #include <cstdint>
 
 #include "kernels.hpp"
@@ -59,11 +61,29 @@ void mg_use_graphs(Multigrid* mg, bool on);
 bool mg_ready(const Multigrid* mg);
 void mg_clear_learned_factor(Multigrid* mg);
 
+// Reliable updates of a mixed-precision FCG solve.  run() adds the FP32
+// correction x to the caller's FP64 solution and zeroes x, forms the FP64
+// residual, writes r / |r| into the right-hand side b, and returns
+// |r_old| / |r_new| (ratio), the recursive-residual level |r|^2 (relative to
+// the new normalisation) that triggers the next refresh (next), and whether
+// the FP64 residual meets the tolerance (converged).
+struct FcgRefresh {
+    double next = 1e-4;  // first trigger, relative to |b|^2 = 1
+    std::function<int(double* ratio, double* next, bool* converged)> run;
+    int refreshes = 0;
+    bool converged = false;
+};
+
 // Flexible CG (one truncation vector) on the FP32 fine operator
 // preconditioned by one V-cycle (precond 0) or by the learned factor alone
-// (precond 1): solves A x = b to ||r|| <= rel_tol ||b||.
+// (precond 1): solves A x = b to ||r|| <= rel_tol ||b||.  With ru, b must be
+// normalised (|b| = 1); the solve instead runs until ru->run reports
+// convergence (or max_it), refreshing the residual from FP64 whenever the
+// recursive |r|^2 falls below ru->next: r = b, the search direction rescaled
+// by the ratio, its previous directions kept (FCG takes alpha and beta from
+// fresh inner products each iteration, so nothing else carries over).
 int mg_fcg_solve(Multigrid* mg, hipStream_t st, const float* b, float* x, double rel_tol, int64_t max_it,
-                 int check_every, int64_t* it_out, int* fallback_used, int precond);
+                 int check_every, int64_t* it_out, int* fallback_used, int precond, FcgRefresh* ru = nullptr);
 
 // Diagnostics for tests: y = A_l x on level l (natural node order for l >= 1,
 // fine order for l = 0), and the transfer operators between l and l+1.

@@ -42,7 +42,9 @@ def test_backward_matches_finite_differences(small, group):
     lo, hi = {"W0": (mpnn.OFF_W0, mpnn.OFF_B0), "b0": (mpnn.OFF_B0, mpnn.OFF_L1),
               "layers": (mpnn.OFF_L1, mpnn.OFF_WH), "head": (mpnn.OFF_WH, mpnn.OFF_WA),
               "Wa": (mpnn.OFF_WA, mpnn.NUM_PARAMS), "all": (0, mpnn.NUM_PARAMS)}[group]
-    rng = np.random.default_rng(hash(group) % 1000)
+    # fixed seeds (str hashes vary per process): a random direction can put a
+    # ReLU kink inside +-h, which spoils the central difference
+    rng = np.random.default_rng({"W0": 11, "b0": 12, "layers": 13, "head": 14, "Wa": 15, "all": 16}[group])
     for _ in range(3):
         d = np.zeros(mpnn.NUM_PARAMS)
         d[lo:hi] = rng.standard_normal(hi - lo)
