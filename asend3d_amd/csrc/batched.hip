@@ -123,7 +123,43 @@ __global__ __launch_bounds__(256) void k_chunk_dot(const int32_t* __restrict__ c
     if (threadIdx.x < kNrhs) partials[(int64_t)ch * 2 * kNrhs + threadIdx.x] = sh[threadIdx.x];
 }
 
-// x += alpha p ; r -= alpha q ; partials of r.(Dinv r) and r.r  (FP32 inner)
+// partials of p.q and p.r per system (the flexible step length p.r / p.q)
+__global__ __launch_bounds__(256) void k_chunk_pqr(const int32_t* __restrict__ chunk_row,
+                                                   const int32_t* __restrict__ chunk_seg, const float* __restrict__ P,
+                                                   const float* __restrict__ Q, const float* __restrict__ R,
+                                                   const double* __restrict__ sv, double* __restrict__ partials) {
+    __shared__ double sh[2][256];
+    const int ch = blockIdx.x;
+    const int j = threadIdx.x % kNrhs;
+    const int64_t e0 = (int64_t)chunk_row[ch] * 3, e1 = (int64_t)chunk_row[ch + 1] * 3;
+    const int s = chunk_seg[ch] * kNrhs + j;
+    double t0 = 0.0, t1 = 0.0;
+    if (sv[s * B_N + B_DONE] == 0.0) {
+#pragma unroll 4
+        for (int64_t e = e0 + threadIdx.x / kNrhs; e < e1; e += 256 / kNrhs) {
+            const int64_t i = e * kNrhs + j;
+            const double pi = P[i];
+            t0 += pi * Q[i];
+            t1 += pi * R[i];
+        }
+    }
+    sh[0][threadIdx.x] = t0;
+    sh[1][threadIdx.x] = t1;
+    __syncthreads();
+    for (int k = 128; k >= kNrhs; k >>= 1) {
+        if (threadIdx.x < k) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + k];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < kNrhs) {
+        partials[(int64_t)ch * 2 * kNrhs + threadIdx.x] = sh[0][threadIdx.x];
+        partials[(int64_t)ch * 2 * kNrhs + kNrhs + threadIdx.x] = sh[1][threadIdx.x];
+    }
+}
+
+// x += alpha p ; r -= alpha q ; partials of (Dinv r).q and r.r  (FP32 inner)
 __global__ __launch_bounds__(256) void k_chunk_update(const int32_t* __restrict__ chunk_row,
                                                       const int32_t* __restrict__ chunk_seg,
                                                       const float* __restrict__ dinv, float* __restrict__ X,
@@ -144,7 +180,7 @@ __global__ __launch_bounds__(256) void k_chunk_update(const int32_t* __restrict_
             X[i] += alpha * P[i];
             const float ri = R[i] - alpha * Q[i];
             R[i] = ri;
-            t0 += (double)ri * (dinv[e] * ri);
+            t0 += (double)(dinv[e] * ri) * Q[i];
             t1 += (double)ri * ri;
         }
     }
@@ -200,6 +236,37 @@ __global__ __launch_bounds__(256) void k_chunk_init(const int32_t* __restrict__ 
     }
 }
 
+// partials of r.(Dinv r) and r.r (after a reliable update reloaded r)
+__global__ __launch_bounds__(256) void k_chunk_rz(const int32_t* __restrict__ chunk_row,
+                                                  const float* __restrict__ dinv, const float* __restrict__ R,
+                                                  double* __restrict__ partials) {
+    __shared__ double sh[2][256];
+    const int ch = blockIdx.x;
+    const int j = threadIdx.x % kNrhs;
+    const int64_t e0 = (int64_t)chunk_row[ch] * 3, e1 = (int64_t)chunk_row[ch + 1] * 3;
+    double t0 = 0.0, t1 = 0.0;
+#pragma unroll 4
+    for (int64_t e = e0 + threadIdx.x / kNrhs; e < e1; e += 256 / kNrhs) {
+        const float ri = R[e * kNrhs + j];
+        t0 += (double)ri * (dinv[e] * ri);
+        t1 += (double)ri * ri;
+    }
+    sh[0][threadIdx.x] = t0;
+    sh[1][threadIdx.x] = t1;
+    __syncthreads();
+    for (int k = 128; k >= kNrhs; k >>= 1) {
+        if (threadIdx.x < k) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + k];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < kNrhs) {
+        partials[(int64_t)ch * 2 * kNrhs + threadIdx.x] = sh[0][threadIdx.x];
+        partials[(int64_t)ch * 2 * kNrhs + kNrhs + threadIdx.x] = sh[1][threadIdx.x];
+    }
+}
+
 // p = Dinv r + beta p
 __global__ __launch_bounds__(256) void k_chunk_dir(const int32_t* __restrict__ chunk_row,
                                                    const int32_t* __restrict__ chunk_seg,
@@ -221,10 +288,15 @@ __global__ __launch_bounds__(256) void k_chunk_dir(const int32_t* __restrict__ c
 // Per-system finalize over the chunks of its variant: one workgroup per
 // variant; the 256 / kNrhs lanes of each right-hand side add a strided share
 // of the chunks, then a fixed-order tree over those lanes (deterministic).
+// The inner solves are flexible CG (one truncation vector): alpha = p.r / p.q
+// and beta = -z.q / p.q from fresh inner products, so a residual replaced by
+// a reliable update needs no repair of the recurrence.
 //   OP 0 (init):   rz, rr0 = rr, it = 0, done when rr0 = 0 or the system is frozen
-//   OP 1 (p.q):    pq, alpha = rz / pq
-//   OP 2 (update): beta = rz'/rz, rz = rz', rr, it += 1, done on rr <= tol2 rr0 or it >= max
+//   OP 1 (p.q, p.r): pq, alpha = pr / pq
+//   OP 2 (update): beta = -zq / pq, rr, it += 1, done on rr <= tol2 rr0 or it >= max
 //   OP 3 (|r|^2):  rr (FP64 residual norm) into out[s]
+//   OP 4 (reliable update): rz, rr = rr0 of the reloaded residual, tol2 =
+//                  out[s], the iteration count kept; done when frozen or r = 0
 __global__ __launch_bounds__(256) void k_sys_fin(const int32_t* __restrict__ seg_chunk,
                                                  const double* __restrict__ partials, int op,
                                                  double* __restrict__ sv, const uint8_t* __restrict__ frozen,
@@ -262,13 +334,18 @@ __global__ __launch_bounds__(256) void k_sys_fin(const int32_t* __restrict__ seg
         v[B_DONE] = (frozen[s] || t1 == 0.0) ? 1.0 : 0.0;
     } else if (op == 1) {
         v[B_PQ] = t0;
-        v[B_ALPHA] = v[B_RZ] / t0;
+        v[B_ALPHA] = t1 / t0;
     } else if (op == 2) {
-        v[B_BETA] = t0 / v[B_RZ];
-        v[B_RZ] = t0;
+        v[B_BETA] = -t0 / v[B_PQ];
         v[B_RR] = t1;
         v[B_IT] += 1.0;
         if (t1 <= v[B_TOL2] * v[B_RR0] || v[B_IT] >= v[B_MAXIT] || !(t1 == t1)) v[B_DONE] = 1.0;
+    } else if (op == 4) {
+        v[B_RZ] = t0;
+        v[B_RR] = t1;
+        v[B_RR0] = t1;
+        v[B_TOL2] = out[s];
+        v[B_DONE] = (frozen[s] || t1 == 0.0 || v[B_IT] >= v[B_MAXIT]) ? 1.0 : 0.0;
     } else {
         out[s] = t0;
     }
@@ -291,6 +368,20 @@ __global__ void k_mr_to_f32(int64_t n_elem, const double* __restrict__ R, const 
         const int64_t row = i / (3 * kNrhs);
         const int s = seg_of_elem_row[row] * kNrhs + (int)(i % kNrhs);
         R32[i] = (float)(R[i] * inv_norm[s]);
+    }
+}
+
+// Reliable update of the search direction per system: P *= ratio[s] (the
+// same direction in the new normalisation of its residual), or, where
+// ratio[s] = 0, a restart P = Dinv R
+__global__ void k_mr_scale(int64_t n_elem, const double* __restrict__ ratio, const int32_t* __restrict__ seg_of_elem_row,
+                           const float* __restrict__ dinv, const float* __restrict__ R, float* __restrict__ P) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_elem;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t row = i / (3 * kNrhs);
+        const int s = seg_of_elem_row[row] * kNrhs + (int)(i % kNrhs);
+        const double q = ratio[s];
+        P[i] = q != 0.0 ? (float)(P[i] * q) : dinv[i / kNrhs] * R[i];
     }
 }
 
@@ -334,7 +425,7 @@ struct Batch {
     // vectors [rows_total][3][kNrhs]
     double *F = nullptr, *U = nullptr, *R64 = nullptr;
     float *B32 = nullptr, *X = nullptr, *R = nullptr, *P = nullptr, *Q = nullptr;
-    double *sv = nullptr, *partials = nullptr, *rr = nullptr, *inv_norm = nullptr;
+    double *sv = nullptr, *partials = nullptr, *rr = nullptr, *inv_norm = nullptr, *ratio = nullptr, *tol2 = nullptr;
     uint8_t* frozen = nullptr;
     double* h_sv = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -467,6 +558,8 @@ int batch_create(Batch** out, const std::vector<OperatorView>& ops) {
     BCHK(hipMalloc((void**)&b->partials, (size_t)b->n_chunks * 2 * kNrhs * sizeof(double)));
     BCHK(hipMalloc((void**)&b->rr, n_sys * sizeof(double)));
     BCHK(hipMalloc((void**)&b->inv_norm, n_sys * sizeof(double)));
+    BCHK(hipMalloc((void**)&b->ratio, n_sys * sizeof(double)));
+    BCHK(hipMalloc((void**)&b->tol2, n_sys * sizeof(double)));
     BCHK(hipMalloc((void**)&b->frozen, n_sys));
     BCHK(hipHostMalloc((void**)&b->h_sv, (size_t)n_sys * B_N * sizeof(double), hipHostMallocDefault));
     BCHK(hipStreamSynchronize(b->st));
@@ -487,7 +580,8 @@ void batch_destroy(Batch* b) {
     bfree(b->seg_of_row);
     bfree(b->slice_var);
     bfree(b->active);
-    for (double** q : {&b->F, &b->U, &b->R64, &b->sv, &b->partials, &b->rr, &b->inv_norm}) bfree(*q);
+    for (double** q : {&b->F, &b->U, &b->R64, &b->sv, &b->partials, &b->rr, &b->inv_norm, &b->ratio, &b->tol2})
+        bfree(*q);
     for (float** q : {&b->B32, &b->X, &b->R, &b->P, &b->Q}) bfree(*q);
     bfree(b->frozen);
     if (b->h_sv) (void)hipHostFree(b->h_sv);
@@ -519,21 +613,48 @@ int batch_solve(Batch* b, const double* h_F, double* h_U, int nrhs, double tol, 
     BCHK(hipMemcpyAsync(ff.data(), b->rr, n_sys * sizeof(double), hipMemcpyDeviceToHost, b->st));
     BCHK(hipStreamSynchronize(b->st));
     rr = ff;
+    // FP32 lockstep block-PCG with FP64 reliable updates.  Every system runs
+    // one PCG whose residual is normalised at its last refresh; a refresh
+    // (U += |r| X in FP64, the FP64 residual of every system, the converged
+    // systems frozen, R = R64 / |R64|, P rescaled by |r_old| / |r_new|, rz and
+    // |r|^2 recomputed) keeps every search direction.  It comes once every
+    // active system has cut its recursive residual by inner_tol or claims
+    // convergence; the FP64 residual alone decides convergence.
     std::vector<uint8_t> frozen(n_sys);
-    std::vector<double> init((size_t)n_sys * B_N, 0.0);
+    std::vector<double> init((size_t)n_sys * B_N, 0.0), ratio(n_sys, 0.0), tol2(n_sys, 0.0), rr_prev(n_sys, 0.0);
     int64_t total_inner = 0, outer = 0;
     std::vector<int64_t> sys_it(n_sys, 0);
-    for (; outer < 60; ++outer) {
+    const double ru2 = inner_tol * inner_tol;
+    // next refresh targets and the refresh itself (host part)
+    auto classify = [&]() {
         int active = 0;
         for (int s = 0; s < n_sys; ++s) {
             const bool conv = (s % kNrhs) >= nrhs || ff[s] == 0.0 || std::sqrt(rr[s] / ff[s]) <= tol;
+            const double inv_old = invn[s];
             frozen[s] = conv ? 1 : 0;
             invn[s] = conv ? 0.0 : 1.0 / std::sqrt(rr[s]);
+            // |r_old| / |r_new|: the factor that keeps P's direction in the new
+            // normalisation; 0 (restart the direction) where the FP32
+            // recursion drifted, i.e. the FP64 residual is more than twice the
+            // recursive one: the old directions no longer fit the residual
+            ratio[s] = (conv || inv_old == 0.0) ? 0.0 : 1.0 / (inv_old * std::sqrt(rr[s]));
+            if (ratio[s] != 0.0) {
+                const double rec = b->h_sv[(size_t)s * B_N + B_RR] * rr_prev[s];
+                if (!(rec > 0.0) || rr[s] > 4.0 * rec) ratio[s] = 0.0;
+            }
+            // recursive convergence target of the renormalised system
+            tol2[s] = conv ? 0.0 : tol * tol * ff[s] / rr[s];
             active += conv ? 0 : 1;
-            init[(size_t)s * B_N + B_TOL2] = inner_tol * inner_tol;
-            init[(size_t)s * B_N + B_MAXIT] = (double)max_it;
         }
-        if (!active) break;
+        return active;
+    };
+    std::fill(invn.begin(), invn.end(), 0.0);
+    int active = classify();
+    for (int s = 0; s < n_sys; ++s) {
+        init[(size_t)s * B_N + B_TOL2] = tol2[s];
+        init[(size_t)s * B_N + B_MAXIT] = (double)max_it;
+    }
+    if (active) {
         BCHK(hipMemcpyAsync(b->frozen, frozen.data(), n_sys, hipMemcpyHostToDevice, b->st));
         BCHK(hipMemcpyAsync(b->inv_norm, invn.data(), n_sys * sizeof(double), hipMemcpyHostToDevice, b->st));
         std::memcpy(b->h_sv, init.data(), init.size() * sizeof(double));
@@ -546,47 +667,68 @@ int batch_solve(Batch* b, const double* h_F, double* h_U, int nrhs, double tol, 
                            b->frozen, (double*)nullptr);
         hipLaunchKernelGGL(k_var_active, dim3((b->n_var + 255) / 256), dim3(256), 0, b->st, b->n_var, b->sv,
                            b->active);
-        for (;;) {
-            for (int k = 0; k < check_every; ++k) {
-                spmm32(b, b->P, b->Q);
-                hipLaunchKernelGGL((k_chunk_dot<float, 0>), dim3(b->n_chunks), dim3(256), 0, b->st, b->chunk_row,
-                                   b->chunk_seg, b->P, b->Q, b->sv, b->partials);
-                hipLaunchKernelGGL(k_sys_fin, dim3(b->n_var), dim3(256), 0, b->st, b->seg_chunk, b->partials, 1,
-                                   b->sv, b->frozen, (double*)nullptr);
-                hipLaunchKernelGGL(k_chunk_update, dim3(b->n_chunks), dim3(256), 0, b->st, b->chunk_row, b->chunk_seg,
-                                   b->dinv, b->X, b->R, b->P, b->Q, b->sv, b->partials);
-                hipLaunchKernelGGL(k_sys_fin, dim3(b->n_var), dim3(256), 0, b->st, b->seg_chunk, b->partials, 2,
-                                   b->sv, b->frozen, (double*)nullptr);
-                hipLaunchKernelGGL(k_var_active, dim3((b->n_var + 255) / 256), dim3(256), 0, b->st, b->n_var, b->sv,
-                                   b->active);
-                hipLaunchKernelGGL(k_chunk_dir, dim3(b->n_chunks), dim3(256), 0, b->st, b->chunk_row, b->chunk_seg,
-                                   b->dinv, b->R, b->P, b->sv);
-            }
-            BCHK(hipGetLastError());
-            BCHK(hipMemcpyAsync(b->h_sv, b->sv, (size_t)n_sys * B_N * sizeof(double), hipMemcpyDeviceToHost, b->st));
-            BCHK(hipStreamSynchronize(b->st));
-            bool all = true;
-            for (int s = 0; s < n_sys; ++s) all = all && b->h_sv[(size_t)s * B_N + B_DONE] != 0.0;
-            if (all) break;
+    }
+    while (active && outer < 200) {
+        for (int k = 0; k < check_every; ++k) {
+            spmm32(b, b->P, b->Q);
+            hipLaunchKernelGGL(k_chunk_pqr, dim3(b->n_chunks), dim3(256), 0, b->st, b->chunk_row, b->chunk_seg, b->P,
+                               b->Q, b->R, b->sv, b->partials);
+            hipLaunchKernelGGL(k_sys_fin, dim3(b->n_var), dim3(256), 0, b->st, b->seg_chunk, b->partials, 1, b->sv,
+                               b->frozen, (double*)nullptr);
+            hipLaunchKernelGGL(k_chunk_update, dim3(b->n_chunks), dim3(256), 0, b->st, b->chunk_row, b->chunk_seg,
+                               b->dinv, b->X, b->R, b->P, b->Q, b->sv, b->partials);
+            hipLaunchKernelGGL(k_sys_fin, dim3(b->n_var), dim3(256), 0, b->st, b->seg_chunk, b->partials, 2, b->sv,
+                               b->frozen, (double*)nullptr);
+            hipLaunchKernelGGL(k_var_active, dim3((b->n_var + 255) / 256), dim3(256), 0, b->st, b->n_var, b->sv,
+                               b->active);
+            hipLaunchKernelGGL(k_chunk_dir, dim3(b->n_chunks), dim3(256), 0, b->st, b->chunk_row, b->chunk_seg,
+                               b->dinv, b->R, b->P, b->sv);
         }
-        int64_t mx = 0;
-        for (int s = 0; s < n_sys; ++s) {
-            const int64_t it = (int64_t)b->h_sv[(size_t)s * B_N + B_IT];
-            sys_it[s] += it;
-            mx = std::max(mx, it);
+        total_inner += check_every;
+        BCHK(hipGetLastError());
+        BCHK(hipMemcpyAsync(b->h_sv, b->sv, (size_t)n_sys * B_N * sizeof(double), hipMemcpyDeviceToHost, b->st));
+        BCHK(hipStreamSynchronize(b->st));
+        // refresh once every active system is done or has cut its residual by inner_tol
+        bool refresh = true, stalled = true;
+        for (int s = 0; s < n_sys && refresh; ++s) {
+            if (frozen[s]) continue;
+            const double* v = b->h_sv + (size_t)s * B_N;
+            const bool done = v[B_DONE] != 0.0;
+            refresh = done || v[B_RR] <= ru2 * v[B_RR0];
+            stalled = stalled && done && v[B_IT] >= v[B_MAXIT];
         }
-        total_inner += mx;
+        if (!refresh) continue;
+        ++outer;
+        for (int s = 0; s < n_sys; ++s) sys_it[s] = (int64_t)b->h_sv[(size_t)s * B_N + B_IT];
         hipLaunchKernelGGL(k_mr_axpy, dim3(nbk(ne)), dim3(256), 0, b->st, ne, b->X, b->inv_norm, b->seg_of_row, b->U);
+        BCHK(hipMemsetAsync(b->X, 0, ne * sizeof(float), b->st));
         residual64(b);
+        rr_prev = rr;
         BCHK(hipMemcpyAsync(rr.data(), b->rr, n_sys * sizeof(double), hipMemcpyDeviceToHost, b->st));
         BCHK(hipStreamSynchronize(b->st));
+        active = classify();
         if (verbose) {
             double worst = 0.0;
             for (int s = 0; s < n_sys; ++s)
                 if ((s % kNrhs) < nrhs && ff[s] > 0.0) worst = std::max(worst, std::sqrt(rr[s] / ff[s]));
-            std::fprintf(stderr, "[batch] refinement %lld: %d active systems, %lld inner iterations, worst residual %.2e\n",
-                         (long long)outer + 1, active, (long long)mx, worst);
+            std::fprintf(stderr, "[batch] refresh %lld: %d active systems, %lld lockstep iterations, worst residual %.2e\n",
+                         (long long)outer, active, (long long)total_inner, worst);
         }
+        if (!active || stalled) break;
+        BCHK(hipMemcpyAsync(b->frozen, frozen.data(), n_sys, hipMemcpyHostToDevice, b->st));
+        BCHK(hipMemcpyAsync(b->inv_norm, invn.data(), n_sys * sizeof(double), hipMemcpyHostToDevice, b->st));
+        BCHK(hipMemcpyAsync(b->ratio, ratio.data(), n_sys * sizeof(double), hipMemcpyHostToDevice, b->st));
+        BCHK(hipMemcpyAsync(b->tol2, tol2.data(), n_sys * sizeof(double), hipMemcpyHostToDevice, b->st));
+        hipLaunchKernelGGL(k_mr_to_f32, dim3(nbk(ne)), dim3(256), 0, b->st, ne, b->R64, b->inv_norm, b->seg_of_row,
+                           b->R);
+        hipLaunchKernelGGL(k_mr_scale, dim3(nbk(ne)), dim3(256), 0, b->st, ne, b->ratio, b->seg_of_row, b->dinv, b->R,
+                           b->P);
+        hipLaunchKernelGGL(k_chunk_rz, dim3(b->n_chunks), dim3(256), 0, b->st, b->chunk_row, b->dinv, b->R,
+                           b->partials);
+        hipLaunchKernelGGL(k_sys_fin, dim3(b->n_var), dim3(256), 0, b->st, b->seg_chunk, b->partials, 4, b->sv,
+                           b->frozen, b->tol2);
+        hipLaunchKernelGGL(k_var_active, dim3((b->n_var + 255) / 256), dim3(256), 0, b->st, b->n_var, b->sv,
+                           b->active);
     }
     BCHK(hipEventRecord(b->ev1, b->st));
     BCHK(hipEventSynchronize(b->ev1));

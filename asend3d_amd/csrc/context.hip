@@ -395,6 +395,59 @@ int run_cg1_pcg(asend_ctx* ctx, int si, const double* b, double* x, bool warm, d
     return ASEND_OK;
 }
 
+// FP32 flexible Jacobi PCG on K32 with the FP64 reliable updates of ru (see
+// solve_device): starts on the normalised residual in d_r32, refreshes when
+// the recursive residual has fallen to ru.next (or claims convergence), and
+// restarts the search direction where the FP32 recursion drifted (FP64
+// residual more than twice the recursive one).  rr: the caller's FP64 |r|^2,
+// updated by the refreshes; ff = |f|^2.
+int run_jacobi_fpcg_ru(asend_ctx* ctx, FcgRefresh& ru, double& rr, double ff, double tol, int64_t max_it,
+                       int check_every, int64_t* it_out) {
+    const int64_t n = ctx->n_dof;
+    float *x = ctx->d_x32, *r = ctx->d_w32, *p = ctx->d_p32, *q = ctx->d_q32;
+    const float* dinv = ctx->d_dinv32;
+    Scalars& h = ctx->h_sc[1];
+    std::memset(&h, 0, sizeof(Scalars));
+    h.tol2 = tol * tol * ff / rr;  // recursive convergence claim, relative to |b|^2 = 1
+    h.max_it = max_it;
+    int rc;
+    if ((rc = write_scalars(ctx, 1))) return rc;
+    Scalars* sc = ctx->d_sc + 1;
+    launch_pcg_init<float>(ctx->st, n, ctx->d_r32, dinv, x, r, p, ctx->d_partials, sc);
+    *it_out = 0;
+    for (;;) {
+        for (int k = 0; k < check_every; ++k) {
+            launch_sell3<float>(ctx->st, 7, ctx->fine(), ctx->d_K32, p, q, r, ctx->d_partials, sc);
+            launch_fpcg_update<float>(ctx->st, n, dinv, x, r, p, q, ctx->d_partials, sc);
+            launch_pcg_dir<float>(ctx->st, n, dinv, r, p, sc);
+        }
+        HIPCHK(hipGetLastError());
+        if ((rc = read_scalars(ctx, 1))) return rc;
+        *it_out = h.it;
+        if (h.it >= max_it) break;
+        if (!(h.done || h.rr <= ru.next * h.ff)) continue;
+        const double rec = h.rr / h.ff * rr;  // recursive residual, FP64 units
+        double ratio = 1.0, next = ru.next;
+        bool conv = false;
+        if ((rc = ru.run(&ratio, &next, &conv))) return rc;
+        ru.refreshes += 1;
+        if (conv) {
+            ru.converged = true;
+            break;
+        }
+        if (rr > 4.0 * rec) ratio = 0.0;  // drifted: restart the direction
+        launch_ru_reload<float>(ctx->st, n, ctx->d_r32, dinv, r, p, ratio);
+        HIPCHK(hipGetLastError());
+        ru.next = next;
+        h.ff = 1.0;
+        h.rr = 1.0;
+        h.done = 0;
+        h.tol2 = tol * tol * ff / rr;
+        if ((rc = write_scalars(ctx, 1))) return rc;
+    }
+    return ASEND_OK;
+}
+
 // FP64 residual r = f - K u, sc[0].rr = ||r||^2.
 int residual64(asend_ctx* ctx, const double* f, const double* u, double* r) {
     launch_sell3<double>(ctx->st, 2, ctx->fine(), ctx->d_K64, u, r, f, ctx->d_partials, ctx->d_sc + 0);
@@ -596,10 +649,7 @@ int solve_device(asend_ctx* ctx, const asend_solve_opts* o, asend_solve_stats* s
         if (use_learned) {
             // done above
         } else if (o->precond == ASEND_PC_JACOBI || learned_pc) {
-            if ((rc = run_jacobi_pcg<float>(ctx, 1, ctx->d_r32, ctx->d_x32, inner_tol, false, max_it - total_it,
-                                            check)))
-                return rc;
-            it_in = ctx->h_sc[1].it;
+            if ((rc = run_jacobi_fpcg_ru(ctx, ru, rr, ff, o->tol, max_it - total_it, check, &it_in))) return rc;
         } else {
             if (o->precond != ASEND_PC_MG) return fail(ctx, ASEND_ERR_ARG, "unknown preconditioner %d", o->precond);
             if ((rc = ensure_mg(ctx))) return rc;

@@ -348,7 +348,9 @@ enum FinOp {
     FIN_CG1_INIT = 7,
     FIN_CG1 = 8,
     FIN_RU_RES = 9,
-    FIN_CG1_RU = 10
+    FIN_CG1_RU = 10,
+    FIN_PQR = 11,
+    FIN_FUPD = 12
 };
 
 __global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ partials, int nparts, int op,
@@ -356,6 +358,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ 
     if ((op == FIN_PQ || op == FIN_UPDATE || op == FIN_CG1) && sc->done) return;
     if (op == FIN_RU_RES && !sc->gate) return;
     if (op == FIN_CG1_RU && (sc->done || sc->gate)) return;
+    if ((op == FIN_PQR || op == FIN_FUPD) && sc->done) return;
     if (op == FIN_MAX) {
         double m = 0.0;
         for (int b = threadIdx.x; b < nparts; b += blockDim.x) m = fmax(m, partials[b]);
@@ -370,8 +373,9 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ 
         }
         return;
     }
-    const int nv =
-        (op == FIN_INIT || op == FIN_UPDATE) ? 2 : ((op == FIN_INIT_WARM || op == FIN_CG1 || op == FIN_CG1_RU) ? 3 : 1);
+    const int nv = (op == FIN_INIT || op == FIN_UPDATE || op == FIN_PQR || op == FIN_FUPD)
+                       ? 2
+                       : ((op == FIN_INIT_WARM || op == FIN_CG1 || op == FIN_CG1_RU) ? 3 : 1);
     double t[3] = {0.0, 0.0, 0.0};
     for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
         t[0] += partials[(size_t)nv * b];
@@ -467,6 +471,19 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ 
             sc->alpha = g / den;
             sc->rz = g;
             sc->it += 1;
+            break;
+        }
+        case FIN_PQR:
+            // flexible step length: t0 = p.q, t1 = p.r
+            sc->pq = t[0];
+            sc->alpha = t[1] / t[0];
+            break;
+        case FIN_FUPD: {
+            // flexible direction: t0 = (Dinv r).q, t1 = r.r
+            sc->beta = -t[0] / sc->pq;
+            sc->rr = t[1];
+            sc->it += 1;
+            if (t[1] <= sc->tol2 * sc->ff || sc->it >= sc->max_it || !(t[1] == t[1])) sc->done = 1;
             break;
         }
         case FIN_UPDATE: {
@@ -569,7 +586,7 @@ __global__ __launch_bounds__(kSellMaxWaves * kWave) void k_sell3(int n_rows, int
                                                   const int32_t* __restrict__ scol, const T* __restrict__ val,
                                                   const T* __restrict__ x, T* __restrict__ y,
                                                   const T* __restrict__ f, double* partials, const Scalars* sc) {
-    if ((MODE == 1 || MODE == 4) && sc->done) return;
+    if ((MODE == 1 || MODE == 4 || MODE == 7) && sc->done) return;
     if (MODE == 6 && (sc->done || sc->gate)) return;
     if (MODE == 5 && !sc->gate) return;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
@@ -605,6 +622,14 @@ __global__ __launch_bounds__(kSellMaxWaves * kWave) void k_sell3(int n_rows, int
                 y[3 * (int64_t)r + 1] = y1;
                 y[3 * (int64_t)r + 2] = y2;
                 part = (double)y0 * y0 + (double)y1 * y1 + (double)y2 * y2;
+            } else if (MODE == 7) {
+                // x = p, f = r: partials p.(K p) and p.r
+                y[3 * (int64_t)r] = a0;
+                y[3 * (int64_t)r + 1] = a1;
+                y[3 * (int64_t)r + 2] = a2;
+                const double p0 = x[3 * (int64_t)r], p1 = x[3 * (int64_t)r + 1], p2 = x[3 * (int64_t)r + 2];
+                part = (double)a0 * p0 + (double)a1 * p1 + (double)a2 * p2;
+                part1 = p0 * f[3 * (int64_t)r] + p1 * f[3 * (int64_t)r + 1] + p2 * f[3 * (int64_t)r + 2];
             } else if (MODE == 4 || MODE == 6) {
                 // x = u (preconditioned residual), f = r: partials r.u, (K u).u, r.r
                 y[3 * (int64_t)r] = a0;
@@ -629,6 +654,11 @@ __global__ __launch_bounds__(kSellMaxWaves * kWave) void k_sell3(int n_rows, int
     if (MODE == 4 || MODE == 6) {
         double t[3] = {part, part1, part2};
         block_partials<3>(t, partials);
+        return;
+    }
+    if (MODE == 7) {
+        double t[2] = {part, part1};
+        block_partials<2>(t, partials);
         return;
     }
     double t[1] = {part};
@@ -677,6 +707,10 @@ void launch_sell3(hipStream_t st, int mode, const Sell3& A, const T* val, const 
             sell3_go<T, 6>(st, nt, nblk, nthr, A, val, x, y, f, partials, sc);
             finalize(st, partials, nblk, FIN_CG1_RU, sc);
             break;
+        case 7:
+            sell3_go<T, 7>(st, nt, nblk, nthr, A, val, x, y, f, partials, sc);
+            finalize(st, partials, nblk, FIN_PQR, sc);
+            break;
         default:
             sell3_go<T, 3>(st, nt, nblk, nthr, A, val, x, y, f, partials, sc);
             break;
@@ -700,6 +734,8 @@ template void launch_sell3<float>(hipStream_t, int, const Sell3&, const float*, 
 //   reliable update) -> finalize: sc->rr, convergence on the FP64 residual
 // MODE 6 (sliced-ELL only): MODE 4 paused while sc->gate is set -> finalize:
 //   the mixed solve's FP32 step scalars, or a refresh request
+// MODE 7 (sliced-ELL only): y = K x with x = p and f = r, partials of p.y and
+//   p.r -> finalize: the flexible PCG step length alpha = p.r / p.q
 // Kernels with a scalars pointer return at once when sc->done is set.
 // ---------------------------------------------------------------------------
 template <typename T, int G, int MODE>
@@ -906,6 +942,50 @@ __global__ __launch_bounds__(kBlock) void k_cg1_update(int64_t n, const T* __res
     }
 }
 
+// Flexible Jacobi PCG (one truncation vector): alpha = p.r / p.q (launch_sell3
+// mode 7), x += alpha p, r -= alpha q, beta = -(Dinv r).q / p.q, then
+// p = Dinv r + beta p (launch_pcg_dir).  Fresh inner products every
+// iteration, so a residual replaced by a reliable update needs no repair.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fpcg_update(int64_t n, const T* __restrict__ dinv, T* __restrict__ x,
+                                                        T* __restrict__ r, const T* __restrict__ p,
+                                                        const T* __restrict__ q, double* partials,
+                                                        const Scalars* sc) {
+    if (sc->done) return;
+    const T alpha = (T)sc->alpha;
+    double t[2] = {0.0, 0.0};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        x[i] += alpha * p[i];
+        const T ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        t[0] += (double)(dinv[i] * ri) * q[i];
+        t[1] += (double)ri * ri;
+    }
+    block_partials<2>(t, partials);
+}
+// reliable update: r = b (the reloaded residual); p *= ratio, or p = Dinv r
+// when ratio = 0 (restart)
+template <typename T>
+__global__ void k_ru_reload(int64_t n, const T* __restrict__ b, const T* __restrict__ dinv, T* __restrict__ r,
+                            T* __restrict__ p, double ratio) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const T bi = b[i];
+        r[i] = bi;
+        p[i] = ratio != 0.0 ? (T)(p[i] * ratio) : dinv[i] * bi;
+    }
+}
+
+template <typename T>
+void launch_fpcg_update(hipStream_t st, int64_t n, const T* dinv, T* x, T* r, const T* p, const T* q,
+                        double* partials, Scalars* sc) {
+    hipLaunchKernelGGL(k_fpcg_update<T>, dim3(kVecBlocks), dim3(kBlock), 0, st, n, dinv, x, r, p, q, partials, sc);
+    finalize(st, partials, kVecBlocks, FIN_FUPD, sc);
+}
+template <typename T>
+void launch_ru_reload(hipStream_t st, int64_t n, const T* b, const T* dinv, T* r, T* p, double ratio) {
+    hipLaunchKernelGGL(k_ru_reload<T>, dim3(kVecBlocks), dim3(kBlock), 0, st, n, b, dinv, r, p, ratio);
+}
+
 template <typename T>
 void launch_cg1_init(hipStream_t st, int64_t n, const T* b, bool cold, const T* dinv, T* x, T* r, T* u, T* p, T* s,
                      double* partials, Scalars* sc) {
@@ -1024,7 +1104,10 @@ void launch_pcg_dir(hipStream_t st, int64_t n, const T* dinv, const T* r, T* p, 
     template void launch_cg1_init<T>(hipStream_t, int64_t, const T*, bool, const T*, T*, T*, T*, T*, T*, double*, \
                                      Scalars*);                                                                   \
     template void launch_cg1_update<T>(hipStream_t, int64_t, const T*, T*, T*, T*, const T*, T*, T*,              \
-                                       const Scalars*);
+                                       const Scalars*);                                                           \
+    template void launch_fpcg_update<T>(hipStream_t, int64_t, const T*, T*, T*, const T*, const T*, double*,      \
+                                        Scalars*);                                                                \
+    template void launch_ru_reload<T>(hipStream_t, int64_t, const T*, const T*, T*, T*, double);
 INST_PCG(double)
 INST_PCG(float)
 

@@ -102,6 +102,14 @@ void launch_pcg_update(hipStream_t st, int64_t n, const T* dinv, T* x, T* r, con
                        double* partials, Scalars* sc);
 template <typename T>
 void launch_pcg_dir(hipStream_t st, int64_t n, const T* dinv, const T* r, T* p, const Scalars* sc);
+// Flexible Jacobi PCG step (after launch_sell3 mode 7): x += alpha p,
+// r -= alpha q, beta = -(Dinv r).q / p.q; then launch_pcg_dir.
+template <typename T>
+void launch_fpcg_update(hipStream_t st, int64_t n, const T* dinv, T* x, T* r, const T* p, const T* q,
+                        double* partials, Scalars* sc);
+// reliable update of a flexible PCG: r = b; p *= ratio (ratio = 0: p = Dinv r)
+template <typename T>
+void launch_ru_reload(hipStream_t st, int64_t n, const T* b, const T* dinv, T* r, T* p, double ratio);
 // Single-reduction Jacobi PCG: start (cold: x = 0, r = b; warm: r = b - K x
 // formed by the caller), then per iteration launch_sell3 mode 4 (w = K u)
 // followed by launch_cg1_update.
