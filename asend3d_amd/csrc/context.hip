@@ -2356,14 +2356,15 @@ int asend_autotune(asend_ctx* ctx, const double* rhs, int what, asend_tuning* ou
     }
     out->spmv_waves_per_block = ctx->spmv_wpb;
     if (what & ASEND_TUNE_SOLVER) {
-        // multigrid-preconditioned mixed solve: Chebyshev degree, inner
-        // tolerance and convergence-check interval, timed on full solves of rhs
+        // multigrid-preconditioned mixed solve: Chebyshev degree, reliable-
+        // update factor and convergence-check interval, timed on full solves
+        // of rhs
         if (!rhs) return fail(ctx, ASEND_ERR_ARG, "solver tuning needs a right-hand side");
         if (!ctx->mg) return fail(ctx, ASEND_ERR_STATE, "solver tuning needs the multigrid hierarchy");
         HIPCHK(hipMemcpy(ctx->d_f, rhs, ctx->n_dof * 8, hipMemcpyHostToDevice));
-        const int degrees[2] = {2, 3};
-        const double tols[3] = {1e-2, 1e-3, 1e-4};
-        const int checks[3] = {2, 4, 8};
+        const int degrees[3] = {1, 2, 3};
+        const double tols[4] = {1e-1, 1e-2, 1e-3, 1e-4};
+        const int checks[3] = {1, 2, 4};
         double best = 1e300;
         int bd = 2, bc = 4;
         double bt = 1e-3;

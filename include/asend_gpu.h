@@ -385,9 +385,10 @@ int asend_set_graphs(asend_ctx* ctx, int on);
 /* Autotuner (no reference counterpart).  ASEND_TUNE_SPMV times the fine SpMV
  * with 1, 2, 4 and 8 slices (wavefronts) per workgroup and keeps the fastest;
  * ASEND_TUNE_SOLVER times full multigrid-preconditioned mixed solves of rhs
- * over Chebyshev degree {2, 3} x inner tolerance {1e-2, 1e-3, 1e-4} x check
- * interval {2, 4, 8} and keeps the fastest converged choice; a later solve
- * uses the tuned inner tolerance / check interval where its options pass 0. */
+ * over Chebyshev degree {1, 2, 3} x reliable-update factor (inner_tol)
+ * {1e-1, 1e-2, 1e-3, 1e-4} x check interval {1, 2, 4} and keeps the fastest
+ * converged choice; a later solve uses the tuned inner tolerance / check
+ * interval where its options pass 0. */
 #define ASEND_TUNE_SPMV 1
 #define ASEND_TUNE_SOLVER 2
 typedef struct asend_tuning {

@@ -173,8 +173,8 @@ def test_autotuner_choices_are_used(ctx):
     t = ctx.autotune(p.f)
     assert t.spmv_waves_per_block in (1, 2, 4, 8)
     assert all(t.spmv_ms_fp32[k] > 0 for k in range(4))
-    assert t.cheb_degree in (2, 3) and t.inner_tol in (1e-2, 1e-3, 1e-4) and t.check_every in (2, 4, 8)
-    assert t.n_trials == 4 + 18 and t.solve_ms > 0
+    assert t.cheb_degree in (1, 2, 3) and t.inner_tol in (1e-1, 1e-2, 1e-3, 1e-4) and t.check_every in (1, 2, 4)
+    assert t.n_trials == 4 + 36 and t.solve_ms > 0
     # a solve that passes 0 for the inner tolerance and check interval uses the tuned ones
     o = ctx.opts(api.SOLVE_IR_MIXED, api.PC_MG, tol=1e-9, inner_tol=0.0, max_it=100000, max_outer=60,
                  check_every=0)
