@@ -158,38 +158,52 @@ __global__ __launch_bounds__(256) void k_layer0(int n, const float* __restrict__
     H[(int64_t)kHid * node + c] = f2bf(fmaxf(s, 0.f));
 }
 
-// a_i = mean over neighbours (excluding i) of h_j
+// a_i = mean over neighbours (excluding i) of h_j.  16 lanes per node, 4
+// features (8 B) per lane, so a wavefront gathers one 128 B row for each of
+// its 4 nodes per load instruction; four neighbour rows in flight per step,
+// summed in row order.
+constexpr int kAggLanes = kHid / 4;  // lanes per node
 __global__ __launch_bounds__(256) void k_aggregate(int n, const int32_t* __restrict__ rp,
                                                    const int32_t* __restrict__ bc, const bf16* __restrict__ H,
                                                    bf16* __restrict__ A) {
-    int node = blockIdx.x * 4 + threadIdx.x / 64, c = threadIdx.x % 64;
+    const int node = blockIdx.x * (256 / kAggLanes) + threadIdx.x / kAggLanes;
+    const int c = 4 * (threadIdx.x % kAggLanes);
     if (node >= n) return;
-    float s = 0.f;
+    float s[4] = {0.f, 0.f, 0.f, 0.f};
     int deg = 0;
     int k = rp[node];
     const int end = rp[node + 1];
-    // four neighbour rows in flight per step; summed in row order
+    auto row = [&](int j) { return *reinterpret_cast<const uint2*>(H + (int64_t)kHid * j + c); };
+    auto add = [&](uint2 v) {
+        s[0] += __uint_as_float(v.x << 16);
+        s[1] += __uint_as_float(v.x & 0xffff0000u);
+        s[2] += __uint_as_float(v.y << 16);
+        s[3] += __uint_as_float(v.y & 0xffff0000u);
+    };
     for (; k + 4 <= end; k += 4) {
         int j[4];
-        float h[4];
+        uint2 h[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) j[u] = bc[k + u];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) h[u] = (j[u] != node) ? bf2f(H[(int64_t)kHid * j[u] + c]) : 0.f;
+        for (int u = 0; u < 4; ++u) h[u] = (j[u] != node) ? row(j[u]) : make_uint2(0u, 0u);
 #pragma unroll
         for (int u = 0; u < 4; ++u)
             if (j[u] != node) {
-                s += h[u];
+                add(h[u]);
                 ++deg;
             }
     }
     for (; k < end; ++k) {
         const int j = bc[k];
         if (j == node) continue;
-        s += bf2f(H[(int64_t)kHid * j + c]);
+        add(row(j));
         ++deg;
     }
-    A[(int64_t)kHid * node + c] = f2bf(deg ? s / (float)deg : 0.f);
+    uint2 o;
+    o.x = (uint32_t)f2bf(deg ? s[0] / (float)deg : 0.f) | ((uint32_t)f2bf(deg ? s[1] / (float)deg : 0.f) << 16);
+    o.y = (uint32_t)f2bf(deg ? s[2] / (float)deg : 0.f) | ((uint32_t)f2bf(deg ? s[3] / (float)deg : 0.f) << 16);
+    *reinterpret_cast<uint2*>(A + (int64_t)kHid * node + c) = o;
 }
 
 // Node GEMM on MFMA: out[node][col] = sum_k Arow[node][k] W[col][k] (+ epilogue)
@@ -782,7 +796,8 @@ void forward(Learned* lp, hipStream_t st, const LearnedGraph& g) {
                        lp->p + kOffB0, lp->H[0]);
     const int gb = blocks_for(n, 4 * kNodesPerWave);
     for (int l = 0; l < kLayers; ++l) {
-        hipLaunchKernelGGL(k_aggregate, dim3(blocks_for(n, 4)), dim3(256), 0, st, n, g.rp, g.bc, lp->H[l], lp->A[l]);
+        hipLaunchKernelGGL(k_aggregate, dim3(blocks_for(n, 256 / kAggLanes)), dim3(256), 0, st, n, g.rp, g.bc,
+                           lp->H[l], lp->A[l]);
         const int off = kOffL1 + l * kLayerSize;
         hipLaunchKernelGGL((k_node_gemm<2 * kHid, kHid, kHid, 0>), dim3(gb), dim3(256), 0, st, n, lp->H[l], lp->A[l],
                            lp->wbf + off, lp->p + off + kHid * 2 * kHid, (void*)lp->H[l + 1]);
