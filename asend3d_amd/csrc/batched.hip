@@ -92,27 +92,19 @@ __global__ __launch_bounds__(kBlock) void k_sell3_mrhs(int n_slices, const int32
 // Per-system scalars, index s = variant * kNrhs + j
 enum { B_RZ = 0, B_PQ, B_ALPHA, B_BETA, B_RR, B_RR0, B_IT, B_DONE, B_MAXIT, B_TOL2, B_N };
 
-// Chunk reductions: each block owns rows [row0, row1) of one variant and
-// accumulates, per rhs j, the products selected by OP into partials[chunk][2][kNrhs].
-//   OP 0: p.q                    (a = P, b = Q)
-//   OP 1: |r|^2 (FP64 residual)  (a = R)
-template <typename T, int OP>
-__global__ __launch_bounds__(256) void k_chunk_dot(const int32_t* __restrict__ chunk_row,
-                                                   const int32_t* __restrict__ chunk_seg, const T* __restrict__ a,
-                                                   const T* __restrict__ b, const double* __restrict__ sv,
-                                                   double* __restrict__ partials) {
+// Chunk reduction of the FP64 residual: each block owns rows [row0, row1) of
+// one variant and accumulates, per rhs j, |r|^2 into partials[chunk][0][j].
+__global__ __launch_bounds__(256) void k_chunk_rr64(const int32_t* __restrict__ chunk_row,
+                                                    const double* __restrict__ a, double* __restrict__ partials) {
     __shared__ double sh[256];
     const int ch = blockIdx.x;
     const int j = threadIdx.x % kNrhs;
     const int64_t e0 = (int64_t)chunk_row[ch] * 3, e1 = (int64_t)chunk_row[ch + 1] * 3;
-    const int seg = chunk_seg[ch];
     double t = 0.0;
-    if (OP == 1 || sv[(seg * kNrhs + j) * B_N + B_DONE] == 0.0) {
 #pragma unroll 4
-        for (int64_t e = e0 + threadIdx.x / kNrhs; e < e1; e += 256 / kNrhs) {
-            const int64_t i = e * kNrhs + j;
-            t += OP == 0 ? (double)a[i] * b[i] : (double)a[i] * a[i];
-        }
+    for (int64_t e = e0 + threadIdx.x / kNrhs; e < e1; e += 256 / kNrhs) {
+        const int64_t i = e * kNrhs + j;
+        t += a[i] * a[i];
     }
     sh[threadIdx.x] = t;
     __syncthreads();
@@ -455,8 +447,7 @@ void residual64(Batch* b) {
         hipLaunchKernelGGL((k_sell3_mrhs<double, 8, 1, true>), dim3(nblk), dim3(kBlock), 0, b->st, b->n_slices, b->sptr,
                            b->scol, b->K64, b->U, b->R64, b->F, j0, (const int32_t*)nullptr,
                            (const uint8_t*)nullptr);
-    hipLaunchKernelGGL((k_chunk_dot<double, 1>), dim3(b->n_chunks), dim3(256), 0, b->st, b->chunk_row, b->chunk_seg,
-                       b->R64, (const double*)nullptr, b->sv, b->partials);
+    hipLaunchKernelGGL(k_chunk_rr64, dim3(b->n_chunks), dim3(256), 0, b->st, b->chunk_row, b->R64, b->partials);
     hipLaunchKernelGGL(k_sys_fin, dim3(b->n_var), dim3(256), 0, b->st,
                        b->seg_chunk, b->partials, 3, b->sv, b->frozen, b->rr);
 }

@@ -344,7 +344,6 @@ enum FinOp {
     FIN_UPDATE = 3,
     FIN_DOT = 4,
     FIN_MAX = 5,
-    FIN_INIT_WARM = 6,
     FIN_CG1_INIT = 7,
     FIN_CG1 = 8,
     FIN_RU_RES = 9,
@@ -375,7 +374,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ 
     }
     const int nv = (op == FIN_INIT || op == FIN_UPDATE || op == FIN_PQR || op == FIN_FUPD)
                        ? 2
-                       : ((op == FIN_INIT_WARM || op == FIN_CG1 || op == FIN_CG1_RU) ? 3 : 1);
+                       : ((op == FIN_CG1 || op == FIN_CG1_RU) ? 3 : 1);
     double t[3] = {0.0, 0.0, 0.0};
     for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
         t[0] += partials[(size_t)nv * b];
@@ -401,14 +400,6 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ 
             sc->ff = t[1];
             sc->it = 0;
             sc->done = (sc->stop_abs ? (t[0] <= sc->tol) : (t[1] <= sc->tol2 * t[1])) || t[1] == 0.0;
-            break;
-        case FIN_INIT_WARM:
-            // started from a nonzero x: the tolerance stays relative to |b|
-            sc->rz = t[0];
-            sc->rr = t[1];
-            sc->ff = t[2];
-            sc->it = 0;
-            sc->done = (sc->stop_abs ? (t[0] <= sc->tol) : (t[1] <= sc->tol2 * t[2])) || t[2] == 0.0;
             break;
         case FIN_CG1_INIT:
             // single-reduction PCG start: t0 = |b|^2
@@ -842,24 +833,6 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(int64_t n, const T* __restr
     block_partials<2>(t, partials);
 }
 
-// warm start: r = b - K x already formed; p = z = Dinv r; partials of r.z,
-// r.r and b.b
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pcg_init_warm(int64_t n, const T* __restrict__ b,
-                                                          const T* __restrict__ dinv, const T* __restrict__ r,
-                                                          T* __restrict__ p, double* partials) {
-    double t[3] = {0.0, 0.0, 0.0};
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const T ri = r[i];
-        const T zi = dinv[i] * ri;
-        p[i] = zi;
-        t[0] += (double)ri * zi;
-        t[1] += (double)ri * ri;
-        t[2] += (double)b[i] * b[i];
-    }
-    block_partials<3>(t, partials);
-}
-
 // x += alpha p; r -= alpha q; partials of r.(Dinv r) and r.r
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(int64_t n, const T* __restrict__ dinv, T* __restrict__ x,
@@ -902,8 +875,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cg1_init(int64_t n, const T* __restrict__ b, int cold,
                                                      const T* __restrict__ dinv, T* __restrict__ x, T* __restrict__ r,
                                                      T* __restrict__ u, T* __restrict__ p, T* __restrict__ s,
-                                                     double* partials, const Scalars* __restrict__ gate) {
-    if (gate && !gate->gate) return;
+                                                     double* partials) {
     double t[1] = {0.0};
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const T bi = b[i];
@@ -990,7 +962,7 @@ template <typename T>
 void launch_cg1_init(hipStream_t st, int64_t n, const T* b, bool cold, const T* dinv, T* x, T* r, T* u, T* p, T* s,
                      double* partials, Scalars* sc) {
     hipLaunchKernelGGL(k_cg1_init<T>, dim3(kVecBlocks), dim3(kBlock), 0, st, n, b, cold ? 1 : 0, dinv, x, r, u, p, s,
-                       partials, (const Scalars*)nullptr);
+                       partials);
     finalize(st, partials, kVecBlocks, FIN_CG1_INIT, sc);
 }
 
@@ -1078,12 +1050,6 @@ void launch_pcg_init(hipStream_t st, int64_t n, const T* b, const T* dinv, T* x,
     finalize(st, partials, kVecBlocks, FIN_INIT, sc);
 }
 template <typename T>
-void launch_pcg_init_warm(hipStream_t st, int64_t n, const T* b, const T* dinv, const T* r, T* p, double* partials,
-                          Scalars* sc) {
-    hipLaunchKernelGGL(k_pcg_init_warm<T>, dim3(kVecBlocks), dim3(kBlock), 0, st, n, b, dinv, r, p, partials);
-    finalize(st, partials, kVecBlocks, FIN_INIT_WARM, sc);
-}
-template <typename T>
 void launch_pcg_update(hipStream_t st, int64_t n, const T* dinv, T* x, T* r, const T* p, const T* q,
                        double* partials, Scalars* sc) {
     hipLaunchKernelGGL(k_pcg_update<T>, dim3(kVecBlocks), dim3(kBlock), 0, st, n, dinv, x, r, p, q, partials, sc);
@@ -1096,8 +1062,6 @@ void launch_pcg_dir(hipStream_t st, int64_t n, const T* dinv, const T* r, T* p, 
 
 #define INST_PCG(T)                                                                                               \
     template void launch_pcg_init<T>(hipStream_t, int64_t, const T*, const T*, T*, T*, T*, double*, Scalars*);    \
-    template void launch_pcg_init_warm<T>(hipStream_t, int64_t, const T*, const T*, const T*, T*, double*,        \
-                                          Scalars*);                                                              \
     template void launch_pcg_update<T>(hipStream_t, int64_t, const T*, T*, T*, const T*, const T*, double*,       \
                                        Scalars*);                                                                 \
     template void launch_pcg_dir<T>(hipStream_t, int64_t, const T*, const T*, T*, const Scalars*);               \

@@ -93,10 +93,6 @@ void launch_spmv3(hipStream_t st, int mode, int n_rows, const int32_t* rp, const
 template <typename T>
 void launch_pcg_init(hipStream_t st, int64_t n, const T* b, const T* dinv, T* x, T* r, T* p, double* partials,
                      Scalars* sc);
-// warm start: r = b - K x already formed, x kept; tolerance relative to |b|
-template <typename T>
-void launch_pcg_init_warm(hipStream_t st, int64_t n, const T* b, const T* dinv, const T* r, T* p, double* partials,
-                          Scalars* sc);
 template <typename T>
 void launch_pcg_update(hipStream_t st, int64_t n, const T* dinv, T* x, T* r, const T* p, const T* q,
                        double* partials, Scalars* sc);
