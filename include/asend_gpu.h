@@ -48,7 +48,7 @@ extern "C" {
 
 /* solver methods */
 #define ASEND_SOLVE_PCG_FP64 0    /* FP64 PCG on the assembled FP64 operator          */
-#define ASEND_SOLVE_IR_MIXED 1    /* FP32 flexible PCG inside FP64 iterative refinement */
+#define ASEND_SOLVE_IR_MIXED 1    /* FP32 flexible PCG with FP64 reliable updates */
 
 /* preconditioners */
 #define ASEND_PC_JACOBI 0         /* point Jacobi                                       */
@@ -70,14 +70,16 @@ typedef struct asend_solve_opts {
     int stop_rule;        /* ASEND_STOP_*                                    */
     int check_every;      /* iterations between convergence checks (>=1)    */
     double tol;           /* outer tolerance for the stop rule               */
-    double inner_tol;     /* relative residual drop per inner FP32 solve     */
+    double inner_tol;     /* mixed: recursive FP32 residual drop that triggers
+                             an FP64 reliable update (refresh)               */
     int64_t max_it;       /* max total (inner) iterations; <=0: n_dof        */
-    int64_t max_outer;    /* max refinement steps (mixed)                    */
+    int64_t max_outer;    /* mixed: max restarts of the FP32 solve (each
+                             runs through its reliable updates)              */
 } asend_solve_opts;
 
 typedef struct asend_solve_stats {
     int64_t iterations;       /* total Krylov iterations (inner for mixed)   */
-    int64_t outer_iterations; /* refinement steps (mixed), 1 for FP64        */
+    int64_t outer_iterations; /* FP64 residual refreshes (mixed), 1 for FP64 */
     double rel_residual;      /* final ||f - K u|| / ||f|| in FP64            */
     double ref_residual;      /* final g^T M^-1 g of the last Krylov loop     */
     int converged;            /* 1 when the stop rule was met                */
