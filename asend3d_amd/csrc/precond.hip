@@ -1269,12 +1269,27 @@ int mg_set_smoother(Multigrid* mg, int kind, int degree, double lo_frac, double 
     return ASEND_OK;
 }
 
+// Block-Jacobi Chebyshev bound of the finest level, estimated on first use
+// after a setup (host synchronisation: call outside graph capture).
+int fine_bound(Multigrid* mg, hipStream_t st) {
+    if (mg->L.size() < 2 || mg->L[0].lmax > 0.0) return ASEND_OK;
+    int rc = estimate_lmax(mg, st, 0, false, &mg->L[0].lmax);
+    if (rc) return rc;
+    const int slot = coef_slot(mg, 0, 0);
+    cheb_coefficients(mg, mg->L[0].lmax, mg->h_coef.data() + slot);
+    MGCHK(hipMemcpyAsync(mg->d_coef + slot, mg->h_coef.data() + slot, 2 * kChebMaxDegree * sizeof(float),
+                         hipMemcpyHostToDevice, st));
+    MGCHK(hipStreamSynchronize(st));
+    return ASEND_OK;
+}
+
 // Recompute the device Chebyshev coefficients from the stored bounds (after a
 // change of degree or interval that keeps the Jacobi kind).
 int mg_refresh_coefficients(Multigrid* mg, hipStream_t st) {
     if (!mg->ready) return ASEND_OK;
     for (size_t l = 0; l + 1 < mg->L.size(); ++l)
-        cheb_coefficients(mg, mg->L[l].lmax, mg->h_coef.data() + coef_slot(mg, (int)l, 0));
+        if (mg->L[l].lmax > 0.0)  // the finest level's may still be pending
+            cheb_coefficients(mg, mg->L[l].lmax, mg->h_coef.data() + coef_slot(mg, (int)l, 0));
     if (mg->lmax_factor > 0.0)
         cheb_coefficients(mg, mg->lmax_factor, mg->h_coef.data() + coef_slot(mg, (int)mg->L.size(), 0));
     MGCHK(hipMemcpyAsync(mg->d_coef, mg->h_coef.data(), mg->h_coef.size() * sizeof(float), hipMemcpyHostToDevice,
@@ -1455,8 +1470,11 @@ int mg_setup(Multigrid* mg, hipStream_t st, const double* d_E, const double* d_n
                            (const int32_t*)nullptr, c.o_rp, c.d_kdiag, c.o_val, c.binv);
     }
     MGCHK(hipGetLastError());
-    // smoother bounds (all but the coarsest level)
-    for (size_t l = 0; l + 1 < mg->L.size(); ++l) {
+    // smoother bounds (all but the coarsest level).  The finest level's is
+    // estimated on first use (fine_bound): when the learned factor smooths
+    // that level, its block-Jacobi bound is never needed.
+    mg->L[0].lmax = 0.0;
+    for (size_t l = 1; l + 1 < mg->L.size(); ++l) {
         int rc = estimate_lmax(mg, st, (int)l, false, &mg->L[l].lmax);
         if (rc) return rc;
         cheb_coefficients(mg, mg->L[l].lmax, mg->h_coef.data() + coef_slot(mg, (int)l, 0));
@@ -1490,6 +1508,10 @@ int mg_fcg_solve(Multigrid* mg, hipStream_t st, const float* b, float* x, double
                  int check_every, int64_t* it_out, int* fallback_used, int precond, FcgRefresh* ru) {
     if (!mg->ready) return mgfail(mg, ASEND_ERR_STATE, "multigrid not set up");
     if (precond == 1 && !mg->has_factor) return mgfail(mg, ASEND_ERR_STATE, "no learned factor set");
+    if (precond == 0 && mg->fine_smoother == 0) {
+        int rc = fine_bound(mg, st);
+        if (rc) return rc;
+    }
     // z = B r: one V-cycle, or the learned factor alone
     auto apply_b = [&](const float* r, float* z) {
         if (precond == 1) launch_factor_apply(st, mg->factor, r, mg->ft, z);
@@ -1733,6 +1755,10 @@ int mg_prolong(Multigrid* mg, hipStream_t st, int l, const float* d_coarse, floa
 
 int mg_vcycle(Multigrid* mg, hipStream_t st, const float* d_b, float* d_x) {
     if (!mg->ready) return mgfail(mg, ASEND_ERR_STATE, "multigrid not set up");
+    if (mg->fine_smoother == 0) {
+        int rc = fine_bound(mg, st);
+        if (rc) return rc;
+    }
     vcycle(mg, st, 0, d_b, d_x);
     return ASEND_OK;
 }
