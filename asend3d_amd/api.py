@@ -91,6 +91,20 @@ class Tuning(ctypes.Structure):
         return d
 
 
+class NlOpts(ctypes.Structure):
+    _fields_ = [("load_steps", ctypes.c_int), ("max_newton", ctypes.c_int), ("du_tol", ctypes.c_double)]
+
+
+class NlStats(ctypes.Structure):
+    _fields_ = [("load_steps", ctypes.c_int), ("newton_iterations", ctypes.c_int),
+                ("max_step_newton", ctypes.c_int), ("unconverged_steps", ctypes.c_int),
+                ("linear_iterations", ctypes.c_int64), ("unconverged_solves", ctypes.c_int),
+                ("last_du", ctypes.c_double), ("device_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class SolveStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int64), ("outer_iterations", ctypes.c_int64),
                 ("rel_residual", ctypes.c_double), ("ref_residual", ctypes.c_double),
@@ -125,7 +139,8 @@ EXPORTED = ["asend_ctx_create", "asend_ctx_destroy", "asend_last_error", "asend_
             "asend_learned_set_params", "asend_learned_train", "asend_learned_eval", "asend_learned_apply",
             "asend_learned_edges", "asend_mg_set_smoother", "asend_learned_vcycle_eval", "asend_set_graphs",
             "asend_set_rayleigh_damping", "asend_newmark_set_solver", "asend_newmark_setup", "asend_newmark_run", "asend_newmark_adjoint", "asend_modal", "asend_batch_create", "asend_batch_destroy",
-            "asend_batch_error", "asend_batch_solve", "asend_autotune"]
+            "asend_batch_error", "asend_batch_solve", "asend_autotune", "asend_nl_assemble",
+            "asend_static_nonlinear"]
 
 
 def lib():
@@ -324,6 +339,25 @@ class Context:
         self._chk(_lib.asend_static_compliance(self.h, _p(f), _p(u), _p(s), ctypes.byref(J), ctypes.byref(opts),
                                                ctypes.byref(st)), "asend_static_compliance")
         return u, s, float(J.value), st
+
+    def nl_assemble(self, u):
+        """Assemble the nonlinear tangent at u (with the penalty) as the
+        context operator; returns the internal force r(u)."""
+        u = _f64(u)
+        r = np.zeros(self.n_dof)
+        self._chk(_lib.asend_nl_assemble(self.h, _p(u), _p(r)), "asend_nl_assemble")
+        return r
+
+    def static_nonlinear(self, f, opts, load_steps=1, max_newton=50, du_tol=1e-12, u0=None):
+        """Geometrically nonlinear static solve (Newton under the quadratic
+        load ramp); returns u and NlStats."""
+        f = _f64(f)
+        u = np.zeros(self.n_dof) if u0 is None else np.array(u0, dtype=np.float64)
+        nl = NlOpts(int(load_steps), int(max_newton), float(du_tol))
+        st = NlStats()
+        self._chk(_lib.asend_static_nonlinear(self.h, _p(f), _p(u), ctypes.byref(nl), ctypes.byref(opts),
+                                              ctypes.byref(st)), "asend_static_nonlinear")
+        return u, st
 
     def bench_spmv(self, precision=32, n_iter=50):
         ms = _dbl(0)

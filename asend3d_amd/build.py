@@ -16,7 +16,7 @@ CSRC = os.path.join(PKG, "csrc")
 OUT = os.path.join(PKG, "libasend_gpu.so")
 BUILD = os.path.join(ROOT, "_build")
 
-HIP_SOURCES = ["kernels.hip", "precond.hip", "learned.hip", "dynamics.hip", "batched.hip", "shell.hip", "brickim.hip", "modal.hip", "context.hip"]
+HIP_SOURCES = ["kernels.hip", "precond.hip", "learned.hip", "dynamics.hip", "batched.hip", "shell.hip", "brickim.hip", "modal.hip", "nonlinear.hip", "context.hip"]
 CXX_SOURCES = ["host_prep.cpp"]
 OFFLOAD_ARCH = "gfx950"
 

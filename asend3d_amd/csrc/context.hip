@@ -91,6 +91,10 @@ struct asend_ctx {
     // learned preconditioner: network, graph data, and the assembly its factor belongs to
     Learned* learned = nullptr;
     int32_t *d_n2e_ptr = nullptr, *d_n2e = nullptr;
+    int max_valence = 0;  // most elements sharing one node
+    // geometrically nonlinear statics: element tangents (576 per element),
+    // element residuals (24), displacement state, applied load, internal force
+    double *d_nl_ke = nullptr, *d_nl_re = nullptr, *d_nl_u = nullptr, *d_nl_f = nullptr, *d_nl_r = nullptr;
     uint8_t* d_clamped = nullptr;
     int32_t *d_enb = nullptr, *d_erev = nullptr, *d_eslot = nullptr;
     int n_edge = 0;
@@ -293,6 +297,11 @@ void free_mesh(asend_ctx* ctx) {
     newmark_free(ctx);
     dfree(ctx->d_n2e_ptr);
     dfree(ctx->d_n2e);
+    dfree(ctx->d_nl_ke);
+    dfree(ctx->d_nl_re);
+    dfree(ctx->d_nl_u);
+    dfree(ctx->d_nl_f);
+    dfree(ctx->d_nl_r);
     dfree(ctx->d_clamped);
     dfree(ctx->d_enb);
     dfree(ctx->d_erev);
@@ -722,13 +731,11 @@ void launch_sens(asend_ctx* ctx, const double* u, const double* lam) {
                          ctx->d_sens);
 }
 
-// K_eff = k_scale K + c_mass M (+ penalty), its Jacobi diagonal and FP32 copy
-int assemble_device(asend_ctx* ctx, double k_scale, double c_mass) {
+// Penalty scale (fixed on the first assembly), penalty diagonal, Jacobi
+// diagonal and FP32 copy of the element operator held in d_K64
+int finish_assembly(asend_ctx* ctx, double k_scale, double c_mass) {
     const int64_t nval = ctx->nval();
-    if (ctx->etype == ASEND_SHELL4 && !ctx->thk_set)
-        return fail(ctx, ASEND_ERR_STATE, "shell4 mesh without thickness (asend_set_thickness)");
-    int rc = integrate_elements(ctx, ctx->d_E, k_scale, c_mass, ctx->d_K64);
-    if (rc) return rc;
+    int rc;
     if (ctx->n_pen > 0) {
         if (ctx->h_pen.back() >= ctx->n_dof) return fail(ctx, ASEND_ERR_STATE, "penalty DOF beyond the mesh");
         if (!ctx->sf_fixed) {
@@ -756,6 +763,55 @@ int assemble_device(asend_ctx* ctx, double k_scale, double c_mass) {
     ctx->asm_count += 1;
     ctx->asm_cmass = c_mass;
     ctx->asm_kscale = k_scale;
+    return ASEND_OK;
+}
+
+// K_eff = k_scale K + c_mass M (+ penalty), its Jacobi diagonal and FP32 copy
+int assemble_device(asend_ctx* ctx, double k_scale, double c_mass) {
+    if (ctx->etype == ASEND_SHELL4 && !ctx->thk_set)
+        return fail(ctx, ASEND_ERR_STATE, "shell4 mesh without thickness (asend_set_thickness)");
+    int rc = integrate_elements(ctx, ctx->d_E, k_scale, c_mass, ctx->d_K64);
+    if (rc) return rc;
+    return finish_assembly(ctx, k_scale, c_mass);
+}
+
+// Geometrically nonlinear brick8: tangent of the internal force at the
+// displacement state d_nl_u into d_K64 (then penalty, diagonal, FP32 copy),
+// the internal force into d_nl_r and, when rhs is set, the Newton load
+// rhs = fact f - r(u) - sf P u (build_elastic_soln_load(true) with
+// nonlinear_geom, S/model_analysis.rs:179-201, and build_elastic_const_load,
+// :275-287).
+// Mesh checks and buffers of the nonlinear path (allocated on first use).
+int nl_prepare(asend_ctx* ctx) {
+    if (!ctx->d_K64) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
+    if (ctx->etype != ASEND_BRICK8)
+        return fail(ctx, ASEND_ERR_UNSUPPORTED, "geometrically nonlinear analysis is implemented for brick8 meshes");
+    if (ctx->max_valence > 8)
+        return fail(ctx, ASEND_ERR_UNSUPPORTED, "nonlinear assembly supports at most 8 elements per node (mesh has %d)",
+                    ctx->max_valence);
+    int rc;
+    const int64_t n = ctx->n_dof;
+    if (!ctx->d_nl_ke &&
+        ((rc = dalloc(ctx, &ctx->d_nl_ke, 576 * ctx->n_el)) || (rc = dalloc(ctx, &ctx->d_nl_re, 24 * ctx->n_el)) ||
+         (rc = dalloc(ctx, &ctx->d_nl_u, n)) || (rc = dalloc(ctx, &ctx->d_nl_f, n)) ||
+         (rc = dalloc(ctx, &ctx->d_nl_r, n))))
+        return rc;
+    return ASEND_OK;
+}
+
+int nl_assemble(asend_ctx* ctx, double fact, double* rhs) {
+    int rc;
+    launch_hex8_nl_element(ctx->st, (int)ctx->n_el, ctx->d_conn_r, ctx->d_xyz_r, ctx->d_E, ctx->d_nu, ctx->d_nl_u,
+                           ctx->d_nl_ke, ctx->d_nl_re);
+    launch_nl_gather(ctx->st, (int)ctx->n_nodes, (int)ctx->n_slices, ctx->d_sptr, ctx->d_scol, ctx->d_rp,
+                     ctx->d_n2e_ptr, ctx->d_n2e, ctx->d_conn_r, ctx->d_nl_ke, ctx->d_nl_re, fact, ctx->d_nl_f,
+                     ctx->d_K64, ctx->d_nl_r, rhs);
+    HIPCHK(hipGetLastError());
+    if ((rc = finish_assembly(ctx, 1.0, 0.0))) return rc;
+    if (rhs) {
+        launch_nl_penalty_load(ctx->st, ctx->d_pen, ctx->d_pen_mult, ctx->n_pen, ctx->d_sc + 2, ctx->d_nl_u, rhs);
+        HIPCHK(hipGetLastError());
+    }
     return ASEND_OK;
 }
 
@@ -900,6 +956,8 @@ int asend_upload_mesh(asend_ctx* ctx, int elem_type, int64_t n_mesh_nodes, const
         for (int64_t e = 0; e < n_el; ++e)
             for (int a = 0; a < 8; ++a) n2e[fill[conn_r[8 * e + a]]++] = (int32_t)e;
     }
+    ctx->max_valence = 0;
+    for (int64_t r = 0; r < n_nodes; ++r) ctx->max_valence = std::max(ctx->max_valence, n2e_ptr[r + 1] - n2e_ptr[r]);
     // brick8 meshes whose elements are all translates of the first one (to
     // 1e-12 of the mesh extent) with one Poisson ratio and at most 8 elements
     // per node assemble from scaled reference blocks
@@ -1343,6 +1401,74 @@ int asend_static_compliance(asend_ctx* ctx, const double* f, double* u, double* 
     if (u) HIPCHK(hipMemcpy(u, ctx->d_u, n * 8, hipMemcpyDeviceToHost));
     if (dJdE) HIPCHK(hipMemcpy(dJdE, ctx->d_sens, ctx->n_el * 8, hipMemcpyDeviceToHost));
     if (stats) *stats = s;
+    return ASEND_OK;
+}
+
+int asend_nl_assemble(asend_ctx* ctx, const double* u, double* r_int) {
+    if (!ctx || !u) return ASEND_ERR_ARG;
+    int rc;
+    if ((rc = nl_prepare(ctx))) return rc;
+    HIPCHK(hipMemcpy(ctx->d_nl_u, u, ctx->n_dof * 8, hipMemcpyHostToDevice));
+    if ((rc = nl_assemble(ctx, 0.0, nullptr))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    if (r_int) HIPCHK(hipMemcpy(r_int, ctx->d_nl_r, ctx->n_dof * 8, hipMemcpyDeviceToHost));
+    return ASEND_OK;
+}
+
+int asend_static_nonlinear(asend_ctx* ctx, const double* f, double* u, const asend_nl_opts* nl,
+                           const asend_solve_opts* opts, asend_nl_stats* stats) {
+    if (!ctx || !f || !u || !opts) return ASEND_ERR_ARG;
+    int rc;
+    if ((rc = nl_prepare(ctx))) return rc;
+    asend_nl_opts o;
+    o.load_steps = 1;
+    o.max_newton = 50;
+    o.du_tol = 1e-12;
+    if (nl) {
+        if (nl->load_steps > 0) o.load_steps = nl->load_steps;
+        if (nl->max_newton > 0) o.max_newton = nl->max_newton;
+        if (nl->du_tol > 0) o.du_tol = nl->du_tol;
+    }
+    const int64_t n = ctx->n_dof;
+    HIPCHK(hipMemcpy(ctx->d_nl_u, u, n * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_nl_f, f, n * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->st));
+    asend_nl_stats st;
+    std::memset(&st, 0, sizeof(st));
+    st.load_steps = o.load_steps;
+    const int L = o.load_steps;
+    for (int s = 0; s < L; ++s) {
+        // load ramp of a static nonlinear solve (S/model_analysis.rs:636-643)
+        const double q = (double)(L - 1 - s);
+        const double fact = 1.0 - q * q / ((double)L * (double)L);
+        int it = 0;
+        bool done = false;
+        while (it < o.max_newton && !done) {
+            if ((rc = nl_assemble(ctx, fact, ctx->d_f))) return rc;
+            asend_solve_stats ls;
+            if ((rc = solve_device(ctx, opts, &ls))) return rc;
+            st.linear_iterations += ls.iterations;
+            if (!ls.converged) st.unconverged_solves += 1;
+            launch_axpy64(ctx->st, n, 1.0, ctx->d_u, ctx->d_nl_u);
+            launch_maxabs(ctx->st, ctx->d_u, n, ctx->d_partials, ctx->d_sc + 2);
+            HIPCHK(hipGetLastError());
+            if ((rc = read_scalars(ctx, 2))) return rc;
+            st.last_du = ctx->h_sc[2].maxabs;
+            ++it;
+            done = st.last_du <= o.du_tol;
+        }
+        st.newton_iterations += it;
+        st.max_step_newton = std::max(st.max_step_newton, it);
+        if (!done) st.unconverged_steps += 1;
+    }
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->st));
+    HIPCHK(hipEventSynchronize(ctx->ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    st.device_ms = ms;
+    HIPCHK(hipMemcpy(u, ctx->d_nl_u, n * 8, hipMemcpyDeviceToHost));
+    if (stats) *stats = st;
     return ASEND_OK;
 }
 

@@ -71,6 +71,18 @@ void launch_shell4_assemble(hipStream_t st, const int32_t* elems, int n_elems, c
                             const double* xyz_r, const double* E, const double* nu, const double* thk,
                             const double* rho, double k_scale, double m_scale, const int32_t* sptr,
                             const uint8_t* slot, double* K);
+// Geometrically nonlinear brick8 (nonlinear.hip): element tangents Ke
+// (576 per element) and residuals Re (24) at displacement u; their row
+// gather into the sliced-ELL operator K with the internal force rint and, if
+// rhs is set, rhs = fact f - rint; the penalty load rhs -= mult sf u.
+void launch_hex8_nl_element(hipStream_t st, int n_el, const int32_t* conn_r, const double* xyz_r, const double* E,
+                            const double* nu, const double* u, double* Ke, double* Re);
+void launch_nl_gather(hipStream_t st, int n_rows, int n_slices, const int32_t* sptr, const int32_t* scol,
+                      const int32_t* rp, const int32_t* n2e_ptr, const int32_t* n2e, const int32_t* conn_r,
+                      const double* Ke, const double* Re, double fact, const double* f, double* K, double* rint,
+                      double* rhs);
+void launch_nl_penalty_load(hipStream_t st, const int64_t* pen, const int32_t* mult, int n_pen, const Scalars* sc,
+                            const double* u, double* rhs);
 void launch_maxabs(hipStream_t st, const double* v, int64_t n, double* partials, Scalars* sc);
 void launch_penalty_diag(hipStream_t st, const int64_t* pen, const int32_t* mult, int n_pen, const int32_t* sptr,
                          const int32_t* kdiag, double* K, const Scalars* sc);

@@ -219,6 +219,50 @@ int asend_element_sens(asend_ctx* ctx, const double* u, const double* lam, doubl
 int asend_static_compliance(asend_ctx* ctx, const double* f, double* u, double* dJdE, double* compliance,
                             const asend_solve_opts* opts, asend_solve_stats* stats);
 
+/* ---- geometrically nonlinear statics (brick8) ---------------------------
+ * Job flags nonlinearGeom and loadRampSteps (S/job.rs:12,17).  Green-Lagrange
+ * strain with the quadratic displacement-gradient term
+ * (S/element_soln_fields.rs:1345-1461), residual and consistent tangent with
+ * its geometric part (S/element_equations.rs:206-418), isotropic material,
+ * dead loads.  Meshes with at most 8 elements per node. */
+typedef struct {
+    int load_steps;  /* L >= 1: load factors 1 - ((L-1-s)/L)^2, s = 0..L-1
+                        (S/model_analysis.rs:636-643); 0 = 1 */
+    int max_newton;  /* Newton iterations per load step; 0 = 50, the reference's */
+    double du_tol;   /* a load step ends when max|du| <= du_tol (absolute);
+                        0 = 1e-12, the reference's */
+} asend_nl_opts;
+
+typedef struct {
+    int load_steps;
+    int newton_iterations;     /* over all load steps */
+    int max_step_newton;       /* most Newton iterations of one load step */
+    int unconverged_steps;     /* load steps that ended at max_newton */
+    int64_t linear_iterations; /* PCG iterations of all Newton solves */
+    int unconverged_solves;    /* Newton solves whose PCG missed its tolerance */
+    double last_du;            /* max|du| of the last Newton iteration */
+    double device_ms;          /* whole solve, HIP events */
+} asend_nl_stats;
+
+/* Tangent stiffness of the internal force at displacement u (n_dof, DOF
+ * order) assembled into the context operator with the penalty (as
+ * asend_assemble does for the linear one, so asend_get_matrix / asend_solve /
+ * asend_spmv see it), and the internal force r(u) into r_int (may be NULL):
+ * build_elastic_soln_load(true) with nonlinearGeom (S/model_analysis.rs:179-201). */
+int asend_nl_assemble(asend_ctx* ctx, const double* u, double* r_int);
+
+/* Static solve with geometric nonlinearity: for each load step, Newton
+ * iterations K_T(u) du = fact f - r(u) - sf P u, u += du, until
+ * max|du| <= du_tol (the elastic branch of Model::solve_step with
+ * nonlinear_geom, S/model_analysis.rs:450-546, under the load ramp of
+ * Model::solve, :633-645).  u: starting displacement on entry (zeros for an
+ * unloaded model), converged displacement on return.  Each Newton solve uses
+ * opts (any method and preconditioner of asend_solve; the tangent is
+ * re-assembled every iteration, so multigrid and learned setups are redone).
+ * The penalty scale is fixed on the context's first assembly. */
+int asend_static_nonlinear(asend_ctx* ctx, const double* f, double* u, const asend_nl_opts* nl,
+                           const asend_solve_opts* opts, asend_nl_stats* stats);
+
 /* Timing helper: average duration (ms) of n_iter back-to-back launches of the
  * SpMV kernel of the given precision, HIP events on the context stream. */
 int asend_bench_spmv(asend_ctx* ctx, int precision, int n_iter, double* avg_ms);
