@@ -16,12 +16,18 @@
   shell    -- Newmark time stepping of a clamped shell4 plate (1 x 1 m,
               256 x 256 elements, t = 5 mm, aluminium) under a 1 ms half-sine
               10 kN centre load, dt = 1e-5 s, 1000 steps, hipGraph steps.
+  nonlinear -- geometrically nonlinear static solve of the 1M-DOF
+              heterogeneous hex8 block (200x40x40, lognormal E) under a 90 MPa
+              tip traction (tip deflection ~10% of the length): Newton under
+              the 5-step quadratic load ramp, each tangent solved by multigrid
+              FP32 FCG with FP64 refreshes.
 
     python -m asend3d_amd.workloads batched [--variants 64] [--sample 2]
     python -m asend3d_amd.workloads newmark [--nx 100 --ny 20 --nz 20 --steps 1000]
     python -m asend3d_amd.workloads shell [--nx 256 --ny 256 --steps 1000]
     python -m asend3d_amd.workloads adjoint [--nx 100 --ny 20 --nz 20 --steps 1000]
     python -m asend3d_amd.workloads modal [--modes 10]
+    python -m asend3d_amd.workloads nonlinear [--nx 200 --ny 40 --nz 40 --load-steps 5]
 
 Each prints one JSON line.
 """
@@ -153,6 +159,32 @@ def modal(n_modes=10, nx=200, ny=40, nz=40):
             "solve_tol": 1e-8}
 
 
+def nonlinear(nx=200, ny=40, nz=40, load_steps=5, traction=90.0, du_tol=1e-6, tol=1e-10, graphs=True):
+    # mm-N-MPa: du_tol 1e-6 mm is ~1e-8 of the tip deflection
+    p = problems.heterogeneous(nx, ny, nz, traction=(0.0, 0.0, -traction))
+    ctx = api.Context(0)
+    problems.upload(ctx, p)
+    ctx.set_grid(nx, ny, nz)
+    ctx.set_graphs(graphs)
+    o = api.Context.opts(api.SOLVE_IR_MIXED, api.PC_MG, tol=tol, inner_tol=1e-2, max_it=200000, max_outer=200,
+                         check_every=4)
+    log(f"nonlinear {nx}x{ny}x{nz}: {p.n_dof} DOF, {load_steps} load steps, traction {traction} MPa")
+    ctx.assemble()
+    lin, lst = ctx.solve(p.f, o)
+    t0 = time.perf_counter()
+    u, st = ctx.static_nonlinear(p.f, o, load_steps=load_steps, du_tol=du_tol)
+    wall = time.perf_counter() - t0
+    tip = p.dof_index[p.mesh.nodes_at_x(nx), 2]
+    out = {"workload": "nonlinear", "mesh": f"{nx}x{ny}x{nz} brick8, lognormal E", "dof": int(p.n_dof),
+           "traction_mpa": traction, "load_steps": load_steps, "du_tol_mm": du_tol, "solve_tol": tol,
+           "tip_w_mm": float(u[tip].mean()), "tip_w_linear_mm": float(lin[tip].mean()),
+           "device_s": st.device_ms / 1e3, "wall_s": wall, "linear_solve_s": lst.device_ms / 1e3,
+           **{k: v for k, v in st.as_dict().items() if k != "device_ms"}}
+    out["ms_per_newton"] = st.device_ms / max(1, st.newton_iterations)
+    ctx.close()
+    return out
+
+
 def shell(nx=256, ny=256, steps=1000, dt=1e-5, graphs=True, tol=1e-8, max_it=2000, load=1.0e4, solver="auto", ru_factor=1e-3):
     p = problems.shell_plate(nx, ny, 1.0, 1.0, 0.005, 70e9, 0.33, 2700.0)
     ctx = api.Context(0)
@@ -186,13 +218,14 @@ def shell(nx=256, ny=256, steps=1000, dt=1e-5, graphs=True, tol=1e-8, max_it=200
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("which", choices=["batched", "newmark", "shell", "adjoint", "modal"])
+    ap.add_argument("which", choices=["batched", "newmark", "shell", "adjoint", "modal", "nonlinear"])
+    ap.add_argument("--load-steps", type=int, default=5)
     ap.add_argument("--modes", type=int, default=10)
     ap.add_argument("--variants", type=int, default=64)
     ap.add_argument("--sample", type=int, default=2)
     ap.add_argument("--nx", type=int, default=None, help="default 100 (newmark), 256 (shell)")
     ap.add_argument("--ny", type=int, default=None, help="default 20 (newmark), 256 (shell)")
-    ap.add_argument("--nz", type=int, default=20)
+    ap.add_argument("--nz", type=int, default=None, help="default 20 (newmark), 40 (nonlinear)")
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--no-graphs", action="store_true")
     ap.add_argument("--solver", choices=["auto", "mixed", "fp64"], default="auto", help="Newmark step solver")
@@ -203,12 +236,14 @@ def main():
         out = batched(a.variants, a.sample)
     elif a.which == "modal":
         out = modal(a.modes)
+    elif a.which == "nonlinear":
+        out = nonlinear(a.nx or 200, a.ny or 40, a.nz or 40, a.load_steps, graphs=not a.no_graphs)
     elif a.which == "adjoint":
-        out = adjoint(a.nx or 100, a.ny or 20, a.nz, a.steps, graphs=not a.no_graphs, solver=a.solver, ru_factor=a.ru_factor)
+        out = adjoint(a.nx or 100, a.ny or 20, a.nz or 20, a.steps, graphs=not a.no_graphs, solver=a.solver, ru_factor=a.ru_factor)
     elif a.which == "shell":
         out = shell(a.nx or 256, a.ny or 256, a.steps, graphs=not a.no_graphs, solver=a.solver, ru_factor=a.ru_factor)
     else:
-        out = newmark(a.nx or 100, a.ny or 20, a.nz, a.steps, graphs=not a.no_graphs, solver=a.solver, ru_factor=a.ru_factor)
+        out = newmark(a.nx or 100, a.ny or 20, a.nz or 20, a.steps, graphs=not a.no_graphs, solver=a.solver, ru_factor=a.ru_factor)
     print(json.dumps(out), flush=True)
 
 
