@@ -92,7 +92,8 @@ class Tuning(ctypes.Structure):
 
 
 class NlOpts(ctypes.Structure):
-    _fields_ = [("load_steps", ctypes.c_int), ("max_newton", ctypes.c_int), ("du_tol", ctypes.c_double)]
+    _fields_ = [("load_steps", ctypes.c_int), ("max_newton", ctypes.c_int), ("du_tol", ctypes.c_double),
+                ("forcing", ctypes.c_double)]
 
 
 class NlStats(ctypes.Structure):
@@ -348,12 +349,13 @@ class Context:
         self._chk(_lib.asend_nl_assemble(self.h, _p(u), _p(r)), "asend_nl_assemble")
         return r
 
-    def static_nonlinear(self, f, opts, load_steps=1, max_newton=50, du_tol=1e-12, u0=None):
+    def static_nonlinear(self, f, opts, load_steps=1, max_newton=50, du_tol=1e-12, u0=None, forcing=0.0):
         """Geometrically nonlinear static solve (Newton under the quadratic
-        load ramp); returns u and NlStats."""
+        load ramp; forcing > 0 makes the Newton solves inexact); returns u and
+        NlStats."""
         f = _f64(f)
         u = np.zeros(self.n_dof) if u0 is None else np.array(u0, dtype=np.float64)
-        nl = NlOpts(int(load_steps), int(max_newton), float(du_tol))
+        nl = NlOpts(int(load_steps), int(max_newton), float(du_tol), float(forcing))
         st = NlStats()
         self._chk(_lib.asend_static_nonlinear(self.h, _p(f), _p(u), ctypes.byref(nl), ctypes.byref(opts),
                                               ctypes.byref(st)), "asend_static_nonlinear")

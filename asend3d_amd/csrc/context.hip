@@ -1424,11 +1424,14 @@ int asend_static_nonlinear(asend_ctx* ctx, const double* f, double* u, const ase
     o.load_steps = 1;
     o.max_newton = 50;
     o.du_tol = 1e-12;
+    o.forcing = 0.0;
     if (nl) {
         if (nl->load_steps > 0) o.load_steps = nl->load_steps;
         if (nl->max_newton > 0) o.max_newton = nl->max_newton;
         if (nl->du_tol > 0) o.du_tol = nl->du_tol;
+        if (nl->forcing > 0) o.forcing = std::min(nl->forcing, 0.5);
     }
+    asend_solve_opts lo = *opts;  // per-iteration tolerance under a forcing term
     const int64_t n = ctx->n_dof;
     HIPCHK(hipMemcpy(ctx->d_nl_u, u, n * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->d_nl_f, f, n * 8, hipMemcpyHostToDevice));
@@ -1444,10 +1447,26 @@ int asend_static_nonlinear(asend_ctx* ctx, const double* f, double* u, const ase
         const double fact = 1.0 - q * q / ((double)L * (double)L);
         int it = 0;
         bool done = false;
+        double eta = o.forcing, load_prev = 0.0;
         while (it < o.max_newton && !done) {
             if ((rc = nl_assemble(ctx, fact, ctx->d_f))) return rc;
+            if (o.forcing > 0) {
+                // Eisenstat-Walker choice 2 (gamma 0.9, exponent 2) with its
+                // safeguard; never tighter than opts->tol
+                launch_dot64(ctx->st, n, ctx->d_f, ctx->d_f, ctx->d_partials, ctx->d_sc + 2);
+                HIPCHK(hipGetLastError());
+                if ((rc = read_scalars(ctx, 2))) return rc;
+                const double load = std::sqrt(ctx->h_sc[2].dot);
+                if (it > 0 && load_prev > 0) {
+                    const double ratio = load / load_prev;
+                    const double e_new = 0.9 * ratio * ratio, e_safe = 0.9 * eta * eta;
+                    eta = std::min(o.forcing, e_safe > 0.1 ? std::max(e_new, e_safe) : e_new);
+                }
+                load_prev = load;
+                lo.tol = std::max(eta, opts->tol);
+            }
             asend_solve_stats ls;
-            if ((rc = solve_device(ctx, opts, &ls))) return rc;
+            if ((rc = solve_device(ctx, &lo, &ls))) return rc;
             st.linear_iterations += ls.iterations;
             if (!ls.converged) st.unconverged_solves += 1;
             launch_axpy64(ctx->st, n, 1.0, ctx->d_u, ctx->d_nl_u);

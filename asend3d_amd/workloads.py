@@ -20,7 +20,8 @@
               heterogeneous hex8 block (200x40x40, lognormal E) under a 90 MPa
               tip traction (tip deflection ~10% of the length): Newton under
               the 5-step quadratic load ramp, each tangent solved by multigrid
-              FP32 FCG with FP64 refreshes.
+              FP32 FCG with FP64 refreshes; then the same with inexact
+              Newton solves (Eisenstat-Walker forcing).
 
     python -m asend3d_amd.workloads batched [--variants 64] [--sample 2]
     python -m asend3d_amd.workloads newmark [--nx 100 --ny 20 --nz 20 --steps 1000]
@@ -181,6 +182,13 @@ def nonlinear(nx=200, ny=40, nz=40, load_steps=5, traction=90.0, du_tol=1e-6, to
            "device_s": st.device_ms / 1e3, "wall_s": wall, "linear_solve_s": lst.device_ms / 1e3,
            **{k: v for k, v in st.as_dict().items() if k != "device_ms"}}
     out["ms_per_newton"] = st.device_ms / max(1, st.newton_iterations)
+    # inexact Newton (Eisenstat-Walker forcing, first solves to 0.1)
+    ui, sti = ctx.static_nonlinear(p.f, o, load_steps=load_steps, du_tol=du_tol, forcing=0.1)
+    out["inexact_newton"] = {"forcing": 0.1, "device_s": sti.device_ms / 1e3,
+                             "newton_iterations": int(sti.newton_iterations),
+                             "linear_iterations": int(sti.linear_iterations),
+                             "unconverged_steps": int(sti.unconverged_steps),
+                             "max_abs_du_vs_exact_mm": float(np.abs(ui - u).max())}
     ctx.close()
     return out
 

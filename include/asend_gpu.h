@@ -231,6 +231,13 @@ typedef struct {
     int max_newton;  /* Newton iterations per load step; 0 = 50, the reference's */
     double du_tol;   /* a load step ends when max|du| <= du_tol (absolute);
                         0 = 1e-12, the reference's */
+    double forcing;  /* 0: every Newton solve to opts->tol (the reference's
+                        full solves).  > 0: inexact Newton, the first solve of
+                        a load step to this relative tolerance (at most 0.5),
+                        later ones by Eisenstat-Walker choice 2 (0.9 times the
+                        squared ratio of successive load norms, safeguarded),
+                        never tighter than opts->tol.  No reference
+                        counterpart; the stopping rule is unchanged. */
 } asend_nl_opts;
 
 typedef struct {

@@ -110,6 +110,11 @@ def test_newton_with_multigrid_mixed_solves(ctx):
     u_o, iters, steps, sf = oracle_newton(p, load_steps=2)
     assert np.abs(u - u_o).max() <= 1e-8 * np.abs(u_o).max()
     assert st.unconverged_steps == 0 and st.unconverged_solves == 0
+    # inexact Newton: same fixed point, fewer Krylov iterations
+    ui, sti = ctx.static_nonlinear(p.f, opts, load_steps=2, forcing=0.1)
+    assert np.abs(ui - u_o).max() <= 1e-8 * np.abs(u_o).max()
+    assert sti.unconverged_steps == 0 and sti.last_du <= 1e-12
+    assert sti.linear_iterations < st.linear_iterations
 
 
 def test_uniaxial_stretch_closed_form(ctx):
