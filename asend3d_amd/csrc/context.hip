@@ -92,6 +92,10 @@ struct asend_ctx {
     Learned* learned = nullptr;
     int32_t *d_n2e_ptr = nullptr, *d_n2e = nullptr;
     int max_valence = 0;  // most elements sharing one node
+    // set inside a nonlinear solve after its first Newton iteration: a new
+    // assembly changes only the finest multigrid level (the coarse levels are
+    // integrated from the moduli, which a Newton iteration does not change)
+    bool mg_fine_only = false;
     // geometrically nonlinear statics: element tangents (576 per element),
     // element residuals (24), displacement state, applied load, internal force
     double *d_nl_ke = nullptr, *d_nl_re = nullptr, *d_nl_u = nullptr, *d_nl_f = nullptr, *d_nl_r = nullptr;
@@ -508,7 +512,14 @@ int ensure_mg(asend_ctx* ctx) {
     if (!ctx->mg) return fail(ctx, ASEND_ERR_STATE, "multigrid preconditioner not built (asend_set_grid)");
     if (!ctx->assembled) return fail(ctx, ASEND_ERR_STATE, "assemble first");
     if (ctx->mg_asm == ctx->asm_count && mg_ready(ctx->mg)) return ASEND_OK;
-    int rc = mg_setup(ctx->mg, ctx->st, ctx->d_E, ctx->d_nu, ctx->d_K32, ctx->d_dinv32, ctx->d_kdiag, ctx->sf,
+    int rc;
+    if (ctx->mg_fine_only && mg_ready(ctx->mg)) {
+        if ((rc = mg_refresh_fine(ctx->mg, ctx->st, ctx->d_K32, ctx->d_dinv32, ctx->d_kdiag)))
+            return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
+        ctx->mg_asm = ctx->asm_count;
+        return ASEND_OK;
+    }
+    rc = mg_setup(ctx->mg, ctx->st, ctx->d_E, ctx->d_nu, ctx->d_K32, ctx->d_dinv32, ctx->d_kdiag, ctx->sf,
                       ctx->n_pen, ctx->h_pen.data(), ctx->h_pen_mult.data());
     if (rc) return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
     ctx->mg_asm = ctx->asm_count;
@@ -1432,6 +1443,10 @@ int asend_static_nonlinear(asend_ctx* ctx, const double* f, double* u, const ase
         if (nl->forcing > 0) o.forcing = std::min(nl->forcing, 0.5);
     }
     asend_solve_opts lo = *opts;  // per-iteration tolerance under a forcing term
+    struct FineOnly {
+        asend_ctx* c;
+        ~FineOnly() { c->mg_fine_only = false; }
+    } fine_only{ctx};
     const int64_t n = ctx->n_dof;
     HIPCHK(hipMemcpy(ctx->d_nl_u, u, n * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->d_nl_f, f, n * 8, hipMemcpyHostToDevice));
@@ -1467,6 +1482,7 @@ int asend_static_nonlinear(asend_ctx* ctx, const double* f, double* u, const ase
             }
             asend_solve_stats ls;
             if ((rc = solve_device(ctx, &lo, &ls))) return rc;
+            ctx->mg_fine_only = true;
             st.linear_iterations += ls.iterations;
             if (!ls.converged) st.unconverged_solves += 1;
             launch_axpy64(ctx->st, n, 1.0, ctx->d_u, ctx->d_nl_u);

@@ -1298,6 +1298,19 @@ int mg_refresh_coefficients(Multigrid* mg, hipStream_t st) {
     return ASEND_OK;
 }
 
+int mg_refresh_fine(Multigrid* mg, hipStream_t st, const float* d_K32, const float* d_dinv32,
+                    const int32_t* d_kdiag0) {
+    if (!mg->ready) return ASEND_ERR_STATE;
+    Level& l0 = mg->L[0];
+    l0.d_val = d_K32;
+    l0.d_dinv = d_dinv32;
+    hipLaunchKernelGGL(k_binv32, dim3((unsigned)((l0.n_nodes + 255) / 256)), dim3(256), 0, st, (int)l0.n_nodes,
+                       mg->fine.sptr, (const int32_t*)nullptr, d_kdiag0, d_K32, l0.binv);
+    MGCHK(hipGetLastError());
+    l0.lmax = 0.0;
+    return ASEND_OK;
+}
+
 void mg_set_fine_geometry(Multigrid* mg, int waves_per_block) { mg->fine.waves_per_block = waves_per_block; }
 
 int mg_smoother_degree(const Multigrid* mg) { return mg->cheb_degree;

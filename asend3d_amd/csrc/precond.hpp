@@ -39,6 +39,11 @@ int mg_set_smoother(Multigrid* mg, int kind, int degree, double lo_frac, double 
 // Re-derive the Chebyshev step coefficients after mg_set_smoother changed the
 // degree or interval (same Jacobi kind: the stored bounds stay valid).
 int mg_refresh_coefficients(Multigrid* mg, hipStream_t st);
+// New finest-level operator values on an unchanged hierarchy (same moduli,
+// penalty and pattern below it): block-Jacobi inverse recomputed, Chebyshev
+// bound re-estimated on first use, coarse levels kept.
+int mg_refresh_fine(Multigrid* mg, hipStream_t st, const float* d_K32, const float* d_dinv32,
+                    const int32_t* d_kdiag0);
 int mg_smoother_degree(const Multigrid* mg);
 // Workgroup geometry of the finest-level SpMV (waves per workgroup).
 void mg_set_fine_geometry(Multigrid* mg, int waves_per_block);
