@@ -145,6 +145,46 @@ def test_uniaxial_stretch_closed_form(ctx):
     assert abs(e - t / Y) > 1e-3
 
 
+def rotation(axis, deg):
+    a = np.asarray(axis, float) / np.linalg.norm(axis)
+    t = np.radians(deg)
+    Kx = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    return np.eye(3) + np.sin(t) * Kx + (1 - np.cos(t)) * Kx @ Kx
+
+
+def test_full_size_rigid_rotation_and_equilibrium(ctx):
+    # size-independent checks on the 1M-DOF block: a finite rigid rotation
+    # carries no internal force, and the Newton solution is in equilibrium
+    gpu_case("nonlinear 200x40x40: rigid rotation, equilibrium of a 2-step solve")
+    p = problems.config2()
+    problems.upload(ctx, p)
+    X = p.mesh.xyz
+    Q = rotation([0.3, 1.0, -0.2], 35.0)
+    d = X @ Q.T - X
+    u = np.zeros(p.n_dof)
+    for c in range(3):
+        u[p.dof_index[:, c]] = d[:, c]
+    ctx.assemble()
+    free = np.ones(p.n_dof, bool)
+    free[p.pen_dofs] = False
+    lin = np.abs(ctx.spmv(u)[free]).max()      # linear internal force of the rotation
+    r = ctx.nl_assemble(u)
+    assert np.abs(r).max() <= 1e-9 * lin
+    ctx.set_grid(200, 40, 40)
+    f = 60.0 * p.f
+    opts = api.Context.opts(method=api.SOLVE_IR_MIXED, precond=api.PC_MG, tol=1e-10, inner_tol=1e-2,
+                            max_outer=200, check_every=4)
+    un, st = ctx.static_nonlinear(f, opts, load_steps=2, du_tol=1e-6, forcing=0.1)
+    assert st.unconverged_steps == 0
+    ri = ctx.nl_assemble(un)
+    pd = np.zeros(p.n_dof)
+    np.add.at(pd, p.pen_dofs, 1.0)
+    res = f - ri - ctx.penalty_scale() * pd * un
+    assert np.linalg.norm(res) <= 1e-8 * np.linalg.norm(f)
+    tip = p.dof_index[p.mesh.nodes_at_x(200), 2]
+    assert un[tip].mean() < -50.0                 # large deflection (mm, length 1000 mm)
+
+
 def test_nonlinear_rejects_other_elements(ctx):
     gpu_case("nonlinear error path brickIM")
     p = problems.cantilever(4, 2, 2, etype=81)
