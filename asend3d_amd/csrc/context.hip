@@ -1440,7 +1440,8 @@ int asend_static_nonlinear(asend_ctx* ctx, const double* f, double* u, const ase
         if (nl->load_steps > 0) o.load_steps = nl->load_steps;
         if (nl->max_newton > 0) o.max_newton = nl->max_newton;
         if (nl->du_tol > 0) o.du_tol = nl->du_tol;
-        if (nl->forcing > 0) o.forcing = std::min(nl->forcing, 0.5);
+        // a forcing term is a relative tolerance: it applies to the relative stop rule only
+        if (nl->forcing > 0 && opts->stop_rule == ASEND_STOP_REL_RESIDUAL) o.forcing = std::min(nl->forcing, 0.5);
     }
     asend_solve_opts lo = *opts;  // per-iteration tolerance under a forcing term
     struct FineOnly {

@@ -236,8 +236,10 @@ typedef struct {
                         a load step to this relative tolerance (at most 0.5),
                         later ones by Eisenstat-Walker choice 2 (0.9 times the
                         squared ratio of successive load norms, safeguarded),
-                        never tighter than opts->tol.  No reference
-                        counterpart; the stopping rule is unchanged. */
+                        never tighter than opts->tol.  Ignored unless
+                        opts->stop_rule is ASEND_STOP_REL_RESIDUAL.  No
+                        reference counterpart; the stopping rule is
+                        unchanged. */
 } asend_nl_opts;
 
 typedef struct {
