@@ -68,6 +68,23 @@ def cpu_baseline(prob, fp64_iters, nx, ny, nz):
     t0 = time.perf_counter()
     O.solid_sens(8, sl.mesh.xyz, sl.mesh.conn, sl.E, sl.nu, sl.dof_index, u, u)
     t_sens = time.perf_counter() - t0
+    # the same PCG with OpenMP over the host's threads (at most 16), for context
+    n_thr = max(1, min(16, os.cpu_count() or 1))
+    O.set_threads(n_thr)
+    n_omp = 300
+    t0 = time.perf_counter()
+    O.jacobi_pcg(K, sl.pen_dofs, sf, sl.f, 1e-30, n_omp)
+    t_omp = (time.perf_counter() - t0) / n_omp
+    O.set_threads(1)
+    cpu_model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     scale = prob.n_dof / sl.n_dof
     scale_el = prob.mesh.n_el / sl.mesh.n_el
     est = t_asm * scale_el + t_it * scale * fp64_iters + t_sens * scale_el
@@ -76,7 +93,9 @@ def cpu_baseline(prob, fp64_iters, nx, ny, nz):
             "sample": (f"oracle on a {sx}x{ny}x{nz} slice ({sl.n_dof} DOF, {sl.mesh.n_el} elements, "
                        f"{t_sample:.1f} s): assembly {t_asm:.2f} s, {n_it} Jacobi PCG iterations "
                        f"{t_it * 1e3:.1f} ms each, element-wise dJ/dE {t_sens:.2f} s; scaled by element ratio "
-                       f"{scale_el:.1f}, DOF ratio {scale:.1f} and {fp64_iters} iterations")}
+                       f"{scale_el:.1f}, DOF ratio {scale:.1f} and {fp64_iters} iterations; host {cpu_model}, "
+                       f"{os.cpu_count()} hardware threads; the PCG iteration with OpenMP on {n_thr} threads takes "
+                       f"{t_omp * 1e3:.2f} ms ({t_it / t_omp:.1f}x the serial one)")}
 
 
 def train_network(ctx, n_meshes, steps):

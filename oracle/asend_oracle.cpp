@@ -571,8 +571,13 @@ inline int64_t find_col(const int64_t* rowptr, const int32_t* cols, int64_t r, i
     return it - cols;
 }
 
+// OpenMP threads of the Jacobi PCG and its SpMV (orc_set_threads; 1 = the
+// serial restatement, whose sums run in index order)
+int g_threads = 1;
+
 void spmv_pen(int64_t n, const int64_t* rowptr, const int32_t* cols, const double* vals, int64_t n_pen,
               const int64_t* pen, double sf, const double* x, double* y) {
+#pragma omp parallel for num_threads(g_threads) schedule(static)
     for (int64_t r = 0; r < n; ++r) {
         double acc = 0.0;
         for (int64_t k = rowptr[r]; k < rowptr[r + 1]; ++k) acc += vals[k] * x[cols[k]];
@@ -1046,6 +1051,11 @@ int orc_conj_grad_sparse(int64_t n, const int64_t* rowptr, const int32_t* cols, 
     return 0;
 }
 
+int orc_set_threads(int n) {
+    g_threads = n > 0 ? n : 1;
+    return 0;
+}
+
 int orc_jacobi_pcg(int64_t n, const int64_t* rowptr, const int32_t* cols, const double* vals, int64_t n_pen,
                    const int64_t* pen_dofs, double sf, const double* rhs, double rel_tol, int64_t max_it,
                    double* x, int64_t* it_out, double* relres_out) {
@@ -1064,19 +1074,23 @@ int orc_jacobi_pcg(int64_t n, const int64_t* rowptr, const int32_t* cols, const 
     while (it < max_it && std::sqrt(rr) > rel_tol * fn) {
         spmv_pen(n, rowptr, cols, vals, n_pen, pen_dofs, sf, p.data(), q.data());
         double pq = 0.0;
+#pragma omp parallel for num_threads(g_threads) schedule(static) reduction(+ : pq)
         for (int64_t i = 0; i < n; ++i) pq += p[i] * q[i];
         double alpha = rz / pq;
         double rzn = 0.0;
-        rr = 0.0;
+        double rr_new = 0.0;
+#pragma omp parallel for num_threads(g_threads) schedule(static) reduction(+ : rzn, rr_new)
         for (int64_t i = 0; i < n; ++i) {
             x[i] += alpha * p[i];
             r[i] -= alpha * q[i];
             z[i] = dinv[i] * r[i];
             rzn += r[i] * z[i];
-            rr += r[i] * r[i];
+            rr_new += r[i] * r[i];
         }
+        rr = rr_new;
         double beta = rzn / rz;
         rz = rzn;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
         for (int64_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
         ++it;
     }

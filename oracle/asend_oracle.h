@@ -133,6 +133,10 @@ int orc_conj_grad_sparse(int64_t n, const int64_t* rowptr, const int32_t* cols, 
 /* Plain Jacobi PCG on K + sf*C^T C stopping on ||f - K x||_2 / ||f||_2 <= rel_tol
  * (the metric's stopping rule; not a reference routine, the CPU twin of the
  * device FP64 Jacobi baseline). */
+/* OpenMP threads of orc_jacobi_pcg and orc_spmv_penalty (default 1: the
+ * serial restatement).  For the multi-core CPU baseline only. */
+int orc_set_threads(int n);
+
 int orc_jacobi_pcg(int64_t n, const int64_t* rowptr, const int32_t* cols, const double* vals,
                    int64_t n_pen, const int64_t* pen_dofs, double sf,
                    const double* rhs, double rel_tol, int64_t max_it,

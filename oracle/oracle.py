@@ -41,6 +41,11 @@ def lib():
     return _lib
 
 
+def set_threads(n):
+    """OpenMP threads of jacobi_pcg / spmv_penalty (1 = serial, the default)."""
+    _chk(lib().orc_set_threads(ctypes.c_int(int(n))), "set_threads")
+
+
 def _ptr(a):
     return ctypes.c_void_p(a.ctypes.data) if a is not None else None
 
