@@ -76,6 +76,7 @@ class BatchStats(ctypes.Structure):
 
 TUNE_SPMV = 1
 TUNE_SOLVER = 2
+TUNE_LEARNED = 4  # tune the solver with the learned smoother
 
 
 class Tuning(ctypes.Structure):

@@ -176,11 +176,13 @@ def main():
         ctx.set_grid(args.nx, args.ny, args.nz)
     tuning = None
     if pc != api.PC_JACOBI:
-        # project autotuner: SpMV workgroup geometry and the multigrid solve's
-        # Chebyshev degree / inner tolerance / check interval (setup, not timed)
+        # project autotuner: SpMV workgroup geometry and the Chebyshev degree /
+        # refresh factor / check interval of the solve that will run, with the
+        # learned smoother when it is the fast path (setup, not timed)
         ctx.assemble()
         t0 = time.perf_counter()
-        tn = ctx.autotune(prob.f)
+        what = api.TUNE_SPMV | api.TUNE_SOLVER | (api.TUNE_LEARNED if pc == api.PC_LEARNED else 0)
+        tn = ctx.autotune(prob.f, what)
         tuning = tn.as_dict()
         tuning["tune_s"] = time.perf_counter() - t0
         log(f"autotuned in {tuning['tune_s']:.1f} s: {tn.spmv_waves_per_block} waves/workgroup, Chebyshev degree "

@@ -441,11 +441,14 @@ int asend_set_graphs(asend_ctx* ctx, int on);
  * with 1, 2, 4 and 8 slices (wavefronts) per workgroup and keeps the fastest;
  * ASEND_TUNE_SOLVER times full multigrid-preconditioned mixed solves of rhs
  * over Chebyshev degree {1, 2, 3} x reliable-update factor (inner_tol)
- * {1e-1, 1e-2, 1e-3, 1e-4} x check interval {1, 2, 4} and keeps the fastest
- * converged choice; a later solve uses the tuned inner tolerance / check
- * interval where its options pass 0. */
+ * {1e-1, 1e-2, 1e-3, 1e-4} x check interval {1, 2, 4} (the faster of two
+ * solves each) and keeps the fastest converged choice; a later solve uses the
+ * tuned inner tolerance / check interval where its options pass 0.  With
+ * ASEND_TUNE_LEARNED as well, the tuned solves use the learned smoother
+ * (ASEND_PC_LEARNED), so the choice fits the learned pipeline. */
 #define ASEND_TUNE_SPMV 1
 #define ASEND_TUNE_SOLVER 2
+#define ASEND_TUNE_LEARNED 4
 typedef struct asend_tuning {
     int spmv_waves_per_block;  /* chosen geometry */
     double spmv_ms_fp32[4];    /* measured, for 1, 2, 4, 8 waves (0 = not tried) */

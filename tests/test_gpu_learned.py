@@ -118,6 +118,25 @@ def test_learned_solve_matches_fp64_and_falls_back(ctx):
         assert st.converged and st.fallback_used == 0, (pc, st.as_dict())
         assert np.abs(u - u_ref).max() <= 1e-6 * np.abs(u_ref).max()
         assert abs(J - J_ref) <= 1e-8 * abs(J_ref)
+    # solver tuning with the learned smoother; a solve passing 0 for the
+    # refresh factor and check interval takes the tuned ones
+    ctx.assemble()
+    tn = ctx.autotune(p.f, api.TUNE_SOLVER | api.TUNE_LEARNED)
+    assert tn.n_trials == 36 and tn.inner_tol in (1e-1, 1e-2, 1e-3, 1e-4) and tn.check_every in (1, 2, 4)
+    o = ctx.opts(api.SOLVE_IR_MIXED, api.PC_LEARNED, tol=1e-8, inner_tol=0.0, max_it=100000, max_outer=40,
+                 check_every=0)
+    u, s, J, st = ctx.static_compliance(p.f, o)
+    assert st.converged and st.fallback_used == 0, st.as_dict()
+    assert np.abs(u - u_ref).max() <= 1e-6 * np.abs(u_ref).max()
+    fresh = api.Context(0)
+    try:
+        problems.upload(fresh, p)
+        fresh.set_grid(*dims)
+        fresh.assemble()
+        with pytest.raises(api.AsendError, match="learned"):
+            fresh.autotune(p.f, api.TUNE_SOLVER | api.TUNE_LEARNED)
+    finally:
+        fresh.close()
     # a network with non-finite weights must not take the solve down
     ctx.learned_set_params(np.full(api.learned_num_params(), np.nan, dtype=np.float32))
     o = ctx.opts(api.SOLVE_IR_MIXED, api.PC_LEARNED, tol=1e-8, inner_tol=1e-3, max_it=400000, max_outer=40,
