@@ -85,6 +85,10 @@ struct asend_ctx {
     double* d_partials = nullptr;
     unsigned* d_counter = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // side stream (default priority) for the multigrid setup that overlaps
+    // the learned smoother's inference and bound, with its fork/join events
+    hipStream_t st2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
     Multigrid* mg = nullptr;
 
@@ -541,6 +545,58 @@ int ensure_factor(asend_ctx* ctx, bool smoother) {
     return ASEND_OK;
 }
 
+// Multigrid setup plus the learned smoother (inference and its Chebyshev
+// bound) for the current assembly, overlapped: the hierarchy's small,
+// latency-bound setup kernels run on the side stream while the network and
+// the bound's power iteration (finest-level SpMVs and factor products) run on
+// the context stream.  Sequential when only one of them is due.  A setup
+// failure sets *mg_failed; a failed bound returns ASEND_ERR_NUMERIC with the
+// hierarchy ready (the caller falls back to Jacobi PCG).
+int ensure_mg_and_factor(asend_ctx* ctx, bool* mg_failed) {
+    *mg_failed = false;
+    int rc;
+    const bool need_mg = !(ctx->mg_asm == ctx->asm_count && mg_ready(ctx->mg));
+    const bool need_factor = !(ctx->factor_asm == ctx->asm_count && mg_factor_has_bounds(ctx->mg));
+    if (!need_mg || !need_factor || ctx->mg_fine_only) {
+        if ((rc = ensure_mg(ctx))) {
+            *mg_failed = true;
+            return rc;
+        }
+        return ensure_factor(ctx, true);
+    }
+    if (!ctx->assembled) {
+        *mg_failed = true;
+        return fail(ctx, ASEND_ERR_STATE, "assemble first");
+    }
+    invalidate_factor(ctx);
+    // the side stream starts after everything queued so far (the assembly)
+    HIPCHK(hipEventRecord(ctx->ev_fork, ctx->st));
+    HIPCHK(hipStreamWaitEvent(ctx->st2, ctx->ev_fork, 0));
+    Factor F;
+    if ((rc = learned_infer(ctx->learned, ctx->st, learned_graph(ctx), &F))) {
+        *mg_failed = true;
+        return fail(ctx, rc, "%s", learned_error(ctx->learned));
+    }
+    if ((rc = mg_learned_bound_begin(ctx->mg, ctx->st, F, ctx->d_K32, ctx->d_dinv32))) {
+        *mg_failed = true;
+        return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
+    }
+    rc = mg_setup(ctx->mg, ctx->st2, ctx->d_E, ctx->d_nu, ctx->d_K32, ctx->d_dinv32, ctx->d_kdiag, ctx->sf,
+                  ctx->n_pen, ctx->h_pen.data(), ctx->h_pen_mult.data());
+    // join: nothing later on the context stream runs before the setup ends
+    HIPCHK(hipEventRecord(ctx->ev_join, ctx->st2));
+    HIPCHK(hipStreamWaitEvent(ctx->st, ctx->ev_join, 0));
+    if (rc) {
+        *mg_failed = true;
+        (void)hipStreamSynchronize(ctx->st);
+        return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
+    }
+    ctx->mg_asm = ctx->asm_count;
+    if ((rc = mg_learned_bound_end(ctx->mg, ctx->st))) return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
+    ctx->factor_asm = ctx->asm_count;
+    return ASEND_OK;
+}
+
 int solve_device(asend_ctx* ctx, const asend_solve_opts* o, asend_solve_stats* st) {
     const int64_t n = ctx->n_dof;
     int64_t max_it = o->max_it > 0 ? o->max_it : n;
@@ -636,9 +692,16 @@ int solve_device(asend_ctx* ctx, const asend_solve_opts* o, asend_solve_stats* s
         int64_t it_in = 0;
         const bool learned_pc = (o->precond == ASEND_PC_LEARNED || o->precond == ASEND_PC_LEARNED_PCG);
         bool use_learned = learned_pc && !s.fallback_used;
+        int rc_pre = ASEND_OK;
         if (learned_pc) {
             if (!ctx->learned) return fail(ctx, ASEND_ERR_STATE, "no learned preconditioner (asend_learned_init)");
-            if ((rc = ensure_mg(ctx))) return rc;
+            if (use_learned && o->precond == ASEND_PC_LEARNED) {
+                bool mg_failed = false;
+                rc_pre = ensure_mg_and_factor(ctx, &mg_failed);
+                if (mg_failed) return rc_pre;
+            } else if ((rc = ensure_mg(ctx))) {
+                return rc;
+            }
         }
         if (use_learned) {
             // learned factor as the multigrid smoother (PC_LEARNED) or alone
@@ -646,7 +709,7 @@ int solve_device(asend_ctx* ctx, const asend_solve_opts* o, asend_solve_stats* s
             // if it breaks down or needs more than the inner budget
             const bool smoother = (o->precond == ASEND_PC_LEARNED);
             const int64_t budget = smoother ? kLearnedInnerBudget : kLearnedPcgBudget;
-            const int rc_factor = ensure_factor(ctx, smoother);
+            const int rc_factor = rc_pre ? rc_pre : ensure_factor(ctx, smoother);
             rc = rc_factor;
             if (rc == ASEND_OK) {
                 if (smoother) mg_set_fine_smoother(ctx->mg, 1);
@@ -848,7 +911,10 @@ int asend_ctx_create(int device, asend_ctx** out) {
         hipHostMalloc((void**)&ctx->h_sc, 4 * sizeof(Scalars), hipHostMallocDefault) != hipSuccess ||
         hipMalloc((void**)&ctx->d_partials, kMaxPartials * 2 * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&ctx->d_counter, 16 * sizeof(unsigned)) != hipSuccess ||
-        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->st2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) {
         asend_ctx_destroy(ctx);
         return ASEND_ERR_HIP;
     }
@@ -876,6 +942,10 @@ void asend_ctx_destroy(asend_ctx* ctx) {
     if (ctx->h_sc) (void)hipHostFree(ctx->h_sc);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->st2) (void)hipStreamSynchronize(ctx->st2);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->st2) (void)hipStreamDestroy(ctx->st2);
     if (ctx->st) (void)hipStreamDestroy(ctx->st);
     delete ctx;
 }

@@ -146,6 +146,10 @@ struct Multigrid {
     // FCG work vectors (fine level)
     float *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
     double* partials = nullptr;
+    // the learned smoother's bound estimated on its own stream while the
+    // hierarchy is set up on another (mg_learned_bound_begin/end): its own
+    // reduction scratch and result
+    double *partials_b = nullptr, *sv_b = nullptr, *h_sv_b = nullptr;
     double* sv = nullptr;      // device scalars of the FCG loop
     double* h_sv = nullptr;    // pinned mirror
     // Chebyshev smoother: point (0) or block (1) Jacobi, degree and interval
@@ -1067,15 +1071,14 @@ void vcycle(Multigrid* mg, hipStream_t st, int l, const float* b, float* x) {
 
 // lambda_max of M A by power iteration (M = D^-1, or the learned factor on
 // the finest level when `learned`).
-int estimate_lmax(Multigrid* mg, hipStream_t st, int l, bool learned, double* out) {
+// kPowerIts steps of the power iteration for lambda_max(M A) on level l,
+// enqueued on st with the given reduction scratch; ||M A x||^2 of the last
+// step lands in nrm2[0].
+void enqueue_power(Multigrid* mg, hipStream_t st, int l, bool learned, double* partials, double* nrm2) {
     Level& lv = mg->L[l];
     const int64_t n = lv.n_dof;
-    // deterministic pseudo-random start vector (a smooth start such as the
-    // constant vector lies near the rigid-body modes and converges slowly)
     hipLaunchKernelGGL(k_fill_hash, dim3(nb_for(n)), dim3(256), 0, st, n, lv.x);
     const int nblk = kVecBlocks;
-    // kPowerIts steps without host round trips; ||M A x|| with ||x|| = 1 of the
-    // last step is the estimate
     for (int it = 0; it < kPowerIts; ++it) {
         apply_level(mg, st, l, lv.x, lv.r);
         float* w = lv.r;
@@ -1085,13 +1088,21 @@ int estimate_lmax(Multigrid* mg, hipStream_t st, int l, bool learned, double* ou
         }
         if (!learned && mg->smoother_kind == 1)
             hipLaunchKernelGGL(k_binv_scale_norm, dim3(nblk), dim3(kBlock), 0, st, (int)lv.n_nodes, lv.binv, w,
-                               mg->partials);
+                               partials);
         else
             hipLaunchKernelGGL(k_dinv_scale_norm, dim3(nblk), dim3(kBlock), 0, st, n, learned ? nullptr : lv.d_dinv,
-                               w, mg->partials);
-        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, st, mg->partials, nblk, 1, mg->sv + SV_N);
-        hipLaunchKernelGGL(k_scale_dev, dim3(nb_for(n)), dim3(256), 0, st, n, w, mg->sv + SV_N, lv.x);
+                               w, partials);
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, st, partials, nblk, 1, nrm2);
+        hipLaunchKernelGGL(k_scale_dev, dim3(nb_for(n)), dim3(256), 0, st, n, w, nrm2, lv.x);
     }
+}
+
+int estimate_lmax(Multigrid* mg, hipStream_t st, int l, bool learned, double* out) {
+    // deterministic pseudo-random start vector (a smooth start such as the
+    // constant vector lies near the rigid-body modes and converges slowly);
+    // kPowerIts steps without host round trips, ||M A x|| with ||x|| = 1 of
+    // the last step is the estimate
+    enqueue_power(mg, st, l, learned, mg->partials, mg->sv + SV_N);
     MGCHK(hipGetLastError());
     MGCHK(hipMemcpyAsync(mg->h_sv, mg->sv + SV_N, sizeof(double), hipMemcpyDeviceToHost, st));
     MGCHK(hipStreamSynchronize(st));
@@ -1231,6 +1242,9 @@ void mg_destroy(Multigrid* mg) {
     fr(mg->sv);
     fr(mg->ft);
     fr(mg->fz);
+    fr(mg->partials_b);
+    fr(mg->sv_b);
+    if (mg->h_sv_b) (void)hipHostFree(mg->h_sv_b);
     fr(mg->d_coef);
     for (auto& g : mg->graphs)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
@@ -1348,6 +1362,43 @@ int mg_set_learned_factor(Multigrid* mg, hipStream_t st, const Factor& F, bool s
     if (rc) {
         mg->has_factor = false;
         return rc;
+    }
+    mg->lmax_factor = lam;
+    return upload_factor_coefficients(mg, st);
+}
+
+int mg_learned_bound_begin(Multigrid* mg, hipStream_t st, const Factor& F, const float* d_K32,
+                           const float* d_dinv32) {
+    Level& l0 = mg->L[0];
+    if (F.n != l0.n_nodes) return mgfail(mg, ASEND_ERR_ARG, "factor size does not match the finest level");
+    if (!mg->ft) {
+        MGCHK(hipMalloc((void**)&mg->ft, l0.n_dof * sizeof(float)));
+        MGCHK(hipMalloc((void**)&mg->fz, l0.n_dof * sizeof(float)));
+    }
+    if (!mg->partials_b) {
+        MGCHK(hipMalloc((void**)&mg->partials_b, kVecBlocks * sizeof(double)));
+        MGCHK(hipMalloc((void**)&mg->sv_b, sizeof(double)));
+        MGCHK(hipHostMalloc((void**)&mg->h_sv_b, sizeof(double), hipHostMallocDefault));
+    }
+    l0.d_val = d_K32;
+    l0.d_dinv = d_dinv32;
+    mg->factor = F;
+    mg->has_factor = true;
+    mg->lmax_factor = 0.0;
+    // the finest level's vectors x, r and the factor's ft, fz are not touched
+    // by the setup of the coarser levels
+    enqueue_power(mg, st, 0, true, mg->partials_b, mg->sv_b);
+    MGCHK(hipGetLastError());
+    MGCHK(hipMemcpyAsync(mg->h_sv_b, mg->sv_b, sizeof(double), hipMemcpyDeviceToHost, st));
+    return ASEND_OK;
+}
+
+int mg_learned_bound_end(Multigrid* mg, hipStream_t st) {
+    MGCHK(hipStreamSynchronize(st));
+    const double lam = std::sqrt(mg->h_sv_b[0]);
+    if (!(lam > 0.0) || !std::isfinite(lam)) {
+        mg->has_factor = false;
+        return mgfail(mg, ASEND_ERR_NUMERIC, "power iteration broke down");
     }
     mg->lmax_factor = lam;
     return upload_factor_coefficients(mg, st);

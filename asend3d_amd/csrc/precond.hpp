@@ -51,6 +51,14 @@ void mg_set_fine_geometry(Multigrid* mg, int waves_per_block);
 // it is in use); with smoother_bounds also estimate lambda_max(P A), which the
 // Chebyshev smoother needs.
 int mg_set_learned_factor(Multigrid* mg, hipStream_t st, const Factor& F, bool smoother_bounds);
+// The same with the bound's power iteration overlapping the hierarchy setup:
+// begin registers F and enqueues the power iteration on st (finest level,
+// own scratch; d_K32 / d_dinv32 are the finest operator the setup will use);
+// mg_setup may then run on another stream; end (after st has been ordered
+// behind that setup) waits for st and uploads the Chebyshev coefficients.
+int mg_learned_bound_begin(Multigrid* mg, hipStream_t st, const Factor& F, const float* d_K32,
+                           const float* d_dinv32);
+int mg_learned_bound_end(Multigrid* mg, hipStream_t st);
 // Training through one V-cycle with the learned smoother: loss = energy of the
 // error left by one cycle on b = A e, relative to e; fills the factor uses
 // (inputs v_k of P and dLoss/dz_k) for learned_step_from_uses.  The returned
