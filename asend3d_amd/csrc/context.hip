@@ -524,8 +524,8 @@ int ensure_mg(asend_ctx* ctx) {
         return ASEND_OK;
     }
     rc = mg_setup(ctx->mg, ctx->st, ctx->d_E, ctx->d_nu, ctx->d_K32, ctx->d_dinv32, ctx->d_kdiag, ctx->sf,
-                      ctx->n_pen, ctx->h_pen.data(), ctx->h_pen_mult.data());
-    if (rc) return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
+                  ctx->n_pen, ctx->h_pen.data(), ctx->h_pen_mult.data());
+    if (rc || (rc = mg_check_bounds(ctx->mg))) return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
     ctx->mg_asm = ctx->asm_count;
     return ASEND_OK;
 }
@@ -591,8 +591,13 @@ int ensure_mg_and_factor(asend_ctx* ctx, bool* mg_failed) {
         (void)hipStreamSynchronize(ctx->st);
         return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
     }
+    const int rc_bound = mg_learned_bound_end(ctx->mg, ctx->st);
+    if ((rc = mg_check_bounds(ctx->mg))) {
+        *mg_failed = true;
+        return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
+    }
     ctx->mg_asm = ctx->asm_count;
-    if ((rc = mg_learned_bound_end(ctx->mg, ctx->st))) return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
+    if (rc_bound) return fail(ctx, rc_bound, "%s", mg_last_error(ctx->mg));
     ctx->factor_asm = ctx->asm_count;
     return ASEND_OK;
 }

@@ -40,6 +40,7 @@ constexpr int kChebMaxDegree = 8;
 constexpr int kMaxCoarseNodes = 30;
 constexpr int kMaxCoarseDenseNodes = 400;
 constexpr int kPowerIts = 20;
+constexpr int kMaxLevels = 32;  // bound of the hierarchy depth (coarsening stops at 17 levels)
 
 // ---------------------------------------------------------------------------
 // Host-side axis maps between a level and the next coarser level.
@@ -150,6 +151,12 @@ struct Multigrid {
     // hierarchy is set up on another (mg_learned_bound_begin/end): its own
     // reduction scratch and result
     double *partials_b = nullptr, *sv_b = nullptr, *h_sv_b = nullptr;
+    // coarse-level smoother bounds: estimated and turned into Chebyshev
+    // coefficients on the device during setup, read back once (d_lam ->
+    // h_lam on lam_stream) when the host first needs them
+    double *d_lam = nullptr, *h_lam = nullptr;
+    bool lam_pending = false;
+    hipStream_t lam_stream = nullptr;
     double* sv = nullptr;      // device scalars of the FCG loop
     double* h_sv = nullptr;    // pinned mirror
     // Chebyshev smoother: point (0) or block (1) Jacobi, degree and interval
@@ -682,6 +689,33 @@ __global__ void k_scale_dev(int64_t n, const float* __restrict__ a, const double
     const float s = (float)(1.0 / sqrt(nrm2[0]));
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         b[i] = s * a[i];
+}
+
+// Chebyshev coefficients of one level from its power-iteration estimate
+// lambda = sqrt(nrm2[0]), the device twin of cheb_coefficients; lambda is
+// kept in lam_out.  A broken-down estimate (not finite or not positive)
+// writes zero coefficients (no smoothing) and is reported when the host
+// reads the bounds back.
+__global__ void k_cheb_coef(const double* __restrict__ nrm2, double lo_frac, double hi_frac, int degree,
+                            float* __restrict__ out, double* __restrict__ lam_out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const double lam = sqrt(nrm2[0]);
+    *lam_out = lam;
+    if (!(lam > 0.0) || !isfinite(lam)) {
+        for (int k = 0; k < 2 * degree; ++k) out[k] = 0.f;
+        return;
+    }
+    const double hi = lam * hi_frac, lo = lam * lo_frac;
+    const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+    double rho0 = 1.0 / sigma;
+    out[0] = 0.f;
+    out[1] = (float)(1.0 / theta);
+    for (int k = 1; k < degree; ++k) {
+        const double rho1 = 1.0 / (2.0 * sigma - rho0);
+        out[2 * k] = (float)(rho1 * rho0);
+        out[2 * k + 1] = (float)(2.0 * rho1 / delta);
+        rho0 = rho1;
+    }
 }
 
 __global__ void k_scale(int64_t n, const float* __restrict__ a, float s, float* __restrict__ b) {
@@ -1225,6 +1259,8 @@ int mg_create(Multigrid** out, int nx, int ny, int nz, const int32_t* rank_of_gr
     MGCHK(hipMalloc((void**)&mg->q, n0 * sizeof(float)));
     MGCHK(hipMalloc((void**)&mg->partials, 2 * 4096 * sizeof(double)));
     MGCHK(hipMalloc((void**)&mg->sv, 32 * sizeof(double)));
+    MGCHK(hipMalloc((void**)&mg->d_lam, kMaxLevels * sizeof(double)));
+    MGCHK(hipHostMalloc((void**)&mg->h_lam, kMaxLevels * sizeof(double), hipHostMallocDefault));
     MGCHK(hipHostMalloc((void**)&mg->h_sv, 32 * sizeof(double), hipHostMallocDefault));
     return ASEND_OK;
 }
@@ -1244,6 +1280,8 @@ void mg_destroy(Multigrid* mg) {
     fr(mg->fz);
     fr(mg->partials_b);
     fr(mg->sv_b);
+    fr(mg->d_lam);
+    if (mg->h_lam) (void)hipHostFree(mg->h_lam);
     if (mg->h_sv_b) (void)hipHostFree(mg->h_sv_b);
     fr(mg->d_coef);
     for (auto& g : mg->graphs)
@@ -1299,8 +1337,25 @@ int fine_bound(Multigrid* mg, hipStream_t st) {
 
 // Recompute the device Chebyshev coefficients from the stored bounds (after a
 // change of degree or interval that keeps the Jacobi kind).
+int mg_check_bounds(Multigrid* mg) {
+    if (!mg->lam_pending) return ASEND_OK;
+    MGCHK(hipStreamSynchronize(mg->lam_stream));
+    mg->lam_pending = false;
+    for (size_t l = 1; l + 1 < mg->L.size(); ++l) {
+        const double lam = mg->h_lam[l];
+        if (!(lam > 0.0) || !std::isfinite(lam)) {
+            mg->ready = false;
+            return mgfail(mg, ASEND_ERR_NUMERIC, "power iteration broke down");
+        }
+        mg->L[l].lmax = lam;
+    }
+    return ASEND_OK;
+}
+
 int mg_refresh_coefficients(Multigrid* mg, hipStream_t st) {
     if (!mg->ready) return ASEND_OK;
+    int rc = mg_check_bounds(mg);
+    if (rc) return rc;
     for (size_t l = 0; l + 1 < mg->L.size(); ++l)
         if (mg->L[l].lmax > 0.0)  // the finest level's may still be pending
             cheb_coefficients(mg, mg->L[l].lmax, mg->h_coef.data() + coef_slot(mg, (int)l, 0));
@@ -1537,14 +1592,17 @@ int mg_setup(Multigrid* mg, hipStream_t st, const double* d_E, const double* d_n
     // smoother bounds (all but the coarsest level).  The finest level's is
     // estimated on first use (fine_bound): when the learned factor smooths
     // that level, its block-Jacobi bound is never needed.
+    // The coarse ones are estimated and turned into coefficients on the
+    // device, without host round trips (mg_check_bounds reads them back).
     mg->L[0].lmax = 0.0;
     for (size_t l = 1; l + 1 < mg->L.size(); ++l) {
-        int rc = estimate_lmax(mg, st, (int)l, false, &mg->L[l].lmax);
-        if (rc) return rc;
-        cheb_coefficients(mg, mg->L[l].lmax, mg->h_coef.data() + coef_slot(mg, (int)l, 0));
+        enqueue_power(mg, st, (int)l, false, mg->partials, mg->sv + SV_N);
+        hipLaunchKernelGGL(k_cheb_coef, dim3(1), dim3(64), 0, st, mg->sv + SV_N, mg->cheb_lo, mg->cheb_hi,
+                           mg->cheb_degree, mg->d_coef + coef_slot(mg, (int)l, 0), mg->d_lam + l);
     }
-    MGCHK(hipMemcpyAsync(mg->d_coef, mg->h_coef.data(), mg->h_coef.size() * sizeof(float), hipMemcpyHostToDevice,
-                         st));
+    MGCHK(hipMemcpyAsync(mg->h_lam, mg->d_lam, mg->L.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    mg->lam_pending = true;
+    mg->lam_stream = st;
     // coarsest: dense FP64 inverse
     Level& lc = mg->L.back();
     hipLaunchKernelGGL(k_fill, dim3(nb_for((int64_t)mg->nco * mg->nco / 2)), dim3(256), 0, st,

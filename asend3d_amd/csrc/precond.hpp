@@ -38,6 +38,9 @@ int mg_set_fine_smoother(Multigrid* mg, int kind);
 int mg_set_smoother(Multigrid* mg, int kind, int degree, double lo_frac, double hi_frac);
 // Re-derive the Chebyshev step coefficients after mg_set_smoother changed the
 // degree or interval (same Jacobi kind: the stored bounds stay valid).
+// Waits for the coarse-level smoother bounds the last setup estimated on the
+// device, keeps them on the host and reports a broken-down estimate.
+int mg_check_bounds(Multigrid* mg);
 int mg_refresh_coefficients(Multigrid* mg, hipStream_t st);
 // New finest-level operator values on an unchanged hierarchy (same moduli,
 // penalty and pattern below it): block-Jacobi inverse recomputed, Chebyshev
