@@ -22,8 +22,10 @@
  *     never throws across the ABI; asend_last_error() gives the message.
  *     Non-convergence is not an error: the call returns ASEND_OK and sets
  *     stats->converged = 0 (the reference only warns, S/matrix_functions.rs:354-356).
- *   - A context is not thread-safe; all its device work runs on one HIP
- *     stream created at default priority.
+ *   - A context is not thread-safe; its device work runs on one HIP stream,
+ *     except the multigrid setup of a learned-smoother solve, which runs on
+ *     a second stream of the context joined back before the solve; both are
+ *     created at default priority.
  */
 #ifndef ASEND_GPU_H
 #define ASEND_GPU_H
