@@ -555,6 +555,10 @@ int ensure_factor(asend_ctx* ctx, bool smoother) {
 int ensure_mg_and_factor(asend_ctx* ctx, bool* mg_failed) {
     *mg_failed = false;
     int rc;
+    if (!ctx->mg) {
+        *mg_failed = true;
+        return fail(ctx, ASEND_ERR_STATE, "multigrid preconditioner not built (asend_set_grid)");
+    }
     const bool need_mg = !(ctx->mg_asm == ctx->asm_count && mg_ready(ctx->mg));
     const bool need_factor = !(ctx->factor_asm == ctx->asm_count && mg_factor_has_bounds(ctx->mg));
     if (!need_mg || !need_factor || ctx->mg_fine_only) {

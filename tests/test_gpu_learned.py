@@ -137,6 +137,17 @@ def test_learned_solve_matches_fp64_and_falls_back(ctx):
             fresh.autotune(p.f, api.TUNE_SOLVER | api.TUNE_LEARNED)
     finally:
         fresh.close()
+    # a learned-smoother solve on a context without the multigrid hierarchy
+    # is refused with a message
+    nogrid = api.Context(0)
+    try:
+        problems.upload(nogrid, p)
+        nogrid.assemble()
+        nogrid.learned_init(9)
+        with pytest.raises(api.AsendError, match="asend_set_grid"):
+            nogrid.static_compliance(p.f, o)
+    finally:
+        nogrid.close()
     # a network with non-finite weights must not take the solve down
     ctx.learned_set_params(np.full(api.learned_num_params(), np.nan, dtype=np.float32))
     o = ctx.opts(api.SOLVE_IR_MIXED, api.PC_LEARNED, tol=1e-8, inner_tol=1e-3, max_it=400000, max_outer=40,
