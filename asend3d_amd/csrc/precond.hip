@@ -1103,8 +1103,6 @@ void vcycle(Multigrid* mg, hipStream_t st, int l, const float* b, float* x) {
     smooth(mg, st, l, b, x, false);
 }
 
-// lambda_max of M A by power iteration (M = D^-1, or the learned factor on
-// the finest level when `learned`).
 // kPowerIts steps of the power iteration for lambda_max(M A) on level l,
 // enqueued on st with the given reduction scratch; ||M A x||^2 of the last
 // step lands in nrm2[0].
@@ -1131,6 +1129,8 @@ void enqueue_power(Multigrid* mg, hipStream_t st, int l, bool learned, double* p
     }
 }
 
+// lambda_max of M A by power iteration (M = the level's Jacobi smoother, or
+// the learned factor on the finest level when `learned`), read back here.
 int estimate_lmax(Multigrid* mg, hipStream_t st, int l, bool learned, double* out) {
     // deterministic pseudo-random start vector (a smooth start such as the
     // constant vector lies near the rigid-body modes and converges slowly);
