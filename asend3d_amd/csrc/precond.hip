@@ -151,6 +151,11 @@ struct Multigrid {
     // hierarchy is set up on another (mg_learned_bound_begin/end): its own
     // reduction scratch and result
     double *partials_b = nullptr, *sv_b = nullptr, *h_sv_b = nullptr;
+    // that power iteration captured as a graph, with what it was captured for
+    hipGraphExec_t pw_exec = nullptr;
+    Factor pw_factor;
+    const float* pw_val = nullptr;
+    int pw_waves = 0;
     // coarse-level smoother bounds: estimated and turned into Chebyshev
     // coefficients on the device during setup, read back once (d_lam ->
     // h_lam on lam_stream) when the host first needs them
@@ -1280,6 +1285,7 @@ void mg_destroy(Multigrid* mg) {
     fr(mg->fz);
     fr(mg->partials_b);
     fr(mg->sv_b);
+    if (mg->pw_exec) (void)hipGraphExecDestroy(mg->pw_exec);
     fr(mg->d_lam);
     if (mg->h_lam) (void)hipHostFree(mg->h_lam);
     if (mg->h_sv_b) (void)hipHostFree(mg->h_sv_b);
@@ -1441,8 +1447,40 @@ int mg_learned_bound_begin(Multigrid* mg, hipStream_t st, const Factor& F, const
     mg->has_factor = true;
     mg->lmax_factor = 0.0;
     // the finest level's vectors x, r and the factor's ft, fz are not touched
-    // by the setup of the coarser levels
-    enqueue_power(mg, st, 0, true, mg->partials_b, mg->sv_b);
+    // by the setup of the coarser levels.  The 20 steps (about 100 launches)
+    // replay as one graph while the factor storage, the operator and the
+    // SpMV geometry stay where they were captured.
+    if (mg->use_graphs) {
+        const Factor& P = mg->pw_factor;
+        const bool same = mg->pw_exec && P.n == F.n && P.n_edge == F.n_edge && P.Ls == F.Ls && P.LsT == F.LsT &&
+                          P.enb == F.enb && P.erev == F.erev && mg->pw_val == d_K32 &&
+                          mg->pw_waves == mg->fine.waves_per_block;
+        if (!same) {
+            if (mg->pw_exec) (void)hipGraphExecDestroy(mg->pw_exec);
+            mg->pw_exec = nullptr;
+            hipGraph_t graph = nullptr;
+            MGCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            enqueue_power(mg, st, 0, true, mg->partials_b, mg->sv_b);
+            hipError_t e1 = hipGetLastError();
+            hipError_t e2 = hipStreamEndCapture(st, &graph);
+            if (e1 != hipSuccess || e2 != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return mgfail(mg, ASEND_ERR_HIP, "capturing the bound's power iteration failed");
+            }
+            hipError_t e3 = hipGraphInstantiate(&mg->pw_exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e3 != hipSuccess) {
+                mg->pw_exec = nullptr;
+                return mgfail(mg, ASEND_ERR_HIP, "instantiating the bound's power iteration failed");
+            }
+            mg->pw_factor = F;
+            mg->pw_val = d_K32;
+            mg->pw_waves = mg->fine.waves_per_block;
+        }
+        MGCHK(hipGraphLaunch(mg->pw_exec, st));
+    } else {
+        enqueue_power(mg, st, 0, true, mg->partials_b, mg->sv_b);
+    }
     MGCHK(hipGetLastError());
     MGCHK(hipMemcpyAsync(mg->h_sv_b, mg->sv_b, sizeof(double), hipMemcpyDeviceToHost, st));
     return ASEND_OK;
