@@ -24,6 +24,22 @@ def test_library_exports_every_declared_symbol():
     assert "gfx950" in api.version()
 
 
+def _fixture_cases():
+    import json
+    return json.load(open(os.path.join(ROOT, "tests", "golden", "reorder_hand_traced.json")))["cases"]
+
+
+@pytest.mark.parametrize("case", _fixture_cases(), ids=lambda c: c["name"])
+def test_reorder_matches_hand_traced_orderings(case):
+    # the host reordering against orderings stepped by hand through the
+    # reference routine (tests/golden/reorder_hand_traced.md)
+    cons = [[np.asarray(t, dtype=np.int32) for t in con] for con in case["constraints"]]
+    ordered, di, ndof = api.reorder_nodes(np.asarray(case["xyz"], float), np.asarray(case["conn"], dtype=np.int32),
+                                          case["dof_per_nd"], cons, case["block_dim"])
+    assert list(ordered) == case["ordered"]
+    assert np.array_equal(di, np.asarray(case["dof_index"])) and ndof == case["n_dof"]
+
+
 @pytest.mark.parametrize("dims,bd", [((20, 4, 4), None), ((13, 7, 3), None), ((9, 9, 9), 12), ((40, 8, 8), 30),
                                      ((6, 3, 2), 2000000000)])
 def test_reorder_matches_oracle_bit_exact(dims, bd):

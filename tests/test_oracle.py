@@ -1,5 +1,7 @@
 """Pins the CPU oracle (oracle/liboracle.so) against closed forms, invariants,
 library routines and finite differences.  CPU only."""
+import os
+
 import numpy as np
 import pytest
 
@@ -114,6 +116,23 @@ def test_brickim_dual_matches_fd():
     Kp, _ = O.solid_element_dE(81, X, 5e9 + h, 0.2)
     Km, _ = O.solid_element_dE(81, X, 5e9 - h, 0.2)
     assert np.allclose(dK, (Kp - Km) / (2 * h), rtol=1e-6, atol=1e-6 * np.abs(dK).max())
+
+
+BRICK_XI = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1], [-1, -1, 1], [1, -1, 1], [1, 1, 1],
+                     [-1, 1, 1]], float)
+
+
+@pytest.mark.parametrize("size", [(1.0, 1.0, 1.0), (0.5, 0.7, 1.3)])
+def test_consistent_mass_closed_form(size):
+    # trilinear consistent mass of a box, integrated exactly by 2x2x2 Gauss:
+    # M_ab = rho V / 64 prod_i (1 + xi_a^i xi_b^i / 3) on each component, no
+    # coupling between components (a lumped or misweighted M fails this)
+    X = (BRICK_XI + 1.0) / 2.0 * np.asarray(size)
+    rho = 3.0
+    M = O.solid_mass(8, X, rho)
+    V = float(np.prod(size))
+    P = np.array([[np.prod(1.0 + BRICK_XI[a] * BRICK_XI[b] / 3.0) for b in range(8)] for a in range(8)])
+    assert np.abs(M - np.kron(rho * V / 64.0 * P, np.eye(3))).max() <= 1e-14 * np.abs(M).max()
 
 
 def test_mass_total():
@@ -256,15 +275,107 @@ def test_brickim_softer_than_brick8_in_bending():
     assert abs(w[81] - eb) < 0.05 * abs(eb)
 
 
-def test_reorder_invariants():
-    m = hex_block(12, 5, 4, 1.0, 0.3, 0.2)
+def _fixture_cases():
+    import json
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reorder_hand_traced.json")
+    return json.load(open(path))["cases"]
+
+
+def _run_reorder(c):
+    cons = [[np.asarray(t, dtype=np.int32) for t in con] for con in c["constraints"]]
+    return O.reorder_nodes(np.asarray(c["xyz"], float), np.asarray(c["conn"], dtype=np.int32), c["dof_per_nd"],
+                           cons, c["block_dim"])
+
+
+@pytest.mark.parametrize("case", _fixture_cases(), ids=lambda c: c["name"])
+def test_reorder_matches_hand_traced_orderings(case):
+    # orderings derived by stepping S/model_prep.rs:20-54,56-258 by hand
+    # (tests/golden/reorder_hand_traced.md): restarts with distance ties,
+    # queue exhaustion, constraint self-links, a two-set tie, 6 DOF per node
+    ordered, di, ndof = _run_reorder(case)
+    assert list(ordered) == case["ordered"]
+    assert np.array_equal(di, np.asarray(case["dof_index"]))
+    assert ndof == case["n_dof"]
+
+
+def _neighbour_sets(n_nodes, conn, constraints):
+    """Node graph of an element mesh and its constraints, built from sets
+    (independent of any insertion order): element co-membership, zipped
+    multi-node term pairs, all pairs otherwise (one-term sets link each node
+    to itself)."""
+    nb = [set() for _ in range(n_nodes)]
+    for el in conn:
+        for a in el:
+            for b in el:
+                if a != b:
+                    nb[a].add(int(b))
+    for con in constraints:
+        for t1 in con:
+            for t2 in con:
+                if len(t1) > 1 and len(t2) > 1:
+                    pairs = zip(t1, t2)
+                else:
+                    pairs = ((a, b) for a in t1 for b in t2)
+                for a, b in pairs:
+                    nb[a].add(int(b))
+                    nb[b].add(int(a))
+    return nb
+
+
+def _nearest_uninserted(xyz, done, frm):
+    d = np.sqrt(((xyz - xyz[frm]) ** 2).sum(axis=1))
+    d[done] = np.inf
+    return int(np.flatnonzero(d == d.min())[0])  # lowest label on a tie
+
+
+@pytest.mark.parametrize("dims,bd,phase", [((12, 5, 4), 18, True), ((7, 6, 3), 11, True), ((9, 4, 4), 5, False)])
+def test_reorder_invariants(dims, bd, phase):
+    # properties of the reference ordering checked by brute force, without
+    # restating it: the start node has the fewest neighbours (first on a tie,
+    # clamped nodes count themselves); every node neighbours an earlier one or
+    # is the uninserted node nearest to the previous one (lowest label on a
+    # tie).  Block restarts come every bd + 1 insertions and take the nearest
+    # node: checked where no queue-exhaustion restart (a restart node whose
+    # neighbours are all inserted, which shifts the block phase) intervenes,
+    # as on the first two meshes; with bd = 5 they do occur.
+    m = hex_block(*dims, 1.0, 0.37, 0.21)
     clamp = m.nodes_at_x(0)
-    bd = 18
-    ordered, di, ndof = O.reorder_nodes(m.xyz, m.conn, 3, [[clamp], [clamp], [clamp]], bd)
-    assert np.array_equal(np.sort(ordered), np.arange(m.n_nodes))
-    assert ndof == 3 * m.n_nodes
-    assert np.array_equal(di[ordered, 0], 3 * np.arange(m.n_nodes))
-    assert np.all(di[:, 3:] == -1)
-    # first node has the least connectivity (a corner, 7 neighbours, not clamped)
-    first = ordered[0]
-    assert first not in set(clamp.tolist())
+    cons = [[clamp], [clamp], [clamp]]
+    ordered, di, ndof = O.reorder_nodes(m.xyz, m.conn, 3, cons, bd)
+    n = m.n_nodes
+    assert np.array_equal(np.sort(ordered), np.arange(n))
+    assert ndof == 3 * n and np.array_equal(di[ordered, 0], 3 * np.arange(n)) and np.all(di[:, 3:] == -1)
+    nb = _neighbour_sets(n, m.conn, [[c.tolist() for c in con] for con in cons])
+    deg = np.array([len(s) for s in nb])
+    assert ordered[0] == int(np.flatnonzero(deg == deg.min())[0])
+    done = np.zeros(n, bool)
+    done[ordered[0]] = True
+    for p in range(1, n):
+        node = int(ordered[p])
+        near = _nearest_uninserted(m.xyz, done, int(ordered[p - 1]))
+        if phase and p % (bd + 1) == 0:
+            assert node == near, p
+        else:
+            assert node == near or any(done[q] for q in nb[node]), p
+        done[node] = True
+
+
+def test_reorder_invariants_fixture_meshes():
+    # the same brute-force checks hold on the hand-traced inputs; on the
+    # disjoint cubes the restart that crosses the gap is a queue-exhaustion
+    # restart (nearest node to the last inserted one)
+    for c in _fixture_cases():
+        ordered, _, _ = _run_reorder(c)
+        xyz = np.asarray(c["xyz"], float)
+        nb = _neighbour_sets(len(xyz), c["conn"], c["constraints"])
+        deg = np.array([len(s) for s in nb])
+        assert ordered[0] == int(np.flatnonzero(deg == deg.min())[0]), c["name"]
+        done = np.zeros(len(xyz), bool)
+        done[ordered[0]] = True
+        for p in range(1, len(xyz)):
+            node = int(ordered[p])
+            near = _nearest_uninserted(xyz, done, int(ordered[p - 1]))
+            assert node == near or any(done[q] for q in nb[node]), (c["name"], p)
+            if p % (c["block_dim"] + 1) == 0:
+                assert node == near, (c["name"], p)
+            done[node] = True

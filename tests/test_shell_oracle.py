@@ -39,6 +39,24 @@ def test_element_symmetry_rigid_modes_and_mass():
     assert np.isclose(Me[0::6, 0::6].sum(), 2700.0 * 0.005 * area, rtol=1e-12)
 
 
+def test_rectangle_mass_closed_form_with_rotary_inertia():
+    # flat mid-plane section: bilinear consistent mass
+    # int N_a N_b dA = A/16 prod_i (1 + xi_a^i xi_b^i / 3), times rho t on
+    # the three translations and rho t^3 / 12 on the three rotations
+    # (get_shell_mass, S/element_properties.rs:1354-1408), no coupling
+    a, b, t, rho = 0.3, 0.2, 0.005, 2700.0
+    X = np.array([[0, a, a, 0], [0, 0, b, b], [0, 0, 0, 0]], float)
+    _, M = O.shell4_element(X, 70e9, 0.33, t, rho)
+    xi = np.array([[-1, -1], [1, -1], [1, 1], [-1, 1]], float)
+    P = np.array([[np.prod(1.0 + xi[i] * xi[j] / 3.0) for j in range(4)] for i in range(4)]) * a * b / 16.0
+    ref = np.kron(P, np.diag([rho * t] * 3 + [rho * t ** 3 / 12.0] * 3))
+    assert np.abs(M - ref).max() <= 1e-13 * np.abs(M).max()
+    # the rotary entries are far below the translational ones: check them on
+    # their own scale too
+    rot = np.ix_(np.r_[3:6, 9:12, 15:18, 21:24], np.r_[3:6, 9:12, 15:18, 21:24])
+    assert np.abs(M[rot] - ref[rot]).max() <= 1e-13 * np.abs(ref[rot]).max()
+
+
 def test_constant_membrane_strain_energy():
     # exact on a parallelogram (the incompatible in-plane modes pass the patch
     # test there; on a general quad the formulation is off by ~1e-6)
