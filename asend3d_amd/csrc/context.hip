@@ -163,6 +163,10 @@ struct asend_ctx {
         double inner_tol = 1e-3;  // reliable-update factor
         float* f32 = nullptr;
         int graph_mixed = -1;
+        // the step graphs also hold the reliable-update factor and the SpMV
+        // workgroup geometry as launch arguments
+        double graph_ru = -1.0;
+        int graph_wpb = -1;
     } nm;
 };
 
@@ -224,16 +228,21 @@ int ensure_work(asend_ctx* ctx) {
     return ASEND_OK;
 }
 
-int read_scalars(asend_ctx* ctx, int idx) {
-    HIPCHK(hipMemcpyAsync(&ctx->h_sc[idx], &ctx->d_sc[idx], sizeof(Scalars), hipMemcpyDeviceToHost, ctx->st));
+// Copy device scalars `dev` into (or from) the pinned host mirror slot idx.
+int read_scalars_from(asend_ctx* ctx, const Scalars* dev, int idx) {
+    HIPCHK(hipMemcpyAsync(&ctx->h_sc[idx], dev, sizeof(Scalars), hipMemcpyDeviceToHost, ctx->st));
     HIPCHK(hipStreamSynchronize(ctx->st));
     return ASEND_OK;
 }
 
-int write_scalars(asend_ctx* ctx, int idx) {
-    HIPCHK(hipMemcpyAsync(&ctx->d_sc[idx], &ctx->h_sc[idx], sizeof(Scalars), hipMemcpyHostToDevice, ctx->st));
+int write_scalars_to(asend_ctx* ctx, Scalars* dev, int idx) {
+    HIPCHK(hipMemcpyAsync(dev, &ctx->h_sc[idx], sizeof(Scalars), hipMemcpyHostToDevice, ctx->st));
     return ASEND_OK;
 }
+
+int read_scalars(asend_ctx* ctx, int idx) { return read_scalars_from(ctx, ctx->d_sc + idx, idx); }
+
+int write_scalars(asend_ctx* ctx, int idx) { return write_scalars_to(ctx, ctx->d_sc + idx, idx); }
 
 void newmark_free(asend_ctx* ctx) {
     auto& nm = ctx->nm;
@@ -255,6 +264,8 @@ void newmark_free(asend_ctx* ctx) {
         *e = nullptr;
     }
     nm.graph_iters = nm.graph_watch = -1;
+    nm.graph_ru = -1.0;
+    nm.graph_wpb = -1;
     nm.ready = false;
 }
 
@@ -338,49 +349,106 @@ __global__ void k_sub_f32(int64_t n, const float* __restrict__ a, const float* _
 // Krylov drivers
 // ---------------------------------------------------------------------------
 
-// Jacobi PCG on the operator of precision T.  b, x on device (length n_dof).
-// Scalars slot `si`.  Returns iterations through the slot.
-template <typename T>
-int run_jacobi_pcg(asend_ctx* ctx, int si, const T* b, T* x, double tol, bool stop_abs, int64_t max_it,
-                   int check_every) {
+// Device buffers of the FP64 Jacobi PCG solve (ASEND_SOLVE_PCG_FP64): the
+// right-hand side f, the solution u, the outer residual r = f - K u, the
+// correction x of one PCG run, its residual r2, direction p and product q
+// (n_dof doubles each), the reduction partials (2 kMaxPartials doubles) and
+// two loop scalars (sc[0] the outer FP64 residual, sc[1] the PCG run).
+struct Fp64Bufs {
+    double *f, *u, *r, *x, *r2, *p, *q, *partials;
+    Scalars* sc;
+};
+
+Fp64Bufs ctx_fp64_bufs(asend_ctx* ctx) {
+    return {ctx->d_f, ctx->d_u, ctx->d_r, ctx->d_x64, ctx->d_r2, ctx->d_p, ctx->d_q, ctx->d_partials, ctx->d_sc};
+}
+
+__global__ void k_fill64(int64_t n, double v, double* __restrict__ x) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = v;
+}
+
+// One Jacobi PCG run on the FP64 operator: B.x solves K x = B.r from x = 0.
+// Host mirror slot 1 receives the run's scalars (iterations, g^T M^-1 g).
+int run_jacobi_pcg64(asend_ctx* ctx, const Fp64Bufs& B, double tol, bool stop_abs, int64_t max_it, int check_every) {
     const int64_t n = ctx->n_dof;
-    const int nr = (int)ctx->n_nodes;
-    const T* K;
-    const T* dinv;
-    T *r, *p, *q;
-    if (sizeof(T) == 8) {
-        K = (const T*)ctx->d_K64;
-        dinv = (const T*)ctx->d_dinv64;
-        r = (T*)ctx->d_r2;
-        p = (T*)ctx->d_p;
-        q = (T*)ctx->d_q;
-    } else {
-        K = (const T*)ctx->d_K32;
-        dinv = (const T*)ctx->d_dinv32;
-        r = (T*)ctx->d_w32;
-        p = (T*)ctx->d_p32;
-        q = (T*)ctx->d_q32;
-    }
-    Scalars& h = ctx->h_sc[si];
+    const double* K = ctx->d_K64;
+    const double* dinv = ctx->d_dinv64;
+    Scalars& h = ctx->h_sc[1];
     std::memset(&h, 0, sizeof(Scalars));
     h.tol = tol;
     h.tol2 = tol * tol;
     h.stop_abs = stop_abs ? 1 : 0;
     h.max_it = max_it;
+    Scalars* sc = B.sc + 1;
     int rc;
-    if ((rc = write_scalars(ctx, si))) return rc;
-    Scalars* sc = ctx->d_sc + si;
-    launch_pcg_init<T>(ctx->st, n, b, dinv, x, r, p, ctx->d_partials, sc);
+    if ((rc = write_scalars_to(ctx, sc, 1))) return rc;
+    launch_pcg_init<double>(ctx->st, n, B.r, dinv, B.x, B.r2, B.p, B.partials, sc);
     for (;;) {
         for (int k = 0; k < check_every; ++k) {
-            launch_sell3<T>(ctx->st, 1, ctx->fine(), K, p, q, nullptr, ctx->d_partials, sc);
-            launch_pcg_update<T>(ctx->st, n, dinv, x, r, p, q, ctx->d_partials, sc);
-            launch_pcg_dir<T>(ctx->st, n, dinv, r, p, sc);
+            launch_sell3<double>(ctx->st, 1, ctx->fine(), K, B.p, B.q, nullptr, B.partials, sc);
+            launch_pcg_update<double>(ctx->st, n, dinv, B.x, B.r2, B.p, B.q, B.partials, sc);
+            launch_pcg_dir<double>(ctx->st, n, dinv, B.r2, B.p, sc);
         }
         HIPCHK(hipGetLastError());
-        if ((rc = read_scalars(ctx, si))) return rc;
+        if ((rc = read_scalars_from(ctx, sc, 1))) return rc;
         if (h.done) break;
     }
+    return ASEND_OK;
+}
+
+// FP64 Jacobi PCG solve of K u = B.f into B.u (conj_grad_sparse with x0 = 0,
+// S/matrix_functions.rs:285-362, with the Jacobi preconditioner).  Under the
+// relative rule a run whose recursive residual drifted from the true one is
+// restarted from the true residual.  Allocates nothing: every buffer it
+// writes is in B.
+int solve_fp64(asend_ctx* ctx, const Fp64Bufs& B, const asend_solve_opts* o, asend_solve_stats* st) {
+    if (o->precond != ASEND_PC_JACOBI) return fail(ctx, ASEND_ERR_UNSUPPORTED, "FP64 PCG supports Jacobi only");
+    const int64_t n = ctx->n_dof;
+    const int64_t max_it = o->max_it > 0 ? o->max_it : n;
+    const int check = o->check_every > 0 ? o->check_every : (ctx->tuned_check > 0 ? ctx->tuned_check : 16);
+    const int64_t max_outer = o->max_outer > 0 ? o->max_outer : 20;
+    asend_solve_stats s;
+    std::memset(&s, 0, sizeof(s));
+    int rc;
+    // ||f||^2 with u = 0: r = f - K*0
+    hipLaunchKernelGGL(k_fill64, dim3(kVecBlocks), dim3(256), 0, ctx->st, n, 0.0, B.u);
+    std::memset(&ctx->h_sc[0], 0, sizeof(Scalars));
+    if ((rc = write_scalars_to(ctx, B.sc, 0))) return rc;
+    launch_sell3<double>(ctx->st, 2, ctx->fine(), ctx->d_K64, B.u, B.r, B.f, B.partials, B.sc);
+    HIPCHK(hipGetLastError());
+    if ((rc = read_scalars_from(ctx, B.sc, 0))) return rc;
+    const double ff = ctx->h_sc[0].rr;
+    if (ff == 0.0) {
+        s.converged = 1;
+        *st = s;
+        return ASEND_OK;
+    }
+    double rr = ff;
+    int64_t total_it = 0, outer = 0;
+    for (; outer < max_outer; ++outer) {
+        if (o->stop_rule == ASEND_STOP_REL_RESIDUAL && std::sqrt(rr / ff) <= o->tol) break;
+        const double tol_in = (o->stop_rule == ASEND_STOP_REL_RESIDUAL) ? o->tol * std::sqrt(ff / rr) : o->tol;
+        if ((rc = run_jacobi_pcg64(ctx, B, tol_in, o->stop_rule == ASEND_STOP_REF_ABS, max_it - total_it, check)))
+            return rc;
+        total_it += ctx->h_sc[1].it;
+        s.ref_residual = ctx->h_sc[1].rz;
+        // u += x ; r = f - K u
+        launch_axpy64(ctx->st, n, 1.0, B.x, B.u);
+        launch_sell3<double>(ctx->st, 2, ctx->fine(), ctx->d_K64, B.u, B.r, B.f, B.partials, B.sc);
+        HIPCHK(hipGetLastError());
+        if ((rc = read_scalars_from(ctx, B.sc, 0))) return rc;
+        rr = ctx->h_sc[0].rr;
+        if (o->stop_rule == ASEND_STOP_REF_ABS || total_it >= max_it) {
+            ++outer;
+            break;
+        }
+    }
+    s.outer_iterations = outer;
+    s.iterations = total_it;
+    s.rel_residual = std::sqrt(rr / ff);
+    s.converged = (o->stop_rule == ASEND_STOP_REL_RESIDUAL) ? (s.rel_residual <= o->tol) : (s.ref_residual <= o->tol);
+    *st = s;
     return ASEND_OK;
 }
 
@@ -607,6 +675,7 @@ int ensure_mg_and_factor(asend_ctx* ctx, bool* mg_failed) {
 }
 
 int solve_device(asend_ctx* ctx, const asend_solve_opts* o, asend_solve_stats* st) {
+    if (o->method == ASEND_SOLVE_PCG_FP64) return solve_fp64(ctx, ctx_fp64_bufs(ctx), o, st);
     const int64_t n = ctx->n_dof;
     int64_t max_it = o->max_it > 0 ? o->max_it : n;
     int check = o->check_every > 0 ? o->check_every : (ctx->tuned_check > 0 ? ctx->tuned_check : 16);
@@ -632,41 +701,6 @@ int solve_device(asend_ctx* ctx, const asend_solve_opts* o, asend_solve_stats* s
     }
     double rr = ff;
     int64_t total_it = 0;
-
-    if (o->method == ASEND_SOLVE_PCG_FP64) {
-        if (o->precond != ASEND_PC_JACOBI) return fail(ctx, ASEND_ERR_UNSUPPORTED, "FP64 PCG supports Jacobi only");
-        // FP64 PCG; restarted from the true residual if the recursive one drifted
-        int64_t outer = 0;
-        for (; outer < max_outer; ++outer) {
-            if (o->stop_rule == ASEND_STOP_REL_RESIDUAL && std::sqrt(rr / ff) <= o->tol) break;
-            double tol_in = (o->stop_rule == ASEND_STOP_REL_RESIDUAL) ? o->tol * std::sqrt(ff / rr) : o->tol;
-            if ((rc = run_jacobi_pcg<double>(ctx, 1, ctx->d_r, ctx->d_x64, tol_in, o->stop_rule == ASEND_STOP_REF_ABS,
-                                             max_it - total_it, check)))
-                return rc;
-            total_it += ctx->h_sc[1].it;
-            s.ref_residual = ctx->h_sc[1].rz;
-            // u += x ; r = f - K u
-            launch_axpy64(ctx->st, n, 1.0, ctx->d_x64, ctx->d_u);
-            if ((rc = residual64(ctx, ctx->d_f, ctx->d_u, ctx->d_r))) return rc;
-            if ((rc = read_scalars(ctx, 0))) return rc;
-            rr = ctx->h_sc[0].rr;
-            if (o->stop_rule == ASEND_STOP_REF_ABS) {
-                ++outer;
-                break;
-            }
-            if (total_it >= max_it) {
-                ++outer;
-                break;
-            }
-        }
-        s.outer_iterations = outer;
-        s.iterations = total_it;
-        s.rel_residual = std::sqrt(rr / ff);
-        s.converged = (o->stop_rule == ASEND_STOP_REL_RESIDUAL) ? (s.rel_residual <= o->tol)
-                                                                 : (s.ref_residual <= o->tol);
-        *st = s;
-        return ASEND_OK;
-    }
 
     if (o->method != ASEND_SOLVE_IR_MIXED) return fail(ctx, ASEND_ERR_ARG, "unknown solve method %d", o->method);
     if (o->stop_rule != ASEND_STOP_REL_RESIDUAL)
@@ -1820,17 +1854,22 @@ int asend_newmark_run(asend_ctx* ctx, int64_t n_steps, const double* f_shape, co
         realloc = true;
     }
     if (realloc) nm.graph_iters = -1;  // recapture: the step graphs hold the old buffers
-    HIPCHK(hipMemcpy(nm.fshape, f_shape, n * 8, hipMemcpyHostToDevice));
-    if (n_steps) HIPCHK(hipMemcpy(nm.fac, load_fac, n_steps * 8, hipMemcpyHostToDevice));
-    if (n_watch) HIPCHK(hipMemcpy(nm.watch, watch, n_watch * 8, hipMemcpyHostToDevice));
-    if (u0) HIPCHK(hipMemcpy(ctx->d_u, u0, n * 8, hipMemcpyHostToDevice));
-    else HIPCHK(hipMemset(ctx->d_u, 0, n * 8));
-    if (v0) HIPCHK(hipMemcpy(nm.v, v0, n * 8, hipMemcpyHostToDevice));
-    else HIPCHK(hipMemset(nm.v, 0, n * 8));
-    if (a0) HIPCHK(hipMemcpy(nm.a, a0, n * 8, hipMemcpyHostToDevice));
-    else HIPCHK(hipMemset(nm.a, 0, n * 8));
-    HIPCHK(hipMemset(nm.step, 0, 8));
-    HIPCHK(hipMemset(ctx->d_x64, 0, n * 8));  // increment of the step before the first
+    // initial state, ordered on the context stream after the assembly queued
+    // there (outside any captured region)
+    hipStream_t s0 = ctx->st;
+    HIPCHK(hipMemcpyAsync(nm.fshape, f_shape, n * 8, hipMemcpyHostToDevice, s0));
+    if (n_steps) HIPCHK(hipMemcpyAsync(nm.fac, load_fac, n_steps * 8, hipMemcpyHostToDevice, s0));
+    if (n_watch) HIPCHK(hipMemcpyAsync(nm.watch, watch, n_watch * 8, hipMemcpyHostToDevice, s0));
+    if (u0) HIPCHK(hipMemcpyAsync(ctx->d_u, u0, n * 8, hipMemcpyHostToDevice, s0));
+    else HIPCHK(hipMemsetAsync(ctx->d_u, 0, n * 8, s0));
+    if (v0) HIPCHK(hipMemcpyAsync(nm.v, v0, n * 8, hipMemcpyHostToDevice, s0));
+    else HIPCHK(hipMemsetAsync(nm.v, 0, n * 8, s0));
+    if (a0) HIPCHK(hipMemcpyAsync(nm.a, a0, n * 8, hipMemcpyHostToDevice, s0));
+    else HIPCHK(hipMemsetAsync(nm.a, 0, n * 8, s0));
+    HIPCHK(hipMemsetAsync(nm.step, 0, 8, s0));
+    HIPCHK(hipMemsetAsync(ctx->d_x64, 0, n * 8, s0));  // increment of the step before the first
+    // the host arrays may be pageable and are the caller's once this returns
+    HIPCHK(hipStreamSynchronize(s0));
     // per-step solve: FP64 Jacobi PCG on K_eff, relative tolerance, at most
     // max_it_step iterations (the kernels skip their work once converged)
     {
@@ -1934,7 +1973,8 @@ int asend_newmark_run(asend_ctx* ctx, int64_t n_steps, const double* f_shape, co
     };
     const bool graph = ctx->use_graphs;
     if (graph && (!nm.exec_head || nm.graph_iters != blk || nm.graph_watch != nw || nm.graph_save != nm.save_u ||
-                  nm.graph_save_y != nm.save_y || nm.graph_mixed != (int)mixed)) {
+                  nm.graph_save_y != nm.save_y || nm.graph_mixed != (int)mixed || nm.graph_ru != mx.ru_factor ||
+                  nm.graph_wpb != A.waves_per_block)) {
         if ((rc = capture(&nm.exec_head, head)) || (rc = capture(&nm.exec_more, iterations)) ||
             (rc = capture(&nm.exec_tail, tail)))
             return rc;
@@ -1943,6 +1983,8 @@ int asend_newmark_run(asend_ctx* ctx, int64_t n_steps, const double* f_shape, co
         nm.graph_save = nm.save_u;
         nm.graph_save_y = nm.save_y;
         nm.graph_mixed = (int)mixed;
+        nm.graph_ru = mx.ru_factor;
+        nm.graph_wpb = A.waves_per_block;
     }
     HIPCHK(hipEventRecord(ctx->ev0, ctx->st));
     for (int64_t s = 0; s < n_steps; ++s) {

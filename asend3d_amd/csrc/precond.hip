@@ -172,7 +172,8 @@ struct Multigrid {
     // captured FCG iteration blocks (hipGraph), reused while their key matches
     struct FcgGraph {
         hipGraphExec_t exec = nullptr;
-        int precond = -1, fine_smoother = -1, check = -1, degree = -1, kind = -1;
+        int precond = -1, fine_smoother = -1, check = -1, degree = -1, kind = -1, waves = -1;
+        const float* val0 = nullptr;  // finest-level operator values
         const float* b = nullptr;
         float* x = nullptr;
         const float* ls = nullptr;
@@ -1722,7 +1723,7 @@ int mg_fcg_solve(Multigrid* mg, hipStream_t st, const float* b, float* x, double
         for (auto& g : mg->graphs)
             if (g.exec && g.precond == precond && g.fine_smoother == mg->fine_smoother && g.check == check_every &&
                 g.degree == mg->cheb_degree && g.kind == mg->smoother_kind && g.b == b && g.x == x && g.ls == ls &&
-                g.lst == lst)
+                g.lst == lst && g.waves == mg->fine.waves_per_block && g.val0 == l0.d_val)
                 gr = &g;
         if (!gr) {
             // capture into the least recently used slot
@@ -1755,6 +1756,8 @@ int mg_fcg_solve(Multigrid* mg, hipStream_t st, const float* b, float* x, double
             gr->x = x;
             gr->ls = ls;
             gr->lst = lst;
+            gr->waves = mg->fine.waves_per_block;
+            gr->val0 = l0.d_val;
         }
         gr->use = ++mg->graph_clock;
     }

@@ -778,20 +778,32 @@ int orc_assemble_solid(int type, int64_t n_nodes, const double* xyz, int64_t n_e
     if (!cols || !vals) return 0;
     std::memcpy(cols, p.cols.data(), sizeof(int32_t) * p.cols.size());
     std::memset(vals, 0, sizeof(double) * p.cols.size());
-    double X[24], Ke[576];
+    /* element matrices of a chunk are integrated in parallel (g_threads) and
+     * scattered serially in element order, so the sums are those of the
+     * serial element loop bit for bit */
+    const int64_t chunk = 4096;
+    std::vector<double> Kc(chunk * 576);
     int64_t gd[24];
-    for (int64_t e = 0; e < n_el; ++e) {
-        const int32_t* c = conn + 8 * e;
-        gather_X(xyz, c, 8, X);
-        solid_ke_iso(type, X, E[e], nu[e], Ke);
-        for (int a = 0; a < 8; ++a)
-            for (int d = 0; d < 3; ++d) gd[3 * a + d] = dof_index[6 * (int64_t)c[a] + d];
-        for (int i = 0; i < 24; ++i)
-            for (int j = 0; j < 24; ++j) {
-                int64_t k = find_col(rowptr, cols, gd[i], (int32_t)gd[j]);
-                if (k < 0) return -3;
-                vals[k] += Ke[i * 24 + j];
-            }
+    for (int64_t e0 = 0; e0 < n_el; e0 += chunk) {
+        const int64_t ne = std::min(chunk, n_el - e0);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+        for (int64_t k = 0; k < ne; ++k) {
+            double X[24];
+            gather_X(xyz, conn + 8 * (e0 + k), 8, X);
+            solid_ke_iso(type, X, E[e0 + k], nu[e0 + k], Kc.data() + 576 * k);
+        }
+        for (int64_t k = 0; k < ne; ++k) {
+            const int32_t* c = conn + 8 * (e0 + k);
+            const double* Ke = Kc.data() + 576 * k;
+            for (int a = 0; a < 8; ++a)
+                for (int d = 0; d < 3; ++d) gd[3 * a + d] = dof_index[6 * (int64_t)c[a] + d];
+            for (int i = 0; i < 24; ++i)
+                for (int j = 0; j < 24; ++j) {
+                    int64_t kk = find_col(rowptr, cols, gd[i], (int32_t)gd[j]);
+                    if (kk < 0) return -3;
+                    vals[kk] += Ke[i * 24 + j];
+                }
+        }
     }
     return 0;
 }
@@ -1108,9 +1120,13 @@ int orc_solid_sens(int type, int64_t n_nodes, const double* xyz, int64_t n_el, c
                    double* dJdE) {
     (void)n_nodes;
     if (type != 8 && type != 81) return -1;
-    int tot = (type == 81) ? 33 : 24;
+    const int tot = (type == 81) ? 33 : 24;
+    /* elements are independent: parallel over g_threads */
+#pragma omp parallel num_threads(g_threads)
+    {
     std::vector<Dual> Kf(tot * tot), Rf(tot);
     std::vector<double> ue(tot), le(tot);
+#pragma omp for schedule(static)
     for (int64_t e = 0; e < n_el; ++e) {
         const int32_t* c = conn + 8 * e;
         double X[24];
@@ -1137,6 +1153,7 @@ int orc_solid_sens(int type, int64_t n_nodes, const double* xyz, int64_t n_el, c
         double s = 0.0;
         for (int i = 0; i < tot; ++i) s += le[i] * Rf[i].d;
         dJdE[e] = -s;
+    }
     }
     return 0;
 }

@@ -42,7 +42,9 @@ def lib():
 
 
 def set_threads(n):
-    """OpenMP threads of jacobi_pcg / spmv_penalty (1 = serial, the default)."""
+    """OpenMP threads of jacobi_pcg, spmv_penalty, the element loop of
+    assemble_solid and solid_sens (1 = serial, the default; results do not
+    depend on the thread count)."""
     _chk(lib().orc_set_threads(ctypes.c_int(int(n))), "set_threads")
 
 

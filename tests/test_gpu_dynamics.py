@@ -81,6 +81,34 @@ def test_newmark_graph_replay_matches_eager(ctx, mode):
             assert np.array_equal(x, x0)
 
 
+def test_newmark_graph_follows_reliable_update_factor(ctx):
+    # the mixed step solver's reliable-update factor is a launch argument of
+    # the captured step graphs: changing it between runs must recapture them,
+    # so graph and eager runs agree bit for bit at both factors
+    gpu_case("newmark graphs across a ru_factor change")
+    p, _, _ = cantilever_case((12, 3, 3))
+    problems.upload(ctx, p, rho=np.full(p.mesh.n_el, RHO))
+    ctx.newmark_setup(5e-5, 0.25, 0.5)
+    n = 30
+    fac = np.sin(np.pi * np.arange(n) / n)
+    watch = tip_dofs(p)
+    try:
+        for ru in (1e-2, 1e-4, 1e-2):
+            ctx.newmark_set_solver(api.NM_MIXED, ru)
+            runs = []
+            for on in (True, False):
+                ctx.set_graphs(on)
+                hist, u, v, a, st = ctx.newmark_run(n, p.f, fac, watch=watch, tol=1e-11, max_it_step=200)
+                assert st.unconverged_steps == 0 and st.step_solver == api.NM_MIXED
+                runs.append((st.total_iterations, hist, u, v, a))
+            assert runs[0][0] == runs[1][0], (ru, runs[0][0], runs[1][0])
+            for x, x0 in zip(runs[0][1:], runs[1][1:]):
+                assert np.array_equal(x, x0)
+    finally:
+        ctx.set_graphs(True)
+        ctx.newmark_set_solver(api.NM_AUTO)
+
+
 @pytest.mark.parametrize("etype", [8, 81])
 def test_newmark_adjoint_matches_oracle(ctx, etype):
     # L = sum_n w_n q.u_n (tip deflection over the history); the device's

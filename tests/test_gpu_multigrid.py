@@ -165,6 +165,43 @@ def test_graph_replay_matches_eager_launches(ctx):
             assert it1 == it0 and np.array_equal(u1, u0)
 
 
+def test_learned_graph_replay_matches_eager_launches(ctx):
+    # the learned pipeline captures the FCG blocks and the smoother bound's
+    # power iteration; replays must equal eager launches bit for bit across
+    # design updates (new moduli, new factor, same buffers) and across a
+    # larger mesh in between (new hierarchy, reallocated factor storage)
+    dims, big = (20, 6, 6), (30, 8, 8)
+    gpu_case(f"learned graph replay {dims} / {big}")
+    p = problems.heterogeneous(*dims, seed=12)
+    q = problems.heterogeneous(*big, seed=13)
+    E2 = p.E * np.exp(0.3 * np.random.default_rng(3).standard_normal(p.E.size))
+    o = ctx.opts(api.SOLVE_IR_MIXED, api.PC_LEARNED, tol=1e-8, inner_tol=1e-2, max_it=100000, max_outer=40,
+                 check_every=2)
+    ctx.learned_init(3)
+    setup(ctx, p, dims)
+    ctx.learned_train(40, lr=1e-3, seed=1, mode=1)
+    res = {}
+    try:
+        for on in (False, True, True):
+            ctx.set_graphs(on)
+            out = []
+            for mesh in ("p", "q", "p"):
+                m, d = (p, dims) if mesh == "p" else (q, big)
+                setup(ctx, m, d)
+                for E in ((m.E, E2) if mesh == "p" else (m.E,)):
+                    ctx.update_moduli(E)
+                    u, s, J, st = ctx.static_compliance(m.f, o)
+                    assert st.converged and st.fallback_used == 0, st.as_dict()
+                    out.append((u.copy(), st.iterations, J))
+            res.setdefault(on, []).append(out)
+    finally:
+        ctx.set_graphs(True)
+    ref = res[False][0]
+    for run in res[True]:
+        for (u1, it1, J1), (u0, it0, J0) in zip(run, ref):
+            assert it1 == it0 and J1 == J0 and np.array_equal(u1, u0)
+
+
 def test_autotuner_choices_are_used(ctx):
     dims = (24, 6, 6)
     gpu_case(f"autotune {dims}")

@@ -144,8 +144,11 @@ def test_reference_stop_rule(ctx):
     assert st1.iterations == 3 and not st1.converged
 
 
-def test_sensitivity_matches_dual_number_oracle(ctx):
-    name, p = list(small_problems())[1]
+@pytest.mark.parametrize("which", [1, 2])
+def test_sensitivity_matches_dual_number_oracle(ctx, which):
+    # which = 2: distorted elements (full Jacobians, so an index slip in J^-1
+    # shows) with a per-element Poisson ratio
+    name, p = list(small_problems())[which]
     gpu_case(f"sens {name}")
     problems.upload(ctx, p)
     ctx.assemble()
@@ -158,9 +161,10 @@ def test_sensitivity_matches_dual_number_oracle(ctx):
     assert np.abs(s - s_ref).max() <= 1e-10 * np.abs(s_ref).max()
 
 
+@pytest.mark.parametrize("which", [1, 2])
 @pytest.mark.parametrize("method", [api.SOLVE_PCG_FP64, api.SOLVE_IR_MIXED])
-def test_compliance_pipeline(ctx, method):
-    name, p = list(small_problems())[1]
+def test_compliance_pipeline(ctx, method, which):
+    name, p = list(small_problems())[which]
     gpu_case(f"compliance pipeline {name} method {method}")
     problems.upload(ctx, p)
     K, sf = oracle_system(p)
