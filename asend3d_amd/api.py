@@ -134,7 +134,7 @@ _lib.asend_ctx_destroy.argtypes = [_vp]
 EXPORTED = ["asend_ctx_create", "asend_ctx_destroy", "asend_last_error", "asend_version", "asend_sync",
             "asend_reorder_nodes", "asend_block_dim_auto", "asend_surface_traction", "asend_lognormal_moduli",
             "asend_upload_mesh", "asend_set_thickness", "asend_set_grid", "asend_update_moduli", "asend_set_penalty",
-            "asend_penalty_scale", "asend_uniform_elements", "asend_assemble", "asend_get_matrix", "asend_spmv", "asend_solve",
+            "asend_penalty_scale", "asend_uniform_elements", "asend_assemble", "asend_get_matrix", "asend_spmv", "asend_solve", "asend_solve_noalloc",
             "asend_element_sens", "asend_static_compliance", "asend_bench_spmv", "asend_mg_info", "asend_mg_apply",
             "asend_mg_restrict", "asend_mg_prolong", "asend_mg_vcycle", "asend_mg_coarse_solve", "asend_bench_copy",
             "asend_learned_num_params", "asend_learned_init", "asend_learned_get_params",
@@ -323,6 +323,16 @@ class Context:
         x = np.zeros(self.n_dof)
         st = SolveStats()
         self._chk(_lib.asend_solve(self.h, _p(rhs), _p(x), ctypes.byref(opts), ctypes.byref(st)), "asend_solve")
+        return x, st
+
+    def solve_noalloc(self, rhs, opts):
+        """asend_solve_noalloc: the FP64 Jacobi PCG solve with its buffers in
+        the library's static per-GPU scratch array (no device allocation)."""
+        rhs = _f64(rhs)
+        x = np.zeros(self.n_dof)
+        st = SolveStats()
+        self._chk(_lib.asend_solve_noalloc(self.h, _p(rhs), _p(x), ctypes.byref(opts), ctypes.byref(st)),
+                  "asend_solve_noalloc")
         return x, st
 
     def element_sens(self, u, lam):

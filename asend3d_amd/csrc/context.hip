@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -934,6 +935,68 @@ int nl_assemble(asend_ctx* ctx, double fact, double* rhs) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free FP64 solve (asend_solve_noalloc): the seven
+// work vectors of the FP64 Jacobi PCG for up to kNoAllocMaxDof DOFs, the
+// reduction partials and the two loop scalars, in one device array that the
+// code object holds on every GPU the library runs on until the process ends
+// (14.8 MB).  Host code never passes `scratch` itself to a kernel: its name
+// is a host-side stand-in, not a device address.  Each device's address is
+// looked up once (scratch_setup, called by asend_ctx_create with that device
+// current) and kept in g_scratch_ptr.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int64_t kNoAllocMaxDof = 1 << 18;
+constexpr int64_t kNoAllocPartials = 2 * (int64_t)kMaxPartials;
+constexpr int64_t kNoAllocScalarDoubles = 64;  // room for the two Scalars
+static_assert(2 * sizeof(Scalars) <= kNoAllocScalarDoubles * sizeof(double), "scalar slot too small");
+constexpr int64_t kNoAllocDoubles = 7 * kNoAllocMaxDof + kNoAllocPartials + kNoAllocScalarDoubles;
+constexpr int kScratchDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+namespace {
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kScratchDevices] = {};
+
+// Device address of `scratch` on `device` (which must be current), looked up
+// on the first call for that device.
+hipError_t scratch_setup(int device) {
+    if (device < 0 || device >= kScratchDevices) return hipSuccess;  // asend_solve_noalloc then refuses
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!ptr) return hipErrorInvalidSymbol;
+    g_scratch_ptr[device] = (double*)ptr;
+    return hipSuccess;
+}
+
+double* scratch_ptr(int device) {
+    if (device < 0 || device >= kScratchDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
+// The FP64 solve's buffers carved out of the device's scratch array.
+Fp64Bufs scratch_fp64_bufs(double* base) {
+    const int64_t v = kNoAllocMaxDof;
+    Fp64Bufs B;
+    B.f = base;
+    B.u = base + v;
+    B.r = base + 2 * v;
+    B.x = base + 3 * v;
+    B.r2 = base + 4 * v;
+    B.p = base + 5 * v;
+    B.q = base + 6 * v;
+    B.partials = base + 7 * v;
+    B.sc = (Scalars*)(base + 7 * v + kNoAllocPartials);
+    return B;
+}
+}  // namespace
+
 // ===========================================================================
 // C ABI
 // ===========================================================================
@@ -946,6 +1009,10 @@ int asend_ctx_create(int device, asend_ctx** out) {
     ctx->device = device;
     hipError_t e = hipSetDevice(device);
     if (e != hipSuccess) {
+        delete ctx;
+        return ASEND_ERR_HIP;
+    }
+    if (scratch_setup(device) != hipSuccess) {
         delete ctx;
         return ASEND_ERR_HIP;
     }
@@ -1478,6 +1545,35 @@ int asend_solve(asend_ctx* ctx, const double* rhs, double* x, const asend_solve_
     HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     s.device_ms = ms;
     HIPCHK(hipMemcpy(x, ctx->d_u, n * 8, hipMemcpyDeviceToHost));
+    if (stats) *stats = s;
+    return ASEND_OK;
+}
+
+int asend_solve_noalloc(asend_ctx* ctx, const double* rhs, double* x, const asend_solve_opts* opts,
+                        asend_solve_stats* stats) {
+    if (!ctx || !rhs || !x || !opts) return ASEND_ERR_ARG;
+    if (!ctx->assembled) return fail(ctx, ASEND_ERR_STATE, "assemble first");
+    if (opts->method != ASEND_SOLVE_PCG_FP64 || opts->precond != ASEND_PC_JACOBI)
+        return fail(ctx, ASEND_ERR_UNSUPPORTED, "asend_solve_noalloc runs FP64 Jacobi PCG only");
+    const int64_t n = ctx->n_dof;
+    if (n > kNoAllocMaxDof)
+        return fail(ctx, ASEND_ERR_ARG, "asend_solve_noalloc: %lld DOFs exceed its scratch limit of %lld DOFs",
+                    (long long)n, (long long)kNoAllocMaxDof);
+    double* base = scratch_ptr(ctx->device);
+    if (!base) return fail(ctx, ASEND_ERR_STATE, "no scratch address for device %d", ctx->device);
+    const Fp64Bufs B = scratch_fp64_bufs(base);
+    HIPCHK(hipMemcpyAsync(B.f, rhs, n * 8, hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->st));
+    asend_solve_stats s;
+    int rc = solve_fp64(ctx, B, opts, &s);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->st));
+    HIPCHK(hipEventSynchronize(ctx->ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    s.device_ms = ms;
+    HIPCHK(hipMemcpyAsync(x, B.u, n * 8, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
     if (stats) *stats = s;
     return ASEND_OK;
 }

@@ -204,6 +204,20 @@ int asend_spmv(asend_ctx* ctx, int precision, const double* x, double* y);
 int asend_solve(asend_ctx* ctx, const double* rhs, double* x, const asend_solve_opts* opts,
                 asend_solve_stats* stats);
 
+/* Allocation-free variant of asend_solve for ASEND_SOLVE_PCG_FP64 with
+ * ASEND_PC_JACOBI (other methods: ASEND_ERR_UNSUPPORTED).  No reference
+ * counterpart: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It calls no hipMalloc/hipFree: the solve's work vectors,
+ * reduction partials and loop scalars live in one static device array per
+ * GPU, sized for at most 262,144 DOFs (14.8 MB, held by the library on every
+ * GPU it runs on until the process ends); a larger model returns
+ * ASEND_ERR_ARG.  Same kernels, iterates and stats as asend_solve.  Calls on
+ * one GPU share that array, from every context: calls that overlap in time on
+ * one GPU race. */
+int asend_solve_noalloc(asend_ctx* ctx, const double* rhs, double* x, const asend_solve_opts* opts,
+                        asend_solve_stats* stats);
+
 /* Compliance adjoint sensitivities for per-element modulus design variables:
  * dJdE[e] = -lambda_e^T (dK_e/dE_e) u_e, FP64 reduction.  Replaces the DV
  * loop of Model::get_obj_gradient with d_relasticd_d
