@@ -82,7 +82,9 @@ TUNE_LEARNED = 4  # tune the solver with the learned smoother
 class Tuning(ctypes.Structure):
     _fields_ = [("spmv_waves_per_block", ctypes.c_int), ("spmv_ms_fp32", ctypes.c_double * 4),
                 ("spmv_ms_fp64", ctypes.c_double * 4), ("cheb_degree", ctypes.c_int), ("inner_tol", ctypes.c_double),
-                ("check_every", ctypes.c_int), ("solve_ms", ctypes.c_double), ("n_trials", ctypes.c_int)]
+                ("check_every", ctypes.c_int), ("solve_ms", ctypes.c_double), ("n_trials", ctypes.c_int),
+                ("learned_active", ctypes.c_int), ("solve_ms_learned", ctypes.c_double),
+                ("solve_ms_no_network", ctypes.c_double)]
 
     def as_dict(self):
         d = {}
@@ -139,7 +141,7 @@ EXPORTED = ["asend_ctx_create", "asend_ctx_destroy", "asend_last_error", "asend_
             "asend_mg_restrict", "asend_mg_prolong", "asend_mg_vcycle", "asend_mg_coarse_solve", "asend_bench_copy",
             "asend_learned_num_params", "asend_learned_init", "asend_learned_get_params",
             "asend_learned_set_params", "asend_learned_train", "asend_learned_eval", "asend_learned_apply",
-            "asend_learned_edges", "asend_mg_set_smoother", "asend_learned_vcycle_eval", "asend_set_graphs",
+            "asend_learned_edges", "asend_mg_set_smoother", "asend_mg_set_cycle_precision", "asend_learned_vcycle_eval", "asend_set_graphs",
             "asend_set_rayleigh_damping", "asend_newmark_set_solver", "asend_newmark_setup", "asend_newmark_run", "asend_newmark_adjoint", "asend_modal", "asend_batch_create", "asend_batch_destroy",
             "asend_batch_error", "asend_batch_solve", "asend_autotune", "asend_nl_assemble",
             "asend_static_nonlinear"]
@@ -496,6 +498,11 @@ class Context:
                   "asend_mg_set_smoother")
 
     # -------------------------------------------------- learned preconditioner --
+    def mg_set_cycle_precision(self, bits=16):
+        """Finest-level arithmetic inside the multigrid cycle: 16 (FP16
+        row-scaled operator copy, the default) or 32."""
+        self._chk(_lib.asend_mg_set_cycle_precision(self.h, ctypes.c_int(bits)), "asend_mg_set_cycle_precision")
+
     def learned_init(self, seed=1):
         self._chk(_lib.asend_learned_init(self.h, ctypes.c_uint64(seed)), "asend_learned_init")
 
@@ -509,8 +516,10 @@ class Context:
         self._chk(_lib.asend_learned_set_params(self.h, _p(a), a.size), "asend_learned_set_params")
 
     def learned_train(self, n_steps, lr=1e-3, seed=1, mode=1):
-        """mode 0: Gaussian pairs, one-step loss; 1: solver samples, one-step
-        loss; 2: end to end through one multigrid V-cycle."""
+        """mode 0: Gaussian pairs, one-step loss; 1: solver-generated samples
+        (V-cycle errors and V-cycle solutions), one-step loss; 2: end to end
+        through one multigrid V-cycle; 3: as 1 with a Gaussian sample before
+        every solver pair."""
         losses = np.zeros(max(n_steps, 1), dtype=np.float64)
         self._chk(_lib.asend_learned_train(self.h, int(n_steps), ctypes.c_float(lr), ctypes.c_uint64(seed),
                                            int(mode), _p(losses)), "asend_learned_train")

@@ -64,6 +64,10 @@ struct asend_ctx {
     double *d_khat = nullptr, *d_mhat = nullptr;
     double* d_K64 = nullptr;
     float* d_K32 = nullptr;
+    // FP16 row-scaled copy read by the multigrid cycle (allocated with the
+    // hierarchy); cycle_half selects it (asend_mg_set_cycle_precision)
+    __half* d_K16 = nullptr;
+    bool cycle_half = true;
     double* d_dinv64 = nullptr;
     float* d_dinv32 = nullptr;
 
@@ -123,6 +127,10 @@ struct asend_ctx {
     int spmv_wpb = 4;
     double tuned_inner_tol = 0.0;
     int tuned_check = 0;
+    // set by the learned autotuning when the pipeline without the network
+    // was the faster one: ASEND_PC_LEARNED solves then run the plain
+    // multigrid smoother (stats.fallback_used = 2); a changed network clears it
+    bool learned_aside = false;
 
     // Newmark time stepping: mass operator, state and per-run buffers
     struct Newmark {
@@ -298,6 +306,7 @@ void free_mesh(asend_ctx* ctx) {
     ctx->uniform_hex = false;
     dfree(ctx->d_K64);
     dfree(ctx->d_K32);
+    dfree(ctx->d_K16);
     dfree(ctx->d_dinv64);
     dfree(ctx->d_dinv32);
     dfree(ctx->d_f);
@@ -734,7 +743,9 @@ int solve_device(asend_ctx* ctx, const asend_solve_opts* o, asend_solve_stats* s
         if (total_it >= max_it) break;
         launch_scale_to_f32(ctx->st, n, ctx->d_r, ctx->d_r32, ctx->d_sc + 0);
         int64_t it_in = 0;
-        const bool learned_pc = (o->precond == ASEND_PC_LEARNED || o->precond == ASEND_PC_LEARNED_PCG);
+        const bool aside = (o->precond == ASEND_PC_LEARNED && ctx->learned_aside);
+        const bool learned_pc = !aside && (o->precond == ASEND_PC_LEARNED || o->precond == ASEND_PC_LEARNED_PCG);
+        if (aside) s.fallback_used = 2;
         bool use_learned = learned_pc && !s.fallback_used;
         int rc_pre = ASEND_OK;
         if (learned_pc) {
@@ -778,7 +789,8 @@ int solve_device(asend_ctx* ctx, const asend_solve_opts* o, asend_solve_stats* s
         } else if (o->precond == ASEND_PC_JACOBI || learned_pc) {
             if ((rc = run_jacobi_fpcg_ru(ctx, ru, rr, ff, o->tol, max_it - total_it, check, &it_in))) return rc;
         } else {
-            if (o->precond != ASEND_PC_MG) return fail(ctx, ASEND_ERR_ARG, "unknown preconditioner %d", o->precond);
+            if (o->precond != ASEND_PC_MG && !aside)
+                return fail(ctx, ASEND_ERR_ARG, "unknown preconditioner %d", o->precond);
             if ((rc = ensure_mg(ctx))) return rc;
             mg_set_fine_smoother(ctx->mg, 0);
             int fb = 0;
@@ -849,6 +861,14 @@ void launch_sens(asend_ctx* ctx, const double* u, const double* lam) {
                          ctx->d_sens);
 }
 
+// FP16 row-scaled copy of the current operator for the multigrid cycle,
+// when the hierarchy exists (after the Jacobi diagonal).
+void refresh_f16(asend_ctx* ctx) {
+    if (ctx->d_K16)
+        launch_to_f16_rowscaled(ctx->st, (int)ctx->n_nodes, (int)ctx->n_slices, ctx->d_sptr, ctx->d_K64,
+                                ctx->d_dinv64, ctx->d_K16);
+}
+
 // Penalty scale (fixed on the first assembly), penalty diagonal, Jacobi
 // diagonal and FP32 copy of the element operator held in d_K64
 int finish_assembly(asend_ctx* ctx, double k_scale, double c_mass) {
@@ -876,6 +896,7 @@ int finish_assembly(asend_ctx* ctx, double k_scale, double c_mass) {
     launch_diag_inv(ctx->st, (int)ctx->n_nodes, ctx->d_sptr, ctx->d_kdiag, ctx->d_K64, ctx->d_dinv64,
                     ctx->d_dinv32);
     launch_to_f32(ctx->st, ctx->d_K64, ctx->d_K32, nval);
+    refresh_f16(ctx);
     HIPCHK(hipGetLastError());
     ctx->assembled = true;
     ctx->asm_count += 1;
@@ -1402,6 +1423,20 @@ int asend_set_grid(asend_ctx* ctx, int64_t nx, int64_t ny, int64_t nz) {
         ctx->mg = nullptr;
         return fail(ctx, rc, "%s", m.c_str());
     }
+    if (!ctx->d_K16 && (rc = dalloc(ctx, &ctx->d_K16, ctx->nval()))) return rc;
+    mg_set_fine_half(ctx->mg, ctx->d_K16, ctx->d_dinv32);
+    mg_set_cycle_half(ctx->mg, ctx->cycle_half);
+    if (ctx->assembled) {
+        refresh_f16(ctx);
+        HIPCHK(hipGetLastError());
+    }
+    return ASEND_OK;
+}
+
+int asend_mg_set_cycle_precision(asend_ctx* ctx, int bits) {
+    if (!ctx || (bits != 16 && bits != 32)) return ASEND_ERR_ARG;
+    ctx->cycle_half = (bits == 16);
+    if (ctx->mg) mg_set_cycle_half(ctx->mg, ctx->cycle_half);
     return ASEND_OK;
 }
 
@@ -2342,6 +2377,7 @@ int asend_modal(asend_ctx* ctx, int n_modes, double shift, const asend_solve_opt
         launch_diag_inv(ctx->st, (int)ctx->n_nodes, ctx->d_sptr, ctx->d_kdiag, ctx->d_K64, ctx->d_dinv64,
                         ctx->d_dinv32);
         launch_to_f32(ctx->st, ctx->d_K64, ctx->d_K32, nval);
+        refresh_f16(ctx);
         HIPCHK(hipGetLastError());
     }
     // Lanczos on B = S (K - shift M)^-1 S, S = diag(sqrt(m)), with full
@@ -2464,6 +2500,7 @@ int asend_learned_num_params(void) { return kNumParams; }
 
 int asend_learned_init(asend_ctx* ctx, uint64_t seed) {
     if (!ctx) return ASEND_ERR_ARG;
+    ctx->learned_aside = false;
     HIPCHK(hipSetDevice(ctx->device));
     invalidate_factor(ctx);
     if (ctx->learned) learned_destroy(ctx->learned);
@@ -2489,6 +2526,7 @@ int asend_learned_get_params(asend_ctx* ctx, float* out, int n) {
 
 int asend_learned_set_params(asend_ctx* ctx, const float* in, int n) {
     if (!ctx || !in) return ASEND_ERR_ARG;
+    ctx->learned_aside = false;
     if (!ctx->learned) return fail(ctx, ASEND_ERR_STATE, "no learned preconditioner (asend_learned_init)");
     if (n != kNumParams) return fail(ctx, ASEND_ERR_ARG, "expected %d parameters", kNumParams);
     invalidate_factor(ctx);
@@ -2497,7 +2535,8 @@ int asend_learned_set_params(asend_ctx* ctx, const float* in, int n) {
 }
 
 int asend_learned_train(asend_ctx* ctx, int n_steps, float lr, uint64_t seed, int mode, double* losses) {
-    if (!ctx || n_steps < 0 || !(lr > 0.f) || mode < 0 || mode > 2) return ASEND_ERR_ARG;
+    if (!ctx || n_steps < 0 || !(lr > 0.f) || mode < 0 || mode > 3) return ASEND_ERR_ARG;
+    ctx->learned_aside = false;
     if (ctx->d_K64 && ctx->etype != ASEND_BRICK8)
         return fail(ctx, ASEND_ERR_UNSUPPORTED, "the learned preconditioner is defined on brick8 meshes");
     if (!ctx->learned) return fail(ctx, ASEND_ERR_STATE, "no learned preconditioner (asend_learned_init)");
@@ -2508,7 +2547,10 @@ int asend_learned_train(asend_ctx* ctx, int n_steps, float lr, uint64_t seed, in
     invalidate_factor(ctx);
     const LearnedGraph g = learned_graph(ctx);
     const int64_t n = ctx->n_dof;
-    const bool solver_samples = (mode == 1);
+    // mode 1: solver samples only, alternating the error a V-cycle leaves and
+    // a V-cycle solution of a random load; mode 3 puts a Gaussian sample
+    // before every such pair
+    const bool solver_samples = (mode == 1 || mode == 3);
     double ema = 0.0;  // running level of the accepted mode-2 losses
     for (int k = 0; k < n_steps; ++k) {
         const uint64_t s = seed * 1000003ull + (uint64_t)k;
@@ -2535,7 +2577,7 @@ int asend_learned_train(asend_ctx* ctx, int n_steps, float lr, uint64_t seed, in
             continue;
         }
         const float* e = nullptr;
-        const int kind = solver_samples ? k % 3 : 0;
+        const int kind = !solver_samples ? 0 : (mode == 1 ? 1 + k % 2 : k % 3);
         if (kind == 1) {
             // error left by one V-cycle B of the solver: e = (I - B A) e0
             learned_sample(ctx->st, g, s, ctx->d_p32);
@@ -2750,6 +2792,13 @@ int asend_autotune(asend_ctx* ctx, const double* rhs, int what, asend_tuning* ou
         double best = 1e300;
         int bd = 2, bc = 4;
         double bt = 1e-3;
+        // learned tuning also tunes the same solve without the network and
+        // keeps the faster pipeline: the network steps aside when it does not
+        // pay for itself (ASEND_PC_LEARNED solves then report fallback_used 2)
+        ctx->learned_aside = false;
+        double best_pc[2] = {1e300, 1e300};  // [0] multigrid smoother, [1] learned
+        bool best_learned = learned;
+        for (int pass = learned ? 1 : 0; pass >= 0; --pass)
         for (int d : degrees) {
             if ((rc = mg_set_smoother(ctx->mg, 1, d, 0.1, 1.15)) || (rc = mg_refresh_coefficients(ctx->mg, ctx->st)))
                 return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
@@ -2759,7 +2808,7 @@ int asend_autotune(asend_ctx* ctx, const double* rhs, int what, asend_tuning* ou
                     asend_solve_opts o;
                     std::memset(&o, 0, sizeof(o));
                     o.method = ASEND_SOLVE_IR_MIXED;
-                    o.precond = learned ? ASEND_PC_LEARNED : ASEND_PC_MG;
+                    o.precond = pass == 1 ? ASEND_PC_LEARNED : ASEND_PC_MG;
                     o.stop_rule = ASEND_STOP_REL_RESIDUAL;
                     o.tol = 1e-8;
                     o.inner_tol = t;
@@ -2782,11 +2831,13 @@ int asend_autotune(asend_ctx* ctx, const double* rhs, int what, asend_tuning* ou
                         ok = ok && st.converged && !st.fallback_used;
                     }
                     out->n_trials += 1;
+                    if (ok) best_pc[pass] = std::min(best_pc[pass], (double)ms);
                     if (ok && ms < best) {
                         best = ms;
                         bd = d;
                         bt = t;
                         bc = c;
+                        best_learned = (pass == 1);
                     }
                 }
         }
@@ -2794,7 +2845,11 @@ int asend_autotune(asend_ctx* ctx, const double* rhs, int what, asend_tuning* ou
             return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
         ctx->tuned_inner_tol = bt;
         ctx->tuned_check = bc;
+        ctx->learned_aside = learned && !best_learned;
         out->solve_ms = best;
+        out->learned_active = learned && best_learned ? 1 : 0;
+        out->solve_ms_learned = learned ? best_pc[1] : 0.0;
+        out->solve_ms_no_network = best_pc[0];
     }
     if (ctx->mg) out->cheb_degree = mg_smoother_degree(ctx->mg);
     out->inner_tol = ctx->tuned_inner_tol;

@@ -714,6 +714,105 @@ template void launch_sell3<float>(hipStream_t, int, const Sell3&, const float*, 
                                   double*, Scalars*);
 
 // ---------------------------------------------------------------------------
+// FP16 row-scaled copy of the sliced-ELL operator and its SpMV.  The
+// multigrid cycle only preconditions the FP32 flexible CG (whose own
+// operator products stay FP32, and whose convergence the FP64 residual
+// decides), so its smoothing and restriction residuals can read 2-byte
+// values: 22 instead of 40 bytes per 3x3 block and lane.  Row scaling by the
+// diagonal keeps every entry O(1): |K_dj| / K_dd <= sqrt(K_jj / K_dd) for an
+// SPD operator, far inside FP16 range, and penalised rows become a unit
+// diagonal with tiny couplings.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kWave) void k_to_f16_rowscaled(int n_rows, const int32_t* __restrict__ sptr,
+                                                            const double* __restrict__ K,
+                                                            const double* __restrict__ dinv64,
+                                                            __half* __restrict__ K16) {
+    const int slice = blockIdx.x, lane = threadIdx.x;
+    const int r = slice * kWave + lane;
+    double s[3] = {0.0, 0.0, 0.0};
+    if (r < n_rows)
+        for (int a = 0; a < 3; ++a) s[a] = dinv64[3 * (int64_t)r + a];
+    for (int k = sptr[slice]; k < sptr[slice + 1]; ++k)
+        for (int c = 0; c < 9; ++c) {
+            const int64_t i = ((int64_t)9 * k + c) * kWave + lane;
+            K16[i] = __float2half((float)(K[i] * s[c / 3]));
+        }
+}
+
+void launch_to_f16_rowscaled(hipStream_t st, int n_rows, int n_slices, const int32_t* sptr, const double* K,
+                             const double* dinv64, __half* K16) {
+    hipLaunchKernelGGL(k_to_f16_rowscaled, dim3(n_slices), dim3(kWave), 0, st, n_rows, sptr, K, dinv64, K16);
+}
+
+template <int MODE, bool NT>
+__global__ __launch_bounds__(kSellMaxWaves * kWave) void k_sell3h(int n_rows, int n_slices,
+                                                                  const int32_t* __restrict__ sptr,
+                                                                  const int32_t* __restrict__ scol,
+                                                                  const __half* __restrict__ val,
+                                                                  const float* __restrict__ dinv,
+                                                                  const float* __restrict__ x, float* __restrict__ y,
+                                                                  const float* __restrict__ f) {
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int slice = lb * (int)(blockDim.x / kWave) + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    const int r = slice * kWave + lane;
+    if (slice >= n_slices) return;
+    const int k0 = sptr[slice], k1 = sptr[slice + 1];
+    const int32_t* cp = scol + (int64_t)k0 * kWave + lane;
+    const __half* vp = val + (int64_t)k0 * 9 * kWave + lane;
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+#pragma unroll 2
+    for (int k = k0; k < k1; ++k) {
+        const int c = sell_load<NT>(cp);
+        const float x0 = x[3 * (int64_t)c], x1 = x[3 * (int64_t)c + 1], x2 = x[3 * (int64_t)c + 2];
+        float v[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            const unsigned short h = sell_load<NT>((const unsigned short*)(vp + q * kWave));
+            v[q] = __half2float(__ushort_as_half(h));
+        }
+        a0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
+        a1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
+        a2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
+        cp += kWave;
+        vp += 9 * kWave;
+    }
+    if (r >= n_rows) return;
+    const int64_t d = 3 * (int64_t)r;
+    a0 /= dinv[d];
+    a1 /= dinv[d + 1];
+    a2 /= dinv[d + 2];
+    if (MODE == 3) {
+        y[d] = f[d] - a0;
+        y[d + 1] = f[d + 1] - a1;
+        y[d + 2] = f[d + 2] - a2;
+    } else {
+        y[d] = a0;
+        y[d + 1] = a1;
+        y[d + 2] = a2;
+    }
+}
+
+void launch_sell3_h(hipStream_t st, int mode, const Sell3& A, const __half* val, const float* dinv32, const float* x,
+                    float* y, const float* f) {
+    const int per = A.waves_per_block;
+    const int nblk = (A.n_slices + per - 1) / per;
+    const int nthr = per * kWave;
+    const bool nt = (int64_t)sizeof(__half) * 9 * kWave * A.n_slots > kSellStreamBytes;
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(nblk), dim3(nthr), 0, st, A.n_rows, A.n_slices, A.sptr, A.scol, val, dinv32, x,
+                           y, f);
+    };
+    if (mode == 3) {
+        if (nt) go(k_sell3h<3, true>);
+        else go(k_sell3h<3, false>);
+    } else {
+        if (nt) go(k_sell3h<0, true>);
+        else go(k_sell3h<0, false>);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Block-row SpMV, b = 3, G lanes per block row, component-major values.
 // MODE 0: y = K x
 // MODE 1: y = K x and (x . y) -> finalize: sc->pq, sc->alpha = sc->rz / pq

@@ -1,6 +1,7 @@
 // Launch wrappers of the solve-path kernels (definitions in kernels.hip).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
 #include <cstdint>
 
 namespace asend {
@@ -95,6 +96,16 @@ void launch_diag_inv(hipStream_t st, int n_rows, const int32_t* sptr, const int3
 template <typename T>
 void launch_sell3(hipStream_t st, int mode, const Sell3& A, const T* val, const T* x, T* y, const T* f,
                   double* partials, Scalars* sc);
+
+// FP16 copy of the fine operator for the multigrid cycle: every value of DOF
+// row d is stored as K_dj / K_dd (so entries are O(1) and within FP16 range
+// whatever the moduli and the penalty scale), one wavefront per slice.
+void launch_to_f16_rowscaled(hipStream_t st, int n_rows, int n_slices, const int32_t* sptr, const double* K,
+                             const double* dinv64, __half* K16);
+// SpMV with that copy, FP32 accumulation, the row scale restored from the
+// FP32 Jacobi diagonal: mode 0: y = K x; mode 3: y = f - K x.
+void launch_sell3_h(hipStream_t st, int mode, const Sell3& A, const __half* val, const float* dinv32, const float* x,
+                    float* y, const float* f);
 
 // mode 0: y = K x; 1: y = K x, sc->pq = x.y, sc->alpha = sc->rz/pq; 2: y = f - K x, sc->rr = |y|^2
 // 3: y = f - K x (no norm).  Modes 1 and 2 launch their single-block finalize kernel after the SpMV.

@@ -317,6 +317,64 @@ __global__ __launch_bounds__(256) void k_factor_transpose(int n, int W, const fl
     }
 }
 
+// FP16 copy of a factor in the row layout: each node's row is scaled by the
+// largest magnitude among its (1 + W) blocks, which keeps every stored entry
+// in [-1, 1] whatever the moduli and penalty (the node blocks carry the
+// inverse Cholesky factor of the operator's diagonal block); sc restores it.
+__global__ __launch_bounds__(256) void k_factor_to16(int n, int W, const float* __restrict__ L,
+                                                     __half* __restrict__ L16, float* __restrict__ sc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int nb = 9 * (W + 1);
+    float m = 0.f;
+    for (int k = 0; k < nb; ++k) m = fmaxf(m, fabsf(L[(int64_t)k * n + i]));
+    const float s = m > 0.f ? 1.f / m : 1.f;
+    for (int k = 0; k < nb; ++k) L16[(int64_t)k * n + i] = __float2half(L[(int64_t)k * n + i] * s);
+    sc[i] = m > 0.f ? m : 1.f;
+}
+
+// z_i = sc_i sum_s L16_is t_{col(s)} with the FP16 copy.  MODE 0 stores z;
+// MODE 1 / 2 fuse the first / a later Chebyshev step: v = c2 z (+ c1 d),
+// d = v, x = v (x += v).
+template <int MODE>
+__global__ __launch_bounds__(256) void k_factor_l16(int n, int W, const __half* __restrict__ L16,
+                                                    const float* __restrict__ sc, const int32_t* __restrict__ enb,
+                                                    const float* __restrict__ t, float* __restrict__ z,
+                                                    const float* __restrict__ coef, float* __restrict__ d) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float t0 = t[3 * i], t1 = t[3 * i + 1], t2 = t[3 * i + 2];
+    float b[9];
+    for (int c = 0; c < 9; ++c) b[c] = __half2float(L16[(int64_t)c * n + i]);
+    float z0 = b[0] * t0 + b[1] * t1 + b[2] * t2;
+    float z1 = b[3] * t0 + b[4] * t1 + b[5] * t2;
+    float z2 = b[6] * t0 + b[7] * t1 + b[8] * t2;
+    for (int s = 0; s < W; ++s) {
+        const int j = enb[(int64_t)s * n + i];
+        if (j < 0) continue;
+        for (int c = 0; c < 9; ++c) b[c] = __half2float(L16[((int64_t)(s + 1) * 9 + c) * n + i]);
+        t0 = t[3 * j];
+        t1 = t[3 * j + 1];
+        t2 = t[3 * j + 2];
+        z0 += b[0] * t0 + b[1] * t1 + b[2] * t2;
+        z1 += b[3] * t0 + b[4] * t1 + b[5] * t2;
+        z2 += b[6] * t0 + b[7] * t1 + b[8] * t2;
+    }
+    const float m = sc[i];
+    const float zz[3] = {m * z0, m * z1, m * z2};
+    if (MODE == 0) {
+        for (int c = 0; c < 3; ++c) z[3 * i + c] = zz[c];
+        return;
+    }
+    const float c1 = coef[0], c2 = coef[1];
+    for (int c = 0; c < 3; ++c) {
+        const float v = (MODE == 1) ? c2 * zz[c] : c1 * d[3 * i + c] + c2 * zz[c];
+        d[3 * i + c] = v;
+        if (MODE == 1) z[3 * i + c] = v;
+        else z[3 * i + c] += v;
+    }
+}
+
 // z_i = sum_s L_is t_{col(s)};  SUB: z = e - (L t)
 template <bool SUB>
 __global__ __launch_bounds__(256) void k_factor_l(int n, int W, const float* __restrict__ Ls,
@@ -669,6 +727,27 @@ __global__ void k_refresh_bf16(const float* __restrict__ p, bf16* __restrict__ w
 
 }  // namespace
 
+void launch_factor_apply16(hipStream_t st, const Factor& F, const float* r, float* t, float* z) {
+    const int nb = blocks_for(F.n, 256);
+    hipLaunchKernelGGL((k_factor_l16<0>), dim3(nb), dim3(256), 0, st, F.n, F.n_edge, F.LsT16, F.scT16, F.enb, r, t,
+                       (const float*)nullptr, (float*)nullptr);
+    hipLaunchKernelGGL((k_factor_l16<0>), dim3(nb), dim3(256), 0, st, F.n, F.n_edge, F.Ls16, F.sc16, F.enb, t, z,
+                       (const float*)nullptr, (float*)nullptr);
+}
+
+void launch_factor_cheb16(hipStream_t st, const Factor& F, const float* r, float* t, const float* coef, float* d,
+                          float* x, bool first) {
+    const int nb = blocks_for(F.n, 256);
+    hipLaunchKernelGGL((k_factor_l16<0>), dim3(nb), dim3(256), 0, st, F.n, F.n_edge, F.LsT16, F.scT16, F.enb, r, t,
+                       (const float*)nullptr, (float*)nullptr);
+    if (first)
+        hipLaunchKernelGGL((k_factor_l16<1>), dim3(nb), dim3(256), 0, st, F.n, F.n_edge, F.Ls16, F.sc16, F.enb, t, x,
+                           coef, d);
+    else
+        hipLaunchKernelGGL((k_factor_l16<2>), dim3(nb), dim3(256), 0, st, F.n, F.n_edge, F.Ls16, F.sc16, F.enb, t, x,
+                           coef, d);
+}
+
 void launch_factor_apply(hipStream_t st, const Factor& F, const float* r, float* t, float* z) {
     const int nb = blocks_for(F.n, 256);
     hipLaunchKernelGGL((k_factor_l<false>), dim3(nb), dim3(256), 0, st, F.n, F.n_edge, F.LsT, F.enb, r,
@@ -691,6 +770,8 @@ struct Learned {
     float *dH = nullptr, *dX = nullptr, *O = nullptr, *dO = nullptr, *dz0 = nullptr;
     bf16* dz = nullptr;
     float *Ci = nullptr, *Ahat = nullptr, *Ls = nullptr, *LsT = nullptr, *DOE = nullptr, *Gea = nullptr;
+    __half *Ls16 = nullptr, *LsT16 = nullptr;  // FP16 copies for the smoother and their row scales
+    float *sc16 = nullptr, *scT16 = nullptr;
     float *e = nullptr, *r = nullptr, *t = nullptr, *y = nullptr, *w = nullptr, *u = nullptr;
     float* wpart = nullptr;
     float* pairbuf = nullptr;  // L^T images of the factor-use vectors (2 kMaxUses x 3n)
@@ -729,6 +810,10 @@ void free_work(Learned* lp) {
     lfree(lp->Ahat);
     lfree(lp->Ls);
     lfree(lp->LsT);
+    lfree(lp->Ls16);
+    lfree(lp->LsT16);
+    lfree(lp->sc16);
+    lfree(lp->scT16);
     lfree(lp->DOE);
     lfree(lp->Gea);
     lfree(lp->e);
@@ -761,6 +846,10 @@ int ensure_cap(Learned* lp, int n, int W) {
     LCHK(hipMalloc((void**)&lp->Ahat, wslots * 9 * c * sizeof(float)));
     LCHK(hipMalloc((void**)&lp->Ls, (wslots + 1) * 9 * c * sizeof(float)));
     LCHK(hipMalloc((void**)&lp->LsT, (wslots + 1) * 9 * c * sizeof(float)));
+    LCHK(hipMalloc((void**)&lp->Ls16, (wslots + 1) * 9 * c * sizeof(__half)));
+    LCHK(hipMalloc((void**)&lp->LsT16, (wslots + 1) * 9 * c * sizeof(__half)));
+    LCHK(hipMalloc((void**)&lp->sc16, c * sizeof(float)));
+    LCHK(hipMalloc((void**)&lp->scT16, c * sizeof(float)));
     LCHK(hipMalloc((void**)&lp->DOE, wslots * 9 * c * sizeof(float)));
     LCHK(hipMalloc((void**)&lp->Gea, c * 81 * sizeof(float)));
     for (float** q : {&lp->e, &lp->r, &lp->t, &lp->y, &lp->w, &lp->u})
@@ -779,6 +868,10 @@ Factor factor_of(const Learned* lp, const LearnedGraph& g) {
     F.n_edge = g.n_edge;
     F.Ls = lp->Ls;
     F.LsT = lp->LsT;
+    F.Ls16 = lp->Ls16;
+    F.LsT16 = lp->LsT16;
+    F.sc16 = lp->sc16;
+    F.scT16 = lp->scT16;
     F.enb = g.enb;
     F.erev = g.erev;
     return F;
@@ -807,6 +900,8 @@ void forward(Learned* lp, hipStream_t st, const LearnedGraph& g) {
     hipLaunchKernelGGL(k_factor, dim3(nb), dim3(256), 0, st, n, W, lp->O, lp->Ci, lp->Ahat, g.enb, lp->wbf + kOffWa,
                        lp->Ls);
     hipLaunchKernelGGL(k_factor_transpose, dim3(nb), dim3(256), 0, st, n, W, lp->Ls, g.enb, g.erev, lp->LsT);
+    hipLaunchKernelGGL(k_factor_to16, dim3(nb), dim3(256), 0, st, n, W, lp->Ls, lp->Ls16, lp->sc16);
+    hipLaunchKernelGGL(k_factor_to16, dim3(nb), dim3(256), 0, st, n, W, lp->LsT, lp->LsT16, lp->scT16);
 }
 
 void backward_from_pairs(Learned* lp, hipStream_t st, const LearnedGraph& g, const PairSet& ps);

@@ -56,11 +56,24 @@ struct Factor {
     const float* LsT = nullptr;  // L^T in the same row layout (block s of row j = (L_ij)^T)
     const int32_t* enb = nullptr;
     const int32_t* erev = nullptr;
+    // FP16 copies of Ls and LsT for the smoother, each node's row scaled by
+    // the largest magnitude in it (sc16 / scT16 restore the scale)
+    const __half* Ls16 = nullptr;
+    const __half* LsT16 = nullptr;
+    const float* sc16 = nullptr;
+    const float* scT16 = nullptr;
 };
 
 // z = L L^T r (t is an n-vector of scratch): two coalesced row gathers, one
 // over L^T and one over L; no atomics.
 void launch_factor_apply(hipStream_t st, const Factor& F, const float* r, float* t, float* z);
+// The same product with the FP16 copies (FP32 arithmetic).
+void launch_factor_apply16(hipStream_t st, const Factor& F, const float* r, float* t, float* z);
+// One Chebyshev step with P = L L^T from the FP16 copies: t = L^T r, then,
+// fused into the second product, z = L t and (coef = {c1, c2} in device
+// memory) first: d = c2 z, x = d; otherwise d = c1 d + c2 z, x += d.
+void launch_factor_cheb16(hipStream_t st, const Factor& F, const float* r, float* t, const float* coef, float* d,
+                          float* x, bool first);
 
 // The factor's uses inside a larger linear computation: P was applied to
 // v[k] (z_k = P v_k) and zbar[k] is dLoss/dz_k, so dLoss/dP = sum_k zbar_k v_k^T.

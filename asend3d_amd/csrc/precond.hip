@@ -141,6 +141,11 @@ struct Multigrid {
     const int32_t* d_conn_r = nullptr;
     int64_t n_el0 = 0;
     Sell3 fine;                        // level-0 operator layout (sliced ELL)
+    // FP16 row-scaled level-0 values and the diagonal that restores them,
+    // read by the cycle's residuals and the smoother bounds when cycle_half
+    const __half* val16 = nullptr;
+    const float* dinv16 = nullptr;
+    bool cycle_half = false;
     // coarsest dense inverse
     int nco = 0;
     double* d_Ainv = nullptr;
@@ -156,6 +161,7 @@ struct Multigrid {
     Factor pw_factor;
     const float* pw_val = nullptr;
     int pw_waves = 0;
+    int pw_half = -1;
     // coarse-level smoother bounds: estimated and turned into Chebyshev
     // coefficients on the device during setup, read back once (d_lam ->
     // h_lam on lam_stream) when the host first needs them
@@ -172,7 +178,7 @@ struct Multigrid {
     // captured FCG iteration blocks (hipGraph), reused while their key matches
     struct FcgGraph {
         hipGraphExec_t exec = nullptr;
-        int precond = -1, fine_smoother = -1, check = -1, degree = -1, kind = -1, waves = -1;
+        int precond = -1, fine_smoother = -1, check = -1, degree = -1, kind = -1, waves = -1, half = -1;
         const float* val0 = nullptr;  // finest-level operator values
         const float* b = nullptr;
         float* x = nullptr;
@@ -205,6 +211,8 @@ struct Multigrid {
 };
 
 namespace {
+
+bool cycle_half(const Multigrid* mg) { return mg->cycle_half && mg->val16 != nullptr; }
 
 int coef_slot(const Multigrid* mg, int level, int step) {
     (void)mg;
@@ -1026,9 +1034,18 @@ int apply_level(Multigrid* mg, hipStream_t st, int l, const float* x, float* y) 
     return ASEND_OK;
 }
 
+// The operator the cycle itself applies on level l: on the finest level the
+// FP16 copy when it is selected (smoothing and restriction residuals, and
+// the power iterations that bound the smoother on it).
+void apply_cycle_op(Multigrid* mg, hipStream_t st, int l, const float* x, float* y) {
+    if (l == 0 && cycle_half(mg)) launch_sell3_h(st, 0, mg->fine, mg->val16, mg->dinv16, x, y, nullptr);
+    else apply_level(mg, st, l, x, y);
+}
+
 void residual_level(Multigrid* mg, hipStream_t st, int l, const float* b, const float* x, float* r) {
     Level& lv = mg->L[l];
-    if (l == 0) launch_sell3<float>(st, 3, mg->fine, lv.d_val, x, r, b, nullptr, nullptr);
+    if (l == 0 && cycle_half(mg)) launch_sell3_h(st, 3, mg->fine, mg->val16, mg->dinv16, x, r, b);
+    else if (l == 0) launch_sell3<float>(st, 3, mg->fine, lv.d_val, x, r, b, nullptr, nullptr);
     else launch_spmv3<float>(st, 3, (int)lv.n_nodes, lv.d_rp, lv.d_bc, lv.d_val, x, r, b, nullptr, nullptr);
 }
 
@@ -1057,6 +1074,11 @@ void smooth(Multigrid* mg, hipStream_t st, int l, const float* b, float* x, bool
             else
                 hipLaunchKernelGGL(k_cheb_block<false>, dim3(nblk), dim3(256), 0, st, (int)lv.n_nodes, lv.binv, r,
                                    coef + 2 * k, lv.d, x);
+            continue;
+        }
+        if (learned && cycle_half(mg) && mg->factor.Ls16) {
+            // FP16 factor, Chebyshev step fused into its second product
+            launch_factor_cheb16(st, mg->factor, r, mg->ft, coef + 2 * k, lv.d, x, first);
             continue;
         }
         if (learned) {
@@ -1118,10 +1140,11 @@ void enqueue_power(Multigrid* mg, hipStream_t st, int l, bool learned, double* p
     hipLaunchKernelGGL(k_fill_hash, dim3(nb_for(n)), dim3(256), 0, st, n, lv.x);
     const int nblk = kVecBlocks;
     for (int it = 0; it < kPowerIts; ++it) {
-        apply_level(mg, st, l, lv.x, lv.r);
+        apply_cycle_op(mg, st, l, lv.x, lv.r);
         float* w = lv.r;
         if (learned) {
-            launch_factor_apply(st, mg->factor, lv.r, mg->ft, mg->fz);
+            if (cycle_half(mg) && mg->factor.Ls16) launch_factor_apply16(st, mg->factor, lv.r, mg->ft, mg->fz);
+            else launch_factor_apply(st, mg->factor, lv.r, mg->ft, mg->fz);
             w = mg->fz;
         }
         if (!learned && mg->smoother_kind == 1)
@@ -1389,6 +1412,21 @@ int mg_refresh_fine(Multigrid* mg, hipStream_t st, const float* d_K32, const flo
 
 void mg_set_fine_geometry(Multigrid* mg, int waves_per_block) { mg->fine.waves_per_block = waves_per_block; }
 
+void mg_set_fine_half(Multigrid* mg, const __half* val16, const float* dinv32) {
+    mg->val16 = val16;
+    mg->dinv16 = dinv32;
+}
+
+// The finest level's smoother bound is estimated on the operator the cycle
+// applies: switching it re-estimates that bound on first use.
+void mg_set_cycle_half(Multigrid* mg, bool on) {
+    if (mg->cycle_half != on && !mg->L.empty()) {
+        mg->L[0].lmax = 0.0;
+        mg->lmax_factor = 0.0;
+    }
+    mg->cycle_half = on;
+}
+
 int mg_smoother_degree(const Multigrid* mg) { return mg->cheb_degree;
 }
 
@@ -1455,7 +1493,7 @@ int mg_learned_bound_begin(Multigrid* mg, hipStream_t st, const Factor& F, const
         const Factor& P = mg->pw_factor;
         const bool same = mg->pw_exec && P.n == F.n && P.n_edge == F.n_edge && P.Ls == F.Ls && P.LsT == F.LsT &&
                           P.enb == F.enb && P.erev == F.erev && mg->pw_val == d_K32 &&
-                          mg->pw_waves == mg->fine.waves_per_block;
+                          mg->pw_waves == mg->fine.waves_per_block && mg->pw_half == (int)cycle_half(mg);
         if (!same) {
             if (mg->pw_exec) (void)hipGraphExecDestroy(mg->pw_exec);
             mg->pw_exec = nullptr;
@@ -1477,6 +1515,7 @@ int mg_learned_bound_begin(Multigrid* mg, hipStream_t st, const Factor& F, const
             mg->pw_factor = F;
             mg->pw_val = d_K32;
             mg->pw_waves = mg->fine.waves_per_block;
+            mg->pw_half = (int)cycle_half(mg);
         }
         MGCHK(hipGraphLaunch(mg->pw_exec, st));
     } else {
@@ -1723,7 +1762,8 @@ int mg_fcg_solve(Multigrid* mg, hipStream_t st, const float* b, float* x, double
         for (auto& g : mg->graphs)
             if (g.exec && g.precond == precond && g.fine_smoother == mg->fine_smoother && g.check == check_every &&
                 g.degree == mg->cheb_degree && g.kind == mg->smoother_kind && g.b == b && g.x == x && g.ls == ls &&
-                g.lst == lst && g.waves == mg->fine.waves_per_block && g.val0 == l0.d_val)
+                g.lst == lst && g.waves == mg->fine.waves_per_block && g.val0 == l0.d_val &&
+                g.half == (int)cycle_half(mg))
                 gr = &g;
         if (!gr) {
             // capture into the least recently used slot
@@ -1758,6 +1798,7 @@ int mg_fcg_solve(Multigrid* mg, hipStream_t st, const float* b, float* x, double
             gr->lst = lst;
             gr->waves = mg->fine.waves_per_block;
             gr->val0 = l0.d_val;
+            gr->half = (int)cycle_half(mg);
         }
         gr->use = ++mg->graph_clock;
     }
@@ -1871,9 +1912,9 @@ int mg_vcycle_uses(Multigrid* mg, hipStream_t st, const float* e, FactorUses* us
         const int k = j % D;
         hipLaunchKernelGGL(k_cheb_adjoint, dim3(nv), dim3(256), 0, st, n, c1[k], c2[k], mg->txb, mg->tdb, mg->tz[j]);
         if (j > 0) {
-            // v_j = b - A x_prev:  xbar -= A P zbar_j
+            // v_j = b - A x_prev:  xbar -= A P zbar_j (A: the cycle's operator)
             launch_factor_apply(st, mg->factor, mg->tz[j], mg->ft, mg->fz);
-            apply_level(mg, st, 0, mg->fz, mg->tq);
+            apply_cycle_op(mg, st, 0, mg->fz, mg->tq);
             hipLaunchKernelGGL(k_lincomb, dim3(nv), dim3(256), 0, st, n, 1.f, mg->txb, -1.f, mg->tq, mg->txb);
         }
         if (j == D) {
@@ -1882,7 +1923,7 @@ int mg_vcycle_uses(Multigrid* mg, hipStream_t st, const float* e, FactorUses* us
             vcycle(mg, st, 1, cv.b, cv.x);
             hipLaunchKernelGGL(k_fill, dim3(nv), dim3(256), 0, st, n, 0.f, mg->tq);
             prolong_add(mg, st, 0, cv.x, mg->tq);
-            apply_level(mg, st, 0, mg->tq, lv.r);
+            apply_cycle_op(mg, st, 0, mg->tq, lv.r);
             hipLaunchKernelGGL(k_lincomb, dim3(nv), dim3(256), 0, st, n, 1.f, mg->txb, -1.f, lv.r, mg->txb);
         }
     }

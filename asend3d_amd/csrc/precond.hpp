@@ -50,6 +50,12 @@ int mg_refresh_fine(Multigrid* mg, hipStream_t st, const float* d_K32, const flo
 int mg_smoother_degree(const Multigrid* mg);
 // Workgroup geometry of the finest-level SpMV (waves per workgroup).
 void mg_set_fine_geometry(Multigrid* mg, int waves_per_block);
+// FP16 row-scaled copy of the finest operator (launch_to_f16_rowscaled) and
+// the FP32 Jacobi diagonal that restores its rows; with cycle_half on, the
+// cycle's finest-level residuals and the smoother bounds read it (the FCG's
+// own operator products stay FP32).
+void mg_set_fine_half(Multigrid* mg, const __half* val16, const float* dinv32);
+void mg_set_cycle_half(Multigrid* mg, bool on);
 // Register the learned factor (device arrays borrowed, must stay valid while
 // it is in use); with smoother_bounds also estimate lambda_max(P A), which the
 // Chebyshev smoother needs.

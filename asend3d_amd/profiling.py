@@ -3,13 +3,16 @@
 launches of named kernels.
 
     python -m asend3d_amd.profiling TRACE.csv [--after NAME:K] [--until NAME:K] [--top 25] [--gaps 15]
+                                     [--by-grid]
 
 --after NAME:K starts the window at the end of the K-th (1-based) launch of
 the kernel whose name contains NAME; --until NAME:K ends it at the end of the
 K-th launch.  Prints a markdown table: kernel, launches, total, average and
 share of the window's kernel time, plus the window's span; --gaps N adds the N
 longest idle intervals of the device (no kernel running) with the kernels on
-either side, which is where the host held the stream.
+either side, which is where the host held the stream.  --by-grid keys the
+table by kernel and grid size, which separates the launches of one kernel on
+the different levels of a multigrid hierarchy.
 """
 import argparse
 import csv
@@ -17,11 +20,15 @@ import sys
 from collections import defaultdict
 
 
-def load(path):
+def load(path, by_grid=False):
     rows = []
     with open(path, newline="") as fh:
         for r in csv.DictReader(fh):
-            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]))
+            name = r["Kernel_Name"]
+            if by_grid:
+                g = r.get("Grid_Size_X") or r.get("Grid_Size") or "?"
+                name = f"{name} @grid {g}"
+            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), name))
     rows.sort()
     return rows
 
@@ -40,7 +47,11 @@ def short(name, width=90):
     """Kernel name without return type, namespaces' anonymous part and
     argument list."""
     n = name.replace("(anonymous namespace)::", "")
-    n = n.split("(")[0]
+    grid = ""
+    if " @grid " in n:
+        n, grid = n.split(" @grid ")
+        grid = f" [grid {grid}]"
+    n = n.split("(")[0] + grid
     if n.startswith("void "):
         n = n[5:]
     return n if len(n) <= width else n[:width - 3] + "..."
@@ -87,8 +98,9 @@ def main():
     ap.add_argument("--until", default=None)
     ap.add_argument("--top", type=int, default=25)
     ap.add_argument("--gaps", type=int, default=0)
+    ap.add_argument("--by-grid", action="store_true")
     a = ap.parse_args()
-    rows = load(a.trace)
+    rows = load(a.trace, a.by_grid)
     t0 = t1 = None
     if a.after:
         name, k = a.after.rsplit(":", 1)

@@ -12,7 +12,11 @@ mixed-precision path: FP32 flexible PCG inside FP64 refinement, preconditioned
 by a multigrid V-cycle whose finest Chebyshev smoother is the learned factor
 L L^T.  The network is first trained on the GPU (8 lognormal 110x22x22 meshes,
 250 steps each = 2000 solver-generated samples; time reported as
-`train_s`, outside `value`), then inferred inside every timed step.  The plain
+`train_s`, outside `value`), then inferred inside every timed step.  The
+autotuner times the pipeline with and without the network and keeps the
+faster; when that is the one without it, the network steps aside
+(`fallback_used` 2, `ablation.learned_active` false) and the step runs the
+same multigrid with its block-Jacobi smoother.  The plain
 FP64 Jacobi PCG pipeline on the same GPU is timed alongside (`fp64_jacobi_s`,
 `speedup_vs_fp64_jacobi`), and so is the same multigrid without the network
 (`mg_no_network_s`).
@@ -186,7 +190,8 @@ def main():
         tuning = tn.as_dict()
         tuning["tune_s"] = time.perf_counter() - t0
         log(f"autotuned in {tuning['tune_s']:.1f} s: {tn.spmv_waves_per_block} waves/workgroup, Chebyshev degree "
-            f"{tn.cheb_degree}, inner tol {tn.inner_tol:g}, check every {tn.check_every}")
+            f"{tn.cheb_degree}, inner tol {tn.inner_tol:g}, check every {tn.check_every}"
+            + (f", learned smoother {'kept' if tn.learned_active else 'stepped aside'}" if pc == api.PC_LEARNED else ""))
     fast = ctx.opts(api.SOLVE_IR_MIXED, pc, tol=args.tol, inner_tol=1e-4 if pc == api.PC_JACOBI else 0.0,
                     max_it=2000000, max_outer=40, check_every=32 if pc == api.PC_JACOBI else 0)
     base = ctx.opts(api.SOLVE_PCG_FP64, api.PC_JACOBI, tol=args.tol, max_it=2000000, check_every=32)
@@ -218,13 +223,19 @@ def main():
     err_s = float(np.abs(s - sb).max() / np.abs(sb).max())
     ablation = None
     if pc == api.PC_LEARNED:
-        # the same pipeline without the network (block-Jacobi Chebyshev smoother)
+        # the same pipeline without the network (block-Jacobi Chebyshev
+        # smoother) at the tuned settings, and each pipeline's best tuned solve
         mg_o = ctx.opts(api.SOLVE_IR_MIXED, api.PC_MG, tol=args.tol, inner_tol=0.0, max_it=2000000, max_outer=40,
                         check_every=0)
         ctx.static_compliance(prob.f, mg_o)
         _, _, _, stm = ctx.static_compliance(prob.f, mg_o)
-        ablation = {"mg_no_network_s": stm.device_ms / 1e3, "mg_no_network_iterations": int(stm.iterations)}
-        log(f"multigrid without the network: {stm.device_ms / 1e3:.3f} s, {stm.iterations} it")
+        ablation = {"mg_no_network_s": stm.device_ms / 1e3, "mg_no_network_iterations": int(stm.iterations),
+                    "learned_active": bool(tuning["learned_active"]),
+                    "tuned_solve_ms_learned": tuning["solve_ms_learned"],
+                    "tuned_solve_ms_no_network": tuning["solve_ms_no_network"]}
+        log(f"multigrid without the network: {stm.device_ms / 1e3:.3f} s, {stm.iterations} it; tuned solves: "
+            f"learned {tuning['solve_ms_learned']:.2f} ms, no network {tuning['solve_ms_no_network']:.2f} ms, "
+            f"network {'kept' if tuning['learned_active'] else 'stepped aside'}")
 
     # roofline of the dominant kernel: FP32 block-row SpMV, HIP events on the context stream
     import ctypes
@@ -265,6 +276,7 @@ def main():
         "config": {"workload": f"hex8 cantilever {args.nx}x{args.ny}x{args.nz} static + compliance adjoint",
                    "n_dof": prob.n_dof, "n_elements": prob.mesh.n_el, "tol": args.tol,
                    "fast_precond": {api.PC_JACOBI: "jacobi", api.PC_MG: "mg", api.PC_LEARNED: "learned"}[pc],
+                   "cycle_storage": "fp16 row-scaled finest operator and learned factor (fp32 arithmetic)",
                    "training": train,
                    "autotune": tuning,
                    "parallelism": "single GPU"},

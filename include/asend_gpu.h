@@ -86,7 +86,10 @@ typedef struct asend_solve_stats {
     double ref_residual;      /* final g^T M^-1 g of the last Krylov loop     */
     int converged;            /* 1 when the stop rule was met                */
     int fallback_used;        /* 1 when the learned preconditioner was
-                                 rejected and Jacobi PCG took over          */
+                                 rejected and Jacobi PCG took over; 2 when
+                                 the learned autotuning measured the
+                                 multigrid smoother without the network
+                                 faster and the solve ran it              */
     double device_ms;         /* HIP-event time of the solve on the stream   */
 } asend_solve_stats;
 
@@ -460,8 +463,10 @@ int asend_set_graphs(asend_ctx* ctx, int on);
  * {1e-1, 1e-2, 1e-3, 1e-4} x check interval {1, 2, 4} (the faster of two
  * solves each) and keeps the fastest converged choice; a later solve uses the
  * tuned inner tolerance / check interval where its options pass 0.  With
- * ASEND_TUNE_LEARNED as well, the tuned solves use the learned smoother
- * (ASEND_PC_LEARNED), so the choice fits the learned pipeline. */
+ * ASEND_TUNE_LEARNED as well, the same grid is timed with the learned
+ * smoother (ASEND_PC_LEARNED) and without it, and the fastest choice of either
+ * is kept: when it is the one without the network, the network steps aside
+ * (learned_active = 0; ASEND_PC_LEARNED solves report fallback_used = 2). */
 #define ASEND_TUNE_SPMV 1
 #define ASEND_TUNE_SOLVER 2
 #define ASEND_TUNE_LEARNED 4
@@ -474,6 +479,13 @@ typedef struct asend_tuning {
     int check_every;
     double solve_ms;           /* best tuned solve */
     int n_trials;
+    /* ASEND_TUNE_LEARNED: 1 when the learned pipeline was the faster one; 0
+     * when the same solve without the network was (ASEND_PC_LEARNED solves
+     * then run the multigrid smoother alone and report fallback_used = 2,
+     * until the network changes) */
+    int learned_active;
+    double solve_ms_learned;     /* best learned solve (learned tuning) */
+    double solve_ms_no_network;  /* best solve without the network */
 } asend_tuning;
 int asend_autotune(asend_ctx* ctx, const double* rhs, int what, asend_tuning* out);
 
@@ -483,6 +495,14 @@ int asend_autotune(asend_ctx* ctx, const double* rhs, int what, asend_tuning* ou
  * assemblies of the same hierarchy; a new kind rebuilds the hierarchy's
  * smoother data at the next solve. */
 int asend_mg_set_smoother(asend_ctx* ctx, int kind, int degree, double lo_frac, double hi_frac);
+
+/* Arithmetic of the finest level inside the multigrid cycle: 16 (default)
+ * reads an FP16 copy of the operator whose rows are scaled by their diagonal
+ * (entries O(1) whatever the moduli and penalty), FP32 accumulation, for the
+ * smoothing and restriction residuals and the smoother bounds; 32 reads the
+ * FP32 operator.  The flexible CG around the cycle keeps its FP32 operator
+ * products and the FP64 residual refreshes decide convergence either way. */
+int asend_mg_set_cycle_precision(asend_ctx* ctx, int bits);
 
 /* ---- learned preconditioner (no reference counterpart) --------------------
  * A 3-layer message-passing network on the mesh node graph (hidden 64, BF16
@@ -507,9 +527,10 @@ int asend_learned_set_params(asend_ctx* ctx, const float* in, int n);
  * mode 0: one pair (r = K e, e) per step, e Gaussian (zero on penalised
  *   nodes); loss = energy left by the best damped step,
  *   min_w ||e - w P K e||_K^2 / ||e||_K^2 (independent of the scale of P).
- * mode 1: as mode 0, but the steps cycle through a Gaussian e, the error
- *   e = (I - B K) e0 left by one multigrid V-cycle B of the solver, and the
- *   approximate solution e = B b of a random load b.
+ * mode 1: solver-generated samples only: the steps alternate the error
+ *   e = (I - B K) e0 left by one multigrid V-cycle B of the solver and the
+ *   approximate solution e = B b of a random load b (same loss).
+ * mode 3: as mode 1 with a Gaussian e before every such pair.
  * mode 2: end to end through the solver: P is the finest Chebyshev smoother
  *   of one V-cycle B; loss = ||(I - B K) e||_K^2 / ||e||_K^2 for a Gaussian e,
  *   differentiated through every use of P in the cycle.
