@@ -601,9 +601,21 @@ int ensure_mg(asend_ctx* ctx) {
         ctx->mg_asm = ctx->asm_count;
         return ASEND_OK;
     }
-    rc = mg_setup(ctx->mg, ctx->st, ctx->d_E, ctx->d_nu, ctx->d_K32, ctx->d_dinv32, ctx->d_kdiag, ctx->sf,
-                  ctx->n_pen, ctx->h_pen.data(), ctx->h_pen_mult.data());
-    if (rc || (rc = mg_check_bounds(ctx->mg))) return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
+    // the finest level's block inverses and Chebyshev bound (finest-level
+    // SpMVs) on the context stream, the coarse levels' setup (small,
+    // latency-bound kernels) on the side stream, joined before the cycle;
+    // every bound is formed on the device, so the host does not wait here
+    HIPCHK(hipEventRecord(ctx->ev_fork, ctx->st));
+    HIPCHK(hipStreamWaitEvent(ctx->st2, ctx->ev_fork, 0));
+    rc = mg_fine_setup(ctx->mg, ctx->st, ctx->d_K32, ctx->d_dinv32, ctx->d_kdiag);
+    const int rc2 = mg_setup(ctx->mg, ctx->st2, ctx->d_E, ctx->d_nu, ctx->d_K32, ctx->d_dinv32, ctx->d_kdiag,
+                             ctx->sf, ctx->n_pen, ctx->h_pen.data(), ctx->h_pen_mult.data(), true);
+    HIPCHK(hipEventRecord(ctx->ev_join, ctx->st2));
+    HIPCHK(hipStreamWaitEvent(ctx->st, ctx->ev_join, 0));
+    if (rc || (rc = rc2) || (rc = mg_setup_finish(ctx->mg, ctx->st))) {
+        (void)hipStreamSynchronize(ctx->st);
+        return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
+    }
     ctx->mg_asm = ctx->asm_count;
     return ASEND_OK;
 }
@@ -664,10 +676,11 @@ int ensure_mg_and_factor(asend_ctx* ctx, bool* mg_failed) {
         return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
     }
     rc = mg_setup(ctx->mg, ctx->st2, ctx->d_E, ctx->d_nu, ctx->d_K32, ctx->d_dinv32, ctx->d_kdiag, ctx->sf,
-                  ctx->n_pen, ctx->h_pen.data(), ctx->h_pen_mult.data());
+                  ctx->n_pen, ctx->h_pen.data(), ctx->h_pen_mult.data(), false);
     // join: nothing later on the context stream runs before the setup ends
     HIPCHK(hipEventRecord(ctx->ev_join, ctx->st2));
     HIPCHK(hipStreamWaitEvent(ctx->st, ctx->ev_join, 0));
+    if (rc == ASEND_OK) rc = mg_setup_finish(ctx->mg, ctx->st);
     if (rc) {
         *mg_failed = true;
         (void)hipStreamSynchronize(ctx->st);
@@ -2868,6 +2881,9 @@ int asend_mg_set_smoother(asend_ctx* ctx, int kind, int degree, double lo_frac, 
     if (!ctx) return ASEND_ERR_ARG;
     if (!ctx->mg) return fail(ctx, ASEND_ERR_STATE, "no multigrid hierarchy (asend_set_grid)");
     int rc = mg_set_smoother(ctx->mg, kind, degree, lo_frac, hi_frac);
+    // same Jacobi kind: the step coefficients of the new degree / interval
+    // from the stored bounds (a new kind re-runs the setup at the next solve)
+    if (rc == ASEND_OK) rc = mg_refresh_coefficients(ctx->mg, ctx->st);
     if (rc) return fail(ctx, rc, "%s", mg_last_error(ctx->mg));
     return ASEND_OK;
 }

@@ -99,6 +99,15 @@ struct DevAxes {
     int* X[3] = {nullptr, nullptr, nullptr};
 };
 
+// Reduction scratch of one power iteration in flight: block partials, the
+// step norms (ping-pong pair) and the result |M A x|^2 of the last step with
+// x of unit norm.
+struct PowerScratch {
+    double* partials = nullptr;
+    double* pp = nullptr;
+    double* out = nullptr;
+};
+
 struct Level {
     int n[3] = {0, 0, 0};   // cells per axis
     int64_t n_nodes = 0, n_dof = 0, n_el = 0, nnzb = 0;
@@ -156,6 +165,16 @@ struct Multigrid {
     // hierarchy is set up on another (mg_learned_bound_begin/end): its own
     // reduction scratch and result
     double *partials_b = nullptr, *sv_b = nullptr, *h_sv_b = nullptr;
+    // power iterations: the finest level's, beside a setup on another stream
+    // (block-Jacobi or learned bound), and the coarse levels', which run on
+    // two streams of their own while the hierarchy is built (level 1 on
+    // pw_stream[0], deeper levels in turn on pw_stream[1]); pw_main also
+    // serves host-read estimates after a setup
+    PowerScratch pw_main, pw_side, pw_deep;
+    double *partials_c = nullptr, *sv_c = nullptr;
+    hipStream_t pw_stream[2] = {nullptr, nullptr};
+    hipEvent_t ev_level[kMaxLevels] = {};
+    hipEvent_t ev_pw_done[2] = {nullptr, nullptr};
     // that power iteration captured as a graph, with what it was captured for
     hipGraphExec_t pw_exec = nullptr;
     Factor pw_factor;
@@ -167,6 +186,9 @@ struct Multigrid {
     // h_lam on lam_stream) when the host first needs them
     double *d_lam = nullptr, *h_lam = nullptr;
     bool lam_pending = false;
+    // the finest level's block-Jacobi bound was enqueued by mg_fine_setup
+    // (coefficients formed on the device, value in d_lam[0])
+    bool fine_pending = false;
     hipStream_t lam_stream = nullptr;
     double* sv = nullptr;      // device scalars of the FCG loop
     double* h_sv = nullptr;    // pinned mirror
@@ -597,25 +619,6 @@ __global__ void k_cheb_dof(int64_t n, const float* __restrict__ dinv, const floa
     }
 }
 
-// Power iteration helper for the block-Jacobi smoother: w = Binv w, partials of |w|^2
-__global__ __launch_bounds__(kBlock) void k_binv_scale_norm(int n_nodes, const float* __restrict__ binv,
-                                                            float* __restrict__ w, double* partials) {
-    double t[1] = {0.0};
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_nodes; i += gridDim.x * blockDim.x) {
-        const float r0 = w[3 * i], r1 = w[3 * i + 1], r2 = w[3 * i + 2];
-        float m[9];
-        for (int k = 0; k < 9; ++k) m[k] = binv[(int64_t)k * n_nodes + i];
-        const float z0 = m[0] * r0 + m[1] * r1 + m[2] * r2;
-        const float z1 = m[3] * r0 + m[4] * r1 + m[5] * r2;
-        const float z2 = m[6] * r0 + m[7] * r1 + m[8] * r2;
-        w[3 * i] = z0;
-        w[3 * i + 1] = z1;
-        w[3 * i + 2] = z2;
-        t[0] += (double)z0 * z0 + (double)z1 * z1 + (double)z2 * z2;
-    }
-    block_partials<1>(t, partials);
-}
-
 // out = a X + b Y
 __global__ void k_lincomb(int64_t n, float a, const float* __restrict__ X, float b, const float* __restrict__ Y,
                           float* __restrict__ out) {
@@ -684,32 +687,59 @@ __global__ void k_cheb_step(int64_t n, const float* __restrict__ dinv, const flo
     }
 }
 
-// Power iteration helper: w holds A v (or P A v); scale by dinv when given,
-// partials of |w|^2.
-__global__ __launch_bounds__(kBlock) void k_dinv_scale_norm(int64_t n, const float* __restrict__ dinv,
-                                                            float* __restrict__ w, double* partials) {
-    double t[1] = {0.0};
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        float v = dinv ? dinv[i] * w[i] : w[i];
-        w[i] = v;
-        t[0] += (double)v * v;
-    }
-    block_partials<1>(t, partials);
-}
-
-// b = a / sqrt(nrm2[0])  (power-iteration normalisation with a device-side norm)
-__global__ void k_scale_dev(int64_t n, const float* __restrict__ a, const double* __restrict__ nrm2,
-                            float* __restrict__ b) {
-    const float s = (float)(1.0 / sqrt(nrm2[0]));
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        b[i] = s * a[i];
-}
-
 // Chebyshev coefficients of one level from its power-iteration estimate
 // lambda = sqrt(nrm2[0]), the device twin of cheb_coefficients; lambda is
 // kept in lam_out.  A broken-down estimate (not finite or not positive)
 // writes zero coefficients (no smoothing) and is reported when the host
 // reads the bounds back.
+// Power iteration helpers.  x is kept at the scale of the previous step:
+// step k computes w = M A (x / |x|), |w|^2 (the estimate of lambda_max^2 of
+// M A), and stores x = w.  M is the 3x3 block-Jacobi (binv), point-Jacobi
+// (dinv) or identity (both null: y already preconditioned) smoother.
+
+// The second half of a step after y = A x: w = s M y with s = 1 / sqrt(*nrm_prev)
+// (1 on the first step, whose x is the raw start), x = w, partial |w|^2 per
+// workgroup (k_power_fin adds them).
+__global__ __launch_bounds__(kBlock) void k_power_finish(int n_nodes, const float* __restrict__ binv,
+                                                         const float* __restrict__ dinv, const float* __restrict__ y,
+                                                         float* __restrict__ x, const double* __restrict__ nrm_prev,
+                                                         double* partials) {
+    const float s = nrm_prev ? (float)(1.0 / sqrt(*nrm_prev)) : 1.f;
+    double t[1] = {0.0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_nodes; i += gridDim.x * blockDim.x) {
+        const float r0 = y[3 * i], r1 = y[3 * i + 1], r2 = y[3 * i + 2];
+        float z0 = r0, z1 = r1, z2 = r2;
+        if (binv) {
+            float m[9];
+            for (int q = 0; q < 9; ++q) m[q] = binv[(int64_t)q * n_nodes + i];
+            z0 = m[0] * r0 + m[1] * r1 + m[2] * r2;
+            z1 = m[3] * r0 + m[4] * r1 + m[5] * r2;
+            z2 = m[6] * r0 + m[7] * r1 + m[8] * r2;
+        } else if (dinv) {
+            z0 *= dinv[3 * i];
+            z1 *= dinv[3 * i + 1];
+            z2 *= dinv[3 * i + 2];
+        }
+        z0 *= s;
+        z1 *= s;
+        z2 *= s;
+        x[3 * i] = z0;
+        x[3 * i + 1] = z1;
+        x[3 * i + 2] = z2;
+        t[0] += (double)z0 * z0 + (double)z1 * z1 + (double)z2 * z2;
+    }
+    block_partials<1>(t, partials);
+}
+
+// *out = sum of the step's partials, in workgroup order
+__global__ __launch_bounds__(kBlock) void k_power_fin(const double* __restrict__ partials, int nparts,
+                                                      double* __restrict__ out) {
+    double t[1] = {0.0};
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) t[0] += partials[b];
+    block_sum<1>(t);
+    if (threadIdx.x == 0) *out = t[0];
+}
+
 __global__ void k_cheb_coef(const double* __restrict__ nrm2, double lo_frac, double hi_frac, int degree,
                             float* __restrict__ out, double* __restrict__ lam_out) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
@@ -1134,27 +1164,27 @@ void vcycle(Multigrid* mg, hipStream_t st, int l, const float* b, float* x) {
 // kPowerIts steps of the power iteration for lambda_max(M A) on level l,
 // enqueued on st with the given reduction scratch; ||M A x||^2 of the last
 // step lands in nrm2[0].
-void enqueue_power(Multigrid* mg, hipStream_t st, int l, bool learned, double* partials, double* nrm2) {
+void enqueue_power(Multigrid* mg, hipStream_t st, int l, bool learned, const PowerScratch& ps) {
     Level& lv = mg->L[l];
     const int64_t n = lv.n_dof;
+    const bool block = !learned && mg->smoother_kind == 1;
+    const float* binv = block ? lv.binv : nullptr;
+    const float* dinv = (learned || block) ? nullptr : lv.d_dinv;
     hipLaunchKernelGGL(k_fill_hash, dim3(nb_for(n)), dim3(256), 0, st, n, lv.x);
-    const int nblk = kVecBlocks;
+    const int nblk = std::min<int>(kVecBlocks, (int)((lv.n_nodes + kBlock - 1) / kBlock));
     for (int it = 0; it < kPowerIts; ++it) {
         apply_cycle_op(mg, st, l, lv.x, lv.r);
-        float* w = lv.r;
+        const float* y = lv.r;
         if (learned) {
             if (cycle_half(mg) && mg->factor.Ls16) launch_factor_apply16(st, mg->factor, lv.r, mg->ft, mg->fz);
             else launch_factor_apply(st, mg->factor, lv.r, mg->ft, mg->fz);
-            w = mg->fz;
+            y = mg->fz;
         }
-        if (!learned && mg->smoother_kind == 1)
-            hipLaunchKernelGGL(k_binv_scale_norm, dim3(nblk), dim3(kBlock), 0, st, (int)lv.n_nodes, lv.binv, w,
-                               partials);
-        else
-            hipLaunchKernelGGL(k_dinv_scale_norm, dim3(nblk), dim3(kBlock), 0, st, n, learned ? nullptr : lv.d_dinv,
-                               w, partials);
-        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, st, partials, nblk, 1, nrm2);
-        hipLaunchKernelGGL(k_scale_dev, dim3(nb_for(n)), dim3(256), 0, st, n, w, nrm2, lv.x);
+        const double* prev = it == 0 ? nullptr : ps.pp + (it - 1) % 2;
+        double* out = it == kPowerIts - 1 ? ps.out : ps.pp + it % 2;
+        hipLaunchKernelGGL(k_power_finish, dim3(nblk), dim3(kBlock), 0, st, (int)lv.n_nodes, binv, dinv, y, lv.x, prev,
+                           ps.partials);
+        hipLaunchKernelGGL(k_power_fin, dim3(1), dim3(kBlock), 0, st, ps.partials, nblk, out);
     }
 }
 
@@ -1165,7 +1195,7 @@ int estimate_lmax(Multigrid* mg, hipStream_t st, int l, bool learned, double* ou
     // constant vector lies near the rigid-body modes and converges slowly);
     // kPowerIts steps without host round trips, ||M A x|| with ||x|| = 1 of
     // the last step is the estimate
-    enqueue_power(mg, st, l, learned, mg->partials, mg->sv + SV_N);
+    enqueue_power(mg, st, l, learned, mg->pw_main);
     MGCHK(hipGetLastError());
     MGCHK(hipMemcpyAsync(mg->h_sv, mg->sv + SV_N, sizeof(double), hipMemcpyDeviceToHost, st));
     MGCHK(hipStreamSynchronize(st));
@@ -1289,6 +1319,24 @@ int mg_create(Multigrid** out, int nx, int ny, int nz, const int32_t* rank_of_gr
     MGCHK(hipMalloc((void**)&mg->partials, 2 * 4096 * sizeof(double)));
     MGCHK(hipMalloc((void**)&mg->sv, 32 * sizeof(double)));
     MGCHK(hipMalloc((void**)&mg->d_lam, kMaxLevels * sizeof(double)));
+    MGCHK(hipMalloc((void**)&mg->partials_b, kVecBlocks * sizeof(double)));
+    MGCHK(hipMalloc((void**)&mg->sv_b, 4 * sizeof(double)));
+    MGCHK(hipHostMalloc((void**)&mg->h_sv_b, sizeof(double), hipHostMallocDefault));
+    MGCHK(hipMalloc((void**)&mg->partials_c, kVecBlocks * sizeof(double)));
+    MGCHK(hipMalloc((void**)&mg->sv_c, 4 * sizeof(double)));
+    mg->pw_main.partials = mg->partials;
+    mg->pw_main.out = mg->sv + SV_N;
+    mg->pw_main.pp = mg->sv + SV_N + 1;
+    mg->pw_side.partials = mg->partials_b;
+    mg->pw_side.out = mg->sv_b;
+    mg->pw_side.pp = mg->sv_b + 1;
+    mg->pw_deep.partials = mg->partials_c;
+    mg->pw_deep.out = mg->sv_c;
+    mg->pw_deep.pp = mg->sv_c + 1;
+    for (auto& sp : mg->pw_stream) MGCHK(hipStreamCreateWithFlags(&sp, hipStreamNonBlocking));
+    for (size_t l = 0; l < mg->L.size(); ++l)
+        MGCHK(hipEventCreateWithFlags(&mg->ev_level[l], hipEventDisableTiming));
+    for (auto& e : mg->ev_pw_done) MGCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     MGCHK(hipHostMalloc((void**)&mg->h_lam, kMaxLevels * sizeof(double), hipHostMallocDefault));
     MGCHK(hipHostMalloc((void**)&mg->h_sv, 32 * sizeof(double), hipHostMallocDefault));
     return ASEND_OK;
@@ -1309,6 +1357,17 @@ void mg_destroy(Multigrid* mg) {
     fr(mg->fz);
     fr(mg->partials_b);
     fr(mg->sv_b);
+    fr(mg->partials_c);
+    fr(mg->sv_c);
+    for (auto& sp : mg->pw_stream) {
+        if (sp) (void)hipStreamSynchronize(sp);
+        if (sp) (void)hipStreamDestroy(sp);
+        sp = nullptr;
+    }
+    for (auto& e : mg->ev_level)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : mg->ev_pw_done)
+        if (e) (void)hipEventDestroy(e);
     if (mg->pw_exec) (void)hipGraphExecDestroy(mg->pw_exec);
     fr(mg->d_lam);
     if (mg->h_lam) (void)hipHostFree(mg->h_lam);
@@ -1355,6 +1414,8 @@ int mg_set_smoother(Multigrid* mg, int kind, int degree, double lo_frac, double 
 // after a setup (host synchronisation: call outside graph capture).
 int fine_bound(Multigrid* mg, hipStream_t st) {
     if (mg->L.size() < 2 || mg->L[0].lmax > 0.0) return ASEND_OK;
+    // enqueued with the setup: its coefficients are already on the device
+    if (mg->fine_pending) return ASEND_OK;
     int rc = estimate_lmax(mg, st, 0, false, &mg->L[0].lmax);
     if (rc) return rc;
     const int slot = coef_slot(mg, 0, 0);
@@ -1371,7 +1432,9 @@ int mg_check_bounds(Multigrid* mg) {
     if (!mg->lam_pending) return ASEND_OK;
     MGCHK(hipStreamSynchronize(mg->lam_stream));
     mg->lam_pending = false;
-    for (size_t l = 1; l + 1 < mg->L.size(); ++l) {
+    const size_t l_first = mg->fine_pending ? 0 : 1;
+    mg->fine_pending = false;
+    for (size_t l = l_first; l + 1 < mg->L.size(); ++l) {
         const double lam = mg->h_lam[l];
         if (!(lam > 0.0) || !std::isfinite(lam)) {
             mg->ready = false;
@@ -1407,6 +1470,7 @@ int mg_refresh_fine(Multigrid* mg, hipStream_t st, const float* d_K32, const flo
                        mg->fine.sptr, (const int32_t*)nullptr, d_kdiag0, d_K32, l0.binv);
     MGCHK(hipGetLastError());
     l0.lmax = 0.0;
+    mg->fine_pending = false;  // a pending bound belongs to the old values
     return ASEND_OK;
 }
 
@@ -1423,6 +1487,7 @@ void mg_set_cycle_half(Multigrid* mg, bool on) {
     if (mg->cycle_half != on && !mg->L.empty()) {
         mg->L[0].lmax = 0.0;
         mg->lmax_factor = 0.0;
+        mg->fine_pending = false;
     }
     mg->cycle_half = on;
 }
@@ -1475,11 +1540,6 @@ int mg_learned_bound_begin(Multigrid* mg, hipStream_t st, const Factor& F, const
         MGCHK(hipMalloc((void**)&mg->ft, l0.n_dof * sizeof(float)));
         MGCHK(hipMalloc((void**)&mg->fz, l0.n_dof * sizeof(float)));
     }
-    if (!mg->partials_b) {
-        MGCHK(hipMalloc((void**)&mg->partials_b, kVecBlocks * sizeof(double)));
-        MGCHK(hipMalloc((void**)&mg->sv_b, sizeof(double)));
-        MGCHK(hipHostMalloc((void**)&mg->h_sv_b, sizeof(double), hipHostMallocDefault));
-    }
     l0.d_val = d_K32;
     l0.d_dinv = d_dinv32;
     mg->factor = F;
@@ -1499,7 +1559,7 @@ int mg_learned_bound_begin(Multigrid* mg, hipStream_t st, const Factor& F, const
             mg->pw_exec = nullptr;
             hipGraph_t graph = nullptr;
             MGCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            enqueue_power(mg, st, 0, true, mg->partials_b, mg->sv_b);
+            enqueue_power(mg, st, 0, true, mg->pw_side);
             hipError_t e1 = hipGetLastError();
             hipError_t e2 = hipStreamEndCapture(st, &graph);
             if (e1 != hipSuccess || e2 != hipSuccess) {
@@ -1519,7 +1579,7 @@ int mg_learned_bound_begin(Multigrid* mg, hipStream_t st, const Factor& F, const
         }
         MGCHK(hipGraphLaunch(mg->pw_exec, st));
     } else {
-        enqueue_power(mg, st, 0, true, mg->partials_b, mg->sv_b);
+        enqueue_power(mg, st, 0, true, mg->pw_side);
     }
     MGCHK(hipGetLastError());
     MGCHK(hipMemcpyAsync(mg->h_sv_b, mg->sv_b, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1625,15 +1685,46 @@ static int prepare_penalty(Multigrid* mg, int n_pen, const int64_t* h_pen, const
     return ASEND_OK;
 }
 
-int mg_setup(Multigrid* mg, hipStream_t st, const double* d_E, const double* d_nu, const float* d_K32,
-             const float* d_dinv32, const int32_t* d_kdiag0, double sf, int n_pen, const int64_t* h_pen,
-             const int32_t* h_mult) {
-    mg->ready = false;
+int mg_fine_setup(Multigrid* mg, hipStream_t st, const float* d_K32, const float* d_dinv32,
+                  const int32_t* d_kdiag0) {
     Level& l0 = mg->L[0];
     l0.d_val = d_K32;
     l0.d_dinv = d_dinv32;
     hipLaunchKernelGGL(k_binv32, dim3((unsigned)((l0.n_nodes + 255) / 256)), dim3(256), 0, st, (int)l0.n_nodes,
                        mg->fine.sptr, (const int32_t*)nullptr, d_kdiag0, d_K32, l0.binv);
+    l0.lmax = 0.0;
+    mg->fine_pending = false;
+    if (mg->L.size() >= 2) {
+        // the bound's own reduction scratch: the coarse levels' run alongside
+        enqueue_power(mg, st, 0, false, mg->pw_side);
+        hipLaunchKernelGGL(k_cheb_coef, dim3(1), dim3(64), 0, st, mg->sv_b, mg->cheb_lo, mg->cheb_hi, mg->cheb_degree,
+                           mg->d_coef + coef_slot(mg, 0, 0), mg->d_lam + 0);
+        mg->fine_pending = true;
+    }
+    MGCHK(hipGetLastError());
+    return ASEND_OK;
+}
+
+int mg_setup_finish(Multigrid* mg, hipStream_t st) {
+    MGCHK(hipMemcpyAsync(mg->h_lam, mg->d_lam, mg->L.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    mg->lam_pending = true;
+    mg->lam_stream = st;
+    return ASEND_OK;
+}
+
+int mg_setup(Multigrid* mg, hipStream_t st, const double* d_E, const double* d_nu, const float* d_K32,
+             const float* d_dinv32, const int32_t* d_kdiag0, double sf, int n_pen, const int64_t* h_pen,
+             const int32_t* h_mult, bool fine_elsewhere) {
+    mg->ready = false;
+    Level& l0 = mg->L[0];
+    l0.d_val = d_K32;
+    l0.d_dinv = d_dinv32;
+    if (!fine_elsewhere) {
+        hipLaunchKernelGGL(k_binv32, dim3((unsigned)((l0.n_nodes + 255) / 256)), dim3(256), 0, st, (int)l0.n_nodes,
+                           mg->fine.sptr, (const int32_t*)nullptr, d_kdiag0, d_K32, l0.binv);
+        mg->L[0].lmax = 0.0;
+        mg->fine_pending = false;
+    }
     {
         int rc = prepare_penalty(mg, n_pen, h_pen, h_mult);
         if (rc) return rc;
@@ -1665,23 +1756,28 @@ int mg_setup(Multigrid* mg, hipStream_t st, const double* d_E, const double* d_n
                            c.o_rp, c.d_kdiag, c.o_val, c.o_dinv);
         hipLaunchKernelGGL(k_binv32, dim3((unsigned)((c.n_nodes + 255) / 256)), dim3(256), 0, st, (int)c.n_nodes,
                            (const int32_t*)nullptr, c.o_rp, c.d_kdiag, c.o_val, c.binv);
+        // smoother bound of this level (all but the coarsest), started as
+        // soon as the level is assembled on a power stream of its own: level
+        // 1 on the first, deeper levels in turn on the second.  The bounds
+        // are turned into Chebyshev coefficients on the device, without host
+        // round trips (mg_setup_finish copies them back, mg_check_bounds
+        // reads them).  The finest level's is enqueued by mg_fine_setup
+        // (fine_elsewhere), or else estimated on first use (fine_bound): when
+        // the learned factor smooths that level, its block-Jacobi bound is
+        // never needed.
+        if (l + 1 < mg->L.size()) {
+            const int k = (l == 1) ? 0 : 1;
+            const PowerScratch& ps = (l == 1) ? mg->pw_main : mg->pw_deep;
+            hipStream_t sp = mg->pw_stream[k];
+            MGCHK(hipEventRecord(mg->ev_level[l], st));
+            MGCHK(hipStreamWaitEvent(sp, mg->ev_level[l], 0));
+            enqueue_power(mg, sp, (int)l, false, ps);
+            hipLaunchKernelGGL(k_cheb_coef, dim3(1), dim3(64), 0, sp, ps.out, mg->cheb_lo, mg->cheb_hi,
+                               mg->cheb_degree, mg->d_coef + coef_slot(mg, (int)l, 0), mg->d_lam + l);
+        }
     }
     MGCHK(hipGetLastError());
-    // smoother bounds (all but the coarsest level).  The finest level's is
-    // estimated on first use (fine_bound): when the learned factor smooths
-    // that level, its block-Jacobi bound is never needed.
-    // The coarse ones are estimated and turned into coefficients on the
-    // device, without host round trips (mg_check_bounds reads them back).
-    mg->L[0].lmax = 0.0;
-    for (size_t l = 1; l + 1 < mg->L.size(); ++l) {
-        enqueue_power(mg, st, (int)l, false, mg->partials, mg->sv + SV_N);
-        hipLaunchKernelGGL(k_cheb_coef, dim3(1), dim3(64), 0, st, mg->sv + SV_N, mg->cheb_lo, mg->cheb_hi,
-                           mg->cheb_degree, mg->d_coef + coef_slot(mg, (int)l, 0), mg->d_lam + l);
-    }
-    MGCHK(hipMemcpyAsync(mg->h_lam, mg->d_lam, mg->L.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-    mg->lam_pending = true;
-    mg->lam_stream = st;
-    // coarsest: dense FP64 inverse
+    // coarsest: dense FP64 inverse (on the setup stream, beside the bounds)
     Level& lc = mg->L.back();
     hipLaunchKernelGGL(k_fill, dim3(nb_for((int64_t)mg->nco * mg->nco / 2)), dim3(256), 0, st,
                        (int64_t)mg->nco * mg->nco * 2, 0.f, (float*)mg->d_Ainv);
@@ -1699,7 +1795,12 @@ int mg_setup(Multigrid* mg, hipStream_t st, const double* d_E, const double* d_n
                                    mg->d_Ainv);
         }
     MGCHK(hipGetLastError());
-    // no host wait: everything that uses the hierarchy is ordered on st
+    // join the power streams: everything that uses the hierarchy is ordered
+    // on st, with no host wait
+    for (int k = 0; k < 2; ++k) {
+        MGCHK(hipEventRecord(mg->ev_pw_done[k], mg->pw_stream[k]));
+        MGCHK(hipStreamWaitEvent(st, mg->ev_pw_done[k], 0));
+    }
     mg->ready = true;
     return ASEND_OK;
 }

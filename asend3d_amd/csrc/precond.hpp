@@ -25,9 +25,16 @@ const char* mg_last_error(const Multigrid* mg);
 // (Re)build coarse operators, smoother bounds and the coarsest inverse for the
 // current fine operator (FP32 copy with the penalty folded in).  pen / mult
 // list the penalised fine DOFs (unique) and their multiplicities.
+// fine_elsewhere: the finest level's block inverses and bound are enqueued by
+// mg_fine_setup on another stream (joined by the caller before the cycle is
+// used); otherwise they are formed here and the bound on first use.  The
+// caller then enqueues mg_setup_finish (after the join, if any), which copies
+// the device-computed bounds back for mg_check_bounds.
 int mg_setup(Multigrid* mg, hipStream_t st, const double* d_E, const double* d_nu, const float* d_K32,
              const float* d_dinv32, const int32_t* d_kdiag0, double sf, int n_pen, const int64_t* h_pen,
-             const int32_t* h_mult);
+             const int32_t* h_mult, bool fine_elsewhere);
+int mg_fine_setup(Multigrid* mg, hipStream_t st, const float* d_K32, const float* d_dinv32, const int32_t* d_kdiag0);
+int mg_setup_finish(Multigrid* mg, hipStream_t st);
 
 // Smoother of the finest level: 0 = Chebyshev with point Jacobi, 1 = Chebyshev
 // with the learned factored preconditioner P = L L^T (mg_set_learned_factor).
