@@ -23,6 +23,7 @@ FP64 Jacobi PCG pipeline on the same GPU is timed alongside (`fp64_jacobi_s`,
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -52,34 +53,60 @@ def measured_traffic(key):
     return None
 
 
-def cpu_baseline(prob, fp64_iters, nx, ny, nz):
-    """Serial oracle (reference-algorithm restatement) on a bounded slice of the
-    workload, extrapolated linearly to the full mesh at the iteration count the
-    device FP64 Jacobi PCG needed."""
+# Iterations of the reference's conj_grad_sparse with its block-banded LDL^T
+# preconditioner (blocks of 6 x 582 = 3,492 DOFs) to a 1e-8 relative residual
+# on the 200x40x40 block: not timed here (hours of serial CPU time), but
+# extrapolated from the oracle's counts on 25- and 50-element-long slices of
+# the same cross-section and block size (247 and 395 iterations, 158 s and
+# 502 s of CPU time) by the power law through those two points,
+# its ~ nx^0.68 (the lower of that and a linear extrapolation, which gives
+# 1,283 at nx = 200).
+REF_ITS_25, REF_ITS_50 = 247, 395
+
+
+def reference_iterations(nx):
+    return REF_ITS_25 * (nx / 25.0) ** (math.log(REF_ITS_50 / REF_ITS_25) / math.log(2.0))
+
+
+def cpu_baseline(prob, u, fp64_iters, nx, ny, nz):
+    """The reference's own algorithm, restated by the oracle and run serially
+    as the reference runs, timed on bounded samples of the workload and
+    extrapolated (each scaling stated in `sample`):
+      - element loop and assembly on an 8-element-long slice, x element ratio;
+      - block-banded LDL^T factor (blocks of 6 x solver_block_dim DOFs, the
+        workload's size) on that slice, x block ratio;
+      - conj_grad_sparse iteration (SpMV, penalty, block solves) on that slice,
+        x DOF ratio, x the iteration count of reference_iterations();
+      - the design-variable loop (per-element modulus DVs) on the full
+        workload: 300 DVs timed, x N_el / 300.
+    The Jacobi PCG of the device baseline, restated on the same slice, is
+    reported beside it."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    from asend3d_amd import problems
-    sx = max(4, nx // 8)
+    from asend3d_amd import api, problems
+    sx = max(2, nx // 25)
     sl = problems.heterogeneous(sx, ny, nz, lx=1000.0 * sx / nx)
+    block_dofs = 6 * api.block_dim_auto(prob.mesh.n_nodes)
     t0 = time.perf_counter()
     K = O.assemble_solid(8, sl.mesh.xyz, sl.mesh.conn, sl.E, sl.nu, sl.dof_index, sl.n_dof)
     t_asm = time.perf_counter() - t0
     sf = 1e5 * O.max_abs(K)
-    n_it = 1500
     t0 = time.perf_counter()
-    u, _, _ = O.jacobi_pcg(K, sl.pen_dofs, sf, sl.f, 1e-30, n_it)
+    ldl = O.LDL(K, sl.pen_dofs, sf, block_dofs)
+    t_fac = time.perf_counter() - t0
+    n_it = 8
+    t0 = time.perf_counter()
+    O.conj_grad_sparse(K, sl.pen_dofs, sf, ldl, sl.f, 0.0, n_it)
     t_it = (time.perf_counter() - t0) / n_it
+    n_jac = 300
     t0 = time.perf_counter()
-    O.solid_sens(8, sl.mesh.xyz, sl.mesh.conn, sl.E, sl.nu, sl.dof_index, u, u)
-    t_sens = time.perf_counter() - t0
-    # the same PCG with OpenMP over the host's threads (at most 16), for context
-    n_thr = max(1, min(16, os.cpu_count() or 1))
-    O.set_threads(n_thr)
-    n_omp = 300
+    O.jacobi_pcg(K, sl.pen_dofs, sf, sl.f, 1e-30, n_jac)
+    t_jac = (time.perf_counter() - t0) / n_jac
+    n_dv = 300
     t0 = time.perf_counter()
-    O.jacobi_pcg(K, sl.pen_dofs, sf, sl.f, 1e-30, n_omp)
-    t_omp = (time.perf_counter() - t0) / n_omp
-    O.set_threads(1)
+    O.dv_loop(8, prob.mesh.xyz, prob.mesh.conn, prob.E, prob.nu, prob.dof_index, prob.n_dof, u, u, prob.tip_elements,
+              n_dv)
+    t_dv = (time.perf_counter() - t0) / n_dv
     cpu_model = "unknown"
     try:
         with open("/proc/cpuinfo") as fh:
@@ -89,17 +116,24 @@ def cpu_baseline(prob, fp64_iters, nx, ny, nz):
                     break
     except OSError:
         pass
-    scale = prob.n_dof / sl.n_dof
-    scale_el = prob.mesh.n_el / sl.mesh.n_el
-    est = t_asm * scale_el + t_it * scale * fp64_iters + t_sens * scale_el
-    t_sample = t_asm + t_it * n_it + t_sens
+    r_el = prob.mesh.n_el / sl.mesh.n_el
+    r_dof = prob.n_dof / sl.n_dof
+    blocks_full = -(-prob.n_dof // block_dofs)
+    blocks_sl = -(-sl.n_dof // block_dofs)
+    its = reference_iterations(nx)
+    est = t_asm * r_el + t_fac * blocks_full / blocks_sl + t_it * r_dof * its + t_dv * prob.mesh.n_el
+    est_jac = t_asm * r_el + t_jac * r_dof * fp64_iters + t_dv * prob.mesh.n_el
     return {"value": est, "unit": "s", "cores": 1, "kind": "port",
-            "sample": (f"oracle on a {sx}x{ny}x{nz} slice ({sl.n_dof} DOF, {sl.mesh.n_el} elements, "
-                       f"{t_sample:.1f} s): assembly {t_asm:.2f} s, {n_it} Jacobi PCG iterations "
-                       f"{t_it * 1e3:.1f} ms each, element-wise dJ/dE {t_sens:.2f} s; scaled by element ratio "
-                       f"{scale_el:.1f}, DOF ratio {scale:.1f} and {fp64_iters} iterations; host {cpu_model}, "
-                       f"{os.cpu_count()} hardware threads; the PCG iteration with OpenMP on {n_thr} threads takes "
-                       f"{t_omp * 1e3:.2f} ms ({t_it / t_omp:.1f}x the serial one)")}
+            "sample": (f"reference algorithm restated by the oracle, serial: assembly {t_asm:.2f} s on a {sx}x{ny}x{nz} "
+                       f"slice ({sl.n_dof} DOF) x{r_el:.0f}; block-banded LDL^T ({block_dofs}-DOF blocks) factor "
+                       f"{t_fac:.2f} s for {blocks_sl} blocks x{blocks_full / blocks_sl:.1f}; conj_grad_sparse "
+                       f"{t_it:.3f} s per iteration on the slice x{r_dof:.1f} DOF ratio x {its:.0f} iterations "
+                       f"(power law through {REF_ITS_25} and {REF_ITS_50} measured on 25- and 50-long slices); "
+                       f"design-variable loop {t_dv * 1e3:.2f} ms per DV (300 DVs timed "
+                       f"on the full mesh) x {prob.mesh.n_el} DVs; host {cpu_model}, {os.cpu_count()} hardware "
+                       f"threads; with Jacobi PCG instead ({fp64_iters} iterations, {t_jac * 1e3:.1f} ms each on the "
+                       f"slice): {est_jac:.0f} s"),
+            "jacobi_variant_s": est_jac}
 
 
 def train_network(ctx, n_meshes, steps):
@@ -257,7 +291,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(prob, stb.iterations, args.nx, args.ny, args.nz)
+        cpu = cpu_baseline(prob, u, stb.iterations, args.nx, args.ny, args.nz)
         log(f"cpu baseline estimate {cpu['value']:.1f} s")
 
     out = {

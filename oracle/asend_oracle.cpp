@@ -1159,6 +1159,63 @@ int orc_solid_sens(int type, int64_t n_nodes, const double* xyz, int64_t n_el, c
 }
 
 
+int orc_dv_loop(int type, int64_t n_nodes, const double* xyz, int64_t n_el, const int32_t* conn, const double* E,
+                const double* nu, const int64_t* dof_index, int64_t n_dof, const double* u, const double* lam,
+                int64_t n_load_el, const int32_t* load_el, int64_t n_dv, double* dJdE) {
+    if (type != 8 && type != 81) return -1;
+    if (n_dv > n_el) return -2;
+    const int tot = (type == 81) ? 33 : 24;
+    std::vector<Dual> crd(3 * n_nodes), drud(n_dof), Kf(tot * tot), Rf(tot);
+    std::vector<char> el_in(n_el, 0);
+    std::vector<double> ue(tot), le(tot);
+    for (int64_t d = 0; d < n_dv; ++d) {
+        /* coordinate duals of every node (no coordinate DVs: derivative 0) */
+        for (int64_t i = 0; i < 3 * n_nodes; ++i) crd[i] = Dual(xyz[i], 0.0);
+        for (int64_t i = 0; i < n_dof; ++i) drud[i] = Dual(0.0, 0.0);
+        for (int64_t e = 0; e < n_el; ++e) el_in[e] = 0;
+        el_in[d] = 1;
+        /* applied loads on the DV's elements (a traction does not depend on
+         * the modulus: the scan is the cost) */
+        int64_t hits = 0;
+        for (int64_t k = 0; k < n_load_el; ++k) hits += el_in[load_el[k]];
+        (void)hits;
+        for (int64_t i = 0; i < n_dof; ++i) drud[i] = Dual(-drud[i].v, -drud[i].d);
+        /* solution-dependent residual of the DV's element with E + eps */
+        const int32_t* c = conn + 8 * d;
+        double X[24];
+        for (int a = 0; a < 8; ++a)
+            for (int k = 0; k < 3; ++k) X[8 * k + a] = crd[3 * (int64_t)c[a] + k].v;
+        const double G = E[d] / (2.0 * (1.0 + nu[d])), dG = 1.0 / (2.0 * (1.0 + nu[d]));
+        Dual mod[3], pr[3], sm[3], C[36];
+        for (int i = 0; i < 3; ++i) { mod[i] = Dual(E[d], 1.0); pr[i] = Dual(nu[d]); sm[i] = Dual(G, dG); }
+        solid_cmat<Dual>(mod, pr, sm, C);
+        int64_t gd[24];
+        for (int a = 0; a < 8; ++a)
+            for (int k = 0; k < 3; ++k) {
+                gd[3 * a + k] = dof_index[6 * (int64_t)c[a] + k];
+                ue[3 * a + k] = u[gd[3 * a + k]];
+                le[3 * a + k] = lam[gd[3 * a + k]];
+            }
+        if (type == 81) {
+            hex_full<Dual>(type, X, nullptr, C, nullptr, Kf.data(), Rf.data());
+            internal_from_external(Kf.data(), tot, 24, ue.data(), ue.data() + 24);
+            internal_from_external(Kf.data(), tot, 24, le.data(), le.data() + 24);
+        }
+        hex_full<Dual>(type, X, nullptr, C, ue.data(), Kf.data(), Rf.data());
+        for (int i = 0; i < 24; ++i) drud[gd[i]] = drud[gd[i]] + Rf[i];
+        for (int64_t i = 0; i < 3 * n_nodes; ++i) crd[i] = Dual(xyz[i], 0.0);
+        /* dL/dd -= lam . dR/dd over every DOF, then the internal-DOF terms */
+        double s = 0.0;
+        for (int64_t i = 0; i < n_dof; ++i) s += lam[i] * drud[i].d;
+        double s_int = 0.0;
+        for (int64_t e = 0; e < n_el; ++e)
+            if (el_in[e] && type == 81)
+                for (int i = 24; i < tot; ++i) s_int += le[i] * Rf[i].d;
+        dJdE[d] = -s - s_int;
+    }
+    return 0;
+}
+
 int orc_shell4_element(const double* X, double E, double nu, double t, double rho, double* Ke, double* Me) {
     double Mtmp[576];
     return shell4_element(X, E, nu, t, rho, Ke, Me ? Me : Mtmp);

@@ -151,6 +151,20 @@ int orc_solid_sens(int type, int64_t n_nodes, const double* xyz, int64_t n_el,
                    const int64_t* dof_index, const double* u, const double* lam, double* dJdE);
 
 
+/* The reference's design-variable loop itself, for per-element modulus DVs
+ * (DV d acts on element d, d < n_dv): per DV the O(N) sweeps of
+ * d_relasticd_d (S/model_analysis.rs:1379-1457) -- node coordinate duals
+ * twice, the dual residual vector over every DOF zeroed and negated, the
+ * element flags reset, the load elements scanned -- the dual-number residual
+ * of the DV's element scattered into it, then dL/dd -= lam . dR/dd over every
+ * DOF and the internal-DOF adjoint term of every element (get_obj_gradient,
+ * :1679-1701).  Same values as orc_solid_sens for those elements (up to the
+ * order of summation); its cost is the reference's O(N_dv (N_dof + N_el +
+ * N_nodes)). */
+int orc_dv_loop(int type, int64_t n_nodes, const double* xyz, int64_t n_el, const int32_t* conn, const double* E,
+                const double* nu, const int64_t* dof_index, int64_t n_dof, const double* u, const double* lam,
+                int64_t n_load_el, const int32_t* load_el, int64_t n_dv, double* dJdE);
+
 /* shell4 (type 41) single isotropic layer of thickness t at the mid-plane:
  * condensed 24x24 stiffness and 24x24 consistent mass of one element
  * (S/element_equations.rs:206-418, :420-626, S/element_soln_fields.rs:1160-1343,

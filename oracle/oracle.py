@@ -272,6 +272,25 @@ def solid_sens(etype, xyz, conn, E, nu, dof_index, u, lam):
     return out
 
 
+def dv_loop(etype, xyz, conn, E, nu, dof_index, n_dof, u, lam, load_el, n_dv):
+    """The reference's design-variable loop for per-element modulus DVs
+    0..n_dv-1 (d_relasticd_d + the DV loop of get_obj_gradient,
+    S/model_analysis.rs:1379-1457, 1679-1701): its O(N_dof) sweeps per DV."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    conn = np.ascontiguousarray(conn, dtype=np.int32)
+    E = np.ascontiguousarray(E, dtype=np.float64)
+    nu = np.ascontiguousarray(nu, dtype=np.float64)
+    di = np.ascontiguousarray(dof_index, dtype=np.int64)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+    le = np.ascontiguousarray(load_el, dtype=np.int32)
+    out = np.zeros(n_dv)
+    _chk(lib().orc_dv_loop(ctypes.c_int(etype), _i64(xyz.shape[0]), _ptr(xyz), _i64(conn.shape[0]), _ptr(conn),
+                           _ptr(E), _ptr(nu), _ptr(di), _i64(n_dof), _ptr(u), _ptr(lam), _i64(le.shape[0]), _ptr(le),
+                           _i64(n_dv), _ptr(out)), "dv_loop")
+    return out
+
+
 def shell4_element(X, E, nu, t, rho):
     """Condensed 24x24 stiffness and 24x24 consistent mass of one shell4
     (X: 3 x 4 SoA coordinates)."""
