@@ -135,6 +135,7 @@ _lib.asend_ctx_destroy.argtypes = [_vp]
 
 EXPORTED = ["asend_ctx_create", "asend_ctx_destroy", "asend_last_error", "asend_version", "asend_sync",
             "asend_reorder_nodes", "asend_block_dim_auto", "asend_surface_traction", "asend_lognormal_moduli",
+            "asend_compliance_clamp_check",
             "asend_upload_mesh", "asend_set_thickness", "asend_set_grid", "asend_update_moduli", "asend_set_penalty",
             "asend_penalty_scale", "asend_uniform_elements", "asend_assemble", "asend_get_matrix", "asend_spmv", "asend_solve", "asend_solve_noalloc",
             "asend_element_sens", "asend_static_compliance", "asend_bench_spmv", "asend_mg_info", "asend_mg_apply",
@@ -230,6 +231,23 @@ def lognormal_moduli(seed, n, median, sigma_ln):
     if rc != 0:
         raise AsendError(f"asend_lognormal_moduli failed ({rc})")
     return out
+
+
+def compliance_clamp_check(etype, E, nu):
+    """(count, first) of the elements whose isotropic compliance the
+    reference's pivot-clamped QR inverse would not invert exactly
+    (S/matrix_functions.rs:167-215 on the matrix of
+    S/element_properties.rs:728-759, or :196-210 for shell4 layers); first is
+    -1 when there are none.  asend_upload_mesh refuses meshes with any."""
+    E = _f64(np.atleast_1d(E))
+    nu = _f64(np.broadcast_to(nu, E.shape))
+    n = ctypes.c_int64(0)
+    first = ctypes.c_int64(-1)
+    rc = _lib.asend_compliance_clamp_check(ctypes.c_int(etype), _i64(E.size), _p(E), _p(nu), ctypes.byref(n),
+                                           ctypes.byref(first))
+    if rc != 0:
+        raise AsendError(f"asend_compliance_clamp_check failed ({rc})")
+    return int(n.value), int(first.value)
 
 
 # ------------------------------------------------------------------ context --

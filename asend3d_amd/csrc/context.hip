@@ -23,6 +23,7 @@
 #include "precond.hpp"
 #include "learned.hpp"
 #include "batched.hpp"
+#include "host_prep.hpp"
 
 using namespace asend;
 
@@ -112,6 +113,7 @@ struct asend_ctx {
     int32_t *d_enb = nullptr, *d_erev = nullptr, *d_eslot = nullptr;
     int n_edge = 0;
     double mean_log_e = 0.0;
+    std::vector<double> h_nu;  // per-element Poisson ratio (moduli updates are checked against it)
     int64_t asm_count = 0, factor_asm = -1;
     int64_t mg_asm = -1;  // assembly the multigrid hierarchy was last set up for
     double asm_cmass = 0.0;   // mass coefficient of the current assembly
@@ -1136,6 +1138,16 @@ int asend_upload_mesh(asend_ctx* ctx, int elem_type, int64_t n_mesh_nodes, const
                             (long long)e);
         }
     }
+    {
+        int64_t first = -1;
+        const int64_t nbad = compliance_clamp_count(elem_type, n_el, E, nu, &first);
+        if (nbad > 0)
+            return fail(ctx, ASEND_ERR_UNSUPPORTED,
+                        "%lld element(s) (first %lld: E = %.6g, nu = %.6g) are past the pivot clamp of the reference's "
+                        "compliance inversion (pivots below 1e-12 replaced by 1e-12), where its stiffness is not the "
+                        "isotropic C integrated here; pose the model in units where 1/E stays above 1e-12 (e.g. MPa)",
+                        (long long)nbad, (long long)first, E[first], nu[first]);
+    }
     HIPCHK(hipSetDevice(ctx->device));
     free_mesh(ctx);
     ctx->etype = elem_type;
@@ -1352,6 +1364,7 @@ int asend_upload_mesh(asend_ctx* ctx, int elem_type, int64_t n_mesh_nodes, const
         double s = 0.0;
         for (int64_t e = 0; e < n_el; ++e) s += std::log(E[e]);
         ctx->mean_log_e = s / (double)n_el;
+        ctx->h_nu.assign(nu, nu + n_el);
     }
     if (shell && (rc = dalloc(ctx, &ctx->d_thk, n_el))) return rc;
     HIPCHK(hipMemcpy(ctx->d_sptr, ctx->h_sptr.data(), ctx->h_sptr.size() * 4, hipMemcpyHostToDevice));
@@ -1456,6 +1469,15 @@ int asend_mg_set_cycle_precision(asend_ctx* ctx, int bits) {
 int asend_update_moduli(asend_ctx* ctx, const double* E) {
     if (!ctx || !E) return ASEND_ERR_ARG;
     if (!ctx->d_E) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
+    {
+        int64_t first = -1;
+        const int64_t nbad = compliance_clamp_count(ctx->etype, ctx->n_el, E, ctx->h_nu.data(), &first);
+        if (nbad > 0)
+            return fail(ctx, ASEND_ERR_UNSUPPORTED,
+                        "%lld element(s) (first %lld: E = %.6g) are past the pivot clamp of the reference's compliance "
+                        "inversion; moduli not updated",
+                        (long long)nbad, (long long)first, E[first]);
+    }
     HIPCHK(hipMemcpyAsync(ctx->d_E, E, ctx->n_el * 8, hipMemcpyHostToDevice, ctx->st));
     HIPCHK(hipStreamSynchronize(ctx->st));
     double s = 0.0;

@@ -312,6 +312,95 @@ int lognormal_moduli(uint64_t seed, int64_t n, double median, double sigma_ln, d
     return ASEND_OK;
 }
 
+// The reference builds an isotropic stiffness by inverting the compliance
+// S(E, nu, G = E/(2(1+nu))): the 6x6 of get_solid_stiff_dfd0
+// (S/element_properties.rs:728-759) for solids, the 3x3 plane-stress layer
+// matrix of get_layer_q_dfd0 (:196-210) for shells.  The inverse is the
+// Givens QR of q_rfactor (S/matrix_functions.rs:167-215): before each
+// rotation of column i1 against a lower row, a diagonal entry of magnitude
+// below 1e-12 (S/constants.rs:13) is replaced by 1e-12; the last column has no
+// rotation and is never checked.  Once that fires, the result is no longer
+// S^-1, while the device integrates the exact isotropic stiffness.  The
+// checks below follow the rotations in the reference's order and arithmetic
+// (theta = atan(b/a), cos and sin of theta) so that the decision at the
+// threshold is the reference's.
+static void iso_compliance(int n, double E, double nu, double* m) {
+    const double G = E / (2.0 * (1.0 + nu));
+    for (int i = 0; i < n * n; ++i) m[i] = 0.0;
+    if (n == 6) {
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) m[6 * i + j] = (i == j) ? 1.0 / E : -nu / E;
+        m[21] = m[28] = m[35] = 1.0 / G;
+    } else {
+        m[0] = m[4] = 1.0 / E;
+        m[1] = m[3] = -nu / E;
+        m[8] = 1.0 / G;
+    }
+}
+
+// true when some checked pivot of the QR of m (n x n) is below 1e-12
+static bool qr_pivot_clamps(int n, double* m) {
+    for (int i1 = 0; i1 < n; ++i1)
+        for (int i2 = i1 + 1; i2 < n; ++i2) {
+            const double a = m[n * i1 + i1];
+            if (std::fabs(a) < 1e-12) return true;
+            const double th = std::atan(m[n * i2 + i1] / a), c = std::cos(th), s = std::sin(th);
+            for (int i3 = i1; i3 < n; ++i3) {
+                const double p = m[n * i1 + i3], q = m[n * i2 + i3];
+                m[n * i1 + i3] = c * p + s * q;
+                m[n * i2 + i3] = -s * p + c * q;
+            }
+        }
+    return false;
+}
+
+// Smallest checked pivot magnitude of the QR of S(1, nu): every pivot of
+// S(E, nu) is this scaled by 1/E up to rounding, which screens out the
+// elements far from the threshold without trigonometry.
+static double min_scaled_pivot(int n, double nu) {
+    double m[36];
+    iso_compliance(n, 1.0, nu, m);
+    double lo = 1e300;
+    for (int i1 = 0; i1 < n; ++i1)
+        for (int i2 = i1 + 1; i2 < n; ++i2) {
+            const double a = m[n * i1 + i1], b = m[n * i2 + i1];
+            lo = std::min(lo, std::fabs(a));
+            const double r = std::hypot(a, b);
+            if (r == 0.0) continue;
+            const double c = a / r, s = b / r;
+            for (int i3 = i1; i3 < n; ++i3) {
+                const double p = m[n * i1 + i3], q = m[n * i2 + i3];
+                m[n * i1 + i3] = c * p + s * q;
+                m[n * i2 + i3] = -s * p + c * q;
+            }
+        }
+    return lo;
+}
+
+int64_t compliance_clamp_count(int elem_type, int64_t n_el, const double* E, const double* nu, int64_t* first) {
+    const int n = (elem_type == ASEND_SHELL4) ? 3 : 6;
+    int64_t count = 0;
+    if (first) *first = -1;
+    double last_nu = std::nan(""), lo = 0.0;
+    for (int64_t e = 0; e < n_el; ++e) {
+        if (!(nu[e] == last_nu)) {
+            last_nu = nu[e];
+            lo = min_scaled_pivot(n, nu[e]);
+        }
+        // pivots are lo/E to within a few ulps: only elements within a
+        // factor 2 of the threshold take the exact replay
+        if (!(std::fabs(E[e]) * 2e-12 < lo)) {
+            double m[36];
+            iso_compliance(n, E[e], nu[e], m);
+            if (qr_pivot_clamps(n, m)) {
+                if (count == 0 && first) *first = e;
+                ++count;
+            }
+        }
+    }
+    return count;
+}
+
 }  // namespace asend
 
 extern "C" {
@@ -339,6 +428,14 @@ int asend_surface_traction(int elem_type, int64_t n_nodes, const double* xyz, in
 
 int asend_lognormal_moduli(uint64_t seed, int64_t n, double median, double sigma_ln, double* out) {
     return asend::lognormal_moduli(seed, n, median, sigma_ln, out);
+}
+
+int asend_compliance_clamp_check(int elem_type, int64_t n_el, const double* E, const double* nu,
+                                 int64_t* n_clamped, int64_t* first) {
+    if (elem_type != ASEND_BRICK8 && elem_type != ASEND_BRICKIM && elem_type != ASEND_SHELL4) return ASEND_ERR_ARG;
+    if (n_el < 0 || (n_el > 0 && (!E || !nu)) || !n_clamped) return ASEND_ERR_ARG;
+    *n_clamped = asend::compliance_clamp_count(elem_type, n_el, E, nu, first);
+    return ASEND_OK;
 }
 
 const char* asend_version(void) { return "asend-mi355x 0.1 (gfx950)"; }

@@ -15,4 +15,9 @@ int surface_traction(int elem_type, int64_t n_nodes, const double* xyz, int64_t 
 
 int lognormal_moduli(uint64_t seed, int64_t n, double median, double sigma_ln, double* out);
 
+// Number of elements whose isotropic compliance the reference's pivot-clamped
+// QR inverse would not invert exactly; *first gets the first such label
+// (-1 when none).
+int64_t compliance_clamp_count(int elem_type, int64_t n_el, const double* E, const double* nu, int64_t* first);
+
 }  // namespace asend

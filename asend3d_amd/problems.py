@@ -66,7 +66,8 @@ def heterogeneous(nx, ny, nz, seed=C2_SEED, median=70e3, sigma_ln=1.0, nu=0.3, l
     70 GPa, traction 1 MPa).  In these units 1/E stays far above the 1e-12
     pivot clamp of the reference's QR inverse (S/matrix_functions.rs:189-191),
     which in SI units would corrupt the material matrix of the stiff tail
-    (E > ~7e11 Pa) of a sigma_ln = 1 distribution."""
+    (E > 6.35e11 Pa at nu = 0.3) of a sigma_ln = 1 distribution; the upload
+    refuses such moduli (api.compliance_clamp_check)."""
     n_el = nx * ny * nz
     E = api.lognormal_moduli(seed, n_el, median, sigma_ln)
     return cantilever(nx, ny, nz, lx, ly, lz, E=E, nu=nu, traction=traction)

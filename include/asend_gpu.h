@@ -136,6 +136,20 @@ int asend_surface_traction(int elem_type, int64_t n_nodes, const double* xyz, in
  * variables, S/model_input.rs:1377-1401). */
 int asend_lognormal_moduli(uint64_t seed, int64_t n, double median, double sigma_ln, double* out);
 
+/* Host-only check of the isotropic materials of n_el elements against the
+ * pivot clamp of the reference's compliance inversion: get_solid_stiff_dfd0
+ * (S/element_properties.rs:728-759; 6x6, brick8/brickIM) and
+ * get_layer_q_dfd0 (:196-210; 3x3 plane stress, shell4) invert
+ * S(E, nu, E/(2(1+nu))) by the Givens QR of q_rfactor
+ * (S/matrix_functions.rs:167-215), which replaces pivots below 1e-12 by
+ * 1e-12.  *n_clamped counts the elements where that fires (for nu = 0.3,
+ * E above ~1e12 in the model's units), where the reference's stiffness is no
+ * longer the isotropic C that the device integrates; *first (may be NULL)
+ * gets the first such label or -1.  asend_upload_mesh and
+ * asend_update_moduli refuse such materials with ASEND_ERR_UNSUPPORTED. */
+int asend_compliance_clamp_check(int elem_type, int64_t n_el, const double* E, const double* nu,
+                                 int64_t* n_clamped, int64_t* first);
+
 /* ---- device model ------------------------------------------------------- */
 
 /* Upload a homogeneous mesh: the replacement for the element/node arrays that
@@ -150,7 +164,10 @@ int asend_lognormal_moduli(uint64_t seed, int64_t n, double median, double sigma
  * (dof_index[6n + c] = 6 rank + c), one isotropic layer per element whose
  * thickness asend_set_thickness supplies; every element must lie in a plane
  * normal to z (the identity element frame).  A new mesh clears the penalty
- * set (asend_set_penalty numbers DOFs on the mesh it is called for). */
+ * set (asend_set_penalty numbers DOFs on the mesh it is called for).
+ * Materials past the compliance-inversion pivot clamp of the reference
+ * (asend_compliance_clamp_check) are refused with ASEND_ERR_UNSUPPORTED: pose
+ * such models in units where 1/E stays above 1e-12 (mm-N-MPa, say). */
 int asend_upload_mesh(asend_ctx* ctx, int elem_type, int64_t n_nodes, const double* xyz,
                       const int64_t* dof_index, int64_t n_dof, int64_t n_el, const int32_t* conn,
                       const double* E, const double* nu, const double* rho);
@@ -166,7 +183,9 @@ int asend_set_thickness(asend_ctx* ctx, const double* t);
  * preconditioners.  No reference counterpart. */
 int asend_set_grid(asend_ctx* ctx, int64_t nx, int64_t ny, int64_t nz);
 
-/* Replace the per-element moduli on the device (design update). */
+/* Replace the per-element moduli on the device (design update).  Refuses,
+ * like asend_upload_mesh, moduli past the compliance pivot clamp, leaving the
+ * previous moduli in place. */
 int asend_update_moduli(asend_ctx* ctx, const double* E);
 
 /* Penalty constraints of single-DOF clamp type: K_eff = K + sf * sum C^T C

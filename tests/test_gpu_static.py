@@ -193,3 +193,34 @@ def test_assembly_deterministic(ctx):
     u1, _ = ctx.solve(p.f, ctx.opts(tol=1e-10, max_it=100000))
     u2, _ = ctx.solve(p.f, ctx.opts(tol=1e-10, max_it=100000))
     assert np.array_equal(u1, u2)
+
+
+def test_si_units_and_pivot_clamp(ctx):
+    """SI units (m, Pa): lognormal moduli around 70 GPa plus one element at
+    6e11 Pa, just under the nu = 0.3 threshold (6.35e11 Pa) of the
+    reference's pivot-clamped compliance inverse, assemble to the oracle's K.
+    An element at 5e12 Pa, where the oracle's (the reference's) clamped
+    inverse is no longer the isotropic C, is refused at upload and at a
+    moduli update, and the context keeps the mesh and moduli it held."""
+    gpu_case("SI units and the compliance pivot clamp")
+    p = problems.cantilever(14, 5, 4, E=api.lognormal_moduli(11, 280, 70e9, 0.5))
+    p.E[7] = 6.0e11
+    assert api.compliance_clamp_check(8, p.E, p.nu) == (0, -1)
+    problems.upload(ctx, p)
+    ctx.assemble()
+    K, sf = oracle_system(p)
+    rp, cols, vals = ctx.get_matrix(with_penalty=False)
+    assert np.array_equal(rp, K.rowptr) and np.array_equal(cols, K.cols)
+    assert np.abs(vals - K.vals).max() <= 1e-12 * np.abs(K.vals).max()
+
+    bad = p.E.copy()
+    bad[3] = 5e12
+    assert api.compliance_clamp_check(8, bad, p.nu) == (1, 3)
+    C = O.solid_cmat(5e12, 0.3)
+    assert not np.allclose(C, 5e12 * O.solid_cmat(1.0, 0.3), rtol=1e-3)
+    with pytest.raises(api.AsendError, match="pivot clamp"):
+        problems.upload(ctx, dataclasses.replace(p, E=bad))
+    with pytest.raises(api.AsendError, match="pivot clamp"):
+        ctx.update_moduli(bad)
+    ctx.assemble()
+    assert np.array_equal(ctx.get_matrix(with_penalty=False)[2], vals)
