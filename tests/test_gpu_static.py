@@ -209,9 +209,16 @@ def test_si_units_and_pivot_clamp(ctx):
     problems.upload(ctx, p)
     ctx.assemble()
     K, sf = oracle_system(p)
-    rp, cols, vals = ctx.get_matrix(with_penalty=False)
+    rp, cols, vals = ctx.get_matrix(with_penalty=True)
     assert np.array_equal(rp, K.rowptr) and np.array_equal(cols, K.cols)
-    assert np.abs(vals - K.vals).max() <= 1e-12 * np.abs(K.vals).max()
+    # K to 1e-12 of max|K|; the penalised diagonals on the scale of sf
+    diag = np.array([K.rowptr[d] + np.searchsorted(K.cols[K.rowptr[d]:K.rowptr[d + 1]], d) for d in p.pen_dofs])
+    ref = K.vals.copy()
+    ref[diag] += sf
+    mask = np.ones(ref.size, bool)
+    mask[diag] = False
+    assert np.abs(vals[mask] - ref[mask]).max() <= 1e-12 * np.abs(K.vals).max()
+    assert np.abs(vals[diag] - ref[diag]).max() <= 1e-14 * sf
 
     bad = p.E.copy()
     bad[3] = 5e12
@@ -223,4 +230,4 @@ def test_si_units_and_pivot_clamp(ctx):
     with pytest.raises(api.AsendError, match="pivot clamp"):
         ctx.update_moduli(bad)
     ctx.assemble()
-    assert np.array_equal(ctx.get_matrix(with_penalty=False)[2], vals)
+    assert np.array_equal(ctx.get_matrix(with_penalty=True)[2], vals)
