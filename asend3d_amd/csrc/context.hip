@@ -24,6 +24,7 @@
 #include "learned.hpp"
 #include "batched.hpp"
 #include "host_prep.hpp"
+#include "device_common.hpp"
 
 using namespace asend;
 
@@ -101,6 +102,7 @@ struct asend_ctx {
     // learned preconditioner: network, graph data, and the assembly its factor belongs to
     Learned* learned = nullptr;
     int32_t *d_n2e_ptr = nullptr, *d_n2e = nullptr;
+    int32_t* d_n2rec = nullptr;  // brick8: record 8 e + a of each node's elements (CSR by d_n2e_ptr)
     int max_valence = 0;  // most elements sharing one node
     // set inside a nonlinear solve after its first Newton iteration: a new
     // assembly changes only the finest multigrid level (the coarse levels are
@@ -108,7 +110,7 @@ struct asend_ctx {
     bool mg_fine_only = false;
     // geometrically nonlinear statics: element tangents (576 per element),
     // element residuals (24), displacement state, applied load, internal force
-    double *d_nl_ke = nullptr, *d_nl_re = nullptr, *d_nl_u = nullptr, *d_nl_f = nullptr, *d_nl_r = nullptr;
+    double *d_kb = nullptr, *d_nl_re = nullptr, *d_nl_u = nullptr, *d_nl_f = nullptr, *d_nl_r = nullptr;
     uint8_t* d_clamped = nullptr;
     int32_t *d_enb = nullptr, *d_erev = nullptr, *d_eslot = nullptr;
     int n_edge = 0;
@@ -328,7 +330,8 @@ void free_mesh(asend_ctx* ctx) {
     newmark_free(ctx);
     dfree(ctx->d_n2e_ptr);
     dfree(ctx->d_n2e);
-    dfree(ctx->d_nl_ke);
+    dfree(ctx->d_n2rec);
+    dfree(ctx->d_kb);
     dfree(ctx->d_nl_re);
     dfree(ctx->d_nl_u);
     dfree(ctx->d_nl_f);
@@ -833,8 +836,8 @@ int solve_device(asend_ctx* ctx, const asend_solve_opts* o, asend_solve_stats* s
 
 // Element integration into K: k_scale K_e(E) + c_mass M_e of every element.
 // Translated-element meshes (brick8, brickIM): one row-wise launch scaling the
-// reference blocks.  Other brick8 meshes: one row-wise gather launch that
-// integrates per element.  brickIM and shell4: K is cleared, then one scatter
+// reference blocks.  Other brick8 meshes: each element integrated once into
+// row records, then one row-wise gather.  brickIM and shell4: K is cleared, then one scatter
 // launch per element colour.
 int integrate_elements(asend_ctx* ctx, const double* E, double k_scale, double c_mass, double* K) {
     if (ctx->uniform_hex) {
@@ -858,9 +861,14 @@ int integrate_elements(asend_ctx* ctx, const double* E, double k_scale, double c
                                    ctx->d_thk, ctx->d_rho, k_scale, c_mass, ctx->d_sptr, ctx->d_slot, K);
         }
     } else {
-        launch_hex8_assemble_rows(ctx->st, (int)ctx->n_nodes, (int)ctx->n_slices, ctx->d_sptr, ctx->d_scol, ctx->d_rp,
-                                  ctx->d_n2e_ptr, ctx->d_n2e, ctx->d_conn_r, ctx->d_xyz_r, E, ctx->d_nu, ctx->d_rho,
-                                  k_scale, c_mass, K);
+        if (!ctx->d_kb) {
+            const int rc = dalloc(ctx, &ctx->d_kb, (int64_t)8 * kRecStride * ctx->n_el);
+            if (rc) return rc;
+        }
+        launch_hex8_elem_rec(ctx->st, (int)ctx->n_el, ctx->d_conn_r, ctx->d_xyz_r, E, ctx->d_nu, ctx->d_rho, k_scale,
+                             c_mass, ctx->d_kb);
+        launch_rec_gather(ctx->st, (int)ctx->n_nodes, (int)ctx->n_slices, ctx->d_sptr, ctx->d_n2e_ptr, ctx->d_n2rec,
+                          ctx->d_slot, ctx->d_kb, nullptr, 0.0, nullptr, K, nullptr, nullptr);
     }
     HIPCHK(hipGetLastError());
     return ASEND_OK;
@@ -945,8 +953,9 @@ int nl_prepare(asend_ctx* ctx) {
                     ctx->max_valence);
     int rc;
     const int64_t n = ctx->n_dof;
-    if (!ctx->d_nl_ke &&
-        ((rc = dalloc(ctx, &ctx->d_nl_ke, 576 * ctx->n_el)) || (rc = dalloc(ctx, &ctx->d_nl_re, 24 * ctx->n_el)) ||
+    if (!ctx->d_kb && (rc = dalloc(ctx, &ctx->d_kb, (int64_t)8 * kRecStride * ctx->n_el))) return rc;
+    if (!ctx->d_nl_re &&
+        ((rc = dalloc(ctx, &ctx->d_nl_re, 24 * ctx->n_el)) ||
          (rc = dalloc(ctx, &ctx->d_nl_u, n)) || (rc = dalloc(ctx, &ctx->d_nl_f, n)) ||
          (rc = dalloc(ctx, &ctx->d_nl_r, n))))
         return rc;
@@ -956,10 +965,9 @@ int nl_prepare(asend_ctx* ctx) {
 int nl_assemble(asend_ctx* ctx, double fact, double* rhs) {
     int rc;
     launch_hex8_nl_element(ctx->st, (int)ctx->n_el, ctx->d_conn_r, ctx->d_xyz_r, ctx->d_E, ctx->d_nu, ctx->d_nl_u,
-                           ctx->d_nl_ke, ctx->d_nl_re);
-    launch_nl_gather(ctx->st, (int)ctx->n_nodes, (int)ctx->n_slices, ctx->d_sptr, ctx->d_scol, ctx->d_rp,
-                     ctx->d_n2e_ptr, ctx->d_n2e, ctx->d_conn_r, ctx->d_nl_ke, ctx->d_nl_re, fact, ctx->d_nl_f,
-                     ctx->d_K64, ctx->d_nl_r, rhs);
+                           ctx->d_kb, ctx->d_nl_re);
+    launch_rec_gather(ctx->st, (int)ctx->n_nodes, (int)ctx->n_slices, ctx->d_sptr, ctx->d_n2e_ptr, ctx->d_n2rec,
+                      ctx->d_slot, ctx->d_kb, ctx->d_nl_re, fact, ctx->d_nl_f, ctx->d_K64, ctx->d_nl_r, rhs);
     HIPCHK(hipGetLastError());
     if ((rc = finish_assembly(ctx, 1.0, 0.0))) return rc;
     if (rhs) {
@@ -1184,14 +1192,19 @@ int asend_upload_mesh(asend_ctx* ctx, int elem_type, int64_t n_mesh_nodes, const
     for (int64_t nd = 0; nd < n_mesh_nodes; ++nd)
         for (int c = 0; c < 3; ++c) xyz_r[3 * (int64_t)rank[nd] + c] = xyz[3 * nd + c];
     // node -> element adjacency
-    std::vector<int32_t> n2e_ptr(n_nodes + 1, 0), n2e;
+    std::vector<int32_t> n2e_ptr(n_nodes + 1, 0), n2e, n2rec;
     for (int64_t i = 0; i < 8 * n_el; ++i) n2e_ptr[conn_r[i] + 1]++;
     for (int64_t r = 0; r < n_nodes; ++r) n2e_ptr[r + 1] += n2e_ptr[r];
     n2e.resize(n2e_ptr[n_nodes]);
+    n2rec.resize(n2e_ptr[n_nodes]);
     {
         std::vector<int32_t> fill(n2e_ptr.begin(), n2e_ptr.end() - 1);
         for (int64_t e = 0; e < n_el; ++e)
-            for (int a = 0; a < 8; ++a) n2e[fill[conn_r[8 * e + a]]++] = (int32_t)e;
+            for (int a = 0; a < 8; ++a) {
+                const int32_t pos = fill[conn_r[8 * e + a]]++;
+                n2e[pos] = (int32_t)e;
+                n2rec[pos] = (int32_t)(8 * e + a);
+            }
     }
     ctx->max_valence = 0;
     for (int64_t r = 0; r < n_nodes; ++r) ctx->max_valence = std::max(ctx->max_valence, n2e_ptr[r + 1] - n2e_ptr[r]);
@@ -1350,12 +1363,14 @@ int asend_upload_mesh(asend_ctx* ctx, int elem_type, int64_t n_mesh_nodes, const
         (rc = dalloc(ctx, &ctx->d_K64, nval)) || (rc = dalloc(ctx, &ctx->d_K32, nval)) ||
         (rc = dalloc(ctx, &ctx->d_dinv64, n_dof)) || (rc = dalloc(ctx, &ctx->d_dinv32, n_dof)) ||
         (rc = dalloc(ctx, &ctx->d_n2e_ptr, n_nodes + 1)) || (rc = dalloc(ctx, &ctx->d_n2e, n2e.size())) ||
+        (rc = dalloc(ctx, &ctx->d_n2rec, n2rec.size())) ||
         (rc = dalloc(ctx, &ctx->d_clamped, n_nodes)) || (rc = dalloc(ctx, &ctx->d_enb, enb.size())) ||
         (rc = dalloc(ctx, &ctx->d_erev, erev.size())) || (rc = dalloc(ctx, &ctx->d_eslot, eslot.size())))
         return rc;
     ctx->n_edge = n_edge;
     HIPCHK(hipMemcpy(ctx->d_n2e_ptr, n2e_ptr.data(), n2e_ptr.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->d_n2e, n2e.data(), n2e.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_n2rec, n2rec.data(), n2rec.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemset(ctx->d_clamped, 0, n_nodes));
     HIPCHK(hipMemcpy(ctx->d_enb, enb.data(), enb.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->d_erev, erev.data(), erev.size() * 4, hipMemcpyHostToDevice));

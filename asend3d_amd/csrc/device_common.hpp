@@ -12,6 +12,13 @@ constexpr int kBlock = 256;  // threads per block for streaming kernels
 constexpr int kNumXcd = 8;
 constexpr int kMaxBlockWaves = 16;  // block reductions accept up to 1024 threads
 
+// Element blocks of brick8 meshes by rows: record (e, a) holds the eight 3x3
+// blocks K_ab (b = 0..7) of element e's row a, row-major, at 9 b of its
+// kRecStride doubles (576 bytes, a multiple of 16); element e's records are
+// contiguous, record (e, a) at kRecStride (8 e + a).  Record (e, a) is
+// consumed by exactly one operator row (node conn[e][a]).
+constexpr int kRecStride = 72;
+
 // Logical block index such that each XCD (blocks are dispatched round-robin
 // over the 8 XCDs) processes one contiguous range of logical blocks, so rows
 // that share x-vector entries share an L2.

@@ -39,13 +39,18 @@ __host__ __device__ inline int64_t sell_index(const int32_t* sptr, int r, int k,
     return ((int64_t)9 * (sptr[r >> 6] + k) + c) * 64 + (r & 63);
 }
 
-// brick8 by rows: writes K (every slot, padding as zeros) = sum of
-// k_scale K_e + c_mass M_e over the elements of each row's node; n2e lists
-// each node's elements
-void launch_hex8_assemble_rows(hipStream_t st, int n_rows, int n_slices, const int32_t* sptr, const int32_t* scol,
-                               const int32_t* rp, const int32_t* n2e_ptr, const int32_t* n2e, const int32_t* conn_r,
-                               const double* xyz_r, const double* E, const double* nu, const double* rho,
-                               double k_scale, double c_mass, double* K);
+// brick8 general meshes, element-once: the blocks k_scale K_e + c_mass
+// rho_e M_e of every element as row records into Kr (8 kRecStride doubles per
+// element), then their row gather into K (every slot, padding as zeros);
+// n2rec lists the records 8 e + a of each row (CSR by n2e_ptr), slot maps
+// block (e, a, b) to its in-row slot.  With Re set the gather also forms the
+// internal force rint of the element residuals Re (24 per element) and, with
+// rhs set, rhs = fact f - rint.
+void launch_hex8_elem_rec(hipStream_t st, int n_el, const int32_t* conn_r, const double* xyz_r, const double* E,
+                          const double* nu, const double* rho, double k_scale, double c_mass, double* Kr);
+void launch_rec_gather(hipStream_t st, int n_rows, int n_slices, const int32_t* sptr, const int32_t* n2e_ptr,
+                       const int32_t* n2rec, const uint8_t* slot, const double* Kr, const double* Re, double fact,
+                       const double* f, double* K, double* rint, double* rhs);
 // Khat (64 x 9) and Mhat (64) of the element conn_r[0..7] with E = rho = 1
 void launch_hex8_reference(hipStream_t st, const int32_t* conn_r, const double* xyz_r, double nu, double* Khat,
                            double* Mhat);
@@ -73,15 +78,11 @@ void launch_shell4_assemble(hipStream_t st, const int32_t* elems, int n_elems, c
                             const double* rho, double k_scale, double m_scale, const int32_t* sptr,
                             const uint8_t* slot, double* K);
 // Geometrically nonlinear brick8 (nonlinear.hip): element tangents Ke
-// (576 per element) and residuals Re (24) at displacement u; their row
-// gather into the sliced-ELL operator K with the internal force rint and, if
-// rhs is set, rhs = fact f - rint; the penalty load rhs -= mult sf u.
+// (row records, 8 kRecStride per element) and residuals Re (24) at
+// displacement u (gathered by launch_rec_gather); the penalty load
+// rhs -= mult sf u.
 void launch_hex8_nl_element(hipStream_t st, int n_el, const int32_t* conn_r, const double* xyz_r, const double* E,
                             const double* nu, const double* u, double* Ke, double* Re);
-void launch_nl_gather(hipStream_t st, int n_rows, int n_slices, const int32_t* sptr, const int32_t* scol,
-                      const int32_t* rp, const int32_t* n2e_ptr, const int32_t* n2e, const int32_t* conn_r,
-                      const double* Ke, const double* Re, double fact, const double* f, double* K, double* rint,
-                      double* rhs);
 void launch_nl_penalty_load(hipStream_t st, const int64_t* pen, const int32_t* mult, int n_pen, const Scalars* sc,
                             const double* u, double* rhs);
 void launch_maxabs(hipStream_t st, const double* v, int64_t n, double* partials, Scalars* sc);

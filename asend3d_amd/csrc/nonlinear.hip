@@ -1,7 +1,7 @@
 // Geometrically nonlinear brick8 statics (total Lagrangian, St. Venant-
-// Kirchhoff): element residual and tangent at a displacement state, their
-// row-wise gather into the sliced-ELL operator, and the Newton right-hand
-// side.
+// Kirchhoff): element residual and tangent at a displacement state (gathered
+// into the sliced-ELL operator by the shared row gather) and the penalty part
+// of the Newton right-hand side.
 //
 // Strain: Green-Lagrange E = (H + H^T + H^T H) / 2 with H the displacement
 // gradient (S/element_soln_fields.rs:1345-1461, n_lgeom terms); stress
@@ -42,7 +42,8 @@ __device__ __forceinline__ double stride8_sum(double v) {
 // adds detJ F S g_t to the residual of node t (summed over gp by a second
 // butterfly) and the group leader stages F, F F^T and S in LDS.  Phase 2,
 // lane = 8 a + b: the 3x3 tangent block of nodes (a, b) summed over the 8
-// points.  Ke[576 e + 9 (8 a + b) + 3 i + k], Re[24 e + 3 t + i].
+// points, stored as row records: Ke[kRecStride (8 e + a) + 9 b + 3 i + k],
+// Re[24 e + 3 t + i].
 __global__ __launch_bounds__(256) void k_hex8_nl_element(int n_el, const int32_t* __restrict__ conn_r,
                                                          const double* __restrict__ xyz_r,
                                                          const double* __restrict__ E,
@@ -182,85 +183,9 @@ __global__ __launch_bounds__(256) void k_hex8_nl_element(int n_el, const int32_t
                                        (i == k ? gsg : 0.0));
     }
     if (live) {
-        double* out = Ke + 576 * (int64_t)e + 9 * lane;
+        double* out = Ke + (int64_t)kRecStride * (8 * (int64_t)e + a) + 9 * b;
 #pragma unroll
         for (int c = 0; c < 9; ++c) out[c] = kb[c];
-    }
-}
-
-// Row-wise gather of the element tangents into the sliced-ELL operator (one
-// lane per block row, one wavefront per 64-row slice), plus the internal
-// force of the row node and the Newton load fact f - r(u).  Every slot,
-// padding included, is written once.
-__global__ __launch_bounds__(256) void k_nl_gather(int n_rows, int n_slices, const int32_t* __restrict__ sptr,
-                                                   const int32_t* __restrict__ scol, const int32_t* __restrict__ rp,
-                                                   const int32_t* __restrict__ n2e_ptr,
-                                                   const int32_t* __restrict__ n2e,
-                                                   const int32_t* __restrict__ conn_r,
-                                                   const double* __restrict__ Ke, const double* __restrict__ Re,
-                                                   double fact, const double* __restrict__ f,
-                                                   double* __restrict__ K, double* __restrict__ rint,
-                                                   double* __restrict__ rhs) {
-    const int lane = threadIdx.x & 63;
-    const int s = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (s >= n_slices) return;
-    const int r = 64 * s + lane;
-    const bool live = r < n_rows;
-    const int s0 = sptr[s], w = sptr[s + 1] - s0;
-    const int nrow = live ? rp[r + 1] - rp[r] : 0;
-    const int q0 = live ? n2e_ptr[r] : 0, nq = live ? n2e_ptr[r + 1] - q0 : 0;
-    int cn[8][8], av[8], ev[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        av[q] = 0;
-        ev[q] = 0;
-#pragma unroll
-        for (int t = 0; t < 8; ++t) cn[q][t] = -1;
-        if (q < nq) {
-            const int e = n2e[q0 + q];
-            ev[q] = e;
-#pragma unroll
-            for (int t = 0; t < 8; ++t) {
-                cn[q][t] = conn_r[8 * (int64_t)e + t];
-                if (cn[q][t] == r) av[q] = t;
-            }
-        }
-    }
-    if (live) {
-        double ri[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-        for (int q = 0; q < 8; ++q)
-            if (q < nq)
-#pragma unroll
-                for (int i = 0; i < 3; ++i) ri[i] += Re[24 * (int64_t)ev[q] + 3 * av[q] + i];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            const int64_t d = 3 * (int64_t)r + i;
-            rint[d] = ri[i];
-            if (rhs) rhs[d] = fact * f[d] - ri[i];
-        }
-    }
-    for (int k = 0; k < w; ++k) {
-        double acc[9];
-#pragma unroll
-        for (int c = 0; c < 9; ++c) acc[c] = 0.0;
-        if (k < nrow) {
-            const int col = scol[64 * (int64_t)(s0 + k) + lane];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                int bq = -1;
-#pragma unroll
-                for (int t = 0; t < 8; ++t) bq = (cn[q][t] == col) ? t : bq;
-                if (bq >= 0) {
-                    const double* blk = Ke + 576 * (int64_t)ev[q] + 9 * (8 * av[q] + bq);
-#pragma unroll
-                    for (int c = 0; c < 9; ++c) acc[c] += blk[c];
-                }
-            }
-        }
-        double* base = K + (int64_t)9 * 64 * (s0 + k) + lane;
-#pragma unroll
-        for (int c = 0; c < 9; ++c) base[64 * c] = acc[c];
     }
 }
 
@@ -282,15 +207,6 @@ void launch_hex8_nl_element(hipStream_t st, int n_el, const int32_t* conn_r, con
     if (n_el <= 0) return;
     hipLaunchKernelGGL(k_hex8_nl_element, dim3((unsigned)((n_el + kNlWaves - 1) / kNlWaves)), dim3(64 * kNlWaves),
                        0, st, n_el, conn_r, xyz_r, E, nu, u, Ke, Re);
-}
-
-void launch_nl_gather(hipStream_t st, int n_rows, int n_slices, const int32_t* sptr, const int32_t* scol,
-                      const int32_t* rp, const int32_t* n2e_ptr, const int32_t* n2e, const int32_t* conn_r,
-                      const double* Ke, const double* Re, double fact, const double* f, double* K, double* rint,
-                      double* rhs) {
-    if (n_slices <= 0) return;
-    hipLaunchKernelGGL(k_nl_gather, dim3((unsigned)((n_slices + 3) / 4)), dim3(256), 0, st, n_rows, n_slices, sptr,
-                       scol, rp, n2e_ptr, n2e, conn_r, Ke, Re, fact, f, K, rint, rhs);
 }
 
 void launch_nl_penalty_load(hipStream_t st, const int64_t* pen, const int32_t* mult, int n_pen, const Scalars* sc,

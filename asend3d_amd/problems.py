@@ -7,6 +7,7 @@ S/const_meth.rs:30-90).  Load: uniform surface traction on the x = lx face
 elements with normDir (1,0,0) and normTol 5 degrees
 (S/element_equations.rs:1833-2056).
 """
+import dataclasses
 from dataclasses import dataclass
 
 import numpy as np
@@ -71,6 +72,20 @@ def heterogeneous(nx, ny, nz, seed=C2_SEED, median=70e3, sigma_ln=1.0, nu=0.3, l
     n_el = nx * ny * nz
     E = api.lognormal_moduli(seed, n_el, median, sigma_ln)
     return cantilever(nx, ny, nz, lx, ly, lz, E=E, nu=nu, traction=traction)
+
+
+def distorted(p, seed, amp=0.2, nu_range=(0.2, 0.4)):
+    """The same problem on a mesh whose interior nodes move by up to amp
+    times the smallest element edge in each direction (seeded), with a
+    per-element Poisson ratio drawn from nu_range: the general, per-element
+    integration path."""
+    rng = np.random.default_rng(seed)
+    m = p.mesh
+    xyz = m.xyz.copy()
+    h = min(m.lx / m.nx, m.ly / m.ny, m.lz / m.nz)
+    inner = np.all((xyz > xyz.min(axis=0) + 1e-9) & (xyz < xyz.max(axis=0) - 1e-9), axis=1)
+    xyz[inner] += rng.uniform(-amp, amp, (int(inner.sum()), 3)) * h
+    return dataclasses.replace(p, mesh=dataclasses.replace(m, xyz=xyz), nu=rng.uniform(*nu_range, m.n_el))
 
 
 def config2():

@@ -22,6 +22,8 @@
               the 5-step quadratic load ramp, each tangent solved by multigrid
               FP32 FCG with FP64 refreshes; then the same with inexact
               Newton solves (Eisenstat-Walker forcing).
+  general  -- assembly of the 200x40x40 heterogeneous block on a distorted
+              mesh (per-element integration and row gather).
 
     python -m asend3d_amd.workloads batched [--variants 64] [--sample 2]
     python -m asend3d_amd.workloads newmark [--nx 100 --ny 20 --nz 20 --steps 1000]
@@ -29,6 +31,7 @@
     python -m asend3d_amd.workloads adjoint [--nx 100 --ny 20 --nz 20 --steps 1000]
     python -m asend3d_amd.workloads modal [--modes 10]
     python -m asend3d_amd.workloads nonlinear [--nx 200 --ny 40 --nz 40 --load-steps 5]
+    python -m asend3d_amd.workloads general [--nx 200 --ny 40 --nz 40]
 
 Each prints one JSON line.
 """
@@ -193,6 +196,26 @@ def nonlinear(nx=200, ny=40, nz=40, load_steps=5, traction=90.0, du_tol=1e-6, to
     return out
 
 
+def general(nx=200, ny=40, nz=40, reps=10, seed=6):
+    """Assembly of the heterogeneous block on a distorted mesh (interior
+    nodes moved by up to 0.2 of an element edge, per-element Poisson ratio):
+    every element integrated once, then the row gather into the operator."""
+    p = problems.distorted(problems.heterogeneous(nx, ny, nz), seed)
+    ctx = api.Context(0)
+    problems.upload(ctx, p)
+    assert not ctx.uniform_elements()
+    ctx.assemble()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.assemble()
+    ctx.sync()
+    wall = (time.perf_counter() - t0) / reps
+    ctx.close()
+    return {"workload": "general", "mesh": f"{nx}x{ny}x{nz} brick8, distorted, lognormal E, per-element nu",
+            "elements": int(p.mesh.n_el), "dof": int(p.n_dof), "assemblies": reps,
+            "wall_ms_per_assembly": wall * 1e3}
+
+
 def shell(nx=256, ny=256, steps=1000, dt=1e-5, graphs=True, tol=1e-8, max_it=2000, load=1.0e4, solver="auto", ru_factor=1e-3):
     p = problems.shell_plate(nx, ny, 1.0, 1.0, 0.005, 70e9, 0.33, 2700.0)
     ctx = api.Context(0)
@@ -226,7 +249,7 @@ def shell(nx=256, ny=256, steps=1000, dt=1e-5, graphs=True, tol=1e-8, max_it=200
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("which", choices=["batched", "newmark", "shell", "adjoint", "modal", "nonlinear"])
+    ap.add_argument("which", choices=["batched", "newmark", "shell", "adjoint", "modal", "nonlinear", "general"])
     ap.add_argument("--load-steps", type=int, default=5)
     ap.add_argument("--modes", type=int, default=10)
     ap.add_argument("--variants", type=int, default=64)
@@ -244,6 +267,8 @@ def main():
         out = batched(a.variants, a.sample)
     elif a.which == "modal":
         out = modal(a.modes)
+    elif a.which == "general":
+        out = general(a.nx or 200, a.ny or 40, a.nz or 40)
     elif a.which == "nonlinear":
         out = nonlinear(a.nx or 200, a.ny or 40, a.nz or 40, a.load_steps, graphs=not a.no_graphs)
     elif a.which == "adjoint":

@@ -3,7 +3,6 @@ tangent and residual kernels, row gather, Newton loop under the load ramp)
 against the CPU oracle (oracle/nonlinear.py) and the closed-form
 St. Venant-Kirchhoff uniaxial stretch.  Every device call goes through the
 C ABI."""
-import dataclasses
 
 import numpy as np
 import pytest
@@ -17,16 +16,6 @@ from asend3d_amd.meshgen import hex_block
 from conftest import gpu_case
 
 pytestmark = pytest.mark.gpu
-
-
-def distorted(p, seed):
-    rng = np.random.default_rng(seed)
-    m = p.mesh
-    xyz = m.xyz.copy()
-    h = min(m.lx / m.nx, m.ly / m.ny, m.lz / m.nz)
-    inner = np.all((xyz > xyz.min(axis=0) + 1e-9) & (xyz < xyz.max(axis=0) - 1e-9), axis=1)
-    xyz[inner] += rng.uniform(-0.2, 0.2, (int(inner.sum()), 3)) * h
-    return dataclasses.replace(p, mesh=dataclasses.replace(m, xyz=xyz), nu=rng.uniform(0.2, 0.4, m.n_el))
 
 
 def csr(ctx, n):
@@ -50,7 +39,7 @@ def test_tangent_and_residual_match_oracle(ctx, which):
     gpu_case(f"nonlinear tangent {which}")
     p = problems.heterogeneous(8, 4, 3, seed=11)
     if "distorted" in which:
-        p = distorted(p, seed=12)
+        p = problems.distorted(p, seed=12)
     problems.upload(ctx, p)
     rng = np.random.default_rng(13)
     # displacement gradients of order 0.1 (elements are 125 x 50 x 67 mm)
