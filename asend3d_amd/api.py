@@ -138,7 +138,7 @@ EXPORTED = ["asend_ctx_create", "asend_ctx_destroy", "asend_last_error", "asend_
             "asend_compliance_clamp_check",
             "asend_upload_mesh", "asend_set_thickness", "asend_set_grid", "asend_update_moduli", "asend_set_penalty",
             "asend_penalty_scale", "asend_uniform_elements", "asend_assemble", "asend_get_matrix", "asend_spmv", "asend_solve", "asend_solve_noalloc",
-            "asend_element_sens", "asend_static_compliance", "asend_bench_spmv", "asend_mg_info", "asend_mg_apply",
+            "asend_element_sens", "asend_element_sens_material", "asend_static_compliance", "asend_bench_spmv", "asend_mg_info", "asend_mg_apply",
             "asend_mg_restrict", "asend_mg_prolong", "asend_mg_vcycle", "asend_mg_coarse_solve", "asend_bench_copy",
             "asend_learned_num_params", "asend_learned_init", "asend_learned_get_params",
             "asend_learned_set_params", "asend_learned_train", "asend_learned_eval", "asend_learned_apply",
@@ -360,6 +360,15 @@ class Context:
         lam = _f64(lam)
         out = np.zeros(self.n_el)
         self._chk(_lib.asend_element_sens(self.h, _p(u), _p(lam), _p(out)), "asend_element_sens")
+        return out
+
+    def element_sens_material(self, u, lam):
+        """(n_el, 9) partials of -lambda^T K_e u w.r.t. E1, E2, E3, nu12, nu13,
+        nu23, G12, G13, G23 (asend_element_sens_material, brick8)."""
+        u = _f64(u)
+        lam = _f64(lam)
+        out = np.zeros((self.n_el, 9))
+        self._chk(_lib.asend_element_sens_material(self.h, _p(u), _p(lam), _p(out)), "asend_element_sens_material")
         return out
 
     def static_compliance(self, f, opts):

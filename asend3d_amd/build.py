@@ -2,6 +2,8 @@
 
     python -m asend3d_amd.build        (or asend3d_amd.build.build())
 
+also links the job-file driver asend3d_amd/asend_run_job against it.
+
 HIP sources are compiled with hipcc for gfx950 only; the host-only preparation
 code with g++.  Objects go to _build/ at the repository root.
 """
@@ -14,6 +16,8 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 OUT = os.path.join(PKG, "libasend_gpu.so")
+CLI = os.path.join(PKG, "asend_run_job")
+CLI_SRC = os.path.join(PKG, "cli", "asend_run_job.cpp")
 BUILD = os.path.join(ROOT, "_build")
 
 HIP_SOURCES = ["kernels.hip", "precond.hip", "learned.hip", "dynamics.hip", "batched.hip", "shell.hip", "brickim.hip", "modal.hip", "nonlinear.hip", "context.hip"]
@@ -65,6 +69,11 @@ def build(verbose=False):
     tmp = OUT + ".tmp"
     _run([HIPCC, f"--offload-arch={OFFLOAD_ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, OUT)
+    # the job-file driver, linked against the library next to it
+    cli_tmp = CLI + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), CLI_SRC, "-o", cli_tmp,
+          "-L", os.path.dirname(OUT), "-lasend_gpu", "-Wl,-rpath,$ORIGIN"])
+    os.replace(cli_tmp, CLI)
     return OUT
 
 

@@ -1,0 +1,932 @@
+// asend_run_job <job.yaml>: the reference's solver executable
+// (S/main.rs:55-70) for static elastic jobs, running the solve path through
+// the C ABI of libasend_gpu.so on the GPU.
+//
+// Job, model, constraint, load, design-variable and objective files are read
+// with the reference's line rules (read_input_line, S/model_input.rs:34-119)
+// and keys (read_job :121-310, read_model_input :312-860,
+// read_constraint_input :872-969, read_load_input :981-1123,
+// read_des_var_input :1218-1296, read_objective_input :1298-1375,
+// read_des_var_values :1377-1401); commands run in file order as in
+// execute_job (S/model_meth.rs:8-138).  Supported subset:
+//   - one element type per model: brick8, brickIM or shell4 (one layer);
+//     isotropic materials (E, nu, G = E/(2(1+nu)) after design variables);
+//   - displacement constraints with one term of coefficient +-1 and rhs 0
+//     (clamps, applied by the penalty method);
+//   - loads nodalForce and surfaceTraction (solid elements);
+//   - solve: static, elastic, linear; solverMethod direct (default) or
+//     iterative (reference stop rule on the device's Jacobi PCG,
+//     convergenceTol, maxIterations; maxIterations 0, which makes the
+//     reference return its zero start vector, means no limit here);
+//   - objective terms of category displacement with operator powerNorm;
+//   - design variables of category modulus, poissonRatio, shearModulus
+//     (brick8) for calcObjGradient;
+//   - writeNodeResults (displacement, reactionForce) and writeObjective
+//     (with the _grad file), numbers formatted as the reference's {:.12e}.
+// Anything else stops the job with an error message, as the reference's
+// panics do.
+#include <array>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "asend_gpu.h"
+
+namespace {
+
+[[noreturn]] void die(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    std::fprintf(stderr, "Error: ");
+    std::vfprintf(stderr, fmt, ap);
+    std::fprintf(stderr, "\n");
+    va_end(ap);
+    std::exit(1);
+}
+
+// ---------------------------------------------------------------- input lines
+// Heading levels are assigned by the column of a key's first character: the
+// first key sets level 0, a key at a new column the next free level, a key at
+// a known column that level (clearing the deeper ones).  Values after the
+// colon, or after "- " on lines without a key, are split on " ,[]\t".
+struct LineReader {
+    std::string head[4];
+    size_t col[4] = {0, 0, 0, 0};
+
+    static void tokens(const std::string& s, size_t from, std::vector<std::string>& out) {
+        const char* sep = " ,[]\t\n\r";
+        size_t i = s.find_first_not_of(sep, from);
+        while (i != std::string::npos) {
+            size_t j = s.find_first_of(sep, i);
+            if (j == std::string::npos) j = s.size();
+            out.push_back(s.substr(i, j - i));
+            i = s.find_first_not_of(sep, j);
+        }
+    }
+
+    // returns true when the line holds a key (the headings were updated)
+    bool read(std::string line, std::vector<std::string>& data) {
+        data.clear();
+        size_t h = line.find('#');
+        if (h != std::string::npos) line = line.substr(0, h);
+        line += " ";
+        size_t c = line.find(':');
+        if (c != std::string::npos) {
+            size_t i2 = line.find_first_not_of(" -\n\t");
+            std::string word = (i2 < c) ? line.substr(i2, c - i2) : std::string();
+            int lev;
+            if (head[0].empty() || col[0] == i2) lev = 0;
+            else if (head[1].empty() || col[1] == i2) lev = 1;
+            else if (head[2].empty() || col[2] == i2) lev = 2;
+            else lev = 3;
+            head[lev] = word;
+            col[lev] = i2;
+            for (int k = lev + 1; k < 4; ++k) {
+                head[k].clear();
+                col[k] = 0;
+            }
+            tokens(line, c + 1, data);
+            return true;
+        }
+        size_t d = line.find("- ");
+        if (d != std::string::npos) tokens(line, d + 1, data);
+        return false;
+    }
+};
+
+std::vector<std::string> read_lines(const std::string& path, const char* what) {
+    std::ifstream in(path);
+    if (!in) die("could not open %s file: %s", what, path.c_str());
+    std::vector<std::string> lines;
+    std::string s;
+    while (std::getline(in, s)) lines.push_back(s);
+    return lines;
+}
+
+double to_d(const std::string& s) {
+    char* end = nullptr;
+    double v = std::strtod(s.c_str(), &end);
+    if (end == s.c_str()) die("could not read '%s' as a number", s.c_str());
+    return v;
+}
+long to_i(const std::string& s) { return (long)std::llround(to_d(s)); }
+bool yes(const std::string& s) { return s.find("yes") != std::string::npos; }
+
+// Rust's {:.12e}: 12 decimals, exponent without sign padding or leading zeros
+std::string fmt_e12(double v) {
+    if (std::isnan(v)) return "NaN";
+    if (std::isinf(v)) return v > 0 ? "inf" : "-inf";
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%.12e", v);
+    std::string s(buf);
+    size_t e = s.find('e');
+    std::string mant = s.substr(0, e);
+    int ex = std::atoi(s.c_str() + e + 1);
+    return mant + "e" + std::to_string(ex);
+}
+
+// ---------------------------------------------------------------- the model
+struct Section {
+    std::string type, mat, el_set;
+    std::vector<std::string> lay_mat;
+    std::vector<double> lay_t;
+    double z_offset = 0.0;
+};
+struct Material {
+    std::string name;
+    double density = 0.0;
+    double E[3] = {0, 0, 0}, nu[3] = {0, 0, 0}, G[3] = {0, 0, 0};
+    bool stiffness = false;
+};
+struct Term {
+    std::string set;
+    int dof = 1;
+    double coef = 1.0;
+};
+struct Constraint {
+    std::string type;
+    std::vector<Term> terms;
+    double rhs = 0.0;
+};
+struct Load {
+    std::string type, node_set, el_set;
+    double load[6] = {0, 0, 0, 0, 0, 0};
+    double norm_dir[3] = {0, 0, 0};
+    double norm_tol = 1.0e-3;
+};
+struct DesignVar {
+    std::string category, el_set, nd_set;
+    int component = 1;
+    std::vector<double> coefs;
+    double value = 0.0;
+};
+struct ObjTerm {
+    std::string category, optr, el_set, nd_set;
+    int component = 1;
+    double coef = 1.0, expnt = 1.0;
+    std::vector<double> tgt;
+    double value = 0.0;
+};
+struct Command {
+    std::string cmd, file, method, node_set = "all";
+    bool elastic = true, dynamic = false, nonlinear = false, thermal = false, write_grad = true;
+    long block_dim = -1, max_it = 0;
+    double conv_tol = 1.0e-12;
+    std::vector<std::string> fields, include;
+    std::vector<long> time_steps;
+};
+struct NamedSet {
+    std::string name;
+    std::vector<int32_t> labels;
+};
+
+struct Job {
+    std::vector<Command> cmds;
+    std::vector<double> xyz;  // 3 per node label
+    std::vector<int> el_type;
+    std::vector<std::vector<int32_t>> conn;
+    std::vector<NamedSet> nsets, esets;
+    std::map<std::string, size_t> ns_map, es_map;
+    std::vector<Section> sections;
+    std::vector<Material> materials;
+    std::vector<Constraint> constraints;
+    std::vector<Load> loads;
+    std::vector<DesignVar> dvars;
+    std::vector<ObjTerm> obj;
+
+    // analysis state
+    asend_ctx* ctx = nullptr;
+    int etype = 0, dpn = 3;
+    int64_t n_dof = 0;
+    std::vector<int64_t> dof_index;  // 6 per node
+    std::vector<double> E, nu, rho, thk;
+    std::vector<int64_t> pen;
+    std::vector<double> u;    // solution, reference DOF numbering
+    std::vector<double> f;    // applied load
+    std::vector<double> dLdD;
+    bool solved = false, have_grad = false;
+    Command solve_cmd;
+
+    ~Job() {
+        if (ctx) asend_ctx_destroy(ctx);
+    }
+
+    const std::vector<int32_t>& node_set(const std::string& n, bool fallback_all) {
+        auto it = ns_map.find(n);
+        if (it == ns_map.end()) {
+            if (!fallback_all) die("there is no Node Set named %s", n.c_str());
+            std::printf("Warning: there is no Node Set named %s. Defaulting to all nodes in writeNodeResults\n",
+                        n.c_str());
+            it = ns_map.find("all");
+        }
+        return nsets[it->second].labels;
+    }
+    const std::vector<int32_t>& element_set(const std::string& n) {
+        auto it = es_map.find(n);
+        if (it == es_map.end()) die("there is no Element Set named %s", n.c_str());
+        return esets[it->second].labels;
+    }
+
+    void check(int rc, const char* what) {
+        if (rc != ASEND_OK) die("%s failed (%d): %s", what, rc, ctx ? asend_last_error(ctx) : "");
+    }
+
+    // ------------------------------------------------------------ readers
+    void read_job(const std::string& path) {
+        LineReader lr;
+        std::vector<std::string> d;
+        for (const auto& line : read_lines(path, "job")) {
+            lr.read(line, d);
+            const std::string& k = lr.head[1];
+            if (k == "command" && d.size() == 1) {
+                cmds.push_back(Command());
+                cmds.back().cmd = d[0];
+                continue;
+            }
+            if (cmds.empty()) continue;
+            Command& c = cmds.back();
+            if (k == "fileName" && d.size() == 1) c.file = d[0];
+            else if (k == "dynamic" && d.size() == 1) c.dynamic = yes(d[0]);
+            else if (k == "elastic" && d.size() == 1) c.elastic = yes(d[0]);
+            else if (k == "nonlinearGeom" && d.size() == 1) c.nonlinear = yes(d[0]);
+            else if ((k == "thermal" || k == "diffusion") && d.size() == 1) c.thermal = yes(d[0]);
+            else if (k == "solverBlockDim" && d.size() == 1) c.block_dim = to_i(d[0]);
+            else if (k == "solverMethod" && d.size() == 1) c.method = d[0];
+            else if (k == "maxIterations" && d.size() == 1) c.max_it = to_i(d[0]);
+            else if (k == "convergenceTol" && d.size() == 1) c.conv_tol = to_d(d[0]);
+            else if (k == "nodeSet" && d.size() == 1) c.node_set = d[0];
+            else if (k == "fields" && d.size() == 1) c.fields.push_back(d[0]);
+            else if (k == "include" && d.size() == 1) c.include.push_back(d[0]);
+            else if (k == "writeGradient" && d.size() == 1) c.write_grad = (d[0] == "yes");
+            else if (k == "timeSteps" && !d.empty() && d[0] != "all" && d[0] != "last") c.time_steps.push_back(0);
+        }
+    }
+
+    void read_model(const std::string& path) {
+        LineReader lr;
+        std::vector<std::string> d;
+        int cur_type = 0;
+        std::vector<NamedSet>* cur_sets = nullptr;
+        bool in_mat = false;
+        for (const auto& line : read_lines(path, "Model input")) {
+            const bool upd = lr.read(line, d);
+            const std::string* h = lr.head;
+            if (h[0] == "nodes" && d.size() == 4) {
+                const long n = to_i(d[0]);
+                if (n < 0) die("negative node label");
+                if ((size_t)(3 * n + 3) > xyz.size()) xyz.resize(3 * n + 3, 0.0);
+                for (int c = 0; c < 3; ++c) xyz[3 * n + c] = to_d(d[1 + c]);
+            } else if (h[0] == "elements") {
+                if (h[1] == "type" && d.size() == 1) {
+                    if (d[0] == "brick8") cur_type = ASEND_BRICK8;
+                    else if (d[0] == "brickIM") cur_type = ASEND_BRICKIM;
+                    else if (d[0] == "shell4") cur_type = ASEND_SHELL4;
+                    else die("element type %s is not supported by this solver", d[0].c_str());
+                } else if (h[1] == "connectivity" && d.size() > 1) {
+                    const long e = to_i(d[0]);
+                    if (e < 0) die("negative element label");
+                    if ((size_t)e >= conn.size()) {
+                        conn.resize(e + 1);
+                        el_type.resize(e + 1, 0);
+                    }
+                    el_type[e] = cur_type;
+                    conn[e].clear();
+                    for (size_t i = 1; i < d.size(); ++i) conn[e].push_back((int32_t)to_i(d[i]));
+                }
+            } else if (h[0] == "sets" && (h[1] == "node" || h[1] == "element") && !h[2].empty()) {
+                auto& sets = (h[1] == "node") ? nsets : esets;
+                if (upd) {
+                    sets.push_back(NamedSet());
+                    sets.back().name = h[2];
+                    cur_sets = &sets;
+                    continue;
+                }
+                if (cur_sets != &sets || sets.empty()) continue;
+                if (d.size() == 1) {
+                    sets.back().labels.push_back((int32_t)to_i(d[0]));
+                } else if (d.size() > 1) {
+                    const long a = to_i(d[0]), b = to_i(d[1]), st = d.size() == 3 ? to_i(d[2]) : 1;
+                    for (long i = a; i < b; i += st) sets.back().labels.push_back((int32_t)i);
+                }
+            } else if (h[0] == "sections") {
+                if (h[1] == "type" && d.size() == 1) {
+                    sections.push_back(Section());
+                    sections.back().type = d[0];
+                } else if (sections.empty()) {
+                    continue;
+                } else if (h[1] == "material" && d.size() == 1) {
+                    sections.back().mat = d[0];
+                } else if (h[1] == "elementSet" && d.size() == 1) {
+                    sections.back().el_set = d[0];
+                } else if (h[1] == "layup") {
+                    if (h[2] == "zOffset" && d.size() == 1) sections.back().z_offset = to_d(d[0]);
+                    else if (h[2] == "layers" && h[3] == "material" && d.size() == 1) {
+                        sections.back().lay_mat.push_back(d[0]);
+                        sections.back().lay_t.push_back(0.0);
+                    } else if (h[2] == "layers" && h[3] == "thickness" && d.size() == 1 &&
+                               !sections.back().lay_t.empty()) {
+                        sections.back().lay_t.back() = to_d(d[0]);
+                    }
+                }
+            } else if (h[0] == "materials" && !h[1].empty()) {
+                if (h[2].empty() && upd) {
+                    materials.push_back(Material());
+                    materials.back().name = h[1];
+                    in_mat = true;
+                } else if (in_mat) {
+                    Material* cur_mat = &materials.back();
+                    if (h[2] == "density" && d.size() == 1) cur_mat->density = to_d(d[0]);
+                    else if (h[2] == "elastic" && (h[3] == "E" || h[3] == "nu" || h[3] == "G") && !d.empty()) {
+                        double* v = (h[3] == "E") ? cur_mat->E : (h[3] == "nu") ? cur_mat->nu : cur_mat->G;
+                        v[0] = to_d(d[0]);
+                        v[1] = d.size() == 3 ? to_d(d[1]) : v[0];
+                        v[2] = d.size() == 3 ? to_d(d[2]) : v[0];
+                    } else if (h[2] == "elastic" && h[3] == "stiffness") {
+                        cur_mat->stiffness = true;
+                    }
+                }
+            }
+        }
+        // individual sets named by label and the "all" sets
+        // (read_model_input, S/model_input.rs:804-860)
+        const size_t nn = xyz.size() / 3, ne = conn.size();
+        for (size_t i = 0; i < nn; ++i) nsets.push_back(NamedSet{std::to_string(i), {(int32_t)i}});
+        NamedSet all_n{"all", {}};
+        for (size_t i = 0; i < nn; ++i) all_n.labels.push_back((int32_t)i);
+        nsets.push_back(all_n);
+        for (size_t i = 0; i < ne; ++i) esets.push_back(NamedSet{std::to_string(i), {(int32_t)i}});
+        NamedSet all_e{"all", {}};
+        for (size_t i = 0; i < ne; ++i) all_e.labels.push_back((int32_t)i);
+        esets.push_back(all_e);
+        ns_map.clear();
+        es_map.clear();
+        for (size_t i = 0; i < nsets.size(); ++i) ns_map[nsets[i].name] = i;
+        for (size_t i = 0; i < esets.size(); ++i) es_map[esets[i].name] = i;
+    }
+
+    // a file with constraints replaces the model's (read_constraint_input
+    // reallocates its lists; one without any leaves them)
+    void read_constraints(const std::string& path) {
+        LineReader lr;
+        std::vector<std::string> d;
+        bool cur_disp = false;
+        const auto lines = read_lines(path, "Constraint input");
+        {
+            LineReader cnt;
+            bool any = false;
+            for (const auto& line : lines) {
+                cnt.read(line, d);
+                any = any || (cnt.head[0] == "constraints" && cnt.head[1] == "type" && d.size() == 1);
+            }
+            if (!any) return;
+            constraints.clear();
+        }
+        for (const auto& line : lines) {
+            lr.read(line, d);
+            const std::string* h = lr.head;
+            if (h[0] != "constraints") continue;
+            if (h[1] == "type" && d.size() == 1) {
+                if (d[0] != "displacement" && d[0] != "temperature" && d[0] != "concentration" && d[0] != "fluid")
+                    die("%s is not a valid constraint type", d[0].c_str());
+                cur_disp = (d[0] == "displacement");
+                if (cur_disp) {
+                    constraints.push_back(Constraint());
+                    constraints.back().type = d[0];
+                }
+            } else if (!cur_disp || constraints.empty()) {
+                continue;
+            } else if (h[1] == "terms") {
+                if (h[2] == "nodeSet" && d.size() == 1) {
+                    constraints.back().terms.push_back(Term());
+                    constraints.back().terms.back().set = d[0];
+                } else if (h[2] == "dof" && d.size() == 1 && !constraints.back().terms.empty()) {
+                    constraints.back().terms.back().dof = (int)to_i(d[0]);
+                } else if (h[2] == "coef" && d.size() == 1 && !constraints.back().terms.empty()) {
+                    constraints.back().terms.back().coef = to_d(d[0]);
+                }
+            } else if (h[1] == "rhs" && d.size() == 1) {
+                constraints.back().rhs = to_d(d[0]);
+            }
+        }
+    }
+
+    // likewise for loads (read_load_input)
+    void read_loads(const std::string& path) {
+        LineReader lr;
+        std::vector<std::string> d;
+        const std::string elastic = "nodalForce bodyForce gravitational centrifugal surfacePressure surfaceTraction";
+        bool cur_el = false;
+        const auto lines = read_lines(path, "Load input");
+        {
+            LineReader cnt;
+            bool any = false;
+            for (const auto& line : lines) {
+                cnt.read(line, d);
+                any = any || (cnt.head[0] == "loads" && cnt.head[1] == "type" && d.size() == 1);
+            }
+            if (!any) return;
+            loads.clear();
+        }
+        for (const auto& line : lines) {
+            lr.read(line, d);
+            const std::string* h = lr.head;
+            if (h[0] != "loads") continue;
+            if (h[1] == "type" && d.size() == 1) {
+                cur_el = elastic.find(d[0]) != std::string::npos;
+                if (cur_el) {
+                    loads.push_back(Load());
+                    loads.back().type = d[0];
+                }
+            } else if (!cur_el || loads.empty()) {
+                continue;
+            } else if (h[1] == "load" && !d.empty()) {
+                for (int i = 0; i < 6; ++i) loads.back().load[i] = (i < (int)d.size()) ? to_d(d[i]) : 0.0;
+            } else if (h[1] == "nodeSet" && d.size() == 1) {
+                loads.back().node_set = d[0];
+            } else if (h[1] == "elementSet" && d.size() == 1) {
+                loads.back().el_set = d[0];
+            } else if (h[1] == "normDir" && d.size() == 3) {
+                for (int i = 0; i < 3; ++i) loads.back().norm_dir[i] = to_d(d[i]);
+            } else if (h[1] == "normTolerance" && d.size() == 1) {
+                loads.back().norm_tol = to_d(d[0]);
+            }
+        }
+    }
+
+    // a design-variable or objective file replaces the previous set
+    void read_dvars(const std::string& path) {
+        LineReader lr;
+        std::vector<std::string> d;
+        dvars.clear();
+        for (const auto& line : read_lines(path, "design variable input")) {
+            lr.read(line, d);
+            const std::string* h = lr.head;
+            if (h[0] != "designVariables") continue;
+            if (h[1] == "category" && d.size() == 1) {
+                dvars.push_back(DesignVar());
+                dvars.back().category = d[0];
+            } else if (dvars.empty()) {
+                continue;
+            } else if (h[1] == "elementSet" && d.size() == 1) {
+                dvars.back().el_set = d[0];
+            } else if (h[1] == "nodeSet" && d.size() == 1) {
+                dvars.back().nd_set = d[0];
+            } else if (h[1] == "component" && d.size() == 1) {
+                dvars.back().component = (int)to_i(d[0]);
+            } else if (h[1] == "coefficients" && d.size() == 1) {
+                dvars.back().coefs.push_back(to_d(d[0]));
+            }
+        }
+    }
+
+    void read_dvar_values(const std::string& path) {
+        LineReader lr;
+        std::vector<std::string> d;
+        for (const auto& line : read_lines(path, "design variable value input")) {
+            lr.read(line, d);
+            if (d.size() == 2) {
+                const long i = to_i(d[0]);
+                if (i < 0 || (size_t)i >= dvars.size()) die("design variable %ld is not defined", i);
+                dvars[i].value = to_d(d[1]);
+            }
+        }
+    }
+
+    void read_objective(const std::string& path) {
+        LineReader lr;
+        std::vector<std::string> d;
+        obj.clear();
+        for (const auto& line : read_lines(path, "Objective input")) {
+            lr.read(line, d);
+            const std::string* h = lr.head;
+            if (h[0] != "objectiveTerms") continue;
+            if (h[1] == "category" && d.size() == 1) {
+                obj.push_back(ObjTerm());
+                obj.back().category = d[0];
+            } else if (obj.empty()) {
+                continue;
+            } else if (h[1] == "operator" && d.size() == 1) {
+                obj.back().optr = d[0];
+            } else if (h[1] == "component" && d.size() == 1) {
+                obj.back().component = (int)to_i(d[0]);
+            } else if (h[1] == "coefficient" && d.size() == 1) {
+                obj.back().coef = to_d(d[0]);
+            } else if (h[1] == "exponent" && d.size() == 1) {
+                obj.back().expnt = to_d(d[0]);
+            } else if (h[1] == "elementSet" && d.size() == 1) {
+                obj.back().el_set = d[0];
+            } else if (h[1] == "nodeSet" && d.size() == 1) {
+                obj.back().nd_set = d[0];
+            } else if (h[1] == "targetValue" && d.size() == 1) {
+                obj.back().tgt.push_back(to_d(d[0]));
+            }
+        }
+    }
+
+    // ------------------------------------------------------------ analysis
+    // element material after design variables (get_solid_stiff_dfd0's
+    // modulus_dv / poisson_dv / shear_mod_dv, S/element_properties.rs:700-726)
+    void element_materials() {
+        const size_t ne = conn.size();
+        std::vector<int> sect(ne, -1);
+        for (size_t s = 0; s < sections.size(); ++s)
+            for (int32_t e : element_set(sections[s].el_set)) sect[e] = (int)s;
+        std::vector<std::array<double, 9>> par(ne);
+        thk.assign(ne, 0.0);
+        rho.assign(ne, 0.0);
+        auto mat_of = [&](const std::string& n) -> const Material& {
+            for (const auto& m : materials)
+                if (m.name == n) return m;
+            die("material %s is not defined", n.c_str());
+        };
+        for (size_t e = 0; e < ne; ++e) {
+            if (sect[e] < 0) die("element %zu has no section", e);
+            const Section& s = sections[sect[e]];
+            const Material* m;
+            if (etype == ASEND_SHELL4) {
+                if (s.lay_mat.size() != 1 || s.z_offset != 0.0)
+                    die("shell4 sections must have one layer centred on the mid-plane");
+                m = &mat_of(s.lay_mat[0]);
+                thk[e] = s.lay_t[0];
+            } else {
+                m = &mat_of(s.mat);
+            }
+            if (m->stiffness) die("materials given by a stiffness matrix are not supported");
+            for (int c = 0; c < 3; ++c) {
+                par[e][c] = m->E[c];
+                par[e][3 + c] = m->nu[c];
+                par[e][6 + c] = m->G[c];
+            }
+            rho[e] = m->density;
+        }
+        for (const auto& dv : dvars) {
+            int base;
+            if (dv.category == "modulus") base = 0;
+            else if (dv.category == "poissonRatio") base = 3;
+            else if (dv.category == "shearModulus") base = 6;
+            else die("design variables of category %s are not supported", dv.category.c_str());
+            if (dv.component < 1 || dv.component > 3) die("design variable component out of range");
+            if (dv.el_set.empty()) die("design variables of category %s need an element set", dv.category.c_str());
+            const auto& set = element_set(dv.el_set);
+            if (dv.coefs.size() >= 2 && dv.coefs.size() != set.size())
+                die("coefficient list provided for a design variable does not match the length of the element set");
+            for (size_t k = 0; k < set.size(); ++k) {
+                const double c = dv.coefs.empty() ? 1.0 : (dv.coefs.size() == 1 ? dv.coefs[0] : dv.coefs[k]);
+                par[set[k]][base + dv.component - 1] += c * dv.value;
+            }
+        }
+        E.assign(ne, 0.0);
+        nu.assign(ne, 0.0);
+        for (size_t e = 0; e < ne; ++e) {
+            const auto& p = par[e];
+            const double Ee = p[0], ne_ = p[3];
+            const double G = Ee / (2.0 * (1.0 + ne_));
+            auto same = [](double a, double b) { return std::fabs(a - b) <= 1e-12 * std::fabs(b); };
+            const bool iso = same(p[1], Ee) && same(p[2], Ee) && same(p[4], ne_) && same(p[5], ne_) &&
+                             (etype == ASEND_SHELL4 ? same(p[6], G)
+                                                    : (same(p[6], G) && same(p[7], G) && same(p[8], G)));
+            if (!iso)
+                die("element %zu: the material after design variables is not isotropic (E = %g %g %g, nu = %g %g %g, "
+                    "G = %g %g %g); the device kernels integrate isotropic materials",
+                    e, p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8]);
+            E[e] = Ee;
+            nu[e] = ne_;
+        }
+    }
+
+    void analysis_prep(const Command& c) {
+        const size_t nn = xyz.size() / 3, ne = conn.size();
+        if (nn == 0 || ne == 0) die("the model has no nodes or no elements");
+        etype = el_type[0];
+        for (size_t e = 0; e < ne; ++e)
+            if (el_type[e] != etype) die("models with more than one element type are not supported");
+        const int npe = (etype == ASEND_SHELL4) ? 4 : 8;
+        dpn = (etype == ASEND_SHELL4) ? 6 : 3;
+        std::vector<int32_t> cflat(npe * ne);
+        for (size_t e = 0; e < ne; ++e) {
+            if ((int)conn[e].size() != npe) die("element %zu has %zu nodes", e, conn[e].size());
+            for (int a = 0; a < npe; ++a) {
+                if (conn[e][a] < 0 || (size_t)conn[e][a] >= nn) die("element %zu names an undefined node", e);
+                cflat[npe * e + a] = conn[e][a];
+            }
+        }
+        // constraint structure for the ordering (add_constraint_conn)
+        std::vector<int64_t> cptr{0}, tptr{0};
+        std::vector<int32_t> setn;
+        for (const auto& k : constraints) {
+            for (const auto& t : k.terms) {
+                for (int32_t n : node_set(t.set, false)) setn.push_back(n);
+                tptr.push_back((int64_t)setn.size());
+            }
+            cptr.push_back((int64_t)tptr.size() - 1);
+        }
+        int64_t bd;
+        if (c.method == "direct" || c.method.empty()) bd = 2000000000;
+        else bd = (c.block_dim > 0) ? c.block_dim : asend_block_dim_auto((int64_t)nn);
+        std::vector<int32_t> ordered(nn);
+        dof_index.assign(6 * nn, -1);
+        int rc = asend_reorder_nodes((int64_t)nn, xyz.data(), (int64_t)ne, npe, dpn, cflat.data(),
+                                     (int64_t)constraints.size(), cptr.data(), tptr.data(), setn.data(), bd,
+                                     ordered.data(), dof_index.data(), &n_dof);
+        if (rc != ASEND_OK) die("node reordering failed (%d)", rc);
+        // penalty DOFs: one-term clamps (Constraint::build_mat)
+        pen.clear();
+        for (const auto& k : constraints) {
+            if (k.terms.size() != 1 || std::fabs(std::fabs(k.terms[0].coef) - 1.0) > 0.0 || k.rhs != 0.0)
+                die("only single-term displacement constraints with coefficient 1 and rhs 0 are supported");
+            const int dof = k.terms[0].dof;
+            if (dof < 1 || dof > dpn) die("constraint dof %d out of range", dof);
+            for (int32_t n : node_set(k.terms[0].set, false)) pen.push_back(dof_index[6 * n + dof - 1]);
+        }
+        element_materials();
+        if (!ctx) check(asend_ctx_create(0, &ctx), "asend_ctx_create");
+        check(asend_upload_mesh(ctx, etype, (int64_t)nn, xyz.data(), dof_index.data(), n_dof, (int64_t)ne,
+                                cflat.data(), E.data(), nu.data(), rho.data()),
+              "asend_upload_mesh");
+        if (etype == ASEND_SHELL4) check(asend_set_thickness(ctx, thk.data()), "asend_set_thickness");
+        check(asend_set_penalty(ctx, (int64_t)pen.size(), pen.data(), 0.0), "asend_set_penalty");
+        // applied loads (build_elastic_app_load)
+        f.assign(n_dof, 0.0);
+        for (const auto& l : loads) {
+            if (l.type == "nodalForce") {
+                for (int32_t n : node_set(l.node_set, false))
+                    for (int c = 0; c < dpn; ++c) f[dof_index[6 * n + c]] += l.load[c];
+            } else if (l.type == "surfaceTraction") {
+                if (etype == ASEND_SHELL4) die("surfaceTraction on shell elements is not supported");
+                const auto& set = element_set(l.el_set);
+                rc = asend_surface_traction(etype, (int64_t)nn, xyz.data(), (int64_t)ne, cflat.data(),
+                                            dof_index.data(), (int64_t)set.size(), set.data(), l.load, l.norm_dir,
+                                            l.norm_tol, n_dof, f.data());
+                if (rc != ASEND_OK) die("surface traction failed (%d)", rc);
+            } else {
+                die("loads of type %s are not supported", l.type.c_str());
+            }
+        }
+    }
+
+    // K x = b through the device; the reference's direct branch solves
+    // exactly, here FP64 Jacobi PCG to a 1e-12 relative residual; the
+    // iterative branch follows conj_grad_sparse's absolute stop rule
+    void device_solve(const std::vector<double>& b, std::vector<double>& x, bool report) {
+        asend_solve_opts o;
+        std::memset(&o, 0, sizeof o);
+        o.method = ASEND_SOLVE_PCG_FP64;
+        o.precond = ASEND_PC_JACOBI;
+        o.check_every = 1;
+        const bool iterative = (solve_cmd.method == "iterative");
+        if (iterative) {
+            o.stop_rule = ASEND_STOP_REF_ABS;
+            o.tol = solve_cmd.conv_tol;
+            o.max_it = solve_cmd.max_it;
+        } else {
+            o.stop_rule = ASEND_STOP_REL_RESIDUAL;
+            o.tol = 1e-12;
+            o.max_it = 100 * n_dof + 1000;
+            o.check_every = 16;
+        }
+        asend_solve_stats st;
+        std::memset(&st, 0, sizeof st);
+        x.assign(n_dof, 0.0);
+        check(asend_solve(ctx, b.data(), x.data(), &o, &st), "asend_solve");
+        if (report && iterative) {
+            if (!st.converged)
+                std::printf("Warning: Conjugate gradient solver did not converge to specified tolerance, %g within "
+                            "tne maximum number of iterations, %ld\n",
+                            solve_cmd.conv_tol, (long)solve_cmd.max_it);
+            std::printf("Total CG iterations: %ld\n", (long)st.iterations);
+            std::printf("Final residual norm: %s\n", fmt_e12(st.ref_residual).c_str());
+        } else if (report) {
+            std::printf("device solve: %ld FP64 PCG iterations, relative residual %s\n", (long)st.iterations,
+                        fmt_e12(st.rel_residual).c_str());
+        }
+    }
+
+    void solve(const Command& c) {
+        if (c.dynamic || c.nonlinear || c.thermal || !c.elastic)
+            die("only static, linear, elastic solves are supported by this solver");
+        solve_cmd = c;
+        analysis_prep(c);
+        check(asend_assemble(ctx, 0.0), "asend_assemble");
+        device_solve(f, u, true);
+        solved = true;
+    }
+
+    // objective terms (calculate_terms for displacement / powerNorm,
+    // S/objective_meth.rs:592-640, get_power_norm :329-337)
+    void objective_values() {
+        if (!solved) die("the objective needs a solution (run solve first)");
+        for (auto& t : obj) {
+            if (t.category != "displacement" || t.optr != "powerNorm")
+                die("objective terms of category %s with operator %s are not supported", t.category.c_str(),
+                    t.optr.c_str());
+            if (t.component < 1 || t.component > dpn) die("objective component out of range");
+            const auto& set = node_set(t.nd_set, false);
+            const double tg = t.tgt.empty() ? 0.0 : t.tgt[0];
+            if (t.tgt.size() > 1 && t.tgt.size() != set.size()) die("target list does not match the node set");
+            double ps = 0.0;
+            for (size_t k = 0; k < set.size(); ++k) {
+                const double q = u[dof_index[6 * set[k] + t.component - 1]];
+                ps += std::pow(q - (t.tgt.size() > 1 ? t.tgt[k] : tg), t.expnt);
+            }
+            t.value = t.coef * std::pow(ps, 1.0 / t.expnt);
+        }
+    }
+
+    // adjoint gradient (get_obj_gradient, S/model_analysis.rs:1544-1706):
+    // dL/du of the terms (d_power_normd_u, S/objective_meth.rs:339-366),
+    // K lambda = dL/du, then for each design variable the element partials
+    // of -lambda^T (dK/dp) u times the DV coefficients
+    void gradient() {
+        objective_values();
+        std::vector<double> dLdu(n_dof, 0.0);
+        for (const auto& t : obj) {
+            const auto& set = node_set(t.nd_set, false);
+            const double tg = t.tgt.empty() ? 0.0 : t.tgt[0];
+            double ps = 0.0;
+            std::vector<double> err(set.size());
+            for (size_t k = 0; k < set.size(); ++k) {
+                err[k] = u[dof_index[6 * set[k] + t.component - 1]] - (t.tgt.size() > 1 ? t.tgt[k] : tg);
+                ps += std::pow(err[k], t.expnt);
+            }
+            const double scale = t.coef * std::pow(ps, 1.0 / t.expnt - 1.0);
+            for (size_t k = 0; k < set.size(); ++k)
+                dLdu[dof_index[6 * set[k] + t.component - 1]] += scale * std::pow(err[k], t.expnt - 1.0);
+        }
+        dLdD.assign(dvars.size(), 0.0);
+        if (!dvars.empty()) {
+            if (etype != ASEND_BRICK8) die("design-variable gradients are implemented for brick8 models");
+            std::vector<double> lam;
+            device_solve(dLdu, lam, false);
+            std::vector<double> part(9 * conn.size());
+            check(asend_element_sens_material(ctx, u.data(), lam.data(), part.data()), "asend_element_sens_material");
+            for (size_t i = 0; i < dvars.size(); ++i) {
+                const auto& dv = dvars[i];
+                const int base = dv.category == "modulus" ? 0 : dv.category == "poissonRatio" ? 3 : 6;
+                const auto& set = element_set(dv.el_set);
+                double acc = 0.0;
+                for (size_t k = 0; k < set.size(); ++k) {
+                    const double c = dv.coefs.empty() ? 1.0 : (dv.coefs.size() == 1 ? dv.coefs[0] : dv.coefs[k]);
+                    acc += c * part[9 * (size_t)set[k] + base + dv.component - 1];
+                }
+                dLdD[i] = acc;
+            }
+        }
+        have_grad = true;
+    }
+
+    // ------------------------------------------------------------ writers
+    void write_node_results(const Command& c) {
+        if (!solved) die("node results need a solution (run solve first)");
+        if (!c.time_steps.empty()) die("time-step node results belong to dynamic jobs");
+        const auto& set = node_set(c.node_set, true);
+        std::vector<double> rf;
+        for (const auto& fld : c.fields) {
+            if (fld == "reactionForce" && rf.empty()) {
+                rf.assign(n_dof, 0.0);
+                check(asend_spmv(ctx, 64, u.data(), rf.data()), "asend_spmv");
+                const double sf = asend_penalty_scale(ctx);
+                for (int64_t d : pen) rf[d] -= sf * u[d];
+            } else if (fld != "displacement" && fld != "reactionForce") {
+                die("unrecognized output field '%s' for write_node_results", fld.c_str());
+            }
+        }
+        std::FILE* fp = std::fopen(c.file.c_str(), "w");
+        if (!fp) die("could not open file %s", c.file.c_str());
+        std::fprintf(fp, "node,");
+        for (const auto& fld : c.fields) std::fprintf(fp, fld == "displacement" ? "U1,U2,U3,R1,R2,R3," : "RF1,RF2,RF3,RM1,RM2,RM3,");
+        std::fprintf(fp, "time\n");
+        for (int32_t n : set) {
+            std::fprintf(fp, "%d,", n);
+            for (const auto& fld : c.fields) {
+                const std::vector<double>& v = (fld == "displacement") ? u : rf;
+                for (int k = 0; k < 6; ++k) {
+                    const double x = (k < dpn) ? v[dof_index[6 * n + k]] : 0.0;
+                    std::fprintf(fp, "%s,", fmt_e12(x).c_str());
+                }
+            }
+            std::fprintf(fp, "%s\n", fmt_e12(-1.0).c_str());
+        }
+        std::fclose(fp);
+    }
+
+    void write_objective(const Command& c) {
+        std::FILE* fp = std::fopen(c.file.c_str(), "w");
+        if (!fp) die("could not open file %s", c.file.c_str());
+        std::fprintf(fp, "objectiveTerm,value");
+        for (const auto& s : c.include) std::fprintf(fp, ",%s", s.c_str());
+        std::fprintf(fp, "\n");
+        for (size_t i = 0; i < obj.size(); ++i) {
+            const auto& t = obj[i];
+            std::fprintf(fp, "%zu,%s", i, fmt_e12(t.value).c_str());
+            for (const auto& s : c.include) {
+                if (s == "category") std::fprintf(fp, ",%s", t.category.c_str());
+                else if (s == "operator") std::fprintf(fp, ",%s", t.optr.c_str());
+                else if (s == "component") std::fprintf(fp, ",%d", t.component);
+                else if (s == "coefficient") std::fprintf(fp, ",%s", fmt_e12(t.coef).c_str());
+                else if (s == "exponent") std::fprintf(fp, ",%s", fmt_e12(t.expnt).c_str());
+                else if (s == "elementSet") std::fprintf(fp, ",%s", t.el_set.c_str());
+                else if (s == "nodeSet") std::fprintf(fp, ",%s", t.nd_set.c_str());
+                else die("unrecognized field '%s' in the list for write_objective()", s.c_str());
+            }
+            std::fprintf(fp, "\n");
+        }
+        std::fclose(fp);
+        if (c.write_grad) {
+            const size_t dot = c.file.find('.');
+            const std::string root = dot == std::string::npos ? c.file : c.file.substr(0, dot);
+            const std::string ext = dot == std::string::npos ? "" : c.file.substr(dot);
+            const std::string gname = root + "_grad" + ext;
+            std::FILE* gp = std::fopen(gname.c_str(), "w");
+            if (!gp) die("could not open file, %s", gname.c_str());
+            std::fprintf(gp, "designVariable,objGrad\n");
+            for (size_t i = 0; i < dvars.size(); ++i)
+                std::fprintf(gp, "%zu,%s\n", i, fmt_e12(have_grad ? dLdD[i] : 0.0).c_str());
+            std::fclose(gp);
+        }
+    }
+
+    // ------------------------------------------------------------ driver
+    void execute() {
+        for (const auto& c : cmds) {
+            if (c.cmd == "readModelInput") {
+                std::printf("reading Model input: %s\n", c.file.c_str());
+                read_model(c.file);
+                read_constraints(c.file);
+                read_loads(c.file);
+            } else if (c.cmd == "readConstraints") {
+                std::printf("reading constraints: %s\n", c.file.c_str());
+                read_constraints(c.file);
+            } else if (c.cmd == "readLoads") {
+                std::printf("reading loads: %s\n", c.file.c_str());
+                read_loads(c.file);
+            } else if (c.cmd == "readDesignVarInput") {
+                std::printf("reading design variable input: %s\n", c.file.c_str());
+                read_dvars(c.file);
+            } else if (c.cmd == "readDesignVarValues") {
+                read_dvar_values(c.file);
+            } else if (c.cmd == "readObjectiveInput") {
+                std::printf("reading Objective input: %s\n", c.file.c_str());
+                read_objective(c.file);
+            } else if (c.cmd == "solvePrep") {
+                // the ordering and upload happen with the solve
+            } else if (c.cmd == "solve") {
+                std::printf("running main analysis \n");
+                solve(c);
+            } else if (c.cmd == "calcObjective") {
+                std::printf("calculating Objective function %s\n", c.file.c_str());
+                objective_values();
+            } else if (c.cmd == "calcObjGradient") {
+                std::printf("calculating Objective gradient %s\n", c.file.c_str());
+                gradient();
+            } else if (c.cmd == "writeNodeResults") {
+                std::printf("writing Node results\n");
+                write_node_results(c);
+            } else if (c.cmd == "writeObjective") {
+                std::printf("writing Objective results\n");
+                write_objective(c);
+            } else {
+                die("job command %s is not supported by this solver", c.cmd.c_str());
+            }
+            std::fflush(stdout);
+        }
+    }
+};
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc < 2) die("job input file not specified in call to analysis solver.");
+    const bool dry = (argc > 2 && std::strcmp(argv[2], "--parse-only") == 0);
+    std::printf("Beginning Job: %s\n", argv[1]);
+    Job job;
+    job.read_job(argv[1]);
+    if (dry) {
+        // parse the model files and report what was read, without a device
+        for (const auto& c : job.cmds) {
+            if (c.cmd == "readModelInput") {
+                job.read_model(c.file);
+                job.read_constraints(c.file);
+                job.read_loads(c.file);
+            } else if (c.cmd == "readConstraints") job.read_constraints(c.file);
+            else if (c.cmd == "readLoads") job.read_loads(c.file);
+            else if (c.cmd == "readDesignVarInput") job.read_dvars(c.file);
+            else if (c.cmd == "readDesignVarValues") job.read_dvar_values(c.file);
+            else if (c.cmd == "readObjectiveInput") job.read_objective(c.file);
+        }
+        std::printf("commands %zu nodes %zu elements %zu node_sets %zu element_sets %zu sections %zu materials %zu "
+                    "constraints %zu loads %zu design_variables %zu objective_terms %zu\n",
+                    job.cmds.size(), job.xyz.size() / 3, job.conn.size(), job.nsets.size(), job.esets.size(),
+                    job.sections.size(), job.materials.size(), job.constraints.size(), job.loads.size(),
+                    job.dvars.size(), job.obj.size());
+        for (const auto& c : job.cmds) std::printf("command %s %s\n", c.cmd.c_str(), c.file.c_str());
+        return 0;
+    }
+    job.execute();
+    return 0;
+}

@@ -1678,6 +1678,26 @@ int asend_element_sens(asend_ctx* ctx, const double* u, const double* lam, doubl
     return ASEND_OK;
 }
 
+int asend_element_sens_material(asend_ctx* ctx, const double* u, const double* lam, double* dJdp) {
+    if (!ctx || !u || !lam || !dJdp) return ASEND_ERR_ARG;
+    if (!ctx->d_E) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
+    if (ctx->etype != ASEND_BRICK8)
+        return fail(ctx, ASEND_ERR_UNSUPPORTED, "material-parameter sensitivities are implemented for brick8 meshes");
+    const int64_t n = ctx->n_dof;
+    double* d_out = nullptr;
+    int rc = dalloc(ctx, &d_out, 9 * ctx->n_el);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(ctx->d_x64, u, n * 8, hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(hipMemcpyAsync(ctx->d_lam, lam, n * 8, hipMemcpyHostToDevice, ctx->st));
+    launch_hex8_sens_mat(ctx->st, (int)ctx->n_el, ctx->d_conn_r, ctx->d_xyz_r, ctx->d_E, ctx->d_nu, ctx->d_x64,
+                         ctx->d_lam, d_out);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(dJdp, d_out, 9 * ctx->n_el * 8, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    dfree(d_out);
+    return ASEND_OK;
+}
+
 int asend_static_compliance(asend_ctx* ctx, const double* f, double* u, double* dJdE, double* compliance,
                             const asend_solve_opts* opts, asend_solve_stats* stats) {
     if (!ctx || !f || !opts) return ASEND_ERR_ARG;

@@ -1412,6 +1412,98 @@ __global__ __launch_bounds__(128) void k_hex8_sens(int n_el, const int32_t* __re
     out[e] = -acc;  // lamL and mu above are already divided by E_e
 }
 
+// Material-parameter sensitivities of -lambda^T K_e u for every brick8
+// element, at its isotropic (E, nu, G = E/(2(1+nu))) point, with respect to
+// the nine parameters of the orthotropic compliance S the reference builds
+// (E1, E2, E3, nu12, nu13, nu23, G12, G13, G23; S/element_properties.rs:
+// 728-759): since dC/dp = -C (dS/dp) C, each is
+// sum_gp detJ sigma(lambda)^T (dS/dp) sigma(u) with sigma = C eps (Voigt
+// [11, 22, 33, 12, 13, 23], engineering shears).  out[9 e + p].
+__global__ __launch_bounds__(128) void k_hex8_sens_mat(int n_el, const int32_t* __restrict__ conn_r,
+                                                       const double* __restrict__ xyz_r,
+                                                       const double* __restrict__ E, const double* __restrict__ nu,
+                                                       const double* __restrict__ u, const double* __restrict__ lam,
+                                                       double* __restrict__ out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_el) return;
+    double X[8][3], U[8][3], L[8][3];
+    for (int a = 0; a < 8; ++a) {
+        const int64_t r = conn_r[8 * (int64_t)e + a];
+        for (int c = 0; c < 3; ++c) {
+            X[a][c] = xyz_r[3 * r + c];
+            U[a][c] = u[3 * r + c];
+            L[a][c] = lam[3 * r + c];
+        }
+    }
+    const double Ee = E[e], ne = nu[e];
+    const double lm = Ee * ne / ((1.0 + ne) * (1.0 - 2.0 * ne));
+    const double G = Ee / (2.0 * (1.0 + ne));
+    const double iE = 1.0 / Ee, iE2 = iE * iE, iG2 = 1.0 / (G * G);
+    const double g = 0.5773502691896257;
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int gp = 0; gp < 8; ++gp) {
+        const double s0 = (gp & 1) ? g : -g, s1 = (gp & 2) ? g : -g, s2 = (gp & 4) ? g : -g;
+        double dN[8][3];
+        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (int a = 0; a < 8; ++a) {
+            const double fx = 1.0 + kHexNdSign[a][0] * s0, fy = 1.0 + kHexNdSign[a][1] * s1,
+                         fz = 1.0 + kHexNdSign[a][2] * s2;
+            dN[a][0] = 0.125 * kHexNdSign[a][0] * fy * fz;
+            dN[a][1] = 0.125 * fx * kHexNdSign[a][1] * fz;
+            dN[a][2] = 0.125 * fx * fy * kHexNdSign[a][2];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) J[i][j] += X[a][i] * dN[a][j];
+        }
+        const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+        const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+        const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+        const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+        const double id = 1.0 / det;
+        const double Ji[3][3] = {{c00 * id, (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id,
+                                  (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id},
+                                 {c01 * id, (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id,
+                                  (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id},
+                                 {c02 * id, (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id,
+                                  (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id}};
+        double gu[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, gl[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (int a = 0; a < 8; ++a) {
+            double gx[3];
+            for (int k = 0; k < 3; ++k) gx[k] = dN[a][0] * Ji[0][k] + dN[a][1] * Ji[1][k] + dN[a][2] * Ji[2][k];
+            for (int i = 0; i < 3; ++i)
+                for (int k = 0; k < 3; ++k) {
+                    gu[i][k] += U[a][i] * gx[k];
+                    gl[i][k] += L[a][i] * gx[k];
+                }
+        }
+        // stresses of u and lambda
+        const double tu = gu[0][0] + gu[1][1] + gu[2][2], tl = gl[0][0] + gl[1][1] + gl[2][2];
+        const double su[6] = {lm * tu + 2.0 * G * gu[0][0], lm * tu + 2.0 * G * gu[1][1], lm * tu + 2.0 * G * gu[2][2],
+                              G * (gu[0][1] + gu[1][0]), G * (gu[0][2] + gu[2][0]), G * (gu[1][2] + gu[2][1])};
+        const double sl[6] = {lm * tl + 2.0 * G * gl[0][0], lm * tl + 2.0 * G * gl[1][1], lm * tl + 2.0 * G * gl[2][2],
+                              G * (gl[0][1] + gl[1][0]), G * (gl[0][2] + gl[2][0]), G * (gl[1][2] + gl[2][1])};
+        const double p01 = sl[0] * su[1] + sl[1] * su[0], p02 = sl[0] * su[2] + sl[2] * su[0],
+                     p12 = sl[1] * su[2] + sl[2] * su[1];
+        // S00 = 1/E1, S01 = -nu12/E1, S02 = -nu13/E1, S11 = 1/E2, S12 = -nu23/E2,
+        // S22 = 1/E3, S33 = 1/G12, S44 = 1/G13, S55 = 1/G23 (symmetric)
+        acc[0] += det * (-iE2 * sl[0] * su[0] + ne * iE2 * (p01 + p02));
+        acc[1] += det * (-iE2 * sl[1] * su[1] + ne * iE2 * p12);
+        acc[2] += det * (-iE2 * sl[2] * su[2]);
+        acc[3] += det * (-iE * p01);
+        acc[4] += det * (-iE * p02);
+        acc[5] += det * (-iE * p12);
+        acc[6] += det * (-iG2 * sl[3] * su[3]);
+        acc[7] += det * (-iG2 * sl[4] * su[4]);
+        acc[8] += det * (-iG2 * sl[5] * su[5]);
+    }
+    for (int p = 0; p < 9; ++p) out[9 * (int64_t)e + p] = acc[p];
+}
+
+void launch_hex8_sens_mat(hipStream_t st, int n_el, const int32_t* conn_r, const double* xyz_r, const double* E,
+                          const double* nu, const double* u, const double* lam, double* out) {
+    hipLaunchKernelGGL(k_hex8_sens_mat, dim3((n_el + 127) / 128), dim3(128), 0, st, n_el, conn_r, xyz_r, E, nu, u,
+                       lam, out);
+}
+
 void launch_hex8_sens(hipStream_t st, int n_el, const int32_t* conn_r, const double* xyz_r, const double* E,
                       const double* nu, const double* u, const double* lam, double* out) {
     hipLaunchKernelGGL(k_hex8_sens, dim3((n_el + 127) / 128), dim3(128), 0, st, n_el, conn_r, xyz_r, E, nu, u, lam,

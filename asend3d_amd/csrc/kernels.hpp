@@ -176,6 +176,10 @@ void launch_axpy_from_f32(hipStream_t st, int64_t n, double* u, const float* d, 
 void launch_dot64(hipStream_t st, int64_t n, const double* a, const double* b, double* partials, Scalars* sc);
 void launch_axpy64(hipStream_t st, int64_t n, double alpha, const double* x, double* y);
 
+// per-element partials of -lambda^T K_e u w.r.t. (E1, E2, E3, nu12, nu13, nu23,
+// G12, G13, G23) of the compliance, out[9 e + p]
+void launch_hex8_sens_mat(hipStream_t st, int n_el, const int32_t* conn_r, const double* xyz_r, const double* E,
+                          const double* nu, const double* u, const double* lam, double* out);
 void launch_hex8_sens(hipStream_t st, int n_el, const int32_t* conn_r, const double* xyz_r, const double* E,
                       const double* nu, const double* u, const double* lam, double* out);
 

@@ -248,6 +248,17 @@ int asend_solve_noalloc(asend_ctx* ctx, const double* rhs, double* x, const asen
  * condensed internal DOFs give); not shell4. */
 int asend_element_sens(asend_ctx* ctx, const double* u, const double* lam, double* dJdE);
 
+/* Per-element material-parameter sensitivities of brick8 meshes:
+ * dJdp[9 e + p] = -lambda_e^T (dK_e/dp) u_e for p = E1, E2, E3, nu12, nu13,
+ * nu23, G12, G13, G23 of the orthotropic compliance the reference inverts
+ * (get_solid_stiff_dfd0, S/element_properties.rs:728-759), evaluated at the
+ * element's isotropic material.  The per-element part of the reference's
+ * gradient for design variables of category modulus, poissonRatio and
+ * shearModulus (d_relasticd_d, S/model_analysis.rs:1379-1457, in
+ * get_obj_gradient :1544-1706); the DV sum over elements and coefficients
+ * is the caller's. */
+int asend_element_sens_material(asend_ctx* ctx, const double* u, const double* lam, double* dJdp);
+
 /* The static compliance pipeline on resident data: assemble, solve K u = f,
  * compliance J = f^T u, adjoint K lambda = dJ/du (= f, so lambda = u without a
  * second solve when the objective is the compliance), dJ/dE.  Replaces

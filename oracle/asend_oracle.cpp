@@ -749,6 +749,31 @@ int orc_solid_element_dE(int type, const double* X, const double* ori, double E,
     return 0;
 }
 
+/* dK_e/dp for one of the nine compliance parameters p = 0..8 (E1, E2, E3,
+ * nu12, nu13, nu23, G12, G13, G23) by dual numbers through the pivot-clamped
+ * QR inverse, as the reference's DiffDoub0 design-variable derivatives of
+ * get_solid_stiff_dfd0 (S/element_properties.rs:700-759). */
+int orc_solid_element_dmat(int type, const double* X, const double* modv, const double* prv, const double* smv,
+                           int param, double* K, double* dK) {
+    if (type != 8 && type != 81) return -1;
+    if (param < 0 || param > 8) return -1;
+    int tot = (type == 81) ? 33 : 24;
+    Dual mod[3], pr[3], sm[3], C[36];
+    for (int i = 0; i < 3; ++i) {
+        mod[i] = Dual(modv[i], param == i ? 1.0 : 0.0);
+        pr[i] = Dual(prv[i], param == 3 + i ? 1.0 : 0.0);
+        sm[i] = Dual(smv[i], param == 6 + i ? 1.0 : 0.0);
+    }
+    solid_cmat<Dual>(mod, pr, sm, C);
+    std::vector<Dual> Kf(tot * tot), Rf(tot);
+    hex_full<Dual>(type, X, nullptr, C, nullptr, Kf.data(), Rf.data());
+    std::vector<Dual> Kc(576);
+    if (type == 81) condense(Kf.data(), tot, 24, Kc.data());
+    else for (int i = 0; i < 576; ++i) Kc[i] = Kf[i];
+    for (int i = 0; i < 576; ++i) { K[i] = Kc[i].v; dK[i] = Kc[i].d; }
+    return 0;
+}
+
 int orc_solid_mass(int type, const double* X, double rho, double* M) {
     if (type != 8 && type != 81) return -1;
     std::memset(M, 0, sizeof(double) * 576);

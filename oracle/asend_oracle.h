@@ -67,6 +67,8 @@ int orc_solid_element(int type, const double* X, const double* ori, const double
  * (S/model_analysis.rs:1379-1457, S/diff_doub_meth.rs:82-185). */
 int orc_solid_element_dE(int type, const double* X, const double* ori, double E, double nu,
                          double* K, double* dK);
+int orc_solid_element_dmat(int type, const double* X, const double* mod, const double* pr, const double* sm,
+                           int param, double* K, double* dK);
 
 /* Consistent solid mass (rho * int N^T N), Element::get_rum_dfd0 solid branch
  * (S/element_equations.rs:583-609).  M out 24x24. */

@@ -125,6 +125,28 @@ def solid_element_dE(etype, X, E, nu):
     return K.reshape(24, 24), dK.reshape(24, 24)
 
 
+def solid_cmat_ortho(mod, pr, sm):
+    """get_solid_stiff_dfd0 with the nine compliance parameters
+    (S/element_properties.rs:728-759): mod = (E1, E2, E3), pr = (nu12, nu13,
+    nu23), sm = (G12, G13, G23)."""
+    mod, pr, sm = (np.ascontiguousarray(v, dtype=np.float64) for v in (mod, pr, sm))
+    C = np.zeros(36)
+    _chk(lib().orc_solid_cmat(_ptr(mod), _ptr(pr), _ptr(sm), _ptr(C)), "solid_cmat")
+    return C.reshape(6, 6)
+
+
+def solid_element_dmat(etype, X, mod, pr, sm, param):
+    """(K, dK/dp) of one element, p = 0..8 indexing (E1, E2, E3, nu12, nu13,
+    nu23, G12, G13, G23), by dual numbers through the QR inverse."""
+    Xs = np.ascontiguousarray(np.asarray(X, dtype=np.float64).T.reshape(-1))
+    mod, pr, sm = (np.ascontiguousarray(v, dtype=np.float64) for v in (mod, pr, sm))
+    K = np.zeros(576)
+    dK = np.zeros(576)
+    _chk(lib().orc_solid_element_dmat(ctypes.c_int(etype), _ptr(Xs), _ptr(mod), _ptr(pr), _ptr(sm),
+                                      ctypes.c_int(param), _ptr(K), _ptr(dK)), "solid_element_dmat")
+    return K.reshape(24, 24), dK.reshape(24, 24)
+
+
 def solid_mass(etype, X, rho):
     Xs = np.ascontiguousarray(np.asarray(X, dtype=np.float64).T.reshape(-1))
     M = np.zeros(576)

@@ -1,0 +1,114 @@
+"""The job-file driver asend_run_job on the C1 job fixture (20x4x4 brick8
+cantilever, clamped, tip traction; design variables on the compliance
+parameters; two displacement powerNorm objective terms), against the oracle:
+node displacements against its exact banded LDL^T solve, reaction forces
+against K u, objective values from its displacements, and the adjoint
+gradient of every design variable from its dual-number element derivatives
+dK_e/dp (E1..3, nu12, nu13, nu23, G12, G13, G23) with lambda = K^-1 dL/du."""
+import csv
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from asend3d_amd import problems
+from conftest import gpu_case
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "asend3d_amd", "asend_run_job")
+FIXTURE = os.path.join(ROOT, "tests", "golden", "c1_job")
+
+# (category offset, component, element set, coefficient) of the fixture's DVs
+DVARS = [(0, 1, "all", 1.0), (0, 2, "all", 1.0), (0, 3, "all", 1.0), (6, 1, "all", 1.0), (6, 3, "all", 1.0),
+         (3, 1, "all", 1.0), (3, 3, "all", 1.0), (0, 1, "5", 1.0), (0, 3, "tipEls", 2.0)]
+
+
+@pytest.fixture(scope="module")
+def oracle_c1():
+    p = problems.config1()
+    m = p.mesh
+    K = O.assemble_solid(8, m.xyz, m.conn, p.E, p.nu, p.dof_index, p.n_dof)
+    sf = 1e5 * O.max_abs(K)
+    f = O.surface_traction(8, m.xyz, m.conn, p.dof_index, p.n_dof, p.tip_elements, (0.0, 0.0, -1.0e6),
+                           (1.0, 0.0, 0.0), 5.0)
+    ldl = O.LDL(K, p.pen_dofs, sf, p.n_dof)
+    u = ldl.solve(f)
+    di = np.asarray(p.dof_index).reshape(-1, 6)[:, :3]
+    tip = m.nodes_at_x(m.nx)
+    J = [u[di[tip, 2]].sum(), 0.5 * np.sqrt((u[di[tip, 0]] ** 2).sum())]
+    dLdu = np.zeros(p.n_dof)
+    dLdu[di[tip, 2]] += 1.0
+    dLdu[di[tip, 0]] += 0.5 * u[di[tip, 0]] / np.sqrt((u[di[tip, 0]] ** 2).sum())
+    lam = ldl.solve(dLdu)
+    E, nu = 200e9, 0.3
+    G = E / (2.0 * (1.0 + nu))
+    part = np.zeros((m.n_el, 9))
+    for e in range(m.n_el):
+        nodes = m.conn[e]
+        ue, le = u[di[nodes]].ravel(), lam[di[nodes]].ravel()
+        for q in range(9):
+            _, dK = O.solid_element_dmat(8, m.xyz[nodes], [E] * 3, [nu] * 3, [G] * 3, q)
+            part[e, q] = -le @ dK @ ue
+    sets = {"all": np.arange(m.n_el), "5": np.array([5]), "tipEls": m.elements_at_x(m.nx - 1)}
+    grad = np.array([c * part[sets[s], base + comp - 1].sum() for base, comp, s, c in DVARS])
+    rf = O.spmv_penalty(K, np.zeros(0, np.int64), 0.0, u)
+    return {"p": p, "u": u, "di": di, "J": np.array(J), "grad": grad, "rf": rf}
+
+
+def run_job(tmp_path, job):
+    work = tmp_path / "job"
+    shutil.copytree(FIXTURE, work)
+    for d in ("results", "results_iterative"):
+        (work / d).mkdir()
+    print(f"[gpu-case] asend_run_job {job}", flush=True)
+    r = subprocess.run([CLI, job], cwd=work, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0, r.stderr
+    return work, r.stdout
+
+
+def read_csv(path):
+    with open(path) as fh:
+        rows = list(csv.reader(fh))
+    return rows[0], rows[1:]
+
+
+@pytest.mark.parametrize("job,tag,bar_u", [("c1_job.yaml", "", 1e-9), ("c1_job_iterative.yaml", "_iterative", 1e-7)])
+def test_cli_c1_matches_oracle(tmp_path, oracle_c1, job, tag, bar_u):
+    gpu_case(f"asend_run_job {job}")
+    work, out = run_job(tmp_path, job)
+    if tag:
+        assert "Total CG iterations: " in out and "Final residual norm: " in out
+        assert "did not converge" not in out
+    o = oracle_c1
+    head, rows = read_csv(work / f"results{tag}" / "nodeResults.csv")
+    assert head == ["node", "U1", "U2", "U3", "R1", "R2", "R3", "RF1", "RF2", "RF3", "RM1", "RM2", "RM3", "time"]
+    assert len(rows) == 525
+    lab = np.array([int(r[0]) for r in rows])
+    assert np.array_equal(lab, np.arange(525))
+    vals = np.array([[float(x) for x in r[1:]] for r in rows])
+    # {:.12e}: 13 significant digits, exponent without padding
+    assert rows[0][-1] == "-1.000000000000e0"
+    u_ref = o["u"][o["di"]]
+    assert np.abs(vals[:, 0:3] - u_ref).max() <= bar_u * np.abs(u_ref).max()
+    assert np.all(vals[:, 3:6] == 0.0)
+    rf_ref = o["rf"][o["di"]]
+    assert np.abs(vals[:, 6:9] - rf_ref).max() <= 1e4 * bar_u * np.abs(rf_ref).max()
+    # equilibrium: the clamp reactions carry the 40 kN tip load
+    clamp = o["p"].clamp_nodes
+    assert abs(vals[clamp, 8].sum() - 4.0e4) <= 1e-6 * 4.0e4
+    head, rows = read_csv(work / f"results{tag}" / "objectiveResults.csv")
+    assert head == ["objectiveTerm", "value", "category", "nodeSet"]
+    J = np.array([float(r[1]) for r in rows])
+    assert [r[2:] for r in rows] == [["displacement", "xMax"], ["displacement", "xMax"]]
+    assert np.allclose(J, o["J"], rtol=bar_u, atol=0.0)
+    head, rows = read_csv(work / f"results{tag}" / "objectiveResults_grad.csv")
+    assert head == ["designVariable", "objGrad"]
+    g = np.array([float(r[1]) for r in rows])
+    assert len(g) == len(DVARS)
+    assert np.abs(g - o["grad"]).max() <= 1e2 * bar_u * np.abs(o["grad"]).max()

@@ -219,3 +219,36 @@ def test_si_units_and_pivot_clamp(ctx):
         ctx.update_moduli(bad)
     ctx.assemble()
     assert np.array_equal(ctx.get_matrix(with_penalty=True)[2], vals)
+
+
+def test_material_sensitivities_match_oracle(ctx):
+    """asend_element_sens_material on a distorted mesh with per-element E and
+    nu: the partials of -lambda^T K_e u with respect to the nine compliance
+    parameters (E1..3, nu12, nu13, nu23, G12, G13, G23) against the oracle's
+    dual-number dK_e/dp through its QR inverse, and their isotropic sum
+    E (dE1 + dE2 + dE3) + G (dG12 + dG13 + dG23) against E dJ/dE of
+    asend_element_sens."""
+    gpu_case("material-parameter sensitivities, distorted 6x3x2")
+    p = problems.distorted(problems.heterogeneous(6, 3, 2, seed=8), seed=9)
+    problems.upload(ctx, p)
+    rng = np.random.default_rng(10)
+    u = rng.standard_normal(p.n_dof)
+    lam = rng.standard_normal(p.n_dof)
+    dev = ctx.element_sens_material(u, lam)
+    di = np.asarray(p.dof_index).reshape(-1, 6)[:, :3]
+    ref = np.zeros_like(dev)
+    for e in range(p.mesh.n_el):
+        nodes = p.mesh.conn[e]
+        E, nu = p.E[e], p.nu[e]
+        G = E / (2.0 * (1.0 + nu))
+        ue = u[di[nodes]].ravel()
+        le = lam[di[nodes]].ravel()
+        for q in range(9):
+            _, dK = O.solid_element_dmat(8, p.mesh.xyz[nodes], [E] * 3, [nu] * 3, [G] * 3, q)
+            ref[e, q] = -le @ dK @ ue
+    for q in range(9):
+        assert np.abs(dev[:, q] - ref[:, q]).max() <= 1e-10 * np.abs(ref[:, q]).max(), q
+    G = p.E / (2.0 * (1.0 + p.nu))
+    iso = p.E * dev[:, 0:3].sum(axis=1) + G * dev[:, 6:9].sum(axis=1)
+    dJdE = ctx.element_sens(u, lam)
+    assert np.abs(iso - p.E * dJdE).max() <= 1e-11 * np.abs(iso).max()

@@ -396,3 +396,32 @@ def test_dv_loop_equals_elementwise_adjoint(etype):
     s2 = O.dv_loop(etype, p.mesh.xyz, p.mesh.conn, p.E, p.nu, p.dof_index, p.n_dof, u, lam, p.tip_elements,
                    p.mesh.n_el)
     assert np.abs(s1 - s2).max() <= 1e-13 * np.abs(s1).max()
+
+
+def test_material_partials_dual_numbers_match_finite_differences():
+    # dK_e/dp for the nine compliance parameters by dual numbers through the
+    # QR inverse, against central differences of the oracle's own K_e(C(p))
+    # on a distorted element (orthotropic base point, so that every parameter
+    # moves C), and the isotropic sum rule d/dE [E1 = E2 = E3 = E,
+    # G = E/(2(1+nu))] = sum_c dK/dE_c + sum_c dK/dG_c / (2(1+nu))
+    X = distorted()
+    mod, pr, sm = np.array([3.1e3, 2.4e3, 1.9e3]), np.array([0.27, 0.21, 0.33]), np.array([1.2e3, 0.9e3, 1.05e3])
+    base = np.concatenate([mod, pr, sm])
+    for p in range(9):
+        _, dK = O.solid_element_dmat(8, X, mod, pr, sm, p)
+        h = 1e-5 * abs(base[p])
+        vp, vm = base.copy(), base.copy()
+        vp[p] += h
+        vm[p] -= h
+        Kp, _ = O.solid_element(8, X, O.solid_cmat_ortho(vp[:3], vp[3:6], vp[6:]))
+        Km, _ = O.solid_element(8, X, O.solid_cmat_ortho(vm[:3], vm[3:6], vm[6:]))
+        fd = (Kp - Km) / (2 * h)
+        assert np.abs(dK - fd).max() <= 1e-6 * np.abs(dK).max(), p
+        assert np.abs(dK).max() > 0
+    E, nu = 2.6e3, 0.29
+    G = E / (2 * (1 + nu))
+    iso = np.array([E, E, E]), np.array([nu, nu, nu]), np.array([G, G, G])
+    _, dKE = O.solid_element_dE(8, X, E, nu)
+    total = sum(O.solid_element_dmat(8, X, *iso, p)[1] for p in range(3))
+    total = total + sum(O.solid_element_dmat(8, X, *iso, p)[1] for p in range(6, 9)) / (2 * (1 + nu))
+    assert np.abs(total - dKE).max() <= 1e-12 * np.abs(dKE).max()
