@@ -135,7 +135,7 @@ _lib.asend_ctx_destroy.argtypes = [_vp]
 
 EXPORTED = ["asend_ctx_create", "asend_ctx_destroy", "asend_last_error", "asend_version", "asend_sync",
             "asend_reorder_nodes", "asend_block_dim_auto", "asend_surface_traction", "asend_lognormal_moduli",
-            "asend_compliance_clamp_check",
+            "asend_compliance_clamp_check", "asend_surface_pressure", "asend_body_load",
             "asend_upload_mesh", "asend_set_thickness", "asend_set_grid", "asend_update_moduli", "asend_set_penalty",
             "asend_penalty_scale", "asend_uniform_elements", "asend_assemble", "asend_get_matrix", "asend_spmv", "asend_solve", "asend_solve_noalloc",
             "asend_element_sens", "asend_element_sens_material", "asend_static_compliance", "asend_bench_spmv", "asend_mg_info", "asend_mg_apply",
@@ -222,6 +222,42 @@ def surface_traction(etype, xyz, conn, dof_index, n_dof, el_set, load, norm_dir,
                                      _i64(n_dof), _p(f))
     if rc != 0:
         raise AsendError(f"asend_surface_traction failed ({rc})")
+    return f
+
+
+def surface_pressure(etype, xyz, conn, dof_index, n_dof, el_set, pressure, norm_dir, norm_tol_deg, f=None):
+    xyz = _f64(xyz)
+    conn = _i32(conn)
+    di = _i64a(dof_index)
+    es = _i32(el_set)
+    nd = _f64(norm_dir)
+    f = np.zeros(n_dof) if f is None else f
+    rc = _lib.asend_surface_pressure(ctypes.c_int(etype), _i64(xyz.shape[0]), _p(xyz), _i64(conn.shape[0]), _p(conn),
+                                     _p(di), _i64(es.shape[0]), _p(es), _dbl(pressure), _p(nd), _dbl(norm_tol_deg),
+                                     _i64(n_dof), _p(f))
+    if rc != 0:
+        raise AsendError(f"asend_surface_pressure failed ({rc})")
+    return f
+
+
+BODY_FORCE, GRAVITATIONAL, CENTRIFUGAL = 0, 1, 2
+
+
+def body_load(etype, kind, xyz, conn, dof_index, n_dof, el_set, rho=None, load=(0.0, 0.0, 0.0),
+              center=(0.0, 0.0, 0.0), axis=(0.0, 0.0, 1.0), ang_vel=0.0, f=None):
+    """asend_body_load: bodyForce (BODY_FORCE), gravitational, centrifugal."""
+    xyz = _f64(xyz)
+    conn = _i32(conn)
+    di = _i64a(dof_index)
+    es = _i32(el_set)
+    rho = None if rho is None else _f64(rho)
+    ld, ce, ax = _f64(load), _f64(center), _f64(axis)
+    f = np.zeros(n_dof) if f is None else f
+    rc = _lib.asend_body_load(ctypes.c_int(etype), ctypes.c_int(kind), _i64(xyz.shape[0]), _p(xyz),
+                              _i64(conn.shape[0]), _p(conn), _p(di), _i64(es.shape[0]), _p(es), _p(rho), _p(ld),
+                              _p(ce), _p(ax), _dbl(ang_vel), _i64(n_dof), _p(f))
+    if rc != 0:
+        raise AsendError(f"asend_body_load failed ({rc})")
     return f
 
 

@@ -13,7 +13,8 @@
 //     isotropic materials (E, nu, G = E/(2(1+nu)) after design variables);
 //   - displacement constraints with one term of coefficient +-1 and rhs 0
 //     (clamps, applied by the penalty method);
-//   - loads nodalForce and surfaceTraction (solid elements);
+//   - loads nodalForce, and for solid elements surfaceTraction,
+//     surfacePressure, bodyForce, gravitational and centrifugal;
 //   - solve: static, elastic, linear; solverMethod direct (default) or
 //     iterative (reference stop rule on the device's Jacobi PCG,
 //     convergenceTol, maxIterations; maxIterations 0, which makes the
@@ -158,7 +159,9 @@ struct Load {
     std::string type, node_set, el_set;
     double load[6] = {0, 0, 0, 0, 0, 0};
     double norm_dir[3] = {0, 0, 0};
-    double norm_tol = 1.0e-3;
+    double norm_tol = 0.0;
+    double center[3] = {0, 0, 0}, axis[3] = {0, 0, 0};
+    double ang_vel = 0.0;
 };
 struct DesignVar {
     std::string category, el_set, nd_set;
@@ -455,6 +458,12 @@ struct Job {
                 for (int i = 0; i < 3; ++i) loads.back().norm_dir[i] = to_d(d[i]);
             } else if (h[1] == "normTolerance" && d.size() == 1) {
                 loads.back().norm_tol = to_d(d[0]);
+            } else if (h[1] == "center" && d.size() == 3) {
+                for (int i = 0; i < 3; ++i) loads.back().center[i] = to_d(d[i]);
+            } else if (h[1] == "axis" && d.size() == 3) {
+                for (int i = 0; i < 3; ++i) loads.back().axis[i] = to_d(d[i]);
+            } else if (h[1] == "angularVelocity" && !d.empty()) {
+                loads.back().ang_vel = to_d(d[0]);
             }
         }
     }
@@ -657,13 +666,26 @@ struct Job {
             if (l.type == "nodalForce") {
                 for (int32_t n : node_set(l.node_set, false))
                     for (int c = 0; c < dpn; ++c) f[dof_index[6 * n + c]] += l.load[c];
-            } else if (l.type == "surfaceTraction") {
-                if (etype == ASEND_SHELL4) die("surfaceTraction on shell elements is not supported");
+            } else if (l.type == "surfaceTraction" || l.type == "surfacePressure") {
+                if (etype == ASEND_SHELL4) die("%s on shell elements is not supported", l.type.c_str());
                 const auto& set = element_set(l.el_set);
-                rc = asend_surface_traction(etype, (int64_t)nn, xyz.data(), (int64_t)ne, cflat.data(),
-                                            dof_index.data(), (int64_t)set.size(), set.data(), l.load, l.norm_dir,
-                                            l.norm_tol, n_dof, f.data());
-                if (rc != ASEND_OK) die("surface traction failed (%d)", rc);
+                if (l.type == "surfaceTraction")
+                    rc = asend_surface_traction(etype, (int64_t)nn, xyz.data(), (int64_t)ne, cflat.data(),
+                                                dof_index.data(), (int64_t)set.size(), set.data(), l.load,
+                                                l.norm_dir, l.norm_tol, n_dof, f.data());
+                else
+                    rc = asend_surface_pressure(etype, (int64_t)nn, xyz.data(), (int64_t)ne, cflat.data(),
+                                                dof_index.data(), (int64_t)set.size(), set.data(), l.load[0],
+                                                l.norm_dir, l.norm_tol, n_dof, f.data());
+                if (rc != ASEND_OK) die("%s load failed (%d)", l.type.c_str(), rc);
+            } else if (l.type == "bodyForce" || l.type == "gravitational" || l.type == "centrifugal") {
+                if (etype == ASEND_SHELL4) die("%s on shell elements is not supported", l.type.c_str());
+                const auto& set = element_set(l.el_set);
+                const int kind = l.type == "bodyForce" ? 0 : l.type == "gravitational" ? 1 : 2;
+                rc = asend_body_load(etype, kind, (int64_t)nn, xyz.data(), (int64_t)ne, cflat.data(), dof_index.data(),
+                                     (int64_t)set.size(), set.data(), rho.data(), l.load, l.center, l.axis, l.ang_vel,
+                                     n_dof, f.data());
+                if (rc != ASEND_OK) die("%s load failed (%d)", l.type.c_str(), rc);
             } else {
                 die("loads of type %s are not supported", l.type.c_str());
             }

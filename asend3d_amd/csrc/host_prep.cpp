@@ -189,10 +189,18 @@ static double hex_detj(const double* X /*[8][3]*/, const double* s, double* N) {
            J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
 }
 
-int surface_traction(int elem_type, int64_t n_nodes, const double* xyz, int64_t n_el, const int32_t* conn,
-                     const int64_t* dof_index, int64_t n_set, const int32_t* el_set, const double* load,
-                     const double* norm_dir, double norm_tol_deg, int64_t n_dof, double* f) {
+// surfaceTraction (traction = load) and surfacePressure (traction = -p n of
+// each face, pressure != nullptr) share the face selection and distribution
+static int surface_load(int elem_type, int64_t n_nodes, const double* xyz, int64_t n_el, const int32_t* conn,
+                        const int64_t* dof_index, int64_t n_set, const int32_t* el_set, const double* load,
+                        const double* pressure, const double* norm_dir_in, double norm_tol_deg, int64_t n_dof,
+                        double* f) {
     if (elem_type != ASEND_BRICK8 && elem_type != ASEND_BRICKIM) return ASEND_ERR_UNSUPPORTED;
+    // the load's normal direction is stored normalised (Load::set_norm_dir, S/load_meth.rs:22-32)
+    const double nd2 = norm_dir_in[0] * norm_dir_in[0] + norm_dir_in[1] * norm_dir_in[1] + norm_dir_in[2] * norm_dir_in[2];
+    if (!(nd2 > 0.0)) return ASEND_ERR_ARG;
+    const double ndm = 1.0 / std::sqrt(nd2);
+    const double norm_dir[3] = {ndm * norm_dir_in[0], ndm * norm_dir_in[1], ndm * norm_dir_in[2]};
     // hex face local nodes (S/element_meth.rs:537-590)
     static const int fl[6][4] = {{3, 2, 1, 0}, {4, 5, 6, 7}, {0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7}};
     // A face lies on the surface when no other face has the same node set
@@ -221,7 +229,6 @@ int surface_traction(int elem_type, int64_t n_nodes, const double* xyz, int64_t 
             on_surf[keys[i + 1].second] = 0;
         }
     const double cos_tol = std::cos(0.017453292519943295 * norm_tol_deg);
-    const double inp = std::sqrt(load[0] * load[0] + load[1] * load[1] + load[2] * load[2]);
     for (int64_t si = 0; si < n_set; ++si) {
         int64_t e = el_set[si];
         if (e < 0 || e >= n_el) return ASEND_ERR_ARG;
@@ -250,8 +257,11 @@ int surface_traction(int elem_type, int64_t n_nodes, const double* xyz, int64_t 
             double dp = (norm_dir[0] * nrm[0] + norm_dir[1] * nrm[1] + norm_dir[2] * nrm[2]) / a2;
             if (!(dp > cos_tol)) continue;
             any = true;
+            double trac[3];
+            for (int d = 0; d < 3; ++d) trac[d] = pressure ? -(*pressure) * (nrm[d] / a2) : load[d];
+            const double inp = std::sqrt(trac[0] * trac[0] + trac[1] * trac[1] + trac[2] * trac[2]);
             for (int i = 0; i < 4; ++i)
-                for (int d = 0; d < 3; ++d) acc[l[i]][d] = load[d];
+                for (int d = 0; d < 3; ++d) acc[l[i]][d] = trac[d];
             // traction spread like a unit-density mass load over the element,
             // kept on the face nodes, then rescaled to area*|t|
             // (S/element_equations.rs:1970-2043)
@@ -284,6 +294,101 @@ int surface_traction(int elem_type, int64_t n_nodes, const double* xyz, int64_t 
                 int64_t g = dof_index[6 * (int64_t)c[a] + d];
                 if (g < 0 || g >= n_dof) return ASEND_ERR_ARG;
                 f[g] += ld[a][d];
+            }
+    }
+    (void)n_nodes;
+    return ASEND_OK;
+}
+
+int surface_traction(int elem_type, int64_t n_nodes, const double* xyz, int64_t n_el, const int32_t* conn,
+                     const int64_t* dof_index, int64_t n_set, const int32_t* el_set, const double* load,
+                     const double* norm_dir, double norm_tol_deg, int64_t n_dof, double* f) {
+    return surface_load(elem_type, n_nodes, xyz, n_el, conn, dof_index, n_set, el_set, load, nullptr, norm_dir,
+                        norm_tol_deg, n_dof, f);
+}
+
+int surface_pressure(int elem_type, int64_t n_nodes, const double* xyz, int64_t n_el, const int32_t* conn,
+                     const int64_t* dof_index, int64_t n_set, const int32_t* el_set, double pressure,
+                     const double* norm_dir, double norm_tol_deg, int64_t n_dof, double* f) {
+    return surface_load(elem_type, n_nodes, xyz, n_el, conn, dof_index, n_set, el_set, nullptr, &pressure, norm_dir,
+                        norm_tol_deg, n_dof, f);
+}
+
+// bodyForce / gravitational / centrifugal element loads of hex elements
+// (Element::get_app_load_dfd0, S/element_equations.rs:1876-1969): a nodal
+// acceleration field pushed through the consistent mass integral
+// f_a = sum_gp detJ N_a rho sum_b N_b acc_b (get_rum_dfd0) over the element's
+// 8 Gauss points.  bodyForce: acc = b at every node, rho = 1, then the
+// element's vector is rescaled to |b| V / |sum_a f_a|; gravitational:
+// acc = g, rho = the element density; centrifugal: acc = omega^2 times the
+// component of (centroid - center) normal to the unit axis, rho = the
+// element density.
+int body_load(int elem_type, int kind, int64_t n_nodes, const double* xyz, int64_t n_el, const int32_t* conn,
+              const int64_t* dof_index, int64_t n_set, const int32_t* el_set, const double* rho, const double* load,
+              const double* center, const double* axis_in, double ang_vel, int64_t n_dof, double* f) {
+    if (elem_type != ASEND_BRICK8 && elem_type != ASEND_BRICKIM) return ASEND_ERR_UNSUPPORTED;
+    if (kind < 0 || kind > 2) return ASEND_ERR_ARG;
+    if (kind > 0 && !rho) return ASEND_ERR_ARG;
+    double axis[3] = {0.0, 0.0, 0.0};
+    if (kind == 2) {
+        // Load::set_axis normalises (S/load_meth.rs:41-51)
+        const double a2 = axis_in[0] * axis_in[0] + axis_in[1] * axis_in[1] + axis_in[2] * axis_in[2];
+        if (!(a2 > 0.0)) return ASEND_ERR_ARG;
+        for (int k = 0; k < 3; ++k) axis[k] = axis_in[k] / std::sqrt(a2);
+    }
+    for (int64_t si = 0; si < n_set; ++si) {
+        const int64_t e = el_set[si];
+        if (e < 0 || e >= n_el) return ASEND_ERR_ARG;
+        const int32_t* c = conn + 8 * e;
+        double X[24];
+        for (int a = 0; a < 8; ++a)
+            for (int k = 0; k < 3; ++k) X[3 * a + k] = xyz[3 * (int64_t)c[a] + k];
+        double acc[3];
+        double dens = 1.0;
+        if (kind == 0) {
+            for (int k = 0; k < 3; ++k) acc[k] = load[k];
+        } else if (kind == 1) {
+            for (int k = 0; k < 3; ++k) acc[k] = load[k];
+            dens = rho[e];
+        } else {
+            double cen[3] = {0, 0, 0}, dp = 0.0;
+            for (int k = 0; k < 3; ++k) {
+                for (int a = 0; a < 8; ++a) cen[k] += X[3 * a + k];
+                cen[k] = cen[k] * 0.125 - center[k];
+                dp += axis[k] * cen[k];
+            }
+            for (int k = 0; k < 3; ++k) acc[k] = ang_vel * ang_vel * (cen[k] - axis[k] * dp);
+            dens = rho[e];
+        }
+        // every node carries the same acceleration, so sum_b N_b acc_b = acc
+        double ld[8][3];
+        std::memset(ld, 0, sizeof(ld));
+        double vol = 0.0;
+        for (int gp = 0; gp < 8; ++gp) {
+            const double s[3] = {(gp & 1) ? kGp : -kGp, (gp & 2) ? kGp : -kGp, (gp & 4) ? kGp : -kGp};
+            double N[8];
+            const double w = hex_detj(X, s, N);
+            double sn = 0.0;
+            for (int b = 0; b < 8; ++b) sn += N[b];
+            vol += w;
+            for (int a = 0; a < 8; ++a)
+                for (int k = 0; k < 3; ++k) ld[a][k] += N[a] * dens * (sn * acc[k]) * w;
+        }
+        if (kind == 0) {
+            double tot[3] = {0, 0, 0};
+            for (int a = 0; a < 8; ++a)
+                for (int k = 0; k < 3; ++k) tot[k] += ld[a][k];
+            const double inp = std::sqrt(acc[0] * acc[0] + acc[1] * acc[1] + acc[2] * acc[2]);
+            const double vm = std::sqrt(tot[0] * tot[0] + tot[1] * tot[1] + tot[2] * tot[2]);
+            const double sc = (vm > 0.0) ? inp * vol / vm : 0.0;
+            for (int a = 0; a < 8; ++a)
+                for (int k = 0; k < 3; ++k) ld[a][k] *= sc;
+        }
+        for (int a = 0; a < 8; ++a)
+            for (int k = 0; k < 3; ++k) {
+                const int64_t g = dof_index[6 * (int64_t)c[a] + k];
+                if (g < 0 || g >= n_dof) return ASEND_ERR_ARG;
+                f[g] += ld[a][k];
             }
     }
     (void)n_nodes;
@@ -428,6 +533,21 @@ int asend_surface_traction(int elem_type, int64_t n_nodes, const double* xyz, in
 
 int asend_lognormal_moduli(uint64_t seed, int64_t n, double median, double sigma_ln, double* out) {
     return asend::lognormal_moduli(seed, n, median, sigma_ln, out);
+}
+
+int asend_surface_pressure(int elem_type, int64_t n_nodes, const double* xyz, int64_t n_el, const int32_t* conn,
+                           const int64_t* dof_index, int64_t n_set, const int32_t* el_set, double pressure,
+                           const double* norm_dir, double norm_tol_deg, int64_t n_dof, double* f) {
+    return asend::surface_pressure(elem_type, n_nodes, xyz, n_el, conn, dof_index, n_set, el_set, pressure, norm_dir,
+                                   norm_tol_deg, n_dof, f);
+}
+
+int asend_body_load(int elem_type, int kind, int64_t n_nodes, const double* xyz, int64_t n_el, const int32_t* conn,
+                    const int64_t* dof_index, int64_t n_set, const int32_t* el_set, const double* rho,
+                    const double* load, const double* center, const double* axis, double angular_velocity,
+                    int64_t n_dof, double* f) {
+    return asend::body_load(elem_type, kind, n_nodes, xyz, n_el, conn, dof_index, n_set, el_set, rho, load, center,
+                            axis, angular_velocity, n_dof, f);
 }
 
 int asend_compliance_clamp_check(int elem_type, int64_t n_el, const double* E, const double* nu,

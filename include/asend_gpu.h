@@ -123,11 +123,36 @@ int64_t asend_block_dim_auto(int64_t n_nodes);
  * (S/element_equations.rs:1833-2056) with find_surface_faces
  * (S/model_prep.rs:510-554): accumulates (+=) the nodal loads of a uniform
  * traction on the surface faces of el_set whose outward normal lies within
- * norm_tol_deg of norm_dir.  Solid hex types only. */
+ * norm_tol_deg of norm_dir (normalised here, as the reference's loads are).
+ * Solid hex types only. */
 int asend_surface_traction(int elem_type, int64_t n_nodes, const double* xyz, int64_t n_el,
                            const int32_t* conn, const int64_t* dof_index, int64_t n_set,
                            const int32_t* el_set, const double* load, const double* norm_dir,
                            double norm_tol_deg, int64_t n_dof, double* f);
+
+/* The surfacePressure branch of the same routine (S/element_equations.rs:
+ * 1970-2043): accumulates (+=) the loads of the traction -pressure n on the
+ * surface faces of el_set whose outward unit normal n lies within
+ * norm_tol_deg of norm_dir (normalised here, as Load::set_norm_dir does,
+ * S/load_meth.rs:22-32).  Solid hex types only. */
+int asend_surface_pressure(int elem_type, int64_t n_nodes, const double* xyz, int64_t n_el,
+                           const int32_t* conn, const int64_t* dof_index, int64_t n_set,
+                           const int32_t* el_set, double pressure, const double* norm_dir,
+                           double norm_tol_deg, int64_t n_dof, double* f);
+
+/* The bodyForce (kind 0), gravitational (kind 1) and centrifugal (kind 2)
+ * branches of Element::get_app_load_dfd0 (S/element_equations.rs:1876-1969)
+ * for solid hex elements: accumulates (+=) the consistent nodal loads of the
+ * elements of el_set.  bodyForce: load = force per unit volume (each
+ * element's vector rescaled to |b| V as the reference does); gravitational:
+ * load = acceleration, times the element density rho[e]; centrifugal:
+ * rho[e] angular_velocity^2 times the distance vector from the axis (through
+ * center along axis, normalised here as Load::set_axis does) to the element
+ * centroid.  rho may be NULL for kind 0. */
+int asend_body_load(int elem_type, int kind, int64_t n_nodes, const double* xyz, int64_t n_el,
+                    const int32_t* conn, const int64_t* dof_index, int64_t n_set, const int32_t* el_set,
+                    const double* rho, const double* load, const double* center, const double* axis,
+                    double angular_velocity, int64_t n_dof, double* f);
 
 /* Seeded per-element moduli E_e = median * exp(sigma_ln * z_e), z_e standard
  * normal from splitmix64(seed) + Box-Muller (cosine branch, one pair per

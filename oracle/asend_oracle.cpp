@@ -857,11 +857,22 @@ int orc_assemble_solid_mass(int type, int64_t n_nodes, const double* xyz, int64_
     return 0;
 }
 
-int orc_surface_traction(int type, int64_t n_nodes, const double* xyz, int64_t n_el, const int32_t* conn,
-                         const int64_t* dof_index, int64_t n_set, const int32_t* el_set, const double* load,
-                         const double* norm_dir, double norm_tol_deg, double* f) {
+/* surfaceTraction (traction = load) and surfacePressure (traction = -p n of
+ * the face, pressure non-null) of get_app_load_dfd0
+ * (S/element_equations.rs:1970-2043); the load's normal direction is stored
+ * normalised (Load::set_norm_dir, S/load_meth.rs:22-32). */
+static int surface_load(int type, int64_t n_nodes, const double* xyz, int64_t n_el, const int32_t* conn,
+                        const int64_t* dof_index, int64_t n_set, const int32_t* el_set, const double* load,
+                        const double* pressure, const double* norm_dir_in, double norm_tol_deg, double* f) {
     (void)n_nodes;
     if (type != 8 && type != 81) return -1;
+    double norm_dir[3];
+    {
+        double m = 0.0;
+        for (int k = 0; k < 3; ++k) m += norm_dir_in[k] * norm_dir_in[k];
+        m = 1.0 / std::sqrt(m);
+        for (int k = 0; k < 3; ++k) norm_dir[k] = m * norm_dir_in[k];
+    }
     /* hex faces, local node order of Element::initialize_faces (S/element_meth.rs:537-590) */
     static const int fl[6][4] = {{3, 2, 1, 0}, {4, 5, 6, 7}, {0, 1, 5, 4},
                                  {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7}};
@@ -917,8 +928,10 @@ int orc_surface_traction(int type, int64_t n_nodes, const double* xyz, int64_t n
             double dp = norm_dir[0] * n[0] + norm_dir[1] * n[1] + norm_dir[2] * n[2];
             if (!(dp > cos_tol)) continue;
             any = true;
+            double trac[3];
+            for (int k = 0; k < 3; ++k) trac[k] = pressure ? -(*pressure) * n[k] : load[k];
             for (int i = 0; i < 4; ++i)
-                for (int k = 0; k < 3; ++k) acc[k * 8 + l[i]] = load[k];
+                for (int k = 0; k < 3; ++k) acc[k * 8 + l[i]] = trac[k];
             /* mass-like distribution with rho = 1 (get_rum_dfd0, actual_props = false) */
             for (int i = 0; i < 24; ++i) el_ld[i] = 0.0;
             for (int gp = 0; gp < 8; ++gp) {
@@ -940,7 +953,7 @@ int orc_surface_traction(int type, int64_t n_nodes, const double* xyz, int64_t n
                     tot[k] += el_ld[3 * a + k];
                 }
             }
-            double inp = std::sqrt(load[0] * load[0] + load[1] * load[1] + load[2] * load[2]);
+            double inp = std::sqrt(trac[0] * trac[0] + trac[1] * trac[1] + trac[2] * trac[2]);
             double vm = std::sqrt(tot[0] * tot[0] + tot[1] * tot[1] + tot[2] * tot[2]);
             double sc = area * inp / vm;
             for (int i = 0; i < 24; ++i) el_ld[i] *= sc;
@@ -948,6 +961,92 @@ int orc_surface_traction(int type, int64_t n_nodes, const double* xyz, int64_t n
         /* An element of the set with no matching face contributes nothing here
          * (the reference would scatter a stale scratch vector instead). */
         if (!any) continue;
+        for (int a = 0; a < 8; ++a)
+            for (int k = 0; k < 3; ++k) f[dof_index[6 * (int64_t)c[a] + k]] += el_ld[3 * a + k];
+    }
+    return 0;
+}
+
+int orc_surface_traction(int type, int64_t n_nodes, const double* xyz, int64_t n_el, const int32_t* conn,
+                         const int64_t* dof_index, int64_t n_set, const int32_t* el_set, const double* load,
+                         const double* norm_dir, double norm_tol_deg, double* f) {
+    return surface_load(type, n_nodes, xyz, n_el, conn, dof_index, n_set, el_set, load, nullptr, norm_dir,
+                        norm_tol_deg, f);
+}
+
+int orc_surface_pressure(int type, int64_t n_nodes, const double* xyz, int64_t n_el, const int32_t* conn,
+                         const int64_t* dof_index, int64_t n_set, const int32_t* el_set, double pressure,
+                         const double* norm_dir, double norm_tol_deg, double* f) {
+    return surface_load(type, n_nodes, xyz, n_el, conn, dof_index, n_set, el_set, nullptr, &pressure, norm_dir,
+                        norm_tol_deg, f);
+}
+
+/* bodyForce (kind 0), gravitational (1), centrifugal (2) loads of
+ * get_app_load_dfd0 (S/element_equations.rs:1876-1969): the element's nodal
+ * acceleration field through get_rum_dfd0 (f_a = sum_gp detJ N_a rho
+ * sum_b N_b acc_b, S/element_equations.rs:420-626) with rho = 1 for
+ * bodyForce (then scaled by |b| V / |sum f|, V from get_volume_dfd0) and the
+ * element density otherwise; the centrifugal field is omega^2 times the part
+ * of (centroid - center) normal to the normalised axis (Load::set_axis). */
+int orc_body_load(int type, int kind, int64_t n_el, const double* xyz, const int32_t* conn,
+                  const int64_t* dof_index, int64_t n_set, const int32_t* el_set, const double* rho,
+                  const double* load, const double* center, const double* axis_in, double ang_vel, double* f) {
+    if (type != 8 && type != 81) return -1;
+    double axis[3] = {0, 0, 0};
+    if (kind == 2) {
+        double m = 0.0;
+        for (int k = 0; k < 3; ++k) m += axis_in[k] * axis_in[k];
+        m = 1.0 / std::sqrt(m);
+        for (int k = 0; k < 3; ++k) axis[k] = m * axis_in[k];
+    }
+    for (int64_t si = 0; si < n_set; ++si) {
+        int64_t e = el_set[si];
+        if (e < 0 || e >= n_el) return -1;
+        const int32_t* c = conn + 8 * e;
+        double X[24];
+        gather_X(xyz, c, 8, X);
+        double acc[24];  /* glob_acc, index dof*8 + nd */
+        double den = 1.0;
+        if (kind == 2) {
+            double cent[3], dp = 0.0;
+            for (int k = 0; k < 3; ++k) {
+                cent[k] = 0.0;
+                for (int a = 0; a < 8; ++a) cent[k] += X[8 * k + a];
+                cent[k] *= 1.0 / 8.0;
+                cent[k] -= center[k];
+                dp += axis[k] * cent[k];
+            }
+            for (int k = 0; k < 3; ++k)
+                for (int a = 0; a < 8; ++a) acc[8 * k + a] = (cent[k] - axis[k] * dp) * ang_vel * ang_vel;
+        } else {
+            for (int k = 0; k < 3; ++k)
+                for (int a = 0; a < 8; ++a) acc[8 * k + a] = load[k];
+        }
+        if (kind != 0) den = rho[e];
+        double el_ld[24], vol = 0.0;
+        for (int i = 0; i < 24; ++i) el_ld[i] = 0.0;
+        for (int gp = 0; gp < 8; ++gp) {
+            double s[3], N[11], dNdx[33], detJ;
+            hex_gauss(gp, s);
+            hex_ip_data(type, X, s, N, dNdx, detJ);
+            vol += detJ;
+            double aip[3] = {0, 0, 0};
+            for (int k = 0; k < 3; ++k)
+                for (int b = 0; b < 8; ++b) aip[k] += acc[8 * k + b] * N[b];
+            for (int a = 0; a < 8; ++a)
+                for (int k = 0; k < 3; ++k) el_ld[3 * a + k] += N[a] * den * aip[k] * detJ;
+        }
+        if (kind == 0) {
+            double tot[3] = {0, 0, 0}, inp = 0.0, vm = 0.0;
+            for (int a = 0; a < 8; ++a)
+                for (int k = 0; k < 3; ++k) tot[k] += el_ld[3 * a + k];
+            for (int k = 0; k < 3; ++k) {
+                inp += load[k] * load[k];
+                vm += tot[k] * tot[k];
+            }
+            double sc = std::sqrt(inp) * vol / std::sqrt(vm);
+            for (int i = 0; i < 24; ++i) el_ld[i] *= sc;
+        }
         for (int a = 0; a < 8; ++a)
             for (int k = 0; k < 3; ++k) f[dof_index[6 * (int64_t)c[a] + k]] += el_ld[3 * a + k];
     }

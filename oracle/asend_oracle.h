@@ -100,6 +100,12 @@ int orc_surface_traction(int type, int64_t n_nodes, const double* xyz, int64_t n
                          int64_t n_set, const int32_t* el_set,
                          const double* load, const double* norm_dir, double norm_tol_deg,
                          double* f);
+int orc_surface_pressure(int type, int64_t n_nodes, const double* xyz, int64_t n_el, const int32_t* conn,
+                         const int64_t* dof_index, int64_t n_set, const int32_t* el_set, double pressure,
+                         const double* norm_dir, double norm_tol_deg, double* f);
+int orc_body_load(int type, int kind, int64_t n_el, const double* xyz, const int32_t* conn,
+                  const int64_t* dof_index, int64_t n_set, const int32_t* el_set, const double* rho,
+                  const double* load, const double* center, const double* axis, double ang_vel, double* f);
 
 /* SparseMat::get_max_abs_val (S/list_ent_meth.rs:81-94). */
 double orc_max_abs(int64_t nnz, const double* vals);

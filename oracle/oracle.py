@@ -212,6 +212,40 @@ def surface_traction(etype, xyz, conn, dof_index, n_dof, el_set, load, norm_dir,
     return f
 
 
+def surface_pressure(etype, xyz, conn, dof_index, n_dof, el_set, pressure, norm_dir, norm_tol_deg):
+    """get_app_load_dfd0 surfacePressure (S/element_equations.rs:1970-2043)."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    conn = np.ascontiguousarray(conn, dtype=np.int32)
+    di = np.ascontiguousarray(dof_index, dtype=np.int64)
+    es = np.ascontiguousarray(el_set, dtype=np.int32)
+    nd = np.ascontiguousarray(norm_dir, dtype=np.float64)
+    f = np.zeros(n_dof)
+    _chk(lib().orc_surface_pressure(ctypes.c_int(etype), _i64(xyz.shape[0]), _ptr(xyz), _i64(conn.shape[0]),
+                                    _ptr(conn), _ptr(di), _i64(es.shape[0]), _ptr(es), _dbl(pressure), _ptr(nd),
+                                    _dbl(norm_tol_deg), _ptr(f)), "surface_pressure")
+    return f
+
+
+BODY_FORCE, GRAVITATIONAL, CENTRIFUGAL = 0, 1, 2
+
+
+def body_load(etype, kind, xyz, conn, dof_index, n_dof, el_set, rho=None, load=(0.0, 0.0, 0.0),
+              center=(0.0, 0.0, 0.0), axis=(0.0, 0.0, 1.0), ang_vel=0.0):
+    """get_app_load_dfd0 bodyForce / gravitational / centrifugal
+    (S/element_equations.rs:1876-1969)."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    conn = np.ascontiguousarray(conn, dtype=np.int32)
+    di = np.ascontiguousarray(dof_index, dtype=np.int64)
+    es = np.ascontiguousarray(el_set, dtype=np.int32)
+    rho = np.ones(conn.shape[0]) if rho is None else np.ascontiguousarray(rho, dtype=np.float64)
+    ld, ce, ax = (np.ascontiguousarray(v, dtype=np.float64) for v in (load, center, axis))
+    f = np.zeros(n_dof)
+    _chk(lib().orc_body_load(ctypes.c_int(etype), ctypes.c_int(kind), _i64(conn.shape[0]), _ptr(xyz), _ptr(conn),
+                             _ptr(di), _i64(es.shape[0]), _ptr(es), _ptr(rho), _ptr(ld), _ptr(ce), _ptr(ax),
+                             _dbl(ang_vel), _ptr(f)), "body_load")
+    return f
+
+
 def max_abs(K):
     """SparseMat::get_max_abs_val (S/list_ent_meth.rs:81-94)."""
     return lib().orc_max_abs(_i64(K.vals.shape[0]), _ptr(K.vals))

@@ -109,3 +109,33 @@ def test_lognormal_moduli_seeded():
     assert abs(z.mean()) < 0.01 and abs(z.std() - 1.0) < 0.01
     c = api.lognormal_moduli(1, 10, 70e3, 1.0)
     assert not np.allclose(a[:10], c)
+
+
+@pytest.mark.parametrize("etype", [8, 81])
+def test_pressure_and_body_loads_match_oracle(etype):
+    # distorted 6x3x2 block, per-element density; every load type on all
+    # elements and on a subset
+    m = hex_block(6, 3, 2, 1.3, 0.4, 0.3)
+    rng = np.random.default_rng(4)
+    inner = np.all((m.xyz > m.xyz.min(axis=0) + 1e-9) & (m.xyz < m.xyz.max(axis=0) - 1e-9), axis=1)
+    xyz = m.xyz.copy()
+    xyz[inner] += rng.uniform(-0.03, 0.03, (int(inner.sum()), 3))
+    clamp = m.nodes_at_x(0)
+    _, di, nd = O.reorder_nodes(xyz, m.conn, 3, [[clamp]] * 3, 24)
+    rho = rng.uniform(500.0, 3000.0, m.n_el)
+    every = np.arange(m.n_el, dtype=np.int32)
+    some = every[1::2]  # odd labels: tip (x = 5) and bottom-layer elements among them
+    for es in (every, some):
+        for ndir in ([1.0, 0.0, 0.0], [0.0, 0.0, -2.0]):
+            a = api.surface_pressure(etype, xyz, m.conn, di, nd, es, 2.5e5, ndir, 10.0)
+            b = O.surface_pressure(etype, xyz, m.conn, di, nd, es, 2.5e5, ndir, 10.0)
+            assert np.abs(b).max() > 0
+            assert np.allclose(a, b, rtol=1e-12, atol=1e-12 * np.abs(b).max())
+        for kind, kw in ((api.BODY_FORCE, {"load": [3.0, -2.0, 9.81]}),
+                         (api.GRAVITATIONAL, {"rho": rho, "load": [0.0, 0.0, -9.81]}),
+                         (api.CENTRIFUGAL, {"rho": rho, "center": [0.2, 0.1, -0.3], "axis": [0.0, 1.0, 1.0],
+                                             "ang_vel": 30.0})):
+            a = api.body_load(etype, kind, xyz, m.conn, di, nd, es, **kw)
+            b = O.body_load(etype, kind, xyz, m.conn, di, nd, es, **kw)
+            assert np.abs(b).max() > 0
+            assert np.allclose(a, b, rtol=1e-12, atol=1e-12 * np.abs(b).max()), kind

@@ -425,3 +425,89 @@ def test_material_partials_dual_numbers_match_finite_differences():
     total = sum(O.solid_element_dmat(8, X, *iso, p)[1] for p in range(3))
     total = total + sum(O.solid_element_dmat(8, X, *iso, p)[1] for p in range(6, 9)) / (2 * (1 + nu))
     assert np.abs(total - dKE).max() <= 1e-12 * np.abs(dKE).max()
+
+
+def _box(nx, ny, nz, lx, ly, lz, distort=0.0, seed=0):
+    from asend3d_amd.meshgen import hex_block
+    m = hex_block(nx, ny, nz, lx, ly, lz)
+    xyz = m.xyz.copy()
+    if distort:
+        rng = np.random.default_rng(seed)
+        inner = np.all((xyz > xyz.min(axis=0) + 1e-9) & (xyz < xyz.max(axis=0) - 1e-9), axis=1)
+        xyz[inner] += rng.uniform(-distort, distort, (int(inner.sum()), 3))
+    di = np.full((m.n_nodes, 6), -1, dtype=np.int64)
+    di[:, :3] = np.arange(3 * m.n_nodes).reshape(-1, 3)
+    return m, xyz, di
+
+
+def _hex_volumes(xyz, conn):
+    # exact volume of a trilinear hex: 8-point Gauss integration of detJ
+    g = 1.0 / np.sqrt(3.0)
+    sgn = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1], [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]])
+    vol = np.zeros(conn.shape[0])
+    for gp in range(8):
+        s = np.array([g if gp & 1 else -g, g if gp & 2 else -g, g if gp & 4 else -g])
+        f = 1.0 + sgn * s
+        dN = 0.125 * np.stack([sgn[:, 0] * f[:, 1] * f[:, 2], f[:, 0] * sgn[:, 1] * f[:, 2],
+                               f[:, 0] * f[:, 1] * sgn[:, 2]], axis=1)
+        J = np.einsum("eai,aj->eij", xyz[conn], dN)
+        vol += np.linalg.det(J)
+    return vol
+
+
+def test_body_loads_closed_forms():
+    # consistent loads of a uniform field: on an undistorted box every node
+    # of an element gets V_e/8 of the element's force; on a distorted mesh the
+    # totals are b sum V_e, g sum rho_e V_e and omega^2 sum rho_e V_e r_perp(c_e)
+    m, xyz, di = _box(4, 3, 2, 2.0, 0.9, 0.4)
+    every = np.arange(m.n_el)
+    n = 3 * m.n_nodes
+    b = np.array([1.5, -0.5, 2.0])
+    f = O.body_load(8, O.BODY_FORCE, xyz, m.conn, di, n, every, load=b).reshape(-1, 3)
+    Ve = 2.0 * 0.9 * 0.4 / m.n_el
+    share = np.zeros(m.n_nodes)
+    np.add.at(share, m.conn.ravel(), Ve / 8.0)
+    assert np.allclose(f, share[:, None] * b, rtol=1e-13, atol=1e-15)
+    m, xyz, di = _box(5, 3, 2, 2.0, 0.9, 0.4, distort=0.05, seed=2)
+    every = np.arange(m.n_el)
+    n = 3 * m.n_nodes
+    vol = _hex_volumes(xyz, m.conn)
+    rho = np.linspace(1.0, 3.0, m.n_el)
+    f = O.body_load(81, O.BODY_FORCE, xyz, m.conn, di, n, every, load=b).reshape(-1, 3)
+    assert np.allclose(f.sum(axis=0), b * vol.sum(), rtol=1e-12)
+    g = np.array([0.0, 0.0, -9.81])
+    f = O.body_load(8, O.GRAVITATIONAL, xyz, m.conn, di, n, every, rho=rho, load=g).reshape(-1, 3)
+    assert np.allclose(f.sum(axis=0), g * (rho * vol).sum(), rtol=1e-12, atol=1e-12)
+    cen, ax, om = np.array([0.1, 0.45, 0.0]), np.array([0.0, 0.0, 3.0]), 7.0
+    f = O.body_load(8, O.CENTRIFUGAL, xyz, m.conn, di, n, every, rho=rho, center=cen, axis=ax,
+                    ang_vel=om).reshape(-1, 3)
+    c = xyz[m.conn].mean(axis=1) - cen
+    r_perp = c.copy()
+    r_perp[:, 2] = 0.0
+    assert np.allclose(f.sum(axis=0), om ** 2 * (rho[:, None] * vol[:, None] * r_perp).sum(axis=0), rtol=1e-12,
+                       atol=1e-12)
+
+
+def test_surface_pressure_closed_forms():
+    # the pressure load is the traction -p n face by face: on the x = Lx face
+    # of a box it equals the traction (-p, 0, 0); on a distorted block the
+    # total over the loaded faces is -p times their vector area
+    # 1/2 d1 x d2 (exact for any quadrilateral)
+    m, xyz, di = _box(4, 3, 2, 2.0, 0.9, 0.4)
+    n = 3 * m.n_nodes
+    tip = m.elements_at_x(m.nx - 1)
+    p = 3.0e5
+    fp = O.surface_pressure(8, xyz, m.conn, di, n, tip, p, [1.0, 0.0, 0.0], 5.0)
+    ft = O.surface_traction(8, xyz, m.conn, di, n, tip, [-p, 0.0, 0.0], [1.0, 0.0, 0.0], 5.0)
+    assert np.allclose(fp, ft, rtol=1e-13, atol=1e-9)
+    assert np.allclose(fp.reshape(-1, 3).sum(axis=0), [-p * 0.9 * 0.4, 0.0, 0.0], rtol=1e-12, atol=1e-6)
+    m, xyz, di = _box(4, 3, 2, 2.0, 0.9, 0.4, distort=0.05, seed=3)
+    xyz[m.nodes_at_x(m.nx), 0] += 0.02 * np.sin(np.arange(len(m.nodes_at_x(m.nx))))
+    n = 3 * m.n_nodes
+    fp = O.surface_pressure(8, xyz, m.conn, di, n, tip, p, [1.0, 0.0, 0.0], 25.0)
+    # +x face of hex_block elements: local nodes (1, 2, 6, 5)
+    area = np.zeros(3)
+    for e in tip:
+        q = xyz[m.conn[e][[1, 2, 6, 5]]]
+        area += 0.5 * np.cross(q[2] - q[0], q[3] - q[1])
+    assert np.allclose(fp.reshape(-1, 3).sum(axis=0), -p * area, rtol=1e-12, atol=1e-6)
