@@ -22,6 +22,7 @@ SHELL4 = 41
 
 SOLVE_PCG_FP64 = 0
 SOLVE_IR_MIXED = 1
+SOLVE_DIRECT = 2  # one-block LDL^T factor (asend_ldl_factor), the reference's direct solverMethod
 NM_FP64 = 0   # Newmark step solver: FP64 single-reduction Jacobi PCG (default)
 NM_MIXED = 1  # FP32 single-reduction Jacobi PCG, FP64 increment, reliable updates
 NM_AUTO = 2   # mixed when the FP64 operator exceeds 128 MiB, else FP64 (default)
@@ -31,6 +32,7 @@ PC_BLOCK_JACOBI = 1
 PC_MG = 2
 PC_LEARNED = 3
 PC_LEARNED_PCG = 4
+PC_LDL = 5  # block LDL^T factor of asend_ldl_factor (the reference's LowerTriMat preconditioner)
 
 STOP_REL_RESIDUAL = 0
 STOP_REF_ABS = 1
@@ -109,6 +111,15 @@ class NlStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class LdlInfo(ctypes.Structure):
+    _fields_ = [("block_dofs", ctypes.c_int64), ("n_blocks", ctypes.c_int64), ("max_bandwidth", ctypes.c_int64),
+                ("bytes", ctypes.c_int64), ("neg_pivots", ctypes.c_int32), ("pos_def", ctypes.c_int32),
+                ("factor_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class SolveStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int64), ("outer_iterations", ctypes.c_int64),
                 ("rel_residual", ctypes.c_double), ("ref_residual", ctypes.c_double),
@@ -138,6 +149,7 @@ EXPORTED = ["asend_ctx_create", "asend_ctx_destroy", "asend_last_error", "asend_
             "asend_compliance_clamp_check", "asend_surface_pressure", "asend_body_load",
             "asend_upload_mesh", "asend_set_thickness", "asend_set_grid", "asend_update_moduli", "asend_set_penalty",
             "asend_penalty_scale", "asend_uniform_elements", "asend_assemble", "asend_get_matrix", "asend_spmv", "asend_solve", "asend_solve_noalloc",
+            "asend_ldl_factor", "asend_ldl_solve", "asend_ldl_release",
             "asend_element_sens", "asend_element_sens_material", "asend_static_compliance", "asend_bench_spmv", "asend_mg_info", "asend_mg_apply",
             "asend_mg_restrict", "asend_mg_prolong", "asend_mg_vcycle", "asend_mg_coarse_solve", "asend_bench_copy",
             "asend_learned_num_params", "asend_learned_init", "asend_learned_get_params",
@@ -397,6 +409,22 @@ class Context:
         out = np.zeros(self.n_el)
         self._chk(_lib.asend_element_sens(self.h, _p(u), _p(lam), _p(out)), "asend_element_sens")
         return out
+
+    def ldl_factor(self, block_dofs=0):
+        """LDL^T factor of K_eff over diagonal blocks of block_dofs DOFs (0: one
+        block, exact) for PC_LDL and SOLVE_DIRECT solves (asend_ldl_factor)."""
+        info = LdlInfo()
+        self._chk(_lib.asend_ldl_factor(self.h, _i64(block_dofs), ctypes.byref(info)), "asend_ldl_factor")
+        return info.as_dict()
+
+    def ldl_solve(self, b):
+        b = _f64(b)
+        x = np.zeros(self.n_dof)
+        self._chk(_lib.asend_ldl_solve(self.h, _p(b), _p(x)), "asend_ldl_solve")
+        return x
+
+    def ldl_release(self):
+        self._chk(_lib.asend_ldl_release(self.h), "asend_ldl_release")
 
     def element_sens_material(self, u, lam):
         """(n_el, 9) partials of -lambda^T K_e u w.r.t. E1, E2, E3, nu12, nu13,

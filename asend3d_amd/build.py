@@ -20,7 +20,7 @@ CLI = os.path.join(PKG, "asend_run_job")
 CLI_SRC = os.path.join(PKG, "cli", "asend_run_job.cpp")
 BUILD = os.path.join(ROOT, "_build")
 
-HIP_SOURCES = ["kernels.hip", "precond.hip", "learned.hip", "dynamics.hip", "batched.hip", "shell.hip", "brickim.hip", "modal.hip", "nonlinear.hip", "context.hip"]
+HIP_SOURCES = ["kernels.hip", "precond.hip", "learned.hip", "dynamics.hip", "batched.hip", "shell.hip", "brickim.hip", "modal.hip", "nonlinear.hip", "ldl.hip", "context.hip"]
 CXX_SOURCES = ["host_prep.cpp"]
 OFFLOAD_ARCH = "gfx950"
 

@@ -132,6 +132,34 @@ std::string fmt_e12(double v) {
     return mant + "e" + std::to_string(ex);
 }
 
+// Rust's {} for f64: the shortest digits that read back to the same double,
+// written out in plain decimal notation without an exponent
+std::string fmt_display(double v) {
+    if (std::isnan(v)) return "NaN";
+    if (std::isinf(v)) return v > 0 ? "inf" : "-inf";
+    if (v == 0.0) return std::signbit(v) ? "-0" : "0";
+    char buf[64];
+    for (int p = 1; p <= 17; ++p) {
+        std::snprintf(buf, sizeof buf, "%.*e", p - 1, v);
+        if (std::strtod(buf, nullptr) == v) break;
+    }
+    std::string s(buf);
+    const bool neg = s[0] == '-';
+    if (neg) s = s.substr(1);
+    const size_t e = s.find('e');
+    const int ex = std::atoi(s.c_str() + e + 1);
+    std::string digits;
+    for (char ch : s.substr(0, e))
+        if (ch != '.') digits += ch;
+    while (digits.size() > 1 && digits.back() == '0') digits.pop_back();
+    const int point = ex + 1;  // digits before the decimal point
+    std::string out;
+    if (point <= 0) out = "0." + std::string(-point, '0') + digits;
+    else if (point >= (int)digits.size()) out = digits + std::string(point - digits.size(), '0');
+    else out = digits.substr(0, point) + "." + digits.substr(point);
+    return (neg ? "-" : "") + out;
+}
+
 // ---------------------------------------------------------------- the model
 struct Section {
     std::string type, mat, el_set;
@@ -214,6 +242,8 @@ struct Job {
     std::vector<double> f;    // applied load
     std::vector<double> dLdD;
     bool solved = false, have_grad = false;
+    int64_t solver_block_dim = 0;        // nodes per reordering block (2e9: direct)
+    bool ldl_ok = true, bw_printed = false;  // LDL^T factor usable / maxBandwidth line printed
     Command solve_cmd;
 
     ~Job() {
@@ -637,7 +667,9 @@ struct Job {
         }
         int64_t bd;
         if (c.method == "direct" || c.method.empty()) bd = 2000000000;
-        else bd = (c.block_dim > 0) ? c.block_dim : asend_block_dim_auto((int64_t)nn);
+        // an unset solverBlockDim (2e9) takes the automatic size (S/model_prep.rs:892-901)
+        else bd = (c.block_dim > 0 && c.block_dim != 2000000000) ? c.block_dim : asend_block_dim_auto((int64_t)nn);
+        solver_block_dim = bd;
         std::vector<int32_t> ordered(nn);
         dof_index.assign(6 * nn, -1);
         int rc = asend_reorder_nodes((int64_t)nn, xyz.data(), (int64_t)ne, npe, dpn, cflat.data(),
@@ -692,20 +724,44 @@ struct Job {
         }
     }
 
-    // K x = b through the device; the reference's direct branch solves
-    // exactly, here FP64 Jacobi PCG to a 1e-12 relative residual; the
-    // iterative branch follows conj_grad_sparse's absolute stop rule
+    // K x = b through the device as solve_step (S/model_analysis.rs:470-501):
+    // the direct branch factors K_eff over one block (ldl_factor) and solves
+    // with the factor (ldl_solve); the iterative branch runs conj_grad_sparse
+    // preconditioned by the factor over blocks of 6 solverBlockDim DOFs,
+    // with its absolute stop rule.  The factor is kept for the adjoint solve
+    // and redone after a new assembly.  A factor that does not fit the device
+    // falls back to FP64 Jacobi PCG (to a 1e-12 relative residual for direct,
+    // the same absolute rule for iterative), saying so.
     void device_solve(const std::vector<double>& b, std::vector<double>& x, bool report) {
+        const bool iterative = (solve_cmd.method == "iterative");
+        asend_ldl_info li;
+        std::memset(&li, 0, sizeof li);
+        const int frc = ldl_ok ? asend_ldl_factor(ctx, iterative ? 6 * (int64_t)solver_block_dim : 0, &li) : -1;
+        if (ldl_ok && frc != ASEND_OK) {
+            if (frc != ASEND_ERR_UNSUPPORTED) check(frc, "asend_ldl_factor");
+            std::printf("note: LDL^T factor not available on the device (%s); FP64 Jacobi PCG instead\n",
+                        asend_last_error(ctx));
+            ldl_ok = false;
+        }
+        if (ldl_ok && !bw_printed) {
+            std::printf("maxBandwidth: %lld\n", (long long)li.max_bandwidth);
+            if (!li.pos_def)
+                std::printf("Warning: preconditioning matrix for conjugate gradient solver is not positive "
+                            "definite.\n");
+            bw_printed = true;
+        }
         asend_solve_opts o;
         std::memset(&o, 0, sizeof o);
         o.method = ASEND_SOLVE_PCG_FP64;
-        o.precond = ASEND_PC_JACOBI;
+        o.precond = ldl_ok ? ASEND_PC_LDL : ASEND_PC_JACOBI;
         o.check_every = 1;
-        const bool iterative = (solve_cmd.method == "iterative");
+        const int64_t max_it = solve_cmd.max_it > 0 ? solve_cmd.max_it : n_dof;  // S/model_prep.rs:237-243
         if (iterative) {
             o.stop_rule = ASEND_STOP_REF_ABS;
             o.tol = solve_cmd.conv_tol;
-            o.max_it = solve_cmd.max_it;
+            o.max_it = max_it;
+        } else if (ldl_ok) {
+            o.method = ASEND_SOLVE_DIRECT;
         } else {
             o.stop_rule = ASEND_STOP_REL_RESIDUAL;
             o.tol = 1e-12;
@@ -718,13 +774,14 @@ struct Job {
         check(asend_solve(ctx, b.data(), x.data(), &o, &st), "asend_solve");
         if (report && iterative) {
             if (!st.converged)
-                std::printf("Warning: Conjugate gradient solver did not converge to specified tolerance, %g within "
-                            "tne maximum number of iterations, %ld\n",
-                            solve_cmd.conv_tol, (long)solve_cmd.max_it);
-            std::printf("Total CG iterations: %ld\n", (long)st.iterations);
-            std::printf("Final residual norm: %s\n", fmt_e12(st.ref_residual).c_str());
+                std::printf("Warning: Conjugate gradient solver did not converge to specified tolerance, %s within "
+                            "tne maximum number of iterations, %lld\n",
+                            fmt_display(solve_cmd.conv_tol).c_str(), (long long)max_it);
+            std::printf("Total CG iterations: %lld\n", (long long)st.iterations);
+            std::printf("Final residual norm: %s\n", fmt_display(st.ref_residual).c_str());
         } else if (report) {
-            std::printf("device solve: %ld FP64 PCG iterations, relative residual %s\n", (long)st.iterations,
+            std::printf("device solve (%s): relative residual %s\n",
+                        o.method == ASEND_SOLVE_DIRECT ? "LDL^T factor" : "FP64 Jacobi PCG",
                         fmt_e12(st.rel_residual).c_str());
         }
     }

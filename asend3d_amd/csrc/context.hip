@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -24,6 +25,7 @@
 #include "learned.hpp"
 #include "batched.hpp"
 #include "host_prep.hpp"
+#include "ldl.hpp"
 #include "device_common.hpp"
 
 using namespace asend;
@@ -116,6 +118,11 @@ struct asend_ctx {
     int n_edge = 0;
     double mean_log_e = 0.0;
     std::vector<double> h_nu;  // per-element Poisson ratio (moduli updates are checked against it)
+    // LDL^T factor over DOF blocks (asend_ldl_factor): the block size asked
+    // for (< 0: none), the factor and the preconditioned vector of its PCG
+    LdlFactor ldl;
+    int64_t ldl_block_dofs = -1;
+    double* d_ldl_z = nullptr;
     int64_t asm_count = 0, factor_asm = -1;
     int64_t mg_asm = -1;  // assembly the multigrid hierarchy was last set up for
     double asm_cmass = 0.0;   // mass coefficient of the current assembly
@@ -282,7 +289,17 @@ void newmark_free(asend_ctx* ctx) {
     nm.ready = false;
 }
 
+void ldl_release(asend_ctx* ctx) {
+    dfree(ctx->ldl.L);
+    dfree(ctx->ldl.P);
+    dfree(ctx->ldl.neg);
+    dfree(ctx->d_ldl_z);
+    ctx->ldl = LdlFactor();
+}
+
 void free_mesh(asend_ctx* ctx) {
+    ldl_release(ctx);
+    ctx->ldl_block_dofs = -1;
     dfree(ctx->d_conn_r);
     dfree(ctx->d_rp);
     dfree(ctx->d_bc);
@@ -412,20 +429,106 @@ int run_jacobi_pcg64(asend_ctx* ctx, const Fp64Bufs& B, double tol, bool stop_ab
     return ASEND_OK;
 }
 
+// The LDL^T factor of the current assembly over the requested DOF blocks
+// (LowerTriMat allocate/populate/ldl_factor, S/lower_tri_mat_meth.rs:15-190,
+// done before each static solve at S/model_analysis.rs:584-589): planned and
+// allocated once per mesh and block size, refilled and refactored whenever
+// the operator was assembled again.
+int ldl_ensure(asend_ctx* ctx) {
+    if (ctx->ldl_block_dofs < 0)
+        return fail(ctx, ASEND_ERR_STATE, "no LDL^T factor requested (asend_ldl_factor)");
+    if (!ctx->assembled) return fail(ctx, ASEND_ERR_STATE, "assemble first");
+    LdlFactor& F = ctx->ldl;
+    if (F.L && F.asm_id == ctx->asm_count) return ASEND_OK;
+    int rc;
+    if (!F.L) {
+        LdlDesc d;
+        int64_t W = 0;
+        const char* msg = ldl_plan(&d, &W, ctx->n_dof, ctx->ldl_block_dofs, ctx->h_rp, ctx->h_bc);
+        if (msg) return fail(ctx, ASEND_ERR_UNSUPPORTED, "LDL^T factor: %s", msg);
+        const size_t bytes = (size_t)d.n_blocks * (size_t)d.stride * sizeof(double);
+        size_t free_b = 0, total_b = 0;
+        HIPCHK(hipMemGetInfo(&free_b, &total_b));
+        if (bytes + ((size_t)1 << 30) > free_b)
+            return fail(ctx, ASEND_ERR_UNSUPPORTED,
+                        "LDL^T factor of %lld blocks, band %lld: %.2f GB of device memory needed, %.2f GB free",
+                        (long long)d.n_blocks, (long long)W, bytes / 1e9, free_b / 1e9);
+        if ((rc = dalloc(ctx, &F.L, (size_t)d.n_blocks * d.stride)) ||
+            (rc = dalloc(ctx, &F.P, (size_t)d.n_blocks * kLdlTileElems)) || (rc = dalloc(ctx, &F.neg, 1)) ||
+            (rc = dalloc(ctx, &ctx->d_ldl_z, (size_t)ctx->n_dof))) {
+            ldl_release(ctx);
+            return rc;
+        }
+        F.d = d;
+        F.max_bandwidth = W;
+        F.bytes = (int64_t)bytes;
+    }
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    const auto t0 = std::chrono::steady_clock::now();
+    ldl_fill(ctx->st, F.d, ctx->fine().n_rows, (int)ctx->n_slices, ctx->d_sptr, ctx->d_scol, ctx->d_K64, F.L);
+    ldl_factor_launch(ctx->st, F.d, F.L, F.P, F.neg);
+    HIPCHK(hipGetLastError());
+    int32_t neg = 0;
+    HIPCHK(hipMemcpyAsync(&neg, F.neg, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    F.factor_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    F.neg_pivots = neg;
+    F.asm_id = ctx->asm_count;
+    return ASEND_OK;
+}
+
+// One PCG run on the FP64 operator preconditioned by the LDL^T factor
+// (conj_grad_sparse, S/matrix_functions.rs:285-362): B.x solves K x = B.r
+// from x = 0; host mirror slot 1 receives the iteration count and the final
+// g^T M^-1 g, the quantity the absolute stop rule tests.
+int run_ldl_pcg64(asend_ctx* ctx, const Fp64Bufs& B, double tol, bool stop_abs, int64_t max_it, int check_every) {
+    const int64_t n = ctx->n_dof;
+    Scalars& h = ctx->h_sc[1];
+    std::memset(&h, 0, sizeof(Scalars));
+    h.tol = tol;
+    h.tol2 = tol * tol;
+    h.stop_abs = stop_abs ? 1 : 0;
+    h.max_it = max_it;
+    Scalars* sc = B.sc + 1;
+    int rc;
+    if ((rc = write_scalars_to(ctx, sc, 1))) return rc;
+    const LdlFactor& F = ctx->ldl;
+    double* z = ctx->d_ldl_z;
+    launch_pcgz_start(ctx->st, n, B.r, B.x, B.r2);
+    ldl_solve_launch(ctx->st, F.d, F.L, B.r2, z, nullptr);
+    launch_pcgz_dots(ctx->st, n, B.r2, z, B.p, true, B.partials, sc);
+    for (;;) {
+        for (int k = 0; k < check_every; ++k) {
+            launch_sell3<double>(ctx->st, 1, ctx->fine(), ctx->d_K64, B.p, B.q, nullptr, B.partials, sc);
+            launch_pcgz_update(ctx->st, n, B.x, B.r2, B.p, B.q, sc);
+            ldl_solve_launch(ctx->st, F.d, F.L, B.r2, z, &sc->done);
+            launch_pcgz_dots(ctx->st, n, B.r2, z, B.p, false, B.partials, sc);
+            launch_pcgz_dir(ctx->st, n, z, B.p, sc);
+        }
+        HIPCHK(hipGetLastError());
+        if ((rc = read_scalars_from(ctx, sc, 1))) return rc;
+        if (h.done) break;
+    }
+    return ASEND_OK;
+}
+
 // FP64 Jacobi PCG solve of K u = B.f into B.u (conj_grad_sparse with x0 = 0,
 // S/matrix_functions.rs:285-362, with the Jacobi preconditioner).  Under the
 // relative rule a run whose recursive residual drifted from the true one is
 // restarted from the true residual.  Allocates nothing: every buffer it
 // writes is in B.
 int solve_fp64(asend_ctx* ctx, const Fp64Bufs& B, const asend_solve_opts* o, asend_solve_stats* st) {
-    if (o->precond != ASEND_PC_JACOBI) return fail(ctx, ASEND_ERR_UNSUPPORTED, "FP64 PCG supports Jacobi only");
+    if (o->precond != ASEND_PC_JACOBI && o->precond != ASEND_PC_LDL)
+        return fail(ctx, ASEND_ERR_UNSUPPORTED, "FP64 PCG supports the Jacobi and LDL^T preconditioners only");
+    const bool ldl = o->precond == ASEND_PC_LDL;
+    int rc;
+    if (ldl && (rc = ldl_ensure(ctx))) return rc;
     const int64_t n = ctx->n_dof;
     const int64_t max_it = o->max_it > 0 ? o->max_it : n;
     const int check = o->check_every > 0 ? o->check_every : (ctx->tuned_check > 0 ? ctx->tuned_check : 16);
     const int64_t max_outer = o->max_outer > 0 ? o->max_outer : 20;
     asend_solve_stats s;
     std::memset(&s, 0, sizeof(s));
-    int rc;
     // ||f||^2 with u = 0: r = f - K*0
     hipLaunchKernelGGL(k_fill64, dim3(kVecBlocks), dim3(256), 0, ctx->st, n, 0.0, B.u);
     std::memset(&ctx->h_sc[0], 0, sizeof(Scalars));
@@ -444,7 +547,8 @@ int solve_fp64(asend_ctx* ctx, const Fp64Bufs& B, const asend_solve_opts* o, ase
     for (; outer < max_outer; ++outer) {
         if (o->stop_rule == ASEND_STOP_REL_RESIDUAL && std::sqrt(rr / ff) <= o->tol) break;
         const double tol_in = (o->stop_rule == ASEND_STOP_REL_RESIDUAL) ? o->tol * std::sqrt(ff / rr) : o->tol;
-        if ((rc = run_jacobi_pcg64(ctx, B, tol_in, o->stop_rule == ASEND_STOP_REF_ABS, max_it - total_it, check)))
+        if ((rc = (ldl ? run_ldl_pcg64 : run_jacobi_pcg64)(ctx, B, tol_in, o->stop_rule == ASEND_STOP_REF_ABS,
+                                                            max_it - total_it, check)))
             return rc;
         total_it += ctx->h_sc[1].it;
         s.ref_residual = ctx->h_sc[1].rz;
@@ -702,8 +806,37 @@ int ensure_mg_and_factor(asend_ctx* ctx, bool* mg_failed) {
     return ASEND_OK;
 }
 
+// Direct solve with the one-block factor: u = (L D L^T)^-1 f (the
+// reference's "direct" branch, ldl_solve at S/model_analysis.rs:495-496);
+// the stats carry the FP64 relative residual of the result.
+int solve_direct(asend_ctx* ctx, asend_solve_stats* st) {
+    int rc;
+    if ((rc = ldl_ensure(ctx))) return rc;
+    if (ctx->ldl.d.n_blocks != 1)
+        return fail(ctx, ASEND_ERR_STATE, "direct solve needs a factor over one block (asend_ldl_factor with "
+                                          "block_dofs 0 or >= n_dof)");
+    const int64_t n = ctx->n_dof;
+    asend_solve_stats s;
+    std::memset(&s, 0, sizeof(s));
+    ldl_solve_launch(ctx->st, ctx->ldl.d, ctx->ldl.L, ctx->d_f, ctx->d_u, nullptr);
+    std::memset(&ctx->h_sc[0], 0, sizeof(Scalars));
+    if ((rc = write_scalars(ctx, 0))) return rc;
+    launch_dot64(ctx->st, n, ctx->d_f, ctx->d_f, ctx->d_partials, ctx->d_sc);
+    HIPCHK(hipGetLastError());
+    if ((rc = read_scalars(ctx, 0))) return rc;
+    const double ff = ctx->h_sc[0].dot;
+    if ((rc = residual64(ctx, ctx->d_f, ctx->d_u, ctx->d_r))) return rc;
+    if ((rc = read_scalars(ctx, 0))) return rc;
+    s.rel_residual = ff > 0.0 ? std::sqrt(ctx->h_sc[0].rr / ff) : 0.0;
+    s.outer_iterations = 1;
+    s.converged = std::isfinite(s.rel_residual) ? 1 : 0;
+    *st = s;
+    return ASEND_OK;
+}
+
 int solve_device(asend_ctx* ctx, const asend_solve_opts* o, asend_solve_stats* st) {
     if (o->method == ASEND_SOLVE_PCG_FP64) return solve_fp64(ctx, ctx_fp64_bufs(ctx), o, st);
+    if (o->method == ASEND_SOLVE_DIRECT) return solve_direct(ctx, st);
     const int64_t n = ctx->n_dof;
     int64_t max_it = o->max_it > 0 ? o->max_it : n;
     int check = o->check_every > 0 ? o->check_every : (ctx->tuned_check > 0 ? ctx->tuned_check : 16);
@@ -1631,6 +1764,54 @@ int asend_solve(asend_ctx* ctx, const double* rhs, double* x, const asend_solve_
     s.device_ms = ms;
     HIPCHK(hipMemcpy(x, ctx->d_u, n * 8, hipMemcpyDeviceToHost));
     if (stats) *stats = s;
+    return ASEND_OK;
+}
+
+int asend_ldl_factor(asend_ctx* ctx, int64_t block_dofs, asend_ldl_info* info) {
+    if (!ctx) return ASEND_ERR_ARG;
+    if (!ctx->assembled) return fail(ctx, ASEND_ERR_STATE, "assemble first");
+    const int64_t bs = (block_dofs <= 0 || block_dofs >= ctx->n_dof) ? ctx->n_dof : block_dofs;
+    if (bs != ctx->ldl_block_dofs) {
+        ldl_release(ctx);
+        ctx->ldl_block_dofs = bs;
+    }
+    int rc;
+    if ((rc = ldl_ensure(ctx))) {
+        ldl_release(ctx);
+        ctx->ldl_block_dofs = -1;
+        return rc;
+    }
+    if (info) {
+        const LdlFactor& F = ctx->ldl;
+        info->block_dofs = F.d.bs;
+        info->n_blocks = F.d.n_blocks;
+        info->max_bandwidth = F.max_bandwidth;
+        info->bytes = F.bytes;
+        info->neg_pivots = F.neg_pivots;
+        info->pos_def = F.neg_pivots == 0 ? 1 : 0;
+        info->factor_ms = F.factor_ms;
+    }
+    return ASEND_OK;
+}
+
+int asend_ldl_solve(asend_ctx* ctx, const double* rhs, double* x) {
+    if (!ctx || !rhs || !x) return ASEND_ERR_ARG;
+    int rc;
+    if ((rc = ldl_ensure(ctx))) return rc;
+    const int64_t n = ctx->n_dof;
+    HIPCHK(hipMemcpyAsync(ctx->d_r, rhs, n * 8, hipMemcpyHostToDevice, ctx->st));
+    ldl_solve_launch(ctx->st, ctx->ldl.d, ctx->ldl.L, ctx->d_r, ctx->d_ldl_z, nullptr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(x, ctx->d_ldl_z, n * 8, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    return ASEND_OK;
+}
+
+int asend_ldl_release(asend_ctx* ctx) {
+    if (!ctx) return ASEND_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    ldl_release(ctx);
+    ctx->ldl_block_dofs = -1;
     return ASEND_OK;
 }
 

@@ -1082,6 +1082,52 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(int64_t n, const T* __restri
         p[i] = dinv[i] * r[i] + beta * p[i];
 }
 
+// PCG with a preconditioner applied by its own kernels (z = M^-1 r formed
+// between these launches; the LDL^T block preconditioner): x = 0, r = b.
+__global__ __launch_bounds__(kBlock) void k_pcgz_start(int64_t n, const double* __restrict__ b,
+                                                       double* __restrict__ x, double* __restrict__ r) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        x[i] = 0.0;
+        r[i] = b[i];
+    }
+}
+
+// partials of r.z and r.r; at the start also p = z
+__global__ __launch_bounds__(kBlock) void k_pcgz_dots(int64_t n, const double* __restrict__ r,
+                                                      const double* __restrict__ z, double* __restrict__ p, int init,
+                                                      double* partials, const Scalars* sc) {
+    if (!init && sc->done) return;
+    double t[2] = {0.0, 0.0};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double ri = r[i], zi = z[i];
+        if (init) p[i] = zi;
+        t[0] += ri * zi;
+        t[1] += ri * ri;
+    }
+    block_partials<2>(t, partials);
+}
+
+// x += alpha p; r -= alpha q
+__global__ __launch_bounds__(kBlock) void k_pcgz_update(int64_t n, double* __restrict__ x, double* __restrict__ r,
+                                                        const double* __restrict__ p, const double* __restrict__ q,
+                                                        const Scalars* __restrict__ sc) {
+    if (sc->done) return;
+    const double alpha = sc->alpha;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        x[i] += alpha * p[i];
+        r[i] -= alpha * q[i];
+    }
+}
+
+// p = z + beta p
+__global__ __launch_bounds__(kBlock) void k_pcgz_dir(int64_t n, const double* __restrict__ z, double* __restrict__ p,
+                                                     const Scalars* __restrict__ sc) {
+    if (sc->done) return;
+    const double beta = sc->beta;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = z[i] + beta * p[i];
+}
+
 // Single-reduction (Chronopoulos-Gear) Jacobi PCG: one SpMV with all three
 // dot products fused (K u with r.u, (K u).u, r.r) and one vector update per
 // iteration, against an SpMV, two vector kernels and two reductions for the
@@ -1278,6 +1324,22 @@ void launch_pcg_update(hipStream_t st, int64_t n, const T* dinv, T* x, T* r, con
 template <typename T>
 void launch_pcg_dir(hipStream_t st, int64_t n, const T* dinv, const T* r, T* p, const Scalars* sc) {
     hipLaunchKernelGGL(k_pcg_dir<T>, dim3(kVecBlocks), dim3(kBlock), 0, st, n, dinv, r, p, sc);
+}
+
+void launch_pcgz_start(hipStream_t st, int64_t n, const double* b, double* x, double* r) {
+    hipLaunchKernelGGL(k_pcgz_start, dim3(kVecBlocks), dim3(kBlock), 0, st, n, b, x, r);
+}
+void launch_pcgz_dots(hipStream_t st, int64_t n, const double* r, const double* z, double* p, bool init,
+                      double* partials, Scalars* sc) {
+    hipLaunchKernelGGL(k_pcgz_dots, dim3(kVecBlocks), dim3(kBlock), 0, st, n, r, z, p, init ? 1 : 0, partials, sc);
+    finalize(st, partials, kVecBlocks, init ? FIN_INIT : FIN_UPDATE, sc);
+}
+void launch_pcgz_update(hipStream_t st, int64_t n, double* x, double* r, const double* p, const double* q,
+                        const Scalars* sc) {
+    hipLaunchKernelGGL(k_pcgz_update, dim3(kVecBlocks), dim3(kBlock), 0, st, n, x, r, p, q, sc);
+}
+void launch_pcgz_dir(hipStream_t st, int64_t n, const double* z, double* p, const Scalars* sc) {
+    hipLaunchKernelGGL(k_pcgz_dir, dim3(kVecBlocks), dim3(kBlock), 0, st, n, z, p, sc);
 }
 
 #define INST_PCG(T)                                                                                               \

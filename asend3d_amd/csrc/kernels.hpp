@@ -173,6 +173,17 @@ void launch_mixed_refresh(hipStream_t st, const MixedSolve& m);
 
 void launch_scale_to_f32(hipStream_t st, int64_t n, const double* r, float* r32, const Scalars* sc);
 void launch_axpy_from_f32(hipStream_t st, int64_t n, double* u, const float* d, const Scalars* sc_outer);
+// PCG whose preconditioner runs as separate launches (z = M^-1 r in between):
+// start x = 0, r = b; dots: r.z and r.r into the loop scalars (init: also
+// p = z, start values; else beta, iteration count and the stop test);
+// update x += alpha p, r -= alpha q; dir p = z + beta p.
+void launch_pcgz_start(hipStream_t st, int64_t n, const double* b, double* x, double* r);
+void launch_pcgz_dots(hipStream_t st, int64_t n, const double* r, const double* z, double* p, bool init,
+                      double* partials, Scalars* sc);
+void launch_pcgz_update(hipStream_t st, int64_t n, double* x, double* r, const double* p, const double* q,
+                        const Scalars* sc);
+void launch_pcgz_dir(hipStream_t st, int64_t n, const double* z, double* p, const Scalars* sc);
+
 void launch_dot64(hipStream_t st, int64_t n, const double* a, const double* b, double* partials, Scalars* sc);
 void launch_axpy64(hipStream_t st, int64_t n, double alpha, const double* x, double* y);
 

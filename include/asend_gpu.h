@@ -51,6 +51,9 @@ extern "C" {
 /* solver methods */
 #define ASEND_SOLVE_PCG_FP64 0    /* FP64 PCG on the assembled FP64 operator          */
 #define ASEND_SOLVE_IR_MIXED 1    /* FP32 flexible PCG with FP64 reliable updates */
+#define ASEND_SOLVE_DIRECT 2      /* u = (L D L^T)^-1 f with the one-block factor of
+                                     asend_ldl_factor (the reference's "direct"
+                                     solverMethod, S/model_analysis.rs:495-496) */
 
 /* preconditioners */
 #define ASEND_PC_JACOBI 0         /* point Jacobi                                       */
@@ -58,6 +61,9 @@ extern "C" {
 #define ASEND_PC_MG 2             /* geometric multigrid, Galerkin coarse operators     */
 #define ASEND_PC_LEARNED 3        /* multigrid whose fine smoother is the learned L L^T */
 #define ASEND_PC_LEARNED_PCG 4    /* the learned L L^T alone                            */
+#define ASEND_PC_LDL 5            /* the block LDL^T factor of asend_ldl_factor (the
+                                     reference's LowerTriMat preconditioner of
+                                     conj_grad_sparse); FP64 PCG only             */
 
 /* stopping rules */
 #define ASEND_STOP_REL_RESIDUAL 0 /* ||f - K u||_2 / ||f||_2 <= tol, evaluated in FP64 */
@@ -250,6 +256,38 @@ int asend_spmv(asend_ctx* ctx, int precision, const double* x, double* y);
  * and the direct ldl_solve branch of solve_step (S/model_analysis.rs:495-501). */
 int asend_solve(asend_ctx* ctx, const double* rhs, double* x, const asend_solve_opts* opts,
                 asend_solve_stats* stats);
+
+/* LDL^T factor of the assembled K_eff restricted to diagonal blocks of
+ * block_dofs consecutive DOFs (the last block may be shorter; block_dofs <= 0
+ * or >= n_dof: one block, an exact factor).  Replaces LowerTriMat
+ * allocate_from_sparse_mat / populate_from_sparse_mat / ldl_factor
+ * (S/lower_tri_mat_meth.rs:15-190) with the reference's block size
+ * 6 * solverBlockDim (S/model_analysis.rs:475-478, S/model_prep.rs:888-903):
+ * each row keeps its columns from its first nonzero inside its block, as the
+ * reference's envelope, stored as a band of 64 x 64 tiles on the device.  The
+ * factor follows the operator: a solve after a new assembly refactors it
+ * with the same blocks.  It is used by ASEND_PC_LDL and ASEND_SOLVE_DIRECT.
+ * Fails with ASEND_ERR_UNSUPPORTED when its band does not fit the device
+ * memory or the solve's window (a band of more than 240 tiles). */
+typedef struct asend_ldl_info {
+    int64_t block_dofs;     /* DOFs per block                                   */
+    int64_t n_blocks;
+    int64_t max_bandwidth;  /* the reference's maxBandwidth: widest row of the
+                               envelope, diagonal included                     */
+    int64_t bytes;          /* device storage of the factor                     */
+    int32_t neg_pivots;     /* pivots d_i < 0                                   */
+    int32_t pos_def;        /* LowerTriMat::pos_def (S/lower_tri_mat_meth.rs:229) */
+    double factor_ms;       /* fill + factorization, wall clock                 */
+} asend_ldl_info;
+int asend_ldl_factor(asend_ctx* ctx, int64_t block_dofs, asend_ldl_info* info);
+
+/* x = (L D L^T)^-1 rhs with the current factor, block by block (ldl_solve,
+ * S/lower_tri_mat_meth.rs:194-227); host arrays of n_dof doubles. */
+int asend_ldl_solve(asend_ctx* ctx, const double* rhs, double* x);
+
+/* Free the factor's device memory (a later ASEND_PC_LDL or direct solve needs
+ * asend_ldl_factor again). */
+int asend_ldl_release(asend_ctx* ctx);
 
 /* Allocation-free variant of asend_solve for ASEND_SOLVE_PCG_FP64 with
  * ASEND_PC_JACOBI (other methods: ASEND_ERR_UNSUPPORTED).  No reference

@@ -1,7 +1,10 @@
 """The job-file driver asend_run_job on the C1 job fixture (20x4x4 brick8
 cantilever, clamped, tip traction; design variables on the compliance
 parameters; two displacement powerNorm objective terms), against the oracle:
-node displacements against its exact banded LDL^T solve, reaction forces
+node displacements against its exact banded LDL^T solve (the direct job
+solves with the device's one-block factor, the iterative job runs
+conj_grad_sparse with the device's block LowerTriMat and must take the
+oracle's iteration count), reaction forces
 against K u, objective values from its displacements, and the adjoint
 gradient of every design variable from its dual-number element derivatives
 dK_e/dp (E1..3, nu12, nu13, nu23, G12, G13, G23) with lambda = K^-1 dL/du."""
@@ -14,7 +17,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from asend3d_amd import problems
+from asend3d_amd import api, problems
 from conftest import gpu_case
 
 pytestmark = pytest.mark.gpu
@@ -57,7 +60,13 @@ def oracle_c1():
     sets = {"all": np.arange(m.n_el), "5": np.array([5]), "tipEls": m.elements_at_x(m.nx - 1)}
     grad = np.array([c * part[sets[s], base + comp - 1].sum() for base, comp, s, c in DVARS])
     rf = O.spmv_penalty(K, np.zeros(0, np.int64), 0.0, u)
-    return {"p": p, "u": u, "di": di, "J": np.array(J), "grad": grad, "rf": rf}
+    # the direct job numbers the DOFs without reordering blocks (solverBlockDim 2e9)
+    clamp = m.nodes_at_x(0)
+    _, di_d, n_d = api.reorder_nodes(m.xyz, m.conn, 3, [[clamp], [clamp], [clamp]], 2000000000)
+    K_d = O.assemble_solid(8, m.xyz, m.conn, p.E, p.nu, di_d, n_d)
+    pen_d = np.concatenate([di_d[clamp, 0], di_d[clamp, 1], di_d[clamp, 2]]).astype(np.int64)
+    return {"p": p, "u": u, "di": di, "J": np.array(J), "grad": grad, "rf": rf, "K": K, "sf": sf, "f": f,
+            "K_direct": K_d, "pen_direct": pen_d}
 
 
 def run_job(tmp_path, job):
@@ -82,10 +91,24 @@ def read_csv(path):
 def test_cli_c1_matches_oracle(tmp_path, oracle_c1, job, tag, bar_u):
     gpu_case(f"asend_run_job {job}")
     work, out = run_job(tmp_path, job)
-    if tag:
-        assert "Total CG iterations: " in out and "Final residual norm: " in out
-        assert "did not converge" not in out
     o = oracle_c1
+    lines = {ln.split(":")[0]: ln.split(":", 1)[1].strip() for ln in out.splitlines() if ":" in ln}
+    if tag:
+        # conj_grad_sparse preconditioned by the LowerTriMat of 6 solverBlockDim
+        # DOFs (automatic size), absolute stop rule: the oracle's iteration count
+        assert "did not converge" not in out
+        p = o["p"]
+        bd = 6 * p.block_dim
+        ldl = O.LDL(o["K"], p.pen_dofs, o["sf"], bd)
+        _, it_ref, _ = O.conj_grad_sparse(o["K"], p.pen_dofs, o["sf"], ldl, o["f"], 1e-24, 200000)
+        assert int(lines["maxBandwidth"]) == ldl.max_bandwidth()
+        assert abs(int(lines["Total CG iterations"]) - it_ref) <= 1, (lines["Total CG iterations"], it_ref)
+        # Rust's {} for f64: plain decimal, no exponent
+        res = lines["Final residual norm"]
+        assert "e" not in res and 0.0 <= float(res) <= 1e-24
+    else:
+        ldl = O.LDL(o["K_direct"], o["pen_direct"], o["sf"], 2 ** 62)
+        assert int(lines["maxBandwidth"]) == ldl.max_bandwidth()
     head, rows = read_csv(work / f"results{tag}" / "nodeResults.csv")
     assert head == ["node", "U1", "U2", "U3", "R1", "R2", "R3", "RF1", "RF2", "RF3", "RM1", "RM2", "RM3", "time"]
     assert len(rows) == 525
