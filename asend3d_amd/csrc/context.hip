@@ -293,6 +293,7 @@ void ldl_release(asend_ctx* ctx) {
     dfree(ctx->ldl.L);
     dfree(ctx->ldl.P);
     dfree(ctx->ldl.neg);
+    dfree(ctx->ldl.work);
     dfree(ctx->d_ldl_z);
     ctx->ldl = LdlFactor();
 }
@@ -455,6 +456,7 @@ int ldl_ensure(asend_ctx* ctx) {
                         (long long)d.n_blocks, (long long)W, bytes / 1e9, free_b / 1e9);
         if ((rc = dalloc(ctx, &F.L, (size_t)d.n_blocks * d.stride)) ||
             (rc = dalloc(ctx, &F.P, (size_t)d.n_blocks * kLdlTileElems)) || (rc = dalloc(ctx, &F.neg, 1)) ||
+            (rc = dalloc(ctx, &F.work, (size_t)ctx->n_dof)) ||
             (rc = dalloc(ctx, &ctx->d_ldl_z, (size_t)ctx->n_dof))) {
             ldl_release(ctx);
             return rc;
@@ -495,13 +497,13 @@ int run_ldl_pcg64(asend_ctx* ctx, const Fp64Bufs& B, double tol, bool stop_abs, 
     const LdlFactor& F = ctx->ldl;
     double* z = ctx->d_ldl_z;
     launch_pcgz_start(ctx->st, n, B.r, B.x, B.r2);
-    ldl_solve_launch(ctx->st, F.d, F.L, B.r2, z, nullptr);
+    ldl_solve_launch(ctx->st, F.d, F.L, B.r2, z, F.work, nullptr);
     launch_pcgz_dots(ctx->st, n, B.r2, z, B.p, true, B.partials, sc);
     for (;;) {
         for (int k = 0; k < check_every; ++k) {
             launch_sell3<double>(ctx->st, 1, ctx->fine(), ctx->d_K64, B.p, B.q, nullptr, B.partials, sc);
             launch_pcgz_update(ctx->st, n, B.x, B.r2, B.p, B.q, sc);
-            ldl_solve_launch(ctx->st, F.d, F.L, B.r2, z, &sc->done);
+            ldl_solve_launch(ctx->st, F.d, F.L, B.r2, z, F.work, &sc->done);
             launch_pcgz_dots(ctx->st, n, B.r2, z, B.p, false, B.partials, sc);
             launch_pcgz_dir(ctx->st, n, z, B.p, sc);
         }
@@ -818,7 +820,7 @@ int solve_direct(asend_ctx* ctx, asend_solve_stats* st) {
     const int64_t n = ctx->n_dof;
     asend_solve_stats s;
     std::memset(&s, 0, sizeof(s));
-    ldl_solve_launch(ctx->st, ctx->ldl.d, ctx->ldl.L, ctx->d_f, ctx->d_u, nullptr);
+    ldl_solve_launch(ctx->st, ctx->ldl.d, ctx->ldl.L, ctx->d_f, ctx->d_u, ctx->ldl.work, nullptr);
     std::memset(&ctx->h_sc[0], 0, sizeof(Scalars));
     if ((rc = write_scalars(ctx, 0))) return rc;
     launch_dot64(ctx->st, n, ctx->d_f, ctx->d_f, ctx->d_partials, ctx->d_sc);
@@ -1800,7 +1802,7 @@ int asend_ldl_solve(asend_ctx* ctx, const double* rhs, double* x) {
     if ((rc = ldl_ensure(ctx))) return rc;
     const int64_t n = ctx->n_dof;
     HIPCHK(hipMemcpyAsync(ctx->d_r, rhs, n * 8, hipMemcpyHostToDevice, ctx->st));
-    ldl_solve_launch(ctx->st, ctx->ldl.d, ctx->ldl.L, ctx->d_r, ctx->d_ldl_z, nullptr);
+    ldl_solve_launch(ctx->st, ctx->ldl.d, ctx->ldl.L, ctx->d_r, ctx->d_ldl_z, ctx->ldl.work, nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(x, ctx->d_ldl_z, n * 8, hipMemcpyDeviceToHost, ctx->st));
     HIPCHK(hipStreamSynchronize(ctx->st));

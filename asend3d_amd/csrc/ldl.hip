@@ -9,7 +9,8 @@
 // LDL^T whose steps are one diagonal-tile factorization, a panel of
 // tile products with D^-1 L_kk^-1 and a trailing update of tile products,
 // all blocks of the numbering at once.  The solve streams a block's tiles
-// with one workgroup, the last wt + 1 solution tiles held in LDS.
+// with one workgroup, the last wt + 1 solution tiles held in LDS, or, for
+// few long blocks, runs one launch per tile row across workgroups.
 #include "ldl.hpp"
 
 #include <algorithm>
@@ -272,6 +273,122 @@ __global__ __launch_bounds__(kBlock) void k_ldl_solve(LdlDesc d, const double* _
     }
 }
 
+// The same solve for few, long blocks (the direct factor is one block): one
+// launch per tile row, right-looking, with the work of a step spread over the
+// tiles below the diagonal; y (work vector) starts as b.  Forward step I:
+// every workgroup k solves the unit lower diagonal tile for z_I = L_II^-1 y_I
+// itself (y_I is final: its updates came from earlier steps), workgroup 0
+// stores x_I = D_I^-1 z_I, and workgroup k subtracts L_KI z_I from y_K,
+// K = I + 1 + k.  Backward step J: y_J = L_JJ^-T x_J (x_J final), then
+// x_K -= L_JK^T y_J for K = J - 1 - k; the solution ends in y and is copied
+// to x.  No step writes what another workgroup of the same step reads, and
+// each vector entry is updated by one workgroup per step in step order, so
+// the result is deterministic.
+__device__ __forceinline__ void tri_solve_lower(const double* __restrict__ sD, double& y, int c) {
+    for (int j = 0; j < kLdlTile; ++j) {
+        const double zj = __shfl(y, j, kWave);
+        if (c > j) y -= sD[c * kLd + j] * zj;
+    }
+}
+
+__device__ __forceinline__ void tri_solve_upper(const double* __restrict__ sD, double& y, int c) {
+    for (int j = kLdlTile - 1; j >= 0; --j) {
+        const double xj = __shfl(y, j, kWave);
+        if (c < j) y -= sD[j * kLd + c] * xj;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_ldl_copy(int64_t n, const double* b, double* x, const int* done) {
+    if (done && *done) return;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = b[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_ldl_fwd_step(LdlDesc d, const double* __restrict__ L,
+                                                         double* __restrict__ y, double* __restrict__ x, int I,
+                                                         const int* __restrict__ done) {
+    if (done && *done) return;
+    const int blk = blockIdx.y;
+    const int nt = ldl_block_nt(d, blk);
+    if (I >= nt) return;
+    const int r = min(d.wt, nt - 1 - I);
+    const int k = blockIdx.x;
+    if (k > 0 && k >= r) return;
+    __shared__ double sD[kLdlTile * kLd];
+    __shared__ double sz[kLdlTile];
+    __shared__ double red[kLdlTile];
+    const int64_t s = (int64_t)blk * d.bs;
+    const int64_t m = std::min(d.bs, d.n - s);
+    const int t = threadIdx.x, w = t >> 6, c = t & 63;
+    stage_tile(sD, L + ldl_tile(d, blk, I, I));
+    __syncthreads();
+    if (w == 0) {
+        const int64_t gi = (int64_t)I * kLdlTile + c;
+        double v = gi < m ? y[s + gi] : 0.0;
+        tri_solve_lower(sD, v, c);
+        sz[c] = v;
+        if (k == 0 && gi < m) x[s + gi] = v / sD[c * kLd + c];
+    }
+    __syncthreads();
+    if (k >= r) return;
+    const int K = I + 1 + k;
+    const double* T = L + ldl_tile(d, blk, K, I);
+    const double zc = sz[c];
+    double acc[16];
+#pragma unroll
+    for (int p = 0; p < 16; ++p) acc[p] = T[(w + 4 * p) * kLdlTile + c] * zc;
+#pragma unroll
+    for (int p = 0; p < 16; ++p) {
+        const double v = wave_sum(acc[p]);
+        if (c == 0) red[w + 4 * p] = v;
+    }
+    __syncthreads();
+    if (t < kLdlTile) {
+        const int64_t gk = (int64_t)K * kLdlTile + t;
+        if (gk < m) y[s + gk] -= red[t];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_ldl_bwd_step(LdlDesc d, const double* __restrict__ L,
+                                                         double* __restrict__ x, double* __restrict__ y, int J,
+                                                         const int* __restrict__ done) {
+    if (done && *done) return;
+    const int blk = blockIdx.y;
+    const int nt = ldl_block_nt(d, blk);
+    if (J >= nt) return;
+    const int r = min(d.wt, J);
+    const int k = blockIdx.x;
+    if (k > 0 && k >= r) return;
+    __shared__ double sD[kLdlTile * kLd];
+    __shared__ double sx[kLdlTile];
+    __shared__ double red[4 * kLdlTile];
+    const int64_t s = (int64_t)blk * d.bs;
+    const int64_t m = std::min(d.bs, d.n - s);
+    const int t = threadIdx.x, w = t >> 6, c = t & 63;
+    stage_tile(sD, L + ldl_tile(d, blk, J, J));
+    __syncthreads();
+    if (w == 0) {
+        const int64_t gj = (int64_t)J * kLdlTile + c;
+        double v = gj < m ? x[s + gj] : 0.0;
+        tri_solve_upper(sD, v, c);
+        sx[c] = v;
+        if (k == 0 && gj < m) y[s + gj] = v;
+    }
+    __syncthreads();
+    if (k >= r) return;
+    const int K = J - 1 - k;
+    const double* T = L + ldl_tile(d, blk, J, K);
+    double acc = 0.0;
+#pragma unroll
+    for (int p = 0; p < 16; ++p) acc = fma(T[(w + 4 * p) * kLdlTile + c], sx[w + 4 * p], acc);
+    red[w * kLdlTile + c] = acc;
+    __syncthreads();
+    if (t < kLdlTile) {
+        const double v = red[t] + red[kLdlTile + t] + red[2 * kLdlTile + t] + red[3 * kLdlTile + t];
+        x[s + (int64_t)K * kLdlTile + t] -= v;  // K < J: a full tile
+    }
+}
+
 size_t solve_lds_bytes(const LdlDesc& d) {
     return ((size_t)(d.wt + 1) * kLdlTile + (size_t)kLdlTile * kLd + 4 * kLdlTile) * sizeof(double);
 }
@@ -330,8 +447,18 @@ void ldl_factor_launch(hipStream_t st, const LdlDesc& d, double* L, double* P, i
     }
 }
 
-void ldl_solve_launch(hipStream_t st, const LdlDesc& d, const double* L, const double* b, double* x,
+void ldl_solve_launch(hipStream_t st, const LdlDesc& d, const double* L, const double* b, double* x, double* work,
                       const int* done) {
+    if (d.n_blocks < kLdlStepBlocks) {
+        hipLaunchKernelGGL(k_ldl_copy, dim3(1024), dim3(kBlock), 0, st, d.n, b, work, done);
+        const int gx = std::max(1, d.wt);
+        for (int I = 0; I < d.nt_full; ++I)
+            hipLaunchKernelGGL(k_ldl_fwd_step, dim3(gx, d.n_blocks), dim3(kBlock), 0, st, d, L, work, x, I, done);
+        for (int J = d.nt_full - 1; J >= 0; --J)
+            hipLaunchKernelGGL(k_ldl_bwd_step, dim3(gx, d.n_blocks), dim3(kBlock), 0, st, d, L, x, work, J, done);
+        hipLaunchKernelGGL(k_ldl_copy, dim3(1024), dim3(kBlock), 0, st, d.n, work, x, done);
+        return;
+    }
     static bool attr_set = false;
     const size_t lds = solve_lds_bytes(d);
     if (!attr_set) {

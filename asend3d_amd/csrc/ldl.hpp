@@ -14,6 +14,10 @@ constexpr int kLdlTileElems = kLdlTile * kLdlTile;
 // Most tiles a tile row may hold below the diagonal: the solve keeps the last
 // kLdlMaxWt + 1 solution tiles of its block in LDS (64 doubles each).
 constexpr int kLdlMaxWt = 239;
+// Below this many blocks the solve runs one launch per tile row with the
+// tiles of a row spread over workgroups; from it on, one workgroup streams
+// each block (enough blocks to fill the chip).
+constexpr int kLdlStepBlocks = 64;
 
 // Banded tile storage of the factor.  Block b covers DOFs [b bs, min((b+1) bs,
 // n)); inside a block, tile (I, J) (rows 64 I.., columns 64 J.., I - wt <= J
@@ -53,6 +57,7 @@ struct LdlFactor {
     double* L = nullptr;        // banded tiles
     double* P = nullptr;        // per block: D^-1 L_kk^-1 of the current step (panel scratch)
     int32_t* neg = nullptr;     // count of negative pivots (pos_def false when > 0)
+    double* work = nullptr;     // n doubles: the solve's work vector
     int64_t asm_id = -1;        // assembly the factor was computed from
     int64_t bytes = 0;
     double factor_ms = 0.0;
@@ -72,9 +77,10 @@ void ldl_fill(hipStream_t st, const LdlDesc& d, int n_rows, int n_slices, const 
               const double* K, double* L);
 void ldl_factor_launch(hipStream_t st, const LdlDesc& d, double* L, double* P, int32_t* neg);
 
-// x = (L D L^T)^-1 b over every block (ldl_solve).  x may alias b.  When
-// done is given and *done is set the launch does nothing (solver loops).
-void ldl_solve_launch(hipStream_t st, const LdlDesc& d, const double* L, const double* b, double* x,
+// x = (L D L^T)^-1 b over every block (ldl_solve).  x may alias b; work is
+// n doubles of scratch (distinct from both).  When done is given and *done
+// is set the launches do nothing (solver loops).
+void ldl_solve_launch(hipStream_t st, const LdlDesc& d, const double* L, const double* b, double* x, double* work,
                       const int* done);
 
 }  // namespace asend

@@ -62,7 +62,7 @@ def config1(etype=8):
 
 
 def heterogeneous(nx, ny, nz, seed=C2_SEED, median=70e3, sigma_ln=1.0, nu=0.3, lx=1000.0, ly=200.0, lz=200.0,
-                  traction=(0.0, 0.0, -1.0)):
+                  traction=(0.0, 0.0, -1.0), block_dim=None):
     """Lognormal per-element moduli, posed in mm-N-MPa (median 70e3 MPa =
     70 GPa, traction 1 MPa).  In these units 1/E stays far above the 1e-12
     pivot clamp of the reference's QR inverse (S/matrix_functions.rs:189-191),
@@ -71,7 +71,7 @@ def heterogeneous(nx, ny, nz, seed=C2_SEED, median=70e3, sigma_ln=1.0, nu=0.3, l
     refuses such moduli (api.compliance_clamp_check)."""
     n_el = nx * ny * nz
     E = api.lognormal_moduli(seed, n_el, median, sigma_ln)
-    return cantilever(nx, ny, nz, lx, ly, lz, E=E, nu=nu, traction=traction)
+    return cantilever(nx, ny, nz, lx, ly, lz, E=E, nu=nu, traction=traction, block_dim=block_dim)
 
 
 def distorted(p, seed, amp=0.2, nu_range=(0.2, 0.4)):

@@ -25,6 +25,11 @@
               Newton solves (Eisenstat-Walker forcing).
   general  -- assembly of the 200x40x40 heterogeneous block on a distorted
               mesh (per-element integration and row gather).
+  reference -- the reference's own solvers on the device for the 200x40x40
+              heterogeneous block: conj_grad_sparse preconditioned by the
+              block LowerTriMat (blocks of 6 x solverBlockDim DOFs) to a 1e-8
+              relative residual, and the direct solver (one LDL^T factor of
+              the whole operator, numbering without reordering blocks).
 
     python -m asend3d_amd.workloads batched [--variants 64] [--sample 2]
     python -m asend3d_amd.workloads newmark [--nx 100 --ny 20 --nz 20 --steps 1000]
@@ -33,6 +38,7 @@
     python -m asend3d_amd.workloads modal [--modes 10]
     python -m asend3d_amd.workloads nonlinear [--nx 200 --ny 40 --nz 40 --load-steps 5]
     python -m asend3d_amd.workloads general [--nx 200 --ny 40 --nz 40]
+    python -m asend3d_amd.workloads reference [--nx 200 --ny 40 --nz 40] [--no-direct]
 
 Each prints one JSON line.
 """
@@ -267,9 +273,54 @@ def shell(nx=256, ny=256, steps=1000, dt=1e-5, graphs=True, tol=1e-8, max_it=200
             "centre_w_min_step": int(np.argmin(w))}
 
 
+def reference(nx=200, ny=40, nz=40, tol=1e-8, direct=True, graphs=True):
+    out = {"workload": "reference", "mesh": [nx, ny, nz], "tol": tol}
+    p = problems.heterogeneous(nx, ny, nz)
+    ctx = api.Context(0)
+    ctx.set_graphs(graphs)
+    problems.upload(ctx, p)
+    ctx.assemble()
+    log(f"{p.n_dof} DOF, solverBlockDim {p.block_dim}")
+    o_fast = api.Context.opts(api.SOLVE_IR_MIXED, api.PC_MG, tol=tol, inner_tol=1e-2)
+    ctx.set_grid(nx, ny, nz)
+    u_fast, st_fast = ctx.solve(p.f, o_fast)
+    t0 = time.perf_counter()
+    info = ctx.ldl_factor(6 * p.block_dim)
+    t_fac = time.perf_counter() - t0
+    log(f"block factor: {info}")
+    o = api.Context.opts(api.SOLVE_PCG_FP64, api.PC_LDL, tol=tol, max_it=100000, check_every=16)
+    u, st = ctx.solve(p.f, o)
+    log(f"block LDL^T PCG: {st.iterations} iterations, {st.device_ms / 1e3:.3f} s, residual {st.rel_residual:.2e}")
+    out["block"] = {"ldl": info, "factor_call_s": t_fac, "iterations": int(st.iterations),
+                    "solve_s": st.device_ms / 1e3, "rel_residual": st.rel_residual,
+                    "max_rel_diff_vs_multigrid": float(np.abs(u - u_fast).max() / np.abs(u_fast).max()),
+                    "multigrid_mixed_s": st_fast.device_ms / 1e3}
+    ctx.ldl_release()
+    ctx.close()
+    if direct:
+        pd = problems.heterogeneous(nx, ny, nz, block_dim=2000000000)
+        c2 = api.Context(0)
+        problems.upload(c2, pd)
+        c2.assemble()
+        t0 = time.perf_counter()
+        info2 = c2.ldl_factor(0)
+        t_fac2 = time.perf_counter() - t0
+        log(f"direct factor: {info2}")
+        u2, st2 = c2.solve(pd.f, api.Context.opts(api.SOLVE_DIRECT, api.PC_JACOBI))
+        log(f"direct solve: {st2.device_ms / 1e3:.3f} s, residual {st2.rel_residual:.2e}")
+        di_b = np.asarray(p.dof_index).reshape(-1, 6)[:, :3]
+        di_d = np.asarray(pd.dof_index).reshape(-1, 6)[:, :3]
+        out["direct"] = {"ldl": info2, "factor_call_s": t_fac2, "solve_s": st2.device_ms / 1e3,
+                         "rel_residual": st2.rel_residual,
+                         "max_rel_diff_vs_block_pcg": float(np.abs(u2[di_d] - u[di_b]).max() / np.abs(u[di_b]).max())}
+        c2.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("which", choices=["batched", "newmark", "shell", "adjoint", "modal", "nonlinear", "general"])
+    ap.add_argument("which", choices=["batched", "newmark", "shell", "adjoint", "modal", "nonlinear", "general",
+                                          "reference"])
     ap.add_argument("--load-steps", type=int, default=5)
     ap.add_argument("--modes", type=int, default=10)
     ap.add_argument("--variants", type=int, default=64)
@@ -279,6 +330,7 @@ def main():
     ap.add_argument("--nz", type=int, default=None, help="default 20 (newmark), 40 (nonlinear)")
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--no-graphs", action="store_true")
+    ap.add_argument("--no-direct", action="store_true", help="reference: skip the direct solver")
     ap.add_argument("--solver", choices=["auto", "mixed", "fp64"], default="auto", help="Newmark step solver")
     ap.add_argument("--ru-factor", type=float, default=1e-3,
                     help="mixed step solver: residual reduction between FP64 reliable updates")
@@ -287,6 +339,8 @@ def main():
         out = batched(a.variants, a.sample)
     elif a.which == "modal":
         out = modal(a.modes)
+    elif a.which == "reference":
+        out = reference(a.nx or 200, a.ny or 40, a.nz or 40, direct=not a.no_direct, graphs=not a.no_graphs)
     elif a.which == "general":
         out = general(a.nx or 200, a.ny or 40, a.nz or 40)
     elif a.which == "nonlinear":
