@@ -55,12 +55,12 @@ def measured_traffic(key):
 
 # Iterations of the reference's conj_grad_sparse with its block-banded LDL^T
 # preconditioner (blocks of 6 x 582 = 3,492 DOFs) to a 1e-8 relative residual
-# on the 200x40x40 block: not timed here (hours of serial CPU time), but
-# extrapolated from the oracle's counts on 25- and 50-element-long slices of
-# the same cross-section and block size (247 and 395 iterations, 158 s and
-# 502 s of CPU time) by the power law through those two points,
-# its ~ nx^0.68 (the lower of that and a linear extrapolation, which gives
-# 1,283 at nx = 200).
+# on the 200x40x40 block: counted by running that algorithm on the device over
+# the full workload (reference_algorithm_on_device; 1,203).  Without that
+# run (--no-reference) the count is extrapolated from the oracle's counts on
+# 25- and 50-element-long slices of the same cross-section and block size
+# (247 and 395 iterations) by the power law through those two points,
+# its ~ nx^0.68, which gives 1,010 at nx = 200.
 REF_ITS_25, REF_ITS_50 = 247, 395
 
 
@@ -68,7 +68,21 @@ def reference_iterations(nx):
     return REF_ITS_25 * (nx / 25.0) ** (math.log(REF_ITS_50 / REF_ITS_25) / math.log(2.0))
 
 
-def cpu_baseline(prob, u, fp64_iters, nx, ny, nz):
+def reference_algorithm_on_device(ctx, prob, tol):
+    """The reference's own solver run on the device over the full workload:
+    conj_grad_sparse preconditioned by the block LowerTriMat (blocks of
+    6 x solverBlockDim DOFs), stopped at the metric's relative residual.  Its
+    iteration count is the one the reference's serial run would take (the
+    device restatement takes the oracle's iteration counts on small meshes)."""
+    info = ctx.ldl_factor(6 * prob.block_dim)
+    _, st = ctx.solve(prob.f, ctx.opts(api.SOLVE_PCG_FP64, api.PC_LDL, tol=tol, max_it=100000, check_every=16))
+    ctx.ldl_release()
+    return {"iterations": int(st.iterations), "factor_s": info["factor_ms"] / 1e3, "pcg_s": st.device_ms / 1e3,
+            "rel_residual": st.rel_residual, "block_dofs": int(info["block_dofs"]),
+            "factor_bytes": int(info["bytes"]), "max_bandwidth": int(info["max_bandwidth"])}
+
+
+def cpu_baseline(prob, u, fp64_iters, nx, ny, nz, ref_its=None):
     """The reference's own algorithm, restated by the oracle and run serially
     as the reference runs, timed on bounded samples of the workload and
     extrapolated (each scaling stated in `sample`):
@@ -76,7 +90,9 @@ def cpu_baseline(prob, u, fp64_iters, nx, ny, nz):
       - block-banded LDL^T factor (blocks of 6 x solver_block_dim DOFs, the
         workload's size) on that slice, x block ratio;
       - conj_grad_sparse iteration (SpMV, penalty, block solves) on that slice,
-        x DOF ratio, x the iteration count of reference_iterations();
+        x DOF ratio, x the iteration count of the same preconditioned PCG run
+        on the device over the full workload (ref_its; reference_iterations()
+        when that run is skipped);
       - the design-variable loop (per-element modulus DVs) on the full
         workload: 300 DVs timed, x N_el / 300.
     The Jacobi PCG of the device baseline, restated on the same slice, is
@@ -120,7 +136,10 @@ def cpu_baseline(prob, u, fp64_iters, nx, ny, nz):
     r_dof = prob.n_dof / sl.n_dof
     blocks_full = -(-prob.n_dof // block_dofs)
     blocks_sl = -(-sl.n_dof // block_dofs)
-    its = reference_iterations(nx)
+    its = ref_its if ref_its else reference_iterations(nx)
+    its_note = (f"the count of the same block-LDL^T PCG run on the device over the full workload to the same "
+                f"residual" if ref_its else
+                f"power law through {REF_ITS_25} and {REF_ITS_50} measured on 25- and 50-long slices")
     est = t_asm * r_el + t_fac * blocks_full / blocks_sl + t_it * r_dof * its + t_dv * prob.mesh.n_el
     est_jac = t_asm * r_el + t_jac * r_dof * fp64_iters + t_dv * prob.mesh.n_el
     return {"value": est, "unit": "s", "cores": 1, "kind": "port",
@@ -128,7 +147,7 @@ def cpu_baseline(prob, u, fp64_iters, nx, ny, nz):
                        f"slice ({sl.n_dof} DOF) x{r_el:.0f}; block-banded LDL^T ({block_dofs}-DOF blocks) factor "
                        f"{t_fac:.2f} s for {blocks_sl} blocks x{blocks_full / blocks_sl:.1f}; conj_grad_sparse "
                        f"{t_it:.3f} s per iteration on the slice x{r_dof:.1f} DOF ratio x {its:.0f} iterations "
-                       f"(power law through {REF_ITS_25} and {REF_ITS_50} measured on 25- and 50-long slices); "
+                       f"({its_note}); "
                        f"design-variable loop {t_dv * 1e3:.2f} ms per DV (300 DVs timed "
                        f"on the full mesh) x {prob.mesh.n_el} DVs; host {cpu_model}, {os.cpu_count()} hardware "
                        f"threads; with Jacobi PCG instead ({fp64_iters} iterations, {t_jac * 1e3:.1f} ms each on the "
@@ -172,6 +191,8 @@ def main():
     ap.add_argument("--nz", type=int, default=40)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-reference", action="store_true",
+                    help="skip the device run of the reference's block-LDL^T PCG (iteration count extrapolated)")
     ap.add_argument("--fast-precond", default="learned", choices=["jacobi", "mg", "learned"])
     ap.add_argument("--train-meshes", type=int, default=8)
     ap.add_argument("--train-steps", type=int, default=250)
@@ -290,8 +311,13 @@ def main():
         f"copy {copy_gbs:.0f} GB/s")
 
     cpu = None
+    ref_alg = None
     if not args.no_cpu:
-        cpu = cpu_baseline(prob, u, stb.iterations, args.nx, args.ny, args.nz)
+        if not args.no_reference:
+            ref_alg = reference_algorithm_on_device(ctx, prob, args.tol)
+            log(f"reference algorithm on the device: {ref_alg}")
+        cpu = cpu_baseline(prob, u, stb.iterations, args.nx, args.ny, args.nz,
+                           ref_its=ref_alg["iterations"] if ref_alg else None)
         log(f"cpu baseline estimate {cpu['value']:.1f} s")
 
     out = {
@@ -338,6 +364,7 @@ def main():
         "copy_achievable_gbs": copy_gbs,
         "spmv32_frac_of_copy": gbs32 / copy_gbs,
         "cpu_baseline": cpu,
+        "reference_algorithm_gpu": ref_alg,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
