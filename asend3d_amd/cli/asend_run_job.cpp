@@ -1,5 +1,5 @@
 // asend_run_job <job.yaml>: the reference's solver executable
-// (S/main.rs:55-70) for static elastic jobs, running the solve path through
+// (S/main.rs:55-70) for linear elastic jobs, running the solve path through
 // the C ABI of libasend_gpu.so on the GPU.
 //
 // Job, model, constraint, load, design-variable and objective files are read
@@ -14,21 +14,30 @@
 //   - displacement constraints with one term of coefficient +-1 and rhs 0
 //     (clamps, applied by the penalty method);
 //   - loads nodalForce, and for solid elements surfaceTraction,
-//     surfacePressure, bodyForce, gravitational and centrifugal;
-//   - solve: static, elastic, linear; solverMethod direct (default) or
-//     iterative (reference stop rule on the device's Jacobi PCG,
-//     convergenceTol, maxIterations; maxIterations 0, which makes the
-//     reference return its zero start vector, means no limit here);
-//   - objective terms of category displacement with operator powerNorm;
+//     surfacePressure, bodyForce, gravitational and centrifugal, each with
+//     its activeTime window;
+//   - solve, linear elastic: static (one solve per staticLoadTime) with
+//     solverMethod direct (default: the device LDL^T factor of the whole
+//     operator) or iterative (conj_grad_sparse with the device block LDL^T
+//     preconditioner and the reference stop rule, convergenceTol,
+//     maxIterations, 0 meaning the DOF count as in the reference); or
+//     dynamic (Newmark with newmarkBeta, newmarkGamma, timeStep, simPeriod,
+//     rayDampCM, rayDampCK; consistent mass), each step solved on the device;
+//     saveSolnHist writes the reference's solnTStep<i>.out state files into
+//     solnHistDir;
+//   - objective terms of category displacement with operator powerNorm
+//     (static solves);
 //   - design variables of category modulus, poissonRatio, shearModulus
 //     (brick8) for calcObjGradient;
-//   - writeNodeResults (displacement, reactionForce) and writeObjective
-//     (with the _grad file), numbers formatted as the reference's {:.12e}.
+//   - writeNodeResults (displacement, velocity, acceleration, reactionForce;
+//     timeSteps read back from the state files) and writeObjective (with
+//     the _grad file), numbers formatted as the reference's {:.12e}.
 // Anything else stops the job with an error message, as the reference's
 // panics do.
 #include <array>
 #include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -190,6 +199,8 @@ struct Load {
     double norm_tol = 0.0;
     double center[3] = {0, 0, 0}, axis[3] = {0, 0, 0};
     double ang_vel = 0.0;
+    double active[2] = {0.0, 1.0e100};  // activeTime (Load::active_time, S/load.rs:25)
+    std::vector<double> vec;            // assembled load vector (reference DOF numbering)
 };
 struct DesignVar {
     std::string category, el_set, nd_set;
@@ -209,6 +220,10 @@ struct Command {
     bool elastic = true, dynamic = false, nonlinear = false, thermal = false, write_grad = true;
     long block_dim = -1, max_it = 0;
     double conv_tol = 1.0e-12;
+    // dynamic and history keys (JobCommand defaults, S/job.rs:55-110)
+    double time_step = 1.0, beta = 0.25, gamma = 0.5, sim_period = 1.0, ray_cm = 0.0, ray_ck = 0.0;
+    bool save_hist = true, lump_mass = false;
+    std::vector<double> load_times;
     std::vector<std::string> fields, include;
     std::vector<long> time_steps;
 };
@@ -239,6 +254,8 @@ struct Job {
     std::vector<double> E, nu, rho, thk;
     std::vector<int64_t> pen;
     std::vector<double> u;    // solution, reference DOF numbering
+    std::vector<double> v, a; // velocity and acceleration (dynamic solves; zero for static)
+    bool dyn_solved = false;  // the last solve was dynamic
     std::vector<double> f;    // applied load
     std::vector<double> dLdD;
     bool solved = false, have_grad = false;
@@ -297,7 +314,24 @@ struct Job {
             else if (k == "fields" && d.size() == 1) c.fields.push_back(d[0]);
             else if (k == "include" && d.size() == 1) c.include.push_back(d[0]);
             else if (k == "writeGradient" && d.size() == 1) c.write_grad = (d[0] == "yes");
-            else if (k == "timeSteps" && !d.empty() && d[0] != "all" && d[0] != "last") c.time_steps.push_back(0);
+            else if (k == "timeStep" && d.size() == 1) c.time_step = to_d(d[0]);
+            else if (k == "newmarkBeta" && d.size() == 1) c.beta = to_d(d[0]);
+            else if (k == "newmarkGamma" && d.size() == 1) c.gamma = to_d(d[0]);
+            else if (k == "simPeriod" && d.size() == 1) c.sim_period = to_d(d[0]);
+            else if (k == "rayDampCM" && d.size() == 1) c.ray_cm = to_d(d[0]);
+            else if (k == "rayDampCK" && d.size() == 1) c.ray_ck = to_d(d[0]);
+            else if (k == "saveSolnHist" && d.size() == 1) c.save_hist = d[0].find("yes") != std::string::npos;
+            else if (k == "solnHistDir" && d.size() == 1) c.file = d[0];
+            else if (k == "lumpMass" && d.size() == 1) c.lump_mass = d[0].find("yes") != std::string::npos;
+            else if (k == "staticLoadTime" && !d.empty())
+                for (const auto& x : d) c.load_times.push_back(to_d(x));
+            else if (k == "timeSteps" && d.size() == 1 && d[0] != "all" && d[0] != "last") {
+                // "all" and "last" are read by the reference but never used
+                c.time_steps.push_back(to_i(d[0]));
+            } else if (k == "timeSteps" && d.size() > 1) {
+                const long a = to_i(d[0]), b = to_i(d[1]), st = d.size() == 3 ? to_i(d[2]) : 1;
+                for (long i = a; i < b; i += st) c.time_steps.push_back(i);
+            }
         }
     }
 
@@ -494,6 +528,9 @@ struct Job {
                 for (int i = 0; i < 3; ++i) loads.back().axis[i] = to_d(d[i]);
             } else if (h[1] == "angularVelocity" && !d.empty()) {
                 loads.back().ang_vel = to_d(d[0]);
+            } else if (h[1] == "activeTime" && !d.empty()) {
+                loads.back().active[0] = to_d(d[0]);
+                loads.back().active[1] = d.size() == 2 ? to_d(d[1]) : 1.0e100;
             }
         }
     }
@@ -692,23 +729,25 @@ struct Job {
               "asend_upload_mesh");
         if (etype == ASEND_SHELL4) check(asend_set_thickness(ctx, thk.data()), "asend_set_thickness");
         check(asend_set_penalty(ctx, (int64_t)pen.size(), pen.data(), 0.0), "asend_set_penalty");
-        // applied loads (build_elastic_app_load)
-        f.assign(n_dof, 0.0);
-        for (const auto& l : loads) {
+        // applied loads (build_elastic_app_load): one vector per load, summed
+        // over the loads active at a time by loads_at()
+        for (auto& l : loads) {
+            l.vec.assign(n_dof, 0.0);
+            double* lf = l.vec.data();
             if (l.type == "nodalForce") {
                 for (int32_t n : node_set(l.node_set, false))
-                    for (int c = 0; c < dpn; ++c) f[dof_index[6 * n + c]] += l.load[c];
+                    for (int c = 0; c < dpn; ++c) lf[dof_index[6 * n + c]] += l.load[c];
             } else if (l.type == "surfaceTraction" || l.type == "surfacePressure") {
                 if (etype == ASEND_SHELL4) die("%s on shell elements is not supported", l.type.c_str());
                 const auto& set = element_set(l.el_set);
                 if (l.type == "surfaceTraction")
                     rc = asend_surface_traction(etype, (int64_t)nn, xyz.data(), (int64_t)ne, cflat.data(),
                                                 dof_index.data(), (int64_t)set.size(), set.data(), l.load,
-                                                l.norm_dir, l.norm_tol, n_dof, f.data());
+                                                l.norm_dir, l.norm_tol, n_dof, lf);
                 else
                     rc = asend_surface_pressure(etype, (int64_t)nn, xyz.data(), (int64_t)ne, cflat.data(),
                                                 dof_index.data(), (int64_t)set.size(), set.data(), l.load[0],
-                                                l.norm_dir, l.norm_tol, n_dof, f.data());
+                                                l.norm_dir, l.norm_tol, n_dof, lf);
                 if (rc != ASEND_OK) die("%s load failed (%d)", l.type.c_str(), rc);
             } else if (l.type == "bodyForce" || l.type == "gravitational" || l.type == "centrifugal") {
                 if (etype == ASEND_SHELL4) die("%s on shell elements is not supported", l.type.c_str());
@@ -716,12 +755,83 @@ struct Job {
                 const int kind = l.type == "bodyForce" ? 0 : l.type == "gravitational" ? 1 : 2;
                 rc = asend_body_load(etype, kind, (int64_t)nn, xyz.data(), (int64_t)ne, cflat.data(), dof_index.data(),
                                      (int64_t)set.size(), set.data(), rho.data(), l.load, l.center, l.axis, l.ang_vel,
-                                     n_dof, f.data());
+                                     n_dof, lf);
                 if (rc != ASEND_OK) die("%s load failed (%d)", l.type.c_str(), rc);
             } else {
                 die("loads of type %s are not supported", l.type.c_str());
             }
         }
+        loads_at(0.0, f);
+    }
+
+    // sum of the loads whose activeTime holds t (build_elastic_app_load's
+    // time >= active_time[0] && time <= active_time[1])
+    void loads_at(double t, std::vector<double>& out) const {
+        out.assign(n_dof, 0.0);
+        for (const auto& l : loads)
+            if (t >= l.active[0] && t <= l.active[1])
+                for (int64_t i = 0; i < n_dof; ++i) out[i] += l.vec[i];
+    }
+
+    // ------------------------------------------------------------ state files
+    // solnTStep<step>.out in the history directory (write_time_step_soln,
+    // S/model_output.rs:14-82): per node label, its num_dof displacements,
+    // velocities and accelerations as big-endian doubles; elements with
+    // internal DOFs would add theirs (brickIM, refused here).
+    std::string state_path(long step) const {
+        return solve_cmd.file + "/solnTStep" + std::to_string(step) + ".out";
+    }
+
+    static void put_be(std::FILE* fp, double v) {
+        uint64_t b;
+        std::memcpy(&b, &v, 8);
+        unsigned char o[8];
+        for (int i = 0; i < 8; ++i) o[i] = (unsigned char)(b >> (8 * (7 - i)));
+        std::fwrite(o, 1, 8, fp);
+    }
+
+    static bool get_be(std::FILE* fp, double* v) {
+        unsigned char o[8];
+        if (std::fread(o, 1, 8, fp) != 8) return false;
+        uint64_t b = 0;
+        for (int i = 0; i < 8; ++i) b = (b << 8) | o[i];
+        std::memcpy(v, &b, 8);
+        return true;
+    }
+
+    void write_state(long step, const std::vector<double>& su, const std::vector<double>& sv,
+                     const std::vector<double>& sa) {
+        if (etype == ASEND_BRICKIM) die("solution history of brickIM models (internal DOFs) is not supported");
+        const std::string path = state_path(step);
+        std::FILE* fp = std::fopen(path.c_str(), "wb");
+        if (!fp) die("couldn't open file, %s", path.c_str());
+        const size_t nn = xyz.size() / 3;
+        for (size_t n = 0; n < nn; ++n)
+            for (const auto* vec : {&su, &sv, &sa})
+                for (int c = 0; c < dpn; ++c) {
+                    const int64_t d = dof_index[6 * n + c];
+                    put_be(fp, d >= 0 ? (*vec)[d] : 0.0);
+                }
+        std::fclose(fp);
+    }
+
+    void read_state(long step, std::vector<double>& su, std::vector<double>& sv, std::vector<double>& sa) {
+        const std::string path = state_path(step);
+        std::FILE* fp = std::fopen(path.c_str(), "rb");
+        if (!fp) die("couldn't open file, %s", path.c_str());
+        su.assign(n_dof, 0.0);
+        sv.assign(n_dof, 0.0);
+        sa.assign(n_dof, 0.0);
+        const size_t nn = xyz.size() / 3;
+        for (size_t n = 0; n < nn; ++n)
+            for (auto* vec : {&su, &sv, &sa})
+                for (int c = 0; c < dpn; ++c) {
+                    double x;
+                    if (!get_be(fp, &x)) die("problem reading file, %s", path.c_str());
+                    const int64_t d = dof_index[6 * n + c];
+                    if (d >= 0) (*vec)[d] = x;
+                }
+        std::fclose(fp);
     }
 
     // K x = b through the device as solve_step (S/model_analysis.rs:470-501):
@@ -786,20 +896,100 @@ struct Job {
         }
     }
 
+    // Model::solve (S/model_analysis.rs:560-669), linear elastic: a static
+    // solve per staticLoadTime with the loads active at that time, or the
+    // Newmark loop over [0, simPeriod) with the loads active at the start of
+    // each step; with saveSolnHist the state after each load time / step
+    // goes to solnTStep<i>.out (the dynamic loop writes the initial state as
+    // step 0 first).
     void solve(const Command& c) {
-        if (c.dynamic || c.nonlinear || c.thermal || !c.elastic)
-            die("only static, linear, elastic solves are supported by this solver");
+        if (c.nonlinear || c.thermal || !c.elastic)
+            die("only linear elastic solves (static or dynamic) are supported by this solver");
         solve_cmd = c;
-        analysis_prep(c);
-        check(asend_assemble(ctx, 0.0), "asend_assemble");
-        device_solve(f, u, true);
+        if (solve_cmd.load_times.empty()) solve_cmd.load_times.push_back(0.0);
+        analysis_prep(solve_cmd);
+        v.assign(n_dof, 0.0);
+        a.assign(n_dof, 0.0);
+        if (c.dynamic) {
+            solve_dynamic();
+        } else {
+            check(asend_assemble(ctx, 0.0), "asend_assemble");
+            for (size_t i = 0; i < solve_cmd.load_times.size(); ++i) {
+                loads_at(solve_cmd.load_times[i], f);
+                device_solve(f, u, true);
+                if (solve_cmd.save_hist) write_state((long)i, u, v, a);
+            }
+        }
         solved = true;
+        dyn_solved = c.dynamic;
+    }
+
+    void solve_dynamic() {
+        const Command& c = solve_cmd;
+        if (c.lump_mass) die("lumpMass is not supported by this solver");
+        if (!(c.time_step > 0.0)) die("dynamic solves need timeStep > 0");
+        check(asend_set_rayleigh_damping(ctx, c.ray_cm, c.ray_ck), "asend_set_rayleigh_damping");
+        check(asend_newmark_setup(ctx, c.time_step, c.beta, c.gamma), "asend_newmark_setup");
+        u.assign(n_dof, 0.0);
+        if (c.save_hist) write_state(0, u, v, a);
+        std::vector<double> u1(n_dof), v1(n_dof), a1(n_dof), fs;
+        const double one = 1.0;
+        double t = 0.0;
+        long step = 1;
+        int64_t its = 0;
+        while (t < c.sim_period) {
+            loads_at(t, fs);
+            asend_newmark_stats st;
+            std::memset(&st, 0, sizeof st);
+            check(asend_newmark_run(ctx, 1, fs.data(), &one, u.data(), v.data(), a.data(), 0, nullptr, nullptr,
+                                    u1.data(), v1.data(), a1.data(), 1e-10, 200000, &st),
+                  "asend_newmark_run");
+            if (st.unconverged_steps) std::printf("Warning: time step %ld: solve did not converge\n", step);
+            its += st.total_iterations;
+            u.swap(u1);
+            v.swap(v1);
+            a.swap(a1);
+            if (c.save_hist) {
+                write_state(step, u, v, a);
+                ++step;
+            }
+            t += c.time_step;
+        }
+        f = fs;
+        std::printf("device Newmark: %lld solver iterations\n", (long long)its);
+    }
+
+    // reaction forces: the element residual of build_elastic_soln_load(false)
+    // negated (S/model_output.rs:206-216): K u, plus M a + D v for a dynamic
+    // solve (D = rayDampCM M + rayDampCK K), without the penalty
+    std::vector<double> reaction() {
+        std::vector<double> rf(n_dof, 0.0);
+        const double sf = asend_penalty_scale(ctx);
+        if (!dyn_solved) {
+            check(asend_spmv(ctx, 64, u.data(), rf.data()), "asend_spmv");
+            for (int64_t d : pen) rf[d] -= sf * u[d];
+            return rf;
+        }
+        std::vector<double> x1(n_dof), x2(n_dof), y(n_dof), y2(n_dof);
+        for (int64_t i = 0; i < n_dof; ++i) {
+            x1[i] = u[i] + solve_cmd.ray_ck * v[i];
+            x2[i] = a[i] + solve_cmd.ray_cm * v[i];
+        }
+        check(asend_assemble(ctx, 0.0), "asend_assemble");
+        check(asend_spmv(ctx, 64, x1.data(), rf.data()), "asend_spmv");
+        for (int64_t d : pen) rf[d] -= sf * x1[d];
+        check(asend_spmv(ctx, 64, x2.data(), y.data()), "asend_spmv");
+        check(asend_assemble(ctx, 1.0), "asend_assemble");
+        check(asend_spmv(ctx, 64, x2.data(), y2.data()), "asend_spmv");
+        for (int64_t i = 0; i < n_dof; ++i) rf[i] += y2[i] - y[i];
+        return rf;
     }
 
     // objective terms (calculate_terms for displacement / powerNorm,
     // S/objective_meth.rs:592-640, get_power_norm :329-337)
     void objective_values() {
         if (!solved) die("the objective needs a solution (run solve first)");
+        if (dyn_solved) die("objectives of dynamic solves are not supported by this solver");
         for (auto& t : obj) {
             if (t.category != "displacement" || t.optr != "powerNorm")
                 die("objective terms of category %s with operator %s are not supported", t.category.c_str(),
@@ -860,36 +1050,65 @@ struct Job {
     }
 
     // ------------------------------------------------------------ writers
+    // write_node_results (S/model_output.rs:84-262): with timeSteps, one file
+    // per listed step, <name up to its first '.'>_timestep<i><rest>, from
+    // that step's state file (which then stays the current state, as the
+    // reference's backstep does), the time column being timeStep * i
+    // (dynamic) or the i-th staticLoadTime; without, the current state and -1
     void write_node_results(const Command& c) {
         if (!solved) die("node results need a solution (run solve first)");
-        if (!c.time_steps.empty()) die("time-step node results belong to dynamic jobs");
+        for (const auto& fld : c.fields)
+            if (fld != "displacement" && fld != "velocity" && fld != "acceleration" && fld != "reactionForce")
+                die("unrecognized output field '%s' for write_node_results", fld.c_str());
+        if (c.time_steps.empty()) {
+            write_node_file(c, c.file, -1.0);
+            return;
+        }
+        const size_t dot = c.file.find('.');
+        const std::string root = dot == std::string::npos ? c.file : c.file.substr(0, dot);
+        const std::string ext = dot == std::string::npos ? "" : c.file.substr(dot);
+        for (long ts : c.time_steps) {
+            read_state(ts, u, v, a);
+            double time;
+            if (dyn_solved) {
+                time = solve_cmd.time_step * (double)ts;
+            } else {
+                const auto& lt = solve_cmd.load_times;
+                time = lt[std::min<size_t>((size_t)std::max<long>(ts, 0), lt.size() - 1)];
+            }
+            write_node_file(c, root + "_timestep" + std::to_string(ts) + ext, time);
+        }
+    }
+
+    void write_node_file(const Command& c, const std::string& name, double time) {
         const auto& set = node_set(c.node_set, true);
         std::vector<double> rf;
-        for (const auto& fld : c.fields) {
-            if (fld == "reactionForce" && rf.empty()) {
-                rf.assign(n_dof, 0.0);
-                check(asend_spmv(ctx, 64, u.data(), rf.data()), "asend_spmv");
-                const double sf = asend_penalty_scale(ctx);
-                for (int64_t d : pen) rf[d] -= sf * u[d];
-            } else if (fld != "displacement" && fld != "reactionForce") {
-                die("unrecognized output field '%s' for write_node_results", fld.c_str());
-            }
-        }
-        std::FILE* fp = std::fopen(c.file.c_str(), "w");
-        if (!fp) die("could not open file %s", c.file.c_str());
+        for (const auto& fld : c.fields)
+            if (fld == "reactionForce" && rf.empty()) rf = reaction();
+        std::FILE* fp = std::fopen(name.c_str(), "w");
+        if (!fp) die("could not open file %s", name.c_str());
         std::fprintf(fp, "node,");
-        for (const auto& fld : c.fields) std::fprintf(fp, fld == "displacement" ? "U1,U2,U3,R1,R2,R3," : "RF1,RF2,RF3,RM1,RM2,RM3,");
+        for (const auto& fld : c.fields) {
+            const char* h = fld == "displacement"   ? "U1,U2,U3,R1,R2,R3,"
+                            : fld == "velocity"     ? "V1,V2,V3,RV1,RV2,RV3,"
+                            : fld == "acceleration" ? "A1,A2,A3,RA1,RA2,RA3,"
+                                                    : "RF1,RF2,RF3,RM1,RM2,RM3,";
+            std::fprintf(fp, "%s", h);
+        }
         std::fprintf(fp, "time\n");
         for (int32_t n : set) {
             std::fprintf(fp, "%d,", n);
             for (const auto& fld : c.fields) {
-                const std::vector<double>& v = (fld == "displacement") ? u : rf;
+                const std::vector<double>& vec = fld == "displacement"   ? u
+                                                 : fld == "velocity"     ? v
+                                                 : fld == "acceleration" ? a
+                                                                         : rf;
                 for (int k = 0; k < 6; ++k) {
-                    const double x = (k < dpn) ? v[dof_index[6 * n + k]] : 0.0;
-                    std::fprintf(fp, "%s,", fmt_e12(x).c_str());
+                    const int64_t d = (k < dpn) ? dof_index[6 * n + k] : -1;
+                    std::fprintf(fp, "%s,", fmt_e12(d >= 0 ? vec[d] : 0.0).c_str());
                 }
             }
-            std::fprintf(fp, "%s\n", fmt_e12(-1.0).c_str());
+            std::fprintf(fp, "%s\n", fmt_e12(time).c_str());
         }
         std::fclose(fp);
     }

@@ -2,7 +2,8 @@
 cantilever (1.0 x 0.2 x 0.2 m, E = 200 GPa, nu = 0.3, rho = 7850) clamped
 at x = 0 with a -1 MPa z traction on the tip face, as the reference's job
 files describe it: model, constraints, loads, design variables, objective
-and two jobs (solverMethod direct and iterative).
+and three jobs (static with solverMethod direct and iterative; dynamic under a
+traction pulse, with state files and time-step node results).
 
 The files follow the layout of the reference's Python writers
 (yaml.dump(..., sort_keys=False) of the same nested keys, quotes removed;
@@ -78,7 +79,8 @@ def main():
                                                                          "convergenceTol": 1e-24})):
         solve = {"command": "solve", "staticLoadTime": [0.0], "loadRampSteps": 1, "timeStep": 1.0,
                  "newmarkBeta": 0.25, "newmarkGamma": 0.5, "simPeriod": 1.0, "saveSolnHist": "yes",
-                 "solnHistDir": "", "solverMethod": method, "solverBlockDim": 2000000000}
+                 "solnHistDir": "results" if method == "direct" else "results_iterative",
+                 "solverMethod": method, "solverBlockDim": 2000000000}
         solve.update(extra)
         tag = "" if method == "direct" else "_iterative"
         job = {"jobCommands": [
@@ -95,6 +97,24 @@ def main():
              "include": ["category", "nodeSet"]},
         ]}
         dump(name, job)
+    # dynamic job: the same cantilever under a tip traction pulse (active for
+    # the steps that start at t <= 9e-4 s), Rayleigh damping, 2e-4 s steps over
+    # 4 ms, state files in results_dynamic/ and node results of five steps
+    pulse = {"loads": [{"type": "surfaceTraction", "activeTime": str([0.0, 9e-4]), "normDir": str([1.0, 0.0, 0.0]),
+                        "normTolerance": 5.0, "load": str([0.0, 0.0, -1.0e6]), "elementSet": "tipEls"}]}
+    dump("c1_loads_pulse.yaml", pulse, width=200)
+    solve = {"command": "solve", "elastic": "yes", "dynamic": "yes", "timeStep": 2e-4, "newmarkBeta": 0.25,
+             "newmarkGamma": 0.5, "simPeriod": 4e-3, "rayDampCM": 5.0, "rayDampCK": 1e-5, "saveSolnHist": "yes",
+             "solnHistDir": "results_dynamic", "solverMethod": "direct"}
+    job = {"jobCommands": [
+        {"command": "readModelInput", "fileName": "c1_model.yaml"},
+        {"command": "readConstraints", "fileName": "c1_constraints.yaml"},
+        {"command": "readLoads", "fileName": "c1_loads_pulse.yaml"},
+        solve,
+        {"command": "writeNodeResults", "fileName": "results_dynamic/nodeResults.csv", "nodeSet": "all",
+         "fields": ["displacement", "velocity", "acceleration", "reactionForce"], "timeSteps": "0 21 5"},
+    ]}
+    dump("c1_job_dynamic.yaml", job)
 
 
 if __name__ == "__main__":

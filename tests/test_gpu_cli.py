@@ -16,6 +16,7 @@ import subprocess
 import numpy as np
 import pytest
 
+import dynamics
 import oracle as O
 from asend3d_amd import api, problems
 from conftest import gpu_case
@@ -72,7 +73,7 @@ def oracle_c1():
 def run_job(tmp_path, job):
     work = tmp_path / "job"
     shutil.copytree(FIXTURE, work)
-    for d in ("results", "results_iterative"):
+    for d in ("results", "results_iterative", "results_dynamic"):
         (work / d).mkdir()
     print(f"[gpu-case] asend_run_job {job}", flush=True)
     r = subprocess.run([CLI, job], cwd=work, capture_output=True, text=True, timeout=300)
@@ -135,3 +136,56 @@ def test_cli_c1_matches_oracle(tmp_path, oracle_c1, job, tag, bar_u):
     g = np.array([float(r[1]) for r in rows])
     assert len(g) == len(DVARS)
     assert np.abs(g - o["grad"]).max() <= 1e2 * bar_u * np.abs(o["grad"]).max()
+
+
+def read_state(path, n_nodes, dpn=3):
+    """solnTStep<i>.out: per node label, dpn displacements, velocities and
+    accelerations as big-endian doubles (S/model_output.rs:14-82)."""
+    raw = np.fromfile(path, dtype=">f8")
+    assert raw.size == n_nodes * 3 * dpn, (path, raw.size)
+    s = raw.reshape(n_nodes, 3, dpn)
+    return s[:, 0], s[:, 1], s[:, 2]
+
+
+def test_cli_dynamic_state_files(tmp_path):
+    """Dynamic job (traction pulse, Rayleigh damping, Newmark from rest):
+    every state file against the oracle's Newmark on the same model, and the
+    time-step node results (U, V, A, reactions K u + M a + D v, time column)."""
+    gpu_case("asend_run_job c1_job_dynamic.yaml")
+    work, out = run_job(tmp_path, "c1_job_dynamic.yaml")
+    p = problems.config1()
+    m = p.mesh
+    di = np.asarray(p.dof_index).reshape(-1, 6)[:, :3]
+    Kc = O.assemble_solid(8, m.xyz, m.conn, p.E, p.nu, p.dof_index, p.n_dof)
+    M = O.assemble_solid_mass(8, m.xyz, m.conn, np.full(m.n_el, 7850.0), p.dof_index, Kc).to_scipy()
+    K = Kc.to_scipy()
+    f = O.surface_traction(8, m.xyz, m.conn, p.dof_index, p.n_dof, p.tip_elements, (0.0, 0.0, -1.0e6),
+                           (1.0, 0.0, 0.0), 5.0)
+    # the reference's loop: loads active at the start time of each step,
+    # time accumulated by repeated addition (S/model_analysis.rs:590-622)
+    fac, t = [], 0.0
+    while t < 4e-3:
+        fac.append(1.0 if 0.0 <= t <= 9e-4 else 0.0)
+        t += 2e-4
+    states = [(np.zeros(p.n_dof), np.zeros(p.n_dof), np.zeros(p.n_dof))]
+    dynamics.newmark(K, M, f, fac, 2e-4, 0.25, 0.5, pen_dofs=p.pen_dofs, damp_cm=5.0, damp_ck=1e-5,
+                     on_step=lambda n, u, v, a, Kp: states.append((u.copy(), v.copy(), a.copy())))
+    assert len(states) == 21
+    scale = [max(np.abs(x).max() for x in (st[k] for st in states)) for k in range(3)]
+    for i, (u, v, a) in enumerate(states):
+        su, sv, sa = read_state(work / "results_dynamic" / f"solnTStep{i}.out", m.n_nodes)
+        for got, ref, sc in ((su, u, scale[0]), (sv, v, scale[1]), (sa, a, scale[2])):
+            assert np.abs(got - ref[di]).max() <= 1e-7 * sc, (i, np.abs(got - ref[di]).max() / sc)
+    D = 5.0 * M + 1e-5 * K
+    for ts in (0, 5, 10, 15, 20):
+        head, rows = read_csv(work / "results_dynamic" / f"nodeResults_timestep{ts}.csv")
+        assert head == ["node", "U1", "U2", "U3", "R1", "R2", "R3", "V1", "V2", "V3", "RV1", "RV2", "RV3",
+                        "A1", "A2", "A3", "RA1", "RA2", "RA3", "RF1", "RF2", "RF3", "RM1", "RM2", "RM3", "time"]
+        vals = np.array([[float(x) for x in r[1:]] for r in rows])
+        u, v, a = states[ts]
+        assert np.abs(vals[:, 0:3] - u[di]).max() <= 1e-7 * scale[0]
+        assert np.abs(vals[:, 6:9] - v[di]).max() <= 1e-7 * scale[1]
+        assert np.abs(vals[:, 12:15] - a[di]).max() <= 1e-7 * scale[2]
+        rf = K @ u + M @ a + D @ v
+        assert np.abs(vals[:, 18:21] - rf[di]).max() <= 1e-5 * max(np.abs(rf).max(), 1e-300)
+        assert np.all(vals[:, -1] == float(f"{2e-4 * ts:.12e}"))
