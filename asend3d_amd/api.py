@@ -524,6 +524,15 @@ class Context:
         """D = cm M + ck K for the next newmark_setup (rayDampCM / rayDampCK)."""
         self._chk(_lib.asend_set_rayleigh_damping(self.h, _dbl(cm), _dbl(ck)), "asend_set_rayleigh_damping")
 
+    def set_material_damping(self, dmat, el_mat=None):
+        """Material damping matrices (n_mat x 6 x 6, strain order 11, 22, 33,
+        12, 13, 23) and each element's matrix (None: matrix 0) for the next
+        newmark_setup (asend_set_material_damping, brick8)."""
+        d = _f64(np.asarray(dmat, dtype=np.float64).reshape(-1, 36))
+        idx = None if el_mat is None else np.ascontiguousarray(el_mat, dtype=np.int32)
+        self._chk(_lib.asend_set_material_damping(self.h, _i64(d.shape[0]), _p(d), _p(idx)),
+                  "asend_set_material_damping")
+
     def newmark_setup(self, dt, beta=0.25, gamma=0.5):
         self._chk(_lib.asend_newmark_setup(self.h, _dbl(dt), _dbl(beta), _dbl(gamma)), "asend_newmark_setup")
 

@@ -22,7 +22,8 @@
 //     preconditioner and the reference stop rule, convergenceTol,
 //     maxIterations, 0 meaning the DOF count as in the reference); or
 //     dynamic (Newmark with newmarkBeta, newmarkGamma, timeStep, simPeriod,
-//     rayDampCM, rayDampCK; consistent mass), each step solved on the device;
+//     rayDampCM, rayDampCK, material damping matrices on brick8; consistent
+//     mass), each step solved on the device;
 //     saveSolnHist writes the reference's solnTStep<i>.out state files into
 //     solnHistDir;
 //   - objective terms of category displacement with operator powerNorm
@@ -181,6 +182,8 @@ struct Material {
     double density = 0.0;
     double E[3] = {0, 0, 0}, nu[3] = {0, 0, 0}, G[3] = {0, 0, 0};
     bool stiffness = false;
+    double damping[36] = {};  // Material::set_damping (symmetric, S/section_meth.rs:73-77)
+    bool damped = false;
 };
 struct Term {
     std::string set;
@@ -416,6 +419,11 @@ struct Job {
                         v[2] = d.size() == 3 ? to_d(d[2]) : v[0];
                     } else if (h[2] == "elastic" && h[3] == "stiffness") {
                         cur_mat->stiffness = true;
+                    } else if (h[2] == "damping" && d.size() == 3) {
+                        const long r = to_i(d[0]) - 1, c = to_i(d[1]) - 1;
+                        if (r < 0 || r > 5 || c < 0 || c > 5) die("damping entry out of range");
+                        cur_mat->damping[6 * r + c] = cur_mat->damping[6 * c + r] = to_d(d[2]);
+                        cur_mat->damped = true;
                     }
                 }
             }
@@ -929,6 +937,7 @@ struct Job {
         if (c.lump_mass) die("lumpMass is not supported by this solver");
         if (!(c.time_step > 0.0)) die("dynamic solves need timeStep > 0");
         check(asend_set_rayleigh_damping(ctx, c.ray_cm, c.ray_ck), "asend_set_rayleigh_damping");
+        material_damping();
         check(asend_newmark_setup(ctx, c.time_step, c.beta, c.gamma), "asend_newmark_setup");
         u.assign(n_dof, 0.0);
         if (c.save_hist) write_state(0, u, v, a);
@@ -959,6 +968,28 @@ struct Job {
         std::printf("device Newmark: %lld solver iterations\n", (long long)its);
     }
 
+    // material damping matrices of the elements' section materials
+    // (get_solid_damp_dfd0, S/element_properties.rs:1537-1575); brick8 only
+    void material_damping() {
+        bool any = false;
+        for (const auto& m : materials) any = any || m.damped;
+        if (!any) return;
+        if (etype != ASEND_BRICK8) die("material damping is supported for brick8 models");
+        std::vector<double> dm(36 * materials.size());
+        for (size_t k = 0; k < materials.size(); ++k)
+            std::copy(materials[k].damping, materials[k].damping + 36, dm.begin() + 36 * k);
+        std::vector<int32_t> idx(conn.size(), 0);
+        for (const auto& sec : sections) {
+            int32_t mi = -1;
+            for (size_t k = 0; k < materials.size(); ++k)
+                if (materials[k].name == sec.mat) mi = (int32_t)k;
+            if (mi < 0) die("material %s is not defined", sec.mat.c_str());
+            for (int32_t e : element_set(sec.el_set)) idx[e] = mi;
+        }
+        check(asend_set_material_damping(ctx, (int64_t)materials.size(), dm.data(), idx.data()),
+              "asend_set_material_damping");
+    }
+
     // reaction forces: the element residual of build_elastic_soln_load(false)
     // negated (S/model_output.rs:206-216): K u, plus M a + D v for a dynamic
     // solve (D = rayDampCM M + rayDampCK K), without the penalty
@@ -970,6 +1001,8 @@ struct Job {
             for (int64_t d : pen) rf[d] -= sf * u[d];
             return rf;
         }
+        for (const auto& m : materials)
+            if (m.damped) die("reactionForce of dynamic solves with material damping is not supported");
         std::vector<double> x1(n_dof), x2(n_dof), y(n_dof), y2(n_dof);
         for (int64_t i = 0; i < n_dof; ++i) {
             x1[i] = u[i] + solve_cmd.ray_ck * v[i];

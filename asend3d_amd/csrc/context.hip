@@ -129,6 +129,12 @@ struct asend_ctx {
     double asm_kscale = 1.0;  // stiffness factor of the current assembly
     // Rayleigh damping D = ray_cm M + ray_ck K of the next Newmark setup
     double ray_cm = 0.0, ray_ck = 0.0;
+    // material damping matrices (n_dmat x 36) and each element's matrix
+    // (null: all matrix 0), added to D of the next Newmark setup
+    double* d_dmat = nullptr;
+    int32_t* d_dmat_idx = nullptr;
+    int64_t n_dmat = 0;
+    double asm_cdamp = 0.0;  // weight of the material damping in the current assembly
     // Newmark step solver (asend_newmark_set_solver)
     int nm_mode = ASEND_NM_AUTO;
     double nm_inner_tol = 1e-3;  // reliable-update factor of the mixed step solver
@@ -170,11 +176,13 @@ struct asend_ctx {
         // u + ck v the adjoint's dR/dE acts on (save_y: where the tail
         // stores it, null when not recording)
         double cm = 0, ck = 0, cd = 0;
-        double keff_k = 1.0, keff_m = 0.0;  // K_eff = keff_k K + keff_m M (+ penalty)
+        double keff_k = 1.0, keff_m = 0.0;  // K_eff = keff_k K + keff_m M (+ keff_d D_mat + penalty)
+        double keff_d = 0.0;
+        bool mat = false;  // material damping matrices are part of D
         double *D64 = nullptr, *rhsd = nullptr, *yhist = nullptr, *save_y = nullptr;
         int64_t yhist_cap = 0;
         const double* graph_save_y = nullptr;
-        bool damped() const { return cm != 0.0 || ck != 0.0; }
+        bool damped() const { return cm != 0.0 || ck != 0.0 || mat; }
         // step solver (ASEND_NM_*): FP32 single-reduction PCG with an FP64
         // increment and reliable updates, FP64 single-reduction PCG, or by
         // operator size; the FP32 work vectors of the mixed solver (5 n
@@ -301,6 +309,9 @@ void ldl_release(asend_ctx* ctx) {
 void free_mesh(asend_ctx* ctx) {
     ldl_release(ctx);
     ctx->ldl_block_dofs = -1;
+    dfree(ctx->d_dmat);
+    dfree(ctx->d_dmat_idx);
+    ctx->n_dmat = 0;
     dfree(ctx->d_conn_r);
     dfree(ctx->d_rp);
     dfree(ctx->d_bc);
@@ -974,8 +985,11 @@ int solve_device(asend_ctx* ctx, const asend_solve_opts* o, asend_solve_stats* s
 // reference blocks.  Other brick8 meshes: each element integrated once into
 // row records, then one row-wise gather.  brickIM and shell4: K is cleared, then one scatter
 // launch per element colour.
-int integrate_elements(asend_ctx* ctx, const double* E, double k_scale, double c_mass, double* K) {
-    if (ctx->uniform_hex) {
+int integrate_elements(asend_ctx* ctx, const double* E, double k_scale, double c_mass, double* K,
+                       double c_damp = 0.0) {
+    if (c_damp != 0.0 && (ctx->etype != ASEND_BRICK8 || ctx->n_dmat == 0))
+        return fail(ctx, ASEND_ERR_UNSUPPORTED, "material damping is implemented for brick8 meshes");
+    if (ctx->uniform_hex && c_damp == 0.0) {
         launch_hex8_assemble_uniform(ctx->st, (int)ctx->n_nodes, (int)ctx->n_slices, ctx->d_sptr, ctx->d_scol,
                                      ctx->d_rp, ctx->d_n2e_ptr, ctx->d_n2e, ctx->d_conn_r, ctx->d_khat, ctx->d_mhat, E,
                                      ctx->d_rho, k_scale, c_mass, K);
@@ -1001,7 +1015,7 @@ int integrate_elements(asend_ctx* ctx, const double* E, double k_scale, double c
             if (rc) return rc;
         }
         launch_hex8_elem_rec(ctx->st, (int)ctx->n_el, ctx->d_conn_r, ctx->d_xyz_r, E, ctx->d_nu, ctx->d_rho, k_scale,
-                             c_mass, ctx->d_kb);
+                             c_mass, ctx->d_dmat, ctx->d_dmat_idx, c_damp, ctx->d_kb);
         launch_rec_gather(ctx->st, (int)ctx->n_nodes, (int)ctx->n_slices, ctx->d_sptr, ctx->d_n2e_ptr, ctx->d_n2rec,
                           ctx->d_slot, ctx->d_kb, nullptr, 0.0, nullptr, K, nullptr, nullptr);
     }
@@ -1060,16 +1074,18 @@ int finish_assembly(asend_ctx* ctx, double k_scale, double c_mass) {
     ctx->asm_count += 1;
     ctx->asm_cmass = c_mass;
     ctx->asm_kscale = k_scale;
+    ctx->asm_cdamp = 0.0;  // assemble_device sets it after this for its own assemblies
     return ASEND_OK;
 }
 
 // K_eff = k_scale K + c_mass M (+ penalty), its Jacobi diagonal and FP32 copy
-int assemble_device(asend_ctx* ctx, double k_scale, double c_mass) {
+int assemble_device(asend_ctx* ctx, double k_scale, double c_mass, double c_damp = 0.0) {
     if (ctx->etype == ASEND_SHELL4 && !ctx->thk_set)
         return fail(ctx, ASEND_ERR_STATE, "shell4 mesh without thickness (asend_set_thickness)");
-    int rc = integrate_elements(ctx, ctx->d_E, k_scale, c_mass, ctx->d_K64);
-    if (rc) return rc;
-    return finish_assembly(ctx, k_scale, c_mass);
+    int rc = integrate_elements(ctx, ctx->d_E, k_scale, c_mass, ctx->d_K64, c_damp);
+    if (rc || (rc = finish_assembly(ctx, k_scale, c_mass))) return rc;
+    ctx->asm_cdamp = c_damp;
+    return ASEND_OK;
 }
 
 // Geometrically nonlinear brick8: tangent of the internal force at the
@@ -2121,6 +2137,40 @@ int asend_batch_solve(asend_batch* ab, int nrhs, const double* const* rhs, doubl
 // ---------------------------------------------------------------------------
 // Newmark dynamics
 // ---------------------------------------------------------------------------
+int asend_set_material_damping(asend_ctx* ctx, int64_t n_mat, const double* dmat, const int32_t* el_mat) {
+    if (!ctx || n_mat < 0 || (n_mat > 0 && !dmat)) return ASEND_ERR_ARG;
+    if (!ctx->d_K64) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
+    if (n_mat == 0) {
+        dfree(ctx->d_dmat);
+        dfree(ctx->d_dmat_idx);
+        ctx->n_dmat = 0;
+        return ASEND_OK;
+    }
+    if (ctx->etype != ASEND_BRICK8)
+        return fail(ctx, ASEND_ERR_UNSUPPORTED, "material damping is implemented for brick8 meshes");
+    for (int64_t m = 0; m < n_mat; ++m)
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j < i; ++j)
+                if (dmat[36 * m + 6 * i + j] != dmat[36 * m + 6 * j + i])
+                    return fail(ctx, ASEND_ERR_ARG, "damping matrix %lld is not symmetric", (long long)m);
+    if (el_mat)
+        for (int64_t e = 0; e < ctx->n_el; ++e)
+            if (el_mat[e] < 0 || el_mat[e] >= n_mat)
+                return fail(ctx, ASEND_ERR_ARG, "element %lld: damping matrix index %d out of range", (long long)e,
+                            el_mat[e]);
+    int rc;
+    if ((rc = dalloc(ctx, &ctx->d_dmat, 36 * n_mat))) return rc;
+    HIPCHK(hipMemcpy(ctx->d_dmat, dmat, 36 * n_mat * sizeof(double), hipMemcpyHostToDevice));
+    if (el_mat) {
+        if ((rc = dalloc(ctx, &ctx->d_dmat_idx, ctx->n_el))) return rc;
+        HIPCHK(hipMemcpy(ctx->d_dmat_idx, el_mat, ctx->n_el * sizeof(int32_t), hipMemcpyHostToDevice));
+    } else {
+        dfree(ctx->d_dmat_idx);
+    }
+    ctx->n_dmat = n_mat;
+    return ASEND_OK;
+}
+
 int asend_set_rayleigh_damping(asend_ctx* ctx, double cm, double ck) {
     if (!ctx || !(cm >= 0.0) || !(ck >= 0.0) || !std::isfinite(cm) || !std::isfinite(ck)) return ASEND_ERR_ARG;
     ctx->ray_cm = cm;
@@ -2156,6 +2206,8 @@ int asend_newmark_setup(asend_ctx* ctx, double dt, double beta, double gamma) {
     nm.cm = ctx->ray_cm;
     nm.ck = ctx->ray_ck;
     nm.cd = nm.c_m * dt * gamma;
+    nm.mat = ctx->n_dmat > 0;
+    nm.keff_d = nm.mat ? nm.cd : 0.0;
     // The reference adds stiffness-proportional damping to the external block
     // of elements with condensed internal DOFs before condensing
     // (S/element_equations.rs:991-1003); that form is not built here.
@@ -2186,10 +2238,11 @@ int asend_newmark_setup(asend_ctx* ctx, double dt, double beta, double gamma) {
         dfree(zeroE);
         if (rc) return rc;
     }
-    // D = cm M + ck K, the damped load's work vector
+    // D = cm M + ck K (+ the material damping matrices), the damped load's
+    // work vector
     if (nm.damped()) {
         if ((rc = dalloc(ctx, &nm.D64, nval)) || (rc = dalloc(ctx, &nm.rhsd, n))) return rc;
-        if ((rc = integrate_elements(ctx, ctx->d_E, nm.ck, nm.cm, nm.D64))) return rc;
+        if ((rc = integrate_elements(ctx, ctx->d_E, nm.ck, nm.cm, nm.D64, nm.mat ? 1.0 : 0.0))) return rc;
     }
     // K_eff = K + c_m M + cd D (+ penalty) = (1 + cd ck) K + (c_m + cd cm) M.
     // The reference scales its penalty once, on the first matrix a solve
@@ -2198,7 +2251,7 @@ int asend_newmark_setup(asend_ctx* ctx, double dt, double beta, double gamma) {
     nm.keff_k = 1.0 + nm.cd * nm.ck;
     nm.keff_m = nm.c_m + nm.cd * nm.cm;
     ctx->sf_fixed = false;
-    if ((rc = assemble_device(ctx, nm.keff_k, nm.keff_m))) return rc;
+    if ((rc = assemble_device(ctx, nm.keff_k, nm.keff_m, nm.keff_d))) return rc;
     HIPCHK(hipStreamSynchronize(ctx->st));
     nm.ready = true;
     return ASEND_OK;
@@ -2218,8 +2271,9 @@ int asend_newmark_run(asend_ctx* ctx, int64_t n_steps, const double* f_shape, co
         if (watch[k] < 0 || watch[k] >= n) return fail(ctx, ASEND_ERR_ARG, "watched DOF out of range");
     int rc;
     // the operator must be K_eff of this setup (a static assembly may have replaced it)
-    if (!ctx->assembled || ctx->asm_cmass != nm.keff_m || ctx->asm_kscale != nm.keff_k) {
-        if ((rc = assemble_device(ctx, nm.keff_k, nm.keff_m))) return rc;
+    if (!ctx->assembled || ctx->asm_cmass != nm.keff_m || ctx->asm_kscale != nm.keff_k ||
+        ctx->asm_cdamp != nm.keff_d) {
+        if ((rc = assemble_device(ctx, nm.keff_k, nm.keff_m, nm.keff_d))) return rc;
     }
     bool realloc = false;
     if (n_steps > nm.fac_cap) {

@@ -70,13 +70,16 @@ __device__ __forceinline__ double group8_sum(double v) {
 // Jacobian at Gauss point gp by an 8-lane butterfly over the nodes t, then
 // the physical gradient g_t, N_t and detJ staged in LDS.  Phase 2, lane =
 // 8 a + b with a <= b: G_ab = sum_gp detJ g_a g_b^T, and for isotropic C the
-// block is K_ab = lam G_ab + mu G_ab^T + mu tr(G_ab) I, plus c_mass rho M_ab I;
-// K_ba = K_ab^T.  The element's eight row records are staged in LDS and
+// block is K_ab = lam G_ab + mu G_ab^T + mu tr(G_ab) I, plus c_mass rho M_ab I
+// and, with c_damp != 0, c_damp times the same integral for a general 6x6
+// material damping matrix; K_ba = K_ab^T.  The element's eight row records are staged in LDS and
 // written as coalesced 16-byte stores.
 __global__ __launch_bounds__(256) void k_hex8_elem_rec(int n_el, const int32_t* __restrict__ conn_r,
                                                        const double* __restrict__ xyz_r,
                                                        const double* __restrict__ E, const double* __restrict__ nu,
                                                        const double* __restrict__ rho, double k_scale, double c_mass,
+                                                       const double* __restrict__ Cd,
+                                                       const int32_t* __restrict__ cd_idx, double c_damp,
                                                        double* __restrict__ Kr) {
     __shared__ __align__(16) double s_g[4][8][8][4];  // (g_t, N_t) at each point
     __shared__ double s_det[4][8];
@@ -149,13 +152,34 @@ __global__ __launch_bounds__(256) void k_hex8_elem_rec(int n_el, const int32_t* 
         const double lam = Ee * ne / ((1.0 + ne) * (1.0 - 2.0 * ne));
         const double mu = Ee / (2.0 * (1.0 + ne));
         const double dg = mu * (G[0] + G[4] + G[8]) + (c_mass != 0.0 ? c_mass * rho[ec] * m : 0.0);
+        // c_damp B^T C_d B for a general symmetric 6x6 matrix in the strain
+        // order (11, 22, 33, 12, 13, 23), engineering shear: block entry
+        // (i, j) = sum over m, n of C_d[V(i,m)][V(j,n)] G_mn
+        double X[9];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) X[c] = 0.0;
+        if (c_damp != 0.0) {
+            const double* Dm = Cd + 36 * (cd_idx ? (int64_t)cd_idx[ec] : 0);
+            constexpr int V[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int m2 = 0; m2 < 3; ++m2)
+#pragma unroll
+                        for (int n2 = 0; n2 < 3; ++n2) acc += Dm[6 * V[i][m2] + V[j][n2]] * G[3 * m2 + n2];
+                    X[3 * i + j] = c_damp * acc;
+                }
+        }
         double* o = &s_out[wv][kRecStride * a + 9 * b];
         double* ot = &s_out[wv][kRecStride * b + 9 * a];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
-                const double v = lam * G[3 * i + j] + mu * G[3 * j + i] + (i == j ? dg : 0.0);
+                const double v = lam * G[3 * i + j] + mu * G[3 * j + i] + (i == j ? dg : 0.0) + X[3 * i + j];
                 o[3 * i + j] = v;
                 if (a != b) ot[3 * j + i] = v;
             }
@@ -419,10 +443,11 @@ __global__ __launch_bounds__(256) void k_hex8_assemble_uniform(
 }
 
 void launch_hex8_elem_rec(hipStream_t st, int n_el, const int32_t* conn_r, const double* xyz_r, const double* E,
-                          const double* nu, const double* rho, double k_scale, double c_mass, double* Kr) {
+                          const double* nu, const double* rho, double k_scale, double c_mass, const double* Cd,
+                          const int32_t* cd_idx, double c_damp, double* Kr) {
     if (n_el <= 0) return;
     hipLaunchKernelGGL(k_hex8_elem_rec, dim3((unsigned)((n_el + 3) / 4)), dim3(256), 0, st, n_el, conn_r, xyz_r, E,
-                       nu, rho, k_scale, c_mass, Kr);
+                       nu, rho, k_scale, c_mass, Cd, cd_idx, c_damp, Kr);
 }
 
 void launch_rec_gather(hipStream_t st, int n_rows, int n_slices, const int32_t* sptr, const int32_t* n2e_ptr,

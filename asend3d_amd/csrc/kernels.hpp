@@ -40,14 +40,17 @@ __host__ __device__ inline int64_t sell_index(const int32_t* sptr, int r, int k,
 }
 
 // brick8 general meshes, element-once: the blocks k_scale K_e + c_mass
-// rho_e M_e of every element as row records into Kr (8 kRecStride doubles per
-// element), then their row gather into K (every slot, padding as zeros);
+// rho_e M_e (+ c_damp D_e, the B^T C_d B integral of the element's damping
+// matrix Cd[36 cd_idx[e]], cd_idx null: matrix 0) of every element as row
+// records into Kr (8 kRecStride doubles per element), then their row gather
+// into K (every slot, padding as zeros);
 // n2rec lists the records 8 e + a of each row (CSR by n2e_ptr), slot maps
 // block (e, a, b) to its in-row slot.  With Re set the gather also forms the
 // internal force rint of the element residuals Re (24 per element) and, with
 // rhs set, rhs = fact f - rint.
 void launch_hex8_elem_rec(hipStream_t st, int n_el, const int32_t* conn_r, const double* xyz_r, const double* E,
-                          const double* nu, const double* rho, double k_scale, double c_mass, double* Kr);
+                          const double* nu, const double* rho, double k_scale, double c_mass, const double* Cd,
+                          const int32_t* cd_idx, double c_damp, double* Kr);
 void launch_rec_gather(hipStream_t st, int n_rows, int n_slices, const int32_t* sptr, const int32_t* n2e_ptr,
                        const int32_t* n2rec, const uint8_t* slot, const double* Kr, const double* Re, double fact,
                        const double* f, double* K, double* rint, double* rhs);

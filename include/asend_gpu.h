@@ -461,14 +461,25 @@ int asend_newmark_set_solver(asend_ctx* ctx, int mode, double ru_factor);
  * S/job.rs:15-16; Element::get_rud_dfd0, S/element_equations.rs:699-753).
  * cm, ck >= 0; default 0.  ck > 0 is implemented for brick8 meshes (the
  * reference adds it to the uncondensed external block of elements with
- * internal DOFs); cm for every element type.  Material damping matrices are
- * not implemented. */
+ * internal DOFs); cm for every element type.  Material damping matrices:
+ * asend_set_material_damping. */
 int asend_set_rayleigh_damping(asend_ctx* ctx, double cm, double ck);
+/* Material damping for the next asend_newmark_setup (material key damping,
+ * Material::set_damping, S/section_meth.rs:73-77; get_solid_damp_dfd0,
+ * S/element_properties.rs:1537-1575; the material-damping force of
+ * get_rud_dfd0, S/element_equations.rs:758-850): n_mat symmetric 6 x 6
+ * matrices (row-major, strain order 11, 22, 33, 12, 13, 23 with engineering
+ * shear, as the stiffness), and the matrix of each element (el_mat, n_el
+ * entries; NULL: every element uses matrix 0).  Each element adds
+ * sum over its Gauss points of w detJ B^T D_m B to D (beside the Rayleigh
+ * terms).  brick8 meshes; n_mat = 0 removes them. */
+int asend_set_material_damping(asend_ctx* ctx, int64_t n_mat, const double* dmat, const int32_t* el_mat);
 /* Assemble M (consistent, rho per element), D (with damping) and
  * K_eff = K + c_m M + c_m dt gamma D, c_m = 1/(dt^2 (gamma - beta)), with the
  * penalty of asend_set_penalty (Element::get_ru_dfd0,
  * S/element_equations.rs:948-1003); requires gamma > beta.  brick8, brickIM
- * and shell4 meshes uploaded with rho (shell4: after asend_set_thickness). */
+ * and shell4 meshes uploaded with rho (shell4: after asend_set_thickness).
+ * D = cm M + ck K plus the material damping of asend_set_material_damping. */
 int asend_newmark_setup(asend_ctx* ctx, double dt, double beta, double gamma);
 /* n_steps steps from (u0, v0, a0) (null = zero).  The load of step n is
  * load_fac[n] * f_shape; the step's residual is f - K u - M a(u) - D v(u) at

@@ -54,11 +54,13 @@ import scipy.sparse.linalg as spla
 
 
 def newmark(K, M, f_shape, load_fac, dt, beta=0.25, gamma=0.5, u0=None, v0=None, a0=None, pen_dofs=(),
-            sf=None, watch=(), first_step="predictor", on_step=None, damp_cm=0.0, damp_ck=0.0):
+            sf=None, watch=(), first_step="predictor", on_step=None, damp_cm=0.0, damp_ck=0.0, damp_mat=None):
     """Run len(load_fac) steps.  K, M: FP64 sparse (no penalty); pen_dofs: DOFs
     clamped by a penalty sf (default 1e5 max|K_eff|, the reference scale
     taken on the first dynamic assembly); damp_cm, damp_ck: Rayleigh
-    coefficients (job keys rayDampCM, rayDampCK).  on_step(n, u, v, a, Kp) is
+    coefficients (job keys rayDampCM, rayDampCK); damp_mat: the assembled
+    material damping matrix (sum of the elements' B^T D_m B integrals,
+    get_rud_dfd0's material term), added to D.  on_step(n, u, v, a, Kp) is
     called after every step.  Returns (hist [n x len(watch)], u, v, a)."""
     K = sp.csr_matrix(K)
     M = sp.csr_matrix(M)
@@ -67,6 +69,8 @@ def newmark(K, M, f_shape, load_fac, dt, beta=0.25, gamma=0.5, u0=None, v0=None,
     c2 = dt * dt * (0.5 + beta - gamma)
     c_m = -c1
     D = (damp_cm * M + damp_ck * K).tocsr()
+    if damp_mat is not None:
+        D = (D + sp.csr_matrix(damp_mat)).tocsr()
     keff = (K + c_m * M + (c_m * dt * gamma) * D).tocsr()
     pen = np.zeros(n)
     if len(pen_dofs):

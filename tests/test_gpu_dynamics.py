@@ -4,6 +4,7 @@ pulse, undamped and with Rayleigh damping; graph replay and eager launches
 give identical histories; the transient adjoint against the oracle's sweep
 with and without damping."""
 import numpy as np
+import scipy.sparse as sp
 import pytest
 
 import dynamics
@@ -194,3 +195,56 @@ def test_damped_newmark_adjoint_matches_oracle(ctx, etype, cm, ck, beta, gamma):
     assert fst.unconverged_steps == 0 and ast.unconverged_steps == 0, (fst.as_dict(), ast.as_dict())
     assert abs(L - L_o) <= 1e-8 * abs(L_o)
     assert np.abs(dL - dL_o).max() <= 1e-7 * np.abs(dL_o).max()
+
+
+def test_material_damping_newmark_matches_oracle(ctx):
+    """Material damping matrices (two SPD 6x6 matrices alternating over the
+    elements) with Rayleigh damping on a distorted mesh: Newmark against the
+    oracle with D = cm M + ck K + sum_e int B^T D_m B."""
+    gpu_case("newmark with material damping, distorted 10x3x3")
+    p = problems.distorted(problems.cantilever(10, 3, 3), seed=21)
+    m = p.mesh
+    Kc = O.assemble_solid(8, m.xyz, m.conn, p.E, p.nu, p.dof_index, p.n_dof)
+    M = O.assemble_solid_mass(8, m.xyz, m.conn, np.full(m.n_el, RHO), p.dof_index, Kc).to_scipy()
+    K = Kc.to_scipy()
+    rng = np.random.default_rng(4)
+    dmat = []
+    for s in (1.0, 3.0):
+        A = rng.standard_normal((6, 6))
+        B = A @ A.T + 6.0 * np.eye(6)
+        dmat.append(s * 2e6 * B / np.abs(B).max())
+    dmat = np.array(dmat)
+    el_mat = (np.arange(m.n_el) % 2).astype(np.int32)
+    di = np.asarray(p.dof_index).reshape(-1, 6)[:, :3]
+    rows, cols, vals = [], [], []
+    for e in range(m.n_el):
+        De, _ = O.solid_element(8, m.xyz[m.conn[e]], dmat[el_mat[e]])
+        dofs = di[m.conn[e]].reshape(-1)
+        rows.append(np.repeat(dofs, 24))
+        cols.append(np.tile(dofs, 24))
+        vals.append(De.reshape(-1))
+    Dg = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                       shape=(p.n_dof, p.n_dof))
+    dt, n = 1e-4, 40
+    fac = np.where(np.arange(n) < 20, np.sin(np.pi * np.arange(n) / 20.0), 0.0)
+    watch = tip_dofs(p)
+    hist_o, u_o, v_o, a_o = dynamics.newmark(K, M, p.f, fac, dt, pen_dofs=p.pen_dofs, watch=watch, damp_cm=2.0,
+                                             damp_ck=1e-6, damp_mat=Dg)
+    hist_r, _, _, _ = dynamics.newmark(K, M, p.f, fac, dt, pen_dofs=p.pen_dofs, watch=watch, damp_cm=2.0,
+                                       damp_ck=1e-6)
+    # the material term matters at this size (the test would see it missing)
+    assert np.abs(hist_o - hist_r).max() >= 1e-3 * np.abs(hist_o).max()
+    problems.upload(ctx, p, rho=np.full(m.n_el, RHO))
+    try:
+        ctx.set_rayleigh_damping(2.0, 1e-6)
+        ctx.set_material_damping(dmat, el_mat)
+        ctx.newmark_setup(dt, 0.25, 0.5)
+        hist, u, v, a, st = ctx.newmark_run(n, p.f, fac, watch=watch, tol=1e-12, max_it_step=4000)
+    finally:
+        ctx.set_rayleigh_damping(0.0, 0.0)
+        ctx.set_material_damping(np.zeros((0, 36)))
+    assert st.unconverged_steps == 0, st.as_dict()
+    scale = np.abs(hist_o).max()
+    assert np.abs(hist - hist_o).max() <= 1e-8 * scale
+    for x, xo in ((u, u_o), (v, v_o), (a, a_o)):
+        assert np.abs(x - xo).max() <= 1e-7 * np.abs(xo).max()
