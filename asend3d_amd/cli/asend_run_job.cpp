@@ -28,8 +28,8 @@
 //     solnHistDir;
 //   - objective terms of category displacement with operator powerNorm
 //     (static solves);
-//   - design variables of category modulus, poissonRatio, shearModulus
-//     (brick8) for calcObjGradient;
+//   - design variables of category modulus, poissonRatio, shearModulus and
+//     density (brick8) for calcObjGradient;
 //   - writeNodeResults (displacement, velocity, acceleration, reactionForce;
 //     timeSteps read back from the state files) and writeObjective (with
 //     the _grad file), numbers formatted as the reference's {:.12e}.
@@ -654,7 +654,21 @@ struct Job {
             if (dv.category == "modulus") base = 0;
             else if (dv.category == "poissonRatio") base = 3;
             else if (dv.category == "shearModulus") base = 6;
+            else if (dv.category == "density") base = -1;
             else die("design variables of category %s are not supported", dv.category.c_str());
+            if (base < 0) {
+                // density = material density + sum of coefficient x value
+                // (get_density_dfd0, S/element_properties.rs:1313-1350)
+                if (dv.el_set.empty()) die("design variables of category density need an element set");
+                const auto& set = element_set(dv.el_set);
+                if (dv.coefs.size() >= 2 && dv.coefs.size() != set.size())
+                    die("coefficient list provided for a design variable does not match the length of the element "
+                        "set");
+                for (size_t k = 0; k < set.size(); ++k)
+                    rho[set[k]] += (dv.coefs.empty() ? 1.0 : (dv.coefs.size() == 1 ? dv.coefs[0] : dv.coefs[k])) *
+                                   dv.value;
+                continue;
+            }
             if (dv.component < 1 || dv.component > 3) die("design variable component out of range");
             if (dv.el_set.empty()) die("design variables of category %s need an element set", dv.category.c_str());
             const auto& set = element_set(dv.el_set);
@@ -1069,6 +1083,10 @@ struct Job {
             check(asend_element_sens_material(ctx, u.data(), lam.data(), part.data()), "asend_element_sens_material");
             for (size_t i = 0; i < dvars.size(); ++i) {
                 const auto& dv = dvars[i];
+                if (dv.category == "density") {
+                    dLdD[i] = density_gradient(dv, lam);
+                    continue;
+                }
                 const int base = dv.category == "modulus" ? 0 : dv.category == "poissonRatio" ? 3 : 6;
                 const auto& set = element_set(dv.el_set);
                 double acc = 0.0;
@@ -1080,6 +1098,40 @@ struct Job {
             }
         }
         have_grad = true;
+    }
+
+    // dL/dD of a density DV in a static solve: the stiffness does not depend
+    // on density, the loads of the gravitational and centrifugal kind do,
+    // linearly in each element's density, so dR/dD = -sum over those loads of
+    // the load of the DV's elements computed with density = coefficient, and
+    // dL/dD = -lam^T dR/dD (the DV loop of get_obj_gradient,
+    // S/model_analysis.rs:1679-1701, with d_relasticd_d's load term
+    // :1379-1457)
+    double density_gradient(const DesignVar& dv, const std::vector<double>& lam) {
+        const auto& set = element_set(dv.el_set);
+        std::vector<double> w(conn.size(), 0.0);
+        for (size_t k = 0; k < set.size(); ++k)
+            w[set[k]] += dv.coefs.empty() ? 1.0 : (dv.coefs.size() == 1 ? dv.coefs[0] : dv.coefs[k]);
+        const size_t nn = xyz.size() / 3, ne = conn.size();
+        std::vector<int32_t> cflat(8 * ne);
+        for (size_t e = 0; e < ne; ++e)
+            for (int a = 0; a < 8; ++a) cflat[8 * e + a] = conn[e][a];
+        const double t = solve_cmd.load_times.empty() ? 0.0 : solve_cmd.load_times.back();
+        double g = 0.0;
+        std::vector<double> df;
+        for (const auto& l : loads) {
+            if (!(t >= l.active[0] && t <= l.active[1])) continue;
+            if (l.type != "gravitational" && l.type != "centrifugal") continue;
+            const auto& lset = element_set(l.el_set);
+            df.assign(n_dof, 0.0);
+            const int rc = asend_body_load(etype, l.type == "gravitational" ? 1 : 2, (int64_t)nn, xyz.data(),
+                                           (int64_t)ne, cflat.data(), dof_index.data(), (int64_t)lset.size(),
+                                           lset.data(), w.data(), l.load, l.center, l.axis, l.ang_vel, n_dof,
+                                           df.data());
+            if (rc != ASEND_OK) die("%s load failed (%d)", l.type.c_str(), rc);
+            for (int64_t i = 0; i < n_dof; ++i) g += lam[i] * df[i];
+        }
+        return g;
     }
 
     // ------------------------------------------------------------ writers

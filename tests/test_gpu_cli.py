@@ -189,3 +189,89 @@ def test_cli_dynamic_state_files(tmp_path):
         rf = K @ u + M @ a + D @ v
         assert np.abs(vals[:, 18:21] - rf[di]).max() <= 1e-5 * max(np.abs(rf).max(), 1e-300)
         assert np.all(vals[:, -1] == float(f"{2e-4 * ts:.12e}"))
+
+
+def write_hexbeam_job(work, load, dvars, obj_comp):
+    """hexBeam (examples/testCases/hexBeam: 10 x 1 x 0.1, E = 1e6, nu = 0,
+    rho = 1) as 100 x 10 x 1 brick8, clamped at x = 0, one body load, the
+    given design variables and a displacement objective on the tip nodes,
+    written in the layout of the reference's Python writers."""
+    import yaml
+    from asend3d_amd.meshgen import hex_block
+
+    def dump(name, data):
+        s = yaml.dump(data, sort_keys=False, width=200).replace("'", "").replace('"', "")
+        (work / name).write_text(s)
+
+    m = hex_block(100, 10, 1, 10.0, 1.0, 0.1)
+    model = {
+        "nodes": [str([i] + [float(v) for v in m.xyz[i]]) for i in range(m.n_nodes)],
+        "elements": [{"type": "brick8", "connectivity": [str([e] + [int(v) for v in m.conn[e]]) for e in range(m.n_el)]}],
+        "sets": {"node": {"xMin": [int(i) for i in m.nodes_at_x(0)], "xMax": [int(i) for i in m.nodes_at_x(m.nx)]}},
+        "sections": [{"type": "solid", "material": "mat", "elementSet": "all"}],
+        "materials": {"mat": {"density": 1.0, "elastic": {"E": str([1e6] * 3), "nu": str([0.0] * 3),
+                                                         "G": str([5e5] * 3)}}},
+    }
+    dump("model.yaml", model)
+    dump("constraints.yaml", {"constraints": [{"type": "displacement", "terms": [{"nodeSet": "xMin", "dof": d,
+                                                                                  "coef": 1.0}], "rhs": 0.0}
+                                              for d in (1, 2, 3)]})
+    dump("loads.yaml", {"loads": [load]})
+    dump("dvars.yaml", {"designVariables": dvars})
+    dump("objective.yaml", {"objectiveTerms": [{"category": "displacement", "operator": "powerNorm",
+                                                "component": obj_comp, "nodeSet": "xMax", "coefficient": 1.0,
+                                                "exponent": 1.0, "targetValue": 0.0}]})
+    (work / "results").mkdir()
+    job = {"jobCommands": [
+        {"command": "readModelInput", "fileName": "model.yaml"},
+        {"command": "readConstraints", "fileName": "constraints.yaml"},
+        {"command": "readLoads", "fileName": "loads.yaml"},
+        {"command": "readDesignVarInput", "fileName": "dvars.yaml"},
+        {"command": "readObjectiveInput", "fileName": "objective.yaml"},
+        {"command": "solve", "solverMethod": "direct", "solnHistDir": "results"},
+        {"command": "calcObjGradient"},
+        {"command": "writeObjective", "fileName": "results/objective.csv"},
+    ]}
+    dump("job.yaml", job)
+    return m
+
+
+@pytest.mark.parametrize("case", ["gravitational", "centrifugal"])
+def test_cli_hexbeam_density_and_modulus_sensitivities(tmp_path, case):
+    """hexBeam tests 3 and 5 (examples/testCases/hexBeam/notes.txt): the body
+    loads scale with density and the deflection with the inverse of the
+    stiffness, so for the tip displacement objective J: dJ/drho = J / rho
+    (the known answers dw/drho = 1.5 and du/drho = 3.33e-4 per node are w/rho
+    and u/rho) and, scaling the three moduli and three shear moduli together,
+    E dJ/dE + G dJ/dG = -J."""
+    gpu_case(f"asend_run_job hexBeam {case} density sensitivity")
+    if case == "gravitational":
+        load = {"type": "gravitational", "elementSet": "all", "load": str([0.0, 0.0, -1.0])}
+        comp = 3
+    else:
+        load = {"type": "centrifugal", "elementSet": "all", "load": str([0.0, 0.0, 0.0]),
+                "center": str([0.0, 0.0, 0.0]), "axis": str([0.0, 0.0, 1.0]), "angularVelocity": 1.0}
+        comp = 1
+    dvars = [{"category": "density", "elementSet": "all", "coefficients": 1.0}] + \
+            [{"category": cat, "component": c, "elementSet": "all", "coefficients": 1.0}
+             for cat in ("modulus", "shearModulus") for c in (1, 2, 3)]
+    write_hexbeam_job(tmp_path, load, dvars, comp)
+    r = subprocess.run([CLI, "job.yaml"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0, r.stderr
+    _, rows = read_csv(tmp_path / "results" / "objective.csv")
+    J = float(rows[0][1])
+    _, rows = read_csv(tmp_path / "results" / "objective_grad.csv")
+    g = np.array([float(x[1]) for x in rows])
+    # per tip node: beam theory w = 1.5 (gravity, 22 tip nodes), u = 3.33e-4
+    # (centrifugal, rho omega^2 L^3 / 3E); the one-element-thick brick mesh
+    # locks in bending, so only the order of magnitude is checked for w
+    per_node = J / 22.0
+    if case == "gravitational":
+        assert -1.6 < per_node < -0.5, per_node
+    else:
+        assert abs(per_node - 3.3333e-4) <= 0.02 * 3.3333e-4, per_node
+    assert abs(g[0] - J / 1.0) <= 1e-8 * abs(J), (g[0], J)
+    # scaling every modulus and shear modulus by s scales the stiffness: dJ/ds = -J
+    dJds = 1e6 * g[1:4].sum() + 5e5 * g[4:7].sum()
+    assert abs(dJds + J) <= 1e-7 * abs(J), (dJds, -J)
