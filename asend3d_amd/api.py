@@ -150,7 +150,8 @@ EXPORTED = ["asend_ctx_create", "asend_ctx_destroy", "asend_last_error", "asend_
             "asend_upload_mesh", "asend_set_thickness", "asend_set_grid", "asend_update_moduli", "asend_set_penalty",
             "asend_penalty_scale", "asend_uniform_elements", "asend_assemble", "asend_get_matrix", "asend_spmv", "asend_solve", "asend_solve_noalloc",
             "asend_ldl_factor", "asend_ldl_solve", "asend_ldl_release",
-            "asend_element_sens", "asend_element_sens_material", "asend_static_compliance", "asend_bench_spmv", "asend_mg_info", "asend_mg_apply",
+            "asend_element_sens", "asend_element_sens_material",
+            "asend_element_centre_fields", "asend_element_centre_adjoint", "asend_static_compliance", "asend_bench_spmv", "asend_mg_info", "asend_mg_apply",
             "asend_mg_restrict", "asend_mg_prolong", "asend_mg_vcycle", "asend_mg_coarse_solve", "asend_bench_copy",
             "asend_learned_num_params", "asend_learned_init", "asend_learned_get_params",
             "asend_learned_set_params", "asend_learned_train", "asend_learned_eval", "asend_learned_apply",
@@ -425,6 +426,20 @@ class Context:
 
     def ldl_release(self):
         self._chk(_lib.asend_ldl_release(self.h), "asend_ldl_release")
+
+    def element_centre_fields(self, u):
+        """(n_el, 13): centre strains (6), stresses (6), element volume (brick8)."""
+        u = _f64(u)
+        out = np.zeros((self.n_el, 13))
+        self._chk(_lib.asend_element_centre_fields(self.h, _p(u), _p(out)), "asend_element_centre_fields")
+        return out
+
+    def element_centre_adjoint(self, h):
+        """sum over elements of B_c^T h_e (h: (n_el, 6) strain-space weights)."""
+        h = _f64(np.asarray(h, dtype=np.float64).reshape(self.n_el, 6))
+        y = np.zeros(self.n_dof)
+        self._chk(_lib.asend_element_centre_adjoint(self.h, _p(h), _p(y)), "asend_element_centre_adjoint")
+        return y
 
     def element_sens_material(self, u, lam):
         """(n_el, 9) partials of -lambda^T K_e u w.r.t. E1, E2, E3, nu12, nu13,

@@ -26,8 +26,9 @@
 //     mass), each step solved on the device;
 //     saveSolnHist writes the reference's solnTStep<i>.out state files into
 //     solnHistDir;
-//   - objective terms of category displacement with operator powerNorm
-//     (static solves);
+//   - objective terms of category displacement, and on brick8 stress,
+//     strain and strainEnergy at the element centres, with operators
+//     powerNorm, volumeIntegral and volumeAverage (static solves);
 //   - design variables of category modulus, poissonRatio, shearModulus and
 //     density (brick8) for calcObjGradient;
 //   - writeNodeResults (displacement, velocity, acceleration, reactionForce;
@@ -1032,47 +1033,190 @@ struct Job {
         return rf;
     }
 
-    // objective terms (calculate_terms for displacement / powerNorm,
-    // S/objective_meth.rs:592-640, get_power_norm :329-337)
+    // Objective terms (get_obj_val, S/objective_meth.rs:592-943): categories
+    // displacement (the nodes of nodeSet, component) and stress, strain,
+    // strainEnergy (brick8, at each element's centre, S/objective_meth.rs:
+    // 665-735); operators powerNorm coef (sum (q - tgt)^p)^(1/p) (:329-337),
+    // volumeIntegral coef (sum q V - tgt)^p and volumeAverage
+    // coef (sum q V / sum V - tgt)^p (:422-480); a displacement term weights
+    // every node by 1.  The q of a term, its targets and weights V follow.
+    struct TermData {
+        std::vector<double> q, tgt, vol;
+        std::vector<int64_t> idx;  // DOF (displacement) or element
+    };
+    std::vector<double> cfields;  // centre fields of the current u (13 per element)
+    bool cfields_ok = false;
+
+    bool element_term(const ObjTerm& t) const {
+        return t.category == "stress" || t.category == "strain" || t.category == "strainEnergy";
+    }
+
+    TermData term_data(const ObjTerm& t) {
+        TermData d;
+        if (t.category == "displacement") {
+            if (t.component < 1 || t.component > dpn) die("objective component out of range");
+            for (int32_t n : node_set(t.nd_set, false)) {
+                d.idx.push_back(dof_index[6 * n + t.component - 1]);
+                d.q.push_back(u[d.idx.back()]);
+                d.vol.push_back(1.0);
+            }
+        } else if (element_term(t)) {
+            if (etype != ASEND_BRICK8) die("objective terms of category %s are implemented for brick8 models",
+                                           t.category.c_str());
+            if (t.category != "strainEnergy" && (t.component < 1 || t.component > 6))
+                die("objective component out of range");
+            if (!cfields_ok) {
+                cfields.assign(13 * conn.size(), 0.0);
+                check(asend_element_centre_fields(ctx, u.data(), cfields.data()), "asend_element_centre_fields");
+                cfields_ok = true;
+            }
+            for (int32_t e : element_set(t.el_set)) {
+                const double* f = &cfields[13 * (size_t)e];
+                double q;
+                if (t.category == "stress") q = f[6 + t.component - 1];
+                else if (t.category == "strain") q = f[t.component - 1];
+                else {
+                    q = 0.0;
+                    for (int k = 0; k < 6; ++k) q += f[6 + k] * f[k];
+                    q *= 0.5;
+                }
+                d.idx.push_back(e);
+                d.q.push_back(q);
+                d.vol.push_back(f[12]);
+            }
+        } else {
+            die("objective terms of category %s are not supported by this solver", t.category.c_str());
+        }
+        if (t.tgt.size() > 1 && t.tgt.size() != d.q.size()) die("target list does not match the set");
+        d.tgt.assign(d.q.size(), t.tgt.empty() ? 0.0 : t.tgt[0]);
+        if (t.tgt.size() > 1) d.tgt = t.tgt;
+        return d;
+    }
+
+    // value of a term and dL/dq of each of its entries
+    double term_value(const ObjTerm& t, const TermData& d, std::vector<double>* w) {
+        const size_t n = d.q.size();
+        if (w) w->assign(n, 0.0);
+        if (t.optr == "powerNorm") {
+            double ps = 0.0;
+            for (size_t k = 0; k < n; ++k) ps += std::pow(d.q[k] - d.tgt[k], t.expnt);
+            if (w) {
+                const double sc = t.coef * std::pow(ps, 1.0 / t.expnt - 1.0);
+                for (size_t k = 0; k < n; ++k) (*w)[k] = sc * std::pow(d.q[k] - d.tgt[k], t.expnt - 1.0);
+            }
+            return t.coef * std::pow(ps, 1.0 / t.expnt);
+        }
+        if (t.optr == "volumeIntegral" || t.optr == "volumeAverage") {
+            double vi = 0.0, tv = 0.0;
+            for (size_t k = 0; k < n; ++k) {
+                vi += d.q[k] * d.vol[k];
+                tv += d.vol[k];
+            }
+            const double tg = t.tgt.empty() ? 0.0 : t.tgt[0];
+            const bool avg = t.optr == "volumeAverage";
+            const double err = (avg ? vi / tv : vi) - tg;
+            if (w) {
+                const double sc = t.coef * t.expnt * std::pow(err, t.expnt - 1.0) / (avg ? tv : 1.0);
+                for (size_t k = 0; k < n; ++k) (*w)[k] = sc * d.vol[k];
+            }
+            return t.coef * std::pow(err, t.expnt);
+        }
+        die("objective operator %s is not supported by this solver", t.optr.c_str());
+    }
+
     void objective_values() {
         if (!solved) die("the objective needs a solution (run solve first)");
         if (dyn_solved) die("objectives of dynamic solves are not supported by this solver");
+        cfields_ok = false;
         for (auto& t : obj) {
-            if (t.category != "displacement" || t.optr != "powerNorm")
-                die("objective terms of category %s with operator %s are not supported", t.category.c_str(),
-                    t.optr.c_str());
-            if (t.component < 1 || t.component > dpn) die("objective component out of range");
-            const auto& set = node_set(t.nd_set, false);
-            const double tg = t.tgt.empty() ? 0.0 : t.tgt[0];
-            if (t.tgt.size() > 1 && t.tgt.size() != set.size()) die("target list does not match the node set");
-            double ps = 0.0;
-            for (size_t k = 0; k < set.size(); ++k) {
-                const double q = u[dof_index[6 * set[k] + t.component - 1]];
-                ps += std::pow(q - (t.tgt.size() > 1 ? t.tgt[k] : tg), t.expnt);
-            }
-            t.value = t.coef * std::pow(ps, 1.0 / t.expnt);
+            const TermData d = term_data(t);
+            t.value = term_value(t, d, nullptr);
         }
     }
 
+    // isotropic C of (E, nu) and dC/dp = -C (dS/dp) C for the nine compliance
+    // parameters p = E1, E2, E3, nu12, nu13, nu23, G12, G13, G23 at that point
+    // (the compliance of get_solid_stiff_dfd0, S/element_properties.rs:728-759)
+    static void iso_c(double E, double nu, double* C) {
+        const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)), mu = E / (2.0 * (1.0 + nu));
+        for (int i = 0; i < 36; ++i) C[i] = 0.0;
+        for (int i = 0; i < 3; ++i) {
+            for (int j = 0; j < 3; ++j) C[6 * i + j] = lam;
+            C[7 * i] += 2.0 * mu;
+            C[7 * (3 + i)] = mu;
+        }
+    }
+    static void dc_dp(double E, double nu, int p, double* dC) {
+        const double G = E / (2.0 * (1.0 + nu));
+        double dS[36] = {}, C[36], T[36];
+        auto sym = [&](int i, int j, double v) { dS[6 * i + j] = dS[6 * j + i] = v; };
+        switch (p) {
+            case 0: sym(0, 0, -1.0 / (E * E)); sym(0, 1, nu / (E * E)); sym(0, 2, nu / (E * E)); break;
+            case 1: sym(1, 1, -1.0 / (E * E)); sym(1, 2, nu / (E * E)); break;
+            case 2: sym(2, 2, -1.0 / (E * E)); break;
+            case 3: sym(0, 1, -1.0 / E); break;
+            case 4: sym(0, 2, -1.0 / E); break;
+            case 5: sym(1, 2, -1.0 / E); break;
+            default: sym(p - 3, p - 3, -1.0 / (G * G)); break;
+        }
+        iso_c(E, nu, C);
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j < 6; ++j) {
+                double a = 0.0;
+                for (int k = 0; k < 6; ++k) a += dS[6 * i + k] * C[6 * k + j];
+                T[6 * i + j] = a;
+            }
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j < 6; ++j) {
+                double a = 0.0;
+                for (int k = 0; k < 6; ++k) a += C[6 * i + k] * T[6 * k + j];
+                dC[6 * i + j] = -a;
+            }
+    }
+
     // adjoint gradient (get_obj_gradient, S/model_analysis.rs:1544-1706):
-    // dL/du of the terms (d_power_normd_u, S/objective_meth.rs:339-366),
-    // K lambda = dL/du, then for each design variable the element partials
-    // of -lambda^T (dK/dp) u times the DV coefficients
+    // dL/du of the terms (getd_ld_u: displacement entries directly, the
+    // element entries through B_c^T of their strain-space weights,
+    // S/objective_meth.rs:944-1160), K lambda = dL/du, then per design
+    // variable the element partials of -lambda^T (dK/dp) u times the DV
+    // coefficients plus the explicit dL/dp of stress and strainEnergy terms
+    // (getd_ld_d, S/objective_meth.rs:1372-1500: dsigma/dp = dC/dp eps)
     void gradient() {
         objective_values();
         std::vector<double> dLdu(n_dof, 0.0);
+        std::vector<TermData> tds;
+        std::vector<std::vector<double>> tws;
+        std::vector<double> h;  // strain-space weights of the element terms
         for (const auto& t : obj) {
-            const auto& set = node_set(t.nd_set, false);
-            const double tg = t.tgt.empty() ? 0.0 : t.tgt[0];
-            double ps = 0.0;
-            std::vector<double> err(set.size());
-            for (size_t k = 0; k < set.size(); ++k) {
-                err[k] = u[dof_index[6 * set[k] + t.component - 1]] - (t.tgt.size() > 1 ? t.tgt[k] : tg);
-                ps += std::pow(err[k], t.expnt);
+            tds.push_back(term_data(t));
+            tws.emplace_back();
+            term_value(t, tds.back(), &tws.back());
+            const TermData& d = tds.back();
+            const std::vector<double>& w = tws.back();
+            if (t.category == "displacement") {
+                for (size_t k = 0; k < d.q.size(); ++k) dLdu[d.idx[k]] += w[k];
+                continue;
             }
-            const double scale = t.coef * std::pow(ps, 1.0 / t.expnt - 1.0);
-            for (size_t k = 0; k < set.size(); ++k)
-                dLdu[dof_index[6 * set[k] + t.component - 1]] += scale * std::pow(err[k], t.expnt - 1.0);
+            if (h.empty()) h.assign(6 * conn.size(), 0.0);
+            for (size_t k = 0; k < d.q.size(); ++k) {
+                const size_t e = (size_t)d.idx[k];
+                const double* f = &cfields[13 * e];
+                double* he = &h[6 * e];
+                if (t.category == "strain") {
+                    he[t.component - 1] += w[k];
+                } else if (t.category == "stress") {
+                    double C[36];
+                    iso_c(E[e], nu[e], C);
+                    for (int j = 0; j < 6; ++j) he[j] += w[k] * C[6 * (t.component - 1) + j];
+                } else {
+                    for (int j = 0; j < 6; ++j) he[j] += w[k] * f[6 + j];
+                }
+            }
+        }
+        if (!h.empty()) {
+            std::vector<double> y(n_dof);
+            check(asend_element_centre_adjoint(ctx, h.data(), y.data()), "asend_element_centre_adjoint");
+            for (int64_t i = 0; i < n_dof; ++i) dLdu[i] += y[i];
         }
         dLdD.assign(dvars.size(), 0.0);
         if (!dvars.empty()) {
@@ -1088,11 +1232,40 @@ struct Job {
                     continue;
                 }
                 const int base = dv.category == "modulus" ? 0 : dv.category == "poissonRatio" ? 3 : 6;
+                const int p = base + dv.component - 1;
                 const auto& set = element_set(dv.el_set);
+                std::vector<double> cw(conn.size(), 0.0);  // DV coefficient of each element
                 double acc = 0.0;
                 for (size_t k = 0; k < set.size(); ++k) {
                     const double c = dv.coefs.empty() ? 1.0 : (dv.coefs.size() == 1 ? dv.coefs[0] : dv.coefs[k]);
-                    acc += c * part[9 * (size_t)set[k] + base + dv.component - 1];
+                    acc += c * part[9 * (size_t)set[k] + p];
+                    cw[set[k]] += c;
+                }
+                // explicit dependence of the element terms on the material
+                for (size_t ti = 0; ti < obj.size(); ++ti) {
+                    const auto& t = obj[ti];
+                    if (t.category != "stress" && t.category != "strainEnergy") continue;
+                    const TermData& d = tds[ti];
+                    for (size_t k = 0; k < d.q.size(); ++k) {
+                        const size_t e = (size_t)d.idx[k];
+                        if (cw[e] == 0.0) continue;
+                        const double* f = &cfields[13 * e];
+                        double dC[36], ds[6];
+                        dc_dp(E[e], nu[e], p, dC);
+                        for (int r = 0; r < 6; ++r) {
+                            ds[r] = 0.0;
+                            for (int c = 0; c < 6; ++c) ds[r] += dC[6 * r + c] * f[c];
+                        }
+                        double dq;
+                        if (t.category == "stress") {
+                            dq = ds[t.component - 1];
+                        } else {
+                            dq = 0.0;
+                            for (int r = 0; r < 6; ++r) dq += ds[r] * f[r];
+                            dq *= 0.5;
+                        }
+                        acc += cw[e] * tws[ti][k] * dq;
+                    }
                 }
                 dLdD[i] = acc;
             }

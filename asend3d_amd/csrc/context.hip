@@ -1897,6 +1897,46 @@ int asend_element_sens_material(asend_ctx* ctx, const double* u, const double* l
     return ASEND_OK;
 }
 
+int asend_element_centre_fields(asend_ctx* ctx, const double* u, double* out) {
+    if (!ctx || !u || !out) return ASEND_ERR_ARG;
+    if (!ctx->d_E) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
+    if (ctx->etype != ASEND_BRICK8)
+        return fail(ctx, ASEND_ERR_UNSUPPORTED, "element centre fields are implemented for brick8 meshes");
+    double* d_out = nullptr;
+    int rc = dalloc(ctx, &d_out, 13 * ctx->n_el);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(ctx->d_x64, u, ctx->n_dof * 8, hipMemcpyHostToDevice, ctx->st));
+    launch_hex8_centre_fields(ctx->st, (int)ctx->n_el, ctx->d_conn_r, ctx->d_xyz_r, ctx->d_E, ctx->d_nu, ctx->d_x64,
+                              d_out);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, d_out, 13 * ctx->n_el * 8, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    dfree(d_out);
+    return ASEND_OK;
+}
+
+int asend_element_centre_adjoint(asend_ctx* ctx, const double* h, double* y) {
+    if (!ctx || !h || !y) return ASEND_ERR_ARG;
+    if (!ctx->d_E) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
+    if (ctx->etype != ASEND_BRICK8)
+        return fail(ctx, ASEND_ERR_UNSUPPORTED, "element centre fields are implemented for brick8 meshes");
+    double *d_h = nullptr, *d_re = nullptr;
+    int rc;
+    if ((rc = dalloc(ctx, &d_h, 6 * ctx->n_el)) || (rc = dalloc(ctx, &d_re, 24 * ctx->n_el))) {
+        dfree(d_h);
+        return rc;
+    }
+    HIPCHK(hipMemcpyAsync(d_h, h, 6 * ctx->n_el * 8, hipMemcpyHostToDevice, ctx->st));
+    launch_hex8_centre_adjoint(ctx->st, (int)ctx->n_el, (int)ctx->n_nodes, ctx->d_conn_r, ctx->d_xyz_r,
+                               ctx->d_n2e_ptr, ctx->d_n2rec, d_h, d_re, ctx->d_x64);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(y, ctx->d_x64, ctx->n_dof * 8, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    dfree(d_h);
+    dfree(d_re);
+    return ASEND_OK;
+}
+
 int asend_static_compliance(asend_ctx* ctx, const double* f, double* u, double* dJdE, double* compliance,
                             const asend_solve_opts* opts, asend_solve_stats* stats) {
     if (!ctx || !f || !opts) return ASEND_ERR_ARG;

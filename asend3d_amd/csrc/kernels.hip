@@ -1585,6 +1585,145 @@ __global__ __launch_bounds__(128) void k_hex8_sens_mat(int n_el, const int32_t* 
     for (int p = 0; p < 9; ++p) out[9 * (int64_t)e + p] = acc[p];
 }
 
+// Element fields at the element centre (natural coordinates 0, 0, 0: the
+// brick8 stress point s_cent of get_stress_strain_dfd0,
+// S/element_soln_fields.rs:1463-1530, S/element_meth.rs:81-83), one thread per
+// element: gradients g_t = J^-T dN_t at the centre, strain
+// (e11, e22, e33, g12, g13, g23) with engineering shear
+// (get_solid_strain_dfd0, :1345-1461), stress = C strain for the element's
+// isotropic C, and the element volume sum_gp detJ (get_volume_dfd0).
+// out[13 e ..]: 6 strains, 6 stresses, volume.
+__device__ __forceinline__ void hex8_centre_grad(const double (&X)[8][3], double (&g)[8][3]) {
+    double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int t = 0; t < 8; ++t) {
+        double sx, sy, sz;
+        hex_sign(t, sx, sy, sz);
+        const double dn[3] = {0.125 * sx, 0.125 * sy, 0.125 * sz};
+        for (int i = 0; i < 3; ++i)
+            for (int c = 0; c < 3; ++c) J[3 * i + c] += X[t][i] * dn[c];
+    }
+    const double c00 = J[4] * J[8] - J[5] * J[7], c01 = J[5] * J[6] - J[3] * J[8], c02 = J[3] * J[7] - J[4] * J[6];
+    const double id = 1.0 / (J[0] * c00 + J[1] * c01 + J[2] * c02);
+    const double Ji[9] = {c00 * id, (J[2] * J[7] - J[1] * J[8]) * id, (J[1] * J[5] - J[2] * J[4]) * id,
+                          c01 * id, (J[0] * J[8] - J[2] * J[6]) * id, (J[2] * J[3] - J[0] * J[5]) * id,
+                          c02 * id, (J[1] * J[6] - J[0] * J[7]) * id, (J[0] * J[4] - J[1] * J[3]) * id};
+    for (int t = 0; t < 8; ++t) {
+        double sx, sy, sz;
+        hex_sign(t, sx, sy, sz);
+        const double dn[3] = {0.125 * sx, 0.125 * sy, 0.125 * sz};
+        for (int j = 0; j < 3; ++j) g[t][j] = dn[0] * Ji[j] + dn[1] * Ji[3 + j] + dn[2] * Ji[6 + j];
+    }
+}
+
+__device__ __forceinline__ void hex8_load_x(const int32_t* __restrict__ conn_r, const double* __restrict__ xyz_r,
+                                            int e, double (&X)[8][3]) {
+    for (int t = 0; t < 8; ++t) {
+        const int node = conn_r[8 * (int64_t)e + t];
+        for (int c = 0; c < 3; ++c) X[t][c] = xyz_r[3 * (int64_t)node + c];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_hex8_centre_fields(int n_el, const int32_t* __restrict__ conn_r,
+                                                            const double* __restrict__ xyz_r,
+                                                            const double* __restrict__ E,
+                                                            const double* __restrict__ nu,
+                                                            const double* __restrict__ u, double* __restrict__ out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_el) return;
+    double X[8][3], g[8][3];
+    hex8_load_x(conn_r, xyz_r, e, X);
+    hex8_centre_grad(X, g);
+    double G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // du_i/dx_j
+    for (int t = 0; t < 8; ++t) {
+        const int node = conn_r[8 * (int64_t)e + t];
+        for (int i = 0; i < 3; ++i) {
+            const double ui = u[3 * (int64_t)node + i];
+            for (int j = 0; j < 3; ++j) G[3 * i + j] += ui * g[t][j];
+        }
+    }
+    const double eps[6] = {G[0], G[4], G[8], G[1] + G[3], G[2] + G[6], G[5] + G[7]};
+    const double Ee = E[e], ne = nu[e];
+    const double lam = Ee * ne / ((1.0 + ne) * (1.0 - 2.0 * ne)), mu = Ee / (2.0 * (1.0 + ne));
+    const double tr = eps[0] + eps[1] + eps[2];
+    double* o = out + 13 * (int64_t)e;
+    for (int k = 0; k < 6; ++k) o[k] = eps[k];
+    for (int k = 0; k < 3; ++k) o[6 + k] = lam * tr + 2.0 * mu * eps[k];
+    for (int k = 3; k < 6; ++k) o[6 + k] = mu * eps[k];
+    // volume: detJ at the 8 Gauss points (weights 1)
+    const double gq = 0.5773502691896257;
+    double vol = 0.0;
+    for (int gp = 0; gp < 8; ++gp) {
+        const double p0 = (gp & 1) ? gq : -gq, p1 = (gp & 2) ? gq : -gq, p2 = (gp & 4) ? gq : -gq;
+        double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int t = 0; t < 8; ++t) {
+            double sx, sy, sz;
+            hex_sign(t, sx, sy, sz);
+            const double fx = 1.0 + sx * p0, fy = 1.0 + sy * p1, fz = 1.0 + sz * p2;
+            const double dn[3] = {0.125 * sx * fy * fz, 0.125 * fx * sy * fz, 0.125 * fx * fy * sz};
+            for (int i = 0; i < 3; ++i)
+                for (int c = 0; c < 3; ++c) J[3 * i + c] += X[t][i] * dn[c];
+        }
+        vol += J[0] * (J[4] * J[8] - J[5] * J[7]) + J[1] * (J[5] * J[6] - J[3] * J[8]) +
+               J[2] * (J[3] * J[7] - J[4] * J[6]);
+    }
+    o[12] = vol;
+}
+
+// Adjoint load of centre-point element quantities: r_e = B_c^T h_e for the
+// element's strain-space weights h_e (6 per element), B_c the centre strain
+// operator of hex8_centre_grad, written as 24 values per element (node t,
+// component i at 3 t + i).
+__global__ __launch_bounds__(256) void k_hex8_centre_adjoint(int n_el, const int32_t* __restrict__ conn_r,
+                                                             const double* __restrict__ xyz_r,
+                                                             const double* __restrict__ h,
+                                                             double* __restrict__ re) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_el) return;
+    double X[8][3], g[8][3];
+    hex8_load_x(conn_r, xyz_r, e, X);
+    hex8_centre_grad(X, g);
+    const double* w = h + 6 * (int64_t)e;
+    double* r = re + 24 * (int64_t)e;
+    for (int t = 0; t < 8; ++t) {
+        r[3 * t + 0] = w[0] * g[t][0] + w[3] * g[t][1] + w[4] * g[t][2];
+        r[3 * t + 1] = w[1] * g[t][1] + w[3] * g[t][0] + w[5] * g[t][2];
+        r[3 * t + 2] = w[2] * g[t][2] + w[4] * g[t][0] + w[5] * g[t][1];
+    }
+}
+
+// Node-row gather of per-element 24-vectors: y[3 r + i] = sum over the
+// records 8 e + a of row r of re[24 e + 3 a + i], in record order.
+__global__ __launch_bounds__(256) void k_elem_vec_gather(int n_rows, const int32_t* __restrict__ n2e_ptr,
+                                                         const int32_t* __restrict__ n2rec,
+                                                         const double* __restrict__ re, double* __restrict__ y) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_rows) return;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int k = n2e_ptr[r]; k < n2e_ptr[r + 1]; ++k) {
+        const int64_t rec = n2rec[k];
+        const int64_t e = rec >> 3, a = rec & 7;
+        for (int i = 0; i < 3; ++i) acc[i] += re[24 * e + 3 * a + i];
+    }
+    for (int i = 0; i < 3; ++i) y[3 * (int64_t)r + i] = acc[i];
+}
+
+void launch_hex8_centre_fields(hipStream_t st, int n_el, const int32_t* conn_r, const double* xyz_r, const double* E,
+                               const double* nu, const double* u, double* out) {
+    if (n_el <= 0) return;
+    hipLaunchKernelGGL(k_hex8_centre_fields, dim3((n_el + 255) / 256), dim3(256), 0, st, n_el, conn_r, xyz_r, E, nu,
+                       u, out);
+}
+
+void launch_hex8_centre_adjoint(hipStream_t st, int n_el, int n_rows, const int32_t* conn_r, const double* xyz_r,
+                                const int32_t* n2e_ptr, const int32_t* n2rec, const double* h, double* re,
+                                double* y) {
+    if (n_el > 0)
+        hipLaunchKernelGGL(k_hex8_centre_adjoint, dim3((n_el + 255) / 256), dim3(256), 0, st, n_el, conn_r, xyz_r, h,
+                           re);
+    hipLaunchKernelGGL(k_elem_vec_gather, dim3((n_rows + 255) / 256), dim3(256), 0, st, n_rows, n2e_ptr, n2rec, re,
+                       y);
+}
+
 void launch_hex8_sens_mat(hipStream_t st, int n_el, const int32_t* conn_r, const double* xyz_r, const double* E,
                           const double* nu, const double* u, const double* lam, double* out) {
     hipLaunchKernelGGL(k_hex8_sens_mat, dim3((n_el + 127) / 128), dim3(128), 0, st, n_el, conn_r, xyz_r, E, nu, u,

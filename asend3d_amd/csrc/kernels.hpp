@@ -192,6 +192,16 @@ void launch_axpy64(hipStream_t st, int64_t n, double alpha, const double* x, dou
 
 // per-element partials of -lambda^T K_e u w.r.t. (E1, E2, E3, nu12, nu13, nu23,
 // G12, G13, G23) of the compliance, out[9 e + p]
+// brick8 centre-point strains, stresses (isotropic C of each element) and
+// element volumes from u: out[13 e ..] = 6 strains, 6 stresses, volume.
+void launch_hex8_centre_fields(hipStream_t st, int n_el, const int32_t* conn_r, const double* xyz_r, const double* E,
+                               const double* nu, const double* u, double* out);
+// y = sum over elements of B_c^T h_e (strain-space weights h, 6 per element)
+// gathered per node row (n2e_ptr / n2rec: the records 8 e + a of each row);
+// re is 24 n_el doubles of scratch.
+void launch_hex8_centre_adjoint(hipStream_t st, int n_el, int n_rows, const int32_t* conn_r, const double* xyz_r,
+                                const int32_t* n2e_ptr, const int32_t* n2rec, const double* h, double* re,
+                                double* y);
 void launch_hex8_sens_mat(hipStream_t st, int n_el, const int32_t* conn_r, const double* xyz_r, const double* E,
                           const double* nu, const double* u, const double* lam, double* out);
 void launch_hex8_sens(hipStream_t st, int n_el, const int32_t* conn_r, const double* xyz_r, const double* E,

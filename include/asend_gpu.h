@@ -322,6 +322,23 @@ int asend_element_sens(asend_ctx* ctx, const double* u, const double* lam, doubl
  * is the caller's. */
 int asend_element_sens_material(asend_ctx* ctx, const double* u, const double* lam, double* dJdp);
 
+/* Element quantities at the brick8 stress point (the element centre, s_cent,
+ * S/element_meth.rs:81-83) for stress, strain and strainEnergy objective
+ * terms (get_obj_val, S/objective_meth.rs:651-700, with
+ * get_stress_strain_dfd0, S/element_soln_fields.rs:1463-1530, and
+ * get_volume_dfd0): out[13 e + k] = strain k (11, 22, 33, 12, 13, 23,
+ * engineering shear), out[13 e + 6 + k] = stress k (C of the element's E, nu),
+ * out[13 e + 12] = element volume (sum over the 8 Gauss points of detJ).
+ * u: n_dof displacements.  brick8. */
+int asend_element_centre_fields(asend_ctx* ctx, const double* u, double* out);
+
+/* dL/du of centre-point objective terms (d_qd_u of getd_ld_u for the element
+ * categories, S/objective_meth.rs:1000-1160): y = sum over elements of
+ * B_c^T h_e with h[6 e ..] the strain-space weights of element e (for a
+ * stress term, C^T times the stress weights; for strainEnergy, the weight
+ * times the stress).  y: n_dof.  brick8. */
+int asend_element_centre_adjoint(asend_ctx* ctx, const double* h, double* y);
+
 /* The static compliance pipeline on resident data: assemble, solve K u = f,
  * compliance J = f^T u, adjoint K lambda = dJ/du (= f, so lambda = u without a
  * second solve when the objective is the compliance), dJ/dE.  Replaces
