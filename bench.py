@@ -74,6 +74,7 @@ def reference_algorithm_on_device(ctx, prob, tol):
     6 x solverBlockDim DOFs), stopped at the metric's relative residual.  Its
     iteration count is the one the reference's serial run would take (the
     device restatement takes the oracle's iteration counts on small meshes)."""
+    from asend3d_amd import api
     info = ctx.ldl_factor(6 * prob.block_dim)
     _, st = ctx.solve(prob.f, ctx.opts(api.SOLVE_PCG_FP64, api.PC_LDL, tol=tol, max_it=100000, check_every=16))
     ctx.ldl_release()
