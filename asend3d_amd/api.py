@@ -151,7 +151,7 @@ EXPORTED = ["asend_ctx_create", "asend_ctx_destroy", "asend_last_error", "asend_
             "asend_penalty_scale", "asend_uniform_elements", "asend_assemble", "asend_get_matrix", "asend_spmv", "asend_solve", "asend_solve_noalloc",
             "asend_ldl_factor", "asend_ldl_solve", "asend_ldl_release",
             "asend_element_sens", "asend_element_sens_material",
-            "asend_element_centre_fields", "asend_element_centre_adjoint", "asend_static_compliance", "asend_bench_spmv", "asend_mg_info", "asend_mg_apply",
+            "asend_element_centre_fields", "asend_element_centre_adjoint", "asend_element_sens_coords", "asend_static_compliance", "asend_bench_spmv", "asend_mg_info", "asend_mg_apply",
             "asend_mg_restrict", "asend_mg_prolong", "asend_mg_vcycle", "asend_mg_coarse_solve", "asend_bench_copy",
             "asend_learned_num_params", "asend_learned_init", "asend_learned_get_params",
             "asend_learned_set_params", "asend_learned_train", "asend_learned_eval", "asend_learned_apply",
@@ -426,6 +426,14 @@ class Context:
 
     def ldl_release(self):
         self._chk(_lib.asend_ldl_release(self.h), "asend_ldl_release")
+
+    def element_sens_coords(self, u, lam):
+        """(n_el, 8, 3): lam_e^T (dK_e/dX_{a,c}) u_e (brick8 shape sensitivities)."""
+        u = _f64(u)
+        lam = _f64(lam)
+        out = np.zeros((self.n_el, 8, 3))
+        self._chk(_lib.asend_element_sens_coords(self.h, _p(u), _p(lam), _p(out)), "asend_element_sens_coords")
+        return out
 
     def element_centre_fields(self, u):
         """(n_el, 13): centre strains (6), stresses (6), element volume (brick8)."""

@@ -29,8 +29,9 @@
 //   - objective terms of category displacement, and on brick8 stress,
 //     strain and strainEnergy at the element centres, with operators
 //     powerNorm, volumeIntegral and volumeAverage (static solves);
-//   - design variables of category modulus, poissonRatio, shearModulus and
-//     density (brick8) for calcObjGradient;
+//   - design variables of category modulus, poissonRatio, shearModulus,
+//     density and nodeCoord (brick8; nodeCoord with nodal forces only) for
+//     calcObjGradient;
 //   - writeNodeResults (displacement, velocity, acceleration, reactionForce;
 //     timeSteps read back from the state files) and writeObjective (with
 //     the _grad file), numbers formatted as the reference's {:.12e}.
@@ -656,6 +657,7 @@ struct Job {
             else if (dv.category == "poissonRatio") base = 3;
             else if (dv.category == "shearModulus") base = 6;
             else if (dv.category == "density") base = -1;
+            else if (dv.category == "nodeCoord") continue;  // applied to the coordinates (node_coords)
             else die("design variables of category %s are not supported", dv.category.c_str());
             if (base < 0) {
                 // density = material density + sum of coefficient x value
@@ -696,6 +698,25 @@ struct Job {
                     e, p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8]);
             E[e] = Ee;
             nu[e] = ne_;
+        }
+    }
+
+    // coordinates after nodeCoord design variables: initial + sum of
+    // coefficient x value (Node::get_crd_dfd0, S/node_meth.rs:255-282); the
+    // reordering keeps the initial coordinates
+    std::vector<double> xyz_dv;
+    void node_coords() {
+        xyz_dv = xyz;
+        for (const auto& dv : dvars) {
+            if (dv.category != "nodeCoord") continue;
+            if (dv.component < 1 || dv.component > 3) die("design variable component out of range");
+            if (dv.nd_set.empty()) die("design variables of category nodeCoord need a node set");
+            const auto& set = node_set(dv.nd_set, false);
+            if (dv.coefs.size() >= 2 && dv.coefs.size() != set.size())
+                die("coefficient list provided for a design variable does not match the length of the node set");
+            for (size_t k = 0; k < set.size(); ++k)
+                xyz_dv[3 * (size_t)set[k] + dv.component - 1] +=
+                    (dv.coefs.empty() ? 1.0 : (dv.coefs.size() == 1 ? dv.coefs[0] : dv.coefs[k])) * dv.value;
         }
     }
 
@@ -746,8 +767,9 @@ struct Job {
             for (int32_t n : node_set(k.terms[0].set, false)) pen.push_back(dof_index[6 * n + dof - 1]);
         }
         element_materials();
+        node_coords();
         if (!ctx) check(asend_ctx_create(0, &ctx), "asend_ctx_create");
-        check(asend_upload_mesh(ctx, etype, (int64_t)nn, xyz.data(), dof_index.data(), n_dof, (int64_t)ne,
+        check(asend_upload_mesh(ctx, etype, (int64_t)nn, xyz_dv.data(), dof_index.data(), n_dof, (int64_t)ne,
                                 cflat.data(), E.data(), nu.data(), rho.data()),
               "asend_upload_mesh");
         if (etype == ASEND_SHELL4) check(asend_set_thickness(ctx, thk.data()), "asend_set_thickness");
@@ -764,11 +786,11 @@ struct Job {
                 if (etype == ASEND_SHELL4) die("%s on shell elements is not supported", l.type.c_str());
                 const auto& set = element_set(l.el_set);
                 if (l.type == "surfaceTraction")
-                    rc = asend_surface_traction(etype, (int64_t)nn, xyz.data(), (int64_t)ne, cflat.data(),
+                    rc = asend_surface_traction(etype, (int64_t)nn, xyz_dv.data(), (int64_t)ne, cflat.data(),
                                                 dof_index.data(), (int64_t)set.size(), set.data(), l.load,
                                                 l.norm_dir, l.norm_tol, n_dof, lf);
                 else
-                    rc = asend_surface_pressure(etype, (int64_t)nn, xyz.data(), (int64_t)ne, cflat.data(),
+                    rc = asend_surface_pressure(etype, (int64_t)nn, xyz_dv.data(), (int64_t)ne, cflat.data(),
                                                 dof_index.data(), (int64_t)set.size(), set.data(), l.load[0],
                                                 l.norm_dir, l.norm_tol, n_dof, lf);
                 if (rc != ASEND_OK) die("%s load failed (%d)", l.type.c_str(), rc);
@@ -776,7 +798,7 @@ struct Job {
                 if (etype == ASEND_SHELL4) die("%s on shell elements is not supported", l.type.c_str());
                 const auto& set = element_set(l.el_set);
                 const int kind = l.type == "bodyForce" ? 0 : l.type == "gravitational" ? 1 : 2;
-                rc = asend_body_load(etype, kind, (int64_t)nn, xyz.data(), (int64_t)ne, cflat.data(), dof_index.data(),
+                rc = asend_body_load(etype, kind, (int64_t)nn, xyz_dv.data(), (int64_t)ne, cflat.data(), dof_index.data(),
                                      (int64_t)set.size(), set.data(), rho.data(), l.load, l.center, l.axis, l.ang_vel,
                                      n_dof, lf);
                 if (rc != ASEND_OK) die("%s load failed (%d)", l.type.c_str(), rc);
@@ -1183,6 +1205,7 @@ struct Job {
     // (getd_ld_d, S/objective_meth.rs:1372-1500: dsigma/dp = dC/dp eps)
     void gradient() {
         objective_values();
+        shape_ok = false;
         std::vector<double> dLdu(n_dof, 0.0);
         std::vector<TermData> tds;
         std::vector<std::vector<double>> tws;
@@ -1231,6 +1254,10 @@ struct Job {
                     dLdD[i] = density_gradient(dv, lam);
                     continue;
                 }
+                if (dv.category == "nodeCoord") {
+                    dLdD[i] = coord_gradient(dv, lam);
+                    continue;
+                }
                 const int base = dv.category == "modulus" ? 0 : dv.category == "poissonRatio" ? 3 : 6;
                 const int p = base + dv.component - 1;
                 const auto& set = element_set(dv.el_set);
@@ -1273,6 +1300,35 @@ struct Job {
         have_grad = true;
     }
 
+    // dL/dD of a nodeCoord DV: -lam^T dR/dD with dR/dD = sum over the set's
+    // nodes of coefficient x d(K u)/dX (the device's element shape
+    // sensitivities; constraints do not depend on the coordinates).  Loads
+    // that depend on the geometry (everything but nodalForce) are refused.
+    std::vector<double> shape_s;
+    bool shape_ok = false;
+    double coord_gradient(const DesignVar& dv, const std::vector<double>& lam) {
+        const double t = solve_cmd.load_times.empty() ? 0.0 : solve_cmd.load_times.back();
+        for (const auto& l : loads)
+            if (t >= l.active[0] && t <= l.active[1] && l.type != "nodalForce")
+                die("nodeCoord design variables with %s loads are not supported", l.type.c_str());
+        if (!shape_ok) {
+            shape_s.assign(24 * conn.size(), 0.0);
+            check(asend_element_sens_coords(ctx, u.data(), lam.data(), shape_s.data()), "asend_element_sens_coords");
+            shape_ok = true;
+        }
+        std::vector<double> w(xyz.size() / 3, 0.0);  // coefficient of each node
+        const auto& set = node_set(dv.nd_set, false);
+        for (size_t k = 0; k < set.size(); ++k)
+            w[set[k]] += dv.coefs.empty() ? 1.0 : (dv.coefs.size() == 1 ? dv.coefs[0] : dv.coefs[k]);
+        double g = 0.0;
+        for (size_t e = 0; e < conn.size(); ++e)
+            for (int a = 0; a < 8; ++a) {
+                const double c = w[conn[e][a]];
+                if (c != 0.0) g -= c * shape_s[24 * e + 3 * a + dv.component - 1];
+            }
+        return g;
+    }
+
     // dL/dD of a density DV in a static solve: the stiffness does not depend
     // on density, the loads of the gravitational and centrifugal kind do,
     // linearly in each element's density, so dR/dD = -sum over those loads of
@@ -1297,7 +1353,7 @@ struct Job {
             if (l.type != "gravitational" && l.type != "centrifugal") continue;
             const auto& lset = element_set(l.el_set);
             df.assign(n_dof, 0.0);
-            const int rc = asend_body_load(etype, l.type == "gravitational" ? 1 : 2, (int64_t)nn, xyz.data(),
+            const int rc = asend_body_load(etype, l.type == "gravitational" ? 1 : 2, (int64_t)nn, xyz_dv.data(),
                                            (int64_t)ne, cflat.data(), dof_index.data(), (int64_t)lset.size(),
                                            lset.data(), w.data(), l.load, l.center, l.axis, l.ang_vel, n_dof,
                                            df.data());

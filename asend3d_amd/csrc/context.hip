@@ -1897,6 +1897,26 @@ int asend_element_sens_material(asend_ctx* ctx, const double* u, const double* l
     return ASEND_OK;
 }
 
+int asend_element_sens_coords(asend_ctx* ctx, const double* u, const double* lam, double* out) {
+    if (!ctx || !u || !lam || !out) return ASEND_ERR_ARG;
+    if (!ctx->d_E) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
+    if (ctx->etype != ASEND_BRICK8)
+        return fail(ctx, ASEND_ERR_UNSUPPORTED, "shape sensitivities are implemented for brick8 meshes");
+    const int64_t n = ctx->n_dof;
+    double* d_out = nullptr;
+    int rc = dalloc(ctx, &d_out, 24 * ctx->n_el);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(ctx->d_x64, u, n * 8, hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(hipMemcpyAsync(ctx->d_lam, lam, n * 8, hipMemcpyHostToDevice, ctx->st));
+    launch_hex8_shape_sens(ctx->st, (int)ctx->n_el, ctx->d_conn_r, ctx->d_xyz_r, ctx->d_E, ctx->d_nu, ctx->d_x64,
+                           ctx->d_lam, d_out);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, d_out, 24 * ctx->n_el * 8, hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    dfree(d_out);
+    return ASEND_OK;
+}
+
 int asend_element_centre_fields(asend_ctx* ctx, const double* u, double* out) {
     if (!ctx || !u || !out) return ASEND_ERR_ARG;
     if (!ctx->d_E) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");

@@ -1707,6 +1707,111 @@ __global__ __launch_bounds__(256) void k_elem_vec_gather(int n_rows, const int32
     for (int i = 0; i < 3; ++i) y[3 * (int64_t)r + i] = acc[i];
 }
 
+// Shape sensitivities of the brick8 element energy form: for element e and
+// node coordinate X_{a,c} (direction d = 3 a + c, one thread per (e, d)),
+// out[24 e + d] = lam_e^T (dK_e/dX_{a,c}) u_e, by forward-mode dual numbers
+// through the Jacobian, its inverse, the physical gradients and detJ at the 8
+// Gauss points of K_e = sum_gp detJ B^T C B (isotropic C of E_e, nu_e): the
+// d_relasticd_d coordinate duals of nodeCoord design variables
+// (S/model_analysis.rs:1379-1457, Node::get_crd_dfd0 S/node_meth.rs:255-282).
+struct Dn {
+    double v, d;
+};
+__device__ __forceinline__ Dn dn_add(Dn a, Dn b) { return {a.v + b.v, a.d + b.d}; }
+__device__ __forceinline__ Dn dn_sub(Dn a, Dn b) { return {a.v - b.v, a.d - b.d}; }
+__device__ __forceinline__ Dn dn_mul(Dn a, Dn b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ __forceinline__ Dn dn_scale(Dn a, double s) { return {a.v * s, a.d * s}; }
+
+__global__ __launch_bounds__(256) void k_hex8_shape_sens(int n_el, const int32_t* __restrict__ conn_r,
+                                                         const double* __restrict__ xyz_r,
+                                                         const double* __restrict__ E, const double* __restrict__ nu,
+                                                         const double* __restrict__ u, const double* __restrict__ lam,
+                                                         double* __restrict__ out) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= 24 * (int64_t)n_el) return;
+    const int e = (int)(tid / 24), dir = (int)(tid % 24);
+    const int sa = dir / 3, sc = dir % 3;  // seeded node and coordinate
+    double X[8][3], U[8][3], Lm[8][3];
+    for (int t = 0; t < 8; ++t) {
+        const int node = conn_r[8 * (int64_t)e + t];
+        for (int c = 0; c < 3; ++c) {
+            X[t][c] = xyz_r[3 * (int64_t)node + c];
+            U[t][c] = u[3 * (int64_t)node + c];
+            Lm[t][c] = lam[3 * (int64_t)node + c];
+        }
+    }
+    const double Ee = E[e], ne = nu[e];
+    const double lm = Ee * ne / ((1.0 + ne) * (1.0 - 2.0 * ne)), mu = Ee / (2.0 * (1.0 + ne));
+    const double g = 0.5773502691896257;
+    double acc = 0.0;
+    for (int gp = 0; gp < 8; ++gp) {
+        const double p0 = (gp & 1) ? g : -g, p1 = (gp & 2) ? g : -g, p2 = (gp & 4) ? g : -g;
+        double dn[8][3];
+        for (int t = 0; t < 8; ++t) {
+            double sx, sy, sz;
+            hex_sign(t, sx, sy, sz);
+            const double fx = 1.0 + sx * p0, fy = 1.0 + sy * p1, fz = 1.0 + sz * p2;
+            dn[t][0] = 0.125 * sx * fy * fz;
+            dn[t][1] = 0.125 * fx * sy * fz;
+            dn[t][2] = 0.125 * fx * fy * sz;
+        }
+        // J[i][c] = sum_t X_t[i] dn_t[c]; its derivative along X_{sa,sc} is
+        // dn_sa[c] in row sc
+        Dn J[9];
+        for (int i = 0; i < 3; ++i)
+            for (int c = 0; c < 3; ++c) {
+                double v = 0.0;
+                for (int t = 0; t < 8; ++t) v += X[t][i] * dn[t][c];
+                J[3 * i + c] = {v, i == sc ? dn[sa][c] : 0.0};
+            }
+        const Dn c00 = dn_sub(dn_mul(J[4], J[8]), dn_mul(J[5], J[7]));
+        const Dn c01 = dn_sub(dn_mul(J[5], J[6]), dn_mul(J[3], J[8]));
+        const Dn c02 = dn_sub(dn_mul(J[3], J[7]), dn_mul(J[4], J[6]));
+        const Dn det = dn_add(dn_add(dn_mul(J[0], c00), dn_mul(J[1], c01)), dn_mul(J[2], c02));
+        const Dn id = {1.0 / det.v, -det.d / (det.v * det.v)};
+        const Dn Ji[9] = {dn_mul(c00, id),
+                          dn_mul(dn_sub(dn_mul(J[2], J[7]), dn_mul(J[1], J[8])), id),
+                          dn_mul(dn_sub(dn_mul(J[1], J[5]), dn_mul(J[2], J[4])), id),
+                          dn_mul(c01, id),
+                          dn_mul(dn_sub(dn_mul(J[0], J[8]), dn_mul(J[2], J[6])), id),
+                          dn_mul(dn_sub(dn_mul(J[2], J[3]), dn_mul(J[0], J[5])), id),
+                          dn_mul(c02, id),
+                          dn_mul(dn_sub(dn_mul(J[1], J[6]), dn_mul(J[0], J[7])), id),
+                          dn_mul(dn_sub(dn_mul(J[0], J[4]), dn_mul(J[1], J[3])), id)};
+        // displacement gradients of u and lam: sum_t U_t[i] g_t[j],
+        // g_t[j] = sum_c dn_t[c] Ji[c][j]
+        Dn Gu[9], Gl[9];
+        for (int k = 0; k < 9; ++k) Gu[k] = Gl[k] = {0.0, 0.0};
+        for (int t = 0; t < 8; ++t) {
+            Dn gt[3];
+            for (int j = 0; j < 3; ++j)
+                gt[j] = dn_add(dn_add(dn_scale(Ji[j], dn[t][0]), dn_scale(Ji[3 + j], dn[t][1])),
+                               dn_scale(Ji[6 + j], dn[t][2]));
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    Gu[3 * i + j] = dn_add(Gu[3 * i + j], dn_scale(gt[j], U[t][i]));
+                    Gl[3 * i + j] = dn_add(Gl[3 * i + j], dn_scale(gt[j], Lm[t][i]));
+                }
+        }
+        const Dn eu[6] = {Gu[0], Gu[4], Gu[8], dn_add(Gu[1], Gu[3]), dn_add(Gu[2], Gu[6]), dn_add(Gu[5], Gu[7])};
+        const Dn el[6] = {Gl[0], Gl[4], Gl[8], dn_add(Gl[1], Gl[3]), dn_add(Gl[2], Gl[6]), dn_add(Gl[5], Gl[7])};
+        const Dn tr = dn_add(dn_add(eu[0], eu[1]), eu[2]);
+        Dn w = {0.0, 0.0};
+        for (int k = 0; k < 3; ++k) w = dn_add(w, dn_mul(el[k], dn_add(dn_scale(tr, lm), dn_scale(eu[k], 2.0 * mu))));
+        for (int k = 3; k < 6; ++k) w = dn_add(w, dn_mul(el[k], dn_scale(eu[k], mu)));
+        acc += det.d * w.v + det.v * w.d;
+    }
+    out[tid] = acc;
+}
+
+void launch_hex8_shape_sens(hipStream_t st, int n_el, const int32_t* conn_r, const double* xyz_r, const double* E,
+                            const double* nu, const double* u, const double* lam, double* out) {
+    if (n_el <= 0) return;
+    const int64_t n = 24 * (int64_t)n_el;
+    hipLaunchKernelGGL(k_hex8_shape_sens, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n_el, conn_r, xyz_r, E,
+                       nu, u, lam, out);
+}
+
 void launch_hex8_centre_fields(hipStream_t st, int n_el, const int32_t* conn_r, const double* xyz_r, const double* E,
                                const double* nu, const double* u, double* out) {
     if (n_el <= 0) return;

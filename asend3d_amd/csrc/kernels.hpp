@@ -192,6 +192,10 @@ void launch_axpy64(hipStream_t st, int64_t n, double alpha, const double* x, dou
 
 // per-element partials of -lambda^T K_e u w.r.t. (E1, E2, E3, nu12, nu13, nu23,
 // G12, G13, G23) of the compliance, out[9 e + p]
+// brick8 shape sensitivities: out[24 e + 3 a + c] = lam_e^T (dK_e/dX_{a,c}) u_e
+// for the 8 nodes a and coordinates c of every element (dual numbers).
+void launch_hex8_shape_sens(hipStream_t st, int n_el, const int32_t* conn_r, const double* xyz_r, const double* E,
+                            const double* nu, const double* u, const double* lam, double* out);
 // brick8 centre-point strains, stresses (isotropic C of each element) and
 // element volumes from u: out[13 e ..] = 6 strains, 6 stresses, volume.
 void launch_hex8_centre_fields(hipStream_t st, int n_el, const int32_t* conn_r, const double* xyz_r, const double* E,

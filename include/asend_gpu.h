@@ -322,6 +322,16 @@ int asend_element_sens(asend_ctx* ctx, const double* u, const double* lam, doubl
  * is the caller's. */
 int asend_element_sens_material(asend_ctx* ctx, const double* u, const double* lam, double* dJdp);
 
+/* Shape sensitivities for nodeCoord design variables (the node-coordinate
+ * duals of d_relasticd_d, S/model_analysis.rs:1379-1457, with
+ * Node::get_crd_dfd0, S/node_meth.rs:255-282): out[24 e + 3 a + c] =
+ * lam_e^T (dK_e/dX_{a,c}) u_e for node a (local index) and coordinate c of
+ * every element e, u and lam in the DOF numbering (n_dof each).  A nodeCoord
+ * DV moving node n along c with coefficient k has dL/dD = -k sum over the
+ * (e, a) at node n of out[24 e + 3 a + c], plus lam^T df/dD for loads that
+ * depend on the geometry.  brick8. */
+int asend_element_sens_coords(asend_ctx* ctx, const double* u, const double* lam, double* out);
+
 /* Element quantities at the brick8 stress point (the element centre, s_cent,
  * S/element_meth.rs:81-83) for stress, strain and strainEnergy objective
  * terms (get_obj_val, S/objective_meth.rs:651-700, with

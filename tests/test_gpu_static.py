@@ -252,3 +252,38 @@ def test_material_sensitivities_match_oracle(ctx):
     iso = p.E * dev[:, 0:3].sum(axis=1) + G * dev[:, 6:9].sum(axis=1)
     dJdE = ctx.element_sens(u, lam)
     assert np.abs(iso - p.E * dJdE).max() <= 1e-11 * np.abs(iso).max()
+
+
+def test_shape_sensitivities_match_finite_differences(ctx):
+    """lam_e^T (dK_e/dX) u_e of every element, node and coordinate (device dual
+    numbers) against central differences of the oracle's element stiffness
+    (get_ruk_dfd0 restated) on a distorted mesh with per-element moduli."""
+    p = problems.distorted(problems.heterogeneous(4, 3, 2, seed=5), seed=8)
+    gpu_case("shape sensitivities 4x3x2 distorted")
+    problems.upload(ctx, p)
+    ctx.assemble()
+    rng = np.random.default_rng(12)
+    u = rng.standard_normal(p.n_dof)
+    lam = rng.standard_normal(p.n_dof)
+    s = ctx.element_sens_coords(u, lam)
+    m = p.mesh
+    di = np.asarray(p.dof_index).reshape(-1, 6)[:, :3]
+    worst = 0.0
+    for e in range(m.n_el):
+        X = m.xyz[m.conn[e]]
+        C = O.solid_cmat(p.E[e], p.nu[e])
+        ue = u[di[m.conn[e]]].reshape(-1)
+        le = lam[di[m.conn[e]]].reshape(-1)
+        h = 1e-6 * np.abs(X).max()
+        fd = np.zeros((8, 3))
+        for a in range(8):
+            for c in range(3):
+                Xp = X.copy()
+                Xm = X.copy()
+                Xp[a, c] += h
+                Xm[a, c] -= h
+                Kp, _ = O.solid_element(8, Xp, C)
+                Km, _ = O.solid_element(8, Xm, C)
+                fd[a, c] = le @ (Kp - Km) @ ue / (2.0 * h)
+        worst = max(worst, np.abs(s[e] - fd).max() / np.abs(fd).max())
+    assert worst <= 1e-6, worst
