@@ -149,7 +149,7 @@ EXPORTED = ["asend_ctx_create", "asend_ctx_destroy", "asend_last_error", "asend_
             "asend_compliance_clamp_check", "asend_surface_pressure", "asend_body_load",
             "asend_upload_mesh", "asend_set_thickness", "asend_set_grid", "asend_update_moduli", "asend_set_penalty",
             "asend_penalty_scale", "asend_uniform_elements", "asend_assemble", "asend_get_matrix", "asend_spmv", "asend_solve", "asend_solve_noalloc",
-            "asend_ldl_factor", "asend_ldl_solve", "asend_ldl_release",
+            "asend_ldl_factor", "asend_ldl_solve", "asend_ldl_release", "asend_cycle_spmv_timing",
             "asend_element_sens", "asend_element_sens_material",
             "asend_element_centre_fields", "asend_element_centre_adjoint", "asend_element_sens_coords", "asend_static_compliance", "asend_bench_spmv", "asend_mg_info", "asend_mg_apply",
             "asend_mg_restrict", "asend_mg_prolong", "asend_mg_vcycle", "asend_mg_coarse_solve", "asend_bench_copy",
@@ -410,6 +410,15 @@ class Context:
         out = np.zeros(self.n_el)
         self._chk(_lib.asend_element_sens(self.h, _p(u), _p(lam), _p(out)), "asend_element_sens")
         return out
+
+    def cycle_spmv_timing(self, enable):
+        """Start (True) or stop (False) the event timing of the cycle's FP16
+        SpMV launches; stop returns (launches, total_ms)."""
+        n = _i64(0)
+        ms = _dbl(0)
+        self._chk(_lib.asend_cycle_spmv_timing(self.h, ctypes.c_int(1 if enable else 0), ctypes.byref(n),
+                                               ctypes.byref(ms)), "asend_cycle_spmv_timing")
+        return None if enable else (int(n.value), float(ms.value))
 
     def ldl_factor(self, block_dofs=0):
         """LDL^T factor of K_eff over diagonal blocks of block_dofs DOFs (0: one

@@ -1785,6 +1785,18 @@ int asend_solve(asend_ctx* ctx, const double* rhs, double* x, const asend_solve_
     return ASEND_OK;
 }
 
+int asend_cycle_spmv_timing(asend_ctx* ctx, int enable, int64_t* launches, double* total_ms) {
+    if (!ctx) return ASEND_ERR_ARG;
+    if (enable) {
+        HIPCHK(hipStreamSynchronize(ctx->st));
+        kernel_timer_start();
+        return ASEND_OK;
+    }
+    if (!launches || !total_ms) return ASEND_ERR_ARG;
+    if (kernel_timer_stop(launches, total_ms)) return fail(ctx, ASEND_ERR_HIP, "cycle SpMV timing events failed");
+    return ASEND_OK;
+}
+
 int asend_ldl_factor(asend_ctx* ctx, int64_t block_dofs, asend_ldl_info* info) {
     if (!ctx) return ASEND_ERR_ARG;
     if (!ctx->assembled) return fail(ctx, ASEND_ERR_STATE, "assemble first");

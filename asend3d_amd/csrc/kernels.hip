@@ -939,8 +939,60 @@ __global__ __launch_bounds__(kSellMaxWaves * kWave) void k_sell3h(int n_rows, in
     }
 }
 
+// Timing of the FP16 cycle SpMV with the residual epilogue (mode 3, the
+// most frequent kernel of the multigrid solve) inside a whole pipeline: while
+// enabled, each launch outside stream capture is bracketed by two events on
+// its own stream; kernel_timer_stop sums their elapsed times.
+namespace {
+struct CycleTimer {
+    bool on = false;
+    std::vector<hipEvent_t> ev;  // pairs
+    size_t used = 0;
+} g_cycle_timer;
+}  // namespace
+
+void kernel_timer_start() {
+    g_cycle_timer.on = true;
+    g_cycle_timer.used = 0;
+}
+
+int kernel_timer_stop(int64_t* launches, double* total_ms) {
+    CycleTimer& t = g_cycle_timer;
+    t.on = false;
+    double ms = 0.0;
+    for (size_t i = 0; i + 1 < t.used; i += 2) {
+        if (hipEventSynchronize(t.ev[i + 1]) != hipSuccess) return -1;
+        float e = 0.0f;
+        if (hipEventElapsedTime(&e, t.ev[i], t.ev[i + 1]) != hipSuccess) return -1;
+        ms += e;
+    }
+    *launches = (int64_t)(t.used / 2);
+    *total_ms = ms;
+    t.used = 0;
+    return 0;
+}
+
+static hipEvent_t cycle_timer_event() {
+    CycleTimer& t = g_cycle_timer;
+    if (t.used == t.ev.size()) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        t.ev.push_back(e);
+    }
+    return t.ev[t.used++];
+}
+
 void launch_sell3_h(hipStream_t st, int mode, const Sell3& A, const __half* val, const float* dinv32, const float* x,
                     float* y, const float* f) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (g_cycle_timer.on && mode == 3) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone) {
+            e0 = cycle_timer_event();
+            e1 = cycle_timer_event();
+            if (e0 && e1) (void)hipEventRecord(e0, st);
+        }
+    }
     const int per = A.waves_per_block;
     const int nblk = (A.n_slices + per - 1) / per;
     const int nthr = per * kWave;
@@ -956,6 +1008,7 @@ void launch_sell3_h(hipStream_t st, int mode, const Sell3& A, const __half* val,
         if (nt) go(k_sell3h<0, true>);
         else go(k_sell3h<0, false>);
     }
+    if (e0 && e1) (void)hipEventRecord(e1, st);
 }
 
 // ---------------------------------------------------------------------------

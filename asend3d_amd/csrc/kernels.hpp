@@ -106,6 +106,10 @@ void launch_sell3(hipStream_t st, int mode, const Sell3& A, const T* val, const 
 // whatever the moduli and the penalty scale), one wavefront per slice.
 void launch_to_f16_rowscaled(hipStream_t st, int n_rows, int n_slices, const int32_t* sptr, const double* K,
                              const double* dinv64, __half* K16);
+// Event timing of the mode-3 FP16 SpMV launches (outside stream capture)
+// between start and stop: launch count and summed kernel time.
+void kernel_timer_start();
+int kernel_timer_stop(int64_t* launches, double* total_ms);
 // SpMV with that copy, FP32 accumulation, the row scale restored from the
 // FP32 Jacobi diagonal: mode 0: y = K x; mode 3: y = f - K x.
 void launch_sell3_h(hipStream_t st, int mode, const Sell3& A, const __half* val, const float* dinv32, const float* x,

@@ -25,6 +25,8 @@
               Newton solves (Eisenstat-Walker forcing).
   general  -- assembly of the 200x40x40 heterogeneous block on a distorted
               mesh (per-element integration and row gather).
+  cycle    -- three multigrid mixed solves of the 200x40x40 block with graphs
+              off (the timed step's solve kernels, for counter runs).
   reference -- the reference's own solvers on the device for the 200x40x40
               heterogeneous block: conj_grad_sparse preconditioned by the
               block LowerTriMat (blocks of 6 x solverBlockDim DOFs) to a 1e-8
@@ -273,6 +275,25 @@ def shell(nx=256, ny=256, steps=1000, dt=1e-5, graphs=True, tol=1e-8, max_it=200
             "centre_w_min_step": int(np.argmin(w))}
 
 
+def cycle(nx=200, ny=40, nz=40, solves=3, tol=1e-8):
+    """Multigrid mixed solves of the heterogeneous block with graphs off: the
+    solve kernels of the bench's timed step without the rest of the bench,
+    for hardware-counter runs."""
+    p = problems.heterogeneous(nx, ny, nz)
+    ctx = api.Context(0)
+    ctx.set_graphs(False)
+    problems.upload(ctx, p)
+    ctx.assemble()
+    ctx.set_grid(nx, ny, nz)
+    o = api.Context.opts(api.SOLVE_IR_MIXED, api.PC_MG, tol=tol, inner_tol=1e-2, check_every=2)
+    runs = []
+    for _ in range(solves):
+        _, st = ctx.solve(p.f, o)
+        runs.append({"ms": st.device_ms, "iterations": int(st.iterations), "rel_residual": st.rel_residual})
+    ctx.close()
+    return {"workload": "cycle", "mesh": [nx, ny, nz], "solves": runs}
+
+
 def reference(nx=200, ny=40, nz=40, tol=1e-8, direct=True, graphs=True):
     out = {"workload": "reference", "mesh": [nx, ny, nz], "tol": tol}
     p = problems.heterogeneous(nx, ny, nz)
@@ -320,7 +341,7 @@ def reference(nx=200, ny=40, nz=40, tol=1e-8, direct=True, graphs=True):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("which", choices=["batched", "newmark", "shell", "adjoint", "modal", "nonlinear", "general",
-                                          "reference"])
+                                          "reference", "cycle"])
     ap.add_argument("--load-steps", type=int, default=5)
     ap.add_argument("--modes", type=int, default=10)
     ap.add_argument("--variants", type=int, default=64)
@@ -339,6 +360,8 @@ def main():
         out = batched(a.variants, a.sample)
     elif a.which == "modal":
         out = modal(a.modes)
+    elif a.which == "cycle":
+        out = cycle(a.nx or 200, a.ny or 40, a.nz or 40)
     elif a.which == "reference":
         out = reference(a.nx or 200, a.ny or 40, a.nz or 40, direct=not a.no_direct, graphs=not a.no_graphs)
     elif a.which == "general":

@@ -257,6 +257,13 @@ int asend_spmv(asend_ctx* ctx, int precision, const double* x, double* y);
 int asend_solve(asend_ctx* ctx, const double* rhs, double* x, const asend_solve_opts* opts,
                 asend_solve_stats* stats);
 
+/* Measurement (no reference counterpart): with enable = 1, every launch of
+ * the multigrid cycle's FP16 SpMV with residual epilogue that is not being
+ * captured into a graph is bracketed by HIP events on its stream; enable = 0
+ * stops and returns the number of launches and their summed kernel time.
+ * For timing that kernel inside a whole pipeline (run it with graphs off). */
+int asend_cycle_spmv_timing(asend_ctx* ctx, int enable, int64_t* launches, double* total_ms);
+
 /* LDL^T factor of the assembled K_eff restricted to diagonal blocks of
  * block_dofs consecutive DOFs (the last block may be shorter; block_dofs <= 0
  * or >= n_dof: one block, an exact factor).  Replaces LowerTriMat
