@@ -146,7 +146,8 @@ _lib.asend_ctx_destroy.argtypes = [_vp]
 
 EXPORTED = ["asend_ctx_create", "asend_ctx_destroy", "asend_last_error", "asend_version", "asend_sync",
             "asend_reorder_nodes", "asend_block_dim_auto", "asend_surface_traction", "asend_lognormal_moduli",
-            "asend_compliance_clamp_check", "asend_surface_pressure", "asend_body_load",
+            "asend_compliance_clamp_check", "asend_surface_pressure", "asend_body_load", "asend_surface_flux",
+            "asend_set_conductivity",
             "asend_upload_mesh", "asend_set_thickness", "asend_set_grid", "asend_update_moduli", "asend_set_penalty",
             "asend_penalty_scale", "asend_uniform_elements", "asend_assemble", "asend_get_matrix", "asend_spmv", "asend_solve", "asend_solve_noalloc",
             "asend_ldl_factor", "asend_ldl_solve", "asend_ldl_release", "asend_cycle_spmv_timing",
@@ -250,6 +251,23 @@ def surface_pressure(etype, xyz, conn, dof_index, n_dof, el_set, pressure, norm_
                                      _i64(n_dof), _p(f))
     if rc != 0:
         raise AsendError(f"asend_surface_pressure failed ({rc})")
+    return f
+
+
+def surface_flux(etype, xyz, conn, dof_index, n_dof, el_set, q, norm_dir, norm_tol_deg, f=None):
+    """asend_surface_flux: the surfaceFlux load into each node's temperature
+    DOF (dof_index[:, 0])."""
+    xyz = _f64(xyz)
+    conn = _i32(conn)
+    di = _i64a(dof_index)
+    es = _i32(el_set)
+    nd = _f64(norm_dir)
+    f = np.zeros(n_dof) if f is None else f
+    rc = _lib.asend_surface_flux(ctypes.c_int(etype), _i64(xyz.shape[0]), _p(xyz), _i64(conn.shape[0]), _p(conn),
+                                 _p(di), _i64(es.shape[0]), _p(es), _dbl(q), _p(nd), _dbl(norm_tol_deg),
+                                 _i64(n_dof), _p(f))
+    if rc != 0:
+        raise AsendError(f"asend_surface_flux failed ({rc})")
     return f
 
 
@@ -564,6 +582,15 @@ class Context:
         idx = None if el_mat is None else np.ascontiguousarray(el_mat, dtype=np.int32)
         self._chk(_lib.asend_set_material_damping(self.h, _i64(d.shape[0]), _p(d), _p(idx)),
                   "asend_set_material_damping")
+
+    def set_conductivity(self, kcond):
+        """Heat conduction (asend_set_conductivity): n_el x 6 conductivities
+        (k11 k22 k33 k12 k13 k23), or None to return to elasticity.  The
+        temperature of node n is DOF dof_index[n, 0]."""
+        k = None if kcond is None else _f64(np.asarray(kcond, dtype=np.float64).reshape(-1, 6))
+        if k is not None and k.shape[0] != self.n_el:
+            raise AsendError(f"conductivities for {k.shape[0]} elements, mesh has {self.n_el}")
+        self._chk(_lib.asend_set_conductivity(self.h, _p(k)), "asend_set_conductivity")
 
     def newmark_setup(self, dt, beta=0.25, gamma=0.5):
         self._chk(_lib.asend_newmark_setup(self.h, _dbl(dt), _dbl(beta), _dbl(gamma)), "asend_newmark_setup")

@@ -135,6 +135,10 @@ struct asend_ctx {
     int32_t* d_dmat_idx = nullptr;
     int64_t n_dmat = 0;
     double asm_cdamp = 0.0;  // weight of the material damping in the current assembly
+    // heat conduction (asend_set_conductivity): 6 conductivities per element;
+    // while set, assemblies integrate the conduction operator (temperature at
+    // the first DOF of each node, the same scalar operator on the other two)
+    double* d_kcond = nullptr;
     // Newmark step solver (asend_newmark_set_solver)
     int nm_mode = ASEND_NM_AUTO;
     double nm_inner_tol = 1e-3;  // reliable-update factor of the mixed step solver
@@ -312,6 +316,7 @@ void free_mesh(asend_ctx* ctx) {
     dfree(ctx->d_dmat);
     dfree(ctx->d_dmat_idx);
     ctx->n_dmat = 0;
+    dfree(ctx->d_kcond);
     dfree(ctx->d_conn_r);
     dfree(ctx->d_rp);
     dfree(ctx->d_bc);
@@ -848,6 +853,10 @@ int solve_direct(asend_ctx* ctx, asend_solve_stats* st) {
 }
 
 int solve_device(asend_ctx* ctx, const asend_solve_opts* o, asend_solve_stats* st) {
+    if (ctx->d_kcond && o->precond != ASEND_PC_JACOBI && o->precond != ASEND_PC_BLOCK_JACOBI &&
+        o->precond != ASEND_PC_LDL)
+        return fail(ctx, ASEND_ERR_UNSUPPORTED,
+                    "conduction solves take the Jacobi or LDL^T preconditioners (the multigrid hierarchy is elastic)");
     if (o->method == ASEND_SOLVE_PCG_FP64) return solve_fp64(ctx, ctx_fp64_bufs(ctx), o, st);
     if (o->method == ASEND_SOLVE_DIRECT) return solve_direct(ctx, st);
     const int64_t n = ctx->n_dof;
@@ -989,7 +998,9 @@ int integrate_elements(asend_ctx* ctx, const double* E, double k_scale, double c
                        double c_damp = 0.0) {
     if (c_damp != 0.0 && (ctx->etype != ASEND_BRICK8 || ctx->n_dmat == 0))
         return fail(ctx, ASEND_ERR_UNSUPPORTED, "material damping is implemented for brick8 meshes");
-    if (ctx->uniform_hex && c_damp == 0.0) {
+    if (ctx->d_kcond && (c_mass != 0.0 || c_damp != 0.0 || k_scale != 1.0))
+        return fail(ctx, ASEND_ERR_UNSUPPORTED, "the conduction operator is assembled alone (no heat capacity term)");
+    if (ctx->uniform_hex && c_damp == 0.0 && !ctx->d_kcond) {
         launch_hex8_assemble_uniform(ctx->st, (int)ctx->n_nodes, (int)ctx->n_slices, ctx->d_sptr, ctx->d_scol,
                                      ctx->d_rp, ctx->d_n2e_ptr, ctx->d_n2e, ctx->d_conn_r, ctx->d_khat, ctx->d_mhat, E,
                                      ctx->d_rho, k_scale, c_mass, K);
@@ -1015,7 +1026,7 @@ int integrate_elements(asend_ctx* ctx, const double* E, double k_scale, double c
             if (rc) return rc;
         }
         launch_hex8_elem_rec(ctx->st, (int)ctx->n_el, ctx->d_conn_r, ctx->d_xyz_r, E, ctx->d_nu, ctx->d_rho, k_scale,
-                             c_mass, ctx->d_dmat, ctx->d_dmat_idx, c_damp, ctx->d_kb);
+                             c_mass, ctx->d_dmat, ctx->d_dmat_idx, c_damp, ctx->d_kcond, ctx->d_kb);
         launch_rec_gather(ctx->st, (int)ctx->n_nodes, (int)ctx->n_slices, ctx->d_sptr, ctx->d_n2e_ptr, ctx->d_n2rec,
                           ctx->d_slot, ctx->d_kb, nullptr, 0.0, nullptr, K, nullptr, nullptr);
     }
@@ -1189,6 +1200,14 @@ Fp64Bufs scratch_fp64_bufs(double* base) {
     B.partials = base + 7 * v;
     B.sc = (Scalars*)(base + 7 * v + kNoAllocPartials);
     return B;
+}
+
+// Entry points built on the elastic operator refuse a context that
+// assembles heat conduction.
+int elastic_only(asend_ctx* ctx, const char* what) {
+    if (!ctx->d_kcond) return ASEND_OK;
+    return fail(ctx, ASEND_ERR_UNSUPPORTED,
+                "%s needs the elastic operator; this context assembles heat conduction (asend_set_conductivity)", what);
 }
 }  // namespace
 
@@ -1634,6 +1653,7 @@ int asend_mg_set_cycle_precision(asend_ctx* ctx, int bits) {
 
 int asend_update_moduli(asend_ctx* ctx, const double* E) {
     if (!ctx || !E) return ASEND_ERR_ARG;
+    if (const int rc0 = elastic_only(ctx, "asend_update_moduli")) return rc0;
     if (!ctx->d_E) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
     {
         int64_t first = -1;
@@ -1876,6 +1896,7 @@ int asend_solve_noalloc(asend_ctx* ctx, const double* rhs, double* x, const asen
 
 int asend_element_sens(asend_ctx* ctx, const double* u, const double* lam, double* dJdE) {
     if (!ctx || !u || !lam || !dJdE) return ASEND_ERR_ARG;
+    if (const int rc0 = elastic_only(ctx, "asend_element_sens")) return rc0;
     if (!ctx->d_E) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
     if (ctx->etype == ASEND_SHELL4)
         return fail(ctx, ASEND_ERR_UNSUPPORTED, "element sensitivities are implemented for brick8 and brickIM meshes");
@@ -1891,6 +1912,7 @@ int asend_element_sens(asend_ctx* ctx, const double* u, const double* lam, doubl
 
 int asend_element_sens_material(asend_ctx* ctx, const double* u, const double* lam, double* dJdp) {
     if (!ctx || !u || !lam || !dJdp) return ASEND_ERR_ARG;
+    if (const int rc0 = elastic_only(ctx, "asend_element_sens_material")) return rc0;
     if (!ctx->d_E) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
     if (ctx->etype != ASEND_BRICK8)
         return fail(ctx, ASEND_ERR_UNSUPPORTED, "material-parameter sensitivities are implemented for brick8 meshes");
@@ -1911,6 +1933,7 @@ int asend_element_sens_material(asend_ctx* ctx, const double* u, const double* l
 
 int asend_element_sens_coords(asend_ctx* ctx, const double* u, const double* lam, double* out) {
     if (!ctx || !u || !lam || !out) return ASEND_ERR_ARG;
+    if (const int rc0 = elastic_only(ctx, "asend_element_sens_coords")) return rc0;
     if (!ctx->d_E) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
     if (ctx->etype != ASEND_BRICK8)
         return fail(ctx, ASEND_ERR_UNSUPPORTED, "shape sensitivities are implemented for brick8 meshes");
@@ -1931,6 +1954,7 @@ int asend_element_sens_coords(asend_ctx* ctx, const double* u, const double* lam
 
 int asend_element_centre_fields(asend_ctx* ctx, const double* u, double* out) {
     if (!ctx || !u || !out) return ASEND_ERR_ARG;
+    if (const int rc0 = elastic_only(ctx, "asend_element_centre_fields")) return rc0;
     if (!ctx->d_E) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
     if (ctx->etype != ASEND_BRICK8)
         return fail(ctx, ASEND_ERR_UNSUPPORTED, "element centre fields are implemented for brick8 meshes");
@@ -1949,6 +1973,7 @@ int asend_element_centre_fields(asend_ctx* ctx, const double* u, double* out) {
 
 int asend_element_centre_adjoint(asend_ctx* ctx, const double* h, double* y) {
     if (!ctx || !h || !y) return ASEND_ERR_ARG;
+    if (const int rc0 = elastic_only(ctx, "asend_element_centre_adjoint")) return rc0;
     if (!ctx->d_E) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
     if (ctx->etype != ASEND_BRICK8)
         return fail(ctx, ASEND_ERR_UNSUPPORTED, "element centre fields are implemented for brick8 meshes");
@@ -1972,6 +1997,7 @@ int asend_element_centre_adjoint(asend_ctx* ctx, const double* h, double* y) {
 int asend_static_compliance(asend_ctx* ctx, const double* f, double* u, double* dJdE, double* compliance,
                             const asend_solve_opts* opts, asend_solve_stats* stats) {
     if (!ctx || !f || !opts) return ASEND_ERR_ARG;
+    if (const int rc0 = elastic_only(ctx, "asend_static_compliance")) return rc0;
     if (!ctx->d_K64) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
     if (ctx->etype == ASEND_SHELL4)
         return fail(ctx, ASEND_ERR_UNSUPPORTED, "element sensitivities are implemented for brick8 and brickIM meshes");
@@ -2002,6 +2028,7 @@ int asend_static_compliance(asend_ctx* ctx, const double* f, double* u, double* 
 
 int asend_nl_assemble(asend_ctx* ctx, const double* u, double* r_int) {
     if (!ctx || !u) return ASEND_ERR_ARG;
+    if (const int rc0 = elastic_only(ctx, "asend_nl_assemble")) return rc0;
     int rc;
     if ((rc = nl_prepare(ctx))) return rc;
     HIPCHK(hipMemcpy(ctx->d_nl_u, u, ctx->n_dof * 8, hipMemcpyHostToDevice));
@@ -2014,6 +2041,7 @@ int asend_nl_assemble(asend_ctx* ctx, const double* u, double* r_int) {
 int asend_static_nonlinear(asend_ctx* ctx, const double* f, double* u, const asend_nl_opts* nl,
                            const asend_solve_opts* opts, asend_nl_stats* stats) {
     if (!ctx || !f || !u || !opts) return ASEND_ERR_ARG;
+    if (const int rc0 = elastic_only(ctx, "asend_static_nonlinear")) return rc0;
     int rc;
     if ((rc = nl_prepare(ctx))) return rc;
     asend_nl_opts o;
@@ -2134,6 +2162,9 @@ extern "C" {
 int asend_batch_create(asend_ctx* const* variants, int n_variants, asend_batch** out) {
     if (!variants || n_variants < 1 || !out) return ASEND_ERR_ARG;
     *out = nullptr;
+    for (int v = 0; v < n_variants; ++v)
+        if (variants[v] && variants[v]->d_kcond)
+            return fail(variants[v], ASEND_ERR_UNSUPPORTED, "batched solves need elastic contexts");
     std::vector<OperatorView> ops(n_variants);
     asend_batch* ab = new asend_batch();
     for (int v = 0; v < n_variants; ++v) {
@@ -2243,6 +2274,28 @@ int asend_set_material_damping(asend_ctx* ctx, int64_t n_mat, const double* dmat
     return ASEND_OK;
 }
 
+int asend_set_conductivity(asend_ctx* ctx, const double* kcond) {
+    if (!ctx) return ASEND_ERR_ARG;
+    if (!ctx->d_K64) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
+    // the operator changes kind: the penalty scale is taken again from the
+    // next assembly (each field has its own constraint list and scale)
+    ctx->sf_fixed = false;
+    ctx->assembled = false;
+    if (!kcond) {
+        dfree(ctx->d_kcond);
+        return ASEND_OK;
+    }
+    if (ctx->etype != ASEND_BRICK8)
+        return fail(ctx, ASEND_ERR_UNSUPPORTED, "heat conduction is implemented for brick8 meshes");
+    for (int64_t i = 0; i < 6 * ctx->n_el; ++i)
+        if (!std::isfinite(kcond[i]))
+            return fail(ctx, ASEND_ERR_ARG, "element %lld: conductivity is not finite", (long long)(i / 6));
+    int rc;
+    if ((rc = dalloc(ctx, &ctx->d_kcond, 6 * ctx->n_el))) return rc;
+    HIPCHK(hipMemcpy(ctx->d_kcond, kcond, 6 * ctx->n_el * sizeof(double), hipMemcpyHostToDevice));
+    return ASEND_OK;
+}
+
 int asend_set_rayleigh_damping(asend_ctx* ctx, double cm, double ck) {
     if (!ctx || !(cm >= 0.0) || !(ck >= 0.0) || !std::isfinite(cm) || !std::isfinite(ck)) return ASEND_ERR_ARG;
     ctx->ray_cm = cm;
@@ -2263,6 +2316,7 @@ int asend_newmark_set_solver(asend_ctx* ctx, int mode, double ru_factor) {
 
 int asend_newmark_setup(asend_ctx* ctx, double dt, double beta, double gamma) {
     if (!ctx || !(dt > 0.0)) return ASEND_ERR_ARG;
+    if (const int rc0 = elastic_only(ctx, "asend_newmark_setup")) return rc0;
     if (!ctx->d_K64) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
     if (!(gamma > beta)) return fail(ctx, ASEND_ERR_ARG, "Newmark needs gamma > beta for an SPD effective operator");
     HIPCHK(hipSetDevice(ctx->device));
@@ -2673,6 +2727,7 @@ static void jacobi_eigen(int m, std::vector<double>& A, std::vector<double>& w, 
 int asend_modal(asend_ctx* ctx, int n_modes, double shift, const asend_solve_opts* opts, double* eig_vals,
                 double* freqs_hz, double* eig_vecs, asend_modal_stats* stats) {
     if (!ctx || n_modes < 1 || !opts || !eig_vals) return ASEND_ERR_ARG;
+    if (const int rc0 = elastic_only(ctx, "asend_modal")) return rc0;
     if (!ctx->d_K64) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
     if (ctx->etype == ASEND_SHELL4 && !ctx->thk_set)
         return fail(ctx, ASEND_ERR_STATE, "shell4 mesh without thickness (asend_set_thickness)");
@@ -2879,6 +2934,7 @@ int asend_learned_num_params(void) { return kNumParams; }
 
 int asend_learned_init(asend_ctx* ctx, uint64_t seed) {
     if (!ctx) return ASEND_ERR_ARG;
+    if (const int rc0 = elastic_only(ctx, "asend_learned_init")) return rc0;
     ctx->learned_aside = false;
     HIPCHK(hipSetDevice(ctx->device));
     invalidate_factor(ctx);
@@ -3129,6 +3185,7 @@ int asend_mg_coarse_solve(asend_ctx* ctx, const float* b, float* x) { return mg_
 // ---------------------------------------------------------------------------
 int asend_autotune(asend_ctx* ctx, const double* rhs, int what, asend_tuning* out) {
     if (!ctx || !out) return ASEND_ERR_ARG;
+    if (const int rc0 = elastic_only(ctx, "asend_autotune")) return rc0;
     if (!ctx->assembled) return fail(ctx, ASEND_ERR_STATE, "assemble first");
     std::memset(out, 0, sizeof(*out));
     int rc;

@@ -189,22 +189,14 @@ static double hex_detj(const double* X /*[8][3]*/, const double* s, double* N) {
            J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
 }
 
-// surfaceTraction (traction = load) and surfacePressure (traction = -p n of
-// each face, pressure != nullptr) share the face selection and distribution
-static int surface_load(int elem_type, int64_t n_nodes, const double* xyz, int64_t n_el, const int32_t* conn,
-                        const int64_t* dof_index, int64_t n_set, const int32_t* el_set, const double* load,
-                        const double* pressure, const double* norm_dir_in, double norm_tol_deg, int64_t n_dof,
-                        double* f) {
-    if (elem_type != ASEND_BRICK8 && elem_type != ASEND_BRICKIM) return ASEND_ERR_UNSUPPORTED;
-    // the load's normal direction is stored normalised (Load::set_norm_dir, S/load_meth.rs:22-32)
-    const double nd2 = norm_dir_in[0] * norm_dir_in[0] + norm_dir_in[1] * norm_dir_in[1] + norm_dir_in[2] * norm_dir_in[2];
-    if (!(nd2 > 0.0)) return ASEND_ERR_ARG;
-    const double ndm = 1.0 / std::sqrt(nd2);
-    const double norm_dir[3] = {ndm * norm_dir_in[0], ndm * norm_dir_in[1], ndm * norm_dir_in[2]};
-    // hex face local nodes (S/element_meth.rs:537-590)
-    static const int fl[6][4] = {{3, 2, 1, 0}, {4, 5, 6, 7}, {0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7}};
-    // A face lies on the surface when no other face has the same node set
-    // (S/model_prep.rs:510-554).  Faces are bucketed by their sorted nodes.
+// hex face local nodes (S/element_meth.rs:537-590)
+static const int kHexFace[6][4] = {{3, 2, 1, 0}, {4, 5, 6, 7}, {0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7}};
+
+// on_surf[6 e + k]: face k of hex element e lies on the surface, i.e. no
+// other face has the same node set (S/model_prep.rs:510-554).  Faces are
+// bucketed by their sorted nodes.
+static std::vector<uint8_t> hex_surface_faces(int64_t n_el, const int32_t* conn) {
+    const auto& fl = kHexFace;
     struct Key {
         int32_t n[4];
         bool operator<(const Key& o) const { return std::lexicographical_compare(n, n + 4, o.n, o.n + 4); }
@@ -228,6 +220,23 @@ static int surface_load(int elem_type, int64_t n_nodes, const double* xyz, int64
             on_surf[keys[i].second] = 0;
             on_surf[keys[i + 1].second] = 0;
         }
+    return on_surf;
+}
+
+// surfaceTraction (traction = load) and surfacePressure (traction = -p n of
+// each face, pressure != nullptr) share the face selection and distribution
+static int surface_load(int elem_type, int64_t n_nodes, const double* xyz, int64_t n_el, const int32_t* conn,
+                        const int64_t* dof_index, int64_t n_set, const int32_t* el_set, const double* load,
+                        const double* pressure, const double* norm_dir_in, double norm_tol_deg, int64_t n_dof,
+                        double* f) {
+    if (elem_type != ASEND_BRICK8 && elem_type != ASEND_BRICKIM) return ASEND_ERR_UNSUPPORTED;
+    // the load's normal direction is stored normalised (Load::set_norm_dir, S/load_meth.rs:22-32)
+    const double nd2 = norm_dir_in[0] * norm_dir_in[0] + norm_dir_in[1] * norm_dir_in[1] + norm_dir_in[2] * norm_dir_in[2];
+    if (!(nd2 > 0.0)) return ASEND_ERR_ARG;
+    const double ndm = 1.0 / std::sqrt(nd2);
+    const double norm_dir[3] = {ndm * norm_dir_in[0], ndm * norm_dir_in[1], ndm * norm_dir_in[2]};
+    const auto& fl = kHexFace;
+    const std::vector<uint8_t> on_surf = hex_surface_faces(n_el, conn);
     const double cos_tol = std::cos(0.017453292519943295 * norm_tol_deg);
     for (int64_t si = 0; si < n_set; ++si) {
         int64_t e = el_set[si];
@@ -312,6 +321,81 @@ int surface_pressure(int elem_type, int64_t n_nodes, const double* xyz, int64_t 
                      const double* norm_dir, double norm_tol_deg, int64_t n_dof, double* f) {
     return surface_load(elem_type, n_nodes, xyz, n_el, conn, dof_index, n_set, el_set, nullptr, &pressure, norm_dir,
                         norm_tol_deg, n_dof, f);
+}
+
+// surfaceFlux element loads of hex elements (Element::get_app_therm_load_dfd0,
+// S/element_equations.rs:2113-2150): the face test adds norm_dir . n of every
+// surface face of the element to one running sum (never reset between
+// faces); a face passes while that sum exceeds cos(norm_tol).  A passing
+// face sets the nodal field to q on its nodes (kept across the element's
+// faces), takes the unit-capacity heat-rate integral
+// sum_gp w detJ N_a sum_b N_b T_b (get_rtm_dfd0 without the material's
+// specific heat), keeps the face's nodes and rescales them to q times the
+// face area.  The element adds the vector of its last passing face.
+int surface_flux(int elem_type, int64_t n_nodes, const double* xyz, int64_t n_el, const int32_t* conn,
+                 const int64_t* dof_index, int64_t n_set, const int32_t* el_set, double q, const double* norm_dir_in,
+                 double norm_tol_deg, int64_t n_dof, double* f) {
+    if (elem_type != ASEND_BRICK8 && elem_type != ASEND_BRICKIM) return ASEND_ERR_UNSUPPORTED;
+    const double nd2 = norm_dir_in[0] * norm_dir_in[0] + norm_dir_in[1] * norm_dir_in[1] + norm_dir_in[2] * norm_dir_in[2];
+    if (!(nd2 > 0.0)) return ASEND_ERR_ARG;
+    const double ndm = 1.0 / std::sqrt(nd2);
+    const double norm_dir[3] = {ndm * norm_dir_in[0], ndm * norm_dir_in[1], ndm * norm_dir_in[2]};
+    const auto& fl = kHexFace;
+    const std::vector<uint8_t> on_surf = hex_surface_faces(n_el, conn);
+    const double cos_tol = std::cos(0.017453292519943295 * norm_tol_deg);
+    for (int64_t si = 0; si < n_set; ++si) {
+        const int64_t e = el_set[si];
+        if (e < 0 || e >= n_el) return ASEND_ERR_ARG;
+        const int32_t* c = conn + 8 * e;
+        double X[24];
+        for (int a = 0; a < 8; ++a)
+            for (int k = 0; k < 3; ++k) X[3 * a + k] = xyz[3 * (int64_t)c[a] + k];
+        double T[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        double ld[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        double dp = 0.0;
+        bool any = false;
+        for (int k = 0; k < 6; ++k) {
+            if (!on_surf[6 * e + k]) continue;
+            const int* l = fl[k];
+            double v1[3], v2[3], nrm[3];
+            for (int i = 0; i < 3; ++i) {
+                v1[i] = X[3 * l[2] + i] - X[3 * l[0] + i];
+                v2[i] = X[3 * l[3] + i] - X[3 * l[1] + i];
+            }
+            nrm[0] = v1[1] * v2[2] - v1[2] * v2[1];
+            nrm[1] = v1[2] * v2[0] - v1[0] * v2[2];
+            nrm[2] = v1[0] * v2[1] - v1[1] * v2[0];
+            const double a2 = std::sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+            for (int i = 0; i < 3; ++i) dp += norm_dir[i] * (nrm[i] / a2);
+            if (!(dp > cos_tol)) continue;
+            any = true;
+            for (int i = 0; i < 4; ++i) T[l[i]] = q;
+            for (int a = 0; a < 8; ++a) ld[a] = 0.0;
+            for (int gp = 0; gp < 8; ++gp) {
+                const double s[3] = {(gp & 1) ? kGp : -kGp, (gp & 2) ? kGp : -kGp, (gp & 4) ? kGp : -kGp};
+                double N[8];
+                const double w = hex_detj(X, s, N);
+                double tp = 0.0;
+                for (int b = 0; b < 8; ++b) tp += N[b] * T[b];
+                for (int a = 0; a < 8; ++a) ld[a] += N[a] * tp * w;
+            }
+            double tot = 0.0;
+            for (int a = 0; a < 8; ++a) {
+                if (a != l[0] && a != l[1] && a != l[2] && a != l[3]) ld[a] = 0.0;
+                tot += ld[a];
+            }
+            const double sc = q * (0.5 * a2) / tot;
+            for (int a = 0; a < 8; ++a) ld[a] *= sc;
+        }
+        if (!any) continue;  // element without a passing face contributes nothing
+        for (int a = 0; a < 8; ++a) {
+            const int64_t g = dof_index[6 * (int64_t)c[a]];
+            if (g < 0 || g >= n_dof) return ASEND_ERR_ARG;
+            f[g] += ld[a];
+        }
+    }
+    (void)n_nodes;
+    return ASEND_OK;
 }
 
 // bodyForce / gravitational / centrifugal element loads of hex elements
@@ -540,6 +624,14 @@ int asend_surface_pressure(int elem_type, int64_t n_nodes, const double* xyz, in
                            const double* norm_dir, double norm_tol_deg, int64_t n_dof, double* f) {
     return asend::surface_pressure(elem_type, n_nodes, xyz, n_el, conn, dof_index, n_set, el_set, pressure, norm_dir,
                                    norm_tol_deg, n_dof, f);
+}
+
+int asend_surface_flux(int elem_type, int64_t n_nodes, const double* xyz, int64_t n_el, const int32_t* conn,
+                       const int64_t* dof_index, int64_t n_set, const int32_t* el_set, double q,
+                       const double* norm_dir, double norm_tol_deg, int64_t n_dof, double* f) {
+    if (!xyz || !conn || !dof_index || (n_set > 0 && !el_set) || !norm_dir || !f) return ASEND_ERR_ARG;
+    return asend::surface_flux(elem_type, n_nodes, xyz, n_el, conn, dof_index, n_set, el_set, q, norm_dir,
+                               norm_tol_deg, n_dof, f);
 }
 
 int asend_body_load(int elem_type, int kind, int64_t n_nodes, const double* xyz, int64_t n_el, const int32_t* conn,

@@ -72,14 +72,19 @@ __device__ __forceinline__ double group8_sum(double v) {
 // 8 a + b with a <= b: G_ab = sum_gp detJ g_a g_b^T, and for isotropic C the
 // block is K_ab = lam G_ab + mu G_ab^T + mu tr(G_ab) I, plus c_mass rho M_ab I
 // and, with c_damp != 0, c_damp times the same integral for a general 6x6
-// material damping matrix; K_ba = K_ab^T.  The element's eight row records are staged in LDS and
-// written as coalesced 16-byte stores.
+// material damping matrix; K_ba = K_ab^T.  With kcond (6 conductivities per
+// element, k11 k22 k33 k12 k13 k23) the block is the heat conduction term
+// s_ab I with s_ab = sum over m, n of k_mn G_ab[m][n] = int g_a^T k g_b: the
+// scalar conduction matrix on each of the three DOFs of a node, of which the
+// first carries the temperature.  The element's eight row records are staged
+// in LDS and written as coalesced 16-byte stores.
 __global__ __launch_bounds__(256) void k_hex8_elem_rec(int n_el, const int32_t* __restrict__ conn_r,
                                                        const double* __restrict__ xyz_r,
                                                        const double* __restrict__ E, const double* __restrict__ nu,
                                                        const double* __restrict__ rho, double k_scale, double c_mass,
                                                        const double* __restrict__ Cd,
                                                        const int32_t* __restrict__ cd_idx, double c_damp,
+                                                       const double* __restrict__ kcond,
                                                        double* __restrict__ Kr) {
     __shared__ __align__(16) double s_g[4][8][8][4];  // (g_t, N_t) at each point
     __shared__ double s_det[4][8];
@@ -175,14 +180,25 @@ __global__ __launch_bounds__(256) void k_hex8_elem_rec(int n_el, const int32_t* 
         }
         double* o = &s_out[wv][kRecStride * a + 9 * b];
         double* ot = &s_out[wv][kRecStride * b + 9 * a];
+        if (kcond) {
+            const double* kc = kcond + 6 * (int64_t)ec;
+            const double s = kc[0] * G[0] + kc[1] * G[4] + kc[2] * G[8] + kc[3] * (G[1] + G[3]) +
+                             kc[4] * (G[2] + G[6]) + kc[5] * (G[5] + G[7]);
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                const double v = lam * G[3 * i + j] + mu * G[3 * j + i] + (i == j ? dg : 0.0) + X[3 * i + j];
-                o[3 * i + j] = v;
-                if (a != b) ot[3 * j + i] = v;
+            for (int c = 0; c < 9; ++c) {
+                o[c] = (c % 4 == 0) ? s : 0.0;
+                if (a != b) ot[c] = (c % 4 == 0) ? s : 0.0;
             }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const double v = lam * G[3 * i + j] + mu * G[3 * j + i] + (i == j ? dg : 0.0) + X[3 * i + j];
+                    o[3 * i + j] = v;
+                    if (a != b) ot[3 * j + i] = v;
+                }
+        }
     }
     __syncthreads();
     if (!live) return;
@@ -444,10 +460,10 @@ __global__ __launch_bounds__(256) void k_hex8_assemble_uniform(
 
 void launch_hex8_elem_rec(hipStream_t st, int n_el, const int32_t* conn_r, const double* xyz_r, const double* E,
                           const double* nu, const double* rho, double k_scale, double c_mass, const double* Cd,
-                          const int32_t* cd_idx, double c_damp, double* Kr) {
+                          const int32_t* cd_idx, double c_damp, const double* kcond, double* Kr) {
     if (n_el <= 0) return;
     hipLaunchKernelGGL(k_hex8_elem_rec, dim3((unsigned)((n_el + 3) / 4)), dim3(256), 0, st, n_el, conn_r, xyz_r, E,
-                       nu, rho, k_scale, c_mass, Cd, cd_idx, c_damp, Kr);
+                       nu, rho, k_scale, c_mass, Cd, cd_idx, c_damp, kcond, Kr);
 }
 
 void launch_rec_gather(hipStream_t st, int n_rows, int n_slices, const int32_t* sptr, const int32_t* n2e_ptr,

@@ -155,6 +155,23 @@ int asend_surface_pressure(int elem_type, int64_t n_nodes, const double* xyz, in
  * rho[e] angular_velocity^2 times the distance vector from the axis (through
  * center along axis, normalised here as Load::set_axis does) to the element
  * centroid.  rho may be NULL for kind 0. */
+/* surfaceFlux (Element::get_app_therm_load_dfd0, S/element_equations.rs:
+ * 2113-2150, and Face::get_area_normal_dfd0): accumulates (+=) into
+ * f[dof_index[6 n]] the nodal heat input of a uniform flux q on the surface
+ * faces of el_set that pass the reference's normal test: the element's
+ * running sum of norm_dir . n over its surface faces (the reference never
+ * resets it between faces) exceeds cos(norm_tol_deg).  Each such face's
+ * vector is the unit-capacity heat-rate integral of a field equal to q on the
+ * face's nodes (and on the nodes of earlier passing faces), kept on the
+ * face's nodes and rescaled to q times the face area; the element adds the
+ * vector of its last passing face.  bodyHeatGen is asend_body_load kind 0
+ * with load (Q, 0, 0) (the same consistent integral, rescaled to Q V), and
+ * nodalHeatGen adds its value at each node's temperature DOF.  Solid hex
+ * types only. */
+int asend_surface_flux(int elem_type, int64_t n_nodes, const double* xyz, int64_t n_el, const int32_t* conn,
+                       const int64_t* dof_index, int64_t n_set, const int32_t* el_set, double q,
+                       const double* norm_dir, double norm_tol_deg, int64_t n_dof, double* f);
+
 int asend_body_load(int elem_type, int kind, int64_t n_nodes, const double* xyz, int64_t n_el,
                     const int32_t* conn, const int64_t* dof_index, int64_t n_set, const int32_t* el_set,
                     const double* rho, const double* load, const double* center, const double* axis,
@@ -241,6 +258,25 @@ int asend_uniform_elements(const asend_ctx* ctx);
  * Produces the FP64 operator (penalty folded into its diagonal), its FP32
  * copy and the Jacobi diagonal. */
 int asend_assemble(asend_ctx* ctx, double c_mass);
+
+/* Heat conduction statics (the thermal branch of Model::solve_step,
+ * S/model_analysis.rs:331-370; Element::get_rtk_dfd0,
+ * S/element_equations.rs:1064-1123; conductivity of
+ * get_conductivity_dfd0, S/element_properties.rs:1715-1760): kcond holds 6
+ * conductivities per element (k11 k22 k33 k12 k13 k23, the material key
+ * thermal: conductivity), after which asend_assemble (c_mass = 0) integrates
+ * sum over the Gauss points of w detJ G^T k G per element.  The temperature
+ * of a node is its first DOF (dof_index[6 n]); the same scalar operator sits
+ * on its other two DOFs, which carry zero loads and, penalised at the
+ * constrained nodes like the first, solve to zero: a PCG over the three
+ * copies takes the iterates of the one-DOF system, and a block of 3 b DOFs
+ * is the reference's thermal LDL^T block of b nodes.  The penalty scale is
+ * taken again from the next assembly (1e5 max|K|, scale_const on the
+ * thermal matrix).  Solves take the (block) Jacobi or LDL^T preconditioners;
+ * entry points built on the elastic operator (sensitivities, Newmark, modal,
+ * nonlinear, multigrid, learned) refuse such a context.  kcond = NULL
+ * returns the context to elasticity.  brick8 meshes. */
+int asend_set_conductivity(asend_ctx* ctx, const double* kcond);
 
 /* Download the assembled FP64 operator as scalar CSR (columns ascending):
  * rowptr[n_dof+1]; pass cols = vals = NULL to query *nnz only. */

@@ -101,6 +101,28 @@ def test_traction_matches_oracle(dims):
     assert np.allclose(a, b, rtol=1e-12, atol=1e-14)
 
 
+@pytest.mark.parametrize("dims", [(6, 3, 2), (3, 1, 1)])
+def test_thermal_loads_match_oracle(dims):
+    """surfaceFlux (asend_surface_flux) and bodyHeatGen (asend_body_load kind
+    0 with load (Q, 0, 0)) against the oracle's restatement, on a mesh with
+    moved nodes, for flux directions that pass on one face and on several (the
+    running normal test)."""
+    import thermal as TH
+    m = hex_block(*dims, 1.3, 0.4, 0.3)
+    rng = np.random.default_rng(4)
+    xyz = m.xyz + 0.01 * rng.standard_normal(m.xyz.shape)
+    _, di, nd = O.reorder_nodes(xyz, m.conn, 3, [[m.nodes_at_x(0)]] * 3, 24)
+    every = np.arange(m.n_el, dtype=np.int32)
+    for ndir, tol in (([1, 0, 0], 5.0), ([-1, 0, 0], 5.0), ([0, 1, 0], 10.0), ([0, 0, -1], 45.0), ([1, 1, 0], 50.0)):
+        a = api.surface_flux(8, xyz, m.conn, di, nd, every, -3.5, ndir, tol)
+        b = TH.to_dofs(TH.surface_flux(xyz, m.conn, every, -3.5, ndir, tol), di, nd)
+        assert np.abs(a - b).max() <= 1e-13 * max(np.abs(b).max(), 1e-300), (ndir, tol)
+        assert not a.reshape(-1, 3)[:, 1:].any()
+    a = api.body_load(8, api.BODY_FORCE, xyz, m.conn, di, nd, every, load=(7.0, 0.0, 0.0))
+    b = TH.to_dofs(TH.body_heat_gen(xyz, m.conn, every, 7.0), di, nd)
+    assert np.abs(a - b).max() <= 1e-13 * np.abs(b).max()
+
+
 def test_lognormal_moduli_seeded():
     a = api.lognormal_moduli(20261202, 200000, 70e3, 1.0)
     b = api.lognormal_moduli(20261202, 200000, 70e3, 1.0)
