@@ -186,6 +186,8 @@ struct Material {
     bool stiffness = false;
     double damping[36] = {};  // Material::set_damping (symmetric, S/section_meth.rs:73-77)
     bool damped = false;
+    double cond[6] = {0, 0, 0, 0, 0, 0};  // thermal conductivity k11 k22 k33 k12 k13 k23
+    bool expands = false;                 // a nonzero thermal expansion coefficient
 };
 struct Term {
     std::string set;
@@ -246,8 +248,10 @@ struct Job {
     std::map<std::string, size_t> ns_map, es_map;
     std::vector<Section> sections;
     std::vector<Material> materials;
-    std::vector<Constraint> constraints;
-    std::vector<Load> loads;
+    std::vector<Constraint> constraints;   // displacement
+    std::vector<Constraint> tconstraints;  // temperature
+    std::vector<Load> loads;               // elastic
+    std::vector<Load> tloads;              // thermal
     std::vector<DesignVar> dvars;
     std::vector<ObjTerm> obj;
 
@@ -267,9 +271,19 @@ struct Job {
     int64_t solver_block_dim = 0;        // nodes per reordering block (2e9: direct)
     bool ldl_ok = true, bw_printed = false;  // LDL^T factor usable / maxBandwidth line printed
     Command solve_cmd;
+    // heat conduction (thermal solves): its own context, the conductivities
+    // of the elements, the node temperatures, and the penalised temperature
+    // DOFs with coefficient x rhs of their constraints
+    asend_ctx* tctx = nullptr;
+    std::vector<double> kc, temp;
+    std::vector<int64_t> tpen, tpen3;  // temperature DOFs; with their two companions
+    std::vector<double> tpen_rhs;
+    std::vector<double> t3;  // temperatures in the conduction context's DOFs
+    bool tldl_ok = true, tbw_printed = false, thermal_solved = false;
 
     ~Job() {
         if (ctx) asend_ctx_destroy(ctx);
+        if (tctx) asend_ctx_destroy(tctx);
     }
 
     const std::vector<int32_t>& node_set(const std::string& n, bool fallback_all) {
@@ -290,6 +304,9 @@ struct Job {
 
     void check(int rc, const char* what) {
         if (rc != ASEND_OK) die("%s failed (%d): %s", what, rc, ctx ? asend_last_error(ctx) : "");
+    }
+    void check_t(int rc, const char* what) {
+        if (rc != ASEND_OK) die("%s failed (%d): %s", what, rc, tctx ? asend_last_error(tctx) : "");
     }
 
     // ------------------------------------------------------------ readers
@@ -426,6 +443,10 @@ struct Job {
                         if (r < 0 || r > 5 || c < 0 || c > 5) die("damping entry out of range");
                         cur_mat->damping[6 * r + c] = cur_mat->damping[6 * c + r] = to_d(d[2]);
                         cur_mat->damped = true;
+                    } else if (h[2] == "thermal" && h[3] == "conductivity" && d.size() == 6) {
+                        for (int i = 0; i < 6; ++i) cur_mat->cond[i] = to_d(d[i]);
+                    } else if (h[2] == "thermal" && h[3] == "expansion" && d.size() == 6) {
+                        for (int i = 0; i < 6; ++i) cur_mat->expands = cur_mat->expands || to_d(d[i]) != 0.0;
                     }
                 }
             }
@@ -463,7 +484,9 @@ struct Job {
             }
             if (!any) return;
             constraints.clear();
+            tconstraints.clear();
         }
+        std::vector<Constraint>* cur = nullptr;
         for (const auto& line : lines) {
             lr.read(line, d);
             const std::string* h = lr.head;
@@ -472,23 +495,24 @@ struct Job {
                 if (d[0] != "displacement" && d[0] != "temperature" && d[0] != "concentration" && d[0] != "fluid")
                     die("%s is not a valid constraint type", d[0].c_str());
                 cur_disp = (d[0] == "displacement");
-                if (cur_disp) {
-                    constraints.push_back(Constraint());
-                    constraints.back().type = d[0];
+                cur = cur_disp ? &constraints : (d[0] == "temperature" ? &tconstraints : nullptr);
+                if (cur) {
+                    cur->push_back(Constraint());
+                    cur->back().type = d[0];
                 }
-            } else if (!cur_disp || constraints.empty()) {
+            } else if (!cur || cur->empty()) {
                 continue;
             } else if (h[1] == "terms") {
                 if (h[2] == "nodeSet" && d.size() == 1) {
-                    constraints.back().terms.push_back(Term());
-                    constraints.back().terms.back().set = d[0];
-                } else if (h[2] == "dof" && d.size() == 1 && !constraints.back().terms.empty()) {
-                    constraints.back().terms.back().dof = (int)to_i(d[0]);
-                } else if (h[2] == "coef" && d.size() == 1 && !constraints.back().terms.empty()) {
-                    constraints.back().terms.back().coef = to_d(d[0]);
+                    cur->back().terms.push_back(Term());
+                    cur->back().terms.back().set = d[0];
+                } else if (h[2] == "dof" && d.size() == 1 && !cur->back().terms.empty()) {
+                    cur->back().terms.back().dof = (int)to_i(d[0]);
+                } else if (h[2] == "coef" && d.size() == 1 && !cur->back().terms.empty()) {
+                    cur->back().terms.back().coef = to_d(d[0]);
                 }
             } else if (h[1] == "rhs" && d.size() == 1) {
-                constraints.back().rhs = to_d(d[0]);
+                cur->back().rhs = to_d(d[0]);
             }
         }
     }
@@ -498,7 +522,8 @@ struct Job {
         LineReader lr;
         std::vector<std::string> d;
         const std::string elastic = "nodalForce bodyForce gravitational centrifugal surfacePressure surfaceTraction";
-        bool cur_el = false;
+        const std::string thermal = "nodalHeatGen bodyHeatGen surfaceFlux";  // read_load_input's lists
+        std::vector<Load>* cur = nullptr;
         const auto lines = read_lines(path, "Load input");
         {
             LineReader cnt;
@@ -509,38 +534,41 @@ struct Job {
             }
             if (!any) return;
             loads.clear();
+            tloads.clear();
         }
         for (const auto& line : lines) {
             lr.read(line, d);
             const std::string* h = lr.head;
             if (h[0] != "loads") continue;
             if (h[1] == "type" && d.size() == 1) {
-                cur_el = elastic.find(d[0]) != std::string::npos;
-                if (cur_el) {
-                    loads.push_back(Load());
-                    loads.back().type = d[0];
+                cur = elastic.find(d[0]) != std::string::npos   ? &loads
+                      : thermal.find(d[0]) != std::string::npos ? &tloads
+                                                                : nullptr;
+                if (cur) {
+                    cur->push_back(Load());
+                    cur->back().type = d[0];
                 }
-            } else if (!cur_el || loads.empty()) {
+            } else if (!cur || cur->empty()) {
                 continue;
             } else if (h[1] == "load" && !d.empty()) {
-                for (int i = 0; i < 6; ++i) loads.back().load[i] = (i < (int)d.size()) ? to_d(d[i]) : 0.0;
+                for (int i = 0; i < 6; ++i) cur->back().load[i] = (i < (int)d.size()) ? to_d(d[i]) : 0.0;
             } else if (h[1] == "nodeSet" && d.size() == 1) {
-                loads.back().node_set = d[0];
+                cur->back().node_set = d[0];
             } else if (h[1] == "elementSet" && d.size() == 1) {
-                loads.back().el_set = d[0];
+                cur->back().el_set = d[0];
             } else if (h[1] == "normDir" && d.size() == 3) {
-                for (int i = 0; i < 3; ++i) loads.back().norm_dir[i] = to_d(d[i]);
+                for (int i = 0; i < 3; ++i) cur->back().norm_dir[i] = to_d(d[i]);
             } else if (h[1] == "normTolerance" && d.size() == 1) {
-                loads.back().norm_tol = to_d(d[0]);
+                cur->back().norm_tol = to_d(d[0]);
             } else if (h[1] == "center" && d.size() == 3) {
-                for (int i = 0; i < 3; ++i) loads.back().center[i] = to_d(d[i]);
+                for (int i = 0; i < 3; ++i) cur->back().center[i] = to_d(d[i]);
             } else if (h[1] == "axis" && d.size() == 3) {
-                for (int i = 0; i < 3; ++i) loads.back().axis[i] = to_d(d[i]);
+                for (int i = 0; i < 3; ++i) cur->back().axis[i] = to_d(d[i]);
             } else if (h[1] == "angularVelocity" && !d.empty()) {
-                loads.back().ang_vel = to_d(d[0]);
+                cur->back().ang_vel = to_d(d[0]);
             } else if (h[1] == "activeTime" && !d.empty()) {
-                loads.back().active[0] = to_d(d[0]);
-                loads.back().active[1] = d.size() == 2 ? to_d(d[1]) : 1.0e100;
+                cur->back().active[0] = to_d(d[0]);
+                cur->back().active[1] = d.size() == 2 ? to_d(d[1]) : 1.0e100;
             }
         }
     }
@@ -626,6 +654,7 @@ struct Job {
         std::vector<std::array<double, 9>> par(ne);
         thk.assign(ne, 0.0);
         rho.assign(ne, 0.0);
+        kc.assign(6 * ne, 0.0);
         auto mat_of = [&](const std::string& n) -> const Material& {
             for (const auto& m : materials)
                 if (m.name == n) return m;
@@ -644,6 +673,7 @@ struct Job {
                 m = &mat_of(s.mat);
             }
             if (m->stiffness) die("materials given by a stiffness matrix are not supported");
+            std::copy(m->cond, m->cond + 6, kc.begin() + 6 * e);
             for (int c = 0; c < 3; ++c) {
                 par[e][c] = m->E[c];
                 par[e][3 + c] = m->nu[c];
@@ -739,13 +769,15 @@ struct Job {
         // constraint structure for the ordering (add_constraint_conn)
         std::vector<int64_t> cptr{0}, tptr{0};
         std::vector<int32_t> setn;
-        for (const auto& k : constraints) {
-            for (const auto& t : k.terms) {
-                for (int32_t n : node_set(t.set, false)) setn.push_back(n);
-                tptr.push_back((int64_t)setn.size());
+        // the displacement constraints, then the temperature ones (reorder_nodes, S/model_prep.rs:98-99)
+        for (const auto* list : {&constraints, &tconstraints})
+            for (const auto& k : *list) {
+                for (const auto& t : k.terms) {
+                    for (int32_t n : node_set(t.set, false)) setn.push_back(n);
+                    tptr.push_back((int64_t)setn.size());
+                }
+                cptr.push_back((int64_t)tptr.size() - 1);
             }
-            cptr.push_back((int64_t)tptr.size() - 1);
-        }
         int64_t bd;
         if (c.method == "direct" || c.method.empty()) bd = 2000000000;
         // an unset solverBlockDim (2e9) takes the automatic size (S/model_prep.rs:892-901)
@@ -754,7 +786,7 @@ struct Job {
         std::vector<int32_t> ordered(nn);
         dof_index.assign(6 * nn, -1);
         int rc = asend_reorder_nodes((int64_t)nn, xyz.data(), (int64_t)ne, npe, dpn, cflat.data(),
-                                     (int64_t)constraints.size(), cptr.data(), tptr.data(), setn.data(), bd,
+                                     (int64_t)(cptr.size() - 1), cptr.data(), tptr.data(), setn.data(), bd,
                                      ordered.data(), dof_index.data(), &n_dof);
         if (rc != ASEND_OK) die("node reordering failed (%d)", rc);
         // penalty DOFs: one-term clamps (Constraint::build_mat)
@@ -768,6 +800,8 @@ struct Job {
         }
         element_materials();
         node_coords();
+        if (c.thermal) thermal_prep(cflat);
+        if (!c.elastic) return;
         if (!ctx) check(asend_ctx_create(0, &ctx), "asend_ctx_create");
         check(asend_upload_mesh(ctx, etype, (int64_t)nn, xyz_dv.data(), dof_index.data(), n_dof, (int64_t)ne,
                                 cflat.data(), E.data(), nu.data(), rho.data()),
@@ -807,6 +841,92 @@ struct Job {
             }
         }
         loads_at(0.0, f);
+    }
+
+    // Heat conduction set-up (the thermal half of analysis_prep): a second
+    // device context over the same DOF numbering with the elements'
+    // conductivities, the temperature penalties (single-term constraints,
+    // coefficient +-1, any rhs: Constraint::build_mat / get_load,
+    // S/const_meth.rs:30-119) and one load vector per thermal load
+    // (build_thermal_app_load, S/model_analysis.rs:77-127).
+    void thermal_prep(const std::vector<int32_t>& cflat) {
+        if (etype != ASEND_BRICK8) die("thermal solves are supported for brick8 models");
+        const size_t nn = xyz.size() / 3, ne = conn.size();
+        tpen.clear();
+        tpen3.clear();
+        tpen_rhs.clear();
+        for (const auto& k : tconstraints) {
+            if (k.terms.size() != 1 || std::fabs(std::fabs(k.terms[0].coef) - 1.0) > 0.0)
+                die("only single-term temperature constraints with coefficient +-1 are supported");
+            for (int32_t n : node_set(k.terms[0].set, false)) {
+                tpen.push_back(dof_index[6 * n]);
+                tpen_rhs.push_back(k.terms[0].coef * k.rhs);
+                for (int c = 0; c < 3; ++c) tpen3.push_back(dof_index[6 * n + c]);
+            }
+        }
+        if (!tctx) check_t(asend_ctx_create(0, &tctx), "asend_ctx_create");
+        const std::vector<double> one(ne, 1.0), nu0(ne, 0.3);
+        check_t(asend_upload_mesh(tctx, etype, (int64_t)nn, xyz_dv.data(), dof_index.data(), n_dof, (int64_t)ne,
+                                  cflat.data(), one.data(), nu0.data(), nullptr),
+                "asend_upload_mesh");
+        check_t(asend_set_conductivity(tctx, kc.data()), "asend_set_conductivity");
+        // the temperature's two companion DOFs are penalised at the same nodes
+        check_t(asend_set_penalty(tctx, (int64_t)tpen3.size(), tpen3.data(), 0.0), "asend_set_penalty");
+        check_t(asend_assemble(tctx, 0.0), "asend_assemble");
+        for (auto& l : tloads) {
+            l.vec.assign(n_dof, 0.0);
+            double* lf = l.vec.data();
+            int rc = ASEND_OK;
+            if (l.type == "nodalHeatGen") {
+                for (int32_t n : node_set(l.node_set, false)) lf[dof_index[6 * n]] += l.load[0];
+            } else if (l.type == "bodyHeatGen") {
+                const auto& set = element_set(l.el_set);
+                const double q[3] = {l.load[0], 0.0, 0.0};
+                const double zero[3] = {0.0, 0.0, 0.0};
+                rc = asend_body_load(etype, 0, (int64_t)nn, xyz_dv.data(), (int64_t)ne, cflat.data(),
+                                     dof_index.data(), (int64_t)set.size(), set.data(), nullptr, q, zero, zero, 0.0,
+                                     n_dof, lf);
+            } else {  // surfaceFlux
+                const auto& set = element_set(l.el_set);
+                rc = asend_surface_flux(etype, (int64_t)nn, xyz_dv.data(), (int64_t)ne, cflat.data(),
+                                        dof_index.data(), (int64_t)set.size(), set.data(), l.load[0], l.norm_dir,
+                                        l.norm_tol, n_dof, lf);
+            }
+            if (rc != ASEND_OK) die("%s load failed (%d)", l.type.c_str(), rc);
+        }
+        t3.assign(n_dof, 0.0);
+        temp.assign(nn, 0.0);
+    }
+
+    // The thermal branch of solve_step (S/model_analysis.rs:331-370): the
+    // load of the thermal loads active at t, minus the element flux of the
+    // current temperatures, plus the penalty load sf (c rhs - T); solved for
+    // the increment, which is added to the temperatures.
+    void thermal_step(double t) {
+        std::vector<double> b(n_dof, 0.0), kt(n_dof, 0.0), dt;
+        for (const auto& l : tloads)
+            if (t >= l.active[0] && t <= l.active[1])
+                for (int64_t i = 0; i < n_dof; ++i) b[i] += l.vec[i];
+        check_t(asend_spmv(tctx, 64, t3.data(), kt.data()), "asend_spmv");  // K T + sf P T
+        const double sf = asend_penalty_scale(tctx);
+        for (int64_t i = 0; i < n_dof; ++i) b[i] -= kt[i];
+        for (size_t k = 0; k < tpen.size(); ++k) b[tpen[k]] += sf * tpen_rhs[k];
+        device_solve(tctx, tldl_ok, tbw_printed, 3 * solver_block_dim, 3, b, dt, true);
+        for (int64_t i = 0; i < n_dof; ++i) t3[i] += dt[i];
+        const size_t nn = xyz.size() / 3;
+        for (size_t n = 0; n < nn; ++n) temp[n] = dof_index[6 * n] >= 0 ? t3[dof_index[6 * n]] : 0.0;
+        thermal_solved = true;
+    }
+
+    // reactionHeatGen: the element flux of the temperatures without the
+    // penalty (build_thermal_soln_load(false) negated, S/model_output.rs:201-208)
+    std::vector<double> reaction_heat() {
+        if (!thermal_solved) die("reactionHeatGen needs a thermal solution");
+        std::vector<double> r(n_dof, 0.0);
+        check_t(asend_spmv(tctx, 64, t3.data(), r.data()), "asend_spmv");
+        const double sf = asend_penalty_scale(tctx);
+        for (int64_t d : tpen3) r[d] -= sf * t3[d];
+        return r;
     }
 
     // sum of the loads whose activeTime holds t (build_elastic_app_load's
@@ -851,12 +971,18 @@ struct Job {
         std::FILE* fp = std::fopen(path.c_str(), "wb");
         if (!fp) die("couldn't open file, %s", path.c_str());
         const size_t nn = xyz.size() / 3;
-        for (size_t n = 0; n < nn; ++n)
+        for (size_t n = 0; n < nn; ++n) {
+            if (solve_cmd.thermal) {  // temperature and its rate (static: 0)
+                put_be(fp, temp[n]);
+                put_be(fp, 0.0);
+            }
+            if (!solve_cmd.elastic) continue;
             for (const auto* vec : {&su, &sv, &sa})
                 for (int c = 0; c < dpn; ++c) {
                     const int64_t d = dof_index[6 * n + c];
                     put_be(fp, d >= 0 ? (*vec)[d] : 0.0);
                 }
+        }
         std::fclose(fp);
     }
 
@@ -868,7 +994,14 @@ struct Job {
         sv.assign(n_dof, 0.0);
         sa.assign(n_dof, 0.0);
         const size_t nn = xyz.size() / 3;
-        for (size_t n = 0; n < nn; ++n)
+        for (size_t n = 0; n < nn; ++n) {
+            if (solve_cmd.thermal) {
+                double tn, rate;
+                if (!get_be(fp, &tn) || !get_be(fp, &rate)) die("problem reading file, %s", path.c_str());
+                temp[n] = tn;
+                if (dof_index[6 * n] >= 0) t3[dof_index[6 * n]] = tn;
+            }
+            if (!solve_cmd.elastic) continue;
             for (auto* vec : {&su, &sv, &sa})
                 for (int c = 0; c < dpn; ++c) {
                     double x;
@@ -876,6 +1009,7 @@ struct Job {
                     const int64_t d = dof_index[6 * n + c];
                     if (d >= 0) (*vec)[d] = x;
                 }
+        }
         std::fclose(fp);
     }
 
@@ -888,18 +1022,26 @@ struct Job {
     // falls back to FP64 Jacobi PCG (to a 1e-12 relative residual for direct,
     // the same absolute rule for iterative), saying so.
     void device_solve(const std::vector<double>& b, std::vector<double>& x, bool report) {
+        device_solve(ctx, ldl_ok, bw_printed, 6 * (int64_t)solver_block_dim, 1, b, x, report);
+    }
+
+    // on context c: LDL^T blocks of block_dofs DOFs for the iterative
+    // method; maxBandwidth is printed in units of bw_div DOFs (3 for the
+    // conduction context, whose temperature DOFs are every third)
+    void device_solve(asend_ctx* c, bool& ldl_ok, bool& bw_printed, int64_t block_dofs, int bw_div,
+                      const std::vector<double>& b, std::vector<double>& x, bool report) {
         const bool iterative = (solve_cmd.method == "iterative");
         asend_ldl_info li;
         std::memset(&li, 0, sizeof li);
-        const int frc = ldl_ok ? asend_ldl_factor(ctx, iterative ? 6 * (int64_t)solver_block_dim : 0, &li) : -1;
+        const int frc = ldl_ok ? asend_ldl_factor(c, iterative ? block_dofs : 0, &li) : -1;
         if (ldl_ok && frc != ASEND_OK) {
-            if (frc != ASEND_ERR_UNSUPPORTED) check(frc, "asend_ldl_factor");
+            if (frc != ASEND_ERR_UNSUPPORTED) die("asend_ldl_factor failed (%d): %s", frc, asend_last_error(c));
             std::printf("note: LDL^T factor not available on the device (%s); FP64 Jacobi PCG instead\n",
-                        asend_last_error(ctx));
+                        asend_last_error(c));
             ldl_ok = false;
         }
         if (ldl_ok && !bw_printed) {
-            std::printf("maxBandwidth: %lld\n", (long long)li.max_bandwidth);
+            std::printf("maxBandwidth: %lld\n", (long long)(li.max_bandwidth / bw_div));
             if (!li.pos_def)
                 std::printf("Warning: preconditioning matrix for conjugate gradient solver is not positive "
                             "definite.\n");
@@ -926,7 +1068,8 @@ struct Job {
         asend_solve_stats st;
         std::memset(&st, 0, sizeof st);
         x.assign(n_dof, 0.0);
-        check(asend_solve(ctx, b.data(), x.data(), &o, &st), "asend_solve");
+        if (const int rc = asend_solve(c, b.data(), x.data(), &o, &st))
+            die("asend_solve failed (%d): %s", rc, asend_last_error(c));
         if (report && iterative) {
             if (!st.converged)
                 std::printf("Warning: Conjugate gradient solver did not converge to specified tolerance, %s within "
@@ -947,21 +1090,38 @@ struct Job {
     // each step; with saveSolnHist the state after each load time / step
     // goes to solnTStep<i>.out (the dynamic loop writes the initial state as
     // step 0 first).
+    // Static steps solve for the increment of the state (solve_step builds
+    // the load minus the element residual of the current state plus the
+    // penalty load, S/model_analysis.rs:331-370,450-534): the thermal
+    // branch first, then the elastic one.  Thermal and elastic solves run
+    // side by side only without thermal expansion (no coupling term).
     void solve(const Command& c) {
-        if (c.nonlinear || c.thermal || !c.elastic)
-            die("only linear elastic solves (static or dynamic) are supported by this solver");
+        if (c.nonlinear) die("only linear solves are supported by this solver");
+        if (!c.elastic && !c.thermal) die("the solve has neither the elastic nor the thermal field");
+        if (c.thermal && c.dynamic) die("transient heat conduction is not supported by this solver");
+        if (c.thermal && c.elastic)
+            for (const auto& m : materials)
+                if (m.expands) die("thermal expansion (thermal-elastic coupling) is not supported by this solver");
         solve_cmd = c;
         if (solve_cmd.load_times.empty()) solve_cmd.load_times.push_back(0.0);
         analysis_prep(solve_cmd);
+        u.assign(n_dof, 0.0);
         v.assign(n_dof, 0.0);
         a.assign(n_dof, 0.0);
         if (c.dynamic) {
             solve_dynamic();
         } else {
-            check(asend_assemble(ctx, 0.0), "asend_assemble");
+            if (c.elastic) check(asend_assemble(ctx, 0.0), "asend_assemble");
+            std::vector<double> b(n_dof), ku(n_dof), du;
             for (size_t i = 0; i < solve_cmd.load_times.size(); ++i) {
-                loads_at(solve_cmd.load_times[i], f);
-                device_solve(f, u, true);
+                if (c.thermal) thermal_step(solve_cmd.load_times[i]);
+                if (c.elastic) {
+                    loads_at(solve_cmd.load_times[i], f);
+                    check(asend_spmv(ctx, 64, u.data(), ku.data()), "asend_spmv");  // K u + sf P u
+                    for (int64_t k = 0; k < n_dof; ++k) b[k] = f[k] - ku[k];
+                    device_solve(b, du, true);
+                    for (int64_t k = 0; k < n_dof; ++k) u[k] += du[k];
+                }
                 if (solve_cmd.save_hist) write_state((long)i, u, v, a);
             }
         }
@@ -1032,6 +1192,7 @@ struct Job {
     // solve (D = rayDampCM M + rayDampCK K), without the penalty
     std::vector<double> reaction() {
         std::vector<double> rf(n_dof, 0.0);
+        if (!solve_cmd.elastic) return rf;  // no displacements: no element forces
         const double sf = asend_penalty_scale(ctx);
         if (!dyn_solved) {
             check(asend_spmv(ctx, 64, u.data(), rf.data()), "asend_spmv");
@@ -1149,6 +1310,7 @@ struct Job {
     void objective_values() {
         if (!solved) die("the objective needs a solution (run solve first)");
         if (dyn_solved) die("objectives of dynamic solves are not supported by this solver");
+        if (!solve_cmd.elastic) die("objectives of thermal solves are not supported by this solver");
         cfields_ok = false;
         for (auto& t : obj) {
             const TermData d = term_data(t);
@@ -1372,7 +1534,8 @@ struct Job {
     void write_node_results(const Command& c) {
         if (!solved) die("node results need a solution (run solve first)");
         for (const auto& fld : c.fields)
-            if (fld != "displacement" && fld != "velocity" && fld != "acceleration" && fld != "reactionForce")
+            if (fld != "displacement" && fld != "velocity" && fld != "acceleration" && fld != "reactionForce" &&
+                fld != "temperature" && fld != "tdot" && fld != "reactionHeatGen")
                 die("unrecognized output field '%s' for write_node_results", fld.c_str());
         if (c.time_steps.empty()) {
             write_node_file(c, c.file, -1.0);
@@ -1396,23 +1559,37 @@ struct Job {
 
     void write_node_file(const Command& c, const std::string& name, double time) {
         const auto& set = node_set(c.node_set, true);
-        std::vector<double> rf;
-        for (const auto& fld : c.fields)
+        std::vector<double> rf, rh;
+        for (const auto& fld : c.fields) {
             if (fld == "reactionForce" && rf.empty()) rf = reaction();
+            if (fld == "reactionHeatGen" && rh.empty()) rh = reaction_heat();
+        }
+        if (temp.size() != xyz.size() / 3) temp.assign(xyz.size() / 3, 0.0);
         std::FILE* fp = std::fopen(name.c_str(), "w");
         if (!fp) die("could not open file %s", name.c_str());
         std::fprintf(fp, "node,");
         for (const auto& fld : c.fields) {
-            const char* h = fld == "displacement"   ? "U1,U2,U3,R1,R2,R3,"
-                            : fld == "velocity"     ? "V1,V2,V3,RV1,RV2,RV3,"
-                            : fld == "acceleration" ? "A1,A2,A3,RA1,RA2,RA3,"
-                                                    : "RF1,RF2,RF3,RM1,RM2,RM3,";
+            const char* h = fld == "displacement"      ? "U1,U2,U3,R1,R2,R3,"
+                            : fld == "velocity"        ? "V1,V2,V3,RV1,RV2,RV3,"
+                            : fld == "acceleration"    ? "A1,A2,A3,RA1,RA2,RA3,"
+                            : fld == "temperature"     ? "T,"
+                            : fld == "tdot"            ? "TDOT,"
+                            : fld == "reactionHeatGen" ? "RHG,"
+                                                       : "RF1,RF2,RF3,RM1,RM2,RM3,";
             std::fprintf(fp, "%s", h);
         }
         std::fprintf(fp, "time\n");
         for (int32_t n : set) {
             std::fprintf(fp, "%d,", n);
             for (const auto& fld : c.fields) {
+                if (fld == "temperature" || fld == "tdot") {
+                    std::fprintf(fp, "%s,", fmt_e12(fld == "temperature" ? temp[n] : 0.0).c_str());
+                    continue;
+                }
+                if (fld == "reactionHeatGen") {
+                    std::fprintf(fp, "%s,", fmt_e12(rh[dof_index[6 * n]]).c_str());
+                    continue;
+                }
                 const std::vector<double>& vec = fld == "displacement"   ? u
                                                  : fld == "velocity"     ? v
                                                  : fld == "acceleration" ? a
@@ -1532,10 +1709,11 @@ int main(int argc, char** argv) {
             else if (c.cmd == "readObjectiveInput") job.read_objective(c.file);
         }
         std::printf("commands %zu nodes %zu elements %zu node_sets %zu element_sets %zu sections %zu materials %zu "
-                    "constraints %zu loads %zu design_variables %zu objective_terms %zu\n",
+                    "constraints %zu loads %zu design_variables %zu objective_terms %zu temperature_constraints %zu "
+                    "thermal_loads %zu\n",
                     job.cmds.size(), job.xyz.size() / 3, job.conn.size(), job.nsets.size(), job.esets.size(),
                     job.sections.size(), job.materials.size(), job.constraints.size(), job.loads.size(),
-                    job.dvars.size(), job.obj.size());
+                    job.dvars.size(), job.obj.size(), job.tconstraints.size(), job.tloads.size());
         for (const auto& c : job.cmds) std::printf("command %s %s\n", c.cmd.c_str(), c.file.c_str());
         return 0;
     }

@@ -38,7 +38,8 @@ def test_c1_fixture_parses(built, tmp_path):
     # plus "all"; three clamps, one traction, nine design variables, two terms
     assert counts == {"commands": 9, "nodes": 525, "elements": 320, "node_sets": 2 + 525 + 1,
                       "element_sets": 1 + 320 + 1, "sections": 1, "materials": 1, "constraints": 3, "loads": 1,
-                      "design_variables": 9, "objective_terms": 2}
+                      "design_variables": 9, "objective_terms": 2, "temperature_constraints": 0,
+                      "thermal_loads": 0}
     assert "command writeObjective results/objectiveResults.csv" in out
 
 
