@@ -183,3 +183,86 @@ def test_small_load_linear_limit():
     # the nonlinear correction is first order in the load
     assert 5.0 < out[0] / out[1] < 20.0
     assert out[0] < 1e-3
+
+
+# ---------------------------------------------------------------- brickIM
+def test_brickim_residual_tangent_energy_and_rotation():
+    """brickIM: the residual over the 8 nodes and 3 bubbles is the gradient
+    of the strain energy, the tangent the Jacobian of the residual, a finite
+    rigid rotation (bubbles at rest) carries no force, and the condensed
+    tangent at rest is the linear oracle's condensed element."""
+    X = distorted(5)
+    rng = np.random.default_rng(31)
+    U = 0.05 * rng.standard_normal((11, 3))
+    E, nu = 70e3, 0.3
+    r, K = NL.element_residual_tangent(X[None], U[None], E, nu, im=True)
+    h = 1e-6
+    fd = np.zeros(33)
+    for k in range(33):
+        Up, Um = U.copy().ravel(), U.copy().ravel()
+        Up[k] += h
+        Um[k] -= h
+        fd[k] = (NL.element_energy(X[None], Up.reshape(1, 11, 3), [E], [nu])[0]
+                 - NL.element_energy(X[None], Um.reshape(1, 11, 3), [E], [nu])[0]) / (2 * h)
+    assert np.abs(r[0] - fd).max() <= 1e-6 * np.abs(r[0]).max()
+    h = 1e-7
+    J = np.zeros((33, 33))
+    for k in range(33):
+        Up, Um = U.copy().ravel(), U.copy().ravel()
+        Up[k] += h
+        Um[k] -= h
+        J[:, k] = (NL.element_residual_tangent(X[None], Up.reshape(1, 11, 3), E, nu, im=True)[0][0]
+                   - NL.element_residual_tangent(X[None], Um.reshape(1, 11, 3), E, nu, im=True)[0][0]) / (2 * h)
+    assert np.abs(K[0] - J).max() <= 1e-6 * np.abs(K[0]).max()
+    R = rotation([1.0, -2.0, 0.5], 40.0)
+    Urot = np.vstack([X @ R.T + np.array([0.3, -0.1, 0.2]) - X, np.zeros((3, 3))])
+    rr, _ = NL.element_residual_tangent(X[None], Urot[None], E, nu, im=True)
+    _, K0 = NL.element_residual_tangent(X[None], np.zeros((1, 11, 3)), E, nu, im=True)
+    assert np.abs(rr).max() <= 1e-9 * np.abs(K0).max() * np.abs(Urot).max()
+    _, Kc, _, _ = NL.condense(np.zeros((1, 33)), K0)
+    Klin, _ = O.solid_element(81, X, lame_c(E, nu))
+    assert np.abs(Kc[0] - Klin).max() <= 1e-10 * np.abs(Klin).max()
+
+
+def hexbeam_moment(nx, ny):
+    """hexBeam test 6 (E/testCases/hexBeam/tipMomentNonlinear.py): 10 x 1 x 0.1
+    brickIM beam, E = 1e6, nu = 0, clamped at x = 0; at the tip, dead forces
+    c (1, 0, 1) on the bottom edge nodes and -c (1, 0, 1) on the top edge
+    nodes, c = 65.4498469 / (11 sqrt 2) for the reference's 11 nodes per
+    edge (the total scaled to ny + 1 nodes), in 5 ramp steps."""
+    m = hex_block(nx, ny, 1, 10.0, 1.0, 0.1)
+    clamp = m.nodes_at_x(0)
+    _, di, n_dof = O.reorder_nodes(m.xyz, m.conn, 3, [[clamp]] * 3, 2000000000)
+    di = np.asarray(di).reshape(-1, 6)
+    tip = m.nodes_at_x(nx)
+    c = 65.4498469 / ((ny + 1) * np.sqrt(2.0))
+    f = np.zeros(n_dof)
+    for nd in tip:
+        s = 1.0 if m.xyz[nd, 2] < 0.05 else -1.0
+        f[di[nd, 0]] += s * c
+        f[di[nd, 2]] += s * c
+    pen = np.concatenate([di[clamp, k] for k in range(3)])
+    E = np.full(m.n_el, 1e6)
+    nu = np.zeros(m.n_el)
+    return m, di, n_dof, f, pen, E, nu
+
+
+def test_brickim_hexbeam_tip_moment_known_answer():
+    """A 45 degree arc: tip displacement [-0.99684, 0, 3.729232]
+    (E/testCases/hexBeam/notes.txt, test 6) on a 20 x 2 x 1 mesh, to 5e-4 of
+    its length (the reference's 100 x 10 x 1 mesh gives [-0.996865, 0, 3.729197])."""
+    m, di, n_dof, f, pen, E, nu = hexbeam_moment(20, 2)
+    u, iters, steps, sf, alpha = NL.newton(m.xyz, m.conn, E, nu, di, n_dof, f, pen, load_steps=5, im=True)
+    w = tip_axis_displacement(m, di, u)
+    want = np.array([-0.99684, 0.0, 3.729232])
+    assert np.abs(w - want).max() <= 5e-4 * np.linalg.norm(want), (w, iters)
+    assert max(iters) < 50
+
+
+def tip_axis_displacement(m, di, u):
+    """Displacement of the tip's beam axis (z = h/2, y = b/2): the mean of the
+    bottom and top tip nodes at y = b/2 (the known answer is the elastica's
+    axis; a face node sits h/2 sin(theta) off it along x)."""
+    tip = m.nodes_at_x(m.nx)
+    at = tip[np.abs(m.xyz[tip, 1] - 0.5) < 1e-9]
+    return np.mean([[u[di[nd, k]] for k in range(3)] for nd in at], axis=0)
