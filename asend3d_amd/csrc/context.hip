@@ -113,6 +113,10 @@ struct asend_ctx {
     // geometrically nonlinear statics: element tangents (576 per element),
     // element residuals (24), displacement state, applied load, internal force
     double *d_kb = nullptr, *d_nl_re = nullptr, *d_nl_u = nullptr, *d_nl_f = nullptr, *d_nl_r = nullptr;
+    // brickIM under geometric nonlinearity: the 9 bubble amplitudes of each
+    // element (kept between calls, zero after an upload) and the maps of
+    // their update (K_ii^-1 K_ie and K_ii^-1 r_i, 225 per element)
+    double *d_nl_alpha = nullptr, *d_nl_wmap = nullptr;
     uint8_t* d_clamped = nullptr;
     int32_t *d_enb = nullptr, *d_erev = nullptr, *d_eslot = nullptr;
     int n_edge = 0;
@@ -370,6 +374,8 @@ void free_mesh(asend_ctx* ctx) {
     dfree(ctx->d_nl_u);
     dfree(ctx->d_nl_f);
     dfree(ctx->d_nl_r);
+    dfree(ctx->d_nl_alpha);
+    dfree(ctx->d_nl_wmap);
     dfree(ctx->d_clamped);
     dfree(ctx->d_enb);
     dfree(ctx->d_erev);
@@ -1108,8 +1114,9 @@ int assemble_device(asend_ctx* ctx, double k_scale, double c_mass, double c_damp
 // Mesh checks and buffers of the nonlinear path (allocated on first use).
 int nl_prepare(asend_ctx* ctx) {
     if (!ctx->d_K64) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
-    if (ctx->etype != ASEND_BRICK8)
-        return fail(ctx, ASEND_ERR_UNSUPPORTED, "geometrically nonlinear analysis is implemented for brick8 meshes");
+    if (ctx->etype != ASEND_BRICK8 && ctx->etype != ASEND_BRICKIM)
+        return fail(ctx, ASEND_ERR_UNSUPPORTED,
+                    "geometrically nonlinear analysis is implemented for brick8 and brickIM meshes");
     if (ctx->max_valence > 8)
         return fail(ctx, ASEND_ERR_UNSUPPORTED, "nonlinear assembly supports at most 8 elements per node (mesh has %d)",
                     ctx->max_valence);
@@ -1121,13 +1128,22 @@ int nl_prepare(asend_ctx* ctx) {
          (rc = dalloc(ctx, &ctx->d_nl_u, n)) || (rc = dalloc(ctx, &ctx->d_nl_f, n)) ||
          (rc = dalloc(ctx, &ctx->d_nl_r, n))))
         return rc;
+    if (ctx->etype == ASEND_BRICKIM && !ctx->d_nl_alpha) {
+        if ((rc = dalloc(ctx, &ctx->d_nl_alpha, 9 * ctx->n_el)) || (rc = dalloc(ctx, &ctx->d_nl_wmap, 225 * ctx->n_el)))
+            return rc;
+        HIPCHK(hipMemset(ctx->d_nl_alpha, 0, 9 * ctx->n_el * sizeof(double)));
+    }
     return ASEND_OK;
 }
 
 int nl_assemble(asend_ctx* ctx, double fact, double* rhs) {
     int rc;
-    launch_hex8_nl_element(ctx->st, (int)ctx->n_el, ctx->d_conn_r, ctx->d_xyz_r, ctx->d_E, ctx->d_nu, ctx->d_nl_u,
-                           ctx->d_kb, ctx->d_nl_re);
+    if (ctx->etype == ASEND_BRICKIM)
+        launch_brickim_nl(ctx->st, (int)ctx->n_el, ctx->d_conn_r, ctx->d_xyz_r, ctx->d_E, ctx->d_nu, ctx->d_nl_u,
+                          ctx->d_nl_alpha, ctx->d_kb, ctx->d_nl_re, ctx->d_nl_wmap);
+    else
+        launch_hex8_nl_element(ctx->st, (int)ctx->n_el, ctx->d_conn_r, ctx->d_xyz_r, ctx->d_E, ctx->d_nu,
+                               ctx->d_nl_u, ctx->d_kb, ctx->d_nl_re);
     launch_rec_gather(ctx->st, (int)ctx->n_nodes, (int)ctx->n_slices, ctx->d_sptr, ctx->d_n2e_ptr, ctx->d_n2rec,
                       ctx->d_slot, ctx->d_kb, ctx->d_nl_re, fact, ctx->d_nl_f, ctx->d_K64, ctx->d_nl_r, rhs);
     HIPCHK(hipGetLastError());
@@ -2099,6 +2115,9 @@ int asend_static_nonlinear(asend_ctx* ctx, const double* f, double* u, const ase
             ctx->mg_fine_only = true;
             st.linear_iterations += ls.iterations;
             if (!ls.converged) st.unconverged_solves += 1;
+            if (ctx->etype == ASEND_BRICKIM)  // bubble amplitudes follow the increment
+                launch_brickim_nl_internal(ctx->st, (int)ctx->n_el, ctx->d_conn_r, ctx->d_u, ctx->d_nl_wmap,
+                                           ctx->d_nl_alpha);
             launch_axpy64(ctx->st, n, 1.0, ctx->d_u, ctx->d_nl_u);
             launch_maxabs(ctx->st, ctx->d_u, n, ctx->d_partials, ctx->d_sc + 2);
             HIPCHK(hipGetLastError());

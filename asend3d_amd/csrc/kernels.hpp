@@ -72,6 +72,13 @@ void launch_brickim_assemble(hipStream_t st, const int32_t* elems, int n_elems, 
                              double k_scale, double c_mass, const int32_t* sptr, const uint8_t* slot, double* K);
 void launch_brickim_reference(hipStream_t st, const int32_t* conn_r, const double* xyz_r, const double* nu,
                               double* Khat, double* Mhat);
+// geometrically nonlinear brickIM: condensed tangent records and residuals at
+// (u, alpha) plus the internal-update maps (225 doubles per element), and the
+// update alpha -= w + W du
+void launch_brickim_nl(hipStream_t st, int n_el, const int32_t* conn_r, const double* xyz_r, const double* E,
+                       const double* nu, const double* u, const double* alpha, double* Ke, double* Re, double* Wmap);
+void launch_brickim_nl_internal(hipStream_t st, int n_el, const int32_t* conn_r, const double* du,
+                                const double* Wmap, double* alpha);
 void launch_brickim_sens(hipStream_t st, int n_el, const int32_t* conn_r, const double* xyz_r, const double* E,
                          const double* nu, const double* u, const double* lam, double* sens);
 // shell4: adds k_scale K_e + m_scale M_e; conn_p holds the 8 pseudo-node rows

@@ -450,7 +450,15 @@ int asend_nl_assemble(asend_ctx* ctx, const double* u, double* r_int);
  * unloaded model), converged displacement on return.  Each Newton solve uses
  * opts (any method and preconditioner of asend_solve; the tangent is
  * re-assembled every iteration, so multigrid and learned setups are redone).
- * The penalty scale is fixed on the context's first assembly. */
+ * The penalty scale is fixed on the context's first assembly.
+ * brick8 and brickIM meshes.  brickIM: the 9 bubble amplitudes of each
+ * element enter the displacement gradient like nodal values; every Newton
+ * iteration condenses them (K_ee - K_ei K_ii^-1 K_ie, r_e - K_ei K_ii^-1 r_i:
+ * condense_mat and update_external, S/element_equations.rs:19-121) and moves
+ * them by -K_ii^-1 (r_i + K_ie du) after the solve (update_internal,
+ * :123-194).  They stay on the device between calls, as the reference's
+ * elements keep them, and are zero after asend_upload_mesh; asend_nl_assemble
+ * reports the condensed tangent and internal force at the amplitudes held. */
 int asend_static_nonlinear(asend_ctx* ctx, const double* f, double* u, const asend_nl_opts* nl,
                            const asend_solve_opts* opts, asend_nl_stats* stats);
 
