@@ -147,7 +147,7 @@ _lib.asend_ctx_destroy.argtypes = [_vp]
 EXPORTED = ["asend_ctx_create", "asend_ctx_destroy", "asend_last_error", "asend_version", "asend_sync",
             "asend_reorder_nodes", "asend_block_dim_auto", "asend_surface_traction", "asend_lognormal_moduli",
             "asend_compliance_clamp_check", "asend_surface_pressure", "asend_body_load", "asend_surface_flux",
-            "asend_set_conductivity",
+            "asend_set_conductivity", "asend_brickim_internal", "asend_brickim_set_internal",
             "asend_upload_mesh", "asend_set_thickness", "asend_set_grid", "asend_update_moduli", "asend_set_penalty",
             "asend_penalty_scale", "asend_uniform_elements", "asend_assemble", "asend_get_matrix", "asend_spmv", "asend_solve", "asend_solve_noalloc",
             "asend_ldl_factor", "asend_ldl_solve", "asend_ldl_release", "asend_cycle_spmv_timing",
@@ -514,6 +514,18 @@ class Context:
         self._chk(_lib.asend_static_nonlinear(self.h, _p(f), _p(u), ctypes.byref(nl), ctypes.byref(opts),
                                               ctypes.byref(st)), "asend_static_nonlinear")
         return u, st
+
+    def brickim_internal(self, u=None):
+        """brickIM bubble amplitudes (n_el x 9): of a linear solve reaching u,
+        or, with u None, those held by the nonlinear path."""
+        a = np.zeros((self.n_el, 9))
+        self._chk(_lib.asend_brickim_internal(self.h, _p(None if u is None else _f64(u)), _p(a)),
+                  "asend_brickim_internal")
+        return a
+
+    def brickim_set_internal(self, alpha):
+        a = _f64(np.asarray(alpha, dtype=np.float64).reshape(self.n_el, 9))
+        self._chk(_lib.asend_brickim_set_internal(self.h, _p(a)), "asend_brickim_set_internal")
 
     def bench_spmv(self, precision=32, n_iter=50):
         ms = _dbl(0)

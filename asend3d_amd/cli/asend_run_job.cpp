@@ -225,7 +225,7 @@ struct ObjTerm {
 struct Command {
     std::string cmd, file, method, node_set = "all";
     bool elastic = true, dynamic = false, nonlinear = false, thermal = false, write_grad = true;
-    long block_dim = -1, max_it = 0;
+    long block_dim = -1, max_it = 0, load_ramp = 1;
     double conv_tol = 1.0e-12;
     // dynamic and history keys (JobCommand defaults, S/job.rs:55-110)
     double time_step = 1.0, beta = 0.25, gamma = 0.5, sim_period = 1.0, ray_cm = 0.0, ray_ck = 0.0;
@@ -265,6 +265,7 @@ struct Job {
     std::vector<double> u;    // solution, reference DOF numbering
     std::vector<double> v, a; // velocity and acceleration (dynamic solves; zero for static)
     bool dyn_solved = false;  // the last solve was dynamic
+    bool nl_solved = false;   // the last solve had nonlinearGeom
     std::vector<double> f;    // applied load
     std::vector<double> dLdD;
     bool solved = false, have_grad = false;
@@ -329,6 +330,7 @@ struct Job {
             else if (k == "nonlinearGeom" && d.size() == 1) c.nonlinear = yes(d[0]);
             else if ((k == "thermal" || k == "diffusion") && d.size() == 1) c.thermal = yes(d[0]);
             else if (k == "solverBlockDim" && d.size() == 1) c.block_dim = to_i(d[0]);
+            else if (k == "loadRampSteps" && d.size() == 1) c.load_ramp = to_i(d[0]);
             else if (k == "solverMethod" && d.size() == 1) c.method = d[0];
             else if (k == "maxIterations" && d.size() == 1) c.max_it = to_i(d[0]);
             else if (k == "convergenceTol" && d.size() == 1) c.conv_tol = to_d(d[0]);
@@ -966,7 +968,6 @@ struct Job {
 
     void write_state(long step, const std::vector<double>& su, const std::vector<double>& sv,
                      const std::vector<double>& sa) {
-        if (etype == ASEND_BRICKIM) die("solution history of brickIM models (internal DOFs) is not supported");
         const std::string path = state_path(step);
         std::FILE* fp = std::fopen(path.c_str(), "wb");
         if (!fp) die("couldn't open file, %s", path.c_str());
@@ -982,6 +983,15 @@ struct Job {
                     const int64_t d = dof_index[6 * n + c];
                     put_be(fp, d >= 0 ? (*vec)[d] : 0.0);
                 }
+        }
+        if (solve_cmd.elastic && etype == ASEND_BRICKIM) {
+            // the elements' internal DOFs after the nodes; a dynamic solve
+            // keeps none (brickIM Newmark runs on the condensed operator)
+            if (solve_cmd.dynamic) die("solution history of dynamic brickIM solves (internal DOFs) is not supported");
+            std::vector<double> al(9 * conn.size());
+            check(asend_brickim_internal(ctx, solve_cmd.nonlinear ? nullptr : su.data(), al.data()),
+                  "asend_brickim_internal");
+            for (double x : al) put_be(fp, x);
         }
         std::fclose(fp);
     }
@@ -1009,6 +1019,12 @@ struct Job {
                     const int64_t d = dof_index[6 * n + c];
                     if (d >= 0) (*vec)[d] = x;
                 }
+        }
+        if (solve_cmd.elastic && etype == ASEND_BRICKIM) {
+            std::vector<double> al(9 * conn.size());
+            for (double& x : al)
+                if (!get_be(fp, &x)) die("problem reading file, %s", path.c_str());
+            if (solve_cmd.nonlinear) check(asend_brickim_set_internal(ctx, al.data()), "asend_brickim_set_internal");
         }
         std::fclose(fp);
     }
@@ -1096,7 +1112,8 @@ struct Job {
     // branch first, then the elastic one.  Thermal and elastic solves run
     // side by side only without thermal expansion (no coupling term).
     void solve(const Command& c) {
-        if (c.nonlinear) die("only linear solves are supported by this solver");
+        if (c.nonlinear && (c.dynamic || !c.elastic))
+            die("geometric nonlinearity is supported for static elastic solves by this solver");
         if (!c.elastic && !c.thermal) die("the solve has neither the elastic nor the thermal field");
         if (c.thermal && c.dynamic) die("transient heat conduction is not supported by this solver");
         if (c.thermal && c.elastic)
@@ -1110,6 +1127,8 @@ struct Job {
         a.assign(n_dof, 0.0);
         if (c.dynamic) {
             solve_dynamic();
+        } else if (c.nonlinear) {
+            solve_nonlinear();
         } else {
             if (c.elastic) check(asend_assemble(ctx, 0.0), "asend_assemble");
             std::vector<double> b(n_dof), ku(n_dof), du;
@@ -1127,6 +1146,64 @@ struct Job {
         }
         solved = true;
         dyn_solved = c.dynamic;
+        nl_solved = c.nonlinear;
+    }
+
+    // Static solve with nonlinearGeom (Model::solve, S/model_analysis.rs:
+    // 625-646): per staticLoadTime, loadRampSteps steps with factors
+    // 1 - ((L - 1 - s) / L)^2, each a Newton loop (max|du| <= 1e-12, at most
+    // 50 iterations) from the current state; brick8 and brickIM.  Each Newton
+    // system is solved as the linear ones: the exact LDL^T factor of the new
+    // tangent (direct) or conj_grad_sparse with its block factor (iterative).
+    void solve_nonlinear() {
+        const Command& c = solve_cmd;
+        if (etype == ASEND_SHELL4) die("geometric nonlinearity is supported for brick8 and brickIM models");
+        const bool iterative = (c.method == "iterative");
+        check(asend_assemble(ctx, 0.0), "asend_assemble");
+        asend_ldl_info li;
+        std::memset(&li, 0, sizeof li);
+        asend_solve_opts o;
+        std::memset(&o, 0, sizeof o);
+        o.check_every = 1;
+        const int frc = asend_ldl_factor(ctx, iterative ? 6 * (int64_t)solver_block_dim : 0, &li);
+        if (frc == ASEND_OK) {
+            std::printf("maxBandwidth: %lld\n", (long long)li.max_bandwidth);
+            o.method = iterative ? ASEND_SOLVE_PCG_FP64 : ASEND_SOLVE_DIRECT;
+            o.precond = ASEND_PC_LDL;
+        } else {
+            if (frc != ASEND_ERR_UNSUPPORTED) check(frc, "asend_ldl_factor");
+            std::printf("note: LDL^T factor not available on the device (%s); FP64 Jacobi PCG instead\n",
+                        asend_last_error(ctx));
+            o.method = ASEND_SOLVE_PCG_FP64;
+            o.precond = ASEND_PC_JACOBI;
+        }
+        if (iterative) {
+            o.stop_rule = ASEND_STOP_REF_ABS;
+            o.tol = c.conv_tol;
+            o.max_it = c.max_it > 0 ? c.max_it : n_dof;
+        } else if (o.method != ASEND_SOLVE_DIRECT) {
+            o.stop_rule = ASEND_STOP_REL_RESIDUAL;
+            o.tol = 1e-12;
+            o.max_it = 100 * n_dof + 1000;
+            o.check_every = 16;
+        }
+        asend_nl_opts nl;
+        std::memset(&nl, 0, sizeof nl);
+        nl.load_steps = (int)std::max<long>(1, c.load_ramp);
+        nl.max_newton = 50;
+        nl.du_tol = 1e-12;
+        for (size_t i = 0; i < c.load_times.size(); ++i) {
+            loads_at(c.load_times[i], f);
+            asend_nl_stats st;
+            std::memset(&st, 0, sizeof st);
+            check(asend_static_nonlinear(ctx, f.data(), u.data(), &nl, &o, &st), "asend_static_nonlinear");
+            std::printf("Nonlinear iterations: %d over %d load steps, last max solution step: %s\n",
+                        st.newton_iterations, st.load_steps, fmt_display(st.last_du).c_str());
+            if (st.unconverged_steps)
+                std::printf("Warning: %d load step(s) ended at 50 nonlinear iterations\n", st.unconverged_steps);
+            if (iterative) std::printf("Total CG iterations: %lld\n", (long long)st.linear_iterations);
+            if (c.save_hist) write_state((long)i, u, v, a);
+        }
     }
 
     void solve_dynamic() {
@@ -1193,6 +1270,10 @@ struct Job {
     std::vector<double> reaction() {
         std::vector<double> rf(n_dof, 0.0);
         if (!solve_cmd.elastic) return rf;  // no displacements: no element forces
+        if (nl_solved) {  // the internal force at the displacements (condensed for brickIM)
+            check(asend_nl_assemble(ctx, u.data(), rf.data()), "asend_nl_assemble");
+            return rf;
+        }
         const double sf = asend_penalty_scale(ctx);
         if (!dyn_solved) {
             check(asend_spmv(ctx, 64, u.data(), rf.data()), "asend_spmv");
@@ -1310,6 +1391,7 @@ struct Job {
     void objective_values() {
         if (!solved) die("the objective needs a solution (run solve first)");
         if (dyn_solved) die("objectives of dynamic solves are not supported by this solver");
+        if (nl_solved) die("objectives of nonlinear solves are not supported by this solver");
         if (!solve_cmd.elastic) die("objectives of thermal solves are not supported by this solver");
         cfields_ok = false;
         for (auto& t : obj) {

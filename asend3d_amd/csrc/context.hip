@@ -2054,6 +2054,44 @@ int asend_nl_assemble(asend_ctx* ctx, const double* u, double* r_int) {
     return ASEND_OK;
 }
 
+int asend_brickim_internal(asend_ctx* ctx, const double* u, double* alpha) {
+    if (!ctx || !alpha) return ASEND_ERR_ARG;
+    if (ctx->etype != ASEND_BRICKIM) return fail(ctx, ASEND_ERR_STATE, "the mesh is not a brickIM mesh");
+    if (const int rc0 = elastic_only(ctx, "asend_brickim_internal")) return rc0;
+    int rc;
+    if ((rc = nl_prepare(ctx))) return rc;
+    const int64_t na = 9 * ctx->n_el;
+    if (!u) {
+        HIPCHK(hipMemcpy(alpha, ctx->d_nl_alpha, na * sizeof(double), hipMemcpyDeviceToHost));
+        return ASEND_OK;
+    }
+    // the maps at rest (u = 0, amplitudes 0) give W = K_ii^-1 K_ie of the
+    // linear element; then alpha = 0 - W u_e by the update kernel
+    double* tmp = nullptr;
+    if ((rc = dalloc(ctx, &tmp, na))) return rc;
+    HIPCHK(hipMemsetAsync(tmp, 0, na * sizeof(double), ctx->st));
+    HIPCHK(hipMemsetAsync(ctx->d_nl_u, 0, ctx->n_dof * sizeof(double), ctx->st));
+    launch_brickim_nl(ctx->st, (int)ctx->n_el, ctx->d_conn_r, ctx->d_xyz_r, ctx->d_E, ctx->d_nu, ctx->d_nl_u, tmp,
+                      ctx->d_kb, ctx->d_nl_re, ctx->d_nl_wmap);
+    HIPCHK(hipMemcpyAsync(ctx->d_nl_r, u, ctx->n_dof * sizeof(double), hipMemcpyHostToDevice, ctx->st));
+    launch_brickim_nl_internal(ctx->st, (int)ctx->n_el, ctx->d_conn_r, ctx->d_nl_r, ctx->d_nl_wmap, tmp);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(alpha, tmp, na * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    dfree(tmp);
+    return ASEND_OK;
+}
+
+int asend_brickim_set_internal(asend_ctx* ctx, const double* alpha) {
+    if (!ctx || !alpha) return ASEND_ERR_ARG;
+    if (ctx->etype != ASEND_BRICKIM) return fail(ctx, ASEND_ERR_STATE, "the mesh is not a brickIM mesh");
+    if (const int rc0 = elastic_only(ctx, "asend_brickim_set_internal")) return rc0;
+    int rc;
+    if ((rc = nl_prepare(ctx))) return rc;
+    HIPCHK(hipMemcpy(ctx->d_nl_alpha, alpha, 9 * ctx->n_el * sizeof(double), hipMemcpyHostToDevice));
+    return ASEND_OK;
+}
+
 int asend_static_nonlinear(asend_ctx* ctx, const double* f, double* u, const asend_nl_opts* nl,
                            const asend_solve_opts* opts, asend_nl_stats* stats) {
     if (!ctx || !f || !u || !opts) return ASEND_ERR_ARG;

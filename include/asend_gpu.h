@@ -462,6 +462,17 @@ int asend_nl_assemble(asend_ctx* ctx, const double* u, double* r_int);
 int asend_static_nonlinear(asend_ctx* ctx, const double* f, double* u, const asend_nl_opts* nl,
                            const asend_solve_opts* opts, asend_nl_stats* stats);
 
+/* brickIM bubble amplitudes, 9 per element in element order, 3 k + component
+ * (the reference's internal DOFs Element::internal_disp in dof_table order,
+ * S/element_meth.rs:193-201; written after the nodes in the state files,
+ * S/model_output.rs:70-78).  With u (n_dof): those that follow a linear solve
+ * reaching u from rest, -K_ii^-1 K_ie u_e (update_internal,
+ * S/element_equations.rs:123-194); with u = NULL: the amplitudes held by the
+ * nonlinear path.  brickIM meshes. */
+int asend_brickim_internal(asend_ctx* ctx, const double* u, double* alpha);
+/* Replace the held amplitudes (read_time_step_soln followed by a backstep). */
+int asend_brickim_set_internal(asend_ctx* ctx, const double* alpha);
+
 /* Timing helper: average duration (ms) of n_iter back-to-back launches of the
  * SpMV kernel of the given precision, HIP events on the context stream. */
 int asend_bench_spmv(asend_ctx* ctx, int precision, int n_iter, double* avg_ms);

@@ -1,0 +1,117 @@
+"""The job-file driver on brickIM jobs: hexBeam test 6
+(E/testCases/hexBeam/tipMomentNonlinear.py: nonlinearGeom, loadRampSteps 5,
+a tip couple of dead nodal forces on the reference's 100 x 10 x 1 brickIM
+mesh), checked against the known answer of E/testCases/hexBeam/notes.txt,
+and the state files of brickIM solves, whose element internal DOFs follow the
+nodes: after the nonlinear solve those the Newton loop holds, after a linear
+one -K_ii^-1 K_ie u_e of the oracle's element."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import nonlinear as NL
+from asend3d_amd.meshgen import hex_block
+from conftest import gpu_case
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "asend3d_amd", "asend_run_job")
+
+
+def dump(path, data):
+    import yaml
+    path.write_text(yaml.dump(data, sort_keys=False, width=200).replace("'", "").replace('"', ""))
+
+
+def write_hexbeam(work, loads, solve):
+    m = hex_block(100, 10, 1, 10.0, 1.0, 0.1)
+    tip = m.nodes_at_x(m.nx)
+    model = {
+        "nodes": [str([i] + [float(v) for v in m.xyz[i]]) for i in range(m.n_nodes)],
+        "elements": [{"type": "brickIM",
+                      "connectivity": [str([e] + [int(v) for v in m.conn[e]]) for e in range(m.n_el)]}],
+        "sets": {"node": {"xMin": [int(i) for i in m.nodes_at_x(0)], "xMax": [int(i) for i in tip],
+                          "xMaxzMin": [int(i) for i in tip if m.xyz[i, 2] < 0.05],
+                          "xzMax": [int(i) for i in tip if m.xyz[i, 2] > 0.05]}},
+        "sections": [{"type": "solid", "material": "myMat", "elementSet": "all"}],
+        "materials": {"myMat": {"density": 1.0, "elastic": {"E": str([1e6] * 3), "nu": str([0.0] * 3),
+                                                            "G": str([5e5] * 3)}}},
+        "constraints": [{"type": "displacement", "terms": [{"nodeSet": "xMin", "dof": d, "coef": 1.0}],
+                         "rhs": 0.0} for d in (1, 2, 3)],
+    }
+    dump(work / "hexBeam.yaml", model)
+    dump(work / "loads.yaml", {"loads": loads})
+    (work / "results").mkdir()
+    job = {"jobCommands": [
+        {"command": "readModelInput", "fileName": "hexBeam.yaml"},
+        {"command": "readLoads", "fileName": "loads.yaml"},
+        dict({"command": "solve", "solverMethod": "direct", "solnHistDir": "results"}, **solve),
+        {"command": "writeNodeResults", "fileName": "results/nodeResults.csv", "nodeSet": "all",
+         "fields": ["displacement"]},
+    ]}
+    dump(work / "job.yaml", job)
+    return m
+
+
+def run(work):
+    r = subprocess.run([CLI, "job.yaml"], cwd=work, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-3000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+def node_displacements(work, m):
+    rows = np.loadtxt(work / "results" / "nodeResults.csv", delimiter=",", skiprows=1)
+    return rows[:, 1:4]
+
+
+def state(work, m, step=0):
+    raw = np.fromfile(work / "results" / f"solnTStep{step}.out", dtype=">f8")
+    assert raw.size == 9 * m.n_nodes + 9 * m.n_el, raw.size
+    nodes = raw[:9 * m.n_nodes].reshape(m.n_nodes, 3, 3)
+    return nodes[:, 0], raw[9 * m.n_nodes:].reshape(m.n_el, 9)
+
+
+def test_cli_hexbeam_tip_moment_nonlinear(tmp_path):
+    gpu_case("asend_run_job hexBeam test 6 (brickIM, nonlinearGeom)")
+    c = float(65.4498469 / (11 * np.sqrt(2.0)))
+    loads = [{"type": "nodalForce", "nodeSet": "xMaxzMin", "load": str([c, 0.0, c, 0.0, 0.0, 0.0])},
+             {"type": "nodalForce", "nodeSet": "xzMax", "load": str([-c, 0.0, -c, 0.0, 0.0, 0.0])}]
+    work = tmp_path / "moment"
+    work.mkdir()
+    m = write_hexbeam(work, loads, {"nonlinearGeom": "yes", "loadRampSteps": 5})
+    out = run(work)
+    U = node_displacements(work, m)
+    tip = m.nodes_at_x(m.nx)
+    axis = tip[np.abs(m.xyz[tip, 1] - 0.5) < 1e-9]
+    w = U[axis].mean(axis=0)
+    want = np.array([-0.99684, 0.0, 3.729232])
+    assert np.abs(w - want).max() <= 5e-5 * np.linalg.norm(want), w
+    assert "Nonlinear iterations:" in out
+    u_state, alpha = state(work, m)
+    assert np.abs(u_state - U).max() <= 1e-12 * np.abs(U).max()
+    assert np.isfinite(alpha).all() and np.abs(alpha).max() > 0.0
+
+
+def test_cli_brickim_linear_state_internal_dofs(tmp_path):
+    gpu_case("asend_run_job brickIM linear state file internal DOFs")
+    loads = [{"type": "nodalForce", "nodeSet": "xMax", "load": str([0.0, 0.0, 0.1, 0.0, 0.0, 0.0])}]
+    work = tmp_path / "linear"
+    work.mkdir()
+    m = write_hexbeam(work, loads, {})
+    run(work)
+    U = node_displacements(work, m)
+    u_state, alpha = state(work, m)
+    assert np.abs(u_state - U).max() <= 1e-12 * np.abs(U).max()
+    # update_internal after a linear solve from rest: alpha = -K_ii^-1 K_ie u_e
+    X = m.xyz[m.conn]
+    E = np.full(m.n_el, 1e6)
+    r, K = NL.element_residual_tangent(X, np.zeros((m.n_el, 11, 3)), E, np.zeros(m.n_el), im=True)
+    _, _, W, _ = NL.condense(r, K)
+    want = -np.einsum("eij,ej->ei", W, U[m.conn].reshape(m.n_el, 24))
+    assert np.abs(alpha - want).max() <= 1e-9 * np.abs(want).max()
+    # the tip deflection of hexBeam test 1 scaled to this load: F L^3 / 3 E I with F = 0.1 x 22 nodes
+    assert U[m.nodes_at_x(m.nx), 2].mean() == pytest.approx(2.2 * 1000 / (3 * 1e6 / 12000), rel=2e-2)
