@@ -37,6 +37,7 @@
 //     the _grad file), numbers formatted as the reference's {:.12e}.
 // Anything else stops the job with an error message, as the reference's
 // panics do.
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdarg>
@@ -223,7 +224,7 @@ struct ObjTerm {
     double value = 0.0;
 };
 struct Command {
-    std::string cmd, file, method, node_set = "all";
+    std::string cmd, file, method, node_set = "all", el_set = "all", position = "centroid";
     bool elastic = true, dynamic = false, nonlinear = false, thermal = false, write_grad = true;
     long block_dim = -1, max_it = 0, load_ramp = 1;
     double conv_tol = 1.0e-12;
@@ -335,6 +336,8 @@ struct Job {
             else if (k == "maxIterations" && d.size() == 1) c.max_it = to_i(d[0]);
             else if (k == "convergenceTol" && d.size() == 1) c.conv_tol = to_d(d[0]);
             else if (k == "nodeSet" && d.size() == 1) c.node_set = d[0];
+            else if (k == "elementSet" && d.size() == 1) c.el_set = d[0];
+            else if (k == "position" && d.size() == 1) c.position = d[0];
             else if (k == "fields" && d.size() == 1) c.fields.push_back(d[0]);
             else if (k == "include" && d.size() == 1) c.include.push_back(d[0]);
             else if (k == "writeGradient" && d.size() == 1) c.write_grad = (d[0] == "yes");
@@ -1686,6 +1689,159 @@ struct Job {
         std::fclose(fp);
     }
 
+    // write_element_results (S/model_output.rs:264-470) at the element
+    // centroids of solid elements: strain (engineering shears; Green-Lagrange
+    // after a nonlinearGeom solve, get_solid_strain_dfd0,
+    // S/element_soln_fields.rs:1345-1461) with its principal values, stress
+    // C strain with von Mises and principal values (Model::mises,
+    // principal_stress / principal_strain, S/model_res_utils.rs:139-196,
+    // ascending), strainEnergyDen = stress . strain / 2.  The bubbles of
+    // brickIM have zero gradient at the centroid.  Host arithmetic on the
+    // solved displacements (output, not the solve path); the principal values
+    // come from Jacobi rotations, the reference's from its own iteration.
+    void write_element_results(const Command& c) {
+        if (!solved) die("element results need a solution (run solve first)");
+        if (etype == ASEND_SHELL4) die("element results of shell elements are not supported by this solver");
+        if (c.position != "centroid") die("element results at position %s are not supported", c.position.c_str());
+        for (const auto& fld : c.fields)
+            if (fld != "stress" && fld != "strain" && fld != "strainEnergyDen")
+                die("element result field %s is not supported by this solver", fld.c_str());
+        if (c.time_steps.empty()) {
+            write_element_file(c, c.file, -1.0);
+            return;
+        }
+        const size_t dot = c.file.find('.');
+        const std::string root = dot == std::string::npos ? c.file : c.file.substr(0, dot);
+        const std::string ext = dot == std::string::npos ? "" : c.file.substr(dot);
+        for (long ts : c.time_steps) {
+            read_state(ts, u, v, a);
+            double time;
+            if (dyn_solved) {
+                time = solve_cmd.time_step * (double)ts;
+            } else {
+                const auto& lt = solve_cmd.load_times;
+                time = lt[std::min<size_t>((size_t)std::max<long>(ts, 0), lt.size() - 1)];
+            }
+            write_element_file(c, root + "_timestep" + std::to_string(ts) + ext, time);
+        }
+    }
+
+    // eigenvalues of a symmetric 3x3 matrix by cyclic Jacobi rotations, ascending
+    static void sym_eig3(const double* m_in, double* ev) {
+        double m[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) m[i][j] = m_in[3 * i + j];
+        for (int sweep = 0; sweep < 50; ++sweep) {
+            const double off = m[0][1] * m[0][1] + m[0][2] * m[0][2] + m[1][2] * m[1][2];
+            const double dg = m[0][0] * m[0][0] + m[1][1] * m[1][1] + m[2][2] * m[2][2];
+            if (off <= 1e-36 * dg || off == 0.0) break;
+            for (int p = 0; p < 2; ++p)
+                for (int q = p + 1; q < 3; ++q) {
+                    if (m[p][q] == 0.0) continue;
+                    const double th = (m[q][q] - m[p][p]) / (2.0 * m[p][q]);
+                    const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+                    const double cs = 1.0 / std::sqrt(t * t + 1.0), sn = t * cs;
+                    for (int k = 0; k < 3; ++k) {  // columns p, q
+                        const double kp = m[k][p], kq = m[k][q];
+                        m[k][p] = cs * kp - sn * kq;
+                        m[k][q] = sn * kp + cs * kq;
+                    }
+                    for (int k = 0; k < 3; ++k) {  // rows p, q
+                        const double pk = m[p][k], qk = m[q][k];
+                        m[p][k] = cs * pk - sn * qk;
+                        m[q][k] = sn * pk + cs * qk;
+                    }
+                }
+        }
+        for (int i = 0; i < 3; ++i) ev[i] = m[i][i];
+        std::sort(ev, ev + 3);
+    }
+
+    void write_element_file(const Command& c, const std::string& name, double time) {
+        const auto& set = element_set(c.el_set);
+        std::FILE* fp = std::fopen(name.c_str(), "w");
+        if (!fp) die("could not open file %s", name.c_str());
+        std::fprintf(fp, "element,int_pt,layer,");
+        for (const auto& fld : c.fields)
+            std::fprintf(fp, "%s", fld == "stress"   ? "S11,S22,S33,S12,S13,S23,MISES,PS1,PS2,PS3,"
+                                   : fld == "strain" ? "E11,E22,E33,E12,E13,E23,PE1,PE2,PE3,"
+                                                     : "SE,");
+        std::fprintf(fp, "time\n");
+        static const double sg[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                                        {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+        for (int32_t e : set) {
+            double X[8][3], U[8][3];
+            for (int t = 0; t < 8; ++t)
+                for (int k = 0; k < 3; ++k) {
+                    const int32_t nd = conn[e][t];
+                    X[t][k] = xyz_dv[3 * (size_t)nd + k];
+                    const int64_t d = dof_index[6 * (size_t)nd + k];
+                    U[t][k] = d >= 0 ? u[d] : 0.0;
+                }
+            // centroid: dN_t/ds_j = sg[t][j] / 8
+            double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+            for (int t = 0; t < 8; ++t)
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) J[i][j] += X[t][i] * 0.125 * sg[t][j];
+            const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1], c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2],
+                         c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+            const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+            const double Ji[3][3] = {{c00 / det, (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det,
+                                      (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det},
+                                     {c01 / det, (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det,
+                                      (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det},
+                                     {c02 / det, (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det,
+                                      (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det}};
+            double H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // H_ij = du_i / dx_j
+            for (int t = 0; t < 8; ++t) {
+                double g[3];
+                for (int j = 0; j < 3; ++j)
+                    g[j] = 0.125 * (sg[t][0] * Ji[0][j] + sg[t][1] * Ji[1][j] + sg[t][2] * Ji[2][j]);
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) H[i][j] += U[t][i] * g[j];
+            }
+            const int pr[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+            double eps[6], sig[6];
+            for (int q = 0; q < 6; ++q) {
+                const int i = pr[q][0], j = pr[q][1];
+                double v = H[i][j] + H[j][i];
+                if (solve_cmd.nonlinear)
+                    for (int k = 0; k < 3; ++k) v += H[k][i] * H[k][j];
+                eps[q] = q < 3 ? 0.5 * v : v;
+            }
+            const double Ee = E[e], ne = nu[e];
+            const double lam = Ee * ne / ((1.0 + ne) * (1.0 - 2.0 * ne)), mu = Ee / (2.0 * (1.0 + ne));
+            const double tr = eps[0] + eps[1] + eps[2];
+            for (int q = 0; q < 6; ++q) sig[q] = q < 3 ? lam * tr + 2.0 * mu * eps[q] : mu * eps[q];
+            std::fprintf(fp, "%d,0,0,", e);
+            for (const auto& fld : c.fields) {
+                if (fld == "strain") {
+                    for (int q = 0; q < 6; ++q) std::fprintf(fp, "%s,", fmt_e12(eps[q]).c_str());
+                    const double m[9] = {eps[0],       0.5 * eps[3], 0.5 * eps[4], 0.5 * eps[3], eps[1],
+                                         0.5 * eps[5], 0.5 * eps[4], 0.5 * eps[5], eps[2]};
+                    double pv[3];
+                    sym_eig3(m, pv);
+                    for (int k = 0; k < 3; ++k) std::fprintf(fp, "%s,", fmt_e12(pv[k]).c_str());
+                } else if (fld == "stress") {
+                    for (int q = 0; q < 6; ++q) std::fprintf(fp, "%s,", fmt_e12(sig[q]).c_str());
+                    const double a1 = sig[0] - sig[1], b1 = sig[1] - sig[2], c1 = sig[2] - sig[0];
+                    const double d1 = sig[3] * sig[3] + sig[4] * sig[4] + sig[5] * sig[5];
+                    std::fprintf(fp, "%s,", fmt_e12(std::sqrt(0.5 * (a1 * a1 + b1 * b1 + c1 * c1 + 6.0 * d1))).c_str());
+                    const double m[9] = {sig[0], sig[3], sig[4], sig[3], sig[1], sig[5], sig[4], sig[5], sig[2]};
+                    double pv[3];
+                    sym_eig3(m, pv);
+                    for (int k = 0; k < 3; ++k) std::fprintf(fp, "%s,", fmt_e12(pv[k]).c_str());
+                } else {
+                    double se = 0.0;
+                    for (int q = 0; q < 6; ++q) se += sig[q] * eps[q];
+                    std::fprintf(fp, "%s,", fmt_e12(0.5 * se).c_str());
+                }
+            }
+            std::fprintf(fp, "%s\n", fmt_e12(time).c_str());
+        }
+        std::fclose(fp);
+    }
+
     void write_objective(const Command& c) {
         std::FILE* fp = std::fopen(c.file.c_str(), "w");
         if (!fp) die("could not open file %s", c.file.c_str());
@@ -1758,6 +1914,9 @@ struct Job {
             } else if (c.cmd == "writeNodeResults") {
                 std::printf("writing Node results\n");
                 write_node_results(c);
+            } else if (c.cmd == "writeElementResults") {
+                std::printf("writing Element results\n");
+                write_element_results(c);
             } else if (c.cmd == "writeObjective") {
                 std::printf("writing Objective results\n");
                 write_objective(c);

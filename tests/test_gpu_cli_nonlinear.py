@@ -26,7 +26,7 @@ def dump(path, data):
     path.write_text(yaml.dump(data, sort_keys=False, width=200).replace("'", "").replace('"', ""))
 
 
-def write_hexbeam(work, loads, solve):
+def write_hexbeam(work, loads, solve, extra=()):
     m = hex_block(100, 10, 1, 10.0, 1.0, 0.1)
     tip = m.nodes_at_x(m.nx)
     model = {
@@ -51,7 +51,7 @@ def write_hexbeam(work, loads, solve):
         dict({"command": "solve", "solverMethod": "direct", "solnHistDir": "results"}, **solve),
         {"command": "writeNodeResults", "fileName": "results/nodeResults.csv", "nodeSet": "all",
          "fields": ["displacement"]},
-    ]}
+    ] + list(extra)}
     dump(work / "job.yaml", job)
     return m
 
@@ -82,7 +82,7 @@ def test_cli_hexbeam_tip_moment_nonlinear(tmp_path):
              {"type": "nodalForce", "nodeSet": "xzMax", "load": str([-c, 0.0, -c, 0.0, 0.0, 0.0])}]
     work = tmp_path / "moment"
     work.mkdir()
-    m = write_hexbeam(work, loads, {"nonlinearGeom": "yes", "loadRampSteps": 5})
+    m = write_hexbeam(work, loads, {"nonlinearGeom": "yes", "loadRampSteps": 5}, ELEMENT_RESULTS)
     out = run(work)
     U = node_displacements(work, m)
     tip = m.nodes_at_x(m.nx)
@@ -94,6 +94,46 @@ def test_cli_hexbeam_tip_moment_nonlinear(tmp_path):
     u_state, alpha = state(work, m)
     assert np.abs(u_state - U).max() <= 1e-12 * np.abs(U).max()
     assert np.isfinite(alpha).all() and np.abs(alpha).max() > 0.0
+    # element centroids lie on the beam axis: under a pure couple the
+    # Green-Lagrange axial strain there vanishes although the tip elements
+    # have turned by 45 degrees (a small-strain measure would give
+    # cos 45 - 1 = -0.29 there)
+    head, el = read_elements(work)
+    tip_els = np.arange(m.nx - 5, m.nx)  # y-row 0
+    assert np.abs(el[tip_els, head.index("E11")]).max() <= 1e-3
+    check_element_identities(head, el, 1e6)
+
+
+ELEMENT_RESULTS = [{"command": "writeElementResults", "fileName": "results/elementResults.csv",
+                    "fields": ["strain", "stress", "strainEnergyDen"]}]
+
+
+def read_elements(work):
+    import csv
+    with open(work / "results" / "elementResults.csv") as fh:
+        rows = list(csv.reader(fh))
+    return rows[0], np.array([[float(x) for x in r] for r in rows[1:]])
+
+
+def check_element_identities(head, el, E):
+    """nu = 0: S = E strain (shears E/2 gamma), Mises and principal values of
+    the stress tensor, strain energy density S.E/2."""
+    col = {h: i for i, h in enumerate(head)}
+    eps = el[:, [col[k] for k in ("E11", "E22", "E33", "E12", "E13", "E23")]]
+    sig = el[:, [col[k] for k in ("S11", "S22", "S33", "S12", "S13", "S23")]]
+    scale = np.abs(sig).max()
+    assert np.abs(sig[:, :3] - E * eps[:, :3]).max() <= 1e-12 * scale + 1e-300
+    assert np.abs(sig[:, 3:] - 0.5 * E * eps[:, 3:]).max() <= 1e-12 * scale + 1e-300
+    T = np.zeros((len(el), 3, 3))
+    for q, (i, j) in enumerate(((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))):
+        T[:, i, j] = T[:, j, i] = sig[:, q]
+    pv = np.linalg.eigvalsh(T)
+    assert np.abs(el[:, [col["PS1"], col["PS2"], col["PS3"]]] - pv).max() <= 1e-10 * scale
+    dev = T - np.trace(T, axis1=1, axis2=2)[:, None, None] * np.eye(3) / 3
+    mises = np.sqrt(1.5 * np.einsum("eij,eij->e", dev, dev))
+    assert np.abs(el[:, col["MISES"]] - mises).max() <= 1e-10 * scale
+    se = 0.5 * np.einsum("ei,ei->e", sig, eps)
+    assert np.abs(el[:, col["SE"]] - se).max() <= 1e-12 * np.abs(se).max() + 1e-300
 
 
 def test_cli_brickim_linear_state_internal_dofs(tmp_path):
@@ -115,3 +155,27 @@ def test_cli_brickim_linear_state_internal_dofs(tmp_path):
     assert np.abs(alpha - want).max() <= 1e-9 * np.abs(want).max()
     # the tip deflection of hexBeam test 1 scaled to this load: F L^3 / 3 E I with F = 0.1 x 22 nodes
     assert U[m.nodes_at_x(m.nx), 2].mean() == pytest.approx(2.2 * 1000 / (3 * 1e6 / 12000), rel=2e-2)
+
+
+def test_cli_hexbeam_axial_element_results(tmp_path):
+    """hexBeam test 4 (Fx = 2 on the tip): stress F/(bh) = 20, strain
+    F/(Ebh) = 2e-5, strain energy density 2e-4, away from the ends."""
+    gpu_case("asend_run_job brickIM element results, axial load")
+    loads = [{"type": "nodalForce", "nodeSet": "xMax", "load": str([2.0 / 22, 0.0, 0.0, 0.0, 0.0, 0.0])}]
+    work = tmp_path / "axial"
+    work.mkdir()
+    m = write_hexbeam(work, loads, {}, ELEMENT_RESULTS)
+    run(work)
+    head, el = read_elements(work)
+    assert el.shape[0] == m.n_el and head[:3] == ["element", "int_pt", "layer"] and head[-1] == "time"
+    col = {h: i for i, h in enumerate(head)}
+    ex = (np.arange(m.n_el) % m.nx)
+    mid = (ex >= 30) & (ex < 70)
+    # the equal tip nodal forces leave a disturbance of ~1e-6 three widths in
+    assert np.abs(el[mid, col["S11"]] - 20.0).max() <= 5e-6 * 20.0
+    assert np.abs(el[mid, col["E11"]] - 2e-5).max() <= 5e-6 * 2e-5
+    assert np.abs(el[mid, col["SE"]] - 2e-4).max() <= 1e-5 * 2e-4
+    assert np.abs(el[mid, col["PS3"]] - 20.0).max() <= 5e-6 * 20.0
+    assert np.abs(el[mid, col["MISES"]] - 20.0).max() <= 5e-6 * 20.0
+    assert (el[:, col["time"]] == -1.0).all()
+    check_element_identities(head, el, 1e6)
