@@ -148,6 +148,7 @@ EXPORTED = ["asend_ctx_create", "asend_ctx_destroy", "asend_last_error", "asend_
             "asend_reorder_nodes", "asend_block_dim_auto", "asend_surface_traction", "asend_lognormal_moduli",
             "asend_compliance_clamp_check", "asend_surface_pressure", "asend_body_load", "asend_surface_flux",
             "asend_set_conductivity", "asend_brickim_internal", "asend_brickim_set_internal",
+            "asend_eigen",
             "asend_upload_mesh", "asend_set_thickness", "asend_set_grid", "asend_update_moduli", "asend_set_penalty",
             "asend_penalty_scale", "asend_uniform_elements", "asend_assemble", "asend_get_matrix", "asend_spmv", "asend_solve", "asend_solve_noalloc",
             "asend_ldl_factor", "asend_ldl_solve", "asend_ldl_release", "asend_cycle_spmv_timing",
@@ -654,6 +655,21 @@ class Context:
         self._chk(_lib.asend_modal(self.h, int(n_modes), _dbl(shift), ctypes.byref(opts), _p(vals), _p(fr),
                                    _p(vecs) if vectors else None, ctypes.byref(st)), "asend_modal")
         return vals, fr, vecs, st
+
+    def eigen(self, kind, n_modes, u=None, shift=0.0, opts=None, vectors=True):
+        """asend_eigen: kind 0 (frequency) or 1 (buckling) about the tangent at
+        u (None: the linear stiffness); returns (eigenvalues, frequencies or
+        load factors, vectors or None, ModalStats)."""
+        if opts is None:
+            opts = self.opts(SOLVE_PCG_FP64, PC_JACOBI, tol=1e-11, max_it=2000000)
+        vals = np.zeros(n_modes)
+        fac = np.zeros(n_modes)
+        vecs = np.zeros((n_modes, self.n_dof)) if vectors else None
+        st = ModalStats()
+        self._chk(_lib.asend_eigen(self.h, int(kind), _p(None if u is None else _f64(u)), int(n_modes), _dbl(shift),
+                                   ctypes.byref(opts), _p(vals), _p(fac), _p(vecs) if vectors else None,
+                                   ctypes.byref(st)), "asend_eigen")
+        return vals, fac, vecs, st
     def autotune(self, rhs=None, what=TUNE_SPMV | TUNE_SOLVER):
         t = Tuning()
         r = None if rhs is None else _f64(rhs)

@@ -2781,9 +2781,17 @@ static void jacobi_eigen(int m, std::vector<double>& A, std::vector<double>& w, 
     for (int i = 0; i < m; ++i) w[i] = A[(size_t)i * m + i];
 }
 
-int asend_modal(asend_ctx* ctx, int n_modes, double shift, const asend_solve_opts* opts, double* eig_vals,
-                double* freqs_hz, double* eig_vecs, asend_modal_stats* stats) {
-    if (!ctx || n_modes < 1 || !opts || !eig_vals) return ASEND_ERR_ARG;
+// Model::eigen_solve (S/model_analysis.rs:713-888).  kind 0 (frequency):
+// lumped mass; kind 1 (buckling): identity "mass" and, per mode, the load
+// factor phi^T K phi / (phi^T K phi - lambda) with K the stiffness at zero
+// displacement and |phi| = 1.  With u the operator is the tangent at u
+// (build_elastic_soln_load with nonlinear_geom, as eigen_solve builds it),
+// else the linear stiffness.  out_fact: frequencies (Hz, -1 for a negative
+// eigenvalue) or load factors.
+static int modal_impl(asend_ctx* ctx, int kind, const double* u, int n_modes, double shift,
+                      const asend_solve_opts* opts, double* eig_vals, double* out_fact, double* eig_vecs,
+                      asend_modal_stats* stats) {
+    if (!ctx || n_modes < 1 || !opts || !eig_vals || (kind != 0 && kind != 1)) return ASEND_ERR_ARG;
     if (const int rc0 = elastic_only(ctx, "asend_modal")) return rc0;
     if (!ctx->d_K64) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
     if (ctx->etype == ASEND_SHELL4 && !ctx->thk_set)
@@ -2809,8 +2817,10 @@ int asend_modal(asend_ctx* ctx, int n_modes, double shift, const asend_solve_opt
     }
     HIPCHK(hipEventRecord(ctx->ev0, ctx->st));
     // consistent mass alone (zero moduli for the brick8 kernels, no stiffness
-    // term for the kernels that condense internal DOFs)
-    if (ctx->etype == ASEND_SHELL4 || ctx->etype == ASEND_BRICKIM) {
+    // term for the kernels that condense internal DOFs); buckling: none
+    if (kind == 1) {
+        rc = ASEND_OK;
+    } else if (ctx->etype == ASEND_SHELL4 || ctx->etype == ASEND_BRICKIM) {
         rc = integrate_elements(ctx, ctx->d_E, 0.0, 1.0, Mtmp);
     } else {
         double* zeroE = nullptr;
@@ -2829,11 +2839,13 @@ int asend_modal(asend_ctx* ctx, int n_modes, double shift, const asend_solve_opt
         std::vector<double> h1(n, 1.0);
         HIPCHK(hipMemcpy(ones, h1.data(), n * 8, hipMemcpyHostToDevice));
     }
-    launch_sell3<double>(ctx->st, 0, ctx->fine(), Mtmp, ones, dm, nullptr, nullptr, nullptr);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(ctx->st));  // the context stream does not order null-stream copies
-    std::vector<double> hm(n);
-    HIPCHK(hipMemcpy(hm.data(), dm, n * 8, hipMemcpyDeviceToHost));
+    std::vector<double> hm(n, 1.0);
+    if (kind == 0) {
+        launch_sell3<double>(ctx->st, 0, ctx->fine(), Mtmp, ones, dm, nullptr, nullptr, nullptr);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(ctx->st));  // the context stream does not order null-stream copies
+        HIPCHK(hipMemcpy(hm.data(), dm, n * 8, hipMemcpyDeviceToHost));
+    }
     double m_avg = 0.0;
     for (double v : hm) m_avg += std::fabs(v);
     m_avg /= (double)n;
@@ -2859,7 +2871,13 @@ int asend_modal(asend_ctx* ctx, int n_modes, double shift, const asend_solve_opt
     }
     dfree(Mtmp);
     // shifted operator K_eff - shift diag(m) (penalty included)
-    if ((rc = assemble_device(ctx, 1.0, 0.0))) {
+    if (u) {
+        HIPCHK(hipMemcpy(ctx->d_nl_u, u, n * 8, hipMemcpyHostToDevice));
+        rc = nl_assemble(ctx, 0.0, nullptr);
+    } else {
+        rc = assemble_device(ctx, 1.0, 0.0);
+    }
+    if (rc) {
         cleanup();
         return rc;
     }
@@ -2933,11 +2951,16 @@ int asend_modal(asend_ctx* ctx, int n_modes, double shift, const asend_solve_opt
     std::sort(order.begin(), order.end(), [&](int x, int y) { return th[x] > th[y]; });
     const int nm_out = std::min(n_modes, steps);
     double max_res = 0.0;
+    std::vector<double> vec_tmp;
+    if (kind == 1 && !eig_vecs) {  // the load factors need the modes
+        vec_tmp.resize((size_t)n_modes * n);
+        eig_vecs = vec_tmp.data();
+    }
     for (int k = 0; k < nm_out; ++k) {
         const int i = order[k];
         const double lam = shift + 1.0 / th[i];
         eig_vals[k] = lam;
-        if (freqs_hz) freqs_hz[k] = lam >= 0.0 ? std::sqrt(lam) / (2.0 * M_PI) : -1.0;
+        if (out_fact && kind == 0) out_fact[k] = lam >= 0.0 ? std::sqrt(lam) / (2.0 * M_PI) : -1.0;
         // residual estimate of the Ritz pair in B: |b_last * z_last| / theta
         max_res = std::max(max_res, std::fabs(b_last * Z[(size_t)(steps - 1) * steps + i]) / std::fabs(th[i]));
         if (eig_vecs) {
@@ -2960,15 +2983,38 @@ int asend_modal(asend_ctx* ctx, int n_modes, double shift, const asend_solve_opt
     }
     for (int k = nm_out; k < n_modes; ++k) {
         eig_vals[k] = 0.0;
-        if (freqs_hz) freqs_hz[k] = -1.0;
+        if (out_fact) out_fact[k] = kind == 0 ? -1.0 : 1.0e100;
+    }
+    if (kind == 1) {
+        // v_kv = phi^T K phi with the stiffness at zero displacement, without
+        // the constraint penalty (elastic_mat alone)
+        if ((rc = assemble_device(ctx, 1.0, 0.0))) {
+            cleanup();
+            return rc;
+        }
+        std::vector<double> y(n);
+        for (int k = 0; k < nm_out; ++k) {
+            const double* ph = eig_vecs + (int64_t)k * n;
+            HIPCHK(hipMemcpy(w, ph, n * 8, hipMemcpyHostToDevice));
+            launch_sell3<double>(ctx->st, 0, ctx->fine(), ctx->d_K64, w, ctx->d_f, nullptr, nullptr, nullptr);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(ctx->st));
+            HIPCHK(hipMemcpy(y.data(), ctx->d_f, n * 8, hipMemcpyDeviceToHost));
+            double vkv = 0.0;
+            for (int64_t i = 0; i < n; ++i) vkv += ph[i] * y[i];
+            for (size_t q = 0; q < ctx->h_pen.size(); ++q)
+                vkv -= ctx->h_pen_mult[q] * ctx->sf * ph[ctx->h_pen[q]] * ph[ctx->h_pen[q]];
+            if (out_fact) out_fact[k] = (vkv - eig_vals[k]) != 0.0 ? vkv / (vkv - eig_vals[k]) : 1.0e100;
+        }
     }
     HIPCHK(hipEventRecord(ctx->ev1, ctx->st));
     HIPCHK(hipEventSynchronize(ctx->ev1));
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     cleanup();
-    // the operator now holds the shift: a static solve must assemble again
-    if (shift != 0.0) ctx->assembled = false;
+    // the operator now holds the shift (or the tangent): a static solve must assemble again
+    if (shift != 0.0 || u) ctx->assembled = false;
+    if (kind == 1) ctx->assembled = true;  // buckling ends with the linear stiffness assembled
     if (stats) {
         asend_modal_stats s;
         std::memset(&s, 0, sizeof(s));
@@ -2982,6 +3028,20 @@ int asend_modal(asend_ctx* ctx, int n_modes, double shift, const asend_solve_opt
         *stats = s;
     }
     return ASEND_OK;
+}
+
+int asend_modal(asend_ctx* ctx, int n_modes, double shift, const asend_solve_opts* opts, double* eig_vals,
+                double* freqs_hz, double* eig_vecs, asend_modal_stats* stats) {
+    return modal_impl(ctx, 0, nullptr, n_modes, shift, opts, eig_vals, freqs_hz, eig_vecs, stats);
+}
+
+int asend_eigen(asend_ctx* ctx, int kind, const double* u, int n_modes, double shift, const asend_solve_opts* opts,
+                double* eig_vals, double* factors, double* eig_vecs, asend_modal_stats* stats) {
+    if (ctx && u) {
+        int rc;
+        if ((rc = nl_prepare(ctx))) return rc;
+    }
+    return modal_impl(ctx, kind, u, n_modes, shift, opts, eig_vals, factors, eig_vecs, stats);
 }
 
 // ---------------------------------------------------------------------------

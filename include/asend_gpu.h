@@ -633,6 +633,22 @@ typedef struct asend_modal_stats {
 int asend_modal(asend_ctx* ctx, int n_modes, double shift, const asend_solve_opts* opts, double* eig_vals,
                 double* freqs_hz, double* eig_vecs, asend_modal_stats* stats);
 
+/* Model::eigen_solve with its state (S/model_analysis.rs:713-888): the
+ * operator is the tangent stiffness at the displacement u (n_dof; the
+ * reference builds it with nonlinear_geom at the current state, so a
+ * preceding static solve stiffens or softens the modes; brickIM uses the
+ * bubble amplitudes held, asend_brickim_set_internal), or the linear
+ * stiffness when u is NULL.  kind 0 (type frequency): as asend_modal,
+ * factors[k] = frequency in Hz.  kind 1 (type buckling): identity mass
+ * (diag_mass = 1), eigenvalues of K_T(u) - shift I plus the shift, and
+ * factors[k] = v / (v - eig_vals[k]) with v = phi_k^T K phi_k, K the
+ * stiffness at zero displacement without the penalty and |phi_k| = 1 (the
+ * reference's load_fact: the multiple of the applied load at which the mode
+ * buckles; 1e100 when v = eig_vals[k]).  Buckling leaves the linear
+ * stiffness assembled. */
+int asend_eigen(asend_ctx* ctx, int kind, const double* u, int n_modes, double shift, const asend_solve_opts* opts,
+                double* eig_vals, double* factors, double* eig_vecs, asend_modal_stats* stats);
+
 /* ---- multigrid diagnostics (no reference counterpart) --------------------
  * Host FP32 arrays.  Level 0 uses the fine DOF numbering, coarser levels the
  * lexicographic node order of their grid.  Used to verify on the device that
