@@ -225,6 +225,11 @@ struct ObjTerm {
 };
 struct Command {
     std::string cmd, file, method, node_set = "all", el_set = "all", position = "centroid";
+    // modalAnalysis (JobCommand defaults, S/job.rs:92-112)
+    std::string modal_type = "buckling";
+    long num_modes = 10;
+    double tgt_eval = 0.0;
+    bool write_modes = true;
     bool elastic = true, dynamic = false, nonlinear = false, thermal = false, write_grad = true;
     long block_dim = -1, max_it = 0, load_ramp = 1;
     double conv_tol = 1.0e-12;
@@ -337,6 +342,10 @@ struct Job {
             else if (k == "convergenceTol" && d.size() == 1) c.conv_tol = to_d(d[0]);
             else if (k == "nodeSet" && d.size() == 1) c.node_set = d[0];
             else if (k == "elementSet" && d.size() == 1) c.el_set = d[0];
+            else if (k == "type" && d.size() == 1) c.modal_type = d[0];
+            else if (k == "numModes" && d.size() == 1) c.num_modes = to_i(d[0]);
+            else if (k == "targetEigenvalue" && d.size() == 1) c.tgt_eval = to_d(d[0]);
+            else if (k == "writeModes" && d.size() == 1) c.write_modes = (d[0] == "yes");
             else if (k == "position" && d.size() == 1) c.position = d[0];
             else if (k == "fields" && d.size() == 1) c.fields.push_back(d[0]);
             else if (k == "include" && d.size() == 1) c.include.push_back(d[0]);
@@ -1842,6 +1851,102 @@ struct Job {
         std::fclose(fp);
     }
 
+    // Model::eigen_solve (S/model_analysis.rs:713-888) through asend_eigen:
+    // the tangent stiffness at the current displacements (zero before any
+    // solve), type frequency (lumped mass) or buckling (identity mass, load
+    // factors), numModes, targetEigenvalue as the shift; the reference's
+    // eigensolve runs for solverMethod direct only.
+    std::vector<double> eig_vals, eig_fac, eig_vecs;
+    Command modal_cmd;
+    void modal_analysis(const Command& c) {
+        const int nm = (int)c.num_modes;
+        if (nm < 1) die("numModes must be positive");
+        modal_cmd = c;
+        if (c.modal_type != "buckling" && c.modal_type != "frequency") {
+            // the reference solves only for these two types; any other leaves
+            // the eigenvalues at zero (and the frequencies at sqrt(0) / 2 pi)
+            std::printf("note: modal analysis type %s: no eigensolve\n", c.modal_type.c_str());
+            eig_vals.assign(nm, 0.0);
+            eig_fac.assign(nm, 0.0);
+            eig_vecs.assign((size_t)nm * std::max<int64_t>(n_dof, 1), 0.0);
+            return;
+        }
+        if (!c.method.empty() && c.method != "direct")
+            die("modal analysis runs with solverMethod direct (the reference's other branch computes nothing)");
+        if (!solved || !solve_cmd.elastic) {
+            Command pc = c;
+            pc.method = "direct";
+            pc.elastic = true;
+            pc.thermal = false;
+            solve_cmd = pc;
+            analysis_prep(pc);
+            u.assign(n_dof, 0.0);
+            v.assign(n_dof, 0.0);
+            a.assign(n_dof, 0.0);
+        }
+        check(asend_assemble(ctx, 0.0), "asend_assemble");
+        asend_ldl_info li;
+        std::memset(&li, 0, sizeof li);
+        check(asend_ldl_factor(ctx, 0, &li), "asend_ldl_factor");
+        if (etype == ASEND_BRICKIM && !nl_solved) {
+            // the elements hold the internal DOFs of the last linear solve
+            std::vector<double> al(9 * conn.size());
+            check(asend_brickim_internal(ctx, u.data(), al.data()), "asend_brickim_internal");
+            check(asend_brickim_set_internal(ctx, al.data()), "asend_brickim_set_internal");
+        }
+        asend_solve_opts o;
+        std::memset(&o, 0, sizeof o);
+        o.method = ASEND_SOLVE_DIRECT;
+        o.precond = ASEND_PC_LDL;
+        eig_vals.assign(nm, 0.0);
+        eig_fac.assign(nm, 0.0);
+        eig_vecs.assign((size_t)nm * n_dof, 0.0);
+        asend_modal_stats st;
+        std::memset(&st, 0, sizeof st);
+        check(asend_eigen(ctx, c.modal_type == "buckling" ? 1 : 0, u.data(), nm, c.tgt_eval, &o, eig_vals.data(),
+                          eig_fac.data(), eig_vecs.data(), &st),
+              "asend_eigen");
+        std::printf("finished eigensolve: %lld Lanczos steps, %lld modes\n", (long long)st.lanczos_steps,
+                    (long long)st.modes_found);
+    }
+
+    // write_modal_results (S/model_output.rs:495-551): mode, eigenvalue and
+    // load factor or frequency; with writeModes (the flag of the
+    // modalAnalysis command, as the reference reads it) one file per mode,
+    // <name up to its first '.'>_mode<i><rest>, Node,U1..R3 per node label
+    // (solid nodes carry no rotations: 0 here, where the reference would
+    // index its mode vector with the unset rotation DOF numbers)
+    void write_modal_results(const Command& c) {
+        if (eig_vals.empty()) die("modal results need a modal analysis (run modalAnalysis first)");
+        std::FILE* fp = std::fopen(c.file.c_str(), "w");
+        if (!fp) die("could not open file %s", c.file.c_str());
+        const bool buck = modal_cmd.modal_type == "buckling";
+        std::fprintf(fp, "mode,eigenvalue,%s\n", buck ? "load_factor" : "frequency");
+        for (size_t i = 0; i < eig_vals.size(); ++i)
+            std::fprintf(fp, "%zu,%s,%s\n", i, fmt_e12(eig_vals[i]).c_str(), fmt_e12(eig_fac[i]).c_str());
+        std::fclose(fp);
+        if (!modal_cmd.write_modes) return;
+        const size_t dot = c.file.find('.');
+        const std::string root = dot == std::string::npos ? c.file : c.file.substr(0, dot);
+        const std::string ext = dot == std::string::npos ? "" : c.file.substr(dot);
+        const size_t nn = xyz.size() / 3;
+        for (size_t i = 0; i < eig_vals.size() && n_dof > 0; ++i) {
+            const std::string name = root + "_mode" + std::to_string(i) + ext;
+            std::FILE* mp = std::fopen(name.c_str(), "w");
+            if (!mp) die("could not open file %s", name.c_str());
+            std::fprintf(mp, "Node,U1,U2,U3,R1,R2,R3\n");
+            for (size_t n = 0; n < nn; ++n) {
+                std::fprintf(mp, "%zu", n);
+                for (int k = 0; k < 6; ++k) {
+                    const int64_t d = k < dpn ? dof_index[6 * n + k] : -1;
+                    std::fprintf(mp, ",%s", fmt_e12(d >= 0 ? eig_vecs[i * (size_t)n_dof + d] : 0.0).c_str());
+                }
+                std::fprintf(mp, "\n");
+            }
+            std::fclose(mp);
+        }
+    }
+
     void write_objective(const Command& c) {
         std::FILE* fp = std::fopen(c.file.c_str(), "w");
         if (!fp) die("could not open file %s", c.file.c_str());
@@ -1914,6 +2019,12 @@ struct Job {
             } else if (c.cmd == "writeNodeResults") {
                 std::printf("writing Node results\n");
                 write_node_results(c);
+            } else if (c.cmd == "modalAnalysis") {
+                std::printf("running modal analysis\n");
+                modal_analysis(c);
+            } else if (c.cmd == "writeModalResults") {
+                std::printf("writing modal results\n");
+                write_modal_results(c);
             } else if (c.cmd == "writeElementResults") {
                 std::printf("writing Element results\n");
                 write_element_results(c);

@@ -179,3 +179,49 @@ def test_cli_hexbeam_axial_element_results(tmp_path):
     assert np.abs(el[mid, col["MISES"]] - 20.0).max() <= 5e-6 * 20.0
     assert (el[:, col["time"]] == -1.0).all()
     check_element_identities(head, el, 1e6)
+
+
+def read_modal(path):
+    import csv
+    with open(path) as fh:
+        rows = list(csv.reader(fh))
+    return rows[0], np.array([[float(x) for x in r] for r in rows[1:]])
+
+
+def test_cli_hexbeam_buckling_and_frequencies(tmp_path):
+    """hexBeam tests 7 and 8 through the job driver: modalAnalysis type
+    buckling after the static solve under half the critical end load (load
+    factor x 1.028088 near pi^2 E I / 4 L^2 = 2.056176; the reference's
+    Rayleigh-quotient factor is 2.1% above), type frequency on the unloaded
+    beam (first frequency 0.161658 Hz), and an unknown type, for which the
+    reference computes nothing."""
+    gpu_case("asend_run_job hexBeam buckling and natural frequencies")
+    loads = [{"type": "surfacePressure", "elementSet": "all", "load": str([20.56176 / 2]),
+              "normDir": str([1.0, 0.0, 0.0]), "normTolerance": 5.0}]
+    work = tmp_path / "buckling"
+    work.mkdir()
+    m = write_hexbeam(work, loads, {}, [{"command": "modalAnalysis", "type": "buckling", "numModes": 3},
+                                        {"command": "writeModalResults", "fileName": "results/buckling.csv"}])
+    run(work)
+    head, rows = read_modal(work / "results" / "buckling.csv")
+    assert head == ["mode", "eigenvalue", "load_factor"] and rows.shape == (3, 3)
+    assert rows[0, 2] * 20.56176 / 2 * 0.1 == pytest.approx(2.056176, rel=3e-2)
+    mode0 = np.loadtxt(work / "results" / "buckling_mode0.csv", delimiter=",", skiprows=1)
+    assert mode0.shape == (m.n_nodes, 7) and np.abs(mode0[:, 4:]).max() == 0.0
+    assert abs(np.linalg.norm(mode0[:, 1:4]) - 1.0) <= 1e-6  # clamped DOFs hold ~0 of the unit mode
+    for kind, first in (("frequency", 0.16165804304147693), ("freq", 0.0)):
+        work = tmp_path / kind
+        work.mkdir()
+        write_hexbeam(work, [], {}, [{"command": "modalAnalysis", "type": kind, "numModes": 2},
+                                     {"command": "writeModalResults", "fileName": "results/modes.csv",
+                                      "writeModes": "no"}])
+        run(work)
+        head, rows = read_modal(work / "results" / "modes.csv")
+        assert head[2] == "frequency"
+        if kind == "frequency":
+            assert rows[0, 2] == pytest.approx(first, rel=1e-2)
+        else:
+            assert (rows[:, 1:] == 0.0).all()
+        # writeModes on the writer command is not read: the reference's writer
+        # takes the flag of the modalAnalysis command (default yes)
+        assert (work / "results" / "modes_mode0.csv").exists()
