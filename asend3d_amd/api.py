@@ -148,7 +148,7 @@ EXPORTED = ["asend_ctx_create", "asend_ctx_destroy", "asend_last_error", "asend_
             "asend_reorder_nodes", "asend_block_dim_auto", "asend_surface_traction", "asend_lognormal_moduli",
             "asend_compliance_clamp_check", "asend_surface_pressure", "asend_body_load", "asend_surface_flux",
             "asend_set_conductivity", "asend_brickim_internal", "asend_brickim_set_internal",
-            "asend_eigen",
+            "asend_eigen", "asend_set_material_stiffness",
             "asend_upload_mesh", "asend_set_thickness", "asend_set_grid", "asend_update_moduli", "asend_set_penalty",
             "asend_penalty_scale", "asend_uniform_elements", "asend_assemble", "asend_get_matrix", "asend_spmv", "asend_solve", "asend_solve_noalloc",
             "asend_ldl_factor", "asend_ldl_solve", "asend_ldl_release", "asend_cycle_spmv_timing",
@@ -595,6 +595,18 @@ class Context:
         idx = None if el_mat is None else np.ascontiguousarray(el_mat, dtype=np.int32)
         self._chk(_lib.asend_set_material_damping(self.h, _i64(d.shape[0]), _p(d), _p(idx)),
                   "asend_set_material_damping")
+
+    def set_material_stiffness(self, cmat, el_mat=None):
+        """General material stiffness matrices (n_mat x 6 x 6, strain order
+        11, 22, 33, 12, 13, 23) and each element's matrix (None: matrix 0),
+        asend_set_material_stiffness (brick8); cmat None removes them."""
+        if cmat is None:
+            self._chk(_lib.asend_set_material_stiffness(self.h, _i64(0), None, None), "asend_set_material_stiffness")
+            return
+        c = _f64(np.asarray(cmat, dtype=np.float64).reshape(-1, 36))
+        idx = None if el_mat is None else np.ascontiguousarray(el_mat, dtype=np.int32)
+        self._chk(_lib.asend_set_material_stiffness(self.h, _i64(c.shape[0]), _p(c), _p(idx)),
+                  "asend_set_material_stiffness")
 
     def set_conductivity(self, kcond):
         """Heat conduction (asend_set_conductivity): n_el x 6 conductivities

@@ -185,6 +185,7 @@ struct Material {
     double density = 0.0;
     double E[3] = {0, 0, 0}, nu[3] = {0, 0, 0}, G[3] = {0, 0, 0};
     bool stiffness = false;
+    double stiff[36] = {};  // stiffness entries (Material::set_stiffness, symmetric)
     double damping[36] = {};  // Material::set_damping (symmetric, S/section_meth.rs:73-77)
     bool damped = false;
     double cond[6] = {0, 0, 0, 0, 0, 0};  // thermal conductivity k11 k22 k33 k12 k13 k23
@@ -267,6 +268,8 @@ struct Job {
     int64_t n_dof = 0;
     std::vector<int64_t> dof_index;  // 6 per node
     std::vector<double> E, nu, rho, thk;
+    std::vector<int> mat_of_el;   // each element's material
+    std::vector<double> cmats;    // 6x6 stiffness per material when any is anisotropic (else empty)
     std::vector<int64_t> pen;
     std::vector<double> u;    // solution, reference DOF numbering
     std::vector<double> v, a; // velocity and acceleration (dynamic solves; zero for static)
@@ -450,7 +453,10 @@ struct Job {
                         v[0] = to_d(d[0]);
                         v[1] = d.size() == 3 ? to_d(d[1]) : v[0];
                         v[2] = d.size() == 3 ? to_d(d[2]) : v[0];
-                    } else if (h[2] == "elastic" && h[3] == "stiffness") {
+                    } else if (h[2] == "elastic" && h[3] == "stiffness" && d.size() == 3) {
+                        const long r = to_i(d[0]) - 1, c = to_i(d[1]) - 1;
+                        if (r < 0 || r > 5 || c < 0 || c > 5) die("stiffness entry out of range");
+                        cur_mat->stiff[6 * r + c] = cur_mat->stiff[6 * c + r] = to_d(d[2]);
                         cur_mat->stiffness = true;
                     } else if (h[2] == "damping" && d.size() == 3) {
                         const long r = to_i(d[0]) - 1, c = to_i(d[1]) - 1;
@@ -666,6 +672,7 @@ struct Job {
         for (size_t s = 0; s < sections.size(); ++s)
             for (int32_t e : element_set(sections[s].el_set)) sect[e] = (int)s;
         std::vector<std::array<double, 9>> par(ne);
+        mat_of_el.assign(ne, 0);
         thk.assign(ne, 0.0);
         rho.assign(ne, 0.0);
         kc.assign(6 * ne, 0.0);
@@ -686,7 +693,7 @@ struct Job {
             } else {
                 m = &mat_of(s.mat);
             }
-            if (m->stiffness) die("materials given by a stiffness matrix are not supported");
+            mat_of_el[e] = (int)(m - materials.data());
             std::copy(m->cond, m->cond + 6, kc.begin() + 6 * e);
             for (int c = 0; c < 3; ++c) {
                 par[e][c] = m->E[c];
@@ -728,21 +735,86 @@ struct Job {
         }
         E.assign(ne, 0.0);
         nu.assign(ne, 0.0);
+        cmats.clear();
+        bool material_dvs = false;
+        for (const auto& dv : dvars)
+            material_dvs = material_dvs || dv.category == "modulus" || dv.category == "poissonRatio" ||
+                           dv.category == "shearModulus";
         for (size_t e = 0; e < ne; ++e) {
             const auto& p = par[e];
+            const Material& me = materials[mat_of_el[e]];
+            if (me.stiffness || !isotropic(p)) {
+                // general material (get_solid_stiff_dfd0): the stiffness entries, or
+                // the inverse of the compliance of the nine engineering constants
+                if (etype != ASEND_BRICK8) die("element %zu: anisotropic materials are supported for brick8 models", e);
+                if (material_dvs) die("material design variables on anisotropic materials are not supported");
+                if (cmats.empty()) cmats.assign(36 * materials.size(), 0.0);
+                double* C = cmats.data() + 36 * mat_of_el[e];
+                if (me.stiffness) std::copy(me.stiff, me.stiff + 36, C);
+                else compliance_inverse(p, C);
+                E[e] = 1.0;
+                nu[e] = 0.3;
+                continue;
+            }
             const double Ee = p[0], ne_ = p[3];
             const double G = Ee / (2.0 * (1.0 + ne_));
-            auto same = [](double a, double b) { return std::fabs(a - b) <= 1e-12 * std::fabs(b); };
-            const bool iso = same(p[1], Ee) && same(p[2], Ee) && same(p[4], ne_) && same(p[5], ne_) &&
-                             (etype == ASEND_SHELL4 ? same(p[6], G)
-                                                    : (same(p[6], G) && same(p[7], G) && same(p[8], G)));
-            if (!iso)
-                die("element %zu: the material after design variables is not isotropic (E = %g %g %g, nu = %g %g %g, "
-                    "G = %g %g %g); the device kernels integrate isotropic materials",
-                    e, p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7], p[8]);
+            (void)G;
             E[e] = Ee;
             nu[e] = ne_;
         }
+        if (!cmats.empty()) {
+            // an element of an isotropic material in a model with anisotropic
+            // ones takes its material matrix too
+            for (size_t e = 0; e < ne; ++e)
+                if (E[e] != 1.0 || nu[e] != 0.3 || cmats[36 * mat_of_el[e]] == 0.0) {
+                    double* C = cmats.data() + 36 * mat_of_el[e];
+                    compliance_inverse(par[e], C);
+                    E[e] = 1.0;
+                    nu[e] = 0.3;
+                }
+        }
+    }
+
+    bool isotropic(const std::array<double, 9>& p) const {
+        const double Ee = p[0], ne_ = p[3];
+        const double G = Ee / (2.0 * (1.0 + ne_));
+        auto same = [](double a, double b) { return std::fabs(a - b) <= 1e-12 * std::fabs(b); };
+        return same(p[1], Ee) && same(p[2], Ee) && same(p[4], ne_) && same(p[5], ne_) &&
+               (etype == ASEND_SHELL4 ? same(p[6], G) : (same(p[6], G) && same(p[7], G) && same(p[8], G)));
+    }
+
+    // C = S^-1, S the compliance of E1..3, nu12, nu13, nu23, G12, G13, G23
+    // (get_solid_stiff_dfd0, S/element_properties.rs:728-761), by Gauss-Jordan
+    // elimination with partial pivoting
+    static void compliance_inverse(const std::array<double, 9>& p, double* C) {
+        double S[6][12] = {};
+        S[0][0] = 1.0 / p[0];
+        S[0][1] = S[1][0] = -p[3] / p[0];
+        S[0][2] = S[2][0] = -p[4] / p[0];
+        S[1][1] = 1.0 / p[1];
+        S[1][2] = S[2][1] = -p[5] / p[1];
+        S[2][2] = 1.0 / p[2];
+        S[3][3] = 1.0 / p[6];
+        S[4][4] = 1.0 / p[7];
+        S[5][5] = 1.0 / p[8];
+        for (int i = 0; i < 6; ++i) S[i][6 + i] = 1.0;
+        for (int c = 0; c < 6; ++c) {
+            int piv = c;
+            for (int r = c + 1; r < 6; ++r)
+                if (std::fabs(S[r][c]) > std::fabs(S[piv][c])) piv = r;
+            if (!(std::fabs(S[piv][c]) > 0.0)) die("singular material compliance");
+            if (piv != c)
+                for (int k = 0; k < 12; ++k) std::swap(S[c][k], S[piv][k]);
+            const double ip = 1.0 / S[c][c];
+            for (int k = 0; k < 12; ++k) S[c][k] *= ip;
+            for (int r = 0; r < 6; ++r) {
+                if (r == c) continue;
+                const double f = S[r][c];
+                for (int k = 0; k < 12; ++k) S[r][k] -= f * S[c][k];
+            }
+        }
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j < 6; ++j) C[6 * i + j] = 0.5 * (S[i][6 + j] + S[j][6 + i]);
     }
 
     // coordinates after nodeCoord design variables: initial + sum of
@@ -821,6 +893,11 @@ struct Job {
                                 cflat.data(), E.data(), nu.data(), rho.data()),
               "asend_upload_mesh");
         if (etype == ASEND_SHELL4) check(asend_set_thickness(ctx, thk.data()), "asend_set_thickness");
+        if (!cmats.empty()) {
+            std::vector<int32_t> idx(mat_of_el.begin(), mat_of_el.end());
+            check(asend_set_material_stiffness(ctx, (int64_t)materials.size(), cmats.data(), idx.data()),
+                  "asend_set_material_stiffness");
+        }
         check(asend_set_penalty(ctx, (int64_t)pen.size(), pen.data(), 0.0), "asend_set_penalty");
         // applied loads (build_elastic_app_load): one vector per load, summed
         // over the loads active at a time by loads_at()
@@ -1818,10 +1895,18 @@ struct Job {
                     for (int k = 0; k < 3; ++k) v += H[k][i] * H[k][j];
                 eps[q] = q < 3 ? 0.5 * v : v;
             }
-            const double Ee = E[e], ne = nu[e];
-            const double lam = Ee * ne / ((1.0 + ne) * (1.0 - 2.0 * ne)), mu = Ee / (2.0 * (1.0 + ne));
-            const double tr = eps[0] + eps[1] + eps[2];
-            for (int q = 0; q < 6; ++q) sig[q] = q < 3 ? lam * tr + 2.0 * mu * eps[q] : mu * eps[q];
+            if (!cmats.empty()) {
+                const double* C = cmats.data() + 36 * mat_of_el[e];
+                for (int q = 0; q < 6; ++q) {
+                    sig[q] = 0.0;
+                    for (int r = 0; r < 6; ++r) sig[q] += E[e] * C[6 * q + r] * eps[r];
+                }
+            } else {
+                const double Ee = E[e], ne = nu[e];
+                const double lam = Ee * ne / ((1.0 + ne) * (1.0 - 2.0 * ne)), mu = Ee / (2.0 * (1.0 + ne));
+                const double tr = eps[0] + eps[1] + eps[2];
+                for (int q = 0; q < 6; ++q) sig[q] = q < 3 ? lam * tr + 2.0 * mu * eps[q] : mu * eps[q];
+            }
             std::fprintf(fp, "%d,0,0,", e);
             for (const auto& fld : c.fields) {
                 if (fld == "strain") {

@@ -143,6 +143,12 @@ struct asend_ctx {
     // while set, assemblies integrate the conduction operator (temperature at
     // the first DOF of each node, the same scalar operator on the other two)
     double* d_kcond = nullptr;
+    // general material stiffness (asend_set_material_stiffness): n_smat 6x6
+    // matrices and each element's matrix; while set, brick8 assembly uses
+    // E_e times them in place of the isotropic material
+    double* d_smat = nullptr;
+    int32_t* d_smat_idx = nullptr;
+    int64_t n_smat = 0;
     // Newmark step solver (asend_newmark_set_solver)
     int nm_mode = ASEND_NM_AUTO;
     double nm_inner_tol = 1e-3;  // reliable-update factor of the mixed step solver
@@ -321,6 +327,9 @@ void free_mesh(asend_ctx* ctx) {
     dfree(ctx->d_dmat_idx);
     ctx->n_dmat = 0;
     dfree(ctx->d_kcond);
+    dfree(ctx->d_smat);
+    dfree(ctx->d_smat_idx);
+    ctx->n_smat = 0;
     dfree(ctx->d_conn_r);
     dfree(ctx->d_rp);
     dfree(ctx->d_bc);
@@ -859,10 +868,11 @@ int solve_direct(asend_ctx* ctx, asend_solve_stats* st) {
 }
 
 int solve_device(asend_ctx* ctx, const asend_solve_opts* o, asend_solve_stats* st) {
-    if (ctx->d_kcond && o->precond != ASEND_PC_JACOBI && o->precond != ASEND_PC_BLOCK_JACOBI &&
-        o->precond != ASEND_PC_LDL)
+    if ((ctx->d_kcond || ctx->n_smat > 0) && o->precond != ASEND_PC_JACOBI &&
+        o->precond != ASEND_PC_BLOCK_JACOBI && o->precond != ASEND_PC_LDL)
         return fail(ctx, ASEND_ERR_UNSUPPORTED,
-                    "conduction solves take the Jacobi or LDL^T preconditioners (the multigrid hierarchy is elastic)");
+                    "conduction solves and general material matrices take the Jacobi or LDL^T preconditioners "
+                    "(the multigrid hierarchy integrates isotropic elasticity)");
     if (o->method == ASEND_SOLVE_PCG_FP64) return solve_fp64(ctx, ctx_fp64_bufs(ctx), o, st);
     if (o->method == ASEND_SOLVE_DIRECT) return solve_direct(ctx, st);
     const int64_t n = ctx->n_dof;
@@ -1006,7 +1016,7 @@ int integrate_elements(asend_ctx* ctx, const double* E, double k_scale, double c
         return fail(ctx, ASEND_ERR_UNSUPPORTED, "material damping is implemented for brick8 meshes");
     if (ctx->d_kcond && (c_mass != 0.0 || c_damp != 0.0 || k_scale != 1.0))
         return fail(ctx, ASEND_ERR_UNSUPPORTED, "the conduction operator is assembled alone (no heat capacity term)");
-    if (ctx->uniform_hex && c_damp == 0.0 && !ctx->d_kcond) {
+    if (ctx->uniform_hex && c_damp == 0.0 && !ctx->d_kcond && ctx->n_smat == 0) {
         launch_hex8_assemble_uniform(ctx->st, (int)ctx->n_nodes, (int)ctx->n_slices, ctx->d_sptr, ctx->d_scol,
                                      ctx->d_rp, ctx->d_n2e_ptr, ctx->d_n2e, ctx->d_conn_r, ctx->d_khat, ctx->d_mhat, E,
                                      ctx->d_rho, k_scale, c_mass, K);
@@ -1032,7 +1042,8 @@ int integrate_elements(asend_ctx* ctx, const double* E, double k_scale, double c
             if (rc) return rc;
         }
         launch_hex8_elem_rec(ctx->st, (int)ctx->n_el, ctx->d_conn_r, ctx->d_xyz_r, E, ctx->d_nu, ctx->d_rho, k_scale,
-                             c_mass, ctx->d_dmat, ctx->d_dmat_idx, c_damp, ctx->d_kcond, ctx->d_kb);
+                             c_mass, ctx->d_dmat, ctx->d_dmat_idx, c_damp, ctx->d_kcond,
+                             ctx->n_smat ? ctx->d_smat : nullptr, ctx->d_smat_idx, ctx->d_kb);
         launch_rec_gather(ctx->st, (int)ctx->n_nodes, (int)ctx->n_slices, ctx->d_sptr, ctx->d_n2e_ptr, ctx->d_n2rec,
                           ctx->d_slot, ctx->d_kb, nullptr, 0.0, nullptr, K, nullptr, nullptr);
     }
@@ -1216,6 +1227,16 @@ Fp64Bufs scratch_fp64_bufs(double* base) {
     B.partials = base + 7 * v;
     B.sc = (Scalars*)(base + 7 * v + kNoAllocPartials);
     return B;
+}
+
+// Entry points whose kernels integrate the isotropic material refuse a
+// context whose elements carry general material matrices.
+int iso_only(asend_ctx* ctx, const char* what) {
+    if (ctx->n_smat == 0) return ASEND_OK;
+    return fail(ctx, ASEND_ERR_UNSUPPORTED,
+                "%s integrates isotropic materials; this context holds general material matrices "
+                "(asend_set_material_stiffness)",
+                what);
 }
 
 // Entry points built on the elastic operator refuse a context that
@@ -1913,6 +1934,7 @@ int asend_solve_noalloc(asend_ctx* ctx, const double* rhs, double* x, const asen
 int asend_element_sens(asend_ctx* ctx, const double* u, const double* lam, double* dJdE) {
     if (!ctx || !u || !lam || !dJdE) return ASEND_ERR_ARG;
     if (const int rc0 = elastic_only(ctx, "asend_element_sens")) return rc0;
+    if (const int rc1 = iso_only(ctx, "asend_element_sens")) return rc1;
     if (!ctx->d_E) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
     if (ctx->etype == ASEND_SHELL4)
         return fail(ctx, ASEND_ERR_UNSUPPORTED, "element sensitivities are implemented for brick8 and brickIM meshes");
@@ -1929,6 +1951,7 @@ int asend_element_sens(asend_ctx* ctx, const double* u, const double* lam, doubl
 int asend_element_sens_material(asend_ctx* ctx, const double* u, const double* lam, double* dJdp) {
     if (!ctx || !u || !lam || !dJdp) return ASEND_ERR_ARG;
     if (const int rc0 = elastic_only(ctx, "asend_element_sens_material")) return rc0;
+    if (const int rc1 = iso_only(ctx, "asend_element_sens_material")) return rc1;
     if (!ctx->d_E) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
     if (ctx->etype != ASEND_BRICK8)
         return fail(ctx, ASEND_ERR_UNSUPPORTED, "material-parameter sensitivities are implemented for brick8 meshes");
@@ -1950,6 +1973,7 @@ int asend_element_sens_material(asend_ctx* ctx, const double* u, const double* l
 int asend_element_sens_coords(asend_ctx* ctx, const double* u, const double* lam, double* out) {
     if (!ctx || !u || !lam || !out) return ASEND_ERR_ARG;
     if (const int rc0 = elastic_only(ctx, "asend_element_sens_coords")) return rc0;
+    if (const int rc1 = iso_only(ctx, "asend_element_sens_coords")) return rc1;
     if (!ctx->d_E) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
     if (ctx->etype != ASEND_BRICK8)
         return fail(ctx, ASEND_ERR_UNSUPPORTED, "shape sensitivities are implemented for brick8 meshes");
@@ -1971,6 +1995,7 @@ int asend_element_sens_coords(asend_ctx* ctx, const double* u, const double* lam
 int asend_element_centre_fields(asend_ctx* ctx, const double* u, double* out) {
     if (!ctx || !u || !out) return ASEND_ERR_ARG;
     if (const int rc0 = elastic_only(ctx, "asend_element_centre_fields")) return rc0;
+    if (const int rc1 = iso_only(ctx, "asend_element_centre_fields")) return rc1;
     if (!ctx->d_E) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
     if (ctx->etype != ASEND_BRICK8)
         return fail(ctx, ASEND_ERR_UNSUPPORTED, "element centre fields are implemented for brick8 meshes");
@@ -1990,6 +2015,7 @@ int asend_element_centre_fields(asend_ctx* ctx, const double* u, double* out) {
 int asend_element_centre_adjoint(asend_ctx* ctx, const double* h, double* y) {
     if (!ctx || !h || !y) return ASEND_ERR_ARG;
     if (const int rc0 = elastic_only(ctx, "asend_element_centre_adjoint")) return rc0;
+    if (const int rc1 = iso_only(ctx, "asend_element_centre_adjoint")) return rc1;
     if (!ctx->d_E) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
     if (ctx->etype != ASEND_BRICK8)
         return fail(ctx, ASEND_ERR_UNSUPPORTED, "element centre fields are implemented for brick8 meshes");
@@ -2014,6 +2040,7 @@ int asend_static_compliance(asend_ctx* ctx, const double* f, double* u, double* 
                             const asend_solve_opts* opts, asend_solve_stats* stats) {
     if (!ctx || !f || !opts) return ASEND_ERR_ARG;
     if (const int rc0 = elastic_only(ctx, "asend_static_compliance")) return rc0;
+    if (const int rc1 = iso_only(ctx, "asend_static_compliance")) return rc1;
     if (!ctx->d_K64) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
     if (ctx->etype == ASEND_SHELL4)
         return fail(ctx, ASEND_ERR_UNSUPPORTED, "element sensitivities are implemented for brick8 and brickIM meshes");
@@ -2045,6 +2072,7 @@ int asend_static_compliance(asend_ctx* ctx, const double* f, double* u, double* 
 int asend_nl_assemble(asend_ctx* ctx, const double* u, double* r_int) {
     if (!ctx || !u) return ASEND_ERR_ARG;
     if (const int rc0 = elastic_only(ctx, "asend_nl_assemble")) return rc0;
+    if (const int rc1 = iso_only(ctx, "asend_nl_assemble")) return rc1;
     int rc;
     if ((rc = nl_prepare(ctx))) return rc;
     HIPCHK(hipMemcpy(ctx->d_nl_u, u, ctx->n_dof * 8, hipMemcpyHostToDevice));
@@ -2096,6 +2124,7 @@ int asend_static_nonlinear(asend_ctx* ctx, const double* f, double* u, const ase
                            const asend_solve_opts* opts, asend_nl_stats* stats) {
     if (!ctx || !f || !u || !opts) return ASEND_ERR_ARG;
     if (const int rc0 = elastic_only(ctx, "asend_static_nonlinear")) return rc0;
+    if (const int rc1 = iso_only(ctx, "asend_static_nonlinear")) return rc1;
     int rc;
     if ((rc = nl_prepare(ctx))) return rc;
     asend_nl_opts o;
@@ -2328,6 +2357,49 @@ int asend_set_material_damping(asend_ctx* ctx, int64_t n_mat, const double* dmat
         dfree(ctx->d_dmat_idx);
     }
     ctx->n_dmat = n_mat;
+    return ASEND_OK;
+}
+
+int asend_set_material_stiffness(asend_ctx* ctx, int64_t n_mat, const double* cmat, const int32_t* el_mat) {
+    if (!ctx || n_mat < 0 || (n_mat > 0 && !cmat)) return ASEND_ERR_ARG;
+    if (!ctx->d_K64) return fail(ctx, ASEND_ERR_STATE, "upload a mesh first");
+    ctx->assembled = false;
+    if (n_mat == 0) {
+        dfree(ctx->d_smat);
+        dfree(ctx->d_smat_idx);
+        ctx->n_smat = 0;
+        return ASEND_OK;
+    }
+    if (ctx->etype != ASEND_BRICK8)
+        return fail(ctx, ASEND_ERR_UNSUPPORTED, "general material matrices are implemented for brick8 meshes");
+    for (int64_t m = 0; m < n_mat; ++m) {
+        // symmetric to rounding (an inverted compliance is symmetric to a few ulps)
+        double amax = 0.0;
+        for (int i = 0; i < 36; ++i) amax = std::max(amax, std::fabs(cmat[36 * m + i]));
+        for (int i = 0; i < 6; ++i) {
+            if (!std::isfinite(cmat[36 * m + 7 * i]) || !(cmat[36 * m + 7 * i] > 0.0))
+                return fail(ctx, ASEND_ERR_ARG, "material matrix %lld: diagonal entry %d is not positive",
+                            (long long)m, i);
+            for (int j = 0; j < i; ++j)
+                if (!(std::fabs(cmat[36 * m + 6 * i + j] - cmat[36 * m + 6 * j + i]) <= 1e-12 * amax))
+                    return fail(ctx, ASEND_ERR_ARG, "material matrix %lld is not symmetric", (long long)m);
+        }
+    }
+    if (el_mat)
+        for (int64_t e = 0; e < ctx->n_el; ++e)
+            if (el_mat[e] < 0 || el_mat[e] >= n_mat)
+                return fail(ctx, ASEND_ERR_ARG, "element %lld: material matrix index %d out of range", (long long)e,
+                            el_mat[e]);
+    int rc;
+    if ((rc = dalloc(ctx, &ctx->d_smat, 36 * n_mat))) return rc;
+    HIPCHK(hipMemcpy(ctx->d_smat, cmat, 36 * n_mat * sizeof(double), hipMemcpyHostToDevice));
+    if (el_mat) {
+        if ((rc = dalloc(ctx, &ctx->d_smat_idx, ctx->n_el))) return rc;
+        HIPCHK(hipMemcpy(ctx->d_smat_idx, el_mat, ctx->n_el * sizeof(int32_t), hipMemcpyHostToDevice));
+    } else {
+        dfree(ctx->d_smat_idx);
+    }
+    ctx->n_smat = n_mat;
     return ASEND_OK;
 }
 
@@ -2654,6 +2726,7 @@ int asend_newmark_adjoint(asend_ctx* ctx, int64_t n_steps, const double* f_shape
                           asend_newmark_stats* fwd_stats, asend_newmark_stats* adj_stats) {
     if (!ctx || n_steps < 1 || !f_shape || !load_fac || !q || !w || !(tol > 0.0) || max_it_step < 1)
         return ASEND_ERR_ARG;
+    if (const int rc1 = iso_only(ctx, "asend_newmark_adjoint")) return rc1;
     auto& nm = ctx->nm;
     if (!nm.ready) return fail(ctx, ASEND_ERR_STATE, "call asend_newmark_setup first");
     if (ctx->etype == ASEND_SHELL4)
@@ -3052,6 +3125,7 @@ int asend_learned_num_params(void) { return kNumParams; }
 int asend_learned_init(asend_ctx* ctx, uint64_t seed) {
     if (!ctx) return ASEND_ERR_ARG;
     if (const int rc0 = elastic_only(ctx, "asend_learned_init")) return rc0;
+    if (const int rc1 = iso_only(ctx, "asend_learned_init")) return rc1;
     ctx->learned_aside = false;
     HIPCHK(hipSetDevice(ctx->device));
     invalidate_factor(ctx);
@@ -3303,6 +3377,7 @@ int asend_mg_coarse_solve(asend_ctx* ctx, const float* b, float* x) { return mg_
 int asend_autotune(asend_ctx* ctx, const double* rhs, int what, asend_tuning* out) {
     if (!ctx || !out) return ASEND_ERR_ARG;
     if (const int rc0 = elastic_only(ctx, "asend_autotune")) return rc0;
+    if (const int rc1 = iso_only(ctx, "asend_autotune")) return rc1;
     if (!ctx->assembled) return fail(ctx, ASEND_ERR_STATE, "assemble first");
     std::memset(out, 0, sizeof(*out));
     int rc;

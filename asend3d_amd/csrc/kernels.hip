@@ -72,7 +72,9 @@ __device__ __forceinline__ double group8_sum(double v) {
 // 8 a + b with a <= b: G_ab = sum_gp detJ g_a g_b^T, and for isotropic C the
 // block is K_ab = lam G_ab + mu G_ab^T + mu tr(G_ab) I, plus c_mass rho M_ab I
 // and, with c_damp != 0, c_damp times the same integral for a general 6x6
-// material damping matrix; K_ba = K_ab^T.  With kcond (6 conductivities per
+// material damping matrix; with Cs (a general 6x6 material stiffness per
+// element, orthotropic materials) k_scale E_e times that integral replaces
+// the isotropic part; K_ba = K_ab^T.  With kcond (6 conductivities per
 // element, k11 k22 k33 k12 k13 k23) the block is the heat conduction term
 // s_ab I with s_ab = sum over m, n of k_mn G_ab[m][n] = int g_a^T k g_b: the
 // scalar conduction matrix on each of the three DOFs of a node, of which the
@@ -85,6 +87,8 @@ __global__ __launch_bounds__(256) void k_hex8_elem_rec(int n_el, const int32_t* 
                                                        const double* __restrict__ Cd,
                                                        const int32_t* __restrict__ cd_idx, double c_damp,
                                                        const double* __restrict__ kcond,
+                                                       const double* __restrict__ Cs,
+                                                       const int32_t* __restrict__ cs_idx,
                                                        double* __restrict__ Kr) {
     __shared__ __align__(16) double s_g[4][8][8][4];  // (g_t, N_t) at each point
     __shared__ double s_det[4][8];
@@ -154,8 +158,9 @@ __global__ __launch_bounds__(256) void k_hex8_elem_rec(int n_el, const int32_t* 
             m += dq * ga.w * gb.w;
         }
         const double Ee = k_scale * E[ec], ne = nu[ec];
-        const double lam = Ee * ne / ((1.0 + ne) * (1.0 - 2.0 * ne));
-        const double mu = Ee / (2.0 * (1.0 + ne));
+        // a general material matrix (Cs) replaces the isotropic one, weighted by k_scale E_e
+        const double lam = Cs ? 0.0 : Ee * ne / ((1.0 + ne) * (1.0 - 2.0 * ne));
+        const double mu = Cs ? 0.0 : Ee / (2.0 * (1.0 + ne));
         const double dg = mu * (G[0] + G[4] + G[8]) + (c_mass != 0.0 ? c_mass * rho[ec] * m : 0.0);
         // c_damp B^T C_d B for a general symmetric 6x6 matrix in the strain
         // order (11, 22, 33, 12, 13, 23), engineering shear: block entry
@@ -163,8 +168,8 @@ __global__ __launch_bounds__(256) void k_hex8_elem_rec(int n_el, const int32_t* 
         double X[9];
 #pragma unroll
         for (int c = 0; c < 9; ++c) X[c] = 0.0;
-        if (c_damp != 0.0) {
-            const double* Dm = Cd + 36 * (cd_idx ? (int64_t)cd_idx[ec] : 0);
+        // w sum over m, n of Dm[V(i,m)][V(j,n)] G_mn into X (strain order 11, 22, 33, 12, 13, 23)
+        auto general_block = [&](const double* Dm, double w) {
             constexpr int V[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};
 #pragma unroll
             for (int i = 0; i < 3; ++i)
@@ -175,9 +180,11 @@ __global__ __launch_bounds__(256) void k_hex8_elem_rec(int n_el, const int32_t* 
                     for (int m2 = 0; m2 < 3; ++m2)
 #pragma unroll
                         for (int n2 = 0; n2 < 3; ++n2) acc += Dm[6 * V[i][m2] + V[j][n2]] * G[3 * m2 + n2];
-                    X[3 * i + j] = c_damp * acc;
+                    X[3 * i + j] += w * acc;
                 }
-        }
+        };
+        if (c_damp != 0.0) general_block(Cd + 36 * (cd_idx ? (int64_t)cd_idx[ec] : 0), c_damp);
+        if (Cs && Ee != 0.0) general_block(Cs + 36 * (cs_idx ? (int64_t)cs_idx[ec] : 0), Ee);
         double* o = &s_out[wv][kRecStride * a + 9 * b];
         double* ot = &s_out[wv][kRecStride * b + 9 * a];
         if (kcond) {
@@ -460,10 +467,11 @@ __global__ __launch_bounds__(256) void k_hex8_assemble_uniform(
 
 void launch_hex8_elem_rec(hipStream_t st, int n_el, const int32_t* conn_r, const double* xyz_r, const double* E,
                           const double* nu, const double* rho, double k_scale, double c_mass, const double* Cd,
-                          const int32_t* cd_idx, double c_damp, const double* kcond, double* Kr) {
+                          const int32_t* cd_idx, double c_damp, const double* kcond, const double* Cs,
+                          const int32_t* cs_idx, double* Kr) {
     if (n_el <= 0) return;
     hipLaunchKernelGGL(k_hex8_elem_rec, dim3((unsigned)((n_el + 3) / 4)), dim3(256), 0, st, n_el, conn_r, xyz_r, E,
-                       nu, rho, k_scale, c_mass, Cd, cd_idx, c_damp, kcond, Kr);
+                       nu, rho, k_scale, c_mass, Cd, cd_idx, c_damp, kcond, Cs, cs_idx, Kr);
 }
 
 void launch_rec_gather(hipStream_t st, int n_rows, int n_slices, const int32_t* sptr, const int32_t* n2e_ptr,

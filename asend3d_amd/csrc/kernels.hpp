@@ -50,7 +50,8 @@ __host__ __device__ inline int64_t sell_index(const int32_t* sptr, int r, int k,
 // rhs set, rhs = fact f - rint.
 void launch_hex8_elem_rec(hipStream_t st, int n_el, const int32_t* conn_r, const double* xyz_r, const double* E,
                           const double* nu, const double* rho, double k_scale, double c_mass, const double* Cd,
-                          const int32_t* cd_idx, double c_damp, const double* kcond, double* Kr);
+                          const int32_t* cd_idx, double c_damp, const double* kcond, const double* Cs,
+                          const int32_t* cs_idx, double* Kr);
 void launch_rec_gather(hipStream_t st, int n_rows, int n_slices, const int32_t* sptr, const int32_t* n2e_ptr,
                        const int32_t* n2rec, const uint8_t* slot, const double* Kr, const double* Re, double fact,
                        const double* f, double* K, double* rint, double* rhs);

@@ -259,6 +259,20 @@ int asend_uniform_elements(const asend_ctx* ctx);
  * copy and the Jacobi diagonal. */
 int asend_assemble(asend_ctx* ctx, double c_mass);
 
+/* Orthotropic and other anisotropic solid materials (get_solid_stiff_dfd0,
+ * S/element_properties.rs:648-764: the stiffness is the inverse of the
+ * compliance built from E1..3, nu12, nu13, nu23, G12, G13, G23, or the
+ * material's stiffness entries): n_mat 6 x 6 matrices, symmetric to 1e-12 (strain order
+ * 11, 22, 33, 12, 13, 23, engineering shear, in the global frame) and each
+ * element's matrix (el_mat; NULL: matrix 0).  While set, brick8 assembly
+ * integrates E_e times the element's matrix in place of the isotropic
+ * material (upload E = 1 to take the matrices as they are; the mass and the
+ * Rayleigh and material damping terms are unchanged).  Entry points whose
+ * kernels integrate the isotropic material (sensitivities, element centre
+ * fields, compliance pipeline, nonlinear, transient adjoint, multigrid,
+ * learned, autotuning) refuse such a context.  n_mat = 0 removes them. */
+int asend_set_material_stiffness(asend_ctx* ctx, int64_t n_mat, const double* cmat, const int32_t* el_mat);
+
 /* Heat conduction statics (the thermal branch of Model::solve_step,
  * S/model_analysis.rs:331-370; Element::get_rtk_dfd0,
  * S/element_equations.rs:1064-1123; conductivity of

@@ -287,3 +287,49 @@ def test_shape_sensitivities_match_finite_differences(ctx):
                 fd[a, c] = le @ (Kp - Km) @ ue / (2.0 * h)
         worst = max(worst, np.abs(s[e] - fd).max() / np.abs(fd).max())
     assert worst <= 1e-6, worst
+
+
+def test_orthotropic_materials_match_oracle(ctx):
+    """General material matrices (asend_set_material_stiffness): two
+    orthotropic materials from the reference's compliance inverse (the
+    oracle's solid_cmat_ortho) on a distorted mesh against the oracle's
+    element integration, scattered; the solve against a sparse direct solve;
+    the isotropic-only paths refuse."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    gpu_case("orthotropic materials het-8x4x3 distorted")
+    p = problems.distorted(problems.heterogeneous(8, 4, 3, seed=11), seed=12)
+    C = np.array([O.solid_cmat_ortho((70e3, 20e3, 10e3), (0.3, 0.25, 0.2), (5e3, 4e3, 3e3)),
+                  O.solid_cmat_ortho((30e3, 60e3, 45e3), (0.1, 0.35, 0.22), (9e3, 7e3, 11e3))])
+    el_mat = np.random.default_rng(5).integers(0, 2, p.mesh.n_el).astype(np.int32)
+    ones = np.ones(p.mesh.n_el)
+    ctx.upload_mesh(8, p.mesh.xyz, p.dof_index, p.n_dof, p.mesh.conn, ones, 0.3 * ones)
+    ctx.set_material_stiffness(C, el_mat)
+    ctx.set_penalty(p.pen_dofs, 0.0)
+    ctx.assemble()
+    di = np.asarray(p.dof_index).reshape(-1, 6)[:, :3]
+    rows, cols, vals = [], [], []
+    for e in range(p.mesh.n_el):
+        c = p.mesh.conn[e]
+        Ke, _ = O.solid_element(8, p.mesh.xyz[c], C[el_mat[e]])
+        d = di[c].reshape(-1)
+        rows.append(np.repeat(d, 24))
+        cols.append(np.tile(d, 24))
+        vals.append(Ke.reshape(-1))
+    K = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(p.n_dof, p.n_dof))
+    rp, cl, vl = ctx.get_matrix(with_penalty=False)
+    Kd = sp.csr_matrix((vl, cl, rp), shape=(p.n_dof, p.n_dof))
+    # each entry sums 81 products C_pq G_mn of both signs: 6e-12 of max|K| measured
+    assert abs(Kd - K).max() <= 2e-11 * abs(K).max()
+    sf = ctx.penalty_scale()
+    assert abs(sf - 1e5 * abs(K).max()) <= 1e-12 * sf
+    P = np.zeros(p.n_dof)
+    np.add.at(P, p.pen_dofs, sf)
+    u_ref = spla.spsolve((K + sp.diags(P)).tocsc(), p.f)
+    ctx.ldl_factor(0)
+    u, st = ctx.solve(p.f, ctx.opts(api.SOLVE_DIRECT, api.PC_JACOBI))
+    assert np.abs(u - u_ref).max() <= 1e-10 * np.abs(u_ref).max()
+    with pytest.raises(api.AsendError, match="isotropic"):
+        ctx.element_sens(u, u)
+    with pytest.raises(api.AsendError, match="Jacobi or LDL"):
+        ctx.solve(p.f, ctx.opts(api.SOLVE_PCG_FP64, api.PC_MG))
