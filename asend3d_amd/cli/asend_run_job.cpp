@@ -189,6 +189,7 @@ struct Material {
     double damping[36] = {};  // Material::set_damping (symmetric, S/section_meth.rs:73-77)
     bool damped = false;
     double cond[6] = {0, 0, 0, 0, 0, 0};  // thermal conductivity k11 k22 k33 k12 k13 k23
+    double spec_heat = 0.0;
     bool expands = false;                 // a nonzero thermal expansion coefficient
 };
 struct Term {
@@ -285,7 +286,8 @@ struct Job {
     // of the elements, the node temperatures, and the penalised temperature
     // DOFs with coefficient x rhs of their constraints
     asend_ctx* tctx = nullptr;
-    std::vector<double> kc, temp;
+    asend_ctx* cctx = nullptr;  // heat capacity alone (transient solves)
+    std::vector<double> kc, temp, tdot, t3dot;
     std::vector<int64_t> tpen, tpen3;  // temperature DOFs; with their two companions
     std::vector<double> tpen_rhs;
     std::vector<double> t3;  // temperatures in the conduction context's DOFs
@@ -294,6 +296,7 @@ struct Job {
     ~Job() {
         if (ctx) asend_ctx_destroy(ctx);
         if (tctx) asend_ctx_destroy(tctx);
+        if (cctx) asend_ctx_destroy(cctx);
     }
 
     const std::vector<int32_t>& node_set(const std::string& n, bool fallback_all) {
@@ -463,6 +466,8 @@ struct Job {
                         if (r < 0 || r > 5 || c < 0 || c > 5) die("damping entry out of range");
                         cur_mat->damping[6 * r + c] = cur_mat->damping[6 * c + r] = to_d(d[2]);
                         cur_mat->damped = true;
+                    } else if (h[2] == "thermal" && h[3] == "specHeat" && d.size() == 1) {
+                        cur_mat->spec_heat = to_d(d[0]);
                     } else if (h[2] == "thermal" && h[3] == "conductivity" && d.size() == 6) {
                         for (int i = 0; i < 6; ++i) cur_mat->cond[i] = to_d(d[i]);
                     } else if (h[2] == "thermal" && h[3] == "expansion" && d.size() == 6) {
@@ -957,13 +962,28 @@ struct Job {
         }
         if (!tctx) check_t(asend_ctx_create(0, &tctx), "asend_ctx_create");
         const std::vector<double> one(ne, 1.0), nu0(ne, 0.3);
+        // heat capacity per element: density x specific heat (get_rtm_dfd0 with actual_props)
+        std::vector<double> rc(ne, 0.0);
+        for (size_t e = 0; e < ne; ++e) rc[e] = rho[e] * materials[mat_of_el[e]].spec_heat;
         check_t(asend_upload_mesh(tctx, etype, (int64_t)nn, xyz_dv.data(), dof_index.data(), n_dof, (int64_t)ne,
-                                  cflat.data(), one.data(), nu0.data(), nullptr),
+                                  cflat.data(), one.data(), nu0.data(), rc.data()),
                 "asend_upload_mesh");
         check_t(asend_set_conductivity(tctx, kc.data()), "asend_set_conductivity");
         // the temperature's two companion DOFs are penalised at the same nodes
         check_t(asend_set_penalty(tctx, (int64_t)tpen3.size(), tpen3.data(), 0.0), "asend_set_penalty");
-        check_t(asend_assemble(tctx, 0.0), "asend_assemble");
+        // a transient job's first matrix, which fixes the penalty scale, holds
+        // the capacity term (build_thermal_soln_load(true) at analysis_prep)
+        const double c1 = solve_cmd.dynamic ? 1.0 / (solve_cmd.gamma * solve_cmd.time_step) : 0.0;
+        check_t(asend_assemble(tctx, c1), "asend_assemble");
+        if (solve_cmd.dynamic) {
+            if (!cctx) check_t(asend_ctx_create(0, &cctx), "asend_ctx_create");
+            const std::vector<double> zk(6 * ne, 0.0);
+            check_t(asend_upload_mesh(cctx, etype, (int64_t)nn, xyz_dv.data(), dof_index.data(), n_dof, (int64_t)ne,
+                                      cflat.data(), one.data(), nu0.data(), rc.data()),
+                    "asend_upload_mesh");
+            check_t(asend_set_conductivity(cctx, zk.data()), "asend_set_conductivity");
+            check_t(asend_assemble(cctx, 1.0), "asend_assemble");
+        }
         for (auto& l : tloads) {
             l.vec.assign(n_dof, 0.0);
             double* lf = l.vec.data();
@@ -986,7 +1006,53 @@ struct Job {
             if (rc != ASEND_OK) die("%s load failed (%d)", l.type.c_str(), rc);
         }
         t3.assign(n_dof, 0.0);
+        t3dot.assign(n_dof, 0.0);
         temp.assign(nn, 0.0);
+        tdot.assign(nn, 0.0);
+    }
+
+    // The thermal Newmark loop of Model::solve (S/model_analysis.rs:596-603,
+    // solve_step :331-370, Node::update_tdot / advance_temp,
+    // S/node_meth.rs:140-171): per step at time t, with the predicted rate
+    // p = -((1 - gamma) / gamma) Tdot, (K + C / (gamma dt) + sf P) dT =
+    // Q(t) - K T - C p + sf (c rhs - T), T += dT,
+    // Tdot = (dT / dt - (1 - gamma) Tdot) / gamma.  K_eff T comes from the
+    // conduction context, C x from the capacity context.
+    void solve_thermal_dynamic() {
+        const Command& c = solve_cmd;
+        if (!(c.time_step > 0.0)) die("dynamic solves need timeStep > 0");
+        const double g = c.gamma, dt = c.time_step, c1 = 1.0 / (g * dt);
+        const size_t nn = xyz.size() / 3;
+        if (c.save_hist) write_state(0, u, v, a);
+        std::vector<double> b(n_dof), ke(n_dof), cx(n_dof), x(n_dof), dT;
+        const double sf = asend_penalty_scale(tctx);
+        double t = 0.0;
+        long step = 1;
+        while (t < c.sim_period) {
+            std::fill(b.begin(), b.end(), 0.0);
+            for (const auto& l : tloads)
+                if (t >= l.active[0] && t <= l.active[1])
+                    for (int64_t i = 0; i < n_dof; ++i) b[i] += l.vec[i];
+            check_t(asend_spmv(tctx, 64, t3.data(), ke.data()), "asend_spmv");  // (K + c1 C + sf P) T
+            for (int64_t i = 0; i < n_dof; ++i) x[i] = c1 * t3[i] + ((1.0 - g) / g) * t3dot[i];
+            check_t(asend_spmv(cctx, 64, x.data(), cx.data()), "asend_spmv");
+            for (int64_t i = 0; i < n_dof; ++i) b[i] += cx[i] - ke[i];
+            for (size_t k = 0; k < tpen.size(); ++k) b[tpen[k]] += sf * tpen_rhs[k];
+            device_solve(tctx, tldl_ok, tbw_printed, 3 * solver_block_dim, 3, b, dT, false);
+            for (int64_t i = 0; i < n_dof; ++i) {
+                t3dot[i] = (dT[i] / dt - (1.0 - g) * t3dot[i]) / g;
+                t3[i] += dT[i];
+            }
+            for (size_t n = 0; n < nn; ++n) {
+                const int64_t d = dof_index[6 * n];
+                temp[n] = d >= 0 ? t3[d] : 0.0;
+                tdot[n] = d >= 0 ? t3dot[d] : 0.0;
+            }
+            thermal_solved = true;
+            if (c.save_hist) write_state(step++, u, v, a);
+            t += dt;
+        }
+        std::printf("device transient heat conduction: %ld steps\n", step - 1);
     }
 
     // The thermal branch of solve_step (S/model_analysis.rs:331-370): the
@@ -1017,6 +1083,13 @@ struct Job {
         check_t(asend_spmv(tctx, 64, t3.data(), r.data()), "asend_spmv");
         const double sf = asend_penalty_scale(tctx);
         for (int64_t d : tpen3) r[d] -= sf * t3[d];
+        if (solve_cmd.dynamic) {  // K T + C Tdot: remove c1 C T, add C Tdot
+            const double c1 = 1.0 / (solve_cmd.gamma * solve_cmd.time_step);
+            std::vector<double> x(n_dof), cx(n_dof);
+            for (int64_t i = 0; i < n_dof; ++i) x[i] = t3dot[i] - c1 * t3[i];
+            check_t(asend_spmv(cctx, 64, x.data(), cx.data()), "asend_spmv");
+            for (int64_t i = 0; i < n_dof; ++i) r[i] += cx[i];
+        }
         return r;
     }
 
@@ -1062,9 +1135,9 @@ struct Job {
         if (!fp) die("couldn't open file, %s", path.c_str());
         const size_t nn = xyz.size() / 3;
         for (size_t n = 0; n < nn; ++n) {
-            if (solve_cmd.thermal) {  // temperature and its rate (static: 0)
+            if (solve_cmd.thermal) {  // temperature and its rate
                 put_be(fp, temp[n]);
-                put_be(fp, 0.0);
+                put_be(fp, tdot[n]);
             }
             if (!solve_cmd.elastic) continue;
             for (const auto* vec : {&su, &sv, &sa})
@@ -1098,7 +1171,11 @@ struct Job {
                 double tn, rate;
                 if (!get_be(fp, &tn) || !get_be(fp, &rate)) die("problem reading file, %s", path.c_str());
                 temp[n] = tn;
-                if (dof_index[6 * n] >= 0) t3[dof_index[6 * n]] = tn;
+                tdot[n] = rate;
+                if (dof_index[6 * n] >= 0) {
+                    t3[dof_index[6 * n]] = tn;
+                    t3dot[dof_index[6 * n]] = rate;
+                }
             }
             if (!solve_cmd.elastic) continue;
             for (auto* vec : {&su, &sv, &sa})
@@ -1204,7 +1281,8 @@ struct Job {
         if (c.nonlinear && (c.dynamic || !c.elastic))
             die("geometric nonlinearity is supported for static elastic solves by this solver");
         if (!c.elastic && !c.thermal) die("the solve has neither the elastic nor the thermal field");
-        if (c.thermal && c.dynamic) die("transient heat conduction is not supported by this solver");
+        if (c.thermal && c.dynamic && c.elastic)
+            die("transient heat conduction beside a dynamic elastic solve is not supported by this solver");
         if (c.thermal && c.elastic)
             for (const auto& m : materials)
                 if (m.expands) die("thermal expansion (thermal-elastic coupling) is not supported by this solver");
@@ -1214,7 +1292,9 @@ struct Job {
         u.assign(n_dof, 0.0);
         v.assign(n_dof, 0.0);
         a.assign(n_dof, 0.0);
-        if (c.dynamic) {
+        if (c.dynamic && c.thermal) {
+            solve_thermal_dynamic();
+        } else if (c.dynamic) {
             solve_dynamic();
         } else if (c.nonlinear) {
             solve_nonlinear();
@@ -1736,6 +1816,7 @@ struct Job {
             if (fld == "reactionHeatGen" && rh.empty()) rh = reaction_heat();
         }
         if (temp.size() != xyz.size() / 3) temp.assign(xyz.size() / 3, 0.0);
+        if (tdot.size() != xyz.size() / 3) tdot.assign(xyz.size() / 3, 0.0);
         std::FILE* fp = std::fopen(name.c_str(), "w");
         if (!fp) die("could not open file %s", name.c_str());
         std::fprintf(fp, "node,");
@@ -1754,7 +1835,7 @@ struct Job {
             std::fprintf(fp, "%d,", n);
             for (const auto& fld : c.fields) {
                 if (fld == "temperature" || fld == "tdot") {
-                    std::fprintf(fp, "%s,", fmt_e12(fld == "temperature" ? temp[n] : 0.0).c_str());
+                    std::fprintf(fp, "%s,", fmt_e12(fld == "temperature" ? temp[n] : tdot[n]).c_str());
                     continue;
                 }
                 if (fld == "reactionHeatGen") {

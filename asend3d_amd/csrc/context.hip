@@ -1014,8 +1014,8 @@ int integrate_elements(asend_ctx* ctx, const double* E, double k_scale, double c
                        double c_damp = 0.0) {
     if (c_damp != 0.0 && (ctx->etype != ASEND_BRICK8 || ctx->n_dmat == 0))
         return fail(ctx, ASEND_ERR_UNSUPPORTED, "material damping is implemented for brick8 meshes");
-    if (ctx->d_kcond && (c_mass != 0.0 || c_damp != 0.0 || k_scale != 1.0))
-        return fail(ctx, ASEND_ERR_UNSUPPORTED, "the conduction operator is assembled alone (no heat capacity term)");
+    if (ctx->d_kcond && c_damp != 0.0)
+        return fail(ctx, ASEND_ERR_UNSUPPORTED, "the conduction operator has no damping term");
     if (ctx->uniform_hex && c_damp == 0.0 && !ctx->d_kcond && ctx->n_smat == 0) {
         launch_hex8_assemble_uniform(ctx->st, (int)ctx->n_nodes, (int)ctx->n_slices, ctx->d_sptr, ctx->d_scol,
                                      ctx->d_rp, ctx->d_n2e_ptr, ctx->d_n2e, ctx->d_conn_r, ctx->d_khat, ctx->d_mhat, E,

@@ -189,8 +189,10 @@ __global__ __launch_bounds__(256) void k_hex8_elem_rec(int n_el, const int32_t* 
         double* ot = &s_out[wv][kRecStride * b + 9 * a];
         if (kcond) {
             const double* kc = kcond + 6 * (int64_t)ec;
-            const double s = kc[0] * G[0] + kc[1] * G[4] + kc[2] * G[8] + kc[3] * (G[1] + G[3]) +
-                             kc[4] * (G[2] + G[6]) + kc[5] * (G[5] + G[7]);
+            // conduction (k_scale) plus heat capacity c_mass rho_e int N_a N_b, rho_e = density x specific heat
+            const double s = k_scale * (kc[0] * G[0] + kc[1] * G[4] + kc[2] * G[8] + kc[3] * (G[1] + G[3]) +
+                                        kc[4] * (G[2] + G[6]) + kc[5] * (G[5] + G[7])) +
+                             (c_mass != 0.0 ? c_mass * rho[ec] * m : 0.0);
 #pragma unroll
             for (int c = 0; c < 9; ++c) {
                 o[c] = (c % 4 == 0) ? s : 0.0;

@@ -278,8 +278,12 @@ int asend_set_material_stiffness(asend_ctx* ctx, int64_t n_mat, const double* cm
  * S/element_equations.rs:1064-1123; conductivity of
  * get_conductivity_dfd0, S/element_properties.rs:1715-1760): kcond holds 6
  * conductivities per element (k11 k22 k33 k12 k13 k23, the material key
- * thermal: conductivity), after which asend_assemble (c_mass = 0) integrates
- * sum over the Gauss points of w detJ G^T k G per element.  The temperature
+ * thermal: conductivity), after which asend_assemble(c_mass) integrates
+ * sum over the Gauss points of w detJ (G^T k G + c_mass rho_e N N^T) per
+ * element: conduction plus c_mass times the heat capacity, rho_e being the
+ * uploaded density times the specific heat (get_rtm_dfd0,
+ * S/element_equations.rs:1125-1204; c_mass = 1/(gamma dt) gives the
+ * operator of a transient step, get_rt_dfd0 :1237-1250).  The temperature
  * of a node is its first DOF (dof_index[6 n]); the same scalar operator sits
  * on its other two DOFs, which carry zero loads and, penalised at the
  * constrained nodes like the first, solve to zero: a PCG over the three

@@ -26,13 +26,19 @@ solverSrc/asend_run_job/src/.
                        field kept across faces; the element adds the vector of
                        its last passing face.
   nodal loads          nodalHeatGen (S/model_analysis.rs:98-104).
+  capacity_element     get_rtm_dfd0 with actual_props (rho c_p int N N^T).
+  transient            the thermal Newmark loop (first-order, gamma).
 
 Parity pins (tests/test_thermal_oracle.py): the unit-cube conduction matrix in
 closed form (1/3, 0, -1/12, -1/12), the patch test of a linear field on a mesh
 with moved interior nodes and an anisotropic conductivity (interior rows
 vanish, boundary rows equal the face-flux integral), 1D closed-form
 temperature profiles, box-volume and face-area sums, and the tensor-product
-mass integrals that fix the flux distribution.
+mass integrals that fix the flux distribution; the box capacity matrix from
+the 1D mass integrals; uniform heating's exact discrete rise from a zero
+starting rate; first-order convergence of the transient (its zero starting
+rate) to the exact solution of the semi-discrete system (matrix
+exponential).
 """
 import numpy as np
 
@@ -178,3 +184,55 @@ def to_dofs(f_node, dof_index, n_dof):
     out = np.zeros(n_dof)
     out[np.asarray(dof_index).reshape(-1, 6)[:, 0]] = f_node
     return out
+
+
+def capacity_element(X, rc):
+    """rc int N N^T (get_rtm_dfd0 with actual_props: rc = density x specific heat)."""
+    Ce = np.zeros((8, 8))
+    for s in _GP:
+        N, dN = _shape(s)
+        Ce += rc * np.linalg.det(X.T @ dN) * np.outer(N, N)
+    return Ce
+
+
+def assemble_capacity(xyz, conn, rc):
+    import scipy.sparse as sp
+    n = xyz.shape[0]
+    rows, cols, vals = [], [], []
+    for e in range(conn.shape[0]):
+        c = conn[e]
+        rows.append(np.repeat(c, 8))
+        cols.append(np.tile(c, 8))
+        vals.append(capacity_element(xyz[c], rc[e]).reshape(-1))
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+
+
+def transient(K, C, pen, sf, t_pen, load_at, dt, gamma, sim_period):
+    """The thermal Newmark loop of Model::solve (S/model_analysis.rs:596-603,
+    solve_step :331-370, Node::update_tdot / advance_temp
+    S/node_meth.rs:140-171), from T = 0, Tdot = 0: per step at time t the
+    load of t minus K T minus C times the predicted rate
+    -((1 - gamma) / gamma) Tdot plus the penalty load sf (T_c - T), solved
+    with K + C / (gamma dt) + sf P for the increment; then
+    Tdot = ((T_new - T) / dt - (1 - gamma) Tdot) / gamma.  Returns the
+    temperatures and rates after each step."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    n = K.shape[0]
+    P = sp.csr_matrix((np.full(len(pen), sf), (pen, pen)), shape=K.shape)
+    A = spla.splu((K + C / (gamma * dt) + P).tocsc())
+    T = np.zeros(n)
+    Td = np.zeros(n)
+    hist = []
+    t = 0.0
+    while t < sim_period:
+        pred = -((1.0 - gamma) / gamma) * Td
+        b = load_at(t) - K @ T - C @ pred - P @ T
+        b[pen] += sf * t_pen
+        dT = A.solve(b)
+        Tn = T + dT
+        Td = ((Tn - T) / dt - (1.0 - gamma) * Td) / gamma
+        T = Tn
+        hist.append((T.copy(), Td.copy()))
+        t += dt
+    return hist

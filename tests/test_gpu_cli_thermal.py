@@ -207,3 +207,53 @@ def test_cli_thermal_job_iterative(tmp_path):
     _, last = read_nodes(work / "results" / "nodes.csv")
     T_ref = (x0 + x1)[rank]
     assert np.abs(last[:, 1] - T_ref).max() <= 1e-7 * np.abs(T_ref).max()
+
+
+def test_cli_transient_thermal_job(tmp_path):
+    """thermal: yes, elastic: no, dynamic: yes (the thermal Newmark loop,
+    gamma 0.6): every state file and the final temperatures, rates and
+    reactionHeatGen (K T + C Tdot) against the oracle's restatement of the
+    loop, with the penalty scale of the first matrix K + C / (gamma dt)."""
+    gpu_case("asend_run_job transient heat conduction")
+    m, xyz = mesh()
+    work = tmp_path / "transient"
+    work.mkdir()
+    y_max_els, mid = write_job(work, m, xyz, "direct")
+    model = (work / "model.yaml").read_text().replace("density: 1.0", "density: 2.0")
+    model = model.replace("thermal:\n      conductivity", "thermal:\n      specHeat: 900.0\n      conductivity")
+    assert "specHeat: 900.0" in model
+    (work / "model.yaml").write_text(model)
+    import yaml
+    job = yaml.safe_load((work / "job.yaml").read_text())
+    cmds = job["jobCommands"]
+    solve = [c for c in cmds if c["command"] == "solve"][0]
+    solve.pop("staticLoadTime")
+    # (safe_load reads yes/no as booleans: write the job's words back)
+    solve.update({"thermal": "yes", "elastic": "no", "dynamic": "yes", "timeStep": 0.05, "newmarkGamma": 0.6,
+                  "simPeriod": 0.5})
+    for c in cmds:
+        if c["command"] == "writeNodeResults":
+            c["fields"] = ["temperature", "tdot", "reactionHeatGen"]
+    dump(work / "job.yaml", job)
+    out = run(work)
+    # time accumulates 0.05 ten times to 0.49999999999999994 < simPeriod: an
+    # eleventh step, in the reference's loop as here
+    assert "transient heat conduction: 11 steps" in out
+    K, _, f0, f1, pen, cr, rank = oracle_problem(m, xyz, y_max_els, mid, 2000000000)
+    C = TH.assemble_capacity(xyz, m.conn, np.full(m.n_el, 1800.0))
+    dt, g = 0.05, 0.6
+    sf = 1e5 * np.abs((K + C / (g * dt)).data).max()
+    load_at = lambda t: f0 if t <= 0.5 else f1  # nodalHeatGen active over [0, 0.5]
+    hist = TH.transient(K, C, pen, sf, cr, load_at, dt, g, 0.5)
+    assert len(hist) == 11
+    for step in (1, 4, 11):
+        raw = np.fromfile(work / "results" / f"solnTStep{step}.out", dtype=">f8").reshape(m.n_nodes, 2)
+        T_ref, Td_ref = hist[step - 1]
+        assert np.abs(raw[:, 0] - T_ref).max() <= 1e-9 * np.abs(T_ref).max(), step
+        assert np.abs(raw[:, 1] - Td_ref).max() <= 1e-8 * np.abs(Td_ref).max(), step
+    head, last = read_nodes(work / "results" / "nodes.csv")
+    assert head[:4] == ["node", "T", "TDOT", "RHG"]
+    T_ref, Td_ref = hist[-1]
+    assert np.abs(last[:, 1] - T_ref).max() <= 1e-9 * np.abs(T_ref).max()
+    want = K @ T_ref + C @ Td_ref
+    assert np.abs(last[:, 3] - want).max() <= 1e-7 * np.abs(want).max()

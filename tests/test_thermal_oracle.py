@@ -218,3 +218,60 @@ def test_bar_profiles_closed_form():
     T = _penalised_solve(K, x1, np.zeros(len(x1)), f)
     exact = q * (L - x) / k11
     assert np.abs(T - exact).max() <= 1e-4 * np.abs(exact).max()
+
+
+def test_capacity_box_closed_form():
+    h = (0.7, 0.3, 1.9)
+    Ce = TH.capacity_element(_box(h), 2.5)
+    M1 = [hd * np.array([[1 / 3, 1 / 6], [1 / 6, 1 / 3]]) for hd in h]
+    for a in range(8):
+        for b in range(8):
+            ia, ib = _bits(a), _bits(b)
+            want = 2.5 * np.prod([M1[d][ia[d], ib[d]] for d in range(3)])
+            assert abs(Ce[a, b] - want) <= 1e-14 * 2.5 * np.prod(h)
+
+
+def test_transient_uniform_heating_is_linear():
+    """No constraint, uniform generation Q: K T = 0 for a uniform T, so
+    rho c_p dT/dt = Q.  The reference starts from a zero rate, so the first
+    step gains gamma dt Q / (rho c_p) and every later one dt Q / (rho c_p):
+    T_n = (Q / rho c_p) dt (n - (1 - gamma)), rate Q / (rho c_p), at every
+    node."""
+    m = hex_block(3, 2, 2, 1.0, 0.5, 0.5)
+    K = TH.assemble_conduction(m.xyz, m.conn, np.tile((2.0, 1.0, 1.5, 0.2, 0.0, 0.1), (m.n_el, 1)))
+    C = TH.assemble_capacity(m.xyz, m.conn, np.full(m.n_el, 4.0))
+    Q = 3.0
+    f = TH.body_heat_gen(m.xyz, m.conn, np.arange(m.n_el), Q)
+    for gamma in (0.5, 0.8):
+        hist = TH.transient(K, C, np.zeros(0, dtype=int), 0.0, np.zeros(0), lambda t: f, 0.1, gamma, 0.5)
+        for k, (T, Td) in enumerate(hist):
+            assert np.abs(T - Q / 4.0 * 0.1 * (k + 1 - (1.0 - gamma))).max() <= 1e-12
+            assert np.abs(Td - Q / 4.0).max() <= 1e-12
+
+
+def test_transient_converges_to_semidiscrete_exact():
+    """gamma = 1/2: the error against the exp(-A t) solution of
+    C T' + (K + P) T = b halves with dt.  First order: the zero starting
+    rate is inconsistent with C T'(0) = b, which costs dt/2 of rise in the
+    first step (the trapezoidal rule alone would give 4)."""
+    import scipy.linalg as sl
+    import scipy.sparse as sp
+    m = hex_block(4, 1, 1, 1.0, 0.25, 0.25)
+    K = TH.assemble_conduction(m.xyz, m.conn, np.tile((1.0, 1.0, 1.0, 0.0, 0.0, 0.0), (m.n_el, 1)))
+    C = TH.assemble_capacity(m.xyz, m.conn, np.ones(m.n_el))
+    pen = m.nodes_at_x(0)
+    sf = 1e3 * np.abs(K.data).max()  # a soft penalty keeps the exact solution smooth
+    f = TH.body_heat_gen(m.xyz, m.conn, np.arange(m.n_el), 2.0)
+    Kp = (K + sp.csr_matrix((np.full(len(pen), sf), (pen, pen)), shape=K.shape)).toarray()
+    Cd = C.toarray()
+    A = np.linalg.solve(Cd, Kp)
+    Tinf = np.linalg.solve(Kp, f)
+    t_end = 0.4
+    exact = Tinf - sl.expm(-A * t_end) @ Tinf
+    errs = []
+    for dt in (0.02, 0.01):
+        hist = TH.transient(K, C, pen, sf, np.zeros(len(pen)), lambda t: f, dt, 0.5, t_end - 1e-9)
+        errs.append(np.abs(hist[-1][0] - exact).max())
+    assert 1.8 <= errs[0] / errs[1] <= 2.3, errs
+    # and the error is the first step's lost rise carried forward: below dt b / C
+    assert errs[1] <= 0.01 * np.abs(exact).max()
