@@ -148,7 +148,7 @@ EXPORTED = ["asend_ctx_create", "asend_ctx_destroy", "asend_last_error", "asend_
             "asend_reorder_nodes", "asend_block_dim_auto", "asend_surface_traction", "asend_lognormal_moduli",
             "asend_compliance_clamp_check", "asend_surface_pressure", "asend_body_load", "asend_surface_flux",
             "asend_set_conductivity", "asend_brickim_internal", "asend_brickim_set_internal",
-            "asend_eigen", "asend_set_material_stiffness",
+            "asend_eigen", "asend_set_material_stiffness", "asend_thermal_strain_load",
             "asend_upload_mesh", "asend_set_thickness", "asend_set_grid", "asend_update_moduli", "asend_set_penalty",
             "asend_penalty_scale", "asend_uniform_elements", "asend_assemble", "asend_get_matrix", "asend_spmv", "asend_solve", "asend_solve_noalloc",
             "asend_ldl_factor", "asend_ldl_solve", "asend_ldl_release", "asend_cycle_spmv_timing",
@@ -269,6 +269,24 @@ def surface_flux(etype, xyz, conn, dof_index, n_dof, el_set, q, norm_dir, norm_t
                                  _i64(n_dof), _p(f))
     if rc != 0:
         raise AsendError(f"asend_surface_flux failed ({rc})")
+    return f
+
+
+def thermal_strain_load(etype, xyz, conn, dof_index, n_dof, E, nu, alpha, temp, cmat=None, cmat_idx=None, f=None):
+    """asend_thermal_strain_load: the expansion load of node temperatures
+    temp with 6 expansion coefficients per element (alpha, n_el x 6)."""
+    xyz = _f64(xyz)
+    conn = _i32(conn)
+    di = _i64a(dof_index)
+    E, nu, al, tp = _f64(E), _f64(nu), _f64(np.asarray(alpha).reshape(-1, 6)), _f64(temp)
+    cm = None if cmat is None else _f64(np.asarray(cmat).reshape(-1, 36))
+    ci = None if cmat_idx is None else _i32(cmat_idx)
+    f = np.zeros(n_dof) if f is None else f
+    rc = _lib.asend_thermal_strain_load(ctypes.c_int(etype), _i64(xyz.shape[0]), _p(xyz), _i64(conn.shape[0]),
+                                        _p(conn), _p(di), _p(E), _p(nu), _p(cm), _p(ci), _p(al), _p(tp),
+                                        _i64(n_dof), _p(f))
+    if rc != 0:
+        raise AsendError(f"asend_thermal_strain_load failed ({rc})")
     return f
 
 

@@ -191,6 +191,7 @@ struct Material {
     double cond[6] = {0, 0, 0, 0, 0, 0};  // thermal conductivity k11 k22 k33 k12 k13 k23
     double spec_heat = 0.0;
     bool expands = false;                 // a nonzero thermal expansion coefficient
+    double expansion[6] = {0, 0, 0, 0, 0, 0};
 };
 struct Term {
     std::string set;
@@ -471,7 +472,10 @@ struct Job {
                     } else if (h[2] == "thermal" && h[3] == "conductivity" && d.size() == 6) {
                         for (int i = 0; i < 6; ++i) cur_mat->cond[i] = to_d(d[i]);
                     } else if (h[2] == "thermal" && h[3] == "expansion" && d.size() == 6) {
-                        for (int i = 0; i < 6; ++i) cur_mat->expands = cur_mat->expands || to_d(d[i]) != 0.0;
+                        for (int i = 0; i < 6; ++i) {
+                            cur_mat->expansion[i] = to_d(d[i]);
+                            cur_mat->expands = cur_mat->expands || cur_mat->expansion[i] != 0.0;
+                        }
                     }
                 }
             }
@@ -1075,6 +1079,31 @@ struct Job {
         thermal_solved = true;
     }
 
+    // the expansion load of the current temperatures (get_ruk_dfd0's thermal
+    // strain, S/element_equations.rs:272-298) when a thermal solve ran and a
+    // material expands; zero otherwise
+    std::vector<double> fth;
+    void expansion_load() {
+        fth.assign(n_dof, 0.0);
+        if (!solve_cmd.thermal || !thermal_solved) return;
+        bool any = false;
+        for (const auto& m : materials) any = any || m.expands;
+        if (!any) return;
+        const size_t ne = conn.size(), nn = xyz.size() / 3;
+        std::vector<double> al(6 * ne);
+        for (size_t e = 0; e < ne; ++e)
+            std::copy(materials[mat_of_el[e]].expansion, materials[mat_of_el[e]].expansion + 6, al.begin() + 6 * e);
+        std::vector<int32_t> cflat(8 * ne), cidx(mat_of_el.begin(), mat_of_el.end());
+        for (size_t e = 0; e < ne; ++e)
+            for (int k = 0; k < 8; ++k) cflat[8 * e + k] = conn[e][k];
+        const int rc = asend_thermal_strain_load(etype, (int64_t)nn, xyz_dv.data(), (int64_t)ne, cflat.data(),
+                                                 dof_index.data(), E.data(), nu.data(),
+                                                 cmats.empty() ? nullptr : cmats.data(),
+                                                 cmats.empty() ? nullptr : cidx.data(), al.data(), temp.data(), n_dof,
+                                                 fth.data());
+        if (rc != ASEND_OK) die("thermal expansion load failed (%d)", rc);
+    }
+
     // reactionHeatGen: the element flux of the temperatures without the
     // penalty (build_thermal_soln_load(false) negated, S/model_output.rs:201-208)
     std::vector<double> reaction_heat() {
@@ -1283,9 +1312,10 @@ struct Job {
         if (!c.elastic && !c.thermal) die("the solve has neither the elastic nor the thermal field");
         if (c.thermal && c.dynamic && c.elastic)
             die("transient heat conduction beside a dynamic elastic solve is not supported by this solver");
-        if (c.thermal && c.elastic)
+        if (c.thermal && c.elastic && (c.nonlinear || c.dynamic))
             for (const auto& m : materials)
-                if (m.expands) die("thermal expansion (thermal-elastic coupling) is not supported by this solver");
+                if (m.expands)
+                    die("thermal expansion is coupled for linear static solves by this solver (not nonlinear or dynamic)");
         solve_cmd = c;
         if (solve_cmd.load_times.empty()) solve_cmd.load_times.push_back(0.0);
         analysis_prep(solve_cmd);
@@ -1305,8 +1335,9 @@ struct Job {
                 if (c.thermal) thermal_step(solve_cmd.load_times[i]);
                 if (c.elastic) {
                     loads_at(solve_cmd.load_times[i], f);
+                    expansion_load();
                     check(asend_spmv(ctx, 64, u.data(), ku.data()), "asend_spmv");  // K u + sf P u
-                    for (int64_t k = 0; k < n_dof; ++k) b[k] = f[k] - ku[k];
+                    for (int64_t k = 0; k < n_dof; ++k) b[k] = f[k] + fth[k] - ku[k];
                     device_solve(b, du, true);
                     for (int64_t k = 0; k < n_dof; ++k) u[k] += du[k];
                 }
@@ -1447,6 +1478,8 @@ struct Job {
         if (!dyn_solved) {
             check(asend_spmv(ctx, 64, u.data(), rf.data()), "asend_spmv");
             for (int64_t d : pen) rf[d] -= sf * u[d];
+            if (fth.size() == (size_t)n_dof)  // the element residual B^T C (strain - alpha T)
+                for (int64_t i = 0; i < n_dof; ++i) rf[i] -= fth[i];
             return rf;
         }
         for (const auto& m : materials)
@@ -1562,6 +1595,9 @@ struct Job {
         if (dyn_solved) die("objectives of dynamic solves are not supported by this solver");
         if (nl_solved) die("objectives of nonlinear solves are not supported by this solver");
         if (!solve_cmd.elastic) die("objectives of thermal solves are not supported by this solver");
+        for (const auto& m : materials)
+            if (solve_cmd.thermal && m.expands)
+                die("objectives of solves with thermal expansion are not supported by this solver");
         cfields_ok = false;
         for (auto& t : obj) {
             const TermData d = term_data(t);
@@ -1870,6 +1906,9 @@ struct Job {
         if (!solved) die("element results need a solution (run solve first)");
         if (etype == ASEND_SHELL4) die("element results of shell elements are not supported by this solver");
         if (c.position != "centroid") die("element results at position %s are not supported", c.position.c_str());
+        for (const auto& m : materials)
+            if (solve_cmd.thermal && m.expands)
+                die("element results of solves with thermal expansion are not supported by this solver");
         for (const auto& fld : c.fields)
             if (fld != "stress" && fld != "strain" && fld != "strainEnergyDen")
                 die("element result field %s is not supported by this solver", fld.c_str());

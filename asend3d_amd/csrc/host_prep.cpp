@@ -398,6 +398,88 @@ int surface_flux(int elem_type, int64_t n_nodes, const double* xyz, int64_t n_el
     return ASEND_OK;
 }
 
+// Thermal strain load of hex elements (the expansion term of
+// get_ruk_dfd0, S/element_equations.rs:272-298: the strain at each Gauss
+// point loses alpha times the interpolated nodal temperature, no reference
+// temperature): f_a = sum_gp w detJ S g_a with S the stress tensor of
+// C (alpha T_gp), C isotropic from (E_e, nu_e) or, when cmat is given, the
+// element's 6 x 6 matrix times E_e; alpha: 6 per element (strain order 11,
+// 22, 33, 12, 13, 23, engineering shear); temp: per node label.
+int thermal_strain_load(int elem_type, int64_t n_nodes, const double* xyz, int64_t n_el, const int32_t* conn,
+                        const int64_t* dof_index, const double* E, const double* nu, const double* cmat,
+                        const int32_t* cmat_idx, const double* alpha, const double* temp, int64_t n_dof, double* f) {
+    if (elem_type != ASEND_BRICK8) return ASEND_ERR_UNSUPPORTED;
+    for (int64_t e = 0; e < n_el; ++e) {
+        const int32_t* c = conn + 8 * e;
+        const double* al = alpha + 6 * e;
+        if (al[0] == 0.0 && al[1] == 0.0 && al[2] == 0.0 && al[3] == 0.0 && al[4] == 0.0 && al[5] == 0.0) continue;
+        double X[24], T[8];
+        for (int a = 0; a < 8; ++a) {
+            if (c[a] < 0 || c[a] >= n_nodes) return ASEND_ERR_ARG;
+            T[a] = temp[c[a]];
+            for (int k = 0; k < 3; ++k) X[3 * a + k] = xyz[3 * (int64_t)c[a] + k];
+        }
+        double Cm[36];
+        if (cmat) {
+            const double* src = cmat + 36 * (cmat_idx ? (int64_t)cmat_idx[e] : 0);
+            for (int i = 0; i < 36; ++i) Cm[i] = E[e] * src[i];
+        } else {
+            const double lam = E[e] * nu[e] / ((1.0 + nu[e]) * (1.0 - 2.0 * nu[e])), mu = E[e] / (2.0 * (1.0 + nu[e]));
+            for (int i = 0; i < 36; ++i) Cm[i] = 0.0;
+            for (int i = 0; i < 3; ++i) {
+                for (int j = 0; j < 3; ++j) Cm[6 * i + j] = lam;
+                Cm[7 * i] += 2.0 * mu;
+                Cm[7 * (i + 3)] = mu;
+            }
+        }
+        double ld[8][3];
+        std::memset(ld, 0, sizeof(ld));
+        for (int gp = 0; gp < 8; ++gp) {
+            const double s[3] = {(gp & 1) ? kGp : -kGp, (gp & 2) ? kGp : -kGp, (gp & 4) ? kGp : -kGp};
+            double N[8], J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, dn[8][3];
+            for (int a = 0; a < 8; ++a) {
+                const double fx = 1.0 + kHexNd[a][0] * s[0], fy = 1.0 + kHexNd[a][1] * s[1],
+                             fz = 1.0 + kHexNd[a][2] * s[2];
+                N[a] = 0.125 * fx * fy * fz;
+                dn[a][0] = 0.125 * kHexNd[a][0] * fy * fz;
+                dn[a][1] = 0.125 * fx * kHexNd[a][1] * fz;
+                dn[a][2] = 0.125 * fx * fy * kHexNd[a][2];
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) J[i][j] += X[3 * a + i] * dn[a][j];
+            }
+            const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1], c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2],
+                         c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+            const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+            const double Ji[3][3] = {{c00 / det, (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det,
+                                      (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det},
+                                     {c01 / det, (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det,
+                                      (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det},
+                                     {c02 / det, (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det,
+                                      (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det}};
+            double Tg = 0.0;
+            for (int a = 0; a < 8; ++a) Tg += N[a] * T[a];
+            double sv[6];
+            for (int i = 0; i < 6; ++i) {
+                sv[i] = 0.0;
+                for (int j = 0; j < 6; ++j) sv[i] += Cm[6 * i + j] * al[j] * Tg;
+            }
+            const double S[3][3] = {{sv[0], sv[3], sv[4]}, {sv[3], sv[1], sv[5]}, {sv[4], sv[5], sv[2]}};
+            for (int a = 0; a < 8; ++a) {
+                double g[3];
+                for (int j = 0; j < 3; ++j) g[j] = dn[a][0] * Ji[0][j] + dn[a][1] * Ji[1][j] + dn[a][2] * Ji[2][j];
+                for (int i = 0; i < 3; ++i) ld[a][i] += det * (S[i][0] * g[0] + S[i][1] * g[1] + S[i][2] * g[2]);
+            }
+        }
+        for (int a = 0; a < 8; ++a)
+            for (int k = 0; k < 3; ++k) {
+                const int64_t g = dof_index[6 * (int64_t)c[a] + k];
+                if (g < 0 || g >= n_dof) return ASEND_ERR_ARG;
+                f[g] += ld[a][k];
+            }
+    }
+    return ASEND_OK;
+}
+
 // bodyForce / gravitational / centrifugal element loads of hex elements
 // (Element::get_app_load_dfd0, S/element_equations.rs:1876-1969): a nodal
 // acceleration field pushed through the consistent mass integral
@@ -632,6 +714,15 @@ int asend_surface_flux(int elem_type, int64_t n_nodes, const double* xyz, int64_
     if (!xyz || !conn || !dof_index || (n_set > 0 && !el_set) || !norm_dir || !f) return ASEND_ERR_ARG;
     return asend::surface_flux(elem_type, n_nodes, xyz, n_el, conn, dof_index, n_set, el_set, q, norm_dir,
                                norm_tol_deg, n_dof, f);
+}
+
+int asend_thermal_strain_load(int elem_type, int64_t n_nodes, const double* xyz, int64_t n_el, const int32_t* conn,
+                              const int64_t* dof_index, const double* E, const double* nu, const double* cmat,
+                              const int32_t* cmat_idx, const double* alpha, const double* temp, int64_t n_dof,
+                              double* f) {
+    if (!xyz || !conn || !dof_index || !E || (!nu && !cmat) || !alpha || !temp || !f) return ASEND_ERR_ARG;
+    return asend::thermal_strain_load(elem_type, n_nodes, xyz, n_el, conn, dof_index, E, nu, cmat, cmat_idx, alpha,
+                                      temp, n_dof, f);
 }
 
 int asend_body_load(int elem_type, int kind, int64_t n_nodes, const double* xyz, int64_t n_el, const int32_t* conn,

@@ -21,6 +21,10 @@ int surface_flux(int elem_type, int64_t n_nodes, const double* xyz, int64_t n_el
                  const int64_t* dof_index, int64_t n_set, const int32_t* el_set, double q, const double* norm_dir,
                  double norm_tol_deg, int64_t n_dof, double* f);
 
+int thermal_strain_load(int elem_type, int64_t n_nodes, const double* xyz, int64_t n_el, const int32_t* conn,
+                        const int64_t* dof_index, const double* E, const double* nu, const double* cmat,
+                        const int32_t* cmat_idx, const double* alpha, const double* temp, int64_t n_dof, double* f);
+
 int body_load(int elem_type, int kind, int64_t n_nodes, const double* xyz, int64_t n_el, const int32_t* conn,
               const int64_t* dof_index, int64_t n_set, const int32_t* el_set, const double* rho, const double* load,
               const double* center, const double* axis, double ang_vel, int64_t n_dof, double* f);

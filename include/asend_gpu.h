@@ -172,6 +172,20 @@ int asend_surface_flux(int elem_type, int64_t n_nodes, const double* xyz, int64_
                        const int64_t* dof_index, int64_t n_set, const int32_t* el_set, double q,
                        const double* norm_dir, double norm_tol_deg, int64_t n_dof, double* f);
 
+/* Thermal expansion load (get_ruk_dfd0, S/element_equations.rs:272-298: the
+ * strain at each Gauss point loses the material's expansion coefficients
+ * times the interpolated nodal temperature, with no reference temperature):
+ * accumulates (+=) f_a = sum_gp w detJ S g_a, S the stress of
+ * C (alpha T_gp).  C: isotropic from (E_e, nu_e), or E_e times the
+ * element's matrix of cmat (n x 36, cmat_idx per element; NULL: matrix 0)
+ * when cmat is given.  alpha: 6 per element (strain order 11, 22, 33, 12,
+ * 13, 23); temp: per node.  A linear static step then solves
+ * K u = f + this load.  brick8 only. */
+int asend_thermal_strain_load(int elem_type, int64_t n_nodes, const double* xyz, int64_t n_el, const int32_t* conn,
+                              const int64_t* dof_index, const double* E, const double* nu, const double* cmat,
+                              const int32_t* cmat_idx, const double* alpha, const double* temp, int64_t n_dof,
+                              double* f);
+
 int asend_body_load(int elem_type, int kind, int64_t n_nodes, const double* xyz, int64_t n_el,
                     const int32_t* conn, const int64_t* dof_index, int64_t n_set, const int32_t* el_set,
                     const double* rho, const double* load, const double* center, const double* axis,

@@ -236,3 +236,26 @@ def transient(K, C, pen, sf, t_pen, load_at, dt, gamma, sim_period):
         hist.append((T.copy(), Td.copy()))
         t += dt
     return hist
+
+
+def thermal_strain_load(xyz, conn, E, nu, alpha, temp):
+    """The expansion term of get_ruk_dfd0 (S/element_equations.rs:272-298):
+    node forces (n_nodes, 3) of sum_gp w detJ B^T C (alpha N.T), isotropic C."""
+    f = np.zeros((xyz.shape[0], 3))
+    for e in range(conn.shape[0]):
+        c = conn[e]
+        X = xyz[c]
+        lam = E[e] * nu[e] / ((1 + nu[e]) * (1 - 2 * nu[e]))
+        mu = E[e] / (2 * (1 + nu[e]))
+        Cm = np.zeros((6, 6))
+        Cm[:3, :3] = lam
+        Cm[np.arange(3), np.arange(3)] += 2 * mu
+        Cm[np.arange(3, 6), np.arange(3, 6)] = mu
+        for s in _GP:
+            N, dN = _shape(s)
+            J = X.T @ dN
+            G = dN @ np.linalg.inv(J)
+            sv = Cm @ (np.asarray(alpha[e]) * (N @ temp[c]))
+            S = np.array([[sv[0], sv[3], sv[4]], [sv[3], sv[1], sv[5]], [sv[4], sv[5], sv[2]]])
+            f[c] += np.linalg.det(J) * (G @ S)
+    return f

@@ -257,3 +257,50 @@ def test_cli_transient_thermal_job(tmp_path):
     assert np.abs(last[:, 1] - T_ref).max() <= 1e-9 * np.abs(T_ref).max()
     want = K @ T_ref + C @ Td_ref
     assert np.abs(last[:, 3] - want).max() <= 1e-7 * np.abs(want).max()
+
+
+def test_cli_thermal_expansion_free_expansion(tmp_path):
+    """thermal + elastic with thermal expansion: every node held at T = 80,
+    rollers on the x = 0, y = 0 and z = 0 faces, no loads.  The body expands
+    stress-free, u = 80 (a1 x, a2 y, a3 z) exactly (a patch test on a mesh
+    with moved interior nodes), and the reactions vanish."""
+    gpu_case("asend_run_job thermal expansion, free expansion")
+    m = hex_block(4, 3, 3, 1.0, 0.8, 0.6)
+    xyz = m.xyz.copy()
+    rng = np.random.default_rng(8)
+    inner = np.all((xyz > 1e-9) & (xyz < np.array([1.0, 0.8, 0.6]) - 1e-9), axis=1)
+    xyz[inner] += rng.uniform(-0.04, 0.04, (int(inner.sum()), 3))
+    al = (1e-5, 2e-5, 3e-5, 0.0, 0.0, 0.0)
+    work = tmp_path / "expansion"
+    work.mkdir()
+    near = lambda k: [int(i) for i in np.flatnonzero(np.abs(xyz[:, k]) < 1e-9)]
+    model = {
+        "nodes": [str([i] + [float(v) for v in xyz[i]]) for i in range(m.n_nodes)],
+        "elements": [{"type": "brick8",
+                      "connectivity": [str([e] + [int(v) for v in m.conn[e]]) for e in range(m.n_el)]}],
+        "sets": {"node": {"x0": near(0), "y0": near(1), "z0": near(2)}},
+        "sections": [{"type": "solid", "material": "mat", "elementSet": "all"}],
+        "materials": {"mat": {"density": 1.0,
+                              "elastic": {"E": str([70e3] * 3), "nu": str([0.3] * 3), "G": str([70e3 / 2.6] * 3)},
+                              "thermal": {"conductivity": str([2.0, 2.0, 2.0, 0.0, 0.0, 0.0]),
+                                          "expansion": str(list(al))}}},
+        "constraints": [{"type": "displacement", "terms": [{"nodeSet": s, "dof": d, "coef": 1.0}], "rhs": 0.0}
+                        for s, d in (("x0", 1), ("y0", 2), ("z0", 3))]
+        + [{"type": "temperature", "terms": [{"nodeSet": "all", "coef": 1.0}], "rhs": 80.0}],
+    }
+    dump(work / "model.yaml", model)
+    (work / "results").mkdir()
+    dump(work / "job.yaml", {"jobCommands": [
+        {"command": "readModelInput", "fileName": "model.yaml"},
+        {"command": "solve", "thermal": "yes", "solverMethod": "direct", "solnHistDir": "results"},
+        {"command": "writeNodeResults", "fileName": "results/nodes.csv", "nodeSet": "all",
+         "fields": ["temperature", "displacement", "reactionForce"]}]})
+    run(work)
+    head, rows = read_nodes(work / "results" / "nodes.csv")
+    assert np.abs(rows[:, 1] - 80.0).max() <= 1e-9 * 80.0
+    U = rows[:, 2:5]
+    want = 80.0 * xyz * np.array(al[:3])
+    assert np.abs(U - want).max() <= 1e-9 * np.abs(want).max()
+    RF = rows[:, 8:11]
+    # stress-free: the reactions are rounding next to the expansion load's scale (E alpha T h^2 ~ 2e-2)
+    assert np.abs(RF).max() <= 1e-8

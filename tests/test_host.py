@@ -123,6 +123,22 @@ def test_thermal_loads_match_oracle(dims):
     assert np.abs(a - b).max() <= 1e-13 * np.abs(b).max()
 
 
+def test_thermal_strain_load_matches_oracle():
+    import thermal as TH
+    m = hex_block(4, 2, 3, 1.3, 0.4, 0.3)
+    rng = np.random.default_rng(6)
+    xyz = m.xyz + 0.01 * rng.standard_normal(m.xyz.shape)
+    _, di, nd = O.reorder_nodes(xyz, m.conn, 3, [[m.nodes_at_x(0)]] * 3, 24)
+    E = rng.uniform(50e3, 90e3, m.n_el)
+    nu = rng.uniform(0.2, 0.35, m.n_el)
+    al = rng.uniform(-1e-5, 3e-5, (m.n_el, 6))
+    T = rng.uniform(-20.0, 120.0, m.n_nodes)
+    a = api.thermal_strain_load(8, xyz, m.conn, di, nd, E, nu, al, T)
+    b = TH.thermal_strain_load(xyz, m.conn, E, nu, al, T)
+    d3 = np.asarray(di).reshape(-1, 6)[:, :3]
+    assert np.abs(a[d3] - b).max() <= 1e-12 * np.abs(b).max()
+
+
 def test_lognormal_moduli_seeded():
     a = api.lognormal_moduli(20261202, 200000, 70e3, 1.0)
     b = api.lognormal_moduli(20261202, 200000, 70e3, 1.0)

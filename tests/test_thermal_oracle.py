@@ -275,3 +275,32 @@ def test_transient_converges_to_semidiscrete_exact():
     assert 1.8 <= errs[0] / errs[1] <= 2.3, errs
     # and the error is the first step's lost rise carried forward: below dt b / C
     assert errs[1] <= 0.01 * np.abs(exact).max()
+
+
+def test_thermal_strain_load_is_free_expansion():
+    """A uniform temperature T with expansion coefficients alpha expands a
+    free body stress-free: u = T (a1 x + a4 y, a2 y + a6 z... ) is the linear
+    field with engineering strains alpha T, so the load must equal K u for
+    the oracle's own (separately pinned) stiffness, on a mesh with moved
+    interior nodes."""
+    import oracle as O
+    m = hex_block(3, 2, 2, 1.5, 1.0, 0.8)
+    xyz = m.xyz.copy()
+    rng = np.random.default_rng(3)
+    inner = np.all((xyz > 1e-9) & (xyz < np.array([1.5, 1.0, 0.8]) - 1e-9), axis=1)
+    xyz[inner] += rng.uniform(-0.1, 0.1, (int(inner.sum()), 3))
+    E = np.full(m.n_el, 70e3)
+    nu = np.full(m.n_el, 0.3)
+    al = np.array([1e-5, 2e-5, 3e-5, 4e-6, 0.0, 5e-6])
+    T = np.full(m.n_nodes, 80.0)
+    f = TH.thermal_strain_load(xyz, m.conn, E, nu, np.tile(al, (m.n_el, 1)), T)
+    # u_x = e11 x + g12 y, u_y = e22 y + g23 z, u_z = e33 z gives strains (e11, e22, e33, g12, 0, g23)
+    e = al * 80.0
+    U = np.column_stack([e[0] * xyz[:, 0] + e[3] * xyz[:, 1], e[1] * xyz[:, 1] + e[5] * xyz[:, 2], e[2] * xyz[:, 2]])
+    _, di6, nd = O.reorder_nodes(xyz, m.conn, 3, [], 1000)
+    K = O.assemble_solid(8, xyz, m.conn, E, nu, di6, nd).to_scipy()
+    di = np.asarray(di6).reshape(-1, 6)[:, :3]
+    u = np.zeros(nd)
+    u[di.reshape(-1)] = U.reshape(-1)
+    Ku = K @ u
+    assert np.abs(f - Ku[di]).max() <= 1e-10 * np.abs(Ku).max()
