@@ -176,6 +176,9 @@ std::string fmt_display(double v) {
 // ---------------------------------------------------------------- the model
 struct Section {
     std::string type, mat, el_set;
+    // local axes as rows (Section::set_orientation, S/section_meth.rs:21-50)
+    double ori[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    bool oriented = false;
     std::vector<std::string> lay_mat;
     std::vector<double> lay_t;
     double z_offset = 0.0;
@@ -271,7 +274,8 @@ struct Job {
     std::vector<int64_t> dof_index;  // 6 per node
     std::vector<double> E, nu, rho, thk;
     std::vector<int> mat_of_el;   // each element's material
-    std::vector<double> cmats;    // 6x6 stiffness per material when any is anisotropic (else empty)
+    std::vector<int> sec_of_el;   // and section
+    std::vector<double> cmats;    // global 6x6 stiffness per section when any is anisotropic (else empty)
     std::vector<int64_t> pen;
     std::vector<double> u;    // solution, reference DOF numbering
     std::vector<double> v, a; // velocity and acceleration (dynamic solves; zero for static)
@@ -434,6 +438,22 @@ struct Job {
                     sections.back().mat = d[0];
                 } else if (h[1] == "elementSet" && d.size() == 1) {
                     sections.back().el_set = d[0];
+                } else if (h[1] == "orientation" && d.size() == 6) {
+                    // x' = xDir normalised, z' = x' x xyVec normalised, y' = z' x x'
+                    double* o = sections.back().ori;
+                    double v[6];
+                    for (int i = 0; i < 6; ++i) v[i] = to_d(d[i]);
+                    double mg = 1.0 / std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                    for (int i = 0; i < 3; ++i) o[i] = mg * v[i];
+                    double z[3] = {o[1] * v[5] - o[2] * v[4], o[2] * v[3] - o[0] * v[5], o[0] * v[4] - o[1] * v[3]};
+                    mg = 1.0 / std::sqrt(z[0] * z[0] + z[1] * z[1] + z[2] * z[2]);
+                    for (int i = 0; i < 3; ++i) o[6 + i] = mg * z[i];
+                    o[3] = o[7] * o[2] - o[8] * o[1];
+                    o[4] = o[8] * o[0] - o[6] * o[2];
+                    o[5] = o[6] * o[1] - o[7] * o[0];
+                    bool id = true;
+                    for (int i = 0; i < 9; ++i) id = id && std::fabs(o[i] - ((i % 4 == 0) ? 1.0 : 0.0)) <= 1e-15;
+                    sections.back().oriented = !id;
                 } else if (h[1] == "layup") {
                     if (h[2] == "zOffset" && d.size() == 1) sections.back().z_offset = to_d(d[0]);
                     else if (h[2] == "layers" && h[3] == "material" && d.size() == 1) {
@@ -682,6 +702,7 @@ struct Job {
             for (int32_t e : element_set(sections[s].el_set)) sect[e] = (int)s;
         std::vector<std::array<double, 9>> par(ne);
         mat_of_el.assign(ne, 0);
+        sec_of_el.assign(ne, 0);
         thk.assign(ne, 0.0);
         rho.assign(ne, 0.0);
         kc.assign(6 * ne, 0.0);
@@ -703,6 +724,7 @@ struct Job {
                 m = &mat_of(s.mat);
             }
             mat_of_el[e] = (int)(m - materials.data());
+            sec_of_el[e] = sect[e];
             std::copy(m->cond, m->cond + 6, kc.begin() + 6 * e);
             for (int c = 0; c < 3; ++c) {
                 par[e][c] = m->E[c];
@@ -757,10 +779,13 @@ struct Job {
                 // the inverse of the compliance of the nine engineering constants
                 if (etype != ASEND_BRICK8) die("element %zu: anisotropic materials are supported for brick8 models", e);
                 if (material_dvs) die("material design variables on anisotropic materials are not supported");
-                if (cmats.empty()) cmats.assign(36 * materials.size(), 0.0);
-                double* C = cmats.data() + 36 * mat_of_el[e];
+                if (cmats.empty()) cmats.assign(36 * sections.size(), 0.0);
+                double* C = cmats.data() + 36 * sec_of_el[e];
                 if (me.stiffness) std::copy(me.stiff, me.stiff + 36, C);
                 else compliance_inverse(p, C);
+                rotate_to_global(sections[sec_of_el[e]], C);
+                if (me.expands && sections[sec_of_el[e]].oriented)
+                    die("thermal expansion in an oriented section is not supported by this solver");
                 E[e] = 1.0;
                 nu[e] = 0.3;
                 continue;
@@ -775,8 +800,8 @@ struct Job {
             // an element of an isotropic material in a model with anisotropic
             // ones takes its material matrix too
             for (size_t e = 0; e < ne; ++e)
-                if (E[e] != 1.0 || nu[e] != 0.3 || cmats[36 * mat_of_el[e]] == 0.0) {
-                    double* C = cmats.data() + 36 * mat_of_el[e];
+                if (E[e] != 1.0 || nu[e] != 0.3 || cmats[36 * sec_of_el[e]] == 0.0) {
+                    double* C = cmats.data() + 36 * sec_of_el[e];
                     compliance_inverse(par[e], C);
                     E[e] = 1.0;
                     nu[e] = 0.3;
@@ -790,6 +815,42 @@ struct Job {
         auto same = [](double a, double b) { return std::fabs(a - b) <= 1e-12 * std::fabs(b); };
         return same(p[1], Ee) && same(p[2], Ee) && same(p[4], ne_) && same(p[5], ne_) &&
                (etype == ASEND_SHELL4 ? same(p[6], G) : (same(p[6], G) && same(p[7], G) && same(p[8], G)));
+    }
+
+    // A section's material matrix in the global frame: the element works in
+    // the local axes R (rows), strain_local = T strain_global (engineering
+    // Voigt), so C_global = T^T C_local T.
+    static void rotate_to_global(const Section& sec, double* C) {
+        if (!sec.oriented) return;
+        const double* R = sec.ori;
+        const int pr[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+        double T[6][6];
+        for (int k = 0; k < 6; ++k) {
+            double eg[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // tensor of engineering basis strain k
+            const int a = pr[k][0], b = pr[k][1];
+            if (k < 3) eg[a][a] = 1.0;
+            else eg[a][b] = eg[b][a] = 0.5;
+            double el[3][3];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    double v = 0.0;
+                    for (int p = 0; p < 3; ++p)
+                        for (int q = 0; q < 3; ++q) v += R[3 * i + p] * eg[p][q] * R[3 * j + q];
+                    el[i][j] = v;
+                }
+            for (int r = 0; r < 6; ++r) T[r][k] = r < 3 ? el[pr[r][0]][pr[r][1]] : 2.0 * el[pr[r][0]][pr[r][1]];
+        }
+        double Cl[36], Cg[36];
+        std::copy(C, C + 36, Cl);
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j < 6; ++j) {
+                double v = 0.0;
+                for (int p = 0; p < 6; ++p)
+                    for (int q = 0; q < 6; ++q) v += T[p][i] * Cl[6 * p + q] * T[q][j];
+                Cg[6 * i + j] = v;
+            }
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j < 6; ++j) C[6 * i + j] = 0.5 * (Cg[6 * i + j] + Cg[6 * j + i]);
     }
 
     // C = S^-1, S the compliance of E1..3, nu12, nu13, nu23, G12, G13, G23
@@ -903,8 +964,8 @@ struct Job {
               "asend_upload_mesh");
         if (etype == ASEND_SHELL4) check(asend_set_thickness(ctx, thk.data()), "asend_set_thickness");
         if (!cmats.empty()) {
-            std::vector<int32_t> idx(mat_of_el.begin(), mat_of_el.end());
-            check(asend_set_material_stiffness(ctx, (int64_t)materials.size(), cmats.data(), idx.data()),
+            std::vector<int32_t> idx(sec_of_el.begin(), sec_of_el.end());
+            check(asend_set_material_stiffness(ctx, (int64_t)sections.size(), cmats.data(), idx.data()),
                   "asend_set_material_stiffness");
         }
         check(asend_set_penalty(ctx, (int64_t)pen.size(), pen.data(), 0.0), "asend_set_penalty");
@@ -1093,7 +1154,7 @@ struct Job {
         std::vector<double> al(6 * ne);
         for (size_t e = 0; e < ne; ++e)
             std::copy(materials[mat_of_el[e]].expansion, materials[mat_of_el[e]].expansion + 6, al.begin() + 6 * e);
-        std::vector<int32_t> cflat(8 * ne), cidx(mat_of_el.begin(), mat_of_el.end());
+        std::vector<int32_t> cflat(8 * ne), cidx(sec_of_el.begin(), sec_of_el.end());
         for (size_t e = 0; e < ne; ++e)
             for (int k = 0; k < 8; ++k) cflat[8 * e + k] = conn[e][k];
         const int rc = asend_thermal_strain_load(etype, (int64_t)nn, xyz_dv.data(), (int64_t)ne, cflat.data(),
@@ -1909,6 +1970,9 @@ struct Job {
         for (const auto& m : materials)
             if (solve_cmd.thermal && m.expands)
                 die("element results of solves with thermal expansion are not supported by this solver");
+        for (const auto& sec : sections)
+            if (sec.oriented)
+                die("element results in oriented sections (local-frame output) are not supported by this solver");
         for (const auto& fld : c.fields)
             if (fld != "stress" && fld != "strain" && fld != "strainEnergyDen")
                 die("element result field %s is not supported by this solver", fld.c_str());
@@ -2016,7 +2080,7 @@ struct Job {
                 eps[q] = q < 3 ? 0.5 * v : v;
             }
             if (!cmats.empty()) {
-                const double* C = cmats.data() + 36 * mat_of_el[e];
+                const double* C = cmats.data() + 36 * sec_of_el[e];
                 for (int q = 0; q < 6; ++q) {
                     sig[q] = 0.0;
                     for (int r = 0; r < 6; ++r) sig[q] += E[e] * C[6 * q + r] * eps[r];

@@ -278,3 +278,52 @@ def test_cli_orthotropic_material(tmp_path):
     eps = el[:, [col[k] for k in ("E11", "E22", "E33", "E12", "E13", "E23")]]
     sig = el[:, [col[k] for k in ("S11", "S22", "S33", "S12", "S13", "S23")]]
     assert np.abs(sig - eps @ C.T).max() <= 1e-12 * np.abs(sig).max()
+
+
+def test_cli_orthotropic_oriented_section(tmp_path):
+    """The orthotropic hexBeam job with the section's orientation turned by
+    30 degrees about z (xDir (cos, sin, 0), xyVec (-sin, cos, 0)): the
+    displacements against the oracle's elements integrated in that local
+    frame (the reference's orientation rows, Section::set_orientation)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    import oracle as O
+    gpu_case("asend_run_job orthotropic hexBeam, oriented section")
+    loads = [{"type": "nodalForce", "nodeSet": "xMax", "load": str([0.0, 0.02, 0.05, 0.0, 0.0, 0.0])}]
+    work = tmp_path / "oriented"
+    work.mkdir()
+    m = write_hexbeam(work, loads, {})
+    c30, s30 = float(np.cos(np.pi / 6)), float(np.sin(np.pi / 6))
+    model = (work / "hexBeam.yaml").read_text()
+    model = model.replace("E: [1000000.0, 1000000.0, 1000000.0]", "E: [1000000.0, 400000.0, 250000.0]")
+    model = model.replace("nu: [0.0, 0.0, 0.0]", "nu: [0.3, 0.2, 0.25]")
+    model = model.replace("G: [500000.0, 500000.0, 500000.0]", "G: [300000.0, 200000.0, 150000.0]")
+    model = model.replace("type: brickIM", "type: brick8")
+    model = model.replace("  elementSet: all\n", "  elementSet: all\n  orientation: " + str([c30, s30, 0.0, -s30, c30, 0.0]) + "\n", 1)
+    assert "orientation:" in model
+    (work / "hexBeam.yaml").write_text(model)
+    run(work)
+    U = node_displacements(work, m)
+    C = O.solid_cmat_ortho((1e6, 4e5, 2.5e5), (0.3, 0.2, 0.25), (3e5, 2e5, 1.5e5))
+    R = np.array([[c30, s30, 0.0], [-s30, c30, 0.0], [0.0, 0.0, 1.0]])
+    clamp = m.nodes_at_x(0)
+    _, di, n_dof = O.reorder_nodes(m.xyz, m.conn, 3, [[clamp]] * 3, 2000000000)
+    di = np.asarray(di).reshape(-1, 6)[:, :3]
+    rows, cols, vals = [], [], []
+    for e in range(m.n_el):
+        c = m.conn[e]
+        K_e, _ = O.solid_element(8, m.xyz[c], C, ori=R.reshape(-1))
+        d = di[c].reshape(-1)
+        rows.append(np.repeat(d, 24))
+        cols.append(np.tile(d, 24))
+        vals.append(K_e.reshape(-1))
+    K = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n_dof, n_dof))
+    sf = 1e5 * abs(K).max()
+    P = np.zeros(n_dof)
+    np.add.at(P, di[clamp].reshape(-1), sf)
+    f = np.zeros(n_dof)
+    for nd in m.nodes_at_x(m.nx):
+        f[di[nd, 1]] += 0.02
+        f[di[nd, 2]] += 0.05
+    U_ref = spla.spsolve((K + sp.diags(P)).tocsc(), f)[di]
+    assert np.abs(U - U_ref).max() <= 1e-9 * np.abs(U_ref).max()
