@@ -236,6 +236,10 @@ struct Command {
     long num_modes = 10;
     double tgt_eval = 0.0;
     bool write_modes = true;
+    // setSolnToMode
+    std::string soln_field = "displacement";
+    long mode = 0;
+    double max_amp = 1.0;
     bool elastic = true, dynamic = false, nonlinear = false, thermal = false, write_grad = true;
     long block_dim = -1, max_it = 0, load_ramp = 1;
     double conv_tol = 1.0e-12;
@@ -357,6 +361,9 @@ struct Job {
             else if (k == "numModes" && d.size() == 1) c.num_modes = to_i(d[0]);
             else if (k == "targetEigenvalue" && d.size() == 1) c.tgt_eval = to_d(d[0]);
             else if (k == "writeModes" && d.size() == 1) c.write_modes = (d[0] == "yes");
+            else if (k == "solnField" && d.size() == 1) c.soln_field = d[0];
+            else if (k == "mode" && d.size() == 1) c.mode = to_i(d[0]);
+            else if (k == "maxAmplitude" && d.size() == 1) c.max_amp = to_d(d[0]);
             else if (k == "position" && d.size() == 1) c.position = d[0];
             else if (k == "fields" && d.size() == 1) c.fields.push_back(d[0]);
             else if (k == "include" && d.size() == 1) c.include.push_back(d[0]);
@@ -1379,10 +1386,14 @@ struct Job {
                     die("thermal expansion is coupled for linear static solves by this solver (not nonlinear or dynamic)");
         solve_cmd = c;
         if (solve_cmd.load_times.empty()) solve_cmd.load_times.push_back(0.0);
+        const int64_t n_prev = n_dof;
         analysis_prep(solve_cmd);
-        u.assign(n_dof, 0.0);
-        v.assign(n_dof, 0.0);
-        a.assign(n_dof, 0.0);
+        // a state set by setSolnToMode / zeroSolution on the same numbering is
+        // the starting state (the reference's solves start from the nodes' state)
+        if (!(state_kept && n_prev == n_dof && (int64_t)u.size() == n_dof)) u.assign(n_dof, 0.0);
+        if (!(state_kept && (int64_t)v.size() == n_dof)) v.assign(n_dof, 0.0);
+        if (!(state_kept && (int64_t)a.size() == n_dof)) a.assign(n_dof, 0.0);
+        state_kept = false;
         if (c.dynamic && c.thermal) {
             solve_thermal_dynamic();
         } else if (c.dynamic) {
@@ -2179,6 +2190,53 @@ struct Job {
                     (long long)st.modes_found);
     }
 
+    // Model::set_soln_to_mode (S/model_analysis.rs:890-934): the displacement,
+    // velocity or acceleration field set to a mode scaled to maxAmplitude
+    // (largest entry); the next static solve starts from it
+    bool state_kept = false;
+    void set_soln_to_mode(const Command& c) {
+        if (c.soln_field != "displacement" && c.soln_field != "velocity" && c.soln_field != "acceleration") {
+            std::printf("Warning: %s is not a valid field to Set to a mode. Aborting setSolnToMode()\n",
+                        c.soln_field.c_str());
+            return;
+        }
+        if (eig_vals.empty() || c.mode < 0 || (size_t)c.mode >= eig_vals.size())
+            die("setSolnToMode needs mode %ld of a modal analysis", c.mode);
+        const double* ph = eig_vecs.data() + (size_t)c.mode * n_dof;
+        double mx = 0.0;
+        for (int64_t i = 0; i < n_dof; ++i) mx = std::max(mx, std::fabs(ph[i]));
+        const double sc = c.max_amp / mx;
+        std::vector<double>& dst = c.soln_field == "displacement" ? u : c.soln_field == "velocity" ? v : a;
+        dst.assign(n_dof, 0.0);
+        for (int64_t i = 0; i < n_dof; ++i) dst[i] = sc * ph[i];
+        state_kept = true;
+    }
+
+    // Model::zero_solution (S/model_analysis.rs:675-712): the listed fields
+    // (and with displacement the elements' internal DOFs) set to zero
+    void zero_solution(const Command& c) {
+        for (const auto& fld : c.fields) {
+            std::vector<double>* vec = fld == "displacement"   ? &u
+                                       : fld == "velocity"     ? &v
+                                       : fld == "acceleration" ? &a
+                                                               : nullptr;
+            if (vec) vec->assign(vec->size(), 0.0);
+            if (fld == "temperature") {
+                std::fill(temp.begin(), temp.end(), 0.0);
+                std::fill(t3.begin(), t3.end(), 0.0);
+            }
+            if (fld == "tdot") {
+                std::fill(tdot.begin(), tdot.end(), 0.0);
+                std::fill(t3dot.begin(), t3dot.end(), 0.0);
+            }
+            if (fld == "displacement" && ctx && etype == ASEND_BRICKIM) {
+                std::vector<double> zero(9 * conn.size(), 0.0);
+                check(asend_brickim_set_internal(ctx, zero.data()), "asend_brickim_set_internal");
+            }
+        }
+        state_kept = true;
+    }
+
     // write_modal_results (S/model_output.rs:495-551): mode, eigenvalue and
     // load factor or frequency; with writeModes (the flag of the
     // modalAnalysis command, as the reference reads it) one file per mode,
@@ -2288,6 +2346,10 @@ struct Job {
             } else if (c.cmd == "writeNodeResults") {
                 std::printf("writing Node results\n");
                 write_node_results(c);
+            } else if (c.cmd == "setSolnToMode") {
+                set_soln_to_mode(c);
+            } else if (c.cmd == "zeroSolution") {
+                zero_solution(c);
             } else if (c.cmd == "modalAnalysis") {
                 std::printf("running modal analysis\n");
                 modal_analysis(c);

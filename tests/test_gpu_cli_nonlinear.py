@@ -327,3 +327,31 @@ def test_cli_orthotropic_oriented_section(tmp_path):
         f[di[nd, 2]] += 0.05
     U_ref = spla.spsolve((K + sp.diags(P)).tocsc(), f)[di]
     assert np.abs(U - U_ref).max() <= 1e-9 * np.abs(U_ref).max()
+
+
+def test_cli_set_soln_to_mode_and_zero_solution(tmp_path):
+    """setSolnToMode puts the first buckling mode, scaled to a largest
+    entry of 0.01, into the displacements (Model::set_soln_to_mode);
+    zeroSolution clears them again."""
+    gpu_case("asend_run_job setSolnToMode and zeroSolution")
+    loads = [{"type": "surfacePressure", "elementSet": "all", "load": str([20.56176 / 2]),
+              "normDir": str([1.0, 0.0, 0.0]), "normTolerance": 5.0}]
+    work = tmp_path / "modeset"
+    work.mkdir()
+    m = write_hexbeam(work, loads, {}, [
+        {"command": "modalAnalysis", "type": "buckling", "numModes": 1},
+        {"command": "writeModalResults", "fileName": "results/buckling.csv"},
+        {"command": "setSolnToMode", "solnField": "displacement", "mode": 0, "maxAmplitude": 0.01},
+        {"command": "writeNodeResults", "fileName": "results/mode_state.csv", "nodeSet": "all",
+         "fields": ["displacement"]},
+        {"command": "zeroSolution", "fields": ["displacement"]},
+        {"command": "writeNodeResults", "fileName": "results/zeroed.csv", "nodeSet": "all",
+         "fields": ["displacement"]}])
+    run(work)
+    U = np.loadtxt(work / "results" / "mode_state.csv", delimiter=",", skiprows=1)[:, 1:4]
+    mode0 = np.loadtxt(work / "results" / "buckling_mode0.csv", delimiter=",", skiprows=1)[:, 1:4]
+    assert abs(np.abs(U).max() - 0.01) <= 1e-15
+    scale = 0.01 / np.abs(mode0).max()
+    assert np.abs(U - scale * mode0).max() <= 1e-12
+    Z = np.loadtxt(work / "results" / "zeroed.csv", delimiter=",", skiprows=1)[:, 1:4]
+    assert not Z.any()
