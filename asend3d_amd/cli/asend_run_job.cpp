@@ -1393,6 +1393,8 @@ struct Job {
         if (!(state_kept && n_prev == n_dof && (int64_t)u.size() == n_dof)) u.assign(n_dof, 0.0);
         if (!(state_kept && (int64_t)v.size() == n_dof)) v.assign(n_dof, 0.0);
         if (!(state_kept && (int64_t)a.size() == n_dof)) a.assign(n_dof, 0.0);
+        if (state_kept && !nd_state.empty()) apply_node_state();  // read node results, in this numbering
+        nd_state.clear();
         state_kept = false;
         if (c.dynamic && c.thermal) {
             solve_thermal_dynamic();
@@ -2190,6 +2192,69 @@ struct Job {
                     (long long)st.modes_found);
     }
 
+    // Model::read_node_results (S/model_input.rs:1403-1466): a node results
+    // CSV (header row with "node", then label,values...) sets the named
+    // nodal fields (U1..R3, V1..RV3, A1..RA3, T, TDOT); other columns are
+    // ignored.  The values apply now and to the next solve's starting state.
+    std::vector<double> nd_state;  // per node: 6 u, 6 v, 6 a, T, Tdot
+    void read_node_results(const std::string& path) {
+        const size_t nn = xyz.size() / 3;
+        if (nd_state.size() != 20 * nn) nd_state.assign(20 * nn, 0.0);
+        std::vector<std::string> head;
+        static const char* names[20] = {"U1", "U2", "U3", "R1", "R2", "R3", "V1", "V2", "V3", "RV1",
+                                        "RV2", "RV3", "A1", "A2", "A3", "RA1", "RA2", "RA3", "T", "TDOT"};
+        auto split = [](const std::string& ln) {
+            std::vector<std::string> out;
+            size_t a = 0;
+            while (true) {
+                const size_t b = ln.find(',', a);
+                out.push_back(ln.substr(a, b == std::string::npos ? std::string::npos : b - a));
+                if (b == std::string::npos) break;
+                a = b + 1;
+            }
+            return out;
+        };
+        for (const auto& line : read_lines(path, "node results")) {
+            if (line.find("node") != std::string::npos) {
+                head = split(line);
+                continue;
+            }
+            if (line.find(',') == std::string::npos) continue;
+            const auto d = split(line);
+            const long n = to_i(d[0]);
+            if (n < 0 || (size_t)n >= nn) die("problem reading the node results file, %s", path.c_str());
+            for (size_t i = 1; i < d.size() && i < head.size(); ++i)
+                for (int k = 0; k < 20; ++k)
+                    if (head[i] == names[k]) nd_state[20 * (size_t)n + k] = to_d(d[i]);
+        }
+        apply_node_state();
+        state_kept = true;
+    }
+
+    void apply_node_state() {
+        if (nd_state.empty() || n_dof == 0) return;
+        const size_t nn = xyz.size() / 3;
+        u.assign(n_dof, 0.0);
+        v.assign(n_dof, 0.0);
+        a.assign(n_dof, 0.0);
+        for (size_t n = 0; n < nn; ++n)
+            for (int k = 0; k < dpn; ++k) {
+                const int64_t dd = dof_index[6 * n + k];
+                if (dd < 0) continue;
+                u[dd] = nd_state[20 * n + k];
+                v[dd] = nd_state[20 * n + 6 + k];
+                a[dd] = nd_state[20 * n + 12 + k];
+            }
+        if (temp.size() == nn)
+            for (size_t n = 0; n < nn; ++n) {
+                temp[n] = nd_state[20 * n + 18];
+                if (tdot.size() == nn) tdot[n] = nd_state[20 * n + 19];
+                const int64_t d = dof_index[6 * n];
+                if (d >= 0 && (int64_t)t3.size() == n_dof) t3[d] = temp[n];
+                if (d >= 0 && (int64_t)t3dot.size() == n_dof) t3dot[d] = tdot[n];
+            }
+    }
+
     // Model::set_soln_to_mode (S/model_analysis.rs:890-934): the displacement,
     // velocity or acceleration field set to a mode scaled to maxAmplitude
     // (largest entry); the next static solve starts from it
@@ -2346,6 +2411,9 @@ struct Job {
             } else if (c.cmd == "writeNodeResults") {
                 std::printf("writing Node results\n");
                 write_node_results(c);
+            } else if (c.cmd == "readNodeResults") {
+                std::printf("reading node results: %s\n", c.file.c_str());
+                read_node_results(c.file);
             } else if (c.cmd == "setSolnToMode") {
                 set_soln_to_mode(c);
             } else if (c.cmd == "zeroSolution") {

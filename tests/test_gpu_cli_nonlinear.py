@@ -355,3 +355,31 @@ def test_cli_set_soln_to_mode_and_zero_solution(tmp_path):
     assert np.abs(U - scale * mode0).max() <= 1e-12
     Z = np.loadtxt(work / "results" / "zeroed.csv", delimiter=",", skiprows=1)[:, 1:4]
     assert not Z.any()
+
+
+def test_cli_read_node_results_restores_state(tmp_path):
+    """readNodeResults (Model::read_node_results, S/model_input.rs:1403-1466)
+    sets the displacements from a node results CSV: after zeroSolution the
+    read state writes back the same file, and a second solve of the same
+    loads starting from it stays at the first solution."""
+    gpu_case("asend_run_job readNodeResults")
+    loads = [{"type": "nodalForce", "nodeSet": "xMax", "load": str([0.0, 0.0, -1.0])}]
+    work = tmp_path / "readback"
+    work.mkdir()
+    m = write_hexbeam(work, loads, {}, [
+        {"command": "zeroSolution", "fields": ["displacement"]},
+        {"command": "readNodeResults", "fileName": "results/nodeResults.csv"},
+        {"command": "writeNodeResults", "fileName": "results/read.csv", "nodeSet": "all",
+         "fields": ["displacement"]},
+        {"command": "solve", "solverMethod": "direct", "solnHistDir": "results"},
+        {"command": "writeNodeResults", "fileName": "results/resolved.csv", "nodeSet": "all",
+         "fields": ["displacement"]}])
+    run(work)
+    first = (work / "results" / "nodeResults.csv").read_text()
+    assert (work / "results" / "read.csv").read_text() == first
+    U0 = np.loadtxt(work / "results" / "nodeResults.csv", delimiter=",", skiprows=1)[:, 1:4]
+    U1 = np.loadtxt(work / "results" / "resolved.csv", delimiter=",", skiprows=1)[:, 1:4]
+    assert np.abs(U0[:, 2]).max() > 0.0
+    # the second solve is one refinement step of the first, whose direct
+    # solve of this slender beam ends at a relative residual near 1e-8
+    assert np.abs(U1 - U0).max() <= 1e-8 * np.abs(U0).max()
