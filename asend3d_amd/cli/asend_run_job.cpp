@@ -1387,6 +1387,12 @@ struct Job {
         solve_cmd = c;
         if (solve_cmd.load_times.empty()) solve_cmd.load_times.push_back(0.0);
         const int64_t n_prev = n_dof;
+        if (init_pending && !first_solve_done) {  // the initial state starts the first solve
+            nd_state = init_state;
+            nd_state.resize(20 * (xyz.size() / 3), 0.0);
+            state_kept = true;
+        }
+        first_solve_done = true;
         analysis_prep(solve_cmd);
         // a state set by setSolnToMode / zeroSolution on the same numbering is
         // the starting state (the reference's solves start from the nodes' state)
@@ -1487,7 +1493,6 @@ struct Job {
         check(asend_set_rayleigh_damping(ctx, c.ray_cm, c.ray_ck), "asend_set_rayleigh_damping");
         material_damping();
         check(asend_newmark_setup(ctx, c.time_step, c.beta, c.gamma), "asend_newmark_setup");
-        u.assign(n_dof, 0.0);
         if (c.save_hist) write_state(0, u, v, a);
         std::vector<double> u1(n_dof), v1(n_dof), a1(n_dof), fs;
         const double one = 1.0;
@@ -2197,6 +2202,38 @@ struct Job {
     // nodal fields (U1..R3, V1..RV3, A1..RA3, T, TDOT); other columns are
     // ignored.  The values apply now and to the next solve's starting state.
     std::vector<double> nd_state;  // per node: 6 u, 6 v, 6 a, T, Tdot
+
+    // Model::read_initial_state (S/model_input.rs:1125-1200), applied by the
+    // first analysis prep (Node::initialize_disp / initialize_temp,
+    // S/node_meth.rs:104-119, S/model_prep.rs:564,598): "initialState" rows of
+    // a node set and up to six values for displacement, velocity and
+    // acceleration (missing ones zero), or one value for temperature and tdot.
+    std::vector<double> init_state;  // per node, as nd_state
+    bool init_pending = false, first_solve_done = false;
+    void read_initial_state(const std::string& path) {
+        LineReader lr;
+        std::vector<std::string> d;
+        const size_t nn = xyz.size() / 3;
+        for (const auto& line : read_lines(path, "initial state")) {
+            lr.read(line, d);
+            const std::string* h = lr.head;
+            if (h[0] != "initialState") continue;
+            int off = -1;
+            size_t nv = 0;
+            if ((h[1] == "displacement" || h[1] == "velocity" || h[1] == "acceleration") && d.size() > 3) {
+                off = h[1] == "displacement" ? 0 : h[1] == "velocity" ? 6 : 12;
+                nv = 6;
+            } else if ((h[1] == "temperature" || h[1] == "tdot") && d.size() == 2) {
+                off = h[1] == "temperature" ? 18 : 19;
+                nv = 1;
+            }
+            if (off < 0) continue;
+            if (init_state.size() != 20 * nn) init_state.resize(20 * nn, 0.0);
+            for (int32_t n : node_set(d[0], false))
+                for (size_t k = 0; k < nv; ++k) init_state[20 * (size_t)n + off + k] = 1 + k < d.size() ? to_d(d[1 + k]) : 0.0;
+            init_pending = true;
+        }
+    }
     void read_node_results(const std::string& path) {
         const size_t nn = xyz.size() / 3;
         if (nd_state.size() != 20 * nn) nd_state.assign(20 * nn, 0.0);
@@ -2383,6 +2420,10 @@ struct Job {
                 read_model(c.file);
                 read_constraints(c.file);
                 read_loads(c.file);
+                read_initial_state(c.file);
+            } else if (c.cmd == "readInitialState") {
+                std::printf("reading initial state: %s\n", c.file.c_str());
+                read_initial_state(c.file);
             } else if (c.cmd == "readConstraints") {
                 std::printf("reading constraints: %s\n", c.file.c_str());
                 read_constraints(c.file);

@@ -383,3 +383,45 @@ def test_cli_read_node_results_restores_state(tmp_path):
     # the second solve is one refinement step of the first, whose direct
     # solve of this slender beam ends at a relative residual near 1e-8
     assert np.abs(U1 - U0).max() <= 1e-8 * np.abs(U0).max()
+
+
+def test_cli_initial_velocity_rigid_translation(tmp_path):
+    """readInitialState (Model::read_initial_state, S/model_input.rs:
+    1125-1200): an initial velocity v0 on all nodes of an unconstrained,
+    unloaded block starts the dynamic solve; Newmark then carries the block
+    as a rigid body, u = v0 t exactly, at constant velocity, and the state
+    file of step 0 holds v0 (as the reference's singleHex dynamic case sets
+    its initial velocity, E/testCases/singleHex/runDynamicElastic.py)."""
+    gpu_case("asend_run_job readInitialState dynamic")
+    m = hex_block(4, 2, 2, 1.0, 0.5, 0.5)
+    work = tmp_path / "initvel"
+    work.mkdir()
+    (work / "results").mkdir()
+    v0 = [0.5, -0.25, 1.0]
+    dump(work / "block.yaml", {
+        "nodes": [str([i] + [float(v) for v in m.xyz[i]]) for i in range(m.n_nodes)],
+        "elements": [{"type": "brick8",
+                      "connectivity": [str([e] + [int(v) for v in m.conn[e]]) for e in range(m.n_el)]}],
+        "sections": [{"type": "solid", "material": "myMat", "elementSet": "all"}],
+        "materials": {"myMat": {"density": 2.0, "elastic": {"E": str([1e3] * 3), "nu": str([0.25] * 3),
+                                                            "G": str([400.0] * 3)}}}})
+    dump(work / "init.yaml", {"initialState": {"velocity": [str(["all"] + v0)]}})
+    dump(work / "job.yaml", {"jobCommands": [
+        {"command": "readModelInput", "fileName": "block.yaml"},
+        {"command": "readInitialState", "fileName": "init.yaml"},
+        {"command": "solve", "solverMethod": "direct", "dynamic": "yes", "timeStep": 0.25, "simPeriod": 1.0,
+         "solnHistDir": "results"},
+        {"command": "writeNodeResults", "fileName": "results/nodeResults.csv", "nodeSet": "all",
+         "fields": ["displacement", "velocity"]}]})
+    run(work)
+    path = work / "results" / "nodeResults.csv"
+    head = [h.strip() for h in path.read_text().splitlines()[0].split(",")]
+    rows = np.loadtxt(path, delimiter=",", skiprows=1, usecols=range(len(head) - (head[-1] == "")))
+    iu, iv = head.index("U1"), head.index("V1")
+    U, V = rows[:, iu:iu + 3], rows[:, iv:iv + 3]
+    assert np.abs(U - np.array(v0) * 1.0).max() <= 1e-9
+    assert np.abs(V - np.array(v0)).max() <= 1e-9
+    raw = np.fromfile(work / "results" / "solnTStep0.out", dtype=">f8")
+    st0 = raw[:9 * m.n_nodes].reshape(m.n_nodes, 3, 3)
+    assert not st0[:, 0].any()
+    assert np.array_equal(st0[:, 1], np.tile(v0, (m.n_nodes, 1)))
